@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Traversal microbenchmark on one MI355X: Mrays/s of the ray-query hot path.
+
+Workload (seeded): 410 level-4 icospheres = 2,099,200 FP32 triangles in one
+mesh inside a 100-unit cube; ray sets 1920x1080 primary closest-hit rays,
+2,000,000 cosine-hemisphere bounce closest-hit rays from primary hits, and
+2,000,000 shadow rays to uniform points on 64 random triangles (unoccluded
+query with target triangle).  One step = the three launches over all
+6,073,600 rays, inputs resident in HBM.
+
+Prints ONE JSON line (see the keys below).  ``roofline`` is priced for the
+dominant kernel from its algorithmic bytes (per-ray work counts from the
+counting variant of the same traversal) over its average launch duration,
+measured with HIP events on the stream the kernels run on.  ``cpu_baseline``
+times the oracle (CPU restatement of the reference traversal) on a bounded
+sample of the same rays with all host threads.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "toy-path-tracing_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+B_RAY_IN = 32
+B_HIT_OUT = 24
+B_OCC_OUT = 1
+B_NODE_FP32 = 112  # 96 B child boxes + 16 B child refs
+B_NODE_Q8 = 64
+B_TRI = 40  # 36 B positions + 4 B triangle id
+B_INST = 64
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nodes", choices=["fp32", "q8", "both"], default="both",
+                    help="node format(s) to time; the reported value is the faster")
+    ap.add_argument("--cpu-sample", type=int, default=300_000, help="rays per set for the CPU baseline (0: skip)")
+    ap.add_argument("--spheres", type=int, default=410)
+    ap.add_argument("--rays", type=int, default=2_000_000, help="bounce and shadow ray counts")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: only one GPU is used in this build (asked for --gpus {args.gpus}); "
+              "the ray batch would shard by rays with the scene replicated per GPU", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import tpt
+    from tpt import scenes
+
+    t0 = time.time()
+    sf = scenes.sphere_field(n_spheres=args.spheres)
+    sc = scenes.single_mesh_scene(sf.mesh)
+    log(f"scene: {sf.mesh.triangle_count()} triangles, {len(sf.mesh.qbvh.nodes)} nodes, build+gen "
+        f"{time.time() - t0:.1f}s")
+    dev = tpt.Device(0)
+    ds = sc.upload(dev)
+    primary = scenes.primary_rays(1920, 1080, **scenes.TRAVERSAL_CAMERA)
+    hp = ds.closest_hit(primary)
+    bounce = scenes.bounce_rays(primary, hp, sf.mesh, args.rays)
+    shadow = scenes.shadow_rays(primary, hp, sf.mesh, args.rays, n_lights=64)
+    log(f"rays: primary {len(primary)} (hit {np.mean(hp['triangle'] != tpt.INVALID_INDEX):.3f}), "
+        f"bounce {len(bounce)}, shadow {len(shadow.rays)}")
+
+    def to_dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).reshape(-1)).to("cuda")
+
+    sets = {
+        "primary": dict(kind="closest", n=len(primary), rays=to_dev(primary), host=primary),
+        "bounce": dict(kind="closest", n=len(bounce), rays=to_dev(bounce), host=bounce),
+        "shadow": dict(kind="any", n=len(shadow.rays), rays=to_dev(shadow.rays), host=shadow.rays,
+                       target=to_dev(shadow.target), target_host=shadow.target),
+    }
+    for s in sets.values():
+        s["out"] = torch.empty(s["n"] * (B_HIT_OUT if s["kind"] == "closest" else B_OCC_OUT), dtype=torch.uint8,
+                               device="cuda")
+    # a dedicated (default-priority) stream: the kernels and the timing events
+    # go on it; torch's legacy default stream would map to the library's own.
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    dev.set_stream(stream.cuda_stream)
+
+    def launch(name, q8):
+        s = sets[name]
+        if s["kind"] == "closest":
+            ds.closest_hit_device(s["rays"], s["out"], s["n"], quantized=q8)
+        else:
+            ds.unoccluded_device(s["rays"], s["target"], None, s["out"], s["n"], quantized=q8)
+
+    # algorithmic bytes per launch from per-ray work counts (untimed)
+    def algo_bytes(name, q8):
+        s = sets[name]
+        c = ds.counts(s["host"], "closest" if s["kind"] == "closest" else "any",
+                      target=s.get("target_host"), quantized=q8).astype(np.float64).sum(0)
+        out_b = B_HIT_OUT if s["kind"] == "closest" else B_OCC_OUT + 8
+        return s["n"] * (B_RAY_IN + out_b) + c[0] * (B_NODE_Q8 if q8 else B_NODE_FP32) + c[1] * B_TRI + c[2] * B_INST, c / s["n"]
+
+    results = {}
+    modes = {"fp32": [False], "q8": [True], "both": [False, True]}[args.nodes]
+    for q8 in modes:
+        label = "q8" if q8 else "fp32"
+        for _ in range(args.warmup):
+            for name in sets:
+                launch(name, q8)
+        torch.cuda.synchronize()
+        ev = {name: [] for name in sets}
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            for name in sets:
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                launch(name, q8)
+                b.record(stream)
+                ev[name].append((a, b))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+        kern_ms = {name: float(np.mean([a.elapsed_time(b) for a, b in ev[name]])) for name in sets}
+        results[label] = dict(wall_s=wall, kern_ms=kern_ms, q8=q8)
+        log(f"{label}: {args.steps} steps in {wall:.3f}s; per-kernel ms " +
+            ", ".join(f"{k}={v:.3f}" for k, v in kern_ms.items()))
+
+    best = min(results, key=lambda k: results[k]["wall_s"])
+    r = results[best]
+    rays_per_step = sum(s["n"] for s in sets.values())
+    ms_per_step = r["wall_s"] / args.steps * 1e3
+    value = rays_per_step / (ms_per_step / 1e3) / 1e6
+
+    # roofline of the dominant kernel of the reported configuration
+    dom = max(r["kern_ms"], key=lambda k: r["kern_ms"][k])
+    bytes_launch, per_ray = algo_bytes(dom, r["q8"])
+    achieved = bytes_launch / (r["kern_ms"][dom] / 1e3) / 1e9
+    per_set = {}
+    for name in sets:
+        per_set[name] = round(sets[name]["n"] / (r["kern_ms"][name] / 1e3) / 1e6, 1)
+    log(f"dominant {dom}: {bytes_launch / 1e6:.1f} MB algorithmic per launch, per-ray "
+        f"nodes/tris/insts {per_ray.round(2).tolist()}, {achieved:.0f} GB/s")
+
+    cpu = None
+    if args.cpu_sample > 0:
+        cpu = cpu_baseline(sc, sets, args.cpu_sample)
+
+    line = {
+        "metric": "Mrays/s",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "traversal microbenchmark: 410 level-4 icospheres (2,099,200 triangles, one mesh, "
+                        "100-unit cube); 1920x1080 primary + 2M bounce closest-hit, 2M shadow any-hit",
+            "nodes": best,
+            "rays_per_step": rays_per_step,
+            "mrays_per_s_by_set": per_set,
+            "kernel_ms": {k: round(v, 4) for k, v in r["kern_ms"].items()},
+            "fp32_vs_q8_wall_s": {k: round(v["wall_s"], 4) for k, v in results.items()},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"{dom} ({'closest' if sets[dom]['kind'] == 'closest' else 'any'}, {best})",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ds.close()
+    dev.close()
+
+
+def cpu_baseline(sc, sets, sample):
+    """The oracle on a seeded sample of each ray set, all host threads."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle  # noqa: E402  (test infrastructure: CPU baseline leg only)
+
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    t = time.time()
+    o = oracle.oracle_scene_from(sc)
+    log(f"cpu baseline: oracle build {time.time() - t:.1f}s (untimed), {threads} threads")
+    rng = np.random.default_rng(123)
+    total_rays, total_s = 0, 0.0
+    for name, s in sets.items():
+        idx = np.sort(rng.choice(s["n"], min(sample, s["n"]), replace=False))
+        rays = s["host"][idx]
+        t = time.perf_counter()
+        if s["kind"] == "closest":
+            o.closest_hit(rays, threads=threads)
+        else:
+            o.unoccluded(rays, s["target_host"][idx], threads=threads)
+        dt = time.perf_counter() - t
+        total_rays += len(rays)
+        total_s += dt
+        log(f"cpu baseline {name}: {len(rays)} rays {dt:.2f}s = {len(rays) / dt / 1e6:.2f} Mrays/s")
+    return {
+        "value": round(total_rays / total_s / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample} seeded rays from each of the primary, bounce and shadow sets "
+                  f"({total_rays} rays, {total_s:.1f}s); shadow answered as the reference does "
+                  "(closest hit + target test)",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,184 @@
+/*
+ * tpt.h — C ABI of the MI355X ray-query backend (libtpt.so).
+ *
+ * The backend replaces the per-ray scene queries of the reference renderer
+ * (crates/renderer/src = R/):
+ *   traverse_qbvh                R/qbvh.rs:611-675
+ *   Scene::closest_hit           R/scene/mod.rs:340-395
+ *   closest_mesh_hit_with_filter R/scene/mod.rs:686-723
+ *   intersect_triangle           R/math/ray.rs:107-209
+ *   unoccluded_ray               R/integrator/mod.rs:239-265
+ *   build_qbvh (host)            R/qbvh.rs:144-166
+ *
+ * Conventions follow the reference's own C shim (crates/ocio-sys/src/shim.h):
+ * opaque handles; create functions return NULL on failure; status functions
+ * return int32_t (0 = ok); a thread-local last-error string; no exception or
+ * panic crosses the boundary.  Inputs are caller-owned and copied (scene) or
+ * only read during the call; outputs are caller-allocated.
+ */
+#ifndef TPT_H
+#define TPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TPT_OK 0
+#define TPT_ERROR 1              /* generic error, see tpt_last_error()          */
+#define TPT_ERROR_NOT_BUILT 2    /* ClosestHitError::QbvhNotBuilt, R/scene/mod.rs:100-113 */
+#define TPT_ERROR_BAD_ARGUMENT 3 /* null pointer, index out of range, bad layout  */
+#define TPT_ERROR_DEVICE 4       /* HIP runtime error                            */
+#define TPT_ERROR_OVERFLOW 5     /* leaf offset/count beyond the 24/7-bit child encoding (R/qbvh.rs:15-33) */
+
+/* ---- per-call flags ---------------------------------------------------- */
+/* Pointers passed to a trace call are device (HBM) pointers; the call only
+ * enqueues work on the device stream and returns without synchronising.
+ * Without this flag pointers are host pointers and the call is synchronous. */
+#define TPT_FLAG_DEVICE_POINTERS 0x1u
+/* Traverse the 8-bit quantised copy of the QBVH nodes instead of the FP32
+ * nodes.  Child boxes are rounded outward, so the set of candidate
+ * triangles only grows; the closest hit is unchanged except on exact ties. */
+#define TPT_FLAG_QUANTIZED 0x2u
+/* tpt_trace_any only: answer with a full closest-hit query followed by the
+ * reference's decision rule (bit-exact with unoccluded_ray, no early exit). */
+#define TPT_FLAG_ANY_EXACT 0x4u
+
+/* ---- records ------------------------------------------------------------ */
+/* Ray: origin, t_max, direction (R/math/ray.rs:5-11 without differentials).
+ * The reference always queries with t_max = +inf (R/scene/mod.rs:352). */
+typedef struct TptRay {
+  float o[3];
+  float tmax;
+  float d[3];
+  float pad;
+} TptRay; /* 32 bytes */
+
+/* SceneHit (R/scene/mod.rs:74-79): TriangleRef = (instance, triangle),
+ * t, barycentric (b0, b1, b2).  A miss has instance = triangle = 0xFFFFFFFF. */
+typedef struct TptHit {
+  uint32_t instance;
+  uint32_t triangle;
+  float t;
+  float b[3];
+} TptHit; /* 24 bytes */
+
+#define TPT_INVALID_INDEX 0xFFFFFFFFu
+
+/* QbvhNode in the reference layout (R/qbvh.rs:54-64): four child boxes as
+ * structure-of-arrays, child references (leaf bit | 7-bit count | 24-bit
+ * offset, or interior node index), and the valid-slot mask. */
+typedef struct TptQbvhNode {
+  float min_x[4], min_y[4], min_z[4];
+  float max_x[4], max_y[4], max_z[4];
+  uint32_t children[4];
+  uint8_t valid_mask;
+  uint8_t pad[15];
+} TptQbvhNode; /* 128 bytes */
+
+/* One mesh (R/scene/mesh.rs:51-57): vertex positions (xyz), triangle indices,
+ * and its built QBVH (nodes may be empty: root-leaf case, R/qbvh.rs:615-620). */
+typedef struct TptMeshDesc {
+  const float* positions;           /* 3 * vertex_count */
+  uint32_t vertex_count;
+  uint32_t triangle_count;
+  const uint32_t* indices;          /* 3 * triangle_count */
+  const TptQbvhNode* nodes;         /* node_count entries, may be NULL if 0 */
+  uint32_t node_count;
+  uint32_t pad;
+  const uint32_t* primitive_indices;/* triangle_count entries (Qbvh::primitive_indices) */
+} TptMeshDesc;
+
+/* Instance (R/scene/mod.rs:62-70): 4x4 matrices in glam's column-major order. */
+typedef struct TptInstanceDesc {
+  uint32_t mesh;
+  uint32_t material;
+  float local_to_world[16];
+  float world_to_local[16];
+} TptInstanceDesc;
+
+/* Scene geometry after Scene::build_qbvh (R/scene/mod.rs:305-316). */
+typedef struct TptSceneDesc {
+  const TptMeshDesc* meshes;
+  uint32_t mesh_count;
+  uint32_t instance_count;
+  const TptInstanceDesc* instances;
+  const TptQbvhNode* tlas_nodes;          /* over instance world bounds */
+  uint32_t tlas_node_count;
+  uint32_t pad;
+  const uint32_t* tlas_primitive_indices; /* instance_count entries */
+} TptSceneDesc;
+
+typedef struct TptDevice TptDevice;
+typedef struct TptScene TptScene;
+typedef struct TptQbvh TptQbvh;
+
+/* ---- errors -------------------------------------------------------------- */
+/* Last error of the calling thread (shim.h:14-15 convention); never NULL. */
+const char* tpt_last_error(void);
+
+/* ---- device -------------------------------------------------------------- */
+TptDevice* tpt_device_create(int32_t device_index);
+void tpt_device_release(TptDevice* device);
+/* Stream that trace calls are enqueued on (a hipStream_t; NULL = the
+ * library's own stream).  Lets a caller time or order work with its events. */
+int32_t tpt_device_set_stream(TptDevice* device, void* hip_stream);
+int32_t tpt_device_synchronize(TptDevice* device);
+
+/* ---- host QBVH build (R/qbvh.rs:144-516) -------------------------------- */
+/* Binned-SAH 4-wide build over primitive bounds (3 floats per min / max).
+ * Returns NULL when n == 0 (build_qbvh returns None) or on error.
+ * threads <= 0 uses the hardware concurrency; the tree does not depend on it. */
+TptQbvh* tpt_qbvh_build(const float* bounds_min, const float* bounds_max, size_t n, int32_t threads);
+void tpt_qbvh_release(TptQbvh* qbvh);
+size_t tpt_qbvh_node_count(const TptQbvh* qbvh);
+size_t tpt_qbvh_primitive_count(const TptQbvh* qbvh);
+/* Copies the nodes (node_count entries) and primitive_indices out. */
+int32_t tpt_qbvh_export(const TptQbvh* qbvh, TptQbvhNode* nodes, uint32_t* primitive_indices);
+
+/* 8-bit quantised copy of n nodes (64 bytes each) as the device traverses
+ * them: per axis an FP32 origin and a power-of-two scale, child corners
+ * origin + q * 2^e rounded outward so each decoded box contains the FP32
+ * box.  No reference counterpart (R/qbvh.rs stores FP32 boxes only); exposed
+ * so a caller can cache or inspect the encoding. */
+int32_t tpt_qbvh_quantize(const TptQbvhNode* nodes, size_t n, void* out_q8_nodes);
+
+/* ---- scene ---------------------------------------------------------------- */
+/* Copies the scene to the device and derives the GPU layouts (leaf-ordered
+ * triangle records, FP32 and 8-bit quantised node arrays). */
+TptScene* tpt_scene_create(TptDevice* device, const TptSceneDesc* desc);
+void tpt_scene_release(TptScene* scene);
+/* Device bytes held by the scene (nodes, quantised nodes, triangles, instances). */
+size_t tpt_scene_device_bytes(const TptScene* scene);
+
+/* ---- queries -------------------------------------------------------------- */
+/* Closest hit for n rays: Scene::closest_hit (R/scene/mod.rs:340-395) with the
+ * per-ray t_max as the initial bound. */
+int32_t tpt_trace_closest(TptScene* scene, const TptRay* rays, size_t n, TptHit* hits, uint32_t flags);
+
+/* Shadow-ray visibility, unoccluded_ray (R/integrator/mod.rs:239-265) for rays
+ * already offset by spawn_ray.  target: per ray (instance, triangle) of the
+ * light triangle, or instance = TPT_INVALID_INDEX for none; NULL = no targets.
+ * distance: per ray light distance (+inf for infinite lights); NULL = +inf.
+ * out[i] = 1 if unoccluded.  Without TPT_FLAG_ANY_EXACT this is an early-exit
+ * any-hit query whose answer differs from the reference only on exact ties. */
+int32_t tpt_trace_any(TptScene* scene, const TptRay* rays, size_t n, const uint32_t* target,
+                      const float* distance, uint8_t* unoccluded, uint32_t flags);
+
+/* The traversal of tpt_trace_closest (query 0) or tpt_trace_any (query 1)
+ * with per-ray work counters instead of results: counts[3*i+0] nodes
+ * visited, [3*i+1] triangles tested, [3*i+2] instances entered.  For query 0
+ * the counts equal those of the reference traversal (same visit order); they
+ * give the algorithmic bytes per ray of the roofline.  No reference
+ * counterpart (the reference keeps no counters). */
+int32_t tpt_trace_counts(TptScene* scene, uint32_t query, const TptRay* rays, size_t n, const uint32_t* target,
+                         const float* distance, uint32_t* counts, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPT_H */

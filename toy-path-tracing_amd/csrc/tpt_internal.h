@@ -1,0 +1,78 @@
+// Internal declarations shared by the host builder, the scene upload and the
+// kernels of libtpt.  Not part of the ABI (include/tpt.h is).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/tpt.h"
+
+namespace tpt {
+
+// Child reference encoding, R/qbvh.rs:15-51: leaf bit | 7-bit count | 24-bit
+// offset, or an interior node index; EMPTY_CHILD = leaf bit with count 0.
+constexpr uint32_t kChildLeafBit = 1u << 31;
+constexpr uint32_t kChildCountShift = 24;
+constexpr uint32_t kChildCountMask = 0x7Fu;
+constexpr uint32_t kChildOffsetMask = (1u << 24) - 1;
+
+struct HostQbvh {
+  std::vector<TptQbvhNode> nodes;
+  std::vector<uint32_t> prim;  // primitive_indices
+};
+
+TptQbvhNode empty_qbvh_node();
+// Returns 0 on success, -1 on bad input, -2 when a leaf exceeds the child
+// encoding (more than 2^24-1 primitive offset or 127 primitives).
+int build_qbvh(const float* bmin, const float* bmax, size_t n, int threads, HostQbvh* out);
+
+// ---- device layouts --------------------------------------------------------
+
+// FP32 node: 4 child boxes as SoA float4 rows + child refs.  128 bytes, so a
+// node is one aligned 128-byte line; the kernel reads the first 112 bytes.
+struct alignas(16) GpuNode {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  uint32_t child[4];
+  uint32_t pad[4];
+};
+static_assert(sizeof(GpuNode) == 128, "GpuNode must be 128 bytes");
+
+// 8-bit quantised node, 64 bytes.  Child box corner = origin + q * 2^e per
+// axis, evaluated in FP32 on the device exactly as on the host encoder; the
+// encoder picks q so that the decoded box contains the FP32 box.
+// flags bit 0: node not quantisable (non-finite bounds) -> use the FP32 node.
+struct alignas(16) GpuNodeQ8 {
+  float origin[3];
+  int8_t exp[3];
+  uint8_t flags;
+  uint8_t q[24];  // lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4]
+  uint32_t pad[2];
+  uint32_t child[4];  // 16-byte aligned at offset 48
+};
+static_assert(sizeof(GpuNodeQ8) == 64, "GpuNodeQ8 must be 64 bytes");
+
+// Leaf-ordered triangle record: the three vertex positions of
+// primitive_indices[k] and that triangle index.  48 bytes, 40 read.
+struct alignas(16) GpuTri {
+  float v[9];
+  uint32_t prim;
+  uint32_t pad[2];
+};
+static_assert(sizeof(GpuTri) == 48, "GpuTri must be 48 bytes");
+
+// Instance: world_to_local (columns 0..3, xyz rows) and where its mesh lives.
+struct alignas(16) GpuInstance {
+  float m[12];
+  uint32_t node_base;  // first node of the mesh's BLAS in the node array
+  uint32_t node_count; // 0: root leaf over all triangles
+  uint32_t tri_base;   // first leaf-ordered triangle of the mesh
+  uint32_t tri_count;
+};
+static_assert(sizeof(GpuInstance) == 64, "GpuInstance must be 64 bytes");
+
+// Host-side quantiser used by the scene upload (exposed for tests).
+void quantize_node(const GpuNode& n, GpuNodeQ8* q);
+GpuNode to_gpu_node(const TptQbvhNode& n, uint32_t interior_base);
+
+}  // namespace tpt

@@ -1,0 +1,1141 @@
+// libtpt: MI355X (gfx950) ray queries over the reference QBVH.
+//
+// Kernels: closest hit (R/scene/mod.rs:340-395 over R/qbvh.rs:611-675 and
+// R/math/ray.rs:107-209) and shadow visibility (R/integrator/mod.rs:239-265),
+// each over FP32 or 8-bit quantised nodes.  One ray per lane, wave64.
+//
+// Bit-exactness with the reference CPU code rests on:
+//  * the same float expression trees (built with -ffp-contract=off; fma only
+//    where the reference calls mul_add; IEEE division and square root);
+//  * the slab test's compare-select min/max (x86 minps/maxps semantics of
+//    wide::f32x4::fast_min/fast_max), written as explicit selects;
+//  * the reference visit order: children sorted near->far by (tmin, slot),
+//    leaves handled far->near before the next pop, interiors pushed far->near.
+//
+// Traversal stack: a per-lane LDS stack of kLdsDepth entries laid out
+// [depth][thread] (bank = thread, conflict-free), continued in a per-lane
+// private array up to the reference's 128 entries (R/qbvh.rs:609).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tpt_internal.h"
+
+namespace tpt {
+
+constexpr int kBlock = 256;
+constexpr int kLdsDepth = 16;
+constexpr int kStackDepth = 128;
+constexpr uint32_t kEmptyChild = kChildLeafBit;
+
+// ---------------------------------------------------------------------------
+// device math
+
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float sel_min(float a, float b) { return a < b ? a : b; }  // minps
+__device__ __forceinline__ float sel_max(float a, float b) { return a > b ? a : b; }  // maxps
+
+__device__ __forceinline__ float difference_of_products(float a, float b, float c, float d) {
+  float cd = c * d;
+  float dop = __builtin_fmaf(a, b, -cd);
+  float err = __builtin_fmaf(-c, d, cd);
+  return dop + err;
+}
+
+// gamma(n) = n*eps/2 / (1 - n*eps/2) in f32, R/math/mod.rs:143-146
+__device__ __forceinline__ constexpr float gamma_n(int n) {
+  return ((float)n * (__FLT_EPSILON__ * 0.5f)) / (1.0f - (float)n * (__FLT_EPSILON__ * 0.5f));
+}
+
+__device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }  // f32::max
+__device__ __forceinline__ float max3abs(float a, float b, float c) {
+  return rmax(fabsf(a), rmax(fabsf(b), fabsf(c)));
+}
+
+struct TriResult {
+  float t, b0, b1, b2;
+};
+
+// intersect_triangle, R/math/ray.rs:107-209.  Same operation order as the
+// reference, including the fp64 recomputation of the edge functions.
+__device__ __forceinline__ bool intersect_triangle(F3 ro, F3 rd, float t_max, F3 v0, F3 v1, F3 v2,
+                                                   TriResult* out) {
+  {
+    F3 a = v2 - v0, b = v1 - v0;
+    F3 c = f3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+    if (c.x * c.x + c.y * c.y + c.z * c.z == 0.0f) return false;
+  }
+  F3 p0 = v0 - ro, p1 = v1 - ro, p2 = v2 - ro;
+  // max_component_index(|d|), then kx = kz+1, ky = kx+1 (mod 3)
+  const float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
+  const int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  float dx, dy, dz;
+  float p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
+  if (kz == 0) {  // (kx, ky, kz) = (1, 2, 0)
+    dx = rd.y; dy = rd.z; dz = rd.x;
+    p0x = p0.y; p0y = p0.z; p0z = p0.x;
+    p1x = p1.y; p1y = p1.z; p1z = p1.x;
+    p2x = p2.y; p2y = p2.z; p2z = p2.x;
+  } else if (kz == 1) {  // (2, 0, 1)
+    dx = rd.z; dy = rd.x; dz = rd.y;
+    p0x = p0.z; p0y = p0.x; p0z = p0.y;
+    p1x = p1.z; p1y = p1.x; p1z = p1.y;
+    p2x = p2.z; p2y = p2.x; p2z = p2.y;
+  } else {  // (0, 1, 2)
+    dx = rd.x; dy = rd.y; dz = rd.z;
+    p0x = p0.x; p0y = p0.y; p0z = p0.z;
+    p1x = p1.x; p1y = p1.y; p1z = p1.z;
+    p2x = p2.x; p2y = p2.y; p2z = p2.z;
+  }
+  if (dz == 0.0f) return false;
+  const float sx = -dx / dz;
+  const float sy = -dy / dz;
+  const float sz = 1.0f / dz;
+  p0x += sx * p0z;
+  p0y += sy * p0z;
+  p1x += sx * p1z;
+  p1y += sy * p1z;
+  p2x += sx * p2z;
+  p2y += sy * p2z;
+  float e0 = difference_of_products(p1x, p2y, p1y, p2x);
+  float e1 = difference_of_products(p2x, p0y, p2y, p0x);
+  float e2 = difference_of_products(p0x, p1y, p0y, p1x);
+  if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+    e0 = (float)((double)p2y * (double)p1x - (double)p2x * (double)p1y);
+    e1 = (float)((double)p0y * (double)p2x - (double)p0x * (double)p2y);
+    e2 = (float)((double)p1y * (double)p0x - (double)p1x * (double)p0y);
+  }
+  if ((e0 < 0.0f || e1 < 0.0f || e2 < 0.0f) && (e0 > 0.0f || e1 > 0.0f || e2 > 0.0f)) return false;
+  const float det = e0 + e1 + e2;
+  if (det == 0.0f) return false;
+  p0z *= sz;
+  p1z *= sz;
+  p2z *= sz;
+  const float t_scaled = e0 * p0z + e1 * p1z + e2 * p2z;
+  if ((det < 0.0f && (t_scaled >= 0.0f || t_scaled < t_max * det)) ||
+      (det > 0.0f && (t_scaled <= 0.0f || t_scaled > t_max * det)))
+    return false;
+  const float inv_det = 1.0f / det;
+  const float t = t_scaled * inv_det;
+  const float max_zt = max3abs(p0z, p1z, p2z);
+  const float delta_z = gamma_n(3) * max_zt;
+  const float max_xt = max3abs(p0x, p1x, p2x);
+  const float max_yt = max3abs(p0y, p1y, p2y);
+  const float delta_x = gamma_n(5) * (max_xt + max_zt);
+  const float delta_y = gamma_n(5) * (max_yt + max_zt);
+  const float delta_e = 2.0f * (gamma_n(2) * max_xt * max_yt + delta_y * max_xt + delta_x * max_yt);
+  const float max_e = max3abs(e0, e1, e2);
+  const float delta_t =
+      3.0f * (gamma_n(3) * max_e * max_zt + delta_e * max_zt + delta_z * max_e) * fabsf(inv_det);
+  if (t <= delta_t) return false;
+  out->t = t;
+  out->b0 = e0 * inv_det;
+  out->b1 = e1 * inv_det;
+  out->b2 = e2 * inv_det;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// scene view passed to kernels by value
+
+struct SceneView {
+  const GpuNode* nodes;      // all BLAS nodes
+  const GpuNodeQ8* qnodes;   // quantised copy, same indices
+  const GpuTri* tris;        // leaf-ordered triangles of all meshes
+  const uint32_t* tri_rank;  // per mesh triangle index -> leaf-ordered slot
+  const GpuInstance* inst;
+  const GpuNode* tlas;
+  const GpuNodeQ8* qtlas;
+  const uint32_t* tlas_prim;  // instance ids in TLAS leaf order
+  uint32_t instance_count;
+  uint32_t tlas_node_count;
+};
+
+struct Ray3 {
+  F3 o, d, inv;
+};
+
+__device__ __forceinline__ float safe_inv(float d) { return d == 0.0f ? __builtin_inff() : 1.0f / d; }
+
+// glam Mat4::transform_point3 / transform_vector3 (x*px, then y*py + r, z*pz + r, w + r)
+__device__ __forceinline__ void to_local(const GpuInstance& in, F3 o, F3 d, Ray3* r) {
+  const float* m = in.m;
+  float rx = m[0] * o.x, ry = m[1] * o.x, rz = m[2] * o.x;
+  rx = m[3] * o.y + rx; ry = m[4] * o.y + ry; rz = m[5] * o.y + rz;
+  rx = m[6] * o.z + rx; ry = m[7] * o.z + ry; rz = m[8] * o.z + rz;
+  rx = m[9] + rx; ry = m[10] + ry; rz = m[11] + rz;
+  r->o = f3(rx, ry, rz);
+  rx = m[0] * d.x; ry = m[1] * d.x; rz = m[2] * d.x;
+  rx = m[3] * d.y + rx; ry = m[4] * d.y + ry; rz = m[5] * d.y + rz;
+  rx = m[6] * d.z + rx; ry = m[7] * d.z + ry; rz = m[8] * d.z + rz;
+  r->d = f3(rx, ry, rz);
+  r->inv = f3(safe_inv(rx), safe_inv(ry), safe_inv(rz));
+}
+
+__device__ __forceinline__ float q8_scale(int e) { return __int_as_float((e + 127) << 23); }
+
+// Node slab test, intersect_4_aabbs (R/qbvh.rs:578-607).  Writes per-slot
+// tmin and child ref; returns the 4-bit hit mask restricted to non-empty slots.
+template <bool kQ8>
+__device__ __forceinline__ uint32_t slab4(const GpuNode* __restrict__ nodes, const GpuNodeQ8* __restrict__ qnodes,
+                                          uint32_t ni, const Ray3& r, float t_max, float tmin[4],
+                                          uint32_t child[4]) {
+  float lx[4], hx[4], ly[4], hy[4], lz[4], hz[4];
+  bool fp32 = !kQ8;
+  if (kQ8) {
+    const uint4* qp = reinterpret_cast<const uint4*>(qnodes + ni);
+    const uint4 a = qp[0], b = qp[1], c = qp[2], ch = qp[3];
+    const uint32_t flags = (a.w >> 24) & 0xFFu;
+    if (flags & 1u) {
+      fp32 = true;
+    } else {
+      const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
+      const float sxs = q8_scale((int)(int8_t)(a.w & 0xFFu));
+      const float sys = q8_scale((int)(int8_t)((a.w >> 8) & 0xFFu));
+      const float szs = q8_scale((int)(int8_t)((a.w >> 16) & 0xFFu));
+      // q bytes: b.x = lox[0..3], b.y = hix, b.z = loy, b.w = hiy, c.x = loz, c.y = hiz
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int sh = 8 * i;
+        lx[i] = ox + (float)((b.x >> sh) & 0xFFu) * sxs;
+        hx[i] = ox + (float)((b.y >> sh) & 0xFFu) * sxs;
+        ly[i] = oy + (float)((b.z >> sh) & 0xFFu) * sys;
+        hy[i] = oy + (float)((b.w >> sh) & 0xFFu) * sys;
+        lz[i] = oz + (float)((c.x >> sh) & 0xFFu) * szs;
+        hz[i] = oz + (float)((c.y >> sh) & 0xFFu) * szs;
+      }
+      (void)c;
+      child[0] = ch.x;
+      child[1] = ch.y;
+      child[2] = ch.z;
+      child[3] = ch.w;
+    }
+  }
+  if (fp32) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + ni);
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4], f = p[5];
+    const uint4 ch = *reinterpret_cast<const uint4*>(p + 6);
+    lx[0] = a.x; lx[1] = a.y; lx[2] = a.z; lx[3] = a.w;
+    hx[0] = b.x; hx[1] = b.y; hx[2] = b.z; hx[3] = b.w;
+    ly[0] = c.x; ly[1] = c.y; ly[2] = c.z; ly[3] = c.w;
+    hy[0] = d.x; hy[1] = d.y; hy[2] = d.z; hy[3] = d.w;
+    lz[0] = e.x; lz[1] = e.y; lz[2] = e.z; lz[3] = e.w;
+    hz[0] = f.x; hz[1] = f.y; hz[2] = f.z; hz[3] = f.w;
+    child[0] = ch.x;
+    child[1] = ch.y;
+    child[2] = ch.z;
+    child[3] = ch.w;
+  }
+  uint32_t bits = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float t1x = (lx[i] - r.o.x) * r.inv.x, t2x = (hx[i] - r.o.x) * r.inv.x;
+    const float t1y = (ly[i] - r.o.y) * r.inv.y, t2y = (hy[i] - r.o.y) * r.inv.y;
+    const float t1z = (lz[i] - r.o.z) * r.inv.z, t2z = (hz[i] - r.o.z) * r.inv.z;
+    const float tminx = sel_min(t1x, t2x), tmaxx = sel_max(t1x, t2x);
+    const float tminy = sel_min(t1y, t2y), tmaxy = sel_max(t1y, t2y);
+    const float tminz = sel_min(t1z, t2z), tmaxz = sel_max(t1z, t2z);
+    const float tn = sel_max(sel_max(sel_max(tminx, tminy), tminz), 0.0f);
+    const float tx = sel_min(sel_min(sel_min(tmaxx, tmaxy), tmaxz), t_max);
+    tmin[i] = tn;
+    if (tn <= tx && child[i] != kEmptyChild) bits |= 1u << i;
+  }
+  return bits;
+}
+
+// Order the hit slots as the reference's stable insertion sort by tmin does,
+// then emit them far->near: interiors are pushed (so the nearest pops first)
+// and leaves are queued for immediate processing.  Returns the leaf queue
+// (up to 4 child refs, processing order in q[0..n)).
+struct LeafQueue {
+  uint32_t q0, q1, q2, q3;
+  int n;
+};
+
+__device__ __forceinline__ void cswap_desc(float& ta, uint32_t& ca, int& sa, float& tb, uint32_t& cb, int& sb) {
+  // descending by (t, slot): swap when (ta, sa) < (tb, sb)
+  const bool sw = (ta < tb) || (ta == tb && sa < sb);
+  if (sw) {
+    float t = ta; ta = tb; tb = t;
+    uint32_t c = ca; ca = cb; cb = c;
+    int s = sa; sa = sb; sb = s;
+  }
+}
+
+struct Stack {
+  uint32_t* lds;  // base for this thread: lds[d * kBlock]
+  uint32_t spill[kStackDepth - kLdsDepth];
+  int sp;
+  // Callers check sp < kStackDepth first (and flag overflow); a full stack
+  // drops the entry rather than writing out of bounds.
+  __device__ __forceinline__ void push(uint32_t v) {
+    if (sp < kLdsDepth) {
+      lds[sp * kBlock] = v;
+      ++sp;
+    } else if (sp < kStackDepth) {
+      spill[sp - kLdsDepth] = v;
+      ++sp;
+    }
+  }
+  __device__ __forceinline__ uint32_t pop() {
+    --sp;
+    return sp < kLdsDepth ? lds[sp * kBlock] : spill[sp - kLdsDepth];
+  }
+};
+
+template <bool kQ8>
+__device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8* qnodes, uint32_t ni, uint32_t base,
+                                           const Ray3& r, float t_max, Stack& st, LeafQueue& lq, bool* overflow) {
+  float tmin[4];
+  uint32_t child[4];
+  const uint32_t bits = slab4<kQ8>(nodes, qnodes, ni, r, t_max, tmin, child);
+  float t[4];
+  uint32_t c[4];
+  int s[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool h = (bits >> i) & 1u;
+    t[i] = h ? tmin[i] : -1.0f;  // misses sort last (hit tmin >= 0)
+    c[i] = child[i];
+    s[i] = h ? i : -1;
+  }
+  cswap_desc(t[0], c[0], s[0], t[1], c[1], s[1]);
+  cswap_desc(t[2], c[2], s[2], t[3], c[3], s[3]);
+  cswap_desc(t[0], c[0], s[0], t[2], c[2], s[2]);
+  cswap_desc(t[1], c[1], s[1], t[3], c[3], s[3]);
+  cswap_desc(t[1], c[1], s[1], t[2], c[2], s[2]);
+  lq.n = 0;
+  uint32_t q[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (s[k] < 0) continue;
+    const uint32_t ck = c[k];
+    if (ck & kChildLeafBit) {
+      if (((ck >> kChildCountShift) & kChildCountMask) == 0) continue;
+      // append to queue (constant-index shift keeps the queue in registers)
+      if (lq.n == 0) q[0] = ck;
+      else if (lq.n == 1) q[1] = ck;
+      else if (lq.n == 2) q[2] = ck;
+      else q[3] = ck;
+      lq.n++;
+    } else {
+      if (st.sp >= kStackDepth) *overflow = true;
+      st.push(ck + base);
+    }
+  }
+  lq.q0 = q[0];
+  lq.q1 = q[1];
+  lq.q2 = q[2];
+  lq.q3 = q[3];
+}
+
+__device__ __forceinline__ uint32_t lq_pop(LeafQueue& lq) {
+  const uint32_t v = lq.q0;
+  lq.q0 = lq.q1;
+  lq.q1 = lq.q2;
+  lq.q2 = lq.q3;
+  lq.n--;
+  return v;
+}
+
+enum QueryMode { kClosest = 0, kAny = 1 };
+
+struct QueryOut {
+  uint32_t inst, tri;
+  float t, b0, b1, b2;
+  bool hit;
+};
+
+// Two-level traversal as one loop: level 0 walks the TLAS, level 1 the BLAS
+// of the instance being entered; both share the stack (BLAS entries sit above
+// `blas_base`).  Every visit follows the reference order, so the last
+// accepted triangle is the reference's answer.
+// kAny: return at the first accepted triangle that is not (skip_inst,
+// skip_tri) and whose t is below `strict_bound` (hits at or beyond it are
+// ignored and the traversal goes on).
+struct Counters {
+  uint32_t nodes, tris, insts;
+};
+
+template <bool kQ8, int kMode, bool kCount>
+__device__ QueryOut traverse(const SceneView& sc, F3 wo, F3 wd, float t_max, uint32_t skip_inst, uint32_t skip_tri,
+                             float strict_bound, Stack& st, bool* overflow, Counters& cnt) {
+  QueryOut res;
+  res.hit = false;
+  res.inst = TPT_INVALID_INDEX;
+  res.tri = TPT_INVALID_INDEX;
+  res.t = __builtin_inff();
+  res.b0 = res.b1 = res.b2 = 0.0f;
+  if (sc.instance_count == 0) return res;
+
+  Ray3 world;
+  world.o = wo;
+  world.d = wd;
+  world.inv = f3(safe_inv(wd.x), safe_inv(wd.y), safe_inv(wd.z));
+  float closest = t_max;
+
+  // TLAS state
+  LeafQueue tq;
+  tq.n = 0;
+  tq.q0 = tq.q1 = tq.q2 = tq.q3 = 0;
+  uint32_t inst_cur = 0, inst_end = 0;
+  st.sp = 0;
+  if (sc.tlas_node_count == 0) {
+    inst_end = sc.instance_count;  // root leaf over all instances
+  } else {
+    st.push(0);
+  }
+  // BLAS state
+  int level = 0;
+  int blas_base = 0;
+  uint32_t cur_inst = 0;
+  Ray3 local;
+  local.o = local.d = local.inv = f3(0.f, 0.f, 0.f);
+  LeafQueue bq;
+  bq.n = 0;
+  bq.q0 = bq.q1 = bq.q2 = bq.q3 = 0;
+  uint32_t tri_cur = 0, tri_end = 0, node_base = 0, tri_base = 0;
+
+  for (;;) {
+    if (level == 1) {
+      if (tri_cur < tri_end) {
+        const GpuTri* tp = sc.tris + tri_base + tri_cur;
+        const float4 a = reinterpret_cast<const float4*>(tp)[0];
+        const float4 b = reinterpret_cast<const float4*>(tp)[1];
+        const float2 c = reinterpret_cast<const float2*>(tp)[4];
+        const uint32_t prim = __float_as_uint(c.y);
+        TriResult h;
+        const bool skip = (kMode == kAny) && cur_inst == skip_inst && prim == skip_tri;
+        if (kCount && !skip) cnt.tris++;
+        if (!skip &&
+            intersect_triangle(local.o, local.d, closest, f3(a.x, a.y, a.z), f3(a.w, b.x, b.y), f3(b.z, b.w, c.x),
+                               &h) &&
+            (kMode != kAny || h.t < strict_bound)) {
+          closest = h.t;
+          res.hit = true;
+          res.inst = cur_inst;
+          res.tri = prim;
+          res.t = h.t;
+          res.b0 = h.b0;
+          res.b1 = h.b1;
+          res.b2 = h.b2;
+          if (kMode == kAny) return res;
+        }
+        ++tri_cur;
+        continue;
+      }
+      if (bq.n > 0) {
+        const uint32_t c = lq_pop(bq);
+        tri_cur = c & kChildOffsetMask;
+        tri_end = tri_cur + ((c >> kChildCountShift) & kChildCountMask);
+        continue;
+      }
+      if (st.sp > blas_base) {
+        const uint32_t ni = st.pop();
+        if (kCount) cnt.nodes++;
+        visit_node<kQ8>(sc.nodes, sc.qnodes, ni, node_base, local, closest, st, bq, overflow);
+        continue;
+      }
+      level = 0;
+      continue;
+    }
+    // level 0
+    if (inst_cur < inst_end) {
+      cur_inst = sc.tlas_prim[inst_cur++];
+      if (kCount) cnt.insts++;
+      const GpuInstance in = sc.inst[cur_inst];
+      to_local(in, world.o, world.d, &local);
+      node_base = in.node_base;
+      tri_base = in.tri_base;
+      blas_base = st.sp;
+      bq.n = 0;
+      if (in.node_count == 0) {
+        tri_cur = 0;
+        tri_end = in.tri_count;
+      } else {
+        tri_cur = tri_end = 0;
+        if (st.sp >= kStackDepth) *overflow = true;
+        st.push(node_base);
+      }
+      level = 1;
+      continue;
+    }
+    if (tq.n > 0) {
+      const uint32_t c = lq_pop(tq);
+      inst_cur = c & kChildOffsetMask;
+      inst_end = inst_cur + ((c >> kChildCountShift) & kChildCountMask);
+      continue;
+    }
+    if (st.sp > 0) {
+      const uint32_t ni = st.pop();
+      if (kCount) cnt.nodes++;
+      visit_node<kQ8>(sc.tlas, sc.qtlas, ni, 0, world, closest, st, tq, overflow);
+      continue;
+    }
+    break;
+  }
+  return res;
+}
+
+__device__ __forceinline__ void load_ray(const TptRay* rays, uint32_t i, F3* o, F3* d, float* tmax) {
+  const float4* p = reinterpret_cast<const float4*>(rays + i);
+  const float4 a = p[0], b = p[1];
+  *o = f3(a.x, a.y, a.z);
+  *tmax = a.w;
+  *d = f3(b.x, b.y, b.z);
+}
+
+// counts (kCount): per ray nodes visited, triangles tested, instances entered.
+template <bool kQ8, bool kCount>
+__global__ __launch_bounds__(kBlock) void closest_kernel(SceneView sc, const TptRay* __restrict__ rays,
+                                                         TptHit* __restrict__ hits, uint32_t n,
+                                                         uint32_t* __restrict__ status, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t lds[kLdsDepth * kBlock];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  Stack st;
+  st.lds = lds + threadIdx.x;
+  st.sp = 0;
+  F3 o, d;
+  float tmax;
+  load_ray(rays, i, &o, &d, &tmax);
+  bool overflow = false;
+  Counters cnt = {0, 0, 0};
+  const QueryOut q = traverse<kQ8, kClosest, kCount>(sc, o, d, tmax, TPT_INVALID_INDEX, TPT_INVALID_INDEX,
+                                                     __builtin_inff(), st, &overflow, cnt);
+  if (kCount) {
+    counts[3 * i] = cnt.nodes;
+    counts[3 * i + 1] = cnt.tris;
+    counts[3 * i + 2] = cnt.insts;
+  }
+  TptHit h;
+  h.instance = q.inst;
+  h.triangle = q.tri;
+  h.t = q.t;
+  h.b[0] = q.b0;
+  h.b[1] = q.b1;
+  h.b[2] = q.b2;
+  hits[i] = h;
+  if (overflow) atomicOr(status, 1u);
+}
+
+// unoccluded_ray (R/integrator/mod.rs:239-265).  Fast form: the target
+// triangle's own t bounds the query and the first other hit ends it; without
+// a target, t_max = distance * (1 - SHADOW_TOLERANCE) (or +inf).  Exact form:
+// a closest-hit query followed by the reference's decision rule.
+template <bool kQ8, bool kExact, bool kCount>
+__global__ __launch_bounds__(kBlock) void any_kernel(SceneView sc, const TptRay* __restrict__ rays,
+                                                     const uint32_t* __restrict__ target,
+                                                     const float* __restrict__ distance,
+                                                     uint8_t* __restrict__ out, uint32_t n,
+                                                     uint32_t* __restrict__ status, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t lds[kLdsDepth * kBlock];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  Stack st;
+  st.lds = lds + threadIdx.x;
+  st.sp = 0;
+  F3 o, d;
+  float tmax_unused;
+  load_ray(rays, i, &o, &d, &tmax_unused);
+  uint32_t ti = TPT_INVALID_INDEX, tt = TPT_INVALID_INDEX;
+  if (target) {
+    ti = target[2 * i];
+    tt = target[2 * i + 1];
+    if (ti >= sc.instance_count) ti = TPT_INVALID_INDEX;
+  }
+  const float dist = distance ? distance[i] : __builtin_inff();
+  bool overflow = false;
+  Counters cnt = {0, 0, 0};
+  uint8_t unocc;
+  if (kExact) {
+    const QueryOut q = traverse<kQ8, kClosest, kCount>(sc, o, d, __builtin_inff(), TPT_INVALID_INDEX,
+                                                       TPT_INVALID_INDEX, __builtin_inff(), st, &overflow, cnt);
+    if (!q.hit)
+      unocc = 1;
+    else if (ti != TPT_INVALID_INDEX)
+      unocc = (q.inst == ti && q.tri == tt) ? 1 : 0;
+    else if (__builtin_isinf(dist))
+      unocc = 0;
+    else
+      unocc = q.t >= dist * (1.0f - 1.0e-3f) ? 1 : 0;
+  } else {
+    float bound = __builtin_inff();
+    float strict_bound = __builtin_inff();
+    if (ti != TPT_INVALID_INDEX) {
+      const GpuInstance in = sc.inst[ti];
+      if (tt < in.tri_count) {
+        Ray3 local;
+        to_local(in, o, d, &local);
+        const GpuTri* tp = sc.tris + in.tri_base + sc.tri_rank[in.tri_base + tt];
+        TriResult h;
+        if (intersect_triangle(local.o, local.d, __builtin_inff(), f3(tp->v[0], tp->v[1], tp->v[2]),
+                               f3(tp->v[3], tp->v[4], tp->v[5]), f3(tp->v[6], tp->v[7], tp->v[8]), &h))
+          bound = h.t;
+      }
+    } else if (!__builtin_isinf(dist)) {
+      bound = dist * (1.0f - 1.0e-3f);
+      strict_bound = bound;  // occluded iff some hit has t < distance * (1 - 1e-3)
+    }
+    const QueryOut q = traverse<kQ8, kAny, kCount>(sc, o, d, bound, ti, tt, strict_bound, st, &overflow, cnt);
+    unocc = q.hit ? 0 : 1;
+  }
+  out[i] = unocc;
+  if (kCount) {
+    counts[3 * i] = cnt.nodes;
+    counts[3 * i + 1] = cnt.tris;
+    counts[3 * i + 2] = cnt.insts;
+  }
+  if (overflow) atomicOr(status, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+bool hip_ok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return false;
+}
+
+}  // namespace
+
+}  // namespace tpt
+
+struct TptDevice {
+  int index = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  uint32_t* status = nullptr;  // device word: bit 0 = traversal stack overflow
+};
+
+struct TptQbvh {
+  tpt::HostQbvh q;
+};
+
+struct TptScene {
+  TptDevice* dev = nullptr;
+  tpt::GpuNode* nodes = nullptr;
+  tpt::GpuNodeQ8* qnodes = nullptr;
+  tpt::GpuTri* tris = nullptr;
+  uint32_t* tri_rank = nullptr;
+  tpt::GpuInstance* inst = nullptr;
+  tpt::GpuNode* tlas = nullptr;
+  tpt::GpuNodeQ8* qtlas = nullptr;
+  uint32_t* tlas_prim = nullptr;
+  uint32_t instance_count = 0;
+  uint32_t tlas_node_count = 0;
+  size_t bytes = 0;
+  // staging for host-pointer calls (grow-only)
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+};
+
+namespace tpt {
+namespace {
+
+template <class T>
+bool upload(const std::vector<T>& v, T** dst, size_t* bytes) {
+  *dst = nullptr;
+  const size_t nb = (v.empty() ? 1 : v.size()) * sizeof(T);
+  if (!hip_ok(hipMalloc((void**)dst, nb), "hipMalloc")) return false;
+  *bytes += nb;
+  if (!v.empty() && !hip_ok(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy"))
+    return false;
+  return true;
+}
+
+void free_scene(TptScene* s) {
+  if (!s) return;
+  (void)hipFree(s->nodes);
+  (void)hipFree(s->qnodes);
+  (void)hipFree(s->tris);
+  (void)hipFree(s->tri_rank);
+  (void)hipFree(s->inst);
+  (void)hipFree(s->tlas);
+  (void)hipFree(s->qtlas);
+  (void)hipFree(s->tlas_prim);
+  (void)hipFree(s->stage);
+  delete s;
+}
+
+SceneView view_of(const TptScene* s) {
+  SceneView v;
+  v.nodes = s->nodes;
+  v.qnodes = s->qnodes;
+  v.tris = s->tris;
+  v.tri_rank = s->tri_rank;
+  v.inst = s->inst;
+  v.tlas = s->tlas;
+  v.qtlas = s->qtlas;
+  v.tlas_prim = s->tlas_prim;
+  v.instance_count = s->instance_count;
+  v.tlas_node_count = s->tlas_node_count;
+  return v;
+}
+
+// Upper bound of the traversal stack: along any root-to-node path, a visit
+// leaves (interior children - 1) entries below the next pop.
+int stack_bound(const std::vector<TptQbvhNode>& nodes, uint32_t root, int depth_guard) {
+  if (nodes.empty() || depth_guard > 256) return 0;
+  const TptQbvhNode& n = nodes[root];
+  int interior = 0, best = 0;
+  for (int i = 0; i < 4; ++i) {
+    if (!(n.valid_mask & (1u << i))) continue;
+    const uint32_t c = n.children[i];
+    if (c & kChildLeafBit) continue;
+    if (c >= nodes.size()) return 1 << 20;
+    interior++;
+    best = std::max(best, stack_bound(nodes, c, depth_guard + 1));
+  }
+  return interior + best;
+}
+
+bool ensure_stage(TptScene* s, size_t nb) {
+  if (s->stage_bytes >= nb) return true;
+  (void)hipFree(s->stage);
+  s->stage = nullptr;
+  s->stage_bytes = 0;
+  if (!hip_ok(hipMalloc(&s->stage, nb), "hipMalloc(stage)")) return false;
+  s->stage_bytes = nb;
+  return true;
+}
+
+bool read_status(TptDevice* d, uint32_t* v) {
+  return hip_ok(hipMemcpyAsync(v, d->status, 4, hipMemcpyDeviceToHost, d->stream), "hipMemcpyAsync(status)") &&
+         hip_ok(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+}
+
+}  // namespace
+}  // namespace tpt
+
+using namespace tpt;
+
+extern "C" {
+
+const char* tpt_last_error(void) { return g_last_error.c_str(); }
+
+TptDevice* tpt_device_create(int32_t device_index) {
+  try {
+    int count = 0;
+    if (!hip_ok(hipGetDeviceCount(&count), "hipGetDeviceCount")) return nullptr;
+    if (device_index < 0 || device_index >= count) {
+      set_error("tpt_device_create: device index out of range");
+      return nullptr;
+    }
+    if (!hip_ok(hipSetDevice(device_index), "hipSetDevice")) return nullptr;
+    TptDevice* d = new TptDevice();
+    d->index = device_index;
+    if (!hip_ok(hipStreamCreateWithFlags(&d->own, hipStreamNonBlocking), "hipStreamCreate")) {
+      delete d;
+      return nullptr;
+    }
+    d->stream = d->own;
+    if (!hip_ok(hipMalloc((void**)&d->status, 4), "hipMalloc(status)") ||
+        !hip_ok(hipMemset(d->status, 0, 4), "hipMemset(status)")) {
+      (void)hipStreamDestroy(d->own);
+      delete d;
+      return nullptr;
+    }
+    return d;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return nullptr;
+  }
+}
+
+void tpt_device_release(TptDevice* d) {
+  if (!d) return;
+  (void)hipSetDevice(d->index);
+  (void)hipStreamSynchronize(d->stream);
+  (void)hipFree(d->status);
+  (void)hipStreamDestroy(d->own);
+  delete d;
+}
+
+int32_t tpt_device_set_stream(TptDevice* d, void* stream) {
+  if (!d) {
+    set_error("tpt_device_set_stream: null device");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  d->stream = stream ? (hipStream_t)stream : d->own;
+  return TPT_OK;
+}
+
+int32_t tpt_device_synchronize(TptDevice* d) {
+  if (!d) {
+    set_error("tpt_device_synchronize: null device");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  return hip_ok(hipStreamSynchronize(d->stream), "hipStreamSynchronize") ? TPT_OK : TPT_ERROR_DEVICE;
+}
+
+TptQbvh* tpt_qbvh_build(const float* bmin, const float* bmax, size_t n, int32_t threads) {
+  try {
+    if (n == 0) {
+      set_error("tpt_qbvh_build: no primitives (build_qbvh returns None)");
+      return nullptr;
+    }
+    if (!bmin || !bmax) {
+      set_error("tpt_qbvh_build: null bounds");
+      return nullptr;
+    }
+    TptQbvh* q = new TptQbvh();
+    const int rc = build_qbvh(bmin, bmax, n, threads, &q->q);
+    if (rc != 0) {
+      set_error(rc == -2 ? "tpt_qbvh_build: leaf offset exceeds the 24-bit child encoding"
+                         : "tpt_qbvh_build: bad input");
+      delete q;
+      return nullptr;
+    }
+    return q;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return nullptr;
+  }
+}
+
+void tpt_qbvh_release(TptQbvh* q) { delete q; }
+size_t tpt_qbvh_node_count(const TptQbvh* q) { return q ? q->q.nodes.size() : 0; }
+size_t tpt_qbvh_primitive_count(const TptQbvh* q) { return q ? q->q.prim.size() : 0; }
+int32_t tpt_qbvh_export(const TptQbvh* q, TptQbvhNode* nodes, uint32_t* prim) {
+  if (!q || (!nodes && !q->q.nodes.empty()) || !prim) {
+    set_error("tpt_qbvh_export: null argument");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  if (!q->q.nodes.empty()) std::memcpy(nodes, q->q.nodes.data(), q->q.nodes.size() * sizeof(TptQbvhNode));
+  std::memcpy(prim, q->q.prim.data(), q->q.prim.size() * 4);
+  return TPT_OK;
+}
+
+int32_t tpt_qbvh_quantize(const TptQbvhNode* nodes, size_t n, void* out) {
+  if ((!nodes || !out) && n) {
+    set_error("tpt_qbvh_quantize: null argument");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  GpuNodeQ8* q = (GpuNodeQ8*)out;
+  for (size_t i = 0; i < n; ++i) quantize_node(to_gpu_node(nodes[i], 0), &q[i]);
+  return TPT_OK;
+}
+
+TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
+  TptScene* s = nullptr;
+  try {
+    if (!dev || !desc) {
+      set_error("tpt_scene_create: null argument");
+      return nullptr;
+    }
+    if (desc->instance_count && (!desc->instances || !desc->tlas_primitive_indices)) {
+      set_error("tpt_scene_create: instances or tlas_primitive_indices missing");
+      return nullptr;
+    }
+    if (!hip_ok(hipSetDevice(dev->index), "hipSetDevice")) return nullptr;
+    std::vector<GpuNode> nodes;
+    std::vector<GpuNodeQ8> qnodes;
+    std::vector<GpuTri> tris;
+    std::vector<uint32_t> rank;
+    std::vector<uint32_t> mesh_node_base(desc->mesh_count), mesh_tri_base(desc->mesh_count);
+    int blas_stack = 0;
+    for (uint32_t m = 0; m < desc->mesh_count; ++m) {
+      const TptMeshDesc& md = desc->meshes[m];
+      if ((md.triangle_count && (!md.indices || !md.positions || !md.primitive_indices)) ||
+          (md.node_count && !md.nodes)) {
+        set_error("tpt_scene_create: mesh " + std::to_string(m) + " has null arrays");
+        return nullptr;
+      }
+      if (md.triangle_count > kChildOffsetMask + 1u) {
+        set_error("tpt_scene_create: mesh exceeds 2^24 triangles (24-bit leaf offset, R/qbvh.rs:15-18)");
+        return nullptr;
+      }
+      const uint32_t nb = (uint32_t)nodes.size();
+      mesh_node_base[m] = nb;
+      mesh_tri_base[m] = (uint32_t)tris.size();
+      std::vector<TptQbvhNode> mn(md.nodes, md.nodes + md.node_count);
+      for (uint32_t k = 0; k < md.node_count; ++k) {
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t ch = mn[k].children[c];
+          if (!(mn[k].valid_mask & (1u << c))) continue;
+          if (ch & kChildLeafBit) {
+            const uint32_t off = ch & kChildOffsetMask, cnt = (ch >> kChildCountShift) & kChildCountMask;
+            if (off + cnt > md.triangle_count) {
+              set_error("tpt_scene_create: leaf range outside the mesh's triangles");
+              return nullptr;
+            }
+          } else if (ch >= md.node_count) {
+            set_error("tpt_scene_create: interior child index out of range");
+            return nullptr;
+          }
+        }
+        nodes.push_back(to_gpu_node(mn[k], nb));
+        GpuNodeQ8 qn;
+        quantize_node(nodes.back(), &qn);
+        qnodes.push_back(qn);
+      }
+      blas_stack = std::max(blas_stack, stack_bound(mn, 0, 0) + 1);
+      const size_t tb = tris.size();
+      tris.resize(tb + md.triangle_count);
+      rank.resize(tb + md.triangle_count);
+      for (uint32_t k = 0; k < md.triangle_count; ++k) {
+        const uint32_t ti = md.primitive_indices[k];
+        if (ti >= md.triangle_count) {
+          set_error("tpt_scene_create: primitive index out of range");
+          return nullptr;
+        }
+        GpuTri& g = tris[tb + k];
+        std::memset(&g, 0, sizeof(g));
+        for (int v = 0; v < 3; ++v) {
+          const uint32_t vi = md.indices[3 * ti + v];
+          if (vi >= md.vertex_count) {
+            set_error("tpt_scene_create: vertex index out of range");
+            return nullptr;
+          }
+          g.v[3 * v + 0] = md.positions[3 * vi + 0];
+          g.v[3 * v + 1] = md.positions[3 * vi + 1];
+          g.v[3 * v + 2] = md.positions[3 * vi + 2];
+        }
+        g.prim = ti;
+        rank[tb + ti] = k;
+      }
+    }
+    std::vector<GpuInstance> inst(desc->instance_count);
+    for (uint32_t i = 0; i < desc->instance_count; ++i) {
+      const TptInstanceDesc& id = desc->instances[i];
+      if (id.mesh >= desc->mesh_count) {
+        set_error("tpt_scene_create: instance mesh index out of range");
+        return nullptr;
+      }
+      GpuInstance& g = inst[i];
+      const float* w = id.world_to_local;  // column-major: column c = w[4c..4c+2]
+      for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 3; ++r) g.m[3 * c + r] = w[4 * c + r];
+      g.node_base = mesh_node_base[id.mesh];
+      g.node_count = desc->meshes[id.mesh].node_count;
+      g.tri_base = mesh_tri_base[id.mesh];
+      g.tri_count = desc->meshes[id.mesh].triangle_count;
+    }
+    std::vector<GpuNode> tlas;
+    std::vector<GpuNodeQ8> qtlas;
+    std::vector<TptQbvhNode> tn(desc->tlas_nodes, desc->tlas_nodes + desc->tlas_node_count);
+    for (uint32_t k = 0; k < desc->tlas_node_count; ++k) {
+      tlas.push_back(to_gpu_node(tn[k], 0));
+      GpuNodeQ8 qn;
+      quantize_node(tlas.back(), &qn);
+      qtlas.push_back(qn);
+    }
+    std::vector<uint32_t> tprim(desc->tlas_primitive_indices, desc->tlas_primitive_indices + desc->instance_count);
+    for (uint32_t v : tprim)
+      if (v >= desc->instance_count) {
+        set_error("tpt_scene_create: TLAS primitive index out of range");
+        return nullptr;
+      }
+    const int tlas_stack = stack_bound(tn, 0, 0) + 1;
+    if (tlas_stack + blas_stack > kStackDepth) {
+      set_error("tpt_scene_create: tree needs more than 128 traversal stack entries (R/qbvh.rs:609)");
+      return nullptr;
+    }
+    s = new TptScene();
+    s->dev = dev;
+    s->instance_count = desc->instance_count;
+    s->tlas_node_count = desc->tlas_node_count;
+    if (!upload(nodes, &s->nodes, &s->bytes) || !upload(qnodes, &s->qnodes, &s->bytes) ||
+        !upload(tris, &s->tris, &s->bytes) || !upload(rank, &s->tri_rank, &s->bytes) ||
+        !upload(inst, &s->inst, &s->bytes) || !upload(tlas, &s->tlas, &s->bytes) ||
+        !upload(qtlas, &s->qtlas, &s->bytes) || !upload(tprim, &s->tlas_prim, &s->bytes)) {
+      free_scene(s);
+      return nullptr;
+    }
+    return s;
+  } catch (const std::exception& e) {
+    set_error(std::string("tpt_scene_create: ") + e.what());
+    free_scene(s);
+    return nullptr;
+  }
+}
+
+void tpt_scene_release(TptScene* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->dev->index);
+  (void)hipStreamSynchronize(s->dev->stream);
+  free_scene(s);
+}
+
+size_t tpt_scene_device_bytes(const TptScene* s) { return s ? s->bytes : 0; }
+
+}  // extern "C"
+
+namespace tpt {
+namespace {
+
+// One implementation behind tpt_trace_closest / tpt_trace_any /
+// tpt_trace_counts.  query 0: closest (hits), 1: shadow (unoccluded).
+int32_t trace_impl(TptScene* s, int query, const TptRay* rays, size_t n, const uint32_t* target,
+                   const float* distance, TptHit* hits, uint8_t* unocc, uint32_t* counts, uint32_t flags,
+                   const char* fn) {
+  if (!s) {
+    set_error(std::string(fn) + ": null scene");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  if (n == 0) return TPT_OK;
+  if (!rays || (query == 0 && !hits && !counts) || (query == 1 && !unocc && !counts)) {
+    set_error(std::string(fn) + ": null rays or output");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  if (n > 0x7FFFFFFFull) {
+    set_error(std::string(fn) + ": too many rays for one call");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  TptDevice* d = s->dev;
+  if (!hip_ok(hipSetDevice(d->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+  const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
+  const bool q8 = flags & TPT_FLAG_QUANTIZED;
+  const bool exact = flags & TPT_FLAG_ANY_EXACT;
+  const TptRay* drays = rays;
+  const uint32_t* dtarget = target;
+  const float* ddist = distance;
+  TptHit* dhits = hits;
+  uint8_t* dout = unocc;
+  uint32_t* dcounts = counts;
+  if (!dev_ptr) {
+    // one staging block: rays | target | distance | output | counts
+    const size_t nb_r = n * sizeof(TptRay);
+    const size_t nb_t = (query == 1 && target) ? n * 8 : 0;
+    const size_t nb_d = (query == 1 && distance) ? n * 4 : 0;
+    const size_t nb_o = ((query == 0 ? n * sizeof(TptHit) : n) + 15) & ~size_t(15);
+    const size_t nb_c = counts ? n * 12 : 0;
+    if (!ensure_stage(s, nb_r + nb_t + nb_d + nb_o + nb_c)) return TPT_ERROR_DEVICE;
+    char* p = (char*)s->stage;
+    drays = (const TptRay*)p;
+    dtarget = nb_t ? (const uint32_t*)(p + nb_r) : nullptr;
+    ddist = nb_d ? (const float*)(p + nb_r + nb_t) : nullptr;
+    char* po = p + nb_r + nb_t + nb_d;
+    dhits = query == 0 ? (TptHit*)po : nullptr;
+    dout = query == 1 ? (uint8_t*)po : nullptr;
+    dcounts = counts ? (uint32_t*)(po + nb_o) : nullptr;
+    if (!hip_ok(hipMemcpyAsync((void*)drays, rays, nb_r, hipMemcpyHostToDevice, d->stream), "hipMemcpyAsync(rays)"))
+      return TPT_ERROR_DEVICE;
+    if (nb_t && !hip_ok(hipMemcpyAsync((void*)dtarget, target, nb_t, hipMemcpyHostToDevice, d->stream),
+                        "hipMemcpyAsync(target)"))
+      return TPT_ERROR_DEVICE;
+    if (nb_d && !hip_ok(hipMemcpyAsync((void*)ddist, distance, nb_d, hipMemcpyHostToDevice, d->stream),
+                        "hipMemcpyAsync(distance)"))
+      return TPT_ERROR_DEVICE;
+    if (!hip_ok(hipMemsetAsync(d->status, 0, 4, d->stream), "hipMemsetAsync(status)")) return TPT_ERROR_DEVICE;
+  }
+  // Outputs the caller did not ask for still need a destination on device.
+  if (query == 0 && !dhits) {
+    if (!ensure_stage(s, n * sizeof(TptHit))) return TPT_ERROR_DEVICE;
+    dhits = (TptHit*)s->stage;
+  }
+  if (query == 1 && !dout) {
+    if (!ensure_stage(s, n)) return TPT_ERROR_DEVICE;
+    dout = (uint8_t*)s->stage;
+  }
+  const SceneView v = view_of(s);
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
+  const uint32_t nn = (uint32_t)n;
+  hipStream_t st = d->stream;
+  if (query == 0) {
+    if (q8) {
+      if (dcounts) hipLaunchKernelGGL((closest_kernel<true, true>), grid, block, 0, st, v, drays, dhits, nn, d->status, dcounts);
+      else hipLaunchKernelGGL((closest_kernel<true, false>), grid, block, 0, st, v, drays, dhits, nn, d->status, dcounts);
+    } else {
+      if (dcounts) hipLaunchKernelGGL((closest_kernel<false, true>), grid, block, 0, st, v, drays, dhits, nn, d->status, dcounts);
+      else hipLaunchKernelGGL((closest_kernel<false, false>), grid, block, 0, st, v, drays, dhits, nn, d->status, dcounts);
+    }
+  } else {
+#define TPT_ANY(Q, E, C) \
+  hipLaunchKernelGGL((any_kernel<Q, E, C>), grid, block, 0, st, v, drays, dtarget, ddist, dout, nn, d->status, dcounts)
+    if (q8 && exact) {
+      if (dcounts) TPT_ANY(true, true, true); else TPT_ANY(true, true, false);
+    } else if (q8) {
+      if (dcounts) TPT_ANY(true, false, true); else TPT_ANY(true, false, false);
+    } else if (exact) {
+      if (dcounts) TPT_ANY(false, true, true); else TPT_ANY(false, true, false);
+    } else {
+      if (dcounts) TPT_ANY(false, false, true); else TPT_ANY(false, false, false);
+    }
+#undef TPT_ANY
+  }
+  if (!hip_ok(hipGetLastError(), "kernel launch")) return TPT_ERROR_DEVICE;
+  if (!dev_ptr) {
+    if (query == 0 && hits &&
+        !hip_ok(hipMemcpyAsync(hits, dhits, n * sizeof(TptHit), hipMemcpyDeviceToHost, st), "hipMemcpyAsync(hits)"))
+      return TPT_ERROR_DEVICE;
+    if (query == 1 && unocc && !hip_ok(hipMemcpyAsync(unocc, dout, n, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(out)"))
+      return TPT_ERROR_DEVICE;
+    if (counts && !hip_ok(hipMemcpyAsync(counts, dcounts, n * 12, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(counts)"))
+      return TPT_ERROR_DEVICE;
+    uint32_t status = 0;
+    if (!read_status(d, &status)) return TPT_ERROR_DEVICE;
+    if (status & 1u) {
+      set_error(std::string(fn) + ": traversal stack overflow (128 entries)");
+      return TPT_ERROR_OVERFLOW;
+    }
+  }
+  return TPT_OK;
+}
+
+}  // namespace
+}  // namespace tpt
+
+extern "C" {
+
+int32_t tpt_trace_closest(TptScene* s, const TptRay* rays, size_t n, TptHit* hits, uint32_t flags) {
+  try {
+    if (!hits && n) {
+      set_error("tpt_trace_closest: null hits");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    return trace_impl(s, 0, rays, n, nullptr, nullptr, hits, nullptr, nullptr, flags, "tpt_trace_closest");
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return TPT_ERROR;
+  }
+}
+
+int32_t tpt_trace_any(TptScene* s, const TptRay* rays, size_t n, const uint32_t* target, const float* distance,
+                      uint8_t* out, uint32_t flags) {
+  try {
+    if (!out && n) {
+      set_error("tpt_trace_any: null output");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    return trace_impl(s, 1, rays, n, target, distance, nullptr, out, nullptr, flags, "tpt_trace_any");
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return TPT_ERROR;
+  }
+}
+
+int32_t tpt_trace_counts(TptScene* s, uint32_t query, const TptRay* rays, size_t n, const uint32_t* target,
+                         const float* distance, uint32_t* counts, uint32_t flags) {
+  try {
+    if (query > 1 || (!counts && n)) {
+      set_error("tpt_trace_counts: query must be 0 or 1 and counts non-null");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    return trace_impl(s, (int)query, rays, n, target, distance, nullptr, nullptr, counts, flags, "tpt_trace_counts");
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return TPT_ERROR;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+"""ctypes binding of libtpt.so (the C ABI declared in include/tpt.h).
+
+The shared library is built in-tree (``toy-path-tracing_amd/libtpt.so``) by
+``__graft_entry__.build()``.  Loading fails loudly when it is missing: there
+is no CPU fallback for any query.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "libtpt.so"
+
+TPT_OK = 0
+TPT_ERROR = 1
+TPT_ERROR_NOT_BUILT = 2
+TPT_ERROR_BAD_ARGUMENT = 3
+TPT_ERROR_DEVICE = 4
+TPT_ERROR_OVERFLOW = 5
+
+TPT_FLAG_DEVICE_POINTERS = 0x1
+TPT_FLAG_QUANTIZED = 0x2
+TPT_FLAG_ANY_EXACT = 0x4
+
+INVALID_INDEX = 0xFFFFFFFF
+
+# numpy mirrors of the ABI records --------------------------------------------
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("pad", "<f4")])
+HIT_DTYPE = np.dtype([("instance", "<u4"), ("triangle", "<u4"), ("t", "<f4"), ("b", "<f4", 3)])
+NODE_DTYPE = np.dtype(
+    [
+        ("min_x", "<f4", 4), ("min_y", "<f4", 4), ("min_z", "<f4", 4),
+        ("max_x", "<f4", 4), ("max_y", "<f4", 4), ("max_z", "<f4", 4),
+        ("children", "<u4", 4), ("valid_mask", "u1"), ("pad", "u1", 15),
+    ]
+)
+Q8_NODE_DTYPE = np.dtype(
+    [
+        ("origin", "<f4", 3), ("exp", "i1", 3), ("flags", "u1"), ("q", "u1", 24),
+        ("pad", "<u4", 2), ("child", "<u4", 4),
+    ]
+)
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 24
+assert NODE_DTYPE.itemsize == 128 and Q8_NODE_DTYPE.itemsize == 64
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.c_void_p),
+        ("vertex_count", ctypes.c_uint32),
+        ("triangle_count", ctypes.c_uint32),
+        ("indices", ctypes.c_void_p),
+        ("nodes", ctypes.c_void_p),
+        ("node_count", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("primitive_indices", ctypes.c_void_p),
+    ]
+
+
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [
+        ("mesh", ctypes.c_uint32),
+        ("material", ctypes.c_uint32),
+        ("local_to_world", ctypes.c_float * 16),
+        ("world_to_local", ctypes.c_float * 16),
+    ]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("meshes", ctypes.c_void_p),
+        ("mesh_count", ctypes.c_uint32),
+        ("instance_count", ctypes.c_uint32),
+        ("instances", ctypes.c_void_p),
+        ("tlas_nodes", ctypes.c_void_p),
+        ("tlas_node_count", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("tlas_primitive_indices", ctypes.c_void_p),
+    ]
+
+
+assert ctypes.sizeof(MeshDesc) == 48 and ctypes.sizeof(SceneDesc) == 48
+assert ctypes.sizeof(InstanceDesc) == 136
+
+
+class TptError(RuntimeError):
+    """A non-zero status from libtpt, carrying tpt_last_error()."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"libtpt status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+# (name, restype, argtypes) for every symbol of include/tpt.h
+_SIGNATURES = [
+    ("tpt_last_error", ctypes.c_char_p, []),
+    ("tpt_device_create", ctypes.c_void_p, [ctypes.c_int32]),
+    ("tpt_device_release", None, [ctypes.c_void_p]),
+    ("tpt_device_set_stream", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("tpt_device_synchronize", ctypes.c_int32, [ctypes.c_void_p]),
+    ("tpt_qbvh_build", ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
+    ("tpt_qbvh_release", None, [ctypes.c_void_p]),
+    ("tpt_qbvh_node_count", ctypes.c_size_t, [ctypes.c_void_p]),
+    ("tpt_qbvh_primitive_count", ctypes.c_size_t, [ctypes.c_void_p]),
+    ("tpt_qbvh_export", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("tpt_qbvh_quantize", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("tpt_scene_create", ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("tpt_scene_release", None, [ctypes.c_void_p]),
+    ("tpt_scene_device_bytes", ctypes.c_size_t, [ctypes.c_void_p]),
+    ("tpt_trace_closest", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_trace_any", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_uint32]),
+    ("tpt_trace_counts", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_uint32]),
+]
+
+
+def lib() -> ctypes.CDLL:
+    """Load libtpt.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "(there is no CPU fallback)"
+            )
+        handle = ctypes.CDLL(os.fspath(LIB_PATH))
+        for name, res, args in _SIGNATURES:
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    return lib().tpt_last_error().decode("utf-8", "replace")
+
+
+def check(status: int) -> None:
+    if status != TPT_OK:
+        raise TptError(status, last_error())
+
+
+def ptr(a) -> int | None:
+    """Address of a numpy array or torch tensor (None for None)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor

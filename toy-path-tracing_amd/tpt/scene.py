@@ -1,0 +1,338 @@
+"""Host-side mirror of the reference scene API for the ray-query path.
+
+Mirrors the geometry part of ``Scene`` (crates/renderer/src/scene/mod.rs):
+``add_mesh``, ``add_instance`` (R/scene/mod.rs:219-252), ``build_qbvh``
+(R/scene/mod.rs:305-316) and ``closest_hit`` (R/scene/mod.rs:340-395, here
+batched), plus the shadow query ``unoccluded`` (R/integrator/mod.rs:239-265).
+Queries run on the GPU through libtpt; the QBVH build runs in libtpt's host
+builder (R/qbvh.rs:144-516).  Matrices are given in the usual math
+convention (column vectors, ``p' = M @ p``) and handed to the ABI in glam's
+column-major order.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from enum import Enum
+
+import numpy as np
+
+from . import _lib
+from ._lib import HIT_DTYPE, INVALID_INDEX, NODE_DTYPE, Q8_NODE_DTYPE, RAY_DTYPE, check, lib, ptr
+
+
+class ClosestHitError(Exception):
+    """ClosestHitError (R/scene/mod.rs:100-113)."""
+
+    class Kind(Enum):
+        QbvhNotBuilt = "scene QBVH has not been built yet"
+
+    def __init__(self, kind: "ClosestHitError.Kind"):
+        super().__init__(kind.value)
+        self.kind = kind
+
+
+@dataclass
+class Qbvh:
+    """Qbvh (R/qbvh.rs:129-133): nodes in the reference layout + primitive_indices."""
+
+    nodes: np.ndarray  # NODE_DTYPE
+    primitive_indices: np.ndarray  # uint32
+
+
+def build_qbvh(bounds_min: np.ndarray, bounds_max: np.ndarray, threads: int = 0) -> Qbvh | None:
+    """build_qbvh (R/qbvh.rs:144-166): None for no primitives."""
+    bmin = np.ascontiguousarray(bounds_min, dtype=np.float32).reshape(-1, 3)
+    bmax = np.ascontiguousarray(bounds_max, dtype=np.float32).reshape(-1, 3)
+    n = bmin.shape[0]
+    if n == 0:
+        return None
+    h = lib().tpt_qbvh_build(ptr(bmin), ptr(bmax), n, threads)
+    if not h:
+        raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
+    try:
+        nn = lib().tpt_qbvh_node_count(h)
+        nodes = np.zeros(nn, dtype=NODE_DTYPE)
+        prim = np.zeros(n, dtype=np.uint32)
+        check(lib().tpt_qbvh_export(h, ptr(nodes) if nn else None, ptr(prim)))
+    finally:
+        lib().tpt_qbvh_release(h)
+    return Qbvh(nodes, prim)
+
+
+def quantize_nodes(nodes: np.ndarray) -> np.ndarray:
+    """8-bit quantised copies of reference-layout nodes, as traversed on the GPU."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    out = np.zeros(len(nodes), dtype=Q8_NODE_DTYPE)
+    if len(nodes):
+        check(lib().tpt_qbvh_quantize(ptr(nodes), len(nodes), ptr(out)))
+    return out
+
+
+@dataclass
+class Mesh:
+    """Mesh geometry (R/scene/mesh.rs:51-57): positions (V,3) and indices (T,3)."""
+
+    positions: np.ndarray
+    indices: np.ndarray
+    normals: np.ndarray | None = None
+    uvs: np.ndarray | None = None
+    qbvh: Qbvh | None = None
+
+    def __post_init__(self):
+        self.positions = np.ascontiguousarray(self.positions, dtype=np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(self.indices, dtype=np.uint32).reshape(-1, 3)
+        if len(self.positions) == 0:
+            raise ValueError("mesh must contain at least one vertex")  # Mesh::new expect
+
+    @property
+    def bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        """compute_bounds (R/scene/mesh.rs:668-680)."""
+        return self.positions.min(axis=0), self.positions.max(axis=0)
+
+    def triangle_count(self) -> int:
+        return len(self.indices)
+
+    def triangle_bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        """Mesh::triangle_bounds (R/scene/mesh.rs:104-111), all triangles."""
+        v = self.positions[self.indices]  # (T,3,3)
+        lo = np.fmin(np.fmin(v[:, 0], v[:, 1]), v[:, 2])
+        hi = np.fmax(np.fmax(v[:, 0], v[:, 1]), v[:, 2])
+        return lo, hi
+
+    def build_qbvh(self, threads: int = 0) -> None:
+        """Mesh::build_qbvh (R/scene/mesh.rs:114-119)."""
+        lo, hi = self.triangle_bounds()
+        self.qbvh = build_qbvh(lo, hi, threads)
+
+
+def glam_cols(m: np.ndarray) -> np.ndarray:
+    """A 4x4 math-convention matrix as glam's column-major 16 floats."""
+    return np.ascontiguousarray(np.asarray(m, dtype=np.float32).T).reshape(16)
+
+
+def transform_point3(m: np.ndarray, p: np.ndarray) -> np.ndarray:
+    """glam Mat4::transform_point3 in f32: ((c0*x + c1*y) + c2*z) + c3."""
+    m = np.asarray(m, dtype=np.float32)
+    p = np.asarray(p, dtype=np.float32)
+    r = m[:3, 0] * p[..., 0:1]
+    r = m[:3, 1] * p[..., 1:2] + r
+    r = m[:3, 2] * p[..., 2:3] + r
+    return m[:3, 3] + r
+
+
+def transform_bounds(lo: np.ndarray, hi: np.ndarray, m: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """transform_bounds (R/scene/mod.rs:869-893): box of the 8 transformed corners."""
+    corners = np.array(
+        [[lo[0], lo[1], lo[2]], [lo[0], lo[1], hi[2]], [lo[0], hi[1], lo[2]], [lo[0], hi[1], hi[2]],
+         [hi[0], lo[1], lo[2]], [hi[0], lo[1], hi[2]], [hi[0], hi[1], lo[2]], [hi[0], hi[1], hi[2]]],
+        dtype=np.float32,
+    )
+    t = transform_point3(m, corners)
+    return np.fmin.reduce(t, axis=0), np.fmax.reduce(t, axis=0)
+
+
+@dataclass
+class Instance:
+    """Instance (R/scene/mod.rs:62-70)."""
+
+    mesh_index: int
+    material_index: int
+    local_to_world: np.ndarray
+    world_to_local: np.ndarray
+    world_bounds: tuple[np.ndarray, np.ndarray]
+
+
+@dataclass
+class Scene:
+    """Geometry of the reference Scene (R/scene/mod.rs:81-98)."""
+
+    meshes: list[Mesh] = field(default_factory=list)
+    instances: list[Instance] = field(default_factory=list)
+    qbvh: Qbvh | None = None
+    built: bool = False
+
+    def add_mesh(self, mesh: Mesh) -> int:
+        self.meshes.append(mesh)
+        return len(self.meshes) - 1
+
+    def add_instance(self, mesh_index: int, material_index: int, local_to_world: np.ndarray,
+                     world_to_local: np.ndarray | None = None) -> int:
+        """Scene::add_instance (R/scene/mod.rs:219-252).  The inverse is taken
+        in f64 and rounded to f32 unless the caller passes it (the renderer's
+        glam inverse crosses the ABI as data)."""
+        l2w = np.asarray(local_to_world, dtype=np.float32)
+        if world_to_local is None:
+            world_to_local = np.linalg.inv(l2w.astype(np.float64)).astype(np.float32)
+        lo, hi = self.meshes[mesh_index].bounds
+        wb = transform_bounds(lo, hi, l2w)
+        self.instances.append(Instance(mesh_index, material_index, l2w, np.asarray(world_to_local, np.float32), wb))
+        self.qbvh = None
+        self.built = False
+        return len(self.instances) - 1
+
+    def build_qbvh(self, threads: int = 0) -> None:
+        """Scene::build_qbvh (R/scene/mod.rs:305-316)."""
+        for m in self.meshes:
+            m.build_qbvh(threads)
+        if self.instances:
+            lo = np.stack([i.world_bounds[0] for i in self.instances])
+            hi = np.stack([i.world_bounds[1] for i in self.instances])
+            self.qbvh = build_qbvh(lo, hi, threads)
+        else:
+            self.qbvh = None
+        self.built = True
+
+    def upload(self, device: "Device") -> "DeviceScene":
+        if not self.built:
+            raise ClosestHitError(ClosestHitError.Kind.QbvhNotBuilt)
+        return DeviceScene(device, self)
+
+
+class Device:
+    """A GPU (tpt_device_create); trace calls go on its stream."""
+
+    def __init__(self, index: int = 0):
+        self.handle = lib().tpt_device_create(index)
+        if not self.handle:
+            raise _lib.TptError(_lib.TPT_ERROR_DEVICE, _lib.last_error())
+        self.index = index
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        check(lib().tpt_device_set_stream(self.handle, stream_ptr))
+
+    def synchronize(self) -> None:
+        check(lib().tpt_device_synchronize(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().tpt_device_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceScene:
+    """A scene resident in device memory (tpt_scene_create)."""
+
+    def __init__(self, device: Device, scene: Scene):
+        self.device = device
+        keep = []  # arrays referenced by the descriptor during the call
+        meshes = (_lib.MeshDesc * max(1, len(scene.meshes)))()
+        for i, m in enumerate(scene.meshes):
+            q = m.qbvh
+            d = meshes[i]
+            d.positions = ptr(m.positions)
+            d.vertex_count = len(m.positions)
+            d.triangle_count = len(m.indices)
+            d.indices = ptr(m.indices)
+            if q is not None:
+                d.nodes = ptr(q.nodes) if len(q.nodes) else None
+                d.node_count = len(q.nodes)
+                d.primitive_indices = ptr(q.primitive_indices)
+                keep.append(q)
+            else:
+                d.node_count = 0
+                d.primitive_indices = None
+        insts = (_lib.InstanceDesc * max(1, len(scene.instances)))()
+        for i, inst in enumerate(scene.instances):
+            insts[i].mesh = inst.mesh_index
+            insts[i].material = inst.material_index
+            insts[i].local_to_world[:] = glam_cols(inst.local_to_world).tolist()
+            insts[i].world_to_local[:] = glam_cols(inst.world_to_local).tolist()
+        desc = _lib.SceneDesc()
+        desc.meshes = ctypes.addressof(meshes)
+        desc.mesh_count = len(scene.meshes)
+        desc.instance_count = len(scene.instances)
+        desc.instances = ctypes.addressof(insts)
+        if scene.qbvh is not None:
+            desc.tlas_nodes = ptr(scene.qbvh.nodes) if len(scene.qbvh.nodes) else None
+            desc.tlas_node_count = len(scene.qbvh.nodes)
+            desc.tlas_primitive_indices = ptr(scene.qbvh.primitive_indices)
+        self.handle = lib().tpt_scene_create(device.handle, ctypes.byref(desc))
+        if not self.handle:
+            raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
+        self.instance_count = len(scene.instances)
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().tpt_scene_device_bytes(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().tpt_scene_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # host-array queries (synchronous) ------------------------------------------
+    def closest_hit(self, rays: np.ndarray, quantized: bool = False) -> np.ndarray:
+        """Scene::closest_hit for a batch of RAY_DTYPE rays -> HIT_DTYPE hits."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        flags = _lib.TPT_FLAG_QUANTIZED if quantized else 0
+        check(lib().tpt_trace_closest(self.handle, ptr(rays), len(rays), ptr(hits), flags))
+        return hits
+
+    def unoccluded(self, rays: np.ndarray, target: np.ndarray | None = None, distance: np.ndarray | None = None,
+                   quantized: bool = False, exact: bool = False) -> np.ndarray:
+        """unoccluded_ray for shadow rays -> bool array."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        out = np.zeros(len(rays), dtype=np.uint8)
+        if target is not None:
+            target = np.ascontiguousarray(target, dtype=np.uint32).reshape(-1, 2)
+        if distance is not None:
+            distance = np.ascontiguousarray(distance, dtype=np.float32)
+        flags = (_lib.TPT_FLAG_QUANTIZED if quantized else 0) | (_lib.TPT_FLAG_ANY_EXACT if exact else 0)
+        check(lib().tpt_trace_any(self.handle, ptr(rays), len(rays), ptr(target), ptr(distance), ptr(out), flags))
+        return out.astype(bool)
+
+    def counts(self, rays: np.ndarray, query: str = "closest", target: np.ndarray | None = None,
+               distance: np.ndarray | None = None, quantized: bool = False, exact: bool = False) -> np.ndarray:
+        """Per-ray (nodes visited, triangles tested, instances entered) of a query."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        out = np.zeros((len(rays), 3), dtype=np.uint32)
+        if target is not None:
+            target = np.ascontiguousarray(target, dtype=np.uint32).reshape(-1, 2)
+        if distance is not None:
+            distance = np.ascontiguousarray(distance, dtype=np.float32)
+        flags = (_lib.TPT_FLAG_QUANTIZED if quantized else 0) | (_lib.TPT_FLAG_ANY_EXACT if exact else 0)
+        q = {"closest": 0, "any": 1}[query]
+        check(lib().tpt_trace_counts(self.handle, q, ptr(rays), len(rays), ptr(target), ptr(distance), ptr(out),
+                                     flags))
+        return out
+
+    # device-pointer queries (asynchronous on the device stream) -----------------
+    def closest_hit_device(self, rays_dev, hits_dev, n: int, quantized: bool = False) -> None:
+        flags = _lib.TPT_FLAG_DEVICE_POINTERS | (_lib.TPT_FLAG_QUANTIZED if quantized else 0)
+        check(lib().tpt_trace_closest(self.handle, ptr(rays_dev), n, ptr(hits_dev), flags))
+
+    def unoccluded_device(self, rays_dev, target_dev, distance_dev, out_dev, n: int, quantized: bool = False,
+                          exact: bool = False) -> None:
+        flags = (_lib.TPT_FLAG_DEVICE_POINTERS | (_lib.TPT_FLAG_QUANTIZED if quantized else 0)
+                 | (_lib.TPT_FLAG_ANY_EXACT if exact else 0))
+        check(lib().tpt_trace_any(self.handle, ptr(rays_dev), n, ptr(target_dev), ptr(distance_dev), ptr(out_dev),
+                                  flags))
+
+
+def make_rays(origins: np.ndarray, directions: np.ndarray, tmax: float | np.ndarray = np.inf) -> np.ndarray:
+    r = np.zeros(len(origins), dtype=RAY_DTYPE)
+    r["o"] = origins
+    r["d"] = directions
+    r["tmax"] = tmax
+    return r
+
+
+__all__ = [
+    "ClosestHitError", "Device", "DeviceScene", "Instance", "Mesh", "Qbvh", "Scene", "build_qbvh", "glam_cols",
+    "make_rays", "quantize_nodes", "transform_bounds", "transform_point3", "HIT_DTYPE", "RAY_DTYPE",
+    "NODE_DTYPE", "Q8_NODE_DTYPE", "INVALID_INDEX",
+]

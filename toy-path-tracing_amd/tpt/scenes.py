@@ -1,0 +1,287 @@
+"""Seeded procedural scenes and ray sets for the ray-query workloads.
+
+The reference scenes load Git-LFS assets that are not available, so every
+workload here is generated from a seed (numpy PCG64), in float64 and rounded
+to float32 once.  Generated data is an input: the GPU path and the oracle see
+the same arrays.
+
+Workloads
+---------
+* ``cornell``: the Cornell box of the plain-method configuration — five walls
+  of [0,1]^3 open at z=0, two rotated boxes (0.3^3 at -18 deg, 0.3x0.6x0.3 at
+  +15 deg) and a 0.2x0.2 ceiling quad light; 36 triangles in 6 instances.
+* ``sphere_field``: the traversal microbenchmark mesh — level-4 icospheres
+  (5120 triangles each), random centre / log-uniform radius / random rotation,
+  per-vertex radial noise of 5% of the radius, flattened into one mesh.
+  410 spheres give 2,099,200 triangles (the 2M-triangle workload; 4096
+  spheres would exceed the 2^24 per-mesh leaf-offset cap of R/qbvh.rs:15-18).
+* ``primary_rays`` / ``bounce_rays`` / ``shadow_rays``: 1920x1080 pinhole
+  rays, cosine-hemisphere bounce rays from primary hits, and shadow rays to
+  uniform points on 64 random "emissive" triangles.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .scene import Mesh, Scene, make_rays
+
+RAY_EPSILON = np.float32(1.0e-4)  # R/integrator/mod.rs:49
+
+
+# --------------------------------------------------------------------------- geometry
+def icosphere(level: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere by repeated 4-way midpoint subdivision (float64, int64)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array(
+        [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array(
+        [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        nf = len(f)
+        m01 = base + inv[:nf]
+        m12 = base + inv[nf:2 * nf]
+        m20 = base + inv[2 * nf:]
+        f = np.concatenate([
+            np.stack([f[:, 0], m01, m20], 1), np.stack([f[:, 1], m12, m01], 1),
+            np.stack([f[:, 2], m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return v, f
+
+
+def random_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], 1)
+
+
+@dataclass
+class SphereField:
+    mesh: Mesh
+    centers: np.ndarray
+    radii: np.ndarray
+
+
+def sphere_field(n_spheres: int = 410, level: int = 4, seed: int = 0, radius=(1.0, 6.0), cube: float = 100.0,
+                 noise: float = 0.05) -> SphereField:
+    """The traversal microbenchmark mesh (one flattened mesh)."""
+    rng = np.random.default_rng(seed)
+    uv, uf = icosphere(level)
+    radii = np.exp(rng.uniform(math.log(radius[0]), math.log(radius[1]), n_spheres))
+    centers = rng.uniform(0.0, 1.0, (n_spheres, 3)) * (cube - 2 * radii[:, None]) + radii[:, None]
+    rots = random_rotations(rng, n_spheres)
+    disp = 1.0 + rng.uniform(-noise, noise, (n_spheres, len(uv)))
+    local = uv[None, :, :] * disp[:, :, None]  # (n, V, 3)
+    pos = np.einsum("nij,nvj->nvi", rots, local) * radii[:, None, None] + centers[:, None, :]
+    idx = uf[None, :, :] + (np.arange(n_spheres) * len(uv))[:, None, None]
+    mesh = Mesh(pos.reshape(-1, 3).astype(np.float32), idx.reshape(-1, 3).astype(np.uint32))
+    return SphereField(mesh, centers, radii)
+
+
+def single_mesh_scene(mesh: Mesh, build: bool = True, threads: int = 0) -> Scene:
+    """A flattened scene: one mesh, one identity instance."""
+    s = Scene()
+    mi = s.add_mesh(mesh)
+    s.add_instance(mi, 0, np.eye(4, dtype=np.float32))
+    if build:
+        s.build_qbvh(threads)
+    return s
+
+
+def _quad(p0, p1, p2, p3) -> tuple[np.ndarray, np.ndarray]:
+    return np.array([p0, p1, p2, p3], dtype=np.float64), np.array([[0, 1, 2], [0, 2, 3]])
+
+
+def _rot_y(deg: float) -> np.ndarray:
+    a = math.radians(deg)
+    m = np.eye(4)
+    m[0, 0] = math.cos(a)
+    m[0, 2] = math.sin(a)
+    m[2, 0] = -math.sin(a)
+    m[2, 2] = math.cos(a)
+    return m
+
+
+def _translate(x, y, z) -> np.ndarray:
+    m = np.eye(4)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def _scale(x, y, z) -> np.ndarray:
+    return np.diag([x, y, z, 1.0])
+
+
+def unit_box() -> Mesh:
+    """Axis-aligned box [-0.5,0.5]^3, 12 outward-facing triangles."""
+    c = np.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)])
+    faces = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    tris = []
+    for a, b, cc, d in faces:
+        tris += [[a, b, cc], [a, cc, d]]
+    return Mesh(c.astype(np.float32), np.array(tris, dtype=np.uint32))
+
+
+@dataclass
+class CornellScene:
+    scene: Scene
+    # instance -> material slot: 0 white, 1 red, 2 green, 3 light
+    instance_material: list[int]
+    light_instance: int
+
+
+def cornell(build: bool = True) -> CornellScene:
+    """Cornell box: 36 triangles, 6 instances (white walls, red, green, two boxes, light)."""
+    s = Scene()
+    # white: floor (y=0), ceiling (y=1), back wall (z=1)
+    wv = np.array([[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1],
+                   [0, 1, 0], [0, 1, 1], [1, 1, 1], [1, 1, 0],
+                   [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=np.float32)
+    wi = np.array([[0, 2, 1], [0, 3, 2], [4, 6, 5], [4, 7, 6], [8, 10, 9], [8, 11, 10]], dtype=np.uint32)
+    white = s.add_mesh(Mesh(wv, wi))
+    rv, ri = _quad([0, 0, 0], [0, 0, 1], [0, 1, 1], [0, 1, 0])  # left wall x=0
+    red = s.add_mesh(Mesh(rv.astype(np.float32), ri))
+    gv, gi = _quad([1, 0, 0], [1, 1, 0], [1, 1, 1], [1, 0, 1])  # right wall x=1
+    green = s.add_mesh(Mesh(gv.astype(np.float32), gi))
+    box = s.add_mesh(unit_box())
+    lv, li = _quad([0.4, 0.999, 0.4], [0.6, 0.999, 0.4], [0.6, 0.999, 0.6], [0.4, 0.999, 0.6])  # faces -Y
+    light = s.add_mesh(Mesh(lv.astype(np.float32), li))
+    eye4 = np.eye(4)
+    s.add_instance(white, 0, eye4)
+    s.add_instance(red, 1, eye4)
+    s.add_instance(green, 2, eye4)
+    short = _translate(0.68, 0.15, 0.36) @ _rot_y(-18.0) @ _scale(0.3, 0.3, 0.3)
+    tall = _translate(0.33, 0.3, 0.64) @ _rot_y(15.0) @ _scale(0.3, 0.6, 0.3)
+    s.add_instance(box, 0, short)
+    s.add_instance(box, 0, tall)
+    li_idx = s.add_instance(light, 3, eye4)
+    if build:
+        s.build_qbvh()
+    return CornellScene(s, [0, 1, 2, 0, 0, 3], li_idx)
+
+
+# --------------------------------------------------------------------------- rays
+def primary_rays(width: int, height: int, eye, look_at, up, fov_y_deg: float) -> np.ndarray:
+    """Pixel-centre pinhole rays (direction_for_sample, R/scene/camera.rs:80-87), row-major."""
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(look_at, np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(up, np.float64))
+    right /= np.linalg.norm(right)
+    iup = np.cross(right, fwd)
+    iup /= np.linalg.norm(iup)
+    th = math.tan(0.5 * math.radians(fov_y_deg))
+    aspect = width / height
+    px = np.arange(width, dtype=np.float64) + 0.5
+    py = np.arange(height, dtype=np.float64) + 0.5
+    sx = ((2.0 * px / width) - 1.0) * aspect * th
+    sy = (1.0 - (2.0 * py / height)) * th
+    d = fwd[None, None, :] + sx[None, :, None] * right[None, None, :] + sy[:, None, None] * iup[None, None, :]
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    d = d.reshape(-1, 3)
+    o = np.broadcast_to(eye, d.shape)
+    return make_rays(o.astype(np.float32), d.astype(np.float32))
+
+
+def triangle_normals(mesh: Mesh, tri: np.ndarray) -> np.ndarray:
+    v = mesh.positions[mesh.indices[tri]].astype(np.float64)
+    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+def spawn_origins(p: np.ndarray, ng: np.ndarray, wi: np.ndarray) -> np.ndarray:
+    """offset_ray_origin (R/integrator/mod.rs:76-85) in f32."""
+    p = p.astype(np.float32)
+    ng = ng.astype(np.float32)
+    wi = wi.astype(np.float32)
+    dot = wi[:, 0] * ng[:, 0] + wi[:, 1] * ng[:, 1] + wi[:, 2] * ng[:, 2]
+    off = np.where((dot >= 0)[:, None], ng, -ng)
+    return p + RAY_EPSILON * off
+
+
+def _hit_points(rays: np.ndarray, hits: np.ndarray, mesh: Mesh):
+    ok = np.nonzero(hits["triangle"] != 0xFFFFFFFF)[0]
+    if len(ok) == 0:
+        raise ValueError("no primary ray hits the scene")
+    o = rays["o"][ok]
+    d = rays["d"][ok]
+    t = hits["t"][ok].astype(np.float32)
+    p = o + t[:, None] * d  # Ray::at
+    ng = triangle_normals(mesh, hits["triangle"][ok])
+    ng = np.where((np.sum(ng * d, axis=1) > 0)[:, None], -ng, ng)  # face the incoming ray
+    return p, ng
+
+
+def _onb(n: np.ndarray):
+    a = np.where((np.abs(n[:, 0]) > 0.9)[:, None], np.array([0.0, 1.0, 0.0]), np.array([1.0, 0.0, 0.0]))
+    t = np.cross(n, a)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(n, t)
+    return t, b
+
+
+def bounce_rays(rays: np.ndarray, hits: np.ndarray, mesh: Mesh, n: int, seed: int = 1) -> np.ndarray:
+    """n cosine-hemisphere rays about the (ray-facing) geometric normal at primary hits."""
+    rng = np.random.default_rng(seed)
+    p, ng = _hit_points(rays, hits, mesh)
+    sel = np.arange(n) % len(p)
+    p, ng = p[sel], ng[sel]
+    u1, u2 = rng.uniform(size=n), rng.uniform(size=n)
+    r = np.sqrt(u1)
+    phi = 2 * np.pi * u2
+    x, y = r * np.cos(phi), r * np.sin(phi)
+    z = np.sqrt(np.maximum(0.0, 1 - u1))
+    t, b = _onb(ng)
+    wi = x[:, None] * t + y[:, None] * b + z[:, None] * ng
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    return make_rays(spawn_origins(p, ng, wi), wi.astype(np.float32))
+
+
+@dataclass
+class ShadowSet:
+    rays: np.ndarray
+    target: np.ndarray  # (n, 2) uint32: (instance, triangle)
+    lights: np.ndarray  # triangle ids of the emissive set
+
+
+def shadow_rays(rays: np.ndarray, hits: np.ndarray, mesh: Mesh, n: int, n_lights: int = 64,
+                seed: int = 2) -> ShadowSet:
+    """n shadow rays from primary hits to uniform points (sample_triangle_point,
+    R/light/area.rs) on n_lights random triangles, targeting that triangle."""
+    rng = np.random.default_rng(seed)
+    p, ng = _hit_points(rays, hits, mesh)
+    sel = np.arange(n) % len(p)
+    p, ng = p[sel], ng[sel]
+    lights = rng.choice(mesh.triangle_count(), size=n_lights, replace=False).astype(np.uint32)
+    k = rng.integers(0, n_lights, size=n)
+    tri = lights[k]
+    v = mesh.positions[mesh.indices[tri]].astype(np.float64)
+    u1, u2 = rng.uniform(size=n), rng.uniform(size=n)
+    su = np.sqrt(u1)
+    q = (1 - su)[:, None] * v[:, 0] + (u2 * su)[:, None] * v[:, 1] + ((1 - u2) * su)[:, None] * v[:, 2]
+    wi = q - p
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    r = make_rays(spawn_origins(p, ng, wi), wi.astype(np.float32))
+    target = np.stack([np.zeros(n, np.uint32), tri], 1).astype(np.uint32)
+    return ShadowSet(r, target, lights)
+
+
+# canonical camera of the traversal microbenchmark: outside the 100-unit cube
+TRAVERSAL_CAMERA = dict(eye=(50.0, 50.0, -140.0), look_at=(50.0, 50.0, 50.0), up=(0.0, 1.0, 0.0), fov_y_deg=25.0)
