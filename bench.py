@@ -49,6 +49,11 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=300_000, help="rays per set for the CPU baseline (0: skip)")
     ap.add_argument("--spheres", type=int, default=410)
     ap.add_argument("--rays", type=int, default=2_000_000, help="bounce and shadow ray counts")
+    ap.add_argument("--refill", type=int, default=0, help="trace-kernel refill threshold (0: library default)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident blocks per CU (0: occupancy)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of this many MiB three times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     return ap.parse_args()
 
 
@@ -64,6 +69,16 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.copy_probe:
+        n = args.copy_probe << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "launches": 3}), flush=True)
+        return
+
     import tpt
     from tpt import scenes
 
@@ -73,6 +88,10 @@ def main():
     log(f"scene: {sf.mesh.triangle_count()} triangles, {len(sf.mesh.qbvh.nodes)} nodes, build+gen "
         f"{time.time() - t0:.1f}s")
     dev = tpt.Device(0)
+    if args.refill:
+        dev.set_param("refill_below", args.refill)
+    if args.blocks_per_cu:
+        dev.set_param("blocks_per_cu", args.blocks_per_cu)
     ds = sc.upload(dev)
     primary = scenes.primary_rays(1920, 1080, **scenes.TRAVERSAL_CAMERA)
     hp = ds.closest_hit(primary)

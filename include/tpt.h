@@ -127,6 +127,12 @@ void tpt_device_release(TptDevice* device);
  * library's own stream).  Lets a caller time or order work with its events. */
 int32_t tpt_device_set_stream(TptDevice* device, void* hip_stream);
 int32_t tpt_device_synchronize(TptDevice* device);
+/* Tuning knobs of the trace kernels (no reference counterpart; set by an
+ * autotuner or the caller):
+ *   "refill_below"  1..64  a persistent wave fetches new rays for its idle
+ *                          lanes when fewer than this many lanes are busy
+ *   "blocks_per_cu" 0..64  resident 256-thread blocks per CU (0: occupancy) */
+int32_t tpt_device_set_param(TptDevice* device, const char* name, int64_t value);
 
 /* ---- host QBVH build (R/qbvh.rs:144-516) -------------------------------- */
 /* Binned-SAH 4-wide build over primitive bounds (3 floats per min / max).

@@ -254,13 +254,50 @@ __device__ __forceinline__ uint32_t slab4(const GpuNode* __restrict__ nodes, con
   return bits;
 }
 
-// Order the hit slots as the reference's stable insertion sort by tmin does,
-// then emit them far->near: interiors are pushed (so the nearest pops first)
-// and leaves are queued for immediate processing.  Returns the leaf queue
-// (up to 4 child refs, processing order in q[0..n)).
+// Per-lane traversal stack: kLdsDepth entries in LDS laid out [depth][thread]
+// (the bank is the thread, so lanes at different depths never conflict),
+// continued in a private array up to the reference's 128 entries
+// (R/qbvh.rs:609).  The stack pointer is a register of its own: it must not
+// share an aggregate with the dynamically indexed spill array, or it would be
+// kept in scratch memory.
+struct LaneStack {
+  uint32_t* lds;    // lds[d * kBlock] is this thread's entry d
+  uint32_t* spill;  // private array of kStackDepth - kLdsDepth entries
+  int sp;
+  bool overflow;
+  __device__ __forceinline__ void push(uint32_t v) {
+    if (sp < kLdsDepth) {
+      lds[sp * kBlock] = v;
+    } else if (sp < kStackDepth) {
+      spill[sp - kLdsDepth] = v;
+    } else {
+      overflow = true;  // the reference panics here; the entry is dropped
+      return;
+    }
+    ++sp;
+  }
+  __device__ __forceinline__ uint32_t pop() {
+    --sp;
+    return sp < kLdsDepth ? lds[sp * kBlock] : spill[sp - kLdsDepth];
+  }
+};
+
+// Leaves of the last visited node, in processing order (q0 first).
 struct LeafQueue {
   uint32_t q0, q1, q2, q3;
   int n;
+  __device__ __forceinline__ void clear() {
+    q0 = q1 = q2 = q3 = 0;
+    n = 0;
+  }
+  __device__ __forceinline__ uint32_t pop() {
+    const uint32_t v = q0;
+    q0 = q1;
+    q1 = q2;
+    q2 = q3;
+    n--;
+    return v;
+  }
 };
 
 __device__ __forceinline__ void cswap_desc(float& ta, uint32_t& ca, int& sa, float& tb, uint32_t& cb, int& sb) {
@@ -273,30 +310,15 @@ __device__ __forceinline__ void cswap_desc(float& ta, uint32_t& ca, int& sa, flo
   }
 }
 
-struct Stack {
-  uint32_t* lds;  // base for this thread: lds[d * kBlock]
-  uint32_t spill[kStackDepth - kLdsDepth];
-  int sp;
-  // Callers check sp < kStackDepth first (and flag overflow); a full stack
-  // drops the entry rather than writing out of bounds.
-  __device__ __forceinline__ void push(uint32_t v) {
-    if (sp < kLdsDepth) {
-      lds[sp * kBlock] = v;
-      ++sp;
-    } else if (sp < kStackDepth) {
-      spill[sp - kLdsDepth] = v;
-      ++sp;
-    }
-  }
-  __device__ __forceinline__ uint32_t pop() {
-    --sp;
-    return sp < kLdsDepth ? lds[sp * kBlock] : spill[sp - kLdsDepth];
-  }
-};
-
+// One node visit.  The reference sorts the hit children by tmin with a stable
+// insertion sort (ties keep slot order; a hit tmin is never NaN) and walks
+// them far->near: leaves are handled at once, interiors pushed so that the
+// nearest pops first.  The same order results from a 4-input sorting network
+// on (tmin, slot) descending; pushes and leaf handling are independent, so
+// interiors are pushed here and leaves queued to run before the next pop.
 template <bool kQ8>
-__device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8* qnodes, uint32_t ni, uint32_t base,
-                                           const Ray3& r, float t_max, Stack& st, LeafQueue& lq, bool* overflow) {
+__device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8* qnodes, uint32_t ni,
+                                           const Ray3& r, float t_max, LaneStack& st, LeafQueue& lq) {
   float tmin[4];
   uint32_t child[4];
   const uint32_t bits = slab4<kQ8>(nodes, qnodes, ni, r, t_max, tmin, child);
@@ -306,7 +328,7 @@ __device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const bool h = (bits >> i) & 1u;
-    t[i] = h ? tmin[i] : -1.0f;  // misses sort last (hit tmin >= 0)
+    t[i] = h ? tmin[i] : -1.0f;  // misses sort last (a hit tmin is >= 0)
     c[i] = child[i];
     s[i] = h ? i : -1;
   }
@@ -315,7 +337,7 @@ __device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8
   cswap_desc(t[0], c[0], s[0], t[2], c[2], s[2]);
   cswap_desc(t[1], c[1], s[1], t[3], c[3], s[3]);
   cswap_desc(t[1], c[1], s[1], t[2], c[2], s[2]);
-  lq.n = 0;
+  lq.clear();
   uint32_t q[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -323,15 +345,13 @@ __device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8
     const uint32_t ck = c[k];
     if (ck & kChildLeafBit) {
       if (((ck >> kChildCountShift) & kChildCountMask) == 0) continue;
-      // append to queue (constant-index shift keeps the queue in registers)
       if (lq.n == 0) q[0] = ck;
       else if (lq.n == 1) q[1] = ck;
       else if (lq.n == 2) q[2] = ck;
       else q[3] = ck;
       lq.n++;
     } else {
-      if (st.sp >= kStackDepth) *overflow = true;
-      st.push(ck + base);
+      st.push(ck);
     }
   }
   lq.q0 = q[0];
@@ -340,152 +360,157 @@ __device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8
   lq.q3 = q[3];
 }
 
-__device__ __forceinline__ uint32_t lq_pop(LeafQueue& lq) {
-  const uint32_t v = lq.q0;
-  lq.q0 = lq.q1;
-  lq.q1 = lq.q2;
-  lq.q2 = lq.q3;
-  lq.n--;
-  return v;
-}
-
 enum QueryMode { kClosest = 0, kAny = 1 };
 
-struct QueryOut {
-  uint32_t inst, tri;
-  float t, b0, b1, b2;
-  bool hit;
-};
-
-// Two-level traversal as one loop: level 0 walks the TLAS, level 1 the BLAS
-// of the instance being entered; both share the stack (BLAS entries sit above
-// `blas_base`).  Every visit follows the reference order, so the last
-// accepted triangle is the reference's answer.
-// kAny: return at the first accepted triangle that is not (skip_inst,
-// skip_tri) and whose t is below `strict_bound` (hits at or beyond it are
-// ignored and the traversal goes on).
 struct Counters {
   uint32_t nodes, tris, insts;
 };
 
-template <bool kQ8, int kMode, bool kCount>
-__device__ QueryOut traverse(const SceneView& sc, F3 wo, F3 wd, float t_max, uint32_t skip_inst, uint32_t skip_tri,
-                             float strict_bound, Stack& st, bool* overflow, Counters& cnt) {
-  QueryOut res;
-  res.hit = false;
-  res.inst = TPT_INVALID_INDEX;
-  res.tri = TPT_INVALID_INDEX;
-  res.t = __builtin_inff();
-  res.b0 = res.b1 = res.b2 = 0.0f;
-  if (sc.instance_count == 0) return res;
-
-  Ray3 world;
-  world.o = wo;
-  world.d = wd;
-  world.inv = f3(safe_inv(wd.x), safe_inv(wd.y), safe_inv(wd.z));
-  float closest = t_max;
-
-  // TLAS state
-  LeafQueue tq;
-  tq.n = 0;
-  tq.q0 = tq.q1 = tq.q2 = tq.q3 = 0;
-  uint32_t inst_cur = 0, inst_end = 0;
-  st.sp = 0;
-  if (sc.tlas_node_count == 0) {
-    inst_end = sc.instance_count;  // root leaf over all instances
-  } else {
-    st.push(0);
-  }
-  // BLAS state
-  int level = 0;
-  int blas_base = 0;
-  uint32_t cur_inst = 0;
-  Ray3 local;
-  local.o = local.d = local.inv = f3(0.f, 0.f, 0.f);
+// Traversal state of one lane.  Single-level scenes (one instance, no TLAS
+// nodes) keep only the BLAS part; two-level scenes interleave TLAS visits and
+// BLAS walks on the same stack, BLAS entries above `blas_base`.
+struct Lane {
+  Ray3 r;         // ray in the current instance's space
+  F3 wo, wd, winv;// world ray (two-level)
+  float closest;  // current t_max
+  float strict;   // kAny: hits at t >= strict are ignored
+  uint32_t skip_inst, skip_tri;
+  // result
+  uint32_t inst, tri;
+  float t, b0, b1, b2;
+  bool hit;
+  // BLAS walk
+  uint32_t tri_cur, tri_end, tri_base;
   LeafQueue bq;
-  bq.n = 0;
-  bq.q0 = bq.q1 = bq.q2 = bq.q3 = 0;
-  uint32_t tri_cur = 0, tri_end = 0, node_base = 0, tri_base = 0;
+  // two-level
+  int level, blas_base;
+  uint32_t cur_inst, inst_cur, inst_end;
+  LeafQueue tq;
+};
 
-  for (;;) {
-    if (level == 1) {
-      if (tri_cur < tri_end) {
-        const GpuTri* tp = sc.tris + tri_base + tri_cur;
-        const float4 a = reinterpret_cast<const float4*>(tp)[0];
-        const float4 b = reinterpret_cast<const float4*>(tp)[1];
-        const float2 c = reinterpret_cast<const float2*>(tp)[4];
-        const uint32_t prim = __float_as_uint(c.y);
-        TriResult h;
-        const bool skip = (kMode == kAny) && cur_inst == skip_inst && prim == skip_tri;
-        if (kCount && !skip) cnt.tris++;
-        if (!skip &&
-            intersect_triangle(local.o, local.d, closest, f3(a.x, a.y, a.z), f3(a.w, b.x, b.y), f3(b.z, b.w, c.x),
-                               &h) &&
-            (kMode != kAny || h.t < strict_bound)) {
-          closest = h.t;
-          res.hit = true;
-          res.inst = cur_inst;
-          res.tri = prim;
-          res.t = h.t;
-          res.b0 = h.b0;
-          res.b1 = h.b1;
-          res.b2 = h.b2;
-          if (kMode == kAny) return res;
-        }
-        ++tri_cur;
-        continue;
-      }
-      if (bq.n > 0) {
-        const uint32_t c = lq_pop(bq);
-        tri_cur = c & kChildOffsetMask;
-        tri_end = tri_cur + ((c >> kChildCountShift) & kChildCountMask);
-        continue;
-      }
-      if (st.sp > blas_base) {
+__device__ __forceinline__ void enter_instance(const SceneView& sc, Lane& L, LaneStack& st, uint32_t ii,
+                                               const F3& wo, const F3& wd) {
+  const GpuInstance in = sc.inst[ii];
+  to_local(in, wo, wd, &L.r);
+  L.cur_inst = ii;
+  L.tri_base = in.tri_base;
+  L.blas_base = st.sp;
+  L.bq.clear();
+  if (in.node_count == 0) {  // root leaf, R/qbvh.rs:615-620
+    L.tri_cur = in.tri_base;
+    L.tri_end = in.tri_base + in.tri_count;
+  } else {
+    L.tri_cur = L.tri_end = 0;
+    st.push(in.node_base);
+  }
+  L.level = 1;
+}
+
+template <bool kTwoLevel>
+__device__ __forceinline__ void lane_begin(const SceneView& sc, Lane& L, LaneStack& st, F3 o, F3 d, float t_max,
+                                           float strict, uint32_t skip_inst, uint32_t skip_tri) {
+  L.closest = t_max;
+  L.strict = strict;
+  L.skip_inst = skip_inst;
+  L.skip_tri = skip_tri;
+  L.hit = false;
+  L.inst = L.tri = TPT_INVALID_INDEX;
+  L.t = __builtin_inff();
+  L.b0 = L.b1 = L.b2 = 0.0f;
+  L.tri_cur = L.tri_end = 0;
+  L.bq.clear();
+  L.tq.clear();
+  st.sp = 0;
+  st.overflow = false;
+  if (kTwoLevel) {
+    L.wo = o;
+    L.wd = d;
+    L.winv = f3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    L.level = 0;
+    L.inst_cur = L.inst_end = 0;
+    if (sc.tlas_node_count == 0)
+      L.inst_end = sc.instance_count;  // TLAS root leaf over all instances
+    else
+      st.push(0);
+  } else {
+    enter_instance(sc, L, st, 0, o, d);
+  }
+}
+
+// One iteration: at most one node visit and one triangle test (the "if-if"
+// form: a lane that pops a node with leaf children tests a triangle in the
+// same iteration).  Returns true when the lane's query is finished.
+template <bool kQ8, int kMode, bool kCount, bool kTwoLevel>
+__device__ __forceinline__ bool lane_step(const SceneView& sc, Lane& L, LaneStack& st, Counters& cnt) {
+  if (kTwoLevel && L.level == 0) {
+    if (L.inst_cur >= L.inst_end) {
+      if (L.tq.n == 0) {
+        if (st.sp == 0) return true;
         const uint32_t ni = st.pop();
         if (kCount) cnt.nodes++;
-        visit_node<kQ8>(sc.nodes, sc.qnodes, ni, node_base, local, closest, st, bq, overflow);
-        continue;
+        Ray3 w;
+        w.o = L.wo;
+        w.d = L.wd;
+        w.inv = L.winv;
+        visit_node<kQ8>(sc.tlas, sc.qtlas, ni, w, L.closest, st, L.tq);
       }
-      level = 0;
-      continue;
-    }
-    // level 0
-    if (inst_cur < inst_end) {
-      cur_inst = sc.tlas_prim[inst_cur++];
-      if (kCount) cnt.insts++;
-      const GpuInstance in = sc.inst[cur_inst];
-      to_local(in, world.o, world.d, &local);
-      node_base = in.node_base;
-      tri_base = in.tri_base;
-      blas_base = st.sp;
-      bq.n = 0;
-      if (in.node_count == 0) {
-        tri_cur = 0;
-        tri_end = in.tri_count;
-      } else {
-        tri_cur = tri_end = 0;
-        if (st.sp >= kStackDepth) *overflow = true;
-        st.push(node_base);
+      if (L.tq.n > 0) {
+        const uint32_t c = L.tq.pop();
+        L.inst_cur = c & kChildOffsetMask;
+        L.inst_end = L.inst_cur + ((c >> kChildCountShift) & kChildCountMask);
       }
-      level = 1;
-      continue;
     }
-    if (tq.n > 0) {
-      const uint32_t c = lq_pop(tq);
-      inst_cur = c & kChildOffsetMask;
-      inst_end = inst_cur + ((c >> kChildCountShift) & kChildCountMask);
-      continue;
-    }
-    if (st.sp > 0) {
+    if (L.inst_cur >= L.inst_end) return false;
+    if (kCount) cnt.insts++;
+    enter_instance(sc, L, st, sc.tlas_prim[L.inst_cur++], L.wo, L.wd);
+  }
+  // BLAS part
+  if (L.tri_cur >= L.tri_end) {
+    if (L.bq.n == 0) {
+      if (st.sp == (kTwoLevel ? L.blas_base : 0)) {
+        if (kTwoLevel) {
+          L.level = 0;
+          return false;
+        }
+        return true;
+      }
       const uint32_t ni = st.pop();
       if (kCount) cnt.nodes++;
-      visit_node<kQ8>(sc.tlas, sc.qtlas, ni, 0, world, closest, st, tq, overflow);
-      continue;
+      visit_node<kQ8>(sc.nodes, sc.qnodes, ni, L.r, L.closest, st, L.bq);
     }
-    break;
+    if (L.bq.n > 0) {
+      const uint32_t c = L.bq.pop();
+      L.tri_cur = L.tri_base + (c & kChildOffsetMask);
+      L.tri_end = L.tri_cur + ((c >> kChildCountShift) & kChildCountMask);
+    }
   }
-  return res;
+  if (L.tri_cur < L.tri_end) {
+    const GpuTri* tp = sc.tris + L.tri_cur;
+    const float4 a = reinterpret_cast<const float4*>(tp)[0];
+    const float4 b = reinterpret_cast<const float4*>(tp)[1];
+    const float2 c = reinterpret_cast<const float2*>(tp)[4];
+    const uint32_t prim = __float_as_uint(c.y);
+    ++L.tri_cur;
+    const bool skip = (kMode == kAny) && L.cur_inst == L.skip_inst && prim == L.skip_tri;
+    if (!skip) {
+      if (kCount) cnt.tris++;
+      TriResult h;
+      if (intersect_triangle(L.r.o, L.r.d, L.closest, f3(a.x, a.y, a.z), f3(a.w, b.x, b.y), f3(b.z, b.w, c.x),
+                             &h) &&
+          (kMode != kAny || h.t < L.strict)) {
+        L.closest = h.t;
+        L.hit = true;
+        L.inst = L.cur_inst;
+        L.tri = prim;
+        L.t = h.t;
+        L.b0 = h.b0;
+        L.b1 = h.b1;
+        L.b2 = h.b2;
+        if (kMode == kAny) return true;
+      }
+    }
+  }
+  return false;
 }
 
 __device__ __forceinline__ void load_ray(const TptRay* rays, uint32_t i, F3* o, F3* d, float* tmax) {
@@ -496,108 +521,186 @@ __device__ __forceinline__ void load_ray(const TptRay* rays, uint32_t i, F3* o, 
   *d = f3(b.x, b.y, b.z);
 }
 
-// counts (kCount): per ray nodes visited, triangles tested, instances entered.
-template <bool kQ8, bool kCount>
-__global__ __launch_bounds__(kBlock) void closest_kernel(SceneView sc, const TptRay* __restrict__ rays,
-                                                         TptHit* __restrict__ hits, uint32_t n,
-                                                         uint32_t* __restrict__ status, uint32_t* __restrict__ counts) {
-  __shared__ uint32_t lds[kLdsDepth * kBlock];
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  Stack st;
-  st.lds = lds + threadIdx.x;
-  st.sp = 0;
+// Work distribution.  Rays are split into kRegions contiguous ranges, one per
+// XCD (workgroups are dispatched round-robin over the 8 XCDs, so block b runs
+// on XCD b % 8): neighbouring rays stay on one XCD's L2.  A wave refills its
+// idle lanes from its range with one atomic per refill; once its range is
+// exhausted it moves on to the next XCD's range.  The last wave to finish
+// resets the counters, so consecutive launches on one stream need no clear.
+constexpr int kRegions = 8;
+struct WorkCounters {
+  uint32_t next[kRegions];
+  uint32_t done_waves;
+  uint32_t pad[7];
+};
+
+struct Query {
+  // closest: hits; any: target, distance, out
+  TptHit* hits;
+  const uint32_t* target;
+  const float* distance;
+  uint8_t* out;
+  uint32_t* counts;
+  uint32_t* status;
+  WorkCounters* work;
+  uint32_t n;
+  uint32_t refill_below;  // refill when fewer lanes than this are busy
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+template <bool kQ8, int kMode, bool kExact, bool kCount, bool kTwoLevel>
+__device__ __forceinline__ void begin_ray(const SceneView& sc, const TptRay* rays, const Query& q, uint32_t rid,
+                                          Lane& L, LaneStack& st) {
   F3 o, d;
   float tmax;
-  load_ray(rays, i, &o, &d, &tmax);
-  bool overflow = false;
-  Counters cnt = {0, 0, 0};
-  const QueryOut q = traverse<kQ8, kClosest, kCount>(sc, o, d, tmax, TPT_INVALID_INDEX, TPT_INVALID_INDEX,
-                                                     __builtin_inff(), st, &overflow, cnt);
-  if (kCount) {
-    counts[3 * i] = cnt.nodes;
-    counts[3 * i + 1] = cnt.tris;
-    counts[3 * i + 2] = cnt.insts;
+  load_ray(rays, rid, &o, &d, &tmax);
+  if (kMode == kClosest) {
+    lane_begin<kTwoLevel>(sc, L, st, o, d, tmax, __builtin_inff(), TPT_INVALID_INDEX, TPT_INVALID_INDEX);
+    return;
   }
-  TptHit h;
-  h.instance = q.inst;
-  h.triangle = q.tri;
-  h.t = q.t;
-  h.b[0] = q.b0;
-  h.b[1] = q.b1;
-  h.b[2] = q.b2;
-  hits[i] = h;
-  if (overflow) atomicOr(status, 1u);
-}
-
-// unoccluded_ray (R/integrator/mod.rs:239-265).  Fast form: the target
-// triangle's own t bounds the query and the first other hit ends it; without
-// a target, t_max = distance * (1 - SHADOW_TOLERANCE) (or +inf).  Exact form:
-// a closest-hit query followed by the reference's decision rule.
-template <bool kQ8, bool kExact, bool kCount>
-__global__ __launch_bounds__(kBlock) void any_kernel(SceneView sc, const TptRay* __restrict__ rays,
-                                                     const uint32_t* __restrict__ target,
-                                                     const float* __restrict__ distance,
-                                                     uint8_t* __restrict__ out, uint32_t n,
-                                                     uint32_t* __restrict__ status, uint32_t* __restrict__ counts) {
-  __shared__ uint32_t lds[kLdsDepth * kBlock];
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  Stack st;
-  st.lds = lds + threadIdx.x;
-  st.sp = 0;
-  F3 o, d;
-  float tmax_unused;
-  load_ray(rays, i, &o, &d, &tmax_unused);
+  // unoccluded_ray (R/integrator/mod.rs:239-265)
   uint32_t ti = TPT_INVALID_INDEX, tt = TPT_INVALID_INDEX;
-  if (target) {
-    ti = target[2 * i];
-    tt = target[2 * i + 1];
+  if (q.target) {
+    ti = q.target[2 * rid];
+    tt = q.target[2 * rid + 1];
     if (ti >= sc.instance_count) ti = TPT_INVALID_INDEX;
   }
-  const float dist = distance ? distance[i] : __builtin_inff();
-  bool overflow = false;
-  Counters cnt = {0, 0, 0};
-  uint8_t unocc;
-  if (kExact) {
-    const QueryOut q = traverse<kQ8, kClosest, kCount>(sc, o, d, __builtin_inff(), TPT_INVALID_INDEX,
-                                                       TPT_INVALID_INDEX, __builtin_inff(), st, &overflow, cnt);
-    if (!q.hit)
-      unocc = 1;
-    else if (ti != TPT_INVALID_INDEX)
-      unocc = (q.inst == ti && q.tri == tt) ? 1 : 0;
-    else if (__builtin_isinf(dist))
-      unocc = 0;
-    else
-      unocc = q.t >= dist * (1.0f - 1.0e-3f) ? 1 : 0;
-  } else {
-    float bound = __builtin_inff();
-    float strict_bound = __builtin_inff();
-    if (ti != TPT_INVALID_INDEX) {
-      const GpuInstance in = sc.inst[ti];
-      if (tt < in.tri_count) {
-        Ray3 local;
-        to_local(in, o, d, &local);
-        const GpuTri* tp = sc.tris + in.tri_base + sc.tri_rank[in.tri_base + tt];
-        TriResult h;
-        if (intersect_triangle(local.o, local.d, __builtin_inff(), f3(tp->v[0], tp->v[1], tp->v[2]),
-                               f3(tp->v[3], tp->v[4], tp->v[5]), f3(tp->v[6], tp->v[7], tp->v[8]), &h))
-          bound = h.t;
-      }
-    } else if (!__builtin_isinf(dist)) {
-      bound = dist * (1.0f - 1.0e-3f);
-      strict_bound = bound;  // occluded iff some hit has t < distance * (1 - 1e-3)
+  if (kExact) {  // the reference's own form: a full closest-hit query
+    lane_begin<kTwoLevel>(sc, L, st, o, d, __builtin_inff(), __builtin_inff(), TPT_INVALID_INDEX, TPT_INVALID_INDEX);
+    // the closest-hit traversal never reads skip_*: they keep the target
+    // for the decision in finish_ray
+    L.skip_inst = ti;
+    L.skip_tri = tt;
+    return;
+  }
+  const float dist = q.distance ? q.distance[rid] : __builtin_inff();
+  float bound = __builtin_inff(), strict = __builtin_inff();
+  if (ti != TPT_INVALID_INDEX) {
+    // The target's own t bounds the query: any other hit nearer than it
+    // makes the target not the closest hit.
+    const GpuInstance in = sc.inst[ti];
+    if (tt < in.tri_count) {
+      Ray3 lr;
+      to_local(in, o, d, &lr);
+      const GpuTri* tp = sc.tris + in.tri_base + sc.tri_rank[in.tri_base + tt];
+      TriResult h;
+      if (intersect_triangle(lr.o, lr.d, __builtin_inff(), f3(tp->v[0], tp->v[1], tp->v[2]),
+                             f3(tp->v[3], tp->v[4], tp->v[5]), f3(tp->v[6], tp->v[7], tp->v[8]), &h))
+        bound = h.t;
     }
-    const QueryOut q = traverse<kQ8, kAny, kCount>(sc, o, d, bound, ti, tt, strict_bound, st, &overflow, cnt);
-    unocc = q.hit ? 0 : 1;
+  } else if (!__builtin_isinf(dist)) {
+    bound = dist * (1.0f - 1.0e-3f);  // SHADOW_TOLERANCE, R/integrator/mod.rs:50
+    strict = bound;                   // occluded iff some hit has t < bound
   }
-  out[i] = unocc;
+  lane_begin<kTwoLevel>(sc, L, st, o, d, bound, strict, ti, tt);
+}
+
+template <int kMode, bool kExact, bool kCount>
+__device__ __forceinline__ void finish_ray(const Query& q, uint32_t rid, const Lane& L, const Counters& cnt) {
+  if (kMode == kClosest) {
+    TptHit h;
+    h.instance = L.inst;
+    h.triangle = L.tri;
+    h.t = L.t;
+    h.b[0] = L.b0;
+    h.b[1] = L.b1;
+    h.b[2] = L.b2;
+    if (q.hits) q.hits[rid] = h;
+  } else {
+    uint8_t unocc;
+    if (kExact) {
+      const float dist = q.distance ? q.distance[rid] : __builtin_inff();
+      if (!L.hit)
+        unocc = 1;
+      else if (L.skip_inst != TPT_INVALID_INDEX)
+        unocc = (L.inst == L.skip_inst && L.tri == L.skip_tri) ? 1 : 0;
+      else if (__builtin_isinf(dist))
+        unocc = 0;
+      else
+        unocc = L.t >= dist * (1.0f - 1.0e-3f) ? 1 : 0;
+    } else {
+      unocc = L.hit ? 0 : 1;
+    }
+    if (q.out) q.out[rid] = unocc;
+  }
   if (kCount) {
-    counts[3 * i] = cnt.nodes;
-    counts[3 * i + 1] = cnt.tris;
-    counts[3 * i + 2] = cnt.insts;
+    q.counts[3 * rid] = cnt.nodes;
+    q.counts[3 * rid + 1] = cnt.tris;
+    q.counts[3 * rid + 2] = cnt.insts;
   }
-  if (overflow) atomicOr(status, 1u);
+}
+
+// Persistent trace kernel: every wave loops, refilling idle lanes with new
+// rays, until all rays are done.  Exits are wave-uniform.
+template <bool kQ8, int kMode, bool kExact, bool kCount, bool kTwoLevel>
+__global__ __launch_bounds__(kBlock) void trace_kernel(SceneView sc, const TptRay* __restrict__ rays, Query q) {
+  __shared__ uint32_t lds[kLdsDepth * kBlock];
+  uint32_t spill[kStackDepth - kLdsDepth];
+  LaneStack st;
+  st.lds = lds + threadIdx.x;
+  st.spill = spill;
+  st.sp = 0;
+  st.overflow = false;
+  Lane L;
+  Counters cnt = {0, 0, 0};
+  const uint32_t lane = lane_id();
+  const uint32_t chunk = (q.n + kRegions - 1) / kRegions;
+  int region = (int)(blockIdx.x % kRegions);
+  int regions_left = kRegions;
+  bool active = false;
+  bool overflow = false;
+  uint32_t rid = 0;
+  for (;;) {
+    uint64_t busy = __ballot(active);
+    if (regions_left > 0 && (uint32_t)__popcll(busy) < q.refill_below) {
+      const uint64_t idle = ~busy;
+      const uint32_t want = (uint32_t)__popcll(idle);
+      const uint32_t lo = (uint32_t)region * chunk;
+      const uint32_t hi = min(q.n, lo + chunk);
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&q.work->next[region], want);
+      base = __shfl(base, 0) + lo;
+      if (!active) {
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+        const uint32_t cand = base + rank;
+        if (cand < hi) {
+          rid = cand;
+          begin_ray<kQ8, kMode, kExact, kCount, kTwoLevel>(sc, rays, q, rid, L, st);
+          if (kCount) cnt = Counters{0, 0, 0};
+          active = true;
+        }
+      }
+      if (base + want >= hi) {  // this range is used up: move to the next XCD's
+        region = (region + 1) % kRegions;
+        regions_left--;
+      }
+      busy = __ballot(active);
+    }
+    if (busy == 0) {
+      if (regions_left == 0) break;
+      continue;
+    }
+    if (active) {
+      // the exact shadow rule needs the full closest-hit traversal
+      if (lane_step<kQ8, (kExact ? kClosest : kMode), kCount, kTwoLevel>(sc, L, st, cnt)) {
+        finish_ray<kMode, kExact, kCount>(q, rid, L, cnt);
+        overflow |= st.overflow;
+        active = false;
+      }
+    }
+  }
+  if (__ballot(overflow) != 0 && lane == 0) atomicOr(q.status, 1u);
+  // the last wave to finish resets the work counters for the next launch
+  if (lane == 0) {
+    __threadfence();
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    if (atomicAdd(&q.work->done_waves, 1u) == waves - 1) {
+      for (int r = 0; r < kRegions; ++r) atomicExch(&q.work->next[r], 0u);
+      atomicExch(&q.work->done_waves, 0u);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -621,9 +724,13 @@ bool hip_ok(hipError_t e, const char* what) {
 
 struct TptDevice {
   int index = 0;
+  int cu_count = 0;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
-  uint32_t* status = nullptr;  // device word: bit 0 = traversal stack overflow
+  uint32_t* status = nullptr;            // device word: bit 0 = traversal stack overflow
+  tpt::WorkCounters* work = nullptr;     // persistent-kernel ray counters (self-resetting)
+  uint32_t refill_below = 40;            // refill a wave when fewer lanes are busy
+  uint32_t blocks_per_cu = 0;            // 0: as many as occupancy allows
 };
 
 struct TptQbvh {
@@ -748,8 +855,17 @@ TptDevice* tpt_device_create(int32_t device_index) {
       return nullptr;
     }
     d->stream = d->own;
+    hipDeviceProp_t prop;
+    if (!hip_ok(hipGetDeviceProperties(&prop, device_index), "hipGetDeviceProperties")) {
+      (void)hipStreamDestroy(d->own);
+      delete d;
+      return nullptr;
+    }
+    d->cu_count = prop.multiProcessorCount;
     if (!hip_ok(hipMalloc((void**)&d->status, 4), "hipMalloc(status)") ||
-        !hip_ok(hipMemset(d->status, 0, 4), "hipMemset(status)")) {
+        !hip_ok(hipMemset(d->status, 0, 4), "hipMemset(status)") ||
+        !hip_ok(hipMalloc((void**)&d->work, sizeof(WorkCounters)), "hipMalloc(work)") ||
+        !hip_ok(hipMemset(d->work, 0, sizeof(WorkCounters)), "hipMemset(work)")) {
       (void)hipStreamDestroy(d->own);
       delete d;
       return nullptr;
@@ -766,8 +882,26 @@ void tpt_device_release(TptDevice* d) {
   (void)hipSetDevice(d->index);
   (void)hipStreamSynchronize(d->stream);
   (void)hipFree(d->status);
+  (void)hipFree(d->work);
   (void)hipStreamDestroy(d->own);
   delete d;
+}
+
+int32_t tpt_device_set_param(TptDevice* d, const char* name, int64_t value) {
+  if (!d || !name) {
+    set_error("tpt_device_set_param: null argument");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  const std::string n(name);
+  if (n == "refill_below" && value >= 1 && value <= 64) {
+    d->refill_below = (uint32_t)value;
+  } else if (n == "blocks_per_cu" && value >= 0 && value <= 64) {
+    d->blocks_per_cu = (uint32_t)value;
+  } else {
+    set_error("tpt_device_set_param: unknown parameter or value out of range: " + n);
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  return TPT_OK;
 }
 
 int32_t tpt_device_set_stream(TptDevice* d, void* stream) {
@@ -983,6 +1117,41 @@ size_t tpt_scene_device_bytes(const TptScene* s) { return s ? s->bytes : 0; }
 namespace tpt {
 namespace {
 
+// Kernel variant for (query, node format, exact shadow rule, counters, levels).
+const void* pick_kernel(int query, bool q8, bool exact, bool cnt, bool two) {
+#define TPT_K(Q8, M, E, C, T) reinterpret_cast<const void*>(&trace_kernel<Q8, M, E, C, T>)
+#define TPT_BY_LEVEL(Q8, M, E, C) (two ? TPT_K(Q8, M, E, C, true) : TPT_K(Q8, M, E, C, false))
+#define TPT_BY_COUNT(Q8, M, E) (cnt ? TPT_BY_LEVEL(Q8, M, E, true) : TPT_BY_LEVEL(Q8, M, E, false))
+  if (query == 0) return q8 ? TPT_BY_COUNT(true, kClosest, false) : TPT_BY_COUNT(false, kClosest, false);
+  if (exact) return q8 ? TPT_BY_COUNT(true, kAny, true) : TPT_BY_COUNT(false, kAny, true);
+  return q8 ? TPT_BY_COUNT(true, kAny, false) : TPT_BY_COUNT(false, kAny, false);
+#undef TPT_BY_COUNT
+#undef TPT_BY_LEVEL
+#undef TPT_K
+}
+
+// Persistent grid: enough resident blocks to fill every CU (or the tuned
+// blocks per CU), never more blocks than the rays need.
+unsigned grid_blocks(TptDevice* d, const void* kern, size_t n) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, int>> cache;
+  int per_cu = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& e : cache)
+      if (e.first == kern) per_cu = e.second;
+    if (per_cu == 0) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+      cache.emplace_back(kern, per_cu);
+    }
+  }
+  if (d->blocks_per_cu) per_cu = (int)d->blocks_per_cu;
+  const size_t want = (n + kBlock - 1) / kBlock;
+  const size_t cap = (size_t)per_cu * (size_t)(d->cu_count > 0 ? d->cu_count : 1);
+  return (unsigned)(want < cap ? want : cap);
+}
+
 // One implementation behind tpt_trace_closest / tpt_trace_any /
 // tpt_trace_counts.  query 0: closest (hits), 1: shadow (unoccluded).
 int32_t trace_impl(TptScene* s, int query, const TptRay* rays, size_t n, const uint32_t* target,
@@ -1048,31 +1217,24 @@ int32_t trace_impl(TptScene* s, int query, const TptRay* rays, size_t n, const u
     dout = (uint8_t*)s->stage;
   }
   const SceneView v = view_of(s);
-  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
-  const uint32_t nn = (uint32_t)n;
+  Query q;
+  q.hits = dhits;
+  q.target = dtarget;
+  q.distance = ddist;
+  q.out = dout;
+  q.counts = dcounts;
+  q.status = d->status;
+  q.work = d->work;
+  q.n = (uint32_t)n;
+  q.refill_below = d->refill_below;
+  const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
+  const bool cnt = dcounts != nullptr;
+  const void* kern = pick_kernel(query, q8, exact, cnt, two);
+  const unsigned blocks = grid_blocks(d, kern, n);
+  void* args[] = {(void*)&v, (void*)&drays, (void*)&q};
+  if (!hip_ok(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, 0, d->stream), "hipLaunchKernel"))
+    return TPT_ERROR_DEVICE;
   hipStream_t st = d->stream;
-  if (query == 0) {
-    if (q8) {
-      if (dcounts) hipLaunchKernelGGL((closest_kernel<true, true>), grid, block, 0, st, v, drays, dhits, nn, d->status, dcounts);
-      else hipLaunchKernelGGL((closest_kernel<true, false>), grid, block, 0, st, v, drays, dhits, nn, d->status, dcounts);
-    } else {
-      if (dcounts) hipLaunchKernelGGL((closest_kernel<false, true>), grid, block, 0, st, v, drays, dhits, nn, d->status, dcounts);
-      else hipLaunchKernelGGL((closest_kernel<false, false>), grid, block, 0, st, v, drays, dhits, nn, d->status, dcounts);
-    }
-  } else {
-#define TPT_ANY(Q, E, C) \
-  hipLaunchKernelGGL((any_kernel<Q, E, C>), grid, block, 0, st, v, drays, dtarget, ddist, dout, nn, d->status, dcounts)
-    if (q8 && exact) {
-      if (dcounts) TPT_ANY(true, true, true); else TPT_ANY(true, true, false);
-    } else if (q8) {
-      if (dcounts) TPT_ANY(true, false, true); else TPT_ANY(true, false, false);
-    } else if (exact) {
-      if (dcounts) TPT_ANY(false, true, true); else TPT_ANY(false, true, false);
-    } else {
-      if (dcounts) TPT_ANY(false, false, true); else TPT_ANY(false, false, false);
-    }
-#undef TPT_ANY
-  }
   if (!hip_ok(hipGetLastError(), "kernel launch")) return TPT_ERROR_DEVICE;
   if (!dev_ptr) {
     if (query == 0 && hits &&

@@ -104,6 +104,7 @@ _SIGNATURES = [
     ("tpt_device_release", None, [ctypes.c_void_p]),
     ("tpt_device_set_stream", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
     ("tpt_device_synchronize", ctypes.c_int32, [ctypes.c_void_p]),
+    ("tpt_device_set_param", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     ("tpt_qbvh_build", ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
     ("tpt_qbvh_release", None, [ctypes.c_void_p]),
     ("tpt_qbvh_node_count", ctypes.c_size_t, [ctypes.c_void_p]),

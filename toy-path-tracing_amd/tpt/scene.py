@@ -201,6 +201,10 @@ class Device:
     def set_stream(self, stream_ptr: int | None) -> None:
         check(lib().tpt_device_set_stream(self.handle, stream_ptr))
 
+    def set_param(self, name: str, value: int) -> None:
+        """Trace-kernel tuning knob (tpt_device_set_param)."""
+        check(lib().tpt_device_set_param(self.handle, name.encode(), int(value)))
+
     def synchronize(self) -> None:
         check(lib().tpt_device_synchronize(self.handle))
 
