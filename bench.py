@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=300_000, help="rays per set for the CPU baseline (0: skip)")
     ap.add_argument("--spheres", type=int, default=410)
     ap.add_argument("--rays", type=int, default=2_000_000, help="bounce and shadow ray counts")
+    ap.add_argument("--render", type=int, default=1, help="also time the Cornell path tracer (1) or not (0)")
     ap.add_argument("--refill", type=int, default=0, help="trace-kernel refill threshold (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident blocks per CU (0: occupancy)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -179,6 +180,8 @@ def main():
     if args.cpu_sample > 0:
         cpu = cpu_baseline(sc, sets, args.cpu_sample)
 
+    render = render_bench(dev, stream, args) if args.render else None
+
     line = {
         "metric": "Mrays/s",
         "value": round(value, 2),
@@ -211,10 +214,60 @@ def main():
             "traffic": None,
         },
         "cpu_baseline": cpu,
+        "render": render,
     }
     print(json.dumps(line), flush=True)
     ds.close()
     dev.close()
+
+
+def render_bench(dev, stream, args):
+    """Plain FP32 megakernel path tracer on the Cornell configuration:
+    512x512, 64 spp, max depth 8, OpenPBR diffuse (specular weight 0)."""
+    import torch
+
+    from tpt import camera, scenes
+
+    w = h = 512
+    spp, depth = 64, 8
+    cs = scenes.cornell()
+    rds = cs.scene.upload(dev)
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    out = torch.empty(w * h * 3, dtype=torch.float32, device="cuda")
+    rds.render_device(cam, w, h, spp, depth, out)
+    torch.cuda.synchronize()
+    reps = max(1, min(args.steps, 5))
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        rds.render_device(cam, w, h, spp, depth, out)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    img = out.view(h, w, 3).cpu().numpy()
+    res = {
+        "workload": "Cornell box 512x512, 64 spp, max depth 8, OpenPBR diffuse (specular weight 0), "
+                    "BSDF sampling + emission (reference NEE policy)",
+        "ms_per_frame": round(ms, 3),
+        "samples_per_s": round(w * h * spp / (ms / 1e3), 1),
+        "image_mean": [round(float(x), 4) for x in img.reshape(-1, 3).mean(0)],
+    }
+    log(f"render: {ms:.2f} ms/frame = {res['samples_per_s'] / 1e6:.1f} Msamples/s")
+    if args.cpu_sample > 0:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle  # noqa: E402  (test infrastructure: CPU baseline leg only)
+
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        r = oracle.OracleRenderer(cs.scene, cs.materials)
+        rows = 16
+        t = time.perf_counter()
+        r.render(cam, w, h, spp, depth, rows=(h // 2 - rows // 2, h // 2 + rows // 2), threads=threads)
+        dt = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": round(rows * w * spp / dt, 1), "unit": "samples/s", "cores": threads,
+                               "kind": "port", "sample": f"{rows} middle rows of the frame ({rows * w * spp} samples)"}
+        log(f"render cpu baseline: {rows * w * spp / dt / 1e6:.2f} Msamples/s on {threads} threads")
+    rds.close()
+    return res
 
 
 def cpu_baseline(sc, sets, sample):

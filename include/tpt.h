@@ -90,6 +90,8 @@ typedef struct TptMeshDesc {
   uint32_t node_count;
   uint32_t pad;
   const uint32_t* primitive_indices;/* triangle_count entries (Qbvh::primitive_indices) */
+  const float* normals;             /* 3 * vertex_count, NULL: no shading normals (Vertex::normal) */
+  const float* uvs;                 /* 2 * vertex_count, NULL: all zero (Vertex::uv) */
 } TptMeshDesc;
 
 /* Instance (R/scene/mod.rs:62-70): 4x4 matrices in glam's column-major order. */
@@ -100,7 +102,31 @@ typedef struct TptInstanceDesc {
   float world_to_local[16];
 } TptInstanceDesc;
 
-/* Scene geometry after Scene::build_qbvh (R/scene/mod.rs:305-316). */
+/* OpenPbrMaterial parameters (R/material/open_pbr.rs:23-82) as the renderer
+ * holds them after prepare_color (rendering-space colours).  tpt_render
+ * supports specular_weight = 0 with no metal, coat, fuzz, transmission,
+ * thin film or thin walls; other values are rejected with
+ * TPT_ERROR_BAD_ARGUMENT. */
+typedef struct TptOpenPbr {
+  float base_weight;
+  float base_color[3];
+  float base_diffuse_roughness;
+  float base_metalness;
+  float specular_weight;
+  float specular_color[3];
+  float specular_roughness;
+  float specular_ior;
+  float specular_roughness_anisotropy;
+  float transmission_weight;
+  float coat_weight;
+  float fuzz_weight;
+  float thin_film_weight;
+  float emission_luminance;
+  float emission_color[3];
+  uint32_t thin_walled;
+} TptOpenPbr; /* 88 bytes */
+
+/* Scene after Scene::build_qbvh (R/scene/mod.rs:305-316). */
 typedef struct TptSceneDesc {
   const TptMeshDesc* meshes;
   uint32_t mesh_count;
@@ -108,9 +134,25 @@ typedef struct TptSceneDesc {
   const TptInstanceDesc* instances;
   const TptQbvhNode* tlas_nodes;          /* over instance world bounds */
   uint32_t tlas_node_count;
-  uint32_t pad;
+  uint32_t material_count;
   const uint32_t* tlas_primitive_indices; /* instance_count entries */
+  const TptOpenPbr* materials;            /* material_count entries (TptInstanceDesc::material) */
 } TptSceneDesc;
+
+/* Pinhole camera after PinholeCamera::new (R/scene/camera.rs:21-42): the
+ * renderer's normalised basis and tan(fov_y / 2), plus pixel_spread =
+ * pixel_spread_angle(resolution) (R/scene/camera.rs:89-96), both evaluated
+ * with the host's libm so the device does no transcendental work. */
+typedef struct TptCamera {
+  float eye[3];
+  float tan_half_fov_y;
+  float forward[3];
+  float pixel_spread;
+  float right[3];
+  float exposure;
+  float image_up[3];
+  float pad;
+} TptCamera; /* 64 bytes */
 
 typedef struct TptDevice TptDevice;
 typedef struct TptScene TptScene;
@@ -131,8 +173,28 @@ int32_t tpt_device_synchronize(TptDevice* device);
  * autotuner or the caller):
  *   "refill_below"  1..64  a persistent wave fetches new rays for its idle
  *                          lanes when fewer than this many lanes are busy
- *   "blocks_per_cu" 0..64  resident 256-thread blocks per CU (0: occupancy) */
+ *   "blocks_per_cu" 0..64  cap on resident 256-thread blocks per CU (0: the
+ *                          occupancy limit) */
 int32_t tpt_device_set_param(TptDevice* device, const char* name, int64_t value);
+
+/* ---- sampler and camera --------------------------------------------------- */
+/* Primary rays of one sample index for every pixel of a width x height frame
+ * (row-major): PathSampler::camera_sample (R/sampler/path_sampler.rs:89-91)
+ * then PinholeCamera::generate_ray_differential (R/scene/camera.rs:48-78).
+ * rays: width*height TptRay (t_max = +inf).  differentials (nullable): 7
+ * floats per pixel, rx_direction[3], ry_direction[3], cone spread angle. */
+int32_t tpt_camera_rays(TptDevice* device, const TptCamera* camera, uint32_t width, uint32_t height,
+                        uint32_t samples_per_pixel, uint32_t sample_index, TptRay* rays, float* differentials,
+                        uint32_t flags);
+/* The integrator's random numbers for one sample index, every pixel:
+ * PathSampler::camera_sample, initial_aux_rng_seed and path_vertex_randoms
+ * (R/sampler/path_sampler.rs:89-131) for depths 0..depth_count-1.
+ * values: per pixel 2 + 14*depth_count floats (camera u,v; per depth
+ * u_category, u_tree, u_light_aux, u_surface.xy, u_lobe, u_layer, u_dir.xy,
+ * u_extra0..3, u_rr); seeds: per pixel 1 + depth_count aux-RNG seeds. */
+int32_t tpt_path_randoms(TptDevice* device, uint32_t width, uint32_t height, uint32_t samples_per_pixel,
+                         uint32_t sample_index, uint32_t depth_count, float* values, uint32_t* seeds,
+                         uint32_t flags);
 
 /* ---- host QBVH build (R/qbvh.rs:144-516) -------------------------------- */
 /* Binned-SAH 4-wide build over primitive bounds (3 floats per min / max).
@@ -173,6 +235,23 @@ int32_t tpt_trace_closest(TptScene* scene, const TptRay* rays, size_t n, TptHit*
  * any-hit query whose answer differs from the reference only on exact ties. */
 int32_t tpt_trace_any(TptScene* scene, const TptRay* rays, size_t n, const uint32_t* target,
                       const float* distance, uint8_t* unoccluded, uint32_t flags);
+
+/* ---- path tracer ------------------------------------------------------------ */
+typedef struct TptRenderDesc {
+  TptCamera camera;
+  uint32_t width;
+  uint32_t height;
+  uint32_t samples_per_pixel;
+  uint32_t max_depth;
+} TptRenderDesc;
+
+/* The render loop of R/main.rs:98-126 with IntegratorKind::Mis
+ * (R/integrator/mis.rs:19-139): for every pixel, samples 0..spp-1 through
+ * PathSampler, generate_ray_differential and trace_radiance, accumulated as
+ * the reference's running mean, times the camera exposure.  rgb: width *
+ * height * 3 floats, row-major.  Light NEE follows the reference policy: an
+ * OpenPBR shading point gets no light-tree sample (R/light_tree/importance.rs:26-36). */
+int32_t tpt_render(TptScene* scene, const TptRenderDesc* desc, float* rgb, uint32_t flags);
 
 /* The traversal of tpt_trace_closest (query 0) or tpt_trace_any (query 1)
  * with per-ray work counters instead of results: counts[3*i+0] nodes

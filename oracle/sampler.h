@@ -1,19 +1,13 @@
-// Oracle: CPU restatement of the reference sampler and pinhole camera.
-//
-// TEST INFRASTRUCTURE ONLY (the checker; never linked into the product).
-//   R/sampler/zsobol.rs        ZSobol index permutation + scrambled 4-D Sobol
-//   R/sampler/pcg.rs           PCG hash
-//   R/sampler/aux_rng.rs       AuxRng
-//   R/sampler/path_sampler.rs  PathSampler dimension layout
-//   R/scene/camera.rs          PinholeCamera::generate_ray_differential
-//   R/math/mod.rs:13-24        sample_tent_1d / 2d
-// Integer code is exact; the camera uses only +,-,*,/,sqrt (correctly
-// rounded), so it is exact too.  The transcendental set-up values
-// (tan(fov/2), atan of the pixel footprint) are inputs computed by the caller.
+// Oracle sampler and camera: ZSobol + scrambled Sobol (R/sampler/zsobol.rs),
+// PCG (R/sampler/pcg.rs), AuxRng (R/sampler/aux_rng.rs), PathSampler
+// (R/sampler/path_sampler.rs), tent filter (R/math/mod.rs:13-24), pinhole
+// camera helpers (R/scene/camera.rs).  TEST INFRASTRUCTURE ONLY.
+#pragma once
+
 #include <cmath>
 #include <cstdint>
 
-namespace {
+namespace orc {
 
 // ---- R/sampler/pcg.rs:1-13
 inline uint32_t pcg_state_transition(uint32_t s) { return s * 747796405u + 2891336453u; }
@@ -180,125 +174,21 @@ inline float tent(float u) {
   return 1.0f - std::sqrt(2.0f - 2.0f * u);
 }
 
-struct F3 {
+struct CF3 {
   float x, y, z;
 };
-inline F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-inline F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline F3 scale(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
-inline float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-inline F3 normalize(F3 a) {  // glam: self * (1 / sqrt(dot))
+inline CF3 add(CF3 a, CF3 b) { return CF3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline CF3 sub(CF3 a, CF3 b) { return CF3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline CF3 scale(float s, CF3 a) { return CF3{s * a.x, s * a.y, s * a.z}; }
+inline float dot(CF3 a, CF3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline CF3 normalize(CF3 a) {  // glam: self * (1 / sqrt(dot))
   float r = 1.0f / std::sqrt(dot(a, a));
-  return {a.x * r, a.y * r, a.z * r};
+  return CF3{a.x * r, a.y * r, a.z * r};
 }
-inline F3 normalize_or_zero(F3 a) {
+inline CF3 normalize_or_zero(CF3 a) {
   float r = 1.0f / std::sqrt(dot(a, a));
-  if (std::isfinite(r) && r > 0.0f) return {a.x * r, a.y * r, a.z * r};
-  return {0.f, 0.f, 0.f};
+  if (std::isfinite(r) && r > 0.0f) return CF3{a.x * r, a.y * r, a.z * r};
+  return CF3{0.f, 0.f, 0.f};
 }
 
-}  // namespace
-
-extern "C" {
-
-uint32_t orc_pcg_hash(uint32_t k) { return pcg_hash(k); }
-uint32_t orc_pcg_state_transition(uint32_t s) { return pcg_state_transition(s); }
-uint32_t orc_pcg_output(uint32_t s) { return pcg_output(s); }
-uint32_t orc_laine_karras(uint32_t v, uint32_t seed) { return laine_karras(v, seed); }
-uint32_t orc_reverse_and_shuffle(uint32_t v, uint32_t seed) { return reverse_and_shuffle(v, seed); }
-float orc_uint_to_float(uint32_t v) { return uint_to_float(v); }
-uint64_t orc_encode_morton_2(uint32_t x, uint32_t y) { return encode_morton_2(x, y); }
-uint64_t orc_mix_bits(uint64_t v) { return mix_bits(v); }
-void orc_shuffled_scrambled_sobol(uint32_t index, uint32_t seed, int depth, uint32_t* out) {
-  shuffled_scrambled_sobol(index, seed, depth, out);
-}
-uint64_t orc_zsobol_sample_index(uint32_t spp, uint32_t rx, uint32_t ry, uint32_t px, uint32_t py, uint32_t si,
-                                 uint32_t dim) {
-  return IndexSampler(spp, rx, ry).sample_index(px, py, si, dim);
-}
-
-// IndexSampler::sample_index for every pixel of (rx, ry) and every sample
-// index < spp in one dimension; out[(y*rx + x)*spp + s].
-void orc_zsobol_sample_indices(uint32_t spp, uint32_t rx, uint32_t ry, uint32_t dim, uint64_t* out) {
-  IndexSampler is(spp, rx, ry);
-  for (uint32_t y = 0; y < ry; ++y)
-    for (uint32_t x = 0; x < rx; ++x)
-      for (uint32_t s = 0; s < spp; ++s) out[((size_t)y * rx + x) * spp + s] = is.sample_index(x, y, s, dim);
-}
-
-// AuxRng (R/sampler/aux_rng.rs:14-30): n floats from a seed.
-void orc_aux_rng(uint32_t seed, int n, float* out) {
-  uint32_t s = pcg_state_transition(seed);
-  for (int i = 0; i < n; ++i) {
-    s = pcg_state_transition(s);
-    out[i] = (float)(pcg_output(s) >> 8) * (1.0f / 16777216.0f);
-  }
-}
-
-// PathSampler values for one (pixel, sample): out[0..2) camera sample,
-// out_seed[0] initial aux seed; per depth d < ndepth, out[2 + 14*d + k] =
-// u_category, u_tree, u_light_aux, u_surface.xy, u_lobe, u_layer, u_dir.xy,
-// u_extra0..3, u_rr and out_seed[1 + d] = aux_rng_seed.
-void orc_path_sampler(uint32_t rx, uint32_t ry, uint32_t spp, uint32_t px, uint32_t py, uint32_t si, int ndepth,
-                      float* out, uint32_t* out_seed) {
-  PathSampler ps(px, py, si, spp, rx, ry);
-  ps.s2(0, out);                       // CAMERA_SAMPLE_DIM
-  out_seed[0] = ps.aux_seed(0x80000000u);  // INITIAL_AUX_RNG_SEED_DIM
-  for (int d = 0; d < ndepth; ++d) {
-    uint32_t base = 2u + (uint32_t)d * 16u;
-    float* o = out + 2 + 14 * d;
-    o[0] = ps.s1(base + 0);
-    o[1] = ps.s1(base + 1);
-    o[2] = ps.s1(base + 2);
-    ps.s2(base + 3, o + 3);
-    o[5] = ps.s1(base + 5);
-    o[6] = ps.s1(base + 6);
-    ps.s2(base + 7, o + 7);
-    o[9] = ps.s1(base + 9);
-    o[10] = ps.s1(base + 10);
-    o[11] = ps.s1(base + 11);
-    o[12] = ps.s1(base + 12);
-    o[13] = ps.s1(base + 13);
-    out_seed[1 + d] = ps.aux_seed(base + 14);
-  }
-}
-
-// PinholeCamera::generate_ray_differential (R/scene/camera.rs:48-78) for every
-// pixel of a (rx, ry) image and one sample index.  cam: eye[3], forward[3],
-// right[3], image_up[3], tan_half_fov_y, spread (= pixel_spread_angle, the
-// atan of the footprint).  out per pixel (row-major): origin[3], dir[3],
-// rx_dir[3], ry_dir[3], cone_spread (13 floats).
-void orc_camera_rays(const float* cam, uint32_t rx, uint32_t ry, uint32_t spp, uint32_t si, float* out) {
-  F3 eye{cam[0], cam[1], cam[2]}, fwd{cam[3], cam[4], cam[5]}, right{cam[6], cam[7], cam[8]},
-      up{cam[9], cam[10], cam[11]};
-  const float th = cam[12], spread = cam[13];
-  const float rxf = (float)rx, ryf = (float)ry;
-  const float aspect = rxf / ryf;
-  auto dir_for = [&](float px, float py) {
-    float sx = ((2.0f * px / rxf) - 1.0f) * aspect * th;
-    float sy = (1.0f - (2.0f * py / ryf)) * th;
-    return normalize(add(add(fwd, scale(sx, right)), scale(sy, up)));
-  };
-  const float sc = std::fmax(1.0f / std::sqrt((float)(spp < 1 ? 1 : spp)), 0.125f);  // differential_scale
-  for (uint32_t y = 0; y < ry; ++y)
-    for (uint32_t x = 0; x < rx; ++x) {
-      PathSampler ps(x, y, si, spp, rx, ry);
-      float u[2];
-      ps.s2(0, u);
-      const float px = ((float)x + 0.5f) + 0.5f * tent(u[0]);
-      const float py = ((float)y + 0.5f) + 0.5f * tent(u[1]);
-      F3 d = dir_for(px, py);
-      F3 dxd = dir_for(px + 1.0f, py);
-      F3 dyd = dir_for(px, py + 1.0f);
-      F3 rxd = normalize_or_zero(add(d, scale(sc, sub(dxd, d))));
-      F3 ryd = normalize_or_zero(add(d, scale(sc, sub(dyd, d))));
-      float* o = out + 13 * ((size_t)y * rx + x);
-      o[0] = eye.x; o[1] = eye.y; o[2] = eye.z;
-      o[3] = d.x; o[4] = d.y; o[5] = d.z;
-      o[6] = rxd.x; o[7] = rxd.y; o[8] = rxd.z;
-      o[9] = ryd.x; o[10] = ryd.y; o[11] = ryd.z;
-      o[12] = std::fmax(spread * sc, 0.0f);  // RayCone::new(0, spread * scale)
-    }
-}
-
-}  // extern "C"
+}  // namespace orc

@@ -55,6 +55,13 @@ def lib() -> ctypes.CDLL:
         L.orc_scene_mesh_qbvh.restype = vp
         L.orc_scene_tlas.argtypes = [vp]
         L.orc_scene_tlas.restype = vp
+        L.orc_render_scene_new.argtypes = [vp, ctypes.c_uint32]
+        L.orc_render_scene_new.restype = vp
+        L.orc_render_scene_free.argtypes = [vp]
+        L.orc_render_set_mesh.argtypes = [vp, ctypes.c_uint32, vp, vp, sz]
+        L.orc_render_set_instance_material.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_render_set_materials.argtypes = [vp, vp, sz]
+        L.orc_render.argtypes = [vp, vp] + [ctypes.c_uint32] * 6 + [vp, ctypes.c_int]
         L.orc_closest_hit.argtypes = [vp, vp, sz, vp, vp, ctypes.c_int]
         L.orc_closest_hit.restype = ctypes.c_int
         L.orc_unoccluded.argtypes = [vp, vp, sz, vp, vp, vp, vp, ctypes.c_int]
@@ -185,3 +192,35 @@ def oracle_scene_from(scene) -> OracleScene:
         o.add_instance(inst.mesh_index, inst.local_to_world, inst.world_to_local)
     o.build()
     return o
+
+
+class OracleRenderer:
+    """The oracle's MIS path tracer over a tpt.Scene with OpenPBR materials."""
+
+    def __init__(self, scene, materials):
+        from tpt.material import records
+        self.geo = oracle_scene_from(scene)
+        self.h = lib().orc_render_scene_new(self.geo.h, len(scene.meshes))
+        self._keep = []
+        for i, m in enumerate(scene.meshes):
+            n = None if m.normals is None else np.ascontiguousarray(m.normals, np.float32)
+            uv = None if m.uvs is None else np.ascontiguousarray(m.uvs, np.float32)
+            self._keep += [n, uv]
+            lib().orc_render_set_mesh(self.h, i, _p(n), _p(uv), len(m.positions))
+        for i, inst in enumerate(scene.instances):
+            lib().orc_render_set_instance_material(self.h, i, inst.material_index)
+        self.recs = records(materials)
+        lib().orc_render_set_materials(self.h, ctypes.addressof(self.recs), len(materials))
+
+    def render(self, cam, width, height, spp, max_depth, rows=None, threads=8):
+        """cam: tpt.camera.pinhole(...) record.  Returns (height, width, 3) f32."""
+        c = np.array(list(cam.eye) + list(cam.forward) + list(cam.right) + list(cam.image_up)
+                     + [cam.tan_half_fov_y, cam.pixel_spread, cam.exposure], np.float32)
+        out = np.zeros((height, width, 3), np.float32)
+        y0, y1 = rows if rows else (0, height)
+        lib().orc_render(self.h, _p(c), width, height, spp, max_depth, y0, y1, _p(out), threads)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_render_scene_free(self.h)

@@ -1,0 +1,737 @@
+// Plain FP32 megakernel path tracer: the render loop of R/main.rs:98-126 with
+// the unidirectional MIS integrator of R/integrator/mis.rs:19-139, over the
+// OpenPBR material subset tpt_render accepts (include/tpt.h).  One thread per
+// pixel runs its samples in order, so the reference's running mean
+// `color += (s - color) / n` is reproduced exactly.
+//
+// Per-sample work: PathSampler (bit-exact ZSobol), pinhole ray, two-level
+// closest hit (the trace kernels' traversal), shading vertex
+// (R/scene/mod.rs:471-544), OpenPBR sample (R/material/open_pbr.rs:489-548 ->
+// R/bsdf/open_pbr.rs:295-331), Russian roulette, emission with MIS.
+//
+// OpenPBR subset: specular weight 0 makes the modulated specular eta exactly
+// 1, so f_avg(eta) = 0 (R/bsdf/open_pbr.rs:2127-2135) and every term that
+// reads the reference's directional-albedo and energy-compensation tables is
+// multiplied by zero; metal, coat, fuzz, transmission, thin film and thin
+// walls are zero.  Light NEE under the reference policy gives no sample for
+// OpenPBR shading points (no light-tree query, no environment), so emission
+// found by BSDF sampling has MIS weight 1 (R/integrator/mis.rs:212-235).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "sampler.h"
+#include "tpt_device.h"
+#include "tpt_internal.h"
+#include "traverse.h"
+
+namespace tpt {
+namespace pt {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTau = 6.28318530717958647692f;
+
+// ---- glam-style vector helpers (component-wise f32, no contraction) ----------
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 scl(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ F3 muls(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 divs(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ F3 hadamard(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ F3 neg(F3 a) { return f3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+  return f3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ float len2(F3 a) { return dot(a, a); }
+__device__ __forceinline__ F3 normalize(F3 a) {  // glam: v * (1 / sqrt(dot))
+  const float r = 1.0f / sqrtf(dot(a, a));
+  return f3(a.x * r, a.y * r, a.z * r);
+}
+__device__ __forceinline__ F3 normalize_or_zero(F3 a) {
+  const float r = 1.0f / sqrtf(dot(a, a));
+  if (__builtin_isfinite(r) && r > 0.0f) return f3(a.x * r, a.y * r, a.z * r);
+  return f3(0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+__device__ __forceinline__ F3 face_forward(F3 n, F3 r) { return dot(n, r) < 0.0f ? neg(n) : n; }
+__device__ __forceinline__ float luminance(F3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+__device__ __forceinline__ F3 interp(float b0, float b1, float b2, F3 a, F3 b, F3 c) {
+  return add(add(scl(b0, a), scl(b1, b)), scl(b2, c));
+}
+
+// ---- OrthonormalBasis (Duff et al.), R/math/orthonormal_basis.rs:11-71 -------
+struct Frame {
+  F3 t, b, n;
+};
+__device__ __forceinline__ Frame frame_from_normal(F3 normal) {
+  const F3 n = len2(normal) > 0.0f ? normalize(normal) : f3(0.0f, 0.0f, 1.0f);
+  const float sign = copysignf(1.0f, n.z);
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  return Frame{f3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x), f3(b, sign + n.y * n.y * a, -n.y), n};
+}
+__device__ __forceinline__ Frame frame_from_normal_tangent(F3 normal, F3 hint) {
+  const F3 n = len2(normal) > 0.0f ? normalize(normal) : f3(0.0f, 0.0f, 1.0f);
+  F3 t = normalize_or_zero(sub(hint, scl(dot(hint, n), n)));
+  if (len2(t) == 0.0f) return frame_from_normal(n);
+  const F3 b = normalize_or_zero(cross(n, t));
+  t = normalize_or_zero(cross(b, n));
+  return Frame{t, b, n};
+}
+__device__ __forceinline__ F3 to_world(const Frame& f, F3 l) {
+  return normalize(add(add(scl(l.x, f.t), scl(l.y, f.b)), scl(l.z, f.n)));
+}
+__device__ __forceinline__ F3 to_frame(const Frame& f, F3 w) { return f3(dot(w, f.t), dot(w, f.b), dot(w, f.n)); }
+
+// ---- GGX (R/bsdf/smith_ggx.rs) -------------------------------------------------
+constexpr float kMinAlpha = 1.0e-4f;
+constexpr float kSmoothAlpha = 1.0e-3f;
+__device__ __forceinline__ bool half_vector(F3 wo, F3 wi, F3* wm) {
+  const F3 m = normalize_or_zero(add(wo, wi));
+  if (len2(m) == 0.0f) return false;
+  *wm = m.z < 0.0f ? neg(m) : m;
+  return true;
+}
+__device__ __forceinline__ float ggx_d(F3 m, float ax, float ay) {
+  if (m.z <= 0.0f) return 0.0f;
+  const float term = m.x * m.x / (ax * ax) + m.y * m.y / (ay * ay) + m.z * m.z;
+  const float denom = kPi * ax * ay * term * term;
+  return denom <= 0.0f ? 0.0f : 1.0f / denom;
+}
+__device__ __forceinline__ float ggx_lambda(F3 w, float ax, float ay) {
+  const float c = fabsf(w.z);
+  if (c <= 0.0f) return __builtin_inff();
+  const float term = 1.0f + (ax * ax * w.x * w.x + ay * ay * w.y * w.y) / (c * c);
+  return 0.5f * (-1.0f + sqrtf(term));
+}
+__device__ __forceinline__ float ggx_g2(F3 wo, F3 wi, float ax, float ay) {
+  const float lo = ggx_lambda(wo, ax, ay), li = ggx_lambda(wi, ax, ay);
+  if (!__builtin_isfinite(lo) || !__builtin_isfinite(li)) return 0.0f;
+  return 1.0f / (1.0f + lo + li);
+}
+__device__ __forceinline__ float pdf_bounded_vndf(F3 wo, F3 wm, float ax, float ay) {
+  if (wo.z <= 0.0f || wm.z <= 0.0f) return 0.0f;
+  ax = fmaxf(ax, kMinAlpha);
+  ay = fmaxf(ay, kMinAlpha);
+  const float a = clampf(fminf(ax, ay), 0.0f, 1.0f);
+  const float s = 1.0f + sqrtf(wo.x * wo.x + wo.y * wo.y);
+  const float a2 = a * a, s2 = s * s;
+  const float k = (1.0f - a2) * s2 / (s2 + a2 * wo.z * wo.z);
+  const float t = sqrtf(ax * ax * wo.x * wo.x + ay * ay * wo.y * wo.y + wo.z * wo.z);
+  const float denom = k * wo.z + t;
+  if (denom <= 0.0f) return 0.0f;
+  return 2.0f * ggx_d(wm, ax, ay) * fmaxf(dot(wo, wm), 0.0f) / denom;
+}
+
+// ---- Fresnel (R/math/mod.rs:50-72, R/bsdf/open_pbr.rs:2161-2172) ----------------
+__device__ __forceinline__ float fresnel_dielectric(float ci, float ei, float et) {
+  ci = clampf(ci, -1.0f, 1.0f);
+  if (ci < 0.0f) {
+    const float t = ei;
+    ei = et;
+    et = t;
+    ci = -ci;
+  }
+  const float si = sqrtf(fmaxf(1.0f - ci * ci, 0.0f));
+  const float st = (ei / et) * si;
+  if (st >= 1.0f) return 1.0f;
+  const float ct = sqrtf(fmaxf(1.0f - st * st, 0.0f));
+  const float rpar = (et * ci - ei * ct) / (et * ci + ei * ct);
+  const float rper = (ei * ci - et * ct) / (ei * ci + et * ct);
+  return 0.5f * (rpar * rpar + rper * rper);
+}
+__device__ __forceinline__ float weighted_fresnel_ratio(float c, float eti, float etp) {
+  eti = fmaxf(eti, 1.0e-4f);
+  etp = fmaxf(etp, 1.0e-4f);
+  if (etp >= 1.0f) return fresnel_dielectric(c, 1.0f, etp);
+  const float cc = clampf(c, 0.0f, 1.0f);
+  const float mu2 = 1.0f - (1.0f - cc * cc) / (eti * eti);
+  if (mu2 <= 0.0f) return 1.0f;
+  return fresnel_dielectric(sqrtf(mu2), 1.0f, 1.0f / etp);
+}
+
+// ---- EON diffuse (R/bsdf/eon.rs:12-197), rho = 1 ----------------------------------
+constexpr float kRcpPi = 1.0f / kPi;
+__device__ __forceinline__ float c1_fon() { return 0.5f - 2.0f / (3.0f * kPi); }
+__device__ __forceinline__ float c2_fon() { return 2.0f / 3.0f - 28.0f / (15.0f * kPi); }
+__device__ __forceinline__ float e_fon(float mu, float r) {
+  const float mc = 1.0f - mu;
+  const float g = mc * (0.05710853f + mc * (0.49188188f + mc * (-0.33218145f + mc * 0.071443f)));
+  return (1.0f + r * g) / (1.0f + c1_fon() * r);
+}
+// f_eon with rho = (1,1,1): every channel is the same scalar.
+__device__ __forceinline__ float f_eon1(float r, F3 wo, F3 wi) {
+  const float mo = wo.z, mi = wi.z;
+  const float s = dot(wo, wi) - mi * mo;
+  const float sot = s > 0.0f ? s / fmaxf(mo, mi) : s;
+  const float af = 1.0f / (1.0f + c1_fon() * r);
+  const float fss = (kRcpPi * af * (1.0f + r * sot)) * 1.0f;
+  const float efo = e_fon(mo, r), efi = e_fon(mi, r);
+  const float avg = af * (1.0f + c2_fon() * r);
+  const float oma = 1.0f - avg;
+  const float denom = 1.0f - 1.0f * oma;
+  const float rho_ms = 1.0f * 1.0f * avg / denom;
+  const float scale = kRcpPi * fmaxf(1.0f - efo, 1.0e-7f) * fmaxf(1.0f - efi, 1.0e-7f) / fmaxf(oma, 1.0e-7f);
+  return fss + rho_ms * scale;
+}
+__device__ __forceinline__ void ltc_coeffs(float mu, float r, float* a, float* b, float* c, float* d) {
+  *a = 1.0f + r * (0.303392f + (-0.518982f + 0.111709f * mu) * mu + (-0.276266f + 0.335918f * mu) * r);
+  *b = r * (-1.16407f + 1.15859f * mu + (0.150815f - 0.150105f * mu) * r) / (mu * mu * mu - 1.43545f);
+  *c = 1.0f + r * (0.20013f + (-0.506373f + 0.261777f * mu) * mu);
+  *d = r * (0.540852f + (-1.01625f + 0.475392f * mu) * mu) / (-1.0743f + (0.0725628f + mu) * mu);
+}
+__device__ __forceinline__ void ltc_x(F3 wo, float* x, float* y) {
+  const float l2 = wo.x * wo.x + wo.y * wo.y;
+  if (l2 > 0.0f) {
+    const float inv = 1.0f / sqrtf(l2);
+    *x = wo.x * inv;
+    *y = wo.y * inv;
+  } else {
+    *x = 1.0f;
+    *y = 0.0f;
+  }
+}
+__device__ __forceinline__ float cltc_pdf(F3 wo, F3 wi, float r) {
+  float xx, xy;
+  ltc_x(wo, &xx, &xy);
+  const float yx = -xy, yy = xx;
+  const F3 wl = f3(xx * wi.x + xy * wi.y, yx * wi.x + yy * wi.y, wi.z);
+  float a, b, c, d;
+  ltc_coeffs(wo.z, r, &a, &b, &c, &d);
+  const float det = c * (a - b * d);
+  const F3 wh = f3(c * (wl.x - b * wl.z), (a - b * d) * wl.y, -c * (d * wl.x - a * wl.z));
+  const float l2 = fmaxf(dot(wh, wh), 1.0e-30f);
+  const float vz = 1.0f / sqrtf(d * d + 1.0f);
+  const float s = 0.5f * (1.0f + vz);
+  const float q = det / l2;
+  return q * q * fmaxf(wh.z, 0.0f) / (kPi * s);
+}
+__device__ __forceinline__ float p_uniform(float mu, float r) {
+  return powf(r, 0.1f) * (0.162925f + (-0.372058f + (0.538233f - 0.290822f * mu) * mu) * mu);
+}
+__device__ __forceinline__ float pdf_eon(F3 wo, F3 wi, float r) {
+  const float pu = p_uniform(wo.z, r);
+  return pu * (1.0f / (2.0f * kPi)) + (1.0f - pu) * cltc_pdf(wo, wi, r);
+}
+__device__ __forceinline__ F3 sample_eon(F3 wo, float r, float u1, float u2, float* pdf) {
+  const float pu = p_uniform(wo.z, r);
+  const float pc = 1.0f - pu;
+  F3 wi;
+  float pdf_c;
+  if (u1 < pu) {
+    const float u1n = u1 / pu;
+    const float st = sqrtf(fmaxf(1.0f - u1n * u1n, 0.0f));
+    const float phi = 2.0f * kPi * u2;
+    wi = f3(st * cosf(phi), st * sinf(phi), u1n);
+    pdf_c = cltc_pdf(wo, wi, r);
+  } else {
+    const float u1n = (u1 - pu) / fmaxf(pc, 1.0e-30f);
+    float a, b, c, d;
+    ltc_coeffs(wo.z, r, &a, &b, &c, &d);
+    const float radius = sqrtf(u1n);
+    const float phi = 2.0f * kPi * u2;
+    const float x0 = radius * cosf(phi), y0 = radius * sinf(phi);
+    const float vz = 1.0f / sqrtf(d * d + 1.0f);
+    const float s = 0.5f * (1.0f + vz);
+    const float edge = sqrtf(fmaxf(1.0f - y0 * y0, 0.0f));
+    const float x = -((1.0f - s) * edge + s * x0);
+    const float y = y0;
+    const F3 wh = f3(x, y, sqrtf(fmaxf(1.0f - (x * x + y * y), 0.0f)));
+    const float pdf_wh = wh.z / (kPi * s);
+    const F3 wl = f3(a * wh.x + b * wh.z, c * wh.y, d * wh.x + wh.z);
+    const float len = sqrtf(dot(wl, wl));
+    const float det = c * (a - b * d);
+    pdf_c = pdf_wh * (len * len * len) / det;
+    const F3 v = divs(wl, len);
+    float xx, xy;
+    ltc_x(wo, &xx, &xy);
+    const float yx = -xy, yy = xx;
+    wi = f3(xx * v.x + yx * v.y, xy * v.x + yy * v.y, v.z);
+  }
+  *pdf = pu * (1.0f / (2.0f * kPi)) + pc * pdf_c;
+  return wi;
+}
+
+// ---- OpenPBR subset ---------------------------------------------------------------------
+struct Bsdf {
+  F3 base_color, spec_color;
+  float base, ax, ay, spec_eta, trans_eta, diff_rough;
+  bool front;
+};
+
+__device__ __forceinline__ float modulated_eta(float eta, float w) {  // R/material/open_pbr.rs:871-878
+  if (fabsf(w - 1.0f) < 1.0e-6f) return eta;
+  const float q = (eta - 1.0f) / (eta + 1.0f);
+  const float f0 = q * q;
+  const float sg = (eta - 1.0f) < 0.0f ? -1.0f : 1.0f;
+  const float eps = sqrtf(clampf(w * f0, 0.0f, 0.99999f)) * sg;
+  return fmaxf((1.0f + eps) / fmaxf(1.0f - eps, 1.0e-6f), 1.0e-4f);
+}
+
+__device__ __forceinline__ Bsdf make_bsdf(const TptOpenPbr& m, bool front) {  // :393-487
+  Bsdf p;
+  p.base_color = f3(m.base_color[0], m.base_color[1], m.base_color[2]);
+  const float rough = clampf(m.specular_roughness, 0.0f, 1.0f);
+  const float an = clampf(m.specular_roughness_anisotropy, 0.0f, 1.0f);
+  const float alpha = rough * rough, oma = 1.0f - an;
+  const float axr = alpha * sqrtf(2.0f / (1.0f + oma * oma));
+  p.ax = clampf(axr, kMinAlpha, 1.0f);
+  p.ay = clampf(oma * axr, kMinAlpha, 1.0f);
+  p.spec_color = f3(clampf(m.specular_color[0], 0.0f, 1.0f), clampf(m.specular_color[1], 0.0f, 1.0f),
+                    clampf(m.specular_color[2], 0.0f, 1.0f));
+  p.spec_eta = modulated_eta(fmaxf(m.specular_ior, 1.0e-4f), fmaxf(m.specular_weight, 0.0f));
+  p.trans_eta = p.spec_eta;
+  p.base = clampf(m.base_weight, 0.0f, 1.0f);
+  p.diff_rough = clampf(m.base_diffuse_roughness, 0.0f, 1.0f);
+  p.front = front;
+  return p;
+}
+
+// layer_weights (R/bsdf/open_pbr.rs:333-390) of the subset: fuzz, coat,
+// metal, transmission and subsurface are zero and e_spec = 0.
+__device__ __forceinline__ void layer_weights(const Bsdf& p, F3* spec_w, F3* diff_w) {
+  const F3 below_metal = f3(1.0f, 1.0f, 1.0f);
+  *spec_w = hadamard(below_metal, p.spec_color);
+  const F3 leak = f3(fmaxf(1.0f - p.spec_color.x * 0.0f, 0.0f), fmaxf(1.0f - p.spec_color.y * 0.0f, 0.0f),
+                     fmaxf(1.0f - p.spec_color.z * 0.0f, 0.0f));
+  const F3 below_trans = muls(hadamard(below_metal, leak), 1.0f - 0.0f);
+  *diff_w = hadamard(muls(below_trans, (1.0f - 0.0f) * p.base), p.base_color);
+}
+struct Probs {
+  float spec, diff, total;
+};
+__device__ __forceinline__ Probs lobe_probs(const Bsdf& p) {  // :392-422
+  F3 sw, dw;
+  layer_weights(p, &sw, &dw);
+  const float zero_lum = luminance(f3(0.0f, 0.0f, 0.0f));
+  const float spec = luminance(sw);
+  const float diff = fmaxf(luminance(dw), 0.0f);
+  const float total = 0.0f + zero_lum + spec + 0.0f + fmaxf(zero_lum, 0.0f) + diff + fmaxf(zero_lum, 0.0f);
+  return Probs{spec, diff, total};
+}
+__device__ __forceinline__ bool spec_smooth(const Bsdf& p) { return fmaxf(p.ax, p.ay) < kSmoothAlpha; }
+__device__ __forceinline__ float spec_fresnel(const Bsdf& p, float c) {
+  return p.front ? weighted_fresnel_ratio(c, p.trans_eta, p.spec_eta)
+                 : weighted_fresnel_ratio(c, 1.0f / p.trans_eta, 1.0f / p.spec_eta);
+}
+__device__ __forceinline__ float spec_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_spec_brdf :841-867
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  F3 wm;
+  if (!half_vector(wo, wi, &wm)) return 0.0f;
+  const float d = ggx_d(wm, p.ax, p.ay);
+  const float g = ggx_g2(wo, wi, p.ax, p.ay);
+  if (d <= 0.0f || g <= 0.0f) return 0.0f;
+  const float ss = spec_fresnel(p, fabsf(dot(wo, wm))) * (d * g / (4.0f * wo.z * wi.z));
+  return ss + 0.0f * 0.0f;  // multiple-scattering factor: f_avg(1) = 0
+}
+__device__ __forceinline__ float spec_pdf(const Bsdf& p, F3 wo, F3 wi) {  // pdf_specular_brdf :1006-1022
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  float pdf_ss = 0.0f;
+  F3 wm;
+  if (half_vector(wo, wi, &wm)) {
+    const float pw = pdf_bounded_vndf(wo, wm, p.ax, p.ay);
+    const float den = 4.0f * fabsf(dot(wo, wm));
+    pdf_ss = den <= 0.0f ? 0.0f : pw / den;
+  }
+  const float pdf_ms = fmaxf(wi.z, 0.0f) / kPi;
+  const float ss_w = 0.0f, ms_w = 0.0f, total = fmaxf(ss_w + ms_w, 1.0e-8f);
+  return (ss_w * pdf_ss + ms_w * pdf_ms) / total;
+}
+__device__ __forceinline__ F3 bsdf_eval(const Bsdf& p, F3 wo, F3 wi) {  // :224-259
+  if (!(wo.z > 0.0f)) return f3(0.0f, 0.0f, 0.0f);
+  F3 sw, dw;
+  layer_weights(p, &sw, &dw);
+  F3 total = f3(0.0f, 0.0f, 0.0f);
+  if (len2(sw) > 0.0f && !spec_smooth(p)) total = add(total, muls(sw, spec_eval(p, wo, wi)));
+  if (len2(dw) > 0.0f) {
+    const float f = (wo.z <= 0.0f || wi.z <= 0.0f) ? 0.0f : f_eon1(p.diff_rough, wo, wi);
+    total = add(total, hadamard(dw, f3(f, f, f)));
+  }
+  return total;
+}
+__device__ __forceinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {  // :261-293
+  if (!(wo.z > 0.0f)) return 0.0f;
+  const Probs pr = lobe_probs(p);
+  if (pr.total <= 0.0f) return 0.0f;
+  float pdf = 0.0f;
+  if (pr.spec > 0.0f && !spec_smooth(p)) pdf += (pr.spec / pr.total) * spec_pdf(p, wo, wi);
+  if (pr.diff > 0.0f) {
+    const float pd = (wo.z <= 0.0f || wi.z <= 0.0f) ? 0.0f : pdf_eon(wo, wi, p.diff_rough);
+    pdf += (pr.diff / pr.total) * pd;
+  }
+  return pdf;
+}
+
+struct Sample {
+  F3 weight, wi;
+  float pdf;
+  bool delta;
+};
+__device__ __forceinline__ bool finalize(const Bsdf& p, F3 wo, F3 wi, Sample* out) {  // :1958-1999
+  const float pdf = bsdf_pdf(p, wo, wi);
+  if (pdf <= 0.0f) return false;
+  const F3 f = bsdf_eval(p, wo, wi);
+  if (len2(f) == 0.0f) return false;
+  const float c = fabsf(wi.z);
+  if (c <= 0.0f) return false;
+  out->weight = muls(f, c / pdf);
+  out->wi = wi;
+  out->pdf = pdf;
+  out->delta = false;
+  return true;
+}
+__device__ __forceinline__ F3 cosine_hemisphere(float u, float v) {  // R/math/mod.rs:26-36
+  const float r = sqrtf(clampf(u, 0.0f, 1.0f));
+  const float phi = kTau * v;
+  const float dx = r * cosf(phi), dy = r * sinf(phi);
+  return f3(dx, dy, sqrtf(fmaxf(1.0f - (dx * dx + dy * dy), 0.0f)));
+}
+// OpenPbrBsdf::sample (:295-331), pick_lobe (:2016-2041), sample_spec_brdf
+// (:1589-1668), sample_diff_brdf (:1936-1940)
+__device__ __forceinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, float ux, float uy, Sample* out) {
+  if (!(wo.z > 0.0f)) return false;
+  const Probs pr = lobe_probs(p);
+  if (pr.total <= 0.0f) return false;
+  float u = u_lobe * pr.total;
+  if (u < 0.0f) return false;  // coat (probability 0)
+  if (u < 0.0f) return false;  // metal
+  bool spec;
+  if (u < pr.spec) {
+    spec = true;
+  } else {
+    spec = false;
+    u -= pr.spec;
+    if (!(u < pr.diff)) return false;  // DiffBtdf: probability 0 -> no sample
+  }
+  const float p_lobe = (spec ? pr.spec : pr.diff) / pr.total;
+  if (p_lobe <= 0.0f) return false;
+  if (spec) {
+    if (spec_smooth(p)) {
+      const F3 wi = f3(-wo.x, -wo.y, wo.z);
+      F3 sw, dw;
+      layer_weights(p, &sw, &dw);
+      out->weight = divs(muls(sw, spec_fresnel(p, fabsf(wi.z))), p_lobe);
+      out->wi = wi;
+      out->pdf = p_lobe;
+      out->delta = true;
+      return true;
+    }
+    const float ss_prob = 0.0f / 1.0e-8f;  // dielectric reflection lobe weights at eta 1
+    if (ux >= ss_prob) {
+      const float uu = ss_prob < 1.0f ? (ux - ss_prob) / (1.0f - ss_prob) : 0.0f;
+      return finalize(p, wo, cosine_hemisphere(uu, uy), out);
+    }
+    return false;
+  }
+  float pdf;
+  const F3 wi = sample_eon(wo, p.diff_rough, ux, uy, &pdf);
+  if (pdf <= 0.0f || wi.z <= 0.0f) return false;
+  return finalize(p, wo, wi, out);
+}
+
+// ---- scene shading -----------------------------------------------------------------------
+struct ShadeView {
+  const GpuShade* inst;
+  const float* pos;
+  const float* nrm;
+  const float* uv;
+  const uint32_t* idx;
+  const TptOpenPbr* mats;
+};
+
+struct Vtx {
+  F3 p, ng, ns, wo;
+  Frame frame;
+  bool front;
+  uint32_t material;
+};
+
+__device__ __forceinline__ F3 load3(const float* a, uint32_t i) { return f3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+__device__ __forceinline__ F3 xform_point(const float* m, F3 p) {
+  float rx = m[0] * p.x, ry = m[1] * p.x, rz = m[2] * p.x;
+  rx = m[3] * p.y + rx; ry = m[4] * p.y + ry; rz = m[5] * p.y + rz;
+  rx = m[6] * p.z + rx; ry = m[7] * p.z + ry; rz = m[8] * p.z + rz;
+  return f3(m[9] + rx, m[10] + ry, m[11] + rz);
+}
+__device__ __forceinline__ F3 xform_normal(const float* n, F3 v) {  // Mat3::mul_vec3
+  return add(add(scl(v.x, f3(n[0], n[1], n[2])), scl(v.y, f3(n[3], n[4], n[5]))), scl(v.z, f3(n[6], n[7], n[8])));
+}
+
+// base_shading_vertex_from_triangle_sample_impl (R/scene/mod.rs:471-544)
+__device__ __forceinline__ Vtx shading_vertex(const ShadeView& sv, uint32_t inst, uint32_t tri, float b0, float b1,
+                                              float b2, F3 dir) {
+  const GpuShade& si = sv.inst[inst];
+  const uint32_t i0 = si.vbase + sv.idx[si.ibase + 3 * tri], i1 = si.vbase + sv.idx[si.ibase + 3 * tri + 1],
+                 i2 = si.vbase + sv.idx[si.ibase + 3 * tri + 2];
+  const F3 p0 = xform_point(si.l2w, load3(sv.pos, i0)), p1 = xform_point(si.l2w, load3(sv.pos, i1)),
+           p2 = xform_point(si.l2w, load3(sv.pos, i2));
+  const bool has_n = si.flags & 1u, has_uv = si.flags & 2u;
+  const F3 zero = f3(0.0f, 0.0f, 0.0f);
+  const F3 n0 = normalize_or_zero(xform_normal(si.nrm, has_n ? load3(sv.nrm, i0) : zero));
+  const F3 n1 = normalize_or_zero(xform_normal(si.nrm, has_n ? load3(sv.nrm, i1) : zero));
+  const F3 n2 = normalize_or_zero(xform_normal(si.nrm, has_n ? load3(sv.nrm, i2) : zero));
+  Vtx v;
+  v.p = interp(b0, b1, b2, p0, p1, p2);
+  const F3 gn = normalize_or_zero(cross(sub(p1, p0), sub(p2, p0)));
+  const F3 sn = normalize_or_zero(interp(b0, b1, b2, n0, n1, n2));
+  const F3 nd = neg(dir);
+  v.front = dot(gn, nd) >= 0.0f;
+  v.ng = face_forward(gn, nd);
+  v.ns = face_forward(len2(sn) > 0.0f ? sn : v.ng, v.ng);
+  // compute_surface_partials over world positions (R/math/mod.rs:112-134)
+  const float u0 = has_uv ? sv.uv[2 * i0] : 0.0f, w0 = has_uv ? sv.uv[2 * i0 + 1] : 0.0f;
+  const float u1 = has_uv ? sv.uv[2 * i1] : 0.0f, w1 = has_uv ? sv.uv[2 * i1 + 1] : 0.0f;
+  const float u2 = has_uv ? sv.uv[2 * i2] : 0.0f, w2 = has_uv ? sv.uv[2 * i2 + 1] : 0.0f;
+  const float du1 = u1 - u0, dv1 = w1 - w0, du2 = u2 - u0, dv2 = w2 - w0;
+  const float det = du1 * dv2 - dv1 * du2;
+  F3 dpdu = zero;
+  bool ok = false;
+  if (!(fabsf(det) <= 1.0e-8f)) {
+    const float inv = 1.0f / det;
+    const F3 dp1 = sub(p1, p0), dp2 = sub(p2, p0);
+    dpdu = muls(sub(scl(dv2, dp1), scl(dv1, dp2)), inv);
+    const F3 dpdv = muls(add(scl(-du2, dp1), scl(du1, dp2)), inv);
+    ok = !(len2(dpdu) == 0.0f || len2(dpdv) == 0.0f);
+  }
+  if (!ok) dpdu = frame_from_normal(v.ns).t;
+  v.frame = frame_from_normal_tangent(v.ns, dpdu);
+  v.wo = normalize_or_zero(nd);
+  v.material = si.material;
+  return v;
+}
+
+// OpenPbrMaterial::le (R/material/open_pbr.rs:351-375); the exit tint is 1
+// without coat and fuzz (0 at a grazing exit).
+__device__ __forceinline__ bool material_le(const TptOpenPbr& m, const Vtx& v, F3* le) {
+  if (m.emission_luminance <= 0.0f) return false;
+  if (!m.thin_walled && !v.front) return false;
+  const F3 wl = normalize_or_zero(to_frame(v.frame, v.wo));
+  const float co = clampf(fabsf(wl.z), 0.0f, 1.0f);
+  const float tint = co <= 0.0f ? 0.0f : 1.0f;
+  const F3 c = muls(f3(m.emission_color[0], m.emission_color[1], m.emission_color[2]),
+                    fmaxf(m.emission_luminance, 0.0f));
+  *le = hadamard(c, f3(tint, tint, tint));
+  return true;
+}
+
+// OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548), radiance mode.
+__device__ __forceinline__ bool material_sample(const TptOpenPbr& m, const Vtx& v, float u_lobe, float ux, float uy,
+                                                Sample* s, F3* wi_world) {
+  if (dot(v.wo, v.ng) <= 0.0f) return false;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  if (wol.z <= 0.0f) return false;
+  const Bsdf p = make_bsdf(m, v.front);
+  if (!bsdf_sample(p, wol, u_lobe, ux, uy, s)) return false;
+  const F3 wi = to_world(v.frame, s->wi);
+  if (dot(wi, v.ng) <= 1.0e-6f) return false;  // GEOMETRIC_NORMAL_COS_EPSILON, reflection lobes
+  *wi_world = wi;
+  return true;
+}
+
+__device__ __forceinline__ F3 offset_origin(F3 p, F3 ng, F3 d) {  // R/integrator/mod.rs:76-85
+  return add(p, scl(1.0e-4f, dot(d, ng) >= 0.0f ? ng : neg(ng)));
+}
+
+struct CamView {
+  F3 eye, fwd, right, up;
+  float tan_half, exposure;
+};
+
+// closest hit of one ray with the trace kernels' lane traversal
+template <bool kTwoLevel>
+__device__ __forceinline__ bool closest(const SceneView& sc, F3 o, F3 d, Lane& L, LaneStack& st, uint32_t* inst,
+                                        uint32_t* tri, float* t, float* b0, float* b1, float* b2) {
+  lane_begin<kTwoLevel>(sc, L, st, o, d, __builtin_inff(), __builtin_inff(), TPT_INVALID_INDEX, TPT_INVALID_INDEX);
+  Counters cnt = {0, 0, 0};
+  while (!lane_step<false, kClosest, false, kTwoLevel>(sc, L, st, cnt)) {
+  }
+  *inst = L.inst;
+  *tri = L.tri;
+  *t = L.t;
+  *b0 = L.b0;
+  *b1 = L.b1;
+  *b2 = L.b2;
+  return L.hit;
+}
+
+template <bool kTwoLevel>
+__global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
+                                                        uint32_t h, uint32_t spp, uint32_t max_depth,
+                                                        float* __restrict__ rgb, uint32_t* __restrict__ status) {
+  __shared__ uint32_t lds[kLdsDepth * kBlock];
+  uint32_t spill[kStackDepth - kLdsDepth];
+  const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+  if (pix >= w * h) return;
+  LaneStack st;
+  st.lds = lds + threadIdx.x;
+  st.spill = spill;
+  st.sp = 0;
+  st.overflow = false;
+  Lane L;
+  bool overflow = false;
+  const uint32_t x = pix % w, y = pix / w;
+  const float rx = (float)w, ry = (float)h, aspect = rx / ry;
+  const smp::ZSobolFrame zf = smp::zsobol_frame(spp, w, h);
+  F3 color = f3(0.0f, 0.0f, 0.0f);
+  for (uint32_t si = 0; si < spp; ++si) {
+    const smp::PathSampler ps{zf, x, y, si};
+    float u[2];
+    ps.s2(smp::kCameraDim, u);
+    const float px = ((float)x + 0.5f) + 0.5f * smp::tent(u[0]);
+    const float py = ((float)y + 0.5f) + 0.5f * smp::tent(u[1]);
+    const float sx = ((2.0f * px / rx) - 1.0f) * aspect * cam.tan_half;
+    const float sy = (1.0f - (2.0f * py / ry)) * cam.tan_half;
+    const F3 dir = normalize(add(add(cam.fwd, scl(sx, cam.right)), scl(sy, cam.up)));
+    // mis::trace_radiance
+    F3 radiance = f3(0.0f, 0.0f, 0.0f), thr = f3(1.0f, 1.0f, 1.0f);
+    uint32_t hi, ht;
+    float tt, hb0, hb1, hb2;
+    if (closest<kTwoLevel>(sc, cam.eye, dir, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) {
+      overflow |= st.overflow;
+      Vtx vtx = shading_vertex(sv, hi, ht, hb0, hb1, hb2, dir);
+      F3 le;
+      if (material_le(sv.mats[vtx.material], vtx, &le)) radiance = add(radiance, le);
+      for (uint32_t depth = 0; depth < max_depth; ++depth) {
+        const uint32_t base = smp::kVertexBase + depth * smp::kVertexStride;
+        Sample s;
+        F3 wi;
+        float ud[2];
+        ps.s2(base + 7, ud);
+        if (!material_sample(sv.mats[vtx.material], vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi)) break;
+        thr = hadamard(thr, s.weight);
+        if (depth + 1 >= 4) {
+          const float q = clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.05f, 1.0f);
+          if (ps.s1(base + 13) > q) break;
+          thr = divs(thr, q);
+        }
+        const F3 no = offset_origin(vtx.p, vtx.ng, wi);
+        if (!closest<kTwoLevel>(sc, no, wi, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) break;  // no environment
+        overflow |= st.overflow;
+        const Vtx nv = shading_vertex(sv, hi, ht, hb0, hb1, hb2, wi);
+        F3 e;
+        if (material_le(sv.mats[nv.material], nv, &e)) {
+          if (s.delta) {
+            radiance = add(radiance, hadamard(thr, e));
+          } else {
+            const float mw = (s.pdf <= 0.0f || s.pdf + 0.0f <= 0.0f) ? 0.0f : s.pdf / (s.pdf + 0.0f);
+            radiance = add(radiance, muls(hadamard(thr, e), mw));
+          }
+        }
+        vtx = nv;
+      }
+    } else {
+      overflow |= st.overflow;
+    }
+    const float cnt = (float)(si + 1);
+    color = add(color, divs(sub(radiance, color), cnt));
+  }
+  float* o = rgb + 3 * (size_t)pix;
+  o[0] = color.x * cam.exposure;
+  o[1] = color.y * cam.exposure;
+  o[2] = color.z * cam.exposure;
+  if (overflow) atomicOr(status, 1u);
+}
+
+}  // namespace pt
+}  // namespace tpt
+
+using namespace tpt;
+
+namespace {
+
+// The OpenPBR parameter subset the path tracer implements exactly.
+bool supported(const TptOpenPbr& m, std::string* why) {
+  if (m.specular_weight != 0.0f) *why = "specular_weight must be 0";
+  else if (m.base_metalness != 0.0f) *why = "base_metalness must be 0";
+  else if (m.transmission_weight != 0.0f) *why = "transmission_weight must be 0";
+  else if (m.coat_weight != 0.0f) *why = "coat_weight must be 0";
+  else if (m.fuzz_weight != 0.0f) *why = "fuzz_weight must be 0";
+  else if (m.thin_film_weight != 0.0f) *why = "thin_film_weight must be 0";
+  else if (m.thin_walled) *why = "thin_walled must be false";
+  else return true;
+  return false;
+}
+
+}  // namespace
+
+extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb, uint32_t flags) {
+  try {
+    if (!s || !desc || !rgb) {
+      set_error("tpt_render: null argument");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    const uint32_t w = desc->width, h = desc->height;
+    if (w == 0 || h == 0) return TPT_OK;
+    if ((uint64_t)w * h > 0x7FFFFFFFull || desc->samples_per_pixel == 0) {
+      set_error("tpt_render: frame too large or samples_per_pixel == 0");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    std::vector<TptOpenPbr> mats(s->material_count);
+    std::vector<GpuShade> shade(s->instance_count);
+    TptDevice* d = s->dev;
+    if (!hip_ok(hipSetDevice(d->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    if (s->material_count &&
+        !hip_ok(hipMemcpy(mats.data(), s->mats, mats.size() * sizeof(TptOpenPbr), hipMemcpyDeviceToHost),
+                "hipMemcpy(materials)"))
+      return TPT_ERROR_DEVICE;
+    if (s->instance_count &&
+        !hip_ok(hipMemcpy(shade.data(), s->shade, shade.size() * sizeof(GpuShade), hipMemcpyDeviceToHost),
+                "hipMemcpy(instances)"))
+      return TPT_ERROR_DEVICE;
+    for (uint32_t i = 0; i < s->instance_count; ++i) {
+      if (shade[i].material >= s->material_count) {
+        set_error("tpt_render: instance " + std::to_string(i) + " has no material");
+        return TPT_ERROR_BAD_ARGUMENT;
+      }
+      std::string why;
+      if (!supported(mats[shade[i].material], &why)) {
+        set_error("tpt_render: material " + std::to_string(shade[i].material) + " not supported: " + why);
+        return TPT_ERROR_BAD_ARGUMENT;
+      }
+    }
+    pt::CamView cam;
+    const TptCamera& c = desc->camera;
+    cam.eye = F3{c.eye[0], c.eye[1], c.eye[2]};
+    cam.fwd = F3{c.forward[0], c.forward[1], c.forward[2]};
+    cam.right = F3{c.right[0], c.right[1], c.right[2]};
+    cam.up = F3{c.image_up[0], c.image_up[1], c.image_up[2]};
+    cam.tan_half = c.tan_half_fov_y;
+    cam.exposure = c.exposure;
+    pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats};
+    const SceneView v = view_of(s);
+    const size_t n = (size_t)w * h;
+    const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
+    float* drgb = rgb;
+    DeviceBuffer tmp;
+    if (!dev_ptr) {
+      if (!tmp.ensure(n * 12)) return TPT_ERROR_DEVICE;
+      drgb = (float*)tmp.ptr;
+      if (!hip_ok(hipMemsetAsync(d->status, 0, 4, d->stream), "hipMemsetAsync(status)")) return TPT_ERROR_DEVICE;
+    }
+    const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
+    const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
+    if (two)
+      hipLaunchKernelGGL(pt::render_kernel<true>, grid, block, 0, d->stream, v, sv, cam, w, h,
+                         desc->samples_per_pixel, desc->max_depth, drgb, d->status);
+    else
+      hipLaunchKernelGGL(pt::render_kernel<false>, grid, block, 0, d->stream, v, sv, cam, w, h,
+                         desc->samples_per_pixel, desc->max_depth, drgb, d->status);
+    if (!hip_ok(hipGetLastError(), "render_kernel launch")) return TPT_ERROR_DEVICE;
+    if (!dev_ptr) {
+      uint32_t status = 0;
+      if (!hip_ok(hipMemcpyAsync(rgb, drgb, n * 12, hipMemcpyDeviceToHost, d->stream), "hipMemcpyAsync(rgb)") ||
+          !hip_ok(hipMemcpyAsync(&status, d->status, 4, hipMemcpyDeviceToHost, d->stream), "hipMemcpyAsync") ||
+          !hip_ok(hipStreamSynchronize(d->stream), "hipStreamSynchronize"))
+        return TPT_ERROR_DEVICE;
+      if (status & 1u) {
+        set_error("tpt_render: traversal stack overflow (128 entries)");
+        return TPT_ERROR_OVERFLOW;
+      }
+    }
+    return TPT_OK;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return TPT_ERROR;
+  }
+}

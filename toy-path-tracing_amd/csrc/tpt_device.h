@@ -1,0 +1,110 @@
+// Device handle and host-side helpers shared by the libtpt compilation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "tpt_internal.h"
+#include "traverse.h"
+
+namespace tpt {
+
+// Thread-local last error (shim.h convention of the reference's C shim).
+void set_error(const std::string& s);
+// true on hipSuccess; otherwise records "<what>: <hip error>" and returns false.
+bool hip_ok(hipError_t e, const char* what);
+
+// Ray counters of the persistent trace kernels: one per XCD range plus the
+// count of finished waves; the last wave resets them.
+constexpr int kRegions = 8;
+struct WorkCounters {
+  uint32_t next[kRegions];
+  uint32_t done_waves;
+  uint32_t pad[7];
+};
+
+// A device allocation owned by the current host scope.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  bool ensure(size_t nb) {
+    if (bytes >= nb) return true;
+    release();
+    if (!hip_ok(hipMalloc(&ptr, nb), "hipMalloc")) {
+      ptr = nullptr;
+      return false;
+    }
+    bytes = nb;
+    return true;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  ~DeviceBuffer() { release(); }
+};
+
+}  // namespace tpt
+
+struct TptDevice {
+  int index = 0;
+  int cu_count = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  uint32_t* status = nullptr;         // device word: bit 0 = traversal stack overflow
+  tpt::WorkCounters* work = nullptr;  // persistent-kernel ray counters (self-resetting)
+  // Defaults from a sweep on MI355X over the 2M-triangle traversal workload
+  // (refill 8..40 x blocks/CU 2..5): fewer resident waves than occupancy
+  // allows keep more of each wave's node and triangle lines in L2.
+  uint32_t refill_below = 24;         // refill a wave when fewer lanes are busy
+  uint32_t blocks_per_cu = 4;         // resident blocks per CU (0: occupancy limit)
+};
+
+// A scene resident in device memory (tpt_scene_create).
+struct TptScene {
+  TptDevice* dev = nullptr;
+  // traversal
+  tpt::GpuNode* nodes = nullptr;
+  tpt::GpuNodeQ8* qnodes = nullptr;
+  tpt::GpuTri* tris = nullptr;
+  uint32_t* tri_rank = nullptr;
+  tpt::GpuInstance* inst = nullptr;
+  tpt::GpuNode* tlas = nullptr;
+  tpt::GpuNodeQ8* qtlas = nullptr;
+  uint32_t* tlas_prim = nullptr;
+  uint32_t instance_count = 0;
+  uint32_t tlas_node_count = 0;
+  // shading (path tracer): per-instance records, mesh vertex arrays in
+  // original order, indices, materials
+  tpt::GpuShade* shade = nullptr;
+  float* vpos = nullptr;
+  float* vnrm = nullptr;
+  float* vuv = nullptr;
+  uint32_t* vidx = nullptr;
+  TptOpenPbr* mats = nullptr;
+  uint32_t material_count = 0;
+  size_t bytes = 0;
+  // staging for host-pointer calls (grow-only)
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+};
+
+namespace tpt {
+inline SceneView view_of(const TptScene* s) {
+  SceneView v;
+  v.nodes = s->nodes;
+  v.qnodes = s->qnodes;
+  v.tris = s->tris;
+  v.tri_rank = s->tri_rank;
+  v.inst = s->inst;
+  v.tlas = s->tlas;
+  v.qtlas = s->qtlas;
+  v.tlas_prim = s->tlas_prim;
+  v.instance_count = s->instance_count;
+  v.tlas_node_count = s->tlas_node_count;
+  return v;
+}
+}  // namespace tpt

@@ -71,6 +71,21 @@ struct alignas(16) GpuInstance {
 };
 static_assert(sizeof(GpuInstance) == 64, "GpuInstance must be 64 bytes");
 
+// Per-instance shading record: local_to_world (columns 0..3, xyz rows),
+// normal_to_world = the transpose of world_to_local's upper 3x3
+// (Mat3::from_mat4(world_to_local.transpose()), R/scene/mod.rs:226), the
+// material, and where the mesh's vertices / indices start.
+struct alignas(16) GpuShade {
+  float l2w[12];
+  float nrm[9];
+  uint32_t material;
+  uint32_t vbase;
+  uint32_t ibase;
+  uint32_t flags;  // bit 0: vertex normals present, bit 1: uvs present
+  uint32_t pad[7];
+};
+static_assert(sizeof(GpuShade) == 128, "GpuShade must be 128 bytes");
+
 // Host-side quantiser used by the scene upload (exposed for tests).
 void quantize_node(const GpuNode& n, GpuNodeQ8* q);
 GpuNode to_gpu_node(const TptQbvhNode& n, uint32_t interior_base);

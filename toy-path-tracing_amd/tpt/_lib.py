@@ -58,6 +58,8 @@ class MeshDesc(ctypes.Structure):
         ("node_count", ctypes.c_uint32),
         ("pad", ctypes.c_uint32),
         ("primitive_indices", ctypes.c_void_p),
+        ("normals", ctypes.c_void_p),
+        ("uvs", ctypes.c_void_p),
     ]
 
 
@@ -78,13 +80,36 @@ class SceneDesc(ctypes.Structure):
         ("instances", ctypes.c_void_p),
         ("tlas_nodes", ctypes.c_void_p),
         ("tlas_node_count", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32),
+        ("material_count", ctypes.c_uint32),
         ("tlas_primitive_indices", ctypes.c_void_p),
+        ("materials", ctypes.c_void_p),
     ]
 
 
-assert ctypes.sizeof(MeshDesc) == 48 and ctypes.sizeof(SceneDesc) == 48
+assert ctypes.sizeof(MeshDesc) == 64 and ctypes.sizeof(SceneDesc) == 56
 assert ctypes.sizeof(InstanceDesc) == 136
+
+
+class Camera(ctypes.Structure):
+    """TptCamera: PinholeCamera after ::new plus the pixel spread angle."""
+
+    _fields_ = [
+        ("eye", ctypes.c_float * 3), ("tan_half_fov_y", ctypes.c_float),
+        ("forward", ctypes.c_float * 3), ("pixel_spread", ctypes.c_float),
+        ("right", ctypes.c_float * 3), ("exposure", ctypes.c_float),
+        ("image_up", ctypes.c_float * 3), ("pad", ctypes.c_float),
+    ]
+
+
+assert ctypes.sizeof(Camera) == 64
+
+
+class RenderDesc(ctypes.Structure):
+    _fields_ = [("camera", Camera), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("samples_per_pixel", ctypes.c_uint32), ("max_depth", ctypes.c_uint32)]
+
+
+assert ctypes.sizeof(RenderDesc) == 80
 
 
 class TptError(RuntimeError):
@@ -119,6 +144,13 @@ _SIGNATURES = [
     ("tpt_trace_any", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_uint32]),
+    ("tpt_camera_rays", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_path_randoms", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_render", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_trace_counts", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_uint32]),

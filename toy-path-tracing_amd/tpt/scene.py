@@ -151,6 +151,12 @@ class Scene:
     instances: list[Instance] = field(default_factory=list)
     qbvh: Qbvh | None = None
     built: bool = False
+    materials: list = field(default_factory=list)  # tpt.material.OpenPbr, by material index
+
+    def add_material(self, material) -> int:
+        """Scene::add_material (R/scene/mod.rs:143-217) for OpenPBR."""
+        self.materials.append(material)
+        return len(self.materials) - 1
 
     def add_mesh(self, mesh: Mesh) -> int:
         self.meshes.append(mesh)
@@ -242,6 +248,12 @@ class DeviceScene:
             else:
                 d.node_count = 0
                 d.primitive_indices = None
+            if m.normals is not None:
+                m.normals = np.ascontiguousarray(m.normals, dtype=np.float32).reshape(-1, 3)
+                d.normals = ptr(m.normals)
+            if m.uvs is not None:
+                m.uvs = np.ascontiguousarray(m.uvs, dtype=np.float32).reshape(-1, 2)
+                d.uvs = ptr(m.uvs)
         insts = (_lib.InstanceDesc * max(1, len(scene.instances)))()
         for i, inst in enumerate(scene.instances):
             insts[i].mesh = inst.mesh_index
@@ -257,6 +269,11 @@ class DeviceScene:
             desc.tlas_nodes = ptr(scene.qbvh.nodes) if len(scene.qbvh.nodes) else None
             desc.tlas_node_count = len(scene.qbvh.nodes)
             desc.tlas_primitive_indices = ptr(scene.qbvh.primitive_indices)
+        if scene.materials:
+            from .material import records
+            mats = records(scene.materials)
+            desc.materials = ctypes.addressof(mats)
+            desc.material_count = len(scene.materials)
         self.handle = lib().tpt_scene_create(device.handle, ctypes.byref(desc))
         if not self.handle:
             raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
@@ -313,6 +330,22 @@ class DeviceScene:
         check(lib().tpt_trace_counts(self.handle, q, ptr(rays), len(rays), ptr(target), ptr(distance), ptr(out),
                                      flags))
         return out
+
+    def render(self, camera, width: int, height: int, spp: int, max_depth: int) -> np.ndarray:
+        """The reference render loop (R/main.rs:98-126) with the MIS integrator
+        -> (height, width, 3) float32, exposure applied."""
+        d = _lib.RenderDesc()
+        d.camera = camera
+        d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+        out = np.zeros((height, width, 3), np.float32)
+        check(lib().tpt_render(self.handle, ctypes.byref(d), ptr(out), 0))
+        return out
+
+    def render_device(self, camera, width: int, height: int, spp: int, max_depth: int, out_dev) -> None:
+        d = _lib.RenderDesc()
+        d.camera = camera
+        d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+        check(lib().tpt_render(self.handle, ctypes.byref(d), ptr(out_dev), _lib.TPT_FLAG_DEVICE_POINTERS))
 
     # device-pointer queries (asynchronous on the device stream) -----------------
     def closest_hit_device(self, rays_dev, hits_dev, n: int, quantized: bool = False) -> None:

@@ -129,40 +129,63 @@ def _scale(x, y, z) -> np.ndarray:
     return np.diag([x, y, z, 1.0])
 
 
+def flat_quads(quads) -> Mesh:
+    """A mesh of planar quads, each with its own four vertices, the quad's
+    normal (from its winding: (p1-p0) x (p2-p0)) and uvs (0,0),(1,0),(1,1),(0,1)."""
+    pos, nrm, uv, idx = [], [], [], []
+    for q in quads:
+        q = np.asarray(q, np.float64)
+        n = np.cross(q[1] - q[0], q[2] - q[0])
+        n /= np.linalg.norm(n)
+        b = len(pos)
+        pos += list(q)
+        nrm += [n] * 4
+        uv += [(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0)]
+        idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+    return Mesh(np.array(pos, np.float32), np.array(idx, np.uint32), normals=np.array(nrm, np.float32),
+                uvs=np.array(uv, np.float32))
+
+
 def unit_box() -> Mesh:
-    """Axis-aligned box [-0.5,0.5]^3, 12 outward-facing triangles."""
-    c = np.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)])
-    faces = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
-    tris = []
-    for a, b, cc, d in faces:
-        tris += [[a, b, cc], [a, cc, d]]
-    return Mesh(c.astype(np.float32), np.array(tris, dtype=np.uint32))
+    """Box [-0.5,0.5]^3: 6 outward-facing quads, 12 triangles, flat normals."""
+    h = 0.5
+    quads = [
+        [(-h, -h, -h), (-h, -h, h), (-h, h, h), (-h, h, -h)],  # -x
+        [(h, -h, -h), (h, h, -h), (h, h, h), (h, -h, h)],  # +x
+        [(-h, -h, -h), (h, -h, -h), (h, -h, h), (-h, -h, h)],  # -y
+        [(-h, h, -h), (-h, h, h), (h, h, h), (h, h, -h)],  # +y
+        [(-h, -h, -h), (-h, h, -h), (h, h, -h), (h, -h, -h)],  # -z
+        [(-h, -h, h), (h, -h, h), (h, h, h), (-h, h, h)],  # +z
+    ]
+    return flat_quads(quads)
 
 
 @dataclass
 class CornellScene:
     scene: Scene
-    # instance -> material slot: 0 white, 1 red, 2 green, 3 light
-    instance_material: list[int]
+    materials: list  # tpt.material.OpenPbr per material index
     light_instance: int
+    camera: dict
 
 
 def cornell(build: bool = True) -> CornellScene:
-    """Cornell box: 36 triangles, 6 instances (white walls, red, green, two boxes, light)."""
+    """Cornell box of the plain-method configuration: 36 triangles in 6
+    instances.  Materials: OpenPBR ``new(c).with_specular_weight(0)`` for
+    white (0.8), red (0.8,0.1,0.1), green (0.1,0.8,0.1); the light is OpenPBR
+    emission 15 with base and specular weight 0.  Walls face into the box."""
+    from .material import OpenPbr
+
     s = Scene()
-    # white: floor (y=0), ceiling (y=1), back wall (z=1)
-    wv = np.array([[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1],
-                   [0, 1, 0], [0, 1, 1], [1, 1, 1], [1, 1, 0],
-                   [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=np.float32)
-    wi = np.array([[0, 2, 1], [0, 3, 2], [4, 6, 5], [4, 7, 6], [8, 10, 9], [8, 11, 10]], dtype=np.uint32)
-    white = s.add_mesh(Mesh(wv, wi))
-    rv, ri = _quad([0, 0, 0], [0, 0, 1], [0, 1, 1], [0, 1, 0])  # left wall x=0
-    red = s.add_mesh(Mesh(rv.astype(np.float32), ri))
-    gv, gi = _quad([1, 0, 0], [1, 1, 0], [1, 1, 1], [1, 0, 1])  # right wall x=1
-    green = s.add_mesh(Mesh(gv.astype(np.float32), gi))
+    white = s.add_mesh(flat_quads([
+        [(0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)],  # floor y=0, normal +y
+        [(0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)],  # ceiling y=1, normal -y
+        [(0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)],  # back wall z=1, normal -z
+    ]))
+    red = s.add_mesh(flat_quads([[(0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)]]))  # x=0, normal +x
+    green = s.add_mesh(flat_quads([[(1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)]]))  # x=1, normal -x
     box = s.add_mesh(unit_box())
-    lv, li = _quad([0.4, 0.999, 0.4], [0.6, 0.999, 0.4], [0.6, 0.999, 0.6], [0.4, 0.999, 0.6])  # faces -Y
-    light = s.add_mesh(Mesh(lv.astype(np.float32), li))
+    light = s.add_mesh(flat_quads([[(0.4, 0.999, 0.4), (0.6, 0.999, 0.4), (0.6, 0.999, 0.6),
+                                     (0.4, 0.999, 0.6)]]))  # normal -y
     eye4 = np.eye(4)
     s.add_instance(white, 0, eye4)
     s.add_instance(red, 1, eye4)
@@ -171,10 +194,17 @@ def cornell(build: bool = True) -> CornellScene:
     tall = _translate(0.33, 0.3, 0.64) @ _rot_y(15.0) @ _scale(0.3, 0.6, 0.3)
     s.add_instance(box, 0, short)
     s.add_instance(box, 0, tall)
-    li_idx = s.add_instance(light, 3, eye4)
+    li = s.add_instance(light, 3, eye4)
     if build:
         s.build_qbvh()
-    return CornellScene(s, [0, 1, 2, 0, 0, 3], li_idx)
+    diffuse = lambda c: OpenPbr.new(c).with_(specular_weight=0.0)
+    mats = [diffuse((0.8, 0.8, 0.8)), diffuse((0.8, 0.1, 0.1)), diffuse((0.1, 0.8, 0.1)),
+            OpenPbr.new((0.0, 0.0, 0.0)).with_(base_weight=0.0, specular_weight=0.0, emission_luminance=15.0,
+                                               emission_color=(1.0, 1.0, 1.0))]
+    for m in mats:
+        s.add_material(m)
+    cam = dict(eye=(0.5, 0.5, -1.4), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov_y_deg=40.0)
+    return CornellScene(s, mats, li, cam)
 
 
 # --------------------------------------------------------------------------- rays
