@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -46,7 +47,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nodes", choices=["fp32", "q8", "both"], default="both",
                     help="node format(s) to time; the reported value is the faster")
-    ap.add_argument("--cpu-sample", type=int, default=300_000, help="rays per set for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=2_100_000,
+                    help="rays per set for the CPU baseline (0: skip; the default covers every ray)")
     ap.add_argument("--spheres", type=int, default=410)
     ap.add_argument("--rays", type=int, default=2_000_000, help="bounce and shadow ray counts")
     ap.add_argument("--render", type=int, default=1, help="also time the Cornell path tracer (1) or not (0)")
@@ -211,7 +213,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(sets[dom]["kind"], best, sets[dom]["n"]),
         },
         "cpu_baseline": cpu,
         "render": render,
@@ -219,6 +221,23 @@ def main():
     print(json.dumps(line), flush=True)
     ds.close()
     dev.close()
+
+
+def measured_traffic(kind, nodes, n_rays):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (each scaled by the same counter's
+    reading for a plain copy of known size), when those runs profiled this
+    kernel on this ray count; otherwise None."""
+    want = {("any", "fp32"): "trace_kernel<false,1,false,false,false>",
+            ("closest", "fp32"): "trace_kernel<false,0,false,false,false>"}.get((kind, nodes))
+    path = ROOT / "profiles" / "trace_any_fp32_T2M_pmc.txt"
+    if want is None or not path.exists():
+        return None
+    for ln in path.read_text().splitlines():
+        m = re.match(r"traffic_bytes_per_launch kernel=(\S+) rays=(\d+) bytes=(\d+)", ln)
+        if m and m.group(1) == want and int(m.group(2)) == n_rays:
+            return int(m.group(3))
+    return None
 
 
 def render_bench(dev, stream, args):
@@ -259,7 +278,7 @@ def render_bench(dev, stream, args):
 
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         r = oracle.OracleRenderer(cs.scene, cs.materials)
-        rows = 16
+        rows = 64
         t = time.perf_counter()
         r.render(cam, w, h, spp, depth, rows=(h // 2 - rows // 2, h // 2 + rows // 2), threads=threads)
         dt = time.perf_counter() - t
