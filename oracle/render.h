@@ -20,8 +20,14 @@
 // maps; base weight / colour / diffuse roughness and emission are free.
 // Light NEE under the reference policy contributes nothing for OpenPBR
 // materials (R/material/open_pbr.rs:592-606 returns no light-tree query) and
-// there is no environment light, so the estimator is BSDF sampling with
-// emission at hits (MIS weight 1, R/light/mod.rs:416-437).
+// there is no environment light, so for them the estimator is BSDF sampling
+// with emission at hits (MIS weight 1, R/light/mod.rs:416-437).
+//
+// The reference's NormalizedLambert, Emissive and Mirror materials
+// (R/material/{normalized_lambert,emissive,mirror}.rs) are carried too:
+// Lambert surfaces build a diffuse light-tree query, so next-event estimation
+// through the SG light tree with the balance heuristic
+// (R/integrator/mis.rs:141-234) is exercised on them.
 #pragma once
 
 #include <cmath>
@@ -29,6 +35,7 @@
 #include <vector>
 
 #include "geometry.h"
+#include "lights.h"
 #include "sampler.h"
 
 namespace orc {
@@ -500,6 +507,15 @@ inline bool bsdf_sample(const PbrBsdf& p, V3 wo, float u_lobe, float ux, float u
 }
 
 // ---- scene shading data ------------------------------------------------------------
+// Material enum of the backend (R/material/mod.rs:100-115 variants carried).
+enum MatKind : uint32_t { kMatOpenPbr = 0, kMatLambert = 1, kMatEmissive = 2, kMatMirror = 3 };
+struct Material {
+  uint32_t kind = kMatOpenPbr;
+  OpenPbr pbr{};
+  V3 color{0.f, 0.f, 0.f};  // Lambert rho, Emissive colour, Mirror colour
+  float strength = 0.0f;    // Emissive
+};
+
 struct ShadeMesh {
   std::vector<float> nrm;  // 3 per vertex (may be empty: geometric normal)
   std::vector<float> uv;   // 2 per vertex (may be empty)
@@ -508,7 +524,15 @@ struct RenderScene {
   const Scene* geo = nullptr;
   std::vector<ShadeMesh> meshes;
   std::vector<uint32_t> inst_material;
-  std::vector<OpenPbr> materials;
+  std::vector<Material> materials;
+  // light tree over the emissive triangles (Scene::build_light_tree) and the
+  // (instance, triangle) -> leaf node table
+  lt::Tree tree;
+  std::vector<uint32_t> leaf_base;  // per instance: offset into leaf_of, or lt::kInvalid
+  std::vector<uint32_t> leaf_of;
+  uint32_t leaf_node(uint32_t inst, uint32_t tri) const {
+    return leaf_base[inst] == lt::kInvalid ? lt::kInvalid : leaf_of[leaf_base[inst] + tri];
+  }
 };
 
 struct Vertex {
@@ -516,6 +540,7 @@ struct Vertex {
   Onb frame;
   bool front_face;
   uint32_t material;
+  uint32_t inst, tri;
 };
 
 inline V3 mat3_mul_normal(const float* w2l, V3 n) {
@@ -525,13 +550,20 @@ inline V3 mat3_mul_normal(const float* w2l, V3 n) {
   return add(add(mul(n.x, c0), mul(n.y, c1)), mul(n.z, c2));
 }
 
+// Scene::triangle_positions (R/scene/mod.rs:432-438): world positions.
+inline void triangle_world(const RenderScene& rs, uint32_t inst, uint32_t tri, V3* p) {
+  const Instance& in = rs.geo->inst[inst];
+  const Mesh& m = rs.geo->meshes[in.mesh];
+  for (int k = 0; k < 3; ++k) p[k] = transform_point3(in.l2w, m.vert(m.idx[3 * tri + k]));
+}
+
 // base_shading_vertex_from_triangle_sample_impl (R/scene/mod.rs:471-544),
-// the fields the integrator reads.
-inline Vertex shading_vertex(const RenderScene& rs, const SceneHit& h, V3 dir) {
-  const Instance& in = rs.geo->inst[h.inst];
+// the fields the integrator reads; `dir` is the incident direction.
+inline Vertex vertex_at(const RenderScene& rs, uint32_t inst, uint32_t tri, float b0, float b1, float b2, V3 dir) {
+  const Instance& in = rs.geo->inst[inst];
   const Mesh& m = rs.geo->meshes[in.mesh];
   const ShadeMesh& sm = rs.meshes[in.mesh];
-  uint32_t i0 = m.idx[3 * h.tri], i1 = m.idx[3 * h.tri + 1], i2 = m.idx[3 * h.tri + 2];
+  uint32_t i0 = m.idx[3 * tri], i1 = m.idx[3 * tri + 1], i2 = m.idx[3 * tri + 2];
   V3 p0 = transform_point3(in.l2w, m.vert(i0)), p1 = transform_point3(in.l2w, m.vert(i1)),
      p2 = transform_point3(in.l2w, m.vert(i2));
   auto nrm = [&](uint32_t i) {
@@ -541,9 +573,9 @@ inline Vertex shading_vertex(const RenderScene& rs, const SceneHit& h, V3 dir) {
   auto uvv = [&](uint32_t i, int k) { return sm.uv.empty() ? 0.0f : sm.uv[2 * i + k]; };
   V3 n0 = nrm(i0), n1 = nrm(i1), n2 = nrm(i2);
   Vertex v;
-  v.p = interp3(h.b0, h.b1, h.b2, p0, p1, p2);
+  v.p = interp3(b0, b1, b2, p0, p1, p2);
   V3 gn = normalize_or_zero3(cross(sub(p1, p0), sub(p2, p0)));
-  V3 sn = normalize_or_zero3(interp3(h.b0, h.b1, h.b2, n0, n1, n2));
+  V3 sn = normalize_or_zero3(interp3(b0, b1, b2, n0, n1, n2));
   V3 nd = neg(dir);
   v.front_face = dot(gn, nd) >= 0.0f;
   v.ng = face_forward(gn, nd);
@@ -565,12 +597,17 @@ inline Vertex shading_vertex(const RenderScene& rs, const SceneHit& h, V3 dir) {
   if (!ok) dpdu = onb_from_normal(v.ns).t;
   v.frame = onb_from_normal_and_tangent(v.ns, dpdu);
   v.wo = normalize_or_zero3(nd);
-  v.material = rs.inst_material[h.inst];
+  v.material = rs.inst_material[inst];
+  v.inst = inst;
+  v.tri = tri;
   return v;
+}
+inline Vertex shading_vertex(const RenderScene& rs, const SceneHit& h, V3 dir) {
+  return vertex_at(rs, h.inst, h.tri, h.b0, h.b1, h.b2, dir);
 }
 
 // OpenPbrMaterial::le (R/material/open_pbr.rs:351-375) with exit tint 1.
-inline bool material_le(const OpenPbr& m, const Vertex& v, V3* le) {
+inline bool pbr_le(const OpenPbr& m, const Vertex& v, V3* le) {
   if (m.emission_luminance <= 0.0f) return false;
   if (!m.thin_walled && !v.front_face) return false;
   V3 wl = normalize_or_zero3(onb_to_local(v.frame, v.wo));
@@ -582,8 +619,8 @@ inline bool material_le(const OpenPbr& m, const Vertex& v, V3* le) {
 
 // OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548) in radiance mode.
 constexpr float kGeomCosEps = 1.0e-6f;  // R/material/mod.rs:31
-inline bool material_sample(const OpenPbr& m, const Vertex& v, float u_lobe, float ux, float uy, BSample* out,
-                            V3* wi_world) {
+inline bool pbr_sample(const OpenPbr& m, const Vertex& v, float u_lobe, float ux, float uy, BSample* out,
+                       V3* wi_world) {
   if (dot(v.wo, v.ng) <= 0.0f) return false;
   V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
@@ -597,30 +634,262 @@ inline bool material_sample(const OpenPbr& m, const Vertex& v, float u_lobe, flo
   return true;
 }
 
+// ---- Material dispatch (R/material/mod.rs:251-412) ------------------------------
+inline bool mat_is_pure_emitter(const Material& m) { return m.kind == kMatEmissive; }
+inline bool mat_may_emit(const Material& m) {
+  if (m.kind == kMatEmissive) return true;
+  if (m.kind == kMatOpenPbr) return m.pbr.emission_luminance > 0.0f;  // open_pbr.rs:377-379
+  return false;
+}
+inline float mat_max_emission(const Material& m) {
+  if (m.kind == kMatEmissive)  // emissive.rs max_emission, no texture
+    return std::fmax(lt::vmax_elem(smul(m.color, m.strength)) * 1.0f, 0.0f);
+  if (m.kind == kMatOpenPbr) {  // open_pbr.rs:381-391
+    if (m.pbr.emission_luminance <= 0.0f) return 0.0f;
+    return std::fmax(lt::vmax_elem(m.pbr.emission_color) * m.pbr.emission_luminance * 1.0f, 0.0f);
+  }
+  return 0.0f;
+}
+inline bool mat_le(const Material& m, const Vertex& v, V3* le) {
+  if (m.kind == kMatEmissive) {  // color * strength * texture factor (1)
+    *le = vmul(smul(m.color, m.strength), v3(1.f, 1.f, 1.f));
+    return true;
+  }
+  if (m.kind == kMatOpenPbr) return pbr_le(m.pbr, v, le);
+  return false;
+}
+// NormalizedLambertMaterial::sample (normalized_lambert.rs:120-158) and
+// MirrorMaterial::sample (mirror.rs:112-143), radiance mode.
+inline bool mat_sample(const Material& m, const Vertex& v, float u_lobe, float ux, float uy, BSample* out,
+                       V3* wi_world) {
+  if (m.kind == kMatOpenPbr) return pbr_sample(m.pbr, v, u_lobe, ux, uy, out, wi_world);
+  if (m.kind == kMatEmissive) return false;
+  V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
+  if (wol.z <= 0.0f) return false;
+  BSample s;
+  if (m.kind == kMatLambert) {  // NormalizedLambertBsdf::sample (bsdf/normalized_lambert.rs:35-60)
+    V3 wi = cosine_hemisphere(ux, uy);
+    float pdf = std::fmax(wi.z, 0.0f) / kPi;
+    if (pdf <= 0.0f) return false;
+    s.weight = m.color;
+    s.wi = wi;
+    s.pdf = pdf;
+    s.delta = false;
+  } else {  // MirrorBsdf::sample (bsdf/mirror.rs:23-39)
+    s.weight = m.color;
+    s.wi = normalize_or_zero3(v3(-wol.x, -wol.y, wol.z));
+    s.pdf = 1.0f;
+    s.delta = true;
+  }
+  V3 wi = onb_to_world(v.frame, s.wi);
+  if (dot(wi, v.ng) <= kGeomCosEps) return false;
+  *out = s;
+  *wi_world = wi;
+  return true;
+}
+// modified_bsdf_eval (R/material/mod.rs:33-53), radiance mode
+inline V3 shading_normal_correction(const Vertex& v, V3 wi, V3 f) {
+  if (len_sq(f) == 0.0f) return v3(0.f, 0.f, 0.f);
+  float cg = std::fabs(dot(wi, v.ng));
+  if (cg <= kGeomCosEps) return v3(0.f, 0.f, 0.f);
+  return smul(f, std::fabs(dot(wi, v.ns)) / cg);
+}
+// Material::eval / pdf for the materials that reach next-event estimation:
+// NormalizedLambert (normalized_lambert.rs:160-193).  Mirror and Emissive
+// evaluate to zero; OpenPBR never gets a light sample under the reference
+// policy (no light-tree query, no environment).
+inline V3 mat_eval(const Material& m, const Vertex& v, V3 wi) {
+  if (m.kind != kMatLambert) return v3(0.f, 0.f, 0.f);
+  if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return v3(0.f, 0.f, 0.f);
+  V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
+  V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
+  V3 f = (wol.z <= 0.0f || wil.z <= 0.0f) ? v3(0.f, 0.f, 0.f) : sdiv(m.color, kPi);
+  return shading_normal_correction(v, wi, f);
+}
+inline float mat_pdf(const Material& m, const Vertex& v, V3 wi) {
+  if (m.kind != kMatLambert) return 0.0f;
+  if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return 0.0f;
+  V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
+  V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
+  if (wol.z <= 0.0f || wil.z <= 0.0f) return 0.0f;
+  return std::fmax(wil.z, 0.0f) / kPi;
+}
+// light_tree_precompute: NormalizedLambert (normalized_lambert.rs:201-218)
+// gives a diffuse query; every other material here gives none.
+inline bool mat_query(const Material& m, const Vertex& v, lt::Query* q) {
+  if (m.kind != kMatLambert) return false;
+  float rho = lt::luminance(m.color);
+  if (rho <= 0.0f) return false;
+  *q = lt::Query{v.p, v.ns, rho};
+  return true;
+}
+
+// Scene::register_area_light_triangles (R/scene/mod.rs:649-683) +
+// build_light_tree (R/light_tree/builder.rs:58-121), triangle leaves.
+inline void build_lights(RenderScene& rs) {
+  std::vector<lt::Leaf> leaves;
+  const Scene& g = *rs.geo;
+  rs.leaf_base.assign(g.inst.size(), lt::kInvalid);
+  rs.leaf_of.clear();
+  for (uint32_t i = 0; i < (uint32_t)g.inst.size(); ++i) {
+    const Material& m = rs.materials[rs.inst_material[i]];
+    if (!mat_may_emit(m)) continue;
+    float me = mat_max_emission(m);
+    if (me <= 0.0f) continue;
+    const Mesh& mesh = g.meshes[g.inst[i].mesh];
+    const uint32_t ntri = (uint32_t)(mesh.idx.size() / 3);
+    rs.leaf_base[i] = (uint32_t)rs.leaf_of.size();
+    rs.leaf_of.resize(rs.leaf_of.size() + ntri, lt::kInvalid);
+    for (uint32_t t = 0; t < ntri; ++t) {
+      V3 p[3];
+      triangle_world(rs, i, t, p);
+      float area = 0.5f * lt::vlen(cross(sub(p[1], p[0]), sub(p[2], p[0])));
+      if (area * me <= 0.0f) continue;
+      lt::Leaf l;
+      if (lt::triangle_leaf(i, t, p[0], p[1], p[2], me, &l)) leaves.push_back(l);
+    }
+  }
+  rs.tree = lt::build_tree(leaves);
+  for (uint32_t k = 0; k < (uint32_t)rs.tree.nodes.size(); ++k) {
+    const lt::Node& n = rs.tree.nodes[k];
+    if (n.leaf) rs.leaf_of[rs.leaf_base[n.inst] + n.tri] = k;
+  }
+}
+
+inline float balance_heuristic(float a, float b) {  // R/math/mod.rs:163-171
+  float sum = a + b;
+  if (a <= 0.0f || sum <= 0.0f) return 0.0f;
+  return a / sum;
+}
+
 constexpr float kRayEpsilon = 1.0e-4f;  // R/integrator/mod.rs:49
 inline V3 offset_origin(V3 p, V3 ng, V3 dir) {
   V3 off = dot(dir, ng) >= 0.0f ? ng : neg(ng);
   return add(p, mul(kRayEpsilon, off));
 }
 
-// mis::trace_radiance (R/integrator/mis.rs:19-139) for the subset.
+// LightSampler with the tree as its only category (R/light/mod.rs:136-213):
+// category pmf = root flux / root flux = 1 when the tree has flux.
+inline float tree_category_pmf(const RenderScene& rs) {
+  float w = std::fmax(rs.tree.root_flux(), 0.0f);
+  return w > 0.0f ? w / w : 0.0f;
+}
+
+// direct_light_mis_contribution (R/integrator/mis.rs:141-210) with
+// sample_light_mis_compensated_lazy (R/light/mod.rs:289-331) and
+// sample_li_for_triangle (R/light/area.rs:12-54).
+inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& vtx, float u_root, float u_tree,
+                       float u_aux, float us0, float us1) {
+  (void)u_root;  // one category: every u selects the tree
+  (void)u_aux;
+  const V3 zero = v3(0.f, 0.f, 0.f);
+  const float cat_pmf = tree_category_pmf(rs);
+  if (cat_pmf <= 0.0f) return zero;  // sample_category: empty sampler
+  lt::Query q;
+  if (!mat_query(m, vtx, &q)) return zero;
+  uint32_t leaf;
+  float leaf_pmf;
+  if (!lt::sample_tree(rs.tree, q, u_tree, &leaf, &leaf_pmf)) return zero;
+  const lt::Node& ln = rs.tree.nodes[leaf];
+  // sample_triangle_point (R/scene/mod.rs:616-633)
+  V3 p[3];
+  triangle_world(rs, ln.inst, ln.tri, p);
+  float su0 = std::sqrt(clampf(us0, 0.0f, 1.0f));
+  float u1 = clampf(us1, 0.0f, 1.0f);
+  float b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
+  V3 lp = interp3(b0, b1, b2, p[0], p[1], p[2]);
+  float area = 0.5f * lt::vlen(cross(sub(p[1], p[0]), sub(p[2], p[0])));
+  float pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
+  if (pdf_area <= 0.0f) return zero;
+  V3 to_light = sub(lp, vtx.p);
+  float d2 = len_sq(to_light);
+  if (d2 <= 0.0f) return zero;
+  float dist = std::sqrt(d2);
+  V3 wi = sdiv(to_light, dist);
+  const Material& lm = rs.materials[rs.inst_material[ln.inst]];
+  if (!mat_may_emit(lm)) return zero;
+  Vertex lv = vertex_at(rs, ln.inst, ln.tri, b0, b1, b2, wi);
+  V3 le;
+  if (!mat_le(lm, lv, &le)) return zero;
+  float cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
+  if (cos_light <= 0.0f) return zero;
+  float li_pdf = pdf_area * d2 / cos_light;
+  float selection = cat_pmf * leaf_pmf;
+  if (li_pdf <= 0.0f || selection <= 0.0f) return zero;
+  V3 f = mat_eval(m, vtx, wi);
+  float bsdf_pdf = len_sq(f) == 0.0f ? 0.0f : mat_pdf(m, vtx, wi);
+  if (len_sq(f) == 0.0f) return zero;
+  float cos_surface = std::fabs(dot(vtx.ng, wi));
+  if (cos_surface <= 0.0f) return zero;
+  // unoccluded_ray (R/integrator/mod.rs:239-265) with the light triangle as target
+  SceneHit sh;
+  if (closest_hit(*rs.geo, offset_origin(vtx.p, vtx.ng, wi), wi, INFINITY, &sh, nullptr) &&
+      !(sh.inst == ln.inst && sh.tri == ln.tri))
+    return zero;
+  float pdf_total = selection * li_pdf;
+  if (pdf_total <= 0.0f) return zero;
+  float mis = balance_heuristic(pdf_total, bsdf_pdf);
+  return smul(vmul(le, f), cos_surface * mis / pdf_total);
+}
+
+// pdf_for_triangle_hit (R/light/mod.rs:407-429) with
+// Scene::area_light_pdf_solid_angle (R/scene/mod.rs:587-611).
+inline float light_pdf_for_hit(const RenderScene& rs, const Material& shading_m, const Vertex& vtx, const Vertex& lv) {
+  if (rs.tree.nodes.empty()) return 0.0f;
+  lt::Query q;
+  if (!mat_query(shading_m, vtx, &q)) return 0.0f;
+  uint32_t leaf = rs.leaf_node(lv.inst, lv.tri);
+  float leaf_pmf = lt::pdf_leaf(rs.tree, q, leaf);
+  if (leaf_pmf <= 0.0f) return 0.0f;
+  float cat_pmf = tree_category_pmf(rs);
+  const float area = rs.tree.nodes[leaf].area;
+  if (area <= 0.0f) return 0.0f;
+  float pdf_area = 1.0f / area;
+  V3 to_light = sub(lv.p, vtx.p);
+  float d2 = len_sq(to_light);
+  if (d2 <= 0.0f) return 0.0f;
+  float dist = std::sqrt(d2);
+  V3 wi = sdiv(to_light, dist);
+  float cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
+  if (cos_light <= 0.0f) return 0.0f;
+  return cat_pmf * leaf_pmf * (pdf_area * d2 / cos_light);
+}
+
+// emitted_radiance_from_bsdf_sample_area (R/integrator/mis.rs:212-234)
+inline V3 emitted_from_bsdf_sample(const RenderScene& rs, V3 throughput, float bsdf_pdf, bool delta,
+                                   const Material& shading_m, const Vertex& vtx, const Vertex& lv,
+                                   const Material& light_m) {
+  V3 le;
+  if (!mat_le(light_m, lv, &le)) return v3(0.f, 0.f, 0.f);
+  if (delta) return vmul(throughput, le);
+  float light_pdf = light_pdf_for_hit(rs, shading_m, vtx, lv);
+  float mis = balance_heuristic(bsdf_pdf, light_pdf);
+  return smul(vmul(throughput, le), mis);
+}
+
+// mis::trace_radiance (R/integrator/mis.rs:19-139).
 inline V3 trace_radiance(const RenderScene& rs, V3 o, V3 d, const PathSampler& ps, uint32_t max_depth) {
   V3 radiance = v3(0.f, 0.f, 0.f), throughput = v3(1.f, 1.f, 1.f);
   SceneHit h;
   if (!closest_hit(*rs.geo, o, d, INFINITY, &h, nullptr)) return v3(0.f, 0.f, 0.f);  // no environment
   Vertex vtx = shading_vertex(rs, h, d);
-  const OpenPbr* mat = &rs.materials[vtx.material];
+  const Material* mat = &rs.materials[vtx.material];
   V3 le;
-  if (material_le(*mat, vtx, &le)) radiance = add(radiance, le);
+  if (mat_le(*mat, vtx, &le)) radiance = add(radiance, le);
   const uint32_t rr_start = 4;
   for (uint32_t depth = 0; depth < max_depth; ++depth) {
     const uint32_t base = 2u + depth * 16u;
-    // direct light: no light-tree query for OpenPBR, no environment -> zero
+    if (!mat_is_pure_emitter(*mat)) {
+      float us[2];
+      ps.s2(base + 3, us);
+      V3 dl = direct_light(rs, *mat, vtx, ps.s1(base + 0), ps.s1(base + 1), ps.s1(base + 2), us[0], us[1]);
+      radiance = add(radiance, vmul(throughput, dl));
+    }
     BSample s;
     V3 wi;
     float udir[2];
     ps.s2(base + 7, udir);
-    if (!material_sample(*mat, vtx, ps.s1(base + 5), udir[0], udir[1], &s, &wi)) break;
+    if (!mat_sample(*mat, vtx, ps.s1(base + 5), udir[0], udir[1], &s, &wi)) break;
     const bool delta = s.delta;
     throughput = vmul(throughput, s.weight);
     if (depth + 1 >= rr_start) {
@@ -632,17 +901,8 @@ inline V3 trace_radiance(const RenderScene& rs, V3 o, V3 d, const PathSampler& p
     SceneHit nh;
     if (!closest_hit(*rs.geo, no, wi, INFINITY, &nh, nullptr)) break;  // environment: none
     Vertex nv = shading_vertex(rs, nh, wi);
-    const OpenPbr* nm = &rs.materials[nv.material];
-    V3 e;
-    if (material_le(*nm, nv, &e)) {
-      if (delta) {
-        radiance = add(radiance, vmul(throughput, e));
-      } else {
-        // balance_heuristic(bsdf_pdf, 0) = 1 for bsdf_pdf > 0, else 0
-        float w = (s.pdf <= 0.0f || s.pdf + 0.0f <= 0.0f) ? 0.0f : s.pdf / (s.pdf + 0.0f);
-        radiance = add(radiance, smul(vmul(throughput, e), w));
-      }
-    }
+    const Material* nm = &rs.materials[nv.material];
+    radiance = add(radiance, emitted_from_bsdf_sample(rs, throughput, s.pdf, delta, *mat, vtx, nv, *nm));
     vtx = nv;
     mat = nm;
   }

@@ -368,13 +368,11 @@ void orc_camera_rays(const float* cam, uint32_t rx, uint32_t ry, uint32_t spp, u
 extern "C" {
 
 // ---- path tracer ------------------------------------------------------------
-// OpenPBR material record as laid out by the product ABI (TptOpenPbr).
-struct OrcOpenPbr {
-  float base_weight, base_color[3], base_diffuse_roughness, base_metalness;
-  float specular_weight, specular_color[3], specular_roughness, specular_ior, specular_roughness_anisotropy;
-  float transmission_weight, coat_weight, fuzz_weight, thin_film_weight;
-  float emission_luminance, emission_color[3];
-  uint32_t thin_walled;
+// Material record as laid out by the product ABI (TptMaterial): kind, pad,
+// then the kind's parameters (TptOpenPbr for OpenPBR).
+struct OrcMaterial {
+  uint32_t kind, pad;
+  float raw[22];
 };
 
 // Shading data for an oracle scene: per mesh vertex normals / uvs (nullable),
@@ -395,28 +393,131 @@ void orc_render_set_mesh(void* h, uint32_t mesh, const float* normals, const flo
 void orc_render_set_instance_material(void* h, uint32_t inst, uint32_t mat) {
   ((RenderScene*)h)->inst_material[inst] = mat;
 }
-void orc_render_set_materials(void* h, const OrcOpenPbr* m, size_t n) {
+// Sets the material table and builds the area-light table and light tree.
+void orc_render_set_materials(void* h, const OrcMaterial* m, size_t n) {
   RenderScene* r = (RenderScene*)h;
-  r->materials.resize(n);
+  r->materials.assign(n, Material{});
   for (size_t i = 0; i < n; ++i) {
-    OpenPbr& o = r->materials[i];
-    o.base_weight = m[i].base_weight;
-    o.base_color = v3(m[i].base_color[0], m[i].base_color[1], m[i].base_color[2]);
-    o.base_diffuse_roughness = m[i].base_diffuse_roughness;
-    o.base_metalness = m[i].base_metalness;
-    o.specular_weight = m[i].specular_weight;
-    o.specular_color = v3(m[i].specular_color[0], m[i].specular_color[1], m[i].specular_color[2]);
-    o.specular_roughness = m[i].specular_roughness;
-    o.specular_ior = m[i].specular_ior;
-    o.specular_roughness_anisotropy = m[i].specular_roughness_anisotropy;
-    o.transmission_weight = m[i].transmission_weight;
-    o.coat_weight = m[i].coat_weight;
-    o.fuzz_weight = m[i].fuzz_weight;
-    o.thin_film_weight = m[i].thin_film_weight;
-    o.emission_luminance = m[i].emission_luminance;
-    o.emission_color = v3(m[i].emission_color[0], m[i].emission_color[1], m[i].emission_color[2]);
-    o.thin_walled = m[i].thin_walled;
+    Material& d = r->materials[i];
+    const float* f = m[i].raw;
+    d.kind = m[i].kind;
+    if (d.kind == kMatOpenPbr) {
+      OpenPbr& o = d.pbr;
+      o.base_weight = f[0];
+      o.base_color = v3(f[1], f[2], f[3]);
+      o.base_diffuse_roughness = f[4];
+      o.base_metalness = f[5];
+      o.specular_weight = f[6];
+      o.specular_color = v3(f[7], f[8], f[9]);
+      o.specular_roughness = f[10];
+      o.specular_ior = f[11];
+      o.specular_roughness_anisotropy = f[12];
+      o.transmission_weight = f[13];
+      o.coat_weight = f[14];
+      o.fuzz_weight = f[15];
+      o.thin_film_weight = f[16];
+      o.emission_luminance = f[17];
+      o.emission_color = v3(f[18], f[19], f[20]);
+      uint32_t tw;
+      std::memcpy(&tw, &f[21], 4);
+      o.thin_walled = tw;
+    } else {
+      d.color = v3(f[0], f[1], f[2]);
+      d.strength = f[3];
+    }
   }
+  build_lights(*r);
+}
+
+// Light tree export, 20 floats per node: flux, mu[3], sigma_s2, nu[3],
+// lambda, radius, area, then as u32 bits: left, right, parent, instance,
+// triangle (kInvalid where absent), leaf flag, pad x3.  Returns the count.
+uint32_t orc_render_light_tree(void* h, float* out, uint32_t cap) {
+  const lt::Tree& t = ((RenderScene*)h)->tree;
+  const uint32_t n = (uint32_t)t.nodes.size();
+  for (uint32_t i = 0; i < n && i < cap; ++i) {
+    const lt::Node& d = t.nodes[i];
+    float* o = out + 20 * i;
+    float v[11] = {d.flux, d.mu.x, d.mu.y, d.mu.z, d.sigma_s2, d.nu.x, d.nu.y, d.nu.z, d.lambda, d.radius, d.area};
+    std::memcpy(o, v, sizeof(v));
+    uint32_t u[9] = {d.left, d.right, d.parent, d.inst, d.tri, d.leaf ? 1u : 0u, 0u, 0u, 0u};
+    std::memcpy(o + 11, u, sizeof(u));
+  }
+  return n;
+}
+
+// Light-tree queries of a diffuse shading point (p, n, rho): the stochastic
+// descent (leaf node + pmf, false if none) and the reverse leaf pmf.
+int orc_render_tree_sample(void* h, const float* p, const float* n, float rho, float u, uint32_t* leaf, float* pmf) {
+  lt::Query q{v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]), rho};
+  return lt::sample_tree(((RenderScene*)h)->tree, q, u, leaf, pmf) ? 1 : 0;
+}
+float orc_render_tree_pdf(void* h, const float* p, const float* n, float rho, uint32_t leaf) {
+  lt::Query q{v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]), rho};
+  return lt::pdf_leaf(((RenderScene*)h)->tree, q, leaf);
+}
+
+// direct_light_mis_contribution at the shading vertex of triangle
+// (inst, tri) at barycentrics b[3], seen along `dir` (incident direction).
+void orc_render_direct_light(void* h, uint32_t inst, uint32_t tri, const float* b, const float* dir, float u_root,
+                             float u_tree, float u_aux, const float* us, float* out) {
+  const RenderScene& rs = *(RenderScene*)h;
+  Vertex v = vertex_at(rs, inst, tri, b[0], b[1], b[2], v3(dir[0], dir[1], dir[2]));
+  V3 r = direct_light(rs, rs.materials[v.material], v, u_root, u_tree, u_aux, us[0], us[1]);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+// emitted_radiance_from_bsdf_sample_area for the shading vertex (inst, tri,
+// b, dir) and the light vertex (linst, ltri, lb, ldir).
+void orc_render_emitted(void* h, uint32_t inst, uint32_t tri, const float* b, const float* dir, uint32_t linst,
+                        uint32_t ltri, const float* lb, const float* ldir, const float* throughput, float bsdf_pdf,
+                        int delta, float* out) {
+  const RenderScene& rs = *(RenderScene*)h;
+  Vertex v = vertex_at(rs, inst, tri, b[0], b[1], b[2], v3(dir[0], dir[1], dir[2]));
+  Vertex lv = vertex_at(rs, linst, ltri, lb[0], lb[1], lb[2], v3(ldir[0], ldir[1], ldir[2]));
+  V3 r = emitted_from_bsdf_sample(rs, v3(throughput[0], throughput[1], throughput[2]), bsdf_pdf, delta != 0,
+                                  rs.materials[v.material], v, lv, rs.materials[lv.material]);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+// One path: trace_radiance from (o, d) with the PathSampler of pixel
+// (px, py), sample si of an (rx, ry) frame at spp samples per pixel.
+void orc_render_trace(void* h, const float* o, const float* d, uint32_t rx, uint32_t ry, uint32_t spp, uint32_t px,
+                      uint32_t py, uint32_t si, uint32_t max_depth, float* out) {
+  const RenderScene& rs = *(RenderScene*)h;
+  PathSampler ps(px, py, si, spp, rx, ry);
+  V3 r = trace_radiance(rs, v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), ps, max_depth);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+
+// SG helpers of R/math/sg.rs by index (test access): 0 expm1_over_x(a),
+// 1 erf(a), 2 sg_integral(a), 3 upper clamped-cosine / 2pi (a), 4 lower (a),
+// 5 clamped-cosine product / pi (z=a, sharpness=b), 6 vmf axis length ->
+// sharpness (a).
+float orc_sg(int fn, float a, float b) {
+  switch (fn) {
+    case 0: return lt::expm1_over_x(a);
+    case 1: return lt::erf_approx(a);
+    case 2: return lt::sg_integral(a);
+    case 3: return lt::upper_cc_over_2pi(a);
+    case 4: return lt::lower_cc_over_2pi(a);
+    case 5: return lt::sg_cc_product_over_pi(a, b);
+    case 6: return lt::vmf_axis_length_to_sharpness(a);
+  }
+  return NAN;
+}
+// sg_product: out = axis[3], sharpness, log_amplitude
+void orc_sg_product(const float* a1, float s1, const float* a2, float s2, float* out) {
+  lt::SgLobe l = lt::sg_product(v3(a1[0], a1[1], a1[2]), s1, v3(a2[0], a2[1], a2[2]), s2);
+  out[0] = l.axis.x;
+  out[1] = l.axis.y;
+  out[2] = l.axis.z;
+  out[3] = l.sharpness;
+  out[4] = l.log_amplitude;
 }
 
 // The pixel loop of R/main.rs:98-126 over rows [y0, y1): PathSampler,

@@ -62,6 +62,19 @@ def lib() -> ctypes.CDLL:
         L.orc_render_set_instance_material.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
         L.orc_render_set_materials.argtypes = [vp, vp, sz]
         L.orc_render.argtypes = [vp, vp] + [ctypes.c_uint32] * 6 + [vp, ctypes.c_int]
+        u32, f32 = ctypes.c_uint32, ctypes.c_float
+        L.orc_render_light_tree.argtypes = [vp, vp, u32]
+        L.orc_render_light_tree.restype = u32
+        L.orc_render_tree_sample.argtypes = [vp, vp, vp, f32, f32, vp, vp]
+        L.orc_render_tree_sample.restype = ctypes.c_int
+        L.orc_render_tree_pdf.argtypes = [vp, vp, vp, f32, u32]
+        L.orc_render_tree_pdf.restype = f32
+        L.orc_render_direct_light.argtypes = [vp, u32, u32, vp, vp, f32, f32, f32, vp, vp]
+        L.orc_render_emitted.argtypes = [vp, u32, u32, vp, vp, u32, u32, vp, vp, vp, f32, ctypes.c_int, vp]
+        L.orc_render_trace.argtypes = [vp, vp, vp] + [u32] * 7 + [vp]
+        L.orc_sg.argtypes = [ctypes.c_int, f32, f32]
+        L.orc_sg.restype = f32
+        L.orc_sg_product.argtypes = [vp, f32, vp, f32, vp]
         L.orc_closest_hit.argtypes = [vp, vp, sz, vp, vp, ctypes.c_int]
         L.orc_closest_hit.restype = ctypes.c_int
         L.orc_unoccluded.argtypes = [vp, vp, sz, vp, vp, vp, vp, ctypes.c_int]
@@ -195,7 +208,8 @@ def oracle_scene_from(scene) -> OracleScene:
 
 
 class OracleRenderer:
-    """The oracle's MIS path tracer over a tpt.Scene with OpenPBR materials."""
+    """The oracle's MIS path tracer over a tpt.Scene and its materials
+    (tpt.material records): light tree built from the emissive instances."""
 
     def __init__(self, scene, materials):
         from tpt.material import records
@@ -209,8 +223,54 @@ class OracleRenderer:
             lib().orc_render_set_mesh(self.h, i, _p(n), _p(uv), len(m.positions))
         for i, inst in enumerate(scene.instances):
             lib().orc_render_set_instance_material(self.h, i, inst.material_index)
+        self.scene_meshes = scene.meshes
         self.recs = records(materials)
         lib().orc_render_set_materials(self.h, ctypes.addressof(self.recs), len(materials))
+
+    def light_tree(self):
+        """Light-tree nodes: structured array (see orc_render_light_tree)."""
+        cap = 2 * sum(len(m.indices) for m in self.scene_meshes) + 1
+        raw = np.zeros((cap, 20), np.float32)
+        n = lib().orc_render_light_tree(self.h, _p(raw), cap)
+        raw = raw[:n]
+        u = raw.view(np.uint32)
+        return dict(flux=raw[:, 0], mu=raw[:, 1:4], sigma_s2=raw[:, 4], nu=raw[:, 5:8], **{"lambda": raw[:, 8]},
+                    radius=raw[:, 9], area=raw[:, 10], left=u[:, 11], right=u[:, 12], parent=u[:, 13],
+                    instance=u[:, 14], triangle=u[:, 15], leaf=u[:, 16].astype(bool))
+
+    # numpy temporaries are bound to names so they outlive the ctypes calls
+    def tree_sample(self, p, n, rho, u):
+        leaf = ctypes.c_uint32(0)
+        pmf = ctypes.c_float(0)
+        pa, na = np.asarray(p, np.float32), np.asarray(n, np.float32)
+        ok = lib().orc_render_tree_sample(self.h, _p(pa), _p(na), rho, u, ctypes.byref(leaf), ctypes.byref(pmf))
+        return (leaf.value, pmf.value) if ok else None
+
+    def tree_pdf(self, p, n, rho, leaf):
+        pa, na = np.asarray(p, np.float32), np.asarray(n, np.float32)
+        return lib().orc_render_tree_pdf(self.h, _p(pa), _p(na), rho, leaf)
+
+    def direct_light(self, inst, tri, bary, direction, u_root, u_tree, u_aux, us):
+        out = np.zeros(3, np.float32)
+        b, d, u = (np.asarray(a, np.float32) for a in (bary, direction, us))
+        lib().orc_render_direct_light(self.h, inst, tri, _p(b), _p(d), u_root, u_tree, u_aux, _p(u), _p(out))
+        return out
+
+    def emitted(self, vertex, light_vertex, throughput, bsdf_pdf, delta):
+        """vertex / light_vertex: (instance, triangle, barycentrics, incident direction)."""
+        (i, t, b, d), (li, lt_, lb, ld) = vertex, light_vertex
+        b, d, lb, ld, th = (np.asarray(a, np.float32) for a in (b, d, lb, ld, throughput))
+        out = np.zeros(3, np.float32)
+        lib().orc_render_emitted(self.h, i, t, _p(b), _p(d), li, lt_, _p(lb), _p(ld), _p(th), bsdf_pdf, int(delta),
+                                 _p(out))
+        return out
+
+    def trace(self, origin, direction, frame, spp, pixel, sample_index, max_depth):
+        out = np.zeros(3, np.float32)
+        o, d = np.asarray(origin, np.float32), np.asarray(direction, np.float32)
+        lib().orc_render_trace(self.h, _p(o), _p(d), frame[0], frame[1], spp, pixel[0], pixel[1], sample_index,
+                               max_depth, _p(out))
+        return out
 
     def render(self, cam, width, height, spp, max_depth, rows=None, threads=8):
         """cam: tpt.camera.pinhole(...) record.  Returns (height, width, 3) f32."""

@@ -1,0 +1,293 @@
+"""Pins the oracle's light sampling: the SG helpers (R/math/sg.rs:350-593
+tests plus closed forms in float64), the light-tree builder
+(R/light_tree/builder.rs:797-888 tests plus structural invariants), the
+stochastic descent (leaf pmfs sum to one, the sampled pmf equals the reverse
+pmf, and sample frequencies over a uniform grid of u equal the pmfs), and
+the MIS integrator's closed forms (R/integrator/mis.rs:377-458) on the
+reference's own test scenes (R/integrator/mod.rs:285-565)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import tpt
+from tpt import scenes
+from tpt.material import Emissive, Mirror, NormalizedLambert, OpenPbr
+
+L = oracle.lib()
+
+
+def sg(fn, a, b=0.0):
+    return float(L.orc_sg(fn, a, b))
+
+
+def close(a, b, tol):
+    return abs(a - b) <= tol
+
+
+# ---------------------------------------------------------------- SG math
+def test_expm1_over_x():  # sg.rs:355-375
+    assert close(sg(0, 0.0), 1.0, 1e-6)
+    assert close(sg(0, 1.0), math.e - 1.0, 1e-5)
+    x = np.float32(-1e-4)
+    assert close(sg(0, float(x)), math.expm1(float(x)) / float(x), 1e-5)
+    for x in np.linspace(-30, 5, 71):
+        x = float(np.float32(x))
+        if x != 0.0:
+            assert close(sg(0, x), math.expm1(x) / x, 2e-6 * max(1.0, math.expm1(x) / x))
+
+
+def test_erf_against_libm():  # sg.rs:378-385 and the approximation's bound
+    assert close(sg(1, 0.0), 0.0, 1e-6) and close(sg(1, 5.0), 1.0, 1e-6)
+    for x in np.linspace(-4.5, 4.5, 181):
+        assert close(sg(1, float(x)), math.erf(float(x)), 2e-4), x
+
+
+def test_sg_integral_closed_form():  # sg.rs:387-398: 4pi at 0, 2pi/k for large k
+    assert close(sg(2, 0.0), 4 * math.pi, 1e-4)
+    for k in [1e-3, 0.1, 0.5, 1.0, 3.0, 10.0, 100.0, 1e4]:
+        exact = 2 * math.pi * (1 - math.exp(-2 * k)) / k
+        assert close(sg(2, k), exact, 1e-5 * exact), k
+
+
+def test_sg_product_identity():  # sg.rs:400-416 plus the exponent identity
+    out = np.zeros(5, np.float32)
+    z = np.array([0, 0, 1], np.float32)
+    L.orc_sg_product(z.ctypes.data, 5.0, z.ctypes.data, 3.0, out.ctypes.data)
+    assert np.allclose(out[:3], z, atol=1e-5) and close(out[3], 8.0, 1e-5) and close(out[4], 0.0, 1e-5)
+    L.orc_sg_product(z.ctypes.data, 5.0, (-z).ctypes.data, 5.0, out.ctypes.data)
+    assert close(out[3], 0.0, 1e-5) and close(out[4], -10.0, 1e-5)
+    # s1 (o.a1 - 1) + s2 (o.a2 - 1) == s (o.axis - 1) + log_amplitude for every direction o
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        a1, a2, o = (v / np.linalg.norm(v) for v in rng.normal(size=(3, 3)))
+        s1, s2 = rng.uniform(0.1, 50, 2)
+        a1, a2 = a1.astype(np.float32), a2.astype(np.float32)
+        L.orc_sg_product(a1.ctypes.data, float(s1), a2.ctypes.data, float(s2), out.ctypes.data)
+        lhs = s1 * (o @ a1 - 1) + s2 * (o @ a2 - 1)
+        rhs = out[3] * (o @ out[:3].astype(np.float64) - 1) + out[4]
+        assert close(lhs, rhs, 1e-4 * (s1 + s2)), (lhs, rhs)
+
+
+def _cc_integral(z, k, n=400):
+    """float64 quadrature of exp(k (o.xi - 1)) max(o.n, 0) over the sphere / pi, xi.n = z."""
+    th = (np.arange(n) + 0.5) * math.pi / n
+    ph = (np.arange(2 * n) + 0.5) * math.pi / n
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    o = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1)
+    xi = np.array([math.sqrt(max(0.0, 1 - z * z)), 0.0, z])
+    f = np.exp(k * (o @ xi - 1)) * np.maximum(o[..., 2], 0) * np.sin(T)
+    return f.sum() * (math.pi / n) ** 2 / math.pi
+
+
+def test_clamped_cosine_product_integral():  # sg.rs:490-535 and quadrature
+    assert close(sg(3, 0.0), 0.5, 1e-4) and close(sg(4, 0.0), 0.5, 1e-4)
+    assert close(sg(5, 1.0, 0.0), 1.0, 1e-3)
+    for z in [-1.0, -0.5, 0.0, 0.5, 1.0]:
+        for k in [0.0, 0.1, 1.0, 10.0, 100.0, 1000.0]:
+            assert sg(5, z, k) >= 0.0
+    prev = sg(5, -1.0, 50.0)
+    for i in range(1, 21):
+        cur = sg(5, -1.0 + 2.0 * i / 20, 50.0)
+        assert cur >= prev - 1e-6
+        prev = cur
+    # the fitted approximation (Tokuyoshi 2024) against float64 quadrature:
+    # within 2% of the peak value (a dropped term shows up far above that)
+    for k in [0.3, 2.0, 8.0, 30.0]:
+        peak = _cc_integral(1.0, k)
+        for z in [-0.8, -0.3, 0.0, 0.4, 0.9, 1.0]:
+            assert close(sg(5, z, k), _cc_integral(z, k), 0.02 * peak), (z, k)
+    # upper / lower closed forms against float64
+    for s in [0.01, 0.3, 0.5, 0.7, 2.0, 20.0]:
+        up = (math.expm1(-s) + s) / (s * s)
+        lo = math.exp(-s) * (-math.expm1(-s) - s * math.exp(-s)) / (s * s)
+        assert close(sg(3, s), up, 1e-5) and close(sg(4, s), lo, 1e-5)
+
+
+def test_vmf_sharpness():  # sg.rs:454-470, Banerjee's approximation r (3 - r^2) / (1 - r^2)
+    vals = [sg(6, r) for r in (0.1, 0.5, 0.9)]
+    assert vals[0] < vals[1] < vals[2]
+    for r in [0.0, 0.2, 0.5, 0.8, 0.99]:
+        assert close(sg(6, r), r * (3 - r * r) / (1 - r * r), 1e-5 * max(1, sg(6, r)))
+    assert sg(6, 1.0) == 2.0 ** 41
+
+
+# ---------------------------------------------------------------- scenes
+UNIT_V = [(0.0, 0.0), (1.0, 0.0), (0.0, 1.0)]
+
+
+def unit_mesh(z):  # R/light/mod.rs test_helpers::unit_mesh
+    pos = np.array([(x, y, z) for x, y in UNIT_V], np.float32)
+    return tpt.Mesh(pos, np.array([[0, 1, 2]], np.uint32), normals=np.tile([0, 0, 1], (3, 1)).astype(np.float32),
+                    uvs=np.array(UNIT_V, np.float32))
+
+
+def renderer(meshes, instances, materials):
+    s = tpt.Scene()
+    for m in meshes:
+        s.add_mesh(m)
+    for m in materials:
+        s.add_material(m)
+    for mesh, mat, l2w in instances:
+        s.add_instance(mesh, mat, l2w)
+    s.build_qbvh()
+    return s, oracle.OracleRenderer(s, s.materials)
+
+
+def floor_with_area_light(albedo, strength):  # R/integrator/mod.rs:411-440
+    return renderer([unit_mesh(0.0), unit_mesh(1.0)], [(0, 0, np.eye(4)), (1, 1, np.eye(4))],
+                    [NormalizedLambert((albedo,) * 3), Emissive((1.0, 1.0, 1.0), strength)])
+
+
+# ---------------------------------------------------------------- light tree
+def test_single_emissive_triangle_is_one_leaf():  # builder.rs:803-815
+    _, r = renderer([unit_mesh(0.0)], [(0, 0, np.eye(4))], [Emissive((1.0, 1.0, 1.0), 1.0)])
+    t = r.light_tree()
+    assert len(t["flux"]) == 1 and t["leaf"][0] and close(t["flux"][0], 0.5, 1e-3)
+
+
+def test_two_triangles_make_an_internal_root():  # builder.rs:817-837
+    m5 = np.eye(4)
+    _, r = renderer([unit_mesh(0.0), unit_mesh(5.0)], [(0, 0, np.eye(4)), (1, 0, np.eye(4))],
+                    [Emissive((1.0, 1.0, 1.0), 1.0)])
+    t = r.light_tree()
+    assert len(t["flux"]) >= 3 and not t["leaf"][0]
+    assert close(t["flux"][t["leaf"]].sum(), t["flux"][0], 1e-4)
+    del m5
+
+
+def test_non_emissive_mesh_is_ignored():  # builder.rs:839-848
+    _, r = renderer([unit_mesh(0.0)], [(0, 0, np.eye(4))], [NormalizedLambert((0.5,) * 3)])
+    assert len(r.light_tree()["flux"]) == 0
+
+
+def emissive_field(n=64, seed=5):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, 10, (n, 3))
+    v = c[:, None, :] + rng.normal(scale=0.4, size=(n, 3, 3))
+    mesh = tpt.Mesh(v.reshape(-1, 3).astype(np.float32), np.arange(3 * n, dtype=np.uint32).reshape(-1, 3))
+    power = [Emissive((1.0, 0.5, 0.25), float(s)) for s in (1.0, 4.0)]
+    return renderer([mesh, unit_mesh(-1.0)], [(0, 0, np.eye(4)), (1, 1, np.eye(4))], power + [NormalizedLambert()])
+
+
+def test_tree_structure_invariants():
+    s, r = emissive_field()
+    t = r.light_tree()
+    n = len(t["flux"])
+    leaves = np.nonzero(t["leaf"])[0]
+    assert len(leaves) == 65 and n == 2 * 65 - 1
+    # every emissive triangle exactly once
+    got = sorted(zip(t["instance"][leaves].tolist(), t["triangle"][leaves].tolist()))
+    assert got == [(0, i) for i in range(64)] + [(1, 0)]
+    inner = np.nonzero(~t["leaf"])[0]
+    assert t["parent"][0] == 0xFFFFFFFF
+    for i in inner:
+        lc, rc = t["left"][i], t["right"][i]
+        assert t["parent"][lc] == i and t["parent"][rc] == i
+        assert lc == i + 1  # depth-first pre-order: left subtree follows its parent
+        fl, fr = t["flux"][lc], t["flux"][rc]
+        assert close(t["flux"][i], fl + fr, 1e-5 * t["flux"][i])
+        mu = (fl * t["mu"][lc] + fr * t["mu"][rc]) / (fl + fr)
+        assert np.allclose(t["mu"][i], mu, atol=1e-4)
+        assert np.isclose(np.linalg.norm(t["nu"][i]), 1.0, atol=1e-5) or np.all(t["nu"][i] == [0, 0, 1])
+    # leaf values: flux = max emission x area; mu = centroid; nu = geometric normal
+    for k in leaves:
+        inst, tri = t["instance"][k], t["triangle"][k]
+        mesh = s.meshes[s.instances[inst].mesh_index]
+        p = mesh.positions[mesh.indices[tri]].astype(np.float64)
+        area = 0.5 * np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0]))
+        emax = 1.0 if inst == 0 else 4.0
+        assert close(t["flux"][k], emax * area, 1e-5 * emax * area)
+        assert close(t["area"][k], area, 1e-5 * area)
+        assert np.allclose(t["mu"][k], p.mean(0), atol=1e-5)
+        ng = np.cross(p[1] - p[0], p[2] - p[0])
+        assert np.allclose(t["nu"][k], ng / np.linalg.norm(ng), atol=1e-5)
+        assert close(t["lambda"][k], 0.5 * (3 - 0.25) / (1 - 0.25), 1e-5)
+
+
+def test_descent_pmfs():
+    """Leaf pmfs sum to one, sample pmf == reverse pmf, and the leaf picked
+    for u on a uniform grid has frequency == its pmf (hierarchical warping
+    partitions [0, 1) into one interval per leaf)."""
+    _, r = emissive_field()
+    t = r.light_tree()
+    leaves = np.nonzero(t["leaf"])[0]
+    rng = np.random.default_rng(11)
+    for _ in range(6):
+        p = rng.uniform(-2, 12, 3)
+        nrm = rng.normal(size=3)
+        nrm /= np.linalg.norm(nrm)
+        pdf = np.array([r.tree_pdf(p, nrm, 0.5, int(k)) for k in leaves])
+        assert close(pdf.sum(), 1.0, 1e-4)
+        grid = 20000
+        counts = {}
+        for i in range(grid):
+            leaf, pmf = r.tree_sample(p, nrm, 0.5, (i + 0.5) / grid)
+            assert close(pmf, r.tree_pdf(p, nrm, 0.5, leaf), 1e-5 * pmf)
+            counts[leaf] = counts.get(leaf, 0) + 1
+        for k, c in zip(leaves, pdf):
+            assert abs(counts.get(int(k), 0) / grid - c) <= 2.0 / grid + 1e-6
+
+
+# ---------------------------------------------------------------- MIS closed forms
+def test_direct_light_balance_heuristic():  # mis.rs:400-420
+    _, r = floor_with_area_light(0.8, 10.0)
+    out = r.direct_light(0, 0, (0.5, 0.25, 0.25), (0, 0, -1), 0.0, 0.0, 0.5, (0.25, 0.5))
+    expected = (4.0 / math.pi) * (2.0 / (2.0 + 1.0 / math.pi))
+    assert np.allclose(out, expected, atol=1e-3)
+
+
+def test_emitted_radiance_balance_heuristic():  # mis.rs:422-444
+    _, r = floor_with_area_light(0.8, 10.0)
+    vtx = (0, 0, (0.5, 0.25, 0.25), (0, 0, -1))
+    lvtx = (1, 0, (0.5, 0.25, 0.25), (0, 0, 1))
+    out = r.emitted(vtx, lvtx, (0.8, 0.8, 0.8), 1.0 / math.pi, False)
+    assert np.allclose(out, 8.0 * ((1.0 / math.pi) / (2.0 + 1.0 / math.pi)), atol=1e-3)
+    out = r.emitted(vtx, lvtx, (0.8, 0.8, 0.8), 1.0, True)  # mis.rs:446-458
+    assert np.allclose(out, 8.0, atol=1e-3)
+
+
+def test_light_after_delta_bounce():  # mis.rs:377-386 with mirror_to_light_scene (mod.rs:285-314)
+    mirror = tpt.Mesh(np.array([(-1, -1, 0), (2, -1, 0), (-1, 2, 0)], np.float32), np.array([[0, 1, 2]], np.uint32),
+                      normals=np.tile([0, 0, 1], (3, 1)).astype(np.float32),
+                      uvs=np.array(UNIT_V, np.float32))
+    light = tpt.Mesh(np.array([(0.45, 0, 1), (0.95, 0, 1), (0.45, 0.5, 1)], np.float32),
+                     np.array([[0, 1, 2]], np.uint32), normals=np.tile([0, 0, 1], (3, 1)).astype(np.float32),
+                     uvs=np.array(UNIT_V, np.float32))
+    color = np.array([0.25, 0.5, 0.75], np.float32)
+    _, r = renderer([mirror, light], [(0, 0, np.eye(4)), (1, 1, np.eye(4))],
+                    [Mirror(tuple(color)), Emissive((1.0, 1.0, 1.0), 4.0)])
+    hit = np.array([0.25, 0.20, 0.0], np.float32)
+    lh = np.array([0.65, 0.20, 1.0], np.float32)
+    refl = (lh - hit) / np.linalg.norm(lh - hit)
+    inc = np.array([refl[0], refl[1], -refl[2]], np.float32)
+    inc /= np.linalg.norm(inc)
+    out = r.trace(hit - inc, inc, (1, 1), 1, (0, 0), 0, 2)
+    assert np.allclose(out, color * 4.0, atol=1e-3)
+
+
+def test_direct_illumination_matches_quadrature():
+    """NEE + BSDF sampling with balance-heuristic weights is an unbiased
+    estimate of direct illumination: the mean over 4096 ZSobol samples of a
+    depth-1 path onto the Lambert floor equals rho/pi * E, with E the float64
+    quadrature of L cos cos' / r^2 over the light triangle.  A wrong MIS
+    weight or pdf shows up as a bias well beyond the 1.5% allowed."""
+    _, r = floor_with_area_light(0.8, 10.0)
+    p = np.array([0.25, 0.25, 0.0])
+    origin = np.array([-2.0, 0.25, 1.0])
+    d = (p - origin) / np.linalg.norm(p - origin)
+    n = 4096
+    acc = np.zeros(3)
+    for si in range(n):
+        acc += r.trace(origin, d, (1, 1), n, (0, 0), si, 1)
+    est = acc / n
+    g = 1200
+    u, v = np.meshgrid((np.arange(g) + 0.5) / g, (np.arange(g) + 0.5) / g, indexing="ij")
+    keep = u + v < 1.0
+    q = np.stack([u[keep], v[keep], np.ones(keep.sum())], -1)
+    r2 = ((q - p) ** 2).sum(-1)
+    e = 10.0 * (1.0 / r2 ** 2).sum() / (g * g)  # cos = cos' = 1 / r, dA = 1/g^2
+    expected = 0.8 / math.pi * e
+    assert np.allclose(est, expected, rtol=0.015), (est, expected)

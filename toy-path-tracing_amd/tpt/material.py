@@ -1,8 +1,11 @@
-"""OpenPBR material parameters (R/material/open_pbr.rs:23-143).
+"""Materials of the backend (R/material/mod.rs:100-115 variants carried).
 
 ``OpenPbr.new(base_color)`` gives the reference's defaults
-(``OpenPbrMaterial::new``); ``with_*`` mirror its builder methods.  The
-record crosses the ABI as ``TptOpenPbr`` (include/tpt.h).
+(``OpenPbrMaterial::new``, R/material/open_pbr.rs:23-143); ``with_*`` mirror
+its builder methods.  ``NormalizedLambert``, ``Emissive`` and ``Mirror``
+are the reference's small materials (R/material/normalized_lambert.rs,
+emissive.rs, mirror.rs).  Every material crosses the ABI as a
+``TptMaterial`` record (kind + parameters, include/tpt.h).
 """
 from __future__ import annotations
 
@@ -25,6 +28,18 @@ class OpenPbrRecord(ctypes.Structure):
 
 
 assert ctypes.sizeof(OpenPbrRecord) == 88
+
+KIND_OPEN_PBR, KIND_NORMALIZED_LAMBERT, KIND_EMISSIVE, KIND_MIRROR = 0, 1, 2, 3
+
+
+class MaterialRecord(ctypes.Structure):
+    """TptMaterial: kind, pad, then the kind's parameters (OpenPBR: the
+    88-byte TptOpenPbr; Lambert: rho[3]; Emissive: color[3], strength;
+    Mirror: color[3])."""
+    _fields_ = [("kind", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("params", ctypes.c_float * 22)]
+
+
+assert ctypes.sizeof(MaterialRecord) == 96
 
 
 @dataclass(frozen=True)
@@ -55,7 +70,7 @@ class OpenPbr:
         """The reference's with_* builders, by field name."""
         return replace(self, **kw)
 
-    def record(self) -> OpenPbrRecord:
+    def open_pbr_record(self) -> OpenPbrRecord:
         r = OpenPbrRecord()
         for name, _ in OpenPbrRecord._fields_:
             v = getattr(self, name)
@@ -65,9 +80,53 @@ class OpenPbr:
                 setattr(r, name, int(v) if name == "thin_walled" else float(v))
         return r
 
+    def record(self) -> MaterialRecord:
+        r = MaterialRecord()
+        r.kind = KIND_OPEN_PBR
+        rec = self.open_pbr_record()  # kept alive across the copy
+        ctypes.memmove(ctypes.addressof(r.params), ctypes.addressof(rec), 88)
+        return r
+
+
+@dataclass(frozen=True)
+class NormalizedLambert:
+    """NormalizedLambertMaterial::new(rho) (R/material/normalized_lambert.rs:27-37)."""
+    rho: tuple = (0.8, 0.8, 0.8)
+
+    def record(self) -> MaterialRecord:
+        r = MaterialRecord()
+        r.kind = KIND_NORMALIZED_LAMBERT
+        r.params[0:3] = [float(x) for x in self.rho]
+        return r
+
+
+@dataclass(frozen=True)
+class Emissive:
+    """EmissiveMaterial::new(color, strength) (R/material/emissive.rs:17-26)."""
+    color: tuple = (1.0, 1.0, 1.0)
+    strength: float = 1.0
+
+    def record(self) -> MaterialRecord:
+        r = MaterialRecord()
+        r.kind = KIND_EMISSIVE
+        r.params[0:4] = [float(x) for x in self.color] + [float(self.strength)]
+        return r
+
+
+@dataclass(frozen=True)
+class Mirror:
+    """MirrorMaterial::new(color) (R/material/mirror.rs)."""
+    color: tuple = (1.0, 1.0, 1.0)
+
+    def record(self) -> MaterialRecord:
+        r = MaterialRecord()
+        r.kind = KIND_MIRROR
+        r.params[0:3] = [float(x) for x in self.color]
+        return r
+
 
 def records(materials) -> ctypes.Array:
-    arr = (OpenPbrRecord * max(1, len(materials)))()
+    arr = (MaterialRecord * max(1, len(materials)))()
     for i, m in enumerate(materials):
         arr[i] = m.record()
     return arr

@@ -207,6 +207,20 @@ def cornell(build: bool = True) -> CornellScene:
     return CornellScene(s, mats, li, cam)
 
 
+def cornell_lambert(build: bool = True) -> CornellScene:
+    """The same Cornell geometry with the reference's NormalizedLambert walls
+    (white 0.8, red, green) and an Emissive light of strength 15: these
+    surfaces build a diffuse light-tree query, so next-event estimation
+    through the light tree is active (it is not for OpenPBR surfaces)."""
+    from .material import Emissive, NormalizedLambert
+
+    cs = cornell(build=build)
+    mats = [NormalizedLambert((0.8, 0.8, 0.8)), NormalizedLambert((0.8, 0.1, 0.1)),
+            NormalizedLambert((0.1, 0.8, 0.1)), Emissive((1.0, 1.0, 1.0), 15.0)]
+    cs.scene.materials[:] = mats
+    return CornellScene(cs.scene, mats, cs.light_instance, cs.camera)
+
+
 # --------------------------------------------------------------------------- rays
 def primary_rays(width: int, height: int, eye, look_at, up, fov_y_deg: float) -> np.ndarray:
     """Pixel-centre pinhole rays (direction_for_sample, R/scene/camera.rs:80-87), row-major."""
