@@ -241,52 +241,61 @@ def measured_traffic(kind, nodes, n_rays):
 
 
 def render_bench(dev, stream, args):
-    """Plain FP32 megakernel path tracer on the Cornell configuration:
-    512x512, 64 spp, max depth 8, OpenPBR diffuse (specular weight 0)."""
+    """Plain FP32 megakernel path tracer on the Cornell configuration
+    (512x512, 64 spp, max depth 8): the OpenPBR diffuse materials of the
+    configuration (specular weight 0; the reference gives OpenPBR surfaces no
+    light-tree NEE), and the same box with the reference's NormalizedLambert
+    walls + Emissive light, where light-tree NEE with MIS runs."""
     import torch
 
     from tpt import camera, scenes
 
     w = h = 512
     spp, depth = 64, 8
-    cs = scenes.cornell()
-    rds = cs.scene.upload(dev)
-    cam = camera.pinhole(**cs.camera, width=w, height=h)
-    out = torch.empty(w * h * 3, dtype=torch.float32, device="cuda")
-    rds.render_device(cam, w, h, spp, depth, out)
-    torch.cuda.synchronize()
-    reps = max(1, min(args.steps, 5))
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record(stream)
-    for _ in range(reps):
-        rds.render_device(cam, w, h, spp, depth, out)
-    b.record(stream)
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / reps
-    img = out.view(h, w, 3).cpu().numpy()
-    res = {
-        "workload": "Cornell box 512x512, 64 spp, max depth 8, OpenPBR diffuse (specular weight 0), "
-                    "BSDF sampling + emission (reference NEE policy)",
-        "ms_per_frame": round(ms, 3),
-        "samples_per_s": round(w * h * spp / (ms / 1e3), 1),
-        "image_mean": [round(float(x), 4) for x in img.reshape(-1, 3).mean(0)],
-    }
-    log(f"render: {ms:.2f} ms/frame = {res['samples_per_s'] / 1e6:.1f} Msamples/s")
-    if args.cpu_sample > 0:
-        sys.path.insert(0, str(ROOT / "tests"))
-        import oracle  # noqa: E402  (test infrastructure: CPU baseline leg only)
+    out = {}
+    for key, make, label in (
+        ("openpbr", scenes.cornell, "OpenPBR diffuse (specular weight 0), BSDF sampling + emission"),
+        ("lambert_nee", scenes.cornell_lambert, "NormalizedLambert + Emissive light, light-tree NEE + MIS"),
+    ):
+        cs = make()
+        rds = cs.scene.upload(dev)
+        cam = camera.pinhole(**cs.camera, width=w, height=h)
+        img_dev = torch.empty(w * h * 3, dtype=torch.float32, device="cuda")
+        rds.render_device(cam, w, h, spp, depth, img_dev)
+        torch.cuda.synchronize()
+        reps = max(1, min(args.steps, 5))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            rds.render_device(cam, w, h, spp, depth, img_dev)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        img = img_dev.view(h, w, 3).cpu().numpy()
+        res = {
+            "workload": f"Cornell box 512x512, 64 spp, max depth 8, {label}",
+            "ms_per_frame": round(ms, 3),
+            "samples_per_s": round(w * h * spp / (ms / 1e3), 1),
+            "image_mean": [round(float(x), 4) for x in img.reshape(-1, 3).mean(0)],
+        }
+        log(f"render {key}: {ms:.2f} ms/frame = {res['samples_per_s'] / 1e6:.1f} Msamples/s")
+        if args.cpu_sample > 0:
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle  # noqa: E402  (test infrastructure: CPU baseline leg only)
 
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        r = oracle.OracleRenderer(cs.scene, cs.materials)
-        rows = 64
-        t = time.perf_counter()
-        r.render(cam, w, h, spp, depth, rows=(h // 2 - rows // 2, h // 2 + rows // 2), threads=threads)
-        dt = time.perf_counter() - t
-        res["cpu_baseline"] = {"value": round(rows * w * spp / dt, 1), "unit": "samples/s", "cores": threads,
-                               "kind": "port", "sample": f"{rows} middle rows of the frame ({rows * w * spp} samples)"}
-        log(f"render cpu baseline: {rows * w * spp / dt / 1e6:.2f} Msamples/s on {threads} threads")
-    rds.close()
-    return res
+            threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+            r = oracle.OracleRenderer(cs.scene, cs.materials)
+            rows = 64
+            t = time.perf_counter()
+            r.render(cam, w, h, spp, depth, rows=(h // 2 - rows // 2, h // 2 + rows // 2), threads=threads)
+            dt = time.perf_counter() - t
+            res["cpu_baseline"] = {"value": round(rows * w * spp / dt, 1), "unit": "samples/s", "cores": threads,
+                                   "kind": "port",
+                                   "sample": f"{rows} middle rows of the frame ({rows * w * spp} samples, {dt:.1f}s)"}
+            log(f"render {key} cpu baseline: {rows * w * spp / dt / 1e6:.2f} Msamples/s on {threads} threads")
+        rds.close()
+        out[key] = res
+    return out
 
 
 def cpu_baseline(sc, sets, sample):

@@ -106,7 +106,7 @@ typedef struct TptInstanceDesc {
  * holds them after prepare_color (rendering-space colours).  tpt_render
  * supports specular_weight = 0 with no metal, coat, fuzz, transmission,
  * thin film or thin walls; other values are rejected with
- * TPT_ERROR_BAD_ARGUMENT. */
+ * TPT_ERROR_BAD_ARGUMENT.  Carried inside TptMaterial. */
 typedef struct TptOpenPbr {
   float base_weight;
   float base_color[3];
@@ -126,6 +126,29 @@ typedef struct TptOpenPbr {
   uint32_t thin_walled;
 } TptOpenPbr; /* 88 bytes */
 
+/* One entry of the scene's material table: the Material enum of
+ * R/material/mod.rs:100-115 for the variants the backend carries.
+ *   TPT_MATERIAL_OPEN_PBR           open_pbr (TptOpenPbr)
+ *   TPT_MATERIAL_NORMALIZED_LAMBERT rho[3]              (R/material/normalized_lambert.rs:18-37)
+ *   TPT_MATERIAL_EMISSIVE           color[3], strength  (R/material/emissive.rs:9-26)
+ *   TPT_MATERIAL_MIRROR             color[3]            (R/material/mirror.rs)
+ * Textures, normal maps and opacity are not part of the record. */
+#define TPT_MATERIAL_OPEN_PBR 0u
+#define TPT_MATERIAL_NORMALIZED_LAMBERT 1u
+#define TPT_MATERIAL_EMISSIVE 2u
+#define TPT_MATERIAL_MIRROR 3u
+typedef struct TptMaterial {
+  uint32_t kind;
+  uint32_t pad;
+  union {
+    TptOpenPbr open_pbr;
+    struct { float rho[3]; } lambert;
+    struct { float color[3]; float strength; } emissive;
+    struct { float color[3]; } mirror;
+    float raw[22];
+  } u;
+} TptMaterial; /* 96 bytes */
+
 /* Scene after Scene::build_qbvh (R/scene/mod.rs:305-316). */
 typedef struct TptSceneDesc {
   const TptMeshDesc* meshes;
@@ -136,8 +159,28 @@ typedef struct TptSceneDesc {
   uint32_t tlas_node_count;
   uint32_t material_count;
   const uint32_t* tlas_primitive_indices; /* instance_count entries */
-  const TptOpenPbr* materials;            /* material_count entries (TptInstanceDesc::material) */
+  const TptMaterial* materials;           /* material_count entries (TptInstanceDesc::material) */
 } TptSceneDesc;
+
+/* A node of the scene's SG light tree (R/light_tree/mod.rs:52-88), as
+ * tpt_scene_create builds it from the emissive triangles
+ * (Scene::build_light_tree, R/light_tree/builder.rs:58-121): depth-first
+ * pre-order, root 0; left/right/parent TPT_INVALID_INDEX where absent;
+ * instance/triangle name the leaf's triangle; area is the leaf triangle's
+ * area (0 for interior nodes). */
+typedef struct TptLightNode {
+  float flux;
+  float mu[3];
+  float sigma_s2;
+  float nu[3];
+  float lambda;
+  float radius;
+  float area;
+  uint32_t left, right, parent;
+  uint32_t instance, triangle;
+  uint32_t leaf;
+  uint32_t pad[3];
+} TptLightNode; /* 80 bytes */
 
 /* Pinhole camera after PinholeCamera::new (R/scene/camera.rs:21-42): the
  * renderer's normalised basis and tan(fov_y / 2), plus pixel_spread =
@@ -214,13 +257,26 @@ int32_t tpt_qbvh_export(const TptQbvh* qbvh, TptQbvhNode* nodes, uint32_t* primi
  * so a caller can cache or inspect the encoding. */
 int32_t tpt_qbvh_quantize(const TptQbvhNode* nodes, size_t n, void* out_q8_nodes);
 
+/* Host build of the SG light tree of a scene description (the same tree
+ * tpt_scene_create builds; no device needed): Scene::build_light_tree
+ * (R/scene/mod.rs:300-303, R/light_tree/builder.rs:58-121) over the emissive
+ * triangles in instance order.  At most `capacity` nodes are written;
+ * *count gets the total (0: no emissive triangle). */
+int32_t tpt_light_tree_build(const TptSceneDesc* desc, TptLightNode* out, uint32_t capacity, uint32_t* count);
+
 /* ---- scene ---------------------------------------------------------------- */
 /* Copies the scene to the device and derives the GPU layouts (leaf-ordered
- * triangle records, FP32 and 8-bit quantised node arrays). */
+ * triangle records, FP32 and 8-bit quantised node arrays), and builds the
+ * SG light tree over the emissive triangles as the reference's
+ * Scene::build_light_tree does (R/scene/mod.rs:300-303; area-light table in
+ * instance order, R/light_tree/builder.rs:58-121). */
 TptScene* tpt_scene_create(TptDevice* device, const TptSceneDesc* desc);
 void tpt_scene_release(TptScene* scene);
 /* Device bytes held by the scene (nodes, quantised nodes, triangles, instances). */
 size_t tpt_scene_device_bytes(const TptScene* scene);
+/* Copies the scene's light tree out (at most `capacity` nodes; *count gets
+ * the total).  Replaces reading Scene::light_tree (R/light_tree/mod.rs:98-121). */
+int32_t tpt_scene_light_tree(const TptScene* scene, TptLightNode* out, uint32_t capacity, uint32_t* count);
 
 /* ---- queries -------------------------------------------------------------- */
 /* Closest hit for n rays: Scene::closest_hit (R/scene/mod.rs:340-395) with the

@@ -82,3 +82,62 @@ def test_unsupported_material_is_rejected(device):
     with pytest.raises(tpt._lib.TptError, match="specular_weight"):
         ds.render(cam, 8, 8, 1, 8)
     ds.close()
+
+
+# ---- next-event estimation through the light tree (NormalizedLambert walls) ----
+@pytest.fixture(scope="module")
+def cornell_nee(device):
+    cs = scenes.cornell_lambert()
+    ds = cs.scene.upload(device)
+    yield cs, ds
+    ds.close()
+
+
+def test_light_tree_on_device_matches_host_build(cornell_nee):
+    cs, ds = cornell_nee
+    from tpt.scene import build_light_tree
+    a, b = ds.light_tree(), build_light_tree(cs.scene)
+    assert len(a) == 3 and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def test_nee_render_matches_oracle(cornell_nee):
+    """Same estimator and sample sequence; the light-tree importances go
+    through exp / expm1 / exp2 (device maths library vs glibc), so values
+    agree to float rounding, not bit for bit: per-channel relative error of
+    the image below 1e-3 on average and the image means equal to 1e-4."""
+    cs, ds = cornell_nee
+    w = h = 32
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    gpu = ds.render(cam, w, h, 8, 8)
+    cpu = oracle.OracleRenderer(cs.scene, cs.materials).render(cam, w, h, 8, 8)
+    assert np.all(np.isfinite(gpu))
+    rel = np.abs(gpu - cpu) / np.maximum(np.abs(cpu), 1e-3)
+    print("mean rel diff", rel.mean(), "max", rel.max(), "bit-identical", np.mean(gpu == cpu))
+    assert rel.mean() < 1e-3
+    assert abs(gpu.mean() - cpu.mean()) <= 1e-4 * cpu.mean()
+
+
+def test_nee_render_relmse_no_worse_than_cpu(cornell_nee):
+    cs, ds = cornell_nee
+    w = h = 32
+    spp = 64
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    ref = ds.render(cam, w, h, 16384, 8)
+    gpu = ds.render(cam, w, h, spp, 8)
+    cpu = oracle.OracleRenderer(cs.scene, cs.materials).render(cam, w, h, spp, 8)
+    e_gpu, e_cpu = relmse(gpu, ref), relmse(cpu, ref)
+    print(f"NEE relMSE at {spp} spp: gpu {e_gpu:.5f} cpu {e_cpu:.5f}")
+    assert e_gpu <= 1.1 * e_cpu + 1e-6
+
+
+def test_nee_lowers_variance(cornell_nee, cornell):
+    """Light-tree NEE with MIS beats BSDF sampling alone on the same
+    geometry: the Lambert scene's relMSE at 16 spp is below the OpenPBR
+    (BSDF-sampling-only) scene's, each against its own 16384-spp image."""
+    errs = []
+    for cs, ds in (cornell_nee, cornell):
+        cam = camera.pinhole(**cs.camera, width=32, height=32)
+        ref = ds.render(cam, 32, 32, 16384, 8)
+        errs.append(relmse(ds.render(cam, 32, 32, 16, 8), ref))
+    print("relMSE at 16 spp: NEE", errs[0], "BSDF only", errs[1])
+    assert errs[0] < errs[1]

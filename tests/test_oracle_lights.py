@@ -291,3 +291,27 @@ def test_direct_illumination_matches_quadrature():
     e = 10.0 * (1.0 / r2 ** 2).sum() / (g * g)  # cos = cos' = 1 / r, dA = 1/g^2
     expected = 0.8 / math.pi * e
     assert np.allclose(est, expected, rtol=0.015), (est, expected)
+
+
+# ---------------------------------------------------------------- product builder
+@pytest.mark.parametrize("which", ["field", "cornell", "cornell_lambert", "two"])
+def test_product_light_tree_matches_oracle(which):
+    """libtpt's host light-tree build (tpt_light_tree_build, the tree
+    tpt_scene_create uploads) equals the oracle's bit for bit."""
+    from tpt.scene import build_light_tree
+    if which == "field":
+        s, r = emissive_field(n=300, seed=9)
+    elif which == "two":
+        s, r = renderer([unit_mesh(0.0), unit_mesh(5.0)], [(0, 0, np.eye(4)), (1, 0, np.eye(4))],
+                        [Emissive((1.0, 1.0, 1.0), 1.0)])
+    else:
+        cs = scenes.cornell() if which == "cornell" else scenes.cornell_lambert()
+        s, r = cs.scene, oracle.OracleRenderer(cs.scene, cs.scene.materials)
+    got = build_light_tree(s)
+    ref = r.light_tree()
+    assert len(got) == len(ref["flux"]) > 0
+    for name in ["flux", "mu", "sigma_s2", "nu", "lambda", "radius", "area"]:
+        assert np.array_equal(got[name].view(np.uint32), np.asarray(ref[name]).view(np.uint32)), name
+    for name in ["left", "right", "parent", "instance", "triangle"]:
+        assert np.array_equal(got[name], ref[name]), name
+    assert np.array_equal(got["leaf"].astype(bool), ref["leaf"])

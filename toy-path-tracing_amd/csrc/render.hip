@@ -1,13 +1,16 @@
 // Plain FP32 megakernel path tracer: the render loop of R/main.rs:98-126 with
 // the unidirectional MIS integrator of R/integrator/mis.rs:19-139, over the
-// OpenPBR material subset tpt_render accepts (include/tpt.h).  One thread per
-// pixel runs its samples in order, so the reference's running mean
-// `color += (s - color) / n` is reproduced exactly.
+// materials tpt_render accepts (include/tpt.h).  One thread per pixel runs its
+// samples in order, so the reference's running mean `color += (s - color) / n`
+// is reproduced exactly.
 //
 // Per-sample work: PathSampler (bit-exact ZSobol), pinhole ray, two-level
 // closest hit (the trace kernels' traversal), shading vertex
-// (R/scene/mod.rs:471-544), OpenPBR sample (R/material/open_pbr.rs:489-548 ->
-// R/bsdf/open_pbr.rs:295-331), Russian roulette, emission with MIS.
+// (R/scene/mod.rs:471-544), next-event estimation through the SG light tree
+// with a shadow ray and balance-heuristic MIS (R/integrator/mis.rs:141-210),
+// material sample (OpenPBR R/material/open_pbr.rs:489-548; NormalizedLambert;
+// Mirror), Russian roulette, emission found by BSDF sampling with its MIS
+// weight (R/integrator/mis.rs:212-234).
 //
 // OpenPBR subset: specular weight 0 makes the modulated specular eta exactly
 // 1, so f_avg(eta) = 0 (R/bsdf/open_pbr.rs:2127-2135) and every term that
@@ -15,7 +18,9 @@
 // multiplied by zero; metal, coat, fuzz, transmission, thin film and thin
 // walls are zero.  Light NEE under the reference policy gives no sample for
 // OpenPBR shading points (no light-tree query, no environment), so emission
-// found by BSDF sampling has MIS weight 1 (R/integrator/mis.rs:212-235).
+// found by BSDF sampling from them has MIS weight 1; NormalizedLambert
+// surfaces build a diffuse light-tree query (R/material/normalized_lambert.rs:
+// 201-218), so for them NEE runs.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -437,7 +442,10 @@ struct ShadeView {
   const float* nrm;
   const float* uv;
   const uint32_t* idx;
-  const TptOpenPbr* mats;
+  const TptMaterial* mats;
+  const GpuLightNode* lights;  // light tree, root 0 (light_count 0: no tree)
+  const uint32_t* tri_leaf;    // GpuShade::light_base + triangle -> leaf node
+  uint32_t light_count;
 };
 
 struct Vtx {
@@ -445,6 +453,7 @@ struct Vtx {
   Frame frame;
   bool front;
   uint32_t material;
+  uint32_t inst, tri;
 };
 
 __device__ __forceinline__ F3 load3(const float* a, uint32_t i) { return f3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
@@ -498,12 +507,21 @@ __device__ __forceinline__ Vtx shading_vertex(const ShadeView& sv, uint32_t inst
   v.frame = frame_from_normal_tangent(v.ns, dpdu);
   v.wo = normalize_or_zero(nd);
   v.material = si.material;
+  v.inst = inst;
+  v.tri = tri;
   return v;
+}
+
+// Scene::triangle_positions (R/scene/mod.rs:432-438) -> world positions
+__device__ __forceinline__ void triangle_world(const ShadeView& sv, uint32_t inst, uint32_t tri, F3* p) {
+  const GpuShade& si = sv.inst[inst];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) p[k] = xform_point(si.l2w, load3(sv.pos, si.vbase + sv.idx[si.ibase + 3 * tri + k]));
 }
 
 // OpenPbrMaterial::le (R/material/open_pbr.rs:351-375); the exit tint is 1
 // without coat and fuzz (0 at a grazing exit).
-__device__ __forceinline__ bool material_le(const TptOpenPbr& m, const Vtx& v, F3* le) {
+__device__ __forceinline__ bool pbr_le(const TptOpenPbr& m, const Vtx& v, F3* le) {
   if (m.emission_luminance <= 0.0f) return false;
   if (!m.thin_walled && !v.front) return false;
   const F3 wl = normalize_or_zero(to_frame(v.frame, v.wo));
@@ -516,8 +534,8 @@ __device__ __forceinline__ bool material_le(const TptOpenPbr& m, const Vtx& v, F
 }
 
 // OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548), radiance mode.
-__device__ __forceinline__ bool material_sample(const TptOpenPbr& m, const Vtx& v, float u_lobe, float ux, float uy,
-                                                Sample* s, F3* wi_world) {
+__device__ __forceinline__ bool pbr_sample(const TptOpenPbr& m, const Vtx& v, float u_lobe, float ux, float uy,
+                                           Sample* s, F3* wi_world) {
   if (dot(v.wo, v.ng) <= 0.0f) return false;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
@@ -527,6 +545,213 @@ __device__ __forceinline__ bool material_sample(const TptOpenPbr& m, const Vtx& 
   if (dot(wi, v.ng) <= 1.0e-6f) return false;  // GEOMETRIC_NORMAL_COS_EPSILON, reflection lobes
   *wi_world = wi;
   return true;
+}
+
+// ---- Material dispatch (R/material/mod.rs:251-412) ---------------------------------
+__device__ __forceinline__ F3 rgb3(const float* c) { return f3(c[0], c[1], c[2]); }
+__device__ __forceinline__ bool mat_may_emit(const TptMaterial& m) {
+  if (m.kind == TPT_MATERIAL_EMISSIVE) return true;
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return m.u.open_pbr.emission_luminance > 0.0f;
+  return false;
+}
+__device__ __forceinline__ bool mat_le(const TptMaterial& m, const Vtx& v, F3* le) {
+  if (m.kind == TPT_MATERIAL_EMISSIVE) {  // EmissiveMaterial::le: color * strength * texture factor 1
+    *le = hadamard(muls(rgb3(m.u.emissive.color), m.u.emissive.strength), f3(1.0f, 1.0f, 1.0f));
+    return true;
+  }
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_le(m.u.open_pbr, v, le);
+  return false;
+}
+// NormalizedLambertMaterial::sample (R/material/normalized_lambert.rs:120-158)
+// and MirrorMaterial::sample (R/material/mirror.rs:112-143), radiance mode.
+__device__ __forceinline__ bool mat_sample(const TptMaterial& m, const Vtx& v, float u_lobe, float ux, float uy,
+                                           Sample* s, F3* wi_world) {
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_sample(m.u.open_pbr, v, u_lobe, ux, uy, s, wi_world);
+  if (m.kind == TPT_MATERIAL_EMISSIVE) return false;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  if (wol.z <= 0.0f) return false;
+  F3 wl;
+  if (m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {  // R/bsdf/normalized_lambert.rs:35-60
+    wl = cosine_hemisphere(ux, uy);
+    const float pdf = fmaxf(wl.z, 0.0f) / kPi;
+    if (pdf <= 0.0f) return false;
+    s->weight = rgb3(m.u.lambert.rho);
+    s->pdf = pdf;
+    s->delta = false;
+  } else {  // R/bsdf/mirror.rs:23-39
+    wl = normalize_or_zero(f3(-wol.x, -wol.y, wol.z));
+    s->weight = rgb3(m.u.mirror.color);
+    s->pdf = 1.0f;
+    s->delta = true;
+  }
+  s->wi = wl;
+  const F3 wi = to_world(v.frame, wl);
+  if (dot(wi, v.ng) <= 1.0e-6f) return false;
+  *wi_world = wi;
+  return true;
+}
+// NormalizedLambertMaterial::eval / pdf (R/material/normalized_lambert.rs:160-193)
+// with modified_bsdf_eval (R/material/mod.rs:33-53); the other materials
+// never receive a light sample (no light-tree query) or evaluate to zero.
+__device__ __forceinline__ F3 lambert_eval(const TptMaterial& m, const Vtx& v, F3 wi, float* pdf) {
+  const F3 zero = f3(0.0f, 0.0f, 0.0f);
+  *pdf = 0.0f;
+  if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return zero;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  const F3 wil = normalize_or_zero(to_frame(v.frame, wi));
+  if (wol.z <= 0.0f || wil.z <= 0.0f) return zero;
+  const F3 f = divs(rgb3(m.u.lambert.rho), kPi);
+  if (len2(f) == 0.0f) return zero;
+  const float cg = fabsf(dot(wi, v.ng));
+  if (cg <= 1.0e-6f) return zero;
+  *pdf = fmaxf(wil.z, 0.0f) / kPi;
+  return muls(f, fabsf(dot(wi, v.ns)) / cg);
+}
+
+// ---- SG light tree (R/math/sg.rs, R/light_tree/{lobe,traversal}.rs) ---------------
+constexpr float kSgMax = 2199023255552.0f;  // 2^41
+constexpr float kFltMinPos = 1.17549435e-38f;
+constexpr float kFltEpsilon = 1.1920929e-7f;
+// erf fit of R/math/sg.rs:74-107
+__device__ __forceinline__ float erf_fit(float x) {
+  const float ax = fabsf(x);
+  if (ax >= 4.0f) return copysignf(1.0f, x);
+  if (ax > 1.0f) {
+    const float p = (((((((2.5890768e-5f * ax + -5.648746e-4f) * ax + 5.6679556e-3f) * ax + -3.5175983e-2f) * ax +
+                        1.5432939e-1f) * ax + 9.156747e-1f) * ax + 1.6284595f) * ax);
+    return copysignf(1.0f - exp2f(-p), x);
+  }
+  const float x2 = x * x;
+  return (((((-5.5885315e-4f * x2 + 4.9073556e-3f) * x2 + -2.6703065e-2f) * x2 + 1.1279922e-1f) * x2 + -3.76123e-1f) *
+              x2 + 1.12837917f) * x;
+}
+struct SgLobe {
+  F3 axis;
+  float sharp, log_amp;
+};
+__device__ __forceinline__ SgLobe sg_product(F3 a1, float s1, F3 a2, float s2) {  // R/math/sg.rs:137-156
+  const F3 ax = add(muls(a1, s1), muls(a2, s2));
+  const float sharp = sqrtf(dot(ax, ax));
+  const F3 d = sub(a1, a2);
+  const float denom = fmaxf(sharp + s1 + s2, kFltMinPos);
+  SgLobe l;
+  l.log_amp = -s1 * s2 * dot(d, d) / denom;
+  l.axis = sharp > kFltMinPos ? divs(ax, sharp) : f3(0.0f, 0.0f, 1.0f);
+  l.sharp = sharp;
+  return l;
+}
+// Clamped-cosine x SG product integral / pi (R/math/sg.rs:236-306)
+__device__ __forceinline__ float cc_product_over_pi(float z, float s) {
+  const float t = s * sqrtf(0.5f * ((s + 2.736083f) * s + 17.021297f) / (((s + 4.0100827f) * s + 15.219156f) * s + 76.0879f));
+  const float tz = t * z;
+  const float lerp = clampf(0.5f * (z * (1.0f - erf_fit(-tz)) + (1.0f - erf_fit(t))) -
+                                0.5f * 0.5641896f * expf(-tz * tz) * expm1f(t * t * (z * z - 1.0f)) / fmaxf(t, kFltMinPos),
+                            0.5f * kFltEpsilon, 1.0f);
+  const float e = expf(-s);
+  float lower, upper;
+  if (s <= 0.5f) {
+    upper = (((((((-1.0f / 362880.0f) * s + 1.0f / 40320.0f) * s - 1.0f / 5040.0f) * s + 1.0f / 720.0f) * s -
+               1.0f / 120.0f) * s + 1.0f / 24.0f) * s - 1.0f / 6.0f) * s + 0.5f;
+    lower = e * (((((((((1.0f / 403200.0f) * s - 1.0f / 45360.0f) * s + 1.0f / 5760.0f) * s - 1.0f / 840.0f) * s +
+                     1.0f / 144.0f) * s - 1.0f / 30.0f) * s + 1.0f / 8.0f) * s - 1.0f / 3.0f) * s + 0.5f);
+  } else {
+    upper = (expm1f(-s) + s) / (s * s);
+    lower = e * (-expm1f(-s) - s * e) / (s * s);
+  }
+  return 2.0f * (lower + (upper - lower) * lerp);
+}
+// sg_light_for_node + diffuse_importance (R/light_tree/lobe.rs:233-287) of a
+// diffuse query (p, n, rho)
+__device__ __forceinline__ float node_importance(const GpuLightNode& nd, F3 p, F3 n, float rho) {
+  if (nd.flux <= 0.0f) return 0.0f;
+  const F3 d = sub(f3(nd.mu[0], nd.mu[1], nd.mu[2]), p);
+  const float r2 = dot(d, d);
+  if (r2 <= 0.0f) return 0.0f;
+  const float inv_r = 1.0f / sqrtf(r2);
+  const float cos_below = fmaxf(-(dot(n, d)) * inv_r, 0.0f);
+  float sigma2 = nd.sigma_s2 * (1.0f - cos_below) + nd.bound_var * cos_below;
+  const float min_sigma2 = r2 / kSgMax;
+  if (sigma2 < min_sigma2) sigma2 = min_sigma2;
+  const F3 axis = muls(d, inv_r);
+  const float sharp = clampf(r2 / sigma2, 0.0f, kSgMax);
+  const SgLobe lobe = sg_product(f3(-nd.nu[0], -nd.nu[1], -nd.nu[2]), nd.lambda, axis, sharp);
+  const float w_amp = nd.flux / (2.0f * kPi * sigma2 * nd.sg_norm);
+  const float w = w_amp * expf(lobe.log_amp);
+  if (!__builtin_isfinite(w) || w <= 0.0f) return 0.0f;
+  if (rho <= 0.0f) return 0.0f;
+  const float c = clampf(dot(lobe.axis, n), -1.0f, 1.0f);
+  return rho * w * (cc_product_over_pi(c, lobe.sharp) * kPi);
+}
+// p(left) of an interior node, with the flux fallback when both children
+// look dark (R/light_tree/traversal.rs:54-80); false if no child has flux.
+__device__ __forceinline__ bool p_left(const ShadeView& sv, const GpuLightNode& nd, F3 p, F3 n, float rho,
+                                       float* pl, bool* by_importance) {
+  const GpuLightNode& L = sv.lights[nd.left];
+  const GpuLightNode& R = sv.lights[nd.right];
+  const float il = node_importance(L, p, n, rho), ir = node_importance(R, p, n, rho);
+  const float total = il + ir;
+  if (total <= 0.0f) {
+    const float lf = fmaxf(L.flux, 0.0f), rf = fmaxf(R.flux, 0.0f);
+    const float tf = lf + rf;
+    if (tf <= 0.0f) return false;
+    *pl = lf / tf;
+    *by_importance = false;
+    return true;
+  }
+  *pl = il / total;
+  *by_importance = true;
+  return true;
+}
+// sample_light_tree (R/light_tree/traversal.rs:26-92)
+__device__ __forceinline__ bool sample_tree(const ShadeView& sv, F3 p, F3 n, float rho, float u, uint32_t* leaf,
+                                            float* pmf_out) {
+  if (sv.light_count == 0) return false;
+  u = clampf(u, 0.0f, 1.0f - kFltEpsilon);
+  uint32_t ni = 0;
+  float pmf = 1.0f;
+  for (;;) {
+    const GpuLightNode& nd = sv.lights[ni];
+    if (nd.left == TPT_INVALID_INDEX) {
+      *leaf = ni;
+      *pmf_out = pmf;
+      return true;
+    }
+    float pl;
+    bool imp;
+    if (!p_left(sv, nd, p, n, rho, &pl, &imp)) return false;
+    if (u < pl) {
+      u /= fmaxf(pl, kFltMinPos);
+      pmf *= pl;
+      ni = nd.left;
+    } else {
+      u = (u - pl) / fmaxf(1.0f - pl, kFltMinPos);
+      pmf *= 1.0f - pl;
+      ni = nd.right;
+    }
+    if (imp) u = clampf(u, 0.0f, 1.0f - kFltEpsilon);
+  }
+}
+// pdf_for_leaf (R/light_tree/traversal.rs:97-140)
+__device__ __forceinline__ float leaf_pmf(const ShadeView& sv, F3 p, F3 n, float rho, uint32_t leaf) {
+  if (sv.light_count == 0 || leaf == TPT_INVALID_INDEX) return 0.0f;
+  float pmf = 1.0f;
+  uint32_t child = leaf, parent = sv.lights[leaf].parent;
+  while (parent != TPT_INVALID_INDEX) {
+    const GpuLightNode& pn = sv.lights[parent];
+    float pl;
+    bool imp;
+    if (!p_left(sv, pn, p, n, rho, &pl, &imp)) return 0.0f;
+    pmf *= child == pn.left ? pl : 1.0f - pl;
+    child = parent;
+    parent = pn.parent;
+  }
+  return pmf;
+}
+
+__device__ __forceinline__ float balance_heuristic(float a, float b) {  // R/math/mod.rs:163-171
+  const float sum = a + b;
+  if (a <= 0.0f || sum <= 0.0f) return 0.0f;
+  return a / sum;
 }
 
 __device__ __forceinline__ F3 offset_origin(F3 p, F3 ng, F3 d) {  // R/integrator/mod.rs:76-85
@@ -553,6 +778,94 @@ __device__ __forceinline__ bool closest(const SceneView& sc, F3 o, F3 d, Lane& L
   *b1 = L.b1;
   *b2 = L.b2;
   return L.hit;
+}
+
+// direct_light_mis_contribution (R/integrator/mis.rs:141-210) for a diffuse
+// light-tree query: tree leaf, uniform point on its triangle
+// (sample_triangle_point, R/scene/mod.rs:616-633; sample_li_for_triangle,
+// R/light/area.rs:12-54), shadow ray with the light triangle as target
+// (unoccluded_ray, R/integrator/mod.rs:239-265), balance heuristic.  The
+// tree is the light sampler's only category (pmf 1).
+template <bool kTwoLevel>
+__device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView& sv, const TptMaterial& m,
+                                           const Vtx& vtx, float u_tree, float us0, float us1, Lane& L,
+                                           LaneStack& st) {
+  const F3 zero = f3(0.0f, 0.0f, 0.0f);
+  if (sv.light_count == 0 || m.kind != TPT_MATERIAL_NORMALIZED_LAMBERT) return zero;
+  const float rho = luminance(rgb3(m.u.lambert.rho));
+  if (rho <= 0.0f) return zero;
+  uint32_t leaf;
+  float lpmf;
+  if (!sample_tree(sv, vtx.p, vtx.ns, rho, u_tree, &leaf, &lpmf)) return zero;
+  const GpuLightNode& ln = sv.lights[leaf];
+  F3 p[3];
+  triangle_world(sv, ln.instance, ln.triangle, p);
+  const float su0 = sqrtf(clampf(us0, 0.0f, 1.0f));
+  const float u1 = clampf(us1, 0.0f, 1.0f);
+  const float b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
+  const F3 lp = interp(b0, b1, b2, p[0], p[1], p[2]);
+  const float area = 0.5f * sqrtf(len2(cross(sub(p[1], p[0]), sub(p[2], p[0]))));
+  const float pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
+  if (pdf_area <= 0.0f) return zero;
+  const F3 to_light = sub(lp, vtx.p);
+  const float d2 = len2(to_light);
+  if (d2 <= 0.0f) return zero;
+  const float dist = sqrtf(d2);
+  const F3 wi = divs(to_light, dist);
+  const TptMaterial& lm = sv.mats[sv.inst[ln.instance].material];
+  if (!mat_may_emit(lm)) return zero;
+  const Vtx lv = shading_vertex(sv, ln.instance, ln.triangle, b0, b1, b2, wi);
+  F3 le;
+  if (!mat_le(lm, lv, &le)) return zero;
+  const float cos_light = fmaxf(dot(lv.ng, neg(wi)), 0.0f);
+  if (cos_light <= 0.0f) return zero;
+  const float li_pdf = pdf_area * d2 / cos_light;
+  const float selection = 1.0f * lpmf;
+  if (li_pdf <= 0.0f || selection <= 0.0f) return zero;
+  float bsdf_pdf;
+  const F3 f = lambert_eval(m, vtx, wi, &bsdf_pdf);
+  if (len2(f) == 0.0f) return zero;
+  const float cos_surface = fabsf(dot(vtx.ng, wi));
+  if (cos_surface <= 0.0f) return zero;
+  uint32_t hi, ht;
+  float tt, hb0, hb1, hb2;
+  if (closest<kTwoLevel>(sc, offset_origin(vtx.p, vtx.ng, wi), wi, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2) &&
+      !(hi == ln.instance && ht == ln.triangle))
+    return zero;
+  const float pdf_total = selection * li_pdf;
+  if (pdf_total <= 0.0f) return zero;
+  const float mis = balance_heuristic(pdf_total, bsdf_pdf);
+  return muls(hadamard(le, f), cos_surface * mis / pdf_total);
+}
+
+// emitted_radiance_from_bsdf_sample_area (R/integrator/mis.rs:212-234) with
+// pdf_for_triangle_hit (R/light/mod.rs:407-429) and the area-light solid-angle
+// pdf (R/scene/mod.rs:587-611).
+__device__ __forceinline__ F3 emitted(const ShadeView& sv, F3 thr, float bsdf_pdf, bool delta, const TptMaterial& m,
+                                      const Vtx& vtx, const Vtx& lv, const TptMaterial& lm) {
+  F3 le;
+  if (!mat_le(lm, lv, &le)) return f3(0.0f, 0.0f, 0.0f);
+  if (delta) return hadamard(thr, le);
+  float light_pdf = 0.0f;
+  if (sv.light_count != 0 && m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {
+    const float rho = luminance(rgb3(m.u.lambert.rho));
+    const uint32_t base = sv.inst[lv.inst].light_base;
+    const uint32_t leaf = base == TPT_INVALID_INDEX ? TPT_INVALID_INDEX : sv.tri_leaf[base + lv.tri];
+    const float lpmf = rho > 0.0f ? leaf_pmf(sv, vtx.p, vtx.ns, rho, leaf) : 0.0f;
+    if (lpmf > 0.0f) {
+      const float area = sv.lights[leaf].area;
+      const F3 to_light = sub(lv.p, vtx.p);
+      const float d2 = len2(to_light);
+      if (area > 0.0f && d2 > 0.0f) {
+        const float pdf_area = 1.0f / area;
+        const float dist = sqrtf(d2);
+        const F3 wi = divs(to_light, dist);
+        const float cos_light = fmaxf(dot(lv.ng, neg(wi)), 0.0f);
+        if (cos_light > 0.0f) light_pdf = 1.0f * lpmf * (pdf_area * d2 / cos_light);
+      }
+    }
+  }
+  return muls(hadamard(thr, le), balance_heuristic(bsdf_pdf, light_pdf));
 }
 
 template <bool kTwoLevel>
@@ -591,14 +904,22 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView 
       overflow |= st.overflow;
       Vtx vtx = shading_vertex(sv, hi, ht, hb0, hb1, hb2, dir);
       F3 le;
-      if (material_le(sv.mats[vtx.material], vtx, &le)) radiance = add(radiance, le);
+      if (mat_le(sv.mats[vtx.material], vtx, &le)) radiance = add(radiance, le);
       for (uint32_t depth = 0; depth < max_depth; ++depth) {
         const uint32_t base = smp::kVertexBase + depth * smp::kVertexStride;
+        const TptMaterial& m = sv.mats[vtx.material];
+        if (m.kind != TPT_MATERIAL_EMISSIVE) {  // should_sample_direct_light: not a pure emitter
+          float us[2];
+          ps.s2(base + 3, us);
+          const F3 dl = direct_light<kTwoLevel>(sc, sv, m, vtx, ps.s1(base + 1), us[0], us[1], L, st);
+          overflow |= st.overflow;
+          radiance = add(radiance, hadamard(thr, dl));
+        }
         Sample s;
         F3 wi;
         float ud[2];
         ps.s2(base + 7, ud);
-        if (!material_sample(sv.mats[vtx.material], vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi)) break;
+        if (!mat_sample(m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi)) break;
         thr = hadamard(thr, s.weight);
         if (depth + 1 >= 4) {
           const float q = clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.05f, 1.0f);
@@ -609,15 +930,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView 
         if (!closest<kTwoLevel>(sc, no, wi, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) break;  // no environment
         overflow |= st.overflow;
         const Vtx nv = shading_vertex(sv, hi, ht, hb0, hb1, hb2, wi);
-        F3 e;
-        if (material_le(sv.mats[nv.material], nv, &e)) {
-          if (s.delta) {
-            radiance = add(radiance, hadamard(thr, e));
-          } else {
-            const float mw = (s.pdf <= 0.0f || s.pdf + 0.0f <= 0.0f) ? 0.0f : s.pdf / (s.pdf + 0.0f);
-            radiance = add(radiance, muls(hadamard(thr, e), mw));
-          }
-        }
+        radiance = add(radiance, emitted(sv, thr, s.pdf, s.delta, m, vtx, nv, sv.mats[nv.material]));
         vtx = nv;
       }
     } else {
@@ -640,8 +953,17 @@ using namespace tpt;
 
 namespace {
 
-// The OpenPBR parameter subset the path tracer implements exactly.
-bool supported(const TptOpenPbr& m, std::string* why) {
+// The materials the path tracer implements exactly: OpenPBR's specular-weight-0
+// subset, NormalizedLambert, Emissive and Mirror.
+bool supported(const TptMaterial& mat, std::string* why) {
+  if (mat.kind == TPT_MATERIAL_NORMALIZED_LAMBERT || mat.kind == TPT_MATERIAL_EMISSIVE ||
+      mat.kind == TPT_MATERIAL_MIRROR)
+    return true;
+  if (mat.kind != TPT_MATERIAL_OPEN_PBR) {
+    *why = "unknown material kind " + std::to_string(mat.kind);
+    return false;
+  }
+  const TptOpenPbr& m = mat.u.open_pbr;
   if (m.specular_weight != 0.0f) *why = "specular_weight must be 0";
   else if (m.base_metalness != 0.0f) *why = "base_metalness must be 0";
   else if (m.transmission_weight != 0.0f) *why = "transmission_weight must be 0";
@@ -667,12 +989,12 @@ extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb
       set_error("tpt_render: frame too large or samples_per_pixel == 0");
       return TPT_ERROR_BAD_ARGUMENT;
     }
-    std::vector<TptOpenPbr> mats(s->material_count);
+    std::vector<TptMaterial> mats(s->material_count);
     std::vector<GpuShade> shade(s->instance_count);
     TptDevice* d = s->dev;
     if (!hip_ok(hipSetDevice(d->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
     if (s->material_count &&
-        !hip_ok(hipMemcpy(mats.data(), s->mats, mats.size() * sizeof(TptOpenPbr), hipMemcpyDeviceToHost),
+        !hip_ok(hipMemcpy(mats.data(), s->mats, mats.size() * sizeof(TptMaterial), hipMemcpyDeviceToHost),
                 "hipMemcpy(materials)"))
       return TPT_ERROR_DEVICE;
     if (s->instance_count &&
@@ -698,7 +1020,7 @@ extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb
     cam.up = F3{c.image_up[0], c.image_up[1], c.image_up[2]};
     cam.tan_half = c.tan_half_fov_y;
     cam.exposure = c.exposure;
-    pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats};
+    pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats, s->lights, s->tri_leaf, s->light_count};
     const SceneView v = view_of(s);
     const size_t n = (size_t)w * h;
     const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;

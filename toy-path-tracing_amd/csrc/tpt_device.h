@@ -6,6 +6,9 @@
 #include <cstdint>
 #include <string>
 
+#include <vector>
+
+#include "light_tree.h"
 #include "tpt_internal.h"
 #include "traverse.h"
 
@@ -84,8 +87,13 @@ struct TptScene {
   float* vnrm = nullptr;
   float* vuv = nullptr;
   uint32_t* vidx = nullptr;
-  TptOpenPbr* mats = nullptr;
+  TptMaterial* mats = nullptr;
   uint32_t material_count = 0;
+  // SG light tree over the emissive triangles and the triangle -> leaf table
+  tpt::GpuLightNode* lights = nullptr;
+  uint32_t light_count = 0;
+  uint32_t* tri_leaf = nullptr;
+  std::vector<TptLightNode> light_nodes;  // host copy (tpt_scene_light_tree)
   size_t bytes = 0;
   // staging for host-pointer calls (grow-only)
   void* stage = nullptr;

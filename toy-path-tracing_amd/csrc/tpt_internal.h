@@ -82,7 +82,8 @@ struct alignas(16) GpuShade {
   uint32_t vbase;
   uint32_t ibase;
   uint32_t flags;  // bit 0: vertex normals present, bit 1: uvs present
-  uint32_t pad[7];
+  uint32_t light_base;  // offset of this instance's triangles in the triangle -> light-leaf table, or ~0
+  uint32_t pad[6];
 };
 static_assert(sizeof(GpuShade) == 128, "GpuShade must be 128 bytes");
 

@@ -261,6 +261,8 @@ void free_scene(TptScene* s) {
   (void)hipFree(s->vuv);
   (void)hipFree(s->vidx);
   (void)hipFree(s->mats);
+  (void)hipFree(s->lights);
+  (void)hipFree(s->tri_leaf);
   delete s;
 }
 
@@ -587,20 +589,53 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
       g.ibase = mesh_ibase[id.mesh];
       g.flags = mesh_flags[id.mesh];
     }
-    std::vector<TptOpenPbr> mats(desc->materials ? desc->materials : nullptr,
-                                 desc->materials ? desc->materials + desc->material_count : nullptr);
+    std::vector<TptMaterial> mats(desc->materials ? desc->materials : nullptr,
+                                  desc->materials ? desc->materials + desc->material_count : nullptr);
+    // SG light tree over the emissive triangles (Scene::build_light_tree)
+    std::vector<TptLightNode> lnodes;
+    std::vector<uint32_t> light_base, tri_leaf;
+    std::string lerr;
+    if (!build_light_tree(*desc, &lnodes, &light_base, &tri_leaf, &lerr)) {
+      set_error("tpt_scene_create: " + lerr);
+      return nullptr;
+    }
+    for (uint32_t i = 0; i < desc->instance_count; ++i) shade[i].light_base = light_base[i];
+    std::vector<GpuLightNode> glights(lnodes.size());
+    for (size_t k = 0; k < lnodes.size(); ++k) {
+      const TptLightNode& a = lnodes[k];
+      GpuLightNode& g = glights[k];
+      std::memset(&g, 0, sizeof(g));
+      for (int c = 0; c < 3; ++c) {
+        g.mu[c] = a.mu[c];
+        g.nu[c] = a.nu[c];
+      }
+      g.flux = a.flux;
+      g.lambda = a.lambda;
+      g.sigma_s2 = a.sigma_s2;
+      g.bound_var = 0.5f * a.radius * a.radius;
+      g.sg_norm = sg_norm_host(a.lambda);
+      g.area = a.area;
+      g.left = a.left;
+      g.right = a.right;
+      g.parent = a.parent;
+      g.instance = a.instance;
+      g.triangle = a.triangle;
+    }
     s = new TptScene();
     s->dev = dev;
     s->instance_count = desc->instance_count;
     s->tlas_node_count = desc->tlas_node_count;
     s->material_count = desc->materials ? desc->material_count : 0;
+    s->light_count = (uint32_t)lnodes.size();
+    s->light_nodes = std::move(lnodes);
     if (!upload(nodes, &s->nodes, &s->bytes) || !upload(qnodes, &s->qnodes, &s->bytes) ||
         !upload(tris, &s->tris, &s->bytes) || !upload(rank, &s->tri_rank, &s->bytes) ||
         !upload(inst, &s->inst, &s->bytes) || !upload(tlas, &s->tlas, &s->bytes) ||
         !upload(qtlas, &s->qtlas, &s->bytes) || !upload(tprim, &s->tlas_prim, &s->bytes) ||
         !upload(shade, &s->shade, &s->bytes) || !upload(vpos, &s->vpos, &s->bytes) ||
         !upload(vnrm, &s->vnrm, &s->bytes) || !upload(vuv, &s->vuv, &s->bytes) ||
-        !upload(vidx, &s->vidx, &s->bytes) || !upload(mats, &s->mats, &s->bytes)) {
+        !upload(vidx, &s->vidx, &s->bytes) || !upload(mats, &s->mats, &s->bytes) ||
+        !upload(glights, &s->lights, &s->bytes) || !upload(tri_leaf, &s->tri_leaf, &s->bytes)) {
       free_scene(s);
       return nullptr;
     }
@@ -620,6 +655,17 @@ void tpt_scene_release(TptScene* s) {
 }
 
 size_t tpt_scene_device_bytes(const TptScene* s) { return s ? s->bytes : 0; }
+
+int32_t tpt_scene_light_tree(const TptScene* s, TptLightNode* out, uint32_t capacity, uint32_t* count) {
+  if (!s || !count || (capacity && !out)) {
+    set_error("tpt_scene_light_tree: null argument");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  *count = (uint32_t)s->light_nodes.size();
+  const uint32_t n = std::min(capacity, *count);
+  if (n) std::memcpy(out, s->light_nodes.data(), n * sizeof(TptLightNode));
+  return TPT_OK;
+}
 
 }  // extern "C"
 

@@ -72,6 +72,14 @@ class InstanceDesc(ctypes.Structure):
     ]
 
 
+# TptLightNode (80 bytes)
+LIGHT_NODE_DTYPE = np.dtype([("flux", "<f4"), ("mu", "<f4", 3), ("sigma_s2", "<f4"), ("nu", "<f4", 3),
+                             ("lambda", "<f4"), ("radius", "<f4"), ("area", "<f4"), ("left", "<u4"),
+                             ("right", "<u4"), ("parent", "<u4"), ("instance", "<u4"), ("triangle", "<u4"),
+                             ("leaf", "<u4"), ("pad", "<u4", 3)])
+assert LIGHT_NODE_DTYPE.itemsize == 80
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [
         ("meshes", ctypes.c_void_p),
@@ -139,6 +147,8 @@ _SIGNATURES = [
     ("tpt_scene_create", ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p]),
     ("tpt_scene_release", None, [ctypes.c_void_p]),
     ("tpt_scene_device_bytes", ctypes.c_size_t, [ctypes.c_void_p]),
+    ("tpt_scene_light_tree", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    ("tpt_light_tree_build", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     ("tpt_trace_closest", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_trace_any", ctypes.c_int32,

@@ -226,54 +226,76 @@ class Device:
             pass
 
 
+def scene_desc(scene: Scene):
+    """TptSceneDesc of a built scene -> (desc, keep): `keep` holds the arrays
+    the descriptor points into and must outlive every call that reads it."""
+    keep = []
+    meshes = (_lib.MeshDesc * max(1, len(scene.meshes)))()
+    for i, m in enumerate(scene.meshes):
+        q = m.qbvh
+        d = meshes[i]
+        d.positions = ptr(m.positions)
+        d.vertex_count = len(m.positions)
+        d.triangle_count = len(m.indices)
+        d.indices = ptr(m.indices)
+        if q is not None:
+            d.nodes = ptr(q.nodes) if len(q.nodes) else None
+            d.node_count = len(q.nodes)
+            d.primitive_indices = ptr(q.primitive_indices)
+            keep.append(q)
+        else:
+            d.node_count = 0
+            d.primitive_indices = None
+        if m.normals is not None:
+            m.normals = np.ascontiguousarray(m.normals, dtype=np.float32).reshape(-1, 3)
+            d.normals = ptr(m.normals)
+        if m.uvs is not None:
+            m.uvs = np.ascontiguousarray(m.uvs, dtype=np.float32).reshape(-1, 2)
+            d.uvs = ptr(m.uvs)
+    insts = (_lib.InstanceDesc * max(1, len(scene.instances)))()
+    for i, inst in enumerate(scene.instances):
+        insts[i].mesh = inst.mesh_index
+        insts[i].material = inst.material_index
+        insts[i].local_to_world[:] = glam_cols(inst.local_to_world).tolist()
+        insts[i].world_to_local[:] = glam_cols(inst.world_to_local).tolist()
+    desc = _lib.SceneDesc()
+    desc.meshes = ctypes.addressof(meshes)
+    desc.mesh_count = len(scene.meshes)
+    desc.instance_count = len(scene.instances)
+    desc.instances = ctypes.addressof(insts)
+    if scene.qbvh is not None:
+        desc.tlas_nodes = ptr(scene.qbvh.nodes) if len(scene.qbvh.nodes) else None
+        desc.tlas_node_count = len(scene.qbvh.nodes)
+        desc.tlas_primitive_indices = ptr(scene.qbvh.primitive_indices)
+    if scene.materials:
+        from .material import records
+        mats = records(scene.materials)
+        keep.append(mats)
+        desc.materials = ctypes.addressof(mats)
+        desc.material_count = len(scene.materials)
+    keep += [meshes, insts]
+    return desc, keep
+
+
+def build_light_tree(scene: Scene) -> np.ndarray:
+    """Scene::build_light_tree on the host (tpt_light_tree_build): the SG light
+    tree over the emissive triangles, LIGHT_NODE_DTYPE records."""
+    desc, keep = scene_desc(scene)
+    count = ctypes.c_uint32(0)
+    check(lib().tpt_light_tree_build(ctypes.byref(desc), None, 0, ctypes.byref(count)))
+    out = np.zeros(count.value, dtype=_lib.LIGHT_NODE_DTYPE)
+    if len(out):
+        check(lib().tpt_light_tree_build(ctypes.byref(desc), ptr(out), count.value, ctypes.byref(count)))
+    del keep
+    return out
+
+
 class DeviceScene:
     """A scene resident in device memory (tpt_scene_create)."""
 
     def __init__(self, device: Device, scene: Scene):
         self.device = device
-        keep = []  # arrays referenced by the descriptor during the call
-        meshes = (_lib.MeshDesc * max(1, len(scene.meshes)))()
-        for i, m in enumerate(scene.meshes):
-            q = m.qbvh
-            d = meshes[i]
-            d.positions = ptr(m.positions)
-            d.vertex_count = len(m.positions)
-            d.triangle_count = len(m.indices)
-            d.indices = ptr(m.indices)
-            if q is not None:
-                d.nodes = ptr(q.nodes) if len(q.nodes) else None
-                d.node_count = len(q.nodes)
-                d.primitive_indices = ptr(q.primitive_indices)
-                keep.append(q)
-            else:
-                d.node_count = 0
-                d.primitive_indices = None
-            if m.normals is not None:
-                m.normals = np.ascontiguousarray(m.normals, dtype=np.float32).reshape(-1, 3)
-                d.normals = ptr(m.normals)
-            if m.uvs is not None:
-                m.uvs = np.ascontiguousarray(m.uvs, dtype=np.float32).reshape(-1, 2)
-                d.uvs = ptr(m.uvs)
-        insts = (_lib.InstanceDesc * max(1, len(scene.instances)))()
-        for i, inst in enumerate(scene.instances):
-            insts[i].mesh = inst.mesh_index
-            insts[i].material = inst.material_index
-            insts[i].local_to_world[:] = glam_cols(inst.local_to_world).tolist()
-            insts[i].world_to_local[:] = glam_cols(inst.world_to_local).tolist()
-        desc = _lib.SceneDesc()
-        desc.meshes = ctypes.addressof(meshes)
-        desc.mesh_count = len(scene.meshes)
-        desc.instance_count = len(scene.instances)
-        desc.instances = ctypes.addressof(insts)
-        if scene.qbvh is not None:
-            desc.tlas_nodes = ptr(scene.qbvh.nodes) if len(scene.qbvh.nodes) else None
-            desc.tlas_node_count = len(scene.qbvh.nodes)
-            desc.tlas_primitive_indices = ptr(scene.qbvh.primitive_indices)
-        if scene.materials:
-            from .material import records
-            mats = records(scene.materials)
-            desc.materials = ctypes.addressof(mats)
-            desc.material_count = len(scene.materials)
+        desc, keep = scene_desc(scene)
         self.handle = lib().tpt_scene_create(device.handle, ctypes.byref(desc))
         if not self.handle:
             raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
@@ -282,6 +304,15 @@ class DeviceScene:
     @property
     def device_bytes(self) -> int:
         return lib().tpt_scene_device_bytes(self.handle)
+
+    def light_tree(self) -> np.ndarray:
+        """The scene's SG light tree (tpt_scene_light_tree) as LIGHT_NODE_DTYPE."""
+        count = ctypes.c_uint32(0)
+        check(lib().tpt_scene_light_tree(self.handle, None, 0, ctypes.byref(count)))
+        out = np.zeros(count.value, dtype=_lib.LIGHT_NODE_DTYPE)
+        check(lib().tpt_scene_light_tree(self.handle, ptr(out) if len(out) else None, count.value,
+                                         ctypes.byref(count)))
+        return out
 
     def close(self) -> None:
         if self.handle:
