@@ -217,7 +217,10 @@ int32_t tpt_device_synchronize(TptDevice* device);
  *   "refill_below"  1..64  a persistent wave fetches new rays for its idle
  *                          lanes when fewer than this many lanes are busy
  *   "blocks_per_cu" 0..64  cap on resident 256-thread blocks per CU (0: the
- *                          occupancy limit) */
+ *                          occupancy limit)
+ * and of the wavefront path tracer (tpt_render_batch):
+ *   "wf_paths"  65536..2^26  paths in flight per pass (default 2^21)
+ *   "wf_graph"  0 or 1       capture one pass as a HIP graph and replay it */
 int32_t tpt_device_set_param(TptDevice* device, const char* name, int64_t value);
 
 /* ---- sampler and camera --------------------------------------------------- */
@@ -310,6 +313,26 @@ typedef struct TptRenderDesc {
  * balance heuristic); an OpenPBR shading point gets no light-tree sample
  * (R/light_tree/importance.rs:26-36, R/material/open_pbr.rs:592-606). */
 int32_t tpt_render(TptScene* scene, const TptRenderDesc* desc, float* rgb, uint32_t flags);
+
+/* One frame (view) of a batched render: its own camera, resolution, samples
+ * per pixel and depth, and its output (width * height * 3 floats; host or
+ * device memory per the call's flags). */
+typedef struct TptFrameDesc {
+  TptRenderDesc render;
+  float* rgb;
+} TptFrameDesc;
+
+/* The same estimator and output as tpt_render for every frame of a batch,
+ * run as a wavefront path tracer: all frames' paths share one set of
+ * launches (per bounce: a ray-queue trace, a shade kernel with light-tree
+ * NEE appending to shadow and next-ray queues, a shadow-queue trace), so
+ * frames of uneven resolution fill the GPU together.  Each frame's image is
+ * the one tpt_render gives for it, bit for bit, with FP32 nodes
+ * (TPT_FLAG_QUANTIZED selects 8-bit nodes for every trace).  Tuned by
+ * tpt_device_set_param "wf_paths" (paths in flight) and "wf_graph" (capture
+ * a pass as a HIP graph).  No reference counterpart beyond the pixel loop of
+ * R/main.rs:98-126, which it runs for many frames at once. */
+int32_t tpt_render_batch(TptScene* scene, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags);
 
 /* The traversal of tpt_trace_closest (query 0) or tpt_trace_any (query 1)
  * with per-ray work counters instead of results: counts[3*i+0] nodes

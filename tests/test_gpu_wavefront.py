@@ -1,0 +1,80 @@
+"""The wavefront path tracer (tpt_render_batch) against the megakernel
+(tpt_render): the same estimator, sample sequence and arithmetic per path, so
+every frame of a batch is the megakernel's image bit for bit with FP32
+nodes, whatever the batch composition, paths in flight or graph setting."""
+import numpy as np
+import pytest
+
+import tpt
+from tpt import camera, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=["openpbr", "lambert_nee"])
+def scene(request, device):
+    cs = scenes.cornell() if request.param == "openpbr" else scenes.cornell_lambert()
+    ds = cs.scene.upload(device)
+    yield cs, ds
+    ds.close()
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+
+
+def test_single_frame_matches_megakernel(scene):
+    cs, ds = scene
+    w, h, spp, depth = 40, 30, 16, 8
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    mega = ds.render(cam, w, h, spp, depth)
+    (wave,) = ds.render_batch([(cam, w, h, spp, depth)])
+    print("identical pixels", np.mean(np.all(mega == wave, axis=2)))
+    assert bits_equal(mega, wave)
+
+
+def test_uneven_batch_matches_per_frame_renders(scene):
+    cs, ds = scene
+    frames = []
+    for (w, h, spp, depth) in [(48, 32, 8, 8), (17, 29, 5, 6), (64, 64, 3, 8), (9, 7, 33, 4)]:
+        frames.append((camera.pinhole(**cs.camera, width=w, height=h), w, h, spp, depth))
+    outs = ds.render_batch(frames)
+    for f, img in zip(frames, outs):
+        assert bits_equal(img, ds.render(*f))
+
+
+@pytest.mark.parametrize("paths,graph", [(65536, 1), (65536, 0), (1 << 21, 0)])
+def test_paths_in_flight_and_graph_do_not_change_the_image(device, scene, paths, graph):
+    cs, ds = scene
+    w = h = 96
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    ref = ds.render(cam, w, h, 12, 8)
+    device.set_param("wf_paths", paths)
+    device.set_param("wf_graph", graph)
+    try:
+        (img,) = ds.render_batch([(cam, w, h, 12, 8)])
+    finally:
+        device.set_param("wf_paths", 1 << 21)
+        device.set_param("wf_graph", 1)
+    assert bits_equal(img, ref)
+
+
+def test_quantized_nodes_render_the_same_image(scene):
+    """8-bit nodes change the traversal order only: hits differ only on exact
+    ties, so the image is the FP32 one up to such paths."""
+    cs, ds = scene
+    w = h = 48
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    (a,) = ds.render_batch([(cam, w, h, 16, 8)])
+    (b,) = ds.render_batch([(cam, w, h, 16, 8)], quantized=True)
+    same = np.mean(np.all(a == b, axis=2))
+    print("q8 identical pixels", same)
+    assert same > 0.99 and abs(a.mean() - b.mean()) <= 1e-3 * a.mean()
+
+
+def test_bad_frames_are_rejected(scene):
+    cs, ds = scene
+    cam = camera.pinhole(**cs.camera, width=8, height=8)
+    with pytest.raises(tpt._lib.TptError):
+        ds.render_batch([(cam, 8, 8, 0, 8)])
+    assert ds.render_batch([]) == []

@@ -1,0 +1,853 @@
+// Device shading code shared by the megakernel (render.hip) and the wavefront
+// path tracer (wavefront.hip): glam-order vector helpers, the Duff frame,
+// GGX / EON / OpenPBR subset, the shading vertex, material dispatch
+// (OpenPBR, NormalizedLambert, Emissive, Mirror), the SG light tree and the
+// next-event-estimation sample.  Reference citations per function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "light_tree.h"
+#include "sampler.h"
+#include "tpt_internal.h"
+#include "traverse.h"
+
+namespace tpt {
+namespace pt {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTau = 6.28318530717958647692f;
+
+// ---- glam-style vector helpers (component-wise f32, no contraction) ----------
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 scl(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ F3 muls(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 divs(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ F3 hadamard(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ F3 neg(F3 a) { return f3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+  return f3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ float len2(F3 a) { return dot(a, a); }
+__device__ __forceinline__ F3 normalize(F3 a) {  // glam: v * (1 / sqrt(dot))
+  const float r = 1.0f / sqrtf(dot(a, a));
+  return f3(a.x * r, a.y * r, a.z * r);
+}
+__device__ __forceinline__ F3 normalize_or_zero(F3 a) {
+  const float r = 1.0f / sqrtf(dot(a, a));
+  if (__builtin_isfinite(r) && r > 0.0f) return f3(a.x * r, a.y * r, a.z * r);
+  return f3(0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+__device__ __forceinline__ F3 face_forward(F3 n, F3 r) { return dot(n, r) < 0.0f ? neg(n) : n; }
+__device__ __forceinline__ float luminance(F3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+__device__ __forceinline__ F3 interp(float b0, float b1, float b2, F3 a, F3 b, F3 c) {
+  return add(add(scl(b0, a), scl(b1, b)), scl(b2, c));
+}
+
+// ---- OrthonormalBasis (Duff et al.), R/math/orthonormal_basis.rs:11-71 -------
+struct Frame {
+  F3 t, b, n;
+};
+__device__ __forceinline__ Frame frame_from_normal(F3 normal) {
+  const F3 n = len2(normal) > 0.0f ? normalize(normal) : f3(0.0f, 0.0f, 1.0f);
+  const float sign = copysignf(1.0f, n.z);
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  return Frame{f3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x), f3(b, sign + n.y * n.y * a, -n.y), n};
+}
+__device__ __forceinline__ Frame frame_from_normal_tangent(F3 normal, F3 hint) {
+  const F3 n = len2(normal) > 0.0f ? normalize(normal) : f3(0.0f, 0.0f, 1.0f);
+  F3 t = normalize_or_zero(sub(hint, scl(dot(hint, n), n)));
+  if (len2(t) == 0.0f) return frame_from_normal(n);
+  const F3 b = normalize_or_zero(cross(n, t));
+  t = normalize_or_zero(cross(b, n));
+  return Frame{t, b, n};
+}
+__device__ __forceinline__ F3 to_world(const Frame& f, F3 l) {
+  return normalize(add(add(scl(l.x, f.t), scl(l.y, f.b)), scl(l.z, f.n)));
+}
+__device__ __forceinline__ F3 to_frame(const Frame& f, F3 w) { return f3(dot(w, f.t), dot(w, f.b), dot(w, f.n)); }
+
+// ---- GGX (R/bsdf/smith_ggx.rs) -------------------------------------------------
+constexpr float kMinAlpha = 1.0e-4f;
+constexpr float kSmoothAlpha = 1.0e-3f;
+__device__ __forceinline__ bool half_vector(F3 wo, F3 wi, F3* wm) {
+  const F3 m = normalize_or_zero(add(wo, wi));
+  if (len2(m) == 0.0f) return false;
+  *wm = m.z < 0.0f ? neg(m) : m;
+  return true;
+}
+__device__ __forceinline__ float ggx_d(F3 m, float ax, float ay) {
+  if (m.z <= 0.0f) return 0.0f;
+  const float term = m.x * m.x / (ax * ax) + m.y * m.y / (ay * ay) + m.z * m.z;
+  const float denom = kPi * ax * ay * term * term;
+  return denom <= 0.0f ? 0.0f : 1.0f / denom;
+}
+__device__ __forceinline__ float ggx_lambda(F3 w, float ax, float ay) {
+  const float c = fabsf(w.z);
+  if (c <= 0.0f) return __builtin_inff();
+  const float term = 1.0f + (ax * ax * w.x * w.x + ay * ay * w.y * w.y) / (c * c);
+  return 0.5f * (-1.0f + sqrtf(term));
+}
+__device__ __forceinline__ float ggx_g2(F3 wo, F3 wi, float ax, float ay) {
+  const float lo = ggx_lambda(wo, ax, ay), li = ggx_lambda(wi, ax, ay);
+  if (!__builtin_isfinite(lo) || !__builtin_isfinite(li)) return 0.0f;
+  return 1.0f / (1.0f + lo + li);
+}
+__device__ __forceinline__ float pdf_bounded_vndf(F3 wo, F3 wm, float ax, float ay) {
+  if (wo.z <= 0.0f || wm.z <= 0.0f) return 0.0f;
+  ax = fmaxf(ax, kMinAlpha);
+  ay = fmaxf(ay, kMinAlpha);
+  const float a = clampf(fminf(ax, ay), 0.0f, 1.0f);
+  const float s = 1.0f + sqrtf(wo.x * wo.x + wo.y * wo.y);
+  const float a2 = a * a, s2 = s * s;
+  const float k = (1.0f - a2) * s2 / (s2 + a2 * wo.z * wo.z);
+  const float t = sqrtf(ax * ax * wo.x * wo.x + ay * ay * wo.y * wo.y + wo.z * wo.z);
+  const float denom = k * wo.z + t;
+  if (denom <= 0.0f) return 0.0f;
+  return 2.0f * ggx_d(wm, ax, ay) * fmaxf(dot(wo, wm), 0.0f) / denom;
+}
+
+// ---- Fresnel (R/math/mod.rs:50-72, R/bsdf/open_pbr.rs:2161-2172) ----------------
+__device__ __forceinline__ float fresnel_dielectric(float ci, float ei, float et) {
+  ci = clampf(ci, -1.0f, 1.0f);
+  if (ci < 0.0f) {
+    const float t = ei;
+    ei = et;
+    et = t;
+    ci = -ci;
+  }
+  const float si = sqrtf(fmaxf(1.0f - ci * ci, 0.0f));
+  const float st = (ei / et) * si;
+  if (st >= 1.0f) return 1.0f;
+  const float ct = sqrtf(fmaxf(1.0f - st * st, 0.0f));
+  const float rpar = (et * ci - ei * ct) / (et * ci + ei * ct);
+  const float rper = (ei * ci - et * ct) / (ei * ci + et * ct);
+  return 0.5f * (rpar * rpar + rper * rper);
+}
+__device__ __forceinline__ float weighted_fresnel_ratio(float c, float eti, float etp) {
+  eti = fmaxf(eti, 1.0e-4f);
+  etp = fmaxf(etp, 1.0e-4f);
+  if (etp >= 1.0f) return fresnel_dielectric(c, 1.0f, etp);
+  const float cc = clampf(c, 0.0f, 1.0f);
+  const float mu2 = 1.0f - (1.0f - cc * cc) / (eti * eti);
+  if (mu2 <= 0.0f) return 1.0f;
+  return fresnel_dielectric(sqrtf(mu2), 1.0f, 1.0f / etp);
+}
+
+// ---- EON diffuse (R/bsdf/eon.rs:12-197), rho = 1 ----------------------------------
+constexpr float kRcpPi = 1.0f / kPi;
+__device__ __forceinline__ float c1_fon() { return 0.5f - 2.0f / (3.0f * kPi); }
+__device__ __forceinline__ float c2_fon() { return 2.0f / 3.0f - 28.0f / (15.0f * kPi); }
+__device__ __forceinline__ float e_fon(float mu, float r) {
+  const float mc = 1.0f - mu;
+  const float g = mc * (0.05710853f + mc * (0.49188188f + mc * (-0.33218145f + mc * 0.071443f)));
+  return (1.0f + r * g) / (1.0f + c1_fon() * r);
+}
+// f_eon with rho = (1,1,1): every channel is the same scalar.
+__device__ __forceinline__ float f_eon1(float r, F3 wo, F3 wi) {
+  const float mo = wo.z, mi = wi.z;
+  const float s = dot(wo, wi) - mi * mo;
+  const float sot = s > 0.0f ? s / fmaxf(mo, mi) : s;
+  const float af = 1.0f / (1.0f + c1_fon() * r);
+  const float fss = (kRcpPi * af * (1.0f + r * sot)) * 1.0f;
+  const float efo = e_fon(mo, r), efi = e_fon(mi, r);
+  const float avg = af * (1.0f + c2_fon() * r);
+  const float oma = 1.0f - avg;
+  const float denom = 1.0f - 1.0f * oma;
+  const float rho_ms = 1.0f * 1.0f * avg / denom;
+  const float scale = kRcpPi * fmaxf(1.0f - efo, 1.0e-7f) * fmaxf(1.0f - efi, 1.0e-7f) / fmaxf(oma, 1.0e-7f);
+  return fss + rho_ms * scale;
+}
+__device__ __forceinline__ void ltc_coeffs(float mu, float r, float* a, float* b, float* c, float* d) {
+  *a = 1.0f + r * (0.303392f + (-0.518982f + 0.111709f * mu) * mu + (-0.276266f + 0.335918f * mu) * r);
+  *b = r * (-1.16407f + 1.15859f * mu + (0.150815f - 0.150105f * mu) * r) / (mu * mu * mu - 1.43545f);
+  *c = 1.0f + r * (0.20013f + (-0.506373f + 0.261777f * mu) * mu);
+  *d = r * (0.540852f + (-1.01625f + 0.475392f * mu) * mu) / (-1.0743f + (0.0725628f + mu) * mu);
+}
+__device__ __forceinline__ void ltc_x(F3 wo, float* x, float* y) {
+  const float l2 = wo.x * wo.x + wo.y * wo.y;
+  if (l2 > 0.0f) {
+    const float inv = 1.0f / sqrtf(l2);
+    *x = wo.x * inv;
+    *y = wo.y * inv;
+  } else {
+    *x = 1.0f;
+    *y = 0.0f;
+  }
+}
+__device__ __forceinline__ float cltc_pdf(F3 wo, F3 wi, float r) {
+  float xx, xy;
+  ltc_x(wo, &xx, &xy);
+  const float yx = -xy, yy = xx;
+  const F3 wl = f3(xx * wi.x + xy * wi.y, yx * wi.x + yy * wi.y, wi.z);
+  float a, b, c, d;
+  ltc_coeffs(wo.z, r, &a, &b, &c, &d);
+  const float det = c * (a - b * d);
+  const F3 wh = f3(c * (wl.x - b * wl.z), (a - b * d) * wl.y, -c * (d * wl.x - a * wl.z));
+  const float l2 = fmaxf(dot(wh, wh), 1.0e-30f);
+  const float vz = 1.0f / sqrtf(d * d + 1.0f);
+  const float s = 0.5f * (1.0f + vz);
+  const float q = det / l2;
+  return q * q * fmaxf(wh.z, 0.0f) / (kPi * s);
+}
+__device__ __forceinline__ float p_uniform(float mu, float r) {
+  return powf(r, 0.1f) * (0.162925f + (-0.372058f + (0.538233f - 0.290822f * mu) * mu) * mu);
+}
+__device__ __forceinline__ float pdf_eon(F3 wo, F3 wi, float r) {
+  const float pu = p_uniform(wo.z, r);
+  return pu * (1.0f / (2.0f * kPi)) + (1.0f - pu) * cltc_pdf(wo, wi, r);
+}
+__device__ __forceinline__ F3 sample_eon(F3 wo, float r, float u1, float u2, float* pdf) {
+  const float pu = p_uniform(wo.z, r);
+  const float pc = 1.0f - pu;
+  F3 wi;
+  float pdf_c;
+  if (u1 < pu) {
+    const float u1n = u1 / pu;
+    const float st = sqrtf(fmaxf(1.0f - u1n * u1n, 0.0f));
+    const float phi = 2.0f * kPi * u2;
+    wi = f3(st * cosf(phi), st * sinf(phi), u1n);
+    pdf_c = cltc_pdf(wo, wi, r);
+  } else {
+    const float u1n = (u1 - pu) / fmaxf(pc, 1.0e-30f);
+    float a, b, c, d;
+    ltc_coeffs(wo.z, r, &a, &b, &c, &d);
+    const float radius = sqrtf(u1n);
+    const float phi = 2.0f * kPi * u2;
+    const float x0 = radius * cosf(phi), y0 = radius * sinf(phi);
+    const float vz = 1.0f / sqrtf(d * d + 1.0f);
+    const float s = 0.5f * (1.0f + vz);
+    const float edge = sqrtf(fmaxf(1.0f - y0 * y0, 0.0f));
+    const float x = -((1.0f - s) * edge + s * x0);
+    const float y = y0;
+    const F3 wh = f3(x, y, sqrtf(fmaxf(1.0f - (x * x + y * y), 0.0f)));
+    const float pdf_wh = wh.z / (kPi * s);
+    const F3 wl = f3(a * wh.x + b * wh.z, c * wh.y, d * wh.x + wh.z);
+    const float len = sqrtf(dot(wl, wl));
+    const float det = c * (a - b * d);
+    pdf_c = pdf_wh * (len * len * len) / det;
+    const F3 v = divs(wl, len);
+    float xx, xy;
+    ltc_x(wo, &xx, &xy);
+    const float yx = -xy, yy = xx;
+    wi = f3(xx * v.x + yx * v.y, xy * v.x + yy * v.y, v.z);
+  }
+  *pdf = pu * (1.0f / (2.0f * kPi)) + pc * pdf_c;
+  return wi;
+}
+
+// ---- OpenPBR subset ---------------------------------------------------------------------
+struct Bsdf {
+  F3 base_color, spec_color;
+  float base, ax, ay, spec_eta, trans_eta, diff_rough;
+  bool front;
+};
+
+__device__ __forceinline__ float modulated_eta(float eta, float w) {  // R/material/open_pbr.rs:871-878
+  if (fabsf(w - 1.0f) < 1.0e-6f) return eta;
+  const float q = (eta - 1.0f) / (eta + 1.0f);
+  const float f0 = q * q;
+  const float sg = (eta - 1.0f) < 0.0f ? -1.0f : 1.0f;
+  const float eps = sqrtf(clampf(w * f0, 0.0f, 0.99999f)) * sg;
+  return fmaxf((1.0f + eps) / fmaxf(1.0f - eps, 1.0e-6f), 1.0e-4f);
+}
+
+__device__ __forceinline__ Bsdf make_bsdf(const TptOpenPbr& m, bool front) {  // :393-487
+  Bsdf p;
+  p.base_color = f3(m.base_color[0], m.base_color[1], m.base_color[2]);
+  const float rough = clampf(m.specular_roughness, 0.0f, 1.0f);
+  const float an = clampf(m.specular_roughness_anisotropy, 0.0f, 1.0f);
+  const float alpha = rough * rough, oma = 1.0f - an;
+  const float axr = alpha * sqrtf(2.0f / (1.0f + oma * oma));
+  p.ax = clampf(axr, kMinAlpha, 1.0f);
+  p.ay = clampf(oma * axr, kMinAlpha, 1.0f);
+  p.spec_color = f3(clampf(m.specular_color[0], 0.0f, 1.0f), clampf(m.specular_color[1], 0.0f, 1.0f),
+                    clampf(m.specular_color[2], 0.0f, 1.0f));
+  p.spec_eta = modulated_eta(fmaxf(m.specular_ior, 1.0e-4f), fmaxf(m.specular_weight, 0.0f));
+  p.trans_eta = p.spec_eta;
+  p.base = clampf(m.base_weight, 0.0f, 1.0f);
+  p.diff_rough = clampf(m.base_diffuse_roughness, 0.0f, 1.0f);
+  p.front = front;
+  return p;
+}
+
+// layer_weights (R/bsdf/open_pbr.rs:333-390) of the subset: fuzz, coat,
+// metal, transmission and subsurface are zero and e_spec = 0.
+__device__ __forceinline__ void layer_weights(const Bsdf& p, F3* spec_w, F3* diff_w) {
+  const F3 below_metal = f3(1.0f, 1.0f, 1.0f);
+  *spec_w = hadamard(below_metal, p.spec_color);
+  const F3 leak = f3(fmaxf(1.0f - p.spec_color.x * 0.0f, 0.0f), fmaxf(1.0f - p.spec_color.y * 0.0f, 0.0f),
+                     fmaxf(1.0f - p.spec_color.z * 0.0f, 0.0f));
+  const F3 below_trans = muls(hadamard(below_metal, leak), 1.0f - 0.0f);
+  *diff_w = hadamard(muls(below_trans, (1.0f - 0.0f) * p.base), p.base_color);
+}
+struct Probs {
+  float spec, diff, total;
+};
+__device__ __forceinline__ Probs lobe_probs(const Bsdf& p) {  // :392-422
+  F3 sw, dw;
+  layer_weights(p, &sw, &dw);
+  const float zero_lum = luminance(f3(0.0f, 0.0f, 0.0f));
+  const float spec = luminance(sw);
+  const float diff = fmaxf(luminance(dw), 0.0f);
+  const float total = 0.0f + zero_lum + spec + 0.0f + fmaxf(zero_lum, 0.0f) + diff + fmaxf(zero_lum, 0.0f);
+  return Probs{spec, diff, total};
+}
+__device__ __forceinline__ bool spec_smooth(const Bsdf& p) { return fmaxf(p.ax, p.ay) < kSmoothAlpha; }
+__device__ __forceinline__ float spec_fresnel(const Bsdf& p, float c) {
+  return p.front ? weighted_fresnel_ratio(c, p.trans_eta, p.spec_eta)
+                 : weighted_fresnel_ratio(c, 1.0f / p.trans_eta, 1.0f / p.spec_eta);
+}
+__device__ __forceinline__ float spec_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_spec_brdf :841-867
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  F3 wm;
+  if (!half_vector(wo, wi, &wm)) return 0.0f;
+  const float d = ggx_d(wm, p.ax, p.ay);
+  const float g = ggx_g2(wo, wi, p.ax, p.ay);
+  if (d <= 0.0f || g <= 0.0f) return 0.0f;
+  const float ss = spec_fresnel(p, fabsf(dot(wo, wm))) * (d * g / (4.0f * wo.z * wi.z));
+  return ss + 0.0f * 0.0f;  // multiple-scattering factor: f_avg(1) = 0
+}
+__device__ __forceinline__ float spec_pdf(const Bsdf& p, F3 wo, F3 wi) {  // pdf_specular_brdf :1006-1022
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  float pdf_ss = 0.0f;
+  F3 wm;
+  if (half_vector(wo, wi, &wm)) {
+    const float pw = pdf_bounded_vndf(wo, wm, p.ax, p.ay);
+    const float den = 4.0f * fabsf(dot(wo, wm));
+    pdf_ss = den <= 0.0f ? 0.0f : pw / den;
+  }
+  const float pdf_ms = fmaxf(wi.z, 0.0f) / kPi;
+  const float ss_w = 0.0f, ms_w = 0.0f, total = fmaxf(ss_w + ms_w, 1.0e-8f);
+  return (ss_w * pdf_ss + ms_w * pdf_ms) / total;
+}
+__device__ __forceinline__ F3 bsdf_eval(const Bsdf& p, F3 wo, F3 wi) {  // :224-259
+  if (!(wo.z > 0.0f)) return f3(0.0f, 0.0f, 0.0f);
+  F3 sw, dw;
+  layer_weights(p, &sw, &dw);
+  F3 total = f3(0.0f, 0.0f, 0.0f);
+  if (len2(sw) > 0.0f && !spec_smooth(p)) total = add(total, muls(sw, spec_eval(p, wo, wi)));
+  if (len2(dw) > 0.0f) {
+    const float f = (wo.z <= 0.0f || wi.z <= 0.0f) ? 0.0f : f_eon1(p.diff_rough, wo, wi);
+    total = add(total, hadamard(dw, f3(f, f, f)));
+  }
+  return total;
+}
+__device__ __forceinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {  // :261-293
+  if (!(wo.z > 0.0f)) return 0.0f;
+  const Probs pr = lobe_probs(p);
+  if (pr.total <= 0.0f) return 0.0f;
+  float pdf = 0.0f;
+  if (pr.spec > 0.0f && !spec_smooth(p)) pdf += (pr.spec / pr.total) * spec_pdf(p, wo, wi);
+  if (pr.diff > 0.0f) {
+    const float pd = (wo.z <= 0.0f || wi.z <= 0.0f) ? 0.0f : pdf_eon(wo, wi, p.diff_rough);
+    pdf += (pr.diff / pr.total) * pd;
+  }
+  return pdf;
+}
+
+struct Sample {
+  F3 weight, wi;
+  float pdf;
+  bool delta;
+};
+__device__ __forceinline__ bool finalize(const Bsdf& p, F3 wo, F3 wi, Sample* out) {  // :1958-1999
+  const float pdf = bsdf_pdf(p, wo, wi);
+  if (pdf <= 0.0f) return false;
+  const F3 f = bsdf_eval(p, wo, wi);
+  if (len2(f) == 0.0f) return false;
+  const float c = fabsf(wi.z);
+  if (c <= 0.0f) return false;
+  out->weight = muls(f, c / pdf);
+  out->wi = wi;
+  out->pdf = pdf;
+  out->delta = false;
+  return true;
+}
+__device__ __forceinline__ F3 cosine_hemisphere(float u, float v) {  // R/math/mod.rs:26-36
+  const float r = sqrtf(clampf(u, 0.0f, 1.0f));
+  const float phi = kTau * v;
+  const float dx = r * cosf(phi), dy = r * sinf(phi);
+  return f3(dx, dy, sqrtf(fmaxf(1.0f - (dx * dx + dy * dy), 0.0f)));
+}
+// OpenPbrBsdf::sample (:295-331), pick_lobe (:2016-2041), sample_spec_brdf
+// (:1589-1668), sample_diff_brdf (:1936-1940)
+__device__ __forceinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, float ux, float uy, Sample* out) {
+  if (!(wo.z > 0.0f)) return false;
+  const Probs pr = lobe_probs(p);
+  if (pr.total <= 0.0f) return false;
+  float u = u_lobe * pr.total;
+  if (u < 0.0f) return false;  // coat (probability 0)
+  if (u < 0.0f) return false;  // metal
+  bool spec;
+  if (u < pr.spec) {
+    spec = true;
+  } else {
+    spec = false;
+    u -= pr.spec;
+    if (!(u < pr.diff)) return false;  // DiffBtdf: probability 0 -> no sample
+  }
+  const float p_lobe = (spec ? pr.spec : pr.diff) / pr.total;
+  if (p_lobe <= 0.0f) return false;
+  if (spec) {
+    if (spec_smooth(p)) {
+      const F3 wi = f3(-wo.x, -wo.y, wo.z);
+      F3 sw, dw;
+      layer_weights(p, &sw, &dw);
+      out->weight = divs(muls(sw, spec_fresnel(p, fabsf(wi.z))), p_lobe);
+      out->wi = wi;
+      out->pdf = p_lobe;
+      out->delta = true;
+      return true;
+    }
+    const float ss_prob = 0.0f / 1.0e-8f;  // dielectric reflection lobe weights at eta 1
+    if (ux >= ss_prob) {
+      const float uu = ss_prob < 1.0f ? (ux - ss_prob) / (1.0f - ss_prob) : 0.0f;
+      return finalize(p, wo, cosine_hemisphere(uu, uy), out);
+    }
+    return false;
+  }
+  float pdf;
+  const F3 wi = sample_eon(wo, p.diff_rough, ux, uy, &pdf);
+  if (pdf <= 0.0f || wi.z <= 0.0f) return false;
+  return finalize(p, wo, wi, out);
+}
+
+// ---- scene shading -----------------------------------------------------------------------
+struct ShadeView {
+  const GpuShade* inst;
+  const float* pos;
+  const float* nrm;
+  const float* uv;
+  const uint32_t* idx;
+  const TptMaterial* mats;
+  const GpuLightNode* lights;  // light tree, root 0 (light_count 0: no tree)
+  const uint32_t* tri_leaf;    // GpuShade::light_base + triangle -> leaf node
+  uint32_t light_count;
+};
+
+struct Vtx {
+  F3 p, ng, ns, wo;
+  Frame frame;
+  bool front;
+  uint32_t material;
+  uint32_t inst, tri;
+};
+
+__device__ __forceinline__ F3 load3(const float* a, uint32_t i) { return f3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+__device__ __forceinline__ F3 xform_point(const float* m, F3 p) {
+  float rx = m[0] * p.x, ry = m[1] * p.x, rz = m[2] * p.x;
+  rx = m[3] * p.y + rx; ry = m[4] * p.y + ry; rz = m[5] * p.y + rz;
+  rx = m[6] * p.z + rx; ry = m[7] * p.z + ry; rz = m[8] * p.z + rz;
+  return f3(m[9] + rx, m[10] + ry, m[11] + rz);
+}
+__device__ __forceinline__ F3 xform_normal(const float* n, F3 v) {  // Mat3::mul_vec3
+  return add(add(scl(v.x, f3(n[0], n[1], n[2])), scl(v.y, f3(n[3], n[4], n[5]))), scl(v.z, f3(n[6], n[7], n[8])));
+}
+
+// base_shading_vertex_from_triangle_sample_impl (R/scene/mod.rs:471-544)
+__device__ __forceinline__ Vtx shading_vertex(const ShadeView& sv, uint32_t inst, uint32_t tri, float b0, float b1,
+                                              float b2, F3 dir) {
+  const GpuShade& si = sv.inst[inst];
+  const uint32_t i0 = si.vbase + sv.idx[si.ibase + 3 * tri], i1 = si.vbase + sv.idx[si.ibase + 3 * tri + 1],
+                 i2 = si.vbase + sv.idx[si.ibase + 3 * tri + 2];
+  const F3 p0 = xform_point(si.l2w, load3(sv.pos, i0)), p1 = xform_point(si.l2w, load3(sv.pos, i1)),
+           p2 = xform_point(si.l2w, load3(sv.pos, i2));
+  const bool has_n = si.flags & 1u, has_uv = si.flags & 2u;
+  const F3 zero = f3(0.0f, 0.0f, 0.0f);
+  const F3 n0 = normalize_or_zero(xform_normal(si.nrm, has_n ? load3(sv.nrm, i0) : zero));
+  const F3 n1 = normalize_or_zero(xform_normal(si.nrm, has_n ? load3(sv.nrm, i1) : zero));
+  const F3 n2 = normalize_or_zero(xform_normal(si.nrm, has_n ? load3(sv.nrm, i2) : zero));
+  Vtx v;
+  v.p = interp(b0, b1, b2, p0, p1, p2);
+  const F3 gn = normalize_or_zero(cross(sub(p1, p0), sub(p2, p0)));
+  const F3 sn = normalize_or_zero(interp(b0, b1, b2, n0, n1, n2));
+  const F3 nd = neg(dir);
+  v.front = dot(gn, nd) >= 0.0f;
+  v.ng = face_forward(gn, nd);
+  v.ns = face_forward(len2(sn) > 0.0f ? sn : v.ng, v.ng);
+  // compute_surface_partials over world positions (R/math/mod.rs:112-134)
+  const float u0 = has_uv ? sv.uv[2 * i0] : 0.0f, w0 = has_uv ? sv.uv[2 * i0 + 1] : 0.0f;
+  const float u1 = has_uv ? sv.uv[2 * i1] : 0.0f, w1 = has_uv ? sv.uv[2 * i1 + 1] : 0.0f;
+  const float u2 = has_uv ? sv.uv[2 * i2] : 0.0f, w2 = has_uv ? sv.uv[2 * i2 + 1] : 0.0f;
+  const float du1 = u1 - u0, dv1 = w1 - w0, du2 = u2 - u0, dv2 = w2 - w0;
+  const float det = du1 * dv2 - dv1 * du2;
+  F3 dpdu = zero;
+  bool ok = false;
+  if (!(fabsf(det) <= 1.0e-8f)) {
+    const float inv = 1.0f / det;
+    const F3 dp1 = sub(p1, p0), dp2 = sub(p2, p0);
+    dpdu = muls(sub(scl(dv2, dp1), scl(dv1, dp2)), inv);
+    const F3 dpdv = muls(add(scl(-du2, dp1), scl(du1, dp2)), inv);
+    ok = !(len2(dpdu) == 0.0f || len2(dpdv) == 0.0f);
+  }
+  if (!ok) dpdu = frame_from_normal(v.ns).t;
+  v.frame = frame_from_normal_tangent(v.ns, dpdu);
+  v.wo = normalize_or_zero(nd);
+  v.material = si.material;
+  v.inst = inst;
+  v.tri = tri;
+  return v;
+}
+
+// Scene::triangle_positions (R/scene/mod.rs:432-438) -> world positions
+__device__ __forceinline__ void triangle_world(const ShadeView& sv, uint32_t inst, uint32_t tri, F3* p) {
+  const GpuShade& si = sv.inst[inst];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) p[k] = xform_point(si.l2w, load3(sv.pos, si.vbase + sv.idx[si.ibase + 3 * tri + k]));
+}
+
+// OpenPbrMaterial::le (R/material/open_pbr.rs:351-375); the exit tint is 1
+// without coat and fuzz (0 at a grazing exit).
+__device__ __forceinline__ bool pbr_le(const TptOpenPbr& m, const Vtx& v, F3* le) {
+  if (m.emission_luminance <= 0.0f) return false;
+  if (!m.thin_walled && !v.front) return false;
+  const F3 wl = normalize_or_zero(to_frame(v.frame, v.wo));
+  const float co = clampf(fabsf(wl.z), 0.0f, 1.0f);
+  const float tint = co <= 0.0f ? 0.0f : 1.0f;
+  const F3 c = muls(f3(m.emission_color[0], m.emission_color[1], m.emission_color[2]),
+                    fmaxf(m.emission_luminance, 0.0f));
+  *le = hadamard(c, f3(tint, tint, tint));
+  return true;
+}
+
+// OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548), radiance mode.
+__device__ __forceinline__ bool pbr_sample(const TptOpenPbr& m, const Vtx& v, float u_lobe, float ux, float uy,
+                                           Sample* s, F3* wi_world) {
+  if (dot(v.wo, v.ng) <= 0.0f) return false;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  if (wol.z <= 0.0f) return false;
+  const Bsdf p = make_bsdf(m, v.front);
+  if (!bsdf_sample(p, wol, u_lobe, ux, uy, s)) return false;
+  const F3 wi = to_world(v.frame, s->wi);
+  if (dot(wi, v.ng) <= 1.0e-6f) return false;  // GEOMETRIC_NORMAL_COS_EPSILON, reflection lobes
+  *wi_world = wi;
+  return true;
+}
+
+// ---- Material dispatch (R/material/mod.rs:251-412) ---------------------------------
+__device__ __forceinline__ F3 rgb3(const float* c) { return f3(c[0], c[1], c[2]); }
+__device__ __forceinline__ bool mat_may_emit(const TptMaterial& m) {
+  if (m.kind == TPT_MATERIAL_EMISSIVE) return true;
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return m.u.open_pbr.emission_luminance > 0.0f;
+  return false;
+}
+__device__ __forceinline__ bool mat_le(const TptMaterial& m, const Vtx& v, F3* le) {
+  if (m.kind == TPT_MATERIAL_EMISSIVE) {  // EmissiveMaterial::le: color * strength * texture factor 1
+    *le = hadamard(muls(rgb3(m.u.emissive.color), m.u.emissive.strength), f3(1.0f, 1.0f, 1.0f));
+    return true;
+  }
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_le(m.u.open_pbr, v, le);
+  return false;
+}
+// NormalizedLambertMaterial::sample (R/material/normalized_lambert.rs:120-158)
+// and MirrorMaterial::sample (R/material/mirror.rs:112-143), radiance mode.
+__device__ __forceinline__ bool mat_sample(const TptMaterial& m, const Vtx& v, float u_lobe, float ux, float uy,
+                                           Sample* s, F3* wi_world) {
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_sample(m.u.open_pbr, v, u_lobe, ux, uy, s, wi_world);
+  if (m.kind == TPT_MATERIAL_EMISSIVE) return false;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  if (wol.z <= 0.0f) return false;
+  F3 wl;
+  if (m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {  // R/bsdf/normalized_lambert.rs:35-60
+    wl = cosine_hemisphere(ux, uy);
+    const float pdf = fmaxf(wl.z, 0.0f) / kPi;
+    if (pdf <= 0.0f) return false;
+    s->weight = rgb3(m.u.lambert.rho);
+    s->pdf = pdf;
+    s->delta = false;
+  } else {  // R/bsdf/mirror.rs:23-39
+    wl = normalize_or_zero(f3(-wol.x, -wol.y, wol.z));
+    s->weight = rgb3(m.u.mirror.color);
+    s->pdf = 1.0f;
+    s->delta = true;
+  }
+  s->wi = wl;
+  const F3 wi = to_world(v.frame, wl);
+  if (dot(wi, v.ng) <= 1.0e-6f) return false;
+  *wi_world = wi;
+  return true;
+}
+// NormalizedLambertMaterial::eval / pdf (R/material/normalized_lambert.rs:160-193)
+// with modified_bsdf_eval (R/material/mod.rs:33-53); the other materials
+// never receive a light sample (no light-tree query) or evaluate to zero.
+__device__ __forceinline__ F3 lambert_eval(const TptMaterial& m, const Vtx& v, F3 wi, float* pdf) {
+  const F3 zero = f3(0.0f, 0.0f, 0.0f);
+  *pdf = 0.0f;
+  if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return zero;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  const F3 wil = normalize_or_zero(to_frame(v.frame, wi));
+  if (wol.z <= 0.0f || wil.z <= 0.0f) return zero;
+  const F3 f = divs(rgb3(m.u.lambert.rho), kPi);
+  if (len2(f) == 0.0f) return zero;
+  const float cg = fabsf(dot(wi, v.ng));
+  if (cg <= 1.0e-6f) return zero;
+  *pdf = fmaxf(wil.z, 0.0f) / kPi;
+  return muls(f, fabsf(dot(wi, v.ns)) / cg);
+}
+
+// ---- SG light tree (R/math/sg.rs, R/light_tree/{lobe,traversal}.rs) ---------------
+constexpr float kSgMax = 2199023255552.0f;  // 2^41
+constexpr float kFltMinPos = 1.17549435e-38f;
+constexpr float kFltEpsilon = 1.1920929e-7f;
+// erf fit of R/math/sg.rs:74-107
+__device__ __forceinline__ float erf_fit(float x) {
+  const float ax = fabsf(x);
+  if (ax >= 4.0f) return copysignf(1.0f, x);
+  if (ax > 1.0f) {
+    const float p = (((((((2.5890768e-5f * ax + -5.648746e-4f) * ax + 5.6679556e-3f) * ax + -3.5175983e-2f) * ax +
+                        1.5432939e-1f) * ax + 9.156747e-1f) * ax + 1.6284595f) * ax);
+    return copysignf(1.0f - exp2f(-p), x);
+  }
+  const float x2 = x * x;
+  return (((((-5.5885315e-4f * x2 + 4.9073556e-3f) * x2 + -2.6703065e-2f) * x2 + 1.1279922e-1f) * x2 + -3.76123e-1f) *
+              x2 + 1.12837917f) * x;
+}
+struct SgLobe {
+  F3 axis;
+  float sharp, log_amp;
+};
+__device__ __forceinline__ SgLobe sg_product(F3 a1, float s1, F3 a2, float s2) {  // R/math/sg.rs:137-156
+  const F3 ax = add(muls(a1, s1), muls(a2, s2));
+  const float sharp = sqrtf(dot(ax, ax));
+  const F3 d = sub(a1, a2);
+  const float denom = fmaxf(sharp + s1 + s2, kFltMinPos);
+  SgLobe l;
+  l.log_amp = -s1 * s2 * dot(d, d) / denom;
+  l.axis = sharp > kFltMinPos ? divs(ax, sharp) : f3(0.0f, 0.0f, 1.0f);
+  l.sharp = sharp;
+  return l;
+}
+// Clamped-cosine x SG product integral / pi (R/math/sg.rs:236-306)
+__device__ __forceinline__ float cc_product_over_pi(float z, float s) {
+  const float t = s * sqrtf(0.5f * ((s + 2.736083f) * s + 17.021297f) / (((s + 4.0100827f) * s + 15.219156f) * s + 76.0879f));
+  const float tz = t * z;
+  const float lerp = clampf(0.5f * (z * (1.0f - erf_fit(-tz)) + (1.0f - erf_fit(t))) -
+                                0.5f * 0.5641896f * expf(-tz * tz) * expm1f(t * t * (z * z - 1.0f)) / fmaxf(t, kFltMinPos),
+                            0.5f * kFltEpsilon, 1.0f);
+  const float e = expf(-s);
+  float lower, upper;
+  if (s <= 0.5f) {
+    upper = (((((((-1.0f / 362880.0f) * s + 1.0f / 40320.0f) * s - 1.0f / 5040.0f) * s + 1.0f / 720.0f) * s -
+               1.0f / 120.0f) * s + 1.0f / 24.0f) * s - 1.0f / 6.0f) * s + 0.5f;
+    lower = e * (((((((((1.0f / 403200.0f) * s - 1.0f / 45360.0f) * s + 1.0f / 5760.0f) * s - 1.0f / 840.0f) * s +
+                     1.0f / 144.0f) * s - 1.0f / 30.0f) * s + 1.0f / 8.0f) * s - 1.0f / 3.0f) * s + 0.5f);
+  } else {
+    upper = (expm1f(-s) + s) / (s * s);
+    lower = e * (-expm1f(-s) - s * e) / (s * s);
+  }
+  return 2.0f * (lower + (upper - lower) * lerp);
+}
+// sg_light_for_node + diffuse_importance (R/light_tree/lobe.rs:233-287) of a
+// diffuse query (p, n, rho)
+__device__ __forceinline__ float node_importance(const GpuLightNode& nd, F3 p, F3 n, float rho) {
+  if (nd.flux <= 0.0f) return 0.0f;
+  const F3 d = sub(f3(nd.mu[0], nd.mu[1], nd.mu[2]), p);
+  const float r2 = dot(d, d);
+  if (r2 <= 0.0f) return 0.0f;
+  const float inv_r = 1.0f / sqrtf(r2);
+  const float cos_below = fmaxf(-(dot(n, d)) * inv_r, 0.0f);
+  float sigma2 = nd.sigma_s2 * (1.0f - cos_below) + nd.bound_var * cos_below;
+  const float min_sigma2 = r2 / kSgMax;
+  if (sigma2 < min_sigma2) sigma2 = min_sigma2;
+  const F3 axis = muls(d, inv_r);
+  const float sharp = clampf(r2 / sigma2, 0.0f, kSgMax);
+  const SgLobe lobe = sg_product(f3(-nd.nu[0], -nd.nu[1], -nd.nu[2]), nd.lambda, axis, sharp);
+  const float w_amp = nd.flux / (2.0f * kPi * sigma2 * nd.sg_norm);
+  const float w = w_amp * expf(lobe.log_amp);
+  if (!__builtin_isfinite(w) || w <= 0.0f) return 0.0f;
+  if (rho <= 0.0f) return 0.0f;
+  const float c = clampf(dot(lobe.axis, n), -1.0f, 1.0f);
+  return rho * w * (cc_product_over_pi(c, lobe.sharp) * kPi);
+}
+// p(left) of an interior node, with the flux fallback when both children
+// look dark (R/light_tree/traversal.rs:54-80); false if no child has flux.
+__device__ __forceinline__ bool p_left(const ShadeView& sv, const GpuLightNode& nd, F3 p, F3 n, float rho,
+                                       float* pl, bool* by_importance) {
+  const GpuLightNode& L = sv.lights[nd.left];
+  const GpuLightNode& R = sv.lights[nd.right];
+  const float il = node_importance(L, p, n, rho), ir = node_importance(R, p, n, rho);
+  const float total = il + ir;
+  if (total <= 0.0f) {
+    const float lf = fmaxf(L.flux, 0.0f), rf = fmaxf(R.flux, 0.0f);
+    const float tf = lf + rf;
+    if (tf <= 0.0f) return false;
+    *pl = lf / tf;
+    *by_importance = false;
+    return true;
+  }
+  *pl = il / total;
+  *by_importance = true;
+  return true;
+}
+// sample_light_tree (R/light_tree/traversal.rs:26-92)
+__device__ __forceinline__ bool sample_tree(const ShadeView& sv, F3 p, F3 n, float rho, float u, uint32_t* leaf,
+                                            float* pmf_out) {
+  if (sv.light_count == 0) return false;
+  u = clampf(u, 0.0f, 1.0f - kFltEpsilon);
+  uint32_t ni = 0;
+  float pmf = 1.0f;
+  for (;;) {
+    const GpuLightNode& nd = sv.lights[ni];
+    if (nd.left == TPT_INVALID_INDEX) {
+      *leaf = ni;
+      *pmf_out = pmf;
+      return true;
+    }
+    float pl;
+    bool imp;
+    if (!p_left(sv, nd, p, n, rho, &pl, &imp)) return false;
+    if (u < pl) {
+      u /= fmaxf(pl, kFltMinPos);
+      pmf *= pl;
+      ni = nd.left;
+    } else {
+      u = (u - pl) / fmaxf(1.0f - pl, kFltMinPos);
+      pmf *= 1.0f - pl;
+      ni = nd.right;
+    }
+    if (imp) u = clampf(u, 0.0f, 1.0f - kFltEpsilon);
+  }
+}
+// pdf_for_leaf (R/light_tree/traversal.rs:97-140)
+__device__ __forceinline__ float leaf_pmf(const ShadeView& sv, F3 p, F3 n, float rho, uint32_t leaf) {
+  if (sv.light_count == 0 || leaf == TPT_INVALID_INDEX) return 0.0f;
+  float pmf = 1.0f;
+  uint32_t child = leaf, parent = sv.lights[leaf].parent;
+  while (parent != TPT_INVALID_INDEX) {
+    const GpuLightNode& pn = sv.lights[parent];
+    float pl;
+    bool imp;
+    if (!p_left(sv, pn, p, n, rho, &pl, &imp)) return 0.0f;
+    pmf *= child == pn.left ? pl : 1.0f - pl;
+    child = parent;
+    parent = pn.parent;
+  }
+  return pmf;
+}
+
+__device__ __forceinline__ float balance_heuristic(float a, float b) {  // R/math/mod.rs:163-171
+  const float sum = a + b;
+  if (a <= 0.0f || sum <= 0.0f) return 0.0f;
+  return a / sum;
+}
+
+__device__ __forceinline__ F3 offset_origin(F3 p, F3 ng, F3 d) {  // R/integrator/mod.rs:76-85
+  return add(p, scl(1.0e-4f, dot(d, ng) >= 0.0f ? ng : neg(ng)));
+}
+
+// The light sample of direct_light_mis_contribution (R/integrator/mis.rs:141-210)
+// up to the shadow test: tree leaf (sample_light_tree), uniform point on its
+// triangle (sample_triangle_point, R/scene/mod.rs:616-633; sample_li_for_triangle,
+// R/light/area.rs:12-54), balance-heuristic weight.  On success `value` is
+// the contribution if the shadow ray from `origin` along `dir` reaches the
+// target triangle first (unoccluded_ray with a target, R/integrator/mod.rs:
+// 239-265).  The tree is the light sampler's only category (pmf 1).
+struct NeeSample {
+  F3 origin, dir, value;
+  uint32_t inst, tri;
+};
+__device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMaterial& m, const Vtx& vtx, float u_tree,
+                                           float us0, float us1, NeeSample* ns) {
+  if (sv.light_count == 0 || m.kind != TPT_MATERIAL_NORMALIZED_LAMBERT) return false;
+  const float rho = luminance(rgb3(m.u.lambert.rho));
+  if (rho <= 0.0f) return false;
+  uint32_t leaf;
+  float lpmf;
+  if (!sample_tree(sv, vtx.p, vtx.ns, rho, u_tree, &leaf, &lpmf)) return false;
+  const GpuLightNode& ln = sv.lights[leaf];
+  F3 p[3];
+  triangle_world(sv, ln.instance, ln.triangle, p);
+  const float su0 = sqrtf(clampf(us0, 0.0f, 1.0f));
+  const float u1 = clampf(us1, 0.0f, 1.0f);
+  const float b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
+  const F3 lp = interp(b0, b1, b2, p[0], p[1], p[2]);
+  const float area = 0.5f * sqrtf(len2(cross(sub(p[1], p[0]), sub(p[2], p[0]))));
+  const float pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
+  if (pdf_area <= 0.0f) return false;
+  const F3 to_light = sub(lp, vtx.p);
+  const float d2 = len2(to_light);
+  if (d2 <= 0.0f) return false;
+  const float dist = sqrtf(d2);
+  const F3 wi = divs(to_light, dist);
+  const TptMaterial& lm = sv.mats[sv.inst[ln.instance].material];
+  if (!mat_may_emit(lm)) return false;
+  const Vtx lv = shading_vertex(sv, ln.instance, ln.triangle, b0, b1, b2, wi);
+  F3 le;
+  if (!mat_le(lm, lv, &le)) return false;
+  const float cos_light = fmaxf(dot(lv.ng, neg(wi)), 0.0f);
+  if (cos_light <= 0.0f) return false;
+  const float li_pdf = pdf_area * d2 / cos_light;
+  const float selection = 1.0f * lpmf;
+  if (li_pdf <= 0.0f || selection <= 0.0f) return false;
+  float bsdf_pdf;
+  const F3 f = lambert_eval(m, vtx, wi, &bsdf_pdf);
+  if (len2(f) == 0.0f) return false;
+  const float cos_surface = fabsf(dot(vtx.ng, wi));
+  if (cos_surface <= 0.0f) return false;
+  const float pdf_total = selection * li_pdf;
+  if (pdf_total <= 0.0f) return false;
+  const float mis = balance_heuristic(pdf_total, bsdf_pdf);
+  ns->origin = offset_origin(vtx.p, vtx.ng, wi);
+  ns->dir = wi;
+  ns->value = muls(hadamard(le, f), cos_surface * mis / pdf_total);
+  ns->inst = ln.instance;
+  ns->tri = ln.triangle;
+  return true;
+}
+
+// emitted_radiance_from_bsdf_sample_area (R/integrator/mis.rs:212-234) with
+// pdf_for_triangle_hit (R/light/mod.rs:407-429) and the area-light solid-angle
+// pdf (R/scene/mod.rs:587-611).  (p, ns): the previous shading vertex's
+// position and shading normal (its light-tree query), m: its material.
+__device__ __forceinline__ F3 emitted(const ShadeView& sv, F3 thr, float bsdf_pdf, bool delta, const TptMaterial& m,
+                                      F3 p, F3 ns, const Vtx& lv, const TptMaterial& lm) {
+  F3 le;
+  if (!mat_le(lm, lv, &le)) return f3(0.0f, 0.0f, 0.0f);
+  if (delta) return hadamard(thr, le);
+  float light_pdf = 0.0f;
+  if (sv.light_count != 0 && m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {
+    const float rho = luminance(rgb3(m.u.lambert.rho));
+    const uint32_t base = sv.inst[lv.inst].light_base;
+    const uint32_t leaf = base == TPT_INVALID_INDEX ? TPT_INVALID_INDEX : sv.tri_leaf[base + lv.tri];
+    const float lpmf = rho > 0.0f ? leaf_pmf(sv, p, ns, rho, leaf) : 0.0f;
+    if (lpmf > 0.0f) {
+      const float area = sv.lights[leaf].area;
+      const F3 to_light = sub(lv.p, p);
+      const float d2 = len2(to_light);
+      if (area > 0.0f && d2 > 0.0f) {
+        const float pdf_area = 1.0f / area;
+        const float dist = sqrtf(d2);
+        const F3 wi = divs(to_light, dist);
+        const float cos_light = fmaxf(dot(lv.ng, neg(wi)), 0.0f);
+        if (cos_light > 0.0f) light_pdf = 1.0f * lpmf * (pdf_area * d2 / cos_light);
+      }
+    }
+  }
+  return muls(hadamard(thr, le), balance_heuristic(bsdf_pdf, light_pdf));
+}
+
+// Pinhole camera: PinholeCamera::generate_ray_differential's direction for a
+// tent-filtered pixel sample (R/scene/camera.rs:48-87, R/math/mod.rs:13-24).
+struct CamView {
+  F3 eye, fwd, right, up;
+  float tan_half, exposure;
+};
+__device__ __forceinline__ F3 camera_dir(const CamView& cam, const smp::PathSampler& ps, uint32_t x, uint32_t y,
+                                         uint32_t w, uint32_t h) {
+  float u[2];
+  ps.s2(smp::kCameraDim, u);
+  const float rx = (float)w, ry = (float)h, aspect = rx / ry;
+  const float px = ((float)x + 0.5f) + 0.5f * smp::tent(u[0]);
+  const float py = ((float)y + 0.5f) + 0.5f * smp::tent(u[1]);
+  const float sx = ((2.0f * px / rx) - 1.0f) * aspect * cam.tan_half;
+  const float sy = (1.0f - (2.0f * py / ry)) * cam.tan_half;
+  return normalize(add(add(cam.fwd, scl(sx, cam.right)), scl(sy, cam.up)));
+}
+
+}  // namespace pt
+}  // namespace tpt

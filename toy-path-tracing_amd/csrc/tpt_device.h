@@ -64,6 +64,11 @@ struct TptDevice {
   // allows keep more of each wave's node and triangle lines in L2.
   uint32_t refill_below = 24;         // refill a wave when fewer lanes are busy
   uint32_t blocks_per_cu = 4;         // resident blocks per CU (0: occupancy limit)
+  // wavefront path tracer: paths in flight per pass, graph capture of a pass,
+  // and its buffers (grow-only)
+  uint32_t wf_paths = 1u << 21;
+  uint32_t wf_graph = 1;
+  tpt::DeviceBuffer wf_buf;
 };
 
 // A scene resident in device memory (tpt_scene_create).
@@ -101,6 +106,12 @@ struct TptScene {
 };
 
 namespace tpt {
+bool launch_trace_queue(TptScene* s, int query, bool q8, bool exact, const TptRay* rays, const uint32_t* n_dev,
+                        uint32_t n_max, TptHit* hits, const uint32_t* target, uint8_t* out, hipStream_t st);
+// true if every instance's material is one the path tracers implement;
+// otherwise records "<fn>: ..." and returns false.
+bool render_materials_ok(TptScene* s, const char* fn);
+
 inline SceneView view_of(const TptScene* s) {
   SceneView v;
   v.nodes = s->nodes;

@@ -48,6 +48,7 @@ struct Query {
   uint32_t* counts;
   uint32_t* status;
   WorkCounters* work;
+  const uint32_t* n_dev;  // when set, the ray count is read here (device-resident queue length)
   uint32_t n;
   uint32_t refill_below;  // refill when fewer lanes than this are busy
 };
@@ -150,7 +151,8 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView sc, const TptRa
   Lane L;
   Counters cnt = {0, 0, 0};
   const uint32_t lane = lane_id();
-  const uint32_t chunk = (q.n + kRegions - 1) / kRegions;
+  const uint32_t n = q.n_dev ? *q.n_dev : q.n;
+  const uint32_t chunk = (n + kRegions - 1) / kRegions;
   int region = (int)(blockIdx.x % kRegions);
   int regions_left = kRegions;
   bool active = false;
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView sc, const TptRa
       const uint64_t idle = ~busy;
       const uint32_t want = (uint32_t)__popcll(idle);
       const uint32_t lo = (uint32_t)region * chunk;
-      const uint32_t hi = min(q.n, lo + chunk);
+      const uint32_t hi = min(n, lo + chunk);
       uint32_t base = 0;
       if (lane == 0) base = atomicAdd(&q.work->next[region], want);
       base = __shfl(base, 0) + lo;
@@ -365,6 +367,10 @@ int32_t tpt_device_set_param(TptDevice* d, const char* name, int64_t value) {
     d->refill_below = (uint32_t)value;
   } else if (n == "blocks_per_cu" && value >= 0 && value <= 64) {
     d->blocks_per_cu = (uint32_t)value;
+  } else if (n == "wf_paths" && value >= 65536 && value <= (1 << 26)) {
+    d->wf_paths = (uint32_t)value;
+  } else if (n == "wf_graph" && (value == 0 || value == 1)) {
+    d->wf_graph = (uint32_t)value;
   } else {
     set_error("tpt_device_set_param: unknown parameter or value out of range: " + n);
     return TPT_ERROR_BAD_ARGUMENT;
@@ -780,6 +786,7 @@ int32_t trace_impl(TptScene* s, int query, const TptRay* rays, size_t n, const u
   q.counts = dcounts;
   q.status = d->status;
   q.work = d->work;
+  q.n_dev = nullptr;
   q.n = (uint32_t)n;
   q.refill_below = d->refill_below;
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
@@ -810,6 +817,32 @@ int32_t trace_impl(TptScene* s, int query, const TptRay* rays, size_t n, const u
 }
 
 }  // namespace
+
+// Wavefront queues: a persistent trace kernel whose ray count is read on the
+// device, so a whole bounce loop can be enqueued (or captured in a graph)
+// without a host round trip.  The grid is sized for n_max rays.
+bool launch_trace_queue(TptScene* s, int query, bool q8, bool exact, const TptRay* rays, const uint32_t* n_dev,
+                        uint32_t n_max, TptHit* hits, const uint32_t* target, uint8_t* out, hipStream_t st) {
+  TptDevice* d = s->dev;
+  const SceneView v = view_of(s);
+  Query q;
+  q.hits = hits;
+  q.target = target;
+  q.distance = nullptr;
+  q.out = out;
+  q.counts = nullptr;
+  q.status = d->status;
+  q.work = d->work;
+  q.n_dev = n_dev;
+  q.n = n_max;
+  q.refill_below = d->refill_below;
+  const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
+  const void* kern = pick_kernel(query, q8, exact, false, two);
+  const unsigned blocks = grid_blocks(d, kern, n_max);
+  void* args[] = {(void*)&v, (void*)&rays, (void*)&q};
+  return hip_ok(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, 0, st), "hipLaunchKernel(trace queue)");
+}
+
 }  // namespace tpt
 
 extern "C" {

@@ -1,0 +1,521 @@
+// Wavefront path tracer: the estimator of render.hip (R/main.rs:98-126 with
+// R/integrator/mis.rs:19-139) reorganised into small kernels over queues of
+// paths, for many frames of different resolution in one call.
+//
+// A pass traces samples [k*S, (k+1)*S) of every pixel of every frame: P =
+// pixels * S paths with their state in structure-of-arrays buffers.  Per
+// bounce b:
+//   trace     closest hit of the ray queue (the persistent trace kernel;
+//             queue length read on the device)
+//   shade     emission at the hit (MIS against the previous vertex), then for
+//             b < max_depth: light-tree NEE -> shadow queue, material sample,
+//             Russian roulette -> next ray queue (wave-compacted appends)
+//   shadow    the shadow queue through the trace kernel's exact rule
+//             (closest hit, then the target test: unoccluded_ray)
+//   add       unoccluded NEE contributions into the path radiance
+// then `fold` adds the pass's samples to each pixel's running mean in sample
+// order (color += (s - color) / n, R/main.rs:116-117).  The arithmetic per
+// path is the megakernel's, in the same order, so the image is the
+// megakernel's bit for bit; only the schedule differs: lanes are refilled
+// with live paths at every bounce and each kernel keeps few registers.
+//
+// Every launch of a pass reads its counts from device memory, so the pass is
+// captured once as a HIP graph and replayed for every pass (no host round
+// trip between bounces).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "shading.h"
+#include "tpt_device.h"
+
+namespace tpt {
+namespace wf {
+
+struct FrameDev {
+  pt::CamView cam;
+  uint32_t w, h, spp, max_depth;
+  smp::ZSobolFrame zf;
+  uint32_t pix_begin;  // first global pixel of this frame
+  uint32_t pad;
+  float* out;          // device rgb, w * h * 3
+};
+
+struct Counters {
+  uint32_t rays[2];  // ping-pong ray-queue lengths
+  uint32_t shadow;   // shadow-queue length
+  uint32_t pass;     // current pass
+  uint32_t pad[4];
+};
+
+struct State {
+  float* thr;       // 3 per path
+  float* rad;       // 3 per path
+  float* prev_p;    // 3 per path: previous shading vertex position
+  float* prev_ns;   // 3 per path: its shading normal
+  float* prev_pdf;  // BSDF pdf of the sample that left it
+  uint32_t* prev_info;  // material index | delta flag (bit 31)
+};
+
+struct Queue {
+  TptRay* rays;
+  uint32_t* path;
+};
+
+struct ShadowQ {
+  TptRay* rays;
+  uint32_t* target;  // (instance, triangle) per entry
+  float* value;      // 3 per entry: throughput x unoccluded contribution
+  uint32_t* path;
+  uint8_t* unocc;
+};
+
+struct Pass {
+  const FrameDev* frames;
+  uint32_t frame_count;
+  uint32_t spp_slots;     // S: samples per pixel per pass
+  uint32_t total_pixels;  // over all frames
+  uint32_t paths;         // total_pixels * S
+};
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Wave-compacted append: every lane of the wave calls it (converged); lanes
+// with want get consecutive slots.  Returns the slot (valid where want).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return 0;
+  const uint32_t lane = lane_id();
+  uint32_t base = 0;
+  const int leader = __ffsll((long long)m) - 1;
+  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  return base + rank;
+}
+
+__device__ __forceinline__ uint32_t frame_of(const Pass& ps, uint32_t gp) {
+  uint32_t f = 0;
+  while (f + 1 < ps.frame_count && gp >= ps.frames[f + 1].pix_begin) ++f;
+  return f;
+}
+
+__device__ __forceinline__ void store3(float* a, uint32_t i, F3 v) {
+  a[3 * i] = v.x;
+  a[3 * i + 1] = v.y;
+  a[3 * i + 2] = v.z;
+}
+__device__ __forceinline__ F3 load3f(const float* a, uint32_t i) { return f3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+
+__device__ __forceinline__ void put_ray(TptRay* r, uint32_t slot, F3 o, F3 d) {
+  TptRay v;
+  v.o[0] = o.x;
+  v.o[1] = o.y;
+  v.o[2] = o.z;
+  v.tmax = __builtin_inff();
+  v.d[0] = d.x;
+  v.d[1] = d.y;
+  v.d[2] = d.z;
+  v.pad = 0.0f;
+  r[slot] = v;
+}
+
+// Resets one ray-queue length and the shadow-queue length.
+__global__ void wf_reset(Counters* c, uint32_t which) {
+  if (threadIdx.x == 0) {
+    c->rays[which] = 0;
+    c->shadow = 0;
+  }
+}
+
+__global__ void wf_next_pass(Counters* c) {
+  if (threadIdx.x == 0) c->pass += 1;
+}
+
+// Camera rays of the pass's samples (PathSampler camera dimension,
+// PinholeCamera::generate_ray_differential), state init, queue 0 append.
+__global__ __launch_bounds__(256) void wf_generate(Pass ps, State st, Queue q, Counters* c) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t k = c->pass;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < ps.paths; base += stride) {
+    const uint32_t path = base + threadIdx.x;
+    bool want = false;
+    F3 o = f3(0.0f, 0.0f, 0.0f), d = o;
+    if (path < ps.paths) {
+      const uint32_t gp = path / ps.spp_slots, s = path % ps.spp_slots;
+      const FrameDev& fr = ps.frames[frame_of(ps, gp)];
+      const uint32_t si = k * ps.spp_slots + s;
+      store3(st.rad, path, f3(0.0f, 0.0f, 0.0f));
+      if (si < fr.spp) {
+        const uint32_t lp = gp - fr.pix_begin, x = lp % fr.w, y = lp / fr.w;
+        const smp::PathSampler smpl{fr.zf, x, y, si};
+        o = fr.cam.eye;
+        d = pt::camera_dir(fr.cam, smpl, x, y, fr.w, fr.h);
+        store3(st.thr, path, f3(1.0f, 1.0f, 1.0f));
+        want = true;
+      }
+    }
+    const uint32_t slot = wave_append(&c->rays[0], want);
+    if (want) {
+      put_ray(q.rays, slot, o, d);
+      q.path[slot] = path;
+    }
+  }
+}
+
+// One bounce of mis::trace_radiance for every path of the input queue.
+__global__ __launch_bounds__(256) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
+                                                const TptHit* __restrict__ hits, Queue qout, ShadowQ sq,
+                                                Counters* c, uint32_t in_idx, uint32_t bounce) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t n = c->rays[in_idx];
+  const uint32_t k = c->pass;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint32_t slot = base + threadIdx.x;
+    bool want_ray = false, want_shadow = false;
+    uint32_t path = 0;
+    F3 no = f3(0.0f, 0.0f, 0.0f), nd = no;
+    pt::NeeSample ns;
+    F3 nee_value = no;
+    if (slot < n) {
+      path = qin.path[slot];
+      const TptHit h = hits[slot];
+      if (h.instance != TPT_INVALID_INDEX) {  // a miss ends the path (no environment light)
+        const uint32_t gp = path / ps.spp_slots, s = path % ps.spp_slots;
+        const FrameDev& fr = ps.frames[frame_of(ps, gp)];
+        const uint32_t lp = gp - fr.pix_begin;
+        const smp::PathSampler smpl{fr.zf, lp % fr.w, lp / fr.w, k * ps.spp_slots + s};
+        const TptRay r = qin.rays[slot];
+        const F3 dir = f3(r.d[0], r.d[1], r.d[2]);
+        const pt::Vtx vtx = pt::shading_vertex(sv, h.instance, h.triangle, h.b[0], h.b[1], h.b[2], dir);
+        const TptMaterial& m = sv.mats[vtx.material];
+        F3 rad = load3f(st.rad, path);
+        F3 thr = load3f(st.thr, path);
+        if (bounce == 0) {
+          F3 le;
+          if (pt::mat_le(m, vtx, &le)) rad = pt::add(rad, le);
+        } else {
+          const uint32_t info = st.prev_info[path];
+          rad = pt::add(rad, pt::emitted(sv, thr, st.prev_pdf[path], (info >> 31) != 0, sv.mats[info & 0x7FFFFFFFu],
+                                         load3f(st.prev_p, path), load3f(st.prev_ns, path), vtx, m));
+        }
+        if (bounce < fr.max_depth) {
+          const uint32_t bd = smp::kVertexBase + bounce * smp::kVertexStride;
+          if (m.kind != TPT_MATERIAL_EMISSIVE) {
+            float us[2];
+            smpl.s2(bd + 3, us);
+            if (pt::nee_sample(sv, m, vtx, smpl.s1(bd + 1), us[0], us[1], &ns)) {
+              want_shadow = true;
+              nee_value = pt::hadamard(thr, ns.value);
+            }
+          }
+          pt::Sample smp_;
+          F3 wi;
+          float ud[2];
+          smpl.s2(bd + 7, ud);
+          if (pt::mat_sample(m, vtx, smpl.s1(bd + 5), ud[0], ud[1], &smp_, &wi)) {
+            thr = pt::hadamard(thr, smp_.weight);
+            bool live = true;
+            if (bounce + 1 >= 4) {
+              const float qs = pt::clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.05f, 1.0f);
+              if (smpl.s1(bd + 13) > qs)
+                live = false;
+              else
+                thr = pt::divs(thr, qs);
+            }
+            if (live) {
+              want_ray = true;
+              no = pt::offset_origin(vtx.p, vtx.ng, wi);
+              nd = wi;
+              store3(st.thr, path, thr);
+              store3(st.prev_p, path, vtx.p);
+              store3(st.prev_ns, path, vtx.ns);
+              st.prev_pdf[path] = smp_.pdf;
+              st.prev_info[path] = vtx.material | (smp_.delta ? 0x80000000u : 0u);
+            }
+          }
+        }
+        store3(st.rad, path, rad);
+      }
+    }
+    const uint32_t rs = wave_append(&c->rays[in_idx ^ 1u], want_ray);
+    if (want_ray) {
+      put_ray(qout.rays, rs, no, nd);
+      qout.path[rs] = path;
+    }
+    const uint32_t ss = wave_append(&c->shadow, want_shadow);
+    if (want_shadow) {
+      put_ray(sq.rays, ss, ns.origin, ns.dir);
+      sq.target[2 * ss] = ns.inst;
+      sq.target[2 * ss + 1] = ns.tri;
+      store3(sq.value, ss, nee_value);
+      sq.path[ss] = path;
+    }
+  }
+}
+
+// Unoccluded NEE contributions into the path radiance (one per path per bounce).
+__global__ __launch_bounds__(256) void wf_shadow_add(State st, ShadowQ sq, const Counters* c) {
+  const uint32_t n = c->shadow;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (!sq.unocc[i]) continue;
+    const uint32_t path = sq.path[i];
+    store3(st.rad, path, pt::add(load3f(st.rad, path), load3f(sq.value, i)));
+  }
+}
+
+// Running mean over the pass's samples, in sample order.
+__global__ __launch_bounds__(256) void wf_fold(Pass ps, State st, float* __restrict__ color, const Counters* c) {
+  const uint32_t k = c->pass;
+  for (uint32_t gp = blockIdx.x * blockDim.x + threadIdx.x; gp < ps.total_pixels; gp += gridDim.x * blockDim.x) {
+    const FrameDev& fr = ps.frames[frame_of(ps, gp)];
+    F3 col = load3f(color, gp);
+    for (uint32_t s = 0; s < ps.spp_slots; ++s) {
+      const uint32_t si = k * ps.spp_slots + s;
+      if (si >= fr.spp) break;
+      const F3 v = load3f(st.rad, gp * ps.spp_slots + s);
+      col = pt::add(col, pt::divs(pt::sub(v, col), (float)(si + 1)));
+    }
+    store3(color, gp, col);
+  }
+}
+
+__global__ __launch_bounds__(256) void wf_output(Pass ps, const float* __restrict__ color) {
+  for (uint32_t gp = blockIdx.x * blockDim.x + threadIdx.x; gp < ps.total_pixels; gp += gridDim.x * blockDim.x) {
+    const FrameDev& fr = ps.frames[frame_of(ps, gp)];
+    const uint32_t lp = gp - fr.pix_begin;
+    fr.out[3 * lp] = color[3 * gp] * fr.cam.exposure;
+    fr.out[3 * lp + 1] = color[3 * gp + 1] * fr.cam.exposure;
+    fr.out[3 * lp + 2] = color[3 * gp + 2] * fr.cam.exposure;
+  }
+}
+
+__global__ void wf_zero(float* a, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0.0f;
+}
+__global__ void wf_zero_u32(uint32_t* a) {
+  if (threadIdx.x == 0) a[0] = 0u;
+}
+__global__ void wf_zero_counters(Counters* c) {
+  if (threadIdx.x < sizeof(Counters) / 4) reinterpret_cast<uint32_t*>(c)[threadIdx.x] = 0u;
+}
+
+}  // namespace wf
+}  // namespace tpt
+
+using namespace tpt;
+
+namespace {
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+template <typename K>
+unsigned resident_blocks(TptDevice* d, K kern, size_t work) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 256, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const size_t cap = (size_t)per_cu * (size_t)(d->cu_count > 0 ? d->cu_count : 1);
+  const size_t want = (work + 255) / 256;
+  return (unsigned)std::max<size_t>(1, std::min(cap, want));
+}
+
+struct Plan {
+  wf::Pass ps;
+  wf::State st;
+  wf::Queue q[2];
+  wf::ShadowQ sq;
+  TptHit* hits;
+  wf::Counters* c;
+  float* color;
+  uint32_t max_depth;
+  bool q8;
+};
+
+// The kernels of one pass, all reading their counts on the device.
+bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream_t st) {
+  TptDevice* d = s->dev;
+  const unsigned g_gen = resident_blocks(d, wf::wf_generate, p.ps.paths);
+  const unsigned g_shade = resident_blocks(d, wf::wf_shade, p.ps.paths);
+  const unsigned g_add = resident_blocks(d, wf::wf_shadow_add, p.ps.paths);
+  const unsigned g_fold = resident_blocks(d, wf::wf_fold, p.ps.total_pixels);
+  hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, 0u);
+  hipLaunchKernelGGL(wf::wf_generate, dim3(g_gen), dim3(256), 0, st, p.ps, p.st, p.q[0], p.c);
+  for (uint32_t b = 0; b <= p.max_depth; ++b) {
+    const uint32_t a = b & 1u;
+    if (!launch_trace_queue(s, 0, p.q8, false, p.q[a].rays, &p.c->rays[a], p.ps.paths, p.hits, nullptr, nullptr, st))
+      return false;
+    hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, a ^ 1u);
+    hipLaunchKernelGGL(wf::wf_shade, dim3(g_shade), dim3(256), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
+                       p.q[a ^ 1u], p.sq, p.c, a, b);
+    if (b < p.max_depth && sv.light_count) {
+      if (!launch_trace_queue(s, 1, p.q8, true, p.sq.rays, &p.c->shadow, p.ps.paths, nullptr, p.sq.target,
+                              p.sq.unocc, st))
+        return false;
+      hipLaunchKernelGGL(wf::wf_shadow_add, dim3(g_add), dim3(256), 0, st, p.st, p.sq, (const wf::Counters*)p.c);
+    }
+  }
+  hipLaunchKernelGGL(wf::wf_fold, dim3(g_fold), dim3(256), 0, st, p.ps, p.st, p.color, (const wf::Counters*)p.c);
+  hipLaunchKernelGGL(wf::wf_next_pass, dim3(1), dim3(64), 0, st, p.c);
+  return hip_ok(hipGetLastError(), "wavefront launch");
+}
+
+}  // namespace
+
+extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags) {
+  try {
+    if (!s || (frame_count && !frames)) {
+      set_error("tpt_render_batch: null argument");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    uint64_t total = 0;
+    uint32_t max_spp = 0, max_depth = 0;
+    for (uint32_t f = 0; f < frame_count; ++f) {
+      const TptRenderDesc& r = frames[f].render;
+      if (!frames[f].rgb || r.samples_per_pixel == 0) {
+        set_error("tpt_render_batch: frame " + std::to_string(f) + ": null rgb or samples_per_pixel == 0");
+        return TPT_ERROR_BAD_ARGUMENT;
+      }
+      total += (uint64_t)r.width * r.height;
+      max_spp = std::max(max_spp, r.samples_per_pixel);
+      max_depth = std::max(max_depth, r.max_depth);
+    }
+    if (total == 0) return TPT_OK;
+    if (total > 0x3FFFFFFFull) {
+      set_error("tpt_render_batch: too many pixels for one call");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    if (!render_materials_ok(s, "tpt_render_batch")) return TPT_ERROR_BAD_ARGUMENT;
+    TptDevice* d = s->dev;
+    if (!hip_ok(hipSetDevice(d->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    const hipStream_t st = d->stream;
+    const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
+    // S samples per pixel per pass, P = pixels * S paths in flight
+    const uint64_t cap = std::max<uint64_t>(d->wf_paths, total);
+    const uint32_t S = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_spp, cap / total));
+    const uint64_t P = total * S;
+    const uint32_t passes = (max_spp + S - 1) / S;
+    // one allocation: counters | frames | color | staged outputs | path state | queues | hits | shadow queue
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      const size_t o = off;
+      off += align256(bytes);
+      return o;
+    };
+    const size_t o_cnt = take(sizeof(wf::Counters));
+    const size_t o_frames = take(frame_count * sizeof(wf::FrameDev));
+    const size_t o_color = take(12 * total);
+    const size_t o_out = dev_ptr ? 0 : take(12 * total);
+    const size_t o_thr = take(12 * P), o_rad = take(12 * P), o_pp = take(12 * P), o_pn = take(12 * P);
+    const size_t o_pdf = take(4 * P), o_info = take(4 * P);
+    const size_t o_q0r = take(sizeof(TptRay) * P), o_q0p = take(4 * P);
+    const size_t o_q1r = take(sizeof(TptRay) * P), o_q1p = take(4 * P);
+    const size_t o_hits = take(sizeof(TptHit) * P);
+    const size_t o_sr = take(sizeof(TptRay) * P), o_st = take(8 * P), o_sv = take(12 * P), o_sp = take(4 * P);
+    const size_t o_su = take(P);
+    if (!d->wf_buf.ensure(off)) return TPT_ERROR_DEVICE;
+    char* base = (char*)d->wf_buf.ptr;
+    // frame table
+    std::vector<wf::FrameDev> fd(frame_count);
+    uint32_t pix = 0;
+    size_t out_off = 0;
+    for (uint32_t f = 0; f < frame_count; ++f) {
+      const TptRenderDesc& r = frames[f].render;
+      wf::FrameDev& x = fd[f];
+      const TptCamera& c = r.camera;
+      x.cam.eye = F3{c.eye[0], c.eye[1], c.eye[2]};
+      x.cam.fwd = F3{c.forward[0], c.forward[1], c.forward[2]};
+      x.cam.right = F3{c.right[0], c.right[1], c.right[2]};
+      x.cam.up = F3{c.image_up[0], c.image_up[1], c.image_up[2]};
+      x.cam.tan_half = c.tan_half_fov_y;
+      x.cam.exposure = c.exposure;
+      x.w = r.width;
+      x.h = r.height;
+      x.spp = r.samples_per_pixel;
+      x.max_depth = r.max_depth;
+      x.zf = smp::zsobol_frame(r.samples_per_pixel, r.width, r.height);
+      x.pix_begin = pix;
+      x.pad = 0;
+      x.out = dev_ptr ? frames[f].rgb : (float*)(base + o_out + out_off);
+      pix += r.width * r.height;
+      out_off += 12 * (size_t)r.width * r.height;
+    }
+    Plan p;
+    p.ps.frames = (const wf::FrameDev*)(base + o_frames);
+    p.ps.frame_count = frame_count;
+    p.ps.spp_slots = S;
+    p.ps.total_pixels = (uint32_t)total;
+    p.ps.paths = (uint32_t)P;
+    p.st = wf::State{(float*)(base + o_thr), (float*)(base + o_rad), (float*)(base + o_pp), (float*)(base + o_pn),
+                     (float*)(base + o_pdf), (uint32_t*)(base + o_info)};
+    p.q[0] = wf::Queue{(TptRay*)(base + o_q0r), (uint32_t*)(base + o_q0p)};
+    p.q[1] = wf::Queue{(TptRay*)(base + o_q1r), (uint32_t*)(base + o_q1p)};
+    p.sq = wf::ShadowQ{(TptRay*)(base + o_sr), (uint32_t*)(base + o_st), (float*)(base + o_sv),
+                       (uint32_t*)(base + o_sp), (uint8_t*)(base + o_su)};
+    p.hits = (TptHit*)(base + o_hits);
+    p.c = (wf::Counters*)(base + o_cnt);
+    p.color = (float*)(base + o_color);
+    p.max_depth = max_depth;
+    p.q8 = flags & TPT_FLAG_QUANTIZED;
+    const pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats, s->lights, s->tri_leaf, s->light_count};
+    if (!hip_ok(hipMemcpyAsync((void*)p.ps.frames, fd.data(), fd.size() * sizeof(wf::FrameDev), hipMemcpyHostToDevice,
+                               st),
+                "hipMemcpyAsync(frames)"))
+      return TPT_ERROR_DEVICE;
+    hipLaunchKernelGGL(wf::wf_zero_counters, dim3(1), dim3(64), 0, st, p.c);
+    hipLaunchKernelGGL(wf::wf_zero, dim3(resident_blocks(d, wf::wf_zero, 3 * total)), dim3(256), 0, st, p.color,
+                       (size_t)(3 * total));
+    if (!dev_ptr) hipLaunchKernelGGL(wf::wf_zero_u32, dim3(1), dim3(64), 0, st, d->status);
+    bool graphed = false;
+    if (d->wf_graph && passes > 1) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const bool ok = enqueue_pass(s, p, sv, st);
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (ok && e == hipSuccess && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+          graphed = true;
+          for (uint32_t k = 0; k < passes && graphed; ++k) graphed = hipGraphLaunch(exec, st) == hipSuccess;
+          if (!graphed) {
+            set_error("tpt_render_batch: hipGraphLaunch failed");
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            return TPT_ERROR_DEVICE;
+          }
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+      }
+      (void)hipGetLastError();  // a refused capture leaves an error behind; the plain launches follow
+    }
+    if (!graphed)
+      for (uint32_t k = 0; k < passes; ++k)
+        if (!enqueue_pass(s, p, sv, st)) return TPT_ERROR_DEVICE;
+    hipLaunchKernelGGL(wf::wf_output, dim3(resident_blocks(d, wf::wf_output, total)), dim3(256), 0, st, p.ps,
+                       (const float*)p.color);
+    if (!hip_ok(hipGetLastError(), "wf_output launch")) return TPT_ERROR_DEVICE;
+    if (!dev_ptr) {
+      uint32_t status = 0;
+      for (uint32_t f = 0; f < frame_count; ++f)
+        if (!hip_ok(hipMemcpyAsync(frames[f].rgb, fd[f].out, 12 * (size_t)fd[f].w * fd[f].h, hipMemcpyDeviceToHost, st),
+                    "hipMemcpyAsync(rgb)"))
+          return TPT_ERROR_DEVICE;
+      if (!hip_ok(hipMemcpyAsync(&status, d->status, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(status)") ||
+          !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+        return TPT_ERROR_DEVICE;
+      if (status & 1u) {
+        set_error("tpt_render_batch: traversal stack overflow (128 entries)");
+        return TPT_ERROR_OVERFLOW;
+      }
+    }
+    return TPT_OK;
+  } catch (const std::exception& e) {
+    set_error(std::string("tpt_render_batch: ") + e.what());
+    return TPT_ERROR;
+  }
+}

@@ -120,6 +120,14 @@ class RenderDesc(ctypes.Structure):
 assert ctypes.sizeof(RenderDesc) == 80
 
 
+class FrameDesc(ctypes.Structure):
+    """TptFrameDesc: one frame of tpt_render_batch."""
+    _fields_ = [("render", RenderDesc), ("rgb", ctypes.c_void_p)]
+
+
+assert ctypes.sizeof(FrameDesc) == 88
+
+
 class TptError(RuntimeError):
     """A non-zero status from libtpt, carrying tpt_last_error()."""
 
@@ -161,6 +169,7 @@ _SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_render", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_render_batch", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("tpt_trace_counts", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_uint32]),
