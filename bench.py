@@ -279,6 +279,24 @@ def render_bench(dev, stream, args):
             "image_mean": [round(float(x), 4) for x in img.reshape(-1, 3).mean(0)],
         }
         log(f"render {key}: {ms:.2f} ms/frame = {res['samples_per_s'] / 1e6:.1f} Msamples/s")
+        # wavefront path tracer on the same frame: FP32 and 8-bit nodes
+        wf_img = torch.empty_like(img_dev)
+        for q8 in (False, True):
+            rds.render_batch_device([(cam, w, h, spp, depth)], [wf_img], quantized=q8)
+            torch.cuda.synchronize()
+            a.record(stream)
+            for _ in range(reps):
+                rds.render_batch_device([(cam, w, h, spp, depth)], [wf_img], quantized=q8)
+            b.record(stream)
+            torch.cuda.synchronize()
+            wms = a.elapsed_time(b) / reps
+            tag = "wavefront_q8" if q8 else "wavefront"
+            res[tag] = {"ms_per_frame": round(wms, 3), "samples_per_s": round(w * h * spp / (wms / 1e3), 1),
+                        "speedup_vs_megakernel": round(ms / wms, 3)}
+            if not q8:
+                res[tag]["identical_to_megakernel"] = bool(torch.equal(wf_img, img_dev))
+            log(f"render {key} {tag}: {wms:.2f} ms/frame = {w * h * spp / (wms / 1e3) / 1e6:.1f} Msamples/s "
+                f"({ms / wms:.2f}x megakernel)")
         if args.cpu_sample > 0:
             sys.path.insert(0, str(ROOT / "tests"))
             import oracle  # noqa: E402  (test infrastructure: CPU baseline leg only)

@@ -326,9 +326,13 @@ typedef struct TptFrameDesc {
  * run as a wavefront path tracer: all frames' paths share one set of
  * launches (per bounce: a ray-queue trace, a shade kernel with light-tree
  * NEE appending to shadow and next-ray queues, a shadow-queue trace), so
- * frames of uneven resolution fill the GPU together.  Each frame's image is
- * the one tpt_render gives for it, bit for bit, with FP32 nodes
- * (TPT_FLAG_QUANTIZED selects 8-bit nodes for every trace).  Tuned by
+ * frames of uneven resolution fill the GPU together.  Shadow rays use the
+ * early-exit any-hit query (the light sample's own t bounds it); with
+ * TPT_FLAG_ANY_EXACT they use the reference's closest-hit rule and each
+ * frame's image is the one tpt_render gives for it, bit for bit, with FP32
+ * nodes (the two rules differ only when another triangle is hit at exactly
+ * the light sample's t).  TPT_FLAG_QUANTIZED selects 8-bit nodes for every
+ * trace.  Tuned by
  * tpt_device_set_param "wf_paths" (paths in flight) and "wf_graph" (capture
  * a pass as a HIP graph).  No reference counterpart beyond the pixel loop of
  * R/main.rs:98-126, which it runs for many frames at once. */

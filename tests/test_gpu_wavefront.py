@@ -1,7 +1,9 @@
 """The wavefront path tracer (tpt_render_batch) against the megakernel
 (tpt_render): the same estimator, sample sequence and arithmetic per path, so
-every frame of a batch is the megakernel's image bit for bit with FP32
-nodes, whatever the batch composition, paths in flight or graph setting."""
+with the exact shadow rule every frame of a batch is the megakernel's image
+bit for bit with FP32 nodes, whatever the batch composition, paths in flight
+or graph setting; the default early-exit shadow rule differs only on exact
+t ties."""
 import numpy as np
 import pytest
 
@@ -28,9 +30,13 @@ def test_single_frame_matches_megakernel(scene):
     w, h, spp, depth = 40, 30, 16, 8
     cam = camera.pinhole(**cs.camera, width=w, height=h)
     mega = ds.render(cam, w, h, spp, depth)
-    (wave,) = ds.render_batch([(cam, w, h, spp, depth)])
+    (wave,) = ds.render_batch([(cam, w, h, spp, depth)], exact_shadow=True)
     print("identical pixels", np.mean(np.all(mega == wave, axis=2)))
     assert bits_equal(mega, wave)
+    (fast,) = ds.render_batch([(cam, w, h, spp, depth)])
+    same = np.mean(np.all(mega == fast, axis=2))
+    print("early-exit shadows: identical pixels", same)
+    assert same > 0.999 and abs(fast.mean() - mega.mean()) <= 1e-4 * mega.mean()
 
 
 def test_uneven_batch_matches_per_frame_renders(scene):
@@ -38,7 +44,7 @@ def test_uneven_batch_matches_per_frame_renders(scene):
     frames = []
     for (w, h, spp, depth) in [(48, 32, 8, 8), (17, 29, 5, 6), (64, 64, 3, 8), (9, 7, 33, 4)]:
         frames.append((camera.pinhole(**cs.camera, width=w, height=h), w, h, spp, depth))
-    outs = ds.render_batch(frames)
+    outs = ds.render_batch(frames, exact_shadow=True)
     for f, img in zip(frames, outs):
         assert bits_equal(img, ds.render(*f))
 
@@ -52,7 +58,7 @@ def test_paths_in_flight_and_graph_do_not_change_the_image(device, scene, paths,
     device.set_param("wf_paths", paths)
     device.set_param("wf_graph", graph)
     try:
-        (img,) = ds.render_batch([(cam, w, h, 12, 8)])
+        (img,) = ds.render_batch([(cam, w, h, 12, 8)], exact_shadow=True)
     finally:
         device.set_param("wf_paths", 1 << 21)
         device.set_param("wf_graph", 1)

@@ -776,10 +776,19 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
   const F3 wi = divs(to_light, dist);
   const TptMaterial& lm = sv.mats[sv.inst[ln.instance].material];
   if (!mat_may_emit(lm)) return false;
-  const Vtx lv = shading_vertex(sv, ln.instance, ln.triangle, b0, b1, b2, wi);
-  F3 le;
-  if (!mat_le(lm, lv, &le)) return false;
-  const float cos_light = fmaxf(dot(lv.ng, neg(wi)), 0.0f);
+  // light vertex (shading_vertex_from_triangle_sample): an Emissive light
+  // needs only its face-forwarded geometric normal, from the same world
+  // positions; other emitters need the full vertex for le
+  F3 le, lng;
+  if (lm.kind == TPT_MATERIAL_EMISSIVE) {
+    lng = face_forward(normalize_or_zero(cross(sub(p[1], p[0]), sub(p[2], p[0]))), neg(wi));
+    le = hadamard(muls(rgb3(lm.u.emissive.color), lm.u.emissive.strength), f3(1.0f, 1.0f, 1.0f));
+  } else {
+    const Vtx lv = shading_vertex(sv, ln.instance, ln.triangle, b0, b1, b2, wi);
+    if (!mat_le(lm, lv, &le)) return false;
+    lng = lv.ng;
+  }
+  const float cos_light = fmaxf(dot(lng, neg(wi)), 0.0f);
   if (cos_light <= 0.0f) return false;
   const float li_pdf = pdf_area * d2 / cos_light;
   const float selection = 1.0f * lpmf;

@@ -54,6 +54,10 @@ static_assert(sizeof(GpuNodeQ8) == 64, "GpuNodeQ8 must be 64 bytes");
 
 // Leaf-ordered triangle record: the three vertex positions of
 // primitive_indices[k] and that triangle index.  48 bytes, 40 read.
+// Leaf-ordered triangle record: vertex positions and the triangle id, whose
+// bit 31 flags a zero-area triangle (never hit, R/math/ray.rs:114-116).
+constexpr uint32_t kTriIdMask = 0x7FFFFFFFu;
+constexpr uint32_t kTriDegenerate = 0x80000000u;
 struct alignas(16) GpuTri {
   float v[9];
   uint32_t prim;

@@ -91,7 +91,8 @@ __device__ __forceinline__ void begin_ray(const SceneView& sc, const TptRay* ray
       to_local(in, o, d, &lr);
       const GpuTri* tp = sc.tris + in.tri_base + sc.tri_rank[in.tri_base + tt];
       TriResult h;
-      if (intersect_triangle(lr.o, lr.d, __builtin_inff(), f3(tp->v[0], tp->v[1], tp->v[2]),
+      if (!(tp->prim & kTriDegenerate) &&
+          intersect_triangle(lr, __builtin_inff(), f3(tp->v[0], tp->v[1], tp->v[2]),
                              f3(tp->v[3], tp->v[4], tp->v[5]), f3(tp->v[6], tp->v[7], tp->v[8]), &h))
         bound = h.t;
     }
@@ -518,7 +519,12 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
           g.v[3 * v + 1] = md.positions[3 * vi + 1];
           g.v[3 * v + 2] = md.positions[3 * vi + 2];
         }
-        g.prim = ti;
+        {  // (v2 - v0) x (v1 - v0) has zero length: the triangle is never hit
+          const float ax = g.v[6] - g.v[0], ay = g.v[7] - g.v[1], az = g.v[8] - g.v[2];
+          const float bx = g.v[3] - g.v[0], by = g.v[4] - g.v[1], bz = g.v[5] - g.v[2];
+          const float cx = ay * bz - by * az, cy = az * bx - bz * ax, cz = ax * by - bx * ay;
+          g.prim = ti | ((cx * cx + cy * cy + cz * cz == 0.0f) ? kTriDegenerate : 0u);
+        }
         rank[tb + ti] = k;
       }
     }

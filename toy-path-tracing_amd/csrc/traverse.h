@@ -56,41 +56,61 @@ struct TriResult {
   float t, b0, b1, b2;
 };
 
-// intersect_triangle, R/math/ray.rs:107-209.  Same operation order as the
-// reference, including the fp64 recomputation of the edge functions.
-__device__ __forceinline__ bool intersect_triangle(F3 ro, F3 rd, float t_max, F3 v0, F3 v1, F3 v2,
-                                                   TriResult* out) {
-  {
-    F3 a = v2 - v0, b = v1 - v0;
-    F3 c = f3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
-    if (c.x * c.x + c.y * c.y + c.z * c.z == 0.0f) return false;
-  }
-  F3 p0 = v0 - ro, p1 = v1 - ro, p2 = v2 - ro;
-  // max_component_index(|d|), then kx = kz+1, ky = kx+1 (mod 3)
-  const float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
+// A ray in the space of the mesh being traversed: origin, direction, the
+// slab test's reciprocal direction, and the ray-only part of the watertight
+// triangle test (R/math/ray.rs:120-144): the axis kz of the largest |d|
+// component and the shear constants sx = -d.x/d.z, sy = -d.y/d.z, sz = 1/d.z
+// of the permuted direction.  They are the same values the reference
+// recomputes for every triangle, evaluated once per ray (kz = 3: d.z == 0,
+// no triangle is hit).
+struct Ray3 {
+  F3 o, d, inv;
+  int kz;
+  float sx, sy, sz;
+};
+
+__device__ __forceinline__ void tri_setup(Ray3* r) {
+  const float ax = fabsf(r->d.x), ay = fabsf(r->d.y), az = fabsf(r->d.z);
   const int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-  float dx, dy, dz;
+  const float dx = kz == 0 ? r->d.y : (kz == 1 ? r->d.z : r->d.x);
+  const float dy = kz == 0 ? r->d.z : (kz == 1 ? r->d.x : r->d.y);
+  const float dz = kz == 0 ? r->d.x : (kz == 1 ? r->d.y : r->d.z);
+  if (dz == 0.0f) {
+    r->kz = 3;
+    r->sx = r->sy = r->sz = 0.0f;
+    return;
+  }
+  r->kz = kz;
+  r->sx = -dx / dz;
+  r->sy = -dy / dz;
+  r->sz = 1.0f / dz;
+}
+
+// intersect_triangle, R/math/ray.rs:107-209.  Same operation order as the
+// reference, including the fp64 recomputation of the edge functions.  The
+// zero-area test ((v2 - v0) x (v1 - v0) has length 0, :114-116) depends on
+// the triangle only and is precomputed into its record (bit 31 of the
+// triangle id); the caller skips such triangles.
+__device__ __forceinline__ bool intersect_triangle(const Ray3& r, float t_max, F3 v0, F3 v1, F3 v2, TriResult* out) {
+  if (r.kz == 3) return false;
+  const F3 ro = r.o;
+  const int kz = r.kz;
+  F3 p0 = v0 - ro, p1 = v1 - ro, p2 = v2 - ro;
   float p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
   if (kz == 0) {  // (kx, ky, kz) = (1, 2, 0)
-    dx = rd.y; dy = rd.z; dz = rd.x;
     p0x = p0.y; p0y = p0.z; p0z = p0.x;
     p1x = p1.y; p1y = p1.z; p1z = p1.x;
     p2x = p2.y; p2y = p2.z; p2z = p2.x;
   } else if (kz == 1) {  // (2, 0, 1)
-    dx = rd.z; dy = rd.x; dz = rd.y;
     p0x = p0.z; p0y = p0.x; p0z = p0.y;
     p1x = p1.z; p1y = p1.x; p1z = p1.y;
     p2x = p2.z; p2y = p2.x; p2z = p2.y;
   } else {  // (0, 1, 2)
-    dx = rd.x; dy = rd.y; dz = rd.z;
     p0x = p0.x; p0y = p0.y; p0z = p0.z;
     p1x = p1.x; p1y = p1.y; p1z = p1.z;
     p2x = p2.x; p2y = p2.y; p2z = p2.z;
   }
-  if (dz == 0.0f) return false;
-  const float sx = -dx / dz;
-  const float sy = -dy / dz;
-  const float sz = 1.0f / dz;
+  const float sx = r.sx, sy = r.sy, sz = r.sz;
   p0x += sx * p0z;
   p0y += sy * p0z;
   p1x += sx * p1z;
@@ -151,9 +171,6 @@ struct SceneView {
   uint32_t tlas_node_count;
 };
 
-struct Ray3 {
-  F3 o, d, inv;
-};
 
 __device__ __forceinline__ float safe_inv(float d) { return d == 0.0f ? __builtin_inff() : 1.0f / d; }
 
@@ -170,6 +187,7 @@ __device__ __forceinline__ void to_local(const GpuInstance& in, F3 o, F3 d, Ray3
   rx = m[6] * d.z + rx; ry = m[7] * d.z + ry; rz = m[8] * d.z + rz;
   r->d = f3(rx, ry, rz);
   r->inv = f3(safe_inv(rx), safe_inv(ry), safe_inv(rz));
+  tri_setup(r);
 }
 
 __device__ __forceinline__ float q8_scale(int e) { return __int_as_float((e + 127) << 23); }
@@ -478,14 +496,15 @@ __device__ __forceinline__ bool lane_step(const SceneView& sc, Lane& L, LaneStac
     const float4 a = reinterpret_cast<const float4*>(tp)[0];
     const float4 b = reinterpret_cast<const float4*>(tp)[1];
     const float2 c = reinterpret_cast<const float2*>(tp)[4];
-    const uint32_t prim = __float_as_uint(c.y);
+    const uint32_t prim = __float_as_uint(c.y) & kTriIdMask;
+    const bool degenerate = (__float_as_uint(c.y) & kTriDegenerate) != 0;
     ++L.tri_cur;
     const bool skip = (kMode == kAny) && L.cur_inst == L.skip_inst && prim == L.skip_tri;
     if (!skip) {
       if (kCount) cnt.tris++;
       TriResult h;
-      if (intersect_triangle(L.r.o, L.r.d, L.closest, f3(a.x, a.y, a.z), f3(a.w, b.x, b.y), f3(b.z, b.w, c.x),
-                             &h) &&
+      if (!degenerate &&
+          intersect_triangle(L.r, L.closest, f3(a.x, a.y, a.z), f3(a.w, b.x, b.y), f3(b.z, b.w, c.x), &h) &&
           (kMode != kAny || h.t < L.strict)) {
         L.closest = h.t;
         L.hit = true;

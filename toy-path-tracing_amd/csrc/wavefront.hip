@@ -10,14 +10,17 @@
 //   shade     emission at the hit (MIS against the previous vertex), then for
 //             b < max_depth: light-tree NEE -> shadow queue, material sample,
 //             Russian roulette -> next ray queue (wave-compacted appends)
-//   shadow    the shadow queue through the trace kernel's exact rule
+//   shadow    the shadow queue through the any-hit trace kernel: the
+//             target's own t bounds the query and the first other hit ends
+//             it; with TPT_FLAG_ANY_EXACT the reference's rule literally
 //             (closest hit, then the target test: unoccluded_ray)
 //   add       unoccluded NEE contributions into the path radiance
 // then `fold` adds the pass's samples to each pixel's running mean in sample
 // order (color += (s - color) / n, R/main.rs:116-117).  The arithmetic per
-// path is the megakernel's, in the same order, so the image is the
-// megakernel's bit for bit; only the schedule differs: lanes are refilled
-// with live paths at every bounce and each kernel keeps few registers.
+// path is the megakernel's, in the same order, so with the exact shadow rule
+// the image is the megakernel's bit for bit (the early-exit rule differs
+// only when another triangle is hit at exactly the light sample's t); only
+// the schedule differs: lanes are refilled with live paths at every bounce.
 //
 // Every launch of a pass reads its counts from device memory, so the pass is
 // captured once as a HIP graph and replayed for every pass (no host round
@@ -335,6 +338,7 @@ struct Plan {
   float* color;
   uint32_t max_depth;
   bool q8;
+  bool exact_shadow;
 };
 
 // The kernels of one pass, all reading their counts on the device.
@@ -354,7 +358,7 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
     hipLaunchKernelGGL(wf::wf_shade, dim3(g_shade), dim3(256), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
                        p.q[a ^ 1u], p.sq, p.c, a, b);
     if (b < p.max_depth && sv.light_count) {
-      if (!launch_trace_queue(s, 1, p.q8, true, p.sq.rays, &p.c->shadow, p.ps.paths, nullptr, p.sq.target,
+      if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow, p.ps.paths, nullptr, p.sq.target,
                               p.sq.unocc, st))
         return false;
       hipLaunchKernelGGL(wf::wf_shadow_add, dim3(g_add), dim3(256), 0, st, p.st, p.sq, (const wf::Counters*)p.c);
@@ -462,6 +466,7 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.color = (float*)(base + o_color);
     p.max_depth = max_depth;
     p.q8 = flags & TPT_FLAG_QUANTIZED;
+    p.exact_shadow = flags & TPT_FLAG_ANY_EXACT;
     const pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats, s->lights, s->tri_leaf, s->light_count};
     if (!hip_ok(hipMemcpyAsync((void*)p.ps.frames, fd.data(), fd.size() * sizeof(wf::FrameDev), hipMemcpyHostToDevice,
                                st),

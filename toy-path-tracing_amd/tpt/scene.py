@@ -378,7 +378,7 @@ class DeviceScene:
         d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
         check(lib().tpt_render(self.handle, ctypes.byref(d), ptr(out_dev), _lib.TPT_FLAG_DEVICE_POINTERS))
 
-    def render_batch(self, frames, quantized: bool = False) -> list:
+    def render_batch(self, frames, quantized: bool = False, exact_shadow: bool = False) -> list:
         """Wavefront path tracer over a batch of frames (tpt_render_batch).
         frames: (camera, width, height, spp, max_depth) tuples -> one
         (height, width, 3) float32 image per frame, as render() gives it."""
@@ -389,11 +389,11 @@ class DeviceScene:
             d.camera = cam
             d.width, d.height, d.samples_per_pixel, d.max_depth = w, h, spp, depth
             arr[i].rgb = ptr(outs[i])
-        flags = _lib.TPT_FLAG_QUANTIZED if quantized else 0
+        flags = (_lib.TPT_FLAG_QUANTIZED if quantized else 0) | (_lib.TPT_FLAG_ANY_EXACT if exact_shadow else 0)
         check(lib().tpt_render_batch(self.handle, arr, len(frames), flags))
         return outs
 
-    def render_batch_device(self, frames, outs_dev, quantized: bool = False) -> None:
+    def render_batch_device(self, frames, outs_dev, quantized: bool = False, exact_shadow: bool = False) -> None:
         """As render_batch into device buffers (torch tensors); asynchronous."""
         arr = (_lib.FrameDesc * max(1, len(frames)))()
         for i, (cam, w, h, spp, depth) in enumerate(frames):
@@ -401,7 +401,8 @@ class DeviceScene:
             d.camera = cam
             d.width, d.height, d.samples_per_pixel, d.max_depth = w, h, spp, depth
             arr[i].rgb = ptr(outs_dev[i])
-        flags = _lib.TPT_FLAG_DEVICE_POINTERS | (_lib.TPT_FLAG_QUANTIZED if quantized else 0)
+        flags = (_lib.TPT_FLAG_DEVICE_POINTERS | (_lib.TPT_FLAG_QUANTIZED if quantized else 0)
+                 | (_lib.TPT_FLAG_ANY_EXACT if exact_shadow else 0))
         check(lib().tpt_render_batch(self.handle, arr, len(frames), flags))
 
     # device-pointer queries (asynchronous on the device stream) -----------------
