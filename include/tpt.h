@@ -43,9 +43,12 @@ extern "C" {
  * nodes.  Child boxes are rounded outward, so the set of candidate
  * triangles only grows; the closest hit is unchanged except on exact ties. */
 #define TPT_FLAG_QUANTIZED 0x2u
-/* tpt_trace_any only: answer with a full closest-hit query followed by the
- * reference's decision rule (bit-exact with unoccluded_ray, no early exit). */
+/* tpt_trace_any and the wavefront's shadow rays: answer with a full
+ * closest-hit query followed by the reference's decision rule (bit-exact
+ * with unoccluded_ray, no early exit). */
 #define TPT_FLAG_ANY_EXACT 0x4u
+/* tpt_render_adjoint: accumulate the per-block gradient sums in FP64. */
+#define TPT_FLAG_GRAD_FP64 0x8u
 
 /* ---- records ------------------------------------------------------------ */
 /* Ray: origin, t_max, direction (R/math/ray.rs:5-11 without differentials).
@@ -313,6 +316,25 @@ typedef struct TptRenderDesc {
  * balance heuristic); an OpenPBR shading point gets no light-tree sample
  * (R/light_tree/importance.rs:26-36, R/material/open_pbr.rs:592-606). */
 int32_t tpt_render(TptScene* scene, const TptRenderDesc* desc, float* rgb, uint32_t flags);
+
+/* Gradient of an image loss with respect to every NormalizedLambert
+ * material's albedo (rho), by path-replay backpropagation: each pixel
+ * sample's path is traced twice from its sampler state (primal radiance,
+ * then a replay that attributes the radiance collected from each Lambert
+ * vertex onward to that vertex's albedo); no path is stored.  Russian
+ * roulette probabilities are held fixed.  dloss_dimage: width * height * 3
+ * floats (dLoss/d of the tpt_render image, exposure included); d_albedo:
+ * 3 * material_count floats (zero for other materials).  Sums are FP32
+ * per block (LDS), then FP64 over blocks; TPT_FLAG_GRAD_FP64 makes the
+ * per-block sums FP64.  No reference counterpart (the reference renderer
+ * is not differentiable). */
+int32_t tpt_render_adjoint(TptScene* scene, const TptRenderDesc* desc, const float* dloss_dimage, float* d_albedo,
+                           uint32_t flags);
+
+/* Replaces the scene's material table (same count; kinds and emission must
+ * stay the same, as the light tree is built from them): the parameter update
+ * of an inverse-rendering loop.  Synchronous. */
+int32_t tpt_scene_set_materials(TptScene* scene, const TptMaterial* materials, uint32_t count);
 
 /* One frame (view) of a batched render: its own camera, resolution, samples
  * per pixel and depth, and its output (width * height * 3 floats; host or

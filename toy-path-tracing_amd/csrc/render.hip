@@ -23,6 +23,7 @@
 // 201-218), so for them NEE runs.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <string>
 
 #include "sampler.h"
@@ -68,6 +69,77 @@ __device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView&
   return ns.value;
 }
 
+// Path-replay adjoint state of one sample: the path's radiance from the
+// primal pass, dLoss/dL of this sample (dLoss/dI_pixel * exposure / spp), and
+// the block's gradient accumulators (3 per material, LDS).
+template <typename Acc>
+struct Adjoint {
+  F3 total;
+  F3 w;
+  Acc* grad;
+};
+
+// mis::trace_radiance (R/integrator/mis.rs:19-139) for one camera ray.  With
+// kAdj, the replay pass of path-replay backpropagation: at every
+// NormalizedLambert vertex, every contribution collected from its NEE
+// onward carries the vertex's albedo exactly once (NEE's f = rho / pi, the
+// BSDF sample's weight = rho; the light-tree pmf, MIS weights and pdfs do not
+// depend on rho), so dL/drho_c += (L_total - L_before)_c / rho_c.  Russian
+// roulette probabilities are held fixed (detached).
+template <bool kTwoLevel, bool kAdj, typename Acc>
+__device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& sv, F3 eye, F3 dir,
+                                         const smp::PathSampler& ps, uint32_t max_depth, Lane& L, LaneStack& st,
+                                         bool& overflow, const Adjoint<Acc>& adj) {
+  F3 radiance = f3(0.0f, 0.0f, 0.0f), thr = f3(1.0f, 1.0f, 1.0f);
+  uint32_t hi, ht;
+  float tt, hb0, hb1, hb2;
+  if (!closest<kTwoLevel>(sc, eye, dir, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) {
+    overflow |= st.overflow;
+    return radiance;  // no environment light
+  }
+  overflow |= st.overflow;
+  Vtx vtx = shading_vertex(sv, hi, ht, hb0, hb1, hb2, dir);
+  F3 le;
+  if (mat_le(sv.mats[vtx.material], vtx, &le)) radiance = add(radiance, le);
+  for (uint32_t depth = 0; depth < max_depth; ++depth) {
+    const uint32_t base = smp::kVertexBase + depth * smp::kVertexStride;
+    const TptMaterial& m = sv.mats[vtx.material];
+    if (kAdj && m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {
+      const F3 rest = sub(adj.total, radiance);
+      Acc* g = adj.grad + 3 * vtx.material;
+      const float* rho = m.u.lambert.rho;
+      if (rho[0] != 0.0f) atomicAdd(&g[0], (Acc)(adj.w.x * rest.x / rho[0]));
+      if (rho[1] != 0.0f) atomicAdd(&g[1], (Acc)(adj.w.y * rest.y / rho[1]));
+      if (rho[2] != 0.0f) atomicAdd(&g[2], (Acc)(adj.w.z * rest.z / rho[2]));
+    }
+    if (m.kind != TPT_MATERIAL_EMISSIVE) {  // should_sample_direct_light: not a pure emitter
+      float us[2];
+      ps.s2(base + 3, us);
+      const F3 dl = direct_light<kTwoLevel>(sc, sv, m, vtx, ps.s1(base + 1), us[0], us[1], L, st);
+      overflow |= st.overflow;
+      radiance = add(radiance, hadamard(thr, dl));
+    }
+    Sample s;
+    F3 wi;
+    float ud[2];
+    ps.s2(base + 7, ud);
+    if (!mat_sample(m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi)) break;
+    thr = hadamard(thr, s.weight);
+    if (depth + 1 >= 4) {
+      const float q = clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.05f, 1.0f);
+      if (ps.s1(base + 13) > q) break;
+      thr = divs(thr, q);
+    }
+    const F3 no = offset_origin(vtx.p, vtx.ng, wi);
+    if (!closest<kTwoLevel>(sc, no, wi, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) break;  // no environment
+    overflow |= st.overflow;
+    const Vtx nv = shading_vertex(sv, hi, ht, hb0, hb1, hb2, wi);
+    radiance = add(radiance, emitted(sv, thr, s.pdf, s.delta, m, vtx.p, vtx.ns, nv, sv.mats[nv.material]));
+    vtx = nv;
+  }
+  return radiance;
+}
+
 template <bool kTwoLevel>
 __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
                                                         uint32_t h, uint32_t spp, uint32_t max_depth,
@@ -85,50 +157,12 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView 
   bool overflow = false;
   const uint32_t x = pix % w, y = pix / w;
   const smp::ZSobolFrame zf = smp::zsobol_frame(spp, w, h);
+  const Adjoint<float> none{};
   F3 color = f3(0.0f, 0.0f, 0.0f);
   for (uint32_t si = 0; si < spp; ++si) {
     const smp::PathSampler ps{zf, x, y, si};
     const F3 dir = camera_dir(cam, ps, x, y, w, h);
-    // mis::trace_radiance
-    F3 radiance = f3(0.0f, 0.0f, 0.0f), thr = f3(1.0f, 1.0f, 1.0f);
-    uint32_t hi, ht;
-    float tt, hb0, hb1, hb2;
-    if (closest<kTwoLevel>(sc, cam.eye, dir, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) {
-      overflow |= st.overflow;
-      Vtx vtx = shading_vertex(sv, hi, ht, hb0, hb1, hb2, dir);
-      F3 le;
-      if (mat_le(sv.mats[vtx.material], vtx, &le)) radiance = add(radiance, le);
-      for (uint32_t depth = 0; depth < max_depth; ++depth) {
-        const uint32_t base = smp::kVertexBase + depth * smp::kVertexStride;
-        const TptMaterial& m = sv.mats[vtx.material];
-        if (m.kind != TPT_MATERIAL_EMISSIVE) {  // should_sample_direct_light: not a pure emitter
-          float us[2];
-          ps.s2(base + 3, us);
-          const F3 dl = direct_light<kTwoLevel>(sc, sv, m, vtx, ps.s1(base + 1), us[0], us[1], L, st);
-          overflow |= st.overflow;
-          radiance = add(radiance, hadamard(thr, dl));
-        }
-        Sample s;
-        F3 wi;
-        float ud[2];
-        ps.s2(base + 7, ud);
-        if (!mat_sample(m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi)) break;
-        thr = hadamard(thr, s.weight);
-        if (depth + 1 >= 4) {
-          const float q = clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.05f, 1.0f);
-          if (ps.s1(base + 13) > q) break;
-          thr = divs(thr, q);
-        }
-        const F3 no = offset_origin(vtx.p, vtx.ng, wi);
-        if (!closest<kTwoLevel>(sc, no, wi, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) break;  // no environment
-        overflow |= st.overflow;
-        const Vtx nv = shading_vertex(sv, hi, ht, hb0, hb1, hb2, wi);
-        radiance = add(radiance, emitted(sv, thr, s.pdf, s.delta, m, vtx.p, vtx.ns, nv, sv.mats[nv.material]));
-        vtx = nv;
-      }
-    } else {
-      overflow |= st.overflow;
-    }
+    const F3 radiance = trace_path<kTwoLevel, false>(sc, sv, cam.eye, dir, ps, max_depth, L, st, overflow, none);
     const float cnt = (float)(si + 1);
     color = add(color, divs(sub(radiance, color), cnt));
   }
@@ -137,6 +171,59 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView 
   o[1] = color.y * cam.exposure;
   o[2] = color.z * cam.exposure;
   if (overflow) atomicOr(status, 1u);
+}
+
+// Path-replay adjoint of the render loop: per pixel sample a primal pass
+// (its radiance) and a replay pass that accumulates dLoss/d(albedo) into
+// the block's LDS sums; each block writes its sums to `partials`.
+template <bool kTwoLevel, typename Acc>
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
+                                                         uint32_t h, uint32_t spp, uint32_t max_depth,
+                                                         const float* __restrict__ dimg, Acc* __restrict__ partials,
+                                                         uint32_t n_grad, uint32_t* __restrict__ status) {
+  __shared__ uint32_t lds[kLdsDepth * kBlock];
+  extern __shared__ __align__(16) unsigned char dyn[];
+  Acc* grad = reinterpret_cast<Acc*>(dyn);
+  for (uint32_t i = threadIdx.x; i < n_grad; i += kBlock) grad[i] = (Acc)0;
+  __syncthreads();
+  uint32_t spill[kStackDepth - kLdsDepth];
+  const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+  if (pix < w * h) {
+    LaneStack st;
+    st.lds = lds + threadIdx.x;
+    st.spill = spill;
+    st.sp = 0;
+    st.overflow = false;
+    Lane L;
+    bool overflow = false;
+    const uint32_t x = pix % w, y = pix / w;
+    const smp::ZSobolFrame zf = smp::zsobol_frame(spp, w, h);
+    const float scale = cam.exposure / (float)spp;  // I = exposure * mean over samples
+    Adjoint<Acc> adj;
+    adj.w = f3(dimg[3 * pix] * scale, dimg[3 * pix + 1] * scale, dimg[3 * pix + 2] * scale);
+    adj.grad = grad;
+    const Adjoint<Acc> none{};
+    for (uint32_t si = 0; si < spp; ++si) {
+      const smp::PathSampler ps{zf, x, y, si};
+      const F3 dir = camera_dir(cam, ps, x, y, w, h);
+      adj.total = trace_path<kTwoLevel, false>(sc, sv, cam.eye, dir, ps, max_depth, L, st, overflow, none);
+      (void)trace_path<kTwoLevel, true>(sc, sv, cam.eye, dir, ps, max_depth, L, st, overflow, adj);
+    }
+    if (overflow) atomicOr(status, 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n_grad; i += kBlock) partials[(size_t)blockIdx.x * n_grad + i] = grad[i];
+}
+
+// Sums the per-block partial gradients in FP64.
+template <typename Acc>
+__global__ void adjoint_reduce(const Acc* __restrict__ partials, uint32_t blocks, uint32_t n_grad,
+                               float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_grad) return;
+  double s = 0.0;
+  for (uint32_t b = 0; b < blocks; ++b) s += (double)partials[(size_t)b * n_grad + i];
+  out[i] = (float)s;
 }
 
 }  // namespace pt
@@ -254,6 +341,146 @@ extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb
     return TPT_OK;
   } catch (const std::exception& e) {
     set_error(e.what());
+    return TPT_ERROR;
+  }
+}
+
+namespace {
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+// The emission parameters of a material (what the light tree is built from).
+bool same_emission(const TptMaterial& a, const TptMaterial& b) {
+  if (a.kind != b.kind) return false;
+  if (a.kind == TPT_MATERIAL_EMISSIVE)
+    return std::memcmp(&a.u.emissive, &b.u.emissive, sizeof(a.u.emissive)) == 0;
+  if (a.kind == TPT_MATERIAL_OPEN_PBR)
+    return a.u.open_pbr.emission_luminance == b.u.open_pbr.emission_luminance &&
+           std::memcmp(a.u.open_pbr.emission_color, b.u.open_pbr.emission_color, 12) == 0;
+  return true;
+}
+
+pt::CamView cam_view(const TptCamera& c) {
+  pt::CamView cam;
+  cam.eye = F3{c.eye[0], c.eye[1], c.eye[2]};
+  cam.fwd = F3{c.forward[0], c.forward[1], c.forward[2]};
+  cam.right = F3{c.right[0], c.right[1], c.right[2]};
+  cam.up = F3{c.image_up[0], c.image_up[1], c.image_up[2]};
+  cam.tan_half = c.tan_half_fov_y;
+  cam.exposure = c.exposure;
+  return cam;
+}
+
+template <typename Acc>
+int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, float* d_albedo, bool dev_ptr) {
+  TptDevice* d = s->dev;
+  const uint32_t w = desc->width, h = desc->height;
+  const size_t n = (size_t)w * h;
+  const uint32_t n_grad = 3 * s->material_count;
+  const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+  const size_t nb_part = align_up((size_t)blocks * n_grad * sizeof(Acc));
+  const size_t nb_img = dev_ptr ? 0 : align_up(n * 12);
+  const size_t nb_out = dev_ptr ? 0 : align_up((size_t)n_grad * 4);
+  DeviceBuffer tmp;
+  if (!tmp.ensure(nb_part + nb_img + nb_out)) return TPT_ERROR_DEVICE;
+  Acc* partials = (Acc*)tmp.ptr;
+  const float* ddimg = dimg;
+  float* dgrad = d_albedo;
+  if (!dev_ptr) {
+    ddimg = (const float*)((char*)tmp.ptr + nb_part);
+    dgrad = (float*)((char*)tmp.ptr + nb_part + nb_img);
+    if (!hip_ok(hipMemcpyAsync((void*)ddimg, dimg, n * 12, hipMemcpyHostToDevice, d->stream), "hipMemcpyAsync(dimg)"))
+      return TPT_ERROR_DEVICE;
+    if (!hip_ok(hipMemsetAsync(d->status, 0, 4, d->stream), "hipMemsetAsync(status)")) return TPT_ERROR_DEVICE;
+  }
+  const pt::CamView cam = cam_view(desc->camera);
+  const pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats, s->lights, s->tri_leaf, s->light_count};
+  const SceneView v = view_of(s);
+  const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
+  const size_t lds_bytes = (size_t)n_grad * sizeof(Acc);
+  if (two)
+    hipLaunchKernelGGL((pt::adjoint_kernel<true, Acc>), dim3(blocks), dim3(kBlock), lds_bytes, d->stream, v, sv, cam, w,
+                       h, desc->samples_per_pixel, desc->max_depth, ddimg, partials, n_grad, d->status);
+  else
+    hipLaunchKernelGGL((pt::adjoint_kernel<false, Acc>), dim3(blocks), dim3(kBlock), lds_bytes, d->stream, v, sv, cam,
+                       w, h, desc->samples_per_pixel, desc->max_depth, ddimg, partials, n_grad, d->status);
+  hipLaunchKernelGGL((pt::adjoint_reduce<Acc>), dim3((n_grad + 255) / 256), dim3(256), 0, d->stream,
+                     (const Acc*)partials, blocks, n_grad, dgrad);
+  if (!hip_ok(hipGetLastError(), "adjoint launch")) return TPT_ERROR_DEVICE;
+  // the scratch buffer is released when this call returns: wait for the kernels
+  uint32_t status = 0;
+  if (!dev_ptr &&
+      (!hip_ok(hipMemcpyAsync(d_albedo, dgrad, (size_t)n_grad * 4, hipMemcpyDeviceToHost, d->stream), "hipMemcpyAsync") ||
+       !hip_ok(hipMemcpyAsync(&status, d->status, 4, hipMemcpyDeviceToHost, d->stream), "hipMemcpyAsync")))
+    return TPT_ERROR_DEVICE;
+  if (!hip_ok(hipStreamSynchronize(d->stream), "hipStreamSynchronize")) return TPT_ERROR_DEVICE;
+  if (status & 1u) {
+    set_error("tpt_render_adjoint: traversal stack overflow (128 entries)");
+    return TPT_ERROR_OVERFLOW;
+  }
+  return TPT_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t tpt_render_adjoint(TptScene* s, const TptRenderDesc* desc, const float* dloss_dimage,
+                                      float* d_albedo, uint32_t flags) {
+  try {
+    if (!s || !desc || !dloss_dimage || !d_albedo) {
+      set_error("tpt_render_adjoint: null argument");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    const uint32_t w = desc->width, h = desc->height;
+    if ((uint64_t)w * h > 0x7FFFFFFFull || desc->samples_per_pixel == 0) {
+      set_error("tpt_render_adjoint: frame too large or samples_per_pixel == 0");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    if (s->material_count == 0 || s->material_count > 4096) {
+      set_error("tpt_render_adjoint: needs 1..4096 materials");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    if (!render_materials_ok(s, "tpt_render_adjoint")) return TPT_ERROR_BAD_ARGUMENT;
+    if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    if ((uint64_t)w * h == 0) {
+      if (flags & TPT_FLAG_DEVICE_POINTERS) return hip_ok(hipMemsetAsync(d_albedo, 0, 12 * (size_t)s->material_count, s->dev->stream), "hipMemsetAsync") ? TPT_OK : TPT_ERROR_DEVICE;
+      std::memset(d_albedo, 0, 12 * (size_t)s->material_count);
+      return TPT_OK;
+    }
+    const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
+    return (flags & TPT_FLAG_GRAD_FP64) ? adjoint_impl<double>(s, desc, dloss_dimage, d_albedo, dev_ptr)
+                                        : adjoint_impl<float>(s, desc, dloss_dimage, d_albedo, dev_ptr);
+  } catch (const std::exception& e) {
+    set_error(std::string("tpt_render_adjoint: ") + e.what());
+    return TPT_ERROR;
+  }
+}
+
+extern "C" int32_t tpt_scene_set_materials(TptScene* s, const TptMaterial* materials, uint32_t count) {
+  try {
+    if (!s || (count && !materials)) {
+      set_error("tpt_scene_set_materials: null argument");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    if (count != s->material_count) {
+      set_error("tpt_scene_set_materials: the material count must stay " + std::to_string(s->material_count));
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    for (uint32_t i = 0; i < count; ++i)
+      if (!same_emission(materials[i], s->host_mats[i])) {
+        set_error("tpt_scene_set_materials: material " + std::to_string(i) +
+                  " changes kind or emission (the light tree is built from emission: create a new scene)");
+        return TPT_ERROR_BAD_ARGUMENT;
+      }
+    if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    if (count && !hip_ok(hipMemcpyAsync(s->mats, materials, count * sizeof(TptMaterial), hipMemcpyHostToDevice,
+                                        s->dev->stream),
+                         "hipMemcpyAsync(materials)"))
+      return TPT_ERROR_DEVICE;
+    if (!hip_ok(hipStreamSynchronize(s->dev->stream), "hipStreamSynchronize")) return TPT_ERROR_DEVICE;
+    s->host_mats.assign(materials, materials + count);
+    return TPT_OK;
+  } catch (const std::exception& e) {
+    set_error(std::string("tpt_scene_set_materials: ") + e.what());
     return TPT_ERROR;
   }
 }

@@ -99,6 +99,7 @@ struct TptScene {
   uint32_t light_count = 0;
   uint32_t* tri_leaf = nullptr;
   std::vector<TptLightNode> light_nodes;  // host copy (tpt_scene_light_tree)
+  std::vector<TptMaterial> host_mats;     // host copy of the material table
   size_t bytes = 0;
   // staging for host-pointer calls (grow-only)
   void* stage = nullptr;

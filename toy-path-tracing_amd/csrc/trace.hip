@@ -640,6 +640,7 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
     s->material_count = desc->materials ? desc->material_count : 0;
     s->light_count = (uint32_t)lnodes.size();
     s->light_nodes = std::move(lnodes);
+    s->host_mats = mats;
     if (!upload(nodes, &s->nodes, &s->bytes) || !upload(qnodes, &s->qnodes, &s->bytes) ||
         !upload(tris, &s->tris, &s->bytes) || !upload(rank, &s->tri_rank, &s->bytes) ||
         !upload(inst, &s->inst, &s->bytes) || !upload(tlas, &s->tlas, &s->bytes) ||

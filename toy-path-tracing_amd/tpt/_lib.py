@@ -24,6 +24,7 @@ TPT_ERROR_OVERFLOW = 5
 
 TPT_FLAG_DEVICE_POINTERS = 0x1
 TPT_FLAG_QUANTIZED = 0x2
+TPT_FLAG_GRAD_FP64 = 0x8
 TPT_FLAG_ANY_EXACT = 0x4
 
 INVALID_INDEX = 0xFFFFFFFF
@@ -170,6 +171,9 @@ _SIGNATURES = [
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_render", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_render_batch", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    ("tpt_render_adjoint", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_scene_set_materials", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_trace_counts", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_uint32]),

@@ -1,0 +1,47 @@
+"""torch.autograd binding of the path-replay adjoint (tpt_render_adjoint).
+
+``render_albedo(albedo, scene, ...)`` renders the scene with the
+NormalizedLambert albedos taken from ``albedo`` (one RGB row per material;
+rows of other materials are ignored) and returns the image as a tensor whose
+backward pass runs the adjoint kernel: ``loss(render_albedo(a, ...)).backward()``
+fills ``a.grad`` with dLoss/d(albedo).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .material import NormalizedLambert
+
+
+def with_albedo(materials, albedo: np.ndarray) -> list:
+    """The material list with every NormalizedLambert albedo replaced by the
+    matching row of ``albedo`` (shape (len(materials), 3))."""
+    out = []
+    for i, m in enumerate(materials):
+        out.append(NormalizedLambert(tuple(float(x) for x in albedo[i])) if isinstance(m, NormalizedLambert) else m)
+    return out
+
+
+class AlbedoRender(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, albedo, device_scene, materials, camera, width, height, spp, max_depth):
+        a = albedo.detach().to("cpu", torch.float32).numpy()
+        device_scene.set_materials(with_albedo(materials, a))
+        img = device_scene.render(camera, width, height, spp, max_depth)
+        ctx.call = (device_scene, materials, a, camera, width, height, spp, max_depth)
+        return torch.from_numpy(img).to(albedo.device)
+
+    @staticmethod
+    def backward(ctx, grad_image):
+        device_scene, materials, a, camera, width, height, spp, max_depth = ctx.call
+        device_scene.set_materials(with_albedo(materials, a))  # the forward's parameters
+        g = device_scene.render_adjoint(camera, width, height, spp, max_depth,
+                                        grad_image.detach().to("cpu", torch.float32).numpy())
+        return torch.from_numpy(g).to(grad_image.device), None, None, None, None, None, None, None
+
+
+def render_albedo(albedo: torch.Tensor, device_scene, materials, camera, width: int, height: int, spp: int,
+                  max_depth: int) -> torch.Tensor:
+    """Differentiable render (see the module docstring)."""
+    return AlbedoRender.apply(albedo, device_scene, materials, camera, width, height, spp, max_depth)

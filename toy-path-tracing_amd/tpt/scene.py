@@ -300,6 +300,7 @@ class DeviceScene:
         if not self.handle:
             raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
         self.instance_count = len(scene.instances)
+        self.material_count = len(scene.materials)
 
     @property
     def device_bytes(self) -> int:
@@ -377,6 +378,26 @@ class DeviceScene:
         d.camera = camera
         d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
         check(lib().tpt_render(self.handle, ctypes.byref(d), ptr(out_dev), _lib.TPT_FLAG_DEVICE_POINTERS))
+
+    def set_materials(self, materials) -> None:
+        """tpt_scene_set_materials: a new material table of the same kinds and
+        emission (e.g. updated albedos of an inverse-rendering step)."""
+        from .material import records
+        recs = records(materials)
+        check(lib().tpt_scene_set_materials(self.handle, ctypes.addressof(recs), len(materials)))
+
+    def render_adjoint(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dimage,
+                       fp64: bool = False) -> np.ndarray:
+        """Path-replay adjoint (tpt_render_adjoint): dLoss/d(albedo) of every
+        NormalizedLambert material -> (material_count, 3) float32."""
+        d = _lib.RenderDesc()
+        d.camera = camera
+        d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+        g = np.ascontiguousarray(dloss_dimage, dtype=np.float32).reshape(height, width, 3)
+        out = np.zeros((self.material_count, 3), np.float32)
+        check(lib().tpt_render_adjoint(self.handle, ctypes.byref(d), ptr(g), ptr(out),
+                                       _lib.TPT_FLAG_GRAD_FP64 if fp64 else 0))
+        return out
 
     def render_batch(self, frames, quantized: bool = False, exact_shadow: bool = False) -> list:
         """Wavefront path tracer over a batch of frames (tpt_render_batch).
