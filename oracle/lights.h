@@ -578,5 +578,210 @@ inline float pdf_leaf(const Tree& t, const Query& q, uint32_t leaf_node) {
   return pmf;
 }
 
+
+// ---- environment light (R/light/environment.rs) ----------------------------------------
+namespace env {
+
+constexpr float kOneMinusEps = 1.0f - kFltEps;
+constexpr float kTauF = 6.28318530717958647692f;
+
+struct Level {
+  size_t w, h;
+  std::vector<float> wt;
+};
+struct Dist {  // HierarchicalDistribution: levels[0] is 1x1, last is the padded leaf
+  std::vector<Level> levels;
+  float total = 0.0f;
+  float texel(size_t i, size_t j) const { return levels.back().wt[j * levels.back().w + i]; }
+};
+
+inline size_t next_pow2(size_t v) {
+  size_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+inline float row_sin_theta(size_t j, size_t h) { return std::sin(((float)j + 0.5f) / (float)h * kPiF); }
+inline float positive_luminance(V3 c) { return fmaxr(luminance(c), 0.0f); }
+
+inline Dist build_distribution(size_t w, size_t h, const std::vector<float>& weights) {  // :426-478
+  Dist d;
+  const size_t pw = next_pow2(w), ph = next_pow2(h);
+  Level leaf{pw, ph, std::vector<float>(pw * ph, 0.0f)};
+  for (size_t j = 0; j < h; ++j)
+    for (size_t i = 0; i < w; ++i) leaf.wt[j * pw + i] = fmaxr(weights[j * w + i], 0.0f);
+  std::vector<Level> lv{leaf};
+  while (lv.back().w > 1 || lv.back().h > 1) {
+    const Level& f = lv.back();
+    const size_t si = f.w > 1 ? 2 : 1, sj = f.h > 1 ? 2 : 1;
+    Level c{f.w / si, f.h / sj, {}};
+    c.wt.assign(c.w * c.h, 0.0f);
+    for (size_t j = 0; j < c.h; ++j)
+      for (size_t i = 0; i < c.w; ++i) {
+        float sum = 0.0f;
+        for (size_t dj = 0; dj < sj; ++dj)
+          for (size_t di = 0; di < si; ++di) sum += f.wt[(j * sj + dj) * f.w + i * si + di];
+        c.wt[j * c.w + i] = sum;
+      }
+    lv.push_back(c);
+  }
+  d.levels.assign(lv.rbegin(), lv.rend());
+  d.total = d.levels[0].wt[0];
+  return d;
+}
+
+struct Env {
+  size_t w = 0, h = 0;
+  float scale = 1.0f;
+  float rot[9], inv_rot[9];  // Mat3 columns
+  std::vector<V3> px;
+  Dist dist, dist_mis;
+};
+
+inline void rot_y(float angle, float* m) {  // Mat3::from_rotation_y, columns
+  const float s = std::sin(angle), c = std::cos(angle);
+  const float v[9] = {c, 0.0f, -s, 0.0f, 1.0f, 0.0f, s, 0.0f, c};
+  for (int k = 0; k < 9; ++k) m[k] = v[k];
+}
+inline V3 mat3_mul(const float* m, V3 v) {  // Mat3 * Vec3: (x*c0 + y*c1) + z*c2
+  return add(add(vscale(v3(m[0], m[1], m[2]), v.x), vscale(v3(m[3], m[4], m[5]), v.y)), vscale(v3(m[6], m[7], m[8]), v.z));
+}
+
+inline Env make(size_t w, size_t h, std::vector<V3> px, float scale, float rotate_y) {  // from_pixels (:47-83)
+  Env e;
+  e.w = w;
+  e.h = h;
+  e.scale = scale;
+  e.px = std::move(px);
+  std::vector<float> wt(w * h), wm(w * h);
+  for (size_t j = 0; j < h; ++j) {  // environment_weights (:362-371)
+    const float st = row_sin_theta(j, h);
+    for (size_t i = 0; i < w; ++i) wt[j * w + i] = positive_luminance(e.px[j * w + i]) * st;
+  }
+  float ws = 0.0f, wsum = 0.0f;  // solid_angle_mean_luminance (:393-408)
+  for (size_t j = 0; j < h; ++j) {
+    const float st = row_sin_theta(j, h);
+    wsum += st * (float)w;
+    for (size_t i = 0; i < w; ++i) ws += positive_luminance(e.px[j * w + i]) * st;
+  }
+  const float mean = wsum > 0.0f ? ws / wsum : 0.0f;
+  const float eps = fmaxr(std::fabs(mean), 1.0f) * 1.0e-6f;  // mis_compensated_environment_weights (:373-391)
+  for (size_t j = 0; j < h; ++j) {
+    const float st = row_sin_theta(j, h);
+    for (size_t i = 0; i < w; ++i) {
+      const float c = positive_luminance(e.px[j * w + i]) - mean;
+      wm[j * w + i] = c > eps ? c * st : 0.0f;
+    }
+  }
+  e.dist = build_distribution(w, h, wt);
+  e.dist_mis = build_distribution(w, h, wm);
+  rot_y(rotate_y, e.rot);
+  rot_y(-rotate_y, e.inv_rot);
+  return e;
+}
+
+inline float total_power(const Env& e) {  // :129-132
+  const float texel = kTauF * kPiF / ((float)e.w * (float)e.h);
+  return e.dist.total * e.scale * texel;
+}
+inline size_t pixel_coord(float t, size_t n) {  // :326-330
+  const float c = clampr(t, 0.0f, 1.0f);
+  const float f = c * (float)n;
+  size_t idx = f >= 1.8446744e19f ? ~size_t(0) : (size_t)f;
+  return std::min(idx, n - 1);
+}
+inline V3 uv_to_direction(float u, float v) {  // :332-337
+  const float phi = u * kTauF, theta = v * kPiF;
+  const float st = std::sin(theta);
+  return v3(st * std::sin(phi), std::cos(theta), st * std::cos(phi));
+}
+inline void direction_to_uv(V3 d, float* u, float* v) {  // :339-348
+  const V3 dir = vnormalize_or_zero(d);
+  const float y = clampr(dir.y, -1.0f, 1.0f);
+  const float theta = std::acos(y);
+  float phi = std::atan2(dir.x, dir.z);
+  if (phi < 0.0f) phi += kTauF;
+  *u = clampr(phi / kTauF, 0.0f, 1.0f);
+  *v = clampr(theta / kPiF, 0.0f, 1.0f);
+}
+inline V3 radiance(const Env& e, V3 dir) {  // :134-140
+  float u, v;
+  direction_to_uv(mat3_mul(e.inv_rot, dir), &u, &v);
+  const V3 p = e.px[pixel_coord(v, e.h) * e.w + pixel_coord(u, e.w)];
+  return vscale(p, e.scale);
+}
+inline float pdf(const Env& e, const Dist& d, V3 dir) {  // pdf_with_distribution (:156-177)
+  if (d.total <= 0.0f) return 0.0f;
+  float u, v;
+  direction_to_uv(mat3_mul(e.inv_rot, dir), &u, &v);
+  const float st = std::sin(v * kPiF);
+  if (st <= 0.0f) return 0.0f;
+  const float wgt = d.texel(pixel_coord(u, e.w), pixel_coord(v, e.h));
+  if (wgt <= 0.0f) return 0.0f;
+  const float p2d = wgt / d.total * (float)e.w * (float)e.h;
+  return p2d / (kTauF * kPiF * st);
+}
+// hsw_step (:480-530)
+inline bool hsw_step(const Level& parent, const Level& child, size_t* i, size_t* j, float* u, float* v) {
+  const size_t si = child.w / parent.w, sj = child.h / parent.h;
+  size_t ci = *i * si, cj = *j * sj;
+  if (sj == 2) {
+    float top = 0.0f, bot = 0.0f;
+    for (size_t di = 0; di < si; ++di) {
+      top += child.wt[cj * child.w + ci + di];
+      bot += child.wt[(cj + 1) * child.w + ci + di];
+    }
+    const float total = top + bot;
+    if (total <= 0.0f) return false;
+    const float pt = top / total;
+    if (*v < pt) {
+      *v = clampr(*v / pt, 0.0f, kOneMinusEps);
+    } else {
+      *v = clampr((*v - pt) / (1.0f - pt), 0.0f, kOneMinusEps);
+      cj += 1;
+    }
+  }
+  if (si == 2) {
+    const float l = child.wt[cj * child.w + ci], r = child.wt[cj * child.w + ci + 1];
+    const float total = l + r;
+    if (total <= 0.0f) return false;
+    const float pl = l / total;
+    if (*u < pl) {
+      *u = clampr(*u / pl, 0.0f, kOneMinusEps);
+    } else {
+      *u = clampr((*u - pl) / (1.0f - pl), 0.0f, kOneMinusEps);
+      ci += 1;
+    }
+  }
+  *i = ci;
+  *j = cj;
+  return true;
+}
+struct Sample {
+  V3 dir, radiance;
+  float pdf;
+};
+// sample_with_distribution (:179-234)
+inline bool sample(const Env& e, const Dist& d, float us0, float us1, Sample* out) {
+  if (d.total <= 0.0f) return false;
+  float u = clampr(us0, 0.0f, 1.0f), v = clampr(us1, 0.0f, 1.0f);
+  size_t i = 0, j = 0;
+  for (size_t l = 1; l < d.levels.size(); ++l)
+    if (!hsw_step(d.levels[l - 1], d.levels[l], &i, &j, &u, &v)) return false;
+  if (i >= e.w || j >= e.h) return false;
+  const float wgt = d.texel(i, j);
+  if (wgt <= 0.0f) return false;
+  const float uc = ((float)i + u) / (float)e.w, vc = ((float)j + v) / (float)e.h;
+  const V3 local = uv_to_direction(clampr(uc, 0.0f, kOneMinusEps), clampr(vc, 0.0f, kOneMinusEps));
+  out->dir = mat3_mul(e.rot, local);
+  const float st = std::sin(vc * kPiF);
+  if (st <= 0.0f) return false;
+  const float p2d = wgt / d.total * (float)e.w * (float)e.h;
+  out->pdf = p2d / (kTauF * kPiF * st);
+  out->radiance = vscale(e.px[j * e.w + i], e.scale);
+  return true;
+}
+
+}  // namespace env
+
 }  // namespace lt
 }  // namespace orc

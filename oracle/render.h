@@ -18,10 +18,10 @@
 // albedo and energy-compensation tables do not influence the result), no
 // metal, coat, fuzz, transmission, thin film, thin walls, textures or normal
 // maps; base weight / colour / diffuse roughness and emission are free.
-// Light NEE under the reference policy contributes nothing for OpenPBR
-// materials (R/material/open_pbr.rs:592-606 returns no light-tree query) and
-// there is no environment light, so for them the estimator is BSDF sampling
-// with emission at hits (MIS weight 1, R/light/mod.rs:416-437).
+// Light NEE under the reference policy samples no light-tree light for
+// OpenPBR materials (R/material/open_pbr.rs:592-606 returns no light-tree
+// query); the environment light, when present, is sampled for every
+// material (its branch needs no query, R/light/mod.rs:289-331).
 //
 // The reference's NormalizedLambert, Emissive and Mirror materials
 // (R/material/{normalized_lambert,emissive,mirror}.rs) are carried too:
@@ -530,6 +530,8 @@ struct RenderScene {
   lt::Tree tree;
   std::vector<uint32_t> leaf_base;  // per instance: offset into leaf_of, or lt::kInvalid
   std::vector<uint32_t> leaf_of;
+  bool has_env = false;
+  lt::env::Env env;
   uint32_t leaf_node(uint32_t inst, uint32_t tri) const {
     return leaf_base[inst] == lt::kInvalid ? lt::kInvalid : leaf_of[leaf_base[inst] + tri];
   }
@@ -694,25 +696,43 @@ inline V3 shading_normal_correction(const Vertex& v, V3 wi, V3 f) {
   if (cg <= kGeomCosEps) return v3(0.f, 0.f, 0.f);
   return smul(f, std::fabs(dot(wi, v.ns)) / cg);
 }
-// Material::eval / pdf for the materials that reach next-event estimation:
-// NormalizedLambert (normalized_lambert.rs:160-193).  Mirror and Emissive
-// evaluate to zero; OpenPBR never gets a light sample under the reference
-// policy (no light-tree query, no environment).
+// Material::eval / pdf (R/material/mod.rs:251-370) in radiance mode:
+// NormalizedLambert (normalized_lambert.rs:160-193), OpenPBR
+// (open_pbr.rs:549-590); Mirror and Emissive evaluate to zero.
 inline V3 mat_eval(const Material& m, const Vertex& v, V3 wi) {
-  if (m.kind != kMatLambert) return v3(0.f, 0.f, 0.f);
-  if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return v3(0.f, 0.f, 0.f);
-  V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
-  V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
-  V3 f = (wol.z <= 0.0f || wil.z <= 0.0f) ? v3(0.f, 0.f, 0.f) : sdiv(m.color, kPi);
-  return shading_normal_correction(v, wi, f);
+  const V3 zero = v3(0.f, 0.f, 0.f);
+  if (m.kind == kMatLambert) {
+    if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return zero;
+    V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
+    V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
+    V3 f = (wol.z <= 0.0f || wil.z <= 0.0f) ? zero : sdiv(m.color, kPi);
+    return shading_normal_correction(v, wi, f);
+  }
+  if (m.kind == kMatOpenPbr) {
+    if (dot(v.wo, v.ng) <= 0.0f) return zero;
+    V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
+    V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
+    if (wol.z <= 0.0f) return zero;
+    return shading_normal_correction(v, wi, bsdf_eval(make_bsdf(m.pbr, v.front_face), wol, wil));
+  }
+  return zero;
 }
 inline float mat_pdf(const Material& m, const Vertex& v, V3 wi) {
-  if (m.kind != kMatLambert) return 0.0f;
-  if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return 0.0f;
-  V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
-  V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
-  if (wol.z <= 0.0f || wil.z <= 0.0f) return 0.0f;
-  return std::fmax(wil.z, 0.0f) / kPi;
+  if (m.kind == kMatLambert) {
+    if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return 0.0f;
+    V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
+    V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
+    if (wol.z <= 0.0f || wil.z <= 0.0f) return 0.0f;
+    return std::fmax(wil.z, 0.0f) / kPi;
+  }
+  if (m.kind == kMatOpenPbr) {
+    if (dot(v.wo, v.ng) <= 0.0f) return 0.0f;
+    V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
+    V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
+    if (wol.z <= 0.0f) return 0.0f;
+    return bsdf_pdf(make_bsdf(m.pbr, v.front_face), wol, wil);
+  }
+  return 0.0f;
 }
 // light_tree_precompute: NormalizedLambert (normalized_lambert.rs:201-218)
 // gives a diffuse query; every other material here gives none.
@@ -768,63 +788,119 @@ inline V3 offset_origin(V3 p, V3 ng, V3 dir) {
   return add(p, mul(kRayEpsilon, off));
 }
 
-// LightSampler with the tree as its only category (R/light/mod.rs:136-213):
-// category pmf = root flux / root flux = 1 when the tree has flux.
-inline float tree_category_pmf(const RenderScene& rs) {
-  float w = std::fmax(rs.tree.root_flux(), 0.0f);
-  return w > 0.0f ? w / w : 0.0f;
+// LightSampler (R/light/mod.rs:136-233): the tree category (weight = root
+// flux) and the environment category (weight = total power), a normalised
+// CDF over them.
+struct Category {
+  int kind;  // 0 tree, 1 environment
+  float pmf;
+};
+inline void light_categories(const RenderScene& rs, float* w_tree, float* w_env, float* total) {
+  *w_tree = std::fmax(rs.tree.root_flux(), 0.0f);
+  *w_env = rs.has_env ? std::fmax(lt::env::total_power(rs.env), 0.0f) : 0.0f;
+  *total = 0.0f;
+  if (*w_tree > 0.0f) *total += *w_tree;
+  if (*w_env > 0.0f) *total += *w_env;
+}
+inline bool sample_category(const RenderScene& rs, float u, Category* out) {
+  float wt, we, total;
+  light_categories(rs, &wt, &we, &total);
+  float w[2];
+  int kind[2];
+  int n = 0;
+  if (wt > 0.0f) { w[n] = wt; kind[n++] = 0; }
+  if (we > 0.0f) { w[n] = we; kind[n++] = 1; }
+  if (n == 0 || total <= 0.0f) return false;
+  float cdf[3] = {0.0f, 0.0f, 0.0f};
+  float acc = 0.0f;
+  for (int k = 0; k < n; ++k) {
+    acc += w[k];
+    cdf[k + 1] = acc;
+  }
+  const float inv = 1.0f / acc;
+  for (int k = 0; k <= n; ++k) cdf[k] *= inv;
+  u = clampf(u, 0.0f, 1.0f);
+  int pp = 0;
+  while (pp <= n && cdf[pp] <= u) ++pp;  // partition_point(c <= u)
+  int idx = pp > 0 ? pp - 1 : 0;
+  if (idx > n - 1) idx = n - 1;
+  out->kind = kind[idx];
+  out->pmf = w[idx] / total;
+  return true;
+}
+inline float category_pmf(const RenderScene& rs, int kind) {
+  float wt, we, total;
+  light_categories(rs, &wt, &we, &total);
+  if (total <= 0.0f) return 0.0f;
+  const float w = kind == 0 ? wt : we;
+  return w > 0.0f ? w / total : 0.0f;
 }
 
 // direct_light_mis_contribution (R/integrator/mis.rs:141-210) with
-// sample_light_mis_compensated_lazy (R/light/mod.rs:289-331) and
-// sample_li_for_triangle (R/light/area.rs:12-54).
+// sample_light_mis_compensated_lazy (R/light/mod.rs:289-331),
+// sample_li_for_triangle (R/light/area.rs:12-54) and the environment's
+// MIS-compensated sample (R/light/environment.rs:260-276).
 inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& vtx, float u_root, float u_tree,
                        float u_aux, float us0, float us1) {
-  (void)u_root;  // one category: every u selects the tree
   (void)u_aux;
   const V3 zero = v3(0.f, 0.f, 0.f);
-  const float cat_pmf = tree_category_pmf(rs);
-  if (cat_pmf <= 0.0f) return zero;  // sample_category: empty sampler
-  lt::Query q;
-  if (!mat_query(m, vtx, &q)) return zero;
-  uint32_t leaf;
-  float leaf_pmf;
-  if (!lt::sample_tree(rs.tree, q, u_tree, &leaf, &leaf_pmf)) return zero;
-  const lt::Node& ln = rs.tree.nodes[leaf];
-  // sample_triangle_point (R/scene/mod.rs:616-633)
-  V3 p[3];
-  triangle_world(rs, ln.inst, ln.tri, p);
-  float su0 = std::sqrt(clampf(us0, 0.0f, 1.0f));
-  float u1 = clampf(us1, 0.0f, 1.0f);
-  float b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
-  V3 lp = interp3(b0, b1, b2, p[0], p[1], p[2]);
-  float area = 0.5f * lt::vlen(cross(sub(p[1], p[0]), sub(p[2], p[0])));
-  float pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
-  if (pdf_area <= 0.0f) return zero;
-  V3 to_light = sub(lp, vtx.p);
-  float d2 = len_sq(to_light);
-  if (d2 <= 0.0f) return zero;
-  float dist = std::sqrt(d2);
-  V3 wi = sdiv(to_light, dist);
-  const Material& lm = rs.materials[rs.inst_material[ln.inst]];
-  if (!mat_may_emit(lm)) return zero;
-  Vertex lv = vertex_at(rs, ln.inst, ln.tri, b0, b1, b2, wi);
-  V3 le;
-  if (!mat_le(lm, lv, &le)) return zero;
-  float cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
-  if (cos_light <= 0.0f) return zero;
-  float li_pdf = pdf_area * d2 / cos_light;
-  float selection = cat_pmf * leaf_pmf;
+  Category cat;
+  if (!sample_category(rs, u_root, &cat)) return zero;
+  V3 le, wi;
+  float li_pdf, selection;
+  uint32_t t_inst = lt::kInvalid, t_tri = lt::kInvalid;
+  if (cat.kind == 0) {
+    lt::Query q;
+    if (!mat_query(m, vtx, &q)) return zero;
+    uint32_t leaf;
+    float leaf_pmf;
+    if (!lt::sample_tree(rs.tree, q, u_tree, &leaf, &leaf_pmf)) return zero;
+    const lt::Node& ln = rs.tree.nodes[leaf];
+    // sample_triangle_point (R/scene/mod.rs:616-633)
+    V3 p[3];
+    triangle_world(rs, ln.inst, ln.tri, p);
+    float su0 = std::sqrt(clampf(us0, 0.0f, 1.0f));
+    float u1 = clampf(us1, 0.0f, 1.0f);
+    float b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
+    V3 lp = interp3(b0, b1, b2, p[0], p[1], p[2]);
+    float area = 0.5f * lt::vlen(cross(sub(p[1], p[0]), sub(p[2], p[0])));
+    float pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
+    if (pdf_area <= 0.0f) return zero;
+    V3 to_light = sub(lp, vtx.p);
+    float d2 = len_sq(to_light);
+    if (d2 <= 0.0f) return zero;
+    float dist = std::sqrt(d2);
+    wi = sdiv(to_light, dist);
+    const Material& lm = rs.materials[rs.inst_material[ln.inst]];
+    if (!mat_may_emit(lm)) return zero;
+    Vertex lv = vertex_at(rs, ln.inst, ln.tri, b0, b1, b2, wi);
+    if (!mat_le(lm, lv, &le)) return zero;
+    float cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
+    if (cos_light <= 0.0f) return zero;
+    li_pdf = pdf_area * d2 / cos_light;
+    selection = cat.pmf * leaf_pmf;
+    t_inst = ln.inst;
+    t_tri = ln.tri;
+  } else {
+    lt::env::Sample es;
+    if (!lt::env::sample(rs.env, rs.env.dist_mis, us0, us1, &es)) return zero;
+    if (es.pdf <= 0.0f) return zero;
+    le = es.radiance;
+    wi = es.dir;
+    li_pdf = es.pdf;
+    selection = cat.pmf;
+  }
   if (li_pdf <= 0.0f || selection <= 0.0f) return zero;
   V3 f = mat_eval(m, vtx, wi);
   float bsdf_pdf = len_sq(f) == 0.0f ? 0.0f : mat_pdf(m, vtx, wi);
   if (len_sq(f) == 0.0f) return zero;
   float cos_surface = std::fabs(dot(vtx.ng, wi));
   if (cos_surface <= 0.0f) return zero;
-  // unoccluded_ray (R/integrator/mod.rs:239-265) with the light triangle as target
+  // unoccluded_ray (R/integrator/mod.rs:239-265): a light triangle target, or
+  // an infinite light that any hit blocks
   SceneHit sh;
   if (closest_hit(*rs.geo, offset_origin(vtx.p, vtx.ng, wi), wi, INFINITY, &sh, nullptr) &&
-      !(sh.inst == ln.inst && sh.tri == ln.tri))
+      !(t_inst != lt::kInvalid && sh.inst == t_inst && sh.tri == t_tri))
     return zero;
   float pdf_total = selection * li_pdf;
   if (pdf_total <= 0.0f) return zero;
@@ -841,7 +917,7 @@ inline float light_pdf_for_hit(const RenderScene& rs, const Material& shading_m,
   uint32_t leaf = rs.leaf_node(lv.inst, lv.tri);
   float leaf_pmf = lt::pdf_leaf(rs.tree, q, leaf);
   if (leaf_pmf <= 0.0f) return 0.0f;
-  float cat_pmf = tree_category_pmf(rs);
+  float cat_pmf = category_pmf(rs, 0);
   const float area = rs.tree.nodes[leaf].area;
   if (area <= 0.0f) return 0.0f;
   float pdf_area = 1.0f / area;
@@ -867,11 +943,26 @@ inline V3 emitted_from_bsdf_sample(const RenderScene& rs, V3 throughput, float b
   return smul(vmul(throughput, le), mis);
 }
 
+// infinite_light_le (R/light/environment.rs:305-311)
+inline V3 env_le(const RenderScene& rs, V3 dir) {
+  return rs.has_env ? lt::env::radiance(rs.env, dir) : v3(0.f, 0.f, 0.f);
+}
+// emitted_radiance_from_bsdf_sample_infinite (R/integrator/mis.rs:236-257)
+// with pdf_for_environment_hit (R/light/mod.rs:431-443), MIS-compensated.
+inline V3 emitted_infinite(const RenderScene& rs, V3 throughput, float bsdf_pdf, bool delta, V3 dir) {
+  V3 le = env_le(rs, dir);
+  if (le.x == 0.0f && le.y == 0.0f && le.z == 0.0f) return v3(0.f, 0.f, 0.f);
+  if (delta) return vmul(throughput, le);
+  float cat = category_pmf(rs, 1);
+  float light_pdf = cat <= 0.0f ? 0.0f : cat * lt::env::pdf(rs.env, rs.env.dist_mis, dir);
+  return smul(vmul(throughput, le), balance_heuristic(bsdf_pdf, light_pdf));
+}
+
 // mis::trace_radiance (R/integrator/mis.rs:19-139).
 inline V3 trace_radiance(const RenderScene& rs, V3 o, V3 d, const PathSampler& ps, uint32_t max_depth) {
   V3 radiance = v3(0.f, 0.f, 0.f), throughput = v3(1.f, 1.f, 1.f);
   SceneHit h;
-  if (!closest_hit(*rs.geo, o, d, INFINITY, &h, nullptr)) return v3(0.f, 0.f, 0.f);  // no environment
+  if (!closest_hit(*rs.geo, o, d, INFINITY, &h, nullptr)) return env_le(rs, d);
   Vertex vtx = shading_vertex(rs, h, d);
   const Material* mat = &rs.materials[vtx.material];
   V3 le;
@@ -899,7 +990,10 @@ inline V3 trace_radiance(const RenderScene& rs, V3 o, V3 d, const PathSampler& p
     }
     V3 no = offset_origin(vtx.p, vtx.ng, wi);
     SceneHit nh;
-    if (!closest_hit(*rs.geo, no, wi, INFINITY, &nh, nullptr)) break;  // environment: none
+    if (!closest_hit(*rs.geo, no, wi, INFINITY, &nh, nullptr)) {
+      radiance = add(radiance, emitted_infinite(rs, throughput, s.pdf, delta, wi));
+      break;
+    }
     Vertex nv = shading_vertex(rs, nh, wi);
     const Material* nm = &rs.materials[nv.material];
     radiance = add(radiance, emitted_from_bsdf_sample(rs, throughput, s.pdf, delta, *mat, vtx, nv, *nm));

@@ -429,6 +429,52 @@ void orc_render_set_materials(void* h, const OrcMaterial* m, size_t n) {
   build_lights(*r);
 }
 
+// Environment light from RGB pixels (EnvironmentLight::from_pixels,
+// R/light/environment.rs:47-83).
+void orc_render_set_environment(void* h, const float* px, uint32_t w, uint32_t hh, float scale, float rotate_y) {
+  RenderScene* r = (RenderScene*)h;
+  std::vector<V3> p((size_t)w * hh);
+  for (size_t i = 0; i < p.size(); ++i) p[i] = v3(px[3 * i], px[3 * i + 1], px[3 * i + 2]);
+  r->env = lt::env::make(w, hh, std::move(p), scale, rotate_y);
+  r->has_env = true;
+}
+// Environment queries for tests: pdf (mis 0/1), radiance, sample (returns
+// 0 if none; out: dir[3], radiance[3], pdf), total power.
+float orc_env_pdf(void* h, const float* d, int mis) {
+  const RenderScene& r = *(RenderScene*)h;
+  return lt::env::pdf(r.env, mis ? r.env.dist_mis : r.env.dist, v3(d[0], d[1], d[2]));
+}
+void orc_env_radiance(void* h, const float* d, float* out) {
+  V3 l = lt::env::radiance(((RenderScene*)h)->env, v3(d[0], d[1], d[2]));
+  out[0] = l.x;
+  out[1] = l.y;
+  out[2] = l.z;
+}
+int orc_env_sample(void* h, float u, float v, int mis, float* out) {
+  const RenderScene& r = *(RenderScene*)h;
+  lt::env::Sample s;
+  if (!lt::env::sample(r.env, mis ? r.env.dist_mis : r.env.dist, u, v, &s)) return 0;
+  const float o[7] = {s.dir.x, s.dir.y, s.dir.z, s.radiance.x, s.radiance.y, s.radiance.z, s.pdf};
+  std::memcpy(out, o, sizeof(o));
+  return 1;
+}
+float orc_env_total_power(void* h) { return lt::env::total_power(((RenderScene*)h)->env); }
+void orc_env_uv(const float* d, float* uv) { lt::env::direction_to_uv(v3(d[0], d[1], d[2]), &uv[0], &uv[1]); }
+void orc_env_dir(float u, float v, float* d) {
+  V3 r = lt::env::uv_to_direction(u, v);
+  d[0] = r.x;
+  d[1] = r.y;
+  d[2] = r.z;
+}
+// emitted_radiance_from_bsdf_sample_infinite for tests
+void orc_render_emitted_infinite(void* h, const float* thr, float bsdf_pdf, int delta, const float* dir, float* out) {
+  V3 r = emitted_infinite(*(RenderScene*)h, v3(thr[0], thr[1], thr[2]), bsdf_pdf, delta != 0,
+                          v3(dir[0], dir[1], dir[2]));
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+
 // Light tree export, 20 floats per node: flux, mu[3], sigma_s2, nu[3],
 // lambda, radius, area, then as u32 bits: left, right, parent, instance,
 // triangle (kInvalid where absent), leaf flag, pad x3.  Returns the count.

@@ -72,6 +72,17 @@ def lib() -> ctypes.CDLL:
         L.orc_render_direct_light.argtypes = [vp, u32, u32, vp, vp, f32, f32, f32, vp, vp]
         L.orc_render_emitted.argtypes = [vp, u32, u32, vp, vp, u32, u32, vp, vp, vp, f32, ctypes.c_int, vp]
         L.orc_render_trace.argtypes = [vp, vp, vp] + [u32] * 7 + [vp]
+        L.orc_render_set_environment.argtypes = [vp, vp, u32, u32, f32, f32]
+        L.orc_env_pdf.argtypes = [vp, vp, ctypes.c_int]
+        L.orc_env_pdf.restype = f32
+        L.orc_env_radiance.argtypes = [vp, vp, vp]
+        L.orc_env_sample.argtypes = [vp, f32, f32, ctypes.c_int, vp]
+        L.orc_env_sample.restype = ctypes.c_int
+        L.orc_env_total_power.argtypes = [vp]
+        L.orc_env_total_power.restype = f32
+        L.orc_env_uv.argtypes = [vp, vp]
+        L.orc_env_dir.argtypes = [f32, f32, vp]
+        L.orc_render_emitted_infinite.argtypes = [vp, vp, f32, ctypes.c_int, vp, vp]
         L.orc_sg.argtypes = [ctypes.c_int, f32, f32]
         L.orc_sg.restype = f32
         L.orc_sg_product.argtypes = [vp, f32, vp, f32, vp]
@@ -81,6 +92,18 @@ def lib() -> ctypes.CDLL:
         L.orc_unoccluded.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def env_uv(d):
+    da, uv = np.asarray(d, np.float32), np.zeros(2, np.float32)
+    lib().orc_env_uv(_p(da), _p(uv))
+    return uv
+
+
+def env_dir(u, v):
+    out = np.zeros(3, np.float32)
+    lib().orc_env_dir(u, v, _p(out))
+    return out
 
 
 def _p(a):
@@ -226,6 +249,33 @@ class OracleRenderer:
         self.scene_meshes = scene.meshes
         self.recs = records(materials)
         lib().orc_render_set_materials(self.h, ctypes.addressof(self.recs), len(materials))
+        env = getattr(scene, "environment", None)
+        if env is not None:
+            px = np.ascontiguousarray(env.pixels, np.float32)
+            lib().orc_render_set_environment(self.h, _p(px), px.shape[1], px.shape[0], env.scale, env.rotate_y)
+
+    # environment light queries (R/light/environment.rs)
+    def env_pdf(self, d, mis=False):
+        da = np.asarray(d, np.float32)
+        return lib().orc_env_pdf(self.h, _p(da), int(mis))
+
+    def env_radiance(self, d):
+        da, out = np.asarray(d, np.float32), np.zeros(3, np.float32)
+        lib().orc_env_radiance(self.h, _p(da), _p(out))
+        return out
+
+    def env_sample(self, u, v, mis=False):
+        out = np.zeros(7, np.float32)
+        ok = lib().orc_env_sample(self.h, u, v, int(mis), _p(out))
+        return (out[:3], out[3:6], float(out[6])) if ok else None
+
+    def env_total_power(self):
+        return lib().orc_env_total_power(self.h)
+
+    def emitted_infinite(self, throughput, bsdf_pdf, delta, direction):
+        t, d, out = np.asarray(throughput, np.float32), np.asarray(direction, np.float32), np.zeros(3, np.float32)
+        lib().orc_render_emitted_infinite(self.h, _p(t), bsdf_pdf, int(delta), _p(d), _p(out))
+        return out
 
     def light_tree(self):
         """Light-tree nodes: structured array (see orc_render_light_tree)."""

@@ -144,6 +144,17 @@ class Instance:
 
 
 @dataclass
+class Environment:
+    """EnvironmentLight::from_pixels (R/light/environment.rs:47-83): a
+    lat-long RGB map (height, width, 3), radiance scale and rotation about +Y
+    (radians)."""
+
+    pixels: np.ndarray
+    scale: float = 1.0
+    rotate_y: float = 0.0
+
+
+@dataclass
 class Scene:
     """Geometry of the reference Scene (R/scene/mod.rs:81-98)."""
 
@@ -151,7 +162,12 @@ class Scene:
     instances: list[Instance] = field(default_factory=list)
     qbvh: Qbvh | None = None
     built: bool = False
-    materials: list = field(default_factory=list)  # tpt.material.OpenPbr, by material index
+    materials: list = field(default_factory=list)  # tpt.material records, by material index
+    environment: Environment | None = None  # Scene::set_environment_light
+
+    def set_environment_light(self, env: Environment) -> None:
+        self.environment = Environment(np.ascontiguousarray(env.pixels, np.float32), float(env.scale),
+                                       float(env.rotate_y))
 
     def add_material(self, material) -> int:
         """Scene::add_material (R/scene/mod.rs:143-217) for OpenPBR."""
