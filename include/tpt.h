@@ -163,6 +163,14 @@ typedef struct TptSceneDesc {
   uint32_t material_count;
   const uint32_t* tlas_primitive_indices; /* instance_count entries */
   const TptMaterial* materials;           /* material_count entries (TptInstanceDesc::material) */
+  /* Environment light (EnvironmentLight::from_pixels, R/light/environment.rs:47-83):
+   * a lat-long RGB map, env_width * env_height * 3 floats row-major (v = 0 is
+   * +Y), radiance scale and rotation about +Y in radians; NULL: none. */
+  const float* env_pixels;
+  uint32_t env_width;
+  uint32_t env_height;
+  float env_scale;
+  float env_rotate_y;
 } TptSceneDesc;
 
 /* A node of the scene's SG light tree (R/light_tree/mod.rs:52-88), as

@@ -57,10 +57,10 @@ __device__ __forceinline__ bool closest(const SceneView& sc, F3 o, F3 d, Lane& L
 // traversal (closest hit, then the target test).
 template <bool kTwoLevel>
 __device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView& sv, const TptMaterial& m,
-                                           const Vtx& vtx, float u_tree, float us0, float us1, Lane& L,
-                                           LaneStack& st) {
+                                           const Vtx& vtx, float u_root, float u_tree, float us0, float us1,
+                                           Lane& L, LaneStack& st) {
   NeeSample ns;
-  if (!nee_sample(sv, m, vtx, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
+  if (!nee_sample(sv, m, vtx, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
   uint32_t hi, ht;
   float tt, hb0, hb1, hb2;
   if (closest<kTwoLevel>(sc, ns.origin, ns.dir, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2) &&
@@ -95,7 +95,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
   float tt, hb0, hb1, hb2;
   if (!closest<kTwoLevel>(sc, eye, dir, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) {
     overflow |= st.overflow;
-    return radiance;  // no environment light
+    return env_le(sv, dir);  // infinite_light_le of the camera ray
   }
   overflow |= st.overflow;
   Vtx vtx = shading_vertex(sv, hi, ht, hb0, hb1, hb2, dir);
@@ -115,7 +115,8 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     if (m.kind != TPT_MATERIAL_EMISSIVE) {  // should_sample_direct_light: not a pure emitter
       float us[2];
       ps.s2(base + 3, us);
-      const F3 dl = direct_light<kTwoLevel>(sc, sv, m, vtx, ps.s1(base + 1), us[0], us[1], L, st);
+      const F3 dl =
+          direct_light<kTwoLevel>(sc, sv, m, vtx, ps.s1(base + 0), ps.s1(base + 1), us[0], us[1], L, st);
       overflow |= st.overflow;
       radiance = add(radiance, hadamard(thr, dl));
     }
@@ -131,7 +132,11 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
       thr = divs(thr, q);
     }
     const F3 no = offset_origin(vtx.p, vtx.ng, wi);
-    if (!closest<kTwoLevel>(sc, no, wi, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) break;  // no environment
+    if (!closest<kTwoLevel>(sc, no, wi, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) {
+      overflow |= st.overflow;
+      radiance = add(radiance, emitted_infinite(sv, thr, s.pdf, s.delta, wi));
+      break;
+    }
     overflow |= st.overflow;
     const Vtx nv = shading_vertex(sv, hi, ht, hb0, hb1, hb2, wi);
     radiance = add(radiance, emitted(sv, thr, s.pdf, s.delta, m, vtx.p, vtx.ns, nv, sv.mats[nv.material]));
@@ -307,7 +312,7 @@ extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb
     cam.up = F3{c.image_up[0], c.image_up[1], c.image_up[2]};
     cam.tan_half = c.tan_half_fov_y;
     cam.exposure = c.exposure;
-    pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats, s->lights, s->tri_leaf, s->light_count};
+    const pt::ShadeView sv = pt::shade_view_of(s);
     const SceneView v = view_of(s);
     const size_t n = (size_t)w * h;
     const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
@@ -394,7 +399,7 @@ int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, 
     if (!hip_ok(hipMemsetAsync(d->status, 0, 4, d->stream), "hipMemsetAsync(status)")) return TPT_ERROR_DEVICE;
   }
   const pt::CamView cam = cam_view(desc->camera);
-  const pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats, s->lights, s->tri_leaf, s->light_count};
+  const pt::ShadeView sv = pt::shade_view_of(s);
   const SceneView v = view_of(s);
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const size_t lds_bytes = (size_t)n_grad * sizeof(Acc);

@@ -9,6 +9,7 @@
 
 #include "light_tree.h"
 #include "sampler.h"
+#include "tpt_device.h"
 #include "tpt_internal.h"
 #include "traverse.h"
 
@@ -428,6 +429,8 @@ struct ShadeView {
   const GpuLightNode* lights;  // light tree, root 0 (light_count 0: no tree)
   const uint32_t* tri_leaf;    // GpuShade::light_base + triangle -> leaf node
   uint32_t light_count;
+  float w_tree, w_env;  // light-sampler category weights: root flux, environment power (0: absent)
+  EnvView env;
 };
 
 struct Vtx {
@@ -572,22 +575,171 @@ __device__ __forceinline__ bool mat_sample(const TptMaterial& m, const Vtx& v, f
   *wi_world = wi;
   return true;
 }
-// NormalizedLambertMaterial::eval / pdf (R/material/normalized_lambert.rs:160-193)
-// with modified_bsdf_eval (R/material/mod.rs:33-53); the other materials
-// never receive a light sample (no light-tree query) or evaluate to zero.
-__device__ __forceinline__ F3 lambert_eval(const TptMaterial& m, const Vtx& v, F3 wi, float* pdf) {
+// Material::eval and pdf (R/material/mod.rs:251-370) in radiance mode for a
+// light sample: NormalizedLambert (R/material/normalized_lambert.rs:160-193)
+// and OpenPBR (R/material/open_pbr.rs:549-590), each with modified_bsdf_eval
+// (R/material/mod.rs:33-53); Mirror and Emissive evaluate to zero.  *pdf is
+// 0 when f is 0 (Material::eval_pdf).
+__device__ __forceinline__ F3 mat_eval_pdf(const TptMaterial& m, const Vtx& v, F3 wi, float* pdf) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
   *pdf = 0.0f;
-  if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return zero;
-  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
-  const F3 wil = normalize_or_zero(to_frame(v.frame, wi));
-  if (wol.z <= 0.0f || wil.z <= 0.0f) return zero;
-  const F3 f = divs(rgb3(m.u.lambert.rho), kPi);
+  F3 f;
+  float p;
+  if (m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {
+    if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return zero;
+    const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+    const F3 wil = normalize_or_zero(to_frame(v.frame, wi));
+    if (wol.z <= 0.0f || wil.z <= 0.0f) return zero;
+    f = divs(rgb3(m.u.lambert.rho), kPi);
+    p = fmaxf(wil.z, 0.0f) / kPi;
+  } else if (m.kind == TPT_MATERIAL_OPEN_PBR) {
+    if (dot(v.wo, v.ng) <= 0.0f) return zero;
+    const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+    const F3 wil = normalize_or_zero(to_frame(v.frame, wi));
+    if (wol.z <= 0.0f) return zero;
+    const Bsdf b = make_bsdf(m.u.open_pbr, v.front);
+    f = bsdf_eval(b, wol, wil);
+    if (len2(f) == 0.0f) return zero;
+    p = bsdf_pdf(b, wol, wil);
+  } else {
+    return zero;
+  }
   if (len2(f) == 0.0f) return zero;
   const float cg = fabsf(dot(wi, v.ng));
   if (cg <= 1.0e-6f) return zero;
-  *pdf = fmaxf(wil.z, 0.0f) / kPi;
+  *pdf = p;
   return muls(f, fabsf(dot(wi, v.ns)) / cg);
+}
+
+// ---- environment light (R/light/environment.rs) ------------------------------------
+__device__ __forceinline__ F3 mat3_mul(const float* m, F3 v) {  // glam Mat3 * Vec3
+  return add(add(scl(v.x, f3(m[0], m[1], m[2])), scl(v.y, f3(m[3], m[4], m[5]))), scl(v.z, f3(m[6], m[7], m[8])));
+}
+__device__ __forceinline__ void env_uv(F3 d, float* u, float* v) {  // direction_to_uv (:339-348)
+  const F3 dir = normalize_or_zero(d);
+  const float theta = acosf(clampf(dir.y, -1.0f, 1.0f));
+  float phi = atan2f(dir.x, dir.z);
+  if (phi < 0.0f) phi += kTau;
+  *u = clampf(phi / kTau, 0.0f, 1.0f);
+  *v = clampf(theta / kPi, 0.0f, 1.0f);
+}
+__device__ __forceinline__ uint32_t env_coord(float t, uint32_t n) {  // pixel_coord (:326-330)
+  const uint32_t i = (uint32_t)(clampf(t, 0.0f, 1.0f) * (float)n);
+  return i < n - 1 ? i : n - 1;
+}
+__device__ __forceinline__ F3 env_radiance(const EnvView& e, F3 dir) {  // :134-140
+  float u, v;
+  env_uv(mat3_mul(e.inv_rot, dir), &u, &v);
+  const uint32_t k = env_coord(v, e.h) * e.w + env_coord(u, e.w);
+  return scl(e.scale, f3(e.px[3 * k], e.px[3 * k + 1], e.px[3 * k + 2]));
+}
+__device__ __forceinline__ float env_texel(const EnvView& e, uint32_t i, uint32_t j) {
+  const uint32_t* leaf = e.levels + 3 * (e.level_count - 1);
+  return e.pyr[leaf[0] + j * leaf[1] + i];
+}
+// pdf_mis_compensated (:156-177 over the compensated pyramid)
+__device__ __forceinline__ float env_pdf(const EnvView& e, F3 dir) {
+  if (e.total_mis <= 0.0f) return 0.0f;
+  float u, v;
+  env_uv(mat3_mul(e.inv_rot, dir), &u, &v);
+  const float st = sinf(v * kPi);
+  if (st <= 0.0f) return 0.0f;
+  const float wgt = env_texel(e, env_coord(u, e.w), env_coord(v, e.h));
+  if (wgt <= 0.0f) return 0.0f;
+  const float p2d = wgt / e.total_mis * (float)e.w * (float)e.h;
+  return p2d / (kTau * kPi * st);
+}
+// sample_with_distribution (:179-234) with hsw_step (:480-530) over the
+// compensated pyramid.
+__device__ __forceinline__ bool env_sample(const EnvView& e, float us0, float us1, F3* dir, F3* rad, float* pdf) {
+  if (e.total_mis <= 0.0f) return false;
+  const float one_minus_eps = 1.0f - 1.1920929e-7f;  // 1 - f32::EPSILON
+  float u = clampf(us0, 0.0f, 1.0f), v = clampf(us1, 0.0f, 1.0f);
+  uint32_t i = 0, j = 0;
+  for (uint32_t l = 1; l < e.level_count; ++l) {
+    const uint32_t pw = e.levels[3 * (l - 1) + 1], ph = e.levels[3 * (l - 1) + 2];
+    const uint32_t off = e.levels[3 * l], cw = e.levels[3 * l + 1], ch = e.levels[3 * l + 2];
+    const uint32_t si = cw / pw, sj = ch / ph;
+    uint32_t ci = i * si, cj = j * sj;
+    if (sj == 2) {
+      float top = 0.0f, bot = 0.0f;
+      for (uint32_t di = 0; di < si; ++di) {
+        top += e.pyr[off + cj * cw + ci + di];
+        bot += e.pyr[off + (cj + 1) * cw + ci + di];
+      }
+      const float total = top + bot;
+      if (total <= 0.0f) return false;
+      const float pt = top / total;
+      if (v < pt) {
+        v = clampf(v / pt, 0.0f, one_minus_eps);
+      } else {
+        v = clampf((v - pt) / (1.0f - pt), 0.0f, one_minus_eps);
+        cj += 1;
+      }
+    }
+    if (si == 2) {
+      const float lw = e.pyr[off + cj * cw + ci], rw = e.pyr[off + cj * cw + ci + 1];
+      const float total = lw + rw;
+      if (total <= 0.0f) return false;
+      const float pl = lw / total;
+      if (u < pl) {
+        u = clampf(u / pl, 0.0f, one_minus_eps);
+      } else {
+        u = clampf((u - pl) / (1.0f - pl), 0.0f, one_minus_eps);
+        ci += 1;
+      }
+    }
+    i = ci;
+    j = cj;
+  }
+  if (i >= e.w || j >= e.h) return false;
+  const float wgt = env_texel(e, i, j);
+  if (wgt <= 0.0f) return false;
+  const float uc = ((float)i + u) / (float)e.w, vc = ((float)j + v) / (float)e.h;
+  const float phi = clampf(uc, 0.0f, one_minus_eps) * kTau, theta = clampf(vc, 0.0f, one_minus_eps) * kPi;
+  const float st = sinf(theta);
+  *dir = mat3_mul(e.rot, f3(st * sinf(phi), cosf(theta), st * cosf(phi)));
+  const float sv = sinf(vc * kPi);
+  if (sv <= 0.0f) return false;
+  const float p2d = wgt / e.total_mis * (float)e.w * (float)e.h;
+  *pdf = p2d / (kTau * kPi * sv);
+  const uint32_t k = j * e.w + i;
+  *rad = scl(e.scale, f3(e.px[3 * k], e.px[3 * k + 1], e.px[3 * k + 2]));
+  return true;
+}
+
+// LightSampler (R/light/mod.rs:136-233): tree (weight root flux) and
+// environment (weight total power) categories, normalised CDF.
+__device__ __forceinline__ bool sample_category(const ShadeView& sv, float u, int* kind, float* pmf) {
+  const float total = (sv.w_tree > 0.0f ? sv.w_tree : 0.0f) + (sv.w_env > 0.0f ? sv.w_env : 0.0f);
+  float w[2];
+  int k[2];
+  int n = 0;
+  if (sv.w_tree > 0.0f) { w[n] = sv.w_tree; k[n++] = 0; }
+  if (sv.w_env > 0.0f) { w[n] = sv.w_env; k[n++] = 1; }
+  if (n == 0 || total <= 0.0f) return false;
+  float cdf[3] = {0.0f, 0.0f, 0.0f};
+  float acc = 0.0f;
+  for (int i = 0; i < n; ++i) {
+    acc += w[i];
+    cdf[i + 1] = acc;
+  }
+  const float inv = 1.0f / acc;
+  for (int i = 0; i <= n; ++i) cdf[i] *= inv;
+  u = clampf(u, 0.0f, 1.0f);
+  int pp = 0;
+  while (pp <= n && cdf[pp] <= u) ++pp;
+  int idx = pp > 0 ? pp - 1 : 0;
+  if (idx > n - 1) idx = n - 1;
+  *kind = k[idx];
+  *pmf = w[idx] / total;
+  return true;
+}
+__device__ __forceinline__ float category_pmf(const ShadeView& sv, int kind) {
+  const float total = (sv.w_tree > 0.0f ? sv.w_tree : 0.0f) + (sv.w_env > 0.0f ? sv.w_env : 0.0f);
+  if (total <= 0.0f) return 0.0f;
+  const float w = kind == 0 ? sv.w_tree : sv.w_env;
+  return w > 0.0f ? w / total : 0.0f;
 }
 
 // ---- SG light tree (R/math/sg.rs, R/light_tree/{lobe,traversal}.rs) ---------------
@@ -741,60 +893,76 @@ __device__ __forceinline__ F3 offset_origin(F3 p, F3 ng, F3 d) {  // R/integrato
 }
 
 // The light sample of direct_light_mis_contribution (R/integrator/mis.rs:141-210)
-// up to the shadow test: tree leaf (sample_light_tree), uniform point on its
-// triangle (sample_triangle_point, R/scene/mod.rs:616-633; sample_li_for_triangle,
-// R/light/area.rs:12-54), balance-heuristic weight.  On success `value` is
-// the contribution if the shadow ray from `origin` along `dir` reaches the
-// target triangle first (unoccluded_ray with a target, R/integrator/mod.rs:
-// 239-265).  The tree is the light sampler's only category (pmf 1).
+// up to the shadow test, with sample_light_mis_compensated_lazy
+// (R/light/mod.rs:289-331): a category, then either a light-tree leaf and a
+// uniform point on its triangle (sample_triangle_point, R/scene/mod.rs:
+// 616-633; sample_li_for_triangle, R/light/area.rs:12-54) or an
+// MIS-compensated environment direction; balance-heuristic weight.  On
+// success `value` is the contribution if the shadow ray from `origin` along
+// `dir` reaches the target triangle first, or escapes for the environment
+// (inst = TPT_INVALID_INDEX) (unoccluded_ray, R/integrator/mod.rs:239-265).
 struct NeeSample {
   F3 origin, dir, value;
   uint32_t inst, tri;
 };
-__device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMaterial& m, const Vtx& vtx, float u_tree,
-                                           float us0, float us1, NeeSample* ns) {
-  if (sv.light_count == 0 || m.kind != TPT_MATERIAL_NORMALIZED_LAMBERT) return false;
-  const float rho = luminance(rgb3(m.u.lambert.rho));
-  if (rho <= 0.0f) return false;
-  uint32_t leaf;
-  float lpmf;
-  if (!sample_tree(sv, vtx.p, vtx.ns, rho, u_tree, &leaf, &lpmf)) return false;
-  const GpuLightNode& ln = sv.lights[leaf];
-  F3 p[3];
-  triangle_world(sv, ln.instance, ln.triangle, p);
-  const float su0 = sqrtf(clampf(us0, 0.0f, 1.0f));
-  const float u1 = clampf(us1, 0.0f, 1.0f);
-  const float b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
-  const F3 lp = interp(b0, b1, b2, p[0], p[1], p[2]);
-  const float area = 0.5f * sqrtf(len2(cross(sub(p[1], p[0]), sub(p[2], p[0]))));
-  const float pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
-  if (pdf_area <= 0.0f) return false;
-  const F3 to_light = sub(lp, vtx.p);
-  const float d2 = len2(to_light);
-  if (d2 <= 0.0f) return false;
-  const float dist = sqrtf(d2);
-  const F3 wi = divs(to_light, dist);
-  const TptMaterial& lm = sv.mats[sv.inst[ln.instance].material];
-  if (!mat_may_emit(lm)) return false;
-  // light vertex (shading_vertex_from_triangle_sample): an Emissive light
-  // needs only its face-forwarded geometric normal, from the same world
-  // positions; other emitters need the full vertex for le
-  F3 le, lng;
-  if (lm.kind == TPT_MATERIAL_EMISSIVE) {
-    lng = face_forward(normalize_or_zero(cross(sub(p[1], p[0]), sub(p[2], p[0]))), neg(wi));
-    le = hadamard(muls(rgb3(lm.u.emissive.color), lm.u.emissive.strength), f3(1.0f, 1.0f, 1.0f));
+__device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMaterial& m, const Vtx& vtx, float u_root,
+                                           float u_tree, float us0, float us1, NeeSample* ns) {
+  int kind;
+  float cat_pmf;
+  if (!sample_category(sv, u_root, &kind, &cat_pmf)) return false;
+  F3 le, wi;
+  float li_pdf, selection;
+  ns->inst = ns->tri = TPT_INVALID_INDEX;
+  if (kind == 0) {
+    if (m.kind != TPT_MATERIAL_NORMALIZED_LAMBERT) return false;  // no light-tree query
+    const float rho = luminance(rgb3(m.u.lambert.rho));
+    if (rho <= 0.0f) return false;
+    uint32_t leaf;
+    float lpmf;
+    if (!sample_tree(sv, vtx.p, vtx.ns, rho, u_tree, &leaf, &lpmf)) return false;
+    const GpuLightNode& ln = sv.lights[leaf];
+    F3 p[3];
+    triangle_world(sv, ln.instance, ln.triangle, p);
+    const float su0 = sqrtf(clampf(us0, 0.0f, 1.0f));
+    const float u1 = clampf(us1, 0.0f, 1.0f);
+    const float b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
+    const F3 lp = interp(b0, b1, b2, p[0], p[1], p[2]);
+    const float area = 0.5f * sqrtf(len2(cross(sub(p[1], p[0]), sub(p[2], p[0]))));
+    const float pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
+    if (pdf_area <= 0.0f) return false;
+    const F3 to_light = sub(lp, vtx.p);
+    const float d2 = len2(to_light);
+    if (d2 <= 0.0f) return false;
+    const float dist = sqrtf(d2);
+    wi = divs(to_light, dist);
+    const TptMaterial& lm = sv.mats[sv.inst[ln.instance].material];
+    if (!mat_may_emit(lm)) return false;
+    // light vertex (shading_vertex_from_triangle_sample): an Emissive light
+    // needs only its face-forwarded geometric normal, from the same world
+    // positions; other emitters need the full vertex for le
+    F3 lng;
+    if (lm.kind == TPT_MATERIAL_EMISSIVE) {
+      lng = face_forward(normalize_or_zero(cross(sub(p[1], p[0]), sub(p[2], p[0]))), neg(wi));
+      le = hadamard(muls(rgb3(lm.u.emissive.color), lm.u.emissive.strength), f3(1.0f, 1.0f, 1.0f));
+    } else {
+      const Vtx lv = shading_vertex(sv, ln.instance, ln.triangle, b0, b1, b2, wi);
+      if (!mat_le(lm, lv, &le)) return false;
+      lng = lv.ng;
+    }
+    const float cos_light = fmaxf(dot(lng, neg(wi)), 0.0f);
+    if (cos_light <= 0.0f) return false;
+    li_pdf = pdf_area * d2 / cos_light;
+    selection = cat_pmf * lpmf;
+    ns->inst = ln.instance;
+    ns->tri = ln.triangle;
   } else {
-    const Vtx lv = shading_vertex(sv, ln.instance, ln.triangle, b0, b1, b2, wi);
-    if (!mat_le(lm, lv, &le)) return false;
-    lng = lv.ng;
+    if (!env_sample(sv.env, us0, us1, &wi, &le, &li_pdf)) return false;
+    if (li_pdf <= 0.0f) return false;
+    selection = cat_pmf;
   }
-  const float cos_light = fmaxf(dot(lng, neg(wi)), 0.0f);
-  if (cos_light <= 0.0f) return false;
-  const float li_pdf = pdf_area * d2 / cos_light;
-  const float selection = 1.0f * lpmf;
   if (li_pdf <= 0.0f || selection <= 0.0f) return false;
   float bsdf_pdf;
-  const F3 f = lambert_eval(m, vtx, wi, &bsdf_pdf);
+  const F3 f = mat_eval_pdf(m, vtx, wi, &bsdf_pdf);
   if (len2(f) == 0.0f) return false;
   const float cos_surface = fabsf(dot(vtx.ng, wi));
   if (cos_surface <= 0.0f) return false;
@@ -804,8 +972,6 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
   ns->origin = offset_origin(vtx.p, vtx.ng, wi);
   ns->dir = wi;
   ns->value = muls(hadamard(le, f), cos_surface * mis / pdf_total);
-  ns->inst = ln.instance;
-  ns->tri = ln.triangle;
   return true;
 }
 
@@ -833,10 +999,25 @@ __device__ __forceinline__ F3 emitted(const ShadeView& sv, F3 thr, float bsdf_pd
         const float dist = sqrtf(d2);
         const F3 wi = divs(to_light, dist);
         const float cos_light = fmaxf(dot(lv.ng, neg(wi)), 0.0f);
-        if (cos_light > 0.0f) light_pdf = 1.0f * lpmf * (pdf_area * d2 / cos_light);
+        if (cos_light > 0.0f) light_pdf = category_pmf(sv, 0) * lpmf * (pdf_area * d2 / cos_light);
       }
     }
   }
+  return muls(hadamard(thr, le), balance_heuristic(bsdf_pdf, light_pdf));
+}
+
+// infinite_light_le (R/light/environment.rs:305-311)
+__device__ __forceinline__ F3 env_le(const ShadeView& sv, F3 dir) {
+  return sv.env.px ? env_radiance(sv.env, dir) : f3(0.0f, 0.0f, 0.0f);
+}
+// emitted_radiance_from_bsdf_sample_infinite (R/integrator/mis.rs:236-257)
+// with pdf_for_environment_hit (R/light/mod.rs:431-443), MIS-compensated.
+__device__ __forceinline__ F3 emitted_infinite(const ShadeView& sv, F3 thr, float bsdf_pdf, bool delta, F3 dir) {
+  const F3 le = env_le(sv, dir);
+  if (le.x == 0.0f && le.y == 0.0f && le.z == 0.0f) return f3(0.0f, 0.0f, 0.0f);
+  if (delta) return hadamard(thr, le);
+  const float cat = category_pmf(sv, 1);
+  const float light_pdf = cat <= 0.0f ? 0.0f : cat * env_pdf(sv.env, dir);
   return muls(hadamard(thr, le), balance_heuristic(bsdf_pdf, light_pdf));
 }
 
@@ -856,6 +1037,24 @@ __device__ __forceinline__ F3 camera_dir(const CamView& cam, const smp::PathSamp
   const float sx = ((2.0f * px / rx) - 1.0f) * aspect * cam.tan_half;
   const float sy = (1.0f - (2.0f * py / ry)) * cam.tan_half;
   return normalize(add(add(cam.fwd, scl(sx, cam.right)), scl(sy, cam.up)));
+}
+
+// The shading view of a scene resident on the device.
+inline ShadeView shade_view_of(const TptScene* s) {
+  ShadeView v{};
+  v.inst = s->shade;
+  v.pos = s->vpos;
+  v.nrm = s->vnrm;
+  v.uv = s->vuv;
+  v.idx = s->vidx;
+  v.mats = s->mats;
+  v.lights = s->lights;
+  v.tri_leaf = s->tri_leaf;
+  v.light_count = s->light_count;
+  v.w_tree = s->w_tree;
+  v.w_env = s->w_env;
+  v.env = s->env;
+  return v;
 }
 
 }  // namespace pt

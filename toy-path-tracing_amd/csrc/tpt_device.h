@@ -52,6 +52,22 @@ struct DeviceBuffer {
 
 }  // namespace tpt
 
+namespace tpt {
+namespace pt {
+// Environment light on the device: RGB texels, the MIS-compensated weight
+// pyramid (levels coarsest first, table of offset / width / height), radiance
+// scale and the rotation about +Y (R/light/environment.rs:9-20).
+struct EnvView {
+  const float* px;  // nullptr: no environment light
+  const float* pyr;
+  const uint32_t* levels;
+  uint32_t w, h, level_count;
+  float scale, total_mis;
+  float rot[9], inv_rot[9];
+};
+}  // namespace pt
+}  // namespace tpt
+
 struct TptDevice {
   int index = 0;
   int cu_count = 0;
@@ -100,6 +116,12 @@ struct TptScene {
   uint32_t* tri_leaf = nullptr;
   std::vector<TptLightNode> light_nodes;  // host copy (tpt_scene_light_tree)
   std::vector<TptMaterial> host_mats;     // host copy of the material table
+  // light-sampler category weights and the environment light
+  float w_tree = 0.0f, w_env = 0.0f;
+  float* env_px = nullptr;
+  float* env_pyr = nullptr;
+  uint32_t* env_levels = nullptr;
+  tpt::pt::EnvView env{};
   size_t bytes = 0;
   // staging for host-pointer calls (grow-only)
   void* stage = nullptr;

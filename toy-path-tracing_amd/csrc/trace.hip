@@ -26,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "environment.h"
 #include "tpt_device.h"
 #include "tpt_internal.h"
 #include "traverse.h"
@@ -266,6 +267,9 @@ void free_scene(TptScene* s) {
   (void)hipFree(s->mats);
   (void)hipFree(s->lights);
   (void)hipFree(s->tri_leaf);
+  (void)hipFree(s->env_px);
+  (void)hipFree(s->env_pyr);
+  (void)hipFree(s->env_levels);
   delete s;
 }
 
@@ -639,6 +643,7 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
     s->tlas_node_count = desc->tlas_node_count;
     s->material_count = desc->materials ? desc->material_count : 0;
     s->light_count = (uint32_t)lnodes.size();
+    s->w_tree = lnodes.empty() ? 0.0f : std::fmax(lnodes[0].flux, 0.0f);
     s->light_nodes = std::move(lnodes);
     s->host_mats = mats;
     if (!upload(nodes, &s->nodes, &s->bytes) || !upload(qnodes, &s->qnodes, &s->bytes) ||
@@ -651,6 +656,35 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
         !upload(glights, &s->lights, &s->bytes) || !upload(tri_leaf, &s->tri_leaf, &s->bytes)) {
       free_scene(s);
       return nullptr;
+    }
+    if (desc->env_pixels) {  // environment light: texels, compensated pyramid, level table
+      HostEnv he;
+      std::string eerr;
+      if (!build_environment(desc->env_pixels, desc->env_width, desc->env_height, desc->env_scale,
+                             desc->env_rotate_y, &he, &eerr)) {
+        set_error("tpt_scene_create: " + eerr);
+        free_scene(s);
+        return nullptr;
+      }
+      if (!upload(he.px, &s->env_px, &s->bytes) || !upload(he.pyr, &s->env_pyr, &s->bytes) ||
+          !upload(he.levels, &s->env_levels, &s->bytes)) {
+        free_scene(s);
+        return nullptr;
+      }
+      pt::EnvView& e = s->env;
+      e.px = s->env_px;
+      e.pyr = s->env_pyr;
+      e.levels = s->env_levels;
+      e.w = he.w;
+      e.h = he.h;
+      e.level_count = he.level_count;
+      e.scale = he.scale;
+      e.total_mis = he.total_mis;
+      for (int k = 0; k < 9; ++k) {
+        e.rot[k] = he.rot[k];
+        e.inv_rot[k] = he.inv_rot[k];
+      }
+      s->w_env = std::fmax(he.power, 0.0f);
     }
     return s;
   } catch (const std::exception& e) {

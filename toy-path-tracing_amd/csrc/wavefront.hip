@@ -187,7 +187,19 @@ __global__ __launch_bounds__(256) void wf_shade(Pass ps, pt::ShadeView sv, State
     if (slot < n) {
       path = qin.path[slot];
       const TptHit h = hits[slot];
-      if (h.instance != TPT_INVALID_INDEX) {  // a miss ends the path (no environment light)
+      if (h.instance == TPT_INVALID_INDEX && sv.env.px) {  // a miss ends the path: environment emission
+        const TptRay r = qin.rays[slot];
+        const F3 dir = f3(r.d[0], r.d[1], r.d[2]);
+        F3 rad = load3f(st.rad, path);
+        if (bounce == 0) {
+          rad = pt::env_le(sv, dir);
+        } else {
+          const uint32_t info = st.prev_info[path];
+          rad = pt::add(rad, pt::emitted_infinite(sv, load3f(st.thr, path), st.prev_pdf[path], (info >> 31) != 0, dir));
+        }
+        store3(st.rad, path, rad);
+      }
+      if (h.instance != TPT_INVALID_INDEX) {
         const uint32_t gp = path / ps.spp_slots, s = path % ps.spp_slots;
         const FrameDev& fr = ps.frames[frame_of(ps, gp)];
         const uint32_t lp = gp - fr.pix_begin;
@@ -211,7 +223,7 @@ __global__ __launch_bounds__(256) void wf_shade(Pass ps, pt::ShadeView sv, State
           if (m.kind != TPT_MATERIAL_EMISSIVE) {
             float us[2];
             smpl.s2(bd + 3, us);
-            if (pt::nee_sample(sv, m, vtx, smpl.s1(bd + 1), us[0], us[1], &ns)) {
+            if (pt::nee_sample(sv, m, vtx, smpl.s1(bd + 0), smpl.s1(bd + 1), us[0], us[1], &ns)) {
               want_shadow = true;
               nee_value = pt::hadamard(thr, ns.value);
             }
@@ -467,7 +479,7 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.max_depth = max_depth;
     p.q8 = flags & TPT_FLAG_QUANTIZED;
     p.exact_shadow = flags & TPT_FLAG_ANY_EXACT;
-    const pt::ShadeView sv{s->shade, s->vpos, s->vnrm, s->vuv, s->vidx, s->mats, s->lights, s->tri_leaf, s->light_count};
+    const pt::ShadeView sv = pt::shade_view_of(s);
     if (!hip_ok(hipMemcpyAsync((void*)p.ps.frames, fd.data(), fd.size() * sizeof(wf::FrameDev), hipMemcpyHostToDevice,
                                st),
                 "hipMemcpyAsync(frames)"))

@@ -92,10 +92,15 @@ class SceneDesc(ctypes.Structure):
         ("material_count", ctypes.c_uint32),
         ("tlas_primitive_indices", ctypes.c_void_p),
         ("materials", ctypes.c_void_p),
+        ("env_pixels", ctypes.c_void_p),
+        ("env_width", ctypes.c_uint32),
+        ("env_height", ctypes.c_uint32),
+        ("env_scale", ctypes.c_float),
+        ("env_rotate_y", ctypes.c_float),
     ]
 
 
-assert ctypes.sizeof(MeshDesc) == 64 and ctypes.sizeof(SceneDesc) == 56
+assert ctypes.sizeof(MeshDesc) == 64 and ctypes.sizeof(SceneDesc) == 80
 assert ctypes.sizeof(InstanceDesc) == 136
 
 

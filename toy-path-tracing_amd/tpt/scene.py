@@ -289,6 +289,13 @@ def scene_desc(scene: Scene):
         keep.append(mats)
         desc.materials = ctypes.addressof(mats)
         desc.material_count = len(scene.materials)
+    if scene.environment is not None:
+        px = np.ascontiguousarray(scene.environment.pixels, np.float32)
+        keep.append(px)
+        desc.env_pixels = ptr(px)
+        desc.env_height, desc.env_width = px.shape[0], px.shape[1]
+        desc.env_scale = scene.environment.scale
+        desc.env_rotate_y = scene.environment.rotate_y
     keep += [meshes, insts]
     return desc, keep
 

@@ -221,6 +221,70 @@ def cornell_lambert(build: bool = True) -> CornellScene:
     return CornellScene(cs.scene, mats, cs.light_instance, cs.camera)
 
 
+def procedural_sky(width: int = 2048, height: int = 1024, sun_elevation_deg: float = 35.0,
+                   sun_azimuth_deg: float = 40.0, sun_radius_deg: float = 0.27, sun_radiance: float = 2.0e4,
+                   zenith=(0.25, 0.45, 0.9), horizon=(0.9, 0.95, 1.0), ground=(0.25, 0.22, 0.2)) -> np.ndarray:
+    """Lat-long HDR sky (height, width, 3) in the environment light's
+    convention (v = 0 is +Y, phi = atan2(x, z)): a zenith-to-horizon gradient,
+    a dim ground below the horizon and an analytic sun disk of the given
+    angular radius.  The disk's texels get the sun radiance weighted by the
+    fraction of the texel the disk covers (supersampled 4x4)."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    theta = v * np.pi
+    y = np.cos(theta)
+    t = np.clip(y, 0.0, 1.0) ** 0.6
+    sky = (1 - t)[:, None] * np.asarray(horizon) + t[:, None] * np.asarray(zenith)
+    img = np.repeat(sky[:, None, :], width, axis=1)
+    img[y < 0] = np.asarray(ground)
+    el, az, rad = np.radians(sun_elevation_deg), np.radians(sun_azimuth_deg), np.radians(sun_radius_deg)
+    sd = np.array([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)])
+    # texels near the sun: supersample coverage
+    th_s = np.arccos(sd[1])
+    ph_s = np.arctan2(sd[0], sd[2]) % (2 * np.pi)
+    j0, j1 = int(max(0, (th_s - 3 * rad) / np.pi * height)), int(min(height, (th_s + 3 * rad) / np.pi * height + 1))
+    for j in range(j0, j1):
+        for i in range(width):
+            ph_c = (i + 0.5) / width * 2 * np.pi
+            if abs((ph_c - ph_s + np.pi) % (2 * np.pi) - np.pi) * np.sin(max(th_s, 1e-3)) > 4 * rad + 2 * np.pi / width:
+                continue
+            ss = (np.arange(4) + 0.5) / 4
+            tt = (j + ss[:, None]) / height * np.pi
+            pp = (i + ss[None, :]) / width * 2 * np.pi
+            d = np.stack([np.sin(tt) * np.sin(pp), np.cos(tt) * np.ones_like(pp), np.sin(tt) * np.cos(pp)], -1)
+            cover = np.mean(d @ sd >= np.cos(rad))
+            if cover > 0:
+                img[j, i] = img[j, i] * (1 - cover) + sun_radiance * cover
+    return img.astype(np.float32)
+
+
+def outdoor(materials: str = "openpbr", sky_size=(256, 128), build: bool = True) -> CornellScene:
+    """The Cornell boxes on a 20 x 20 floor under the procedural sky (no
+    walls, no area light): light comes from the environment only, so
+    environment NEE is what lights OpenPBR surfaces here."""
+    from .material import NormalizedLambert, OpenPbr
+    from .scene import Environment
+
+    s = Scene()
+    floor = s.add_mesh(flat_quads([[(-10, 0, -10), (-10, 0, 10), (10, 0, 10), (10, 0, -10)]]))
+    box = s.add_mesh(unit_box())
+    s.add_instance(floor, 0, np.eye(4))
+    s.add_instance(box, 1, _translate(0.68, 0.15, 0.36) @ _rot_y(-18.0) @ _scale(0.3, 0.3, 0.3))
+    s.add_instance(box, 2, _translate(0.33, 0.3, 0.64) @ _rot_y(15.0) @ _scale(0.3, 0.6, 0.3))
+    if build:
+        s.build_qbvh()
+    cols = [(0.8, 0.8, 0.8), (0.8, 0.1, 0.1), (0.1, 0.8, 0.1)]
+    if materials == "openpbr":
+        mats = [OpenPbr.new(c).with_(specular_weight=0.0) for c in cols]
+    else:
+        mats = [NormalizedLambert(c) for c in cols]
+    for m in mats:
+        s.add_material(m)
+    s.set_environment_light(Environment(procedural_sky(*sky_size), 1.0, 0.3))
+    cam = dict(eye=(0.5, 0.9, -1.6), look_at=(0.5, 0.3, 0.5), up=(0.0, 1.0, 0.0), fov_y_deg=40.0)
+    return CornellScene(s, mats, -1, cam)
+
+
 # --------------------------------------------------------------------------- rays
 def primary_rays(width: int, height: int, eye, look_at, up, fov_y_deg: float) -> np.ndarray:
     """Pixel-centre pinhole rays (direction_for_sample, R/scene/camera.rs:80-87), row-major."""
