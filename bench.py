@@ -279,8 +279,12 @@ def render_bench(dev, stream, args):
             "image_mean": [round(float(x), 4) for x in img.reshape(-1, 3).mean(0)],
         }
         log(f"render {key}: {ms:.2f} ms/frame = {res['samples_per_s'] / 1e6:.1f} Msamples/s")
-        # wavefront path tracer on the same frame: FP32 and 8-bit nodes
+        # wavefront path tracer on the same frame: FP32 and 8-bit nodes, after
+        # the autotuner has picked paths in flight and the trace launch shape
         wf_img = torch.empty_like(img_dev)
+        tune = rds.autotune([(cam, w, h, spp, depth)], tune_spp=8)
+        res["autotune"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()}
+        log(f"render {key} autotune: {res['autotune']}")
         for q8 in (False, True):
             rds.render_batch_device([(cam, w, h, spp, depth)], [wf_img], quantized=q8)
             torch.cuda.synchronize()
@@ -313,6 +317,9 @@ def render_bench(dev, stream, args):
             log(f"render {key} cpu baseline: {rows * w * spp / dt / 1e6:.2f} Msamples/s on {threads} threads")
         rds.close()
         out[key] = res
+        dev.set_param("wf_paths", 1 << 21)  # the traversal defaults for whatever runs next
+        dev.set_param("blocks_per_cu", 4)
+        dev.set_param("refill_below", 24)
     return out
 
 

@@ -368,6 +368,22 @@ typedef struct TptFrameDesc {
  * R/main.rs:98-126, which it runs for many frames at once. */
 int32_t tpt_render_batch(TptScene* scene, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags);
 
+/* Autotuner of tpt_render_batch for a workload: coordinate descent over the
+ * device's "wf_paths", "blocks_per_cu" and "refill_below" parameters, each
+ * candidate timed with HIP events on the given frames (samples per pixel
+ * capped at tune_spp, 0: as given; outputs go to an internal buffer).  The
+ * fastest setting stays on the device.  No reference counterpart. */
+typedef struct TptTuneResult {
+  uint32_t wf_paths;
+  uint32_t blocks_per_cu;
+  uint32_t refill_below;
+  uint32_t trials;
+  float ms_default;  /* the starting setting */
+  float ms_best;
+} TptTuneResult;
+int32_t tpt_autotune_render(TptScene* scene, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags,
+                            uint32_t tune_spp, TptTuneResult* out);
+
 /* The traversal of tpt_trace_closest (query 0) or tpt_trace_any (query 1)
  * with per-ray work counters instead of results: counts[3*i+0] nodes
  * visited, [3*i+1] triangles tested, [3*i+2] instances entered.  For query 0

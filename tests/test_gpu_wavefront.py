@@ -84,3 +84,22 @@ def test_bad_frames_are_rejected(scene):
     with pytest.raises(tpt._lib.TptError):
         ds.render_batch([(cam, 8, 8, 0, 8)])
     assert ds.render_batch([]) == []
+
+
+def test_autotuner_picks_a_setting_no_slower_and_keeps_the_image(device, scene):
+    cs, ds = scene
+    w = h = 128
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    ref = ds.render(cam, w, h, 8, 8)
+    try:
+        res = ds.autotune([(cam, w, h, 64, 8)], tune_spp=8)
+        print("autotune", res)
+        assert res["ms_best"] <= res["ms_default"] and res["trials"] >= 10
+        assert res["wf_paths"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22)
+        assert res["blocks_per_cu"] in (0, 2, 3, 4, 6, 8) and res["refill_below"] in (8, 16, 24, 32, 48)
+        (img,) = ds.render_batch([(cam, w, h, 8, 8)], exact_shadow=True)
+        assert bits_equal(img, ref)
+    finally:
+        device.set_param("wf_paths", 1 << 21)
+        device.set_param("blocks_per_cu", 4)
+        device.set_param("refill_below", 24)

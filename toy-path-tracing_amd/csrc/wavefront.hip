@@ -369,7 +369,7 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
     hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, a ^ 1u);
     hipLaunchKernelGGL(wf::wf_shade, dim3(g_shade), dim3(256), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
                        p.q[a ^ 1u], p.sq, p.c, a, b);
-    if (b < p.max_depth && sv.light_count) {
+    if (b < p.max_depth && (sv.light_count || sv.env.px)) {  // some light can be sampled
       if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow, p.ps.paths, nullptr, p.sq.target,
                               p.sq.unocc, st))
         return false;
@@ -533,6 +533,98 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     return TPT_OK;
   } catch (const std::exception& e) {
     set_error(std::string("tpt_render_batch: ") + e.what());
+    return TPT_ERROR;
+  }
+}
+
+namespace {
+
+// Time one batched render (device outputs) on the device stream, ms.
+bool time_batch(TptScene* s, std::vector<TptFrameDesc>& frames, uint32_t flags, int reps, float* ms) {
+  TptDevice* d = s->dev;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (!hip_ok(hipEventCreate(&a), "hipEventCreate") || !hip_ok(hipEventCreate(&b), "hipEventCreate")) return false;
+  bool ok = tpt_render_batch(s, frames.data(), (uint32_t)frames.size(), flags | TPT_FLAG_DEVICE_POINTERS) == TPT_OK;
+  ok = ok && hip_ok(hipEventRecord(a, d->stream), "hipEventRecord");
+  for (int r = 0; ok && r < reps; ++r)
+    ok = tpt_render_batch(s, frames.data(), (uint32_t)frames.size(), flags | TPT_FLAG_DEVICE_POINTERS) == TPT_OK;
+  ok = ok && hip_ok(hipEventRecord(b, d->stream), "hipEventRecord") &&
+       hip_ok(hipEventSynchronize(b), "hipEventSynchronize") && hip_ok(hipEventElapsedTime(ms, a, b), "hipEventElapsedTime");
+  if (ok) *ms /= (float)reps;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ok;
+}
+
+}  // namespace
+
+// Autotuner of the wavefront path tracer: coordinate descent over paths in
+// flight, resident trace blocks per CU and the trace kernels' refill
+// threshold, each candidate timed (HIP events) on the caller's frames at
+// tune_spp samples per pixel; the fastest setting is left on the device.
+extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags,
+                                       uint32_t tune_spp, TptTuneResult* out) {
+  try {
+    if (!s || !frames || frame_count == 0 || !out) {
+      set_error("tpt_autotune_render: null argument");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    TptDevice* d = s->dev;
+    if (!hip_ok(hipSetDevice(d->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    size_t pixels = 0;
+    for (uint32_t f = 0; f < frame_count; ++f) pixels += (size_t)frames[f].render.width * frames[f].render.height;
+    DeviceBuffer img;
+    if (!img.ensure(std::max<size_t>(12 * pixels, 256))) return TPT_ERROR_DEVICE;
+    std::vector<TptFrameDesc> fr(frames, frames + frame_count);
+    size_t off = 0;
+    for (auto& f : fr) {
+      f.rgb = (float*)img.ptr + 3 * off;
+      off += (size_t)f.render.width * f.render.height;
+      if (tune_spp) f.render.samples_per_pixel = std::min(f.render.samples_per_pixel, tune_spp);
+    }
+    const uint32_t keep_paths = d->wf_paths, keep_bpc = d->blocks_per_cu, keep_refill = d->refill_below;
+    uint32_t trials = 0;
+    float best = 0.0f;
+    if (!time_batch(s, fr, flags, 2, &best)) return TPT_ERROR_DEVICE;
+    ++trials;
+    const float base_ms = best;
+    auto tryv = [&](uint32_t* knob, uint32_t v) -> int32_t {
+      const uint32_t old = *knob;
+      *knob = v;
+      float ms = 0.0f;
+      if (!time_batch(s, fr, flags, 2, &ms)) {
+        *knob = old;
+        return TPT_ERROR_DEVICE;
+      }
+      ++trials;
+      if (ms < best) {
+        best = ms;
+      } else {
+        *knob = old;
+      }
+      return TPT_OK;
+    };
+    const uint32_t paths[] = {1u << 19, 1u << 20, 1u << 21, 1u << 22};
+    const uint32_t bpcs[] = {0u, 2u, 3u, 4u, 6u, 8u};
+    const uint32_t refills[] = {8u, 16u, 24u, 32u, 48u};
+    for (uint32_t v : paths)
+      if (v != d->wf_paths && tryv(&d->wf_paths, v) != TPT_OK) return TPT_ERROR_DEVICE;
+    for (uint32_t v : bpcs)
+      if (v != d->blocks_per_cu && tryv(&d->blocks_per_cu, v) != TPT_OK) return TPT_ERROR_DEVICE;
+    for (uint32_t v : refills)
+      if (v != d->refill_below && tryv(&d->refill_below, v) != TPT_OK) return TPT_ERROR_DEVICE;
+    out->wf_paths = d->wf_paths;
+    out->blocks_per_cu = d->blocks_per_cu;
+    out->refill_below = d->refill_below;
+    out->trials = trials;
+    out->ms_default = base_ms;
+    out->ms_best = best;
+    (void)keep_paths;
+    (void)keep_bpc;
+    (void)keep_refill;
+    return TPT_OK;
+  } catch (const std::exception& e) {
+    set_error(std::string("tpt_autotune_render: ") + e.what());
     return TPT_ERROR;
   }
 }

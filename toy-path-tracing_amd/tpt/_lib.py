@@ -126,6 +126,11 @@ class RenderDesc(ctypes.Structure):
 assert ctypes.sizeof(RenderDesc) == 80
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("wf_paths", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("refill_below", ctypes.c_uint32),
+                ("trials", ctypes.c_uint32), ("ms_default", ctypes.c_float), ("ms_best", ctypes.c_float)]
+
+
 class FrameDesc(ctypes.Structure):
     """TptFrameDesc: one frame of tpt_render_batch."""
     _fields_ = [("render", RenderDesc), ("rgb", ctypes.c_void_p)]
@@ -179,6 +184,8 @@ _SIGNATURES = [
     ("tpt_render_adjoint", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_scene_set_materials", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_autotune_render", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     ("tpt_trace_counts", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_uint32]),

@@ -437,6 +437,19 @@ class DeviceScene:
         check(lib().tpt_render_batch(self.handle, arr, len(frames), flags))
         return outs
 
+    def autotune(self, frames, quantized: bool = False, tune_spp: int = 8) -> dict:
+        """tpt_autotune_render over these frames; the chosen parameters stay
+        on the device.  -> dict of the result fields."""
+        arr = (_lib.FrameDesc * max(1, len(frames)))()
+        for i, (cam, w, h, spp, depth) in enumerate(frames):
+            d = arr[i].render
+            d.camera = cam
+            d.width, d.height, d.samples_per_pixel, d.max_depth = w, h, spp, depth
+        res = _lib.TuneResult()
+        flags = _lib.TPT_FLAG_QUANTIZED if quantized else 0
+        check(lib().tpt_autotune_render(self.handle, arr, len(frames), flags, tune_spp, ctypes.byref(res)))
+        return {name: getattr(res, name) for name, _ in _lib.TuneResult._fields_}
+
     def render_batch_device(self, frames, outs_dev, quantized: bool = False, exact_shadow: bool = False) -> None:
         """As render_batch into device buffers (torch tensors); asynchronous."""
         arr = (_lib.FrameDesc * max(1, len(frames)))()
