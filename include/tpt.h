@@ -319,10 +319,13 @@ typedef struct TptRenderDesc {
  * (R/integrator/mis.rs:19-139): for every pixel, samples 0..spp-1 through
  * PathSampler, generate_ray_differential and trace_radiance, accumulated as
  * the reference's running mean, times the camera exposure.  rgb: width *
- * height * 3 floats, row-major.  Light NEE follows the reference policy: a
- * NormalizedLambert shading point samples the SG light tree (shadow ray,
- * balance heuristic); an OpenPBR shading point gets no light-tree sample
- * (R/light_tree/importance.rs:26-36, R/material/open_pbr.rs:592-606). */
+ * height * 3 floats, row-major.  Light NEE follows the reference policy: the
+ * light sampler picks the light-tree or the environment category by power
+ * (LightSampler, R/light/mod.rs); a NormalizedLambert shading point samples the SG
+ * light tree (shadow ray, balance heuristic); an OpenPBR shading point gets
+ * no light-tree sample (R/light_tree/importance.rs:26-36,
+ * R/material/open_pbr.rs:592-606); every material gets environment
+ * samples. */
 int32_t tpt_render(TptScene* scene, const TptRenderDesc* desc, float* rgb, uint32_t flags);
 
 /* Gradient of an image loss with respect to every NormalizedLambert
