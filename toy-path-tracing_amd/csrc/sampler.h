@@ -92,11 +92,22 @@ __device__ __forceinline__ void sobol_point(uint32_t index, uint32_t seed, int d
   }
 }
 
-// The 24 permutations of four base-4 digits, in the reference's table order.
-__constant__ static const uint8_t kDigitPerm[24][4] = {
-    {0, 1, 2, 3}, {0, 1, 3, 2}, {0, 2, 1, 3}, {0, 2, 3, 1}, {0, 3, 2, 1}, {0, 3, 1, 2}, {1, 0, 2, 3}, {1, 0, 3, 2},
-    {1, 2, 0, 3}, {1, 2, 3, 0}, {1, 3, 2, 0}, {1, 3, 0, 2}, {2, 1, 0, 3}, {2, 1, 3, 0}, {2, 0, 1, 3}, {2, 0, 3, 1},
-    {2, 3, 0, 1}, {2, 3, 1, 0}, {3, 1, 2, 0}, {3, 1, 0, 2}, {3, 2, 1, 0}, {3, 2, 0, 1}, {3, 0, 2, 1}, {3, 0, 1, 2}};
+// The 24 permutations of four base-4 digits, in the reference's table order
+// (R/sampler/zsobol.rs), packed 8 bits per permutation (the image of digit d
+// in bits 2d..2d+1), permutations 0-7, 8-15 and 16-23 in three words.  A
+// lookup is a select and a shift: no memory access inside the digit loop
+// (a per-lane indexed table load there stalls the wave once per digit).
+__host__ __device__ __forceinline__ uint32_t permute_digit(uint32_t p, uint32_t digit) {
+  const uint64_t w = p < 8u ? 0xb1e19c6c78d8b4e4ull : (p < 16u ? 0x72d236c68d2d39c9ull : 0x93634b1b87271e4eull);
+  return (uint32_t)(w >> (((p & 7u) << 3) | (digit << 1))) & 3u;
+}
+
+// (h >> 24) % 24 in 32-bit arithmetic: h >> 24 = xh * 2^32 + xl with
+// xh < 256, and 2^32 = 16 (mod 24).
+__host__ __device__ __forceinline__ uint32_t top40_mod24(uint64_t h) {
+  const uint32_t xl = (uint32_t)(h >> 24), xh = (uint32_t)(h >> 56);
+  return (xh * 16u + xl % 24u) % 24u;
+}
 
 // Per-frame constants of the ZSobol index: log2 of the pow2-rounded spp and
 // the number of base-4 digits of the Morton index.
@@ -125,7 +136,7 @@ __host__ __device__ inline ZSobolFrame zsobol_frame(uint32_t spp, uint32_t width
 
 // Sample index of (pixel, sample, dimension): Morton-ordered pixel + sample,
 // each base-4 digit permuted by a hash of the higher digits and the dimension.
-__device__ __forceinline__ uint64_t zsobol_index(const ZSobolFrame& f, uint32_t px, uint32_t py, uint32_t si,
+__host__ __device__ __forceinline__ uint64_t zsobol_index(const ZSobolFrame& f, uint32_t px, uint32_t py, uint32_t si,
                                                  uint32_t dim) {
   const uint64_t m = (morton2(px, py) << f.log2_spp) | (uint64_t)si;
   const bool odd = (f.log2_spp & 1u) != 0;
@@ -134,8 +145,8 @@ __device__ __forceinline__ uint64_t zsobol_index(const ZSobolFrame& f, uint32_t 
   for (int i = (int)f.n_digits - 1; i >= (odd ? 1 : 0); --i) {
     const uint32_t sh = 2u * (uint32_t)i - (odd ? 1u : 0u);
     const uint32_t digit = (uint32_t)((m >> sh) & 3u);
-    const uint32_t p = (uint32_t)((mix64((m >> (sh + 2)) ^ dh) >> 24) % 24u);
-    out |= (uint64_t)kDigitPerm[p][digit] << sh;
+    const uint32_t p = top40_mod24(mix64((m >> (sh + 2)) ^ dh));
+    out |= (uint64_t)permute_digit(p, digit) << sh;
   }
   if (odd) out |= (m & 1u) ^ (mix64((m >> 1) ^ dh) & 1u);
   return out;

@@ -87,18 +87,45 @@ __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
-// Wave-compacted append: every lane of the wave calls it (converged); lanes
-// with want get consecutive slots.  Returns the slot (valid where want).
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
-  const uint64_t m = __ballot(want);
-  if (m == 0) return 0;
-  const uint32_t lane = lane_id();
-  uint32_t base = 0;
-  const int leader = __ffsll((long long)m) - 1;
-  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-  return base + rank;
+// Block-aggregated append to up to two queues: every thread of the block
+// calls it (block-uniform control flow); threads with want_a (want_b) get
+// consecutive slots of queue a (b).  One global atomic per queue per block
+// call: the queue counters are single words updated by every block on all
+// XCDs, and appends of one atomic per wave serialise on them.
+constexpr uint32_t kAppendBlock = 256;
+struct AppendSlots {
+  uint32_t a, b;
+};
+__device__ __forceinline__ AppendSlots block_append(uint32_t* ca, bool want_a, uint32_t* cb, bool want_b) {
+  constexpr uint32_t kWaves = kAppendBlock / 64;
+  __shared__ uint32_t cnt[2][kWaves];
+  __shared__ uint32_t base[2];
+  const uint64_t ma = __ballot(want_a), mb = __ballot(want_b);
+  const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+  if (lane == 0) {
+    cnt[0][wave] = (uint32_t)__popcll(ma);
+    cnt[1][wave] = (uint32_t)__popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ta = 0, tb = 0;
+    for (uint32_t w = 0; w < kWaves; ++w) {
+      ta += cnt[0][w];
+      tb += cnt[1][w];
+    }
+    base[0] = ta ? atomicAdd(ca, ta) : 0u;
+    base[1] = tb ? atomicAdd(cb, tb) : 0u;
+  }
+  __syncthreads();
+  AppendSlots r{base[0], base[1]};
+  for (uint32_t w = 0; w < wave; ++w) {
+    r.a += cnt[0][w];
+    r.b += cnt[1][w];
+  }
+  r.a += __builtin_amdgcn_mbcnt_hi((uint32_t)(ma >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ma, 0u));
+  r.b += __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
+  __syncthreads();  // cnt and base are rewritten by the next call
+  return r;
 }
 
 __device__ __forceinline__ uint32_t frame_of(const Pass& ps, uint32_t gp) {
@@ -141,7 +168,7 @@ __global__ void wf_next_pass(Counters* c) {
 
 // Camera rays of the pass's samples (PathSampler camera dimension,
 // PinholeCamera::generate_ray_differential), state init, queue 0 append.
-__global__ __launch_bounds__(256) void wf_generate(Pass ps, State st, Queue q, Counters* c) {
+__global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Queue q, Counters* c) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t k = c->pass;
   for (uint32_t base = blockIdx.x * blockDim.x; base < ps.paths; base += stride) {
@@ -162,7 +189,7 @@ __global__ __launch_bounds__(256) void wf_generate(Pass ps, State st, Queue q, C
         want = true;
       }
     }
-    const uint32_t slot = wave_append(&c->rays[0], want);
+    const uint32_t slot = block_append(&c->rays[0], want, &c->rays[0], false).a;
     if (want) {
       put_ray(q.rays, slot, o, d);
       q.path[slot] = path;
@@ -171,7 +198,7 @@ __global__ __launch_bounds__(256) void wf_generate(Pass ps, State st, Queue q, C
 }
 
 // One bounce of mis::trace_radiance for every path of the input queue.
-__global__ __launch_bounds__(256) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
+__global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
                                                 const TptHit* __restrict__ hits, Queue qout, ShadowQ sq,
                                                 Counters* c, uint32_t in_idx, uint32_t bounce) {
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -257,12 +284,12 @@ __global__ __launch_bounds__(256) void wf_shade(Pass ps, pt::ShadeView sv, State
         store3(st.rad, path, rad);
       }
     }
-    const uint32_t rs = wave_append(&c->rays[in_idx ^ 1u], want_ray);
+    const AppendSlots slots = block_append(&c->rays[in_idx ^ 1u], want_ray, &c->shadow, want_shadow);
+    const uint32_t rs = slots.a, ss = slots.b;
     if (want_ray) {
       put_ray(qout.rays, rs, no, nd);
       qout.path[rs] = path;
     }
-    const uint32_t ss = wave_append(&c->shadow, want_shadow);
     if (want_shadow) {
       put_ray(sq.rays, ss, ns.origin, ns.dir);
       sq.target[2 * ss] = ns.inst;
@@ -329,14 +356,14 @@ namespace {
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 template <typename K>
-unsigned resident_blocks(TptDevice* d, K kern, size_t work) {
+unsigned resident_blocks(TptDevice* d, K kern, size_t work, unsigned block = 256) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 256, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), (int)block, 0) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   const size_t cap = (size_t)per_cu * (size_t)(d->cu_count > 0 ? d->cu_count : 1);
-  const size_t want = (work + 255) / 256;
+  const size_t want = (work + block - 1) / block;
   return (unsigned)std::max<size_t>(1, std::min(cap, want));
 }
 
@@ -356,18 +383,18 @@ struct Plan {
 // The kernels of one pass, all reading their counts on the device.
 bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream_t st) {
   TptDevice* d = s->dev;
-  const unsigned g_gen = resident_blocks(d, wf::wf_generate, p.ps.paths);
-  const unsigned g_shade = resident_blocks(d, wf::wf_shade, p.ps.paths);
+  const unsigned g_gen = resident_blocks(d, wf::wf_generate, p.ps.paths, wf::kAppendBlock);
+  const unsigned g_shade = resident_blocks(d, wf::wf_shade, p.ps.paths, wf::kAppendBlock);
   const unsigned g_add = resident_blocks(d, wf::wf_shadow_add, p.ps.paths);
   const unsigned g_fold = resident_blocks(d, wf::wf_fold, p.ps.total_pixels);
   hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, 0u);
-  hipLaunchKernelGGL(wf::wf_generate, dim3(g_gen), dim3(256), 0, st, p.ps, p.st, p.q[0], p.c);
+  hipLaunchKernelGGL(wf::wf_generate, dim3(g_gen), dim3(wf::kAppendBlock), 0, st, p.ps, p.st, p.q[0], p.c);
   for (uint32_t b = 0; b <= p.max_depth; ++b) {
     const uint32_t a = b & 1u;
     if (!launch_trace_queue(s, 0, p.q8, false, p.q[a].rays, &p.c->rays[a], p.ps.paths, p.hits, nullptr, nullptr, st))
       return false;
     hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, a ^ 1u);
-    hipLaunchKernelGGL(wf::wf_shade, dim3(g_shade), dim3(256), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
+    hipLaunchKernelGGL(wf::wf_shade, dim3(g_shade), dim3(wf::kAppendBlock), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
                        p.q[a ^ 1u], p.sq, p.c, a, b);
     if (b < p.max_depth && (sv.light_count || sv.env.px)) {  // some light can be sampled
       if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow, p.ps.paths, nullptr, p.sq.target,
