@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--render", type=int, default=1, help="also time the Cornell path tracer (1) or not (0)")
     ap.add_argument("--refill", type=int, default=0, help="trace-kernel refill threshold (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident blocks per CU (0: occupancy)")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="tpt_device_set_param before the traversal runs (repeatable)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of this many MiB three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -95,6 +97,9 @@ def main():
         dev.set_param("refill_below", args.refill)
     if args.blocks_per_cu:
         dev.set_param("blocks_per_cu", args.blocks_per_cu)
+    for kv in args.set:
+        name, value = kv.split("=", 1)
+        dev.set_param(name, int(value))
     ds = sc.upload(dev)
     primary = scenes.primary_rays(1920, 1080, **scenes.TRAVERSAL_CAMERA)
     hp = ds.closest_hit(primary)

@@ -21,11 +21,16 @@ bool hip_ok(hipError_t e, const char* what);
 
 // Ray counters of the persistent trace kernels: one per XCD range plus the
 // count of finished waves; the last wave resets them.
-constexpr int kRegions = 8;
+constexpr int kMaxRegions = 64;  // work regions of the persistent trace kernel
+// One counter per 256-byte line: every wave of every XCD updates these with
+// device-scope atomics, and counters sharing a line serialise on it.
+struct alignas(256) WorkLine {
+  uint32_t v;
+  uint32_t pad[63];
+};
 struct WorkCounters {
-  uint32_t next[kRegions];
-  uint32_t done_waves;
-  uint32_t pad[7];
+  WorkLine next[kMaxRegions];
+  WorkLine done_waves;
 };
 
 // A device allocation owned by the current host scope.
@@ -79,6 +84,8 @@ struct TptDevice {
   // (refill 8..40 x blocks/CU 2..5): fewer resident waves than occupancy
   // allows keep more of each wave's node and triangle lines in L2.
   uint32_t refill_below = 24;         // refill a wave when fewer lanes are busy
+  uint32_t refill_grab = 0;           // ray ids a wave takes per work-counter atomic (at least its idle lanes)
+  uint32_t regions = 8;               // work regions (multiple of the 8 XCDs keeps block b on region b's XCD)
   uint32_t blocks_per_cu = 4;         // resident blocks per CU (0: occupancy limit)
   // wavefront path tracer: paths in flight per pass, graph capture of a pass,
   // and its buffers (grow-only)

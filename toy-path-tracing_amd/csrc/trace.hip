@@ -33,8 +33,8 @@
 
 namespace tpt {
 
-// Work distribution.  Rays are split into kRegions contiguous ranges, one per
-// XCD (workgroups are dispatched round-robin over the 8 XCDs, so block b runs
+// Work distribution.  Rays are split into `regions` contiguous ranges (8 by
+// default: one per XCD (workgroups are dispatched round-robin over the 8 XCDs, so block b runs
 // on XCD b % 8): neighbouring rays stay on one XCD's L2.  A wave refills its
 // idle lanes from its range with one atomic per refill; once its range is
 // exhausted it moves on to the next XCD's range.  The last wave to finish
@@ -52,6 +52,8 @@ struct Query {
   const uint32_t* n_dev;  // when set, the ray count is read here (device-resident queue length)
   uint32_t n;
   uint32_t refill_below;  // refill when fewer lanes than this are busy
+  uint32_t grab;          // ray ids taken per work-counter atomic (at least the idle lanes)
+  uint32_t regions;       // work regions, 1..kMaxRegions
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
@@ -154,41 +156,64 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView sc, const TptRa
   Counters cnt = {0, 0, 0};
   const uint32_t lane = lane_id();
   const uint32_t n = q.n_dev ? *q.n_dev : q.n;
-  const uint32_t chunk = (n + kRegions - 1) / kRegions;
-  int region = (int)(blockIdx.x % kRegions);
-  int regions_left = kRegions;
+  const uint32_t chunk = (n + q.regions - 1) / q.regions;
+  uint32_t region = blockIdx.x % q.regions;
+  uint32_t regions_left = q.regions;
   bool active = false;
   bool overflow = false;
   uint32_t rid = 0;
+  uint32_t res_lo = 0, res_hi = 0;  // wave-uniform reserve: ray ids taken, not yet started
   for (;;) {
     uint64_t busy = __ballot(active);
-    if (regions_left > 0 && (uint32_t)__popcll(busy) < q.refill_below) {
+    const uint32_t rem = res_hi - res_lo;
+    if ((regions_left > 0 || rem > 0) && (uint32_t)__popcll(busy) < q.refill_below) {
       const uint64_t idle = ~busy;
       const uint32_t want = (uint32_t)__popcll(idle);
-      const uint32_t lo = (uint32_t)region * chunk;
-      const uint32_t hi = min(n, lo + chunk);
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(&q.work->next[region], want);
-      base = __shfl(base, 0) + lo;
+      uint32_t nb = 0, ne = 0;  // newly taken ids [nb, ne)
+      const bool take = rem < want && regions_left > 0;
+      if (take) {
+        const uint32_t lo = (uint32_t)region * chunk;
+        const uint32_t hi = min(n, lo + chunk);
+        const uint32_t grab = max(want - rem, q.grab);
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&q.work->next[region].v, grab);
+        b = __shfl(b, 0);
+        const uint64_t end = (uint64_t)lo + b + grab;
+        nb = (uint32_t)min<uint64_t>((uint64_t)lo + b, hi);
+        ne = (uint32_t)min<uint64_t>(end, hi);
+        if (end >= hi) {  // this range is used up: move to the next XCD's
+          region = region + 1 == q.regions ? 0u : region + 1;
+          regions_left--;
+        }
+      }
       if (!active) {
+        // lanes of rank < rem start reserved ids, the next ones new ids
         const uint32_t rank =
             __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-        const uint32_t cand = base + rank;
-        if (cand < hi) {
-          rid = cand;
+        bool got = false;
+        if (rank < rem) {
+          rid = res_lo + rank;
+          got = true;
+        } else if (rank - rem < ne - nb) {
+          rid = nb + (rank - rem);
+          got = true;
+        }
+        if (got) {
           begin_ray<kQ8, kMode, kExact, kCount, kTwoLevel>(sc, rays, q, rid, L, st);
           if (kCount) cnt = Counters{0, 0, 0};
           active = true;
         }
       }
-      if (base + want >= hi) {  // this range is used up: move to the next XCD's
-        region = (region + 1) % kRegions;
-        regions_left--;
+      const uint32_t k1 = min(want, rem);
+      res_lo += k1;
+      if (take) {
+        res_lo = nb + min(want - k1, ne - nb);
+        res_hi = ne;
       }
       busy = __ballot(active);
     }
     if (busy == 0) {
-      if (regions_left == 0) break;
+      if (regions_left == 0 && res_hi == res_lo) break;
       continue;
     }
     if (active) {
@@ -205,9 +230,9 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView sc, const TptRa
   if (lane == 0) {
     __threadfence();
     const uint32_t waves = gridDim.x * (kBlock / 64);
-    if (atomicAdd(&q.work->done_waves, 1u) == waves - 1) {
-      for (int r = 0; r < kRegions; ++r) atomicExch(&q.work->next[r], 0u);
-      atomicExch(&q.work->done_waves, 0u);
+    if (atomicAdd(&q.work->done_waves.v, 1u) == waves - 1) {
+      for (uint32_t r = 0; r < q.regions; ++r) atomicExch(&q.work->next[r].v, 0u);
+      atomicExch(&q.work->done_waves.v, 0u);
     }
   }
 }
@@ -368,6 +393,14 @@ int32_t tpt_device_set_param(TptDevice* d, const char* name, int64_t value) {
     return TPT_ERROR_BAD_ARGUMENT;
   }
   const std::string n(name);
+  if (n == "regions" && value >= 1 && value <= kMaxRegions) {
+    d->regions = (uint32_t)value;
+    return TPT_OK;
+  }
+  if (n == "refill_grab" && value >= 0 && value <= 65536) {
+    d->refill_grab = (uint32_t)value;
+    return TPT_OK;
+  }
   if (n == "refill_below" && value >= 1 && value <= 64) {
     d->refill_below = (uint32_t)value;
   } else if (n == "blocks_per_cu" && value >= 0 && value <= 64) {
@@ -830,6 +863,8 @@ int32_t trace_impl(TptScene* s, int query, const TptRay* rays, size_t n, const u
   q.n_dev = nullptr;
   q.n = (uint32_t)n;
   q.refill_below = d->refill_below;
+  q.grab = d->refill_grab;
+  q.regions = d->regions;
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const bool cnt = dcounts != nullptr;
   const void* kern = pick_kernel(query, q8, exact, cnt, two);
@@ -877,6 +912,8 @@ bool launch_trace_queue(TptScene* s, int query, bool q8, bool exact, const TptRa
   q.n_dev = n_dev;
   q.n = n_max;
   q.refill_below = d->refill_below;
+  q.grab = d->refill_grab;
+  q.regions = d->regions;
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const void* kern = pick_kernel(query, q8, exact, false, two);
   const unsigned blocks = grid_blocks(d, kern, n_max);

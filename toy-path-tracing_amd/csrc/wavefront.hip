@@ -46,11 +46,16 @@ struct FrameDev {
   float* out;          // device rgb, w * h * 3
 };
 
+// Queue lengths, one per 256-byte line: every block appends with a
+// device-scope atomic, and counters sharing a line serialise on it.
+struct alignas(256) CountLine {
+  uint32_t v;
+  uint32_t pad[63];
+};
 struct Counters {
-  uint32_t rays[2];  // ping-pong ray-queue lengths
-  uint32_t shadow;   // shadow-queue length
-  uint32_t pass;     // current pass
-  uint32_t pad[4];
+  CountLine rays[2];  // ping-pong ray-queue lengths
+  CountLine shadow;   // shadow-queue length
+  CountLine pass;     // current pass
 };
 
 struct State {
@@ -157,20 +162,20 @@ __device__ __forceinline__ void put_ray(TptRay* r, uint32_t slot, F3 o, F3 d) {
 // Resets one ray-queue length and the shadow-queue length.
 __global__ void wf_reset(Counters* c, uint32_t which) {
   if (threadIdx.x == 0) {
-    c->rays[which] = 0;
-    c->shadow = 0;
+    c->rays[which].v = 0;
+    c->shadow.v = 0;
   }
 }
 
 __global__ void wf_next_pass(Counters* c) {
-  if (threadIdx.x == 0) c->pass += 1;
+  if (threadIdx.x == 0) c->pass.v += 1;
 }
 
 // Camera rays of the pass's samples (PathSampler camera dimension,
 // PinholeCamera::generate_ray_differential), state init, queue 0 append.
 __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Queue q, Counters* c) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t k = c->pass;
+  const uint32_t k = c->pass.v;
   for (uint32_t base = blockIdx.x * blockDim.x; base < ps.paths; base += stride) {
     const uint32_t path = base + threadIdx.x;
     bool want = false;
@@ -189,7 +194,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Q
         want = true;
       }
     }
-    const uint32_t slot = block_append(&c->rays[0], want, &c->rays[0], false).a;
+    const uint32_t slot = block_append(&c->rays[0].v, want, &c->rays[0].v, false).a;
     if (want) {
       put_ray(q.rays, slot, o, d);
       q.path[slot] = path;
@@ -202,8 +207,8 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
                                                 const TptHit* __restrict__ hits, Queue qout, ShadowQ sq,
                                                 Counters* c, uint32_t in_idx, uint32_t bounce) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t n = c->rays[in_idx];
-  const uint32_t k = c->pass;
+  const uint32_t n = c->rays[in_idx].v;
+  const uint32_t k = c->pass.v;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t slot = base + threadIdx.x;
     bool want_ray = false, want_shadow = false;
@@ -284,7 +289,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
         store3(st.rad, path, rad);
       }
     }
-    const AppendSlots slots = block_append(&c->rays[in_idx ^ 1u], want_ray, &c->shadow, want_shadow);
+    const AppendSlots slots = block_append(&c->rays[in_idx ^ 1u].v, want_ray, &c->shadow.v, want_shadow);
     const uint32_t rs = slots.a, ss = slots.b;
     if (want_ray) {
       put_ray(qout.rays, rs, no, nd);
@@ -302,7 +307,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
 
 // Unoccluded NEE contributions into the path radiance (one per path per bounce).
 __global__ __launch_bounds__(256) void wf_shadow_add(State st, ShadowQ sq, const Counters* c) {
-  const uint32_t n = c->shadow;
+  const uint32_t n = c->shadow.v;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (!sq.unocc[i]) continue;
     const uint32_t path = sq.path[i];
@@ -312,7 +317,7 @@ __global__ __launch_bounds__(256) void wf_shadow_add(State st, ShadowQ sq, const
 
 // Running mean over the pass's samples, in sample order.
 __global__ __launch_bounds__(256) void wf_fold(Pass ps, State st, float* __restrict__ color, const Counters* c) {
-  const uint32_t k = c->pass;
+  const uint32_t k = c->pass.v;
   for (uint32_t gp = blockIdx.x * blockDim.x + threadIdx.x; gp < ps.total_pixels; gp += gridDim.x * blockDim.x) {
     const FrameDev& fr = ps.frames[frame_of(ps, gp)];
     F3 col = load3f(color, gp);
@@ -343,7 +348,12 @@ __global__ void wf_zero_u32(uint32_t* a) {
   if (threadIdx.x == 0) a[0] = 0u;
 }
 __global__ void wf_zero_counters(Counters* c) {
-  if (threadIdx.x < sizeof(Counters) / 4) reinterpret_cast<uint32_t*>(c)[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) {
+    c->rays[0].v = 0u;
+    c->rays[1].v = 0u;
+    c->shadow.v = 0u;
+    c->pass.v = 0u;
+  }
 }
 
 }  // namespace wf
@@ -391,13 +401,13 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
   hipLaunchKernelGGL(wf::wf_generate, dim3(g_gen), dim3(wf::kAppendBlock), 0, st, p.ps, p.st, p.q[0], p.c);
   for (uint32_t b = 0; b <= p.max_depth; ++b) {
     const uint32_t a = b & 1u;
-    if (!launch_trace_queue(s, 0, p.q8, false, p.q[a].rays, &p.c->rays[a], p.ps.paths, p.hits, nullptr, nullptr, st))
+    if (!launch_trace_queue(s, 0, p.q8, false, p.q[a].rays, &p.c->rays[a].v, p.ps.paths, p.hits, nullptr, nullptr, st))
       return false;
     hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, a ^ 1u);
     hipLaunchKernelGGL(wf::wf_shade, dim3(g_shade), dim3(wf::kAppendBlock), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
                        p.q[a ^ 1u], p.sq, p.c, a, b);
     if (b < p.max_depth && (sv.light_count || sv.env.px)) {  // some light can be sampled
-      if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow, p.ps.paths, nullptr, p.sq.target,
+      if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow.v, p.ps.paths, nullptr, p.sq.target,
                               p.sq.unocc, st))
         return false;
       hipLaunchKernelGGL(wf::wf_shadow_add, dim3(g_add), dim3(256), 0, st, p.st, p.sq, (const wf::Counters*)p.c);
