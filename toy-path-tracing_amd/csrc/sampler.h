@@ -92,16 +92,6 @@ __device__ __forceinline__ void sobol_point(uint32_t index, uint32_t seed, int d
   }
 }
 
-// The 24 permutations of four base-4 digits, in the reference's table order
-// (R/sampler/zsobol.rs), packed 8 bits per permutation (the image of digit d
-// in bits 2d..2d+1), permutations 0-7, 8-15 and 16-23 in three words.  A
-// lookup is a select and a shift: no memory access inside the digit loop
-// (a per-lane indexed table load there stalls the wave once per digit).
-__host__ __device__ __forceinline__ uint32_t permute_digit(uint32_t p, uint32_t digit) {
-  const uint64_t w = p < 8u ? 0xb1e19c6c78d8b4e4ull : (p < 16u ? 0x72d236c68d2d39c9ull : 0x93634b1b87271e4eull);
-  return (uint32_t)(w >> (((p & 7u) << 3) | (digit << 1))) & 3u;
-}
-
 // (h >> 24) % 24 in 32-bit arithmetic: h >> 24 = xh * 2^32 + xl with
 // xh < 256, and 2^32 = 16 (mod 24).
 __host__ __device__ __forceinline__ uint32_t top40_mod24(uint64_t h) {
@@ -134,19 +124,70 @@ __host__ __device__ inline ZSobolFrame zsobol_frame(uint32_t spp, uint32_t width
   return f;
 }
 
+// The 24 permutations of four base-4 digits, in the reference's table order
+// (R/sampler/zsobol.rs), packed 8 bits per permutation (the image of digit d
+// in bits 2d..2d+1), permutations 0-7, 8-15 and 16-23 in three 64-bit
+// words: perm_byte(p) is a select and a shift, not a per-lane table load
+// inside the digit loop.
+__host__ __device__ __forceinline__ uint32_t perm_byte(uint32_t p) {
+  const uint64_t w = p < 8u ? 0xb1e19c6c78d8b4e4ull : (p < 16u ? 0x72d236c68d2d39c9ull : 0x93634b1b87271e4eull);
+  return (uint32_t)(w >> ((p & 7u) << 3)) & 0xffu;
+}
+
+// Permutation byte of one digit: hash of the higher digits (value `higher`)
+// and the dimension hash dh.
+__host__ __device__ __forceinline__ uint32_t digit_perm_byte(uint64_t higher, uint64_t dh) {
+  return perm_byte(top40_mod24(mix64(higher ^ dh)));
+}
+
+// Digit-permutation rows.  For the top kZTabDigits base-4 digits of any
+// ZSobol index the higher digits form a value below kZTabPrefixes, so their
+// permutations depend on (dimension, value) only: row `dim` holds
+// digit_perm_byte(v, 0x55555555 * dim) for every v < kZTabPrefixes, and a
+// lookup replaces a 64-bit hash for those digits.
+constexpr uint32_t kZTabDigits = 6;
+constexpr uint32_t kZTabPrefixes = 1u << (2 * (kZTabDigits - 1));
+
 // Sample index of (pixel, sample, dimension): Morton-ordered pixel + sample,
 // each base-4 digit permuted by a hash of the higher digits and the dimension.
+// row: the dimension's digit-permutation row, or null (every digit hashed).
 __host__ __device__ __forceinline__ uint64_t zsobol_index(const ZSobolFrame& f, uint32_t px, uint32_t py, uint32_t si,
-                                                 uint32_t dim) {
+                                                          uint32_t dim, const uint8_t* row = nullptr) {
   const uint64_t m = (morton2(px, py) << f.log2_spp) | (uint64_t)si;
   const bool odd = (f.log2_spp & 1u) != 0;
+  const uint32_t sub = odd ? 1u : 0u;
   const uint64_t dh = 0x55555555ull * (uint64_t)dim;
+  const int last = odd ? 1 : 0;
+  const int top = (int)f.n_digits - 1;
+  int i = top;
   uint64_t out = 0;
-  for (int i = (int)f.n_digits - 1; i >= (odd ? 1 : 0); --i) {
-    const uint32_t sh = 2u * (uint32_t)i - (odd ? 1u : 0u);
+  if (row) {
+    // m < 2^(2 n_digits - sub), so digit i's higher digits are below 4^(top - i)
+    if (top - (int)kZTabDigits + 1 >= last) {
+      // the usual case, unrolled: the row reads issue together
+      uint32_t b[kZTabDigits];
+#pragma unroll
+      for (int j = 0; j < (int)kZTabDigits; ++j) b[j] = row[(uint32_t)(m >> (2u * (uint32_t)(top - j) - sub + 2u))];
+#pragma unroll
+      for (int j = 0; j < (int)kZTabDigits; ++j) {
+        const uint32_t sh = 2u * (uint32_t)(top - j) - sub;
+        out |= (uint64_t)((b[j] >> (2u * (uint32_t)((m >> sh) & 3u))) & 3u) << sh;
+      }
+      i = top - (int)kZTabDigits;
+    } else {
+      for (; i >= last; --i) {
+        const uint32_t sh = 2u * (uint32_t)i - sub;
+        const uint32_t digit = (uint32_t)((m >> sh) & 3u);
+        const uint32_t b = row[(uint32_t)(m >> (sh + 2))];
+        out |= (uint64_t)((b >> (2u * digit)) & 3u) << sh;
+      }
+    }
+  }
+  for (; i >= last; --i) {
+    const uint32_t sh = 2u * (uint32_t)i - sub;
     const uint32_t digit = (uint32_t)((m >> sh) & 3u);
-    const uint32_t p = top40_mod24(mix64((m >> (sh + 2)) ^ dh));
-    out |= (uint64_t)permute_digit(p, digit) << sh;
+    const uint32_t b = digit_perm_byte(m >> (sh + 2), dh);
+    out |= (uint64_t)((b >> (2u * digit)) & 3u) << sh;
   }
   if (odd) out |= (m & 1u) ^ (mix64((m >> 1) ^ dh) & 1u);
   return out;
@@ -155,19 +196,22 @@ __host__ __device__ __forceinline__ uint64_t zsobol_index(const ZSobolFrame& f, 
 __host__ __device__ __forceinline__ uint32_t fold64(uint64_t v) { return (uint32_t)v ^ (uint32_t)(v >> 32); }
 
 // PathSampler: one pixel sample's view of the sequence (scramble seed 0).
+// row: the dimension's digit-permutation row (null: hash every digit).
 struct PathSampler {
   ZSobolFrame f;
   uint32_t px, py, si;
-  __device__ __forceinline__ uint32_t index(uint32_t dim) const { return fold64(zsobol_index(f, px, py, si, dim)); }
+  __device__ __forceinline__ uint32_t index(uint32_t dim, const uint8_t* row = nullptr) const {
+    return fold64(zsobol_index(f, px, py, si, dim, row));
+  }
   __device__ __forceinline__ static uint32_t seed(uint32_t dim) { return fold64(mix64((uint64_t)dim)); }
-  __device__ __forceinline__ float s1(uint32_t dim) const {
+  __device__ __forceinline__ float s1(uint32_t dim, const uint8_t* row = nullptr) const {
     uint32_t v[2];
-    sobol_point(index(dim), seed(dim), 1, v);
+    sobol_point(index(dim, row), seed(dim), 1, v);
     return u32_to_unit(v[0]);
   }
-  __device__ __forceinline__ void s2(uint32_t dim, float* o) const {
+  __device__ __forceinline__ void s2(uint32_t dim, float* o, const uint8_t* row = nullptr) const {
     uint32_t v[2];
-    sobol_point(index(dim), seed(dim), 2, v);
+    sobol_point(index(dim, row), seed(dim), 2, v);
     o[0] = u32_to_unit(v[0]);
     o[1] = u32_to_unit(v[1]);
   }

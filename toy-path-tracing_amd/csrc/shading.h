@@ -1028,9 +1028,9 @@ struct CamView {
   float tan_half, exposure;
 };
 __device__ __forceinline__ F3 camera_dir(const CamView& cam, const smp::PathSampler& ps, uint32_t x, uint32_t y,
-                                         uint32_t w, uint32_t h) {
+                                         uint32_t w, uint32_t h, const uint8_t* row = nullptr) {
   float u[2];
-  ps.s2(smp::kCameraDim, u);
+  ps.s2(smp::kCameraDim, u, row);
   const float rx = (float)w, ry = (float)h, aspect = rx / ry;
   const float px = ((float)x + 0.5f) + 0.5f * smp::tent(u[0]);
   const float py = ((float)y + 0.5f) + 0.5f * smp::tent(u[1]);

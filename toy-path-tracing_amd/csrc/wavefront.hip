@@ -171,9 +171,36 @@ __global__ void wf_next_pass(Counters* c) {
   if (threadIdx.x == 0) c->pass.v += 1;
 }
 
+// Digit-permutation rows (smp::zsobol_index) of dimensions 0..dims-1.
+__global__ void wf_ztab(uint8_t* __restrict__ tab, uint32_t dims) {
+  const uint32_t n = dims * smp::kZTabPrefixes;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t dim = i / smp::kZTabPrefixes, v = i % smp::kZTabPrefixes;
+    tab[i] = (uint8_t)smp::digit_perm_byte(v, 0x55555555ull * dim);
+  }
+}
+
+// Row `dim` of the digit-permutation table into LDS: copied when the table
+// holds it, computed otherwise.  All threads of the block call it.
+__device__ __forceinline__ void stage_row(uint8_t* lds, const uint8_t* __restrict__ tab, uint32_t tab_dims,
+                                          uint32_t dim) {
+  if (dim < tab_dims) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(tab + (size_t)dim * smp::kZTabPrefixes);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(lds);
+    for (uint32_t i = threadIdx.x; i < smp::kZTabPrefixes / 4; i += blockDim.x) dst[i] = src[i];
+  } else {
+    for (uint32_t v = threadIdx.x; v < smp::kZTabPrefixes; v += blockDim.x)
+      lds[v] = (uint8_t)smp::digit_perm_byte(v, 0x55555555ull * dim);
+  }
+}
+
 // Camera rays of the pass's samples (PathSampler camera dimension,
 // PinholeCamera::generate_ray_differential), state init, queue 0 append.
-__global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Queue q, Counters* c) {
+__global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Queue q, Counters* c,
+                                                            const uint8_t* __restrict__ ztab, uint32_t ztab_dims) {
+  __shared__ __align__(16) uint8_t zrow[smp::kZTabPrefixes];
+  stage_row(zrow, ztab, ztab_dims, smp::kCameraDim);
+  __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t k = c->pass.v;
   for (uint32_t base = blockIdx.x * blockDim.x; base < ps.paths; base += stride) {
@@ -189,7 +216,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Q
         const uint32_t lp = gp - fr.pix_begin, x = lp % fr.w, y = lp / fr.w;
         const smp::PathSampler smpl{fr.zf, x, y, si};
         o = fr.cam.eye;
-        d = pt::camera_dir(fr.cam, smpl, x, y, fr.w, fr.h);
+        d = pt::camera_dir(fr.cam, smpl, x, y, fr.w, fr.h, zrow);
         store3(st.thr, path, f3(1.0f, 1.0f, 1.0f));
         want = true;
       }
@@ -205,7 +232,16 @@ __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Q
 // One bounce of mis::trace_radiance for every path of the input queue.
 __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
                                                 const TptHit* __restrict__ hits, Queue qout, ShadowQ sq,
-                                                Counters* c, uint32_t in_idx, uint32_t bounce) {
+                                                Counters* c, uint32_t in_idx, uint32_t bounce,
+                                                const uint8_t* __restrict__ ztab, uint32_t ztab_dims) {
+  // digit-permutation rows of this bounce's sampler dimensions (vertex base +0, +1, +3, +5, +7, +13)
+  __shared__ __align__(16) uint8_t zrows[6][smp::kZTabPrefixes];
+  {
+    const uint32_t bd = smp::kVertexBase + bounce * smp::kVertexStride;
+    const uint32_t off[6] = {0, 1, 3, 5, 7, 13};
+    for (int r = 0; r < 6; ++r) stage_row(zrows[r], ztab, ztab_dims, bd + off[r]);
+    __syncthreads();
+  }
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t n = c->rays[in_idx].v;
   const uint32_t k = c->pass.v;
@@ -254,8 +290,8 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
           const uint32_t bd = smp::kVertexBase + bounce * smp::kVertexStride;
           if (m.kind != TPT_MATERIAL_EMISSIVE) {
             float us[2];
-            smpl.s2(bd + 3, us);
-            if (pt::nee_sample(sv, m, vtx, smpl.s1(bd + 0), smpl.s1(bd + 1), us[0], us[1], &ns)) {
+            smpl.s2(bd + 3, us, zrows[2]);
+            if (pt::nee_sample(sv, m, vtx, smpl.s1(bd + 0, zrows[0]), smpl.s1(bd + 1, zrows[1]), us[0], us[1], &ns)) {
               want_shadow = true;
               nee_value = pt::hadamard(thr, ns.value);
             }
@@ -263,13 +299,13 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
           pt::Sample smp_;
           F3 wi;
           float ud[2];
-          smpl.s2(bd + 7, ud);
-          if (pt::mat_sample(m, vtx, smpl.s1(bd + 5), ud[0], ud[1], &smp_, &wi)) {
+          smpl.s2(bd + 7, ud, zrows[4]);
+          if (pt::mat_sample(m, vtx, smpl.s1(bd + 5, zrows[3]), ud[0], ud[1], &smp_, &wi)) {
             thr = pt::hadamard(thr, smp_.weight);
             bool live = true;
             if (bounce + 1 >= 4) {
               const float qs = pt::clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.05f, 1.0f);
-              if (smpl.s1(bd + 13) > qs)
+              if (smpl.s1(bd + 13, zrows[5]) > qs)
                 live = false;
               else
                 thr = pt::divs(thr, qs);
@@ -383,6 +419,8 @@ struct Plan {
   wf::Queue q[2];
   wf::ShadowQ sq;
   TptHit* hits;
+  uint8_t* ztab;      // digit-permutation rows
+  uint32_t ztab_dims;
   wf::Counters* c;
   float* color;
   uint32_t max_depth;
@@ -398,14 +436,15 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
   const unsigned g_add = resident_blocks(d, wf::wf_shadow_add, p.ps.paths);
   const unsigned g_fold = resident_blocks(d, wf::wf_fold, p.ps.total_pixels);
   hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, 0u);
-  hipLaunchKernelGGL(wf::wf_generate, dim3(g_gen), dim3(wf::kAppendBlock), 0, st, p.ps, p.st, p.q[0], p.c);
+  hipLaunchKernelGGL(wf::wf_generate, dim3(g_gen), dim3(wf::kAppendBlock), 0, st, p.ps, p.st, p.q[0], p.c,
+                     (const uint8_t*)p.ztab, p.ztab_dims);
   for (uint32_t b = 0; b <= p.max_depth; ++b) {
     const uint32_t a = b & 1u;
     if (!launch_trace_queue(s, 0, p.q8, false, p.q[a].rays, &p.c->rays[a].v, p.ps.paths, p.hits, nullptr, nullptr, st))
       return false;
     hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, a ^ 1u);
     hipLaunchKernelGGL(wf::wf_shade, dim3(g_shade), dim3(wf::kAppendBlock), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
-                       p.q[a ^ 1u], p.sq, p.c, a, b);
+                       p.q[a ^ 1u], p.sq, p.c, a, b, (const uint8_t*)p.ztab, p.ztab_dims);
     if (b < p.max_depth && (sv.light_count || sv.env.px)) {  // some light can be sampled
       if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow.v, p.ps.paths, nullptr, p.sq.target,
                               p.sq.unocc, st))
@@ -471,6 +510,10 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     const size_t o_hits = take(sizeof(TptHit) * P);
     const size_t o_sr = take(sizeof(TptRay) * P), o_st = take(8 * P), o_sv = take(12 * P), o_sp = take(4 * P);
     const size_t o_su = take(P);
+    // digit-permutation rows of every sampler dimension a path of up to 256 bounces reads
+    // (deeper bounces compute their rows in the kernel)
+    const uint32_t ztab_dims = smp::kVertexBase + smp::kVertexStride * (std::min<uint32_t>(max_depth, 256u) + 1);
+    const size_t o_ztab = take((size_t)ztab_dims * smp::kZTabPrefixes);
     if (!d->wf_buf.ensure(off)) return TPT_ERROR_DEVICE;
     char* base = (char*)d->wf_buf.ptr;
     // frame table
@@ -511,6 +554,8 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.sq = wf::ShadowQ{(TptRay*)(base + o_sr), (uint32_t*)(base + o_st), (float*)(base + o_sv),
                        (uint32_t*)(base + o_sp), (uint8_t*)(base + o_su)};
     p.hits = (TptHit*)(base + o_hits);
+    p.ztab = (uint8_t*)(base + o_ztab);
+    p.ztab_dims = ztab_dims;
     p.c = (wf::Counters*)(base + o_cnt);
     p.color = (float*)(base + o_color);
     p.max_depth = max_depth;
@@ -522,6 +567,8 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
                 "hipMemcpyAsync(frames)"))
       return TPT_ERROR_DEVICE;
     hipLaunchKernelGGL(wf::wf_zero_counters, dim3(1), dim3(64), 0, st, p.c);
+    hipLaunchKernelGGL(wf::wf_ztab, dim3(resident_blocks(d, wf::wf_ztab, (size_t)ztab_dims * smp::kZTabPrefixes)),
+                       dim3(256), 0, st, p.ztab, ztab_dims);
     hipLaunchKernelGGL(wf::wf_zero, dim3(resident_blocks(d, wf::wf_zero, 3 * total)), dim3(256), 0, st, p.color,
                        (size_t)(3 * total));
     if (!dev_ptr) hipLaunchKernelGGL(wf::wf_zero_u32, dim3(1), dim3(64), 0, st, d->status);
