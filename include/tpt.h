@@ -106,10 +106,14 @@ typedef struct TptInstanceDesc {
 } TptInstanceDesc;
 
 /* OpenPbrMaterial parameters (R/material/open_pbr.rs:23-82) as the renderer
- * holds them after prepare_color (rendering-space colours).  tpt_render
- * supports specular_weight = 0 with no metal, coat, fuzz, transmission,
- * thin film or thin walls; other values are rejected with
- * TPT_ERROR_BAD_ARGUMENT.  Carried inside TptMaterial. */
+ * holds them after prepare_color (rendering-space colours): base, metal,
+ * specular, coat, fuzz and emission.  Materials with transmission, thin
+ * film or thin walls are rejected with TPT_ERROR_BAD_ARGUMENT (textures,
+ * normal maps and subsurface are not part of the record).  The coat and
+ * fuzz fields follow the reference's field order after thin_walled; the
+ * reference's defaults (OpenPbrMaterial::new) are coat_color 1,
+ * coat_roughness 0, coat_ior 1.6, coat_darkening 1, fuzz_color 1,
+ * fuzz_roughness 0.5.  Carried inside TptMaterial. */
 typedef struct TptOpenPbr {
   float base_weight;
   float base_color[3];
@@ -127,7 +131,15 @@ typedef struct TptOpenPbr {
   float emission_luminance;
   float emission_color[3];
   uint32_t thin_walled;
-} TptOpenPbr; /* 88 bytes */
+  float coat_color[3];
+  float coat_roughness;
+  float coat_roughness_anisotropy;
+  float coat_ior;
+  float coat_darkening;
+  float fuzz_color[3];
+  float fuzz_roughness;
+  float pad;
+} TptOpenPbr; /* 136 bytes */
 
 /* One entry of the scene's material table: the Material enum of
  * R/material/mod.rs:100-115 for the variants the backend carries.
@@ -148,9 +160,9 @@ typedef struct TptMaterial {
     struct { float rho[3]; } lambert;
     struct { float color[3]; float strength; } emissive;
     struct { float color[3]; } mirror;
-    float raw[22];
+    float raw[34];
   } u;
-} TptMaterial; /* 96 bytes */
+} TptMaterial; /* 144 bytes */
 
 /* Scene after Scene::build_qbvh (R/scene/mod.rs:305-316). */
 typedef struct TptSceneDesc {
@@ -386,6 +398,34 @@ typedef struct TptTuneResult {
 } TptTuneResult;
 int32_t tpt_autotune_render(TptScene* scene, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags,
                             uint32_t tune_spp, TptTuneResult* out);
+
+/* ---- material BSDF queries ------------------------------------------------------- */
+/* One query of a material's BSDF in the shading frame (z = shading normal):
+ * the outgoing direction wo, an incident direction wi for eval and pdf, and
+ * the randoms u = (u_lobe, u0, u1) for a sample (MaterialSampleRandoms'
+ * u_lobe and u_dir); front_face as the shading vertex has it. */
+typedef struct TptBsdfQuery {
+  float wo[3];
+  float wi[3];
+  float u[3];
+  uint32_t material;
+  uint32_t front_face;
+  uint32_t pad;
+} TptBsdfQuery; /* 48 bytes */
+typedef struct TptBsdfResult {
+  float f[3];       /* eval(wo, wi), radiance mode, before the shading-normal correction */
+  float pdf;        /* pdf(wo, wi) */
+  float weight[3];  /* sample(wo, u): f |cos| / pdf, or the delta lobe's weight */
+  float sample_pdf; /* its pdf (the lobe probability for a delta sample) */
+  float wi[3];      /* its direction */
+  uint32_t flags;   /* bit 0: a sample was produced; bit 1: it is a delta sample */
+} TptBsdfResult; /* 48 bytes */
+/* The BSDF the path tracer builds for a material (Material::eval / pdf /
+ * sample, R/material/mod.rs:251-370, at the BSDF level: OpenPbrBsdf
+ * R/bsdf/open_pbr.rs:224-331 after OpenPbrMaterial::make_bsdf
+ * R/material/open_pbr.rs:393-487, NormalizedLambertBsdf, MirrorBsdf), on the
+ * device, for n queries.  Host or device pointers per flags. */
+int32_t tpt_material_bsdf(TptScene* scene, const TptBsdfQuery* queries, size_t n, TptBsdfResult* out, uint32_t flags);
 
 /* The traversal of tpt_trace_closest (query 0) or tpt_trace_any (query 1)
  * with per-ray work counters instead of results: counts[3*i+0] nodes

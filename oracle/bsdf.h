@@ -1,0 +1,1464 @@
+// Oracle: BSDFs of the reference for the carried materials, restated line by
+// line, with the GGX tables they read.  TEST INFRASTRUCTURE ONLY.
+//
+//   R/math/orthonormal_basis.rs          ONB (Duff et al.)
+//   R/bsdf/smith_ggx.rs                  GGX D / Lambda / G1 / G2, VNDF and bounded VNDF
+//   R/math/mod.rs:26-72                  cosine hemisphere, dielectric Fresnel
+//   R/bsdf/eon.rs                        EON diffuse
+//   R/bsdf/directional_albedo.rs         dielectric GGX directional albedo table, conductor and
+//                                        dielectric energy-compensation tables (host builds, lookups)
+//   R/bsdf/dielectric_ggx.rs:121-587     the dielectric GGX eval / rough sampling those builds use
+//   R/bsdf/open_pbr.rs                   OpenPbrBsdf: coat, F82 metal, dielectric specular with
+//                                        multiple-scattering compensation, fuzz (LTC), EON diffuse
+//   R/material/open_pbr.rs:393-941       OpenPbrMaterial::make_bsdf, roughness coupling, coat darkening
+//
+// Carried OpenPBR parameters: base, metalness, specular (weight, colour,
+// roughness, anisotropy, IOR), coat (weight, colour, roughness, anisotropy,
+// IOR, darkening), fuzz (weight, colour, roughness), diffuse roughness and
+// emission.  Transmission, thin film, thin walls, subsurface, textures and
+// normal maps are not carried (the backend rejects such materials), so the
+// BTDF lobes, dispersion and thin-film Fresnel never run.
+//
+// Transcendental functions (sin, cos, atan2, exp, log, pow) are the host
+// libm's, as the reference's f32 methods are on Linux.  f32::powi of a
+// run-time base is LLVM's binary-exponentiation expansion (powi_rt); of
+// compile-time constants LLVM folds it through the double pow (powi_const).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include "geometry.h"
+
+namespace orc {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTau = 6.28318530717958647692f;
+
+// ---- glam-style helpers --------------------------------------------------
+inline V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+inline V3 vmul(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline V3 smul(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline V3 sdiv(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+inline float len_sq(V3 a) { return dot(a, a); }
+inline V3 normalize3(V3 a) {  // v * (1 / |v|)
+  float r = 1.0f / std::sqrt(dot(a, a));
+  return v3(a.x * r, a.y * r, a.z * r);
+}
+inline V3 normalize_or_zero3(V3 a) {
+  float r = 1.0f / std::sqrt(dot(a, a));
+  if (std::isfinite(r) && r > 0.0f) return v3(a.x * r, a.y * r, a.z * r);
+  return v3(0.f, 0.f, 0.f);
+}
+inline float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }  // f32::clamp
+inline V3 face_forward(V3 n, V3 ref) { return dot(n, ref) < 0.0f ? neg(n) : n; }  // R/math/mod.rs:42-48
+inline float luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }  // R/math/sg.rs:314-316
+inline V3 interp3(float b0, float b1, float b2, V3 a, V3 b, V3 c) {
+  return add(add(mul(b0, a), mul(b1, b)), mul(b2, c));
+}
+
+// ---- OrthonormalBasis, R/math/orthonormal_basis.rs:11-71 -------------------
+struct Onb {
+  V3 t, b, n;
+};
+inline Onb onb_from_normal(V3 normal) {
+  V3 n = len_sq(normal) > 0.0f ? normalize3(normal) : v3(0.f, 0.f, 1.f);
+  float sign = std::copysign(1.0f, n.z);
+  float a = -1.0f / (sign + n.z);
+  float b = n.x * n.y * a;
+  V3 t = v3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  V3 bt = v3(b, sign + n.y * n.y * a, -n.y);
+  return Onb{t, bt, n};
+}
+inline Onb onb_from_normal_and_tangent(V3 normal, V3 hint) {
+  V3 n = len_sq(normal) > 0.0f ? normalize3(normal) : v3(0.f, 0.f, 1.f);
+  V3 t = normalize_or_zero3(sub(hint, mul(dot(hint, n), n)));
+  if (len_sq(t) == 0.0f) return onb_from_normal(n);
+  V3 b = normalize_or_zero3(cross(n, t));
+  t = normalize_or_zero3(cross(b, n));
+  return Onb{t, b, n};
+}
+inline V3 onb_to_world(const Onb& f, V3 l) { return normalize3(add(add(mul(l.x, f.t), mul(l.y, f.b)), mul(l.z, f.n))); }
+inline V3 onb_to_local(const Onb& f, V3 w) { return v3(dot(w, f.t), dot(w, f.b), dot(w, f.n)); }
+
+// ---- GGX, R/bsdf/smith_ggx.rs ------------------------------------------------
+constexpr float kMinAlpha = 1.0e-4f;
+constexpr float kSmoothAlpha = 1.0e-3f;
+inline bool reflection_half_vector(V3 wo, V3 wi, V3* wm) {
+  V3 m = normalize_or_zero3(add(wo, wi));
+  if (len_sq(m) == 0.0f) return false;
+  *wm = m.z < 0.0f ? neg(m) : m;
+  return true;
+}
+inline float ggx_d(V3 m, float ax, float ay) {
+  if (m.z <= 0.0f) return 0.0f;
+  float term = m.x * m.x / (ax * ax) + m.y * m.y / (ay * ay) + m.z * m.z;
+  float denom = kPi * ax * ay * term * term;
+  return denom <= 0.0f ? 0.0f : 1.0f / denom;
+}
+inline float ggx_lambda(V3 w, float ax, float ay) {
+  float c = std::fabs(w.z);
+  if (c <= 0.0f) return INFINITY;
+  float term = 1.0f + (ax * ax * w.x * w.x + ay * ay * w.y * w.y) / (c * c);
+  return 0.5f * (-1.0f + std::sqrt(term));
+}
+inline float ggx_g2(V3 wo, V3 wi, float ax, float ay) {
+  float lo = ggx_lambda(wo, ax, ay), li = ggx_lambda(wi, ax, ay);
+  if (!std::isfinite(lo) || !std::isfinite(li)) return 0.0f;
+  return 1.0f / (1.0f + lo + li);
+}
+inline float pdf_wm_bounded_vndf(V3 wo, V3 wm, float ax, float ay) {
+  if (wo.z <= 0.0f || wm.z <= 0.0f) return 0.0f;
+  ax = std::fmax(ax, kMinAlpha);
+  ay = std::fmax(ay, kMinAlpha);
+  float a = clampf(std::fmin(ax, ay), 0.0f, 1.0f);
+  float s = 1.0f + std::sqrt(wo.x * wo.x + wo.y * wo.y);
+  float a2 = a * a, s2 = s * s;
+  float k = (1.0f - a2) * s2 / (s2 + a2 * wo.z * wo.z);
+  float t = std::sqrt(ax * ax * wo.x * wo.x + ay * ay * wo.y * wo.y + wo.z * wo.z);
+  float denom = k * wo.z + t;
+  if (denom <= 0.0f) return 0.0f;
+  return 2.0f * ggx_d(wm, ax, ay) * std::fmax(dot(wo, wm), 0.0f) / denom;
+}
+
+// ---- fresnel, R/math/mod.rs:50-72 -----------------------------------------------
+inline float fresnel_dielectric(float ci, float ei, float et) {
+  ci = clampf(ci, -1.0f, 1.0f);
+  if (ci < 0.0f) {
+    std::swap(ei, et);
+    ci = -ci;
+  }
+  float si = std::sqrt(std::fmax(1.0f - ci * ci, 0.0f));
+  float st = (ei / et) * si;
+  if (st >= 1.0f) return 1.0f;
+  float ct = std::sqrt(std::fmax(1.0f - st * st, 0.0f));
+  float rpar = (et * ci - ei * ct) / (et * ci + ei * ct);
+  float rper = (ei * ci - et * ct) / (ei * ci + et * ct);
+  return 0.5f * (rpar * rpar + rper * rper);
+}
+// weighted_fresnel_ratio, R/bsdf/open_pbr.rs:2161-2172
+inline float weighted_fresnel_ratio(float c, float eti, float etp) {
+  eti = std::fmax(eti, 1.0e-4f);
+  etp = std::fmax(etp, 1.0e-4f);
+  if (etp >= 1.0f) return fresnel_dielectric(c, 1.0f, etp);
+  float cc = clampf(c, 0.0f, 1.0f);
+  float mu2 = 1.0f - (1.0f - cc * cc) / (eti * eti);
+  if (mu2 <= 0.0f) return 1.0f;
+  return fresnel_dielectric(std::sqrt(mu2), 1.0f, 1.0f / etp);
+}
+// f_avg_dielectric, R/bsdf/open_pbr.rs:2127-2135
+inline float f_avg_dielectric(float eta) {
+  if (eta >= 1.0f) return clampf((eta - 1.0f) / (4.08567f + 1.00071f * eta), 0.0f, 1.0f);
+  float e = std::fmax(eta, 1.0e-6f);
+  return clampf(0.997118f + 0.1014f * e - 0.965241f * e * e - 0.130607f * e * e * e, 0.0f, 1.0f);
+}
+
+// ---- EON, R/bsdf/eon.rs:12-197 -----------------------------------------------
+constexpr float kRcpPi = 1.0f / kPi;
+const float kC1Fon = 0.5f - 2.0f / (3.0f * kPi);
+const float kC2Fon = 2.0f / 3.0f - 28.0f / (15.0f * kPi);
+inline float e_fon_approx(float mu, float r) {
+  float mc = 1.0f - mu;
+  float g = mc * (0.05710853f + mc * (0.49188188f + mc * (-0.33218145f + mc * 0.071443f)));
+  return (1.0f + r * g) / (1.0f + kC1Fon * r);
+}
+inline V3 f_eon(V3 rho, float r, V3 wo, V3 wi) {
+  float mo = wo.z, mi = wi.z;
+  float s = dot(wo, wi) - mi * mo;
+  float sot = s > 0.0f ? s / std::fmax(mo, mi) : s;
+  float af = 1.0f / (1.0f + kC1Fon * r);
+  V3 fss = mul(kRcpPi * af * (1.0f + r * sot), rho);
+  float efo = e_fon_approx(mo, r), efi = e_fon_approx(mi, r);
+  float avg = af * (1.0f + kC2Fon * r);
+  float oma = 1.0f - avg;
+  V3 denom = sub(v3(1.f, 1.f, 1.f), smul(rho, oma));
+  V3 rho_ms = v3(rho.x * rho.x * avg / denom.x, rho.y * rho.y * avg / denom.y, rho.z * rho.z * avg / denom.z);
+  float scale = kRcpPi * std::fmax(1.0f - efo, 1.0e-7f) * std::fmax(1.0f - efi, 1.0e-7f) / std::fmax(oma, 1.0e-7f);
+  return add(fss, smul(rho_ms, scale));
+}
+inline void ltc_coeffs(float mu, float r, float* a, float* b, float* c, float* d) {
+  *a = 1.0f + r * (0.303392f + (-0.518982f + 0.111709f * mu) * mu + (-0.276266f + 0.335918f * mu) * r);
+  *b = r * (-1.16407f + 1.15859f * mu + (0.150815f - 0.150105f * mu) * r) / (mu * mu * mu - 1.43545f);
+  *c = 1.0f + r * (0.20013f + (-0.506373f + 0.261777f * mu) * mu);
+  *d = r * (0.540852f + (-1.01625f + 0.475392f * mu) * mu) / (-1.0743f + (0.0725628f + mu) * mu);
+}
+inline void ltc_basis_x(V3 wo, float* x, float* y) {
+  float l2 = wo.x * wo.x + wo.y * wo.y;
+  if (l2 > 0.0f) {
+    float inv = 1.0f / std::sqrt(l2);
+    *x = wo.x * inv;
+    *y = wo.y * inv;
+  } else {
+    *x = 1.0f;
+    *y = 0.0f;
+  }
+}
+inline V3 ltc_from(V3 wo, V3 v) {
+  float xx, xy;
+  ltc_basis_x(wo, &xx, &xy);
+  float yx = -xy, yy = xx;
+  return v3(xx * v.x + yx * v.y, xy * v.x + yy * v.y, v.z);
+}
+inline V3 ltc_to(V3 wo, V3 v) {
+  float xx, xy;
+  ltc_basis_x(wo, &xx, &xy);
+  float yx = -xy, yy = xx;
+  return v3(xx * v.x + xy * v.y, yx * v.x + yy * v.y, v.z);
+}
+inline V3 cltc_sample(V3 wo, float r, float u1, float u2, float* pdf) {
+  float a, b, c, d;
+  ltc_coeffs(wo.z, r, &a, &b, &c, &d);
+  float radius = std::sqrt(u1);
+  float phi = 2.0f * kPi * u2;
+  float x0 = radius * std::cos(phi), y0 = radius * std::sin(phi);
+  float vz = 1.0f / std::sqrt(d * d + 1.0f);
+  float s = 0.5f * (1.0f + vz);
+  float edge = std::sqrt(std::fmax(1.0f - y0 * y0, 0.0f));
+  float x = -((1.0f - s) * edge + s * x0);
+  float y = y0;
+  float whz = std::sqrt(std::fmax(1.0f - (x * x + y * y), 0.0f));
+  V3 wh = v3(x, y, whz);
+  float pdf_wh = wh.z / (kPi * s);
+  V3 wl = v3(a * wh.x + b * wh.z, c * wh.y, d * wh.x + wh.z);
+  float len = std::sqrt(dot(wl, wl));
+  float det = c * (a - b * d);
+  *pdf = pdf_wh * (len * len * len) / det;
+  return ltc_from(wo, sdiv(wl, len));
+}
+inline float cltc_pdf(V3 wo, V3 wi, float r) {
+  V3 wl = ltc_to(wo, wi);
+  float a, b, c, d;
+  ltc_coeffs(wo.z, r, &a, &b, &c, &d);
+  float det = c * (a - b * d);
+  V3 wh = v3(c * (wl.x - b * wl.z), (a - b * d) * wl.y, -c * (d * wl.x - a * wl.z));
+  float l2 = std::fmax(dot(wh, wh), 1.0e-30f);
+  float vz = 1.0f / std::sqrt(d * d + 1.0f);
+  float s = 0.5f * (1.0f + vz);
+  float q = det / l2;
+  return q * q * std::fmax(wh.z, 0.0f) / (kPi * s);
+}
+inline float p_uniform(float mu, float r) {
+  return std::pow(r, 0.1f) * (0.162925f + (-0.372058f + (0.538233f - 0.290822f * mu) * mu) * mu);
+}
+inline V3 uniform_lobe_sample(float u1, float u2) {
+  float st = std::sqrt(std::fmax(1.0f - u1 * u1, 0.0f));
+  float phi = 2.0f * kPi * u2;
+  return v3(st * std::cos(phi), st * std::sin(phi), u1);
+}
+inline V3 sample_eon(V3 wo, float r, float u1, float u2, float* pdf) {
+  float pu = p_uniform(wo.z, r);
+  float pc = 1.0f - pu;
+  V3 wi;
+  float pdf_c;
+  if (u1 < pu) {
+    float u1n = u1 / pu;
+    wi = uniform_lobe_sample(u1n, u2);
+    pdf_c = cltc_pdf(wo, wi, r);
+  } else {
+    float den = std::fmax(pc, 1.0e-30f);
+    float u1n = (u1 - pu) / den;
+    wi = cltc_sample(wo, r, u1n, u2, &pdf_c);
+  }
+  *pdf = pu * (1.0f / (2.0f * kPi)) + pc * pdf_c;
+  return wi;
+}
+inline float pdf_eon(V3 wo, V3 wi, float r) {
+  float pu = p_uniform(wo.z, r);
+  return pu * (1.0f / (2.0f * kPi)) + (1.0f - pu) * cltc_pdf(wo, wi, r);
+}
+
+
+// ---- shared sampling helpers -----------------------------------------------------------
+struct BSample {
+  V3 weight, wi;
+  float pdf;
+  bool delta;
+};
+// sample_cosine_weighted_hemisphere, R/math/mod.rs:26-36
+inline V3 cosine_hemisphere(float u, float v) {
+  float r = std::sqrt(clampf(u, 0.0f, 1.0f));
+  float phi = kTau * v;
+  float dx = r * std::cos(phi), dy = r * std::sin(phi);
+  float z = std::sqrt(std::fmax(1.0f - (dx * dx + dy * dy), 0.0f));
+  return v3(dx, dy, z);
+}
+inline float cosine_hemisphere_pdf(float c) { return std::fmax(c, 0.0f) / kPi; }  // R/math/mod.rs:38-40
+
+// f32::powi with a run-time base: LLVM expands it by binary exponentiation
+// (x^n = product of the squarings selected by the bits of n, lowest first).
+inline float powi_rt(float x, int n) {
+  float res = 0.0f, cur = x;
+  bool have = false;
+  while (n) {
+    if (n & 1) {
+      res = have ? res * cur : cur;
+      have = true;
+    }
+    n >>= 1;
+    if (n) cur = cur * cur;
+  }
+  return have ? res : 1.0f;
+}
+// f32::powi of compile-time constants, folded by LLVM through the double pow.
+inline float powi_const(float x, int n) { return (float)std::pow((double)x, (double)n); }
+inline V3 vec3_powf(V3 v, float e) {  // R/bsdf/open_pbr.rs:2179-2185
+  return v3(std::pow(std::fmax(v.x, 0.0f), e), std::pow(std::fmax(v.y, 0.0f), e), std::pow(std::fmax(v.z, 0.0f), e));
+}
+inline V3 vclamp01(V3 v) { return v3(clampf(v.x, 0.f, 1.f), clampf(v.y, 0.f, 1.f), clampf(v.z, 0.f, 1.f)); }
+inline V3 vsplat(float s) { return v3(s, s, s); }
+inline V3 vlerp(V3 a, V3 b, float t) { return add(a, smul(sub(b, a), t)); }  // a + (b - a) * t
+inline float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+
+// ---- GGX sampling, R/bsdf/smith_ggx.rs ---------------------------------------------------
+inline float ggx_g1(V3 w, float ax, float ay) {  // :45-51
+  float l = ggx_lambda(w, ax, ay);
+  if (!std::isfinite(l)) return 0.0f;
+  return 1.0f / (1.0f + l);
+}
+inline V3 reflect_local(V3 wo, V3 wm) {  // :12-14
+  return normalize_or_zero3(add(neg(wo), smul(wm, 2.0f * dot(wo, wm))));
+}
+inline float pdf_wm_vndf(V3 wo, V3 wm, float ax, float ay) {  // :62-75
+  if (!(wo.z > 0.0f) || wm.z <= 0.0f) return 0.0f;
+  float d = ggx_d(wm, ax, ay), g1 = ggx_g1(wo, ax, ay), c = std::fmax(dot(wo, wm), 0.0f);
+  if (d <= 0.0f || g1 <= 0.0f || c <= 0.0f) return 0.0f;
+  return d * g1 * c / wo.z;
+}
+// Heitz 2018 VNDF sampling, :78-119
+inline bool sample_wm_vndf(V3 wo, float ax, float ay, float u0, float u1, V3* out) {
+  if (wo.z <= 0.0f) return false;
+  ax = std::fmax(ax, kMinAlpha);
+  ay = std::fmax(ay, kMinAlpha);
+  V3 ws = normalize_or_zero3(v3(ax * wo.x, ay * wo.y, wo.z));
+  if (len_sq(ws) == 0.0f) return false;
+  float lensq = ws.x * ws.x + ws.y * ws.y;
+  V3 t1 = lensq > 0.0f ? sdiv(v3(-ws.y, ws.x, 0.0f), std::sqrt(lensq)) : v3(1.f, 0.f, 0.f);
+  V3 t2 = cross(ws, t1);
+  float r = std::sqrt(clampf(u0, 0.0f, 1.0f));
+  float phi = kTau * u1;
+  float p1 = r * std::cos(phi);
+  float p2 = r * std::sin(phi);
+  float s = 0.5f * (1.0f + ws.z);
+  p2 = (1.0f - s) * std::sqrt(std::fmax(1.0f - p1 * p1, 0.0f)) + s * p2;
+  V3 nh = add(add(smul(t1, p1), smul(t2, p2)), smul(ws, std::sqrt(std::fmax(1.0f - p1 * p1 - p2 * p2, 0.0f))));
+  V3 wm = normalize_or_zero3(v3(ax * nh.x, ay * nh.y, std::fmax(nh.z, 0.0f)));
+  if (len_sq(wm) == 0.0f || wm.z <= 0.0f) return false;
+  *out = wm;
+  return true;
+}
+// bounded VNDF sampling (Eto and Tokuyoshi), :121-150
+inline bool sample_wm_bounded_vndf(V3 wo, float ax, float ay, float u0, float u1, V3* out) {
+  if (wo.z <= 0.0f) return false;
+  ax = std::fmax(ax, kMinAlpha);
+  ay = std::fmax(ay, kMinAlpha);
+  V3 ws = normalize_or_zero3(v3(ax * wo.x, ay * wo.y, wo.z));
+  if (len_sq(ws) == 0.0f) return false;
+  float phi = kTau * u0;
+  float a = clampf(std::fmin(ax, ay), 0.0f, 1.0f);
+  float s = 1.0f + std::sqrt(wo.x * wo.x + wo.y * wo.y);
+  float a2 = a * a, s2 = s * s;
+  float k = (1.0f - a2) * s2 / (s2 + a2 * wo.z * wo.z);
+  float lb = wo.z > 0.0f ? -k * ws.z : -ws.z;
+  float z = std::fma(lb, u1, 1.0f - u1);  // f32::mul_add
+  float st = std::sqrt(clampf(1.0f - z * z, 0.0f, 1.0f));
+  V3 os = v3(st * std::cos(phi), st * std::sin(phi), z);
+  V3 ms = add(ws, os);
+  V3 wm = normalize_or_zero3(v3(ax * ms.x, ay * ms.y, ms.z));
+  if (len_sq(wm) == 0.0f || wm.z <= 0.0f) return false;
+  *out = wm;
+  return true;
+}
+// refract_about_wm, R/bsdf/dielectric_ggx.rs:723-738
+inline bool refract_about_wm(V3 wo, V3 wm, float eta, V3* out) {
+  float co = dot(wo, wm);
+  if (co <= 0.0f) return false;
+  float s2t = eta * eta * std::fmax(1.0f - co * co, 0.0f);
+  if (s2t >= 1.0f) return false;
+  float ct = std::sqrt(std::fmax(1.0f - s2t, 0.0f));
+  V3 wi = normalize_or_zero3(add(smul(wo, -eta), smul(wm, eta * co - ct)));
+  if (len_sq(wi) == 0.0f) return false;
+  *out = wi;
+  return true;
+}
+
+// ---- LUTs, R/bsdf/directional_albedo.rs ------------------------------------------------
+constexpr float kUniformHemiPdf = 1.0f / (2.0f * kPi);  // :30
+inline float radical_inverse_vdc(uint32_t bits) {       // :343-345
+  uint32_t r = 0;
+  for (int i = 0; i < 32; ++i) r |= ((bits >> i) & 1u) << (31 - i);
+  return (float)r * 2.3283064e-10f;
+}
+inline float radical_inverse_base3(uint32_t n) {  // :347-356
+  float q = 0.0f, bk = 1.0f / 3.0f;
+  while (n > 0) {
+    q += (float)(n % 3u) * bk;
+    n /= 3u;
+    bk /= 3.0f;
+  }
+  return q;
+}
+inline float lut_cell_center(uint32_t i, uint32_t res) { return ((float)i + 0.5f) / (float)res; }  // :410-412
+struct AxisLerp {
+  uint32_t lo, hi;
+  float t;
+};
+inline AxisLerp axis_lerp(float value, uint32_t res) {  // compute_lut_axis_lerp :388-408
+  if (res == 1) return AxisLerp{0, 0, 0.0f};
+  float pos = clampf(value, 0.0f, 1.0f) * (float)res - 0.5f;
+  if (pos <= 0.0f) return AxisLerp{0, 0, 0.0f};
+  uint32_t mx = res - 1;
+  if (pos >= (float)mx) return AxisLerp{mx, mx, 0.0f};
+  uint32_t lo = (uint32_t)std::floor(pos);
+  return AxisLerp{lo, lo + 1, pos - (float)lo};
+}
+// alpha_xy_from_roughness of the LUT module (:358-370)
+inline void lut_alpha_xy(float roughness, float anisotropy, float* ax, float* ay) {
+  roughness = clampf(roughness, 0.0f, 1.0f);
+  anisotropy = clampf(anisotropy, -1.0f, 1.0f);
+  float alpha = roughness * roughness;
+  float aspect = std::sqrt(1.0f - 0.9f * std::fabs(anisotropy));
+  float x, y;
+  if (anisotropy >= 0.0f) {
+    x = alpha / aspect;
+    y = alpha * aspect;
+  } else {
+    x = alpha * aspect;
+    y = alpha / aspect;
+  }
+  *ax = clampf(x, kMinAlpha, 1.0f);
+  *ay = clampf(y, kMinAlpha, 1.0f);
+}
+inline float sanitize_eta(float eta) { return (std::isfinite(eta) && eta > 0.0f) ? eta : 1.5f; }  // :372-378
+
+// Reflection-only DielectricGgxBsdf::eval (R/bsdf/dielectric_ggx.rs:121-176),
+// colour 1, front face, no energy compensation.
+inline float dielectric_reflect_eval(float eta, float ax, float ay, V3 wo, V3 wi) {
+  ax = std::fmax(ax, kMinAlpha);
+  ay = std::fmax(ay, kMinAlpha);
+  if (!(wo.z > 0.0f) || eta <= 0.0f) return 0.0f;
+  if (std::fmax(ax, ay) < kSmoothAlpha) return 0.0f;
+  if (!(wo.z * wi.z > 0.0f)) return 0.0f;
+  V3 wm;
+  if (!reflection_half_vector(wo, wi, &wm)) return 0.0f;
+  float cwm = dot(wo, wm);
+  if (cwm <= 0.0f) return 0.0f;
+  float co = std::fabs(wo.z), ci = std::fabs(wi.z);
+  if (co <= 0.0f || ci <= 0.0f) return 0.0f;
+  float d = ggx_d(wm, ax, ay);
+  if (d <= 0.0f) return 0.0f;
+  float g = ggx_g2(wo, wi, ax, ay);
+  if (g <= 0.0f) return 0.0f;
+  float f = fresnel_dielectric(cwm, 1.0f, eta);
+  return d * g * f / (4.0f * co * ci);
+}
+
+// DielectricGgxDirectionalAlbedoLut (:128-290): sqrt(cos) x phi x roughness x
+// anisotropy = 64 x 16 x 64 x 16 cells.  OpenPBR looks it up at anisotropy 0
+// only (R/bsdf/open_pbr.rs:431,446), which interpolates slices 7 and 8 with
+// weight 1/2 each, so those two slices are built (kAlbedoAniso) and stored as
+// slice 0 and 1.
+constexpr uint32_t kAlbedoCos = 64, kAlbedoPhi = 16, kAlbedoRough = 64, kAlbedoAnisoRes = 16, kAlbedoAniso = 2;
+constexpr uint32_t kAlbedoLen = kAlbedoCos * kAlbedoPhi * kAlbedoRough * kAlbedoAniso;
+constexpr uint32_t kAlbedoSamples = 64;
+struct AlbedoLut {
+  float eta = 1.5f;
+  std::vector<float> v;  // [((cos * 16 + phi) * 64 + rough) * 2 + (aniso - 7)]
+};
+template <class F>
+inline void parallel_for(uint32_t n, F f) {
+  unsigned th = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < th; ++t)
+    pool.emplace_back([&, t] {
+      for (uint32_t i = t; i < n; i += th) f(i);
+    });
+  for (auto& p : pool) p.join();
+}
+inline AlbedoLut build_albedo_lut(float eta_in) {
+  AlbedoLut L;
+  L.eta = sanitize_eta(eta_in);
+  const float eta = L.eta;
+  V3 hs[kAlbedoSamples];  // generate_uniform_hemisphere_samples :327-341
+  for (uint32_t i = 0; i < kAlbedoSamples; ++i) {
+    float u = ((float)i + 0.5f) / (float)kAlbedoSamples, vv = radical_inverse_vdc(i);
+    float z = clampf(u, 0.0f, 1.0f), r = std::sqrt(std::fmax(1.0f - z * z, 0.0f)), phi = kTau * vv;
+    hs[i] = v3(r * std::cos(phi), r * std::sin(phi), z);
+  }
+  L.v.assign(kAlbedoLen, 0.0f);
+  parallel_for(kAlbedoLen, [&](uint32_t idx) {
+    uint32_t an = idx % kAlbedoAniso, rest = idx / kAlbedoAniso;
+    uint32_t ri = rest % kAlbedoRough;
+    rest /= kAlbedoRough;
+    uint32_t pi = rest % kAlbedoPhi, ci = rest / kAlbedoPhi;
+    float sc = lut_cell_center(ci, kAlbedoCos);
+    float ct = sc * sc, st = std::sqrt(std::fmax(1.0f - ct * ct, 0.0f));
+    float phi = lut_cell_center(pi, kAlbedoPhi) * (kPi / 2.0f);
+    V3 wo = v3(st * std::cos(phi), st * std::sin(phi), ct);
+    float rough = lut_cell_center(ri, kAlbedoRough);
+    float aniso = 2.0f * lut_cell_center(an + 7, kAlbedoAnisoRes) - 1.0f;
+    // estimate_dielectric_ggx_directional_albedo :292-325
+    float val;
+    if (wo.z <= 0.0f) {
+      val = 0.0f;
+    } else {
+      float ax, ay;
+      lut_alpha_xy(rough, aniso, &ax, &ay);
+      if (std::fmax(ax, ay) < kSmoothAlpha) {
+        val = fresnel_dielectric(clampf(wo.z, 0.0f, 1.0f), 1.0f, eta);
+      } else {
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < kAlbedoSamples; ++s)
+          sum += dielectric_reflect_eval(eta, ax, ay, wo, hs[s]) * std::fmax(hs[s].z, 0.0f) / kUniformHemiPdf;
+        val = clampf(sum / (float)kAlbedoSamples, 0.0f, 1.0f);
+      }
+    }
+    L.v[idx] = val;
+  });
+  return L;
+}
+inline float fold_phi(V3 w) {  // fold_phi_to_first_quadrant :380-386
+  if (w.x == 0.0f && w.y == 0.0f) return 0.0f;
+  return std::atan2(std::fabs(w.y), std::fabs(w.x));
+}
+// DielectricGgxDirectionalAlbedoLut::lookup (:170-226) at anisotropy 0.
+inline float albedo_lookup(const AlbedoLut& L, V3 w, float roughness) {
+  if (w.z <= 0.0f) return 0.0f;
+  roughness = clampf(roughness, 0.0f, 1.0f);
+  float ax, ay;
+  lut_alpha_xy(roughness, 0.0f, &ax, &ay);
+  if (std::fmax(ax, ay) < kSmoothAlpha) return fresnel_dielectric(clampf(w.z, 0.0f, 1.0f), 1.0f, L.eta);
+  float sc = std::sqrt(clampf(w.z, 0.0f, 1.0f));
+  float phi = fold_phi(w) / (kPi / 2.0f);
+  AxisLerp a = axis_lerp(sc, kAlbedoCos), b = axis_lerp(phi, kAlbedoPhi), c = axis_lerp(roughness, kAlbedoRough);
+  AxisLerp d = axis_lerp(0.5f, kAlbedoAnisoRes);  // slices 7, 8 -> stored 0, 1
+  uint32_t ia[2] = {a.lo, a.hi}, ib[2] = {b.lo, b.hi}, ic[2] = {c.lo, c.hi}, id[2] = {d.lo - 7, d.hi - 7};
+  float wa[2] = {1.0f - a.t, a.t}, wb[2] = {1.0f - b.t, b.t}, wc[2] = {1.0f - c.t, c.t}, wd[2] = {1.0f - d.t, d.t};
+  float value = 0.0f;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k)
+        for (int l = 0; l < 2; ++l)
+          value += wa[i] * wb[j] * wc[k] * wd[l] *
+                   L.v[((ia[i] * kAlbedoPhi + ib[j]) * kAlbedoRough + ic[k]) * kAlbedoAniso + id[l]];
+  return clampf(value, 0.0f, 1.0f);
+}
+
+// ConductorGgxEnergyCompensationLut (:694-776): 64 x 64 directional albedo of
+// a white-Fresnel GGX conductor, plus its cosine-weighted average.
+constexpr uint32_t kCondCos = 64, kCondRough = 64, kCondSamples = 256;
+struct CondLut {
+  std::vector<float> e, eavg;
+};
+inline CondLut build_cond_lut() {
+  CondLut L;
+  L.e.assign(kCondCos * kCondRough, 0.0f);
+  float us[kCondSamples][2];  // generate_stratified_unit_square_samples_ec :823-829
+  for (uint32_t i = 0; i < kCondSamples; ++i) {
+    us[i][0] = ((float)i + 0.5f) / (float)kCondSamples;
+    us[i][1] = radical_inverse_vdc(i);
+  }
+  parallel_for(kCondCos * kCondRough, [&](uint32_t idx) {
+    uint32_t ci = idx % kCondCos, ri = idx / kCondCos;
+    float sc = lut_cell_center(ci, kCondCos);
+    float ct = sc * sc;
+    float rough = lut_cell_center(ri, kCondRough);
+    // estimate_conductor_ggx_directional_albedo_white_fresnel :778-821
+    float val;
+    float alpha = clampf(rough * rough, kMinAlpha, 1.0f);
+    if (ct <= 0.0f) {
+      val = 0.0f;
+    } else if (alpha < kSmoothAlpha) {
+      val = 1.0f;
+    } else {
+      float st = std::sqrt(std::fmax(1.0f - ct * ct, 0.0f));
+      V3 wo = v3(st, 0.0f, ct);
+      float g1 = ggx_g1(wo, alpha, alpha);
+      if (g1 <= 0.0f) {
+        val = 0.0f;
+      } else {
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < kCondSamples; ++s) {
+          V3 wm;
+          float term = 0.0f;
+          if (sample_wm_vndf(wo, alpha, alpha, us[s][0], us[s][1], &wm)) {
+            float c = dot(wo, wm);
+            if (c > 0.0f) {
+              V3 wi = reflect_local(wo, wm);
+              if (wi.z > 0.0f) term = ggx_g2(wo, wi, alpha, alpha) / g1;
+            }
+          }
+          sum += term;
+        }
+        val = clampf(sum / (float)kCondSamples, 0.0f, 1.0f);
+      }
+    }
+    L.e[idx] = val;
+  });
+  L.eavg.assign(kCondRough, 0.0f);
+  for (uint32_t r = 0; r < kCondRough; ++r) {
+    float w = 0.0f;
+    for (uint32_t c = 0; c < kCondCos; ++c) {
+      float sc = lut_cell_center(c, kCondCos), ct = sc * sc;
+      w += L.e[r * kCondCos + c] * ct * sc;
+    }
+    L.eavg[r] = clampf(4.0f / (float)kCondCos * w, 0.0f, 1.0f);
+  }
+  return L;
+}
+inline float cond_lookup_e(const CondLut& L, float c, float r) {  // :738-754
+  c = clampf(c, 0.0f, 1.0f);
+  r = clampf(r, 0.0f, 1.0f);
+  AxisLerp a = axis_lerp(std::sqrt(c), kCondCos), b = axis_lerp(r, kCondRough);
+  uint32_t ia[2] = {a.lo, a.hi}, ib[2] = {b.lo, b.hi};
+  float wa[2] = {1.0f - a.t, a.t}, wb[2] = {1.0f - b.t, b.t};
+  float value = 0.0f;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) value += wa[i] * wb[j] * L.e[ib[j] * kCondCos + ia[i]];
+  return clampf(value, 0.0f, 1.0f);
+}
+inline float cond_lookup_eavg(const CondLut& L, float r) {  // :756-762
+  r = clampf(r, 0.0f, 1.0f);
+  AxisLerp a = axis_lerp(r, kCondRough);
+  return clampf((1.0f - a.t) * L.eavg[a.lo] + a.t * L.eavg[a.hi], 0.0f, 1.0f);
+}
+
+// DielectricGgxEnergyCompensationLut (:832-990): 32^3 full-sphere albedo of a
+// rough dielectric over (sqrt(cos), roughness, eta axis), plus averages.
+constexpr uint32_t kDielCos = 32, kDielRough = 32, kDielEta = 32, kDielSamples = 4096;
+struct DielLut {
+  std::vector<float> e, eavg;
+};
+inline float diel_axis_to_eta(float t) {  // :855-863
+  t = clampf(t, 0.0f, 1.0f);
+  if (t >= 0.5f) return 1.0f + (t - 0.5f) * 2.0f * (3.0f - 1.0f);
+  float inv = 1.0f + (0.5f - t) * 2.0f * (3.0f - 1.0f);
+  return 1.0f / inv;
+}
+inline float diel_eta_to_axis(float eta) {  // :845-853
+  eta = clampf(eta, 1.0f / 3.0f, 3.0f);
+  if (eta >= 1.0f) return 0.5f + 0.5f * ((eta - 1.0f) / (3.0f - 1.0f));
+  float inv = 1.0f / eta;
+  return 0.5f - 0.5f * ((inv - 1.0f) / (3.0f - 1.0f));
+}
+// DielectricGgxBsdf::sample_rough (R/bsdf/dielectric_ggx.rs:501-587), front
+// face, colour 1, no energy compensation, radiance mode: the weight's x and
+// whether the sample transmits.
+inline bool dielectric_sample_rough(float eta, float alpha, V3 wo, float uc, float u0, float u1, float* w, bool* trans) {
+  const float ei = 1.0f, et = eta, eta_rel = 1.0f / eta;
+  V3 wm;
+  if (!sample_wm_vndf(wo, alpha, alpha, u0, u1, &wm)) return false;
+  float cwm = dot(wo, wm);
+  if (cwm <= 0.0f) return false;
+  float f = fresnel_dielectric(cwm, ei, et);
+  float g1 = ggx_g1(wo, alpha, alpha);
+  if (g1 <= 0.0f) return false;
+  float pr = std::fmax(f, 0.0f), pt = std::fmax(1.0f - f, 0.0f);
+  float psum = pr + pt;
+  if (psum <= 0.0f) return false;
+  pr /= psum;
+  pt /= psum;
+  if (uc < pr) {
+    V3 wi = reflect_local(wo, wm);
+    if (!(wi.z > 0.0f)) return false;
+    float g2 = ggx_g2(wo, wi, alpha, alpha);
+    float pw = pdf_wm_vndf(wo, wm, alpha, alpha);
+    if (pw <= 0.0f) return false;
+    float pdf = pr * pw / (4.0f * cwm);
+    if (pdf <= 0.0f) return false;
+    *w = f * g2 / (pr * g1);
+    *trans = false;
+    return true;
+  }
+  V3 wi;
+  if (!refract_about_wm(wo, wm, eta_rel, &wi)) return false;
+  if (wi.z >= 0.0f) return false;
+  float cwi = dot(wi, wm);
+  float den = cwi + eta_rel * cwm;
+  if (std::fabs(den) < 1.0e-6f) return false;
+  float g2 = ggx_g2(wo, wi, alpha, alpha);
+  float pw = pdf_wm_vndf(wo, wm, alpha, alpha);
+  if (pw <= 0.0f) return false;
+  float pdf = pt * pw * std::fabs(cwi) / (den * den);
+  if (pdf <= 0.0f) return false;
+  float rs = 1.0f / (eta_rel * eta_rel);  // TransportMode::Radiance.transmission_scale
+  *w = 1.0f * (rs * (1.0f - f) * g2 / (pt * g1));
+  *trans = true;
+  return true;
+}
+inline DielLut build_diel_lut() {
+  DielLut L;
+  L.e.assign(kDielCos * kDielRough * kDielEta, 0.0f);
+  parallel_for(kDielCos * kDielRough * kDielEta, [&](uint32_t idx) {
+    uint32_t ci = idx % kDielCos, rest = idx / kDielCos;
+    uint32_t ri = rest % kDielRough, ei = rest / kDielRough;
+    float sc = lut_cell_center(ci, kDielCos), ct = sc * sc;
+    float rough = lut_cell_center(ri, kDielRough);
+    float eta = diel_axis_to_eta(lut_cell_center(ei, kDielEta));
+    // estimate_dielectric_ggx_directional_albedo_full_sphere :975-1023
+    float val;
+    if (ct <= 0.0f || eta <= 0.0f) {
+      val = 0.0f;
+    } else {
+      float alpha = clampf(rough * rough, kMinAlpha, 1.0f);
+      if (alpha < kSmoothAlpha) {
+        val = 1.0f;
+      } else {
+        float st = std::sqrt(std::fmax(1.0f - ct * ct, 0.0f));
+        V3 wo = v3(st, 0.0f, ct);
+        float inv_eta_sq = 1.0f / (eta * eta);
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < kDielSamples; ++s) {
+          float uc = ((float)s + 0.5f) / (float)kDielSamples;
+          float u = radical_inverse_vdc(s + 1), v = radical_inverse_base3(s + 1);
+          float w;
+          bool tr;
+          if (!dielectric_sample_rough(eta, alpha, wo, uc, u, v, &w, &tr)) continue;
+          if (!std::isfinite(w)) continue;
+          sum += w * (tr ? inv_eta_sq : 1.0f);
+        }
+        val = clampf(sum / (float)kDielSamples, 0.0f, 1.0f);
+      }
+    }
+    L.e[idx] = val;
+  });
+  L.eavg.assign(kDielRough * kDielEta, 0.0f);
+  for (uint32_t e = 0; e < kDielEta; ++e)
+    for (uint32_t r = 0; r < kDielRough; ++r) {
+      float w = 0.0f;
+      for (uint32_t c = 0; c < kDielCos; ++c) {
+        float sc = lut_cell_center(c, kDielCos), ct = sc * sc;
+        w += L.e[(e * kDielRough + r) * kDielCos + c] * ct * sc;
+      }
+      L.eavg[e * kDielRough + r] = clampf(4.0f / (float)kDielCos * w, 0.0f, 1.0f);
+    }
+  return L;
+}
+inline float diel_lookup_e(const DielLut& L, float c, float r, float eta) {  // :919-944
+  c = clampf(c, 0.0f, 1.0f);
+  r = clampf(r, 0.0f, 1.0f);
+  AxisLerp a = axis_lerp(std::sqrt(c), kDielCos), b = axis_lerp(r, kDielRough), e = axis_lerp(diel_eta_to_axis(eta), kDielEta);
+  uint32_t ie[2] = {e.lo, e.hi}, ib[2] = {b.lo, b.hi}, ia[2] = {a.lo, a.hi};
+  float we[2] = {1.0f - e.t, e.t}, wb[2] = {1.0f - b.t, b.t}, wa[2] = {1.0f - a.t, a.t};
+  float value = 0.0f;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) value += we[i] * wb[j] * wa[k] * L.e[(ie[i] * kDielRough + ib[j]) * kDielCos + ia[k]];
+  return clampf(value, 0.0f, 1.0f);
+}
+inline float diel_lookup_eavg(const DielLut& L, float r, float eta) {  // :946-961
+  r = clampf(r, 0.0f, 1.0f);
+  AxisLerp b = axis_lerp(r, kDielRough), e = axis_lerp(diel_eta_to_axis(eta), kDielEta);
+  uint32_t ie[2] = {e.lo, e.hi}, ib[2] = {b.lo, b.hi};
+  float we[2] = {1.0f - e.t, e.t}, wb[2] = {1.0f - b.t, b.t};
+  float value = 0.0f;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) value += we[i] * wb[j] * L.eavg[ie[i] * kDielRough + ib[j]];
+  return clampf(value, 0.0f, 1.0f);
+}
+
+// The tables one material reads.  A table a material does not need is an
+// all-zero stand-in that is only ever read with weight zero: the conductor
+// table is read with a non-zero weight only for metalness > 0, the dielectric
+// energy-compensation table only where f_avg(eta) > 0 (eta != 1) or for a
+// coat, and the coat albedo table only for coat > 0.
+struct Luts {
+  const AlbedoLut* spec = nullptr;
+  const AlbedoLut* coat = nullptr;
+  const CondLut* cond = nullptr;
+  const DielLut* diel = nullptr;
+};
+// Process-wide tables, built once (the reference caches them per process,
+// R/bsdf/directional_albedo.rs:37-116).
+const AlbedoLut& albedo_lut_for(float eta);
+const CondLut& cond_lut();
+const DielLut& diel_lut();
+const AlbedoLut& zero_albedo_lut();
+const CondLut& zero_cond_lut();
+const DielLut& zero_diel_lut();
+
+// ---- OpenPBR material, R/material/open_pbr.rs -----------------------------------------------
+// OpenPbrMaterial parameters (:23-82) carried by the backend; transmission,
+// thin film, thin walls, subsurface, textures and normal maps are rejected.
+struct OpenPbr {
+  float base_weight;
+  V3 base_color;
+  float base_diffuse_roughness;
+  float base_metalness;
+  float specular_weight;
+  V3 specular_color;
+  float specular_roughness;
+  float specular_ior;
+  float specular_roughness_anisotropy;
+  float transmission_weight, coat_weight, fuzz_weight, thin_film_weight;
+  float emission_luminance;
+  V3 emission_color;
+  uint32_t thin_walled;
+  V3 coat_color;
+  float coat_roughness, coat_roughness_anisotropy, coat_ior, coat_darkening;
+  V3 fuzz_color;
+  float fuzz_roughness;
+};
+
+inline float modulated_eta(float eta, float w) {  // :871-878
+  if (std::fabs(w - 1.0f) < 1.0e-6f) return eta;
+  float f0 = powi_rt((eta - 1.0f) / (eta + 1.0f), 2);
+  float sg = (eta - 1.0f) > 0.0f ? 1.0f : ((eta - 1.0f) < 0.0f ? -1.0f : 1.0f);  // f32::signum (+0 -> +1)
+  float eps = std::sqrt(clampf(w * f0, 0.0f, 0.99999f)) * sg;
+  return std::fmax((1.0f + eps) / std::fmax(1.0f - eps, 1.0e-6f), 1.0e-4f);
+}
+inline void alpha_xy(float roughness, float anisotropy, float* ax, float* ay) {  // :861-869
+  float r = clampf(roughness, 0.0f, 1.0f);
+  float a = clampf(anisotropy, 0.0f, 1.0f);
+  float alpha = r * r;
+  float oma = 1.0f - a;
+  float x = alpha * std::sqrt(2.0f / (1.0f + oma * oma));
+  float y = oma * x;
+  *ax = clampf(x, kMinAlpha, 1.0f);
+  *ay = clampf(y, kMinAlpha, 1.0f);
+}
+// zeltner_dir_albedo / LTC coefficients, R/bsdf/open_pbr.rs:2070-2091 (the
+// material module's copy :890-898 is the same expression)
+inline float zeltner_dir_albedo(float ndv, float rough) {
+  float x = clampf(ndv, 0.0f, 1.0f), y = clampf(rough, 0.01f, 1.0f);
+  float s = y * (0.0206607f + 1.58491f * y) / (0.0379424f + y * (1.32227f + y));
+  float m = y * (-0.193854f + y * (-1.14885f + y * (1.7932f - 0.95943f * y * y))) / (0.046391f + y);
+  float o = y * (0.000654023f + (-0.0207818f + 0.119681f * y) * y) / (1.26264f + y * (-1.92021f + y));
+  float q = (x - m) / s;
+  return std::exp(-0.5f * (q * q)) / (s * std::sqrt(2.0f * kPi)) + o;
+}
+inline float zeltner_a_inv(float ndv, float rough) {
+  float x = clampf(ndv, 0.0f, 1.0f), y = clampf(rough, 0.01f, 1.0f);
+  return (2.58126f * x + 0.813703f * y) * y / (1.0f + 0.310327f * x * x + 2.60994f * x * y);
+}
+inline float zeltner_b_inv(float ndv, float rough) {
+  float x = clampf(ndv, 0.0f, 1.0f), y = clampf(rough, 0.01f, 1.0f);
+  return std::sqrt(1.0f - x) * (y - 1.0f) * y * y * y / (0.0000254053f + 1.71228f * x - 1.71506f * x * y + 1.34174f * y * y);
+}
+inline float dielectric_fresnel_avg_ln(float eta) {  // dielectric_fresnel_avg :880-886
+  eta = clampf(eta, 1.0f, 3.0f);
+  return clampf(std::log(std::fmax((10893.0f * eta - 1438.2f) / (-774.4f * eta * eta + 10212.0f * eta + 1.0f), 1.0e-6f)),
+                0.0f, 1.0f);
+}
+
+// OpenPbrBsdfParams (R/bsdf/open_pbr.rs:33-71) of the carried subset.
+struct PbrBsdf {
+  V3 base_color;
+  float base, specular;
+  V3 specular_color;
+  float ax, ay, specular_eta, transmission_eta, metalness;
+  float coat;
+  V3 coat_color, coat_darkening;
+  float cax, cay, coat_eta;
+  float fuzz;
+  V3 fuzz_color;
+  float fuzz_roughness;
+  float diffuse_roughness;
+  bool front_face;
+  Luts luts;
+};
+
+// apply_fuzz_roughening / effective_*_roughness (:736-778)
+inline float fuzz_roughening(const OpenPbr& m, float r, float fw, float fr) {
+  if (fw <= 0.0f) return r;
+  V3 fc = vclamp01(m.fuzz_color);
+  float e = zeltner_dir_albedo(1.0f, fr);
+  float rf = clampf(luminance(smul(fc, e)), 0.0f, 1.0f);
+  float rough = std::pow(std::fmin(powi_rt(r, 4) + 2.0f * powi_rt(rf, 4), 1.0f), 0.25f);
+  return lerpf(r, rough, fw);
+}
+// coat_darkening_factor (:900-941); wo_local: the shading-frame outgoing direction.
+inline V3 coat_darkening_factor(const OpenPbr& m, V3 base_color, V3 wo_local) {
+  float cw = clampf(m.coat_weight, 0.0f, 1.0f), delta = clampf(m.coat_darkening, 0.0f, 1.0f);
+  if (cw <= 0.0f || delta <= 0.0f) return vsplat(1.0f);
+  float ec = std::fmax(m.coat_ior, 1.0f);
+  float ef = dielectric_fresnel_avg_ln(ec);
+  float kr = 1.0f - (1.0f - ef) / (ec * ec);
+  float ks = fresnel_dielectric(clampf(std::fabs(wo_local.z), 0.0f, 1.0f), 1.0f, ec);
+  float sior = std::fmax(m.specular_ior, 1.0e-4f);
+  float eta_s = lerpf(sior, sior / ec, cw);
+  float f0 = powi_rt((eta_s - 1.0f) / (eta_s + 1.0f), 2);
+  float fs = clampf(std::fmax(m.specular_weight, 0.0f) * f0, 0.0f, 1.0f);
+  float r_spec = clampf(m.specular_roughness * 1.0f, 0.0f, 1.0f);
+  float metal = clampf(m.base_metalness * 1.0f, 0.0f, 1.0f);
+  float rd = lerpf(1.0f, r_spec, fs);
+  float rb = lerpf(rd, r_spec, metal);
+  float k = clampf(lerpf(ks, kr, rb), 0.0f, 1.0f);
+  V3 bw = smul(base_color, clampf(m.base_weight, 0.0f, 1.0f));
+  V3 ssc = vsplat(0.8f);  // subsurface_color default, weight 0
+  V3 e_dielec = vlerp(vlerp(bw, vclamp01(ssc), 0.0f), vsplat(1.0f - f0), clampf(m.transmission_weight, 0.0f, 1.0f));
+  V3 e_metal = vclamp01(smul(bw, std::fmax(m.specular_weight, 0.0f)));
+  V3 e_base = vclamp01(vlerp(e_dielec, e_metal, metal));
+  V3 cc = vclamp01(m.coat_color);
+  V3 kc = smul(cc, k);
+  V3 den = vmax(sub(vsplat(1.0f), vmul(e_base, kc)), vsplat(1.0e-4f));
+  float omk = std::fmax(1.0f - k, 0.0f);
+  V3 df = v3(omk / den.x, omk / den.y, omk / den.z);
+  return vclamp01(vlerp(vsplat(1.0f), df, cw * delta));
+}
+// OpenPbrMaterial::make_bsdf (:393-487)
+inline PbrBsdf make_bsdf(const OpenPbr& m, bool front_face, V3 wo_local, const Luts& luts) {
+  PbrBsdf p;
+  V3 base_color = vmul(m.base_color, vsplat(1.0f));
+  float metal = clampf(m.base_metalness * 1.0f, 0.0f, 1.0f);
+  float fw = clampf(m.fuzz_weight * 1.0f, 0.0f, 1.0f);
+  float fr = clampf(m.fuzz_roughness * 1.0f, 0.0f, 1.0f);
+  float coat_r = fuzz_roughening(m, clampf(m.coat_roughness, 0.0f, 1.0f), fw, fr);
+  float r_base = clampf(m.specular_roughness * 1.0f, 0.0f, 1.0f);
+  float coat_rough = std::pow(std::fmin(powi_rt(r_base, 4) + 2.0f * powi_rt(coat_r, 4), 1.0f), 0.25f);
+  float with_coat = lerpf(r_base, coat_rough, clampf(m.coat_weight, 0.0f, 1.0f));
+  float spec_r = fuzz_roughening(m, with_coat, fw, fr);
+  alpha_xy(spec_r, m.specular_roughness_anisotropy, &p.ax, &p.ay);
+  alpha_xy(coat_r, m.coat_roughness_anisotropy, &p.cax, &p.cay);
+  p.base_color = base_color;
+  p.base = clampf(m.base_weight, 0.0f, 1.0f);
+  p.specular = std::fmax(m.specular_weight, 0.0f);
+  p.specular_color = vclamp01(m.specular_color);
+  p.specular_eta = modulated_eta(std::fmax(m.specular_ior, 1.0e-4f), std::fmax(m.specular_weight, 0.0f));
+  p.transmission_eta = p.specular_eta;
+  p.metalness = metal;
+  p.coat = clampf(m.coat_weight, 0.0f, 1.0f);
+  p.coat_color = vclamp01(m.coat_color);
+  p.coat_darkening = coat_darkening_factor(m, base_color, wo_local);
+  p.coat_eta = std::fmax(m.coat_ior, 1.0e-4f);
+  p.fuzz = fw;
+  p.fuzz_color = vclamp01(m.fuzz_color);
+  p.fuzz_roughness = fr;
+  p.diffuse_roughness = clampf(m.base_diffuse_roughness, 0.0f, 1.0f);
+  p.front_face = front_face;
+  p.luts = luts;
+  return p;
+}
+
+// The tables a material reads (see Luts).
+inline Luts luts_for(const OpenPbr& m) {
+  Luts l;
+  const float eta = modulated_eta(std::fmax(m.specular_ior, 1.0e-4f), std::fmax(m.specular_weight, 0.0f));
+  const bool coat = clampf(m.coat_weight, 0.0f, 1.0f) > 0.0f;
+  l.spec = &albedo_lut_for(eta);
+  l.coat = coat ? &albedo_lut_for(std::fmax(m.coat_ior, 1.0e-4f)) : &zero_albedo_lut();
+  l.cond = clampf(m.base_metalness, 0.0f, 1.0f) > 0.0f ? &cond_lut() : &zero_cond_lut();
+  l.diel = (eta != 1.0f || coat) ? &diel_lut() : &zero_diel_lut();
+  return l;
+}
+// OpenPbrMaterial::emission_exit_tint (:608-638); wo_local normalised.
+inline V3 emission_exit_tint(const OpenPbr& m, V3 wo_local) {
+  float co = clampf(std::fabs(wo_local.z), 0.0f, 1.0f);
+  if (co <= 0.0f) return vsplat(0.0f);
+  V3 tint = vsplat(1.0f);
+  float cw = clampf(m.coat_weight, 0.0f, 1.0f);
+  if (cw > 0.0f) {
+    float ec = std::fmax(m.coat_ior, 1.0e-4f);
+    float mu = std::fmax(std::sqrt(std::fmax(1.0f - (1.0f - co * co) / (ec * ec), 0.0f)), 1.0e-4f);
+    V3 tr = vec3_powf(vclamp01(m.coat_color), 1.0f / mu);
+    tint = vmul(tint, vlerp(vsplat(1.0f), tr, cw));
+  }
+  float fw = clampf(m.fuzz_weight * 1.0f, 0.0f, 1.0f);
+  if (fw > 0.0f) {
+    float ef = zeltner_dir_albedo(co, clampf(m.fuzz_roughness * 1.0f, 0.0f, 1.0f));
+    tint = vmul(tint, vsplat(lerpf(1.0f, std::fmax(1.0f - ef, 0.0f), fw)));
+  }
+  return tint;
+}
+
+// ---- OpenPbrBsdf, R/bsdf/open_pbr.rs ------------------------------------------------------
+constexpr float kCos82 = 1.0f / 7.0f;    // :30
+constexpr float kMsDenomEps = 1.0e-6f;  // :2125
+inline V3 schlick_vec3(V3 f0, float c) {  // :2148-2151
+  float om = 1.0f - clampf(c, 0.0f, 1.0f);
+  float p5 = powi_rt(om, 5);
+  return add(f0, smul(sub(vsplat(1.0f), f0), p5));
+}
+inline float refracted_cos(float mu, float eta) {  // refracted_cos_in_dielectric :2174-2177
+  float s2 = std::fmax(1.0f - powi_rt(clampf(mu, 0.0f, 1.0f), 2), 0.0f) / std::fmax(eta * eta, kMsDenomEps);
+  return std::sqrt(std::fmax(1.0f - s2, 0.0f));
+}
+inline float rough_proxy(float ax, float ay) { return clampf(std::sqrt(std::fmax(ax, ay)), 0.0f, 1.0f); }
+inline bool coat_smooth(const PbrBsdf& p) { return p.coat <= 0.0f || std::fmax(p.cax, p.cay) < kSmoothAlpha; }
+inline bool metal_smooth(const PbrBsdf& p) { return p.metalness <= 0.0f || std::fmax(p.ax, p.ay) < kSmoothAlpha; }
+inline bool spec_smooth(const PbrBsdf& p) { return p.specular <= 0.0f || std::fmax(p.ax, p.ay) < kSmoothAlpha; }
+
+struct MsParams {
+  float eta_o, eta_rel, req, e_avg_o, ratio_r, ome_o, ome_t;
+};
+inline MsParams ms_params(const PbrBsdf& p, float ax, float ay, float eta) {  // dielectric_ms_params :593-625
+  MsParams s;
+  s.eta_o = p.front_face ? std::fmax(eta, 1.0e-4f) : 1.0f / std::fmax(eta, 1.0e-4f);
+  s.eta_rel = 1.0f / s.eta_o;
+  s.req = std::pow(ax * ay, 0.25f);
+  s.e_avg_o = diel_lookup_eavg(*p.luts.diel, s.req, s.eta_o);
+  float e_avg_t = diel_lookup_eavg(*p.luts.diel, s.req, s.eta_rel);
+  s.ome_o = std::fmax(1.0f - s.e_avg_o, kMsDenomEps);
+  s.ome_t = std::fmax(1.0f - e_avg_t, kMsDenomEps);
+  float fo = f_avg_dielectric(s.eta_o), ft = f_avg_dielectric(s.eta_rel);
+  float a = (1.0f - fo) / s.ome_o;
+  float b = (1.0f - ft) * s.eta_o * s.eta_o / s.ome_t;
+  float x = a + b > kMsDenomEps ? b / (a + b) : 0.5f;
+  s.ratio_r = clampf(1.0f - x * (1.0f - fo), 0.0f, 1.0f);
+  return s;
+}
+inline float ms_factor(const MsParams& s) {  // dielectric_reflection_ms_factor :573-577
+  float fa = clampf(f_avg_dielectric(s.eta_o), 0.0f, 1.0f);
+  float den = 1.0f - fa * std::fmax(1.0f - s.e_avg_o, 0.0f);
+  return clampf(fa * fa * s.e_avg_o / std::fmax(den, kMsDenomEps), 0.0f, 1.0f);
+}
+struct TwoLobe {
+  float ss, ms, total;
+};
+inline TwoLobe refl_lobe_weights(const MsParams& s) {  // :627-636
+  float fa = clampf(f_avg_dielectric(s.eta_o), 0.0f, 1.0f);
+  float ss = clampf(fa * s.e_avg_o, 0.0f, 1.0f);
+  float ms = std::fmax(1.0f - s.e_avg_o, 0.0f) * ms_factor(s);
+  return TwoLobe{ss, ms, std::fmax(ss + ms, 1.0e-8f)};
+}
+inline TwoLobe cond_lobe_weights(const PbrBsdf& p) {  // :579-591
+  float e = clampf(cond_lookup_eavg(*p.luts.cond, std::pow(p.ax * p.ay, 0.25f)), 0.0f, 1.0f);
+  float ms = std::fmax(1.0f - e, 0.0f);
+  return TwoLobe{e, ms, std::fmax(e + ms, 1.0e-8f)};
+}
+inline float albedo_with_ms(const PbrBsdf& p, V3 wo, float ax, float ay, float eta, float ss) {  // :464-481
+  if (wo.z <= 0.0f || std::fmax(ax, ay) < kSmoothAlpha) return clampf(ss, 0.0f, 1.0f);
+  MsParams s = ms_params(p, ax, ay, eta);
+  float eo = diel_lookup_e(*p.luts.diel, wo.z, s.req, s.eta_o);
+  return clampf(ss + ms_factor(s) * std::fmax(1.0f - eo, 0.0f), 0.0f, 1.0f);
+}
+inline float coat_albedo(const PbrBsdf& p, V3 wo) {  // lookup_coat_albedo :424-437
+  if (wo.z <= 0.0f) return 0.0f;
+  float ss = albedo_lookup(*p.luts.coat, wo, rough_proxy(p.cax, p.cay));
+  return albedo_with_ms(p, wo, p.cax, p.cay, p.coat_eta, ss);
+}
+inline float spec_albedo(const PbrBsdf& p, V3 wo) {  // lookup_spec_albedo :439-454
+  if (wo.z <= 0.0f) return 0.0f;
+  float ss = albedo_lookup(*p.luts.spec, wo, rough_proxy(p.ax, p.ay));
+  return albedo_with_ms(p, wo, p.ax, p.ay, p.specular_eta, ss);
+}
+inline float fuzz_albedo(const PbrBsdf& p, V3 wo) {  // lookup_fuzz_albedo :456-462
+  if (p.fuzz <= 0.0f) return 0.0f;
+  return zeltner_dir_albedo(clampf(wo.z, 0.0f, 1.0f), clampf(p.fuzz_roughness, 0.01f, 1.0f));
+}
+inline V3 coat_absorb_tint(const PbrBsdf& p, V3 wo, V3 wi) {  // :483-492
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return vsplat(1.0f);
+  float mo = refracted_cos(wo.z, p.coat_eta), mi = refracted_cos(wi.z, p.coat_eta);
+  float ex = 0.5f * (1.0f / std::fmax(mi, 1.0e-4f) + 1.0f / std::fmax(mo, 1.0e-4f));
+  return vec3_powf(p.coat_color, ex);
+}
+
+struct LayerW {
+  float coat_amp;
+  V3 metal, spec_brdf, fuzz, diff_brdf;
+};
+// layer_weights (:333-390); transmission = subsurface = 0, not thin-walled.
+inline LayerW layer_weights(const PbrBsdf& p, V3 wo, const V3* wi) {
+  const V3 one = vsplat(1.0f);
+  float e_fuzz = fuzz_albedo(p, wo);
+  V3 fuzz_w = smul(p.fuzz_color, p.fuzz);
+  float bfs = std::fmax(1.0f - p.fuzz * e_fuzz, 0.0f);
+  V3 below_fuzz = vsplat(bfs);
+  float coat_amp = p.coat;
+  float e_coat = coat_amp > 0.0f ? coat_albedo(p, wo) : 0.0f;
+  V3 under_coat = below_fuzz;
+  if (coat_amp > 0.0f) {
+    V3 tint = wi ? coat_absorb_tint(p, wo, *wi) : p.coat_color;
+    V3 thr = vmul(vmul(p.coat_darkening, tint), vsplat(std::fmax(1.0f - e_coat, 0.0f)));
+    under_coat = vmul(below_fuzz, add(smul(one, 1.0f - coat_amp), smul(thr, coat_amp)));
+  }
+  LayerW w;
+  w.metal = smul(under_coat, p.metalness);
+  V3 below_metal = smul(under_coat, 1.0f - p.metalness);
+  float e_spec = spec_albedo(p, wo);
+  w.spec_brdf = vmul(below_metal, p.specular_color);
+  V3 leak = vmax(sub(one, smul(p.specular_color, e_spec)), vsplat(0.0f));
+  V3 below_spec = vmul(below_metal, leak);
+  V3 below_trans = smul(below_spec, 1.0f - 0.0f);
+  w.diff_brdf = vmul(smul(below_trans, (1.0f - 0.0f) * p.base), p.base_color);
+  w.coat_amp = coat_amp * bfs;
+  w.fuzz = fuzz_w;
+  return w;
+}
+struct Probs {
+  float coat, metal, spec, fuzz, diff, total;
+};
+inline Probs lobe_probs(const PbrBsdf& p, V3 wo) {  // :392-422
+  LayerW w = layer_weights(p, wo, nullptr);
+  Probs r;
+  r.coat = p.coat > 0.0f ? w.coat_amp : 0.0f;
+  r.metal = luminance(vmul(w.metal, add(vsplat(0.5f), smul(p.base_color, 0.5f))));
+  r.spec = luminance(w.spec_brdf);
+  const float spec_btdf = 0.0f;
+  r.fuzz = std::fmax(luminance(w.fuzz), 0.0f);
+  r.diff = std::fmax(luminance(w.diff_brdf), 0.0f);
+  const float diff_btdf = std::fmax(luminance(vsplat(0.0f)), 0.0f);
+  r.total = r.coat + r.metal + r.spec + spec_btdf + r.fuzz + r.diff + diff_btdf;
+  return r;
+}
+
+inline float eval_refl_ms(const PbrBsdf& p, V3 wo, V3 wi, float ax, float ay, float eta) {  // :648-670
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  MsParams s = ms_params(p, ax, ay, eta);
+  float eo = diel_lookup_e(*p.luts.diel, wo.z, s.req, s.eta_o);
+  float ei = diel_lookup_e(*p.luts.diel, wi.z, s.req, s.eta_o);
+  return ms_factor(s) * (1.0f - eo) * (1.0f - ei) / (kPi * s.ome_o);
+}
+inline float eval_coat(const PbrBsdf& p, V3 wo, V3 wi) {  // :698-722 (one channel)
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  V3 wm;
+  if (!reflection_half_vector(wo, wi, &wm)) return 0.0f;
+  float d = ggx_d(wm, p.cax, p.cay), g = ggx_g2(wo, wi, p.cax, p.cay);
+  if (d <= 0.0f || g <= 0.0f) return 0.0f;
+  float f = fresnel_dielectric(std::fabs(dot(wo, wm)), 1.0f, p.coat_eta);
+  float ss = d * g * f / (4.0f * wo.z * wi.z);
+  return ss + eval_refl_ms(p, wo, wi, p.cax, p.cay, p.coat_eta);
+}
+inline float pdf_coat(const PbrBsdf& p, V3 wo, V3 wi) {  // :724-742
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  TwoLobe w = refl_lobe_weights(ms_params(p, p.cax, p.cay, p.coat_eta));
+  float pms = cosine_hemisphere_pdf(wi.z);
+  V3 wm;
+  if (!reflection_half_vector(wo, wi, &wm)) return w.ms * pms / w.total;
+  float pw = pdf_wm_bounded_vndf(wo, wm, p.cax, p.cay);
+  float den = 4.0f * std::fabs(dot(wo, wm));
+  if (den <= 0.0f) return w.ms * pms / w.total;
+  return (w.ss * (pw / den) + w.ms * pms) / w.total;
+}
+inline V3 metal_fresnel(const PbrBsdf& p, float c) {  // metal_f82_tint_fresnel :792-802
+  V3 f0 = vmax(smul(p.base_color, p.base), vsplat(0.0f));
+  V3 fs = schlick_vec3(f0, c), f82 = schlick_vec3(f0, kCos82);
+  V3 ft = vmul(vclamp01(p.specular_color), f82);
+  float cc = clampf(c, 0.0f, 1.0f);
+  float den = kCos82 * powi_const(1.0f - kCos82, 6);
+  float corr = cc * powi_rt(1.0f - cc, 6) / std::fmax(den, 1.0e-8f);
+  V3 r = sub(fs, smul(sub(f82, ft), corr));
+  return vclamp01(smul(r, p.specular));
+}
+inline V3 metal_fresnel_avg(const PbrBsdf& p) {  // metal_f82_tint_fresnel_avg :832-839
+  V3 f0 = vmax(smul(p.base_color, p.base), vsplat(0.0f));
+  V3 f82 = schlick_vec3(f0, kCos82);
+  V3 ft = vmul(vclamp01(p.specular_color), f82);
+  float den = kCos82 * powi_const(1.0f - kCos82, 6);
+  V3 b = sdiv(sub(f82, ft), std::fmax(den, 1.0e-8f));
+  V3 r = sub(add(f0, sdiv(sub(vsplat(1.0f), f0), 21.0f)), sdiv(b, 126.0f));
+  return vclamp01(smul(r, p.specular));
+}
+inline V3 eval_metal(const PbrBsdf& p, V3 wo, V3 wi) {  // :744-759, 804-816
+  const V3 zero = vsplat(0.0f);
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return zero;
+  V3 wm;
+  if (!reflection_half_vector(wo, wi, &wm)) return zero;
+  float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
+  if (d <= 0.0f || g <= 0.0f) return zero;
+  V3 f = metal_fresnel(p, std::fabs(dot(wo, wm)));
+  V3 ss = smul(f, d * g / (4.0f * wo.z * wi.z));
+  // eval_metal_ms
+  float req = std::pow(p.ax * p.ay, 0.25f);
+  float eo = cond_lookup_e(*p.luts.cond, wo.z, req), ei = cond_lookup_e(*p.luts.cond, wi.z, req);
+  float ea = cond_lookup_eavg(*p.luts.cond, req);
+  float omea = std::fmax(1.0f - ea, kMsDenomEps);
+  V3 fa = metal_fresnel_avg(p);
+  // compute_conductor_f_ms :2137-2146
+  float om = std::fmax(1.0f - ea, 0.0f);
+  V3 den = sub(vsplat(1.0f), smul(fa, om));
+  V3 ds = v3(std::fmax(den.x, kMsDenomEps), std::fmax(den.y, kMsDenomEps), std::fmax(den.z, kMsDenomEps));
+  V3 fms = smul(vmul(fa, fa), ea);
+  fms = v3(fms.x / ds.x, fms.y / ds.y, fms.z / ds.z);
+  return add(ss, smul(fms, (1.0f - eo) * (1.0f - ei) / (kPi * omea)));
+}
+inline float base_dielectric_fresnel(const PbrBsdf& p, float c, float ep, float epr) {  // :902-929
+  float un = p.front_face ? weighted_fresnel_ratio(c, ep, epr) : weighted_fresnel_ratio(c, 1.0f / ep, 1.0f / epr);
+  float coat = clampf(p.coat, 0.0f, 1.0f);
+  if (coat <= 0.0f) return un;
+  float ce = std::fmax(p.coat_eta, 1.0e-4f);
+  float eb = ep / ce, ebp = epr / ce;
+  float a, b;
+  if (p.coat_eta > ep) {
+    a = 1.0f / std::fmax(eb, 1.0e-4f);
+    b = 1.0f / std::fmax(ebp, 1.0e-4f);
+  } else {
+    a = eb;
+    b = ebp;
+  }
+  float co = p.front_face ? weighted_fresnel_ratio(c, a, b)
+                          : weighted_fresnel_ratio(c, 1.0f / std::fmax(a, 1.0e-4f), 1.0f / std::fmax(b, 1.0e-4f));
+  return lerpf(un, co, coat);
+}
+inline float eval_spec(const PbrBsdf& p, V3 wo, V3 wi) {  // eval_spec_brdf :841-867 (one channel)
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  V3 wm;
+  if (!reflection_half_vector(wo, wi, &wm)) return 0.0f;
+  float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
+  if (d <= 0.0f || g <= 0.0f) return 0.0f;
+  float f = base_dielectric_fresnel(p, std::fabs(dot(wo, wm)), p.transmission_eta, p.specular_eta);
+  float ss = f * (d * g / (4.0f * wo.z * wi.z));
+  return ss + eval_refl_ms(p, wo, wi, p.ax, p.ay, p.specular_eta);
+}
+inline float pdf_spec_ss(const PbrBsdf& p, V3 wo, V3 wi) {  // :981-994
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  V3 wm;
+  if (!reflection_half_vector(wo, wi, &wm)) return 0.0f;
+  float pw = pdf_wm_bounded_vndf(wo, wm, p.ax, p.ay);
+  float den = 4.0f * std::fabs(dot(wo, wm));
+  if (den <= 0.0f) return 0.0f;
+  return pw / den;
+}
+inline float pdf_metal(const PbrBsdf& p, V3 wo, V3 wi) {  // :996-1004
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  TwoLobe w = cond_lobe_weights(p);
+  return (w.ss * pdf_spec_ss(p, wo, wi) + w.ms * cosine_hemisphere_pdf(wi.z)) / w.total;
+}
+inline float pdf_spec(const PbrBsdf& p, V3 wo, V3 wi) {  // pdf_specular_brdf :1006-1022
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  TwoLobe w = refl_lobe_weights(ms_params(p, p.ax, p.ay, p.specular_eta));
+  return (w.ss * pdf_spec_ss(p, wo, wi) + w.ms * cosine_hemisphere_pdf(wi.z)) / w.total;
+}
+// OpenPbrFuzzBsdf (:148-205) with the LTC of Zeltner et al.
+inline float phi_of(V3 v) {  // :2060-2063
+  float ph = std::atan2(v.y, v.x);
+  return ph < 0.0f ? ph + 2.0f * kPi : ph;
+}
+inline V3 rotate_z(V3 v, float angle) {  // :2065-2068
+  float s = std::sin(angle), c = std::cos(angle);
+  return v3(c * v.x - s * v.y, s * v.x + c * v.y, v.z);
+}
+inline float eval_ltc(V3 wi, float a, float b) {  // :2097-2110
+  V3 w = v3(a * wi.x + b * wi.z, a * wi.y, wi.z);
+  float l = std::sqrt(dot(w, w));
+  if (l <= 0.0f) return 0.0f;
+  w = sdiv(w, l);
+  if (w.z <= 0.0f) return 0.0f;
+  float det = a * a;
+  float jac = det / (l * l * l);
+  return (w.z / kPi) * jac;
+}
+inline float fuzz_eval(const PbrBsdf& p, V3 wo, V3 wi, float* pdf) {  // eval (one channel) and pdf
+  float r = clampf(p.fuzz_roughness, 0.01f, 1.0f);
+  if (wo.z <= 0.0f || wi.z <= 0.0f) {
+    *pdf = 0.0f;
+    return 0.0f;
+  }
+  V3 ws = rotate_z(wi, -phi_of(wo));
+  float a = zeltner_a_inv(wo.z, r), b = zeltner_b_inv(wo.z, r);
+  float v = eval_ltc(ws, a, b);
+  *pdf = v;
+  float rr = zeltner_dir_albedo(wo.z, r);
+  return rr * v / std::fmax(wi.z, 1.0e-6f);
+}
+inline bool fuzz_sample(const PbrBsdf& p, V3 wo, float u0, float u1, V3* wi) {  // :184-200, sample_ltc :2112-2123
+  float r = clampf(p.fuzz_roughness, 0.01f, 1.0f);
+  if (wo.z <= 0.0f) return false;
+  float a = zeltner_a_inv(wo.z, r), b = zeltner_b_inv(wo.z, r);
+  float phi = 2.0f * kPi * u0;
+  float ct = std::sqrt(u1), st = std::sqrt(std::fmax(1.0f - u1, 0.0f));
+  V3 o = v3(std::cos(phi) * st, std::sin(phi) * st, ct);
+  V3 ws = normalize_or_zero3(v3(o.x / a - o.z * b / a, o.y / a, o.z));
+  if (ws.z <= 0.0f) return false;
+  V3 w = rotate_z(ws, phi_of(wo));
+  if (w.z <= 0.0f) return false;
+  *wi = w;
+  return true;
+}
+
+// OpenPbrBsdf::eval (:224-259) / pdf (:261-293), radiance mode.
+inline V3 bsdf_eval(const PbrBsdf& p, V3 wo, V3 wi) {
+  V3 total = vsplat(0.0f);
+  if (!(wo.z > 0.0f)) return total;
+  LayerW w = layer_weights(p, wo, &wi);
+  if (w.coat_amp > 0.0f && !coat_smooth(p)) total = add(total, smul(vsplat(eval_coat(p, wo, wi)), w.coat_amp));
+  if (len_sq(w.metal) > 0.0f && !metal_smooth(p)) total = add(total, vmul(w.metal, eval_metal(p, wo, wi)));
+  if (len_sq(w.spec_brdf) > 0.0f && !spec_smooth(p)) total = add(total, vmul(w.spec_brdf, vsplat(eval_spec(p, wo, wi))));
+  if (len_sq(w.fuzz) > 0.0f) {
+    float pd;
+    total = add(total, vmul(w.fuzz, vsplat(fuzz_eval(p, wo, wi, &pd))));
+  }
+  if (len_sq(w.diff_brdf) > 0.0f) {
+    V3 f = (wo.z <= 0.0f || wi.z <= 0.0f) ? vsplat(0.0f) : f_eon(vsplat(1.0f), p.diffuse_roughness, wo, wi);
+    total = add(total, vmul(w.diff_brdf, f));
+  }
+  return total;
+}
+inline float bsdf_pdf(const PbrBsdf& p, V3 wo, V3 wi) {
+  if (!(wo.z > 0.0f)) return 0.0f;
+  Probs pr = lobe_probs(p, wo);
+  if (pr.total <= 0.0f) return 0.0f;
+  float pdf = 0.0f;
+  if (pr.coat > 0.0f && !coat_smooth(p)) pdf += (pr.coat / pr.total) * pdf_coat(p, wo, wi);
+  if (pr.metal > 0.0f && !metal_smooth(p)) pdf += (pr.metal / pr.total) * pdf_metal(p, wo, wi);
+  if (pr.spec > 0.0f && !spec_smooth(p)) pdf += (pr.spec / pr.total) * pdf_spec(p, wo, wi);
+  if (pr.fuzz > 0.0f) {
+    float pd;
+    fuzz_eval(p, wo, wi, &pd);
+    pdf += (pr.fuzz / pr.total) * pd;
+  }
+  if (pr.diff > 0.0f) {
+    float pd = (wo.z <= 0.0f || wi.z <= 0.0f) ? 0.0f : pdf_eon(wo, wi, p.diffuse_roughness);
+    pdf += (pr.diff / pr.total) * pd;
+  }
+  return pdf;
+}
+// finalize_rough_sample (:1958-1999)
+inline bool finalize(const PbrBsdf& p, V3 wo, V3 wi, BSample* out) {
+  float pdf = bsdf_pdf(p, wo, wi);
+  if (pdf <= 0.0f) return false;
+  V3 f = bsdf_eval(p, wo, wi);
+  if (len_sq(f) == 0.0f) return false;
+  float c = std::fabs(wi.z);
+  if (c <= 0.0f) return false;
+  out->weight = smul(f, c / pdf);
+  out->wi = wi;
+  out->pdf = pdf;
+  out->delta = false;
+  return true;
+}
+inline bool delta_sample(V3 weight, V3 wi, float p_lobe, BSample* out) {
+  out->weight = weight;
+  out->wi = wi;
+  out->pdf = p_lobe;
+  out->delta = true;
+  return true;
+}
+// the two-lobe (single scattering by bounded VNDF, multiple scattering by
+// cosine) sampling of sample_coat / sample_metal / sample_spec_brdf
+inline bool sample_two_lobe(const PbrBsdf& p, V3 wo, float ax, float ay, TwoLobe w, float ux, float uy, BSample* out) {
+  float ssp = w.ss / w.total;
+  if (ux >= ssp) {
+    float u = ssp < 1.0f ? (ux - ssp) / (1.0f - ssp) : 0.0f;
+    return finalize(p, wo, cosine_hemisphere(u, uy), out);
+  }
+  float u = ssp > 0.0f ? ux / ssp : 0.0f;
+  V3 wm;
+  if (!sample_wm_bounded_vndf(wo, ax, ay, u, uy, &wm)) return false;
+  V3 wi = reflect_local(wo, wm);
+  if (wi.z <= 0.0f) return false;
+  return finalize(p, wo, wi, out);
+}
+// OpenPbrBsdf::sample (:295-331) with pick_lobe (:2016-2041)
+inline bool bsdf_sample(const PbrBsdf& p, V3 wo, float u_lobe, float ux, float uy, BSample* out) {
+  if (!(wo.z > 0.0f)) return false;
+  Probs pr = lobe_probs(p, wo);
+  if (pr.total <= 0.0f) return false;
+  float u = u_lobe * pr.total;
+  int lobe;  // 0 coat, 1 metal, 2 spec, 3 spec btdf, 4 fuzz, 5 diff, 6 diff btdf
+  float plv;
+  if (u < pr.coat) {
+    lobe = 0;
+    plv = pr.coat;
+  } else if ((u -= pr.coat) < pr.metal) {
+    lobe = 1;
+    plv = pr.metal;
+  } else if ((u -= pr.metal) < pr.spec) {
+    lobe = 2;
+    plv = pr.spec;
+  } else if ((u -= pr.spec) < 0.0f) {
+    lobe = 3;
+    plv = 0.0f;
+  } else if ((u -= 0.0f) < pr.fuzz) {
+    lobe = 4;
+    plv = pr.fuzz;
+  } else if ((u -= pr.fuzz) < pr.diff) {
+    lobe = 5;
+    plv = pr.diff;
+  } else {
+    lobe = 6;
+    plv = 0.0f;
+  }
+  float p_lobe = plv / pr.total;
+  if (p_lobe <= 0.0f) return false;
+  if (lobe == 0) {  // sample_coat :1467-1530
+    if (coat_smooth(p)) {
+      V3 wi = v3(-wo.x, -wo.y, wo.z);
+      float f = fresnel_dielectric(std::fabs(wi.z), 1.0f, p.coat_eta);
+      LayerW w = layer_weights(p, wo, &wi);
+      return delta_sample(vsplat(w.coat_amp * f / p_lobe), wi, p_lobe, out);
+    }
+    return sample_two_lobe(p, wo, p.cax, p.cay, refl_lobe_weights(ms_params(p, p.cax, p.cay, p.coat_eta)), ux, uy, out);
+  }
+  if (lobe == 1) {  // sample_metal :1532-1587
+    if (metal_smooth(p)) {
+      V3 wi = v3(-wo.x, -wo.y, wo.z);
+      V3 f = metal_fresnel(p, std::fabs(wi.z));
+      LayerW w = layer_weights(p, wo, &wi);
+      return delta_sample(sdiv(vmul(w.metal, f), p_lobe), wi, p_lobe, out);
+    }
+    return sample_two_lobe(p, wo, p.ax, p.ay, cond_lobe_weights(p), ux, uy, out);
+  }
+  if (lobe == 2) {  // sample_spec_brdf :1589-1668
+    if (spec_smooth(p)) {
+      V3 wi = v3(-wo.x, -wo.y, wo.z);
+      float f = base_dielectric_fresnel(p, std::fabs(wi.z), p.transmission_eta, p.specular_eta);
+      LayerW w = layer_weights(p, wo, &wi);
+      return delta_sample(sdiv(vmul(w.spec_brdf, vsplat(f)), p_lobe), wi, p_lobe, out);
+    }
+    return sample_two_lobe(p, wo, p.ax, p.ay, refl_lobe_weights(ms_params(p, p.ax, p.ay, p.specular_eta)), ux, uy, out);
+  }
+  if (lobe == 4) {  // sample_fuzz :1929-1934
+    V3 wi;
+    if (!fuzz_sample(p, wo, ux, uy, &wi)) return false;
+    return finalize(p, wo, wi, out);
+  }
+  if (lobe == 5) {  // sample_diff_brdf :1936-1940 -> EonBsdf::sample (R/bsdf/eon.rs:40-58)
+    float pdf;
+    V3 wi = sample_eon(wo, p.diffuse_roughness, ux, uy, &pdf);
+    if (pdf <= 0.0f || wi.z <= 0.0f) return false;
+    return finalize(p, wo, wi, out);
+  }
+  return false;  // spec / diffuse BTDF: not carried (transmission 0, not thin-walled)
+}
+
+
+// ---- table cache --------------------------------------------------------------------------
+}  // namespace orc
+#include <map>
+#include <memory>
+#include <mutex>
+namespace orc {
+inline std::mutex& lut_mutex() {
+  static std::mutex m;
+  return m;
+}
+inline const AlbedoLut& albedo_lut_for(float eta) {
+  static std::map<uint32_t, std::unique_ptr<AlbedoLut>> cache;
+  const float e = sanitize_eta(eta);
+  uint32_t key;
+  std::memcpy(&key, &e, 4);
+  std::lock_guard<std::mutex> g(lut_mutex());
+  auto it = cache.find(key);
+  if (it == cache.end()) it = cache.emplace(key, std::make_unique<AlbedoLut>(build_albedo_lut(e))).first;
+  return *it->second;
+}
+inline const CondLut& cond_lut() {
+  static const CondLut l = build_cond_lut();
+  return l;
+}
+inline const DielLut& diel_lut() {
+  static const DielLut l = build_diel_lut();
+  return l;
+}
+inline const AlbedoLut& zero_albedo_lut() {
+  static const AlbedoLut l = [] {
+    AlbedoLut z;
+    z.v.assign(kAlbedoLen, 0.0f);
+    return z;
+  }();
+  return l;
+}
+inline const CondLut& zero_cond_lut() {
+  static const CondLut l = [] {
+    CondLut z;
+    z.e.assign(kCondCos * kCondRough, 0.0f);
+    z.eavg.assign(kCondRough, 0.0f);
+    return z;
+  }();
+  return l;
+}
+inline const DielLut& zero_diel_lut() {
+  static const DielLut l = [] {
+    DielLut z;
+    z.e.assign(kDielCos * kDielRough * kDielEta, 0.0f);
+    z.eavg.assign(kDielRough * kDielEta, 0.0f);
+    return z;
+  }();
+  return l;
+}
+}  // namespace orc

@@ -601,7 +601,7 @@ inline size_t next_pow2(size_t v) {
   return p;
 }
 inline float row_sin_theta(size_t j, size_t h) { return std::sin(((float)j + 0.5f) / (float)h * kPiF); }
-inline float positive_luminance(V3 c) { return fmaxr(luminance(c), 0.0f); }
+inline float positive_luminance(V3 c) { return fmaxr(lt::luminance(c), 0.0f); }
 
 inline Dist build_distribution(size_t w, size_t h, const std::vector<float>& weights) {  // :426-478
   Dist d;

@@ -5,19 +5,12 @@
 //   R/integrator/mis.rs:19-139, 212-257  trace_radiance, emission MIS
 //   R/integrator/mod.rs:52-85            spawn_ray / offset_ray_origin
 //   R/scene/mod.rs:426-544               shading vertex
-//   R/math/orthonormal_basis.rs          ONB (Duff et al.)
-//   R/material/open_pbr.rs:351-606       OpenPbrMaterial le / make_bsdf / sample / eval / pdf
+//   R/material/open_pbr.rs:351-606       OpenPbrMaterial le (with exit tint) / sample / eval / pdf
 //   R/material/mod.rs:33-79              shading-normal correction
-//   R/bsdf/open_pbr.rs                   OpenPbrBsdf (layer weights, lobe choice, lobes)
-//   R/bsdf/eon.rs                        EON diffuse
-//   R/bsdf/smith_ggx.rs                  GGX D / Lambda / G2, bounded VNDF pdf
+//   bsdf.h                               the BSDFs and GGX tables
 //
-// Supported OpenPBR subset (the same one the GPU path supports): specular
-// weight 0 (the modulated specular eta is then exactly 1, so f_avg = 0 and
-// every LUT-dependent term is multiplied by zero: the reference's directional
-// albedo and energy-compensation tables do not influence the result), no
-// metal, coat, fuzz, transmission, thin film, thin walls, textures or normal
-// maps; base weight / colour / diffuse roughness and emission are free.
+// OpenPBR materials carry base, metal, specular, coat, fuzz and emission
+// (bsdf.h lists what is not carried).
 // Light NEE under the reference policy samples no light-tree light for
 // OpenPBR materials (R/material/open_pbr.rs:592-606 returns no light-tree
 // query); the environment light, when present, is sampled for every
@@ -34,477 +27,12 @@
 #include <cstdint>
 #include <vector>
 
+#include "bsdf.h"
 #include "geometry.h"
 #include "lights.h"
 #include "sampler.h"
 
 namespace orc {
-
-constexpr float kPi = 3.14159265358979323846f;
-constexpr float kTau = 6.28318530717958647692f;
-
-// ---- glam-style helpers --------------------------------------------------
-inline V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
-inline V3 vmul(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
-inline V3 smul(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
-inline V3 sdiv(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
-inline float len_sq(V3 a) { return dot(a, a); }
-inline V3 normalize3(V3 a) {  // v * (1 / |v|)
-  float r = 1.0f / std::sqrt(dot(a, a));
-  return v3(a.x * r, a.y * r, a.z * r);
-}
-inline V3 normalize_or_zero3(V3 a) {
-  float r = 1.0f / std::sqrt(dot(a, a));
-  if (std::isfinite(r) && r > 0.0f) return v3(a.x * r, a.y * r, a.z * r);
-  return v3(0.f, 0.f, 0.f);
-}
-inline float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }  // f32::clamp
-inline V3 face_forward(V3 n, V3 ref) { return dot(n, ref) < 0.0f ? neg(n) : n; }  // R/math/mod.rs:42-48
-inline float luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }  // R/math/sg.rs:314-316
-inline V3 interp3(float b0, float b1, float b2, V3 a, V3 b, V3 c) {
-  return add(add(mul(b0, a), mul(b1, b)), mul(b2, c));
-}
-
-// ---- OrthonormalBasis, R/math/orthonormal_basis.rs:11-71 -------------------
-struct Onb {
-  V3 t, b, n;
-};
-inline Onb onb_from_normal(V3 normal) {
-  V3 n = len_sq(normal) > 0.0f ? normalize3(normal) : v3(0.f, 0.f, 1.f);
-  float sign = std::copysign(1.0f, n.z);
-  float a = -1.0f / (sign + n.z);
-  float b = n.x * n.y * a;
-  V3 t = v3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
-  V3 bt = v3(b, sign + n.y * n.y * a, -n.y);
-  return Onb{t, bt, n};
-}
-inline Onb onb_from_normal_and_tangent(V3 normal, V3 hint) {
-  V3 n = len_sq(normal) > 0.0f ? normalize3(normal) : v3(0.f, 0.f, 1.f);
-  V3 t = normalize_or_zero3(sub(hint, mul(dot(hint, n), n)));
-  if (len_sq(t) == 0.0f) return onb_from_normal(n);
-  V3 b = normalize_or_zero3(cross(n, t));
-  t = normalize_or_zero3(cross(b, n));
-  return Onb{t, b, n};
-}
-inline V3 onb_to_world(const Onb& f, V3 l) { return normalize3(add(add(mul(l.x, f.t), mul(l.y, f.b)), mul(l.z, f.n))); }
-inline V3 onb_to_local(const Onb& f, V3 w) { return v3(dot(w, f.t), dot(w, f.b), dot(w, f.n)); }
-
-// ---- GGX, R/bsdf/smith_ggx.rs ------------------------------------------------
-constexpr float kMinAlpha = 1.0e-4f;
-constexpr float kSmoothAlpha = 1.0e-3f;
-inline bool reflection_half_vector(V3 wo, V3 wi, V3* wm) {
-  V3 m = normalize_or_zero3(add(wo, wi));
-  if (len_sq(m) == 0.0f) return false;
-  *wm = m.z < 0.0f ? neg(m) : m;
-  return true;
-}
-inline float ggx_d(V3 m, float ax, float ay) {
-  if (m.z <= 0.0f) return 0.0f;
-  float term = m.x * m.x / (ax * ax) + m.y * m.y / (ay * ay) + m.z * m.z;
-  float denom = kPi * ax * ay * term * term;
-  return denom <= 0.0f ? 0.0f : 1.0f / denom;
-}
-inline float ggx_lambda(V3 w, float ax, float ay) {
-  float c = std::fabs(w.z);
-  if (c <= 0.0f) return INFINITY;
-  float term = 1.0f + (ax * ax * w.x * w.x + ay * ay * w.y * w.y) / (c * c);
-  return 0.5f * (-1.0f + std::sqrt(term));
-}
-inline float ggx_g2(V3 wo, V3 wi, float ax, float ay) {
-  float lo = ggx_lambda(wo, ax, ay), li = ggx_lambda(wi, ax, ay);
-  if (!std::isfinite(lo) || !std::isfinite(li)) return 0.0f;
-  return 1.0f / (1.0f + lo + li);
-}
-inline float pdf_wm_bounded_vndf(V3 wo, V3 wm, float ax, float ay) {
-  if (wo.z <= 0.0f || wm.z <= 0.0f) return 0.0f;
-  ax = std::fmax(ax, kMinAlpha);
-  ay = std::fmax(ay, kMinAlpha);
-  float a = clampf(std::fmin(ax, ay), 0.0f, 1.0f);
-  float s = 1.0f + std::sqrt(wo.x * wo.x + wo.y * wo.y);
-  float a2 = a * a, s2 = s * s;
-  float k = (1.0f - a2) * s2 / (s2 + a2 * wo.z * wo.z);
-  float t = std::sqrt(ax * ax * wo.x * wo.x + ay * ay * wo.y * wo.y + wo.z * wo.z);
-  float denom = k * wo.z + t;
-  if (denom <= 0.0f) return 0.0f;
-  return 2.0f * ggx_d(wm, ax, ay) * std::fmax(dot(wo, wm), 0.0f) / denom;
-}
-
-// ---- fresnel, R/math/mod.rs:50-72 -----------------------------------------------
-inline float fresnel_dielectric(float ci, float ei, float et) {
-  ci = clampf(ci, -1.0f, 1.0f);
-  if (ci < 0.0f) {
-    std::swap(ei, et);
-    ci = -ci;
-  }
-  float si = std::sqrt(std::fmax(1.0f - ci * ci, 0.0f));
-  float st = (ei / et) * si;
-  if (st >= 1.0f) return 1.0f;
-  float ct = std::sqrt(std::fmax(1.0f - st * st, 0.0f));
-  float rpar = (et * ci - ei * ct) / (et * ci + ei * ct);
-  float rper = (ei * ci - et * ct) / (ei * ci + et * ct);
-  return 0.5f * (rpar * rpar + rper * rper);
-}
-// weighted_fresnel_ratio, R/bsdf/open_pbr.rs:2161-2172
-inline float weighted_fresnel_ratio(float c, float eti, float etp) {
-  eti = std::fmax(eti, 1.0e-4f);
-  etp = std::fmax(etp, 1.0e-4f);
-  if (etp >= 1.0f) return fresnel_dielectric(c, 1.0f, etp);
-  float cc = clampf(c, 0.0f, 1.0f);
-  float mu2 = 1.0f - (1.0f - cc * cc) / (eti * eti);
-  if (mu2 <= 0.0f) return 1.0f;
-  return fresnel_dielectric(std::sqrt(mu2), 1.0f, 1.0f / etp);
-}
-// f_avg_dielectric, R/bsdf/open_pbr.rs:2127-2135
-inline float f_avg_dielectric(float eta) {
-  if (eta >= 1.0f) return clampf((eta - 1.0f) / (4.08567f + 1.00071f * eta), 0.0f, 1.0f);
-  float e = std::fmax(eta, 1.0e-6f);
-  return clampf(0.997118f + 0.1014f * e - 0.965241f * e * e - 0.130607f * e * e * e, 0.0f, 1.0f);
-}
-
-// ---- EON, R/bsdf/eon.rs:12-197 -----------------------------------------------
-constexpr float kRcpPi = 1.0f / kPi;
-const float kC1Fon = 0.5f - 2.0f / (3.0f * kPi);
-const float kC2Fon = 2.0f / 3.0f - 28.0f / (15.0f * kPi);
-inline float e_fon_approx(float mu, float r) {
-  float mc = 1.0f - mu;
-  float g = mc * (0.05710853f + mc * (0.49188188f + mc * (-0.33218145f + mc * 0.071443f)));
-  return (1.0f + r * g) / (1.0f + kC1Fon * r);
-}
-inline V3 f_eon(V3 rho, float r, V3 wo, V3 wi) {
-  float mo = wo.z, mi = wi.z;
-  float s = dot(wo, wi) - mi * mo;
-  float sot = s > 0.0f ? s / std::fmax(mo, mi) : s;
-  float af = 1.0f / (1.0f + kC1Fon * r);
-  V3 fss = mul(kRcpPi * af * (1.0f + r * sot), rho);
-  float efo = e_fon_approx(mo, r), efi = e_fon_approx(mi, r);
-  float avg = af * (1.0f + kC2Fon * r);
-  float oma = 1.0f - avg;
-  V3 denom = sub(v3(1.f, 1.f, 1.f), smul(rho, oma));
-  V3 rho_ms = v3(rho.x * rho.x * avg / denom.x, rho.y * rho.y * avg / denom.y, rho.z * rho.z * avg / denom.z);
-  float scale = kRcpPi * std::fmax(1.0f - efo, 1.0e-7f) * std::fmax(1.0f - efi, 1.0e-7f) / std::fmax(oma, 1.0e-7f);
-  return add(fss, smul(rho_ms, scale));
-}
-inline void ltc_coeffs(float mu, float r, float* a, float* b, float* c, float* d) {
-  *a = 1.0f + r * (0.303392f + (-0.518982f + 0.111709f * mu) * mu + (-0.276266f + 0.335918f * mu) * r);
-  *b = r * (-1.16407f + 1.15859f * mu + (0.150815f - 0.150105f * mu) * r) / (mu * mu * mu - 1.43545f);
-  *c = 1.0f + r * (0.20013f + (-0.506373f + 0.261777f * mu) * mu);
-  *d = r * (0.540852f + (-1.01625f + 0.475392f * mu) * mu) / (-1.0743f + (0.0725628f + mu) * mu);
-}
-inline void ltc_basis_x(V3 wo, float* x, float* y) {
-  float l2 = wo.x * wo.x + wo.y * wo.y;
-  if (l2 > 0.0f) {
-    float inv = 1.0f / std::sqrt(l2);
-    *x = wo.x * inv;
-    *y = wo.y * inv;
-  } else {
-    *x = 1.0f;
-    *y = 0.0f;
-  }
-}
-inline V3 ltc_from(V3 wo, V3 v) {
-  float xx, xy;
-  ltc_basis_x(wo, &xx, &xy);
-  float yx = -xy, yy = xx;
-  return v3(xx * v.x + yx * v.y, xy * v.x + yy * v.y, v.z);
-}
-inline V3 ltc_to(V3 wo, V3 v) {
-  float xx, xy;
-  ltc_basis_x(wo, &xx, &xy);
-  float yx = -xy, yy = xx;
-  return v3(xx * v.x + xy * v.y, yx * v.x + yy * v.y, v.z);
-}
-inline V3 cltc_sample(V3 wo, float r, float u1, float u2, float* pdf) {
-  float a, b, c, d;
-  ltc_coeffs(wo.z, r, &a, &b, &c, &d);
-  float radius = std::sqrt(u1);
-  float phi = 2.0f * kPi * u2;
-  float x0 = radius * std::cos(phi), y0 = radius * std::sin(phi);
-  float vz = 1.0f / std::sqrt(d * d + 1.0f);
-  float s = 0.5f * (1.0f + vz);
-  float edge = std::sqrt(std::fmax(1.0f - y0 * y0, 0.0f));
-  float x = -((1.0f - s) * edge + s * x0);
-  float y = y0;
-  float whz = std::sqrt(std::fmax(1.0f - (x * x + y * y), 0.0f));
-  V3 wh = v3(x, y, whz);
-  float pdf_wh = wh.z / (kPi * s);
-  V3 wl = v3(a * wh.x + b * wh.z, c * wh.y, d * wh.x + wh.z);
-  float len = std::sqrt(dot(wl, wl));
-  float det = c * (a - b * d);
-  *pdf = pdf_wh * (len * len * len) / det;
-  return ltc_from(wo, sdiv(wl, len));
-}
-inline float cltc_pdf(V3 wo, V3 wi, float r) {
-  V3 wl = ltc_to(wo, wi);
-  float a, b, c, d;
-  ltc_coeffs(wo.z, r, &a, &b, &c, &d);
-  float det = c * (a - b * d);
-  V3 wh = v3(c * (wl.x - b * wl.z), (a - b * d) * wl.y, -c * (d * wl.x - a * wl.z));
-  float l2 = std::fmax(dot(wh, wh), 1.0e-30f);
-  float vz = 1.0f / std::sqrt(d * d + 1.0f);
-  float s = 0.5f * (1.0f + vz);
-  float q = det / l2;
-  return q * q * std::fmax(wh.z, 0.0f) / (kPi * s);
-}
-inline float p_uniform(float mu, float r) {
-  return std::pow(r, 0.1f) * (0.162925f + (-0.372058f + (0.538233f - 0.290822f * mu) * mu) * mu);
-}
-inline V3 uniform_lobe_sample(float u1, float u2) {
-  float st = std::sqrt(std::fmax(1.0f - u1 * u1, 0.0f));
-  float phi = 2.0f * kPi * u2;
-  return v3(st * std::cos(phi), st * std::sin(phi), u1);
-}
-inline V3 sample_eon(V3 wo, float r, float u1, float u2, float* pdf) {
-  float pu = p_uniform(wo.z, r);
-  float pc = 1.0f - pu;
-  V3 wi;
-  float pdf_c;
-  if (u1 < pu) {
-    float u1n = u1 / pu;
-    wi = uniform_lobe_sample(u1n, u2);
-    pdf_c = cltc_pdf(wo, wi, r);
-  } else {
-    float den = std::fmax(pc, 1.0e-30f);
-    float u1n = (u1 - pu) / den;
-    wi = cltc_sample(wo, r, u1n, u2, &pdf_c);
-  }
-  *pdf = pu * (1.0f / (2.0f * kPi)) + pc * pdf_c;
-  return wi;
-}
-inline float pdf_eon(V3 wo, V3 wi, float r) {
-  float pu = p_uniform(wo.z, r);
-  return pu * (1.0f / (2.0f * kPi)) + (1.0f - pu) * cltc_pdf(wo, wi, r);
-}
-
-// ---- materials -------------------------------------------------------------------
-// OpenPbrMaterial parameters (R/material/open_pbr.rs:23-82), the fields the
-// supported subset reads; unsupported ones must be zero.
-struct OpenPbr {
-  float base_weight;
-  V3 base_color;
-  float base_diffuse_roughness;
-  float base_metalness;
-  float specular_weight;
-  V3 specular_color;
-  float specular_roughness;
-  float specular_ior;
-  float specular_roughness_anisotropy;
-  float transmission_weight, coat_weight, fuzz_weight, thin_film_weight;
-  float emission_luminance;
-  V3 emission_color;
-  uint32_t thin_walled;
-};
-
-// OpenPbrBsdfParams of the subset (R/material/open_pbr.rs:393-487).
-struct PbrBsdf {
-  V3 base_color;
-  float base;
-  V3 specular_color;
-  float ax, ay;
-  float specular_eta, transmission_eta;
-  float diffuse_roughness;
-  bool front_face;
-};
-
-inline float modulated_eta(float eta, float w) {  // R/material/open_pbr.rs:871-878
-  if (std::fabs(w - 1.0f) < 1.0e-6f) return eta;
-  float f0 = std::pow((eta - 1.0f) / (eta + 1.0f), 2.0f);
-  float sg = (eta - 1.0f) > 0.0f ? 1.0f : ((eta - 1.0f) < 0.0f ? -1.0f : 1.0f);  // f32::signum (+0 -> +1)
-  float eps = std::sqrt(clampf(w * f0, 0.0f, 0.99999f)) * sg;
-  return std::fmax((1.0f + eps) / std::fmax(1.0f - eps, 1.0e-6f), 1.0e-4f);
-}
-inline void alpha_xy(float roughness, float anisotropy, float* ax, float* ay) {  // :861-869
-  float r = clampf(roughness, 0.0f, 1.0f);
-  float a = clampf(anisotropy, 0.0f, 1.0f);
-  float alpha = r * r;
-  float oma = 1.0f - a;
-  float x = alpha * std::sqrt(2.0f / (1.0f + oma * oma));
-  float y = oma * x;
-  *ax = clampf(x, kMinAlpha, 1.0f);
-  *ay = clampf(y, kMinAlpha, 1.0f);
-}
-
-inline PbrBsdf make_bsdf(const OpenPbr& m, bool front_face) {
-  PbrBsdf p;
-  p.base_color = m.base_color;  // * ONE (no texture)
-  float rough = clampf(m.specular_roughness, 0.0f, 1.0f);  // coat = fuzz = 0: no roughening
-  alpha_xy(rough, m.specular_roughness_anisotropy, &p.ax, &p.ay);
-  p.specular_color = v3(clampf(m.specular_color.x, 0.f, 1.f), clampf(m.specular_color.y, 0.f, 1.f),
-                        clampf(m.specular_color.z, 0.f, 1.f));
-  p.specular_eta = modulated_eta(std::fmax(m.specular_ior, 1.0e-4f), std::fmax(m.specular_weight, 0.0f));
-  p.transmission_eta = p.specular_eta;
-  p.base = clampf(m.base_weight, 0.0f, 1.0f);
-  p.diffuse_roughness = clampf(m.base_diffuse_roughness, 0.0f, 1.0f);
-  p.front_face = front_face;
-  return p;
-}
-
-struct LayerW {
-  V3 spec_brdf, diff_brdf;
-};
-// layer_weights (R/bsdf/open_pbr.rs:333-390) with fuzz = coat = metal =
-// transmission = subsurface = 0 and e_spec = 0 (eta 1).
-inline LayerW layer_weights(const PbrBsdf& p) {
-  V3 below_metal = v3(1.f, 1.f, 1.f);
-  float e_spec = 0.0f;
-  V3 spec_w = vmul(below_metal, p.specular_color);
-  V3 leak = vmax(sub(v3(1.f, 1.f, 1.f), smul(p.specular_color, e_spec)), v3(0.f, 0.f, 0.f));
-  V3 below_spec = vmul(below_metal, leak);
-  V3 below_trans = smul(below_spec, 1.0f - 0.0f);
-  V3 diff_w = vmul(smul(below_trans, (1.0f - 0.0f) * p.base), p.base_color);
-  return LayerW{spec_w, diff_w};
-}
-struct Probs {
-  float spec, diff, total;
-};
-inline Probs lobe_probs(const PbrBsdf& p) {  // :392-422
-  LayerW w = layer_weights(p);
-  float coat = 0.0f;
-  float metal = luminance(v3(0.f, 0.f, 0.f));
-  float spec = luminance(w.spec_brdf);
-  float spec_btdf = 0.0f;
-  float fuzz = std::fmax(luminance(v3(0.f, 0.f, 0.f)), 0.0f);
-  float diff = std::fmax(luminance(w.diff_brdf), 0.0f);
-  float diff_btdf = std::fmax(luminance(v3(0.f, 0.f, 0.f)), 0.0f);
-  float total = coat + metal + spec + spec_btdf + fuzz + diff + diff_btdf;
-  return Probs{spec, diff, total};
-}
-inline bool spec_smooth(const PbrBsdf& p) { return std::fmax(p.ax, p.ay) < kSmoothAlpha; }
-
-// Spec lobe pieces at eta 1 (dielectric_ms_params / lobe weights :593-636):
-// f_avg = 0, so ss = ms = 0, total = 1e-8 and the ms factor is 0.
-inline float spec_eval_scalar(const PbrBsdf& p, V3 wo, V3 wi) {  // eval_spec_brdf :841-867
-  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
-  V3 wm;
-  if (!reflection_half_vector(wo, wi, &wm)) return 0.0f;
-  float d = ggx_d(wm, p.ax, p.ay);
-  float g = ggx_g2(wo, wi, p.ax, p.ay);
-  if (d <= 0.0f || g <= 0.0f) return 0.0f;
-  float f = p.front_face ? weighted_fresnel_ratio(std::fabs(dot(wo, wm)), p.transmission_eta, p.specular_eta)
-                         : weighted_fresnel_ratio(std::fabs(dot(wo, wm)), 1.0f / p.transmission_eta,
-                                                  1.0f / p.specular_eta);
-  float ss = f * (d * g / (4.0f * wo.z * wi.z));
-  float ms = 0.0f * 0.0f;  // dielectric_reflection_ms_factor = 0 when f_avg = 0
-  return ss + ms;
-}
-inline float spec_pdf(const PbrBsdf& p, V3 wo, V3 wi) {  // pdf_specular_brdf :1006-1022
-  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
-  float pdf_ss = 0.0f;
-  V3 wm;
-  if (reflection_half_vector(wo, wi, &wm)) {
-    float pw = pdf_wm_bounded_vndf(wo, wm, p.ax, p.ay);
-    float den = 4.0f * std::fabs(dot(wo, wm));
-    pdf_ss = den <= 0.0f ? 0.0f : pw / den;
-  }
-  float pdf_ms = std::fmax(wi.z, 0.0f) / kPi;
-  const float ss_w = 0.0f, ms_w = 0.0f, total = std::fmax(ss_w + ms_w, 1.0e-8f);
-  return (ss_w * pdf_ss + ms_w * pdf_ms) / total;
-}
-
-// OpenPbrBsdf::eval (:224-259) / pdf (:261-293)
-inline V3 bsdf_eval(const PbrBsdf& p, V3 wo, V3 wi) {
-  if (!(wo.z > 0.0f)) return v3(0.f, 0.f, 0.f);
-  LayerW w = layer_weights(p);
-  V3 total = v3(0.f, 0.f, 0.f);
-  if (len_sq(w.spec_brdf) > 0.0f && !spec_smooth(p)) total = add(total, smul(w.spec_brdf, spec_eval_scalar(p, wo, wi)));
-  if (len_sq(w.diff_brdf) > 0.0f) {
-    V3 f = (wo.z <= 0.0f || wi.z <= 0.0f) ? v3(0.f, 0.f, 0.f) : f_eon(v3(1.f, 1.f, 1.f), p.diffuse_roughness, wo, wi);
-    total = add(total, vmul(w.diff_brdf, f));
-  }
-  return total;
-}
-inline float bsdf_pdf(const PbrBsdf& p, V3 wo, V3 wi) {
-  if (!(wo.z > 0.0f)) return 0.0f;
-  Probs pr = lobe_probs(p);
-  if (pr.total <= 0.0f) return 0.0f;
-  float pdf = 0.0f;
-  if (pr.spec > 0.0f && !spec_smooth(p)) pdf += (pr.spec / pr.total) * spec_pdf(p, wo, wi);
-  if (pr.diff > 0.0f) {
-    float pd = (wo.z <= 0.0f || wi.z <= 0.0f) ? 0.0f : pdf_eon(wo, wi, p.diffuse_roughness);
-    pdf += (pr.diff / pr.total) * pd;
-  }
-  return pdf;
-}
-
-struct BSample {
-  V3 weight, wi;
-  float pdf;
-  bool delta;
-};
-// finalize_rough_sample (:1958-1999)
-inline bool finalize(const PbrBsdf& p, V3 wo, V3 wi, BSample* out) {
-  float pdf = bsdf_pdf(p, wo, wi);
-  if (pdf <= 0.0f) return false;
-  V3 f = bsdf_eval(p, wo, wi);
-  if (len_sq(f) == 0.0f) return false;
-  float c = std::fabs(wi.z);
-  if (c <= 0.0f) return false;
-  out->weight = smul(f, c / pdf);
-  out->wi = wi;
-  out->pdf = pdf;
-  out->delta = false;
-  return true;
-}
-// sample_cosine_weighted_hemisphere, R/math/mod.rs:26-36
-inline V3 cosine_hemisphere(float u, float v) {
-  float r = std::sqrt(clampf(u, 0.0f, 1.0f));
-  float phi = kTau * v;
-  float dx = r * std::cos(phi), dy = r * std::sin(phi);
-  float z = std::sqrt(std::fmax(1.0f - (dx * dx + dy * dy), 0.0f));
-  return v3(dx, dy, z);
-}
-// OpenPbrBsdf::sample (:295-331) with pick_lobe (:2016-2041)
-inline bool bsdf_sample(const PbrBsdf& p, V3 wo, float u_lobe, float ux, float uy, BSample* out) {
-  if (!(wo.z > 0.0f)) return false;
-  Probs pr = lobe_probs(p);
-  if (pr.total <= 0.0f) return false;
-  float u = u_lobe * pr.total;
-  // coat, metal: 0
-  bool spec;
-  if (u < 0.0f) return false;  // coat (prob 0): u < 0 never holds for u_lobe >= 0
-  u -= 0.0f;
-  if (u < 0.0f) return false;  // metal
-  u -= 0.0f;
-  if (u < pr.spec) {
-    spec = true;
-  } else {
-    spec = false;  // spec_btdf, fuzz (prob 0) are skipped; diff_brdf else diff_btdf
-    u -= pr.spec;
-    u -= 0.0f;
-    u -= 0.0f;
-    if (!(u < pr.diff)) {
-      // ChosenLobe::DiffBtdf: probability 0 -> p_lobe = 0 -> None
-      return false;
-    }
-  }
-  float p_lobe = (spec ? pr.spec : pr.diff) / pr.total;
-  if (p_lobe <= 0.0f) return false;
-  if (spec) {  // sample_spec_brdf :1589-1668
-    if (spec_smooth(p)) {
-      V3 wi = v3(-wo.x, -wo.y, wo.z);
-      float f = p.front_face ? weighted_fresnel_ratio(std::fabs(wi.z), p.transmission_eta, p.specular_eta)
-                             : weighted_fresnel_ratio(std::fabs(wi.z), 1.0f / p.transmission_eta,
-                                                      1.0f / p.specular_eta);
-      LayerW w = layer_weights(p);
-      out->weight = sdiv(smul(w.spec_brdf, f), p_lobe);
-      out->wi = wi;
-      out->pdf = p_lobe;
-      out->delta = true;
-      return true;
-    }
-    const float ss_prob = 0.0f / 1.0e-8f;  // weights.ss / weights.total
-    if (ux >= ss_prob) {
-      float uu = ss_prob < 1.0f ? (ux - ss_prob) / (1.0f - ss_prob) : 0.0f;
-      return finalize(p, wo, cosine_hemisphere(uu, uy), out);
-    }
-    return false;  // unreachable at eta 1 (ss probability 0)
-  }
-  // sample_diff_brdf :1936-1940 -> EonBsdf::sample (R/bsdf/eon.rs:40-58)
-  if (wo.z <= 0.0f) return false;
-  float pdf;
-  V3 wi = sample_eon(wo, p.diffuse_roughness, ux, uy, &pdf);
-  if (pdf <= 0.0f || wi.z <= 0.0f) return false;
-  return finalize(p, wo, wi, out);
-}
 
 // ---- scene shading data ------------------------------------------------------------
 // Material enum of the backend (R/material/mod.rs:100-115 variants carried).
@@ -512,6 +40,7 @@ enum MatKind : uint32_t { kMatOpenPbr = 0, kMatLambert = 1, kMatEmissive = 2, kM
 struct Material {
   uint32_t kind = kMatOpenPbr;
   OpenPbr pbr{};
+  Luts luts;  // OpenPBR: the GGX tables it reads
   V3 color{0.f, 0.f, 0.f};  // Lambert rho, Emissive colour, Mirror colour
   float strength = 0.0f;    // Emissive
 };
@@ -608,29 +137,28 @@ inline Vertex shading_vertex(const RenderScene& rs, const SceneHit& h, V3 dir) {
   return vertex_at(rs, h.inst, h.tri, h.b0, h.b1, h.b2, dir);
 }
 
-// OpenPbrMaterial::le (R/material/open_pbr.rs:351-375) with exit tint 1.
+// OpenPbrMaterial::le (R/material/open_pbr.rs:351-375), no emission texture.
 inline bool pbr_le(const OpenPbr& m, const Vertex& v, V3* le) {
   if (m.emission_luminance <= 0.0f) return false;
   if (!m.thin_walled && !v.front_face) return false;
   V3 wl = normalize_or_zero3(onb_to_local(v.frame, v.wo));
-  float co = clampf(std::fabs(wl.z), 0.0f, 1.0f);
-  V3 tint = co <= 0.0f ? v3(0.f, 0.f, 0.f) : v3(1.f, 1.f, 1.f);
-  *le = vmul(smul(m.emission_color, std::fmax(m.emission_luminance, 0.0f)), tint);
+  *le = vmul(smul(vmul(m.emission_color, vsplat(1.0f)), std::fmax(m.emission_luminance, 0.0f)),
+             emission_exit_tint(m, wl));
   return true;
 }
 
 // OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548) in radiance mode.
 constexpr float kGeomCosEps = 1.0e-6f;  // R/material/mod.rs:31
-inline bool pbr_sample(const OpenPbr& m, const Vertex& v, float u_lobe, float ux, float uy, BSample* out,
+inline bool pbr_sample(const OpenPbr& m, const Luts& luts, const Vertex& v, float u_lobe, float ux, float uy, BSample* out,
                        V3* wi_world) {
   if (dot(v.wo, v.ng) <= 0.0f) return false;
   V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
-  PbrBsdf p = make_bsdf(m, v.front_face);
+  PbrBsdf p = make_bsdf(m, v.front_face, wol, luts);
   BSample s;
   if (!bsdf_sample(p, wol, u_lobe, ux, uy, &s)) return false;
   V3 wi = onb_to_world(v.frame, s.wi);
-  if (dot(wi, v.ng) <= kGeomCosEps) return false;  // reflection lobes only in the subset
+  if (dot(wi, v.ng) <= kGeomCosEps) return false;  // every carried lobe reflects
   *out = s;  // modified_bsdf_sample_weight leaves radiance-mode weights unchanged
   *wi_world = wi;
   return true;
@@ -664,7 +192,7 @@ inline bool mat_le(const Material& m, const Vertex& v, V3* le) {
 // MirrorMaterial::sample (mirror.rs:112-143), radiance mode.
 inline bool mat_sample(const Material& m, const Vertex& v, float u_lobe, float ux, float uy, BSample* out,
                        V3* wi_world) {
-  if (m.kind == kMatOpenPbr) return pbr_sample(m.pbr, v, u_lobe, ux, uy, out, wi_world);
+  if (m.kind == kMatOpenPbr) return pbr_sample(m.pbr, m.luts, v, u_lobe, ux, uy, out, wi_world);
   if (m.kind == kMatEmissive) return false;
   V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
@@ -713,7 +241,7 @@ inline V3 mat_eval(const Material& m, const Vertex& v, V3 wi) {
     V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
     V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
     if (wol.z <= 0.0f) return zero;
-    return shading_normal_correction(v, wi, bsdf_eval(make_bsdf(m.pbr, v.front_face), wol, wil));
+    return shading_normal_correction(v, wi, bsdf_eval(make_bsdf(m.pbr, v.front_face, wol, m.luts), wol, wil));
   }
   return zero;
 }
@@ -730,7 +258,7 @@ inline float mat_pdf(const Material& m, const Vertex& v, V3 wi) {
     V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
     V3 wil = normalize_or_zero3(onb_to_local(v.frame, wi));
     if (wol.z <= 0.0f) return 0.0f;
-    return bsdf_pdf(make_bsdf(m.pbr, v.front_face), wol, wil);
+    return bsdf_pdf(make_bsdf(m.pbr, v.front_face, wol, m.luts), wol, wil);
   }
   return 0.0f;
 }
@@ -1004,3 +532,4 @@ inline V3 trace_radiance(const RenderScene& rs, V3 o, V3 d, const PathSampler& p
 }
 
 }  // namespace orc
+

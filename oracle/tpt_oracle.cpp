@@ -372,7 +372,7 @@ extern "C" {
 // then the kind's parameters (TptOpenPbr for OpenPBR).
 struct OrcMaterial {
   uint32_t kind, pad;
-  float raw[22];
+  float raw[34];
 };
 
 // Shading data for an oracle scene: per mesh vertex normals / uvs (nullable),
@@ -421,6 +421,14 @@ void orc_render_set_materials(void* h, const OrcMaterial* m, size_t n) {
       uint32_t tw;
       std::memcpy(&tw, &f[21], 4);
       o.thin_walled = tw;
+      o.coat_color = v3(f[22], f[23], f[24]);
+      o.coat_roughness = f[25];
+      o.coat_roughness_anisotropy = f[26];
+      o.coat_ior = f[27];
+      o.coat_darkening = f[28];
+      o.fuzz_color = v3(f[29], f[30], f[31]);
+      o.fuzz_roughness = f[32];
+      d.luts = luts_for(o);
     } else {
       d.color = v3(f[0], f[1], f[2]);
       d.strength = f[3];
@@ -603,6 +611,133 @@ void orc_render(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t sp
     o[1] = color.y * exposure;
     o[2] = color.z * exposure;
   });
+}
+
+
+// ---- OpenPBR BSDF and GGX tables ---------------------------------------------------------
+static OpenPbr parse_open_pbr(const OrcMaterial* m) {
+  const float* f = m->raw;
+  OpenPbr o{};
+  o.base_weight = f[0];
+  o.base_color = v3(f[1], f[2], f[3]);
+  o.base_diffuse_roughness = f[4];
+  o.base_metalness = f[5];
+  o.specular_weight = f[6];
+  o.specular_color = v3(f[7], f[8], f[9]);
+  o.specular_roughness = f[10];
+  o.specular_ior = f[11];
+  o.specular_roughness_anisotropy = f[12];
+  o.transmission_weight = f[13];
+  o.coat_weight = f[14];
+  o.fuzz_weight = f[15];
+  o.thin_film_weight = f[16];
+  o.emission_luminance = f[17];
+  o.emission_color = v3(f[18], f[19], f[20]);
+  uint32_t tw;
+  std::memcpy(&tw, &f[21], 4);
+  o.thin_walled = tw;
+  o.coat_color = v3(f[22], f[23], f[24]);
+  o.coat_roughness = f[25];
+  o.coat_roughness_anisotropy = f[26];
+  o.coat_ior = f[27];
+  o.coat_darkening = f[28];
+  o.fuzz_color = v3(f[29], f[30], f[31]);
+  o.fuzz_roughness = f[32];
+  return o;
+}
+// Table sizes: albedo (per eta), conductor e / e_avg, dielectric e / e_avg.
+void orc_lut_sizes(uint32_t* out) {
+  out[0] = kAlbedoLen;
+  out[1] = kCondCos * kCondRough;
+  out[2] = kCondRough;
+  out[3] = kDielCos * kDielRough * kDielEta;
+  out[4] = kDielRough * kDielEta;
+}
+void orc_lut_albedo(float eta, float* out) {
+  const AlbedoLut& l = albedo_lut_for(eta);
+  std::memcpy(out, l.v.data(), 4 * l.v.size());
+}
+void orc_lut_cond(float* e, float* eavg) {
+  const CondLut& l = cond_lut();
+  std::memcpy(e, l.e.data(), 4 * l.e.size());
+  std::memcpy(eavg, l.eavg.data(), 4 * l.eavg.size());
+}
+void orc_lut_diel(float* e, float* eavg) {
+  const DielLut& l = diel_lut();
+  std::memcpy(e, l.e.data(), 4 * l.e.size());
+  std::memcpy(eavg, l.eavg.data(), 4 * l.eavg.size());
+}
+// which: 0 albedo(eta; w, roughness), 1 conductor e(cos, r), 2 conductor e_avg(r),
+// 3 dielectric e(cos, r, eta), 4 dielectric e_avg(r, eta)
+float orc_lut_lookup(int which, float eta, const float* w, float a, float b) {
+  switch (which) {
+    case 0: return albedo_lookup(albedo_lut_for(eta), v3(w[0], w[1], w[2]), a);
+    case 1: return cond_lookup_e(cond_lut(), a, b);
+    case 2: return cond_lookup_eavg(cond_lut(), a);
+    case 3: return diel_lookup_e(diel_lut(), a, b, eta);
+    default: return diel_lookup_eavg(diel_lut(), a, eta);
+  }
+}
+// f (3 per direction, radiance mode, no shading-normal correction) and pdf of
+// OpenPbrMaterial::make_bsdf(...).eval / pdf in the shading frame, for n
+// incident directions.
+void orc_openpbr_eval(const OrcMaterial* m, int front_face, const float* wo, const float* wi, size_t n, float* f,
+                      float* pdf) {
+  const OpenPbr o = parse_open_pbr(m);
+  const V3 w = v3(wo[0], wo[1], wo[2]);
+  const PbrBsdf p = make_bsdf(o, front_face != 0, w, luts_for(o));
+  for (size_t i = 0; i < n; ++i) {
+    const V3 d = v3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+    const V3 e = bsdf_eval(p, w, d);
+    f[3 * i] = e.x;
+    f[3 * i + 1] = e.y;
+    f[3 * i + 2] = e.z;
+    pdf[i] = bsdf_pdf(p, w, d);
+  }
+}
+// OpenPbrBsdf::sample for n (u_lobe, u0, u1) triples; out per sample:
+// ok, weight[3], wi[3], pdf, delta (9 floats).
+void orc_openpbr_sample(const OrcMaterial* m, int front_face, const float* wo, const float* u, size_t n, float* out) {
+  const OpenPbr o = parse_open_pbr(m);
+  const V3 w = v3(wo[0], wo[1], wo[2]);
+  const PbrBsdf p = make_bsdf(o, front_face != 0, w, luts_for(o));
+  for (size_t i = 0; i < n; ++i) {
+    BSample s{};
+    const bool ok = bsdf_sample(p, w, u[3 * i], u[3 * i + 1], u[3 * i + 2], &s);
+    float* r = out + 9 * i;
+    r[0] = ok ? 1.0f : 0.0f;
+    r[1] = ok ? s.weight.x : 0.0f;
+    r[2] = ok ? s.weight.y : 0.0f;
+    r[3] = ok ? s.weight.z : 0.0f;
+    r[4] = ok ? s.wi.x : 0.0f;
+    r[5] = ok ? s.wi.y : 0.0f;
+    r[6] = ok ? s.wi.z : 0.0f;
+    r[7] = ok ? s.pdf : 0.0f;
+    r[8] = ok && s.delta ? 1.0f : 0.0f;
+  }
+}
+// Pieces of the BSDF the reference's own tests check: 0 metal Fresnel at cos
+// (3), 1 metal Fresnel average (3), 2 emission exit tint at wo (3), 3 coat
+// darkening factor (3), 4 dielectric ms params (eta_o, ratio_r, ome_o) for
+// (alpha = a, eta = b).
+void orc_openpbr_probe(const OrcMaterial* m, int which, const float* wo, float a, float b, float* out) {
+  const OpenPbr o = parse_open_pbr(m);
+  const V3 w = v3(wo[0], wo[1], wo[2]);
+  const PbrBsdf p = make_bsdf(o, true, w, luts_for(o));
+  V3 r = vsplat(0.0f);
+  if (which == 0) r = metal_fresnel(p, a);
+  if (which == 1) r = metal_fresnel_avg(p);
+  if (which == 2) r = emission_exit_tint(o, w);
+  if (which == 3) r = p.coat_darkening;
+  if (which == 4) {
+    PbrBsdf q = p;
+    q.luts.diel = &diel_lut();
+    const MsParams s = ms_params(q, a, a, b);
+    r = v3(s.eta_o, s.ratio_r, s.ome_o);
+  }
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
 }
 
 }  // extern "C"

@@ -86,6 +86,15 @@ def lib() -> ctypes.CDLL:
         L.orc_sg.argtypes = [ctypes.c_int, f32, f32]
         L.orc_sg.restype = f32
         L.orc_sg_product.argtypes = [vp, f32, vp, f32, vp]
+        L.orc_lut_sizes.argtypes = [vp]
+        L.orc_lut_albedo.argtypes = [f32, vp]
+        L.orc_lut_cond.argtypes = [vp, vp]
+        L.orc_lut_diel.argtypes = [vp, vp]
+        L.orc_lut_lookup.argtypes = [ctypes.c_int, f32, vp, f32, f32]
+        L.orc_lut_lookup.restype = f32
+        L.orc_openpbr_eval.argtypes = [vp, ctypes.c_int, vp, vp, sz, vp, vp]
+        L.orc_openpbr_sample.argtypes = [vp, ctypes.c_int, vp, vp, sz, vp]
+        L.orc_openpbr_probe.argtypes = [vp, ctypes.c_int, vp, f32, f32, vp]
         L.orc_closest_hit.argtypes = [vp, vp, sz, vp, vp, ctypes.c_int]
         L.orc_closest_hit.restype = ctypes.c_int
         L.orc_unoccluded.argtypes = [vp, vp, sz, vp, vp, vp, vp, ctypes.c_int]
@@ -108,6 +117,76 @@ def env_dir(u, v):
 
 def _p(a):
     return None if a is None else a.ctypes.data
+
+
+# ---- GGX tables and the OpenPBR BSDF (oracle/bsdf.h) ------------------------------------------
+def lut_sizes():
+    out = np.zeros(5, np.uint32)
+    lib().orc_lut_sizes(_p(out))
+    return [int(x) for x in out]
+
+
+def lut_albedo(eta):
+    """Dielectric GGX directional albedo table at eta: (64, 16, 64, 2) =
+    (sqrt cos, phi, roughness, anisotropy slices 7 and 8)."""
+    out = np.zeros(lut_sizes()[0], np.float32)
+    lib().orc_lut_albedo(eta, _p(out))
+    return out.reshape(64, 16, 64, 2)
+
+
+def lut_conductor():
+    """(e (roughness 64, sqrt cos 64), e_avg (64))."""
+    n = lut_sizes()
+    e, ea = np.zeros(n[1], np.float32), np.zeros(n[2], np.float32)
+    lib().orc_lut_cond(_p(e), _p(ea))
+    return e.reshape(64, 64), ea
+
+
+def lut_dielectric():
+    """(e (eta 32, roughness 32, sqrt cos 32), e_avg (eta 32, roughness 32))."""
+    n = lut_sizes()
+    e, ea = np.zeros(n[3], np.float32), np.zeros(n[4], np.float32)
+    lib().orc_lut_diel(_p(e), _p(ea))
+    return e.reshape(32, 32, 32), ea.reshape(32, 32)
+
+
+def lut_lookup(which, eta=1.5, w=(0.0, 0.0, 1.0), a=0.0, b=0.0):
+    wa = np.asarray(w, np.float32)
+    return lib().orc_lut_lookup(which, eta, _p(wa), a, b)
+
+
+def _mat_buf(material):
+    rec = material.record()
+    buf = np.frombuffer(bytes(rec), np.uint8).copy()
+    return buf
+
+
+def openpbr_eval(material, wo, wi, front_face=True):
+    """OpenPbrBsdf eval (n, 3) and pdf (n,) in the shading frame."""
+    m = _mat_buf(material)
+    woa = np.asarray(wo, np.float32)
+    wia = np.ascontiguousarray(np.asarray(wi, np.float32).reshape(-1, 3))
+    f, pdf = np.zeros((len(wia), 3), np.float32), np.zeros(len(wia), np.float32)
+    lib().orc_openpbr_eval(_p(m), int(front_face), _p(woa), _p(wia), len(wia), _p(f), _p(pdf))
+    return f, pdf
+
+
+def openpbr_sample(material, wo, u, front_face=True):
+    """OpenPbrBsdf sample for rows (u_lobe, u0, u1) -> dict of ok, weight,
+    wi, pdf, delta."""
+    m = _mat_buf(material)
+    woa = np.asarray(wo, np.float32)
+    ua = np.ascontiguousarray(np.asarray(u, np.float32).reshape(-1, 3))
+    out = np.zeros((len(ua), 9), np.float32)
+    lib().orc_openpbr_sample(_p(m), int(front_face), _p(woa), _p(ua), len(ua), _p(out))
+    return dict(ok=out[:, 0] > 0, weight=out[:, 1:4], wi=out[:, 4:7], pdf=out[:, 7], delta=out[:, 8] > 0)
+
+
+def openpbr_probe(material, which, wo=(0.0, 0.0, 1.0), a=0.0, b=0.0):
+    m = _mat_buf(material)
+    woa, out = np.asarray(wo, np.float32), np.zeros(3, np.float32)
+    lib().orc_openpbr_probe(_p(m), which, _p(woa), a, b, _p(out))
+    return out
 
 
 HIT_DTYPE = np.dtype([("instance", "<u4"), ("triangle", "<u4"), ("t", "<f4"), ("b", "<f4", 3)])

@@ -1,11 +1,13 @@
 """GPU path tracer vs the oracle's restatement of the reference renderer
-(R/main.rs:98-126 + R/integrator/mis.rs on OpenPBR).
+(R/main.rs:98-126 + R/integrator/mis.rs on OpenPBR with the reference's
+default parameters: specular weight 1, roughness 0.3, IOR 1.5).
 
 The two run the same estimator on the same sample sequence: the sampler,
-camera, traversal, shading frame and BSDF arithmetic agree bit for bit, and
-only the transcendental functions of direction sampling (cos / sin of the
-device maths library vs the host libm) may differ in the last place, after
-which a path can take a different branch.  So most pixels are bit-identical
+camera, traversal, shading frame and most BSDF arithmetic agree bit for bit,
+and the transcendental functions (cos / sin of direction sampling, atan2 of
+the albedo-table lookup, pow, exp of the device maths library vs the host
+libm) may differ in the last place, after which a path can take a different
+branch.  So most pixels are bit-identical
 and the images agree to the Monte Carlo noise level; and against a
 high-sample-count reference the GPU image's relMSE (mean over pixels and
 channels of (x - r)^2 / (r^2 + 1e-2)) is no worse than the CPU image's.
@@ -75,11 +77,11 @@ def test_render_is_deterministic_and_running_mean(cornell):
 
 def test_unsupported_material_is_rejected(device):
     cs = scenes.cornell(build=False)
-    cs.scene.materials[0] = OpenPbr.new((0.8, 0.8, 0.8))  # specular weight 1: needs the albedo tables
+    cs.scene.materials[0] = OpenPbr.new((0.8, 0.8, 0.8)).with_(transmission_weight=0.5)  # no BTDF lobes
     cs.scene.build_qbvh()
     ds = cs.scene.upload(device)
     cam = camera.pinhole(**cs.camera, width=8, height=8)
-    with pytest.raises(tpt._lib.TptError, match="specular_weight"):
+    with pytest.raises(tpt._lib.TptError, match="transmission_weight"):
         ds.render(cam, 8, 8, 1, 8)
     ds.close()
 

@@ -124,7 +124,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     F3 wi;
     float ud[2];
     ps.s2(base + 7, ud);
-    if (!mat_sample(m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi)) break;
+    if (!mat_sample(sv.tab, m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi)) break;
     thr = hadamard(thr, s.weight);
     if (depth + 1 >= 4) {
       const float q = clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.05f, 1.0f);
@@ -238,8 +238,8 @@ using namespace tpt;
 
 namespace {
 
-// The materials the path tracer implements exactly: OpenPBR's specular-weight-0
-// subset, NormalizedLambert, Emissive and Mirror.
+// The materials the path tracer implements: OpenPBR without transmission,
+// thin film or thin walls, NormalizedLambert, Emissive and Mirror.
 bool supported(const TptMaterial& mat, std::string* why) {
   if (mat.kind == TPT_MATERIAL_NORMALIZED_LAMBERT || mat.kind == TPT_MATERIAL_EMISSIVE ||
       mat.kind == TPT_MATERIAL_MIRROR)
@@ -249,15 +249,77 @@ bool supported(const TptMaterial& mat, std::string* why) {
     return false;
   }
   const TptOpenPbr& m = mat.u.open_pbr;
-  if (m.specular_weight != 0.0f) *why = "specular_weight must be 0";
-  else if (m.base_metalness != 0.0f) *why = "base_metalness must be 0";
-  else if (m.transmission_weight != 0.0f) *why = "transmission_weight must be 0";
-  else if (m.coat_weight != 0.0f) *why = "coat_weight must be 0";
-  else if (m.fuzz_weight != 0.0f) *why = "fuzz_weight must be 0";
+  if (m.transmission_weight != 0.0f) *why = "transmission_weight must be 0";
   else if (m.thin_film_weight != 0.0f) *why = "thin_film_weight must be 0";
   else if (m.thin_walled) *why = "thin_walled must be false";
   else return true;
   return false;
+}
+
+// The inputs of a material's GGX tables (see upload_ggx_tables).
+bool same_ggx_inputs(const TptMaterial& a, const TptMaterial& b) {
+  if (a.kind != TPT_MATERIAL_OPEN_PBR) return true;
+  const TptOpenPbr &x = a.u.open_pbr, &y = b.u.open_pbr;
+  return x.specular_ior == y.specular_ior && x.specular_weight == y.specular_weight &&
+         x.coat_weight == y.coat_weight && x.coat_ior == y.coat_ior && x.base_metalness == y.base_metalness;
+}
+
+// Material BSDF queries in the shading frame (tpt_material_bsdf).
+__global__ __launch_bounds__(256) void bsdf_query_kernel(pt::Tables tb, const TptMaterial* __restrict__ mats,
+                                                         uint32_t material_count, const TptBsdfQuery* __restrict__ q,
+                                                         size_t n, TptBsdfResult* __restrict__ out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const TptBsdfQuery a = q[i];
+    TptBsdfResult r;
+    for (int k = 0; k < 3; ++k) r.f[k] = r.weight[k] = r.wi[k] = 0.0f;
+    r.pdf = r.sample_pdf = 0.0f;
+    r.flags = 0;
+    if (a.material < material_count) {
+      const TptMaterial& m = mats[a.material];
+      const F3 wo = f3(a.wo[0], a.wo[1], a.wo[2]), wi = f3(a.wi[0], a.wi[1], a.wi[2]);
+      pt::Sample smp{};
+      bool ok = false;
+      F3 f = f3(0.0f, 0.0f, 0.0f);
+      if (m.kind == TPT_MATERIAL_OPEN_PBR) {
+        const pt::Bsdf b = pt::make_bsdf(tb, a.material, m.u.open_pbr, a.front_face != 0, wo);
+        f = pt::bsdf_eval(b, wo, wi);
+        r.pdf = pt::bsdf_pdf(b, wo, wi);
+        ok = pt::bsdf_sample(b, wo, a.u[0], a.u[1], a.u[2], &smp);
+      } else if (m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {  // R/bsdf/normalized_lambert.rs:35-90
+        if (wo.z > 0.0f && wi.z > 0.0f) {
+          f = pt::divs(pt::rgb3(m.u.lambert.rho), pt::kPi);
+          r.pdf = fmaxf(wi.z, 0.0f) / pt::kPi;
+        }
+        if (wo.z > 0.0f) {
+          smp.wi = pt::cosine_hemisphere(a.u[1], a.u[2]);
+          smp.pdf = fmaxf(smp.wi.z, 0.0f) / pt::kPi;
+          smp.weight = pt::rgb3(m.u.lambert.rho);
+          smp.delta = false;
+          ok = smp.pdf > 0.0f;
+        }
+      } else if (m.kind == TPT_MATERIAL_MIRROR && wo.z > 0.0f) {  // R/bsdf/mirror.rs:23-39
+        smp.wi = pt::normalize_or_zero(f3(-wo.x, -wo.y, wo.z));
+        smp.pdf = 1.0f;
+        smp.weight = pt::rgb3(m.u.mirror.color);
+        smp.delta = true;
+        ok = true;
+      }
+      r.f[0] = f.x;
+      r.f[1] = f.y;
+      r.f[2] = f.z;
+      if (ok) {
+        r.weight[0] = smp.weight.x;
+        r.weight[1] = smp.weight.y;
+        r.weight[2] = smp.weight.z;
+        r.wi[0] = smp.wi.x;
+        r.wi[1] = smp.wi.y;
+        r.wi[2] = smp.wi.z;
+        r.sample_pdf = smp.pdf;
+        r.flags = 1u | (smp.delta ? 2u : 0u);
+      }
+    }
+    out[i] = r;
+  }
 }
 
 }  // namespace
@@ -482,10 +544,53 @@ extern "C" int32_t tpt_scene_set_materials(TptScene* s, const TptMaterial* mater
                          "hipMemcpyAsync(materials)"))
       return TPT_ERROR_DEVICE;
     if (!hip_ok(hipStreamSynchronize(s->dev->stream), "hipStreamSynchronize")) return TPT_ERROR_DEVICE;
+    bool tables = false;
+    for (uint32_t i = 0; i < count; ++i) tables |= !same_ggx_inputs(materials[i], s->host_mats[i]);
     s->host_mats.assign(materials, materials + count);
+    if (tables && !upload_ggx_tables(s, s->host_mats)) return TPT_ERROR_DEVICE;
     return TPT_OK;
   } catch (const std::exception& e) {
     set_error(std::string("tpt_scene_set_materials: ") + e.what());
+    return TPT_ERROR;
+  }
+}
+
+extern "C" int32_t tpt_material_bsdf(TptScene* s, const TptBsdfQuery* queries, size_t n, TptBsdfResult* out,
+                                     uint32_t flags) {
+  try {
+    if (!s || (n && (!queries || !out))) {
+      set_error("tpt_material_bsdf: null argument");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    if (n == 0) return TPT_OK;
+    if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    const hipStream_t st = s->dev->stream;
+    const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
+    DeviceBuffer qb, rb;
+    const TptBsdfQuery* dq = queries;
+    TptBsdfResult* dr = out;
+    if (!dev_ptr) {
+      if (!qb.ensure(n * sizeof(TptBsdfQuery)) || !rb.ensure(n * sizeof(TptBsdfResult))) return TPT_ERROR_DEVICE;
+      if (!hip_ok(hipMemcpyAsync(qb.ptr, queries, n * sizeof(TptBsdfQuery), hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync(queries)"))
+        return TPT_ERROR_DEVICE;
+      dq = (const TptBsdfQuery*)qb.ptr;
+      dr = (TptBsdfResult*)rb.ptr;
+    }
+    const pt::ShadeView sv = pt::shade_view_of(s);
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(bsdf_query_kernel, dim3(blocks), dim3(256), 0, st, sv.tab, (const TptMaterial*)s->mats,
+                       s->material_count, dq, n, dr);
+    if (!hip_ok(hipGetLastError(), "bsdf_query_kernel launch")) return TPT_ERROR_DEVICE;
+    if (!dev_ptr) {
+      if (!hip_ok(hipMemcpyAsync(out, dr, n * sizeof(TptBsdfResult), hipMemcpyDeviceToHost, st),
+                  "hipMemcpyAsync(results)") ||
+          !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+        return TPT_ERROR_DEVICE;
+    }
+    return TPT_OK;
+  } catch (const std::exception& e) {
+    set_error(std::string("tpt_material_bsdf: ") + e.what());
     return TPT_ERROR;
   }
 }

@@ -241,11 +241,166 @@ __device__ __forceinline__ F3 sample_eon(F3 wo, float r, float u1, float u2, flo
   return wi;
 }
 
-// ---- OpenPBR subset ---------------------------------------------------------------------
+// ---- OpenPBR (R/material/open_pbr.rs, R/bsdf/open_pbr.rs) ----------------------------------
+// Carried: base (EON diffuse), F82 metal, dielectric specular and coat (GGX
+// with multiple-scattering compensation), fuzz (LTC); transmission, thin
+// film, thin walls, subsurface, textures and normal maps are rejected on the
+// host.  The GGX tables (ggx_tables.h) are read through Tables.
+struct Tables {
+  const float* albedo;     // directional-albedo tables, ggx::kAlbedoLen each; table 0 is all zero
+  const uint32_t* mat;     // per material: spec table, coat table
+  const float* cond;       // conductor e then e_avg
+  const float* diel;       // dielectric e then e_avg
+};
+constexpr uint32_t kAlbCos = 64, kAlbPhi = 16, kAlbRough = 64, kAlbAniso = 2, kAlbLen = 64 * 16 * 64 * 2;
+constexpr uint32_t kCondCos = 64, kCondRough = 64, kDielCos = 32, kDielRough = 32, kDielEta = 32;
+
+struct AxisLerp {
+  uint32_t lo, hi;
+  float t;
+};
+__device__ __forceinline__ AxisLerp axis_lerp(float value, uint32_t res) {  // compute_lut_axis_lerp
+  const float pos = clampf(value, 0.0f, 1.0f) * (float)res - 0.5f;
+  if (pos <= 0.0f) return AxisLerp{0, 0, 0.0f};
+  const uint32_t mx = res - 1;
+  if (pos >= (float)mx) return AxisLerp{mx, mx, 0.0f};
+  const uint32_t lo = (uint32_t)floorf(pos);
+  return AxisLerp{lo, lo + 1, pos - (float)lo};
+}
+// DielectricGgxDirectionalAlbedoLut::lookup at anisotropy 0
+// (R/bsdf/directional_albedo.rs:170-226): slices 7, 8 stored as 0, 1.
+__device__ __forceinline__ float albedo_lookup(const float* t, float eta, F3 w, float roughness) {
+  if (w.z <= 0.0f) return 0.0f;
+  roughness = clampf(roughness, 0.0f, 1.0f);
+  const float alpha = roughness * roughness;  // aspect 1 at anisotropy 0
+  if (fmaxf(clampf(alpha, kMinAlpha, 1.0f), clampf(alpha, kMinAlpha, 1.0f)) < kSmoothAlpha)
+    return fresnel_dielectric(clampf(w.z, 0.0f, 1.0f), 1.0f, eta);
+  const float sc = sqrtf(clampf(w.z, 0.0f, 1.0f));
+  const float phi = (w.x == 0.0f && w.y == 0.0f) ? 0.0f : atan2f(fabsf(w.y), fabsf(w.x));
+  const AxisLerp a = axis_lerp(sc, kAlbCos), b = axis_lerp(phi / (kPi / 2.0f), kAlbPhi),
+                 c = axis_lerp(roughness, kAlbRough);
+  const uint32_t ia[2] = {a.lo, a.hi}, ib[2] = {b.lo, b.hi}, ic[2] = {c.lo, c.hi};
+  const float wa[2] = {1.0f - a.t, a.t}, wb[2] = {1.0f - b.t, b.t}, wc[2] = {1.0f - c.t, c.t};
+  const float wd[2] = {1.0f - 0.5f, 0.5f};  // axis_lerp(0.5, 16): cells 7 and 8, blend 1/2
+  float value = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const float* cell = t + ((ia[i] * kAlbPhi + ib[j]) * kAlbRough + ic[k]) * kAlbAniso;
+#pragma unroll
+        for (int l = 0; l < 2; ++l) value += wa[i] * wb[j] * wc[k] * wd[l] * cell[l];
+      }
+  return clampf(value, 0.0f, 1.0f);
+}
+__device__ __forceinline__ float cond_e(const float* t, float c, float r) {  // :738-754
+  c = clampf(c, 0.0f, 1.0f);
+  r = clampf(r, 0.0f, 1.0f);
+  const AxisLerp a = axis_lerp(sqrtf(c), kCondCos), b = axis_lerp(r, kCondRough);
+  const uint32_t ia[2] = {a.lo, a.hi}, ib[2] = {b.lo, b.hi};
+  const float wa[2] = {1.0f - a.t, a.t}, wb[2] = {1.0f - b.t, b.t};
+  float value = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) value += wa[i] * wb[j] * t[ib[j] * kCondCos + ia[i]];
+  return clampf(value, 0.0f, 1.0f);
+}
+__device__ __forceinline__ float cond_eavg(const float* t, float r) {  // :756-762
+  const float* ea = t + kCondCos * kCondRough;
+  const AxisLerp a = axis_lerp(clampf(r, 0.0f, 1.0f), kCondRough);
+  return clampf((1.0f - a.t) * ea[a.lo] + a.t * ea[a.hi], 0.0f, 1.0f);
+}
+__device__ __forceinline__ float diel_eta_axis(float eta) {  // :845-853
+  eta = clampf(eta, 1.0f / 3.0f, 3.0f);
+  if (eta >= 1.0f) return 0.5f + 0.5f * ((eta - 1.0f) / (3.0f - 1.0f));
+  const float inv = 1.0f / eta;
+  return 0.5f - 0.5f * ((inv - 1.0f) / (3.0f - 1.0f));
+}
+__device__ __forceinline__ float diel_e(const float* t, float c, float r, float eta) {  // :919-944
+  c = clampf(c, 0.0f, 1.0f);
+  r = clampf(r, 0.0f, 1.0f);
+  const AxisLerp a = axis_lerp(sqrtf(c), kDielCos), b = axis_lerp(r, kDielRough),
+                 e = axis_lerp(diel_eta_axis(eta), kDielEta);
+  const uint32_t ie[2] = {e.lo, e.hi}, ib[2] = {b.lo, b.hi}, ia[2] = {a.lo, a.hi};
+  const float we[2] = {1.0f - e.t, e.t}, wb[2] = {1.0f - b.t, b.t}, wa[2] = {1.0f - a.t, a.t};
+  float value = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) value += we[i] * wb[j] * wa[k] * t[(ie[i] * kDielRough + ib[j]) * kDielCos + ia[k]];
+  return clampf(value, 0.0f, 1.0f);
+}
+__device__ __forceinline__ float diel_eavg(const float* t, float r, float eta) {  // :946-961
+  const float* ea = t + kDielCos * kDielRough * kDielEta;
+  r = clampf(r, 0.0f, 1.0f);
+  const AxisLerp b = axis_lerp(r, kDielRough), e = axis_lerp(diel_eta_axis(eta), kDielEta);
+  const uint32_t ie[2] = {e.lo, e.hi}, ib[2] = {b.lo, b.hi};
+  const float we[2] = {1.0f - e.t, e.t}, wb[2] = {1.0f - b.t, b.t};
+  float value = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) value += we[i] * wb[j] * ea[ie[i] * kDielRough + ib[j]];
+  return clampf(value, 0.0f, 1.0f);
+}
+
+// f32::powi of a run-time base (LLVM's binary exponentiation)
+__device__ __forceinline__ float pow2i(float x) { return x * x; }
+__device__ __forceinline__ float pow4i(float x) {
+  const float x2 = x * x;
+  return x2 * x2;
+}
+__device__ __forceinline__ float pow5i(float x) { return x * pow4i(x); }
+__device__ __forceinline__ float pow6i(float x) {
+  const float x2 = x * x;
+  return x2 * (x2 * x2);
+}
+__device__ __forceinline__ F3 f3s(float s) { return f3(s, s, s); }
+__device__ __forceinline__ F3 clamp01(F3 v) { return f3(clampf(v.x, 0.f, 1.f), clampf(v.y, 0.f, 1.f), clampf(v.z, 0.f, 1.f)); }
+__device__ __forceinline__ F3 vmaxs(F3 v, float s) { return f3(fmaxf(v.x, s), fmaxf(v.y, s), fmaxf(v.z, s)); }
+__device__ __forceinline__ F3 lerp3(F3 a, F3 b, float t) { return add(a, muls(sub(b, a), t)); }
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ F3 pow3(F3 v, float e) {  // vec3_powf
+  return f3(powf(fmaxf(v.x, 0.0f), e), powf(fmaxf(v.y, 0.0f), e), powf(fmaxf(v.z, 0.0f), e));
+}
+__device__ __forceinline__ float f_avg_dielectric(float eta) {  // R/bsdf/open_pbr.rs:2127-2135
+  if (eta >= 1.0f) return clampf((eta - 1.0f) / (4.08567f + 1.00071f * eta), 0.0f, 1.0f);
+  const float e = fmaxf(eta, 1.0e-6f);
+  return clampf(0.997118f + 0.1014f * e - 0.965241f * e * e - 0.130607f * e * e * e, 0.0f, 1.0f);
+}
+__device__ __forceinline__ float zeltner_albedo(float ndv, float rough) {  // :2070-2078
+  const float x = clampf(ndv, 0.0f, 1.0f), y = clampf(rough, 0.01f, 1.0f);
+  const float s = y * (0.0206607f + 1.58491f * y) / (0.0379424f + y * (1.32227f + y));
+  const float m = y * (-0.193854f + y * (-1.14885f + y * (1.7932f - 0.95943f * y * y))) / (0.046391f + y);
+  const float o = y * (0.000654023f + (-0.0207818f + 0.119681f * y) * y) / (1.26264f + y * (-1.92021f + y));
+  const float q = (x - m) / s;
+  return expf(-0.5f * (q * q)) / (s * sqrtf(2.0f * kPi)) + o;
+}
+__device__ __forceinline__ float zeltner_a(float ndv, float rough) {
+  const float x = clampf(ndv, 0.0f, 1.0f), y = clampf(rough, 0.01f, 1.0f);
+  return (2.58126f * x + 0.813703f * y) * y / (1.0f + 0.310327f * x * x + 2.60994f * x * y);
+}
+__device__ __forceinline__ float zeltner_b(float ndv, float rough) {
+  const float x = clampf(ndv, 0.0f, 1.0f), y = clampf(rough, 0.01f, 1.0f);
+  return sqrtf(1.0f - x) * (y - 1.0f) * y * y * y /
+         (0.0000254053f + 1.71228f * x - 1.71506f * x * y + 1.34174f * y * y);
+}
+
 struct Bsdf {
-  F3 base_color, spec_color;
-  float base, ax, ay, spec_eta, trans_eta, diff_rough;
+  F3 base_color, spec_color, coat_color, coat_dark, fuzz_color;
+  float base, specular, ax, ay, spec_eta, trans_eta, metal;
+  float coat, cax, cay, coat_eta;
+  float fuzz, fuzz_rough, diff_rough;
   bool front;
+  const float* spec_tab;  // directional-albedo table of spec_eta
+  const float* coat_tab;  // of coat_eta
+  const float* cond;
+  const float* diel;
 };
 
 __device__ __forceinline__ float modulated_eta(float eta, float w) {  // R/material/open_pbr.rs:871-878
@@ -256,94 +411,358 @@ __device__ __forceinline__ float modulated_eta(float eta, float w) {  // R/mater
   const float eps = sqrtf(clampf(w * f0, 0.0f, 0.99999f)) * sg;
   return fmaxf((1.0f + eps) / fmaxf(1.0f - eps, 1.0e-6f), 1.0e-4f);
 }
-
-__device__ __forceinline__ Bsdf make_bsdf(const TptOpenPbr& m, bool front) {  // :393-487
+__device__ __forceinline__ void alpha_xy(float rough, float aniso, float* ax, float* ay) {  // :861-869
+  const float r = clampf(rough, 0.0f, 1.0f), a = clampf(aniso, 0.0f, 1.0f);
+  const float alpha = r * r, oma = 1.0f - a;
+  const float x = alpha * sqrtf(2.0f / (1.0f + oma * oma));
+  *ax = clampf(x, kMinAlpha, 1.0f);
+  *ay = clampf(oma * x, kMinAlpha, 1.0f);
+}
+__device__ __forceinline__ float fuzz_roughening(const TptOpenPbr& m, float r, float fw, float fr) {  // :767-778
+  if (fw <= 0.0f) return r;
+  const F3 fc = clamp01(f3(m.fuzz_color[0], m.fuzz_color[1], m.fuzz_color[2]));
+  const float rf = clampf(luminance(muls(fc, zeltner_albedo(1.0f, fr))), 0.0f, 1.0f);
+  const float rough = powf(fminf(pow4i(r) + 2.0f * pow4i(rf), 1.0f), 0.25f);
+  return lerpf(r, rough, fw);
+}
+__device__ __forceinline__ float dielectric_fresnel_avg_ln(float eta) {  // :880-886
+  eta = clampf(eta, 1.0f, 3.0f);
+  return clampf(logf(fmaxf((10893.0f * eta - 1438.2f) / (-774.4f * eta * eta + 10212.0f * eta + 1.0f), 1.0e-6f)),
+                0.0f, 1.0f);
+}
+// coat_darkening_factor (:900-941)
+__device__ __forceinline__ F3 coat_darkening(const TptOpenPbr& m, F3 base_color, F3 wol) {
+  const float cw = clampf(m.coat_weight, 0.0f, 1.0f), delta = clampf(m.coat_darkening, 0.0f, 1.0f);
+  if (cw <= 0.0f || delta <= 0.0f) return f3s(1.0f);
+  const float ec = fmaxf(m.coat_ior, 1.0f);
+  const float kr = 1.0f - (1.0f - dielectric_fresnel_avg_ln(ec)) / (ec * ec);
+  const float ks = fresnel_dielectric(clampf(fabsf(wol.z), 0.0f, 1.0f), 1.0f, ec);
+  const float sior = fmaxf(m.specular_ior, 1.0e-4f);
+  const float eta_s = lerpf(sior, sior / ec, cw);
+  const float f0 = pow2i((eta_s - 1.0f) / (eta_s + 1.0f));
+  const float fs = clampf(fmaxf(m.specular_weight, 0.0f) * f0, 0.0f, 1.0f);
+  const float r_spec = clampf(m.specular_roughness * 1.0f, 0.0f, 1.0f);
+  const float metal = clampf(m.base_metalness * 1.0f, 0.0f, 1.0f);
+  const float rb = lerpf(lerpf(1.0f, r_spec, fs), r_spec, metal);
+  const float k = clampf(lerpf(ks, kr, rb), 0.0f, 1.0f);
+  const F3 bw = muls(base_color, clampf(m.base_weight, 0.0f, 1.0f));
+  const F3 e_dielec = lerp3(lerp3(bw, clamp01(f3s(0.8f)), 0.0f), f3s(1.0f - f0), clampf(m.transmission_weight, 0.0f, 1.0f));
+  const F3 e_metal = clamp01(muls(bw, fmaxf(m.specular_weight, 0.0f)));
+  const F3 e_base = clamp01(lerp3(e_dielec, e_metal, metal));
+  const F3 kc = muls(clamp01(f3(m.coat_color[0], m.coat_color[1], m.coat_color[2])), k);
+  const F3 den = vmaxs(sub(f3s(1.0f), hadamard(e_base, kc)), 1.0e-4f);
+  const float omk = fmaxf(1.0f - k, 0.0f);
+  return clamp01(lerp3(f3s(1.0f), f3(omk / den.x, omk / den.y, omk / den.z), cw * delta));
+}
+// OpenPbrMaterial::make_bsdf (:393-487) for material `mi`; wol: the shading-frame wo
+__device__ __forceinline__ Bsdf make_bsdf(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol) {
   Bsdf p;
-  p.base_color = f3(m.base_color[0], m.base_color[1], m.base_color[2]);
-  const float rough = clampf(m.specular_roughness, 0.0f, 1.0f);
-  const float an = clampf(m.specular_roughness_anisotropy, 0.0f, 1.0f);
-  const float alpha = rough * rough, oma = 1.0f - an;
-  const float axr = alpha * sqrtf(2.0f / (1.0f + oma * oma));
-  p.ax = clampf(axr, kMinAlpha, 1.0f);
-  p.ay = clampf(oma * axr, kMinAlpha, 1.0f);
-  p.spec_color = f3(clampf(m.specular_color[0], 0.0f, 1.0f), clampf(m.specular_color[1], 0.0f, 1.0f),
-                    clampf(m.specular_color[2], 0.0f, 1.0f));
+  p.base_color = hadamard(f3(m.base_color[0], m.base_color[1], m.base_color[2]), f3s(1.0f));
+  p.metal = clampf(m.base_metalness * 1.0f, 0.0f, 1.0f);
+  const float fw = clampf(m.fuzz_weight * 1.0f, 0.0f, 1.0f), fr = clampf(m.fuzz_roughness * 1.0f, 0.0f, 1.0f);
+  const float coat_r = fuzz_roughening(m, clampf(m.coat_roughness, 0.0f, 1.0f), fw, fr);
+  const float r_base = clampf(m.specular_roughness * 1.0f, 0.0f, 1.0f);
+  const float coat_rough = powf(fminf(pow4i(r_base) + 2.0f * pow4i(coat_r), 1.0f), 0.25f);
+  const float spec_r = fuzz_roughening(m, lerpf(r_base, coat_rough, clampf(m.coat_weight, 0.0f, 1.0f)), fw, fr);
+  alpha_xy(spec_r, m.specular_roughness_anisotropy, &p.ax, &p.ay);
+  alpha_xy(coat_r, m.coat_roughness_anisotropy, &p.cax, &p.cay);
+  p.base = clampf(m.base_weight, 0.0f, 1.0f);
+  p.specular = fmaxf(m.specular_weight, 0.0f);
+  p.spec_color = clamp01(f3(m.specular_color[0], m.specular_color[1], m.specular_color[2]));
   p.spec_eta = modulated_eta(fmaxf(m.specular_ior, 1.0e-4f), fmaxf(m.specular_weight, 0.0f));
   p.trans_eta = p.spec_eta;
-  p.base = clampf(m.base_weight, 0.0f, 1.0f);
+  p.coat = clampf(m.coat_weight, 0.0f, 1.0f);
+  p.coat_color = clamp01(f3(m.coat_color[0], m.coat_color[1], m.coat_color[2]));
+  p.coat_dark = coat_darkening(m, p.base_color, wol);
+  p.coat_eta = fmaxf(m.coat_ior, 1.0e-4f);
+  p.fuzz = fw;
+  p.fuzz_color = clamp01(f3(m.fuzz_color[0], m.fuzz_color[1], m.fuzz_color[2]));
+  p.fuzz_rough = fr;
   p.diff_rough = clampf(m.base_diffuse_roughness, 0.0f, 1.0f);
   p.front = front;
+  p.spec_tab = tb.albedo + (size_t)tb.mat[2 * mi] * kAlbLen;
+  p.coat_tab = tb.albedo + (size_t)tb.mat[2 * mi + 1] * kAlbLen;
+  p.cond = tb.cond;
+  p.diel = tb.diel;
   return p;
 }
 
-// layer_weights (R/bsdf/open_pbr.rs:333-390) of the subset: fuzz, coat,
-// metal, transmission and subsurface are zero and e_spec = 0.
-__device__ __forceinline__ void layer_weights(const Bsdf& p, F3* spec_w, F3* diff_w) {
-  const F3 below_metal = f3(1.0f, 1.0f, 1.0f);
-  *spec_w = hadamard(below_metal, p.spec_color);
-  const F3 leak = f3(fmaxf(1.0f - p.spec_color.x * 0.0f, 0.0f), fmaxf(1.0f - p.spec_color.y * 0.0f, 0.0f),
-                     fmaxf(1.0f - p.spec_color.z * 0.0f, 0.0f));
+constexpr float kCos82 = 1.0f / 7.0f;
+constexpr float kF82Den = 0x1.d01982p-5f;  // 1/7 * (6/7)^6 as the reference's constant-folded powi gives it
+constexpr float kMsEps = 1.0e-6f;
+__device__ __forceinline__ F3 schlick3(F3 f0, float c) {
+  const float p5 = pow5i(1.0f - clampf(c, 0.0f, 1.0f));
+  return add(f0, muls(sub(f3s(1.0f), f0), p5));
+}
+__device__ __forceinline__ float refracted_cos(float mu, float eta) {  // :2174-2177
+  const float s2 = fmaxf(1.0f - pow2i(clampf(mu, 0.0f, 1.0f)), 0.0f) / fmaxf(eta * eta, kMsEps);
+  return sqrtf(fmaxf(1.0f - s2, 0.0f));
+}
+__device__ __forceinline__ float rough_proxy(float ax, float ay) { return clampf(sqrtf(fmaxf(ax, ay)), 0.0f, 1.0f); }
+__device__ __forceinline__ bool coat_smooth(const Bsdf& p) { return p.coat <= 0.0f || fmaxf(p.cax, p.cay) < kSmoothAlpha; }
+__device__ __forceinline__ bool metal_smooth(const Bsdf& p) { return p.metal <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
+__device__ __forceinline__ bool spec_smooth(const Bsdf& p) { return p.specular <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
+
+struct MsParams {
+  float eta_o, req, e_avg_o, ome_o;
+};
+__device__ __forceinline__ MsParams ms_params(const Bsdf& p, float ax, float ay, float eta) {  // :593-625
+  MsParams s;
+  s.eta_o = p.front ? fmaxf(eta, 1.0e-4f) : 1.0f / fmaxf(eta, 1.0e-4f);
+  s.req = powf(ax * ay, 0.25f);
+  s.e_avg_o = diel_eavg(p.diel, s.req, s.eta_o);
+  s.ome_o = fmaxf(1.0f - s.e_avg_o, kMsEps);
+  return s;  // the transmission ratio is not needed without transmission
+}
+__device__ __forceinline__ float ms_factor(const MsParams& s) {  // :573-577
+  const float fa = clampf(f_avg_dielectric(s.eta_o), 0.0f, 1.0f);
+  const float den = 1.0f - fa * fmaxf(1.0f - s.e_avg_o, 0.0f);
+  return clampf(fa * fa * s.e_avg_o / fmaxf(den, kMsEps), 0.0f, 1.0f);
+}
+struct TwoLobe {
+  float ss, ms, total;
+};
+__device__ __forceinline__ TwoLobe refl_lobes(const MsParams& s) {  // :627-636
+  const float fa = clampf(f_avg_dielectric(s.eta_o), 0.0f, 1.0f);
+  const float ss = clampf(fa * s.e_avg_o, 0.0f, 1.0f);
+  const float ms = fmaxf(1.0f - s.e_avg_o, 0.0f) * ms_factor(s);
+  return TwoLobe{ss, ms, fmaxf(ss + ms, 1.0e-8f)};
+}
+__device__ __forceinline__ TwoLobe cond_lobes(const Bsdf& p) {  // :579-591
+  const float e = clampf(cond_eavg(p.cond, powf(p.ax * p.ay, 0.25f)), 0.0f, 1.0f);
+  const float ms = fmaxf(1.0f - e, 0.0f);
+  return TwoLobe{e, ms, fmaxf(e + ms, 1.0e-8f)};
+}
+__device__ __forceinline__ float albedo_ms(const Bsdf& p, F3 wo, float ax, float ay, float eta, float ss) {  // :464-481
+  if (wo.z <= 0.0f || fmaxf(ax, ay) < kSmoothAlpha) return clampf(ss, 0.0f, 1.0f);
+  const MsParams s = ms_params(p, ax, ay, eta);
+  const float eo = diel_e(p.diel, wo.z, s.req, s.eta_o);
+  return clampf(ss + ms_factor(s) * fmaxf(1.0f - eo, 0.0f), 0.0f, 1.0f);
+}
+__device__ __forceinline__ F3 coat_tint(const Bsdf& p, F3 wo, F3 wi) {  // :483-492
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return f3s(1.0f);
+  const float mo = refracted_cos(wo.z, p.coat_eta), mi = refracted_cos(wi.z, p.coat_eta);
+  return pow3(p.coat_color, 0.5f * (1.0f / fmaxf(mi, 1.0e-4f) + 1.0f / fmaxf(mo, 1.0e-4f)));
+}
+struct LayerW {
+  float coat_amp;
+  F3 metal, spec, fuzz, diff;
+};
+// layer_weights (:333-390) without transmission, subsurface, thin walls
+__device__ __forceinline__ LayerW layer_weights(const Bsdf& p, F3 wo, const F3* wi) {
+  const float e_fuzz = p.fuzz <= 0.0f ? 0.0f : zeltner_albedo(clampf(wo.z, 0.0f, 1.0f), clampf(p.fuzz_rough, 0.01f, 1.0f));
+  const float bfs = fmaxf(1.0f - p.fuzz * e_fuzz, 0.0f);
+  F3 under = f3s(bfs);
+  if (p.coat > 0.0f) {
+    const float e_coat =
+        wo.z <= 0.0f ? 0.0f
+                     : albedo_ms(p, wo, p.cax, p.cay, p.coat_eta, albedo_lookup(p.coat_tab, p.coat_eta, wo, rough_proxy(p.cax, p.cay)));
+    const F3 tint = wi ? coat_tint(p, wo, *wi) : p.coat_color;
+    const F3 thr = hadamard(hadamard(p.coat_dark, tint), f3s(fmaxf(1.0f - e_coat, 0.0f)));
+    under = hadamard(f3s(bfs), add(muls(f3s(1.0f), 1.0f - p.coat), muls(thr, p.coat)));
+  }
+  LayerW w;
+  w.metal = muls(under, p.metal);
+  const F3 below_metal = muls(under, 1.0f - p.metal);
+  const float e_spec =
+      wo.z <= 0.0f ? 0.0f
+                   : albedo_ms(p, wo, p.ax, p.ay, p.spec_eta, albedo_lookup(p.spec_tab, p.spec_eta, wo, rough_proxy(p.ax, p.ay)));
+  w.spec = hadamard(below_metal, p.spec_color);
+  const F3 leak = vmaxs(sub(f3s(1.0f), muls(p.spec_color, e_spec)), 0.0f);
   const F3 below_trans = muls(hadamard(below_metal, leak), 1.0f - 0.0f);
-  *diff_w = hadamard(muls(below_trans, (1.0f - 0.0f) * p.base), p.base_color);
+  w.diff = hadamard(muls(below_trans, (1.0f - 0.0f) * p.base), p.base_color);
+  w.coat_amp = p.coat * bfs;
+  w.fuzz = muls(p.fuzz_color, p.fuzz);
+  return w;
 }
 struct Probs {
-  float spec, diff, total;
+  float coat, metal, spec, fuzz, diff, total;
 };
-__device__ __forceinline__ Probs lobe_probs(const Bsdf& p) {  // :392-422
-  F3 sw, dw;
-  layer_weights(p, &sw, &dw);
-  const float zero_lum = luminance(f3(0.0f, 0.0f, 0.0f));
-  const float spec = luminance(sw);
-  const float diff = fmaxf(luminance(dw), 0.0f);
-  const float total = 0.0f + zero_lum + spec + 0.0f + fmaxf(zero_lum, 0.0f) + diff + fmaxf(zero_lum, 0.0f);
-  return Probs{spec, diff, total};
+__device__ __forceinline__ Probs lobe_probs(const Bsdf& p, F3 wo) {  // :392-422
+  const LayerW w = layer_weights(p, wo, nullptr);
+  Probs r;
+  r.coat = p.coat > 0.0f ? w.coat_amp : 0.0f;
+  r.metal = luminance(hadamard(w.metal, add(f3s(0.5f), muls(p.base_color, 0.5f))));
+  r.spec = luminance(w.spec);
+  r.fuzz = fmaxf(luminance(w.fuzz), 0.0f);
+  r.diff = fmaxf(luminance(w.diff), 0.0f);
+  const float zero = fmaxf(luminance(f3s(0.0f)), 0.0f);
+  r.total = r.coat + r.metal + r.spec + 0.0f + r.fuzz + r.diff + zero;
+  return r;
 }
-__device__ __forceinline__ bool spec_smooth(const Bsdf& p) { return fmaxf(p.ax, p.ay) < kSmoothAlpha; }
-__device__ __forceinline__ float spec_fresnel(const Bsdf& p, float c) {
-  return p.front ? weighted_fresnel_ratio(c, p.trans_eta, p.spec_eta)
-                 : weighted_fresnel_ratio(c, 1.0f / p.trans_eta, 1.0f / p.spec_eta);
+__device__ __forceinline__ float refl_ms_eval(const Bsdf& p, F3 wo, F3 wi, float ax, float ay, float eta) {  // :648-670
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  const MsParams s = ms_params(p, ax, ay, eta);
+  const float eo = diel_e(p.diel, wo.z, s.req, s.eta_o), ei = diel_e(p.diel, wi.z, s.req, s.eta_o);
+  return ms_factor(s) * (1.0f - eo) * (1.0f - ei) / (kPi * s.ome_o);
+}
+__device__ __forceinline__ float coat_eval(const Bsdf& p, F3 wo, F3 wi) {  // :698-722
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  F3 wm;
+  if (!half_vector(wo, wi, &wm)) return 0.0f;
+  const float d = ggx_d(wm, p.cax, p.cay), g = ggx_g2(wo, wi, p.cax, p.cay);
+  if (d <= 0.0f || g <= 0.0f) return 0.0f;
+  const float ss = d * g * fresnel_dielectric(fabsf(dot(wo, wm)), 1.0f, p.coat_eta) / (4.0f * wo.z * wi.z);
+  return ss + refl_ms_eval(p, wo, wi, p.cax, p.cay, p.coat_eta);
+}
+__device__ __forceinline__ float cos_pdf(float c) { return fmaxf(c, 0.0f) / kPi; }
+__device__ __forceinline__ float two_lobe_pdf(TwoLobe w, F3 wo, F3 wi, float ax, float ay, bool early_ms) {
+  const float pms = cos_pdf(wi.z);
+  F3 wm;
+  if (!half_vector(wo, wi, &wm)) return early_ms ? w.ms * pms / w.total : (w.ss * 0.0f + w.ms * pms) / w.total;
+  const float pw = pdf_bounded_vndf(wo, wm, ax, ay);
+  const float den = 4.0f * fabsf(dot(wo, wm));
+  if (den <= 0.0f) return early_ms ? w.ms * pms / w.total : (w.ss * 0.0f + w.ms * pms) / w.total;
+  return (w.ss * (pw / den) + w.ms * pms) / w.total;
+}
+__device__ __forceinline__ F3 metal_fresnel(const Bsdf& p, float c) {  // :792-802
+  const F3 f0 = vmaxs(muls(p.base_color, p.base), 0.0f);
+  const F3 f82 = schlick3(f0, kCos82);
+  const F3 ft = hadamard(p.spec_color, f82);
+  const float cc = clampf(c, 0.0f, 1.0f);
+  const float corr = cc * pow6i(1.0f - cc) / fmaxf(kF82Den, 1.0e-8f);
+  return clamp01(muls(sub(schlick3(f0, c), muls(sub(f82, ft), corr)), p.specular));
+}
+__device__ __forceinline__ F3 metal_fresnel_avg(const Bsdf& p) {  // :832-839
+  const F3 f0 = vmaxs(muls(p.base_color, p.base), 0.0f);
+  const F3 f82 = schlick3(f0, kCos82);
+  const F3 b = divs(sub(f82, hadamard(p.spec_color, f82)), fmaxf(kF82Den, 1.0e-8f));
+  return clamp01(muls(sub(add(f0, divs(sub(f3s(1.0f), f0), 21.0f)), divs(b, 126.0f)), p.specular));
+}
+__device__ __forceinline__ F3 metal_eval(const Bsdf& p, F3 wo, F3 wi) {  // :744-759, 804-816, 2137-2146
+  const F3 zero = f3s(0.0f);
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return zero;
+  F3 wm;
+  if (!half_vector(wo, wi, &wm)) return zero;
+  const float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
+  if (d <= 0.0f || g <= 0.0f) return zero;
+  const F3 ss = muls(metal_fresnel(p, fabsf(dot(wo, wm))), d * g / (4.0f * wo.z * wi.z));
+  const float req = powf(p.ax * p.ay, 0.25f);
+  const float eo = cond_e(p.cond, wo.z, req), ei = cond_e(p.cond, wi.z, req), ea = cond_eavg(p.cond, req);
+  const float omea = fmaxf(1.0f - ea, kMsEps);
+  const F3 fa = metal_fresnel_avg(p);
+  const F3 den = sub(f3s(1.0f), muls(fa, fmaxf(1.0f - ea, 0.0f)));
+  const F3 ds = vmaxs(den, kMsEps);
+  const F3 fms0 = muls(hadamard(fa, fa), ea);
+  const F3 fms = f3(fms0.x / ds.x, fms0.y / ds.y, fms0.z / ds.z);
+  return add(ss, muls(fms, (1.0f - eo) * (1.0f - ei) / (kPi * omea)));
+}
+__device__ __forceinline__ float base_fresnel(const Bsdf& p, float c) {  // base_dielectric_fresnel :902-929
+  const float ep = p.trans_eta, epr = p.spec_eta;
+  const float un = p.front ? weighted_fresnel_ratio(c, ep, epr) : weighted_fresnel_ratio(c, 1.0f / ep, 1.0f / epr);
+  const float coat = clampf(p.coat, 0.0f, 1.0f);
+  if (coat <= 0.0f) return un;
+  const float ce = fmaxf(p.coat_eta, 1.0e-4f);
+  const float eb = ep / ce, ebp = epr / ce;
+  const bool flip = p.coat_eta > ep;
+  const float a = flip ? 1.0f / fmaxf(eb, 1.0e-4f) : eb, b = flip ? 1.0f / fmaxf(ebp, 1.0e-4f) : ebp;
+  const float co = p.front ? weighted_fresnel_ratio(c, a, b)
+                           : weighted_fresnel_ratio(c, 1.0f / fmaxf(a, 1.0e-4f), 1.0f / fmaxf(b, 1.0e-4f));
+  return lerpf(un, co, coat);
 }
 __device__ __forceinline__ float spec_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_spec_brdf :841-867
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
   F3 wm;
   if (!half_vector(wo, wi, &wm)) return 0.0f;
-  const float d = ggx_d(wm, p.ax, p.ay);
-  const float g = ggx_g2(wo, wi, p.ax, p.ay);
+  const float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
   if (d <= 0.0f || g <= 0.0f) return 0.0f;
-  const float ss = spec_fresnel(p, fabsf(dot(wo, wm))) * (d * g / (4.0f * wo.z * wi.z));
-  return ss + 0.0f * 0.0f;  // multiple-scattering factor: f_avg(1) = 0
+  const float ss = base_fresnel(p, fabsf(dot(wo, wm))) * (d * g / (4.0f * wo.z * wi.z));
+  return ss + refl_ms_eval(p, wo, wi, p.ax, p.ay, p.spec_eta);
 }
-__device__ __forceinline__ float spec_pdf(const Bsdf& p, F3 wo, F3 wi) {  // pdf_specular_brdf :1006-1022
+__device__ __forceinline__ float spec_ss_pdf(const Bsdf& p, F3 wo, F3 wi) {  // :981-994
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
-  float pdf_ss = 0.0f;
   F3 wm;
-  if (half_vector(wo, wi, &wm)) {
-    const float pw = pdf_bounded_vndf(wo, wm, p.ax, p.ay);
-    const float den = 4.0f * fabsf(dot(wo, wm));
-    pdf_ss = den <= 0.0f ? 0.0f : pw / den;
-  }
-  const float pdf_ms = fmaxf(wi.z, 0.0f) / kPi;
-  const float ss_w = 0.0f, ms_w = 0.0f, total = fmaxf(ss_w + ms_w, 1.0e-8f);
-  return (ss_w * pdf_ss + ms_w * pdf_ms) / total;
+  if (!half_vector(wo, wi, &wm)) return 0.0f;
+  const float den = 4.0f * fabsf(dot(wo, wm));
+  if (den <= 0.0f) return 0.0f;
+  return pdf_bounded_vndf(wo, wm, p.ax, p.ay) / den;
 }
-__device__ __forceinline__ F3 bsdf_eval(const Bsdf& p, F3 wo, F3 wi) {  // :224-259
-  if (!(wo.z > 0.0f)) return f3(0.0f, 0.0f, 0.0f);
-  F3 sw, dw;
-  layer_weights(p, &sw, &dw);
-  F3 total = f3(0.0f, 0.0f, 0.0f);
-  if (len2(sw) > 0.0f && !spec_smooth(p)) total = add(total, muls(sw, spec_eval(p, wo, wi)));
-  if (len2(dw) > 0.0f) {
+__device__ __forceinline__ float phi_of(F3 v) {  // :2060-2063
+  const float ph = atan2f(v.y, v.x);
+  return ph < 0.0f ? ph + 2.0f * kPi : ph;
+}
+__device__ __forceinline__ F3 rotate_z(F3 v, float angle) {  // :2065-2068
+  const float s = sinf(angle), c = cosf(angle);
+  return f3(c * v.x - s * v.y, s * v.x + c * v.y, v.z);
+}
+// OpenPbrFuzzBsdf eval (one channel) and pdf (:159-182), LTC :2097-2110
+__device__ __forceinline__ float fuzz_eval(const Bsdf& p, F3 wo, F3 wi, float* pdf) {
+  *pdf = 0.0f;
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  const float r = clampf(p.fuzz_rough, 0.01f, 1.0f);
+  const F3 ws = rotate_z(wi, -phi_of(wo));
+  const float a = zeltner_a(wo.z, r), b = zeltner_b(wo.z, r);
+  F3 w = f3(a * ws.x + b * ws.z, a * ws.y, ws.z);
+  const float l = sqrtf(dot(w, w));
+  if (l <= 0.0f) return 0.0f;
+  w = divs(w, l);
+  if (w.z <= 0.0f) return 0.0f;
+  const float v = (w.z / kPi) * (a * a / (l * l * l));
+  *pdf = v;
+  return zeltner_albedo(wo.z, r) * v / fmaxf(wi.z, 1.0e-6f);
+}
+__device__ __forceinline__ bool fuzz_sample(const Bsdf& p, F3 wo, float u0, float u1, F3* wi) {  // :184-200, 2112-2123
+  if (wo.z <= 0.0f) return false;
+  const float r = clampf(p.fuzz_rough, 0.01f, 1.0f);
+  const float a = zeltner_a(wo.z, r), b = zeltner_b(wo.z, r);
+  const float phi = 2.0f * kPi * u0;
+  const float ct = sqrtf(u1), st = sqrtf(fmaxf(1.0f - u1, 0.0f));
+  const F3 o = f3(cosf(phi) * st, sinf(phi) * st, ct);
+  const F3 ws = normalize_or_zero(f3(o.x / a - o.z * b / a, o.y / a, o.z));
+  if (ws.z <= 0.0f) return false;
+  const F3 w = rotate_z(ws, phi_of(wo));
+  if (w.z <= 0.0f) return false;
+  *wi = w;
+  return true;
+}
+
+// OpenPbrBsdf::eval (:224-259), radiance mode
+__device__ __noinline__ F3 bsdf_eval(const Bsdf& p, F3 wo, F3 wi) {
+  F3 total = f3s(0.0f);
+  if (!(wo.z > 0.0f)) return total;
+  const LayerW w = layer_weights(p, wo, &wi);
+  if (w.coat_amp > 0.0f && !coat_smooth(p)) total = add(total, muls(f3s(coat_eval(p, wo, wi)), w.coat_amp));
+  if (len2(w.metal) > 0.0f && !metal_smooth(p)) total = add(total, hadamard(w.metal, metal_eval(p, wo, wi)));
+  if (len2(w.spec) > 0.0f && !spec_smooth(p)) total = add(total, hadamard(w.spec, f3s(spec_eval(p, wo, wi))));
+  if (len2(w.fuzz) > 0.0f) {
+    float pd;
+    total = add(total, hadamard(w.fuzz, f3s(fuzz_eval(p, wo, wi, &pd))));
+  }
+  if (len2(w.diff) > 0.0f) {
     const float f = (wo.z <= 0.0f || wi.z <= 0.0f) ? 0.0f : f_eon1(p.diff_rough, wo, wi);
-    total = add(total, hadamard(dw, f3(f, f, f)));
+    total = add(total, hadamard(w.diff, f3s(f)));
   }
   return total;
 }
-__device__ __forceinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {  // :261-293
+// OpenPbrBsdf::pdf (:261-293)
+__device__ __noinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {
   if (!(wo.z > 0.0f)) return 0.0f;
-  const Probs pr = lobe_probs(p);
+  const Probs pr = lobe_probs(p, wo);
   if (pr.total <= 0.0f) return 0.0f;
   float pdf = 0.0f;
-  if (pr.spec > 0.0f && !spec_smooth(p)) pdf += (pr.spec / pr.total) * spec_pdf(p, wo, wi);
+  if (pr.coat > 0.0f && !coat_smooth(p) && wo.z > 0.0f && wi.z > 0.0f)
+    pdf += (pr.coat / pr.total) * two_lobe_pdf(refl_lobes(ms_params(p, p.cax, p.cay, p.coat_eta)), wo, wi, p.cax, p.cay, true);
+  if (pr.metal > 0.0f && !metal_smooth(p)) {
+    float pm = 0.0f;
+    if (wo.z > 0.0f && wi.z > 0.0f) {
+      const TwoLobe w = cond_lobes(p);
+      pm = (w.ss * spec_ss_pdf(p, wo, wi) + w.ms * cos_pdf(wi.z)) / w.total;
+    }
+    pdf += (pr.metal / pr.total) * pm;
+  }
+  if (pr.spec > 0.0f && !spec_smooth(p)) {
+    float ps = 0.0f;
+    if (wo.z > 0.0f && wi.z > 0.0f) {
+      const TwoLobe w = refl_lobes(ms_params(p, p.ax, p.ay, p.spec_eta));
+      ps = (w.ss * spec_ss_pdf(p, wo, wi) + w.ms * cos_pdf(wi.z)) / w.total;
+    }
+    pdf += (pr.spec / pr.total) * ps;
+  }
+  if (pr.fuzz > 0.0f) {
+    float pd;
+    fuzz_eval(p, wo, wi, &pd);
+    pdf += (pr.fuzz / pr.total) * pd;
+  }
   if (pr.diff > 0.0f) {
     const float pd = (wo.z <= 0.0f || wi.z <= 0.0f) ? 0.0f : pdf_eon(wo, wi, p.diff_rough);
     pdf += (pr.diff / pr.total) * pd;
@@ -375,47 +794,118 @@ __device__ __forceinline__ F3 cosine_hemisphere(float u, float v) {  // R/math/m
   const float dx = r * cosf(phi), dy = r * sinf(phi);
   return f3(dx, dy, sqrtf(fmaxf(1.0f - (dx * dx + dy * dy), 0.0f)));
 }
-// OpenPbrBsdf::sample (:295-331), pick_lobe (:2016-2041), sample_spec_brdf
-// (:1589-1668), sample_diff_brdf (:1936-1940)
-__device__ __forceinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, float ux, float uy, Sample* out) {
+// sample_wm_bounded_vndf (R/bsdf/smith_ggx.rs:121-150)
+__device__ __forceinline__ bool sample_bounded_vndf(F3 wo, float ax, float ay, float u0, float u1, F3* out) {
+  if (wo.z <= 0.0f) return false;
+  ax = fmaxf(ax, kMinAlpha);
+  ay = fmaxf(ay, kMinAlpha);
+  const F3 ws = normalize_or_zero(f3(ax * wo.x, ay * wo.y, wo.z));
+  if (len2(ws) == 0.0f) return false;
+  const float phi = kTau * u0;
+  const float a = clampf(fminf(ax, ay), 0.0f, 1.0f);
+  const float s = 1.0f + sqrtf(wo.x * wo.x + wo.y * wo.y);
+  const float a2 = a * a, s2 = s * s;
+  const float k = (1.0f - a2) * s2 / (s2 + a2 * wo.z * wo.z);
+  const float lb = wo.z > 0.0f ? -k * ws.z : -ws.z;
+  const float z = fmaf(lb, u1, 1.0f - u1);
+  const float st = sqrtf(clampf(1.0f - z * z, 0.0f, 1.0f));
+  const F3 ms = add(ws, f3(st * cosf(phi), st * sinf(phi), z));
+  const F3 wm = normalize_or_zero(f3(ax * ms.x, ay * ms.y, ms.z));
+  if (len2(wm) == 0.0f || wm.z <= 0.0f) return false;
+  *out = wm;
+  return true;
+}
+__device__ __forceinline__ F3 reflect_local(F3 wo, F3 wm) {  // R/bsdf/smith_ggx.rs:12-14
+  return normalize_or_zero(add(neg(wo), muls(wm, 2.0f * dot(wo, wm))));
+}
+__device__ __forceinline__ bool sample_two_lobe(const Bsdf& p, F3 wo, float ax, float ay, TwoLobe w, float ux, float uy,
+                                                Sample* out) {
+  const float ssp = w.ss / w.total;
+  if (ux >= ssp) {
+    const float u = ssp < 1.0f ? (ux - ssp) / (1.0f - ssp) : 0.0f;
+    return finalize(p, wo, cosine_hemisphere(u, uy), out);
+  }
+  F3 wm;
+  if (!sample_bounded_vndf(wo, ax, ay, ssp > 0.0f ? ux / ssp : 0.0f, uy, &wm)) return false;
+  const F3 wi = reflect_local(wo, wm);
+  if (wi.z <= 0.0f) return false;
+  return finalize(p, wo, wi, out);
+}
+__device__ __forceinline__ bool delta_sample(F3 weight, F3 wi, float p_lobe, Sample* out) {
+  out->weight = weight;
+  out->wi = wi;
+  out->pdf = p_lobe;
+  out->delta = true;
+  return true;
+}
+// OpenPbrBsdf::sample (:295-331), pick_lobe (:2016-2041), sample_coat /
+// sample_metal / sample_spec_brdf / sample_fuzz / sample_diff_brdf
+// (:1467-1668, 1929-1940)
+__device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, float ux, float uy, Sample* out) {
   if (!(wo.z > 0.0f)) return false;
-  const Probs pr = lobe_probs(p);
+  const Probs pr = lobe_probs(p, wo);
   if (pr.total <= 0.0f) return false;
   float u = u_lobe * pr.total;
-  if (u < 0.0f) return false;  // coat (probability 0)
-  if (u < 0.0f) return false;  // metal
-  bool spec;
-  if (u < pr.spec) {
-    spec = true;
+  int lobe;
+  float plv;
+  if (u < pr.coat) {
+    lobe = 0;
+    plv = pr.coat;
+  } else if ((u -= pr.coat) < pr.metal) {
+    lobe = 1;
+    plv = pr.metal;
+  } else if ((u -= pr.metal) < pr.spec) {
+    lobe = 2;
+    plv = pr.spec;
+  } else if ((u -= pr.spec) < 0.0f) {
+    lobe = 3;
+    plv = 0.0f;
+  } else if ((u -= 0.0f) < pr.fuzz) {
+    lobe = 4;
+    plv = pr.fuzz;
+  } else if ((u -= pr.fuzz) < pr.diff) {
+    lobe = 5;
+    plv = pr.diff;
   } else {
-    spec = false;
-    u -= pr.spec;
-    if (!(u < pr.diff)) return false;  // DiffBtdf: probability 0 -> no sample
+    lobe = 6;
+    plv = 0.0f;
   }
-  const float p_lobe = (spec ? pr.spec : pr.diff) / pr.total;
+  const float p_lobe = plv / pr.total;
   if (p_lobe <= 0.0f) return false;
-  if (spec) {
-    if (spec_smooth(p)) {
-      const F3 wi = f3(-wo.x, -wo.y, wo.z);
-      F3 sw, dw;
-      layer_weights(p, &sw, &dw);
-      out->weight = divs(muls(sw, spec_fresnel(p, fabsf(wi.z))), p_lobe);
-      out->wi = wi;
-      out->pdf = p_lobe;
-      out->delta = true;
-      return true;
+  const F3 mirror = f3(-wo.x, -wo.y, wo.z);
+  if (lobe == 0) {
+    if (coat_smooth(p)) {
+      const float f = fresnel_dielectric(fabsf(mirror.z), 1.0f, p.coat_eta);
+      return delta_sample(f3s(layer_weights(p, wo, &mirror).coat_amp * f / p_lobe), mirror, p_lobe, out);
     }
-    const float ss_prob = 0.0f / 1.0e-8f;  // dielectric reflection lobe weights at eta 1
-    if (ux >= ss_prob) {
-      const float uu = ss_prob < 1.0f ? (ux - ss_prob) / (1.0f - ss_prob) : 0.0f;
-      return finalize(p, wo, cosine_hemisphere(uu, uy), out);
-    }
-    return false;
+    return sample_two_lobe(p, wo, p.cax, p.cay, refl_lobes(ms_params(p, p.cax, p.cay, p.coat_eta)), ux, uy, out);
   }
-  float pdf;
-  const F3 wi = sample_eon(wo, p.diff_rough, ux, uy, &pdf);
-  if (pdf <= 0.0f || wi.z <= 0.0f) return false;
-  return finalize(p, wo, wi, out);
+  if (lobe == 1) {
+    if (metal_smooth(p)) {
+      const F3 f = metal_fresnel(p, fabsf(mirror.z));
+      return delta_sample(divs(hadamard(layer_weights(p, wo, &mirror).metal, f), p_lobe), mirror, p_lobe, out);
+    }
+    return sample_two_lobe(p, wo, p.ax, p.ay, cond_lobes(p), ux, uy, out);
+  }
+  if (lobe == 2) {
+    if (spec_smooth(p)) {
+      const float f = base_fresnel(p, fabsf(mirror.z));
+      return delta_sample(divs(hadamard(layer_weights(p, wo, &mirror).spec, f3s(f)), p_lobe), mirror, p_lobe, out);
+    }
+    return sample_two_lobe(p, wo, p.ax, p.ay, refl_lobes(ms_params(p, p.ax, p.ay, p.spec_eta)), ux, uy, out);
+  }
+  if (lobe == 4) {
+    F3 wi;
+    if (!fuzz_sample(p, wo, ux, uy, &wi)) return false;
+    return finalize(p, wo, wi, out);
+  }
+  if (lobe == 5) {
+    float pdf;
+    const F3 wi = sample_eon(wo, p.diff_rough, ux, uy, &pdf);
+    if (pdf <= 0.0f || wi.z <= 0.0f) return false;
+    return finalize(p, wo, wi, out);
+  }
+  return false;
 }
 
 // ---- scene shading -----------------------------------------------------------------------
@@ -431,6 +921,7 @@ struct ShadeView {
   uint32_t light_count;
   float w_tree, w_env;  // light-sampler category weights: root flux, environment power (0: absent)
   EnvView env;
+  Tables tab;  // GGX tables of the OpenPBR materials
 };
 
 struct Vtx {
@@ -504,30 +995,46 @@ __device__ __forceinline__ void triangle_world(const ShadeView& sv, uint32_t ins
   for (int k = 0; k < 3; ++k) p[k] = xform_point(si.l2w, load3(sv.pos, si.vbase + sv.idx[si.ibase + 3 * tri + k]));
 }
 
-// OpenPbrMaterial::le (R/material/open_pbr.rs:351-375); the exit tint is 1
-// without coat and fuzz (0 at a grazing exit).
+// OpenPbrMaterial::emission_exit_tint (R/material/open_pbr.rs:608-638)
+__device__ __forceinline__ F3 exit_tint(const TptOpenPbr& m, F3 wol) {
+  const float co = clampf(fabsf(wol.z), 0.0f, 1.0f);
+  if (co <= 0.0f) return f3s(0.0f);
+  F3 tint = f3s(1.0f);
+  const float cw = clampf(m.coat_weight, 0.0f, 1.0f);
+  if (cw > 0.0f) {
+    const float ec = fmaxf(m.coat_ior, 1.0e-4f);
+    const float mu = fmaxf(sqrtf(fmaxf(1.0f - (1.0f - co * co) / (ec * ec), 0.0f)), 1.0e-4f);
+    const F3 tr = pow3(clamp01(f3(m.coat_color[0], m.coat_color[1], m.coat_color[2])), 1.0f / mu);
+    tint = hadamard(tint, lerp3(f3s(1.0f), tr, cw));
+  }
+  const float fw = clampf(m.fuzz_weight * 1.0f, 0.0f, 1.0f);
+  if (fw > 0.0f) {
+    const float ef = zeltner_albedo(co, clampf(m.fuzz_roughness * 1.0f, 0.0f, 1.0f));
+    tint = hadamard(tint, f3s(lerpf(1.0f, fmaxf(1.0f - ef, 0.0f), fw)));
+  }
+  return tint;
+}
+// OpenPbrMaterial::le (R/material/open_pbr.rs:351-375), no emission texture
 __device__ __forceinline__ bool pbr_le(const TptOpenPbr& m, const Vtx& v, F3* le) {
   if (m.emission_luminance <= 0.0f) return false;
   if (!m.thin_walled && !v.front) return false;
   const F3 wl = normalize_or_zero(to_frame(v.frame, v.wo));
-  const float co = clampf(fabsf(wl.z), 0.0f, 1.0f);
-  const float tint = co <= 0.0f ? 0.0f : 1.0f;
-  const F3 c = muls(f3(m.emission_color[0], m.emission_color[1], m.emission_color[2]),
+  const F3 c = muls(hadamard(f3(m.emission_color[0], m.emission_color[1], m.emission_color[2]), f3s(1.0f)),
                     fmaxf(m.emission_luminance, 0.0f));
-  *le = hadamard(c, f3(tint, tint, tint));
+  *le = hadamard(c, exit_tint(m, wl));
   return true;
 }
 
 // OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548), radiance mode.
-__device__ __forceinline__ bool pbr_sample(const TptOpenPbr& m, const Vtx& v, float u_lobe, float ux, float uy,
-                                           Sample* s, F3* wi_world) {
+__device__ __forceinline__ bool pbr_sample(const Tables& tb, const TptOpenPbr& m, const Vtx& v, float u_lobe, float ux,
+                                           float uy, Sample* s, F3* wi_world) {
   if (dot(v.wo, v.ng) <= 0.0f) return false;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
-  const Bsdf p = make_bsdf(m, v.front);
+  const Bsdf p = make_bsdf(tb, v.material, m, v.front, wol);
   if (!bsdf_sample(p, wol, u_lobe, ux, uy, s)) return false;
   const F3 wi = to_world(v.frame, s->wi);
-  if (dot(wi, v.ng) <= 1.0e-6f) return false;  // GEOMETRIC_NORMAL_COS_EPSILON, reflection lobes
+  if (dot(wi, v.ng) <= 1.0e-6f) return false;  // GEOMETRIC_NORMAL_COS_EPSILON; every carried lobe reflects
   *wi_world = wi;
   return true;
 }
@@ -549,9 +1056,9 @@ __device__ __forceinline__ bool mat_le(const TptMaterial& m, const Vtx& v, F3* l
 }
 // NormalizedLambertMaterial::sample (R/material/normalized_lambert.rs:120-158)
 // and MirrorMaterial::sample (R/material/mirror.rs:112-143), radiance mode.
-__device__ __forceinline__ bool mat_sample(const TptMaterial& m, const Vtx& v, float u_lobe, float ux, float uy,
-                                           Sample* s, F3* wi_world) {
-  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_sample(m.u.open_pbr, v, u_lobe, ux, uy, s, wi_world);
+__device__ __forceinline__ bool mat_sample(const Tables& tb, const TptMaterial& m, const Vtx& v, float u_lobe, float ux,
+                                           float uy, Sample* s, F3* wi_world) {
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_sample(tb, m.u.open_pbr, v, u_lobe, ux, uy, s, wi_world);
   if (m.kind == TPT_MATERIAL_EMISSIVE) return false;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
@@ -580,7 +1087,7 @@ __device__ __forceinline__ bool mat_sample(const TptMaterial& m, const Vtx& v, f
 // and OpenPBR (R/material/open_pbr.rs:549-590), each with modified_bsdf_eval
 // (R/material/mod.rs:33-53); Mirror and Emissive evaluate to zero.  *pdf is
 // 0 when f is 0 (Material::eval_pdf).
-__device__ __forceinline__ F3 mat_eval_pdf(const TptMaterial& m, const Vtx& v, F3 wi, float* pdf) {
+__device__ __forceinline__ F3 mat_eval_pdf(const Tables& tb, const TptMaterial& m, const Vtx& v, F3 wi, float* pdf) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
   *pdf = 0.0f;
   F3 f;
@@ -597,7 +1104,7 @@ __device__ __forceinline__ F3 mat_eval_pdf(const TptMaterial& m, const Vtx& v, F
     const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
     const F3 wil = normalize_or_zero(to_frame(v.frame, wi));
     if (wol.z <= 0.0f) return zero;
-    const Bsdf b = make_bsdf(m.u.open_pbr, v.front);
+    const Bsdf b = make_bsdf(tb, v.material, m.u.open_pbr, v.front, wol);
     f = bsdf_eval(b, wol, wil);
     if (len2(f) == 0.0f) return zero;
     p = bsdf_pdf(b, wol, wil);
@@ -962,7 +1469,7 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
   }
   if (li_pdf <= 0.0f || selection <= 0.0f) return false;
   float bsdf_pdf;
-  const F3 f = mat_eval_pdf(m, vtx, wi, &bsdf_pdf);
+  const F3 f = mat_eval_pdf(sv.tab, m, vtx, wi, &bsdf_pdf);
   if (len2(f) == 0.0f) return false;
   const float cos_surface = fabsf(dot(vtx.ng, wi));
   if (cos_surface <= 0.0f) return false;
@@ -1054,6 +1561,7 @@ inline ShadeView shade_view_of(const TptScene* s) {
   v.w_tree = s->w_tree;
   v.w_env = s->w_env;
   v.env = s->env;
+  v.tab = Tables{s->ggx_albedo, s->ggx_mat, s->ggx_cond, s->ggx_diel};
   return v;
 }
 

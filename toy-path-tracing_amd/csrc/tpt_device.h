@@ -129,6 +129,13 @@ struct TptScene {
   float* env_pyr = nullptr;
   uint32_t* env_levels = nullptr;
   tpt::pt::EnvView env{};
+  // GGX tables of the OpenPBR materials (ggx_tables.h): directional-albedo
+  // tables (table 0 all zero), per material its spec and coat table, the
+  // conductor and dielectric energy-compensation tables (zero when unused)
+  float* ggx_albedo = nullptr;
+  uint32_t* ggx_mat = nullptr;
+  float* ggx_cond = nullptr;
+  float* ggx_diel = nullptr;
   size_t bytes = 0;
   // staging for host-pointer calls (grow-only)
   void* stage = nullptr;
@@ -140,6 +147,8 @@ bool launch_trace_queue(TptScene* s, int query, bool q8, bool exact, const TptRa
                         uint32_t n_max, TptHit* hits, const uint32_t* target, uint8_t* out, hipStream_t st);
 // true if every instance's material is one the path tracers implement;
 // otherwise records "<fn>: ..." and returns false.
+// Builds and uploads the GGX tables of the scene's materials (trace.hip).
+bool upload_ggx_tables(TptScene* s, const std::vector<TptMaterial>& mats);
 bool render_materials_ok(TptScene* s, const char* fn);
 
 inline SceneView view_of(const TptScene* s) {

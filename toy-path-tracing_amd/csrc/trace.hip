@@ -21,12 +21,14 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "environment.h"
+#include "ggx_tables.h"
 #include "tpt_device.h"
 #include "tpt_internal.h"
 #include "traverse.h"
@@ -295,8 +297,68 @@ void free_scene(TptScene* s) {
   (void)hipFree(s->env_px);
   (void)hipFree(s->env_pyr);
   (void)hipFree(s->env_levels);
+  (void)hipFree(s->ggx_albedo);
+  (void)hipFree(s->ggx_mat);
+  (void)hipFree(s->ggx_cond);
+  (void)hipFree(s->ggx_diel);
   delete s;
 }
+
+}  // namespace
+
+// GGX tables of the scene's OpenPBR materials (see TptScene): every
+// distinct specular IOR and, for coated materials, coat IOR gets its
+// directional-albedo table; the energy-compensation tables are uploaded
+// when some material reads them with a non-zero weight (metalness > 0 for
+// the conductor's; a specular IOR other than 1 or a coat for the
+// dielectric's) and as zeros otherwise.
+bool upload_ggx_tables(TptScene* s, const std::vector<TptMaterial>& mats) {
+  std::vector<float> albedo(ggx::kAlbedoLen, 0.0f);
+  std::map<uint32_t, uint32_t> index;
+  auto table_of = [&](float eta) {
+    const float e = (std::isfinite(eta) && eta > 0.0f) ? eta : 1.5f;
+    uint32_t key;
+    std::memcpy(&key, &e, 4);
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    const std::vector<float>& t = ggx::albedo_table(e);
+    const uint32_t id = (uint32_t)(albedo.size() / ggx::kAlbedoLen);
+    albedo.insert(albedo.end(), t.begin(), t.end());
+    index.emplace(key, id);
+    return id;
+  };
+  std::vector<uint32_t> mt(2 * std::max<size_t>(1, mats.size()), 0u);
+  bool need_cond = false, need_diel = false;
+  for (size_t i = 0; i < mats.size(); ++i) {
+    if (mats[i].kind != TPT_MATERIAL_OPEN_PBR) continue;
+    const TptOpenPbr& m = mats[i].u.open_pbr;
+    const float eta = ggx::modulated_eta(std::fmax(m.specular_ior, 1.0e-4f), std::fmax(m.specular_weight, 0.0f));
+    mt[2 * i] = table_of(eta);
+    const float cw = m.coat_weight < 0.0f ? 0.0f : (m.coat_weight > 1.0f ? 1.0f : m.coat_weight);
+    if (cw > 0.0f) {
+      mt[2 * i + 1] = table_of(std::fmax(m.coat_ior, 1.0e-4f));
+      need_diel = true;
+    }
+    if (eta != 1.0f) need_diel = true;
+    if (m.base_metalness > 0.0f) need_cond = true;
+  }
+  const size_t ncond = ggx::kCondCos * ggx::kCondRough + ggx::kCondRough;
+  const size_t ndiel = ggx::kDielCos * ggx::kDielRough * ggx::kDielEta + ggx::kDielRough * ggx::kDielEta;
+  const std::vector<float> cond = need_cond ? ggx::conductor_table() : std::vector<float>(ncond, 0.0f);
+  const std::vector<float> diel = need_diel ? ggx::dielectric_table() : std::vector<float>(ndiel, 0.0f);
+  (void)hipFree(s->ggx_albedo);
+  (void)hipFree(s->ggx_mat);
+  (void)hipFree(s->ggx_cond);
+  (void)hipFree(s->ggx_diel);
+  s->ggx_albedo = nullptr;
+  s->ggx_mat = nullptr;
+  s->ggx_cond = nullptr;
+  s->ggx_diel = nullptr;
+  return upload(albedo, &s->ggx_albedo, &s->bytes) && upload(mt, &s->ggx_mat, &s->bytes) &&
+         upload(cond, &s->ggx_cond, &s->bytes) && upload(diel, &s->ggx_diel, &s->bytes);
+}
+
+namespace {
 
 // Upper bound of the traversal stack: along any root-to-node path, a visit
 // leaves (interior children - 1) entries below the next pop.
@@ -687,6 +749,10 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
         !upload(vnrm, &s->vnrm, &s->bytes) || !upload(vuv, &s->vuv, &s->bytes) ||
         !upload(vidx, &s->vidx, &s->bytes) || !upload(mats, &s->mats, &s->bytes) ||
         !upload(glights, &s->lights, &s->bytes) || !upload(tri_leaf, &s->tri_leaf, &s->bytes)) {
+      free_scene(s);
+      return nullptr;
+    }
+    if (!upload_ggx_tables(s, mats)) {
       free_scene(s);
       return nullptr;
     }

@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
           F3 wi;
           float ud[2];
           smpl.s2(bd + 7, ud, zrows[4]);
-          if (pt::mat_sample(m, vtx, smpl.s1(bd + 5, zrows[3]), ud[0], ud[1], &smp_, &wi)) {
+          if (pt::mat_sample(sv.tab, m, vtx, smpl.s1(bd + 5, zrows[3]), ud[0], ud[1], &smp_, &wi)) {
             thr = pt::hadamard(thr, smp_.weight);
             bool live = true;
             if (bounce + 1 >= 4) {

@@ -126,6 +126,13 @@ class RenderDesc(ctypes.Structure):
 assert ctypes.sizeof(RenderDesc) == 80
 
 
+BSDF_QUERY_DTYPE = np.dtype([("wo", "<f4", 3), ("wi", "<f4", 3), ("u", "<f4", 3), ("material", "<u4"),
+                             ("front_face", "<u4"), ("pad", "<u4")])
+BSDF_RESULT_DTYPE = np.dtype([("f", "<f4", 3), ("pdf", "<f4"), ("weight", "<f4", 3), ("sample_pdf", "<f4"),
+                              ("wi", "<f4", 3), ("flags", "<u4")])
+assert BSDF_QUERY_DTYPE.itemsize == 48 and BSDF_RESULT_DTYPE.itemsize == 48
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [("wf_paths", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("refill_below", ctypes.c_uint32),
                 ("trials", ctypes.c_uint32), ("ms_default", ctypes.c_float), ("ms_best", ctypes.c_float)]
@@ -184,6 +191,8 @@ _SIGNATURES = [
     ("tpt_render_adjoint", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_scene_set_materials", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_material_bsdf", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_autotune_render", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     ("tpt_trace_counts", ctypes.c_int32,

@@ -24,22 +24,27 @@ class OpenPbrRecord(ctypes.Structure):
         ("fuzz_weight", ctypes.c_float), ("thin_film_weight", ctypes.c_float),
         ("emission_luminance", ctypes.c_float), ("emission_color", ctypes.c_float * 3),
         ("thin_walled", ctypes.c_uint32),
+        ("coat_color", ctypes.c_float * 3), ("coat_roughness", ctypes.c_float),
+        ("coat_roughness_anisotropy", ctypes.c_float), ("coat_ior", ctypes.c_float),
+        ("coat_darkening", ctypes.c_float),
+        ("fuzz_color", ctypes.c_float * 3), ("fuzz_roughness", ctypes.c_float),
+        ("pad", ctypes.c_float),
     ]
 
 
-assert ctypes.sizeof(OpenPbrRecord) == 88
+assert ctypes.sizeof(OpenPbrRecord) == 136
 
 KIND_OPEN_PBR, KIND_NORMALIZED_LAMBERT, KIND_EMISSIVE, KIND_MIRROR = 0, 1, 2, 3
 
 
 class MaterialRecord(ctypes.Structure):
     """TptMaterial: kind, pad, then the kind's parameters (OpenPBR: the
-    88-byte TptOpenPbr; Lambert: rho[3]; Emissive: color[3], strength;
+    136-byte TptOpenPbr; Lambert: rho[3]; Emissive: color[3], strength;
     Mirror: color[3])."""
-    _fields_ = [("kind", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("params", ctypes.c_float * 22)]
+    _fields_ = [("kind", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("params", ctypes.c_float * 34)]
 
 
-assert ctypes.sizeof(MaterialRecord) == 96
+assert ctypes.sizeof(MaterialRecord) == 144
 
 
 @dataclass(frozen=True)
@@ -60,6 +65,13 @@ class OpenPbr:
     emission_luminance: float = 0.0
     emission_color: tuple = field(default=(1.0, 1.0, 1.0))
     thin_walled: bool = False
+    coat_color: tuple = (1.0, 1.0, 1.0)
+    coat_roughness: float = 0.0
+    coat_roughness_anisotropy: float = 0.0
+    coat_ior: float = 1.6
+    coat_darkening: float = 1.0
+    fuzz_color: tuple = (1.0, 1.0, 1.0)
+    fuzz_roughness: float = 0.5
 
     @staticmethod
     def new(base_color) -> "OpenPbr":
@@ -73,6 +85,8 @@ class OpenPbr:
     def open_pbr_record(self) -> OpenPbrRecord:
         r = OpenPbrRecord()
         for name, _ in OpenPbrRecord._fields_:
+            if name == "pad":
+                continue
             v = getattr(self, name)
             if isinstance(v, tuple):
                 getattr(r, name)[:] = [float(x) for x in v]
@@ -84,7 +98,7 @@ class OpenPbr:
         r = MaterialRecord()
         r.kind = KIND_OPEN_PBR
         rec = self.open_pbr_record()  # kept alive across the copy
-        ctypes.memmove(ctypes.addressof(r.params), ctypes.addressof(rec), 88)
+        ctypes.memmove(ctypes.addressof(r.params), ctypes.addressof(rec), ctypes.sizeof(OpenPbrRecord))
         return r
 
 

@@ -450,6 +450,20 @@ class DeviceScene:
         check(lib().tpt_autotune_render(self.handle, arr, len(frames), flags, tune_spp, ctypes.byref(res)))
         return {name: getattr(res, name) for name, _ in _lib.TuneResult._fields_}
 
+    def material_bsdf(self, material, wo, wi, u, front_face: bool = True) -> np.ndarray:
+        """tpt_material_bsdf: the BSDF of material index `material` in the
+        shading frame, for rows of wo, wi (eval / pdf) and u = (u_lobe, u0,
+        u1) (sample) -> structured array of BSDF_RESULT_DTYPE."""
+        wo, wi, u = (np.asarray(a, np.float32).reshape(-1, 3) for a in (wo, wi, u))
+        n = max(len(wo), len(wi), len(u))
+        q = np.zeros(n, _lib.BSDF_QUERY_DTYPE)
+        q["wo"], q["wi"], q["u"] = wo, wi, u
+        q["material"] = np.asarray(material, np.uint32)
+        q["front_face"] = int(bool(front_face))
+        out = np.zeros(n, _lib.BSDF_RESULT_DTYPE)
+        check(lib().tpt_material_bsdf(self.handle, q.ctypes.data, n, out.ctypes.data, 0))
+        return out
+
     def render_batch_device(self, frames, outs_dev, quantized: bool = False, exact_shadow: bool = False) -> None:
         """As render_batch into device buffers (torch tensors); asynchronous."""
         arr = (_lib.FrameDesc * max(1, len(frames)))()

@@ -168,11 +168,13 @@ class CornellScene:
     camera: dict
 
 
-def cornell(build: bool = True) -> CornellScene:
+def cornell(build: bool = True, specular_weight: float = 1.0) -> CornellScene:
     """Cornell box of the plain-method configuration: 36 triangles in 6
-    instances.  Materials: OpenPBR ``new(c).with_specular_weight(0)`` for
-    white (0.8), red (0.8,0.1,0.1), green (0.1,0.8,0.1); the light is OpenPBR
-    emission 15 with base and specular weight 0.  Walls face into the box."""
+    instances.  Materials: OpenPBR ``new(c)`` (the reference's defaults:
+    specular weight 1, roughness 0.3, IOR 1.5; ``specular_weight`` overrides
+    the weight) for white (0.8), red (0.8,0.1,0.1), green (0.1,0.8,0.1); the
+    light is OpenPBR emission 15 with base and specular weight 0.  Walls face
+    into the box."""
     from .material import OpenPbr
 
     s = Scene()
@@ -197,7 +199,7 @@ def cornell(build: bool = True) -> CornellScene:
     li = s.add_instance(light, 3, eye4)
     if build:
         s.build_qbvh()
-    diffuse = lambda c: OpenPbr.new(c).with_(specular_weight=0.0)
+    diffuse = lambda c: OpenPbr.new(c).with_(specular_weight=float(specular_weight))
     mats = [diffuse((0.8, 0.8, 0.8)), diffuse((0.8, 0.1, 0.1)), diffuse((0.1, 0.8, 0.1)),
             OpenPbr.new((0.0, 0.0, 0.0)).with_(base_weight=0.0, specular_weight=0.0, emission_luminance=15.0,
                                                emission_color=(1.0, 1.0, 1.0))]
