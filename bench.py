@@ -246,9 +246,10 @@ def measured_traffic(kind, nodes, n_rays):
 
 
 def render_bench(dev, stream, args):
-    """Plain FP32 megakernel path tracer on the Cornell configuration
-    (512x512, 64 spp, max depth 8): the OpenPBR diffuse materials of the
-    configuration (specular weight 0; the reference gives OpenPBR surfaces no
+    """FP32 path tracer (megakernel, then the autotuned wavefront) on the
+    Cornell configuration (512x512, 64 spp, max depth 8): OpenPBR walls with
+    the reference's default parameters (OpenPbrMaterial::new: specular weight
+    1, roughness 0.3, IOR 1.5; the reference gives OpenPBR surfaces no
     light-tree NEE), and the same box with the reference's NormalizedLambert
     walls + Emissive light, where light-tree NEE with MIS runs."""
     import torch
@@ -259,7 +260,8 @@ def render_bench(dev, stream, args):
     spp, depth = 64, 8
     out = {}
     for key, make, label in (
-        ("openpbr", scenes.cornell, "OpenPBR diffuse (specular weight 0), BSDF sampling + emission"),
+        ("openpbr", scenes.cornell, "OpenPBR defaults (EON base + GGX specular with its albedo and "
+                                    "energy-compensation tables), BSDF sampling + emission"),
         ("lambert_nee", scenes.cornell_lambert, "NormalizedLambert + Emissive light, light-tree NEE + MIS"),
     ):
         cs = make()
