@@ -247,11 +247,13 @@ def measured_traffic(kind, nodes, n_rays):
 
 def render_bench(dev, stream, args):
     """FP32 path tracer (megakernel, then the autotuned wavefront) on the
-    Cornell configuration (512x512, 64 spp, max depth 8): OpenPBR walls with
-    the reference's default parameters (OpenPbrMaterial::new: specular weight
-    1, roughness 0.3, IOR 1.5; the reference gives OpenPBR surfaces no
-    light-tree NEE), and the same box with the reference's NormalizedLambert
-    walls + Emissive light, where light-tree NEE with MIS runs."""
+    Cornell configuration (512x512, 64 spp, max depth 8): the configuration's
+    diffuse OpenPBR walls (new(c).with_specular_weight(0); the reference
+    gives OpenPBR surfaces no light-tree NEE), the same walls with the
+    reference's OpenPbrMaterial::new defaults (specular weight 1, roughness
+    0.3, IOR 1.5: the full BSDF with its GGX tables), and the same box with
+    the reference's NormalizedLambert walls + Emissive light, where light-tree
+    NEE with MIS runs."""
     import torch
 
     from tpt import camera, scenes
@@ -260,8 +262,10 @@ def render_bench(dev, stream, args):
     spp, depth = 64, 8
     out = {}
     for key, make, label in (
-        ("openpbr", scenes.cornell, "OpenPBR defaults (EON base + GGX specular with its albedo and "
-                                    "energy-compensation tables), BSDF sampling + emission"),
+        ("openpbr", scenes.cornell, "OpenPBR diffuse (new(c).with_specular_weight(0)), BSDF sampling + emission"),
+        ("openpbr_glossy", lambda: scenes.cornell(specular_weight=1.0),
+         "OpenPBR defaults (EON base + GGX specular with its albedo and energy-compensation tables), "
+         "BSDF sampling + emission"),
         ("lambert_nee", scenes.cornell_lambert, "NormalizedLambert + Emissive light, light-tree NEE + MIS"),
     ):
         cs = make()

@@ -1,6 +1,7 @@
 """GPU path tracer vs the oracle's restatement of the reference renderer
-(R/main.rs:98-126 + R/integrator/mis.rs on OpenPBR with the reference's
-default parameters: specular weight 1, roughness 0.3, IOR 1.5).
+(R/main.rs:98-126 + R/integrator/mis.rs on OpenPBR: the configuration's
+diffuse walls, specular weight 0, and the reference's defaults, specular
+weight 1, roughness 0.3, IOR 1.5).
 
 The two run the same estimator on the same sample sequence: the sampler,
 camera, traversal, shading frame and most BSDF arithmetic agree bit for bit,
@@ -27,9 +28,11 @@ def relmse(x, ref):
     return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
 
 
-@pytest.fixture(scope="module")
-def cornell(device):
-    cs = scenes.cornell()
+@pytest.fixture(scope="module", params=[0.0, 1.0], ids=["diffuse", "glossy"])
+def cornell(device, request):
+    # the configuration's diffuse OpenPBR walls, and the reference's
+    # OpenPbrMaterial::new defaults (specular weight 1: the GGX tables)
+    cs = scenes.cornell(specular_weight=request.param)
     ds = cs.scene.upload(device)
     yield cs, ds
     ds.close()

@@ -168,11 +168,12 @@ class CornellScene:
     camera: dict
 
 
-def cornell(build: bool = True, specular_weight: float = 1.0) -> CornellScene:
+def cornell(build: bool = True, specular_weight: float = 0.0) -> CornellScene:
     """Cornell box of the plain-method configuration: 36 triangles in 6
-    instances.  Materials: OpenPBR ``new(c)`` (the reference's defaults:
-    specular weight 1, roughness 0.3, IOR 1.5; ``specular_weight`` overrides
-    the weight) for white (0.8), red (0.8,0.1,0.1), green (0.1,0.8,0.1); the
+    instances.  Materials: OpenPBR ``new(c).with_specular_weight(w)`` for
+    white (0.8), red (0.8,0.1,0.1), green (0.1,0.8,0.1), with w = 0 (the
+    configuration's "diffuse OpenPBR"; ``specular_weight=1`` gives the
+    reference's OpenPbrMaterial::new defaults: roughness 0.3, IOR 1.5); the
     light is OpenPBR emission 15 with base and specular weight 0.  Walls face
     into the box."""
     from .material import OpenPbr
