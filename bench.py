@@ -253,7 +253,8 @@ def render_bench(dev, stream, args):
     reference's OpenPbrMaterial::new defaults (specular weight 1, roughness
     0.3, IOR 1.5: the full BSDF with its GGX tables), and the same box with
     the reference's NormalizedLambert walls + Emissive light, where light-tree
-    NEE with MIS runs."""
+    NEE with MIS runs; and the textured box (every texture slot, EWA lookups
+    with ray differentials)."""
     import torch
 
     from tpt import camera, scenes
@@ -267,6 +268,9 @@ def render_bench(dev, stream, args):
          "OpenPBR defaults (EON base + GGX specular with its albedo and energy-compensation tables), "
          "BSDF sampling + emission"),
         ("lambert_nee", scenes.cornell_lambert, "NormalizedLambert + Emissive light, light-tree NEE + MIS"),
+        ("textured", scenes.cornell_textured,
+         "OpenPBR with base-colour, roughness, metalness, fuzz and emission textures (EWA over mip pyramids, "
+         "ray differentials and cones), a Mirror box and a NormalizedLambert box with light-tree NEE"),
     ):
         cs = make()
         rds = cs.scene.upload(dev)

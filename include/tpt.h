@@ -108,12 +108,19 @@ typedef struct TptInstanceDesc {
 /* OpenPbrMaterial parameters (R/material/open_pbr.rs:23-82) as the renderer
  * holds them after prepare_color (rendering-space colours): base, metal,
  * specular, coat, fuzz and emission.  Materials with transmission, thin
- * film or thin walls are rejected with TPT_ERROR_BAD_ARGUMENT (textures,
- * normal maps and subsurface are not part of the record).  The coat and
- * fuzz fields follow the reference's field order after thin_walled; the
+ * film or thin walls are rejected with TPT_ERROR_BAD_ARGUMENT (normal maps,
+ * opacity and subsurface are not part of the record).  The coat and fuzz
+ * fields follow the reference's field order after thin_walled; the
  * reference's defaults (OpenPbrMaterial::new) are coat_color 1,
  * coat_roughness 0, coat_ior 1.6, coat_darkening 1, fuzz_color 1,
- * fuzz_roughness 0.5.  Carried inside TptMaterial. */
+ * fuzz_roughness 0.5.  The *_texture fields index the scene's texture
+ * table (TptSceneDesc::textures) or are TPT_INVALID_INDEX for none: the
+ * reference's base_color_texture, base_metalness_texture,
+ * specular_roughness_texture, fuzz_weight_texture, fuzz_roughness_texture
+ * and emission_color_texture (R/material/open_pbr.rs:66-72), each
+ * multiplying its parameter by the filtered texel (RGB textures for the
+ * two colours, scalar textures for the rest; *_at, :670-790).  Carried
+ * inside TptMaterial. */
 typedef struct TptOpenPbr {
   float base_weight;
   float base_color[3];
@@ -138,8 +145,14 @@ typedef struct TptOpenPbr {
   float coat_darkening;
   float fuzz_color[3];
   float fuzz_roughness;
-  float pad;
-} TptOpenPbr; /* 136 bytes */
+  uint32_t base_color_texture;
+  uint32_t base_metalness_texture;
+  uint32_t specular_roughness_texture;
+  uint32_t fuzz_weight_texture;
+  uint32_t fuzz_roughness_texture;
+  uint32_t emission_color_texture;
+  uint32_t pad;
+} TptOpenPbr; /* 160 bytes */
 
 /* One entry of the scene's material table: the Material enum of
  * R/material/mod.rs:100-115 for the variants the backend carries.
@@ -147,7 +160,8 @@ typedef struct TptOpenPbr {
  *   TPT_MATERIAL_NORMALIZED_LAMBERT rho[3]              (R/material/normalized_lambert.rs:18-37)
  *   TPT_MATERIAL_EMISSIVE           color[3], strength  (R/material/emissive.rs:9-26)
  *   TPT_MATERIAL_MIRROR             color[3]            (R/material/mirror.rs)
- * Textures, normal maps and opacity are not part of the record. */
+ * Normal maps and opacity are not part of the record; OpenPBR carries its
+ * textures as indices into the scene's texture table. */
 #define TPT_MATERIAL_OPEN_PBR 0u
 #define TPT_MATERIAL_NORMALIZED_LAMBERT 1u
 #define TPT_MATERIAL_EMISSIVE 2u
@@ -160,9 +174,23 @@ typedef struct TptMaterial {
     struct { float rho[3]; } lambert;
     struct { float color[3]; float strength; } emissive;
     struct { float color[3]; } mirror;
-    float raw[34];
+    float raw[40];
   } u;
-} TptMaterial; /* 144 bytes */
+} TptMaterial; /* 168 bytes */
+
+/* A filtered, mip-mapped image (Texture<Vec3> or ScalarTexture,
+ * R/material/texture.rs:53-126): width * height texels of `channels` floats
+ * (3: RGB, 1: scalar), row-major, row 0 first; texel (x, y) covers
+ * uv [x/w, (x+1)/w) x [y/h, (y+1)/h).  tpt_scene_create copies level 0 to
+ * the device and builds the box-filtered pyramid there (build_mip_levels,
+ * :433-466). */
+typedef struct TptTextureDesc {
+  const float* pixels;
+  uint32_t width;
+  uint32_t height;
+  uint32_t channels;
+  uint32_t pad;
+} TptTextureDesc; /* 24 bytes */
 
 /* Scene after Scene::build_qbvh (R/scene/mod.rs:305-316). */
 typedef struct TptSceneDesc {
@@ -183,6 +211,11 @@ typedef struct TptSceneDesc {
   uint32_t env_height;
   float env_scale;
   float env_rotate_y;
+  /* Texture table indexed by the materials' *_texture fields (may be NULL
+   * with texture_count 0). */
+  const TptTextureDesc* textures;
+  uint32_t texture_count;
+  uint32_t pad;
 } TptSceneDesc;
 
 /* A node of the scene's SG light tree (R/light_tree/mod.rs:52-88), as
@@ -341,18 +374,30 @@ typedef struct TptRenderDesc {
 int32_t tpt_render(TptScene* scene, const TptRenderDesc* desc, float* rgb, uint32_t flags);
 
 /* Gradient of an image loss with respect to every NormalizedLambert
- * material's albedo (rho), by path-replay backpropagation: each pixel
- * sample's path is traced twice from its sampler state (primal radiance,
- * then a replay that attributes the radiance collected from each Lambert
- * vertex onward to that vertex's albedo); no path is stored.  Russian
- * roulette probabilities are held fixed.  dloss_dimage: width * height * 3
- * floats (dLoss/d of the tpt_render image, exposure included); d_albedo:
- * 3 * material_count floats (zero for other materials).  Sums are FP32
- * per block (LDS), then FP64 over blocks; TPT_FLAG_GRAD_FP64 makes the
- * per-block sums FP64.  No reference counterpart (the reference renderer
- * is not differentiable). */
+ * material's albedo (rho) and every OpenPBR material's base_color, by
+ * path-replay backpropagation: each pixel sample's path is traced twice from
+ * its sampler state (primal radiance, then a replay that attributes the
+ * radiance collected from each vertex onward to that vertex's parameters);
+ * no path is stored.  Sampling decisions, MIS weights and Russian roulette
+ * probabilities are held fixed.  OpenPBR materials are differentiated where
+ * their BSDF is affine in the base colour (base_metalness 0 and no coat
+ * darkening: coat_weight or coat_darkening 0); other OpenPBR rows are zero.
+ * dloss_dimage: width * height * 3 floats (dLoss/d of the tpt_render image,
+ * exposure included); d_albedo: 3 * material_count floats (zero for other
+ * materials).  Sums are FP32 per block (LDS), then FP64 over blocks;
+ * TPT_FLAG_GRAD_FP64 makes the per-block sums FP64.  No reference
+ * counterpart (the reference renderer is not differentiable). */
 int32_t tpt_render_adjoint(TptScene* scene, const TptRenderDesc* desc, const float* dloss_dimage, float* d_albedo,
                            uint32_t flags);
+/* As tpt_render_adjoint, and also the gradient with respect to the texels of
+ * the OpenPBR base-colour textures: d_texels gets the level-0 texels of every
+ * texture of the scene in table order (width * height * channels floats
+ * each; zero for textures that no differentiated material reads as its base
+ * colour).  The lookups' gradients are scattered through the EWA weights
+ * into every mip level and then through the box filter down to level 0.  No
+ * reference counterpart. */
+int32_t tpt_render_adjoint_texels(TptScene* scene, const TptRenderDesc* desc, const float* dloss_dimage,
+                                  float* d_albedo, float* d_texels, uint32_t flags);
 
 /* Replaces the scene's material table (same count; kinds and emission must
  * stay the same, as the light tree is built from them): the parameter update
@@ -426,6 +471,47 @@ typedef struct TptBsdfResult {
  * R/material/open_pbr.rs:393-487, NormalizedLambertBsdf, MirrorBsdf), on the
  * device, for n queries.  Host or device pointers per flags. */
 int32_t tpt_material_bsdf(TptScene* scene, const TptBsdfQuery* queries, size_t n, TptBsdfResult* out, uint32_t flags);
+
+/* ---- textures ------------------------------------------------------------------ */
+/* One texture lookup: uv and the screen-space derivatives of uv (the
+ * shading vertex's uv_dx() / uv_dy(), R/material/mod.rs:163-169). */
+typedef struct TptTextureQuery {
+  float uv[2];
+  float dstdx[2];
+  float dstdy[2];
+} TptTextureQuery; /* 24 bytes */
+/* Lookup modes, the Texture methods of R/material/texture.rs:
+ *   TPT_TEXTURE_FILTERED     sample_filtered: EWA over the mip pyramid
+ *                            (alpha 2, anisotropy at most 8), :140-193, :245-300
+ *   TPT_TEXTURE_BILINEAR     sample: bilinear on level 0, :130-132, :225-243
+ *   TPT_TEXTURE_NEAREST      sample_nearest, :134-142
+ *   TPT_TEXTURE_TRILINEAR    sample_trilinear, :200-223
+ *   TPT_TEXTURE_MIP_BILINEAR sample_mip_bilinear, :147-159
+ * The renderer uses TPT_TEXTURE_FILTERED (the materials' *_at fetches). */
+#define TPT_TEXTURE_FILTERED 0u
+#define TPT_TEXTURE_BILINEAR 1u
+#define TPT_TEXTURE_NEAREST 2u
+#define TPT_TEXTURE_TRILINEAR 3u
+#define TPT_TEXTURE_MIP_BILINEAR 4u
+/* n lookups of one texture of the scene on the device in the given mode;
+ * out gets `channels` floats per query.  Host or device pointers per flags. */
+int32_t tpt_texture_sample(TptScene* scene, uint32_t texture, uint32_t mode, const TptTextureQuery* queries, size_t n,
+                           float* out, uint32_t flags);
+/* Shape of a texture's pyramid: level count (Texture::level_count), channel
+ * count and the level-0 maximum (Texture::max_value, :121-124). */
+int32_t tpt_texture_info(const TptScene* scene, uint32_t texture, uint32_t* levels, uint32_t* channels,
+                         float* max_value);
+/* Replaces a texture's level-0 texels (same size and channels; host or
+ * device pointer per flags) and rebuilds its pyramid on the device: the
+ * texel update of an inverse-rendering loop.  An emission texture is
+ * rejected (the light tree is built from its maximum).  Synchronous.  No
+ * reference counterpart (the reference's textures are immutable). */
+int32_t tpt_scene_set_texture(TptScene* scene, uint32_t texture, const float* pixels, uint32_t flags);
+/* Copies mip level `level` of a texture to host memory: width * height *
+ * channels floats (capacity in floats; *width / *height get its size).
+ * Replaces reading TextureLevel::pixels (R/material/texture.rs:62-67). */
+int32_t tpt_texture_level(TptScene* scene, uint32_t texture, uint32_t level, float* out, size_t capacity,
+                          uint32_t* width, uint32_t* height);
 
 /* The traversal of tpt_trace_closest (query 0) or tpt_trace_any (query 1)
  * with per-ray work counters instead of results: counts[3*i+0] nodes

@@ -278,6 +278,8 @@ struct BSample {
   V3 weight, wi;
   float pdf;
   bool delta;
+  bool glossy = false;  // BsdfFlags::GLOSSY: a rough coat, metal, specular or fuzz lobe sample
+  float cone_spread = 0.0f;  // MaterialSample::cone_spread (set by the material)
 };
 // sample_cosine_weighted_hemisphere, R/math/mod.rs:26-36
 inline V3 cosine_hemisphere(float u, float v) {
@@ -784,6 +786,7 @@ const DielLut& zero_diel_lut();
 // ---- OpenPBR material, R/material/open_pbr.rs -----------------------------------------------
 // OpenPbrMaterial parameters (:23-82) carried by the backend; transmission,
 // thin film, thin walls, subsurface, textures and normal maps are rejected.
+constexpr uint32_t kNoTexture = 0xFFFFFFFFu;
 struct OpenPbr {
   float base_weight;
   V3 base_color;
@@ -802,7 +805,12 @@ struct OpenPbr {
   float coat_roughness, coat_roughness_anisotropy, coat_ior, coat_darkening;
   V3 fuzz_color;
   float fuzz_roughness;
+  // texture slots (indices into the scene's textures, kNoTexture for none):
+  // base_color, base_metalness, specular_roughness, fuzz_weight,
+  // fuzz_roughness, emission_color (R/material/open_pbr.rs:66-72)
+  uint32_t tex[6] = {kNoTexture, kNoTexture, kNoTexture, kNoTexture, kNoTexture, kNoTexture};
 };
+enum TexSlot { kTexBaseColor = 0, kTexMetalness, kTexSpecRough, kTexFuzzWeight, kTexFuzzRough, kTexEmission };
 
 inline float modulated_eta(float eta, float w) {  // :871-878
   if (std::fabs(w - 1.0f) < 1.0e-6f) return eta;
@@ -860,6 +868,7 @@ struct PbrBsdf {
   float diffuse_roughness;
   bool front_face;
   Luts luts;
+  float spec_roughness;  // effective_specular_roughness (the sample's cone spread reads it)
 };
 
 // apply_fuzz_roughening / effective_*_roughness (:736-778)
@@ -914,6 +923,7 @@ inline PbrBsdf make_bsdf(const OpenPbr& m, bool front_face, V3 wo_local, const L
   float spec_r = fuzz_roughening(m, with_coat, fw, fr);
   alpha_xy(spec_r, m.specular_roughness_anisotropy, &p.ax, &p.ay);
   alpha_xy(coat_r, m.coat_roughness_anisotropy, &p.cax, &p.cay);
+  p.spec_roughness = spec_r;
   p.base_color = base_color;
   p.base = clampf(m.base_weight, 0.0f, 1.0f);
   p.specular = std::fmax(m.specular_weight, 0.0f);
@@ -941,7 +951,11 @@ inline Luts luts_for(const OpenPbr& m) {
   const bool coat = clampf(m.coat_weight, 0.0f, 1.0f) > 0.0f;
   l.spec = &albedo_lut_for(eta);
   l.coat = coat ? &albedo_lut_for(std::fmax(m.coat_ior, 1.0e-4f)) : &zero_albedo_lut();
-  l.cond = clampf(m.base_metalness, 0.0f, 1.0f) > 0.0f ? &cond_lut() : &zero_cond_lut();
+  // metalness clamp(base_metalness * texel, 0, 1) is positive only for a
+  // non-zero parameter
+  const bool metal = clampf(m.base_metalness, 0.0f, 1.0f) > 0.0f ||
+                     (m.tex[kTexMetalness] != kNoTexture && m.base_metalness != 0.0f);
+  l.cond = metal ? &cond_lut() : &zero_cond_lut();
   l.diel = (eta != 1.0f || coat) ? &diel_lut() : &zero_diel_lut();
   return l;
 }
@@ -1365,6 +1379,7 @@ inline bool bsdf_sample(const PbrBsdf& p, V3 wo, float u_lobe, float ux, float u
   }
   float p_lobe = plv / pr.total;
   if (p_lobe <= 0.0f) return false;
+  out->glossy = lobe != 5;  // delta samples carry DELTA instead (read with !delta)
   if (lobe == 0) {  // sample_coat :1467-1530
     if (coat_smooth(p)) {
       V3 wi = v3(-wo.x, -wo.y, wo.z);

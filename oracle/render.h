@@ -31,6 +31,7 @@
 #include "geometry.h"
 #include "lights.h"
 #include "sampler.h"
+#include "texture.h"
 
 namespace orc {
 
@@ -61,6 +62,7 @@ struct RenderScene {
   std::vector<uint32_t> leaf_of;
   bool has_env = false;
   lt::env::Env env;
+  std::vector<texture::Texture> textures;  // indexed by the OpenPBR texture slots
   uint32_t leaf_node(uint32_t inst, uint32_t tri) const {
     return leaf_base[inst] == lt::kInvalid ? lt::kInvalid : leaf_of[leaf_base[inst] + tri];
   }
@@ -72,7 +74,100 @@ struct Vertex {
   bool front_face;
   uint32_t material;
   uint32_t inst, tri;
+  // uv and its screen-space derivatives, the position and normal partials
+  // (ShadingVertex, R/material/mod.rs:120-170)
+  float u = 0.0f, v = 0.0f, dudx = 0.0f, dvdx = 0.0f, dudy = 0.0f, dvdy = 0.0f;
+  V3 dpdx{0.f, 0.f, 0.f}, dpdy{0.f, 0.f, 0.f}, dndu{0.f, 0.f, 0.f}, dndv{0.f, 0.f, 0.f};
 };
+
+// Ray with its optional differential and its cone (R/math/ray.rs:8-80).
+struct RayCone {
+  float width = 0.0f, spread = 0.0f;
+};
+inline RayCone make_cone(float width, float spread) {  // RayCone::new
+  return RayCone{std::fmax(width, 0.0f), std::fmax(spread, 0.0f)};
+}
+inline float cone_width_at(const RayCone& c, float t) {  // RayCone::width_at
+  if (std::isfinite(t)) return std::fmax(c.width + c.spread * std::fmax(t, 0.0f), 0.0f);
+  return c.width;
+}
+struct Ray {
+  V3 o, d;
+  bool has_diff = false;
+  V3 rxo{0.f, 0.f, 0.f}, ryo{0.f, 0.f, 0.f}, rxd{0.f, 0.f, 0.f}, ryd{0.f, 0.f, 0.f};
+  RayCone cone;
+};
+
+// compute_surface_partials (R/math/mod.rs:112-134) of three values over uvs.
+inline bool surface_partials(V3 a0, V3 a1, V3 a2, const float* uv, V3* dadu, V3* dadv) {
+  V3 d1 = sub(a1, a0), d2 = sub(a2, a0);
+  float du1 = uv[2] - uv[0], dv1 = uv[3] - uv[1], du2 = uv[4] - uv[0], dv2 = uv[5] - uv[1];
+  float det = du1 * dv2 - dv1 * du2;
+  if (std::fabs(det) <= 1.0e-8f) return false;
+  float inv = 1.0f / det;
+  V3 a = smul(sub(mul(dv2, d1), mul(dv1, d2)), inv);
+  V3 b = smul(add(mul(-du2, d1), mul(du1, d2)), inv);
+  if (len_sq(a) == 0.0f || len_sq(b) == 0.0f) return false;
+  *dadu = a;
+  *dadv = b;
+  return true;
+}
+
+inline float finite_clamped_derivative(float x) {  // R/scene/mod.rs:861-867
+  return std::isfinite(x) ? clampf(x, -1.0e8f, 1.0e8f) : 0.0f;
+}
+inline bool v3_finite(V3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+
+// differentials_from_dp (R/scene/mod.rs:787-819)
+inline bool differentials_from_dp(Vertex& v, V3 dpdu, V3 dpdv, V3 dpdx, V3 dpdy) {
+  float a00 = dot(dpdu, dpdu), a01 = dot(dpdu, dpdv), a11 = dot(dpdv, dpdv);
+  float det = difference_of_products(a00, a11, a01, a01);
+  if (std::fabs(det) <= 1.0e-12f || !std::isfinite(det)) return false;
+  float inv = 1.0f / det;
+  float b0x = dot(dpdu, dpdx), b1x = dot(dpdv, dpdx), b0y = dot(dpdu, dpdy), b1y = dot(dpdv, dpdy);
+  v.dpdx = dpdx;
+  v.dpdy = dpdy;
+  v.dudx = finite_clamped_derivative(difference_of_products(a11, b0x, a01, b1x) * inv);
+  v.dvdx = finite_clamped_derivative(difference_of_products(a00, b1x, a01, b0x) * inv);
+  v.dudy = finite_clamped_derivative(difference_of_products(a11, b0y, a01, b1y) * inv);
+  v.dvdy = finite_clamped_derivative(difference_of_products(a00, b1y, a01, b0y) * inv);
+  return true;
+}
+// compute_shading_differentials (R/scene/mod.rs:748-785) with
+// cone_shading_differentials (:821-849) and ray_parameter_to_point (:851-859);
+// n is the face-forwarded geometric normal.  No ray: zero differentials.
+inline void shading_differentials(Vertex& v, V3 n, V3 dpdu, V3 dpdv, const Ray* ray) {
+  if (!ray) return;
+  if (ray->has_diff) {
+    float plane_d = -dot(n, v.p);
+    float rxden = dot(n, ray->rxd), ryden = dot(n, ray->ryd);
+    if (std::fabs(rxden) > 1.0e-8f && std::fabs(ryden) > 1.0e-8f) {
+      float tx = (-dot(n, ray->rxo) - plane_d) / rxden;
+      float ty = (-dot(n, ray->ryo) - plane_d) / ryden;
+      V3 px = add(ray->rxo, mul(tx, ray->rxd)), py = add(ray->ryo, mul(ty, ray->ryd));
+      V3 dpdx = sub(px, v.p), dpdy = sub(py, v.p);
+      if (std::isfinite(tx) && std::isfinite(ty) && v3_finite(dpdx) && v3_finite(dpdy) &&
+          differentials_from_dp(v, dpdu, dpdv, dpdx, dpdy))
+        return;
+    }
+  }
+  float dl2 = len_sq(ray->d);
+  if (dl2 <= 0.0f) return;
+  float t = dot(sub(v.p, ray->o), ray->d) / dl2;
+  if (!std::isfinite(t)) return;
+  float width = cone_width_at(ray->cone, t);
+  if (width <= 0.0f) return;
+  float cos_theta = std::fmax(std::fabs(dot(normalize_or_zero3(ray->d), n)), 1.0e-3f);
+  float projected = width / cos_theta;
+  float dudx = projected / std::fmax(std::sqrt(len_sq(dpdu)), 1.0e-6f);
+  float dvdy = projected / std::fmax(std::sqrt(len_sq(dpdv)), 1.0e-6f);
+  v.dpdx = mul(dudx, dpdu);
+  v.dpdy = mul(dvdy, dpdv);
+  v.dudx = dudx;
+  v.dvdx = 0.0f;
+  v.dudy = 0.0f;
+  v.dvdy = dvdy;
+}
 
 inline V3 mat3_mul_normal(const float* w2l, V3 n) {
   // normal_to_world = Mat3::from_mat4(world_to_local.transpose()): column c is
@@ -89,8 +184,10 @@ inline void triangle_world(const RenderScene& rs, uint32_t inst, uint32_t tri, V
 }
 
 // base_shading_vertex_from_triangle_sample_impl (R/scene/mod.rs:471-544),
-// the fields the integrator reads; `dir` is the incident direction.
-inline Vertex vertex_at(const RenderScene& rs, uint32_t inst, uint32_t tri, float b0, float b1, float b2, V3 dir) {
+// the fields the integrator reads; `dir` is the incident direction, `ray`
+// the ray that hit (null for a light sample's vertex: zero differentials).
+inline Vertex vertex_at(const RenderScene& rs, uint32_t inst, uint32_t tri, float b0, float b1, float b2, V3 dir,
+                        const Ray* ray = nullptr) {
   const Instance& in = rs.geo->inst[inst];
   const Mesh& m = rs.geo->meshes[in.mesh];
   const ShadeMesh& sm = rs.meshes[in.mesh];
@@ -112,29 +209,62 @@ inline Vertex vertex_at(const RenderScene& rs, uint32_t inst, uint32_t tri, floa
   v.ng = face_forward(gn, nd);
   V3 raw_ns = len_sq(sn) > 0.0f ? sn : v.ng;
   v.ns = face_forward(raw_ns, v.ng);
-  // compute_surface_partials (R/math/mod.rs:112-134) over world positions
-  float du1 = uvv(i1, 0) - uvv(i0, 0), dv1 = uvv(i1, 1) - uvv(i0, 1);
-  float du2 = uvv(i2, 0) - uvv(i0, 0), dv2 = uvv(i2, 1) - uvv(i0, 1);
-  float det = du1 * dv2 - dv1 * du2;
-  V3 dpdu;
-  bool ok = false;
-  if (!(std::fabs(det) <= 1.0e-8f)) {
-    float inv = 1.0f / det;
-    V3 dp1 = sub(p1, p0), dp2 = sub(p2, p0);
-    dpdu = smul(sub(mul(dv2, dp1), mul(dv1, dp2)), inv);
-    V3 dpdv = smul(add(mul(-du2, dp1), mul(du1, dp2)), inv);
-    ok = !(len_sq(dpdu) == 0.0f || len_sq(dpdv) == 0.0f);
+  const float uvs[6] = {uvv(i0, 0), uvv(i0, 1), uvv(i1, 0), uvv(i1, 1), uvv(i2, 0), uvv(i2, 1)};
+  // interpolate_vec2: (b0 uv0 + b1 uv1) + b2 uv2
+  v.u = b0 * uvs[0] + b1 * uvs[2] + b2 * uvs[4];
+  v.v = b0 * uvs[1] + b1 * uvs[3] + b2 * uvs[5];
+  // normal partials, flipped with the shading normal
+  if (!surface_partials(n0, n1, n2, uvs, &v.dndu, &v.dndv)) v.dndu = v.dndv = v3(0.f, 0.f, 0.f);
+  if (dot(v.ns, raw_ns) < 0.0f) {
+    v.dndu = neg(v.dndu);
+    v.dndv = neg(v.dndv);
   }
-  if (!ok) dpdu = onb_from_normal(v.ns).t;
+  // compute_surface_partials (R/math/mod.rs:112-134) over world positions,
+  // else the Duff frame's tangent and bitangent
+  V3 dpdu, dpdv;
+  if (!surface_partials(p0, p1, p2, uvs, &dpdu, &dpdv)) {
+    Onb f = onb_from_normal(v.ns);
+    dpdu = f.t;
+    dpdv = f.b;
+  }
   v.frame = onb_from_normal_and_tangent(v.ns, dpdu);
+  if (len_sq(dpdu) == 0.0f) dpdu = v.frame.t;
+  if (len_sq(dpdv) == 0.0f) dpdv = v.frame.b;
+  shading_differentials(v, v.ng, dpdu, dpdv, ray);
   v.wo = normalize_or_zero3(nd);
   v.material = rs.inst_material[inst];
   v.inst = inst;
   v.tri = tri;
   return v;
 }
-inline Vertex shading_vertex(const RenderScene& rs, const SceneHit& h, V3 dir) {
-  return vertex_at(rs, h.inst, h.tri, h.b0, h.b1, h.b2, dir);
+inline Vertex shading_vertex(const RenderScene& rs, const SceneHit& h, const Ray& ray) {
+  return vertex_at(rs, h.inst, h.tri, h.b0, h.b1, h.b2, ray.d, &ray);
+}
+
+// OpenPbrMaterial::*_at (R/material/open_pbr.rs:670-790, le :351-375): a
+// textured parameter is the parameter times Texture::sample_filtered at the
+// vertex's uv with derivatives uv_dx() = (dudx, dvdx), uv_dy() = (dudy, dvdy);
+// the clamps stay where the parameters are read.
+inline Material material_at(const RenderScene& rs, const Material& m, const Vertex& v) {
+  if (m.kind != kMatOpenPbr) return m;
+  Material r = m;
+  const OpenPbr& p = m.pbr;
+  auto fetch = [&](int slot) {
+    return texture::sample_filtered(rs.textures[p.tex[slot]], v.u, v.v, v.dudx, v.dvdx, v.dudy, v.dvdy);
+  };
+  if (p.tex[kTexBaseColor] != kNoTexture) {
+    texture::Px t = fetch(kTexBaseColor);
+    r.pbr.base_color = vmul(p.base_color, v3(t.c[0], t.c[1], t.c[2]));
+  }
+  if (p.tex[kTexMetalness] != kNoTexture) r.pbr.base_metalness = p.base_metalness * fetch(kTexMetalness).c[0];
+  if (p.tex[kTexSpecRough] != kNoTexture) r.pbr.specular_roughness = p.specular_roughness * fetch(kTexSpecRough).c[0];
+  if (p.tex[kTexFuzzWeight] != kNoTexture) r.pbr.fuzz_weight = p.fuzz_weight * fetch(kTexFuzzWeight).c[0];
+  if (p.tex[kTexFuzzRough] != kNoTexture) r.pbr.fuzz_roughness = p.fuzz_roughness * fetch(kTexFuzzRough).c[0];
+  if (p.tex[kTexEmission] != kNoTexture) {
+    texture::Px t = fetch(kTexEmission);
+    r.pbr.emission_color = vmul(p.emission_color, v3(t.c[0], t.c[1], t.c[2]));
+  }
+  return r;
 }
 
 // OpenPbrMaterial::le (R/material/open_pbr.rs:351-375), no emission texture.
@@ -159,6 +289,9 @@ inline bool pbr_sample(const OpenPbr& m, const Luts& luts, const Vertex& v, floa
   if (!bsdf_sample(p, wol, u_lobe, ux, uy, &s)) return false;
   V3 wi = onb_to_world(v.frame, s.wi);
   if (dot(wi, v.ng) <= kGeomCosEps) return false;  // every carried lobe reflects
+  // cone_spread: 2 * effective specular roughness for GLOSSY samples, else
+  // GLOSSY_DIFFUSE_CONE_SPREAD 0.5 (:509-522)
+  s.cone_spread = (s.glossy && !s.delta) ? 2.0f * clampf(p.spec_roughness, 0.0f, 1.0f) : 0.5f;
   *out = s;  // modified_bsdf_sample_weight leaves radiance-mode weights unchanged
   *wi_world = wi;
   return true;
@@ -171,12 +304,14 @@ inline bool mat_may_emit(const Material& m) {
   if (m.kind == kMatOpenPbr) return m.pbr.emission_luminance > 0.0f;  // open_pbr.rs:377-379
   return false;
 }
-inline float mat_max_emission(const Material& m) {
+inline float mat_max_emission(const RenderScene& rs, const Material& m) {
   if (m.kind == kMatEmissive)  // emissive.rs max_emission, no texture
     return std::fmax(lt::vmax_elem(smul(m.color, m.strength)) * 1.0f, 0.0f);
-  if (m.kind == kMatOpenPbr) {  // open_pbr.rs:381-391
+  if (m.kind == kMatOpenPbr) {  // open_pbr.rs:381-391: texture factor = the emission texture's max_value
     if (m.pbr.emission_luminance <= 0.0f) return 0.0f;
-    return std::fmax(lt::vmax_elem(m.pbr.emission_color) * m.pbr.emission_luminance * 1.0f, 0.0f);
+    const uint32_t et = m.pbr.tex[kTexEmission];
+    const float tf = et != kNoTexture ? rs.textures[et].max_value : 1.0f;
+    return std::fmax(lt::vmax_elem(m.pbr.emission_color) * m.pbr.emission_luminance * tf, 0.0f);
   }
   return 0.0f;
 }
@@ -205,11 +340,13 @@ inline bool mat_sample(const Material& m, const Vertex& v, float u_lobe, float u
     s.wi = wi;
     s.pdf = pdf;
     s.delta = false;
+    s.cone_spread = 0.5f;  // DIFFUSE_CONE_SPREAD, normalized_lambert.rs:16
   } else {  // MirrorBsdf::sample (bsdf/mirror.rs:23-39)
     s.weight = m.color;
     s.wi = normalize_or_zero3(v3(-wol.x, -wol.y, wol.z));
     s.pdf = 1.0f;
     s.delta = true;
+    s.cone_spread = 0.0f;  // mirror.rs:134
   }
   V3 wi = onb_to_world(v.frame, s.wi);
   if (dot(wi, v.ng) <= kGeomCosEps) return false;
@@ -282,7 +419,7 @@ inline void build_lights(RenderScene& rs) {
   for (uint32_t i = 0; i < (uint32_t)g.inst.size(); ++i) {
     const Material& m = rs.materials[rs.inst_material[i]];
     if (!mat_may_emit(m)) continue;
-    float me = mat_max_emission(m);
+    float me = mat_max_emission(rs, m);
     if (me <= 0.0f) continue;
     const Mesh& mesh = g.meshes[g.inst[i].mesh];
     const uint32_t ntri = (uint32_t)(mesh.idx.size() / 3);
@@ -399,9 +536,10 @@ inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& v
     if (d2 <= 0.0f) return zero;
     float dist = std::sqrt(d2);
     wi = sdiv(to_light, dist);
-    const Material& lm = rs.materials[rs.inst_material[ln.inst]];
-    if (!mat_may_emit(lm)) return zero;
+    const Material& lm0 = rs.materials[rs.inst_material[ln.inst]];
+    if (!mat_may_emit(lm0)) return zero;
     Vertex lv = vertex_at(rs, ln.inst, ln.tri, b0, b1, b2, wi);
+    const Material lm = material_at(rs, lm0, lv);
     if (!mat_le(lm, lv, &le)) return zero;
     float cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
     if (cos_light <= 0.0f) return zero;
@@ -486,29 +624,67 @@ inline V3 emitted_infinite(const RenderScene& rs, V3 throughput, float bsdf_pdf,
   return smul(vmul(throughput, le), balance_heuristic(bsdf_pdf, light_pdf));
 }
 
-// mis::trace_radiance (R/integrator/mis.rs:19-139).
-inline V3 trace_radiance(const RenderScene& rs, V3 o, V3 d, const PathSampler& ps, uint32_t max_depth) {
+// specular_ray_differential (R/integrator/mod.rs:88-138) for a reflection
+// (every delta lobe carried reflects) with reflected_differential_directions
+// (:140-157) and differential_is_reasonable (:208-219).
+inline bool specular_differential(const Ray& in, const Vertex& v, V3 wi_world, Ray* out) {
+  if (!in.has_diff) return false;
+  V3 wo = v.wo, wi = normalize_or_zero3(wi_world);
+  V3 dndx = add(mul(v.dudx, v.dndu), mul(v.dvdx, v.dndv));
+  V3 dndy = add(mul(v.dudy, v.dndu), mul(v.dvdy, v.dndv));
+  V3 dwodx = sub(neg(in.rxd), wo), dwody = sub(neg(in.ryd), wo);
+  V3 rxo = offset_origin(add(v.p, v.dpdx), v.ng, wi), ryo = offset_origin(add(v.p, v.dpdy), v.ng, wi);
+  V3 n = v.ns;
+  float dwn_dx = dot(dwodx, n) + dot(wo, dndx), dwn_dy = dot(dwody, n) + dot(wo, dndy);
+  float wn = dot(wo, n);
+  V3 rxd = add(sub(wi, dwodx), mul(2.0f, add(mul(wn, dndx), mul(dwn_dx, n))));
+  V3 ryd = add(sub(wi, dwody), mul(2.0f, add(mul(wn, dndy), mul(dwn_dy, n))));
+  const float kMax = 1.0e16f;
+  if (!v3_finite(rxo) || !v3_finite(ryo) || !v3_finite(rxd) || !v3_finite(ryd) || len_sq(rxo) > kMax ||
+      len_sq(ryo) > kMax || len_sq(rxd) > kMax || len_sq(ryd) > kMax)
+    return false;
+  out->rxo = rxo;
+  out->ryo = ryo;
+  out->rxd = rxd;
+  out->ryd = ryd;
+  return true;
+}
+// spawn_scattered_ray (R/integrator/mod.rs:58-75)
+inline Ray spawn_scattered(const Ray& in, float hit_t, const Vertex& v, const BSample& s, V3 wi) {
+  Ray r;
+  r.o = offset_origin(v.p, v.ng, wi);
+  r.d = wi;
+  r.cone = make_cone(cone_width_at(in.cone, hit_t), in.cone.spread + s.cone_spread);
+  if (s.delta) r.has_diff = specular_differential(in, v, wi, &r);
+  return r;
+}
+
+// mis::trace_radiance (R/integrator/mis.rs:19-139) for a camera ray with its
+// differential and cone.
+inline V3 trace_radiance(const RenderScene& rs, const Ray& camera_ray, const PathSampler& ps, uint32_t max_depth) {
   V3 radiance = v3(0.f, 0.f, 0.f), throughput = v3(1.f, 1.f, 1.f);
   SceneHit h;
-  if (!closest_hit(*rs.geo, o, d, INFINITY, &h, nullptr)) return env_le(rs, d);
-  Vertex vtx = shading_vertex(rs, h, d);
-  const Material* mat = &rs.materials[vtx.material];
+  if (!closest_hit(*rs.geo, camera_ray.o, camera_ray.d, INFINITY, &h, nullptr)) return env_le(rs, camera_ray.d);
+  Ray ray = camera_ray;
+  float hit_t = h.t;
+  Vertex vtx = shading_vertex(rs, h, ray);
+  Material mat = material_at(rs, rs.materials[vtx.material], vtx);
   V3 le;
-  if (mat_le(*mat, vtx, &le)) radiance = add(radiance, le);
+  if (mat_le(mat, vtx, &le)) radiance = add(radiance, le);
   const uint32_t rr_start = 4;
   for (uint32_t depth = 0; depth < max_depth; ++depth) {
     const uint32_t base = 2u + depth * 16u;
-    if (!mat_is_pure_emitter(*mat)) {
+    if (!mat_is_pure_emitter(mat)) {
       float us[2];
       ps.s2(base + 3, us);
-      V3 dl = direct_light(rs, *mat, vtx, ps.s1(base + 0), ps.s1(base + 1), ps.s1(base + 2), us[0], us[1]);
+      V3 dl = direct_light(rs, mat, vtx, ps.s1(base + 0), ps.s1(base + 1), ps.s1(base + 2), us[0], us[1]);
       radiance = add(radiance, vmul(throughput, dl));
     }
     BSample s;
     V3 wi;
     float udir[2];
     ps.s2(base + 7, udir);
-    if (!mat_sample(*mat, vtx, ps.s1(base + 5), udir[0], udir[1], &s, &wi)) break;
+    if (!mat_sample(mat, vtx, ps.s1(base + 5), udir[0], udir[1], &s, &wi)) break;
     const bool delta = s.delta;
     throughput = vmul(throughput, s.weight);
     if (depth + 1 >= rr_start) {
@@ -516,19 +692,28 @@ inline V3 trace_radiance(const RenderScene& rs, V3 o, V3 d, const PathSampler& p
       if (ps.s1(base + 13) > q) break;
       throughput = sdiv(throughput, q);
     }
-    V3 no = offset_origin(vtx.p, vtx.ng, wi);
+    const Ray next = spawn_scattered(ray, hit_t, vtx, s, wi);
     SceneHit nh;
-    if (!closest_hit(*rs.geo, no, wi, INFINITY, &nh, nullptr)) {
+    if (!closest_hit(*rs.geo, next.o, next.d, INFINITY, &nh, nullptr)) {
       radiance = add(radiance, emitted_infinite(rs, throughput, s.pdf, delta, wi));
       break;
     }
-    Vertex nv = shading_vertex(rs, nh, wi);
-    const Material* nm = &rs.materials[nv.material];
-    radiance = add(radiance, emitted_from_bsdf_sample(rs, throughput, s.pdf, delta, *mat, vtx, nv, *nm));
+    Vertex nv = shading_vertex(rs, nh, next);
+    const Material nm = material_at(rs, rs.materials[nv.material], nv);
+    radiance = add(radiance, emitted_from_bsdf_sample(rs, throughput, s.pdf, delta, mat, vtx, nv, nm));
     vtx = nv;
     mat = nm;
+    ray = next;
+    hit_t = nh.t;
   }
   return radiance;
+}
+// A ray without differential and with a zero cone (single-path test entry).
+inline V3 trace_radiance(const RenderScene& rs, V3 o, V3 d, const PathSampler& ps, uint32_t max_depth) {
+  Ray r;
+  r.o = o;
+  r.d = d;
+  return trace_radiance(rs, r, ps, max_depth);
 }
 
 }  // namespace orc

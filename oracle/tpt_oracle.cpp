@@ -21,6 +21,7 @@
 #include "geometry.h"
 #include "sampler.h"
 #include "render.h"
+#include "texture.h"
 
 using namespace orc;
 
@@ -372,8 +373,9 @@ extern "C" {
 // then the kind's parameters (TptOpenPbr for OpenPBR).
 struct OrcMaterial {
   uint32_t kind, pad;
-  float raw[34];
+  float raw[40];
 };
+static OpenPbr parse_open_pbr(const OrcMaterial* m);
 
 // Shading data for an oracle scene: per mesh vertex normals / uvs (nullable),
 // per instance material index, and the material table.
@@ -393,6 +395,13 @@ void orc_render_set_mesh(void* h, uint32_t mesh, const float* normals, const flo
 void orc_render_set_instance_material(void* h, uint32_t inst, uint32_t mat) {
   ((RenderScene*)h)->inst_material[inst] = mat;
 }
+// Appends a texture to the scene's table (before orc_render_set_materials:
+// the light tree reads the emission textures' maxima).
+uint32_t orc_render_add_texture(void* h, uint32_t w, uint32_t hh, int channels, const float* px) {
+  RenderScene* r = (RenderScene*)h;
+  r->textures.push_back(texture::from_pixels(w, hh, channels, px));
+  return (uint32_t)(r->textures.size() - 1);
+}
 // Sets the material table and builds the area-light table and light tree.
 void orc_render_set_materials(void* h, const OrcMaterial* m, size_t n) {
   RenderScene* r = (RenderScene*)h;
@@ -402,32 +411,8 @@ void orc_render_set_materials(void* h, const OrcMaterial* m, size_t n) {
     const float* f = m[i].raw;
     d.kind = m[i].kind;
     if (d.kind == kMatOpenPbr) {
-      OpenPbr& o = d.pbr;
-      o.base_weight = f[0];
-      o.base_color = v3(f[1], f[2], f[3]);
-      o.base_diffuse_roughness = f[4];
-      o.base_metalness = f[5];
-      o.specular_weight = f[6];
-      o.specular_color = v3(f[7], f[8], f[9]);
-      o.specular_roughness = f[10];
-      o.specular_ior = f[11];
-      o.specular_roughness_anisotropy = f[12];
-      o.transmission_weight = f[13];
-      o.coat_weight = f[14];
-      o.fuzz_weight = f[15];
-      o.thin_film_weight = f[16];
-      o.emission_luminance = f[17];
-      o.emission_color = v3(f[18], f[19], f[20]);
-      uint32_t tw;
-      std::memcpy(&tw, &f[21], 4);
-      o.thin_walled = tw;
-      o.coat_color = v3(f[22], f[23], f[24]);
-      o.coat_roughness = f[25];
-      o.coat_roughness_anisotropy = f[26];
-      o.coat_ior = f[27];
-      o.coat_darkening = f[28];
-      o.fuzz_color = v3(f[29], f[30], f[31]);
-      o.fuzz_roughness = f[32];
+      d.pbr = parse_open_pbr(&m[i]);
+      const OpenPbr& o = d.pbr;
       d.luts = luts_for(o);
     } else {
       d.color = v3(f[0], f[1], f[2]);
@@ -517,7 +502,7 @@ void orc_render_direct_light(void* h, uint32_t inst, uint32_t tri, const float* 
                              float u_tree, float u_aux, const float* us, float* out) {
   const RenderScene& rs = *(RenderScene*)h;
   Vertex v = vertex_at(rs, inst, tri, b[0], b[1], b[2], v3(dir[0], dir[1], dir[2]));
-  V3 r = direct_light(rs, rs.materials[v.material], v, u_root, u_tree, u_aux, us[0], us[1]);
+  V3 r = direct_light(rs, material_at(rs, rs.materials[v.material], v), v, u_root, u_tree, u_aux, us[0], us[1]);
   out[0] = r.x;
   out[1] = r.y;
   out[2] = r.z;
@@ -531,7 +516,8 @@ void orc_render_emitted(void* h, uint32_t inst, uint32_t tri, const float* b, co
   Vertex v = vertex_at(rs, inst, tri, b[0], b[1], b[2], v3(dir[0], dir[1], dir[2]));
   Vertex lv = vertex_at(rs, linst, ltri, lb[0], lb[1], lb[2], v3(ldir[0], ldir[1], ldir[2]));
   V3 r = emitted_from_bsdf_sample(rs, v3(throughput[0], throughput[1], throughput[2]), bsdf_pdf, delta != 0,
-                                  rs.materials[v.material], v, lv, rs.materials[lv.material]);
+                                  material_at(rs, rs.materials[v.material], v), v, lv,
+                                  material_at(rs, rs.materials[lv.material], lv));
   out[0] = r.x;
   out[1] = r.y;
   out[2] = r.z;
@@ -583,8 +569,9 @@ void orc_render(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t sp
   const RenderScene& rs = *(RenderScene*)h;
   CF3 eye{cam[0], cam[1], cam[2]}, fwd{cam[3], cam[4], cam[5]}, right{cam[6], cam[7], cam[8]},
       up{cam[9], cam[10], cam[11]};
-  const float th = cam[12], exposure = cam[14];
+  const float th = cam[12], spread = cam[13], exposure = cam[14];
   const float rxf = (float)w, ryf = (float)hgt, aspect = rxf / ryf;
+  const float sc = std::fmax(1.0f / std::sqrt((float)(spp < 1 ? 1 : spp)), 0.125f);  // differential_scale
   auto dir_for = [&](float px, float py) {
     float sx = ((2.0f * px / rxf) - 1.0f) * aspect * th;
     float sy = (1.0f - (2.0f * py / ryf)) * th;
@@ -601,8 +588,19 @@ void orc_render(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t sp
       ps.s2(0, u);
       const float px = ((float)x + 0.5f) + 0.5f * tent(u[0]);
       const float py = ((float)y + 0.5f) + 0.5f * tent(u[1]);
+      // generate_ray_differential (R/scene/camera.rs:48-78)
       CF3 d = dir_for(px, py);
-      V3 s = trace_radiance(rs, v3(eye.x, eye.y, eye.z), v3(d.x, d.y, d.z), ps, max_depth);
+      CF3 rxd = normalize_or_zero(add(d, scale(sc, sub(dir_for(px + 1.0f, py), d))));
+      CF3 ryd = normalize_or_zero(add(d, scale(sc, sub(dir_for(px, py + 1.0f), d))));
+      Ray ray;
+      ray.o = v3(eye.x, eye.y, eye.z);
+      ray.d = v3(d.x, d.y, d.z);
+      ray.has_diff = true;
+      ray.rxo = ray.ryo = ray.o;
+      ray.rxd = v3(rxd.x, rxd.y, rxd.z);
+      ray.ryd = v3(ryd.x, ryd.y, ryd.z);
+      ray.cone = make_cone(0.0f, spread * sc);
+      V3 s = trace_radiance(rs, ray, ps, max_depth);
       const float cnt = (float)(si + 1);
       color = add(color, sdiv(sub(s, color), cnt));
     }
@@ -643,6 +641,7 @@ static OpenPbr parse_open_pbr(const OrcMaterial* m) {
   o.coat_darkening = f[28];
   o.fuzz_color = v3(f[29], f[30], f[31]);
   o.fuzz_roughness = f[32];
+  std::memcpy(o.tex, &f[33], sizeof(o.tex));
   return o;
 }
 // Table sizes: albedo (per eta), conductor e / e_avg, dielectric e / e_avg.
@@ -738,6 +737,46 @@ void orc_openpbr_probe(const OrcMaterial* m, int which, const float* wo, float a
   out[0] = r.x;
   out[1] = r.y;
   out[2] = r.z;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Textures (texture.h): the reference's Texture / ScalarTexture.
+extern "C" {
+
+void* orc_texture_new(uint32_t w, uint32_t h, int channels, const float* px) {
+  return new texture::Texture(texture::from_pixels(w, h, channels, px));
+}
+void orc_texture_free(void* t) { delete (texture::Texture*)t; }
+uint32_t orc_texture_levels(void* t) { return (uint32_t)((texture::Texture*)t)->levels.size(); }
+float orc_texture_max_value(void* t) { return ((texture::Texture*)t)->max_value; }
+void orc_texture_level_size(void* t, uint32_t level, uint32_t* w, uint32_t* h) {
+  const texture::Level& l = ((texture::Texture*)t)->levels[level];
+  *w = (uint32_t)l.width;
+  *h = (uint32_t)l.height;
+}
+void orc_texture_level(void* t, uint32_t level, float* out) {
+  const texture::Level& l = ((texture::Texture*)t)->levels[level];
+  std::copy(l.px.begin(), l.px.end(), out);
+}
+// Lookups: q = (u, v, dstdx.x, dstdx.y, dstdy.x, dstdy.y) per query; mode as
+// TPT_TEXTURE_*: 0 filtered (EWA), 1 bilinear, 2 nearest, 3 trilinear,
+// 4 mip-bilinear.  out: channels floats per query.
+void orc_texture_sample(void* th, int mode, const float* q, size_t n, float* out) {
+  const texture::Texture& t = *(texture::Texture*)th;
+  for (size_t i = 0; i < n; ++i) {
+    const float* a = q + 6 * i;
+    texture::Px p;
+    switch (mode) {
+      case 1: p = texture::sample(t, a[0], a[1]); break;
+      case 2: p = texture::sample_nearest(t, a[0], a[1]); break;
+      case 3: p = texture::sample_trilinear(t, a[0], a[1], a[2], a[3], a[4], a[5]); break;
+      case 4: p = texture::sample_mip_bilinear(t, a[0], a[1], a[2], a[3], a[4], a[5]); break;
+      default: p = texture::sample_filtered(t, a[0], a[1], a[2], a[3], a[4], a[5]); break;
+    }
+    for (int k = 0; k < t.channels; ++k) out[t.channels * i + k] = p.c[k];
+  }
 }
 
 }  // extern "C"

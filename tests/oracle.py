@@ -95,6 +95,18 @@ def lib() -> ctypes.CDLL:
         L.orc_openpbr_eval.argtypes = [vp, ctypes.c_int, vp, vp, sz, vp, vp]
         L.orc_openpbr_sample.argtypes = [vp, ctypes.c_int, vp, vp, sz, vp]
         L.orc_openpbr_probe.argtypes = [vp, ctypes.c_int, vp, f32, f32, vp]
+        L.orc_render_add_texture.argtypes = [vp, u32, u32, ctypes.c_int, vp]
+        L.orc_render_add_texture.restype = u32
+        L.orc_texture_new.argtypes = [u32, u32, ctypes.c_int, vp]
+        L.orc_texture_new.restype = vp
+        L.orc_texture_free.argtypes = [vp]
+        L.orc_texture_levels.argtypes = [vp]
+        L.orc_texture_levels.restype = u32
+        L.orc_texture_max_value.argtypes = [vp]
+        L.orc_texture_max_value.restype = f32
+        L.orc_texture_level_size.argtypes = [vp, u32, vp, vp]
+        L.orc_texture_level.argtypes = [vp, u32, vp]
+        L.orc_texture_sample.argtypes = [vp, ctypes.c_int, vp, sz, vp]
         L.orc_closest_hit.argtypes = [vp, vp, sz, vp, vp, ctypes.c_int]
         L.orc_closest_hit.restype = ctypes.c_int
         L.orc_unoccluded.argtypes = [vp, vp, sz, vp, vp, vp, vp, ctypes.c_int]
@@ -187,6 +199,50 @@ def openpbr_probe(material, which, wo=(0.0, 0.0, 1.0), a=0.0, b=0.0):
     woa, out = np.asarray(wo, np.float32), np.zeros(3, np.float32)
     lib().orc_openpbr_probe(_p(m), which, _p(woa), a, b, _p(out))
     return out
+
+
+# ---- textures (oracle/texture.h) --------------------------------------------------------------
+class OracleTexture:
+    """The reference's Texture / ScalarTexture on the CPU: pixels (h, w, 3) or
+    (h, w) float32."""
+
+    def __init__(self, pixels):
+        px = np.ascontiguousarray(pixels, np.float32)
+        self.channels = 3 if px.ndim == 3 else 1
+        self._px = px
+        self.h = lib().orc_texture_new(px.shape[1], px.shape[0], self.channels, _p(px))
+
+    @property
+    def levels(self) -> int:
+        return lib().orc_texture_levels(self.h)
+
+    @property
+    def max_value(self) -> float:
+        return lib().orc_texture_max_value(self.h)
+
+    def level(self, i):
+        w, h = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        lib().orc_texture_level_size(self.h, i, ctypes.byref(w), ctypes.byref(h))
+        out = np.zeros((h.value, w.value, self.channels) if self.channels == 3 else (h.value, w.value), np.float32)
+        lib().orc_texture_level(self.h, i, _p(out))
+        return out
+
+    def sample(self, uv, dstdx=None, dstdy=None, mode=0):
+        """mode: 0 filtered (EWA), 1 bilinear, 2 nearest, 3 trilinear, 4 mip-bilinear."""
+        uv = np.asarray(uv, np.float32).reshape(-1, 2)
+        q = np.zeros((len(uv), 6), np.float32)
+        q[:, 0:2] = uv
+        if dstdx is not None:
+            q[:, 2:4] = np.asarray(dstdx, np.float32).reshape(-1, 2)
+        if dstdy is not None:
+            q[:, 4:6] = np.asarray(dstdy, np.float32).reshape(-1, 2)
+        out = np.zeros((len(uv), self.channels), np.float32)
+        lib().orc_texture_sample(self.h, mode, _p(q), len(uv), _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_texture_free(self.h)
 
 
 HIT_DTYPE = np.dtype([("instance", "<u4"), ("triangle", "<u4"), ("t", "<f4"), ("b", "<f4", 3)])
@@ -326,6 +382,9 @@ class OracleRenderer:
         for i, inst in enumerate(scene.instances):
             lib().orc_render_set_instance_material(self.h, i, inst.material_index)
         self.scene_meshes = scene.meshes
+        for t in getattr(scene, "textures", []):  # before the materials: the light tree reads them
+            px = np.ascontiguousarray(t.pixels, np.float32)
+            lib().orc_render_add_texture(self.h, px.shape[1], px.shape[0], 3 if px.ndim == 3 else 1, _p(px))
         self.recs = records(materials)
         lib().orc_render_set_materials(self.h, ctypes.addressof(self.recs), len(materials))
         env = getattr(scene, "environment", None)

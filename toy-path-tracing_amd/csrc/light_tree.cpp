@@ -281,7 +281,17 @@ bool build_light_tree(const TptSceneDesc& desc, std::vector<TptLightNode>* out, 
       emax = fmx(fmx(c.x, fmx(c.y, c.z)) * 1.0f, 0.0f);
     } else if (m.kind == TPT_MATERIAL_OPEN_PBR && m.u.open_pbr.emission_luminance > 0.0f) {
       const float* ec = m.u.open_pbr.emission_color;
-      emax = fmx(fmx(ec[0], fmx(ec[1], ec[2])) * m.u.open_pbr.emission_luminance * 1.0f, 0.0f);
+      // texture factor: the emission texture's level-0 maximum (open_pbr.rs:381-391)
+      float tf = 1.0f;
+      const uint32_t et = m.u.open_pbr.emission_color_texture;
+      if (et != TPT_INVALID_INDEX) {
+        if (et >= desc.texture_count || !desc.textures) {
+          *err = "light tree: emission_color_texture index out of range";
+          return false;
+        }
+        tf = texture_max_value(desc.textures[et]);
+      }
+      emax = fmx(fmx(ec[0], fmx(ec[1], ec[2])) * m.u.open_pbr.emission_luminance * tf, 0.0f);
     } else {
       continue;
     }

@@ -55,12 +55,13 @@ __device__ __forceinline__ bool closest(const SceneView& sc, F3 o, F3 d, Lane& L
 
 // direct_light_mis_contribution with the shadow ray traced by the lane
 // traversal (closest hit, then the target test).
-template <bool kTwoLevel>
+template <bool kTwoLevel, bool kTex>
 __device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView& sv, const TptMaterial& m,
                                            const Vtx& vtx, float u_root, float u_tree, float us0, float us1,
-                                           Lane& L, LaneStack& st) {
+                                           Lane& L, LaneStack& st, F3* dir = nullptr) {
   NeeSample ns;
-  if (!nee_sample(sv, m, vtx, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
+  if (!nee_sample<kTex>(sv, m, vtx, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
+  if (dir) *dir = ns.dir;
   uint32_t hi, ht;
   float tt, hb0, hb1, hb2;
   if (closest<kTwoLevel>(sc, ns.origin, ns.dir, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2) &&
@@ -70,40 +71,56 @@ __device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView&
 }
 
 // Path-replay adjoint state of one sample: the path's radiance from the
-// primal pass, dLoss/dL of this sample (dLoss/dI_pixel * exposure / spp), and
-// the block's gradient accumulators (3 per material, LDS).
+// primal pass, dLoss/dL of this sample (dLoss/dI_pixel * exposure / spp), the
+// block's gradient accumulators (3 per material, LDS) and the texel gradient
+// array (laid out like the texels; null when not requested).
 template <typename Acc>
 struct Adjoint {
   F3 total;
   F3 w;
   Acc* grad;
+  float* texgrad;
 };
 
 // mis::trace_radiance (R/integrator/mis.rs:19-139) for one camera ray.  With
+// kTex the ray's differential and cone follow the path
+// (spawn_scattered_ray) and the materials' textures are applied at every
+// vertex.  With
 // kAdj, the replay pass of path-replay backpropagation: at every
 // NormalizedLambert vertex, every contribution collected from its NEE
 // onward carries the vertex's albedo exactly once (NEE's f = rho / pi, the
 // BSDF sample's weight = rho; the light-tree pmf, MIS weights and pdfs do not
 // depend on rho), so dL/drho_c += (L_total - L_before)_c / rho_c.  Russian
-// roulette probabilities are held fixed (detached).
-template <bool kTwoLevel, bool kAdj, typename Acc>
-__device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& sv, F3 eye, F3 dir,
+// roulette probabilities are held fixed (detached).  At an OpenPBR vertex
+// whose BSDF is affine in the base colour c (base_color_linear), channel k of
+// a contribution that went through the BSDF value f(w) changes by
+// (df_k/dc_k)(w) / f_k(w) per unit of c_k: the vertex's NEE contribution at
+// the light direction and everything collected after it (the BSDF-sampled
+// continuation) at the sampled direction, sampling and MIS weights held
+// fixed; c = parameter * texture value, so dL/dparameter = dL/dc * texture
+// value and dL/dtexel = dL/dc * parameter * d(lookup)/d(texel).
+template <bool kTwoLevel, bool kAdj, bool kTex, typename Acc>
+__device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& sv, const RayDiff& camera_ray,
                                          const smp::PathSampler& ps, uint32_t max_depth, Lane& L, LaneStack& st,
                                          bool& overflow, const Adjoint<Acc>& adj) {
   F3 radiance = f3(0.0f, 0.0f, 0.0f), thr = f3(1.0f, 1.0f, 1.0f);
   uint32_t hi, ht;
   float tt, hb0, hb1, hb2;
-  if (!closest<kTwoLevel>(sc, eye, dir, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) {
+  if (!closest<kTwoLevel>(sc, camera_ray.o, camera_ray.d, L, st, &hi, &ht, &tt, &hb0, &hb1, &hb2)) {
     overflow |= st.overflow;
-    return env_le(sv, dir);  // infinite_light_le of the camera ray
+    return env_le(sv, camera_ray.d);  // infinite_light_le of the camera ray
   }
   overflow |= st.overflow;
-  Vtx vtx = shading_vertex(sv, hi, ht, hb0, hb1, hb2, dir);
+  RayDiff ray = camera_ray;
+  float hit_t = tt;
+  VtxTex tx;
+  Vtx vtx = shading_vertex_t<kTex>(sv, hi, ht, hb0, hb1, hb2, ray.d, &ray, &tx);
+  VtxMat<kTex> vm(sv, vtx.material, &tx);
   F3 le;
-  if (mat_le(sv.mats[vtx.material], vtx, &le)) radiance = add(radiance, le);
+  if (mat_le(vm.get(), vtx, &le)) radiance = add(radiance, le);
   for (uint32_t depth = 0; depth < max_depth; ++depth) {
     const uint32_t base = smp::kVertexBase + depth * smp::kVertexStride;
-    const TptMaterial& m = sv.mats[vtx.material];
+    const TptMaterial& m = vm.get();
     if (kAdj && m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {
       const F3 rest = sub(adj.total, radiance);
       Acc* g = adj.grad + 3 * vtx.material;
@@ -112,19 +129,44 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
       if (rho[1] != 0.0f) atomicAdd(&g[1], (Acc)(adj.w.y * rest.y / rho[1]));
       if (rho[2] != 0.0f) atomicAdd(&g[2], (Acc)(adj.w.z * rest.z / rho[2]));
     }
+    F3 nee_c = f3(0.0f, 0.0f, 0.0f), nee_dir = nee_c;
     if (m.kind != TPT_MATERIAL_EMISSIVE) {  // should_sample_direct_light: not a pure emitter
       float us[2];
       ps.s2(base + 3, us);
-      const F3 dl =
-          direct_light<kTwoLevel>(sc, sv, m, vtx, ps.s1(base + 0), ps.s1(base + 1), us[0], us[1], L, st);
+      const F3 dl = direct_light<kTwoLevel, kTex>(sc, sv, m, vtx, ps.s1(base + 0), ps.s1(base + 1), us[0], us[1], L,
+                                                  st, kAdj ? &nee_dir : nullptr);
       overflow |= st.overflow;
-      radiance = add(radiance, hadamard(thr, dl));
+      nee_c = hadamard(thr, dl);
+      radiance = add(radiance, nee_c);
     }
     Sample s;
     F3 wi;
     float ud[2];
     ps.s2(base + 7, ud);
-    if (!mat_sample(sv.tab, m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi)) break;
+    const bool sampled = mat_sample(sv.tab, m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi);
+    if (kAdj && base_color_linear(m)) {
+      const F3 rest = sub(adj.total, radiance);  // collected after this vertex's NEE
+      F3 g = f3(0.0f, 0.0f, 0.0f);
+      if (len2(nee_c) > 0.0f)
+        g = hadamard(nee_c, base_color_ratio(sv.tab, m, vtx, normalize_or_zero(to_frame(vtx.frame, nee_dir))));
+      if (sampled && !s.delta) g = add(g, hadamard(rest, base_color_ratio(sv.tab, m, vtx, s.wi)));
+      g = hadamard(g, adj.w);  // dL/dc of this sample
+      const TptOpenPbr& p0 = sv.mats[vtx.material].u.open_pbr;  // the parameters, untextured
+      F3 tv = f3(1.0f, 1.0f, 1.0f);
+      if (kTex && p0.base_color_texture != TPT_INVALID_INDEX) {
+        const tex::V2 uv = tex::v2(tx.u, tx.v), dx = tex::v2(tx.dudx, tx.dvdx), dy = tex::v2(tx.dudy, tx.dvdy);
+        const pt::GpuTexture& t = sv.tex.tex[p0.base_color_texture];
+        tv = tex::filtered<3>(sv.tex, t, uv, dx, dy);
+        if (adj.texgrad)
+          tex::scatter_filtered<3>(sv.tex, t, uv, dx, dy, hadamard(g, rgb3(p0.base_color)), adj.texgrad);
+      }
+      Acc* gm = adj.grad + 3 * vtx.material;
+      const F3 gp = hadamard(g, tv);
+      if (gp.x != 0.0f) atomicAdd(&gm[0], (Acc)gp.x);
+      if (gp.y != 0.0f) atomicAdd(&gm[1], (Acc)gp.y);
+      if (gp.z != 0.0f) atomicAdd(&gm[2], (Acc)gp.z);
+    }
+    if (!sampled) break;
     thr = hadamard(thr, s.weight);
     if (depth + 1 >= 4) {
       const float q = clampf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.05f, 1.0f);
@@ -138,14 +180,36 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
       break;
     }
     overflow |= st.overflow;
-    const Vtx nv = shading_vertex(sv, hi, ht, hb0, hb1, hb2, wi);
-    radiance = add(radiance, emitted(sv, thr, s.pdf, s.delta, m, vtx.p, vtx.ns, nv, sv.mats[nv.material]));
+    RayDiff next;
+    if (kTex) next = spawn_scattered(ray, hit_t, vtx, tx, s, no, wi);
+    VtxTex ntx;
+    const Vtx nv = shading_vertex_t<kTex>(sv, hi, ht, hb0, hb1, hb2, wi, &next, &ntx);
+    const VtxMat<kTex> nvm(sv, nv.material, &ntx);
+    radiance = add(radiance, emitted(sv, thr, s.pdf, s.delta, m, vtx.p, vtx.ns, nv, nvm.get()));
     vtx = nv;
+    vm = nvm;
+    if (kTex) {
+      tx = ntx;
+      ray = next;
+      hit_t = tt;
+    }
   }
   return radiance;
 }
 
-template <bool kTwoLevel>
+// The camera ray of a pixel sample: with its differential and cone when the
+// path tracer carries them.
+template <bool kTex>
+__device__ __forceinline__ RayDiff camera_ray(const CamView& cam, const smp::PathSampler& ps, uint32_t x, uint32_t y,
+                                              uint32_t w, uint32_t h, uint32_t spp) {
+  if (kTex) return camera_ray_diff(cam, ps, x, y, w, h, spp);
+  RayDiff r;
+  r.o = cam.eye;
+  r.d = camera_dir(cam, ps, x, y, w, h);
+  return r;
+}
+
+template <bool kTwoLevel, bool kTex>
 __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
                                                         uint32_t h, uint32_t spp, uint32_t max_depth,
                                                         float* __restrict__ rgb, uint32_t* __restrict__ status) {
@@ -166,8 +230,8 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView 
   F3 color = f3(0.0f, 0.0f, 0.0f);
   for (uint32_t si = 0; si < spp; ++si) {
     const smp::PathSampler ps{zf, x, y, si};
-    const F3 dir = camera_dir(cam, ps, x, y, w, h);
-    const F3 radiance = trace_path<kTwoLevel, false>(sc, sv, cam.eye, dir, ps, max_depth, L, st, overflow, none);
+    const RayDiff ray = camera_ray<kTex>(cam, ps, x, y, w, h, spp);
+    const F3 radiance = trace_path<kTwoLevel, false, kTex>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
     const float cnt = (float)(si + 1);
     color = add(color, divs(sub(radiance, color), cnt));
   }
@@ -181,11 +245,12 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView 
 // Path-replay adjoint of the render loop: per pixel sample a primal pass
 // (its radiance) and a replay pass that accumulates dLoss/d(albedo) into
 // the block's LDS sums; each block writes its sums to `partials`.
-template <bool kTwoLevel, typename Acc>
+template <bool kTwoLevel, bool kTex, typename Acc>
 __global__ __launch_bounds__(kBlock) void adjoint_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
                                                          uint32_t h, uint32_t spp, uint32_t max_depth,
                                                          const float* __restrict__ dimg, Acc* __restrict__ partials,
-                                                         uint32_t n_grad, uint32_t* __restrict__ status) {
+                                                         uint32_t n_grad, float* __restrict__ texgrad,
+                                                         uint32_t* __restrict__ status) {
   __shared__ uint32_t lds[kLdsDepth * kBlock];
   extern __shared__ __align__(16) unsigned char dyn[];
   Acc* grad = reinterpret_cast<Acc*>(dyn);
@@ -207,12 +272,13 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(SceneView sc, ShadeView
     Adjoint<Acc> adj;
     adj.w = f3(dimg[3 * pix] * scale, dimg[3 * pix + 1] * scale, dimg[3 * pix + 2] * scale);
     adj.grad = grad;
+    adj.texgrad = texgrad;
     const Adjoint<Acc> none{};
     for (uint32_t si = 0; si < spp; ++si) {
       const smp::PathSampler ps{zf, x, y, si};
-      const F3 dir = camera_dir(cam, ps, x, y, w, h);
-      adj.total = trace_path<kTwoLevel, false>(sc, sv, cam.eye, dir, ps, max_depth, L, st, overflow, none);
-      (void)trace_path<kTwoLevel, true>(sc, sv, cam.eye, dir, ps, max_depth, L, st, overflow, adj);
+      const RayDiff ray = camera_ray<kTex>(cam, ps, x, y, w, h, spp);
+      adj.total = trace_path<kTwoLevel, false, kTex>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
+      (void)trace_path<kTwoLevel, true, kTex>(sc, sv, ray, ps, max_depth, L, st, overflow, adj);
     }
     if (overflow) atomicOr(status, 1u);
   }
@@ -261,7 +327,8 @@ bool same_ggx_inputs(const TptMaterial& a, const TptMaterial& b) {
   if (a.kind != TPT_MATERIAL_OPEN_PBR) return true;
   const TptOpenPbr &x = a.u.open_pbr, &y = b.u.open_pbr;
   return x.specular_ior == y.specular_ior && x.specular_weight == y.specular_weight &&
-         x.coat_weight == y.coat_weight && x.coat_ior == y.coat_ior && x.base_metalness == y.base_metalness;
+         x.coat_weight == y.coat_weight && x.coat_ior == y.coat_ior && x.base_metalness == y.base_metalness &&
+         x.base_metalness_texture == y.base_metalness_texture;
 }
 
 // Material BSDF queries in the shading frame (tpt_material_bsdf).
@@ -374,6 +441,7 @@ extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb
     cam.up = F3{c.image_up[0], c.image_up[1], c.image_up[2]};
     cam.tan_half = c.tan_half_fov_y;
     cam.exposure = c.exposure;
+    cam.spread = c.pixel_spread;
     const pt::ShadeView sv = pt::shade_view_of(s);
     const SceneView v = view_of(s);
     const size_t n = (size_t)w * h;
@@ -387,12 +455,11 @@ extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb
     }
     const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
     const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
-    if (two)
-      hipLaunchKernelGGL(pt::render_kernel<true>, grid, block, 0, d->stream, v, sv, cam, w, h,
-                         desc->samples_per_pixel, desc->max_depth, drgb, d->status);
-    else
-      hipLaunchKernelGGL(pt::render_kernel<false>, grid, block, 0, d->stream, v, sv, cam, w, h,
-                         desc->samples_per_pixel, desc->max_depth, drgb, d->status);
+    const bool tex = s->texture_count > 0;
+    auto kernel = two ? (tex ? pt::render_kernel<true, true> : pt::render_kernel<true, false>)
+                      : (tex ? pt::render_kernel<false, true> : pt::render_kernel<false, false>);
+    hipLaunchKernelGGL(kernel, grid, block, 0, d->stream, v, sv, cam, w, h, desc->samples_per_pixel, desc->max_depth,
+                       drgb, d->status);
     if (!hip_ok(hipGetLastError(), "render_kernel launch")) return TPT_ERROR_DEVICE;
     if (!dev_ptr) {
       uint32_t status = 0;
@@ -423,7 +490,8 @@ bool same_emission(const TptMaterial& a, const TptMaterial& b) {
     return std::memcmp(&a.u.emissive, &b.u.emissive, sizeof(a.u.emissive)) == 0;
   if (a.kind == TPT_MATERIAL_OPEN_PBR)
     return a.u.open_pbr.emission_luminance == b.u.open_pbr.emission_luminance &&
-           std::memcmp(a.u.open_pbr.emission_color, b.u.open_pbr.emission_color, 12) == 0;
+           std::memcmp(a.u.open_pbr.emission_color, b.u.open_pbr.emission_color, 12) == 0 &&
+           a.u.open_pbr.emission_color_texture == b.u.open_pbr.emission_color_texture;
   return true;
 }
 
@@ -435,11 +503,13 @@ pt::CamView cam_view(const TptCamera& c) {
   cam.up = F3{c.image_up[0], c.image_up[1], c.image_up[2]};
   cam.tan_half = c.tan_half_fov_y;
   cam.exposure = c.exposure;
+  cam.spread = c.pixel_spread;
   return cam;
 }
 
 template <typename Acc>
-int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, float* d_albedo, bool dev_ptr) {
+int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, float* d_albedo, float* d_texels,
+                     bool dev_ptr) {
   TptDevice* d = s->dev;
   const uint32_t w = desc->width, h = desc->height;
   const size_t n = (size_t)w * h;
@@ -448,11 +518,18 @@ int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, 
   const size_t nb_part = align_up((size_t)blocks * n_grad * sizeof(Acc));
   const size_t nb_img = dev_ptr ? 0 : align_up(n * 12);
   const size_t nb_out = dev_ptr ? 0 : align_up((size_t)n_grad * 4);
+  // texel gradients: an array laid out like the texels, and (host outputs)
+  // staging for the packed level-0 gradients
+  const size_t n_tg = d_texels ? texel_grad_floats(s) : 0;
+  const size_t nb_tbuf = d_texels ? align_up(texel_floats(s) * 4) : 0;
+  const size_t nb_tout = (d_texels && !dev_ptr) ? align_up(n_tg * 4) : 0;
   DeviceBuffer tmp;
-  if (!tmp.ensure(nb_part + nb_img + nb_out)) return TPT_ERROR_DEVICE;
+  if (!tmp.ensure(nb_part + nb_img + nb_out + nb_tbuf + nb_tout)) return TPT_ERROR_DEVICE;
   Acc* partials = (Acc*)tmp.ptr;
   const float* ddimg = dimg;
   float* dgrad = d_albedo;
+  float* tbuf = d_texels ? (float*)((char*)tmp.ptr + nb_part + nb_img + nb_out) : nullptr;
+  float* tout = d_texels ? (dev_ptr ? d_texels : (float*)((char*)tmp.ptr + nb_part + nb_img + nb_out + nb_tbuf)) : nullptr;
   if (!dev_ptr) {
     ddimg = (const float*)((char*)tmp.ptr + nb_part);
     dgrad = (float*)((char*)tmp.ptr + nb_part + nb_img);
@@ -460,24 +537,28 @@ int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, 
       return TPT_ERROR_DEVICE;
     if (!hip_ok(hipMemsetAsync(d->status, 0, 4, d->stream), "hipMemsetAsync(status)")) return TPT_ERROR_DEVICE;
   }
+  if (tbuf && !hip_ok(hipMemsetAsync(tbuf, 0, nb_tbuf, d->stream), "hipMemsetAsync(texel gradients)"))
+    return TPT_ERROR_DEVICE;
   const pt::CamView cam = cam_view(desc->camera);
   const pt::ShadeView sv = pt::shade_view_of(s);
   const SceneView v = view_of(s);
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const size_t lds_bytes = (size_t)n_grad * sizeof(Acc);
-  if (two)
-    hipLaunchKernelGGL((pt::adjoint_kernel<true, Acc>), dim3(blocks), dim3(kBlock), lds_bytes, d->stream, v, sv, cam, w,
-                       h, desc->samples_per_pixel, desc->max_depth, ddimg, partials, n_grad, d->status);
-  else
-    hipLaunchKernelGGL((pt::adjoint_kernel<false, Acc>), dim3(blocks), dim3(kBlock), lds_bytes, d->stream, v, sv, cam,
-                       w, h, desc->samples_per_pixel, desc->max_depth, ddimg, partials, n_grad, d->status);
+  const bool tex = s->texture_count > 0;
+  auto kernel = two ? (tex ? pt::adjoint_kernel<true, true, Acc> : pt::adjoint_kernel<true, false, Acc>)
+                    : (tex ? pt::adjoint_kernel<false, true, Acc> : pt::adjoint_kernel<false, false, Acc>);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), lds_bytes, d->stream, v, sv, cam, w, h,
+                     desc->samples_per_pixel, desc->max_depth, ddimg, partials, n_grad, tbuf, d->status);
   hipLaunchKernelGGL((pt::adjoint_reduce<Acc>), dim3((n_grad + 255) / 256), dim3(256), 0, d->stream,
                      (const Acc*)partials, blocks, n_grad, dgrad);
   if (!hip_ok(hipGetLastError(), "adjoint launch")) return TPT_ERROR_DEVICE;
+  if (tbuf && !texel_grads_to_level0(s, tbuf, tout, d->stream)) return TPT_ERROR_DEVICE;
   // the scratch buffer is released when this call returns: wait for the kernels
   uint32_t status = 0;
   if (!dev_ptr &&
       (!hip_ok(hipMemcpyAsync(d_albedo, dgrad, (size_t)n_grad * 4, hipMemcpyDeviceToHost, d->stream), "hipMemcpyAsync") ||
+       (d_texels && n_tg &&
+        !hip_ok(hipMemcpyAsync(d_texels, tout, n_tg * 4, hipMemcpyDeviceToHost, d->stream), "hipMemcpyAsync")) ||
        !hip_ok(hipMemcpyAsync(&status, d->status, 4, hipMemcpyDeviceToHost, d->stream), "hipMemcpyAsync")))
     return TPT_ERROR_DEVICE;
   if (!hip_ok(hipStreamSynchronize(d->stream), "hipStreamSynchronize")) return TPT_ERROR_DEVICE;
@@ -490,8 +571,9 @@ int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, 
 
 }  // namespace
 
-extern "C" int32_t tpt_render_adjoint(TptScene* s, const TptRenderDesc* desc, const float* dloss_dimage,
-                                      float* d_albedo, uint32_t flags) {
+namespace {
+int32_t render_adjoint(TptScene* s, const TptRenderDesc* desc, const float* dloss_dimage, float* d_albedo,
+                       float* d_texels, uint32_t flags) {
   try {
     if (!s || !desc || !dloss_dimage || !d_albedo) {
       set_error("tpt_render_adjoint: null argument");
@@ -508,18 +590,39 @@ extern "C" int32_t tpt_render_adjoint(TptScene* s, const TptRenderDesc* desc, co
     }
     if (!render_materials_ok(s, "tpt_render_adjoint")) return TPT_ERROR_BAD_ARGUMENT;
     if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    const size_t n_tg = d_texels ? texel_grad_floats(s) : 0;
     if ((uint64_t)w * h == 0) {
-      if (flags & TPT_FLAG_DEVICE_POINTERS) return hip_ok(hipMemsetAsync(d_albedo, 0, 12 * (size_t)s->material_count, s->dev->stream), "hipMemsetAsync") ? TPT_OK : TPT_ERROR_DEVICE;
+      if (flags & TPT_FLAG_DEVICE_POINTERS)
+        return hip_ok(hipMemsetAsync(d_albedo, 0, 12 * (size_t)s->material_count, s->dev->stream), "hipMemsetAsync") &&
+                       (!n_tg || hip_ok(hipMemsetAsync(d_texels, 0, 4 * n_tg, s->dev->stream), "hipMemsetAsync"))
+                   ? TPT_OK
+                   : TPT_ERROR_DEVICE;
       std::memset(d_albedo, 0, 12 * (size_t)s->material_count);
+      if (n_tg) std::memset(d_texels, 0, 4 * n_tg);
       return TPT_OK;
     }
     const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
-    return (flags & TPT_FLAG_GRAD_FP64) ? adjoint_impl<double>(s, desc, dloss_dimage, d_albedo, dev_ptr)
-                                        : adjoint_impl<float>(s, desc, dloss_dimage, d_albedo, dev_ptr);
+    return (flags & TPT_FLAG_GRAD_FP64) ? adjoint_impl<double>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr)
+                                        : adjoint_impl<float>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr);
   } catch (const std::exception& e) {
     set_error(std::string("tpt_render_adjoint: ") + e.what());
     return TPT_ERROR;
   }
+}
+}  // namespace
+
+extern "C" int32_t tpt_render_adjoint(TptScene* s, const TptRenderDesc* desc, const float* dloss_dimage,
+                                      float* d_albedo, uint32_t flags) {
+  return render_adjoint(s, desc, dloss_dimage, d_albedo, nullptr, flags);
+}
+
+extern "C" int32_t tpt_render_adjoint_texels(TptScene* s, const TptRenderDesc* desc, const float* dloss_dimage,
+                                             float* d_albedo, float* d_texels, uint32_t flags) {
+  if (s && !d_texels) {
+    set_error("tpt_render_adjoint_texels: null texel gradient output");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  return render_adjoint(s, desc, dloss_dimage, d_albedo, d_texels, flags);
 }
 
 extern "C" int32_t tpt_scene_set_materials(TptScene* s, const TptMaterial* materials, uint32_t count) {
@@ -538,6 +641,13 @@ extern "C" int32_t tpt_scene_set_materials(TptScene* s, const TptMaterial* mater
                   " changes kind or emission (the light tree is built from emission: create a new scene)");
         return TPT_ERROR_BAD_ARGUMENT;
       }
+    for (uint32_t i = 0; i < count; ++i) {
+      const std::string terr = material_texture_error(materials[i], i, s->host_textures);
+      if (!terr.empty()) {
+        set_error("tpt_scene_set_materials: " + terr);
+        return TPT_ERROR_BAD_ARGUMENT;
+      }
+    }
     if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
     if (count && !hip_ok(hipMemcpyAsync(s->mats, materials, count * sizeof(TptMaterial), hipMemcpyHostToDevice,
                                         s->dev->stream),

@@ -9,6 +9,7 @@
 
 #include "light_tree.h"
 #include "sampler.h"
+#include "texture.h"
 #include "tpt_device.h"
 #include "tpt_internal.h"
 #include "traverse.h"
@@ -396,6 +397,7 @@ struct Bsdf {
   float base, specular, ax, ay, spec_eta, trans_eta, metal;
   float coat, cax, cay, coat_eta;
   float fuzz, fuzz_rough, diff_rough;
+  float spec_rough;  // effective specular roughness (a glossy sample's cone spread)
   bool front;
   const float* spec_tab;  // directional-albedo table of spec_eta
   const float* coat_tab;  // of coat_eta
@@ -466,6 +468,7 @@ __device__ __forceinline__ Bsdf make_bsdf(const Tables& tb, uint32_t mi, const T
   const float spec_r = fuzz_roughening(m, lerpf(r_base, coat_rough, clampf(m.coat_weight, 0.0f, 1.0f)), fw, fr);
   alpha_xy(spec_r, m.specular_roughness_anisotropy, &p.ax, &p.ay);
   alpha_xy(coat_r, m.coat_roughness_anisotropy, &p.cax, &p.cay);
+  p.spec_rough = spec_r;
   p.base = clampf(m.base_weight, 0.0f, 1.0f);
   p.specular = fmaxf(m.specular_weight, 0.0f);
   p.spec_color = clamp01(f3(m.specular_color[0], m.specular_color[1], m.specular_color[2]));
@@ -734,6 +737,18 @@ __device__ __noinline__ F3 bsdf_eval(const Bsdf& p, F3 wo, F3 wi) {
   }
   return total;
 }
+// d eval / d base_color (diagonal) for a material whose coat does not darken
+// (coat weight or coat_darkening 0) and whose metalness is 0: then channel k
+// of the base colour enters eval only through the diffuse lobe's weight
+// below_transmission * base * base_color (layer_weights) times the unit-albedo
+// EON lobe, so the derivative is that weight at base colour 1 times the lobe.
+__device__ __noinline__ F3 bsdf_eval_dbase(const Bsdf& p, F3 wo, F3 wi) {
+  if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) return f3s(0.0f);
+  Bsdf q = p;
+  q.base_color = f3s(1.0f);
+  const LayerW w = layer_weights(q, wo, &wi);
+  return muls(w.diff, f_eon1(p.diff_rough, wo, wi));
+}
 // OpenPbrBsdf::pdf (:261-293)
 __device__ __noinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {
   if (!(wo.z > 0.0f)) return 0.0f;
@@ -774,6 +789,8 @@ struct Sample {
   F3 weight, wi;
   float pdf;
   bool delta;
+  bool glossy;  // BsdfFlags::GLOSSY: a rough coat, metal, specular or fuzz lobe (read with !delta)
+  float cone;   // MaterialSample::cone_spread
 };
 __device__ __forceinline__ bool finalize(const Bsdf& p, F3 wo, F3 wi, Sample* out) {  // :1958-1999
   const float pdf = bsdf_pdf(p, wo, wi);
@@ -872,6 +889,7 @@ __device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, flo
   }
   const float p_lobe = plv / pr.total;
   if (p_lobe <= 0.0f) return false;
+  out->glossy = lobe != 5;
   const F3 mirror = f3(-wo.x, -wo.y, wo.z);
   if (lobe == 0) {
     if (coat_smooth(p)) {
@@ -921,7 +939,8 @@ struct ShadeView {
   uint32_t light_count;
   float w_tree, w_env;  // light-sampler category weights: root flux, environment power (0: absent)
   EnvView env;
-  Tables tab;  // GGX tables of the OpenPBR materials
+  Tables tab;   // GGX tables of the OpenPBR materials
+  TexView tex;  // textures of the OpenPBR texture slots
 };
 
 struct Vtx {
@@ -943,9 +962,98 @@ __device__ __forceinline__ F3 xform_normal(const float* n, F3 v) {  // Mat3::mul
   return add(add(scl(v.x, f3(n[0], n[1], n[2])), scl(v.y, f3(n[3], n[4], n[5]))), scl(v.z, f3(n[6], n[7], n[8])));
 }
 
-// base_shading_vertex_from_triangle_sample_impl (R/scene/mod.rs:471-544)
-__device__ __forceinline__ Vtx shading_vertex(const ShadeView& sv, uint32_t inst, uint32_t tri, float b0, float b1,
-                                              float b2, F3 dir) {
+// A ray's differential and cone (Ray / RayDifferential / RayCone,
+// R/math/ray.rs:8-80), carried by the path tracers when the scene has
+// textures.
+struct RayDiff {
+  F3 o, d;
+  bool has;  // a differential is present
+  F3 rxo, ryo, rxd, ryd;
+  float cw, cs;  // cone width and spread angle
+};
+// The shading vertex's texture inputs: uv, its screen-space derivatives and
+// the position / normal partials (ShadingVertex, R/material/mod.rs:120-170).
+struct VtxTex {
+  float u, v, dudx, dvdx, dudy, dvdy;
+  F3 dpdx, dpdy, dndu, dndv;
+};
+
+__device__ __forceinline__ float cone_width_at(float cw, float cs, float t) {  // RayCone::width_at
+  return __builtin_isfinite(t) ? fmaxf(cw + cs * fmaxf(t, 0.0f), 0.0f) : cw;
+}
+__device__ __forceinline__ bool finite3(F3 a) {
+  return __builtin_isfinite(a.x) && __builtin_isfinite(a.y) && __builtin_isfinite(a.z);
+}
+// compute_surface_partials (R/math/mod.rs:112-134) of three values over uvs
+__device__ __forceinline__ bool surface_partials(F3 a0, F3 a1, F3 a2, float du1, float dv1, float du2, float dv2,
+                                                 F3* dadu, F3* dadv) {
+  const float det = du1 * dv2 - dv1 * du2;
+  if (fabsf(det) <= 1.0e-8f) return false;
+  const float inv = 1.0f / det;
+  const F3 d1 = sub(a1, a0), d2 = sub(a2, a0);
+  const F3 a = muls(sub(scl(dv2, d1), scl(dv1, d2)), inv);
+  const F3 b = muls(add(scl(-du2, d1), scl(du1, d2)), inv);
+  if (len2(a) == 0.0f || len2(b) == 0.0f) return false;
+  *dadu = a;
+  *dadv = b;
+  return true;
+}
+__device__ __forceinline__ float clamped_derivative(float x) {  // finite_clamped_derivative, R/scene/mod.rs:861-867
+  return __builtin_isfinite(x) ? clampf(x, -1.0e8f, 1.0e8f) : 0.0f;
+}
+// compute_shading_differentials (R/scene/mod.rs:748-785) with
+// differentials_from_dp (:787-819), cone_shading_differentials (:821-849) and
+// ray_parameter_to_point (:851-859); n: the face-forwarded geometric normal.
+// No ray: zero differentials.
+__device__ __forceinline__ void shading_differentials(VtxTex& t, F3 p, F3 n, F3 dpdu, F3 dpdv, const RayDiff* ray) {
+  t.dudx = t.dvdx = t.dudy = t.dvdy = 0.0f;
+  t.dpdx = t.dpdy = f3(0.0f, 0.0f, 0.0f);
+  if (!ray) return;
+  if (ray->has) {
+    const float plane_d = -dot(n, p);
+    const float rxden = dot(n, ray->rxd), ryden = dot(n, ray->ryd);
+    if (fabsf(rxden) > 1.0e-8f && fabsf(ryden) > 1.0e-8f) {
+      const float tx = (-dot(n, ray->rxo) - plane_d) / rxden, ty = (-dot(n, ray->ryo) - plane_d) / ryden;
+      const F3 dpdx = sub(add(ray->rxo, scl(tx, ray->rxd)), p), dpdy = sub(add(ray->ryo, scl(ty, ray->ryd)), p);
+      if (__builtin_isfinite(tx) && __builtin_isfinite(ty) && finite3(dpdx) && finite3(dpdy)) {
+        const float a00 = dot(dpdu, dpdu), a01 = dot(dpdu, dpdv), a11 = dot(dpdv, dpdv);
+        const float det = difference_of_products(a00, a11, a01, a01);
+        if (!(fabsf(det) <= 1.0e-12f) && __builtin_isfinite(det)) {
+          const float inv = 1.0f / det;
+          const float b0x = dot(dpdu, dpdx), b1x = dot(dpdv, dpdx), b0y = dot(dpdu, dpdy), b1y = dot(dpdv, dpdy);
+          t.dpdx = dpdx;
+          t.dpdy = dpdy;
+          t.dudx = clamped_derivative(difference_of_products(a11, b0x, a01, b1x) * inv);
+          t.dvdx = clamped_derivative(difference_of_products(a00, b1x, a01, b0x) * inv);
+          t.dudy = clamped_derivative(difference_of_products(a11, b0y, a01, b1y) * inv);
+          t.dvdy = clamped_derivative(difference_of_products(a00, b1y, a01, b0y) * inv);
+          return;
+        }
+      }
+    }
+  }
+  const float dl2 = len2(ray->d);
+  if (dl2 <= 0.0f) return;
+  const float tp = dot(sub(p, ray->o), ray->d) / dl2;
+  if (!__builtin_isfinite(tp)) return;
+  const float width = cone_width_at(ray->cw, ray->cs, tp);
+  if (width <= 0.0f) return;
+  const float cos_theta = fmaxf(fabsf(dot(normalize_or_zero(ray->d), n)), 1.0e-3f);
+  const float projected = width / cos_theta;
+  const float dudx = projected / fmaxf(sqrtf(len2(dpdu)), 1.0e-6f);
+  const float dvdy = projected / fmaxf(sqrtf(len2(dpdv)), 1.0e-6f);
+  t.dpdx = scl(dudx, dpdu);
+  t.dpdy = scl(dvdy, dpdv);
+  t.dudx = dudx;
+  t.dvdy = dvdy;
+}
+
+// base_shading_vertex_from_triangle_sample_impl (R/scene/mod.rs:471-544).
+// With kTex also the texture inputs into *tx, for the ray that hit (null for
+// a light sample's vertex).
+template <bool kTex>
+__device__ __forceinline__ Vtx shading_vertex_t(const ShadeView& sv, uint32_t inst, uint32_t tri, float b0, float b1,
+                                                float b2, F3 dir, const RayDiff* ray, VtxTex* tx) {
   const GpuShade& si = sv.inst[inst];
   const uint32_t i0 = si.vbase + sv.idx[si.ibase + 3 * tri], i1 = si.vbase + sv.idx[si.ibase + 3 * tri + 1],
                  i2 = si.vbase + sv.idx[si.ibase + 3 * tri + 2];
@@ -963,30 +1071,92 @@ __device__ __forceinline__ Vtx shading_vertex(const ShadeView& sv, uint32_t inst
   const F3 nd = neg(dir);
   v.front = dot(gn, nd) >= 0.0f;
   v.ng = face_forward(gn, nd);
-  v.ns = face_forward(len2(sn) > 0.0f ? sn : v.ng, v.ng);
-  // compute_surface_partials over world positions (R/math/mod.rs:112-134)
+  const F3 raw_ns = len2(sn) > 0.0f ? sn : v.ng;
+  v.ns = face_forward(raw_ns, v.ng);
+  // compute_surface_partials (R/math/mod.rs:112-134) over world positions,
+  // else the Duff frame's tangent and bitangent
   const float u0 = has_uv ? sv.uv[2 * i0] : 0.0f, w0 = has_uv ? sv.uv[2 * i0 + 1] : 0.0f;
   const float u1 = has_uv ? sv.uv[2 * i1] : 0.0f, w1 = has_uv ? sv.uv[2 * i1 + 1] : 0.0f;
   const float u2 = has_uv ? sv.uv[2 * i2] : 0.0f, w2 = has_uv ? sv.uv[2 * i2 + 1] : 0.0f;
   const float du1 = u1 - u0, dv1 = w1 - w0, du2 = u2 - u0, dv2 = w2 - w0;
-  const float det = du1 * dv2 - dv1 * du2;
-  F3 dpdu = zero;
-  bool ok = false;
-  if (!(fabsf(det) <= 1.0e-8f)) {
-    const float inv = 1.0f / det;
-    const F3 dp1 = sub(p1, p0), dp2 = sub(p2, p0);
-    dpdu = muls(sub(scl(dv2, dp1), scl(dv1, dp2)), inv);
-    const F3 dpdv = muls(add(scl(-du2, dp1), scl(du1, dp2)), inv);
-    ok = !(len2(dpdu) == 0.0f || len2(dpdv) == 0.0f);
+  F3 dpdu, dpdv;
+  if (!surface_partials(p0, p1, p2, du1, dv1, du2, dv2, &dpdu, &dpdv)) {
+    const Frame f = frame_from_normal(v.ns);
+    dpdu = f.t;
+    dpdv = f.b;
   }
-  if (!ok) dpdu = frame_from_normal(v.ns).t;
   v.frame = frame_from_normal_tangent(v.ns, dpdu);
   v.wo = normalize_or_zero(nd);
   v.material = si.material;
   v.inst = inst;
   v.tri = tri;
+  if (kTex) {
+    tx->u = b0 * u0 + b1 * u1 + b2 * u2;  // interpolate_vec2
+    tx->v = b0 * w0 + b1 * w1 + b2 * w2;
+    if (!surface_partials(n0, n1, n2, du1, dv1, du2, dv2, &tx->dndu, &tx->dndv)) tx->dndu = tx->dndv = zero;
+    if (dot(v.ns, raw_ns) < 0.0f) {
+      tx->dndu = neg(tx->dndu);
+      tx->dndv = neg(tx->dndv);
+    }
+    if (len2(dpdu) == 0.0f) dpdu = v.frame.t;
+    if (len2(dpdv) == 0.0f) dpdv = v.frame.b;
+    shading_differentials(*tx, v.p, v.ng, dpdu, dpdv, ray);
+  }
   return v;
 }
+__device__ __forceinline__ Vtx shading_vertex(const ShadeView& sv, uint32_t inst, uint32_t tri, float b0, float b1,
+                                              float b2, F3 dir) {
+  return shading_vertex_t<false>(sv, inst, tri, b0, b1, b2, dir, nullptr, nullptr);
+}
+
+// OpenPbrMaterial::*_at (R/material/open_pbr.rs:670-790) and le's emission
+// texture (:351-375): each textured parameter times Texture::sample_filtered
+// at the vertex (uv_dx = (dudx, dvdx), uv_dy = (dudy, dvdy)); the clamps stay
+// where the parameters are read.
+__device__ __forceinline__ void apply_textures(const ShadeView& sv, TptMaterial& mat, const VtxTex& t) {
+  if (mat.kind != TPT_MATERIAL_OPEN_PBR) return;
+  TptOpenPbr& m = mat.u.open_pbr;
+  const tex::V2 uv = tex::v2(t.u, t.v), dx = tex::v2(t.dudx, t.dvdx), dy = tex::v2(t.dudy, t.dvdy);
+  if (m.base_color_texture != TPT_INVALID_INDEX) {
+    const F3 c = tex::filtered<3>(sv.tex, sv.tex.tex[m.base_color_texture], uv, dx, dy);
+    m.base_color[0] *= c.x;
+    m.base_color[1] *= c.y;
+    m.base_color[2] *= c.z;
+  }
+  if (m.base_metalness_texture != TPT_INVALID_INDEX)
+    m.base_metalness *= tex::filtered<1>(sv.tex, sv.tex.tex[m.base_metalness_texture], uv, dx, dy).x;
+  if (m.specular_roughness_texture != TPT_INVALID_INDEX)
+    m.specular_roughness *= tex::filtered<1>(sv.tex, sv.tex.tex[m.specular_roughness_texture], uv, dx, dy).x;
+  if (m.fuzz_weight_texture != TPT_INVALID_INDEX)
+    m.fuzz_weight *= tex::filtered<1>(sv.tex, sv.tex.tex[m.fuzz_weight_texture], uv, dx, dy).x;
+  if (m.fuzz_roughness_texture != TPT_INVALID_INDEX)
+    m.fuzz_roughness *= tex::filtered<1>(sv.tex, sv.tex.tex[m.fuzz_roughness_texture], uv, dx, dy).x;
+  if (m.emission_color_texture != TPT_INVALID_INDEX) {
+    const F3 c = tex::filtered<3>(sv.tex, sv.tex.tex[m.emission_color_texture], uv, dx, dy);
+    m.emission_color[0] *= c.x;
+    m.emission_color[1] *= c.y;
+    m.emission_color[2] *= c.z;
+  }
+}
+// The material of a vertex: the table entry, with its textures applied when
+// the scene has textures.
+template <bool kTex>
+struct VtxMat;
+template <>
+struct VtxMat<false> {
+  const TptMaterial* m;
+  __device__ __forceinline__ VtxMat(const ShadeView& sv, uint32_t mi, const VtxTex*) : m(&sv.mats[mi]) {}
+  __device__ __forceinline__ const TptMaterial& get() const { return *m; }
+};
+template <>
+struct VtxMat<true> {
+  TptMaterial m;
+  __device__ __forceinline__ VtxMat(const ShadeView& sv, uint32_t mi, const VtxTex* t) : m(sv.mats[mi]) {
+    apply_textures(sv, m, *t);
+  }
+  __device__ __forceinline__ const TptMaterial& get() const { return m; }
+};
+
 
 // Scene::triangle_positions (R/scene/mod.rs:432-438) -> world positions
 __device__ __forceinline__ void triangle_world(const ShadeView& sv, uint32_t inst, uint32_t tri, F3* p) {
@@ -1035,6 +1205,9 @@ __device__ __forceinline__ bool pbr_sample(const Tables& tb, const TptOpenPbr& m
   if (!bsdf_sample(p, wol, u_lobe, ux, uy, s)) return false;
   const F3 wi = to_world(v.frame, s->wi);
   if (dot(wi, v.ng) <= 1.0e-6f) return false;  // GEOMETRIC_NORMAL_COS_EPSILON; every carried lobe reflects
+  // cone_spread: 2 * effective specular roughness for glossy samples, else
+  // GLOSSY_DIFFUSE_CONE_SPREAD (:21, :509-522)
+  s->cone = (s->glossy && !s->delta) ? 2.0f * clampf(p.spec_rough, 0.0f, 1.0f) : 0.5f;
   *wi_world = wi;
   return true;
 }
@@ -1070,11 +1243,13 @@ __device__ __forceinline__ bool mat_sample(const Tables& tb, const TptMaterial& 
     s->weight = rgb3(m.u.lambert.rho);
     s->pdf = pdf;
     s->delta = false;
+    s->cone = 0.5f;  // DIFFUSE_CONE_SPREAD (R/material/normalized_lambert.rs:16)
   } else {  // R/bsdf/mirror.rs:23-39
     wl = normalize_or_zero(f3(-wol.x, -wol.y, wol.z));
     s->weight = rgb3(m.u.mirror.color);
     s->pdf = 1.0f;
     s->delta = true;
+    s->cone = 0.0f;  // R/material/mirror.rs:134
   }
   s->wi = wl;
   const F3 wi = to_world(v.frame, wl);
@@ -1116,6 +1291,31 @@ __device__ __forceinline__ F3 mat_eval_pdf(const Tables& tb, const TptMaterial& 
   if (cg <= 1.0e-6f) return zero;
   *pdf = p;
   return muls(f, fabsf(dot(wi, v.ns)) / cg);
+}
+
+// OpenPBR materials whose eval is affine in the base colour per channel
+// (metalness 0; no coat darkening): the ones the adjoint differentiates.
+__device__ __host__ __forceinline__ bool base_color_linear(const TptMaterial& m) {
+  if (m.kind != TPT_MATERIAL_OPEN_PBR) return false;
+  const TptOpenPbr& p = m.u.open_pbr;
+  const float cw = p.coat_weight < 0.0f ? 0.0f : (p.coat_weight > 1.0f ? 1.0f : p.coat_weight);
+  const float cd = p.coat_darkening < 0.0f ? 0.0f : (p.coat_darkening > 1.0f ? 1.0f : p.coat_darkening);
+  return p.base_metalness == 0.0f && (cw <= 0.0f || cd <= 0.0f);
+}
+// (d f_k / d base_color_k) / f_k of an OpenPBR material's BSDF at a
+// direction (0 where f_k is 0): the factor by which a contribution that
+// went through this BSDF value scales per unit relative change of the base
+// colour.  The shading-normal correction is common to both and cancels.
+// wil: the shading-frame direction (a BSDF sample's own, or a light
+// direction brought into the frame as mat_eval_pdf does).
+__device__ __forceinline__ F3 base_color_ratio(const Tables& tb, const TptMaterial& m, const Vtx& v, F3 wil) {
+  const F3 zero = f3(0.0f, 0.0f, 0.0f);
+  if (dot(v.wo, v.ng) <= 0.0f) return zero;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  if (wol.z <= 0.0f) return zero;
+  const Bsdf b = make_bsdf(tb, v.material, m.u.open_pbr, v.front, wol);
+  const F3 f = bsdf_eval(b, wol, wil), d = bsdf_eval_dbase(b, wol, wil);
+  return f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
 }
 
 // ---- environment light (R/light/environment.rs) ------------------------------------
@@ -1399,6 +1599,46 @@ __device__ __forceinline__ F3 offset_origin(F3 p, F3 ng, F3 d) {  // R/integrato
   return add(p, scl(1.0e-4f, dot(d, ng) >= 0.0f ? ng : neg(ng)));
 }
 
+// specular_ray_differential (R/integrator/mod.rs:88-138) for a reflection
+// (every delta lobe carried reflects), reflected_differential_directions
+// (:140-157), differential_is_reasonable (:208-219).
+__device__ __forceinline__ bool specular_differential(const RayDiff& in, const Vtx& v, const VtxTex& t, F3 wi_world,
+                                                      RayDiff* out) {
+  if (!in.has) return false;
+  const F3 wo = v.wo, wi = normalize_or_zero(wi_world);
+  const F3 dndx = add(scl(t.dudx, t.dndu), scl(t.dvdx, t.dndv));
+  const F3 dndy = add(scl(t.dudy, t.dndu), scl(t.dvdy, t.dndv));
+  const F3 dwodx = sub(neg(in.rxd), wo), dwody = sub(neg(in.ryd), wo);
+  const F3 rxo = offset_origin(add(v.p, t.dpdx), v.ng, wi), ryo = offset_origin(add(v.p, t.dpdy), v.ng, wi);
+  const F3 n = v.ns;
+  const float dwn_dx = dot(dwodx, n) + dot(wo, dndx), dwn_dy = dot(dwody, n) + dot(wo, dndy);
+  const float wn = dot(wo, n);
+  const F3 rxd = add(sub(wi, dwodx), scl(2.0f, add(scl(wn, dndx), scl(dwn_dx, n))));
+  const F3 ryd = add(sub(wi, dwody), scl(2.0f, add(scl(wn, dndy), scl(dwn_dy, n))));
+  const float kMax = 1.0e16f;
+  if (!finite3(rxo) || !finite3(ryo) || !finite3(rxd) || !finite3(ryd) || len2(rxo) > kMax || len2(ryo) > kMax ||
+      len2(rxd) > kMax || len2(ryd) > kMax)
+    return false;
+  out->rxo = rxo;
+  out->ryo = ryo;
+  out->rxd = rxd;
+  out->ryd = ryd;
+  return true;
+}
+// spawn_scattered_ray (R/integrator/mod.rs:58-75): the next ray's cone and,
+// after a delta sample, its specular differential.  o / d: its origin and
+// direction (spawn_ray).
+__device__ __forceinline__ RayDiff spawn_scattered(const RayDiff& in, float hit_t, const Vtx& v, const VtxTex& t,
+                                                   const Sample& s, F3 o, F3 wi) {
+  RayDiff r;
+  r.o = o;
+  r.d = wi;
+  r.cw = fmaxf(cone_width_at(in.cw, in.cs, hit_t), 0.0f);
+  r.cs = fmaxf(in.cs + s.cone, 0.0f);
+  r.has = s.delta && specular_differential(in, v, t, wi, &r);
+  return r;
+}
+
 // The light sample of direct_light_mis_contribution (R/integrator/mis.rs:141-210)
 // up to the shadow test, with sample_light_mis_compensated_lazy
 // (R/light/mod.rs:289-331): a category, then either a light-tree leaf and a
@@ -1412,6 +1652,7 @@ struct NeeSample {
   F3 origin, dir, value;
   uint32_t inst, tri;
 };
+template <bool kTex = false>
 __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMaterial& m, const Vtx& vtx, float u_root,
                                            float u_tree, float us0, float us1, NeeSample* ns) {
   int kind;
@@ -1452,8 +1693,11 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
       lng = face_forward(normalize_or_zero(cross(sub(p[1], p[0]), sub(p[2], p[0]))), neg(wi));
       le = hadamard(muls(rgb3(lm.u.emissive.color), lm.u.emissive.strength), f3(1.0f, 1.0f, 1.0f));
     } else {
-      const Vtx lv = shading_vertex(sv, ln.instance, ln.triangle, b0, b1, b2, wi);
-      if (!mat_le(lm, lv, &le)) return false;
+      // the light vertex has no ray: zero differentials (a bilinear emission lookup)
+      VtxTex lt;
+      const Vtx lv = shading_vertex_t<kTex>(sv, ln.instance, ln.triangle, b0, b1, b2, wi, nullptr, &lt);
+      const VtxMat<kTex> lmt(sv, sv.inst[ln.instance].material, &lt);
+      if (!mat_le(lmt.get(), lv, &le)) return false;
       lng = lv.ng;
     }
     const float cos_light = fmaxf(dot(lng, neg(wi)), 0.0f);
@@ -1533,6 +1777,7 @@ __device__ __forceinline__ F3 emitted_infinite(const ShadeView& sv, F3 thr, floa
 struct CamView {
   F3 eye, fwd, right, up;
   float tan_half, exposure;
+  float spread;  // pixel_spread_angle (R/scene/camera.rs:89-96)
 };
 __device__ __forceinline__ F3 camera_dir(const CamView& cam, const smp::PathSampler& ps, uint32_t x, uint32_t y,
                                          uint32_t w, uint32_t h, const uint8_t* row = nullptr) {
@@ -1544,6 +1789,35 @@ __device__ __forceinline__ F3 camera_dir(const CamView& cam, const smp::PathSamp
   const float sx = ((2.0f * px / rx) - 1.0f) * aspect * cam.tan_half;
   const float sy = (1.0f - (2.0f * py / ry)) * cam.tan_half;
   return normalize(add(add(cam.fwd, scl(sx, cam.right)), scl(sy, cam.up)));
+}
+// generate_ray_differential (R/scene/camera.rs:48-78): the camera ray with
+// its differential toward the next pixel in x and y, scaled by
+// differential_scale(spp) = max(1 / sqrt(spp), 0.125), and its cone
+// (width 0, spread pixel_spread_angle * scale).
+__device__ __forceinline__ RayDiff camera_ray_diff(const CamView& cam, const smp::PathSampler& ps, uint32_t x,
+                                                   uint32_t y, uint32_t w, uint32_t h, uint32_t spp,
+                                                   const uint8_t* row = nullptr) {
+  float u[2];
+  ps.s2(smp::kCameraDim, u, row);
+  const float rx = (float)w, ry = (float)h, aspect = rx / ry;
+  const float px = ((float)x + 0.5f) + 0.5f * smp::tent(u[0]);
+  const float py = ((float)y + 0.5f) + 0.5f * smp::tent(u[1]);
+  auto dir_for = [&](float qx, float qy) {
+    const float sx = ((2.0f * qx / rx) - 1.0f) * aspect * cam.tan_half;
+    const float sy = (1.0f - (2.0f * qy / ry)) * cam.tan_half;
+    return normalize(add(add(cam.fwd, scl(sx, cam.right)), scl(sy, cam.up)));
+  };
+  const float sc = fmaxf(1.0f / sqrtf((float)(spp < 1u ? 1u : spp)), 0.125f);
+  RayDiff r;
+  r.o = cam.eye;
+  r.d = dir_for(px, py);
+  r.has = true;
+  r.rxo = r.ryo = cam.eye;
+  r.rxd = normalize_or_zero(add(r.d, scl(sc, sub(dir_for(px + 1.0f, py), r.d))));
+  r.ryd = normalize_or_zero(add(r.d, scl(sc, sub(dir_for(px, py + 1.0f), r.d))));
+  r.cw = 0.0f;
+  r.cs = fmaxf(cam.spread * sc, 0.0f);
+  return r;
 }
 
 // The shading view of a scene resident on the device.
@@ -1562,6 +1836,7 @@ inline ShadeView shade_view_of(const TptScene* s) {
   v.w_env = s->w_env;
   v.env = s->env;
   v.tab = Tables{s->ggx_albedo, s->ggx_mat, s->ggx_cond, s->ggx_diel};
+  v.tex = TexView{s->tex_texels, s->tex_levels, s->textures, s->texture_count};
   return v;
 }
 

@@ -70,6 +70,24 @@ struct EnvView {
   float scale, total_mis;
   float rot[9], inv_rot[9];
 };
+// Textures on the device: every mip level of every texture in one texel
+// array (RGB texels padded to float4 so one texel is one 16-byte load; scalar
+// texels one float), a level table (offset in floats, size) and per texture
+// its first level, level count, channels and level-0 maximum
+// (R/material/texture.rs:53-67).
+struct GpuTexLevel {
+  uint32_t offset, width, height, pow2;  // pow2: bit 0 width, bit 1 height is a power of two
+};
+struct GpuTexture {
+  uint32_t first_level, levels, channels;
+  float max_value;
+};
+struct TexView {
+  const float* texels;
+  const GpuTexLevel* levels;
+  const GpuTexture* tex;
+  uint32_t count;
+};
 }  // namespace pt
 }  // namespace tpt
 
@@ -136,6 +154,14 @@ struct TptScene {
   uint32_t* ggx_mat = nullptr;
   float* ggx_cond = nullptr;
   float* ggx_diel = nullptr;
+  // textures (TptSceneDesc::textures): texels of every level, level table,
+  // per-texture records; host copies of the two tables
+  float* tex_texels = nullptr;
+  tpt::pt::GpuTexLevel* tex_levels = nullptr;
+  tpt::pt::GpuTexture* textures = nullptr;
+  uint32_t texture_count = 0;
+  std::vector<tpt::pt::GpuTexLevel> host_tex_levels;
+  std::vector<tpt::pt::GpuTexture> host_textures;
   size_t bytes = 0;
   // staging for host-pointer calls (grow-only)
   void* stage = nullptr;
@@ -149,6 +175,23 @@ bool launch_trace_queue(TptScene* s, int query, bool q8, bool exact, const TptRa
 // otherwise records "<fn>: ..." and returns false.
 // Builds and uploads the GGX tables of the scene's materials (trace.hip).
 bool upload_ggx_tables(TptScene* s, const std::vector<TptMaterial>& mats);
+// Copies the scene description's textures to the device and builds their
+// mip pyramids there (texture.hip); records "<fn>: ..." on failure.
+bool upload_textures(TptScene* s, const TptSceneDesc& desc);
+// Level-0 maximum of a texture description (Texture::max_value): the fold
+// of f32::max over every channel, from 0.
+float texture_max_value(const TptTextureDesc& t);
+// Floats of the scene's texel array (every level; RGB texels 4 floats).
+size_t texel_floats(const TptScene* s);
+// Floats of the level-0 texels of every texture packed (channels per texel).
+size_t texel_grad_floats(const TptScene* s);
+// Turns a gradient array laid out like the texels (every level) into
+// level-0 gradients (the adjoint of the pyramid build, coarsest level first;
+// modifies g) and packs them into out, texture after texture; enqueued on st.
+bool texel_grads_to_level0(TptScene* s, float* g, float* out, hipStream_t st);
+// Checks the materials' texture indices against the scene's texture table
+// (colour slots need RGB, scalar slots one channel); "" when valid.
+std::string material_texture_error(const TptMaterial& m, uint32_t index, const std::vector<pt::GpuTexture>& tex);
 bool render_materials_ok(TptScene* s, const char* fn);
 
 inline SceneView view_of(const TptScene* s) {

@@ -301,6 +301,9 @@ void free_scene(TptScene* s) {
   (void)hipFree(s->ggx_mat);
   (void)hipFree(s->ggx_cond);
   (void)hipFree(s->ggx_diel);
+  (void)hipFree(s->tex_texels);
+  (void)hipFree(s->tex_levels);
+  (void)hipFree(s->textures);
   delete s;
 }
 
@@ -340,7 +343,10 @@ bool upload_ggx_tables(TptScene* s, const std::vector<TptMaterial>& mats) {
       need_diel = true;
     }
     if (eta != 1.0f) need_diel = true;
-    if (m.base_metalness > 0.0f) need_cond = true;
+    // metalness clamp(base_metalness * texel, 0, 1) can be positive only
+    // for a non-zero parameter
+    if (m.base_metalness > 0.0f || (m.base_metalness_texture != TPT_INVALID_INDEX && m.base_metalness != 0.0f))
+      need_cond = true;
   }
   const size_t ncond = ggx::kCondCos * ggx::kCondRough + ggx::kCondRough;
   const size_t ndiel = ggx::kDielCos * ggx::kDielRough * ggx::kDielEta + ggx::kDielRough * ggx::kDielEta;
@@ -755,6 +761,18 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
     if (!upload_ggx_tables(s, mats)) {
       free_scene(s);
       return nullptr;
+    }
+    if (!upload_textures(s, *desc)) {
+      free_scene(s);
+      return nullptr;
+    }
+    for (uint32_t i = 0; i < (uint32_t)mats.size(); ++i) {
+      const std::string terr = material_texture_error(mats[i], i, s->host_textures);
+      if (!terr.empty()) {
+        set_error("tpt_scene_create: " + terr);
+        free_scene(s);
+        return nullptr;
+      }
     }
     if (desc->env_pixels) {  // environment light: texels, compensated pyramid, level table
       HostEnv he;

@@ -65,6 +65,12 @@ struct State {
   float* prev_ns;   // 3 per path: its shading normal
   float* prev_pdf;  // BSDF pdf of the sample that left it
   uint32_t* prev_info;  // material index | delta flag (bit 31)
+  // textured scenes: the current ray's differential (12 arrays of one float
+  // per path: rx origin, ry origin, rx direction, ry direction), whether it
+  // has one, and its cone (2 arrays: width, spread); null otherwise
+  float* diff;
+  uint32_t* dflag;
+  float* cone;
 };
 
 struct Queue {
@@ -194,8 +200,41 @@ __device__ __forceinline__ void stage_row(uint8_t* lds, const uint8_t* __restric
   }
 }
 
+// The per-path ray differential and cone of textured scenes (SoA).
+__device__ __forceinline__ void store_diff(const State& st, uint32_t P, uint32_t path, const pt::RayDiff& r) {
+  st.dflag[path] = r.has ? 1u : 0u;
+  if (r.has) {
+    const F3 v[4] = {r.rxo, r.ryo, r.rxd, r.ryd};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      st.diff[(size_t)(3 * k) * P + path] = v[k].x;
+      st.diff[(size_t)(3 * k + 1) * P + path] = v[k].y;
+      st.diff[(size_t)(3 * k + 2) * P + path] = v[k].z;
+    }
+  }
+  st.cone[path] = r.cw;
+  st.cone[(size_t)P + path] = r.cs;
+}
+__device__ __forceinline__ void load_diff(const State& st, uint32_t P, uint32_t path, pt::RayDiff* r) {
+  r->has = st.dflag[path] != 0u;
+  if (r->has) {
+    F3 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      v[k] = f3(st.diff[(size_t)(3 * k) * P + path], st.diff[(size_t)(3 * k + 1) * P + path],
+                st.diff[(size_t)(3 * k + 2) * P + path]);
+    r->rxo = v[0];
+    r->ryo = v[1];
+    r->rxd = v[2];
+    r->ryd = v[3];
+  }
+  r->cw = st.cone[path];
+  r->cs = st.cone[(size_t)P + path];
+}
+
 // Camera rays of the pass's samples (PathSampler camera dimension,
 // PinholeCamera::generate_ray_differential), state init, queue 0 append.
+template <bool kTex>
 __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Queue q, Counters* c,
                                                             const uint8_t* __restrict__ ztab, uint32_t ztab_dims) {
   __shared__ __align__(16) uint8_t zrow[smp::kZTabPrefixes];
@@ -216,7 +255,13 @@ __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Q
         const uint32_t lp = gp - fr.pix_begin, x = lp % fr.w, y = lp / fr.w;
         const smp::PathSampler smpl{fr.zf, x, y, si};
         o = fr.cam.eye;
-        d = pt::camera_dir(fr.cam, smpl, x, y, fr.w, fr.h, zrow);
+        if (kTex) {
+          const pt::RayDiff r = pt::camera_ray_diff(fr.cam, smpl, x, y, fr.w, fr.h, fr.spp, zrow);
+          d = r.d;
+          store_diff(st, ps.paths, path, r);
+        } else {
+          d = pt::camera_dir(fr.cam, smpl, x, y, fr.w, fr.h, zrow);
+        }
         store3(st.thr, path, f3(1.0f, 1.0f, 1.0f));
         want = true;
       }
@@ -230,6 +275,9 @@ __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Q
 }
 
 // One bounce of mis::trace_radiance for every path of the input queue.
+// With kTex the ray differentials and cones follow the paths and the
+// materials' textures are applied at every vertex.
+template <bool kTex>
 __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
                                                 const TptHit* __restrict__ hits, Queue qout, ShadowQ sq,
                                                 Counters* c, uint32_t in_idx, uint32_t bounce,
@@ -274,8 +322,16 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
         const smp::PathSampler smpl{fr.zf, lp % fr.w, lp / fr.w, k * ps.spp_slots + s};
         const TptRay r = qin.rays[slot];
         const F3 dir = f3(r.d[0], r.d[1], r.d[2]);
-        const pt::Vtx vtx = pt::shading_vertex(sv, h.instance, h.triangle, h.b[0], h.b[1], h.b[2], dir);
-        const TptMaterial& m = sv.mats[vtx.material];
+        pt::RayDiff in;
+        if (kTex) {
+          in.o = f3(r.o[0], r.o[1], r.o[2]);
+          in.d = dir;
+          load_diff(st, ps.paths, path, &in);
+        }
+        pt::VtxTex tx;
+        const pt::Vtx vtx = pt::shading_vertex_t<kTex>(sv, h.instance, h.triangle, h.b[0], h.b[1], h.b[2], dir, &in, &tx);
+        const pt::VtxMat<kTex> vm(sv, vtx.material, &tx);
+        const TptMaterial& m = vm.get();
         F3 rad = load3f(st.rad, path);
         F3 thr = load3f(st.thr, path);
         if (bounce == 0) {
@@ -291,7 +347,8 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
           if (m.kind != TPT_MATERIAL_EMISSIVE) {
             float us[2];
             smpl.s2(bd + 3, us, zrows[2]);
-            if (pt::nee_sample(sv, m, vtx, smpl.s1(bd + 0, zrows[0]), smpl.s1(bd + 1, zrows[1]), us[0], us[1], &ns)) {
+            if (pt::nee_sample<kTex>(sv, m, vtx, smpl.s1(bd + 0, zrows[0]), smpl.s1(bd + 1, zrows[1]), us[0], us[1],
+                                     &ns)) {
               want_shadow = true;
               nee_value = pt::hadamard(thr, ns.value);
             }
@@ -314,6 +371,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
               want_ray = true;
               no = pt::offset_origin(vtx.p, vtx.ng, wi);
               nd = wi;
+              if (kTex) store_diff(st, ps.paths, path, pt::spawn_scattered(in, h.t, vtx, tx, smp_, no, wi));
               store3(st.thr, path, thr);
               store3(st.prev_p, path, vtx.p);
               store3(st.prev_ns, path, vtx.ns);
@@ -426,24 +484,27 @@ struct Plan {
   uint32_t max_depth;
   bool q8;
   bool exact_shadow;
+  bool tex;  // the scene has textures: ray differentials and textured materials
 };
 
 // The kernels of one pass, all reading their counts on the device.
 bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream_t st) {
   TptDevice* d = s->dev;
-  const unsigned g_gen = resident_blocks(d, wf::wf_generate, p.ps.paths, wf::kAppendBlock);
-  const unsigned g_shade = resident_blocks(d, wf::wf_shade, p.ps.paths, wf::kAppendBlock);
+  auto gen = p.tex ? wf::wf_generate<true> : wf::wf_generate<false>;
+  auto shade = p.tex ? wf::wf_shade<true> : wf::wf_shade<false>;
+  const unsigned g_gen = resident_blocks(d, gen, p.ps.paths, wf::kAppendBlock);
+  const unsigned g_shade = resident_blocks(d, shade, p.ps.paths, wf::kAppendBlock);
   const unsigned g_add = resident_blocks(d, wf::wf_shadow_add, p.ps.paths);
   const unsigned g_fold = resident_blocks(d, wf::wf_fold, p.ps.total_pixels);
   hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, 0u);
-  hipLaunchKernelGGL(wf::wf_generate, dim3(g_gen), dim3(wf::kAppendBlock), 0, st, p.ps, p.st, p.q[0], p.c,
+  hipLaunchKernelGGL(gen, dim3(g_gen), dim3(wf::kAppendBlock), 0, st, p.ps, p.st, p.q[0], p.c,
                      (const uint8_t*)p.ztab, p.ztab_dims);
   for (uint32_t b = 0; b <= p.max_depth; ++b) {
     const uint32_t a = b & 1u;
     if (!launch_trace_queue(s, 0, p.q8, false, p.q[a].rays, &p.c->rays[a].v, p.ps.paths, p.hits, nullptr, nullptr, st))
       return false;
     hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, a ^ 1u);
-    hipLaunchKernelGGL(wf::wf_shade, dim3(g_shade), dim3(wf::kAppendBlock), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
+    hipLaunchKernelGGL(shade, dim3(g_shade), dim3(wf::kAppendBlock), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
                        p.q[a ^ 1u], p.sq, p.c, a, b, (const uint8_t*)p.ztab, p.ztab_dims);
     if (b < p.max_depth && (sv.light_count || sv.env.px)) {  // some light can be sampled
       if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow.v, p.ps.paths, nullptr, p.sq.target,
@@ -510,6 +571,8 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     const size_t o_hits = take(sizeof(TptHit) * P);
     const size_t o_sr = take(sizeof(TptRay) * P), o_st = take(8 * P), o_sv = take(12 * P), o_sp = take(4 * P);
     const size_t o_su = take(P);
+    const bool tex = s->texture_count > 0;
+    const size_t o_diff = tex ? take(12 * 4 * P) : 0, o_dflag = tex ? take(4 * P) : 0, o_cone = tex ? take(8 * P) : 0;
     // digit-permutation rows of every sampler dimension a path of up to 256 bounces reads
     // (deeper bounces compute their rows in the kernel)
     const uint32_t ztab_dims = smp::kVertexBase + smp::kVertexStride * (std::min<uint32_t>(max_depth, 256u) + 1);
@@ -530,6 +593,7 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
       x.cam.up = F3{c.image_up[0], c.image_up[1], c.image_up[2]};
       x.cam.tan_half = c.tan_half_fov_y;
       x.cam.exposure = c.exposure;
+      x.cam.spread = c.pixel_spread;
       x.w = r.width;
       x.h = r.height;
       x.spp = r.samples_per_pixel;
@@ -548,7 +612,9 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.ps.total_pixels = (uint32_t)total;
     p.ps.paths = (uint32_t)P;
     p.st = wf::State{(float*)(base + o_thr), (float*)(base + o_rad), (float*)(base + o_pp), (float*)(base + o_pn),
-                     (float*)(base + o_pdf), (uint32_t*)(base + o_info)};
+                     (float*)(base + o_pdf), (uint32_t*)(base + o_info),
+                     tex ? (float*)(base + o_diff) : nullptr, tex ? (uint32_t*)(base + o_dflag) : nullptr,
+                     tex ? (float*)(base + o_cone) : nullptr};
     p.q[0] = wf::Queue{(TptRay*)(base + o_q0r), (uint32_t*)(base + o_q0p)};
     p.q[1] = wf::Queue{(TptRay*)(base + o_q1r), (uint32_t*)(base + o_q1p)};
     p.sq = wf::ShadowQ{(TptRay*)(base + o_sr), (uint32_t*)(base + o_st), (float*)(base + o_sv),
@@ -561,6 +627,7 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.max_depth = max_depth;
     p.q8 = flags & TPT_FLAG_QUANTIZED;
     p.exact_shadow = flags & TPT_FLAG_ANY_EXACT;
+    p.tex = tex;
     const pt::ShadeView sv = pt::shade_view_of(s);
     if (!hip_ok(hipMemcpyAsync((void*)p.ps.frames, fd.data(), fd.size() * sizeof(wf::FrameDev), hipMemcpyHostToDevice,
                                st),

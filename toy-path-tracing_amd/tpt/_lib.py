@@ -97,10 +97,28 @@ class SceneDesc(ctypes.Structure):
         ("env_height", ctypes.c_uint32),
         ("env_scale", ctypes.c_float),
         ("env_rotate_y", ctypes.c_float),
+        ("textures", ctypes.c_void_p),
+        ("texture_count", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
     ]
 
 
-assert ctypes.sizeof(MeshDesc) == 64 and ctypes.sizeof(SceneDesc) == 80
+class TextureDesc(ctypes.Structure):
+    _fields_ = [
+        ("pixels", ctypes.c_void_p),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("channels", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
+# TptTextureQuery (24 bytes) and the lookup modes
+TEXTURE_QUERY_DTYPE = np.dtype([("uv", "<f4", 2), ("dstdx", "<f4", 2), ("dstdy", "<f4", 2)])
+assert TEXTURE_QUERY_DTYPE.itemsize == 24
+TEXTURE_FILTERED, TEXTURE_BILINEAR, TEXTURE_NEAREST, TEXTURE_TRILINEAR, TEXTURE_MIP_BILINEAR = 0, 1, 2, 3, 4
+
+assert ctypes.sizeof(MeshDesc) == 64 and ctypes.sizeof(SceneDesc) == 96 and ctypes.sizeof(TextureDesc) == 24
 assert ctypes.sizeof(InstanceDesc) == 136
 
 
@@ -190,9 +208,20 @@ _SIGNATURES = [
     ("tpt_render_batch", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("tpt_render_adjoint", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_render_adjoint_texels", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_scene_set_materials", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_scene_set_texture", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_material_bsdf", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_texture_sample", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+      ctypes.c_uint32]),
+    ("tpt_texture_info", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("tpt_texture_level", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+      ctypes.c_void_p]),
     ("tpt_autotune_render", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     ("tpt_trace_counts", ctypes.c_int32,

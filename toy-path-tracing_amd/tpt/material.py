@@ -28,23 +28,29 @@ class OpenPbrRecord(ctypes.Structure):
         ("coat_roughness_anisotropy", ctypes.c_float), ("coat_ior", ctypes.c_float),
         ("coat_darkening", ctypes.c_float),
         ("fuzz_color", ctypes.c_float * 3), ("fuzz_roughness", ctypes.c_float),
-        ("pad", ctypes.c_float),
+        ("base_color_texture", ctypes.c_uint32), ("base_metalness_texture", ctypes.c_uint32),
+        ("specular_roughness_texture", ctypes.c_uint32), ("fuzz_weight_texture", ctypes.c_uint32),
+        ("fuzz_roughness_texture", ctypes.c_uint32), ("emission_color_texture", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
     ]
 
 
-assert ctypes.sizeof(OpenPbrRecord) == 136
+assert ctypes.sizeof(OpenPbrRecord) == 160
+NO_TEXTURE = 0xFFFFFFFF
+TEXTURE_FIELDS = ("base_color_texture", "base_metalness_texture", "specular_roughness_texture",
+                  "fuzz_weight_texture", "fuzz_roughness_texture", "emission_color_texture")
 
 KIND_OPEN_PBR, KIND_NORMALIZED_LAMBERT, KIND_EMISSIVE, KIND_MIRROR = 0, 1, 2, 3
 
 
 class MaterialRecord(ctypes.Structure):
     """TptMaterial: kind, pad, then the kind's parameters (OpenPBR: the
-    136-byte TptOpenPbr; Lambert: rho[3]; Emissive: color[3], strength;
+    160-byte TptOpenPbr; Lambert: rho[3]; Emissive: color[3], strength;
     Mirror: color[3])."""
-    _fields_ = [("kind", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("params", ctypes.c_float * 34)]
+    _fields_ = [("kind", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("params", ctypes.c_float * 40)]
 
 
-assert ctypes.sizeof(MaterialRecord) == 144
+assert ctypes.sizeof(MaterialRecord) == 168
 
 
 @dataclass(frozen=True)
@@ -72,6 +78,14 @@ class OpenPbr:
     coat_darkening: float = 1.0
     fuzz_color: tuple = (1.0, 1.0, 1.0)
     fuzz_roughness: float = 0.5
+    # indices into the scene's texture table (Scene.add_texture), None: no
+    # texture (R/material/open_pbr.rs:66-72)
+    base_color_texture: int | None = None
+    base_metalness_texture: int | None = None
+    specular_roughness_texture: int | None = None
+    fuzz_weight_texture: int | None = None
+    fuzz_roughness_texture: int | None = None
+    emission_color_texture: int | None = None
 
     @staticmethod
     def new(base_color) -> "OpenPbr":
@@ -88,7 +102,9 @@ class OpenPbr:
             if name == "pad":
                 continue
             v = getattr(self, name)
-            if isinstance(v, tuple):
+            if name in TEXTURE_FIELDS:
+                setattr(r, name, NO_TEXTURE if v is None else int(v))
+            elif isinstance(v, tuple):
                 getattr(r, name)[:] = [float(x) for x in v]
             else:
                 setattr(r, name, int(v) if name == "thin_walled" else float(v))

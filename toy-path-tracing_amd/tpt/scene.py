@@ -164,6 +164,14 @@ class Scene:
     built: bool = False
     materials: list = field(default_factory=list)  # tpt.material records, by material index
     environment: Environment | None = None  # Scene::set_environment_light
+    textures: list = field(default_factory=list)  # tpt.texture.Texture, indexed by the materials' *_texture
+
+    def add_texture(self, texture) -> int:
+        """Adds a texture to the scene's table -> its index (what an OpenPbr
+        *_texture field takes; the reference holds Arc<Texture> in the
+        material, R/material/open_pbr.rs:66-72)."""
+        self.textures.append(texture)
+        return len(self.textures) - 1
 
     def set_environment_light(self, env: Environment) -> None:
         self.environment = Environment(np.ascontiguousarray(env.pixels, np.float32), float(env.scale),
@@ -296,6 +304,17 @@ def scene_desc(scene: Scene):
         desc.env_height, desc.env_width = px.shape[0], px.shape[1]
         desc.env_scale = scene.environment.scale
         desc.env_rotate_y = scene.environment.rotate_y
+    if scene.textures:
+        texs = (_lib.TextureDesc * len(scene.textures))()
+        for i, t in enumerate(scene.textures):
+            px = np.ascontiguousarray(t.pixels, np.float32)
+            keep.append(px)
+            texs[i].pixels = ptr(px)
+            texs[i].height, texs[i].width = px.shape[0], px.shape[1]
+            texs[i].channels = 3 if px.ndim == 3 else 1
+        keep.append(texs)
+        desc.textures = ctypes.addressof(texs)
+        desc.texture_count = len(scene.textures)
     keep += [meshes, insts]
     return desc, keep
 
@@ -324,6 +343,7 @@ class DeviceScene:
             raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
         self.instance_count = len(scene.instances)
         self.material_count = len(scene.materials)
+        self.textures = list(scene.textures)  # host copies (shapes of the gradient outputs)
 
     @property
     def device_bytes(self) -> int:
@@ -422,6 +442,33 @@ class DeviceScene:
                                        _lib.TPT_FLAG_GRAD_FP64 if fp64 else 0))
         return out
 
+    def render_adjoint_texels(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dimage,
+                              fp64: bool = False):
+        """tpt_render_adjoint_texels -> (d_albedo (material_count, 3), [per
+        texture level-0 gradient, shaped like its pixels])."""
+        d = _lib.RenderDesc()
+        d.camera = camera
+        d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+        g = np.ascontiguousarray(dloss_dimage, dtype=np.float32).reshape(height, width, 3)
+        out = np.zeros((self.material_count, 3), np.float32)
+        shapes = [t.pixels.shape for t in self.textures]
+        flat = np.zeros(max(1, sum(int(np.prod(sh)) for sh in shapes)), np.float32)
+        check(lib().tpt_render_adjoint_texels(self.handle, ctypes.byref(d), ptr(g), ptr(out), ptr(flat),
+                                              _lib.TPT_FLAG_GRAD_FP64 if fp64 else 0))
+        grads, o = [], 0
+        for sh in shapes:
+            k = int(np.prod(sh))
+            grads.append(flat[o:o + k].reshape(sh).copy())
+            o += k
+        return out, grads
+
+    def set_texture(self, texture: int, pixels) -> None:
+        """tpt_scene_set_texture: new level-0 texels (same shape) for a texture."""
+        px = np.ascontiguousarray(pixels, np.float32)
+        if px.shape != self.textures[texture].pixels.shape:
+            raise ValueError("set_texture: the texture's shape must stay " + str(self.textures[texture].pixels.shape))
+        check(lib().tpt_scene_set_texture(self.handle, texture, ptr(px), 0))
+
     def render_batch(self, frames, quantized: bool = False, exact_shadow: bool = False) -> list:
         """Wavefront path tracer over a batch of frames (tpt_render_batch).
         frames: (camera, width, height, spp, max_depth) tuples -> one
@@ -449,6 +496,38 @@ class DeviceScene:
         flags = _lib.TPT_FLAG_QUANTIZED if quantized else 0
         check(lib().tpt_autotune_render(self.handle, arr, len(frames), flags, tune_spp, ctypes.byref(res)))
         return {name: getattr(res, name) for name, _ in _lib.TuneResult._fields_}
+
+    def texture_info(self, texture: int) -> dict:
+        """tpt_texture_info -> {levels, channels, max_value}."""
+        lv, ch, mx = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_float(0.0)
+        check(lib().tpt_texture_info(self.handle, texture, ctypes.byref(lv), ctypes.byref(ch), ctypes.byref(mx)))
+        return {"levels": lv.value, "channels": ch.value, "max_value": mx.value}
+
+    def texture_level(self, texture: int, level: int) -> np.ndarray:
+        """tpt_texture_level: mip level `level` as (h, w, 3) or (h, w) float32."""
+        w, h = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        check(lib().tpt_texture_level(self.handle, texture, level, None, 0, ctypes.byref(w), ctypes.byref(h)))
+        ch = self.texture_info(texture)["channels"]
+        out = np.zeros((h.value, w.value, ch) if ch == 3 else (h.value, w.value), np.float32)
+        check(lib().tpt_texture_level(self.handle, texture, level, ptr(out), out.size, ctypes.byref(w),
+                                      ctypes.byref(h)))
+        return out
+
+    def texture_sample(self, texture: int, uv, dstdx=None, dstdy=None, mode: int = 0) -> np.ndarray:
+        """tpt_texture_sample: lookups of one texture at rows of uv with the
+        uv derivatives dstdx / dstdy (zero when None) in `mode`
+        (tpt._lib.TEXTURE_*) -> (n, channels) float32."""
+        uv = np.asarray(uv, np.float32).reshape(-1, 2)
+        q = np.zeros(len(uv), _lib.TEXTURE_QUERY_DTYPE)
+        q["uv"] = uv
+        if dstdx is not None:
+            q["dstdx"] = np.asarray(dstdx, np.float32).reshape(-1, 2)
+        if dstdy is not None:
+            q["dstdy"] = np.asarray(dstdy, np.float32).reshape(-1, 2)
+        ch = self.texture_info(texture)["channels"]
+        out = np.zeros((len(uv), ch), np.float32)
+        check(lib().tpt_texture_sample(self.handle, texture, mode, q.ctypes.data, len(uv), ptr(out), 0))
+        return out
 
     def material_bsdf(self, material, wo, wi, u, front_face: bool = True) -> np.ndarray:
         """tpt_material_bsdf: the BSDF of material index `material` in the

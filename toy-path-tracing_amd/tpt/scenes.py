@@ -396,3 +396,66 @@ def shadow_rays(rays: np.ndarray, hits: np.ndarray, mesh: Mesh, n: int, n_lights
 
 # canonical camera of the traversal microbenchmark: outside the 100-unit cube
 TRAVERSAL_CAMERA = dict(eye=(50.0, 50.0, -140.0), look_at=(50.0, 50.0, 50.0), up=(0.0, 1.0, 0.0), fov_y_deg=25.0)
+
+
+def cornell_textured(build: bool = True) -> CornellScene:
+    """The Cornell box with every texture path of the backend in use: a
+    tiled RGB checkerboard (64 x 64, uv x 4: wrap addressing) as the floor's
+    base colour with a scalar roughness map, a non-power-of-two RGB gradient
+    (48 x 40) as the back wall's base colour with a metalness map, fuzz weight
+    and roughness maps on the ceiling, and an RGB emission texture (8 x 8) on
+    the light.  The short box is a Mirror (delta bounces carry specular ray
+    differentials) and the tall box NormalizedLambert (next-event estimation
+    through the light tree, whose flux reads the emission texture's maximum).
+    Materials: 0 floor, 1 back wall, 2 ceiling, 3 red, 4 green, 5 mirror,
+    6 Lambert, 7 light."""
+    from .material import Mirror, NormalizedLambert, OpenPbr
+    from .texture import Texture, checkerboard
+
+    s = Scene()
+    floor_m = flat_quads([[(0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)]])
+    floor_m.uvs = floor_m.uvs * np.float32(4.0)
+    back_m = flat_quads([[(0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)]])
+    ceil_m = flat_quads([[(0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)]])
+    red = flat_quads([[(0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)]])
+    green = flat_quads([[(1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)]])
+    light = flat_quads([[(0.4, 0.999, 0.4), (0.6, 0.999, 0.4), (0.6, 0.999, 0.6), (0.4, 0.999, 0.6)]])
+    meshes = [s.add_mesh(m) for m in (floor_m, back_m, ceil_m, red, green)]
+    box = s.add_mesh(unit_box())
+    lm = s.add_mesh(light)
+    eye4 = np.eye(4)
+    for mi, mesh in enumerate(meshes):
+        s.add_instance(mesh, mi, eye4)
+    s.add_instance(box, 5, _translate(0.68, 0.15, 0.36) @ _rot_y(-18.0) @ _scale(0.3, 0.3, 0.3))
+    s.add_instance(box, 6, _translate(0.33, 0.3, 0.64) @ _rot_y(15.0) @ _scale(0.3, 0.6, 0.3))
+    li = s.add_instance(lm, 7, eye4)
+    rng = np.random.default_rng(11)
+    t_checker = s.add_texture(checkerboard(64, 64, 8, (0.9, 0.85, 0.8), (0.15, 0.2, 0.3)))
+    t_rough = s.add_texture(Texture(rng.uniform(0.2, 1.0, (32, 32)).astype(np.float32)))
+    gy, gx = np.mgrid[0:40, 0:48].astype(np.float32)
+    grad = np.stack([gx / 47.0, gy / 39.0, 0.5 + 0.5 * np.sin(gx * 0.3) * np.cos(gy * 0.2)], -1) * 0.8 + 0.1
+    t_grad = s.add_texture(Texture(grad.astype(np.float32)))
+    t_metal = s.add_texture(Texture((np.add.outer(np.arange(16), np.arange(16)) % 5 == 0).astype(np.float32)))
+    t_emit = s.add_texture(Texture((0.8 * np.stack([np.linspace(0.6, 1.0, 64).reshape(8, 8), np.full((8, 8), 0.9),
+                                                    np.linspace(1.0, 0.5, 64).reshape(8, 8)], -1)).astype(np.float32)))
+    if build:
+        s.build_qbvh()
+    mats = [
+        OpenPbr.new((1.0, 1.0, 1.0)).with_(base_color_texture=t_checker, specular_roughness=0.8,
+                                           specular_roughness_texture=t_rough),
+        OpenPbr.new((1.0, 1.0, 1.0)).with_(base_color_texture=t_grad, base_metalness=0.9, specular_roughness=0.25,
+                                           base_metalness_texture=t_metal),
+        OpenPbr.new((0.8, 0.8, 0.8)).with_(specular_weight=0.0, fuzz_weight=1.0, fuzz_roughness=0.9,
+                                           fuzz_color=(0.9, 0.7, 0.6), fuzz_weight_texture=t_rough,
+                                           fuzz_roughness_texture=t_metal),
+        OpenPbr.new((0.8, 0.1, 0.1)).with_(specular_weight=0.0),
+        OpenPbr.new((0.1, 0.8, 0.1)).with_(specular_weight=0.0),
+        Mirror((0.9, 0.9, 0.9)),
+        NormalizedLambert((0.7, 0.7, 0.7)),
+        OpenPbr.new((0.0, 0.0, 0.0)).with_(base_weight=0.0, specular_weight=0.0, emission_luminance=15.0,
+                                           emission_color=(1.0, 1.0, 1.0), emission_color_texture=t_emit),
+    ]
+    for m in mats:
+        s.add_material(m)
+    cam = dict(eye=(0.5, 0.5, -1.4), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov_y_deg=40.0)
+    return CornellScene(s, mats, li, cam)
