@@ -1,10 +1,12 @@
-"""torch.autograd binding of the path-replay adjoint (tpt_render_adjoint).
+"""torch.autograd binding of the path-replay adjoint (tpt_render_adjoint,
+tpt_render_adjoint_texels).
 
 ``render_albedo(albedo, scene, ...)`` renders the scene with the
 NormalizedLambert albedos taken from ``albedo`` (one RGB row per material;
 rows of other materials are ignored) and returns the image as a tensor whose
 backward pass runs the adjoint kernel: ``loss(render_albedo(a, ...)).backward()``
-fills ``a.grad`` with dLoss/d(albedo).
+fills ``a.grad`` with dLoss/d(albedo).  ``render_texture(texels, index, ...)``
+does the same for the level-0 texels of one OpenPBR base-colour texture.
 """
 from __future__ import annotations
 
@@ -45,3 +47,30 @@ def render_albedo(albedo: torch.Tensor, device_scene, materials, camera, width: 
                   max_depth: int) -> torch.Tensor:
     """Differentiable render (see the module docstring)."""
     return AlbedoRender.apply(albedo, device_scene, materials, camera, width, height, spp, max_depth)
+
+
+class TextureRender(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, texels, texture, device_scene, camera, width, height, spp, max_depth):
+        t = texels.detach().to("cpu", torch.float32).numpy()
+        device_scene.set_texture(texture, t)
+        img = device_scene.render(camera, width, height, spp, max_depth)
+        ctx.call = (device_scene, texture, t, camera, width, height, spp, max_depth)
+        return torch.from_numpy(img).to(texels.device)
+
+    @staticmethod
+    def backward(ctx, grad_image):
+        device_scene, texture, t, camera, width, height, spp, max_depth = ctx.call
+        device_scene.set_texture(texture, t)  # the forward's texels
+        _, grads = device_scene.render_adjoint_texels(camera, width, height, spp, max_depth,
+                                                      grad_image.detach().to("cpu", torch.float32).numpy())
+        return torch.from_numpy(grads[texture]).to(grad_image.device), None, None, None, None, None, None, None
+
+
+def render_texture(texels: torch.Tensor, texture: int, device_scene, camera, width: int, height: int, spp: int,
+                   max_depth: int) -> torch.Tensor:
+    """Differentiable render in the level-0 texels of one base-colour texture
+    of the scene (index ``texture``): the forward uploads ``texels``
+    (tpt_scene_set_texture) and renders; the backward runs the path-replay
+    adjoint and returns dLoss/d(texels) (tpt_render_adjoint_texels)."""
+    return TextureRender.apply(texels, texture, device_scene, camera, width, height, spp, max_depth)
