@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--spheres", type=int, default=410)
     ap.add_argument("--rays", type=int, default=2_000_000, help="bounce and shadow ray counts")
     ap.add_argument("--render", type=int, default=1, help="also time the Cornell path tracer (1) or not (0)")
+    ap.add_argument("--render-scenes", default="openpbr,openpbr_glossy,lambert_nee,textured",
+                    help="comma-separated render lines to time (openpbr, openpbr_glossy, lambert_nee, textured)")
+    ap.add_argument("--adjoint", type=int, default=1, help="also time the path-replay adjoint (1) or not (0)")
     ap.add_argument("--refill", type=int, default=0, help="trace-kernel refill threshold (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident blocks per CU (0: occupancy)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
@@ -188,6 +191,7 @@ def main():
         cpu = cpu_baseline(sc, sets, args.cpu_sample)
 
     render = render_bench(dev, stream, args) if args.render else None
+    adjoint = adjoint_bench(dev, stream, args) if args.adjoint else None
 
     line = {
         "metric": "Mrays/s",
@@ -222,6 +226,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "render": render,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     ds.close()
@@ -272,6 +277,8 @@ def render_bench(dev, stream, args):
          "OpenPBR with base-colour, roughness, metalness, fuzz and emission textures (EWA over mip pyramids, "
          "ray differentials and cones), a Mirror box and a NormalizedLambert box with light-tree NEE"),
     ):
+        if key not in args.render_scenes.split(","):
+            continue
         cs = make()
         rds = cs.scene.upload(dev)
         cam = camera.pinhole(**cs.camera, width=w, height=h)
@@ -336,6 +343,61 @@ def render_bench(dev, stream, args):
         dev.set_param("blocks_per_cu", 4)
         dev.set_param("refill_below", 24)
     return out
+
+
+def adjoint_bench(dev, stream, args):
+    """Path-replay adjoint on the inverse-rendering Cornell box: 16 OpenPBR
+    materials with learnable base colours plus a learnable 512x512 RGB
+    base-colour texture, 256x256, 16 spp, depth 6 (an L2 loss's image
+    gradient as input): gradient evaluations per second.  One evaluation =
+    primal + replay of every pixel sample, the parameter and texel gradients
+    (EWA and pyramid adjoints) on the device; specular_roughness and
+    metalness are not differentiated by this build."""
+    import torch
+
+    from tpt import camera, scenes
+
+    cs = scenes.cornell_inverse()
+    ds = cs.scene.upload(dev)
+    w = h = 256
+    spp, depth = 16, 6
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    img = torch.empty(w * h * 3, dtype=torch.float32, device="cuda")
+    ds.render_device(cam, w, h, spp, depth, img)
+    torch.cuda.synchronize()
+    dloss = (img - 0.5).contiguous()  # dL/dI of 0.5 * |I - 0.5|^2
+    n_tex = sum(int(np.prod(t.pixels.shape)) for t in cs.scene.textures)
+    d_alb = torch.empty(3 * len(cs.materials), dtype=torch.float32, device="cuda")
+    d_tex = torch.empty(n_tex, dtype=torch.float32, device="cuda")
+    ds.render_adjoint_texels_device(cam, w, h, spp, depth, dloss, d_alb, d_tex)
+    torch.cuda.synchronize()
+    reps = max(1, min(args.steps, 5))
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        ds.render_adjoint_texels_device(cam, w, h, spp, depth, dloss, d_alb, d_tex)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    # the forward render of the same frame, for the replay's cost relative to it
+    a.record(stream)
+    for _ in range(reps):
+        ds.render_device(cam, w, h, spp, depth, img)
+    b.record(stream)
+    torch.cuda.synchronize()
+    fms = a.elapsed_time(b) / reps
+    ds.close()
+    res = {
+        "workload": "inverse-rendering Cornell box: 16 OpenPBR materials (learnable base_color) + learnable "
+                    "512x512 RGB base-colour texture, 256x256, 16 spp, depth 6, L2 image loss",
+        "parameters": 3 * 16 + n_tex,
+        "ms_per_gradient": round(ms, 3),
+        "grad_per_s": round(1e3 / ms, 2),
+        "samples_per_s": round(w * h * spp / (ms / 1e3), 1),
+        "forward_ms": round(fms, 3),
+    }
+    log(f"adjoint: {ms:.2f} ms per gradient = {1e3 / ms:.1f} grad/s (forward {fms:.2f} ms)")
+    return res
 
 
 def cpu_baseline(sc, sets, sample):

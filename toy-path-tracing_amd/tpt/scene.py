@@ -462,6 +462,18 @@ class DeviceScene:
             o += k
         return out, grads
 
+    def render_adjoint_texels_device(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dev,
+                                     d_albedo_dev, d_texels_dev, fp64: bool = False) -> None:
+        """As render_adjoint_texels with device buffers (torch tensors):
+        dloss (h*w*3), d_albedo (material_count*3), d_texels (every
+        texture's level-0 texels packed)."""
+        d = _lib.RenderDesc()
+        d.camera = camera
+        d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+        flags = _lib.TPT_FLAG_DEVICE_POINTERS | (_lib.TPT_FLAG_GRAD_FP64 if fp64 else 0)
+        check(lib().tpt_render_adjoint_texels(self.handle, ctypes.byref(d), ptr(dloss_dev), ptr(d_albedo_dev),
+                                              ptr(d_texels_dev), flags))
+
     def set_texture(self, texture: int, pixels) -> None:
         """tpt_scene_set_texture: new level-0 texels (same shape) for a texture."""
         px = np.ascontiguousarray(pixels, np.float32)

@@ -459,3 +459,59 @@ def cornell_textured(build: bool = True) -> CornellScene:
         s.add_material(m)
     cam = dict(eye=(0.5, 0.5, -1.4), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov_y_deg=40.0)
     return CornellScene(s, mats, li, cam)
+
+
+def cornell_inverse(build: bool = True, seed: int = 0, tex_size: int = 512) -> CornellScene:
+    """The inverse-rendering Cornell box: the Cornell geometry split into 16
+    OpenPBR materials (five walls, the six faces of the short box, five faces
+    of the tall box) with seeded diffuse base colours, a tex_size² RGB
+    base-colour texture (seeded smooth noise) on the back wall, and the
+    OpenPBR emitter as material 16.  Every material is affine in its base
+    colour (metalness 0, no coat), so the adjoint differentiates all 16 base
+    colours and the texels."""
+    from .material import OpenPbr
+    from .texture import Texture
+
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    walls = [
+        [(0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)],  # floor
+        [(0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)],  # ceiling
+        [(0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)],  # back
+        [(0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)],  # left
+        [(1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)],  # right
+    ]
+    mi = 0
+    for q in walls:
+        s.add_instance(s.add_mesh(flat_quads([q])), mi, np.eye(4))
+        mi += 1
+    box = unit_box()
+    faces = [flat_quads([box.positions[4 * f:4 * f + 4].tolist()]) for f in range(6)]
+    short = _translate(0.68, 0.15, 0.36) @ _rot_y(-18.0) @ _scale(0.3, 0.3, 0.3)
+    tall = _translate(0.33, 0.3, 0.64) @ _rot_y(15.0) @ _scale(0.3, 0.6, 0.3)
+    for f in range(6):
+        s.add_instance(s.add_mesh(faces[f]), mi, short)
+        mi += 1
+    for f in (0, 1, 3, 4, 5):  # the tall box's bottom face (-y) is hidden
+        s.add_instance(s.add_mesh(faces[f]), mi, tall)
+        mi += 1
+    li = s.add_instance(s.add_mesh(flat_quads([[(0.4, 0.999, 0.4), (0.6, 0.999, 0.4), (0.6, 0.999, 0.6),
+                                                 (0.4, 0.999, 0.6)]])), mi, np.eye(4))
+    # smooth seeded noise: a sum of a few random sinusoids per channel
+    y, x = np.mgrid[0:tex_size, 0:tex_size].astype(np.float64) / tex_size
+    px = np.zeros((tex_size, tex_size, 3))
+    for c in range(3):
+        for _ in range(4):
+            fx, fy, ph = rng.uniform(1, 8), rng.uniform(1, 8), rng.uniform(0, 2 * np.pi)
+            px[..., c] += np.sin(2 * np.pi * (fx * x + fy * y) + ph)
+    tex = s.add_texture(Texture((0.55 + 0.1 * px).clip(0.05, 1.0).astype(np.float32)))
+    if build:
+        s.build_qbvh()
+    mats = [OpenPbr.new(tuple(rng.uniform(0.2, 0.9, 3).tolist())).with_(specular_weight=0.0) for _ in range(mi)]
+    mats[2] = mats[2].with_(base_color_texture=tex)
+    mats.append(OpenPbr.new((0.0, 0.0, 0.0)).with_(base_weight=0.0, specular_weight=0.0, emission_luminance=15.0,
+                                                   emission_color=(1.0, 1.0, 1.0)))
+    for m in mats:
+        s.add_material(m)
+    cam = dict(eye=(0.5, 0.5, -1.4), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov_y_deg=40.0)
+    return CornellScene(s, mats, li, cam)
