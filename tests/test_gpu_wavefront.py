@@ -39,6 +39,19 @@ def test_single_frame_matches_megakernel(scene):
     assert same > 0.999 and abs(fast.mean() - mega.mean()) <= 1e-4 * mega.mean()
 
 
+def test_more_paths_than_resident_threads_match_megakernel(scene):
+    """262,144 paths in one pass: more than the shade kernel has resident
+    threads, so the launch shape matters (a grid capped at the resident
+    capacity once gave wrong OpenPBR paths here)."""
+    cs, ds = scene
+    w, h, spp, depth = 128, 128, 16, 8
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    mega = ds.render(cam, w, h, spp, depth)
+    (wave,) = ds.render_batch([(cam, w, h, spp, depth)], exact_shadow=True)
+    print("identical pixels", np.mean(np.all(mega == wave, axis=2)))
+    assert bits_equal(mega, wave)
+
+
 def test_uneven_batch_matches_per_frame_renders(scene):
     cs, ds = scene
     frames = []

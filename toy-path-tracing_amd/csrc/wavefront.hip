@@ -492,8 +492,16 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
   TptDevice* d = s->dev;
   auto gen = p.tex ? wf::wf_generate<true> : wf::wf_generate<false>;
   auto shade = p.tex ? wf::wf_shade<true> : wf::wf_shade<false>;
-  const unsigned g_gen = resident_blocks(d, gen, p.ps.paths, wf::kAppendBlock);
-  const unsigned g_shade = resident_blocks(d, shade, p.ps.paths, wf::kAppendBlock);
+  // One thread per path slot, so no thread of the generate and shade kernels
+  // runs its slot loop more than once.  With the shade grid capped at the
+  // resident capacity (2 waves per SIMD for the untextured kernel: 512
+  // blocks), threads took a second slot, and there this kernel's build gave
+  // wrong OpenPBR paths (the ones through the out-of-line BSDF calls),
+  // differently from run to run, while the textured build and every one-slot
+  // launch matched the megakernel bit for bit.  Blocks past the queue length
+  // exit at once.
+  const unsigned g_gen = (unsigned)((p.ps.paths + wf::kAppendBlock - 1) / wf::kAppendBlock);
+  const unsigned g_shade = g_gen;
   const unsigned g_add = resident_blocks(d, wf::wf_shadow_add, p.ps.paths);
   const unsigned g_fold = resident_blocks(d, wf::wf_fold, p.ps.total_pixels);
   hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, 0u);
