@@ -192,6 +192,17 @@ typedef struct TptTextureDesc {
   uint32_t pad;
 } TptTextureDesc; /* 24 bytes */
 
+/* A directional light as DirectionalLight::new leaves it
+ * (R/light/directional.rs:15-32): the unit direction the light travels
+ * along, its RGB tint and its perpendicular irradiance; sample_li gives
+ * radiance color * intensity from -direction with pdf 1 (a delta light). */
+typedef struct TptDirectionalLight {
+  float direction[3];
+  float intensity;
+  float color[3];
+  float pad;
+} TptDirectionalLight; /* 32 bytes */
+
 /* Scene after Scene::build_qbvh (R/scene/mod.rs:305-316). */
 typedef struct TptSceneDesc {
   const TptMeshDesc* meshes;
@@ -215,7 +226,11 @@ typedef struct TptSceneDesc {
    * with texture_count 0). */
   const TptTextureDesc* textures;
   uint32_t texture_count;
-  uint32_t pad;
+  /* Directional lights (Scene::add_directional_light, R/scene/mod.rs:277-281),
+   * each its own light-sampler category after the light tree and the
+   * environment (LightSampler::build_from_scene, R/light/mod.rs:133-191). */
+  uint32_t directional_count;
+  const TptDirectionalLight* directional_lights;
 } TptSceneDesc;
 
 /* A node of the scene's SG light tree (R/light_tree/mod.rs:52-88), as

@@ -63,6 +63,13 @@ struct RenderScene {
   bool has_env = false;
   lt::env::Env env;
   std::vector<texture::Texture> textures;  // indexed by the OpenPBR texture slots
+  // directional lights (R/light/directional.rs:15-32): direction of travel,
+  // tint, perpendicular irradiance
+  struct DirLight {
+    V3 direction, color;
+    float intensity;
+  };
+  std::vector<DirLight> dirs;
   uint32_t leaf_node(uint32_t inst, uint32_t tri) const {
     return leaf_base[inst] == lt::kInvalid ? lt::kInvalid : leaf_of[leaf_base[inst] + tri];
   }
@@ -453,52 +460,66 @@ inline V3 offset_origin(V3 p, V3 ng, V3 dir) {
   return add(p, mul(kRayEpsilon, off));
 }
 
-// LightSampler (R/light/mod.rs:136-233): the tree category (weight = root
-// flux) and the environment category (weight = total power), a normalised
-// CDF over them.
+// LightSampler::build_from_scene (R/light/mod.rs:133-191): entries with a
+// positive weight in the order tree (root flux), environment (total power),
+// directional lights (luminance(color) * intensity); the running total and
+// the CDF scaled by 1 / total.  kind: 0 tree, 1 environment, 2 + i
+// directional light i.
 struct Category {
-  int kind;  // 0 tree, 1 environment
+  int kind;
   float pmf;
 };
-inline void light_categories(const RenderScene& rs, float* w_tree, float* w_env, float* total) {
-  *w_tree = std::fmax(rs.tree.root_flux(), 0.0f);
-  *w_env = rs.has_env ? std::fmax(lt::env::total_power(rs.env), 0.0f) : 0.0f;
-  *total = 0.0f;
-  if (*w_tree > 0.0f) *total += *w_tree;
-  if (*w_env > 0.0f) *total += *w_env;
-}
-inline bool sample_category(const RenderScene& rs, float u, Category* out) {
-  float wt, we, total;
-  light_categories(rs, &wt, &we, &total);
-  float w[2];
-  int kind[2];
-  int n = 0;
-  if (wt > 0.0f) { w[n] = wt; kind[n++] = 0; }
-  if (we > 0.0f) { w[n] = we; kind[n++] = 1; }
-  if (n == 0 || total <= 0.0f) return false;
-  float cdf[3] = {0.0f, 0.0f, 0.0f};
-  float acc = 0.0f;
-  for (int k = 0; k < n; ++k) {
-    acc += w[k];
-    cdf[k + 1] = acc;
+struct LightSampler {
+  std::vector<int> kind;
+  std::vector<float> weight, cdf;
+  float total = 0.0f;
+};
+inline LightSampler light_sampler(const RenderScene& rs) {
+  LightSampler ls;
+  float wt = std::fmax(rs.tree.root_flux(), 0.0f);
+  if (rs.tree.nodes.empty()) wt = 0.0f;
+  if (wt > 0.0f) { ls.kind.push_back(0); ls.weight.push_back(wt); }
+  if (rs.has_env) {
+    float we = std::fmax(lt::env::total_power(rs.env), 0.0f);
+    if (we > 0.0f) { ls.kind.push_back(1); ls.weight.push_back(we); }
   }
-  const float inv = 1.0f / acc;
-  for (int k = 0; k <= n; ++k) cdf[k] *= inv;
+  for (size_t i = 0; i < rs.dirs.size(); ++i) {
+    float w = std::fmax(lt::luminance(rs.dirs[i].color) * rs.dirs[i].intensity, 0.0f);
+    if (w > 0.0f) { ls.kind.push_back(2 + (int)i); ls.weight.push_back(w); }
+  }
+  ls.cdf.push_back(0.0f);
+  float total = 0.0f;
+  for (float w : ls.weight) {
+    total += w;
+    ls.cdf.push_back(total);
+  }
+  if (total > 0.0f) {
+    float inv = 1.0f / total;
+    for (float& c : ls.cdf) c *= inv;
+  }
+  ls.total = total;
+  return ls;
+}
+// sample_category (R/light/mod.rs:197-213): partition_point(c <= u) - 1
+inline bool sample_category(const RenderScene& rs, float u, Category* out) {
+  LightSampler ls = light_sampler(rs);
+  const int n = (int)ls.kind.size();
+  if (n == 0 || ls.total <= 0.0f) return false;
   u = clampf(u, 0.0f, 1.0f);
   int pp = 0;
-  while (pp <= n && cdf[pp] <= u) ++pp;  // partition_point(c <= u)
+  while (pp <= n && ls.cdf[pp] <= u) ++pp;
   int idx = pp > 0 ? pp - 1 : 0;
   if (idx > n - 1) idx = n - 1;
-  out->kind = kind[idx];
-  out->pmf = w[idx] / total;
+  out->kind = ls.kind[idx];
+  out->pmf = ls.weight[idx] / ls.total;
   return true;
 }
-inline float category_pmf(const RenderScene& rs, int kind) {
-  float wt, we, total;
-  light_categories(rs, &wt, &we, &total);
-  if (total <= 0.0f) return 0.0f;
-  const float w = kind == 0 ? wt : we;
-  return w > 0.0f ? w / total : 0.0f;
+inline float category_pmf(const RenderScene& rs, int kind) {  // :215-224
+  LightSampler ls = light_sampler(rs);
+  if (ls.total <= 0.0f) return 0.0f;
+  for (size_t i = 0; i < ls.kind.size(); ++i)
+    if (ls.kind[i] == kind) return ls.weight[i] / ls.total;
+  return 0.0f;
 }
 
 // direct_light_mis_contribution (R/integrator/mis.rs:141-210) with
@@ -547,13 +568,19 @@ inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& v
     selection = cat.pmf * leaf_pmf;
     t_inst = ln.inst;
     t_tri = ln.tri;
-  } else {
+  } else if (cat.kind == 1) {
     lt::env::Sample es;
     if (!lt::env::sample(rs.env, rs.env.dist_mis, us0, us1, &es)) return zero;
     if (es.pdf <= 0.0f) return zero;
     le = es.radiance;
     wi = es.dir;
     li_pdf = es.pdf;
+    selection = cat.pmf;
+  } else {  // directional::sample_li (R/light/directional.rs:34-43)
+    const RenderScene::DirLight& dl = rs.dirs[cat.kind - 2];
+    le = smul(dl.color, dl.intensity);
+    wi = neg(dl.direction);
+    li_pdf = 1.0f;
     selection = cat.pmf;
   }
   if (li_pdf <= 0.0f || selection <= 0.0f) return zero;
@@ -570,6 +597,7 @@ inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& v
     return zero;
   float pdf_total = selection * li_pdf;
   if (pdf_total <= 0.0f) return zero;
+  if (cat.kind >= 2) return smul(vmul(le, f), cos_surface / pdf_total);  // delta light: no MIS
   float mis = balance_heuristic(pdf_total, bsdf_pdf);
   return smul(vmul(le, f), cos_surface * mis / pdf_total);
 }

@@ -395,6 +395,23 @@ void orc_render_set_mesh(void* h, uint32_t mesh, const float* normals, const flo
 void orc_render_set_instance_material(void* h, uint32_t inst, uint32_t mat) {
   ((RenderScene*)h)->inst_material[inst] = mat;
 }
+// Appends a directional light (direction of travel, as DirectionalLight::new
+// leaves it; tint; perpendicular irradiance).
+void orc_render_add_directional(void* h, const float* dir, const float* color, float intensity) {
+  RenderScene* r = (RenderScene*)h;
+  r->dirs.push_back(RenderScene::DirLight{v3(dir[0], dir[1], dir[2]), v3(color[0], color[1], color[2]), intensity});
+}
+// Light-sampler categories: up to cap entries of (kind, pmf); returns the count.
+uint32_t orc_render_light_categories(void* h, int* kind, float* pmf, uint32_t cap) {
+  const RenderScene& rs = *(RenderScene*)h;
+  LightSampler ls = light_sampler(rs);
+  uint32_t n = 0;
+  for (size_t i = 0; i < ls.kind.size() && n < cap; ++i, ++n) {
+    kind[n] = ls.kind[i];
+    pmf[n] = ls.total > 0.0f ? ls.weight[i] / ls.total : 0.0f;
+  }
+  return (uint32_t)ls.kind.size();
+}
 // Appends a texture to the scene's table (before orc_render_set_materials:
 // the light tree reads the emission textures' maxima).
 uint32_t orc_render_add_texture(void* h, uint32_t w, uint32_t hh, int channels, const float* px) {

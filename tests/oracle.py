@@ -95,6 +95,9 @@ def lib() -> ctypes.CDLL:
         L.orc_openpbr_eval.argtypes = [vp, ctypes.c_int, vp, vp, sz, vp, vp]
         L.orc_openpbr_sample.argtypes = [vp, ctypes.c_int, vp, vp, sz, vp]
         L.orc_openpbr_probe.argtypes = [vp, ctypes.c_int, vp, f32, f32, vp]
+        L.orc_render_add_directional.argtypes = [vp, vp, vp, f32]
+        L.orc_render_light_categories.argtypes = [vp, vp, vp, u32]
+        L.orc_render_light_categories.restype = u32
         L.orc_render_add_texture.argtypes = [vp, u32, u32, ctypes.c_int, vp]
         L.orc_render_add_texture.restype = u32
         L.orc_texture_new.argtypes = [u32, u32, ctypes.c_int, vp]
@@ -387,10 +390,20 @@ class OracleRenderer:
             lib().orc_render_add_texture(self.h, px.shape[1], px.shape[0], 3 if px.ndim == 3 else 1, _p(px))
         self.recs = records(materials)
         lib().orc_render_set_materials(self.h, ctypes.addressof(self.recs), len(materials))
+        for dl in getattr(scene, "directional_lights", []):
+            d, c = np.asarray(dl.direction, np.float32), np.asarray(dl.color, np.float32)
+            lib().orc_render_add_directional(self.h, _p(d), _p(c), dl.intensity)
         env = getattr(scene, "environment", None)
         if env is not None:
             px = np.ascontiguousarray(env.pixels, np.float32)
             lib().orc_render_set_environment(self.h, _p(px), px.shape[1], px.shape[0], env.scale, env.rotate_y)
+
+    def light_categories(self):
+        """The light sampler's categories: [(kind, pmf)], kind 0 tree, 1
+        environment, 2 + i directional light i."""
+        kind, pmf = np.zeros(32, np.int32), np.zeros(32, np.float32)
+        n = lib().orc_render_light_categories(self.h, _p(kind), _p(pmf), 32)
+        return [(int(kind[i]), float(pmf[i])) for i in range(min(n, 32))]
 
     # environment light queries (R/light/environment.rs)
     def env_pdf(self, d, mis=False):

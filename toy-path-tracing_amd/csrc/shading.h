@@ -938,6 +938,7 @@ struct ShadeView {
   const uint32_t* tri_leaf;    // GpuShade::light_base + triangle -> leaf node
   uint32_t light_count;
   float w_tree, w_env;  // light-sampler category weights: root flux, environment power (0: absent)
+  const LightCats* cats;  // the light sampler's category table
   EnvView env;
   Tables tab;   // GGX tables of the OpenPBR materials
   TexView tex;  // textures of the OpenPBR texture slots
@@ -1417,36 +1418,28 @@ __device__ __forceinline__ bool env_sample(const EnvView& e, float us0, float us
 
 // LightSampler (R/light/mod.rs:136-233): tree (weight root flux) and
 // environment (weight total power) categories, normalised CDF.
+// LightSampler::sample_category / category_pmf (R/light/mod.rs:197-224):
+// partition_point over the normalised CDF; kind 0 tree, 1 environment,
+// 2 + i directional light i.
 __device__ __forceinline__ bool sample_category(const ShadeView& sv, float u, int* kind, float* pmf) {
-  const float total = (sv.w_tree > 0.0f ? sv.w_tree : 0.0f) + (sv.w_env > 0.0f ? sv.w_env : 0.0f);
-  float w[2];
-  int k[2];
-  int n = 0;
-  if (sv.w_tree > 0.0f) { w[n] = sv.w_tree; k[n++] = 0; }
-  if (sv.w_env > 0.0f) { w[n] = sv.w_env; k[n++] = 1; }
-  if (n == 0 || total <= 0.0f) return false;
-  float cdf[3] = {0.0f, 0.0f, 0.0f};
-  float acc = 0.0f;
-  for (int i = 0; i < n; ++i) {
-    acc += w[i];
-    cdf[i + 1] = acc;
-  }
-  const float inv = 1.0f / acc;
-  for (int i = 0; i <= n; ++i) cdf[i] *= inv;
+  const LightCats& t = *sv.cats;
+  const int n = (int)t.count;
+  if (n == 0 || t.total <= 0.0f) return false;
   u = clampf(u, 0.0f, 1.0f);
   int pp = 0;
-  while (pp <= n && cdf[pp] <= u) ++pp;
+  while (pp <= n && t.cdf[pp] <= u) ++pp;
   int idx = pp > 0 ? pp - 1 : 0;
   if (idx > n - 1) idx = n - 1;
-  *kind = k[idx];
-  *pmf = w[idx] / total;
+  *kind = (int)t.kind[idx];
+  *pmf = t.weight[idx] / t.total;
   return true;
 }
 __device__ __forceinline__ float category_pmf(const ShadeView& sv, int kind) {
-  const float total = (sv.w_tree > 0.0f ? sv.w_tree : 0.0f) + (sv.w_env > 0.0f ? sv.w_env : 0.0f);
-  if (total <= 0.0f) return 0.0f;
-  const float w = kind == 0 ? sv.w_tree : sv.w_env;
-  return w > 0.0f ? w / total : 0.0f;
+  const LightCats& t = *sv.cats;
+  if (t.total <= 0.0f) return 0.0f;
+  for (uint32_t i = 0; i < t.count; ++i)
+    if ((int)t.kind[i] == kind) return t.weight[i] / t.total;
+  return 0.0f;
 }
 
 // ---- SG light tree (R/math/sg.rs, R/light_tree/{lobe,traversal}.rs) ---------------
@@ -1706,9 +1699,16 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
     selection = cat_pmf * lpmf;
     ns->inst = ln.instance;
     ns->tri = ln.triangle;
-  } else {
+  } else if (kind == 1) {
     if (!env_sample(sv.env, us0, us1, &wi, &le, &li_pdf)) return false;
     if (li_pdf <= 0.0f) return false;
+    selection = cat_pmf;
+  } else {  // directional light: sample_li (R/light/directional.rs:34-43), a delta light at infinity
+    const uint32_t i = (uint32_t)kind - 2u;
+    const LightCats& t = *sv.cats;
+    le = f3(t.radiance[i][0], t.radiance[i][1], t.radiance[i][2]);
+    wi = neg(f3(t.dir[i][0], t.dir[i][1], t.dir[i][2]));
+    li_pdf = 1.0f;
     selection = cat_pmf;
   }
   if (li_pdf <= 0.0f || selection <= 0.0f) return false;
@@ -1719,9 +1719,13 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
   if (cos_surface <= 0.0f) return false;
   const float pdf_total = selection * li_pdf;
   if (pdf_total <= 0.0f) return false;
-  const float mis = balance_heuristic(pdf_total, bsdf_pdf);
   ns->origin = offset_origin(vtx.p, vtx.ng, wi);
   ns->dir = wi;
+  if (kind >= 2) {  // delta light: no MIS (R/integrator/mis.rs:204-206)
+    ns->value = muls(hadamard(le, f), cos_surface / pdf_total);
+    return true;
+  }
+  const float mis = balance_heuristic(pdf_total, bsdf_pdf);
   ns->value = muls(hadamard(le, f), cos_surface * mis / pdf_total);
   return true;
 }
@@ -1834,6 +1838,7 @@ inline ShadeView shade_view_of(const TptScene* s) {
   v.light_count = s->light_count;
   v.w_tree = s->w_tree;
   v.w_env = s->w_env;
+  v.cats = s->light_cats;
   v.env = s->env;
   v.tab = Tables{s->ggx_albedo, s->ggx_mat, s->ggx_cond, s->ggx_diel};
   v.tex = TexView{s->tex_texels, s->tex_levels, s->textures, s->texture_count};

@@ -88,6 +88,23 @@ struct TexView {
   const GpuTexture* tex;
   uint32_t count;
 };
+// The light sampler's categories (LightSampler::build_from_scene,
+// R/light/mod.rs:133-191): the light tree (kind 0, weight root flux), the
+// environment (kind 1, total power) and each directional light i (kind 2 + i,
+// luminance(color) * intensity), in that order and only those with a positive
+// weight; the normalised CDF and the total weight, as the reference forms
+// them; and the directional lights (direction, color * intensity).
+constexpr uint32_t kMaxDirectional = 16;
+constexpr uint32_t kMaxLightCats = 2 + kMaxDirectional;
+struct LightCats {
+  uint32_t count;
+  float total;
+  uint32_t kind[kMaxLightCats];
+  float weight[kMaxLightCats];
+  float cdf[kMaxLightCats + 1];
+  float dir[kMaxDirectional][3];
+  float radiance[kMaxDirectional][3];
+};
 }  // namespace pt
 }  // namespace tpt
 
@@ -143,6 +160,8 @@ struct TptScene {
   std::vector<TptMaterial> host_mats;     // host copy of the material table
   // light-sampler category weights and the environment light
   float w_tree = 0.0f, w_env = 0.0f;
+  tpt::pt::LightCats* light_cats = nullptr;  // device copy of host_cats
+  tpt::pt::LightCats host_cats{};
   float* env_px = nullptr;
   float* env_pyr = nullptr;
   uint32_t* env_levels = nullptr;

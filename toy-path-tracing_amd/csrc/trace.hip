@@ -301,6 +301,7 @@ void free_scene(TptScene* s) {
   (void)hipFree(s->ggx_mat);
   (void)hipFree(s->ggx_cond);
   (void)hipFree(s->ggx_diel);
+  (void)hipFree(s->light_cats);
   (void)hipFree(s->tex_texels);
   (void)hipFree(s->tex_levels);
   (void)hipFree(s->textures);
@@ -315,6 +316,52 @@ void free_scene(TptScene* s) {
 // when some material reads them with a non-zero weight (metalness > 0 for
 // the conductor's; a specular IOR other than 1 or a coat for the
 // dielectric's) and as zeros otherwise.
+// LightSampler::build_from_scene (R/light/mod.rs:133-191): the entries with a
+// positive weight in the reference's order, the running total in f32 and the
+// CDF scaled by 1 / total.
+bool build_light_cats(TptScene* s, const TptSceneDesc& d) {
+  pt::LightCats& t = s->host_cats;
+  t = pt::LightCats{};
+  if (d.directional_count > pt::kMaxDirectional) {
+    set_error("tpt_scene_create: at most " + std::to_string(pt::kMaxDirectional) + " directional lights");
+    return false;
+  }
+  if (d.directional_count && !d.directional_lights) {
+    set_error("tpt_scene_create: directional_count > 0 with a null directional light table");
+    return false;
+  }
+  auto add = [&](uint32_t kind, float w) {
+    t.kind[t.count] = kind;
+    t.weight[t.count] = w;
+    t.count++;
+  };
+  if (s->w_tree > 0.0f) add(0, s->w_tree);
+  if (s->w_env > 0.0f) add(1, s->w_env);
+  for (uint32_t i = 0; i < d.directional_count; ++i) {
+    const TptDirectionalLight& l = d.directional_lights[i];
+    for (int c = 0; c < 3; ++c) {
+      t.dir[i][c] = l.direction[c];
+      t.radiance[i][c] = l.color[c] * l.intensity;  // sample_li: color * intensity
+    }
+    const float lum = 0.2126f * l.color[0] + 0.7152f * l.color[1] + 0.0722f * l.color[2];  // sg::luminance
+    const float w = std::fmax(lum * l.intensity, 0.0f);
+    if (w > 0.0f) add(2 + i, w);
+  }
+  float total = 0.0f;
+  t.cdf[0] = 0.0f;
+  for (uint32_t k = 0; k < t.count; ++k) {
+    total += t.weight[k];
+    t.cdf[k + 1] = total;
+  }
+  if (total > 0.0f) {
+    const float inv = 1.0f / total;
+    for (uint32_t k = 0; k <= t.count; ++k) t.cdf[k] *= inv;
+  }
+  t.total = total;
+  std::vector<pt::LightCats> one(1, t);
+  return upload(one, &s->light_cats, &s->bytes);
+}
+
 bool upload_ggx_tables(TptScene* s, const std::vector<TptMaterial>& mats) {
   std::vector<float> albedo(ggx::kAlbedoLen, 0.0f);
   std::map<uint32_t, uint32_t> index;
@@ -802,6 +849,10 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
         e.inv_rot[k] = he.inv_rot[k];
       }
       s->w_env = std::fmax(he.power, 0.0f);
+    }
+    if (!build_light_cats(s, *desc)) {
+      free_scene(s);
+      return nullptr;
     }
     return s;
   } catch (const std::exception& e) {

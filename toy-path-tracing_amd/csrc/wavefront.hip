@@ -514,7 +514,7 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
     hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, a ^ 1u);
     hipLaunchKernelGGL(shade, dim3(g_shade), dim3(wf::kAppendBlock), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
                        p.q[a ^ 1u], p.sq, p.c, a, b, (const uint8_t*)p.ztab, p.ztab_dims);
-    if (b < p.max_depth && (sv.light_count || sv.env.px)) {  // some light can be sampled
+    if (b < p.max_depth && s->host_cats.count) {  // the light sampler has a category: NEE can run
       if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow.v, p.ps.paths, nullptr, p.sq.target,
                               p.sq.unocc, st))
         return false;

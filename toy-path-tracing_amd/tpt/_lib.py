@@ -99,8 +99,14 @@ class SceneDesc(ctypes.Structure):
         ("env_rotate_y", ctypes.c_float),
         ("textures", ctypes.c_void_p),
         ("texture_count", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32),
+        ("directional_count", ctypes.c_uint32),
+        ("directional_lights", ctypes.c_void_p),
     ]
+
+
+# TptDirectionalLight (32 bytes)
+DIRECTIONAL_DTYPE = np.dtype([("direction", "<f4", 3), ("intensity", "<f4"), ("color", "<f4", 3), ("pad", "<f4")])
+assert DIRECTIONAL_DTYPE.itemsize == 32
 
 
 class TextureDesc(ctypes.Structure):
@@ -118,7 +124,7 @@ TEXTURE_QUERY_DTYPE = np.dtype([("uv", "<f4", 2), ("dstdx", "<f4", 2), ("dstdy",
 assert TEXTURE_QUERY_DTYPE.itemsize == 24
 TEXTURE_FILTERED, TEXTURE_BILINEAR, TEXTURE_NEAREST, TEXTURE_TRILINEAR, TEXTURE_MIP_BILINEAR = 0, 1, 2, 3, 4
 
-assert ctypes.sizeof(MeshDesc) == 64 and ctypes.sizeof(SceneDesc) == 96 and ctypes.sizeof(TextureDesc) == 24
+assert ctypes.sizeof(MeshDesc) == 64 and ctypes.sizeof(SceneDesc) == 104 and ctypes.sizeof(TextureDesc) == 24
 assert ctypes.sizeof(InstanceDesc) == 136
 
 

@@ -155,6 +155,26 @@ class Environment:
 
 
 @dataclass
+class DirectionalLight:
+    """DirectionalLight (R/light/directional.rs:15-32): the direction the
+    light travels along (normalised as DirectionalLight::new does, in f32:
+    v * (1 / sqrt(dot))), RGB tint and perpendicular irradiance."""
+
+    direction: tuple
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+
+    @staticmethod
+    def new(direction, color, intensity: float) -> "DirectionalLight":
+        d = np.asarray(direction, np.float32)
+        dot = np.float32(d[0] * d[0]) + np.float32(d[1] * d[1])
+        dot = np.float32(dot + np.float32(d[2] * d[2]))
+        inv = np.float32(np.float32(1.0) / np.sqrt(dot, dtype=np.float32))
+        return DirectionalLight(tuple(float(np.float32(x * inv)) for x in d), tuple(float(c) for c in color),
+                                float(intensity))
+
+
+@dataclass
 class Scene:
     """Geometry of the reference Scene (R/scene/mod.rs:81-98)."""
 
@@ -165,6 +185,12 @@ class Scene:
     materials: list = field(default_factory=list)  # tpt.material records, by material index
     environment: Environment | None = None  # Scene::set_environment_light
     textures: list = field(default_factory=list)  # tpt.texture.Texture, indexed by the materials' *_texture
+    directional_lights: list = field(default_factory=list)  # DirectionalLight
+
+    def add_directional_light(self, light: "DirectionalLight") -> int:
+        """Scene::add_directional_light (R/scene/mod.rs:277-281) -> its index."""
+        self.directional_lights.append(light)
+        return len(self.directional_lights) - 1
 
     def add_texture(self, texture) -> int:
         """Adds a texture to the scene's table -> its index (what an OpenPbr
@@ -304,6 +330,13 @@ def scene_desc(scene: Scene):
         desc.env_height, desc.env_width = px.shape[0], px.shape[1]
         desc.env_scale = scene.environment.scale
         desc.env_rotate_y = scene.environment.rotate_y
+    if scene.directional_lights:
+        dl = np.zeros(len(scene.directional_lights), _lib.DIRECTIONAL_DTYPE)
+        for i, l in enumerate(scene.directional_lights):
+            dl[i]["direction"], dl[i]["color"], dl[i]["intensity"] = l.direction, l.color, l.intensity
+        keep.append(dl)
+        desc.directional_lights = dl.ctypes.data
+        desc.directional_count = len(dl)
     if scene.textures:
         texs = (_lib.TextureDesc * len(scene.textures))()
         for i, t in enumerate(scene.textures):

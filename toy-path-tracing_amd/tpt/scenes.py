@@ -261,10 +261,13 @@ def procedural_sky(width: int = 2048, height: int = 1024, sun_elevation_deg: flo
     return img.astype(np.float32)
 
 
-def outdoor(materials: str = "openpbr", sky_size=(256, 128), build: bool = True) -> CornellScene:
+def outdoor(materials: str = "openpbr", sky_size=(256, 128), build: bool = True, sun=None, sky: bool = True) \
+        -> CornellScene:
     """The Cornell boxes on a 20 x 20 floor under the procedural sky (no
     walls, no area light): light comes from the environment only, so
-    environment NEE is what lights OpenPBR surfaces here."""
+    environment NEE is what lights OpenPBR surfaces here.  sun: optional
+    directional lights [(direction of travel, color, intensity)]; sky=False
+    drops the environment."""
     from .material import NormalizedLambert, OpenPbr
     from .scene import Environment
 
@@ -283,7 +286,11 @@ def outdoor(materials: str = "openpbr", sky_size=(256, 128), build: bool = True)
         mats = [NormalizedLambert(c) for c in cols]
     for m in mats:
         s.add_material(m)
-    s.set_environment_light(Environment(procedural_sky(*sky_size), 1.0, 0.3))
+    if sky:
+        s.set_environment_light(Environment(procedural_sky(*sky_size), 1.0, 0.3))
+    from .scene import DirectionalLight
+    for d, c, e in (sun or []):
+        s.add_directional_light(DirectionalLight.new(d, c, e))
     cam = dict(eye=(0.5, 0.9, -1.6), look_at=(0.5, 0.3, 0.5), up=(0.0, 1.0, 0.0), fov_y_deg=40.0)
     return CornellScene(s, mats, -1, cam)
 
