@@ -398,6 +398,10 @@ struct Bsdf {
   float coat, cax, cay, coat_eta;
   float fuzz, fuzz_rough, diff_rough;
   float spec_rough;  // effective specular roughness (a glossy sample's cone spread)
+  // directional albedos at the make_bsdf wo (they depend on wo only, and
+  // every query of one Bsdf uses that wo): fuzz (Zeltner), coat and specular
+  // (GGX tables with multiple-scattering compensation)
+  float e_fuzz, e_coat, e_spec;
   bool front;
   const float* spec_tab;  // directional-albedo table of spec_eta
   const float* coat_tab;  // of coat_eta
@@ -456,8 +460,9 @@ __device__ __forceinline__ F3 coat_darkening(const TptOpenPbr& m, F3 base_color,
   const float omk = fmaxf(1.0f - k, 0.0f);
   return clamp01(lerp3(f3s(1.0f), f3(omk / den.x, omk / den.y, omk / den.z), cw * delta));
 }
-// OpenPbrMaterial::make_bsdf (:393-487) for material `mi`; wol: the shading-frame wo
-__device__ __forceinline__ Bsdf make_bsdf(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol) {
+// OpenPbrMaterial::make_bsdf (:393-487) for material `mi`: the parameters
+__device__ __forceinline__ Bsdf make_bsdf_params(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front,
+                                                 F3 wol) {
   Bsdf p;
   p.base_color = hadamard(f3(m.base_color[0], m.base_color[1], m.base_color[2]), f3s(1.0f));
   p.metal = clampf(m.base_metalness * 1.0f, 0.0f, 1.0f);
@@ -551,25 +556,34 @@ struct LayerW {
   float coat_amp;
   F3 metal, spec, fuzz, diff;
 };
-// layer_weights (:333-390) without transmission, subsurface, thin walls
+// make_bsdf: the parameters and the wo-only directional albedos that
+// layer_weights reads (the same expressions, evaluated once per vertex).
+__device__ __forceinline__ Bsdf make_bsdf(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol) {
+  Bsdf p = make_bsdf_params(tb, mi, m, front, wol);
+  p.e_fuzz = p.fuzz <= 0.0f ? 0.0f : zeltner_albedo(clampf(wol.z, 0.0f, 1.0f), clampf(p.fuzz_rough, 0.01f, 1.0f));
+  p.e_coat = (p.coat > 0.0f && wol.z > 0.0f)
+                 ? albedo_ms(p, wol, p.cax, p.cay, p.coat_eta,
+                             albedo_lookup(p.coat_tab, p.coat_eta, wol, rough_proxy(p.cax, p.cay)))
+                 : 0.0f;
+  p.e_spec = wol.z <= 0.0f ? 0.0f
+                           : albedo_ms(p, wol, p.ax, p.ay, p.spec_eta,
+                                       albedo_lookup(p.spec_tab, p.spec_eta, wol, rough_proxy(p.ax, p.ay)));
+  return p;
+}
+// layer_weights (:333-390) without transmission, subsurface, thin walls; wo
+// is the make_bsdf wo (its albedos come from p)
 __device__ __forceinline__ LayerW layer_weights(const Bsdf& p, F3 wo, const F3* wi) {
-  const float e_fuzz = p.fuzz <= 0.0f ? 0.0f : zeltner_albedo(clampf(wo.z, 0.0f, 1.0f), clampf(p.fuzz_rough, 0.01f, 1.0f));
-  const float bfs = fmaxf(1.0f - p.fuzz * e_fuzz, 0.0f);
+  const float bfs = fmaxf(1.0f - p.fuzz * p.e_fuzz, 0.0f);
   F3 under = f3s(bfs);
   if (p.coat > 0.0f) {
-    const float e_coat =
-        wo.z <= 0.0f ? 0.0f
-                     : albedo_ms(p, wo, p.cax, p.cay, p.coat_eta, albedo_lookup(p.coat_tab, p.coat_eta, wo, rough_proxy(p.cax, p.cay)));
     const F3 tint = wi ? coat_tint(p, wo, *wi) : p.coat_color;
-    const F3 thr = hadamard(hadamard(p.coat_dark, tint), f3s(fmaxf(1.0f - e_coat, 0.0f)));
+    const F3 thr = hadamard(hadamard(p.coat_dark, tint), f3s(fmaxf(1.0f - p.e_coat, 0.0f)));
     under = hadamard(f3s(bfs), add(muls(f3s(1.0f), 1.0f - p.coat), muls(thr, p.coat)));
   }
   LayerW w;
   w.metal = muls(under, p.metal);
   const F3 below_metal = muls(under, 1.0f - p.metal);
-  const float e_spec =
-      wo.z <= 0.0f ? 0.0f
-                   : albedo_ms(p, wo, p.ax, p.ay, p.spec_eta, albedo_lookup(p.spec_tab, p.spec_eta, wo, rough_proxy(p.ax, p.ay)));
+  const float e_spec = p.e_spec;
   w.spec = hadamard(below_metal, p.spec_color);
   const F3 leak = vmaxs(sub(f3s(1.0f), muls(p.spec_color, e_spec)), 0.0f);
   const F3 below_trans = muls(hadamard(below_metal, leak), 1.0f - 0.0f);
