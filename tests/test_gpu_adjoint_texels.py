@@ -68,9 +68,16 @@ def setup(device, request):
     ds.close()
 
 
+def frame(ds, cam, spp, depth):
+    """One image through the wavefront path tracer: identical to
+    ds.render (checked in test_gpu_wavefront.py) and, at 32 x 32 pixels with
+    thousands of samples each, far faster than one thread per pixel."""
+    return ds.render_batch([(cam, W, H, spp, depth)])[0].astype(np.float64)
+
+
 def render(ds, mats, cam, base, spp, depth):
     ds.set_materials(with_base(mats, base))
-    return ds.render(cam, W, H, spp, depth).astype(np.float64)
+    return frame(ds, cam, spp, depth)
 
 
 def rel_l2(a, b):
@@ -111,10 +118,10 @@ def check(setup, depth, spp, h, tol):
         px = px0.copy()
         px[y, x, c] += h
         ds.set_texture(tex[0], px)
-        lp = ((ds.render(cam, W, H, spp, depth).astype(np.float64) - target) ** 2).sum()
+        lp = ((frame(ds, cam, spp, depth) - target) ** 2).sum()
         px[y, x, c] -= 2 * h
         ds.set_texture(tex[0], px)
-        lm = ((ds.render(cam, W, H, spp, depth).astype(np.float64) - target) ** 2).sum()
+        lm = ((frame(ds, cam, spp, depth) - target) ** 2).sum()
         fdt.append((lp - lm) / (2 * h))
         adt.append(gt[y, x, c])
     ds.set_texture(tex[0], px0)

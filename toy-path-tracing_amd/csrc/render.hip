@@ -60,7 +60,7 @@ __device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView&
                                            const Vtx& vtx, float u_root, float u_tree, float us0, float us1,
                                            Lane& L, LaneStack& st, F3* dir = nullptr) {
   NeeSample ns;
-  if (!nee_sample<kTex>(sv, m, vtx, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
+  if (!nee_sample<kTex, true>(sv, m, vtx, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
   if (dir) *dir = ns.dir;
   uint32_t hi, ht;
   float tt, hb0, hb1, hb2;
@@ -143,7 +143,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     F3 wi;
     float ud[2];
     ps.s2(base + 7, ud);
-    const bool sampled = mat_sample(sv.tab, m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi);
+    const bool sampled = mat_sample<true>(sv.tab, m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi);
     if (kAdj && base_color_linear(m)) {
       const F3 rest = sub(adj.total, radiance);  // collected after this vertex's NEE
       F3 g = f3(0.0f, 0.0f, 0.0f);
@@ -348,7 +348,8 @@ __global__ __launch_bounds__(256) void bsdf_query_kernel(pt::Tables tb, const Tp
       bool ok = false;
       F3 f = f3(0.0f, 0.0f, 0.0f);
       if (m.kind == TPT_MATERIAL_OPEN_PBR) {
-        const pt::Bsdf b = pt::make_bsdf(tb, a.material, m.u.open_pbr, a.front_face != 0, wo);
+        pt::Bsdf b;
+        pt::make_bsdf(tb, a.material, m.u.open_pbr, a.front_face != 0, wo, &b);
         f = pt::bsdf_eval(b, wo, wi);
         r.pdf = pt::bsdf_pdf(b, wo, wi);
         ok = pt::bsdf_sample(b, wo, a.u[0], a.u[1], a.u[2], &smp);

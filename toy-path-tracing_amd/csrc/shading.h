@@ -392,6 +392,15 @@ __device__ __forceinline__ float zeltner_b(float ndv, float rough) {
          (0.0000254053f + 1.71228f * x - 1.71506f * x * y + 1.34174f * y * y);
 }
 
+struct MsParams {
+  float eta_o, req, e_avg_o, ome_o;
+};
+struct TwoLobe {
+  float ss, ms, total;
+};
+struct Probs {
+  float coat, metal, spec, fuzz, diff, total;
+};
 struct Bsdf {
   F3 base_color, spec_color, coat_color, coat_dark, fuzz_color;
   float base, specular, ax, ay, spec_eta, trans_eta, metal;
@@ -402,6 +411,16 @@ struct Bsdf {
   // every query of one Bsdf uses that wo): fuzz (Zeltner), coat and specular
   // (GGX tables with multiple-scattering compensation)
   float e_fuzz, e_coat, e_spec;
+  // more per-vertex constants (make_bsdf, where the lobe reads them): the
+  // specular and coat lobes' multiple-scattering parameters, factor, table
+  // value at wo and two-lobe weights; the conductor's table terms at wo, its
+  // average Fresnel and multiple-scattering colour; the lobe probabilities
+  MsParams ms_s, ms_c;
+  float msf_s, msf_c, eo_s, eo_c;
+  TwoLobe tl_s, tl_c, tl_m;
+  float m_req, m_eo, m_omea;
+  F3 m_fms;
+  Probs pr;
   bool front;
   const float* spec_tab;  // directional-albedo table of spec_eta
   const float* coat_tab;  // of coat_eta
@@ -511,9 +530,6 @@ __device__ __forceinline__ bool coat_smooth(const Bsdf& p) { return p.coat <= 0.
 __device__ __forceinline__ bool metal_smooth(const Bsdf& p) { return p.metal <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
 __device__ __forceinline__ bool spec_smooth(const Bsdf& p) { return p.specular <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
 
-struct MsParams {
-  float eta_o, req, e_avg_o, ome_o;
-};
 __device__ __forceinline__ MsParams ms_params(const Bsdf& p, float ax, float ay, float eta) {  // :593-625
   MsParams s;
   s.eta_o = p.front ? fmaxf(eta, 1.0e-4f) : 1.0f / fmaxf(eta, 1.0e-4f);
@@ -527,9 +543,6 @@ __device__ __forceinline__ float ms_factor(const MsParams& s) {  // :573-577
   const float den = 1.0f - fa * fmaxf(1.0f - s.e_avg_o, 0.0f);
   return clampf(fa * fa * s.e_avg_o / fmaxf(den, kMsEps), 0.0f, 1.0f);
 }
-struct TwoLobe {
-  float ss, ms, total;
-};
 __device__ __forceinline__ TwoLobe refl_lobes(const MsParams& s) {  // :627-636
   const float fa = clampf(f_avg_dielectric(s.eta_o), 0.0f, 1.0f);
   const float ss = clampf(fa * s.e_avg_o, 0.0f, 1.0f);
@@ -541,11 +554,10 @@ __device__ __forceinline__ TwoLobe cond_lobes(const Bsdf& p) {  // :579-591
   const float ms = fmaxf(1.0f - e, 0.0f);
   return TwoLobe{e, ms, fmaxf(e + ms, 1.0e-8f)};
 }
-__device__ __forceinline__ float albedo_ms(const Bsdf& p, F3 wo, float ax, float ay, float eta, float ss) {  // :464-481
+// albedo_ms (:464-481) with the lobe's cached ms_factor and table value at wo
+__device__ __forceinline__ float albedo_ms(F3 wo, float ax, float ay, float msf, float eo, float ss) {
   if (wo.z <= 0.0f || fmaxf(ax, ay) < kSmoothAlpha) return clampf(ss, 0.0f, 1.0f);
-  const MsParams s = ms_params(p, ax, ay, eta);
-  const float eo = diel_e(p.diel, wo.z, s.req, s.eta_o);
-  return clampf(ss + ms_factor(s) * fmaxf(1.0f - eo, 0.0f), 0.0f, 1.0f);
+  return clampf(ss + msf * fmaxf(1.0f - eo, 0.0f), 0.0f, 1.0f);
 }
 __device__ __forceinline__ F3 coat_tint(const Bsdf& p, F3 wo, F3 wi) {  // :483-492
   if (wo.z <= 0.0f || wi.z <= 0.0f) return f3s(1.0f);
@@ -556,20 +568,6 @@ struct LayerW {
   float coat_amp;
   F3 metal, spec, fuzz, diff;
 };
-// make_bsdf: the parameters and the wo-only directional albedos that
-// layer_weights reads (the same expressions, evaluated once per vertex).
-__device__ __forceinline__ Bsdf make_bsdf(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol) {
-  Bsdf p = make_bsdf_params(tb, mi, m, front, wol);
-  p.e_fuzz = p.fuzz <= 0.0f ? 0.0f : zeltner_albedo(clampf(wol.z, 0.0f, 1.0f), clampf(p.fuzz_rough, 0.01f, 1.0f));
-  p.e_coat = (p.coat > 0.0f && wol.z > 0.0f)
-                 ? albedo_ms(p, wol, p.cax, p.cay, p.coat_eta,
-                             albedo_lookup(p.coat_tab, p.coat_eta, wol, rough_proxy(p.cax, p.cay)))
-                 : 0.0f;
-  p.e_spec = wol.z <= 0.0f ? 0.0f
-                           : albedo_ms(p, wol, p.ax, p.ay, p.spec_eta,
-                                       albedo_lookup(p.spec_tab, p.spec_eta, wol, rough_proxy(p.ax, p.ay)));
-  return p;
-}
 // layer_weights (:333-390) without transmission, subsurface, thin walls; wo
 // is the make_bsdf wo (its albedos come from p)
 __device__ __forceinline__ LayerW layer_weights(const Bsdf& p, F3 wo, const F3* wi) {
@@ -592,9 +590,6 @@ __device__ __forceinline__ LayerW layer_weights(const Bsdf& p, F3 wo, const F3* 
   w.fuzz = muls(p.fuzz_color, p.fuzz);
   return w;
 }
-struct Probs {
-  float coat, metal, spec, fuzz, diff, total;
-};
 __device__ __forceinline__ Probs lobe_probs(const Bsdf& p, F3 wo) {  // :392-422
   const LayerW w = layer_weights(p, wo, nullptr);
   Probs r;
@@ -607,11 +602,11 @@ __device__ __forceinline__ Probs lobe_probs(const Bsdf& p, F3 wo) {  // :392-422
   r.total = r.coat + r.metal + r.spec + 0.0f + r.fuzz + r.diff + zero;
   return r;
 }
-__device__ __forceinline__ float refl_ms_eval(const Bsdf& p, F3 wo, F3 wi, float ax, float ay, float eta) {  // :648-670
+// refl_ms_eval (:648-670) with the lobe's cached parameters, factor and table value at wo
+__device__ __forceinline__ float refl_ms_eval(const Bsdf& p, F3 wo, F3 wi, const MsParams& s, float msf, float eo) {
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
-  const MsParams s = ms_params(p, ax, ay, eta);
-  const float eo = diel_e(p.diel, wo.z, s.req, s.eta_o), ei = diel_e(p.diel, wi.z, s.req, s.eta_o);
-  return ms_factor(s) * (1.0f - eo) * (1.0f - ei) / (kPi * s.ome_o);
+  const float ei = diel_e(p.diel, wi.z, s.req, s.eta_o);
+  return msf * (1.0f - eo) * (1.0f - ei) / (kPi * s.ome_o);
 }
 __device__ __forceinline__ float coat_eval(const Bsdf& p, F3 wo, F3 wi) {  // :698-722
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
@@ -620,7 +615,7 @@ __device__ __forceinline__ float coat_eval(const Bsdf& p, F3 wo, F3 wi) {  // :6
   const float d = ggx_d(wm, p.cax, p.cay), g = ggx_g2(wo, wi, p.cax, p.cay);
   if (d <= 0.0f || g <= 0.0f) return 0.0f;
   const float ss = d * g * fresnel_dielectric(fabsf(dot(wo, wm)), 1.0f, p.coat_eta) / (4.0f * wo.z * wi.z);
-  return ss + refl_ms_eval(p, wo, wi, p.cax, p.cay, p.coat_eta);
+  return ss + refl_ms_eval(p, wo, wi, p.ms_c, p.msf_c, p.eo_c);
 }
 __device__ __forceinline__ float cos_pdf(float c) { return fmaxf(c, 0.0f) / kPi; }
 __device__ __forceinline__ float two_lobe_pdf(TwoLobe w, F3 wo, F3 wi, float ax, float ay, bool early_ms) {
@@ -654,15 +649,72 @@ __device__ __forceinline__ F3 metal_eval(const Bsdf& p, F3 wo, F3 wi) {  // :744
   const float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
   if (d <= 0.0f || g <= 0.0f) return zero;
   const F3 ss = muls(metal_fresnel(p, fabsf(dot(wo, wm))), d * g / (4.0f * wo.z * wi.z));
+  const float ei = cond_e(p.cond, wi.z, p.m_req);
+  return add(ss, muls(p.m_fms, (1.0f - p.m_eo) * (1.0f - ei) / (kPi * p.m_omea)));
+}
+// the conductor's per-vertex terms of metal_eval: equivalent roughness, table
+// values at wo, 1 - energy average and multiple-scattering colour
+__device__ __forceinline__ void metal_constants(Bsdf& p, F3 wo) {
   const float req = powf(p.ax * p.ay, 0.25f);
-  const float eo = cond_e(p.cond, wo.z, req), ei = cond_e(p.cond, wi.z, req), ea = cond_eavg(p.cond, req);
-  const float omea = fmaxf(1.0f - ea, kMsEps);
+  p.m_req = req;
+  p.m_eo = cond_e(p.cond, wo.z, req);
+  const float ea = cond_eavg(p.cond, req);
+  p.m_omea = fmaxf(1.0f - ea, kMsEps);
   const F3 fa = metal_fresnel_avg(p);
   const F3 den = sub(f3s(1.0f), muls(fa, fmaxf(1.0f - ea, 0.0f)));
   const F3 ds = vmaxs(den, kMsEps);
   const F3 fms0 = muls(hadamard(fa, fa), ea);
-  const F3 fms = f3(fms0.x / ds.x, fms0.y / ds.y, fms0.z / ds.z);
-  return add(ss, muls(fms, (1.0f - eo) * (1.0f - ei) / (kPi * omea)));
+  p.m_fms = f3(fms0.x / ds.x, fms0.y / ds.y, fms0.z / ds.z);
+}
+// make_bsdf: the parameters plus every wo-only term that layer_weights, the
+// lobe evaluations, pdf and sample read (the same expressions, evaluated once
+// per vertex under the conditions their readers test).
+__device__ __forceinline__ void make_bsdf(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol,
+                                          Bsdf* out) {
+  Bsdf& p = *out;
+  p = make_bsdf_params(tb, mi, m, front, wol);
+  const bool rough_s = !(fmaxf(p.ax, p.ay) < kSmoothAlpha), rough_c = !(fmaxf(p.cax, p.cay) < kSmoothAlpha);
+  if (rough_s) {
+    p.ms_s = ms_params(p, p.ax, p.ay, p.spec_eta);
+    p.msf_s = ms_factor(p.ms_s);
+    p.tl_s = refl_lobes(p.ms_s);
+    if (wol.z > 0.0f) p.eo_s = diel_e(p.diel, wol.z, p.ms_s.req, p.ms_s.eta_o);
+  }
+  if (p.coat > 0.0f && rough_c) {
+    p.ms_c = ms_params(p, p.cax, p.cay, p.coat_eta);
+    p.msf_c = ms_factor(p.ms_c);
+    p.tl_c = refl_lobes(p.ms_c);
+    if (wol.z > 0.0f) p.eo_c = diel_e(p.diel, wol.z, p.ms_c.req, p.ms_c.eta_o);
+  }
+  if (p.metal > 0.0f && rough_s) {
+    p.tl_m = cond_lobes(p);
+    if (wol.z > 0.0f) metal_constants(p, wol);
+  }
+  p.e_fuzz = p.fuzz <= 0.0f ? 0.0f : zeltner_albedo(clampf(wol.z, 0.0f, 1.0f), clampf(p.fuzz_rough, 0.01f, 1.0f));
+  p.e_coat = (p.coat > 0.0f && wol.z > 0.0f)
+                 ? albedo_ms(wol, p.cax, p.cay, p.msf_c, p.eo_c,
+                             albedo_lookup(p.coat_tab, p.coat_eta, wol, rough_proxy(p.cax, p.cay)))
+                 : 0.0f;
+  p.e_spec = wol.z <= 0.0f ? 0.0f
+                           : albedo_ms(wol, p.ax, p.ay, p.msf_s, p.eo_s,
+                                       albedo_lookup(p.spec_tab, p.spec_eta, wol, rough_proxy(p.ax, p.ay)));
+  if (wol.z > 0.0f) p.pr = lobe_probs(p, wol);
+}
+// The same as a call: the Bsdf lives in memory anyway (eval, pdf and sample
+// take it by reference), and a caller that keeps a whole path in registers
+// (the one-thread-per-pixel kernels) keeps its register budget for the path
+// loop.  The wavefront shade kernel, whose live state is small, inlines it.
+__device__ __noinline__ void make_bsdf_call(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol,
+                                            Bsdf* out) {
+  make_bsdf(tb, mi, m, front, wol, out);
+}
+template <bool kOutline>
+__device__ __forceinline__ void bsdf_at(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol,
+                                        Bsdf* out) {
+  if constexpr (kOutline)
+    make_bsdf_call(tb, mi, m, front, wol, out);
+  else
+    make_bsdf(tb, mi, m, front, wol, out);
 }
 __device__ __forceinline__ float base_fresnel(const Bsdf& p, float c) {  // base_dielectric_fresnel :902-929
   const float ep = p.trans_eta, epr = p.spec_eta;
@@ -684,7 +736,7 @@ __device__ __forceinline__ float spec_eval(const Bsdf& p, F3 wo, F3 wi) {  // ev
   const float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
   if (d <= 0.0f || g <= 0.0f) return 0.0f;
   const float ss = base_fresnel(p, fabsf(dot(wo, wm))) * (d * g / (4.0f * wo.z * wi.z));
-  return ss + refl_ms_eval(p, wo, wi, p.ax, p.ay, p.spec_eta);
+  return ss + refl_ms_eval(p, wo, wi, p.ms_s, p.msf_s, p.eo_s);
 }
 __device__ __forceinline__ float spec_ss_pdf(const Bsdf& p, F3 wo, F3 wi) {  // :981-994
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
@@ -766,15 +818,15 @@ __device__ __noinline__ F3 bsdf_eval_dbase(const Bsdf& p, F3 wo, F3 wi) {
 // OpenPbrBsdf::pdf (:261-293)
 __device__ __noinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {
   if (!(wo.z > 0.0f)) return 0.0f;
-  const Probs pr = lobe_probs(p, wo);
+  const Probs& pr = p.pr;
   if (pr.total <= 0.0f) return 0.0f;
   float pdf = 0.0f;
   if (pr.coat > 0.0f && !coat_smooth(p) && wo.z > 0.0f && wi.z > 0.0f)
-    pdf += (pr.coat / pr.total) * two_lobe_pdf(refl_lobes(ms_params(p, p.cax, p.cay, p.coat_eta)), wo, wi, p.cax, p.cay, true);
+    pdf += (pr.coat / pr.total) * two_lobe_pdf(p.tl_c, wo, wi, p.cax, p.cay, true);
   if (pr.metal > 0.0f && !metal_smooth(p)) {
     float pm = 0.0f;
     if (wo.z > 0.0f && wi.z > 0.0f) {
-      const TwoLobe w = cond_lobes(p);
+      const TwoLobe& w = p.tl_m;
       pm = (w.ss * spec_ss_pdf(p, wo, wi) + w.ms * cos_pdf(wi.z)) / w.total;
     }
     pdf += (pr.metal / pr.total) * pm;
@@ -782,7 +834,7 @@ __device__ __noinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {
   if (pr.spec > 0.0f && !spec_smooth(p)) {
     float ps = 0.0f;
     if (wo.z > 0.0f && wi.z > 0.0f) {
-      const TwoLobe w = refl_lobes(ms_params(p, p.ax, p.ay, p.spec_eta));
+      const TwoLobe& w = p.tl_s;
       ps = (w.ss * spec_ss_pdf(p, wo, wi) + w.ms * cos_pdf(wi.z)) / w.total;
     }
     pdf += (pr.spec / pr.total) * ps;
@@ -874,7 +926,7 @@ __device__ __forceinline__ bool delta_sample(F3 weight, F3 wi, float p_lobe, Sam
 // (:1467-1668, 1929-1940)
 __device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, float ux, float uy, Sample* out) {
   if (!(wo.z > 0.0f)) return false;
-  const Probs pr = lobe_probs(p, wo);
+  const Probs& pr = p.pr;
   if (pr.total <= 0.0f) return false;
   float u = u_lobe * pr.total;
   int lobe;
@@ -910,21 +962,21 @@ __device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, flo
       const float f = fresnel_dielectric(fabsf(mirror.z), 1.0f, p.coat_eta);
       return delta_sample(f3s(layer_weights(p, wo, &mirror).coat_amp * f / p_lobe), mirror, p_lobe, out);
     }
-    return sample_two_lobe(p, wo, p.cax, p.cay, refl_lobes(ms_params(p, p.cax, p.cay, p.coat_eta)), ux, uy, out);
+    return sample_two_lobe(p, wo, p.cax, p.cay, p.tl_c, ux, uy, out);
   }
   if (lobe == 1) {
     if (metal_smooth(p)) {
       const F3 f = metal_fresnel(p, fabsf(mirror.z));
       return delta_sample(divs(hadamard(layer_weights(p, wo, &mirror).metal, f), p_lobe), mirror, p_lobe, out);
     }
-    return sample_two_lobe(p, wo, p.ax, p.ay, cond_lobes(p), ux, uy, out);
+    return sample_two_lobe(p, wo, p.ax, p.ay, p.tl_m, ux, uy, out);
   }
   if (lobe == 2) {
     if (spec_smooth(p)) {
       const float f = base_fresnel(p, fabsf(mirror.z));
       return delta_sample(divs(hadamard(layer_weights(p, wo, &mirror).spec, f3s(f)), p_lobe), mirror, p_lobe, out);
     }
-    return sample_two_lobe(p, wo, p.ax, p.ay, refl_lobes(ms_params(p, p.ax, p.ay, p.spec_eta)), ux, uy, out);
+    return sample_two_lobe(p, wo, p.ax, p.ay, p.tl_s, ux, uy, out);
   }
   if (lobe == 4) {
     F3 wi;
@@ -1211,12 +1263,14 @@ __device__ __forceinline__ bool pbr_le(const TptOpenPbr& m, const Vtx& v, F3* le
 }
 
 // OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548), radiance mode.
+template <bool kOutline>
 __device__ __forceinline__ bool pbr_sample(const Tables& tb, const TptOpenPbr& m, const Vtx& v, float u_lobe, float ux,
                                            float uy, Sample* s, F3* wi_world) {
   if (dot(v.wo, v.ng) <= 0.0f) return false;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
-  const Bsdf p = make_bsdf(tb, v.material, m, v.front, wol);
+  Bsdf p;
+  bsdf_at<kOutline>(tb, v.material, m, v.front, wol, &p);
   if (!bsdf_sample(p, wol, u_lobe, ux, uy, s)) return false;
   const F3 wi = to_world(v.frame, s->wi);
   if (dot(wi, v.ng) <= 1.0e-6f) return false;  // GEOMETRIC_NORMAL_COS_EPSILON; every carried lobe reflects
@@ -1244,9 +1298,11 @@ __device__ __forceinline__ bool mat_le(const TptMaterial& m, const Vtx& v, F3* l
 }
 // NormalizedLambertMaterial::sample (R/material/normalized_lambert.rs:120-158)
 // and MirrorMaterial::sample (R/material/mirror.rs:112-143), radiance mode.
+// kOutline: build the OpenPBR Bsdf in a call (bsdf_at)
+template <bool kOutline = false>
 __device__ __forceinline__ bool mat_sample(const Tables& tb, const TptMaterial& m, const Vtx& v, float u_lobe, float ux,
                                            float uy, Sample* s, F3* wi_world) {
-  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_sample(tb, m.u.open_pbr, v, u_lobe, ux, uy, s, wi_world);
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_sample<kOutline>(tb, m.u.open_pbr, v, u_lobe, ux, uy, s, wi_world);
   if (m.kind == TPT_MATERIAL_EMISSIVE) return false;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
@@ -1277,6 +1333,7 @@ __device__ __forceinline__ bool mat_sample(const Tables& tb, const TptMaterial& 
 // and OpenPBR (R/material/open_pbr.rs:549-590), each with modified_bsdf_eval
 // (R/material/mod.rs:33-53); Mirror and Emissive evaluate to zero.  *pdf is
 // 0 when f is 0 (Material::eval_pdf).
+template <bool kOutline = false>
 __device__ __forceinline__ F3 mat_eval_pdf(const Tables& tb, const TptMaterial& m, const Vtx& v, F3 wi, float* pdf) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
   *pdf = 0.0f;
@@ -1294,7 +1351,8 @@ __device__ __forceinline__ F3 mat_eval_pdf(const Tables& tb, const TptMaterial& 
     const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
     const F3 wil = normalize_or_zero(to_frame(v.frame, wi));
     if (wol.z <= 0.0f) return zero;
-    const Bsdf b = make_bsdf(tb, v.material, m.u.open_pbr, v.front, wol);
+    Bsdf b;
+    bsdf_at<kOutline>(tb, v.material, m.u.open_pbr, v.front, wol, &b);
     f = bsdf_eval(b, wol, wil);
     if (len2(f) == 0.0f) return zero;
     p = bsdf_pdf(b, wol, wil);
@@ -1328,7 +1386,8 @@ __device__ __forceinline__ F3 base_color_ratio(const Tables& tb, const TptMateri
   if (dot(v.wo, v.ng) <= 0.0f) return zero;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return zero;
-  const Bsdf b = make_bsdf(tb, v.material, m.u.open_pbr, v.front, wol);
+  Bsdf b;
+  make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
   const F3 f = bsdf_eval(b, wol, wil), d = bsdf_eval_dbase(b, wol, wil);
   return f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
 }
@@ -1659,7 +1718,7 @@ struct NeeSample {
   F3 origin, dir, value;
   uint32_t inst, tri;
 };
-template <bool kTex = false>
+template <bool kTex = false, bool kOutline = false>
 __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMaterial& m, const Vtx& vtx, float u_root,
                                            float u_tree, float us0, float us1, NeeSample* ns) {
   int kind;
@@ -1727,7 +1786,7 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
   }
   if (li_pdf <= 0.0f || selection <= 0.0f) return false;
   float bsdf_pdf;
-  const F3 f = mat_eval_pdf(sv.tab, m, vtx, wi, &bsdf_pdf);
+  const F3 f = mat_eval_pdf<kOutline>(sv.tab, m, vtx, wi, &bsdf_pdf);
   if (len2(f) == 0.0f) return false;
   const float cos_surface = fabsf(dot(vtx.ng, wi));
   if (cos_surface <= 0.0f) return false;
