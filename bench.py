@@ -304,7 +304,8 @@ def render_bench(dev, stream, args):
         # wavefront path tracer on the same frame: FP32 and 8-bit nodes, after
         # the autotuner has picked paths in flight and the trace launch shape
         wf_img = torch.empty_like(img_dev)
-        tune = rds.autotune([(cam, w, h, spp, depth)], tune_spp=8)
+        # 32 samples per pixel: 8 Mi paths, two passes at the largest paths-in-flight candidate
+        tune = rds.autotune([(cam, w, h, spp, depth)], tune_spp=32)
         res["autotune"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()}
         log(f"render {key} autotune: {res['autotune']}")
         for q8 in (False, True):

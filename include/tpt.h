@@ -446,8 +446,12 @@ int32_t tpt_render_batch(TptScene* scene, const TptFrameDesc* frames, uint32_t f
 /* Autotuner of tpt_render_batch for a workload: coordinate descent over the
  * device's "wf_paths", "blocks_per_cu" and "refill_below" parameters, each
  * candidate timed with HIP events on the given frames (samples per pixel
- * capped at tune_spp, 0: as given; outputs go to an internal buffer).  The
- * fastest setting stays on the device.  No reference counterpart. */
+ * capped at tune_spp, 0: as given; outputs go to an internal buffer).  A
+ * candidate replaces the current setting only when at least 1% faster, and
+ * paths-in-flight candidates above the tuning batch's path count are not
+ * tried (give tune_spp enough samples that the batch holds the frames' pass
+ * structure).  The fastest setting stays on the device.  No reference
+ * counterpart. */
 typedef struct TptTuneResult {
   uint32_t wf_paths;
   uint32_t blocks_per_cu;

@@ -105,7 +105,7 @@ def test_autotuner_picks_a_setting_no_slower_and_keeps_the_image(device, scene):
     cam = camera.pinhole(**cs.camera, width=w, height=h)
     ref = ds.render(cam, w, h, 8, 8)
     try:
-        res = ds.autotune([(cam, w, h, 64, 8)], tune_spp=8)
+        res = ds.autotune([(cam, w, h, 64, 8)], tune_spp=64)  # 1 Mi paths: two paths-in-flight candidates fit
         print("autotune", res)
         assert res["ms_best"] <= res["ms_default"] and res["trials"] >= 10
         assert res["wf_paths"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22)

@@ -717,10 +717,16 @@ bool time_batch(TptScene* s, std::vector<TptFrameDesc>& frames, uint32_t flags, 
 
 }  // namespace
 
+// A candidate replaces the current setting only when it is at least 1% faster:
+// run-to-run noise of a few tenths of a percent must not walk the setting away
+// from the starting one.
+constexpr float kTuneMargin = 0.99f;
 // Autotuner of the wavefront path tracer: coordinate descent over paths in
 // flight, resident trace blocks per CU and the trace kernels' refill
 // threshold, each candidate timed (HIP events) on the caller's frames at
-// tune_spp samples per pixel; the fastest setting is left on the device.
+// tune_spp samples per pixel; the fastest setting is left on the device.  A
+// paths-in-flight candidate larger than the tuning batch's path count runs
+// the batch in one pass, like the batch size itself, so it is not tried.
 extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags,
                                        uint32_t tune_spp, TptTuneResult* out) {
   try {
@@ -756,7 +762,7 @@ extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, 
         return TPT_ERROR_DEVICE;
       }
       ++trials;
-      if (ms < best) {
+      if (ms < best * kTuneMargin) {
         best = ms;
       } else {
         *knob = old;
@@ -766,8 +772,10 @@ extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, 
     const uint32_t paths[] = {1u << 19, 1u << 20, 1u << 21, 1u << 22};
     const uint32_t bpcs[] = {0u, 2u, 3u, 4u, 6u, 8u};
     const uint32_t refills[] = {8u, 16u, 24u, 32u, 48u};
+    size_t batch_paths = 0;
+    for (const auto& f : fr) batch_paths += (size_t)f.render.width * f.render.height * f.render.samples_per_pixel;
     for (uint32_t v : paths)
-      if (v != d->wf_paths && tryv(&d->wf_paths, v) != TPT_OK) return TPT_ERROR_DEVICE;
+      if (v != d->wf_paths && v <= batch_paths && tryv(&d->wf_paths, v) != TPT_OK) return TPT_ERROR_DEVICE;
     for (uint32_t v : bpcs)
       if (v != d->blocks_per_cu && tryv(&d->blocks_per_cu, v) != TPT_OK) return TPT_ERROR_DEVICE;
     for (uint32_t v : refills)

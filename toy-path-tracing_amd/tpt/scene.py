@@ -529,9 +529,11 @@ class DeviceScene:
         check(lib().tpt_render_batch(self.handle, arr, len(frames), flags))
         return outs
 
-    def autotune(self, frames, quantized: bool = False, tune_spp: int = 8) -> dict:
-        """tpt_autotune_render over these frames; the chosen parameters stay
-        on the device.  -> dict of the result fields."""
+    def autotune(self, frames, quantized: bool = False, tune_spp: int = 32) -> dict:
+        """tpt_autotune_render over these frames at up to tune_spp samples per
+        pixel (paths-in-flight candidates above the tuning batch's path count
+        are skipped, so the batch should hold a few passes); the chosen
+        parameters stay on the device.  -> dict of the result fields."""
         arr = (_lib.FrameDesc * max(1, len(frames)))()
         for i, (cam, w, h, spp, depth) in enumerate(frames):
             d = arr[i].render
