@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--render-scenes", default="openpbr,openpbr_glossy,lambert_nee,textured",
                     help="comma-separated render lines to time (openpbr, openpbr_glossy, lambert_nee, textured)")
     ap.add_argument("--adjoint", type=int, default=1, help="also time the path-replay adjoint (1) or not (0)")
+    ap.add_argument("--city", type=int, default=1,
+                    help="also time the four-frame city batch (1) or not (0): megakernel vs wavefront fp32 / q8")
+    ap.add_argument("--city-spp", type=int, default=4, help="samples per pixel of the city frames")
     ap.add_argument("--refill", type=int, default=0, help="trace-kernel refill threshold (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident blocks per CU (0: occupancy)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
@@ -192,6 +195,7 @@ def main():
 
     render = render_bench(dev, stream, args) if args.render else None
     adjoint = adjoint_bench(dev, stream, args) if args.adjoint else None
+    city = city_bench(dev, stream, args) if args.city else None
 
     line = {
         "metric": "Mrays/s",
@@ -227,6 +231,7 @@ def main():
         "cpu_baseline": cpu,
         "render": render,
         "adjoint": adjoint,
+        "city": city,
     }
     print(json.dumps(line), flush=True)
     ds.close()
@@ -344,6 +349,91 @@ def render_bench(dev, stream, args):
         dev.set_param("blocks_per_cu", 4)
         dev.set_param("refill_below", 24)
     return out
+
+
+def city_bench(dev, stream, args):
+    """The wavefront-batching workload: the procedural city (about 9.4 M
+    triangles, 2048 emissive triangles in the light tree, a 2048x1024 sky
+    with a sun, eight OpenPBR variants) rendered as four frames of uneven
+    resolution (1920x1080, 1280x720, 640x360, 3840x2160: 11,520,000 pixels)
+    at max depth 16.  The plain FP32 megakernel renders the frames one launch
+    each; the wavefront path tracer renders all four in one batch, with FP32
+    and with 8-bit quantised nodes, after the autotuner.  Rendered again with
+    the exact shadow rule, both wavefront images are compared with the
+    megakernel's bit for bit (identical images: equal relMSE); with 8-bit
+    nodes a path can take the other triangle of an exact tie, so the line
+    reports how many pixels differ."""
+    import torch
+
+    from tpt import camera, scenes
+
+    spp, depth = args.city_spp, 16
+    t = time.perf_counter()
+    cs = scenes.city()
+    t_build = time.perf_counter() - t
+    tris = sum(m.triangle_count() for m in cs.scene.meshes)
+    log(f"city: {tris} triangles, {cs.emissive_triangles} emissive, generated and built in {t_build:.1f}s")
+    rds = cs.scene.upload(dev)
+    frames = [(camera.pinhole(**v, width=w, height=h), w, h, spp, depth) for v, w, h in cs.cameras]
+    pixels = sum(w * h for _, w, h, _, _ in frames)
+    samples = pixels * spp
+    mega = [torch.empty(w * h * 3, dtype=torch.float32, device="cuda") for _, w, h, _, _ in frames]
+    reps = max(1, min(args.steps, 3))
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    def megakernel():
+        for (cam, w, h, s_, d), o in zip(frames, mega):
+            rds.render_device(cam, w, h, s_, d, o)
+
+    ms = timed(megakernel)
+    res = {
+        "workload": f"procedural city, {tris} triangles, {cs.emissive_triangles} emissive triangles + 2048x1024 sky "
+                    f"with sun, 8 OpenPBR variants; 4 frames (1920x1080, 1280x720, 640x360, 3840x2160), "
+                    f"{spp} spp, max depth {depth}",
+        "scene_build_s": round(t_build, 2),
+        "megakernel": {"ms_per_batch": round(ms, 3), "samples_per_s": round(samples / (ms / 1e3), 1)},
+    }
+    log(f"city megakernel fp32: {ms:.1f} ms per 4 frames = {samples / (ms / 1e3) / 1e6:.1f} Msamples/s")
+    tune = rds.autotune(frames, quantized=True, tune_spp=1)
+    res["autotune"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()}
+    log(f"city autotune: {res['autotune']}")
+    wf = [torch.empty_like(o) for o in mega]
+    for q8 in (False, True):
+        wms = timed(lambda: rds.render_batch_device(frames, wf, quantized=q8))
+        tag = "wavefront_q8" if q8 else "wavefront"
+        res[tag] = {"ms_per_batch": round(wms, 3), "samples_per_s": round(samples / (wms / 1e3), 1),
+                    "speedup_vs_megakernel": round(ms / wms, 3)}
+        # the timed runs use the early-exit shadow walk; with the exact shadow
+        # rule (closest hit, as the megakernel) the images must be the megakernel's
+        rds.render_batch_device(frames, wf, quantized=q8, exact_shadow=True)
+        torch.cuda.synchronize()
+        res[tag]["identical_to_megakernel"] = all(bool(torch.equal(x, y)) for x, y in zip(wf, mega))
+        # 8-bit nodes change the child visit order: at exact ties (a shared
+        # edge) another triangle can be the hit, and that pixel's path differs
+        differ = sum(int((x != y).view(-1, 3).any(1).sum()) for x, y in zip(wf, mega))
+        res[tag]["pixels_differing_from_megakernel"] = differ
+        res[tag]["mean_relative_difference"] = round(
+            sum(float((x - y).abs().sum()) for x, y in zip(wf, mega)) / max(sum(float(y.sum()) for y in mega), 1e-30),
+            8)
+        log(f"city {tag}: {wms:.1f} ms per 4 frames = {samples / (wms / 1e3) / 1e6:.1f} Msamples/s "
+            f"({ms / wms:.2f}x megakernel), exact-shadow image identical: {res[tag]['identical_to_megakernel']} "
+            f"({differ} of {pixels} pixels differ)")
+    res["image_means"] = [[round(float(x), 4) for x in o.view(-1, 3).mean(0).tolist()] for o in mega]
+    rds.close()
+    dev.set_param("wf_paths", 1 << 21)
+    dev.set_param("blocks_per_cu", 4)
+    dev.set_param("refill_below", 24)
+    return res
 
 
 def adjoint_bench(dev, stream, args):

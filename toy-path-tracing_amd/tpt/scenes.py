@@ -15,6 +15,13 @@ Workloads
   per-vertex radial noise of 5% of the radius, flattened into one mesh.
   410 spheres give 2,099,200 triangles (the 2M-triangle workload; 4096
   spheres would exceed the 2^24 per-mesh leaf-offset cap of R/qbvh.rs:15-18).
+* ``city``: the wavefront-batching workload -- a 256 x 256 grid of bevelled
+  (octagonal) towers with log-normal heights (mu 2, sigma 0.6), 400,000
+  scattered rocks (noisy icosahedra, 20 triangles each) and a textured
+  ground, about 9.4 M triangles in eleven flattened meshes; 2048 emissive
+  street-lamp triangles in the light tree and a 2048 x 1024 procedural sky with
+  an analytic sun of 0.27 degree radius; eight OpenPBR variants; four
+  cameras of uneven resolution (1920x1080, 1280x720, 640x360, 3840x2160).
 * ``primary_rays`` / ``bounce_rays`` / ``shadow_rays``: 1920x1080 pinhole
   rays, cosine-hemisphere bounce rays from primary hits, and shadow rays to
   uniform points on 64 random "emissive" triangles.
@@ -522,3 +529,164 @@ def cornell_inverse(build: bool = True, seed: int = 0, tex_size: int = 512) -> C
         s.add_material(m)
     cam = dict(eye=(0.5, 0.5, -1.4), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov_y_deg=40.0)
     return CornellScene(s, mats, li, cam)
+
+
+# --------------------------------------------------------------------------- city
+@dataclass
+class CityScene:
+    scene: Scene
+    materials: list
+    cameras: list  # (camera kwargs, width, height) per frame
+    emissive_triangles: int
+
+
+CITY_FRAMES = ((1920, 1080), (1280, 720), (640, 360), (3840, 2160))
+
+
+def _octagon_towers(rng, centers, half, chamfer, heights):
+    """Bevelled towers: octagonal prisms (the footprint square with its
+    vertical edges chamfered), 8 side quads + a 6-triangle roof, open at the
+    ground: 22 triangles and 16 vertices per tower."""
+    n = len(centers)
+    a, c = half[:, None], chamfer[:, None]
+    # footprint octagon, counter-clockwise seen from above (+y)
+    ox = np.stack([a, a - c, -(a - c), -a, -a, -(a - c), a - c, a], 1)[..., 0]
+    oz = np.stack([a - c, a, a, a - c, -(a - c), -a, -a, -(a - c)], 1)[..., 0]
+    x = ox + centers[:, 0:1]
+    z = oz + centers[:, 1:2]
+    y0 = np.zeros_like(x)
+    y1 = np.broadcast_to(heights[:, None], x.shape)
+    bottom = np.stack([x, y0, z], -1)  # (n, 8, 3)
+    top = np.stack([x, y1, z], -1)
+    pos = np.concatenate([bottom, top], 1)  # vertices 0..7 bottom, 8..15 top
+    k = np.arange(8)
+    k1 = (k + 1) % 8
+    # side quads, wound so that (p1 - p0) x (p2 - p0) points out of the tower
+    sides = np.concatenate([np.stack([k, k + 8, k1 + 8], 1), np.stack([k, k1 + 8, k1], 1)], 0)
+    roof = np.array([[8, 8 + j + 1, 8 + j] for j in range(1, 7)])
+    tri = np.concatenate([sides, roof], 0)  # (22, 3)
+    idx = tri[None, :, :] + (np.arange(n) * 16)[:, None, None]
+    return pos.reshape(-1, 3), idx.reshape(-1, 3)
+
+
+def _rocks(rng, n, extent, size=(0.15, 0.8), noise=0.25):
+    """n noisy icosahedra (level-0 icospheres, 20 triangles), random
+    rotation, log-uniform size, resting on the ground anywhere in the city."""
+    uv, uf = icosphere(0)
+    r = np.exp(rng.uniform(math.log(size[0]), math.log(size[1]), n))
+    cxz = rng.uniform(-extent, extent, (n, 2))
+    rots = random_rotations(rng, n)
+    disp = 1.0 + rng.uniform(-noise, noise, (n, len(uv)))
+    local = uv[None, :, :] * disp[:, :, None]
+    pos = np.einsum("nij,nvj->nvi", rots, local) * r[:, None, None]
+    pos[:, :, 0] += cxz[:, 0:1]
+    pos[:, :, 1] += 0.5 * r[:, None]
+    pos[:, :, 2] += cxz[:, 1:2]
+    idx = uf[None, :, :] + (np.arange(n) * len(uv))[:, None, None]
+    return pos.reshape(-1, 3), idx.reshape(-1, 3)
+
+
+def city_texture(size: int = 512) -> np.ndarray:
+    """A facade / paving texture (size x size RGB): a grid of lit and unlit
+    window cells with mortar lines and seeded per-cell tints."""
+    rng = np.random.default_rng(7)
+    cells = 16
+    y, x = np.mgrid[0:size, 0:size]
+    cx, cy = x * cells // size, y * cells // size
+    fx, fy = (x * cells % size) / size, (y * cells % size) / size
+    tint = rng.uniform(0.3, 0.9, (cells, cells, 3))
+    img = tint[cy, cx]
+    mortar = (fx < 0.08) | (fy < 0.08)
+    img[mortar] = (0.12, 0.12, 0.13)
+    return np.ascontiguousarray(img.astype(np.float32))
+
+
+def city(grid: int = 256, rocks: int = 400_000, seed: int = 0, cell: float = 4.0, lights: int = 2048,
+         sky_size=(2048, 1024), tex_size: int = 512, build: bool = True, threads: int = 0) -> CityScene:
+    """The seeded procedural city.  Towers on a grid x grid lattice of
+    `cell`-unit plots (footprint half-size 0.25-0.42 cell, chamfer 15-30% of
+    it, height exp(N(2, 0.6))), each tower given one of the eight variants;
+    `rocks` sheen rocks; `lights` emissive triangles as lights / 2 street
+    lamps (downward-facing 0.6 x 0.6 quads 4-6 units above random plot
+    corners, luminance 20); a textured ground.  Flattened meshes, one per tower variant, plus the rocks, the
+    ground and the emitters, each under an identity instance.
+
+    Materials: 0 diffuse, 1-3 metal at specular_roughness 0.05 / 0.3 / 0.8,
+    4 coat 0.5, 5 smooth dielectric of IOR 1.5 (reflection: the backend
+    carries no transmission lobe), 6 sheen (fuzz), 7 base-colour textured,
+    8 the lamp emitter.  Every surface is wound outward ((p1 - p0) x (p2 - p0)
+    points out of the tower, up from the ground, down from a lamp)."""
+    from .material import OpenPbr
+    from .scene import Environment
+    from .texture import Texture
+
+    rng = np.random.default_rng(seed)
+    n = grid * grid
+    extent = 0.5 * grid * cell
+    gi, gj = np.meshgrid(np.arange(grid), np.arange(grid), indexing="ij")
+    centers = np.stack([(gi.ravel() + 0.5) * cell - extent, (gj.ravel() + 0.5) * cell - extent], 1)
+    half = rng.uniform(0.25, 0.42, n) * cell
+    chamfer = rng.uniform(0.15, 0.30, n) * half
+    heights = np.exp(rng.normal(2.0, 0.6, n))
+    variant = rng.integers(0, 8, n)
+    s = Scene()
+    tex = s.add_texture(Texture(city_texture(tex_size)))
+
+    def add(pos, idx, mat, uvs=None):
+        mesh = Mesh(pos.astype(np.float32), idx.astype(np.uint32), uvs=None if uvs is None else uvs.astype(np.float32))
+        s.add_instance(s.add_mesh(mesh), mat, np.eye(4))
+
+    for v in range(8):
+        sel = np.nonzero(variant == v)[0]
+        if len(sel) == 0:
+            continue
+        pos, idx = _octagon_towers(rng, centers[sel], half[sel], chamfer[sel], heights[sel])
+        uvs = np.stack([(pos[:, 0] + pos[:, 2]) * 0.05, pos[:, 1] * 0.05], 1) if v == 7 else None
+        add(pos, idx, v, uvs)
+    if rocks:  # sheen rocks
+        rp, ri = _rocks(rng, rocks, extent)
+        add(rp, ri, 6)
+    # ground: textured paving
+    e = extent * 1.05
+    gp = np.array([[-e, 0, -e], [-e, 0, e], [e, 0, e], [e, 0, -e]], np.float64)
+    add(gp, np.array([[0, 1, 2], [0, 2, 3]]), 7, gp[:, [0, 2]] * 0.02)
+    # street lamps: downward-facing emissive quads over random plot corners
+    corners = (grid - 1) ** 2
+    quads = min(lights // 2, corners)
+    pick = rng.choice(corners, quads, replace=False)
+    c = np.stack([(pick // (grid - 1) + 1) * cell - extent, (pick % (grid - 1) + 1) * cell - extent], 1)
+    h = rng.uniform(4.0, 6.0, quads)
+    q = 0.3
+    lp = np.stack([np.stack([c[:, 0] - q, h, c[:, 1] - q], 1), np.stack([c[:, 0] + q, h, c[:, 1] - q], 1),
+                   np.stack([c[:, 0] + q, h, c[:, 1] + q], 1), np.stack([c[:, 0] - q, h, c[:, 1] + q], 1)], 1)
+    li = np.array([[0, 1, 2], [0, 2, 3]])[None] + (np.arange(quads) * 4)[:, None, None]
+    add(lp.reshape(-1, 3), li.reshape(-1, 3), 8)
+    if build:
+        s.build_qbvh(threads)
+    mats = [
+        OpenPbr.new((0.7, 0.7, 0.68)).with_(specular_weight=0.0),
+        OpenPbr.new((0.95, 0.8, 0.55)).with_(base_metalness=1.0, specular_roughness=0.05),
+        OpenPbr.new((0.9, 0.9, 0.92)).with_(base_metalness=1.0, specular_roughness=0.3),
+        OpenPbr.new((0.75, 0.45, 0.35)).with_(base_metalness=1.0, specular_roughness=0.8),
+        OpenPbr.new((0.55, 0.08, 0.06)).with_(coat_weight=0.5),
+        OpenPbr.new((0.02, 0.02, 0.02)).with_(specular_ior=1.5, specular_roughness=0.0),
+        OpenPbr.new((0.35, 0.33, 0.3)).with_(specular_weight=0.0, fuzz_weight=1.0, fuzz_roughness=0.5,
+                                             fuzz_color=(0.8, 0.75, 0.6)),
+        OpenPbr.new((1.0, 1.0, 1.0)).with_(base_color_texture=tex, specular_roughness=0.5),
+        OpenPbr.new((0.0, 0.0, 0.0)).with_(base_weight=0.0, specular_weight=0.0, emission_luminance=20.0,
+                                           emission_color=(1.0, 0.85, 0.6)),
+    ]
+    for m in mats:
+        s.add_material(m)
+    s.set_environment_light(Environment(procedural_sky(*sky_size), 1.0, 0.0))
+    views = [
+        dict(eye=(-0.55 * extent, 0.12 * extent, -0.55 * extent), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
+             fov_y_deg=45.0),
+        # street level on the street line x = 0
+        dict(eye=(0.0, 1.7, -2.0), look_at=(6.0, 8.0, 60.0), up=(0.0, 1.0, 0.0), fov_y_deg=60.0),
+        dict(eye=(0.3 * extent, 0.3 * extent, 0.1 * extent), look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
+             fov_y_deg=35.0),
+        dict(eye=(0.0, 0.06 * extent, -0.9 * extent), look_at=(0.0, 10.0, 0.0), up=(0.0, 1.0, 0.0), fov_y_deg=50.0),
+    ]
+    cams = [(v, w, h) for v, (w, h) in zip(views, CITY_FRAMES)]
+    return CityScene(s, mats, cams, 2 * quads)
