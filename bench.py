@@ -348,6 +348,7 @@ def render_bench(dev, stream, args):
         dev.set_param("wf_paths", 1 << 21)  # the traversal defaults for whatever runs next
         dev.set_param("blocks_per_cu", 4)
         dev.set_param("refill_below", 24)
+        dev.set_param("wf_sort", 0)
     return out
 
 
@@ -408,6 +409,14 @@ def city_bench(dev, stream, args):
     res["autotune"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()}
     log(f"city autotune: {res['autotune']}")
     wf = [torch.empty_like(o) for o in mega]
+    # the material sort of the shade queue, both settings (the autotuner keeps the faster)
+    chosen = tune["wf_sort"]
+    res["material_sort_ms"] = {}
+    for v in (0, 1):
+        dev.set_param("wf_sort", v)
+        res["material_sort_ms"]["on" if v else "off"] = round(timed(lambda: rds.render_batch_device(frames, wf)), 3)
+    dev.set_param("wf_sort", chosen)
+    log(f"city wavefront fp32 with / without the material sort: {res['material_sort_ms']}")
     for q8 in (False, True):
         wms = timed(lambda: rds.render_batch_device(frames, wf, quantized=q8))
         tag = "wavefront_q8" if q8 else "wavefront"
@@ -433,6 +442,7 @@ def city_bench(dev, stream, args):
     dev.set_param("wf_paths", 1 << 21)
     dev.set_param("blocks_per_cu", 4)
     dev.set_param("refill_below", 24)
+    dev.set_param("wf_sort", 0)
     return res
 
 

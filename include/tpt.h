@@ -291,7 +291,10 @@ int32_t tpt_device_synchronize(TptDevice* device);
  *                          occupancy limit)
  * and of the wavefront path tracer (tpt_render_batch):
  *   "wf_paths"  65536..2^26  paths in flight per pass (default 2^21)
- *   "wf_graph"  0 or 1       capture one pass as a HIP graph and replay it */
+ *   "wf_graph"  0 or 1       capture one pass as a HIP graph and replay it
+ *   "wf_sort"   0 or 1       shade each bounce's hits grouped by material
+ *                            (a counting sort of the hit queue; scenes of at
+ *                            most 255 materials; the image does not change) */
 int32_t tpt_device_set_param(TptDevice* device, const char* name, int64_t value);
 
 /* ---- sampler and camera --------------------------------------------------- */
@@ -444,7 +447,7 @@ typedef struct TptFrameDesc {
 int32_t tpt_render_batch(TptScene* scene, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags);
 
 /* Autotuner of tpt_render_batch for a workload: coordinate descent over the
- * device's "wf_paths", "blocks_per_cu" and "refill_below" parameters, each
+ * device's "wf_paths", "blocks_per_cu", "refill_below" and "wf_sort" parameters, each
  * candidate timed with HIP events on the given frames (samples per pixel
  * capped at tune_spp, 0: as given; outputs go to an internal buffer).  A
  * candidate replaces the current setting only when at least 1% faster, and
@@ -457,6 +460,7 @@ typedef struct TptTuneResult {
   uint32_t blocks_per_cu;
   uint32_t refill_below;
   uint32_t trials;
+  uint32_t wf_sort;
   float ms_default;  /* the starting setting */
   float ms_best;
 } TptTuneResult;

@@ -42,6 +42,23 @@ def test_wavefront_batch_matches_megakernel(city):
         assert np.all(np.isfinite(img)) and np.all(img >= 0) and img.mean() > 0
 
 
+def test_material_sorted_shading_keeps_the_image(city, device):
+    """wf_sort 1: each bounce's hits are shaded grouped by material (a
+    counting sort of the hit queue); a path's result does not depend on the
+    order paths are shaded in, so the image is the megakernel's bit for bit."""
+    c, ds, frames = city
+    mega = [ds.render(*f) for f in frames]
+    plain_q8 = ds.render_batch(frames, quantized=True, exact_shadow=True)
+    device.set_param("wf_sort", 1)
+    try:
+        for q8, want in ((False, mega), (True, plain_q8)):
+            got = ds.render_batch(frames, quantized=q8, exact_shadow=True)
+            for a, b in zip(want, got):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"q8={q8}"
+    finally:
+        device.set_param("wf_sort", 0)
+
+
 def test_q8_batch_differs_only_through_ties(city):
     """8-bit nodes change the order children are visited in, so where two
     triangles are hit at the same distance (a shared edge; t within 2 ulp)

@@ -99,6 +99,22 @@ def test_bad_frames_are_rejected(scene):
     assert ds.render_batch([]) == []
 
 
+@pytest.mark.parametrize("make", ["cornell_textured", "cornell_lambert", "glossy"])
+def test_material_sorted_shading_matches_megakernel(device, make):
+    cs = scenes.cornell(specular_weight=1.0) if make == "glossy" else getattr(scenes, make)()
+    ds = cs.scene.upload(device)
+    try:
+        w, h = 96, 64
+        cam = camera.pinhole(**cs.camera, width=w, height=h)
+        ref = ds.render(cam, w, h, 16, 8)
+        device.set_param("wf_sort", 1)
+        (img,) = ds.render_batch([(cam, w, h, 16, 8)], exact_shadow=True)
+        assert bits_equal(img, ref)
+    finally:
+        device.set_param("wf_sort", 0)
+        ds.close()
+
+
 def test_autotuner_picks_a_setting_no_slower_and_keeps_the_image(device, scene):
     cs, ds = scene
     w = h = 128
@@ -110,9 +126,11 @@ def test_autotuner_picks_a_setting_no_slower_and_keeps_the_image(device, scene):
         assert res["ms_best"] <= res["ms_default"] and res["trials"] >= 10
         assert res["wf_paths"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22)
         assert res["blocks_per_cu"] in (0, 2, 3, 4, 6, 8) and res["refill_below"] in (8, 16, 24, 32, 48)
+        assert res["wf_sort"] in (0, 1)
         (img,) = ds.render_batch([(cam, w, h, 8, 8)], exact_shadow=True)
         assert bits_equal(img, ref)
     finally:
         device.set_param("wf_paths", 1 << 21)
         device.set_param("blocks_per_cu", 4)
         device.set_param("refill_below", 24)
+        device.set_param("wf_sort", 0)

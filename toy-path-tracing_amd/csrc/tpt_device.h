@@ -126,6 +126,7 @@ struct TptDevice {
   // and its buffers (grow-only)
   uint32_t wf_paths = 1u << 21;
   uint32_t wf_graph = 1;
+  uint32_t wf_sort = 0;  // shade each bounce's hits grouped by material (counting sort of the queue)
   tpt::DeviceBuffer wf_buf;
 };
 

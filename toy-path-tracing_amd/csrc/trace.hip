@@ -524,6 +524,8 @@ int32_t tpt_device_set_param(TptDevice* d, const char* name, int64_t value) {
     d->wf_paths = (uint32_t)value;
   } else if (n == "wf_graph" && (value == 0 || value == 1)) {
     d->wf_graph = (uint32_t)value;
+  } else if (n == "wf_sort" && (value == 0 || value == 1)) {
+    d->wf_sort = (uint32_t)value;
   } else {
     set_error("tpt_device_set_param: unknown parameter or value out of range: " + n);
     return TPT_ERROR_BAD_ARGUMENT;

@@ -277,11 +277,73 @@ __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Q
 // One bounce of mis::trace_radiance for every path of the input queue.
 // With kTex the ray differentials and cones follow the paths and the
 // materials' textures are applied at every vertex.
+// ---- material sort of a bounce's hit queue ----------------------------------
+// Counting sort by the hit's material (misses in the last bin): wf_sort_count
+// counts the keys, wf_sort_scatter gives every block a range per key and
+// writes the queue entries' indices in key order.  Entries of one key keep no
+// particular order (blocks reserve their ranges atomically); the shade results
+// of a path do not depend on the order paths are shaded in.
+constexpr uint32_t kMaxSortBins = 256;
+struct SortBuf {
+  uint32_t* hist;    // kMaxSortBins counts
+  uint32_t* cursor;  // kMaxSortBins reservation cursors
+  uint32_t* order;   // queue indices in material order (P)
+};
+__device__ __forceinline__ uint32_t sort_key(const TptHit& h, const GpuShade* inst, uint32_t bins) {
+  return h.instance == TPT_INVALID_INDEX ? bins - 1u : inst[h.instance].material;
+}
+__global__ __launch_bounds__(256) void wf_sort_zero(SortBuf sb) {
+  sb.hist[threadIdx.x] = 0u;
+  sb.cursor[threadIdx.x] = 0u;
+}
+__global__ __launch_bounds__(256) void wf_sort_count(const TptHit* __restrict__ hits, const GpuShade* inst,
+                                                     const Counters* c, uint32_t in_idx, uint32_t bins, SortBuf sb) {
+  __shared__ uint32_t cnt[kMaxSortBins];
+  cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t n = c->rays[in_idx].v;
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[sort_key(hits[i], inst, bins)], 1u);
+  __syncthreads();
+  if (threadIdx.x < bins && cnt[threadIdx.x]) atomicAdd(&sb.hist[threadIdx.x], cnt[threadIdx.x]);
+}
+__global__ __launch_bounds__(256) void wf_sort_scatter(const TptHit* __restrict__ hits, const GpuShade* inst,
+                                                       const Counters* c, uint32_t in_idx, uint32_t bins, SortBuf sb) {
+  __shared__ uint32_t cnt[kMaxSortBins], first[kMaxSortBins];
+  const uint32_t n = c->rays[in_idx].v;
+  if (blockIdx.x * 256u >= n) return;  // uniform per block
+  cnt[threadIdx.x] = 0u;
+  // exclusive prefix of the key counts (256 entries, one per thread)
+  first[threadIdx.x] = threadIdx.x < bins ? sb.hist[threadIdx.x] : 0u;
+  __syncthreads();
+  for (uint32_t d = 1; d < kMaxSortBins; d <<= 1) {
+    const uint32_t v = threadIdx.x >= d ? first[threadIdx.x - d] : 0u;
+    __syncthreads();
+    first[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const uint32_t excl = first[threadIdx.x] - (threadIdx.x < bins ? sb.hist[threadIdx.x] : 0u);
+  __syncthreads();
+  first[threadIdx.x] = excl;
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  uint32_t key = 0, rank = 0;
+  if (i < n) {
+    key = sort_key(hits[i], inst, bins);
+    rank = atomicAdd(&cnt[key], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < bins && cnt[threadIdx.x]) first[threadIdx.x] += atomicAdd(&sb.cursor[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (i < n) sb.order[first[key] + rank] = i;
+}
+
+// order: the queue indices in material order (nullptr: queue order)
 template <bool kTex>
 __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
                                                 const TptHit* __restrict__ hits, Queue qout, ShadowQ sq,
                                                 Counters* c, uint32_t in_idx, uint32_t bounce,
-                                                const uint8_t* __restrict__ ztab, uint32_t ztab_dims) {
+                                                const uint8_t* __restrict__ ztab, uint32_t ztab_dims,
+                                                const uint32_t* __restrict__ order) {
   // digit-permutation rows of this bounce's sampler dimensions (vertex base +0, +1, +3, +5, +7, +13)
   __shared__ __align__(16) uint8_t zrows[6][smp::kZTabPrefixes];
   {
@@ -301,10 +363,11 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
     pt::NeeSample ns;
     F3 nee_value = no;
     if (slot < n) {
-      path = qin.path[slot];
-      const TptHit h = hits[slot];
+      const uint32_t e = order ? order[slot] : slot;
+      path = qin.path[e];
+      const TptHit h = hits[e];
       if (h.instance == TPT_INVALID_INDEX && sv.env.px) {  // a miss ends the path: environment emission
-        const TptRay r = qin.rays[slot];
+        const TptRay r = qin.rays[e];
         const F3 dir = f3(r.d[0], r.d[1], r.d[2]);
         F3 rad = load3f(st.rad, path);
         if (bounce == 0) {
@@ -320,7 +383,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
         const FrameDev& fr = ps.frames[frame_of(ps, gp)];
         const uint32_t lp = gp - fr.pix_begin;
         const smp::PathSampler smpl{fr.zf, lp % fr.w, lp / fr.w, k * ps.spp_slots + s};
-        const TptRay r = qin.rays[slot];
+        const TptRay r = qin.rays[e];
         const F3 dir = f3(r.d[0], r.d[1], r.d[2]);
         pt::RayDiff in;
         if (kTex) {
@@ -485,6 +548,8 @@ struct Plan {
   bool q8;
   bool exact_shadow;
   bool tex;  // the scene has textures: ray differentials and textured materials
+  wf::SortBuf sort;
+  uint32_t sort_bins;  // material count + 1 (misses); 0: shade in queue order
 };
 
 // The kernels of one pass, all reading their counts on the device.
@@ -512,8 +577,18 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
     if (!launch_trace_queue(s, 0, p.q8, false, p.q[a].rays, &p.c->rays[a].v, p.ps.paths, p.hits, nullptr, nullptr, st))
       return false;
     hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, a ^ 1u);
+    const uint32_t* order = nullptr;
+    if (p.sort_bins) {
+      const unsigned g_sort = (unsigned)((p.ps.paths + 255u) / 256u);
+      hipLaunchKernelGGL(wf::wf_sort_zero, dim3(1), dim3(256), 0, st, p.sort);
+      hipLaunchKernelGGL(wf::wf_sort_count, dim3(g_sort), dim3(256), 0, st, (const TptHit*)p.hits, sv.inst,
+                         (const wf::Counters*)p.c, a, p.sort_bins, p.sort);
+      hipLaunchKernelGGL(wf::wf_sort_scatter, dim3(g_sort), dim3(256), 0, st, (const TptHit*)p.hits, sv.inst,
+                         (const wf::Counters*)p.c, a, p.sort_bins, p.sort);
+      order = p.sort.order;
+    }
     hipLaunchKernelGGL(shade, dim3(g_shade), dim3(wf::kAppendBlock), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
-                       p.q[a ^ 1u], p.sq, p.c, a, b, (const uint8_t*)p.ztab, p.ztab_dims);
+                       p.q[a ^ 1u], p.sq, p.c, a, b, (const uint8_t*)p.ztab, p.ztab_dims, order);
     if (b < p.max_depth && s->host_cats.count) {  // the light sampler has a category: NEE can run
       if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow.v, p.ps.paths, nullptr, p.sq.target,
                               p.sq.unocc, st))
@@ -581,6 +656,8 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     const size_t o_su = take(P);
     const bool tex = s->texture_count > 0;
     const size_t o_diff = tex ? take(12 * 4 * P) : 0, o_dflag = tex ? take(4 * P) : 0, o_cone = tex ? take(8 * P) : 0;
+    const uint32_t sort_bins = (d->wf_sort && s->material_count + 1 <= wf::kMaxSortBins) ? s->material_count + 1 : 0;
+    const size_t o_sort = sort_bins ? take(4 * (2 * wf::kMaxSortBins + P)) : 0;
     // digit-permutation rows of every sampler dimension a path of up to 256 bounces reads
     // (deeper bounces compute their rows in the kernel)
     const uint32_t ztab_dims = smp::kVertexBase + smp::kVertexStride * (std::min<uint32_t>(max_depth, 256u) + 1);
@@ -636,6 +713,10 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.q8 = flags & TPT_FLAG_QUANTIZED;
     p.exact_shadow = flags & TPT_FLAG_ANY_EXACT;
     p.tex = tex;
+    p.sort_bins = sort_bins;
+    p.sort = sort_bins ? wf::SortBuf{(uint32_t*)(base + o_sort), (uint32_t*)(base + o_sort) + wf::kMaxSortBins,
+                                     (uint32_t*)(base + o_sort) + 2 * wf::kMaxSortBins}
+                       : wf::SortBuf{nullptr, nullptr, nullptr};
     const pt::ShadeView sv = pt::shade_view_of(s);
     if (!hip_ok(hipMemcpyAsync((void*)p.ps.frames, fd.data(), fd.size() * sizeof(wf::FrameDev), hipMemcpyHostToDevice,
                                st),
@@ -722,8 +803,8 @@ bool time_batch(TptScene* s, std::vector<TptFrameDesc>& frames, uint32_t flags, 
 // from the starting one.
 constexpr float kTuneMargin = 0.99f;
 // Autotuner of the wavefront path tracer: coordinate descent over paths in
-// flight, resident trace blocks per CU and the trace kernels' refill
-// threshold, each candidate timed (HIP events) on the caller's frames at
+// flight, resident trace blocks per CU, the trace kernels' refill threshold and
+// the material sort of the shade queue, each candidate timed (HIP events) on the caller's frames at
 // tune_spp samples per pixel; the fastest setting is left on the device.  A
 // paths-in-flight candidate larger than the tuning batch's path count runs
 // the batch in one pass, like the batch size itself, so it is not tried.
@@ -780,10 +861,12 @@ extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, 
       if (v != d->blocks_per_cu && tryv(&d->blocks_per_cu, v) != TPT_OK) return TPT_ERROR_DEVICE;
     for (uint32_t v : refills)
       if (v != d->refill_below && tryv(&d->refill_below, v) != TPT_OK) return TPT_ERROR_DEVICE;
+    if (tryv(&d->wf_sort, d->wf_sort ^ 1u) != TPT_OK) return TPT_ERROR_DEVICE;
     out->wf_paths = d->wf_paths;
     out->blocks_per_cu = d->blocks_per_cu;
     out->refill_below = d->refill_below;
     out->trials = trials;
+    out->wf_sort = d->wf_sort;
     out->ms_default = base_ms;
     out->ms_best = best;
     (void)keep_paths;

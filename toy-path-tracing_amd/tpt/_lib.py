@@ -159,7 +159,8 @@ assert BSDF_QUERY_DTYPE.itemsize == 48 and BSDF_RESULT_DTYPE.itemsize == 48
 
 class TuneResult(ctypes.Structure):
     _fields_ = [("wf_paths", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("refill_below", ctypes.c_uint32),
-                ("trials", ctypes.c_uint32), ("ms_default", ctypes.c_float), ("ms_best", ctypes.c_float)]
+                ("trials", ctypes.c_uint32), ("wf_sort", ctypes.c_uint32), ("ms_default", ctypes.c_float),
+                ("ms_best", ctypes.c_float)]
 
 
 class FrameDesc(ctypes.Structure):
