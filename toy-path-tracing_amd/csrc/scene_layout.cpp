@@ -32,7 +32,8 @@ GpuNode to_gpu_node(const TptQbvhNode& n, uint32_t interior_base) {
 namespace {
 
 // Decoded corner, bit-identical on host and device: q * 2^e is exact for
-// q <= 255 and e in [-126, 120], and the single addition rounds to nearest.
+// q <= 255 and e in [-126, 120], and the single addition rounds to nearest
+// (the device's fused multiply-add of the same exact product rounds once too).
 inline float decode(float origin, int q, int e) {
   const float s = std::ldexp(1.0f, e);
   return origin + (float)q * s;

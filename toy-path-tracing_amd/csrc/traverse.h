@@ -211,16 +211,19 @@ __device__ __forceinline__ uint32_t slab4(const GpuNode* __restrict__ nodes, con
       const float sxs = q8_scale((int)(int8_t)(a.w & 0xFFu));
       const float sys = q8_scale((int)(int8_t)((a.w >> 8) & 0xFFu));
       const float szs = q8_scale((int)(int8_t)((a.w >> 16) & 0xFFu));
-      // q bytes: b.x = lox[0..3], b.y = hix, b.z = loy, b.w = hiy, c.x = loz, c.y = hiz
+      // q bytes: b.x = lox[0..3], b.y = hix, b.z = loy, b.w = hiy, c.x = loz, c.y = hiz.
+      // A plane is origin + q * 2^e: the product of a byte and a power of two
+      // is exact, so one fused multiply-add gives the bits of the rounded sum
+      // origin + (q * 2^e) that the builder's containment check used.
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int sh = 8 * i;
-        lx[i] = ox + (float)((b.x >> sh) & 0xFFu) * sxs;
-        hx[i] = ox + (float)((b.y >> sh) & 0xFFu) * sxs;
-        ly[i] = oy + (float)((b.z >> sh) & 0xFFu) * sys;
-        hy[i] = oy + (float)((b.w >> sh) & 0xFFu) * sys;
-        lz[i] = oz + (float)((c.x >> sh) & 0xFFu) * szs;
-        hz[i] = oz + (float)((c.y >> sh) & 0xFFu) * szs;
+        lx[i] = __builtin_fmaf((float)((b.x >> sh) & 0xFFu), sxs, ox);
+        hx[i] = __builtin_fmaf((float)((b.y >> sh) & 0xFFu), sxs, ox);
+        ly[i] = __builtin_fmaf((float)((b.z >> sh) & 0xFFu), sys, oy);
+        hy[i] = __builtin_fmaf((float)((b.w >> sh) & 0xFFu), sys, oy);
+        lz[i] = __builtin_fmaf((float)((c.x >> sh) & 0xFFu), szs, oz);
+        hz[i] = __builtin_fmaf((float)((c.y >> sh) & 0xFFu), szs, oz);
       }
       (void)c;
       child[0] = ch.x;
