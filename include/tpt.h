@@ -417,6 +417,23 @@ int32_t tpt_render_adjoint(TptScene* scene, const TptRenderDesc* desc, const flo
 int32_t tpt_render_adjoint_texels(TptScene* scene, const TptRenderDesc* desc, const float* dloss_dimage,
                                   float* d_albedo, float* d_texels, uint32_t flags);
 
+/* Path-replay adjoint with respect to every OpenPBR material's base colour
+ * (RGB), specular_roughness and base_metalness: d_params holds 5 floats per
+ * material (material_count * 5; NormalizedLambert rows: albedo, 0, 0).  A
+ * vertex's contributions are scaled by (df/dtheta) / f of its BSDF at the
+ * light and sampled directions, df/dtheta by a central difference of step
+ * 1e-3 of the BSDF rebuilt at theta +- step (one-sided at the [0, 1] ends of
+ * roughness and metalness and at base colour 0); the base colour of a
+ * material whose BSDF is affine in it takes the exact ratio as in
+ * tpt_render_adjoint.  Sampling, Russian roulette and MIS weights are held
+ * fixed.  A roughness or metalness slot with a texture gets no gradient (the
+ * base-colour texture does, into d_texels as in tpt_render_adjoint_texels;
+ * d_texels may be NULL).  At most 2048 materials.  The parameters of the
+ * configuration's inverse-rendering workload (R/material/open_pbr.rs:23-896
+ * parameter meaning); no reference counterpart. */
+int32_t tpt_render_adjoint_params(TptScene* scene, const TptRenderDesc* desc, const float* dloss_dimage,
+                                  float* d_params, float* d_texels, uint32_t flags);
+
 /* Replaces the scene's material table (same count; kinds and emission must
  * stay the same, as the light tree is built from them): the parameter update
  * of an inverse-rendering loop.  Synchronous. */

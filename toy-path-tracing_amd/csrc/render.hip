@@ -78,8 +78,9 @@ template <typename Acc>
 struct Adjoint {
   F3 total;
   F3 w;
-  Acc* grad;
+  Acc* grad;        // rows of `rows` gradients per material
   float* texgrad;
+  uint32_t rows;    // 3: albedo / base colour; 5 (kParamRows): + specular_roughness, base_metalness
 };
 
 // mis::trace_radiance (R/integrator/mis.rs:19-139) for one camera ray.  With
@@ -123,7 +124,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     const TptMaterial& m = vm.get();
     if (kAdj && m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {
       const F3 rest = sub(adj.total, radiance);
-      Acc* g = adj.grad + 3 * vtx.material;
+      Acc* g = adj.grad + adj.rows * vtx.material;
       const float* rho = m.u.lambert.rho;
       if (rho[0] != 0.0f) atomicAdd(&g[0], (Acc)(adj.w.x * rest.x / rho[0]));
       if (rho[1] != 0.0f) atomicAdd(&g[1], (Acc)(adj.w.y * rest.y / rho[1]));
@@ -144,14 +145,41 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     float ud[2];
     ps.s2(base + 7, ud);
     const bool sampled = mat_sample<true>(sv.tab, m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi);
-    if (kAdj && base_color_linear(m)) {
+    const bool lin = kAdj && base_color_linear(m);
+    if (kAdj && (lin || (adj.rows == kParamRows && m.kind == TPT_MATERIAL_OPEN_PBR))) {
       const F3 rest = sub(adj.total, radiance);  // collected after this vertex's NEE
+      const bool use_nee = len2(nee_c) > 0.0f, use_smp = sampled && !s.delta;
+      const F3 wn = normalize_or_zero(to_frame(vtx.frame, nee_dir));
       F3 g = f3(0.0f, 0.0f, 0.0f);
-      if (len2(nee_c) > 0.0f)
-        g = hadamard(nee_c, base_color_ratio(sv.tab, m, vtx, normalize_or_zero(to_frame(vtx.frame, nee_dir))));
-      if (sampled && !s.delta) g = add(g, hadamard(rest, base_color_ratio(sv.tab, m, vtx, s.wi)));
-      g = hadamard(g, adj.w);  // dL/dc of this sample
       const TptOpenPbr& p0 = sv.mats[vtx.material].u.open_pbr;  // the parameters, untextured
+      Acc* gm = adj.grad + adj.rows * vtx.material;
+      if (adj.rows == kParamRows) {
+        // every parameter by central differences of the BSDF: dL/dtheta_k =
+        // sum over channels of w * (NEE * r_k(light dir) + rest * r_k(sampled dir))
+        F3 rn[kParamRows], rs[kParamRows];
+        const F3 zero = f3(0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < kParamRows; ++k) rn[k] = rs[k] = zero;
+        if (use_nee) bsdf_param_ratios(sv.tab, m, vtx, wn, rn);
+        if (use_smp) bsdf_param_ratios(sv.tab, m, vtx, s.wi, rs);
+        float gk[kParamRows];
+        for (int k = 0; k < kParamRows; ++k)
+          gk[k] = dot(adj.w, add(hadamard(nee_c, rn[k]), hadamard(rest, rs[k])));
+        if (lin) {  // the base colour exactly (the ratio is diagonal and exact for affine BSDFs)
+          if (use_nee) g = hadamard(nee_c, base_color_ratio(sv.tab, m, vtx, wn));
+          if (use_smp) g = add(g, hadamard(rest, base_color_ratio(sv.tab, m, vtx, s.wi)));
+          g = hadamard(g, adj.w);
+        } else {
+          g = f3(gk[0], gk[1], gk[2]);
+        }
+        // roughness and metalness: the effective value is the parameter
+        // times the texture's; textured slots get no row
+        if (p0.specular_roughness_texture == TPT_INVALID_INDEX && gk[3] != 0.0f) atomicAdd(&gm[3], (Acc)gk[3]);
+        if (p0.base_metalness_texture == TPT_INVALID_INDEX && gk[4] != 0.0f) atomicAdd(&gm[4], (Acc)gk[4]);
+      } else {
+        if (use_nee) g = hadamard(nee_c, base_color_ratio(sv.tab, m, vtx, wn));
+        if (use_smp) g = add(g, hadamard(rest, base_color_ratio(sv.tab, m, vtx, s.wi)));
+        g = hadamard(g, adj.w);  // dL/dc of this sample
+      }
       F3 tv = f3(1.0f, 1.0f, 1.0f);
       if (kTex && p0.base_color_texture != TPT_INVALID_INDEX) {
         const tex::V2 uv = tex::v2(tx.u, tx.v), dx = tex::v2(tx.dudx, tx.dvdx), dy = tex::v2(tx.dudy, tx.dvdy);
@@ -160,7 +188,6 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
         if (adj.texgrad)
           tex::scatter_filtered<3>(sv.tex, t, uv, dx, dy, hadamard(g, rgb3(p0.base_color)), adj.texgrad);
       }
-      Acc* gm = adj.grad + 3 * vtx.material;
       const F3 gp = hadamard(g, tv);
       if (gp.x != 0.0f) atomicAdd(&gm[0], (Acc)gp.x);
       if (gp.y != 0.0f) atomicAdd(&gm[1], (Acc)gp.y);
@@ -249,7 +276,7 @@ template <bool kTwoLevel, bool kTex, typename Acc>
 __global__ __launch_bounds__(kBlock) void adjoint_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
                                                          uint32_t h, uint32_t spp, uint32_t max_depth,
                                                          const float* __restrict__ dimg, Acc* __restrict__ partials,
-                                                         uint32_t n_grad, float* __restrict__ texgrad,
+                                                         uint32_t n_grad, uint32_t rows, float* __restrict__ texgrad,
                                                          uint32_t* __restrict__ status) {
   __shared__ uint32_t lds[kLdsDepth * kBlock];
   extern __shared__ __align__(16) unsigned char dyn[];
@@ -273,6 +300,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(SceneView sc, ShadeView
     adj.w = f3(dimg[3 * pix] * scale, dimg[3 * pix + 1] * scale, dimg[3 * pix + 2] * scale);
     adj.grad = grad;
     adj.texgrad = texgrad;
+    adj.rows = rows;
     const Adjoint<Acc> none{};
     for (uint32_t si = 0; si < spp; ++si) {
       const smp::PathSampler ps{zf, x, y, si};
@@ -510,11 +538,11 @@ pt::CamView cam_view(const TptCamera& c) {
 
 template <typename Acc>
 int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, float* d_albedo, float* d_texels,
-                     bool dev_ptr) {
+                     bool dev_ptr, uint32_t rows) {
   TptDevice* d = s->dev;
   const uint32_t w = desc->width, h = desc->height;
   const size_t n = (size_t)w * h;
-  const uint32_t n_grad = 3 * s->material_count;
+  const uint32_t n_grad = rows * s->material_count;
   const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
   const size_t nb_part = align_up((size_t)blocks * n_grad * sizeof(Acc));
   const size_t nb_img = dev_ptr ? 0 : align_up(n * 12);
@@ -549,7 +577,7 @@ int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, 
   auto kernel = two ? (tex ? pt::adjoint_kernel<true, true, Acc> : pt::adjoint_kernel<true, false, Acc>)
                     : (tex ? pt::adjoint_kernel<false, true, Acc> : pt::adjoint_kernel<false, false, Acc>);
   hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), lds_bytes, d->stream, v, sv, cam, w, h,
-                     desc->samples_per_pixel, desc->max_depth, ddimg, partials, n_grad, tbuf, d->status);
+                     desc->samples_per_pixel, desc->max_depth, ddimg, partials, n_grad, rows, tbuf, d->status);
   hipLaunchKernelGGL((pt::adjoint_reduce<Acc>), dim3((n_grad + 255) / 256), dim3(256), 0, d->stream,
                      (const Acc*)partials, blocks, n_grad, dgrad);
   if (!hip_ok(hipGetLastError(), "adjoint launch")) return TPT_ERROR_DEVICE;
@@ -574,7 +602,7 @@ int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, 
 
 namespace {
 int32_t render_adjoint(TptScene* s, const TptRenderDesc* desc, const float* dloss_dimage, float* d_albedo,
-                       float* d_texels, uint32_t flags) {
+                       float* d_texels, uint32_t flags, uint32_t rows = 3) {
   try {
     if (!s || !desc || !dloss_dimage || !d_albedo) {
       set_error("tpt_render_adjoint: null argument");
@@ -585,26 +613,35 @@ int32_t render_adjoint(TptScene* s, const TptRenderDesc* desc, const float* dlos
       set_error("tpt_render_adjoint: frame too large or samples_per_pixel == 0");
       return TPT_ERROR_BAD_ARGUMENT;
     }
-    if (s->material_count == 0 || s->material_count > 4096) {
-      set_error("tpt_render_adjoint: needs 1..4096 materials");
+    const uint32_t max_mats = rows == 3 ? 4096u : 2048u;  // the block's gradient rows live in LDS
+    if (s->material_count == 0 || s->material_count > max_mats) {
+      set_error("tpt_render_adjoint: needs 1.." + std::to_string(max_mats) + " materials");
       return TPT_ERROR_BAD_ARGUMENT;
     }
     if (!render_materials_ok(s, "tpt_render_adjoint")) return TPT_ERROR_BAD_ARGUMENT;
     if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    if (rows == pt::kParamRows && !s->cond_tables) {
+      // the metalness derivative reads the conductor tables even where a
+      // material's metalness is 0 (tables are otherwise built only for
+      // materials with a non-zero metalness)
+      s->force_cond = true;
+      if (!upload_ggx_tables(s, s->host_mats)) return TPT_ERROR_DEVICE;
+    }
     const size_t n_tg = d_texels ? texel_grad_floats(s) : 0;
     if ((uint64_t)w * h == 0) {
       if (flags & TPT_FLAG_DEVICE_POINTERS)
-        return hip_ok(hipMemsetAsync(d_albedo, 0, 12 * (size_t)s->material_count, s->dev->stream), "hipMemsetAsync") &&
+        return hip_ok(hipMemsetAsync(d_albedo, 0, 4 * rows * (size_t)s->material_count, s->dev->stream),
+                      "hipMemsetAsync") &&
                        (!n_tg || hip_ok(hipMemsetAsync(d_texels, 0, 4 * n_tg, s->dev->stream), "hipMemsetAsync"))
                    ? TPT_OK
                    : TPT_ERROR_DEVICE;
-      std::memset(d_albedo, 0, 12 * (size_t)s->material_count);
+      std::memset(d_albedo, 0, 4 * rows * (size_t)s->material_count);
       if (n_tg) std::memset(d_texels, 0, 4 * n_tg);
       return TPT_OK;
     }
     const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
-    return (flags & TPT_FLAG_GRAD_FP64) ? adjoint_impl<double>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr)
-                                        : adjoint_impl<float>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr);
+    return (flags & TPT_FLAG_GRAD_FP64) ? adjoint_impl<double>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr, rows)
+                                        : adjoint_impl<float>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr, rows);
   } catch (const std::exception& e) {
     set_error(std::string("tpt_render_adjoint: ") + e.what());
     return TPT_ERROR;
@@ -624,6 +661,11 @@ extern "C" int32_t tpt_render_adjoint_texels(TptScene* s, const TptRenderDesc* d
     return TPT_ERROR_BAD_ARGUMENT;
   }
   return render_adjoint(s, desc, dloss_dimage, d_albedo, d_texels, flags);
+}
+
+extern "C" int32_t tpt_render_adjoint_params(TptScene* s, const TptRenderDesc* desc, const float* dloss_dimage,
+                                             float* d_params, float* d_texels, uint32_t flags) {
+  return render_adjoint(s, desc, dloss_dimage, d_params, d_texels, flags, pt::kParamRows);
 }
 
 extern "C" int32_t tpt_scene_set_materials(TptScene* s, const TptMaterial* materials, uint32_t count) {

@@ -1392,6 +1392,46 @@ __device__ __forceinline__ F3 base_color_ratio(const Tables& tb, const TptMateri
   return f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
 }
 
+// OpenPBR parameters the adjoint differentiates numerically: 0-2 the base
+// colour channels, 3 specular_roughness, 4 base_metalness.
+constexpr int kParamRows = 5;
+constexpr float kParamStep = 1.0e-3f;
+__device__ __forceinline__ float& pbr_param(TptOpenPbr& p, int k) {
+  return k < 3 ? p.base_color[k] : (k == 3 ? p.specular_roughness : p.base_metalness);
+}
+// The per-channel ratios (df/dtheta_k) / f of an OpenPBR BSDF value at wil,
+// for every parameter k at once: df/dtheta_k by a central difference of step
+// kParamStep of the BSDF rebuilt at theta_k +- step (one-sided at 0, and at 1
+// for roughness and metalness, which the BSDF clamps there); 0 where f is 0.
+// r[k] receives the ratios of parameter k.
+__device__ __noinline__ void bsdf_param_ratios(const Tables& tb, const TptMaterial& m, const Vtx& v, F3 wil,
+                                               F3 r[kParamRows]) {
+  const F3 zero = f3(0.0f, 0.0f, 0.0f);
+  for (int k = 0; k < kParamRows; ++k) r[k] = zero;
+  if (dot(v.wo, v.ng) <= 0.0f) return;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  if (wol.z <= 0.0f) return;
+  Bsdf b;
+  make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
+  const F3 f = bsdf_eval(b, wol, wil);
+  if (!(f.x > 0.0f) && !(f.y > 0.0f) && !(f.z > 0.0f)) return;
+  for (int k = 0; k < kParamRows; ++k) {
+    TptOpenPbr a = m.u.open_pbr, c = m.u.open_pbr;
+    const float x = pbr_param(a, k);
+    const float hi = k < 3 ? x + kParamStep : fminf(x + kParamStep, 1.0f);
+    const float lo = fmaxf(x - kParamStep, 0.0f);
+    if (!(hi > lo)) continue;
+    pbr_param(a, k) = hi;
+    pbr_param(c, k) = lo;
+    make_bsdf_call(tb, v.material, a, v.front, wol, &b);
+    const F3 fa = bsdf_eval(b, wol, wil);
+    make_bsdf_call(tb, v.material, c, v.front, wol, &b);
+    const F3 fc = bsdf_eval(b, wol, wil);
+    const F3 d = divs(sub(fa, fc), hi - lo);
+    r[k] = f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
+  }
+}
+
 // ---- environment light (R/light/environment.rs) ------------------------------------
 __device__ __forceinline__ F3 mat3_mul(const float* m, F3 v) {  // glam Mat3 * Vec3
   return add(add(scl(v.x, f3(m[0], m[1], m[2])), scl(v.y, f3(m[3], m[4], m[5]))), scl(v.z, f3(m[6], m[7], m[8])));

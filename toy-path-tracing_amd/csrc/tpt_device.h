@@ -173,6 +173,8 @@ struct TptScene {
   float* ggx_albedo = nullptr;
   uint32_t* ggx_mat = nullptr;
   float* ggx_cond = nullptr;
+  bool cond_tables = false;  // ggx_cond holds the conductor tables (not zeros)
+  bool force_cond = false;   // build them whatever the materials (the parameter adjoint asked)
   float* ggx_diel = nullptr;
   // textures (TptSceneDesc::textures): texels of every level, level table,
   // per-texture records; host copies of the two tables

@@ -397,6 +397,8 @@ bool upload_ggx_tables(TptScene* s, const std::vector<TptMaterial>& mats) {
   }
   const size_t ncond = ggx::kCondCos * ggx::kCondRough + ggx::kCondRough;
   const size_t ndiel = ggx::kDielCos * ggx::kDielRough * ggx::kDielEta + ggx::kDielRough * ggx::kDielEta;
+  need_cond = need_cond || s->force_cond;
+  s->cond_tables = need_cond;
   const std::vector<float> cond = need_cond ? ggx::conductor_table() : std::vector<float>(ncond, 0.0f);
   const std::vector<float> diel = need_diel ? ggx::dielectric_table() : std::vector<float>(ndiel, 0.0f);
   (void)hipFree(s->ggx_albedo);

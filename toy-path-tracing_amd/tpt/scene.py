@@ -495,6 +495,28 @@ class DeviceScene:
             o += k
         return out, grads
 
+    def render_adjoint_params(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dimage,
+                              fp64: bool = False):
+        """tpt_render_adjoint_params -> (d_params (material_count, 5): base
+        colour RGB, specular_roughness, base_metalness per material; [per
+        texture level-0 gradient, shaped like its pixels])."""
+        d = _lib.RenderDesc()
+        d.camera = camera
+        d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+        g = np.ascontiguousarray(dloss_dimage, dtype=np.float32).reshape(height, width, 3)
+        out = np.zeros((self.material_count, 5), np.float32)
+        shapes = [t.pixels.shape for t in self.textures]
+        flat = np.zeros(max(1, sum(int(np.prod(sh)) for sh in shapes)), np.float32)
+        check(lib().tpt_render_adjoint_params(self.handle, ctypes.byref(d), ptr(g), ptr(out),
+                                              ptr(flat) if shapes else None,
+                                              _lib.TPT_FLAG_GRAD_FP64 if fp64 else 0))
+        grads, o = [], 0
+        for sh in shapes:
+            k = int(np.prod(sh))
+            grads.append(flat[o:o + k].reshape(sh).copy())
+            o += k
+        return out, grads
+
     def render_adjoint_texels_device(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dev,
                                      d_albedo_dev, d_texels_dev, fp64: bool = False) -> None:
         """As render_adjoint_texels with device buffers (torch tensors):
