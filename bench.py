@@ -447,58 +447,64 @@ def city_bench(dev, stream, args):
 
 
 def adjoint_bench(dev, stream, args):
-    """Path-replay adjoint on the inverse-rendering Cornell box: 16 OpenPBR
-    materials with learnable base colours plus a learnable 512x512 RGB
-    base-colour texture, 256x256, 16 spp, depth 6 (an L2 loss's image
-    gradient as input): gradient evaluations per second.  One evaluation =
-    primal + replay of every pixel sample, the parameter and texel gradients
-    (EWA and pyramid adjoints) on the device; specular_roughness and
-    metalness are not differentiated by this build."""
+    """Path-replay adjoint on the inverse-rendering Cornell box, 256x256,
+    16 spp, depth 6 (an L2 loss's image gradient as input): gradient
+    evaluations per second, for (1) 16 diffuse OpenPBR materials with
+    learnable base colours plus a learnable 512x512 RGB base-colour texture
+    (exact base-colour ratios), and (2) the configuration's full parameter
+    set: 16 glossy materials with learnable base colour, specular roughness
+    and metalness (seeded), plus the texture.  One evaluation = primal +
+    replay of every pixel sample, parameter and texel gradients (EWA and
+    pyramid adjoints) on the device."""
     import torch
 
     from tpt import camera, scenes
 
-    cs = scenes.cornell_inverse()
-    ds = cs.scene.upload(dev)
     w = h = 256
     spp, depth = 16, 6
-    cam = camera.pinhole(**cs.camera, width=w, height=h)
-    img = torch.empty(w * h * 3, dtype=torch.float32, device="cuda")
-    ds.render_device(cam, w, h, spp, depth, img)
-    torch.cuda.synchronize()
-    dloss = (img - 0.5).contiguous()  # dL/dI of 0.5 * |I - 0.5|^2
-    n_tex = sum(int(np.prod(t.pixels.shape)) for t in cs.scene.textures)
-    d_alb = torch.empty(3 * len(cs.materials), dtype=torch.float32, device="cuda")
-    d_tex = torch.empty(n_tex, dtype=torch.float32, device="cuda")
-    ds.render_adjoint_texels_device(cam, w, h, spp, depth, dloss, d_alb, d_tex)
-    torch.cuda.synchronize()
     reps = max(1, min(args.steps, 5))
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record(stream)
-    for _ in range(reps):
-        ds.render_adjoint_texels_device(cam, w, h, spp, depth, dloss, d_alb, d_tex)
-    b.record(stream)
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / reps
-    # the forward render of the same frame, for the replay's cost relative to it
-    a.record(stream)
-    for _ in range(reps):
-        ds.render_device(cam, w, h, spp, depth, img)
-    b.record(stream)
-    torch.cuda.synchronize()
-    fms = a.elapsed_time(b) / reps
-    ds.close()
-    res = {
-        "workload": "inverse-rendering Cornell box: 16 OpenPBR materials (learnable base_color) + learnable "
-                    "512x512 RGB base-colour texture, 256x256, 16 spp, depth 6, L2 image loss",
-        "parameters": 3 * 16 + n_tex,
-        "ms_per_gradient": round(ms, 3),
-        "grad_per_s": round(1e3 / ms, 2),
-        "samples_per_s": round(w * h * spp / (ms / 1e3), 1),
-        "forward_ms": round(fms, 3),
-    }
-    log(f"adjoint: {ms:.2f} ms per gradient = {1e3 / ms:.1f} grad/s (forward {fms:.2f} ms)")
-    return res
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    out = {}
+    for key, glossy in (("base_color_texels", False), ("params_texels", True)):
+        cs = scenes.cornell_inverse(glossy=glossy)
+        ds = cs.scene.upload(dev)
+        cam = camera.pinhole(**cs.camera, width=w, height=h)
+        img = torch.empty(w * h * 3, dtype=torch.float32, device="cuda")
+        fms = timed(lambda: ds.render_device(cam, w, h, spp, depth, img))
+        dloss = (img - 0.5).contiguous()  # dL/dI of 0.5 * |I - 0.5|^2
+        n_tex = sum(int(np.prod(t.pixels.shape)) for t in cs.scene.textures)
+        d_tex = torch.empty(n_tex, dtype=torch.float32, device="cuda")
+        rows = 5 if glossy else 3
+        d_par = torch.empty(rows * len(cs.materials), dtype=torch.float32, device="cuda")
+        if glossy:
+            ms = timed(lambda: ds.render_adjoint_params_device(cam, w, h, spp, depth, dloss, d_par, d_tex))
+        else:
+            ms = timed(lambda: ds.render_adjoint_texels_device(cam, w, h, spp, depth, dloss, d_par, d_tex))
+        ds.close()
+        label = ("16 glossy OpenPBR materials (learnable base_color, specular_roughness, base_metalness)" if glossy
+                 else "16 OpenPBR materials (learnable base_color)")
+        out[key] = {
+            "workload": f"inverse-rendering Cornell box: {label} + learnable 512x512 RGB base-colour texture, "
+                        "256x256, 16 spp, depth 6, L2 image loss",
+            "parameters": rows * 16 + n_tex,
+            "ms_per_gradient": round(ms, 3),
+            "grad_per_s": round(1e3 / ms, 2),
+            "samples_per_s": round(w * h * spp / (ms / 1e3), 1),
+            "forward_ms": round(fms, 3),
+        }
+        log(f"adjoint {key}: {ms:.2f} ms per gradient = {1e3 / ms:.1f} grad/s (forward {fms:.2f} ms)")
+    return out
 
 
 def cpu_baseline(sc, sets, sample):

@@ -86,3 +86,23 @@ def test_albedo_entry_point_rows_unchanged(setup):
     g3 = ds.render_adjoint(cam, W, H, 64, 3, dl)
     assert np.allclose(g5[:, :3], g3, rtol=1e-5, atol=1e-7 * np.abs(g3).max())
     assert np.any(g5[:3, 3] != 0) and np.any(g5[:3, 4] != 0)
+
+
+def test_torch_autograd_params(setup):
+    import torch
+
+    from tpt.autograd import render_params
+
+    cs, ds, cam = setup
+    _, mats = materials(BASE)
+    rows = np.zeros((len(mats), 5), np.float32)
+    rows[:3] = BASE
+    target = torch.from_numpy(render(ds, TARGET, cam, 64, 3).astype(np.float32))
+    p = torch.tensor(rows, requires_grad=True)
+    img = render_params(p, ds, mats, cam, W, H, 64, 3)
+    loss = ((img - target) ** 2).sum()
+    loss.backward()
+    ds.set_materials(mats)
+    want, _ = ds.render_adjoint_params(cam, W, H, 64, 3, (2.0 * (img.detach().numpy() - target.numpy())))
+    assert np.allclose(p.grad.numpy(), want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+    assert np.all(p.grad.numpy()[3] == 0)

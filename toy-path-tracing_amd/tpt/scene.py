@@ -529,6 +529,17 @@ class DeviceScene:
         check(lib().tpt_render_adjoint_texels(self.handle, ctypes.byref(d), ptr(dloss_dev), ptr(d_albedo_dev),
                                               ptr(d_texels_dev), flags))
 
+    def render_adjoint_params_device(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dev,
+                                     d_params_dev, d_texels_dev=None, fp64: bool = False) -> None:
+        """As render_adjoint_params with device buffers (torch tensors):
+        dloss (h*w*3), d_params (material_count*5), d_texels (optional)."""
+        d = _lib.RenderDesc()
+        d.camera = camera
+        d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+        flags = _lib.TPT_FLAG_DEVICE_POINTERS | (_lib.TPT_FLAG_GRAD_FP64 if fp64 else 0)
+        check(lib().tpt_render_adjoint_params(self.handle, ctypes.byref(d), ptr(dloss_dev), ptr(d_params_dev),
+                                              ptr(d_texels_dev) if d_texels_dev is not None else None, flags))
+
     def set_texture(self, texture: int, pixels) -> None:
         """tpt_scene_set_texture: new level-0 texels (same shape) for a texture."""
         px = np.ascontiguousarray(pixels, np.float32)

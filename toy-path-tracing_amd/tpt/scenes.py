@@ -475,14 +475,17 @@ def cornell_textured(build: bool = True) -> CornellScene:
     return CornellScene(s, mats, li, cam)
 
 
-def cornell_inverse(build: bool = True, seed: int = 0, tex_size: int = 512) -> CornellScene:
+def cornell_inverse(build: bool = True, seed: int = 0, tex_size: int = 512, glossy: bool = False) -> CornellScene:
     """The inverse-rendering Cornell box: the Cornell geometry split into 16
     OpenPBR materials (five walls, the six faces of the short box, five faces
     of the tall box) with seeded diffuse base colours, a tex_size² RGB
     base-colour texture (seeded smooth noise) on the back wall, and the
     OpenPBR emitter as material 16.  Every material is affine in its base
     colour (metalness 0, no coat), so the adjoint differentiates all 16 base
-    colours and the texels."""
+    colours and the texels.  glossy=True: the inverse-rendering
+    configuration's parameter set instead -- every material also draws a
+    specular_roughness in [0.2, 0.8] and a metalness in [0, 1] from the seed
+    (specular weight 1), all three learnable (tpt_render_adjoint_params)."""
     from .material import OpenPbr
     from .texture import Texture
 
@@ -522,6 +525,10 @@ def cornell_inverse(build: bool = True, seed: int = 0, tex_size: int = 512) -> C
     if build:
         s.build_qbvh()
     mats = [OpenPbr.new(tuple(rng.uniform(0.2, 0.9, 3).tolist())).with_(specular_weight=0.0) for _ in range(mi)]
+    if glossy:
+        g = np.random.default_rng(seed + 1)
+        mats = [m.with_(specular_weight=1.0, specular_roughness=float(g.uniform(0.2, 0.8)),
+                        base_metalness=float(g.uniform(0.0, 1.0))) for m in mats]
     mats[2] = mats[2].with_(base_color_texture=tex)
     mats.append(OpenPbr.new((0.0, 0.0, 0.0)).with_(base_weight=0.0, specular_weight=0.0, emission_luminance=15.0,
                                                    emission_color=(1.0, 1.0, 1.0)))
