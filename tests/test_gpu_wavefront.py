@@ -124,7 +124,7 @@ def test_autotuner_picks_a_setting_no_slower_and_keeps_the_image(device, scene):
         res = ds.autotune([(cam, w, h, 64, 8)], tune_spp=64)  # 1 Mi paths: two paths-in-flight candidates fit
         print("autotune", res)
         assert res["ms_best"] <= res["ms_default"] and res["trials"] >= 10
-        assert res["wf_paths"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22)
+        assert res["wf_paths"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
         assert res["blocks_per_cu"] in (0, 2, 3, 4, 6, 8) and res["refill_below"] in (8, 16, 24, 32, 48)
         assert res["wf_sort"] in (0, 1)
         (img,) = ds.render_batch([(cam, w, h, 8, 8)], exact_shadow=True)

@@ -850,7 +850,7 @@ extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, 
       }
       return TPT_OK;
     };
-    const uint32_t paths[] = {1u << 19, 1u << 20, 1u << 21, 1u << 22};
+    const uint32_t paths[] = {1u << 19, 1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24};
     const uint32_t bpcs[] = {0u, 2u, 3u, 4u, 6u, 8u};
     const uint32_t refills[] = {8u, 16u, 24u, 32u, 48u};
     size_t batch_paths = 0;
