@@ -466,8 +466,8 @@ inline V3 offset_origin(V3 p, V3 ng, V3 dir) {
 // the CDF scaled by 1 / total.  kind: 0 tree, 1 environment, 2 + i
 // directional light i.
 struct Category {
-  int kind;
-  float pmf;
+  int kind = -1;
+  float pmf = 0.0f;
 };
 struct LightSampler {
   std::vector<int> kind;
