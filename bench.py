@@ -233,6 +233,22 @@ def main():
         "adjoint": adjoint,
         "city": city,
     }
+    # the three throughputs side by side: samples/s of the wavefront path
+    # tracer with 8-bit nodes against the plain FP32 megakernel on the city
+    # batch (equal relMSE: the same estimator on the same samples), Mrays/s,
+    # and gradients/s of the full-parameter adjoint
+    summary = {"mrays_per_s": line["value"]}
+    if city:
+        summary["city_samples_per_s"] = {
+            "megakernel_fp32": city["megakernel"]["samples_per_s"],
+            "wavefront_fp32": city["wavefront"]["samples_per_s"],
+            "wavefront_q8": city["wavefront_q8"]["samples_per_s"],
+            "q8_wavefront_vs_fp32_megakernel": city["wavefront_q8"]["speedup_vs_megakernel"],
+            "fp32_wavefront_vs_fp32_megakernel": city["wavefront"]["speedup_vs_megakernel"],
+        }
+    if adjoint:
+        summary["grad_per_s"] = {k: v["grad_per_s"] for k, v in adjoint.items()}
+    line["summary"] = summary
     print(json.dumps(line), flush=True)
     ds.close()
     dev.close()
