@@ -159,8 +159,9 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
         F3 rn[kParamRows], rs[kParamRows];
         const F3 zero = f3(0.0f, 0.0f, 0.0f);
         for (int k = 0; k < kParamRows; ++k) rn[k] = rs[k] = zero;
-        if (use_nee) bsdf_param_ratios(sv.tab, m, vtx, wn, rn);
-        if (use_smp) bsdf_param_ratios(sv.tab, m, vtx, s.wi, rs);
+        const int first = lin ? 3 : 0;  // an affine base colour takes the exact ratio below
+        if (use_nee) bsdf_param_ratios(sv.tab, m, vtx, wn, rn, first);
+        if (use_smp) bsdf_param_ratios(sv.tab, m, vtx, s.wi, rs, first);
         float gk[kParamRows];
         for (int k = 0; k < kParamRows; ++k)
           gk[k] = dot(adj.w, add(hadamard(nee_c, rn[k]), hadamard(rest, rs[k])));

@@ -1403,9 +1403,10 @@ __device__ __forceinline__ float& pbr_param(TptOpenPbr& p, int k) {
 // for every parameter k at once: df/dtheta_k by a central difference of step
 // kParamStep of the BSDF rebuilt at theta_k +- step (one-sided at 0, and at 1
 // for roughness and metalness, which the BSDF clamps there); 0 where f is 0.
-// r[k] receives the ratios of parameter k.
+// r[k] receives the ratios of parameter k, for k from `first` on (the others
+// stay 0).
 __device__ __noinline__ void bsdf_param_ratios(const Tables& tb, const TptMaterial& m, const Vtx& v, F3 wil,
-                                               F3 r[kParamRows]) {
+                                               F3 r[kParamRows], int first) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
   for (int k = 0; k < kParamRows; ++k) r[k] = zero;
   if (dot(v.wo, v.ng) <= 0.0f) return;
@@ -1415,7 +1416,7 @@ __device__ __noinline__ void bsdf_param_ratios(const Tables& tb, const TptMateri
   make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
   const F3 f = bsdf_eval(b, wol, wil);
   if (!(f.x > 0.0f) && !(f.y > 0.0f) && !(f.z > 0.0f)) return;
-  for (int k = 0; k < kParamRows; ++k) {
+  for (int k = first; k < kParamRows; ++k) {
     TptOpenPbr a = m.u.open_pbr, c = m.u.open_pbr;
     const float x = pbr_param(a, k);
     const float hi = k < 3 ? x + kParamStep : fminf(x + kParamStep, 1.0f);
