@@ -67,6 +67,15 @@ struct Ray3 {
   F3 o, d, inv;
   int kz;
   float sx, sy, sz;
+  // Constants of the conservative slab test (slab_cons): per axis the near
+  // plane's t is fma(plane, ns, no) and the far plane's fma(plane, fs, fo),
+  // rows selects the node rows that hold each axis's near plane (bit a: the
+  // direction's axis a is negative, so its near plane is the child's max),
+  // cons is false where the test's error bounds do not hold (a zero or
+  // out-of-range direction component): such a ray takes the exact test.
+  float ns[3], no[3], fs[3], fo[3];
+  uint32_t rows;
+  bool cons;
 };
 
 __device__ __forceinline__ void tri_setup(Ray3* r) {
@@ -174,6 +183,50 @@ struct SceneView {
 
 __device__ __forceinline__ float safe_inv(float d) { return d == 0.0f ? __builtin_inff() : 1.0f / d; }
 
+// Conservative slab test: per-ray constants.
+//
+// The reference's slab test (R/qbvh.rs:578-607) computes, per plane p of axis
+// a, r = fl(fl(p - o) * inv) and accepts a child when
+// max(near_x, near_y, near_z, 0) <= min(far_x, far_y, far_z, t_max).  The
+// conservative test computes each near plane as f_n = fma(p, ns, no) with
+// ns = fl(inv (1 - k)), no = fl(-o ns - m) and each far plane as
+// f_f = fma(p, fs, fo) with fs = fl(inv (1 + k)), fo = fl(-o fs + m), where
+// k = 2^-20 and m = |o inv| 2^-21 + 2^-100.  With u = 2^-24 and T the exact t
+// of the plane, |r - T| <= (2u + u^2)|T| and
+//   f_n <= (1 - k)(1 + u)^2 T + u(1 + u)(|o inv|(1 + u) + m) - m(1 + u)
+// so for T >= 0, f_n <= r as soon as k >= 4u (the T terms) and
+// m >= 2u |o inv| (the constant terms); for T < 0 the same bound makes f_n
+// negative, where the reference's near value is negative too and both are
+// clamped to 0.  Symmetrically f_f >= r whenever T >= 0, and a far plane
+// with T < 0 makes the reference reject the child.  Hence every child the
+// reference accepts is accepted (the test answers a superset), with one
+// fused multiply-add per plane, IEEE min3 / max3 and no compare-selects.
+// The bounds need finite products: rays with a direction component of zero
+// or of reciprocal magnitude outside [2^-60, 2^60], or |o inv| >= 2^100, keep
+// the reference's test (cons = false).
+__device__ __forceinline__ void cons_setup(Ray3* r) {
+  constexpr float kLo = 1.0f - 0x1p-20f, kHi = 1.0f + 0x1p-20f;
+  const float o[3] = {r->o.x, r->o.y, r->o.z};
+  const float inv[3] = {r->inv.x, r->inv.y, r->inv.z};
+  bool ok = true;
+  uint32_t rows = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float ai = fabsf(inv[a]);
+    const float ao = fabsf(o[a] * inv[a]);
+    ok = ok && ai >= 0x1p-60f && ai <= 0x1p60f && ao < 0x1p100f;
+    const float m = __builtin_fmaf(ao, 0x1p-21f, 0x1p-100f);
+    const float sn = inv[a] * kLo, sf = inv[a] * kHi;
+    r->ns[a] = sn;
+    r->fs[a] = sf;
+    r->no[a] = __builtin_fmaf(-o[a], sn, -m);
+    r->fo[a] = __builtin_fmaf(-o[a], sf, m);
+    rows |= (inv[a] < 0.0f ? 1u : 0u) << a;
+  }
+  r->rows = rows;
+  r->cons = ok;
+}
+
 // glam Mat4::transform_point3 / transform_vector3 (x*px, then y*py + r, z*pz + r, w + r)
 __device__ __forceinline__ void to_local(const GpuInstance& in, F3 o, F3 d, Ray3* r) {
   const float* m = in.m;
@@ -188,9 +241,88 @@ __device__ __forceinline__ void to_local(const GpuInstance& in, F3 o, F3 d, Ray3
   r->d = f3(rx, ry, rz);
   r->inv = f3(safe_inv(rx), safe_inv(ry), safe_inv(rz));
   tri_setup(r);
+  cons_setup(r);
 }
 
 __device__ __forceinline__ float q8_scale(int e) { return __int_as_float((e + 127) << 23); }
+
+// Conservative slab test of one node (see cons_setup): the hit mask of a
+// superset of the children the reference's test accepts, and per child the
+// conservative entry distance lo (<= the reference's tmin; used only to order
+// children).  8-bit nodes are tested against their decoded boxes (which
+// contain the FP32 boxes, and the reference's test is monotone in the box),
+// ray-relative: per node and axis S = 2^e ns and B = fma(origin, ns, no), so
+// a plane origin + q 2^e costs one byte conversion and one fma(q, S, B).
+// That form rounds S (exactly: a power-of-two scale) and B once, and drops the
+// rounding of origin + q 2^e; the extra error is below
+// 3u(|B| + 255|S|) + u 255 |S|, so B is moved outward by
+// 2^-20 (|B| + 255|S|) (16u of the same quantity) for the near and far planes.
+template <bool kQ8>
+__device__ __forceinline__ uint32_t slab4_cons(const GpuNode* __restrict__ nodes, const GpuNodeQ8* __restrict__ qnodes,
+                                               uint32_t ni, const Ray3& r, float t_max, float lo[4],
+                                               uint32_t child[4]) {
+  float tn[3][4], tf[3][4];
+  bool fp32 = !kQ8;
+  if (kQ8) {
+    const uint4* qp = reinterpret_cast<const uint4*>(qnodes + ni);
+    const uint4 a = qp[0], b = qp[1], c = qp[2], ch = qp[3];
+    if ((a.w >> 24) & 1u) {
+      fp32 = true;
+    } else {
+      const float org[3] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)};
+      // q words: b.x = lox[0..3], b.y = hix, b.z = loy, b.w = hiy, c.x = loz, c.y = hiz
+      const uint32_t wlo[3] = {b.x, b.z, c.x}, whi[3] = {b.y, b.w, c.y};
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
+        const int e = (int)(int8_t)((a.w >> (8 * ax)) & 0xFFu);
+        const bool neg = (r.rows >> ax) & 1u;
+        const uint32_t wn = neg ? whi[ax] : wlo[ax], wf = neg ? wlo[ax] : whi[ax];
+        const float sn = __builtin_ldexpf(r.ns[ax], e), sf = __builtin_ldexpf(r.fs[ax], e);
+        float bn = __builtin_fmaf(org[ax], r.ns[ax], r.no[ax]);
+        float bf = __builtin_fmaf(org[ax], r.fs[ax], r.fo[ax]);
+        bn = __builtin_fmaf(-0x1p-20f, __builtin_fmaf(255.0f, fabsf(sn), fabsf(bn)), bn);
+        bf = __builtin_fmaf(0x1p-20f, __builtin_fmaf(255.0f, fabsf(sf), fabsf(bf)), bf);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          tn[ax][i] = __builtin_fmaf((float)((wn >> (8 * i)) & 0xFFu), sn, bn);
+          tf[ax][i] = __builtin_fmaf((float)((wf >> (8 * i)) & 0xFFu), sf, bf);
+        }
+      }
+      child[0] = ch.x;
+      child[1] = ch.y;
+      child[2] = ch.z;
+      child[3] = ch.w;
+    }
+  }
+  if (fp32) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + ni);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const uint32_t neg = (r.rows >> ax) & 1u;
+      const float4 nr = p[2 * ax + neg], fr = p[2 * ax + 1 - neg];
+      const float nv[4] = {nr.x, nr.y, nr.z, nr.w}, fv[4] = {fr.x, fr.y, fr.z, fr.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        tn[ax][i] = __builtin_fmaf(nv[i], r.ns[ax], r.no[ax]);
+        tf[ax][i] = __builtin_fmaf(fv[i], r.fs[ax], r.fo[ax]);
+      }
+    }
+    const uint4 ch = *reinterpret_cast<const uint4*>(p + 6);
+    child[0] = ch.x;
+    child[1] = ch.y;
+    child[2] = ch.z;
+    child[3] = ch.w;
+  }
+  uint32_t bits = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float l = fmaxf(fmaxf(tn[0][i], tn[1][i]), fmaxf(tn[2][i], 0.0f));
+    const float h = fminf(fminf(tf[0][i], tf[1][i]), fminf(tf[2][i], t_max));
+    lo[i] = l;
+    if (l <= h && child[i] != kEmptyChild) bits |= 1u << i;
+  }
+  return bits;
+}
 
 // Node slab test, intersect_4_aabbs (R/qbvh.rs:578-607).  Writes per-slot
 // tmin and child ref; returns the 4-bit hit mask restricted to non-empty slots.
@@ -333,12 +465,23 @@ __device__ __forceinline__ void cswap_desc(float& ta, uint32_t& ca, int& sa, flo
 // the end of the walk, so that it pops first.
 enum VisitOrder { kVisitReference = 0, kVisitNearLast = 1, kVisitSlot = 2 };
 
-template <bool kQ8, int kOrder = kVisitReference>
+//
+// kCons: test the children with the conservative slab test (slab4_cons) on
+// rays that allow it; it accepts every child the reference's test accepts,
+// so the walk only tests more triangles, never fewer, and orders children by
+// its conservative entry distances (an order that can differ from the
+// reference's only between children whose entry distances tie within
+// rounding).
+template <bool kQ8, int kOrder = kVisitReference, bool kCons = false>
 __device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8* qnodes, uint32_t ni,
                                            const Ray3& r, float t_max, LaneStack& st, LeafQueue& lq) {
   float tmin[4];
   uint32_t child[4];
-  const uint32_t bits = slab4<kQ8>(nodes, qnodes, ni, r, t_max, tmin, child);
+  uint32_t bits;
+  if (kCons && r.cons)
+    bits = slab4_cons<kQ8>(nodes, qnodes, ni, r, t_max, tmin, child);
+  else
+    bits = slab4<kQ8>(nodes, qnodes, ni, r, t_max, tmin, child);
   float t[4];
   uint32_t c[4];
   int s[4];
@@ -512,8 +655,12 @@ __device__ __forceinline__ bool lane_step(const SceneView& sc, Lane& L, LaneStac
       }
       const uint32_t ni = st.pop();
       if (kCount) cnt.nodes++;
-      visit_node<kQ8, kMode == kAny ? kAnyVisitOrder : kVisitReference>(sc.nodes, sc.qnodes, ni, L.r, L.closest, st,
-                                                                      L.bq);
+      // The conservative slab test serves the early-exit any-hit walk (its
+      // answer depends only on which triangles are tested) and 8-bit nodes
+      // (whose boxes already only grow the candidate set); FP32 closest-hit
+      // walks keep the reference's test and visit order.
+      visit_node<kQ8, kMode == kAny ? kAnyVisitOrder : kVisitReference, (kMode == kAny || kQ8)>(
+          sc.nodes, sc.qnodes, ni, L.r, L.closest, st, L.bq);
     }
     if (L.bq.n > 0) {
       const uint32_t c = L.bq.pop();

@@ -237,12 +237,15 @@ __device__ __forceinline__ void load_diff(const State& st, uint32_t P, uint32_t 
 template <bool kTex>
 __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Queue q, Counters* c,
                                                             const uint8_t* __restrict__ ztab, uint32_t ztab_dims) {
+  // One path slot per thread (the host launches ceil(paths / block) blocks):
+  // no thread carries state from one slot to another.
+  const uint32_t base = blockIdx.x * blockDim.x;
+  if (base >= ps.paths) return;  // uniform per block, before any barrier
   __shared__ __align__(16) uint8_t zrow[smp::kZTabPrefixes];
   stage_row(zrow, ztab, ztab_dims, smp::kCameraDim);
   __syncthreads();
-  const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t k = c->pass.v;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < ps.paths; base += stride) {
+  {
     const uint32_t path = base + threadIdx.x;
     bool want = false;
     F3 o = f3(0.0f, 0.0f, 0.0f), d = o;
@@ -344,6 +347,13 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
                                                 Counters* c, uint32_t in_idx, uint32_t bounce,
                                                 const uint8_t* __restrict__ ztab, uint32_t ztab_dims,
                                                 const uint32_t* __restrict__ order) {
+  // One queue slot per thread (the host launches ceil(paths / block) blocks
+  // and the queue never holds more than `paths` entries): no thread carries
+  // state from one slot to another.  Blocks past the queue length leave at
+  // once (uniform per block, before any barrier).
+  const uint32_t n = c->rays[in_idx].v;
+  const uint32_t base = blockIdx.x * blockDim.x;
+  if (base >= n) return;
   // digit-permutation rows of this bounce's sampler dimensions (vertex base +0, +1, +3, +5, +7, +13)
   __shared__ __align__(16) uint8_t zrows[6][smp::kZTabPrefixes];
   {
@@ -352,10 +362,8 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
     for (int r = 0; r < 6; ++r) stage_row(zrows[r], ztab, ztab_dims, bd + off[r]);
     __syncthreads();
   }
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t n = c->rays[in_idx].v;
   const uint32_t k = c->pass.v;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+  {
     const uint32_t slot = base + threadIdx.x;
     bool want_ray = false, want_shadow = false;
     uint32_t path = 0;
@@ -557,16 +565,18 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
   TptDevice* d = s->dev;
   auto gen = p.tex ? wf::wf_generate<true> : wf::wf_generate<false>;
   auto shade = p.tex ? wf::wf_shade<true> : wf::wf_shade<false>;
-  // One thread per path slot, so no thread of the generate and shade kernels
-  // runs its slot loop more than once.  With the shade grid capped at the
-  // resident capacity (2 waves per SIMD for the untextured kernel: 512
-  // blocks), threads took a second slot, and there this kernel's build gave
-  // wrong OpenPBR paths (the ones through the out-of-line BSDF calls),
-  // differently from run to run, while the textured build and every one-slot
-  // launch matched the megakernel bit for bit.  Blocks past the queue length
-  // exit at once.
+  // One thread per path slot: generate and shade take exactly one slot per
+  // thread (they have no slot loop).  An earlier form looped over slots with
+  // the shade grid capped at the resident capacity; there the untextured
+  // build gave wrong OpenPBR paths on a thread's second slot, differently
+  // from run to run (the cause was not found), so no launch may hand a
+  // thread more than one slot.  Blocks past the queue length exit at once.
   const unsigned g_gen = (unsigned)((p.ps.paths + wf::kAppendBlock - 1) / wf::kAppendBlock);
   const unsigned g_shade = g_gen;
+  if ((uint64_t)g_gen * wf::kAppendBlock < p.ps.paths) {
+    set_error("tpt_render_batch: generate / shade grid smaller than the paths in flight");
+    return false;
+  }
   const unsigned g_add = resident_blocks(d, wf::wf_shadow_add, p.ps.paths);
   const unsigned g_fold = resident_blocks(d, wf::wf_fold, p.ps.total_pixels);
   hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, 0u);

@@ -1,23 +1,44 @@
 """torch.autograd binding of the path-replay adjoint (tpt_render_adjoint,
-tpt_render_adjoint_texels).
+tpt_render_adjoint_texels, tpt_render_adjoint_params).
 
 ``render_albedo(albedo, scene, ...)`` renders the scene with the
 NormalizedLambert albedos taken from ``albedo`` (one RGB row per material;
-rows of other materials are ignored) and returns the image as a tensor whose
-backward pass runs the adjoint kernel: ``loss(render_albedo(a, ...)).backward()``
-fills ``a.grad`` with dLoss/d(albedo).  ``render_texture(texels, index, ...)``
-does the same for the level-0 texels of one OpenPBR base-colour texture.
-``render_params(params, scene, ...)`` takes one row of five OpenPBR
-parameters per material (base colour RGB, specular_roughness,
-base_metalness; rows of other materials are ignored) and differentiates
-through tpt_render_adjoint_params.
+rows of other materials are ignored and get zero gradient) and returns the
+image as a tensor whose backward pass runs the adjoint kernel:
+``loss(render_albedo(a, ...)).backward()`` fills ``a.grad`` with
+dLoss/d(albedo).  ``render_texture(texels, index, ...)`` does the same for the
+level-0 texels of one OpenPBR base-colour texture.  ``render_params(params,
+scene, ...)`` takes one row of five OpenPBR parameters per material (base
+colour RGB, specular_roughness, base_metalness; rows of other materials are
+ignored and get zero gradient) and differentiates through
+tpt_render_adjoint_params.
+
+The image, the incoming image gradient and the parameter / texel gradients
+stay in device memory: the library's calls are enqueued on torch's current
+stream with device pointers (no host copy of an image or a gradient).  Only
+the material table itself (a few hundred bytes per material) goes through
+host memory, because tpt_scene_set_materials takes host records.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
-from .material import NormalizedLambert
+from .material import NormalizedLambert, OpenPbr
+
+
+def _dev(device_scene) -> torch.device:
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _bind_stream(device_scene) -> None:
+    """Enqueue the library's work on torch's current stream, so the device
+    buffers it reads and writes are ordered with torch's own kernels."""
+    device_scene.device.set_stream(torch.cuda.current_stream().cuda_stream)
+
+
+def _grad_image(grad_image: torch.Tensor) -> torch.Tensor:
+    return grad_image.detach().to(_dev(None), torch.float32).contiguous()
 
 
 def with_albedo(materials, albedo: np.ndarray) -> list:
@@ -29,22 +50,41 @@ def with_albedo(materials, albedo: np.ndarray) -> list:
     return out
 
 
+def _row_mask(materials, kind, cols: int, device) -> torch.Tensor:
+    """1 for the rows of materials of the given class, 0 elsewhere."""
+    m = torch.tensor([1.0 if isinstance(x, kind) else 0.0 for x in materials], dtype=torch.float32)
+    return m.to(device)[:, None].expand(len(materials), cols)
+
+
+def _render(device_scene, camera, width, height, spp, max_depth) -> torch.Tensor:
+    _bind_stream(device_scene)
+    img = torch.empty((height, width, 3), dtype=torch.float32, device=_dev(device_scene))
+    device_scene.render_device(camera, width, height, spp, max_depth, img)
+    return img
+
+
 class AlbedoRender(torch.autograd.Function):
     @staticmethod
     def forward(ctx, albedo, device_scene, materials, camera, width, height, spp, max_depth):
         a = albedo.detach().to("cpu", torch.float32).numpy()
+        _bind_stream(device_scene)
         device_scene.set_materials(with_albedo(materials, a))
-        img = device_scene.render(camera, width, height, spp, max_depth)
+        img = _render(device_scene, camera, width, height, spp, max_depth)
         ctx.call = (device_scene, materials, a, camera, width, height, spp, max_depth)
-        return torch.from_numpy(img).to(albedo.device)
+        return img.to(albedo.device)
 
     @staticmethod
     def backward(ctx, grad_image):
         device_scene, materials, a, camera, width, height, spp, max_depth = ctx.call
+        _bind_stream(device_scene)
         device_scene.set_materials(with_albedo(materials, a))  # the forward's parameters
-        g = device_scene.render_adjoint(camera, width, height, spp, max_depth,
-                                        grad_image.detach().to("cpu", torch.float32).numpy())
-        return torch.from_numpy(g).to(grad_image.device), None, None, None, None, None, None, None
+        g_img = _grad_image(grad_image)
+        g = torch.zeros((len(materials), 3), dtype=torch.float32, device=g_img.device)
+        device_scene.render_adjoint_device(camera, width, height, spp, max_depth, g_img, g)
+        # the forward reads only NormalizedLambert rows (tpt_render_adjoint
+        # also fills OpenPBR base-colour rows)
+        g = g * _row_mask(materials, NormalizedLambert, 3, g.device)
+        return g.to(grad_image.device), None, None, None, None, None, None, None
 
 
 def render_albedo(albedo: torch.Tensor, device_scene, materials, camera, width: int, height: int, spp: int,
@@ -53,22 +93,36 @@ def render_albedo(albedo: torch.Tensor, device_scene, materials, camera, width: 
     return AlbedoRender.apply(albedo, device_scene, materials, camera, width, height, spp, max_depth)
 
 
+def _texel_offsets(device_scene) -> list:
+    offs, o = [], 0
+    for t in device_scene.textures:
+        offs.append(o)
+        o += int(np.prod(t.pixels.shape))
+    return offs + [o]
+
+
 class TextureRender(torch.autograd.Function):
     @staticmethod
     def forward(ctx, texels, texture, device_scene, camera, width, height, spp, max_depth):
-        t = texels.detach().to("cpu", torch.float32).numpy()
-        device_scene.set_texture(texture, t)
-        img = device_scene.render(camera, width, height, spp, max_depth)
+        t = texels.detach().to(_dev(device_scene), torch.float32).contiguous()
+        _bind_stream(device_scene)
+        device_scene.set_texture_device(texture, t)
+        img = _render(device_scene, camera, width, height, spp, max_depth)
         ctx.call = (device_scene, texture, t, camera, width, height, spp, max_depth)
-        return torch.from_numpy(img).to(texels.device)
+        return img.to(texels.device)
 
     @staticmethod
     def backward(ctx, grad_image):
         device_scene, texture, t, camera, width, height, spp, max_depth = ctx.call
-        device_scene.set_texture(texture, t)  # the forward's texels
-        _, grads = device_scene.render_adjoint_texels(camera, width, height, spp, max_depth,
-                                                      grad_image.detach().to("cpu", torch.float32).numpy())
-        return torch.from_numpy(grads[texture]).to(grad_image.device), None, None, None, None, None, None, None
+        _bind_stream(device_scene)
+        device_scene.set_texture_device(texture, t)  # the forward's texels
+        g_img = _grad_image(grad_image)
+        offs = _texel_offsets(device_scene)
+        d_alb = torch.zeros((device_scene.material_count, 3), dtype=torch.float32, device=g_img.device)
+        d_tex = torch.zeros(max(1, offs[-1]), dtype=torch.float32, device=g_img.device)
+        device_scene.render_adjoint_texels_device(camera, width, height, spp, max_depth, g_img, d_alb, d_tex)
+        g = d_tex[offs[texture]:offs[texture + 1]].reshape(device_scene.textures[texture].pixels.shape)
+        return g.to(grad_image.device), None, None, None, None, None, None, None
 
 
 def render_texture(texels: torch.Tensor, texture: int, device_scene, camera, width: int, height: int, spp: int,
@@ -84,8 +138,6 @@ def with_params(materials, params: np.ndarray) -> list:
     """The material list with every OpenPBR material's base colour, specular
     roughness and metalness replaced by the matching row of ``params``
     (shape (len(materials), 5))."""
-    from .material import OpenPbr
-
     out = []
     for i, m in enumerate(materials):
         if isinstance(m, OpenPbr):
@@ -100,18 +152,24 @@ class ParamsRender(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params, device_scene, materials, camera, width, height, spp, max_depth):
         p = params.detach().to("cpu", torch.float32).numpy()
+        _bind_stream(device_scene)
         device_scene.set_materials(with_params(materials, p))
-        img = device_scene.render(camera, width, height, spp, max_depth)
+        img = _render(device_scene, camera, width, height, spp, max_depth)
         ctx.call = (device_scene, materials, p, camera, width, height, spp, max_depth)
-        return torch.from_numpy(img).to(params.device)
+        return img.to(params.device)
 
     @staticmethod
     def backward(ctx, grad_image):
         device_scene, materials, p, camera, width, height, spp, max_depth = ctx.call
+        _bind_stream(device_scene)
         device_scene.set_materials(with_params(materials, p))  # the forward's parameters
-        g, _ = device_scene.render_adjoint_params(camera, width, height, spp, max_depth,
-                                                  grad_image.detach().to("cpu", torch.float32).numpy())
-        return torch.from_numpy(g).to(grad_image.device), None, None, None, None, None, None, None
+        g_img = _grad_image(grad_image)
+        g = torch.zeros((len(materials), 5), dtype=torch.float32, device=g_img.device)
+        device_scene.render_adjoint_params_device(camera, width, height, spp, max_depth, g_img, g)
+        # the forward reads only OpenPBR rows (tpt_render_adjoint_params also
+        # fills NormalizedLambert albedo rows)
+        g = g * _row_mask(materials, OpenPbr, 5, g.device)
+        return g.to(grad_image.device), None, None, None, None, None, None, None
 
 
 def render_params(params: torch.Tensor, device_scene, materials, camera, width: int, height: int, spp: int,

@@ -517,6 +517,16 @@ class DeviceScene:
             o += k
         return out, grads
 
+    def render_adjoint_device(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dev,
+                              d_albedo_dev, fp64: bool = False) -> None:
+        """As render_adjoint with device buffers (torch tensors): dloss
+        (h*w*3), d_albedo (material_count*3); asynchronous."""
+        d = _lib.RenderDesc()
+        d.camera = camera
+        d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+        flags = _lib.TPT_FLAG_DEVICE_POINTERS | (_lib.TPT_FLAG_GRAD_FP64 if fp64 else 0)
+        check(lib().tpt_render_adjoint(self.handle, ctypes.byref(d), ptr(dloss_dev), ptr(d_albedo_dev), flags))
+
     def render_adjoint_texels_device(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dev,
                                      d_albedo_dev, d_texels_dev, fp64: bool = False) -> None:
         """As render_adjoint_texels with device buffers (torch tensors):
@@ -546,6 +556,13 @@ class DeviceScene:
         if px.shape != self.textures[texture].pixels.shape:
             raise ValueError("set_texture: the texture's shape must stay " + str(self.textures[texture].pixels.shape))
         check(lib().tpt_scene_set_texture(self.handle, texture, ptr(px), 0))
+
+    def set_texture_device(self, texture: int, pixels_dev) -> None:
+        """tpt_scene_set_texture from a device buffer (a torch tensor of the
+        texture's level-0 shape); synchronous."""
+        if tuple(pixels_dev.shape) != tuple(self.textures[texture].pixels.shape):
+            raise ValueError("set_texture: the texture's shape must stay " + str(self.textures[texture].pixels.shape))
+        check(lib().tpt_scene_set_texture(self.handle, texture, ptr(pixels_dev), _lib.TPT_FLAG_DEVICE_POINTERS))
 
     def render_batch(self, frames, quantized: bool = False, exact_shadow: bool = False) -> list:
         """Wavefront path tracer over a batch of frames (tpt_render_batch).
