@@ -562,6 +562,22 @@ int32_t tpt_texture_level(TptScene* scene, uint32_t texture, uint32_t level, flo
 int32_t tpt_trace_counts(TptScene* scene, uint32_t query, const TptRay* rays, size_t n, const uint32_t* target,
                          const float* distance, uint32_t* counts, uint32_t flags);
 
+/* ---- allocation-free queries ------------------------------------------------ */
+/* tpt_trace_closest (query 0: hits) or tpt_trace_any (query 1: unoccluded,
+ * with optional target / distance, TPT_FLAG_ANY_EXACT honoured) for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  No counterpart in the
+ * reference.  Device pointers only (flags must hold TPT_FLAG_DEVICE_POINTERS;
+ * TPT_FLAG_QUANTIZED selects 8-bit nodes).  The trace kernel's scratch (its
+ * work counters and status word, 16,896 bytes) is a static device array that
+ * each GPU holds until the process ends; the call clears it with a kernel,
+ * launches the trace on the device's stream and returns without
+ * synchronising or allocating.  At most 2^31 - 1 rays per call.  Calls that
+ * overlap on one GPU share that array and race; a traversal-stack overflow
+ * is not reported (as with device pointers in tpt_trace_closest). */
+int32_t tpt_trace_noalloc(TptScene* scene, uint32_t query, const TptRay* rays, size_t n, const uint32_t* target,
+                          const float* distance, TptHit* hits, uint8_t* unoccluded, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

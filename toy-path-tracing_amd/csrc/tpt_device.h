@@ -128,7 +128,18 @@ struct TptDevice {
   uint32_t wf_graph = 1;
   uint32_t wf_sort = 0;  // shade each bounce's hits grouped by material (counting sort of the queue)
   tpt::DeviceBuffer wf_buf;
+  // tpt_trace_noalloc: this device's address of the file-scope scratch array
+  // (looked up once by noalloc_setup) and the resident blocks per CU of its
+  // kernel variants
+  void* noalloc_scratch = nullptr;
+  int noalloc_per_cu[12] = {};
 };
+
+namespace tpt {
+// Looks up the allocation-free entry point's scratch on the current device
+// (tpt_device_create calls it once per device handle).
+bool noalloc_setup(TptDevice* d);
+}  // namespace tpt
 
 // A scene resident in device memory (tpt_scene_create).
 struct TptScene {

@@ -279,7 +279,9 @@ TptDevice* tpt_device_create(int32_t device_index) {
     if (!hip_ok(hipMalloc((void**)&d->status, 4), "hipMalloc(status)") ||
         !hip_ok(hipMemset(d->status, 0, 4), "hipMemset(status)") ||
         !hip_ok(hipMalloc((void**)&d->work, sizeof(WorkCounters)), "hipMalloc(work)") ||
-        !hip_ok(hipMemset(d->work, 0, sizeof(WorkCounters)), "hipMemset(work)")) {
+        !hip_ok(hipMemset(d->work, 0, sizeof(WorkCounters)), "hipMemset(work)") || !noalloc_setup(d)) {
+      (void)hipFree(d->status);
+      (void)hipFree(d->work);
       (void)hipStreamDestroy(d->own);
       delete d;
       return nullptr;
