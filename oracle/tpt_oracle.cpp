@@ -581,8 +581,11 @@ void orc_sg_product(const float* a1, float s1, const float* a2, float s2, float*
 // camera ray, trace_radiance, running mean, exposure.  rgb: 3 floats per
 // pixel of the full frame (only the rows rendered are written).
 // cam: eye, forward, right, image_up, tan_half_fov_y, pixel_spread, exposure.
-void orc_render(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth, uint32_t y0,
-                uint32_t y1, float* rgb, int threads) {
+// Pixels [x0, x1) x [y0, y1) of a w x hgt frame (the pixel loop of
+// R/main.rs:98-126 restricted to a rectangle; every pixel's samples depend on
+// the full frame size only), written row-major into a (y1-y0) x (x1-x0) crop.
+void orc_render_rect(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth,
+                     uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, float* rgb, int threads) {
   const RenderScene& rs = *(RenderScene*)h;
   CF3 eye{cam[0], cam[1], cam[2]}, fwd{cam[3], cam[4], cam[5]}, right{cam[6], cam[7], cam[8]},
       up{cam[9], cam[10], cam[11]};
@@ -595,9 +598,11 @@ void orc_render(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t sp
     return normalize(add(add(fwd, scale(sx, right)), scale(sy, up)));
   };
   if (y1 > hgt) y1 = hgt;
-  const size_t n = (size_t)(y1 > y0 ? y1 - y0 : 0) * w;
+  if (x1 > w) x1 = w;
+  const uint32_t cw = x1 > x0 ? x1 - x0 : 0;
+  const size_t n = (size_t)(y1 > y0 ? y1 - y0 : 0) * cw;
   parallel_for(n, threads, [&](size_t k) {
-    const uint32_t x = (uint32_t)(k % w), y = y0 + (uint32_t)(k / w);
+    const uint32_t x = x0 + (uint32_t)(k % cw), y = y0 + (uint32_t)(k / cw);
     V3 color = v3(0.f, 0.f, 0.f);
     for (uint32_t si = 0; si < spp; ++si) {
       PathSampler ps(x, y, si, spp, w, hgt);
@@ -621,11 +626,19 @@ void orc_render(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t sp
       const float cnt = (float)(si + 1);
       color = add(color, sdiv(sub(s, color), cnt));
     }
-    float* o = rgb + 3 * ((size_t)y * w + x);
+    float* o = rgb + 3 * ((size_t)(y - y0) * cw + (x - x0));
     o[0] = color.x * exposure;
     o[1] = color.y * exposure;
     o[2] = color.z * exposure;
   });
+}
+
+// Rows [y0, y1) of the frame into the full-frame buffer rgb (w * hgt * 3).
+void orc_render(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth, uint32_t y0,
+                uint32_t y1, float* rgb, int threads) {
+  if (y1 > hgt) y1 = hgt;
+  if (y1 <= y0) return;
+  orc_render_rect(h, cam, w, hgt, spp, max_depth, 0, w, y0, y1, rgb + 3 * (size_t)y0 * w, threads);
 }
 
 

@@ -62,6 +62,7 @@ def lib() -> ctypes.CDLL:
         L.orc_render_set_instance_material.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
         L.orc_render_set_materials.argtypes = [vp, vp, sz]
         L.orc_render.argtypes = [vp, vp] + [ctypes.c_uint32] * 6 + [vp, ctypes.c_int]
+        L.orc_render_rect.argtypes = [vp, vp] + [ctypes.c_uint32] * 8 + [vp, ctypes.c_int]
         u32, f32 = ctypes.c_uint32, ctypes.c_float
         L.orc_render_light_tree.argtypes = [vp, vp, u32]
         L.orc_render_light_tree.restype = u32
@@ -480,6 +481,14 @@ class OracleRenderer:
         out = np.zeros((height, width, 3), np.float32)
         y0, y1 = rows if rows else (0, height)
         lib().orc_render(self.h, _p(c), width, height, spp, max_depth, y0, y1, _p(out), threads)
+        return out
+
+    def render_rect(self, cam, width, height, spp, max_depth, x0, x1, y0, y1, threads=8):
+        """Pixels [x0, x1) x [y0, y1) of a width x height frame -> (y1-y0, x1-x0, 3)."""
+        c = np.array(list(cam.eye) + list(cam.forward) + list(cam.right) + list(cam.image_up)
+                     + [cam.tan_half_fov_y, cam.pixel_spread, cam.exposure], np.float32)
+        out = np.zeros((y1 - y0, x1 - x0, 3), np.float32)
+        lib().orc_render_rect(self.h, _p(c), width, height, spp, max_depth, x0, x1, y0, y1, _p(out), threads)
         return out
 
     def __del__(self):

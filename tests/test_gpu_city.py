@@ -106,8 +106,11 @@ def test_early_exit_shadow_batch_is_close(city):
 
 
 def test_lamps_light_the_street_at_night(device):
-    """Without the sky, a street-level frame is lit by the lamps alone (NEE
-    through the light tree of the 64 emitter triangles)."""
+    """Without the sky, a frame over the streets is lit by the lamps alone.
+    Every city material is OpenPBR, which under the reference's policy takes
+    no light-tree sample (OpenPbrMaterial has no light-tree precompute,
+    R/material/open_pbr.rs:592-606), so the lamps reach the image through
+    BSDF-sampled paths that hit an emitter (emission weight 1)."""
     c = scenes.city(grid=12, rocks=300, lights=64, sky_size=(256, 128), tex_size=64)
     c.scene.environment = None
     ds = c.scene.upload(device)

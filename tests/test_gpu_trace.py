@@ -198,5 +198,47 @@ def test_full_size_traversal_subsample(device):
     assert diff.sum() <= len(pr) // 100000 + 2
     assert np.all(np.isinf(got["t"][got["triangle"] == MISS]))
     sel = np.random.default_rng(0).choice(len(pr), 30000, replace=False)
-    ref = oracle.oracle_scene_from(sc).closest_hit(pr[sel])
+    orc = oracle.oracle_scene_from(sc)
+    ref = orc.closest_hit(pr[sel])
     _assert_hits_equal(got[sel], ref, "2M primary subsample")
+
+    # the 2,000,000 incoherent bounce rays: 30k subsample bit-exact against
+    # the oracle, every ray q8 == fp32 except ties
+    br = scenes.bounce_rays(pr, got, sf.mesh, 2_000_000)
+    gb = ds.closest_hit(br)
+    qb = ds.closest_hit(br, quantized=True)
+    diff = qb["triangle"] != gb["triangle"]
+    assert np.all(_ulp_diff(qb["t"][diff], gb["t"][diff]) <= 2), "bounce: q8 differs beyond a tie"
+    assert np.all(qb["triangle"][diff] != MISS) and np.all(gb["triangle"][diff] != MISS)
+    assert diff.sum() <= len(br) // 100000 + 2, diff.sum()
+    same = ~diff
+    assert np.array_equal(qb["t"][same].view(np.uint32), gb["t"][same].view(np.uint32))
+    sel = np.random.default_rng(1).choice(len(br), 30000, replace=False)
+    _assert_hits_equal(gb[sel], orc.closest_hit(br[sel]), "2M bounce subsample")
+
+    # the 2,000,000 shadow rays to 64 emissive triangles: the exact rule on a
+    # 30k subsample against the oracle's unoccluded_ray; the early-exit
+    # answers (fp32 and q8, all rays) against the exact rule, differing only
+    # on ties, each checked to have another hit at the target's distance
+    ss = scenes.shadow_rays(pr, got, sf.mesh, 2_000_000, n_lights=64)
+    exact = ds.unoccluded(ss.rays, ss.target, exact=True)
+    sel = np.random.default_rng(2).choice(len(ss.rays), 30000, replace=False)
+    assert np.array_equal(exact[sel], orc.unoccluded(ss.rays[sel], ss.target[sel])), "2M shadow subsample"
+    assert 0.01 < exact.mean() < 0.99
+    for q in (False, True):
+        fast = ds.unoccluded(ss.rays, ss.target, quantized=q)
+        bad = np.nonzero(fast != exact)[0]
+        assert len(bad) <= len(ss.rays) // 100000 + 2, (q, len(bad))
+        if len(bad):
+            # a tie: the closest hit is at the target triangle's own t within
+            # 2 ulp (which of the two the reference reports depends on the
+            # visit order)
+            h = orc.closest_hit(ss.rays[bad])
+            assert np.all(h["triangle"] != MISS)
+            for i, k in enumerate(bad):
+                v = sf.mesh.positions[sf.mesh.indices[ss.target[k, 1]]]
+                r = ss.rays[k]
+                tt = oracle.intersect_triangle(r["o"], r["d"], np.inf, v[0], v[1], v[2])
+                assert tt is not None, (q, k)
+                assert _ulp_diff(np.float32(tt[0]), h["t"][i:i + 1])[0] <= 2, (q, k, tt, h[i])
+    ds.close()

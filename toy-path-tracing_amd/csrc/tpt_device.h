@@ -116,12 +116,15 @@ struct TptDevice {
   uint32_t* status = nullptr;         // device word: bit 0 = traversal stack overflow
   tpt::WorkCounters* work = nullptr;  // persistent-kernel ray counters (self-resetting)
   // Defaults from a sweep on MI355X over the 2M-triangle traversal workload
-  // (refill 8..40 x blocks/CU 2..5): fewer resident waves than occupancy
-  // allows keep more of each wave's node and triangle lines in L2.
-  uint32_t refill_below = 24;         // refill a wave when fewer lanes are busy
+  // (refill 16..62 x blocks/CU 2..8, conservative any-hit slab test): the
+  // kernels are bound by VALU issue with divergent lanes, so every resident
+  // wave the occupancy allows helps, and refilling a wave once fewer than 40
+  // of its 64 lanes are busy keeps more lanes working (shadow rays 1.107 ->
+  // 0.948 ms against 4 blocks per CU and a threshold of 24).
+  uint32_t refill_below = 40;         // refill a wave when fewer lanes are busy
   uint32_t refill_grab = 0;           // ray ids a wave takes per work-counter atomic (at least its idle lanes)
   uint32_t regions = 8;               // work regions (multiple of the 8 XCDs keeps block b on region b's XCD)
-  uint32_t blocks_per_cu = 4;         // resident blocks per CU (0: occupancy limit)
+  uint32_t blocks_per_cu = 0;         // resident blocks per CU (0: occupancy limit)
   // wavefront path tracer: paths in flight per pass, graph capture of a pass,
   // and its buffers (grow-only)
   uint32_t wf_paths = 1u << 21;
