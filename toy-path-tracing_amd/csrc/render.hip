@@ -148,7 +148,10 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     const bool lin = kAdj && base_color_linear(m);
     if (kAdj && (lin || (adj.rows == kParamRows && m.kind == TPT_MATERIAL_OPEN_PBR))) {
       const F3 rest = sub(adj.total, radiance);  // collected after this vertex's NEE
+      // a smooth lobe's mirror sample (delta) carries the lobe's weight
+      // numerator, a function of every parameter but an affine base colour
       const bool use_nee = len2(nee_c) > 0.0f, use_smp = sampled && !s.delta;
+      const bool use_dlt = sampled && s.delta && m.kind == TPT_MATERIAL_OPEN_PBR && adj.rows == kParamRows;
       const F3 wn = normalize_or_zero(to_frame(vtx.frame, nee_dir));
       F3 g = f3(0.0f, 0.0f, 0.0f);
       const TptOpenPbr& p0 = sv.mats[vtx.material].u.open_pbr;  // the parameters, untextured
@@ -162,6 +165,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
         const int first = lin ? 3 : 0;  // an affine base colour takes the exact ratio below
         if (use_nee) bsdf_param_ratios(sv.tab, m, vtx, wn, rn, first);
         if (use_smp) bsdf_param_ratios(sv.tab, m, vtx, s.wi, rs, first);
+        else if (use_dlt) bsdf_delta_param_ratios(sv.tab, m, vtx, s.lobe, rs, first);
         float gk[kParamRows];
         for (int k = 0; k < kParamRows; ++k)
           gk[k] = dot(adj.w, add(hadamard(nee_c, rn[k]), hadamard(rest, rs[k])));

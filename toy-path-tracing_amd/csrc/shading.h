@@ -857,6 +857,7 @@ struct Sample {
   bool delta;
   bool glossy;  // BsdfFlags::GLOSSY: a rough coat, metal, specular or fuzz lobe (read with !delta)
   float cone;   // MaterialSample::cone_spread
+  int lobe;     // OpenPBR: the lobe picked (0 coat, 1 metal, 2 specular, 4 fuzz, 5 diffuse)
 };
 __device__ __forceinline__ bool finalize(const Bsdf& p, F3 wo, F3 wi, Sample* out) {  // :1958-1999
   const float pdf = bsdf_pdf(p, wo, wi);
@@ -914,6 +915,17 @@ __device__ __forceinline__ bool sample_two_lobe(const Bsdf& p, F3 wo, float ax, 
   if (wi.z <= 0.0f) return false;
   return finalize(p, wo, wi, out);
 }
+// The weight numerator of a smooth lobe's mirror sample (the sample's weight
+// is this over the lobe probability): the lobe's layer weight at the mirror
+// direction times its Fresnel term there (sample_coat / sample_metal /
+// sample_spec_brdf for smooth lobes, R/bsdf/open_pbr.rs:1467-1668).
+__device__ __forceinline__ F3 delta_numerator(const Bsdf& p, F3 wo, int lobe) {
+  const F3 mirror = f3(-wo.x, -wo.y, wo.z);
+  if (lobe == 0) return f3s(layer_weights(p, wo, &mirror).coat_amp * fresnel_dielectric(fabsf(mirror.z), 1.0f, p.coat_eta));
+  if (lobe == 1) return hadamard(layer_weights(p, wo, &mirror).metal, metal_fresnel(p, fabsf(mirror.z)));
+  if (lobe == 2) return hadamard(layer_weights(p, wo, &mirror).spec, f3s(base_fresnel(p, fabsf(mirror.z))));
+  return f3s(0.0f);
+}
 __device__ __forceinline__ bool delta_sample(F3 weight, F3 wi, float p_lobe, Sample* out) {
   out->weight = weight;
   out->wi = wi;
@@ -956,26 +968,18 @@ __device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, flo
   const float p_lobe = plv / pr.total;
   if (p_lobe <= 0.0f) return false;
   out->glossy = lobe != 5;
+  out->lobe = lobe;
   const F3 mirror = f3(-wo.x, -wo.y, wo.z);
   if (lobe == 0) {
-    if (coat_smooth(p)) {
-      const float f = fresnel_dielectric(fabsf(mirror.z), 1.0f, p.coat_eta);
-      return delta_sample(f3s(layer_weights(p, wo, &mirror).coat_amp * f / p_lobe), mirror, p_lobe, out);
-    }
+    if (coat_smooth(p)) return delta_sample(divs(delta_numerator(p, wo, 0), p_lobe), mirror, p_lobe, out);
     return sample_two_lobe(p, wo, p.cax, p.cay, p.tl_c, ux, uy, out);
   }
   if (lobe == 1) {
-    if (metal_smooth(p)) {
-      const F3 f = metal_fresnel(p, fabsf(mirror.z));
-      return delta_sample(divs(hadamard(layer_weights(p, wo, &mirror).metal, f), p_lobe), mirror, p_lobe, out);
-    }
+    if (metal_smooth(p)) return delta_sample(divs(delta_numerator(p, wo, 1), p_lobe), mirror, p_lobe, out);
     return sample_two_lobe(p, wo, p.ax, p.ay, p.tl_m, ux, uy, out);
   }
   if (lobe == 2) {
-    if (spec_smooth(p)) {
-      const float f = base_fresnel(p, fabsf(mirror.z));
-      return delta_sample(divs(hadamard(layer_weights(p, wo, &mirror).spec, f3s(f)), p_lobe), mirror, p_lobe, out);
-    }
+    if (spec_smooth(p)) return delta_sample(divs(delta_numerator(p, wo, 2), p_lobe), mirror, p_lobe, out);
     return sample_two_lobe(p, wo, p.ax, p.ay, p.tl_s, ux, uy, out);
   }
   if (lobe == 4) {
@@ -1428,6 +1432,40 @@ __device__ __noinline__ void bsdf_param_ratios(const Tables& tb, const TptMateri
     const F3 fa = bsdf_eval(b, wol, wil);
     make_bsdf_call(tb, v.material, c, v.front, wol, &b);
     const F3 fc = bsdf_eval(b, wol, wil);
+    const F3 d = divs(sub(fa, fc), hi - lo);
+    r[k] = f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
+  }
+}
+
+// The same ratios for a smooth lobe's mirror sample (lobe as bsdf_sample
+// picked it): the sample's weight is the lobe's numerator (delta_numerator,
+// a layer weight times a Fresnel term, both functions of the parameters)
+// over its lobe probability, which the adjoint holds fixed with the rest of
+// the sampling, so a contribution that went through it changes by
+// (d numerator / d theta_k) / numerator per unit of theta_k.
+__device__ __noinline__ void bsdf_delta_param_ratios(const Tables& tb, const TptMaterial& m, const Vtx& v, int lobe,
+                                                     F3 r[kParamRows], int first) {
+  const F3 zero = f3(0.0f, 0.0f, 0.0f);
+  for (int k = 0; k < kParamRows; ++k) r[k] = zero;
+  if (dot(v.wo, v.ng) <= 0.0f) return;
+  const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
+  if (wol.z <= 0.0f) return;
+  Bsdf b;
+  make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
+  const F3 f = delta_numerator(b, wol, lobe);
+  if (!(f.x > 0.0f) && !(f.y > 0.0f) && !(f.z > 0.0f)) return;
+  for (int k = first; k < kParamRows; ++k) {
+    TptOpenPbr a = m.u.open_pbr, c = m.u.open_pbr;
+    const float x = pbr_param(a, k);
+    const float hi = k < 3 ? x + kParamStep : fminf(x + kParamStep, 1.0f);
+    const float lo = fmaxf(x - kParamStep, 0.0f);
+    if (!(hi > lo)) continue;
+    pbr_param(a, k) = hi;
+    pbr_param(c, k) = lo;
+    make_bsdf_call(tb, v.material, a, v.front, wol, &b);
+    const F3 fa = delta_numerator(b, wol, lobe);
+    make_bsdf_call(tb, v.material, c, v.front, wol, &b);
+    const F3 fc = delta_numerator(b, wol, lobe);
     const F3 d = divs(sub(fa, fc), hi - lo);
     r[k] = f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
   }
