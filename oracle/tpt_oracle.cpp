@@ -769,6 +769,66 @@ void orc_openpbr_probe(const OrcMaterial* m, int which, const float* wo, float a
   out[2] = r.z;
 }
 
+
+// ---- ray differentials and cones (test entry points) ------------------------------------
+// Ray record of the two entries below, 21 floats: origin[3], dir[3], has
+// differential (0 / 1), rx origin[3], ry origin[3], rx dir[3], ry dir[3],
+// cone width, cone spread.
+static Ray ray_from(const float* r) {
+  Ray ray;
+  ray.o = v3(r[0], r[1], r[2]);
+  ray.d = v3(r[3], r[4], r[5]);
+  ray.has_diff = r[6] != 0.0f;
+  ray.rxo = v3(r[7], r[8], r[9]);
+  ray.ryo = v3(r[10], r[11], r[12]);
+  ray.rxd = v3(r[13], r[14], r[15]);
+  ray.ryd = v3(r[16], r[17], r[18]);
+  ray.cone = make_cone(r[19], r[20]);
+  return ray;
+}
+static void ray_to(const Ray& ray, float* r) {
+  const V3 v[6] = {ray.o, ray.d, ray.rxo, ray.ryo, ray.rxd, ray.ryd};
+  r[0] = v[0].x; r[1] = v[0].y; r[2] = v[0].z;
+  r[3] = v[1].x; r[4] = v[1].y; r[5] = v[1].z;
+  r[6] = ray.has_diff ? 1.0f : 0.0f;
+  for (int k = 0; k < 4; ++k) {
+    r[7 + 3 * k] = v[2 + k].x;
+    r[8 + 3 * k] = v[2 + k].y;
+    r[9 + 3 * k] = v[2 + k].z;
+  }
+  r[19] = ray.cone.width;
+  r[20] = ray.cone.spread;
+}
+// compute_shading_differentials (R/scene/mod.rs:748-785 with the cone
+// fallback :821-859) at point p with face-forwarded geometric normal n and
+// position partials dpdu, dpdv for a ray (21 floats).  out: dudx, dvdx,
+// dudy, dvdy, dpdx[3], dpdy[3].
+void orc_shading_differentials(const float* p, const float* n, const float* dpdu, const float* dpdv,
+                               const float* ray, float* out) {
+  Vertex v;
+  v.p = v3(p[0], p[1], p[2]);
+  const Ray r = ray_from(ray);
+  shading_differentials(v, v3(n[0], n[1], n[2]), v3(dpdu[0], dpdu[1], dpdu[2]), v3(dpdv[0], dpdv[1], dpdv[2]), &r);
+  out[0] = v.dudx; out[1] = v.dvdx; out[2] = v.dudy; out[3] = v.dvdy;
+  out[4] = v.dpdx.x; out[5] = v.dpdx.y; out[6] = v.dpdx.z;
+  out[7] = v.dpdy.x; out[8] = v.dpdy.y; out[9] = v.dpdy.z;
+}
+// spawn_scattered_ray (R/integrator/mod.rs:58-75) with the specular ray
+// differential of a delta sample (:88-157).  vtx: p, ng, ns, wo, dpdx, dpdy,
+// dndu, dndv (3 floats each), dudx, dvdx, dudy, dvdy; the sample's delta
+// flag, cone spread and direction wi; out: the spawned ray (21 floats).
+void orc_spawn_scattered(const float* ray, float hit_t, const float* vtx, int delta, float cone_spread,
+                         const float* wi, float* out) {
+  Vertex v;
+  V3* f[8] = {&v.p, &v.ng, &v.ns, &v.wo, &v.dpdx, &v.dpdy, &v.dndu, &v.dndv};
+  for (int k = 0; k < 8; ++k) *f[k] = v3(vtx[3 * k], vtx[3 * k + 1], vtx[3 * k + 2]);
+  v.dudx = vtx[24]; v.dvdx = vtx[25]; v.dudy = vtx[26]; v.dvdy = vtx[27];
+  BSample s;
+  s.delta = delta != 0;
+  s.cone_spread = cone_spread;
+  ray_to(spawn_scattered(ray_from(ray), hit_t, v, s, v3(wi[0], wi[1], wi[2])), out);
+}
+
 }  // extern "C"
 
 // ===========================================================================

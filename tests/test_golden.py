@@ -72,3 +72,23 @@ def test_cornell_render():
     cam = camera.pinhole(**cs.camera, width=w, height=h)
     img = oracle.OracleRenderer(cs.scene, cs.materials).render(cam, w, h, spp, depth, threads=3)
     assert np.array_equal(bits(img), bits(z["image"]))
+
+
+def test_reference_images_match_the_scenes_and_the_oracle_converges_to_them():
+    """The CPU 16,384-spp reference images (tests/golden/reference_images.npz)
+    belong to the current scene generators, and the oracle's own renders
+    converge to them: relMSE falls roughly as 1/spp from 16 to 256 spp."""
+    import golden_images
+    from tpt import camera
+    from make_reference_images import CASES
+
+    for name in ("cornell_diffuse", "cornell_lambert_nee"):
+        make, size, depth = CASES[name]
+        cs = make()
+        ref, size, depth, spp = golden_images.reference(name, cs.scene)
+        assert spp == 16384 and ref.shape == (size, size, 3) and np.all(np.isfinite(ref)) and ref.mean() > 0
+        cam = camera.pinhole(**cs.camera, width=size, height=size)
+        r = oracle.OracleRenderer(cs.scene, cs.materials)
+        e16 = golden_images.relmse(r.render(cam, size, size, 16, depth), ref)
+        e256 = golden_images.relmse(r.render(cam, size, size, 256, depth), ref)
+        assert e256 < e16 / 6.0, (name, e16, e256)

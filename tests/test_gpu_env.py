@@ -8,6 +8,7 @@ finite differences with the sky lighting the scene."""
 import numpy as np
 import pytest
 
+import golden_images
 import oracle
 from tpt import camera, scenes
 from tpt.autograd import with_albedo
@@ -41,13 +42,9 @@ def test_env_render_matches_oracle(outdoor):
 
 def test_env_relmse_no_worse_than_cpu(outdoor):
     cs, ds = outdoor
-    w = h = 24
-    cam = camera.pinhole(**cs.camera, width=w, height=h)
-    ref = ds.render(cam, w, h, 8192, 6)
-    gpu = ds.render(cam, w, h, 32, 6)
-    cpu = oracle.OracleRenderer(cs.scene, cs.materials).render(cam, w, h, 32, 6)
-    e_gpu, e_cpu = relmse(gpu, ref), relmse(cpu, ref)
-    print(f"relMSE at 32 spp: gpu {e_gpu:.5f} cpu {e_cpu:.5f}")
+    name = "outdoor_openpbr" if cs.materials[0].__class__.__name__ == "OpenPbr" else "outdoor_lambert"
+    e_gpu, e_cpu = golden_images.gpu_and_cpu_relmse(name, cs, ds, 32)
+    print(f"{name} relMSE at 32 spp against the CPU reference: gpu {e_gpu:.5f} cpu {e_cpu:.5f}")
     assert e_gpu <= 1.1 * e_cpu + 1e-6
 
 

@@ -9,13 +9,15 @@ and the transcendental functions (cos / sin of direction sampling, atan2 of
 the albedo-table lookup, pow, exp of the device maths library vs the host
 libm) may differ in the last place, after which a path can take a different
 branch.  So most pixels are bit-identical
-and the images agree to the Monte Carlo noise level; and against a
-high-sample-count reference the GPU image's relMSE (mean over pixels and
-channels of (x - r)^2 / (r^2 + 1e-2)) is no worse than the CPU image's.
+and the images agree to the Monte Carlo noise level; and against the CPU
+oracle's 16,384-spp image (tests/golden/reference_images.npz) the GPU image's
+relMSE (mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2)) is no
+worse than the CPU image's at equal samples per pixel.
 """
 import numpy as np
 import pytest
 
+import golden_images
 import oracle
 import tpt
 from tpt import camera, scenes
@@ -53,15 +55,12 @@ def test_render_matches_oracle_pixel_by_pixel(cornell):
 
 
 def test_render_relmse_no_worse_than_cpu(cornell):
+    """Against the CPU oracle's 16,384-spp image (a committed fixture, not a
+    GPU render), the GPU image at 64 spp is no worse than the CPU's."""
     cs, ds = cornell
-    w = h = 32
-    spp = 64
-    cam = camera.pinhole(**cs.camera, width=w, height=h)
-    ref = ds.render(cam, w, h, 16384, 8)  # high-sample reference image
-    gpu = ds.render(cam, w, h, spp, 8)
-    cpu = oracle.OracleRenderer(cs.scene, cs.materials).render(cam, w, h, spp, 8)
-    e_gpu, e_cpu = relmse(gpu, ref), relmse(cpu, ref)
-    print(f"relMSE at {spp} spp: gpu {e_gpu:.4f} cpu {e_cpu:.4f}")
+    name = "cornell_diffuse" if cs.materials[0].specular_weight == 0.0 else "cornell_glossy"
+    e_gpu, e_cpu = golden_images.gpu_and_cpu_relmse(name, cs, ds, 64)
+    print(f"{name} relMSE at 64 spp against the CPU reference: gpu {e_gpu:.5f} cpu {e_cpu:.5f}")
     assert e_gpu <= 1.1 * e_cpu + 1e-6
 
 
@@ -124,14 +123,8 @@ def test_nee_render_matches_oracle(cornell_nee):
 
 def test_nee_render_relmse_no_worse_than_cpu(cornell_nee):
     cs, ds = cornell_nee
-    w = h = 32
-    spp = 64
-    cam = camera.pinhole(**cs.camera, width=w, height=h)
-    ref = ds.render(cam, w, h, 16384, 8)
-    gpu = ds.render(cam, w, h, spp, 8)
-    cpu = oracle.OracleRenderer(cs.scene, cs.materials).render(cam, w, h, spp, 8)
-    e_gpu, e_cpu = relmse(gpu, ref), relmse(cpu, ref)
-    print(f"NEE relMSE at {spp} spp: gpu {e_gpu:.5f} cpu {e_cpu:.5f}")
+    e_gpu, e_cpu = golden_images.gpu_and_cpu_relmse("cornell_lambert_nee", cs, ds, 64)
+    print(f"NEE relMSE at 64 spp against the CPU reference: gpu {e_gpu:.5f} cpu {e_cpu:.5f}")
     assert e_gpu <= 1.1 * e_cpu + 1e-6
 
 

@@ -12,6 +12,7 @@ queries, each still a convex combination of texels)."""
 import numpy as np
 import pytest
 
+import golden_images
 import oracle
 import tpt
 from tpt import _lib, camera, scenes
@@ -178,12 +179,6 @@ def test_textured_wavefront_equals_megakernel(cornell_tex):
 
 def test_textured_render_relmse_no_worse_than_cpu(cornell_tex):
     cs, ds = cornell_tex
-    w = h = 32
-    spp = 64
-    cam = camera.pinhole(**cs.camera, width=w, height=h)
-    ref = ds.render(cam, w, h, 16384, 8)
-    gpu = ds.render(cam, w, h, spp, 8)
-    cpu = oracle.OracleRenderer(cs.scene, cs.materials).render(cam, w, h, spp, 8)
-    e_gpu, e_cpu = relmse(gpu, ref), relmse(cpu, ref)
-    print(f"relMSE at {spp} spp: gpu {e_gpu:.4f} cpu {e_cpu:.4f}")
+    e_gpu, e_cpu = golden_images.gpu_and_cpu_relmse("cornell_textured", cs, ds, 64)
+    print(f"textured relMSE at 64 spp against the CPU reference: gpu {e_gpu:.5f} cpu {e_cpu:.5f}")
     assert e_gpu <= 1.1 * e_cpu + 1e-6
