@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--city", type=int, default=1,
                     help="also time the four-frame city batch (1) or not (0): megakernel vs wavefront fp32 / q8")
     ap.add_argument("--city-spp", type=int, default=4, help="samples per pixel of the city frames")
+    ap.add_argument("--city-adjoint", type=int, default=1,
+                    help="also time the batched multi-view adjoint on the inverse-rendering city (1) or not (0)")
+    ap.add_argument("--city-adjoint-spp", type=int, default=32, help="samples per pixel of the batched adjoint views")
     ap.add_argument("--refill", type=int, default=0, help="trace-kernel refill threshold (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident blocks per CU (0: occupancy)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
@@ -196,6 +199,7 @@ def main():
     render = render_bench(dev, stream, args) if args.render else None
     adjoint = adjoint_bench(dev, stream, args) if args.adjoint else None
     city = city_bench(dev, stream, args) if args.city else None
+    city_adjoint = city_adjoint_bench(dev, stream, args) if args.city_adjoint else None
 
     line = {
         "metric": "Mrays/s",
@@ -232,6 +236,7 @@ def main():
         "render": render,
         "adjoint": adjoint,
         "city": city,
+        "city_adjoint": city_adjoint,
     }
     # the three throughputs side by side: samples/s of the wavefront path
     # tracer with 8-bit nodes against the plain FP32 megakernel on the city
@@ -248,6 +253,8 @@ def main():
         }
     if adjoint:
         summary["grad_per_s"] = {k: v["grad_per_s"] for k, v in adjoint.items()}
+    if city_adjoint:
+        summary.setdefault("grad_per_s", {})["city_32_views"] = city_adjoint["grad_per_s"]
     line["summary"] = summary
     print(json.dumps(line), flush=True)
     ds.close()
@@ -460,6 +467,78 @@ def city_bench(dev, stream, args):
     dev.set_param("refill_below", 24)
     dev.set_param("wf_sort", 0)
     return res
+
+
+def city_adjoint_bench(dev, stream, args):
+    """Batched differentiable rendering at scale: the inverse-rendering city
+    (scenes.city_inverse: the city's geometry, 64 learnable OpenPBR
+    materials, four 1024x1024 FP32 base-colour textures from seeded Perlin
+    noise, the lamps and the sky), 32 views whose side is log-uniform in
+    256..1024, 32 spp, depth 8.  One gradient = the forward render of all
+    views in one wavefront batch (the autotuner's schedule) + one batched
+    path-replay adjoint launch over every view (tpt_render_adjoint_batch)
+    with respect to 320 parameters and 12.58 M texels.  The FP32 compensated
+    accumulation's error against FP64 accumulation is measured at 2 spp."""
+    import torch
+
+    from tpt import camera, scenes
+
+    spp, depth = args.city_adjoint_spp, 8
+    t = time.perf_counter()
+    c = scenes.city_inverse()
+    t_build = time.perf_counter() - t
+    ds = c.scene.upload(dev)
+    frames = [(camera.pinhole(**v, width=w, height=h), w, h, spp, depth) for v, w, h in c.views]
+    pixels = sum(w * h for _, w, h, _, _ in frames)
+    n_tex = sum(int(np.prod(tx.pixels.shape)) for tx in c.scene.textures)
+    log(f"city adjoint: {len(frames)} views, {pixels} pixels, {len(c.materials)} materials, {n_tex} texels, "
+        f"generated and built in {t_build:.1f}s")
+    tune = ds.autotune([(cam, w, h, s_, d) for cam, w, h, s_, d in frames], tune_spp=2)
+    outs = [torch.empty(w * h * 3, dtype=torch.float32, device="cuda") for _, w, h, _, _ in frames]
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    ds.render_batch_device(frames, outs)
+    b.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = a.elapsed_time(b)
+    # dL/dI of 0.5 |I - 0.5|^2 for every view
+    views = [f + ((o - 0.5).contiguous(),) for f, o in zip(frames, outs)]
+    d_par = torch.zeros(5 * len(c.materials), dtype=torch.float32, device="cuda")
+    d_tex = torch.zeros(n_tex, dtype=torch.float32, device="cuda")
+    a.record(stream)
+    ds.render_adjoint_batch_device(views, d_par, d_tex)
+    b.record(stream)
+    torch.cuda.synchronize()
+    adj_ms = a.elapsed_time(b)
+    ms = fwd_ms + adj_ms
+    log(f"city adjoint: forward {fwd_ms:.0f} ms + batched adjoint {adj_ms:.0f} ms = {1e3 / ms:.3f} grad/s")
+    # FP32 compensated vs FP64 accumulation, 2 spp
+    small = [(cam, w, h, 2, d, g) for (cam, w, h, _, d), g in zip(frames, [v[5] for v in views])]
+    p32, p64 = torch.zeros_like(d_par), torch.zeros_like(d_par)
+    t32, t64 = torch.zeros_like(d_tex), torch.zeros_like(d_tex)
+    ds.render_adjoint_batch_device(small, p32, t32)
+    ds.render_adjoint_batch_device(small, p64, t64, fp64=True)
+    torch.cuda.synchronize()
+    err_p = float((p32 - p64).abs().max() / p64.abs().max().clamp_min(1e-30))
+    err_t = float((t32 - t64).abs().max() / t64.abs().max().clamp_min(1e-30))
+    ds.close()
+    log(f"city adjoint: FP32 compensated vs FP64 accumulation, max error / max gradient: parameters {err_p:.2e}, "
+        f"texels {err_t:.2e}")
+    return {
+        "workload": f"inverse-rendering city ({sum(m.triangle_count() for m in c.scene.meshes)} triangles, "
+                    f"{len(c.materials) - 1} learnable OpenPBR materials + the lamp emitter, 4 x 1024x1024 FP32 "
+                    f"base-colour textures, 2048x1024 sky), {len(frames)} views of side 256..1024 (log-uniform), "
+                    f"{pixels} pixels, {spp} spp, depth {depth}, L2 image loss",
+        "parameters": 5 * (len(c.materials) - 1) + n_tex,
+        "forward_ms": round(fwd_ms, 1),
+        "adjoint_ms": round(adj_ms, 1),
+        "ms_per_gradient": round(ms, 1),
+        "grad_per_s": round(1e3 / ms, 4),
+        "adjoint_samples_per_s": round(pixels * spp / (adj_ms / 1e3), 1),
+        "forward_schedule": tune,
+        "fp32_vs_fp64_max_error": {"parameters": err_p, "texels": err_t},
+    }
 
 
 def adjoint_bench(dev, stream, args):

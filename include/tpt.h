@@ -434,6 +434,28 @@ int32_t tpt_render_adjoint_texels(TptScene* scene, const TptRenderDesc* desc, co
 int32_t tpt_render_adjoint_params(TptScene* scene, const TptRenderDesc* desc, const float* dloss_dimage,
                                   float* d_params, float* d_texels, uint32_t flags);
 
+/* One view of a batched adjoint: its render description (camera,
+ * resolution, samples per pixel, depth) and dLoss/d(image) of that view
+ * (width * height * 3 floats; host or device memory per the call's flags). */
+typedef struct TptAdjointFrameDesc {
+  TptRenderDesc render;
+  const float* dloss_dimage;
+} TptAdjointFrameDesc;
+
+/* tpt_render_adjoint_params for many views of uneven resolution in one
+ * launch: d_params (material_count * 5 floats) and d_texels (nullable, as in
+ * tpt_render_adjoint_texels) receive the sum over the views of each view's
+ * gradient.  Every pixel of every view is a thread; the block's gradient
+ * rows accumulate per sample index in FP32 (LDS) and are folded into FP32
+ * compensated sums (TwoSum) after every sample index, the blocks' sums and
+ * compensations then added in FP64; texel gradients are FP32 atomics.  With
+ * TPT_FLAG_GRAD_FP64 the block sums and texel gradients are FP64 instead (the
+ * reference for the FP32 accumulation's error).  At most 1024 materials and
+ * 2^31 pixels per batch.  No reference counterpart (the pixel loop of
+ * R/main.rs:98-126 per view, differentiated). */
+int32_t tpt_render_adjoint_batch(TptScene* scene, const TptAdjointFrameDesc* frames, uint32_t frame_count,
+                                 float* d_params, float* d_texels, uint32_t flags);
+
 /* Replaces the scene's material table (same count; kinds and emission must
  * stay the same, as the light tree is built from them): the parameter update
  * of an inverse-rendering loop.  Synchronous. */

@@ -23,8 +23,11 @@
 // 201-218), so for them NEE runs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <string>
+#include <vector>
 
 #include "sampler.h"
 #include "tpt_device.h"
@@ -74,12 +77,12 @@ __device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView&
 // primal pass, dLoss/dL of this sample (dLoss/dI_pixel * exposure / spp), the
 // block's gradient accumulators (3 per material, LDS) and the texel gradient
 // array (laid out like the texels; null when not requested).
-template <typename Acc>
+template <typename Acc, typename TexAcc = float>
 struct Adjoint {
   F3 total;
   F3 w;
   Acc* grad;        // rows of `rows` gradients per material
-  float* texgrad;
+  TexAcc* texgrad;  // laid out like the texels (every level)
   uint32_t rows;    // 3: albedo / base colour; 5 (kParamRows): + specular_roughness, base_metalness
 };
 
@@ -100,10 +103,10 @@ struct Adjoint {
 // continuation) at the sampled direction, sampling and MIS weights held
 // fixed; c = parameter * texture value, so dL/dparameter = dL/dc * texture
 // value and dL/dtexel = dL/dc * parameter * d(lookup)/d(texel).
-template <bool kTwoLevel, bool kAdj, bool kTex, typename Acc>
+template <bool kTwoLevel, bool kAdj, bool kTex, typename Acc, typename TexAcc = float>
 __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& sv, const RayDiff& camera_ray,
                                          const smp::PathSampler& ps, uint32_t max_depth, Lane& L, LaneStack& st,
-                                         bool& overflow, const Adjoint<Acc>& adj) {
+                                         bool& overflow, const Adjoint<Acc, TexAcc>& adj) {
   F3 radiance = f3(0.0f, 0.0f, 0.0f), thr = f3(1.0f, 1.0f, 1.0f);
   uint32_t hi, ht;
   float tt, hb0, hb1, hb2;
@@ -163,9 +166,10 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
         const F3 zero = f3(0.0f, 0.0f, 0.0f);
         for (int k = 0; k < kParamRows; ++k) rn[k] = rs[k] = zero;
         const int first = lin ? 3 : 0;  // an affine base colour takes the exact ratio below
-        if (use_nee) bsdf_param_ratios(sv.tab, m, vtx, wn, rn, first);
-        if (use_smp) bsdf_param_ratios(sv.tab, m, vtx, s.wi, rs, first);
-        else if (use_dlt) bsdf_delta_param_ratios(sv.tab, m, vtx, s.lobe, rs, first);
+        // one rebuild of the BSDF per parameter and side serves both the light
+        // direction and the sampled direction
+        if (use_nee || use_smp) bsdf_param_ratios2(sv.tab, m, vtx, use_nee, wn, rn, use_smp, s.wi, rs, first);
+        if (use_dlt) bsdf_delta_param_ratios(sv.tab, m, vtx, s.lobe, rs, first);
         float gk[kParamRows];
         for (int k = 0; k < kParamRows; ++k)
           gk[k] = dot(adj.w, add(hadamard(nee_c, rn[k]), hadamard(rest, rs[k])));
@@ -191,7 +195,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
         const pt::GpuTexture& t = sv.tex.tex[p0.base_color_texture];
         tv = tex::filtered<3>(sv.tex, t, uv, dx, dy);
         if (adj.texgrad)
-          tex::scatter_filtered<3>(sv.tex, t, uv, dx, dy, hadamard(g, rgb3(p0.base_color)), adj.texgrad);
+          tex::scatter_filtered<3, TexAcc>(sv.tex, t, uv, dx, dy, hadamard(g, rgb3(p0.base_color)), adj.texgrad);
       }
       const F3 gp = hadamard(g, tv);
       if (gp.x != 0.0f) atomicAdd(&gm[0], (Acc)gp.x);
@@ -327,6 +331,147 @@ __global__ void adjoint_reduce(const Acc* __restrict__ partials, uint32_t blocks
   if (i >= n_grad) return;
   double s = 0.0;
   for (uint32_t b = 0; b < blocks; ++b) s += (double)partials[(size_t)b * n_grad + i];
+  out[i] = (float)s;
+}
+
+
+// ---- batched adjoint over many views ----------------------------------------------
+// One view of a batched adjoint: camera, size, samples per pixel, depth, its
+// sampler frame, its first pixel in the batch and dLoss/d(image).
+struct AdjFrame {
+  CamView cam;
+  uint32_t w, h, spp, max_depth;
+  smp::ZSobolFrame zf;
+  uint32_t pix_begin;
+  uint32_t pad;
+  const float* dimg;
+};
+
+// The frame of global pixel gp (frames in pixel order; a binary search).
+__device__ __forceinline__ uint32_t adj_frame_of(const AdjFrame* fr, uint32_t n, uint32_t gp) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (fr[mid].pix_begin <= gp) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// TwoSum (Knuth): s + e == a + b exactly, s = fl(a + b).
+__device__ __forceinline__ float two_sum(float a, float b, float* e) {
+  const float s = a + b;
+  const float bp = s - a;
+  *e = (a - (s - bp)) + (b - bp);
+  return s;
+}
+
+// Path-replay adjoint of many views in one launch: one thread per pixel of
+// every view (the pixel loop of R/main.rs:98-126 per view), a primal pass
+// and a replay pass per pixel sample as adjoint_kernel.  The block's
+// gradient rows accumulate per sample index in an FP32 LDS epoch buffer
+// (LDS atomics over at most one sample of the block's pixels); after every
+// sample index the epoch is folded into the block's compensated FP32 sums
+// (sum, compensation) with TwoSum, so the long accumulation over samples
+// carries no rounding drift; each block writes sum and compensation, which a
+// second kernel adds in FP64.  kFp64: FP64 LDS sums and FP64 texel
+// gradients instead (the reference accumulation of the error report).
+template <bool kTwoLevel, bool kTex, bool kFp64>
+__global__ __launch_bounds__(kBlock) void adjoint_batch_kernel(SceneView sc, ShadeView sv,
+                                                               const AdjFrame* __restrict__ frames, uint32_t nframes,
+                                                               uint32_t total, uint32_t max_spp,
+                                                               void* __restrict__ partials, uint32_t n_grad,
+                                                               void* __restrict__ texgrad,
+                                                               uint32_t* __restrict__ status) {
+  using Acc = typename std::conditional<kFp64, double, float>::type;
+  __shared__ uint32_t lds[kLdsDepth * kBlock];
+  extern __shared__ __align__(16) unsigned char dyn[];
+  Acc* epoch = reinterpret_cast<Acc*>(dyn);                  // n_grad
+  float* sum = reinterpret_cast<float*>(epoch + n_grad);     // n_grad (FP32 mode)
+  float* comp = sum + n_grad;                                // n_grad (FP32 mode)
+  for (uint32_t i = threadIdx.x; i < n_grad; i += kBlock) {
+    epoch[i] = (Acc)0;
+    if (!kFp64) sum[i] = comp[i] = 0.0f;
+  }
+  __syncthreads();
+  uint32_t spill[kStackDepth - kLdsDepth];
+  const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = gp < total;
+  LaneStack st;
+  st.lds = lds + threadIdx.x;
+  st.spill = spill;
+  st.sp = 0;
+  st.overflow = false;
+  Lane L;
+  bool overflow = false;
+  AdjFrame fr;
+  uint32_t x = 0, y = 0;
+  Adjoint<Acc, Acc> adj;
+  adj.grad = epoch;
+  adj.texgrad = reinterpret_cast<Acc*>(texgrad);
+  adj.rows = kParamRows;
+  if (live) {
+    fr = frames[adj_frame_of(frames, nframes, gp)];
+    const uint32_t lp = gp - fr.pix_begin;
+    x = lp % fr.w;
+    y = lp / fr.w;
+    const float scale = fr.cam.exposure / (float)fr.spp;
+    adj.w = f3(fr.dimg[3 * lp] * scale, fr.dimg[3 * lp + 1] * scale, fr.dimg[3 * lp + 2] * scale);
+  }
+  const Adjoint<Acc, Acc> none{};
+  for (uint32_t si = 0; si < max_spp; ++si) {
+    if (live && si < fr.spp) {
+      const smp::PathSampler ps{fr.zf, x, y, si};
+      const RayDiff ray = camera_ray<kTex>(fr.cam, ps, x, y, fr.w, fr.h, fr.spp);
+      adj.total = trace_path<kTwoLevel, false, kTex>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, none);
+      (void)trace_path<kTwoLevel, true, kTex>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, adj);
+    }
+    if (!kFp64) {  // fold the sample's epoch into the compensated sums
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < n_grad; i += kBlock) {
+        const float v = (float)epoch[i];
+        if (v != 0.0f) {
+          float e;
+          sum[i] = two_sum(sum[i], v, &e);
+          comp[i] += e;
+          epoch[i] = (Acc)0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (overflow) atomicOr(status, 1u);
+  __syncthreads();
+  if (kFp64) {
+    double* out = reinterpret_cast<double*>(partials) + (size_t)blockIdx.x * n_grad;
+    for (uint32_t i = threadIdx.x; i < n_grad; i += kBlock) out[i] = (double)epoch[i];
+  } else {
+    float* out = reinterpret_cast<float*>(partials) + (size_t)blockIdx.x * 2 * n_grad;
+    for (uint32_t i = threadIdx.x; i < n_grad; i += kBlock) {
+      out[2 * i] = sum[i];
+      out[2 * i + 1] = comp[i];
+    }
+  }
+}
+
+// Adds the blocks' partial sums in FP64: (sum, compensation) pairs, or FP64
+// partials.
+template <bool kFp64>
+__global__ void adjoint_batch_reduce(const void* __restrict__ partials, uint32_t blocks, uint32_t n_grad,
+                                     float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_grad) return;
+  double s = 0.0;
+  if (kFp64) {
+    const double* p = reinterpret_cast<const double*>(partials);
+    for (uint32_t b = 0; b < blocks; ++b) s += p[(size_t)b * n_grad + i];
+  } else {
+    const float* p = reinterpret_cast<const float*>(partials);
+    for (uint32_t b = 0; b < blocks; ++b) {
+      const float* q = p + (size_t)b * 2 * n_grad + 2 * i;
+      s += (double)q[0] + (double)q[1];
+    }
+  }
   out[i] = (float)s;
 }
 
@@ -671,6 +816,142 @@ extern "C" int32_t tpt_render_adjoint_texels(TptScene* s, const TptRenderDesc* d
 extern "C" int32_t tpt_render_adjoint_params(TptScene* s, const TptRenderDesc* desc, const float* dloss_dimage,
                                              float* d_params, float* d_texels, uint32_t flags) {
   return render_adjoint(s, desc, dloss_dimage, d_params, d_texels, flags, pt::kParamRows);
+}
+
+namespace {
+// The batched adjoint (tpt_render_adjoint_batch), kFp64: FP64 block sums and
+// texel gradients.
+template <bool kFp64>
+int32_t adjoint_batch_impl(TptScene* s, const TptAdjointFrameDesc* frames, uint32_t nframes, float* d_params,
+                           float* d_texels, bool dev_ptr) {
+  using Acc = typename std::conditional<kFp64, double, float>::type;
+  TptDevice* d = s->dev;
+  const uint32_t n_grad = pt::kParamRows * s->material_count;
+  std::vector<pt::AdjFrame> host(nframes);
+  uint64_t total = 0;
+  uint32_t max_spp = 0;
+  size_t img_floats = 0;
+  for (uint32_t f = 0; f < nframes; ++f) {
+    const TptRenderDesc& r = frames[f].render;
+    pt::AdjFrame& a = host[f];
+    a.cam = cam_view(r.camera);
+    a.w = r.width;
+    a.h = r.height;
+    a.spp = r.samples_per_pixel;
+    a.max_depth = r.max_depth;
+    a.zf = smp::zsobol_frame(r.samples_per_pixel, r.width, r.height);
+    a.pix_begin = (uint32_t)total;
+    a.pad = 0;
+    a.dimg = frames[f].dloss_dimage;
+    total += (uint64_t)r.width * r.height;
+    img_floats += (size_t)r.width * r.height * 3;
+    max_spp = std::max(max_spp, r.samples_per_pixel);
+  }
+  const unsigned blocks = (unsigned)((total + kBlock - 1) / kBlock);
+  const size_t nb_frames = align_up(nframes * sizeof(pt::AdjFrame));
+  const size_t nb_part = align_up((size_t)blocks * n_grad * (kFp64 ? 8 : 8));  // FP64 partials or (sum, comp) pairs
+  const size_t nb_img = dev_ptr ? 0 : align_up(img_floats * 4);
+  const size_t nb_out = dev_ptr ? 0 : align_up((size_t)n_grad * 4);
+  const size_t n_tg = d_texels ? texel_grad_floats(s) : 0;
+  const size_t nb_tbuf = d_texels ? align_up(texel_floats(s) * sizeof(Acc)) : 0;
+  const size_t nb_tout = (d_texels && !dev_ptr) ? align_up(n_tg * 4) : 0;
+  DeviceBuffer tmp;
+  if (!tmp.ensure(nb_frames + nb_part + nb_img + nb_out + nb_tbuf + nb_tout)) return TPT_ERROR_DEVICE;
+  char* base = (char*)tmp.ptr;
+  pt::AdjFrame* dframes = (pt::AdjFrame*)base;
+  void* partials = base + nb_frames;
+  float* dimg_stage = (float*)(base + nb_frames + nb_part);
+  float* dgrad = dev_ptr ? d_params : (float*)(base + nb_frames + nb_part + nb_img);
+  Acc* tbuf = d_texels ? (Acc*)(base + nb_frames + nb_part + nb_img + nb_out) : nullptr;
+  float* tout = d_texels ? (dev_ptr ? d_texels : (float*)(base + nb_frames + nb_part + nb_img + nb_out + nb_tbuf))
+                         : nullptr;
+  hipStream_t st = d->stream;
+  if (!dev_ptr) {  // stage every view's dLoss/d(image)
+    size_t o = 0;
+    for (uint32_t f = 0; f < nframes; ++f) {
+      const size_t nf = (size_t)host[f].w * host[f].h * 3;
+      if (!hip_ok(hipMemcpyAsync(dimg_stage + o, frames[f].dloss_dimage, nf * 4, hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync(dloss)"))
+        return TPT_ERROR_DEVICE;
+      host[f].dimg = dimg_stage + o;
+      o += nf;
+    }
+  }
+  if (!hip_ok(hipMemcpyAsync(dframes, host.data(), nframes * sizeof(pt::AdjFrame), hipMemcpyHostToDevice, st),
+              "hipMemcpyAsync(frames)") ||
+      !hip_ok(hipMemsetAsync(d->status, 0, 4, st), "hipMemsetAsync(status)"))
+    return TPT_ERROR_DEVICE;
+  if (tbuf && !hip_ok(hipMemsetAsync(tbuf, 0, nb_tbuf, st), "hipMemsetAsync(texel gradients)")) return TPT_ERROR_DEVICE;
+  const pt::ShadeView sv = pt::shade_view_of(s);
+  const SceneView v = view_of(s);
+  const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
+  const bool tex = s->texture_count > 0;
+  const size_t lds_bytes = (size_t)n_grad * (kFp64 ? sizeof(double) : 3 * sizeof(float));
+  auto kernel = two ? (tex ? pt::adjoint_batch_kernel<true, true, kFp64> : pt::adjoint_batch_kernel<true, false, kFp64>)
+                    : (tex ? pt::adjoint_batch_kernel<false, true, kFp64> : pt::adjoint_batch_kernel<false, false, kFp64>);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), lds_bytes, st, v, sv, (const pt::AdjFrame*)dframes, nframes,
+                     (uint32_t)total, max_spp, partials, n_grad, (void*)tbuf, d->status);
+  hipLaunchKernelGGL((pt::adjoint_batch_reduce<kFp64>), dim3((n_grad + 255) / 256), dim3(256), 0, st,
+                     (const void*)partials, blocks, n_grad, dgrad);
+  if (!hip_ok(hipGetLastError(), "adjoint batch launch")) return TPT_ERROR_DEVICE;
+  if (tbuf && !texel_grads_to_level0<Acc>(s, tbuf, tout, st)) return TPT_ERROR_DEVICE;
+  uint32_t status = 0;
+  if (!dev_ptr &&
+      (!hip_ok(hipMemcpyAsync(d_params, dgrad, (size_t)n_grad * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync") ||
+       (d_texels && n_tg &&
+        !hip_ok(hipMemcpyAsync(d_texels, tout, n_tg * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))))
+    return TPT_ERROR_DEVICE;
+  if (!hip_ok(hipMemcpyAsync(&status, d->status, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(status)") ||
+      !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))  // the scratch is released on return
+    return TPT_ERROR_DEVICE;
+  if (status & 1u) {
+    set_error("tpt_render_adjoint_batch: traversal stack overflow (128 entries)");
+    return TPT_ERROR_OVERFLOW;
+  }
+  return TPT_OK;
+}
+}  // namespace
+
+extern "C" int32_t tpt_render_adjoint_batch(TptScene* s, const TptAdjointFrameDesc* frames, uint32_t frame_count,
+                                            float* d_params, float* d_texels, uint32_t flags) {
+  try {
+    if (!s || !frames || !frame_count || !d_params) {
+      set_error("tpt_render_adjoint_batch: null argument or no frame");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    uint64_t total = 0;
+    for (uint32_t f = 0; f < frame_count; ++f) {
+      const TptRenderDesc& r = frames[f].render;
+      if (!frames[f].dloss_dimage || r.width == 0 || r.height == 0 || r.samples_per_pixel == 0) {
+        set_error("tpt_render_adjoint_batch: frame " + std::to_string(f) +
+                  " has a null dloss_dimage, an empty size or samples_per_pixel 0");
+        return TPT_ERROR_BAD_ARGUMENT;
+      }
+      total += (uint64_t)r.width * r.height;
+    }
+    if (total > 0x7FFFFF00ull) {
+      set_error("tpt_render_adjoint_batch: more than 2^31 pixels in one batch");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    // the block's rows live in LDS next to the traversal stacks: epoch,
+    // sum and compensation (FP32) or FP64 sums, 5 rows per material
+    if (s->material_count == 0 || s->material_count > 1024) {
+      set_error("tpt_render_adjoint_batch: needs 1..1024 materials");
+      return TPT_ERROR_BAD_ARGUMENT;
+    }
+    if (!render_materials_ok(s, "tpt_render_adjoint_batch")) return TPT_ERROR_BAD_ARGUMENT;
+    if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
+    if (!s->cond_tables) {  // the metalness derivative reads the conductor tables
+      s->force_cond = true;
+      if (!upload_ggx_tables(s, s->host_mats)) return TPT_ERROR_DEVICE;
+    }
+    const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
+    return (flags & TPT_FLAG_GRAD_FP64) ? adjoint_batch_impl<true>(s, frames, frame_count, d_params, d_texels, dev_ptr)
+                                        : adjoint_batch_impl<false>(s, frames, frame_count, d_params, d_texels, dev_ptr);
+  } catch (const std::exception& e) {
+    set_error(std::string("tpt_render_adjoint_batch: ") + e.what());
+    return TPT_ERROR;
+  }
 }
 
 extern "C" int32_t tpt_scene_set_materials(TptScene* s, const TptMaterial* materials, uint32_t count) {

@@ -1403,23 +1403,30 @@ constexpr float kParamStep = 1.0e-3f;
 __device__ __forceinline__ float& pbr_param(TptOpenPbr& p, int k) {
   return k < 3 ? p.base_color[k] : (k == 3 ? p.specular_roughness : p.base_metalness);
 }
-// The per-channel ratios (df/dtheta_k) / f of an OpenPBR BSDF value at wil,
-// for every parameter k at once: df/dtheta_k by a central difference of step
+// The per-channel ratios (df/dtheta_k) / f of an OpenPBR BSDF value, for
+// every parameter k at once: df/dtheta_k by a central difference of step
 // kParamStep of the BSDF rebuilt at theta_k +- step (one-sided at 0, and at 1
 // for roughness and metalness, which the BSDF clamps there); 0 where f is 0.
-// r[k] receives the ratios of parameter k, for k from `first` on (the others
-// stay 0).
-__device__ __noinline__ void bsdf_param_ratios(const Tables& tb, const TptMaterial& m, const Vtx& v, F3 wil,
-                                               F3 r[kParamRows], int first) {
+// ra[k] / rb[k] receive the ratios of parameter k at directions wia / wib
+// (each only when na / nb), for k from `first` on (the others stay 0).  Each
+// rebuilt BSDF serves both directions, so a vertex with a light sample and a
+// BSDF sample builds it once per parameter and side.
+__device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMaterial& m, const Vtx& v, bool na, F3 wia,
+                                                F3 ra[kParamRows], bool nb, F3 wib, F3 rb[kParamRows], int first) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
-  for (int k = 0; k < kParamRows; ++k) r[k] = zero;
+  for (int k = 0; k < kParamRows; ++k) ra[k] = rb[k] = zero;
   if (dot(v.wo, v.ng) <= 0.0f) return;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return;
   Bsdf b;
   make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
-  const F3 f = bsdf_eval(b, wol, wil);
-  if (!(f.x > 0.0f) && !(f.y > 0.0f) && !(f.z > 0.0f)) return;
+  const F3 fa0 = na ? bsdf_eval(b, wol, wia) : zero, fb0 = nb ? bsdf_eval(b, wol, wib) : zero;
+  na = na && ((fa0.x > 0.0f) || (fa0.y > 0.0f) || (fa0.z > 0.0f));
+  nb = nb && ((fb0.x > 0.0f) || (fb0.y > 0.0f) || (fb0.z > 0.0f));
+  if (!na && !nb) return;
+  auto ratio = [](F3 d, F3 f) {
+    return f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
+  };
   for (int k = first; k < kParamRows; ++k) {
     TptOpenPbr a = m.u.open_pbr, c = m.u.open_pbr;
     const float x = pbr_param(a, k);
@@ -1429,13 +1436,14 @@ __device__ __noinline__ void bsdf_param_ratios(const Tables& tb, const TptMateri
     pbr_param(a, k) = hi;
     pbr_param(c, k) = lo;
     make_bsdf_call(tb, v.material, a, v.front, wol, &b);
-    const F3 fa = bsdf_eval(b, wol, wil);
+    const F3 fah = na ? bsdf_eval(b, wol, wia) : zero, fbh = nb ? bsdf_eval(b, wol, wib) : zero;
     make_bsdf_call(tb, v.material, c, v.front, wol, &b);
-    const F3 fc = bsdf_eval(b, wol, wil);
-    const F3 d = divs(sub(fa, fc), hi - lo);
-    r[k] = f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
+    const F3 fal = na ? bsdf_eval(b, wol, wia) : zero, fbl = nb ? bsdf_eval(b, wol, wib) : zero;
+    if (na) ra[k] = ratio(divs(sub(fah, fal), hi - lo), fa0);
+    if (nb) rb[k] = ratio(divs(sub(fbh, fbl), hi - lo), fb0);
   }
 }
+
 
 // The same ratios for a smooth lobe's mirror sample (lobe as bsdf_sample
 // picked it): the sample's weight is the lobe's numerator (delta_numerator,

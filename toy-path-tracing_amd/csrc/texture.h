@@ -245,23 +245,23 @@ __device__ __forceinline__ F3 nearest(const TexView& tv, const GpuTexture& t, V2
 // like the texels (same level offsets, RGB texels 4 floats apart), following
 // the lookup's own branch decisions; global float atomics (texels are shared
 // by many lookups).
-template <int C>
+template <int C, typename T = float>
 __device__ __forceinline__ void scatter_texel(const GpuTexLevel& L, uint32_t x, uint32_t y, F3 g, float w,
-                                              float* __restrict__ gbuf) {
+                                              T* __restrict__ gbuf) {
   const size_t i = (size_t)y * L.width + x;
   if (C == 3) {
-    float* a = gbuf + L.offset + 4 * i;
-    atomicAdd(a, g.x * w);
-    atomicAdd(a + 1, g.y * w);
-    atomicAdd(a + 2, g.z * w);
+    T* a = gbuf + L.offset + 4 * i;
+    atomicAdd(a, (T)(g.x * w));
+    atomicAdd(a + 1, (T)(g.y * w));
+    atomicAdd(a + 2, (T)(g.z * w));
   } else {
-    atomicAdd(gbuf + L.offset + i, g.x * w);
+    atomicAdd(gbuf + L.offset + i, (T)(g.x * w));
   }
 }
 // bilerp_level's adjoint: the four texels with the bilinear weights
-template <int C>
+template <int C, typename T = float>
 __device__ __forceinline__ void scatter_bilerp(const TexView& tv, const GpuTexture& t, uint32_t level, V2 uv, F3 g,
-                                               float* __restrict__ gbuf) {
+                                               T* __restrict__ gbuf) {
   const GpuTexLevel& L = level_of(tv, t, level);
   const float u = wrap_unit(uv.x), v = wrap_unit(uv.y);
   const float x = u * (float)L.width - 0.5f, y = v * (float)L.height - 0.5f;
@@ -270,18 +270,18 @@ __device__ __forceinline__ void scatter_bilerp(const TexView& tv, const GpuTextu
   const bool px2 = L.pow2 & 1u, py2 = (L.pow2 >> 1) & 1u;
   const uint32_t xa = wrap_index(x0, L.width, px2), xb = wrap_index(x0 + 1, L.width, px2);
   const uint32_t ya = wrap_index(y0, L.height, py2), yb = wrap_index(y0 + 1, L.height, py2);
-  scatter_texel<C>(L, xa, ya, g, (1.0f - tx) * (1.0f - ty), gbuf);
-  scatter_texel<C>(L, xb, ya, g, tx * (1.0f - ty), gbuf);
-  scatter_texel<C>(L, xa, yb, g, (1.0f - tx) * ty, gbuf);
-  scatter_texel<C>(L, xb, yb, g, tx * ty, gbuf);
+  scatter_texel<C, T>(L, xa, ya, g, (1.0f - tx) * (1.0f - ty), gbuf);
+  scatter_texel<C, T>(L, xb, ya, g, tx * (1.0f - ty), gbuf);
+  scatter_texel<C, T>(L, xa, yb, g, (1.0f - tx) * ty, gbuf);
+  scatter_texel<C, T>(L, xb, yb, g, tx * ty, gbuf);
 }
 // ewa_level's adjoint: the same footprint, weights w / weight_sum (a first
 // pass sums the weights, a second scatters)
-template <int C>
+template <int C, typename T = float>
 __device__ __noinline__ void scatter_ewa_level(const TexView& tv, const GpuTexture& t, uint32_t level, V2 uv, V2 d0,
-                                               V2 d1, F3 g, float* __restrict__ gbuf) {
+                                               V2 d1, F3 g, T* __restrict__ gbuf) {
   if (level >= t.levels) {
-    scatter_texel<C>(tv.levels[t.first_level + t.levels - 1u], 0, 0, g, 1.0f, gbuf);
+    scatter_texel<C, T>(tv.levels[t.first_level + t.levels - 1u], 0, 0, g, 1.0f, gbuf);
     return;
   }
   const GpuTexLevel& L = tv.levels[t.first_level + level];
@@ -293,12 +293,12 @@ __device__ __noinline__ void scatter_ewa_level(const TexView& tv, const GpuTextu
   float b = -2.0f * (d0.x * d0.y + d1.x * d1.y);
   float c = d0.x * d0.x + d1.x * d1.x + 1.0f;
   const float inv_f = 1.0f / (a * c - 0.25f * b * b);
-  if (!__builtin_isfinite(inv_f)) return scatter_bilerp<C>(tv, t, level, uv, g, gbuf);
+  if (!__builtin_isfinite(inv_f)) return scatter_bilerp<C, T>(tv, t, level, uv, g, gbuf);
   a *= inv_f;
   b *= inv_f;
   c *= inv_f;
   const float det = -b * b + 4.0f * a * c;
-  if (det <= 0.0f || !__builtin_isfinite(det)) return scatter_bilerp<C>(tv, t, level, uv, g, gbuf);
+  if (det <= 0.0f || !__builtin_isfinite(det)) return scatter_bilerp<C, T>(tv, t, level, uv, g, gbuf);
   const float inv_det = 1.0f / det;
   const float u_sqrt = sqrtf(fmaxf(det * c, 0.0f)), v_sqrt = sqrtf(fmaxf(a * det, 0.0f));
   const float fs0 = ceilf(st.x - 2.0f * inv_det * u_sqrt), fs1 = floorf(st.x + 2.0f * inv_det * u_sqrt);
@@ -306,7 +306,7 @@ __device__ __noinline__ void scatter_ewa_level(const TexView& tv, const GpuTextu
   const float lim = 1073741824.0f;
   if (!(fs1 - fs0 <= (float)kMaxEwaSpan) || !(ft1 - ft0 <= (float)kMaxEwaSpan) || !(fabsf(fs0) < lim) ||
       !(fabsf(fs1) < lim) || !(fabsf(ft0) < lim) || !(fabsf(ft1) < lim))
-    return scatter_bilerp<C>(tv, t, level, uv, g, gbuf);
+    return scatter_bilerp<C, T>(tv, t, level, uv, g, gbuf);
   const int s0 = (int)fs0, s1 = (int)fs1, t0 = (int)ft0, t1 = (int)ft1;
   const float cutoff = expf(-kEwaAlpha);
   float wsum = 0.0f;
@@ -318,7 +318,7 @@ __device__ __noinline__ void scatter_ewa_level(const TexView& tv, const GpuTextu
       if (!(r2 >= 1.0f)) wsum += expf(-kEwaAlpha * r2) - cutoff;
     }
   }
-  if (!(wsum > 0.0f)) return scatter_bilerp<C>(tv, t, level, uv, g, gbuf);
+  if (!(wsum > 0.0f)) return scatter_bilerp<C, T>(tv, t, level, uv, g, gbuf);
   const float inv = 1.0f / wsum;
   const bool px2 = L.pow2 & 1u, py2 = (L.pow2 >> 1) & 1u;
   uint32_t y = wrap_index(t0, L.height, py2);
@@ -328,17 +328,17 @@ __device__ __noinline__ void scatter_ewa_level(const TexView& tv, const GpuTextu
     for (int is = s0; is <= s1; ++is) {
       const float ss = (float)is - st.x;
       const float r2 = a * ss * ss + b * ss * tt + c * tt * tt;
-      if (!(r2 >= 1.0f)) scatter_texel<C>(L, x, y, g, (expf(-kEwaAlpha * r2) - cutoff) * inv, gbuf);
+      if (!(r2 >= 1.0f)) scatter_texel<C, T>(L, x, y, g, (expf(-kEwaAlpha * r2) - cutoff) * inv, gbuf);
       x = x + 1u == L.width ? 0u : x + 1u;
     }
     y = y + 1u == L.height ? 0u : y + 1u;
   }
 }
 // sample_filtered's adjoint: levels level and level + 1 with weights 1 - t and t
-template <int C>
+template <int C, typename T = float>
 __device__ __noinline__ void scatter_filtered(const TexView& tv, const GpuTexture& t, V2 uv, V2 d0, V2 d1, F3 g,
-                                              float* __restrict__ gbuf) {
-  if (!finite2(d0) || !finite2(d1)) return scatter_bilerp<C>(tv, t, 0, uv, g, gbuf);
+                                              T* __restrict__ gbuf) {
+  if (!finite2(d0) || !finite2(d1)) return scatter_bilerp<C, T>(tv, t, 0, uv, g, gbuf);
   if (len2(d0) < len2(d1)) {
     const V2 k = d0;
     d0 = d1;
@@ -351,16 +351,17 @@ __device__ __noinline__ void scatter_filtered(const TexView& tv, const GpuTextur
     d1 = v2(d1.x * scale, d1.y * scale);
     shorter *= scale;
   }
-  if (shorter <= 0.0f) return scatter_bilerp<C>(tv, t, 0, uv, g, gbuf);
+  if (shorter <= 0.0f) return scatter_bilerp<C, T>(tv, t, 0, uv, g, gbuf);
   const float lod = fmaxf((float)(t.levels - 1u) + log2f(fmaxf(shorter, kMinFilterWidth)), 0.0f);
   const float fl = floorf(lod);
   const uint32_t level = fl >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)fl;
   const float tl = lod - fl;
   // lerp(a, b, t) = a + (b - a) t: d/da = 1 - t, d/db = t
-  scatter_ewa_level<C>(tv, t, level, uv, d0, d1, f3(g.x * (1.0f - tl), g.y * (1.0f - tl), g.z * (1.0f - tl)), gbuf);
+  scatter_ewa_level<C, T>(tv, t, level, uv, d0, d1, f3(g.x * (1.0f - tl), g.y * (1.0f - tl), g.z * (1.0f - tl)),
+                          gbuf);
   if (tl != 0.0f)
-    scatter_ewa_level<C>(tv, t, level == 0xFFFFFFFFu ? level : level + 1u, uv, d0, d1, f3(g.x * tl, g.y * tl, g.z * tl),
-                         gbuf);
+    scatter_ewa_level<C, T>(tv, t, level == 0xFFFFFFFFu ? level : level + 1u, uv, d0, d1,
+                            f3(g.x * tl, g.y * tl, g.z * tl), gbuf);
 }
 
 // Texture::sample_filtered of texture `ti` (RGB, or the scalar in .x)

@@ -81,8 +81,8 @@ __global__ __launch_bounds__(256) void texture_query_kernel(TexView tv, uint32_t
 // Adjoint of one pyramid level: each level-L gradient flows to its four
 // wrapped level-(L-1) texels with weight 0.25 (children shared by two parents
 // under wrap addressing get both).
-template <int C>
-__global__ __launch_bounds__(256) void mip_adjoint_kernel(float* __restrict__ g, GpuTexLevel src, GpuTexLevel dst) {
+template <int C, typename T>
+__global__ __launch_bounds__(256) void mip_adjoint_kernel(T* __restrict__ g, GpuTexLevel src, GpuTexLevel dst) {
   const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u), y = blockIdx.y * 16u + (threadIdx.x >> 4);
   if (x >= dst.width || y >= dst.height) return;
   const uint32_t xs[2] = {tex::wrap_index((int)(2u * x), src.width, src.pow2 & 1u),
@@ -91,8 +91,8 @@ __global__ __launch_bounds__(256) void mip_adjoint_kernel(float* __restrict__ g,
                           tex::wrap_index((int)(2u * y + 1u), src.height, (src.pow2 >> 1) & 1u)};
   const size_t o = (size_t)y * dst.width + x;
   for (int c = 0; c < C; ++c) {
-    const float v = 0.25f * g[dst.offset + (C == 3 ? 4 * o + c : o)];
-    if (v == 0.0f) continue;
+    const T v = (T)0.25 * g[dst.offset + (C == 3 ? 4 * o + c : o)];
+    if (v == (T)0) continue;
     for (int j = 0; j < 2; ++j)
       for (int i = 0; i < 2; ++i) {
         const size_t k = (size_t)ys[j] * src.width + xs[i];
@@ -101,16 +101,17 @@ __global__ __launch_bounds__(256) void mip_adjoint_kernel(float* __restrict__ g,
   }
 }
 // Level 0 of a gradient array (texel layout) packed to channels floats per texel.
-__global__ __launch_bounds__(256) void pack_level_kernel(const float* __restrict__ g, GpuTexLevel L, uint32_t channels,
+template <typename T>
+__global__ __launch_bounds__(256) void pack_level_kernel(const T* __restrict__ g, GpuTexLevel L, uint32_t channels,
                                                          float* __restrict__ out) {
   const size_t n = (size_t)L.width * L.height;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     if (channels == 3) {
-      out[3 * i] = g[L.offset + 4 * i];
-      out[3 * i + 1] = g[L.offset + 4 * i + 1];
-      out[3 * i + 2] = g[L.offset + 4 * i + 2];
+      out[3 * i] = (float)g[L.offset + 4 * i];
+      out[3 * i + 1] = (float)g[L.offset + 4 * i + 1];
+      out[3 * i + 2] = (float)g[L.offset + 4 * i + 2];
     } else {
-      out[i] = g[L.offset + i];
+      out[i] = (float)g[L.offset + i];
     }
   }
 }
@@ -286,7 +287,8 @@ size_t texel_grad_floats(const TptScene* s) {
   return n;
 }
 
-bool texel_grads_to_level0(TptScene* s, float* g, float* out, hipStream_t st) {
+template <typename T>
+bool texel_grads_to_level0(TptScene* s, T* g, float* out, hipStream_t st) {
   size_t o = 0;
   for (uint32_t i = 0; i < s->texture_count; ++i) {
     const GpuTexture& t = s->host_textures[i];
@@ -294,18 +296,20 @@ bool texel_grads_to_level0(TptScene* s, float* g, float* out, hipStream_t st) {
       const GpuTexLevel &dst = s->host_tex_levels[t.first_level + l], &src = s->host_tex_levels[t.first_level + l - 1];
       const dim3 grid((dst.width + 15u) / 16u, (dst.height + 15u) / 16u);
       if (t.channels == 3)
-        hipLaunchKernelGGL(mip_adjoint_kernel<3>, grid, dim3(256), 0, st, g, src, dst);
+        hipLaunchKernelGGL((mip_adjoint_kernel<3, T>), grid, dim3(256), 0, st, g, src, dst);
       else
-        hipLaunchKernelGGL(mip_adjoint_kernel<1>, grid, dim3(256), 0, st, g, src, dst);
+        hipLaunchKernelGGL((mip_adjoint_kernel<1, T>), grid, dim3(256), 0, st, g, src, dst);
     }
     const GpuTexLevel& L0 = s->host_tex_levels[t.first_level];
     const size_t n = (size_t)L0.width * L0.height;
-    hipLaunchKernelGGL(pack_level_kernel, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
-                       (const float*)g, L0, t.channels, out + o);
+    hipLaunchKernelGGL(pack_level_kernel<T>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
+                       (const T*)g, L0, t.channels, out + o);
     o += n * t.channels;
   }
   return hip_ok(hipGetLastError(), "texel gradient launch");
 }
+template bool texel_grads_to_level0<float>(TptScene*, float*, float*, hipStream_t);
+template bool texel_grads_to_level0<double>(TptScene*, double*, float*, hipStream_t);
 
 }  // namespace tpt
 

@@ -224,7 +224,8 @@ size_t texel_grad_floats(const TptScene* s);
 // Turns a gradient array laid out like the texels (every level) into
 // level-0 gradients (the adjoint of the pyramid build, coarsest level first;
 // modifies g) and packs them into out, texture after texture; enqueued on st.
-bool texel_grads_to_level0(TptScene* s, float* g, float* out, hipStream_t st);
+template <typename T>
+bool texel_grads_to_level0(TptScene* s, T* g, float* out, hipStream_t st);
 // Checks the materials' texture indices against the scene's texture table
 // (colour slots need RGB, scalar slots one channel); "" when valid.
 std::string material_texture_error(const TptMaterial& m, uint32_t index, const std::vector<pt::GpuTexture>& tex);

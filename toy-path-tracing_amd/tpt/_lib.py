@@ -171,6 +171,14 @@ class FrameDesc(ctypes.Structure):
 assert ctypes.sizeof(FrameDesc) == 88
 
 
+class AdjointFrameDesc(ctypes.Structure):
+    """TptAdjointFrameDesc: one view of tpt_render_adjoint_batch."""
+    _fields_ = [("render", RenderDesc), ("dloss_dimage", ctypes.c_void_p)]
+
+
+assert ctypes.sizeof(AdjointFrameDesc) == 88
+
+
 class TptError(RuntimeError):
     """A non-zero status from libtpt, carrying tpt_last_error()."""
 
@@ -219,6 +227,8 @@ _SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_render_adjoint_params", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_render_adjoint_batch", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_scene_set_materials", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_scene_set_texture", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_material_bsdf", ctypes.c_int32,

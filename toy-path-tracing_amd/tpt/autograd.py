@@ -11,7 +11,11 @@ level-0 texels of one OpenPBR base-colour texture.  ``render_params(params,
 scene, ...)`` takes one row of five OpenPBR parameters per material (base
 colour RGB, specular_roughness, base_metalness; rows of other materials are
 ignored and get zero gradient) and differentiates through
-tpt_render_adjoint_params.
+tpt_render_adjoint_params.  ``render_views(params, texels, scene, materials,
+views)`` renders many views of uneven resolution in one wavefront batch and
+differentiates all of them in one batched adjoint launch
+(tpt_render_adjoint_batch), with respect to the parameter rows and every
+texture's texels.
 
 The image, the incoming image gradient and the parameter / texel gradients
 stay in device memory: the library's calls are enqueued on torch's current
@@ -177,3 +181,52 @@ def render_params(params: torch.Tensor, device_scene, materials, camera, width: 
     """Differentiable render with respect to OpenPBR base colour, roughness
     and metalness (see the module docstring)."""
     return ParamsRender.apply(params, device_scene, materials, camera, width, height, spp, max_depth)
+
+
+class ViewsRender(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, texels, device_scene, materials, views):
+        p = params.detach().to("cpu", torch.float32).numpy()
+        _bind_stream(device_scene)
+        device_scene.set_materials(with_params(materials, p))
+        offs = _texel_offsets(device_scene)
+        tx = None
+        if texels is not None:
+            tx = texels.detach().to(_dev(device_scene), torch.float32).contiguous()
+            for t in range(len(device_scene.textures)):
+                device_scene.set_texture_device(t, tx[offs[t]:offs[t + 1]].view(device_scene.textures[t].pixels.shape))
+        outs = [torch.empty((h, w, 3), dtype=torch.float32, device=_dev(device_scene)) for _, w, h, _, _ in views]
+        device_scene.render_batch_device(list(views), outs)
+        ctx.call = (device_scene, materials, p, tx, views, texels is not None)
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *grad_images):
+        device_scene, materials, p, tx, views, has_tex = ctx.call
+        _bind_stream(device_scene)
+        device_scene.set_materials(with_params(materials, p))  # the forward's parameters
+        offs = _texel_offsets(device_scene)
+        if tx is not None:
+            for t in range(len(device_scene.textures)):
+                device_scene.set_texture_device(t, tx[offs[t]:offs[t + 1]].view(device_scene.textures[t].pixels.shape))
+        dev = _dev(device_scene)
+        gi = [(g if g is not None else torch.zeros((h, w, 3), device=dev)).detach().to(dev, torch.float32).contiguous()
+              for g, (_, w, h, _, _) in zip(grad_images, views)]
+        adj_views = [(cam, w, h, spp, depth, g) for (cam, w, h, spp, depth), g in zip(views, gi)]
+        d_p = torch.zeros((len(materials), 5), dtype=torch.float32, device=dev)
+        d_t = torch.zeros(max(1, offs[-1]), dtype=torch.float32, device=dev) if has_tex else None
+        device_scene.render_adjoint_batch_device(adj_views, d_p, d_t)
+        d_p = d_p * _row_mask(materials, OpenPbr, 5, dev)
+        return d_p, (d_t[:offs[-1]] if has_tex else None), None, None, None
+
+
+def render_views(params: torch.Tensor, texels, device_scene, materials, views):
+    """Differentiable batched render of many views in one launch: ``views``
+    are (camera, width, height, spp, max_depth) tuples; ``params`` one row of
+    five OpenPBR parameters per material (as render_params); ``texels`` the
+    level-0 texels of every texture of the scene concatenated (flat, in the
+    scene's texture order) or None.  The forward is one wavefront batch
+    (tpt_render_batch); the backward one batched path-replay adjoint over
+    every view (tpt_render_adjoint_batch).  Returns one (height, width, 3)
+    image per view."""
+    return ViewsRender.apply(params, texels, device_scene, materials, tuple(views))

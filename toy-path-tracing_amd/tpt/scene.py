@@ -557,6 +557,48 @@ class DeviceScene:
             raise ValueError("set_texture: the texture's shape must stay " + str(self.textures[texture].pixels.shape))
         check(lib().tpt_scene_set_texture(self.handle, texture, ptr(px), 0))
 
+    def render_adjoint_batch(self, views, fp64: bool = False, texels: bool = True):
+        """tpt_render_adjoint_batch over host arrays.  views: (camera, width,
+        height, spp, max_depth, dloss_dimage) tuples -> (d_params
+        (material_count, 5), [per texture level-0 gradient] or None), the sums
+        over the views."""
+        arr = (_lib.AdjointFrameDesc * len(views))()
+        keep = []
+        for i, (cam, w, h, spp, depth, dl) in enumerate(views):
+            d = arr[i].render
+            d.camera = cam
+            d.width, d.height, d.samples_per_pixel, d.max_depth = w, h, spp, depth
+            g = np.ascontiguousarray(dl, dtype=np.float32).reshape(h, w, 3)
+            keep.append(g)
+            arr[i].dloss_dimage = g.ctypes.data
+        out = np.zeros((self.material_count, 5), np.float32)
+        shapes = [t.pixels.shape for t in self.textures]
+        flat = np.zeros(max(1, sum(int(np.prod(sh)) for sh in shapes)), np.float32) if texels and shapes else None
+        check(lib().tpt_render_adjoint_batch(self.handle, arr, len(views), ptr(out), ptr(flat),
+                                             _lib.TPT_FLAG_GRAD_FP64 if fp64 else 0))
+        if flat is None:
+            return out, None
+        grads, o = [], 0
+        for sh in shapes:
+            k = int(np.prod(sh))
+            grads.append(flat[o:o + k].reshape(sh).copy())
+            o += k
+        return out, grads
+
+    def render_adjoint_batch_device(self, views, d_params_dev, d_texels_dev=None, fp64: bool = False) -> None:
+        """As render_adjoint_batch with device buffers (torch tensors): each
+        view's dloss (h*w*3), d_params (material_count*5), d_texels
+        (optional, every texture's level-0 texels packed)."""
+        arr = (_lib.AdjointFrameDesc * len(views))()
+        for i, (cam, w, h, spp, depth, dl) in enumerate(views):
+            d = arr[i].render
+            d.camera = cam
+            d.width, d.height, d.samples_per_pixel, d.max_depth = w, h, spp, depth
+            arr[i].dloss_dimage = ptr(dl)
+        flags = _lib.TPT_FLAG_DEVICE_POINTERS | (_lib.TPT_FLAG_GRAD_FP64 if fp64 else 0)
+        check(lib().tpt_render_adjoint_batch(self.handle, arr, len(views), ptr(d_params_dev), ptr(d_texels_dev),
+                                             flags))
+
     def set_texture_device(self, texture: int, pixels_dev) -> None:
         """tpt_scene_set_texture from a device buffer (a torch tensor of the
         texture's level-0 shape); synchronous."""

@@ -697,3 +697,132 @@ def city(grid: int = 256, rocks: int = 400_000, seed: int = 0, cell: float = 4.0
     ]
     cams = [(v, w, h) for v, (w, h) in zip(views, CITY_FRAMES)]
     return CityScene(s, mats, cams, 2 * quads)
+
+
+# ------------------------------------------------------------ city, inverse rendering
+def perlin_texture(size: int, seed: int, cells: int = 8, octaves: int = 4) -> np.ndarray:
+    """A seeded RGB gradient-noise (Perlin) texture, size x size, values in
+    [0.1, 0.9]: per channel, `octaves` octaves of tileable gradient noise on a
+    cells x cells lattice doubling per octave, quintic fade."""
+    rng = np.random.default_rng(seed)
+    img = np.zeros((size, size, 3))
+    y, x = np.mgrid[0:size, 0:size].astype(np.float64) / size
+    for ch in range(3):
+        acc = np.zeros((size, size))
+        amp, n = 1.0, cells
+        for _ in range(octaves):
+            ang = rng.uniform(0.0, 2.0 * np.pi, (n, n))
+            gx, gy = np.cos(ang), np.sin(ang)
+            fx, fy = x * n, y * n
+            ix, iy = np.floor(fx).astype(int), np.floor(fy).astype(int)
+            tx, ty = fx - ix, fy - iy
+
+            def dot(ox, oy):
+                i, j = (ix + ox) % n, (iy + oy) % n
+                return gx[j, i] * (tx - ox) + gy[j, i] * (ty - oy)
+
+            fade = lambda t: t * t * t * (t * (t * 6 - 15) + 10)
+            u, v = fade(tx), fade(ty)
+            n0 = dot(0, 0) * (1 - u) + dot(1, 0) * u
+            n1 = dot(0, 1) * (1 - u) + dot(1, 1) * u
+            acc += amp * (n0 * (1 - v) + n1 * v)
+            amp *= 0.5
+            n *= 2
+        img[..., ch] = acc
+    img = (img - img.min()) / max(img.max() - img.min(), 1e-9)
+    return np.ascontiguousarray((0.1 + 0.8 * img).astype(np.float32))
+
+
+@dataclass
+class CityInverseScene:
+    scene: Scene
+    materials: list
+    views: list  # (camera kwargs, width, height) per view
+    learnable: int  # the first `learnable` materials are OpenPBR with learnable parameters
+
+
+def city_inverse(grid: int = 256, rocks: int = 400_000, seed: int = 0, cell: float = 4.0, lights: int = 2048,
+                 materials: int = 64, tex_size: int = 1024, textures: int = 4, views: int = 32,
+                 view_sizes=(256, 1024), sky_size=(2048, 1024), build: bool = True, threads: int = 0) \
+        -> CityInverseScene:
+    """The batched inverse-rendering city: the city's towers (as in city())
+    split into materials - 2 seeded groups, the rocks and the ground, each an
+    OpenPBR material with seeded base colour (uniform 0.2-0.9), specular
+    roughness (0.2-0.8) and metalness (0-1), all learnable; `textures`
+    seeded Perlin-noise RGB textures of tex_size^2 FP32 texels as the base
+    colours of the first textures - 1 tower groups and of the ground; the
+    street lamps (one more material, the emitter) and the sky.  `views`
+    cameras at seeded positions over the city looking at seeded points near
+    its centre, square images whose side is log-uniform in view_sizes."""
+    from .material import OpenPbr
+    from .scene import Environment
+    from .texture import Texture
+
+    rng = np.random.default_rng(seed)
+    n = grid * grid
+    extent = 0.5 * grid * cell
+    gi, gj = np.meshgrid(np.arange(grid), np.arange(grid), indexing="ij")
+    centers = np.stack([(gi.ravel() + 0.5) * cell - extent, (gj.ravel() + 0.5) * cell - extent], 1)
+    half = rng.uniform(0.25, 0.42, n) * cell
+    chamfer = rng.uniform(0.15, 0.30, n) * half
+    heights = np.exp(rng.normal(2.0, 0.6, n))
+    groups = materials - 2
+    group = rng.integers(0, groups, n)
+    s = Scene()
+    tex = [s.add_texture(Texture(perlin_texture(tex_size, seed + 100 + t))) for t in range(textures)]
+
+    def add(pos, idx, mat, uvs=None):
+        mesh = Mesh(pos.astype(np.float32), idx.astype(np.uint32), uvs=None if uvs is None else uvs.astype(np.float32))
+        s.add_instance(s.add_mesh(mesh), mat, np.eye(4))
+
+    textured = {g: tex[g] for g in range(min(textures - 1, groups))}
+    for g in range(groups):
+        sel = np.nonzero(group == g)[0]
+        if len(sel) == 0:
+            continue
+        pos, idx = _octagon_towers(rng, centers[sel], half[sel], chamfer[sel], heights[sel])
+        uvs = np.stack([(pos[:, 0] + pos[:, 2]) * 0.05, pos[:, 1] * 0.05], 1) if g in textured else None
+        add(pos, idx, g, uvs)
+    if rocks:
+        rp, ri = _rocks(rng, rocks, extent)
+        add(rp, ri, groups)
+    e = extent * 1.05
+    gp = np.array([[-e, 0, -e], [-e, 0, e], [e, 0, e], [e, 0, -e]], np.float64)
+    add(gp, np.array([[0, 1, 2], [0, 2, 3]]), groups + 1, gp[:, [0, 2]] * 0.02)
+    corners = (grid - 1) ** 2
+    quads = min(lights // 2, corners)
+    pick = rng.choice(corners, quads, replace=False)
+    c = np.stack([(pick // (grid - 1) + 1) * cell - extent, (pick % (grid - 1) + 1) * cell - extent], 1)
+    h = rng.uniform(4.0, 6.0, quads)
+    q = 0.3
+    lp = np.stack([np.stack([c[:, 0] - q, h, c[:, 1] - q], 1), np.stack([c[:, 0] + q, h, c[:, 1] - q], 1),
+                   np.stack([c[:, 0] + q, h, c[:, 1] + q], 1), np.stack([c[:, 0] - q, h, c[:, 1] + q], 1)], 1)
+    li = np.array([[0, 1, 2], [0, 2, 3]])[None] + (np.arange(quads) * 4)[:, None, None]
+    add(lp.reshape(-1, 3), li.reshape(-1, 3), materials)
+    if build:
+        s.build_qbvh(threads)
+    mr = np.random.default_rng(seed + 1)
+    mats = []
+    for i in range(materials):
+        m = OpenPbr.new(tuple(mr.uniform(0.2, 0.9, 3).tolist())).with_(
+            specular_roughness=float(mr.uniform(0.2, 0.8)), base_metalness=float(mr.uniform(0.0, 1.0)))
+        if i in textured:
+            m = m.with_(base_color_texture=textured[i])
+        mats.append(m)
+    mats[groups + 1] = mats[groups + 1].with_(base_color_texture=tex[textures - 1])
+    mats.append(OpenPbr.new((0.0, 0.0, 0.0)).with_(base_weight=0.0, specular_weight=0.0, emission_luminance=20.0,
+                                                   emission_color=(1.0, 0.85, 0.6)))
+    for m in mats:
+        s.add_material(m)
+    s.set_environment_light(Environment(procedural_sky(*sky_size), 1.0, 0.0))
+    vr = np.random.default_rng(seed + 2)
+    cams = []
+    for _ in range(views):
+        side = int(round(np.exp(vr.uniform(np.log(view_sizes[0]), np.log(view_sizes[1])))))
+        r = vr.uniform(0.1, 0.6) * extent
+        a = vr.uniform(0.0, 2.0 * np.pi)
+        eye = (float(r * np.cos(a)), float(vr.uniform(3.0, 0.25 * extent)), float(r * np.sin(a)))
+        look = (float(vr.uniform(-0.1, 0.1) * extent), 0.0, float(vr.uniform(-0.1, 0.1) * extent))
+        cams.append((dict(eye=eye, look_at=look, up=(0.0, 1.0, 0.0), fov_y_deg=float(vr.uniform(40.0, 70.0))),
+                     side, side))
+    return CityInverseScene(s, mats, cams, materials)
