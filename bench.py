@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--city-adjoint", type=int, default=1,
                     help="also time the batched multi-view adjoint on the inverse-rendering city (1) or not (0)")
     ap.add_argument("--city-adjoint-spp", type=int, default=32, help="samples per pixel of the batched adjoint views")
+    ap.add_argument("--nee-proxy", type=int, default=1,
+                    help="also measure the OpenPBR proxy NEE policy against the reference policy (1) or not (0)")
     ap.add_argument("--refill", type=int, default=0, help="trace-kernel refill threshold (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident blocks per CU (0: occupancy)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
@@ -197,6 +199,7 @@ def main():
         cpu = cpu_baseline(sc, sets, args.cpu_sample)
 
     render = render_bench(dev, stream, args) if args.render else None
+    nee_proxy = nee_proxy_bench(dev, stream, args) if args.nee_proxy else None
     adjoint = adjoint_bench(dev, stream, args) if args.adjoint else None
     city = city_bench(dev, stream, args) if args.city else None
     city_adjoint = city_adjoint_bench(dev, stream, args) if args.city_adjoint else None
@@ -234,6 +237,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "render": render,
+        "nee_proxy": nee_proxy,
         "adjoint": adjoint,
         "city": city,
         "city_adjoint": city_adjoint,
@@ -251,6 +255,12 @@ def main():
             "q8_wavefront_vs_fp32_megakernel": city["wavefront_q8"]["speedup_vs_megakernel"],
             "fp32_wavefront_vs_fp32_megakernel": city["wavefront"]["speedup_vs_megakernel"],
         }
+    if nee_proxy:
+        summary["nee_proxy_relmse_gain_at_equal_time"] = {k: v["relmse_gain_at_equal_time"]
+                                                          for k, v in nee_proxy.items()}
+    if city and "nee_proxy" in city:
+        for k, v in city["nee_proxy"].items():
+            summary.setdefault("nee_proxy_relmse_gain_at_equal_time", {})["city_" + k] = v["relmse_gain_at_equal_time"]
     if adjoint:
         summary["grad_per_s"] = {k: v["grad_per_s"] for k, v in adjoint.items()}
     if city_adjoint:
@@ -375,6 +385,78 @@ def render_bench(dev, stream, args):
     return out
 
 
+def relmse(x, ref):
+    """Mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2)."""
+    return float(((x - ref) ** 2 / (ref * ref + 1e-2)).mean())
+
+
+def policy_comparison(rds, stream, frame, render, ref_spp):
+    """The two NEE policies on one frame: time `render(frame)` (a device
+    render of one frame) under each, then the relMSE of each image against a
+    ref_spp-sample image of the proxy policy (both policies converge to the
+    same image; the proxy's has the lower variance).  relMSE falls as
+    1 / samples, so at the reference policy's frame time the proxy's relMSE
+    is its own times its time over the reference's: the gain at equal time is
+    the reference policy's relMSE over that."""
+    import torch
+
+    cam, w, h, spp, depth = frame
+    img = torch.empty(w * h * 3, dtype=torch.float32, device="cuda")
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {}
+    imgs = {}
+    for policy in ("reference", "openpbr_proxy"):
+        rds.set_nee_policy(policy)
+        render(frame, img)
+        torch.cuda.synchronize()
+        a.record(stream)
+        render(frame, img)
+        b.record(stream)
+        torch.cuda.synchronize()
+        out[policy] = {"ms_per_frame": round(a.elapsed_time(b), 3)}
+        imgs[policy] = img.clone()
+    ref = torch.empty_like(img)
+    render((cam, w, h, ref_spp, depth), ref)  # proxy policy (set last above)
+    torch.cuda.synchronize()
+    rds.set_nee_policy("reference")
+    for policy in imgs:
+        out[policy]["relmse"] = relmse(imgs[policy], ref)
+        out[policy]["image_mean"] = round(float(imgs[policy].mean()), 5)
+    t_ref, t_prox = out["reference"]["ms_per_frame"], out["openpbr_proxy"]["ms_per_frame"]
+    e_ref, e_prox = out["reference"]["relmse"], out["openpbr_proxy"]["relmse"]
+    out["reference_image"] = f"{ref_spp} spp, proxy policy (mean {float(ref.mean()):.5f})"
+    out["relmse_gain_at_equal_spp"] = round(e_ref / max(e_prox, 1e-30), 2)
+    out["relmse_gain_at_equal_time"] = round(e_ref / max(e_prox * t_prox / t_ref, 1e-30), 2)
+    for policy in imgs:
+        out[policy]["relmse"] = round(out[policy]["relmse"], 6)
+    return out
+
+
+def nee_proxy_bench(dev, stream, args):
+    """The OpenPBR proxy NEE policy (tpt_scene_set_nee_policy) against the
+    reference policy on the Cornell configuration (512x512, 64 spp, max depth
+    8; diffuse OpenPBR walls, and the reference's OpenPBR defaults): frame
+    time of the megakernel under each policy and relMSE against a
+    2048-spp image, and the gain at equal time."""
+    from tpt import camera, scenes
+
+    w = h = 512
+    out = {}
+    for key, make in (("cornell_openpbr", scenes.cornell),
+                      ("cornell_openpbr_glossy", lambda: scenes.cornell(specular_weight=1.0))):
+        cs = make()
+        rds = cs.scene.upload(dev)
+        cam = camera.pinhole(**cs.camera, width=w, height=h)
+        res = policy_comparison(rds, stream, (cam, w, h, 64, 8),
+                                lambda f, o: rds.render_device(f[0], f[1], f[2], f[3], f[4], o), 2048)
+        res["workload"] = f"Cornell box 512x512, 64 spp, max depth 8 ({key}), megakernel"
+        log(f"nee proxy {key}: reference {res['reference']} proxy {res['openpbr_proxy']} gain at equal time "
+            f"{res['relmse_gain_at_equal_time']}x")
+        rds.close()
+        out[key] = res
+    return out
+
+
 def city_bench(dev, stream, args):
     """The wavefront-batching workload: the procedural city (about 9.4 M
     triangles, 2048 emissive triangles in the light tree, a 2048x1024 sky
@@ -461,7 +543,28 @@ def city_bench(dev, stream, args):
             f"({ms / wms:.2f}x megakernel), exact-shadow image identical: {res[tag]['identical_to_megakernel']} "
             f"({differ} of {pixels} pixels differ)")
     res["image_means"] = [[round(float(x), 4) for x in o.view(-1, 3).mean(0).tolist()] for o in mega]
+    if args.nee_proxy:
+        # the NEE policies on the 640x360 frame: the city as configured (sky
+        # and sun) and at night (the same geometry without the sky: lamps only)
+        cam3, w3, h3, _, _ = frames[2]
+
+        def wave(f, o):
+            rds_cur.render_batch_device([f], [o.view(f[2], f[1], 3)])
+
+        res["nee_proxy"] = {}
+        rds_cur = rds
+        res["nee_proxy"]["day"] = policy_comparison(rds, stream, (cam3, w3, h3, spp, depth), wave, 1024)
+        res["nee_proxy"]["day"]["workload"] = f"city frame 640x360, {spp} spp, max depth {depth}, wavefront, sky + lamps"
+        log(f"city nee proxy day: {res['nee_proxy']['day']}")
     rds.close()
+    if args.nee_proxy:
+        cs.scene.environment = None
+        rds_cur = cs.scene.upload(dev)
+        res["nee_proxy"]["night"] = policy_comparison(rds_cur, stream, (cam3, w3, h3, spp, depth), wave, 1024)
+        res["nee_proxy"]["night"]["workload"] = (f"city frame 640x360, {spp} spp, max depth {depth}, wavefront, "
+                                                 "lamps only (the sky removed)")
+        log(f"city nee proxy night: {res['nee_proxy']['night']}")
+        rds_cur.close()
     dev.set_param("wf_paths", 1 << 21)
     dev.set_param("blocks_per_cu", 4)
     dev.set_param("refill_below", 24)

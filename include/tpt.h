@@ -387,8 +387,9 @@ typedef struct TptRenderDesc {
  * (LightSampler, R/light/mod.rs); a NormalizedLambert shading point samples the SG
  * light tree (shadow ray, balance heuristic); an OpenPBR shading point gets
  * no light-tree sample (R/light_tree/importance.rs:26-36,
- * R/material/open_pbr.rs:592-606); every material gets environment
- * samples. */
+ * R/material/open_pbr.rs:592-606) unless the scene's policy is
+ * TPT_NEE_OPENPBR_PROXY (tpt_scene_set_nee_policy); every material gets
+ * environment samples. */
 int32_t tpt_render(TptScene* scene, const TptRenderDesc* desc, float* rgb, uint32_t flags);
 
 /* Gradient of an image loss with respect to every NormalizedLambert
@@ -460,6 +461,28 @@ int32_t tpt_render_adjoint_batch(TptScene* scene, const TptAdjointFrameDesc* fra
  * stay the same, as the light tree is built from them): the parameter update
  * of an inverse-rendering loop.  Synchronous. */
 int32_t tpt_scene_set_materials(TptScene* scene, const TptMaterial* materials, uint32_t count);
+
+/* ---- next-event-estimation policy ------------------------------------------ */
+/* What light-tree query an OpenPBR shading point builds
+ * (LightTreePrecompute, R/light_tree/lobe.rs:20-64):
+ *   TPT_NEE_REFERENCE      none, as the reference (R/material/open_pbr.rs:
+ *                          592-606): no light-tree NEE sample, and emitters
+ *                          found by BSDF sampling get MIS weight 1;
+ *   TPT_NEE_OPENPBR_PROXY  a diffuse lobe and one merged glossy lobe, built
+ *                          as StandardSurface's precompute builds its own
+ *                          (R/material/standard_surface.rs:599-698) from the
+ *                          OpenPBR parameters.  An extension: the estimator
+ *                          stays unbiased (the NEE pmf and the reverse pmf of
+ *                          BSDF-sampled hits use the same query); only its
+ *                          variance changes.
+ * NormalizedLambert builds its diffuse query under both. */
+#define TPT_NEE_REFERENCE 0u
+#define TPT_NEE_OPENPBR_PROXY 1u
+
+/* Selects the scene's NEE policy for every later render, batch and adjoint
+ * call (default TPT_NEE_REFERENCE).  No reference counterpart: the reference
+ * has the first policy only. */
+int32_t tpt_scene_set_nee_policy(TptScene* scene, uint32_t policy);
 
 /* One frame (view) of a batched render: its own camera, resolution, samples
  * per pixel and depth, and its output (width * height * 3 floats; host or

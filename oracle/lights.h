@@ -4,7 +4,8 @@
 //   R/math/sg.rs:26-346               SG / vMF helpers (expm1_over_x, erf, products, integrals)
 //   R/light_tree/builder.rs:58-765    binned SAOH build (triangle leaves)
 //   R/light_tree/traversal.rs:26-160  stochastic descent, leaf pmf
-//   R/light_tree/lobe.rs:233-287      sg_light_for_node, diffuse_importance
+//   R/light_tree/lobe.rs:112-163,233-436  make_glossy_lobe, sg_light_for_node, diffuse /
+//                                     glossy importance, sggx_reflection_pdf, merge_glossy_roughness
 //   R/light/mod.rs:136-233            LightSampler (tree category only)
 //   R/light/area.rs:12-54             sample_li_for_triangle
 //   R/scene/mod.rs:558-683            area-light table, pdfs, sample_triangle_point
@@ -490,17 +491,140 @@ inline float diffuse_importance(float rho, V3 n, float w, const SgLobe& lobe) {
   return rho * w * integral;
 }
 
-// A light-tree query of a diffuse shading point (LightTreePrecompute with a
-// diffuse lobe only: the materials of this backend that have one).
+// ---- glossy lobe (R/light_tree/lobe.rs:112-163, 290-341, 396-436) -----------------
+// sg_normalized_hemispherical_integral (R/math/sg.rs:155-171)
+inline float sg_normalized_hemispherical_integral(float cosine, float sharpness) {
+  const float A = 0.65173288f, B = 1.341828f, C = 7.221669f;
+  float steepness = sharpness * std::sqrt((0.5f * sharpness + A) / ((sharpness + B) * sharpness + C));
+  cosine = clampr(cosine, -1.0f, 1.0f);
+  float denom = erf_approx(steepness);
+  float numer = erf_approx(steepness * cosine);
+  float v = std::fabs(denom) > 0.0f ? 0.5f + 0.5f * (numer / denom) : 0.5f;
+  return clampr(v, 0.0f, 1.0f);
+}
+// vmf_hemispherical_integral (R/math/sg.rs:189-194)
+inline float vmf_hemispherical_integral(float cosine, float sharpness) {
+  float v = sg_normalized_hemispherical_integral(cosine, sharpness);
+  float e = std::exp(-sharpness);
+  float num = e + (1.0f - e) * v;
+  return num / (e + 1.0f);
+}
+// glam Mat2: two columns; a * b takes column j of b through a
+struct M2 {
+  float c0x, c0y, c1x, c1y;
+};
+inline M2 m2_mul(const M2& a, const M2& b) {
+  return M2{a.c0x * b.c0x + a.c1x * b.c0y, a.c0y * b.c0x + a.c1y * b.c0y, a.c0x * b.c1x + a.c1x * b.c1y,
+            a.c0y * b.c1x + a.c1y * b.c1y};
+}
+inline M2 m2_transpose(const M2& a) { return M2{a.c0x, a.c1x, a.c0y, a.c1y}; }
+// OrthonormalBasis: tangent, bitangent, normal
+struct Frame3 {
+  V3 t, b, n;
+};
+inline V3 frame_to_local(const Frame3& f, V3 w) { return v3(dot(w, f.t), dot(w, f.b), dot(w, f.n)); }
+struct GlossyLobe {  // GlossyLobePrecompute (R/light_tree/lobe.rs:36-46)
+  float rho;
+  V3 wi_ts;
+  M2 jj;
+  float det_jj4;
+  float beta_x, beta_y;
+  float alpha2_max;
+  V3 xi_p;
+  float kappa_p;
+};
+// make_glossy_lobe (R/light_tree/lobe.rs:112-163); false for None
+inline bool make_glossy_lobe(float rho, const Frame3& frame, V3 wo_world, float alpha_x, float alpha_y,
+                             GlossyLobe* g) {
+  if (rho <= 0.0f) return false;
+  V3 wi_ts = frame_to_local(frame, wo_world);
+  if (wi_ts.z <= 0.0f) return false;
+  float vlen = std::sqrt(wi_ts.x * wi_ts.x + wi_ts.y * wi_ts.y);
+  float vx = vlen > 0.0f ? wi_ts.x / vlen : 1.0f, vy = vlen > 0.0f ? wi_ts.y / vlen : 0.0f;
+  M2 r{vx, vy, -vy, vx};
+  M2 d{0.5f, 0.0f, 0.0f, 0.5f / fmaxr(wi_ts.z, 1.0e-4f)};
+  M2 jac = m2_mul(r, d);
+  g->jj = m2_mul(jac, m2_transpose(jac));
+  g->det_jj4 = 1.0f / (4.0f * wi_ts.z * wi_ts.z);
+  float a2x = clampr(alpha_x * alpha_x, 0.0f, 1.0f - 1.0e-4f);
+  float a2y = clampr(alpha_y * alpha_y, 0.0f, 1.0f - 1.0e-4f);
+  g->beta_x = a2x / (1.0f - a2x);
+  g->beta_y = a2y / (1.0f - a2y);
+  g->alpha2_max = fmaxr(fmaxr(a2x, a2y), 1.0e-6f);
+  V3 n = frame.n;
+  g->xi_p = vnormalize_or_zero(add(vneg(wo_world), vscale(n, 2.0f * dot(wo_world, n))));
+  g->kappa_p = clampr((1.0f - g->alpha2_max) / (2.0f * g->alpha2_max), 0.0f, kSgSharpnessMax);
+  g->rho = rho;
+  g->wi_ts = wi_ts;
+  return true;
+}
+// sggx_reflection_pdf (R/light_tree/lobe.rs:396-419)
+inline float sggx_reflection_pdf(V3 wi, V3 h, const M2& m) {
+  float det = fmaxr(m.c0x * m.c1y - m.c0y * m.c1x, 1.0e-30f);
+  M2 adj{m.c1y, -m.c0y, -m.c1x, m.c0x};
+  // adj * m_xy
+  float ax = adj.c0x * h.x + adj.c1x * h.y, ay = adj.c0y * h.x + adj.c1y * h.y;
+  float length2 = (h.x * ax + h.y * ay) / det + h.z * h.z;
+  float sggx = 1.0f / (kPiF * std::sqrt(det) * (length2 * length2));
+  float inv_norm = std::sqrt(fmaxr(wi.x * wi.x * m.c0x + 2.0f * wi.x * wi.y * m.c0y + wi.y * wi.y * m.c1y + wi.z * wi.z,
+                                   0.0f));
+  return sggx / (4.0f * fmaxr(inv_norm, 1.0e-30f));
+}
+// glossy_importance (R/light_tree/lobe.rs:290-341)
+inline float glossy_importance(const GlossyLobe& g, const Frame3& frame, V3 n, float w, const SgLobe& lobe) {
+  if (g.rho <= 0.0f || w <= 0.0f) return 0.0f;
+  float inv_kappa = 1.0f / fmaxr(lobe.sharpness, 1.0e-30f);
+  M2 fp{g.beta_x + g.jj.c0x * inv_kappa, 0.0f + g.jj.c0y * inv_kappa, 0.0f + g.jj.c1x * inv_kappa,
+        g.beta_y + g.jj.c1y * inv_kappa};
+  float det = g.beta_x * g.beta_y + 2.0f * inv_kappa * (g.beta_x * g.jj.c1y + g.beta_y * g.jj.c0x) +
+              inv_kappa * inv_kappa * g.det_jj4 / 4.0f;
+  float tr = fp.c0x + fp.c1y;
+  float denom = fmaxr(1.0f + tr + det, 1.0e-12f);
+  M2 a_bar{(fp.c0x + det) / denom, fp.c0y / denom, fp.c1x / denom, (fp.c1y + det) / denom};
+  V3 xi_ts = frame_to_local(frame, lobe.axis);
+  V3 hu = add(g.wi_ts, xi_ts);
+  if (dot(hu, hu) <= 1.0e-20f) return 0.0f;
+  V3 h = vnormalize(hu);
+  float lobe_value = sggx_reflection_pdf(g.wi_ts, h, a_bar);
+  V3 prod = add(vscale(g.xi_p, g.kappa_p), vscale(lobe.axis, lobe.sharpness));
+  float prod_sharp = vlen(prod);
+  if (prod_sharp <= 0.0f) return 0.0f;
+  V3 prod_dir = vdiv(prod, prod_sharp);
+  float v_vis = vmf_hemispherical_integral(clampr(dot(prod_dir, n), -1.0f, 1.0f), prod_sharp);
+  float sg_full = sg_integral(lobe.sharpness);
+  return g.rho * w * v_vis * lobe_value * sg_full;
+}
+// merge_glossy_roughness (R/light_tree/lobe.rs:421-436)
+inline void merge_glossy_roughness(float rho_a, float ax_a, float ay_a, float rho_b, float ax_b, float ay_b,
+                                   float* ax, float* ay) {
+  float total = fmaxr(rho_a + rho_b, 1.0e-30f);
+  float wa = rho_a / total, wb = rho_b / total;
+  float a2x = wa * ax_a * ax_a + wb * ax_b * ax_b;
+  float a2y = wa * ay_a * ay_a + wb * ay_b * ay_b;
+  *ax = std::sqrt(a2x);
+  *ay = std::sqrt(a2y);
+}
+
+// A light-tree query (LightTreePrecompute, R/light_tree/lobe.rs:20-28): a
+// diffuse lobe (rho > 0) and / or a glossy lobe (has_glossy) in `frame`.
 struct Query {
   V3 p, n;
   float rho;
+  bool has_glossy = false;
+  Frame3 frame{};
+  GlossyLobe glossy{};
 };
-inline float node_importance(const Query& q, const Node& nd) {  // evaluate_node_importance
+// evaluate_node_importance (R/light_tree/importance.rs:41-48) with the
+// material's light_tree_importance: the diffuse lobe alone
+// (R/material/normalized_lambert.rs), or the lobes summed and clamped at
+// zero (R/material/standard_surface.rs:700-717).
+inline float node_importance(const Query& q, const Node& nd) {
   float w;
   SgLobe lobe;
   if (!sg_light_for_node(q.p, q.n, nd, &w, &lobe)) return 0.0f;
-  return diffuse_importance(q.rho, q.n, w, lobe);
+  float imp = diffuse_importance(q.rho, q.n, w, lobe);
+  if (!q.has_glossy) return imp;
+  return fmaxr(imp + glossy_importance(q.glossy, q.frame, q.n, w, lobe), 0.0f);
 }
 
 // sample_light_tree (R/light_tree/traversal.rs:26-92)

@@ -38,6 +38,9 @@ namespace orc {
 // ---- scene shading data ------------------------------------------------------------
 // Material enum of the backend (R/material/mod.rs:100-115 variants carried).
 enum MatKind : uint32_t { kMatOpenPbr = 0, kMatLambert = 1, kMatEmissive = 2, kMatMirror = 3 };
+// next-event-estimation policy: the reference's (OpenPBR builds no light-tree
+// query) or the OpenPBR proxy query (openpbr_proxy_query)
+enum NeePolicy : uint32_t { kNeeReference = 0, kNeeOpenPbrProxy = 1 };
 struct Material {
   uint32_t kind = kMatOpenPbr;
   OpenPbr pbr{};
@@ -70,6 +73,7 @@ struct RenderScene {
     float intensity;
   };
   std::vector<DirLight> dirs;
+  uint32_t nee_policy = 0;  // kNeeReference or kNeeOpenPbrProxy
   uint32_t leaf_node(uint32_t inst, uint32_t tri) const {
     return leaf_base[inst] == lt::kInvalid ? lt::kInvalid : leaf_of[leaf_base[inst] + tri];
   }
@@ -406,9 +410,61 @@ inline float mat_pdf(const Material& m, const Vertex& v, V3 wi) {
   }
   return 0.0f;
 }
+// The OpenPBR light-tree query of the proxy policy (an extension of this
+// backend; the reference's OpenPBR builds none, R/material/open_pbr.rs:
+// 592-606): StandardSurface::light_tree_precompute
+// (R/material/standard_surface.rs:599-698) with OpenPBR's parameters.
+//   under    = lerp(1, coat_color, coat_weight / 2)
+//   diffuse  = (1 - metal) (1 - transmission_weight) base_weight lum(base_color under)
+//   metal    = metal base_weight lum(base_color under)
+//   specular = (1 - metal) specular_weight lum(specular_color under) F0(specular_ior)
+//   coat     = coat_weight F0(coat_ior)
+// metal and specular at the specular roughness, coat at the coat roughness
+// (alpha_xy), merged in that order by merge_glossy_roughness into one glossy
+// lobe; its frame is (t, normalize(n x t), n) with t the vertex frame's
+// tangent and n the normalised shading normal.
+inline float dielectric_f0(float ior) {
+  float q = (ior - 1.0f) / (ior + 1.0f);
+  return q * q;
+}
+inline bool openpbr_proxy_query(const OpenPbr& m, const Vertex& v, lt::Query* q) {
+  float metal = clampf(m.base_metalness, 0.0f, 1.0f), base = clampf(m.base_weight, 0.0f, 1.0f);
+  V3 under = vlerp(vsplat(1.0f), vclamp01(m.coat_color), clampf(m.coat_weight, 0.0f, 1.0f) * 0.5f);
+  float lb = luminance(vmul(m.base_color, under));
+  float rho_d = std::fmax((1.0f - metal) * (1.0f - clampf(m.transmission_weight, 0.0f, 1.0f)) * base * lb, 0.0f);
+  float rho_metal = metal * base * lb;
+  float rho_spec = (1.0f - metal) * std::fmax(m.specular_weight, 0.0f) * luminance(vmul(vclamp01(m.specular_color), under)) *
+                   dielectric_f0(std::fmax(m.specular_ior, 1.0e-4f));
+  float rho_coat = clampf(m.coat_weight, 0.0f, 1.0f) * dielectric_f0(std::fmax(m.coat_ior, 1.0e-4f));
+  float sx, sy, cx, cy;
+  alpha_xy(clampf(m.specular_roughness, 0.0f, 1.0f), m.specular_roughness_anisotropy, &sx, &sy);
+  alpha_xy(clampf(m.coat_roughness, 0.0f, 1.0f), m.coat_roughness_anisotropy, &cx, &cy);
+  float rho = 0.0f, ax = 1.0f, ay = 1.0f;
+  const float rl[3] = {rho_metal, rho_spec, rho_coat}, xl[3] = {sx, sx, cx}, yl[3] = {sy, sy, cy};
+  for (int i = 0; i < 3; ++i) {
+    if (!(rl[i] > 0.0f)) continue;
+    if (rho > 0.0f) {
+      lt::merge_glossy_roughness(rho, ax, ay, rl[i], xl[i], yl[i], &ax, &ay);
+    } else {
+      ax = xl[i];
+      ay = yl[i];
+    }
+    rho += rl[i];
+  }
+  *q = lt::Query{v.p, v.ns, rho_d};
+  if (rho > 0.0f) {
+    V3 fn = len_sq(v.ns) > 0.0f ? normalize3(v.ns) : v3(0.f, 0.f, 1.f);
+    lt::Frame3 f{v.frame.t, normalize_or_zero3(cross(fn, v.frame.t)), fn};
+    q->has_glossy = lt::make_glossy_lobe(rho, f, v.wo, ax, ay, &q->glossy);
+    q->frame = f;
+  }
+  return q->rho > 0.0f || q->has_glossy;
+}
 // light_tree_precompute: NormalizedLambert (normalized_lambert.rs:201-218)
-// gives a diffuse query; every other material here gives none.
-inline bool mat_query(const Material& m, const Vertex& v, lt::Query* q) {
+// gives a diffuse query; OpenPBR gives the proxy query under the proxy
+// policy; every other material here gives none.
+inline bool mat_query(const RenderScene& rs, const Material& m, const Vertex& v, lt::Query* q) {
+  if (m.kind == kMatOpenPbr && rs.nee_policy == kNeeOpenPbrProxy) return openpbr_proxy_query(m.pbr, v, q);
   if (m.kind != kMatLambert) return false;
   float rho = lt::luminance(m.color);
   if (rho <= 0.0f) return false;
@@ -537,7 +593,7 @@ inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& v
   uint32_t t_inst = lt::kInvalid, t_tri = lt::kInvalid;
   if (cat.kind == 0) {
     lt::Query q;
-    if (!mat_query(m, vtx, &q)) return zero;
+    if (!mat_query(rs, m, vtx, &q)) return zero;
     uint32_t leaf;
     float leaf_pmf;
     if (!lt::sample_tree(rs.tree, q, u_tree, &leaf, &leaf_pmf)) return zero;
@@ -607,7 +663,7 @@ inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& v
 inline float light_pdf_for_hit(const RenderScene& rs, const Material& shading_m, const Vertex& vtx, const Vertex& lv) {
   if (rs.tree.nodes.empty()) return 0.0f;
   lt::Query q;
-  if (!mat_query(shading_m, vtx, &q)) return 0.0f;
+  if (!mat_query(rs, shading_m, vtx, &q)) return 0.0f;
   uint32_t leaf = rs.leaf_node(lv.inst, lv.tri);
   float leaf_pmf = lt::pdf_leaf(rs.tree, q, leaf);
   if (leaf_pmf <= 0.0f) return 0.0f;

@@ -513,6 +513,29 @@ float orc_render_tree_pdf(void* h, const float* p, const float* n, float rho, ui
   return lt::pdf_leaf(((RenderScene*)h)->tree, q, leaf);
 }
 
+// NEE policy of the renderer (kNeeReference 0, kNeeOpenPbrProxy 1).
+void orc_render_set_nee_policy(void* h, uint32_t policy) { ((RenderScene*)h)->nee_policy = policy; }
+// The light-tree query of the shading vertex of triangle (inst, tri) at
+// barycentrics b, seen along dir, under the scene's policy: the stochastic
+// descent (returns 0 when the material builds no query or the descent gives
+// none) and the reverse pmf of a leaf node (0 without a query).
+int orc_render_vertex_tree_sample(void* h, uint32_t inst, uint32_t tri, const float* b, const float* dir, float u,
+                                  uint32_t* leaf, float* pmf) {
+  const RenderScene& rs = *(RenderScene*)h;
+  Vertex v = vertex_at(rs, inst, tri, b[0], b[1], b[2], v3(dir[0], dir[1], dir[2]));
+  lt::Query q;
+  if (!mat_query(rs, material_at(rs, rs.materials[v.material], v), v, &q)) return 0;
+  return lt::sample_tree(rs.tree, q, u, leaf, pmf) ? 1 : 0;
+}
+float orc_render_vertex_tree_pdf(void* h, uint32_t inst, uint32_t tri, const float* b, const float* dir,
+                                 uint32_t leaf) {
+  const RenderScene& rs = *(RenderScene*)h;
+  Vertex v = vertex_at(rs, inst, tri, b[0], b[1], b[2], v3(dir[0], dir[1], dir[2]));
+  lt::Query q;
+  if (!mat_query(rs, material_at(rs, rs.materials[v.material], v), v, &q)) return 0.0f;
+  return lt::pdf_leaf(rs.tree, q, leaf);
+}
+
 // direct_light_mis_contribution at the shading vertex of triangle
 // (inst, tri) at barycentrics b[3], seen along `dir` (incident direction).
 void orc_render_direct_light(void* h, uint32_t inst, uint32_t tri, const float* b, const float* dir, float u_root,
@@ -564,8 +587,33 @@ float orc_sg(int fn, float a, float b) {
     case 4: return lt::lower_cc_over_2pi(a);
     case 5: return lt::sg_cc_product_over_pi(a, b);
     case 6: return lt::vmf_axis_length_to_sharpness(a);
+    case 7: return lt::vmf_hemispherical_integral(a, b);
+    case 8: return lt::sg_normalized_hemispherical_integral(a, b);
   }
   return NAN;
+}
+// Glossy-lobe pieces of R/light_tree/lobe.rs (test access).
+// sggx_reflection_pdf(wi, h, m) with m = glam Mat2 columns (c0x, c0y, c1x, c1y).
+float orc_sggx_reflection_pdf(const float* wi, const float* h, const float* m) {
+  return lt::sggx_reflection_pdf(v3(wi[0], wi[1], wi[2]), v3(h[0], h[1], h[2]), lt::M2{m[0], m[1], m[2], m[3]});
+}
+// merge_glossy_roughness: out = (alpha_x, alpha_y)
+void orc_merge_glossy_roughness(float ra, float axa, float aya, float rb, float axb, float ayb, float* out) {
+  lt::merge_glossy_roughness(ra, axa, aya, rb, axb, ayb, &out[0], &out[1]);
+}
+// make_glossy_lobe(rho, frame = (t, b, n) in frame[9], wo, ax, ay), then
+// glossy_importance against the SG light (w, axis, sharpness) with normal n;
+// -1 when make_glossy_lobe gives None.
+float orc_glossy_importance(float rho, const float* frame, const float* wo, float ax, float ay, const float* n,
+                            float w, const float* axis, float sharpness) {
+  lt::Frame3 f{v3(frame[0], frame[1], frame[2]), v3(frame[3], frame[4], frame[5]), v3(frame[6], frame[7], frame[8])};
+  lt::GlossyLobe g;
+  if (!lt::make_glossy_lobe(rho, f, v3(wo[0], wo[1], wo[2]), ax, ay, &g)) return -1.0f;
+  lt::SgLobe l;
+  l.axis = v3(axis[0], axis[1], axis[2]);
+  l.sharpness = sharpness;
+  l.log_amplitude = 0.0f;
+  return lt::glossy_importance(g, f, v3(n[0], n[1], n[2]), w, l);
 }
 // sg_product: out = axis[3], sharpness, log_amplitude
 void orc_sg_product(const float* a1, float s1, const float* a2, float s2, float* out) {

@@ -70,6 +70,18 @@ def lib() -> ctypes.CDLL:
         L.orc_render_tree_sample.restype = ctypes.c_int
         L.orc_render_tree_pdf.argtypes = [vp, vp, vp, f32, u32]
         L.orc_render_tree_pdf.restype = f32
+        L.orc_render_set_nee_policy.argtypes = [vp, u32]
+        L.orc_render_set_nee_policy.restype = None
+        L.orc_render_vertex_tree_sample.argtypes = [vp, u32, u32, vp, vp, f32, vp, vp]
+        L.orc_render_vertex_tree_sample.restype = ctypes.c_int
+        L.orc_render_vertex_tree_pdf.argtypes = [vp, u32, u32, vp, vp, u32]
+        L.orc_render_vertex_tree_pdf.restype = f32
+        L.orc_sggx_reflection_pdf.argtypes = [vp, vp, vp]
+        L.orc_sggx_reflection_pdf.restype = f32
+        L.orc_merge_glossy_roughness.argtypes = [f32, f32, f32, f32, f32, f32, vp]
+        L.orc_merge_glossy_roughness.restype = None
+        L.orc_glossy_importance.argtypes = [f32, vp, vp, f32, f32, vp, f32, vp, f32]
+        L.orc_glossy_importance.restype = f32
         L.orc_render_direct_light.argtypes = [vp, u32, u32, vp, vp, f32, f32, f32, vp, vp]
         L.orc_render_emitted.argtypes = [vp, u32, u32, vp, vp, u32, u32, vp, vp, vp, f32, ctypes.c_int, vp]
         L.orc_render_trace.argtypes = [vp, vp, vp] + [u32] * 7 + [vp]
@@ -373,7 +385,7 @@ class OracleRenderer:
     """The oracle's MIS path tracer over a tpt.Scene and its materials
     (tpt.material records): light tree built from the emissive instances."""
 
-    def __init__(self, scene, materials):
+    def __init__(self, scene, materials, nee_policy: str = "reference"):
         from tpt.material import records
         self.geo = oracle_scene_from(scene)
         self.h = lib().orc_render_scene_new(self.geo.h, len(scene.meshes))
@@ -398,6 +410,25 @@ class OracleRenderer:
         if env is not None:
             px = np.ascontiguousarray(env.pixels, np.float32)
             lib().orc_render_set_environment(self.h, _p(px), px.shape[1], px.shape[0], env.scale, env.rotate_y)
+        self.set_nee_policy(nee_policy)
+
+    def set_nee_policy(self, policy: str) -> None:
+        """"reference" (OpenPBR builds no light-tree query) or
+        "openpbr_proxy" (the OpenPBR proxy query)."""
+        lib().orc_render_set_nee_policy(self.h, {"reference": 0, "openpbr_proxy": 1}[policy])
+
+    def vertex_tree_sample(self, inst, tri, bary, direction, u):
+        """The stochastic descent with the query of a shading vertex under the
+        renderer's policy: (leaf node, pmf) or None."""
+        leaf, pmf = ctypes.c_uint32(0), ctypes.c_float(0)
+        b, d = np.asarray(bary, np.float32), np.asarray(direction, np.float32)
+        ok = lib().orc_render_vertex_tree_sample(self.h, inst, tri, _p(b), _p(d), u, ctypes.byref(leaf),
+                                                 ctypes.byref(pmf))
+        return (leaf.value, pmf.value) if ok else None
+
+    def vertex_tree_pdf(self, inst, tri, bary, direction, leaf):
+        b, d = np.asarray(bary, np.float32), np.asarray(direction, np.float32)
+        return lib().orc_render_vertex_tree_pdf(self.h, inst, tri, _p(b), _p(d), leaf)
 
     def light_categories(self):
         """The light sampler's categories: [(kind, pmf)], kind 0 tree, 1

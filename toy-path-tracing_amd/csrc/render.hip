@@ -60,10 +60,10 @@ __device__ __forceinline__ bool closest(const SceneView& sc, F3 o, F3 d, Lane& L
 // traversal (closest hit, then the target test).
 template <bool kTwoLevel, bool kTex>
 __device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView& sv, const TptMaterial& m,
-                                           const Vtx& vtx, float u_root, float u_tree, float us0, float us1,
-                                           Lane& L, LaneStack& st, F3* dir = nullptr) {
+                                           const Vtx& vtx, const TreeQuery& tq, float u_root, float u_tree,
+                                           float us0, float us1, Lane& L, LaneStack& st, F3* dir = nullptr) {
   NeeSample ns;
-  if (!nee_sample<kTex, true>(sv, m, vtx, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
+  if (!nee_sample<kTex, true>(sv, m, vtx, tq, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
   if (dir) *dir = ns.dir;
   uint32_t hi, ht;
   float tt, hb0, hb1, hb2;
@@ -134,11 +134,12 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
       if (rho[2] != 0.0f) atomicAdd(&g[2], (Acc)(adj.w.z * rest.z / rho[2]));
     }
     F3 nee_c = f3(0.0f, 0.0f, 0.0f), nee_dir = nee_c;
+    const TreeQuery tq = tree_query(sv.nee_policy, m, vtx);
     if (m.kind != TPT_MATERIAL_EMISSIVE) {  // should_sample_direct_light: not a pure emitter
       float us[2];
       ps.s2(base + 3, us);
-      const F3 dl = direct_light<kTwoLevel, kTex>(sc, sv, m, vtx, ps.s1(base + 0), ps.s1(base + 1), us[0], us[1], L,
-                                                  st, kAdj ? &nee_dir : nullptr);
+      const F3 dl = direct_light<kTwoLevel, kTex>(sc, sv, m, vtx, tq, ps.s1(base + 0), ps.s1(base + 1), us[0], us[1],
+                                                  L, st, kAdj ? &nee_dir : nullptr);
       overflow |= st.overflow;
       nee_c = hadamard(thr, dl);
       radiance = add(radiance, nee_c);
@@ -221,7 +222,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     VtxTex ntx;
     const Vtx nv = shading_vertex_t<kTex>(sv, hi, ht, hb0, hb1, hb2, wi, &next, &ntx);
     const VtxMat<kTex> nvm(sv, nv.material, &ntx);
-    radiance = add(radiance, emitted(sv, thr, s.pdf, s.delta, m, vtx.p, vtx.ns, nv, nvm.get()));
+    radiance = add(radiance, emitted(sv, thr, s.pdf, s.delta, tq, nv, nvm.get()));
     vtx = nv;
     vm = nvm;
     if (kTex) {
@@ -992,6 +993,19 @@ extern "C" int32_t tpt_scene_set_materials(TptScene* s, const TptMaterial* mater
     set_error(std::string("tpt_scene_set_materials: ") + e.what());
     return TPT_ERROR;
   }
+}
+
+extern "C" int32_t tpt_scene_set_nee_policy(TptScene* s, uint32_t policy) {
+  if (!s) {
+    set_error("tpt_scene_set_nee_policy: null scene");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  if (policy != TPT_NEE_REFERENCE && policy != TPT_NEE_OPENPBR_PROXY) {
+    set_error("tpt_scene_set_nee_policy: unknown policy " + std::to_string(policy));
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  s->nee_policy = policy;
+  return TPT_OK;
 }
 
 extern "C" int32_t tpt_material_bsdf(TptScene* s, const TptBsdfQuery* queries, size_t n, TptBsdfResult* out,

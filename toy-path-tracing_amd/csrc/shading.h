@@ -1009,6 +1009,7 @@ struct ShadeView {
   uint32_t light_count;
   float w_tree, w_env;  // light-sampler category weights: root flux, environment power (0: absent)
   const LightCats* cats;  // the light sampler's category table
+  uint32_t nee_policy;    // TPT_NEE_REFERENCE or TPT_NEE_OPENPBR_PROXY
   EnvView env;
   Tables tab;   // GGX tables of the OpenPBR materials
   TexView tex;  // textures of the OpenPBR texture slots
@@ -1654,15 +1655,187 @@ __device__ __forceinline__ float cc_product_over_pi(float z, float s) {
   }
   return 2.0f * (lower + (upper - lower) * lerp);
 }
-// sg_light_for_node + diffuse_importance (R/light_tree/lobe.rs:233-287) of a
-// diffuse query (p, n, rho)
-__device__ __forceinline__ float node_importance(const GpuLightNode& nd, F3 p, F3 n, float rho) {
+// ---- light-tree query (LightTreePrecompute, R/light_tree/lobe.rs:20-28) -----------
+// The compact query of a shading point, from which the stochastic descent and
+// the reverse pmf evaluate node importances.  NormalizedLambert gives a
+// diffuse lobe (R/material/normalized_lambert.rs:201-218).  OpenPBR gives no
+// query under TPT_NEE_REFERENCE (R/material/open_pbr.rs:592-606) and, under
+// TPT_NEE_OPENPBR_PROXY, a diffuse lobe and one merged glossy lobe assembled
+// as StandardSurface::light_tree_precompute assembles its own
+// (R/material/standard_surface.rs:599-698), from the OpenPBR parameters.
+// The wavefront path tracer stores this record for the reverse pmf at the
+// next vertex, so it holds inputs only; lobe_query derives the rest.
+struct TreeQuery {
+  F3 p, n;             // position, shading normal
+  F3 t, wo;            // frame tangent, direction towards the viewer (glossy lobe only)
+  float rho_d, rho_g;  // diffuse / merged glossy weights (0: lobe absent)
+  float ax, ay;        // merged glossy roughness
+};
+// merge_glossy_roughness (R/light_tree/lobe.rs:421-436)
+__device__ __forceinline__ void merge_glossy(float ra, float* ax, float* ay, float rb, float bx, float by) {
+  const float total = fmaxf(ra + rb, 1.0e-30f);
+  const float wa = ra / total, wb = rb / total;
+  *ax = sqrtf(wa * *ax * *ax + wb * bx * bx);
+  *ay = sqrtf(wa * *ay * *ay + wb * by * by);
+}
+__device__ __forceinline__ float dielectric_f0(float ior) {
+  const float q = (ior - 1.0f) / (ior + 1.0f);
+  return q * q;
+}
+// The OpenPBR proxy: rho_diffuse, rho_metal, rho_spec and rho_coat of the
+// StandardSurface template with OpenPBR's parameters in place of Standard
+// Surface's (base_weight scales the metal lobe too, as in OpenPBR's metal
+// term; transmission_weight removes its share of the diffuse base; the coat
+// reflectance is its own ior's normal-incidence Fresnel); the metal and
+// specular lobes take the specular roughness, the coat lobe the coat
+// roughness (alpha_xy, OpenPBR's anisotropy mapping).  Fuzz and thin film
+// take no lobe.
+__device__ __forceinline__ void openpbr_proxy(const TptOpenPbr& m, TreeQuery* q) {
+  const F3 bc = rgb3(m.base_color);
+  const float metal = clampf(m.base_metalness, 0.0f, 1.0f), base = clampf(m.base_weight, 0.0f, 1.0f);
+  const F3 under = lerp3(f3s(1.0f), clamp01(rgb3(m.coat_color)), clampf(m.coat_weight, 0.0f, 1.0f) * 0.5f);
+  const float lb = luminance(hadamard(bc, under));
+  q->rho_d = fmaxf((1.0f - metal) * (1.0f - clampf(m.transmission_weight, 0.0f, 1.0f)) * base * lb, 0.0f);
+  const float rho_metal = metal * base * lb;
+  const float rho_spec = (1.0f - metal) * fmaxf(m.specular_weight, 0.0f) *
+                         luminance(hadamard(clamp01(rgb3(m.specular_color)), under)) *
+                         dielectric_f0(fmaxf(m.specular_ior, 1.0e-4f));
+  const float rho_coat = clampf(m.coat_weight, 0.0f, 1.0f) * dielectric_f0(fmaxf(m.coat_ior, 1.0e-4f));
+  float sx, sy, cx, cy;
+  alpha_xy(clampf(m.specular_roughness, 0.0f, 1.0f), m.specular_roughness_anisotropy, &sx, &sy);
+  alpha_xy(clampf(m.coat_roughness, 0.0f, 1.0f), m.coat_roughness_anisotropy, &cx, &cy);
+  float rho = 0.0f, ax = 1.0f, ay = 1.0f;
+  const float rl[3] = {rho_metal, rho_spec, rho_coat};
+  const float xl[3] = {sx, sx, cx}, yl[3] = {sy, sy, cy};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (!(rl[i] > 0.0f)) continue;
+    if (rho > 0.0f) {
+      merge_glossy(rho, &ax, &ay, rl[i], xl[i], yl[i]);
+    } else {
+      ax = xl[i];
+      ay = yl[i];
+    }
+    rho += rl[i];
+  }
+  q->rho_g = rho;
+  q->ax = ax;
+  q->ay = ay;
+}
+// build_query (R/light_tree/importance.rs:26-36) for a material at a vertex;
+// policy: TPT_NEE_REFERENCE or TPT_NEE_OPENPBR_PROXY.
+__device__ __forceinline__ TreeQuery tree_query(uint32_t policy, const TptMaterial& m, const Vtx& v) {
+  TreeQuery q;
+  q.p = v.p;
+  q.n = v.ns;
+  q.t = v.frame.t;
+  q.wo = v.wo;
+  q.rho_d = q.rho_g = 0.0f;
+  q.ax = q.ay = 1.0f;
+  if (m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT)
+    q.rho_d = luminance(rgb3(m.u.lambert.rho));
+  else if (m.kind == TPT_MATERIAL_OPEN_PBR && policy == TPT_NEE_OPENPBR_PROXY)
+    openpbr_proxy(m.u.open_pbr, &q);
+  return q;
+}
+
+// The lobe precomputes of a query: the diffuse weight, and make_glossy_lobe
+// (R/light_tree/lobe.rs:112-163) in the frame (t, n x t, n) rebuilt from the
+// query (rho_g 0: no glossy lobe, also when the viewer is below the frame).
+struct LobeQuery {
+  F3 p, n;
+  float rho_d, rho_g;
+  Frame f;
+  F3 wi;                         // the view direction in the frame
+  float jxx, jxy, jyy, det_jj4;  // J J^T (symmetric) and det(J J^T) * 4
+  float bx, by;                  // alpha^2 / (1 - alpha^2)
+  F3 xi;                         // mirror direction of the view about n
+  float kappa;                   // its vMF sharpness
+};
+__device__ __forceinline__ LobeQuery lobe_query(const TreeQuery& q) {
+  LobeQuery l;
+  l.p = q.p;
+  l.n = q.n;
+  l.rho_d = q.rho_d;
+  l.rho_g = 0.0f;
+  if (!(q.rho_g > 0.0f)) return l;
+  const F3 fn = len2(q.n) > 0.0f ? normalize(q.n) : f3(0.0f, 0.0f, 1.0f);
+  l.f = Frame{q.t, normalize_or_zero(cross(fn, q.t)), fn};
+  l.wi = to_frame(l.f, q.wo);
+  if (l.wi.z <= 0.0f) return l;
+  l.rho_g = q.rho_g;
+  const float vlen = sqrtf(l.wi.x * l.wi.x + l.wi.y * l.wi.y);
+  const float vx = vlen > 0.0f ? l.wi.x / vlen : 1.0f, vy = vlen > 0.0f ? l.wi.y / vlen : 0.0f;
+  // J = R(v) diag(1/2, 1/(2 cos)), J J^T = R diag(1/4, d^2) R^T
+  const float d = 0.5f / fmaxf(l.wi.z, 1.0e-4f), d2 = d * d;
+  l.jxx = 0.25f * vx * vx + d2 * vy * vy;
+  l.jxy = (0.25f - d2) * vx * vy;
+  l.jyy = 0.25f * vy * vy + d2 * vx * vx;
+  l.det_jj4 = 1.0f / (4.0f * l.wi.z * l.wi.z);
+  const float a2x = clampf(q.ax * q.ax, 0.0f, 1.0f - 1.0e-4f), a2y = clampf(q.ay * q.ay, 0.0f, 1.0f - 1.0e-4f);
+  l.bx = a2x / (1.0f - a2x);
+  l.by = a2y / (1.0f - a2y);
+  const float a2max = fmaxf(fmaxf(a2x, a2y), 1.0e-6f);
+  l.xi = normalize_or_zero(add(neg(q.wo), scl(2.0f * dot(q.wo, fn), fn)));
+  l.kappa = clampf((1.0f - a2max) / (2.0f * a2max), 0.0f, kSgMax);
+  return l;
+}
+
+// expm1_over_x and sg_integral (R/math/sg.rs:51-60,117-119)
+__device__ __forceinline__ float expm1_over_x(float x) {
+  const float u = expf(x);
+  if (u == 1.0f) return 1.0f;
+  const float y = u - 1.0f;
+  return fabsf(x) < 1.0f ? y / logf(u) : y / x;
+}
+__device__ __forceinline__ float sg_integral(float s) { return 4.0f * kPi * expm1_over_x(-2.0f * s); }
+// vmf_hemispherical_integral with sg_normalized_hemispherical_integral
+// (R/math/sg.rs:155-171,189-194)
+__device__ __forceinline__ float vmf_hemispherical(float c, float s) {
+  const float steep = s * sqrtf((0.5f * s + 0.65173288f) / ((s + 1.341828f) * s + 7.221669f));
+  c = clampf(c, -1.0f, 1.0f);
+  const float den = erf_fit(steep), num = erf_fit(steep * c);
+  const float v = clampf(fabsf(den) > 0.0f ? 0.5f + 0.5f * (num / den) : 0.5f, 0.0f, 1.0f);
+  const float e = expf(-s);
+  return (e + (1.0f - e) * v) / (e + 1.0f);
+}
+// glossy_importance (R/light_tree/lobe.rs:290-341) with sggx_reflection_pdf
+// (:396-419): NDF filtering of the merged lobe by the light's SG, the
+// reflection-lobe visibility and the SG's normalisation.
+__device__ __forceinline__ float glossy_importance(const LobeQuery& g, float w, const SgLobe& lobe) {
+  if (w <= 0.0f) return 0.0f;
+  const float ik = 1.0f / fmaxf(lobe.sharp, 1.0e-30f);
+  const float p00 = g.bx + g.jxx * ik, p01 = g.jxy * ik, p11 = g.by + g.jyy * ik;
+  const float det = g.bx * g.by + 2.0f * ik * (g.bx * g.jyy + g.by * g.jxx) + ik * ik * g.det_jj4 / 4.0f;
+  const float denom = fmaxf(1.0f + (p00 + p11) + det, 1.0e-12f);
+  const float a00 = (p00 + det) / denom, a01 = p01 / denom, a11 = (p11 + det) / denom;
+  const F3 xl = to_frame(g.f, lobe.axis);
+  const F3 hu = add(g.wi, xl);
+  if (len2(hu) <= 1.0e-20f) return 0.0f;
+  const F3 h = normalize(hu);
+  const float da = fmaxf(a00 * a11 - a01 * a01, 1.0e-30f);
+  const float l2 = (h.x * (a11 * h.x - a01 * h.y) + h.y * (a00 * h.y - a01 * h.x)) / da + h.z * h.z;
+  const float sggx = 1.0f / (kPi * sqrtf(da) * (l2 * l2));
+  const float inv_norm =
+      sqrtf(fmaxf(g.wi.x * g.wi.x * a00 + 2.0f * g.wi.x * g.wi.y * a01 + g.wi.y * g.wi.y * a11 + g.wi.z * g.wi.z, 0.0f));
+  const float lobe_value = sggx / (4.0f * fmaxf(inv_norm, 1.0e-30f));
+  const F3 prod = add(muls(g.xi, g.kappa), muls(lobe.axis, lobe.sharp));
+  const float ps = sqrtf(dot(prod, prod));
+  if (ps <= 0.0f) return 0.0f;
+  const float vis = vmf_hemispherical(clampf(dot(divs(prod, ps), g.n), -1.0f, 1.0f), ps);
+  return g.rho_g * w * vis * lobe_value * sg_integral(lobe.sharp);
+}
+// sg_light_for_node (R/light_tree/lobe.rs:233-274) and the query's lobe
+// importances: diffuse_importance (:276-287), glossy_importance, summed and
+// clamped at zero as StandardSurface::light_tree_importance does
+// (R/material/standard_surface.rs:700-717).
+__device__ __forceinline__ float node_importance(const GpuLightNode& nd, const LobeQuery& q) {
   if (nd.flux <= 0.0f) return 0.0f;
-  const F3 d = sub(f3(nd.mu[0], nd.mu[1], nd.mu[2]), p);
+  const F3 d = sub(f3(nd.mu[0], nd.mu[1], nd.mu[2]), q.p);
   const float r2 = dot(d, d);
   if (r2 <= 0.0f) return 0.0f;
   const float inv_r = 1.0f / sqrtf(r2);
-  const float cos_below = fmaxf(-(dot(n, d)) * inv_r, 0.0f);
+  const float cos_below = fmaxf(-(dot(q.n, d)) * inv_r, 0.0f);
   float sigma2 = nd.sigma_s2 * (1.0f - cos_below) + nd.bound_var * cos_below;
   const float min_sigma2 = r2 / kSgMax;
   if (sigma2 < min_sigma2) sigma2 = min_sigma2;
@@ -1672,17 +1845,22 @@ __device__ __forceinline__ float node_importance(const GpuLightNode& nd, F3 p, F
   const float w_amp = nd.flux / (2.0f * kPi * sigma2 * nd.sg_norm);
   const float w = w_amp * expf(lobe.log_amp);
   if (!__builtin_isfinite(w) || w <= 0.0f) return 0.0f;
-  if (rho <= 0.0f) return 0.0f;
-  const float c = clampf(dot(lobe.axis, n), -1.0f, 1.0f);
-  return rho * w * (cc_product_over_pi(c, lobe.sharp) * kPi);
+  float imp = 0.0f;
+  if (q.rho_d > 0.0f) {
+    const float c = clampf(dot(lobe.axis, q.n), -1.0f, 1.0f);
+    imp = q.rho_d * w * (cc_product_over_pi(c, lobe.sharp) * kPi);
+  }
+  if (q.rho_g > 0.0f) imp = fmaxf(imp + glossy_importance(q, w, lobe), 0.0f);
+  return imp;
 }
+__device__ __forceinline__ bool has_lobe(const LobeQuery& q) { return q.rho_d > 0.0f || q.rho_g > 0.0f; }
 // p(left) of an interior node, with the flux fallback when both children
 // look dark (R/light_tree/traversal.rs:54-80); false if no child has flux.
-__device__ __forceinline__ bool p_left(const ShadeView& sv, const GpuLightNode& nd, F3 p, F3 n, float rho,
-                                       float* pl, bool* by_importance) {
+__device__ __forceinline__ bool p_left(const ShadeView& sv, const GpuLightNode& nd, const LobeQuery& q, float* pl,
+                                       bool* by_importance) {
   const GpuLightNode& L = sv.lights[nd.left];
   const GpuLightNode& R = sv.lights[nd.right];
-  const float il = node_importance(L, p, n, rho), ir = node_importance(R, p, n, rho);
+  const float il = node_importance(L, q), ir = node_importance(R, q);
   const float total = il + ir;
   if (total <= 0.0f) {
     const float lf = fmaxf(L.flux, 0.0f), rf = fmaxf(R.flux, 0.0f);
@@ -1697,7 +1875,7 @@ __device__ __forceinline__ bool p_left(const ShadeView& sv, const GpuLightNode& 
   return true;
 }
 // sample_light_tree (R/light_tree/traversal.rs:26-92)
-__device__ __forceinline__ bool sample_tree(const ShadeView& sv, F3 p, F3 n, float rho, float u, uint32_t* leaf,
+__device__ __forceinline__ bool sample_tree(const ShadeView& sv, const LobeQuery& q, float u, uint32_t* leaf,
                                             float* pmf_out) {
   if (sv.light_count == 0) return false;
   u = clampf(u, 0.0f, 1.0f - kFltEpsilon);
@@ -1712,7 +1890,7 @@ __device__ __forceinline__ bool sample_tree(const ShadeView& sv, F3 p, F3 n, flo
     }
     float pl;
     bool imp;
-    if (!p_left(sv, nd, p, n, rho, &pl, &imp)) return false;
+    if (!p_left(sv, nd, q, &pl, &imp)) return false;
     if (u < pl) {
       u /= fmaxf(pl, kFltMinPos);
       pmf *= pl;
@@ -1726,7 +1904,7 @@ __device__ __forceinline__ bool sample_tree(const ShadeView& sv, F3 p, F3 n, flo
   }
 }
 // pdf_for_leaf (R/light_tree/traversal.rs:97-140)
-__device__ __forceinline__ float leaf_pmf(const ShadeView& sv, F3 p, F3 n, float rho, uint32_t leaf) {
+__device__ __forceinline__ float leaf_pmf(const ShadeView& sv, const LobeQuery& q, uint32_t leaf) {
   if (sv.light_count == 0 || leaf == TPT_INVALID_INDEX) return 0.0f;
   float pmf = 1.0f;
   uint32_t child = leaf, parent = sv.lights[leaf].parent;
@@ -1734,7 +1912,7 @@ __device__ __forceinline__ float leaf_pmf(const ShadeView& sv, F3 p, F3 n, float
     const GpuLightNode& pn = sv.lights[parent];
     float pl;
     bool imp;
-    if (!p_left(sv, pn, p, n, rho, &pl, &imp)) return 0.0f;
+    if (!p_left(sv, pn, q, &pl, &imp)) return 0.0f;
     pmf *= child == pn.left ? pl : 1.0f - pl;
     child = parent;
     parent = pn.parent;
@@ -1806,8 +1984,9 @@ struct NeeSample {
   uint32_t inst, tri;
 };
 template <bool kTex = false, bool kOutline = false>
-__device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMaterial& m, const Vtx& vtx, float u_root,
-                                           float u_tree, float us0, float us1, NeeSample* ns) {
+__device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMaterial& m, const Vtx& vtx,
+                                           const TreeQuery& tq, float u_root, float u_tree, float us0, float us1,
+                                           NeeSample* ns) {
   int kind;
   float cat_pmf;
   if (!sample_category(sv, u_root, &kind, &cat_pmf)) return false;
@@ -1815,12 +1994,11 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
   float li_pdf, selection;
   ns->inst = ns->tri = TPT_INVALID_INDEX;
   if (kind == 0) {
-    if (m.kind != TPT_MATERIAL_NORMALIZED_LAMBERT) return false;  // no light-tree query
-    const float rho = luminance(rgb3(m.u.lambert.rho));
-    if (rho <= 0.0f) return false;
+    const LobeQuery q = lobe_query(tq);
+    if (!has_lobe(q)) return false;  // no light-tree query
     uint32_t leaf;
     float lpmf;
-    if (!sample_tree(sv, vtx.p, vtx.ns, rho, u_tree, &leaf, &lpmf)) return false;
+    if (!sample_tree(sv, q, u_tree, &leaf, &lpmf)) return false;
     const GpuLightNode& ln = sv.lights[leaf];
     F3 p[3];
     triangle_world(sv, ln.instance, ln.triangle, p);
@@ -1892,22 +2070,22 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
 
 // emitted_radiance_from_bsdf_sample_area (R/integrator/mis.rs:212-234) with
 // pdf_for_triangle_hit (R/light/mod.rs:407-429) and the area-light solid-angle
-// pdf (R/scene/mod.rs:587-611).  (p, ns): the previous shading vertex's
-// position and shading normal (its light-tree query), m: its material.
-__device__ __forceinline__ F3 emitted(const ShadeView& sv, F3 thr, float bsdf_pdf, bool delta, const TptMaterial& m,
-                                      F3 p, F3 ns, const Vtx& lv, const TptMaterial& lm) {
+// pdf (R/scene/mod.rs:587-611).  tq: the previous shading vertex's
+// light-tree query (the one its NEE sampled with).
+__device__ __forceinline__ F3 emitted(const ShadeView& sv, F3 thr, float bsdf_pdf, bool delta, const TreeQuery& tq,
+                                      const Vtx& lv, const TptMaterial& lm) {
   F3 le;
   if (!mat_le(lm, lv, &le)) return f3(0.0f, 0.0f, 0.0f);
   if (delta) return hadamard(thr, le);
   float light_pdf = 0.0f;
-  if (sv.light_count != 0 && m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {
-    const float rho = luminance(rgb3(m.u.lambert.rho));
+  if (sv.light_count != 0 && (tq.rho_d > 0.0f || tq.rho_g > 0.0f)) {
+    const LobeQuery q = lobe_query(tq);
     const uint32_t base = sv.inst[lv.inst].light_base;
     const uint32_t leaf = base == TPT_INVALID_INDEX ? TPT_INVALID_INDEX : sv.tri_leaf[base + lv.tri];
-    const float lpmf = rho > 0.0f ? leaf_pmf(sv, p, ns, rho, leaf) : 0.0f;
+    const float lpmf = has_lobe(q) ? leaf_pmf(sv, q, leaf) : 0.0f;
     if (lpmf > 0.0f) {
       const float area = sv.lights[leaf].area;
-      const F3 to_light = sub(lv.p, p);
+      const F3 to_light = sub(lv.p, q.p);
       const float d2 = len2(to_light);
       if (area > 0.0f && d2 > 0.0f) {
         const float pdf_area = 1.0f / area;
@@ -1999,6 +2177,7 @@ inline ShadeView shade_view_of(const TptScene* s) {
   v.w_tree = s->w_tree;
   v.w_env = s->w_env;
   v.cats = s->light_cats;
+  v.nee_policy = s->nee_policy;
   v.env = s->env;
   v.tab = Tables{s->ggx_albedo, s->ggx_mat, s->ggx_cond, s->ggx_diel};
   v.tex = TexView{s->tex_texels, s->tex_levels, s->textures, s->texture_count};

@@ -175,6 +175,7 @@ struct TptScene {
   std::vector<TptMaterial> host_mats;     // host copy of the material table
   // light-sampler category weights and the environment light
   float w_tree = 0.0f, w_env = 0.0f;
+  uint32_t nee_policy = TPT_NEE_REFERENCE;  // tpt_scene_set_nee_policy
   tpt::pt::LightCats* light_cats = nullptr;  // device copy of host_cats
   tpt::pt::LightCats host_cats{};
   float* env_px = nullptr;

@@ -65,6 +65,13 @@ struct State {
   float* prev_ns;   // 3 per path: its shading normal
   float* prev_pdf;  // BSDF pdf of the sample that left it
   uint32_t* prev_info;  // material index | delta flag (bit 31)
+  // the rest of its light-tree query (TreeQuery): 4 per path (diffuse and
+  // glossy weights, glossy roughness); with the OpenPBR proxy policy also the
+  // frame tangent and the view direction (3 each; null otherwise), written
+  // when the glossy weight is positive
+  float* prev_q;
+  float* prev_t;
+  float* prev_wo;
   // textured scenes: the current ray's differential (12 arrays of one float
   // per path: rx origin, ry origin, rx direction, ry direction), whether it
   // has one, and its cone (2 arrays: width, spread); null otherwise
@@ -410,15 +417,29 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
           if (pt::mat_le(m, vtx, &le)) rad = pt::add(rad, le);
         } else {
           const uint32_t info = st.prev_info[path];
-          rad = pt::add(rad, pt::emitted(sv, thr, st.prev_pdf[path], (info >> 31) != 0, sv.mats[info & 0x7FFFFFFFu],
-                                         load3f(st.prev_p, path), load3f(st.prev_ns, path), vtx, m));
+          pt::TreeQuery pq;
+          const float4 q4 = reinterpret_cast<const float4*>(st.prev_q)[path];
+          pq.rho_d = q4.x;
+          pq.rho_g = q4.y;
+          pq.ax = q4.z;
+          pq.ay = q4.w;
+          pq.p = load3f(st.prev_p, path);
+          pq.n = load3f(st.prev_ns, path);
+          if (pq.rho_g > 0.0f) {
+            pq.t = load3f(st.prev_t, path);
+            pq.wo = load3f(st.prev_wo, path);
+          } else {
+            pq.t = pq.wo = f3(0.0f, 0.0f, 0.0f);
+          }
+          rad = pt::add(rad, pt::emitted(sv, thr, st.prev_pdf[path], (info >> 31) != 0, pq, vtx, m));
         }
         if (bounce < fr.max_depth) {
           const uint32_t bd = smp::kVertexBase + bounce * smp::kVertexStride;
+          const pt::TreeQuery tq = pt::tree_query(sv.nee_policy, m, vtx);
           if (m.kind != TPT_MATERIAL_EMISSIVE) {
             float us[2];
             smpl.s2(bd + 3, us, zrows[2]);
-            if (pt::nee_sample<kTex>(sv, m, vtx, smpl.s1(bd + 0, zrows[0]), smpl.s1(bd + 1, zrows[1]), us[0], us[1],
+            if (pt::nee_sample<kTex>(sv, m, vtx, tq, smpl.s1(bd + 0, zrows[0]), smpl.s1(bd + 1, zrows[1]), us[0], us[1],
                                      &ns)) {
               want_shadow = true;
               nee_value = pt::hadamard(thr, ns.value);
@@ -446,6 +467,11 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
               store3(st.thr, path, thr);
               store3(st.prev_p, path, vtx.p);
               store3(st.prev_ns, path, vtx.ns);
+              reinterpret_cast<float4*>(st.prev_q)[path] = make_float4(tq.rho_d, tq.rho_g, tq.ax, tq.ay);
+              if (tq.rho_g > 0.0f) {
+                store3(st.prev_t, path, tq.t);
+                store3(st.prev_wo, path, tq.wo);
+              }
               st.prev_pdf[path] = smp_.pdf;
               st.prev_info[path] = vtx.material | (smp_.delta ? 0x80000000u : 0u);
             }
@@ -658,7 +684,9 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     const size_t o_color = take(12 * total);
     const size_t o_out = dev_ptr ? 0 : take(12 * total);
     const size_t o_thr = take(12 * P), o_rad = take(12 * P), o_pp = take(12 * P), o_pn = take(12 * P);
-    const size_t o_pdf = take(4 * P), o_info = take(4 * P);
+    const size_t o_pdf = take(4 * P), o_info = take(4 * P), o_pq = take(16 * P);
+    const bool proxy = s->nee_policy == TPT_NEE_OPENPBR_PROXY;
+    const size_t o_pt = proxy ? take(12 * P) : 0, o_pwo = proxy ? take(12 * P) : 0;
     const size_t o_q0r = take(sizeof(TptRay) * P), o_q0p = take(4 * P);
     const size_t o_q1r = take(sizeof(TptRay) * P), o_q1p = take(4 * P);
     const size_t o_hits = take(sizeof(TptHit) * P);
@@ -707,7 +735,8 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.ps.total_pixels = (uint32_t)total;
     p.ps.paths = (uint32_t)P;
     p.st = wf::State{(float*)(base + o_thr), (float*)(base + o_rad), (float*)(base + o_pp), (float*)(base + o_pn),
-                     (float*)(base + o_pdf), (uint32_t*)(base + o_info),
+                     (float*)(base + o_pdf), (uint32_t*)(base + o_info), (float*)(base + o_pq),
+                     proxy ? (float*)(base + o_pt) : nullptr, proxy ? (float*)(base + o_pwo) : nullptr,
                      tex ? (float*)(base + o_diff) : nullptr, tex ? (uint32_t*)(base + o_dflag) : nullptr,
                      tex ? (float*)(base + o_cone) : nullptr};
     p.q[0] = wf::Queue{(TptRay*)(base + o_q0r), (uint32_t*)(base + o_q0p)};

@@ -230,6 +230,7 @@ _SIGNATURES = [
     ("tpt_render_adjoint_batch", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_scene_set_materials", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    ("tpt_scene_set_nee_policy", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_scene_set_texture", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
     ("tpt_material_bsdf", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_uint32]),

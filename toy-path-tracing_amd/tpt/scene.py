@@ -462,6 +462,16 @@ class DeviceScene:
         recs = records(materials)
         check(lib().tpt_scene_set_materials(self.handle, ctypes.addressof(recs), len(materials)))
 
+    NEE_POLICIES = {"reference": 0, "openpbr_proxy": 1}
+
+    def set_nee_policy(self, policy: str) -> None:
+        """tpt_scene_set_nee_policy: "reference" (OpenPBR shading points build
+        no light-tree query, as the reference) or "openpbr_proxy" (they build
+        a diffuse + merged-glossy query, so emissive triangles get light-tree
+        NEE from OpenPBR surfaces).  Applies to every later render, batch and
+        adjoint call on this scene."""
+        check(lib().tpt_scene_set_nee_policy(self.handle, self.NEE_POLICIES[policy]))
+
     def render_adjoint(self, camera, width: int, height: int, spp: int, max_depth: int, dloss_dimage,
                        fp64: bool = False) -> np.ndarray:
         """Path-replay adjoint (tpt_render_adjoint): dLoss/d(albedo) of every
