@@ -91,7 +91,12 @@ def test_inverse_cornell_adjoint_matches_central_differences(setup):
     img = render(ds, cs, theta, cam)
     print("inverse Cornell: adjoint at 16,384 spp", flush=True)
     d_params, d_tex = ds.render_adjoint_params(cam, W, H, SPP, DEPTH, (2.0 * (img - target)).astype(np.float32))
-    assert d_params.shape == (17, 5) and np.all(d_params[16] == 0)
+    assert d_params.shape == (17, 5)
+    # the lamp (row 16: base and specular weight 0) scatters only through its
+    # zero-weight specular lobe, whose mirror sample carries the IOR-1 Fresnel
+    # term (zero up to rounding) times the lobe weight (1 - metalness): its
+    # row is that rounding-level derivative, not a learnable signal
+    assert np.abs(d_params[16]).max() <= 1e-6 * np.abs(d_params[:16]).max(), d_params[16]
 
     # seeded materials: four base-colour channels, three metalness and three
     # roughness entries
