@@ -107,9 +107,15 @@ typedef struct TptInstanceDesc {
 
 /* OpenPbrMaterial parameters (R/material/open_pbr.rs:23-82) as the renderer
  * holds them after prepare_color (rendering-space colours): base, metal,
- * specular, coat, fuzz and emission.  Materials with transmission, thin
- * film or thin walls are rejected with TPT_ERROR_BAD_ARGUMENT (normal maps,
- * opacity and subsurface are not part of the record).  The coat and fuzz
+ * specular, transmission (the specular BTDF, rough or smooth, and thin
+ * walls), coat, fuzz and emission.  Materials with a thin film
+ * (thin_film_weight != 0) or dispersion (transmission_dispersion_scale > 0)
+ * are rejected with TPT_ERROR_BAD_ARGUMENT (normal maps, opacity, subsurface
+ * and the interior medium are not part of the record; the reference
+ * forces the last two off, validate_and_warn :303-321).  The fields after
+ * emission_color_texture carry transmission_color (default 1),
+ * transmission_dispersion_scale (0), transmission_dispersion_abbe_number
+ * (20), thin_film_thickness (0.5, micrometres) and thin_film_ior (1.4).  The coat and fuzz
  * fields follow the reference's field order after thin_walled; the
  * reference's defaults (OpenPbrMaterial::new) are coat_color 1,
  * coat_roughness 0, coat_ior 1.6, coat_darkening 1, fuzz_color 1,
@@ -151,8 +157,13 @@ typedef struct TptOpenPbr {
   uint32_t fuzz_weight_texture;
   uint32_t fuzz_roughness_texture;
   uint32_t emission_color_texture;
-  uint32_t pad;
-} TptOpenPbr; /* 160 bytes */
+  float transmission_color[3];
+  float transmission_dispersion_scale;
+  float transmission_dispersion_abbe_number;
+  float thin_film_thickness;
+  float thin_film_ior;
+  uint32_t pad[2];
+} TptOpenPbr; /* 192 bytes */
 
 /* One entry of the scene's material table: the Material enum of
  * R/material/mod.rs:100-115 for the variants the backend carries.
@@ -176,7 +187,7 @@ typedef struct TptMaterial {
     struct { float color[3]; } mirror;
     float raw[40];
   } u;
-} TptMaterial; /* 168 bytes */
+} TptMaterial; /* 200 bytes */
 
 /* A filtered, mip-mapped image (Texture<Vec3> or ScalarTexture,
  * R/material/texture.rs:53-126): width * height texels of `channels` floats
@@ -548,7 +559,7 @@ typedef struct TptBsdfResult {
   float weight[3];  /* sample(wo, u): f |cos| / pdf, or the delta lobe's weight */
   float sample_pdf; /* its pdf (the lobe probability for a delta sample) */
   float wi[3];      /* its direction */
-  uint32_t flags;   /* bit 0: a sample was produced; bit 1: it is a delta sample */
+  uint32_t flags;   /* bit 0: a sample was produced; bit 1: it is a delta sample; bit 2: it refracts */
 } TptBsdfResult; /* 48 bytes */
 /* The BSDF the path tracer builds for a material (Material::eval / pdf /
  * sample, R/material/mod.rs:251-370, at the BSDF level: OpenPbrBsdf

@@ -14,41 +14,52 @@
 
 namespace orc {
 
-constexpr float kInf = std::numeric_limits<float>::infinity();
+// The oracle's arithmetic type.  float: the reference computes in f32, and
+// this is the instantiation every parity check uses (liboracle.so).  With
+// ORC_FP64 (liboracle64.so) every value is widened to double, the same
+// algorithm in FP64, which serves as the gradients' FP64 reference; the
+// sampler's outputs stay the f32 values of the reference either way.
+#ifdef ORC_FP64
+using real = double;
+#else
+using real = float;
+#endif
+
+constexpr real kInf = std::numeric_limits<real>::infinity();
 
 // ---------------------------------------------------------------------------
 // glam::Vec3 semantics (glam 0.32, scalar Vec3): component-wise f32 ops,
 // dot = x*x' + y*y' + z*z' left to right, min/max = f32::min/max (NaN-ignoring).
 struct V3 {
-  float x, y, z;
+  real x, y, z;
 };
-inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+inline V3 v3(real x, real y, real z) { return V3{x, y, z}; }
 inline V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 inline V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-inline V3 mul(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
-inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 mul(real s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+inline real dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline V3 cross(V3 a, V3 b) {
   return v3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
 }
-inline float rmin(float a, float b) { return std::fmin(a, b); }  // Rust f32::min
-inline float rmax(float a, float b) { return std::fmax(a, b); }  // Rust f32::max
+inline real rmin(real a, real b) { return std::fmin(a, b); }  // Rust f32::min
+inline real rmax(real a, real b) { return std::fmax(a, b); }  // Rust f32::max
 inline V3 vmin(V3 a, V3 b) { return v3(rmin(a.x, b.x), rmin(a.y, b.y), rmin(a.z, b.z)); }
 inline V3 vmax(V3 a, V3 b) { return v3(rmax(a.x, b.x), rmax(a.y, b.y), rmax(a.z, b.z)); }
 inline V3 vabs(V3 a) { return v3(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
-inline float max_element(V3 a) { return rmax(a.x, rmax(a.y, a.z)); }
-inline float comp(V3 v, int axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
+inline real max_element(V3 a) { return rmax(a.x, rmax(a.y, a.z)); }
+inline real comp(V3 v, int axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
 
 // Mat4 column-major (glam layout: x_axis, y_axis, z_axis, w_axis).
 // transform_point3: res = x*px; res = y*py + res; res = z*pz + res; res = w + res.
-inline V3 transform_point3(const float* m, V3 p) {
-  float rx = m[0] * p.x, ry = m[1] * p.x, rz = m[2] * p.x;
+inline V3 transform_point3(const real* m, V3 p) {
+  real rx = m[0] * p.x, ry = m[1] * p.x, rz = m[2] * p.x;
   rx = m[4] * p.y + rx; ry = m[5] * p.y + ry; rz = m[6] * p.y + rz;
   rx = m[8] * p.z + rx; ry = m[9] * p.z + ry; rz = m[10] * p.z + rz;
   rx = m[12] + rx; ry = m[13] + ry; rz = m[14] + rz;
   return v3(rx, ry, rz);
 }
-inline V3 transform_vector3(const float* m, V3 p) {
-  float rx = m[0] * p.x, ry = m[1] * p.x, rz = m[2] * p.x;
+inline V3 transform_vector3(const real* m, V3 p) {
+  real rx = m[0] * p.x, ry = m[1] * p.x, rz = m[2] * p.x;
   rx = m[4] * p.y + rx; ry = m[5] * p.y + ry; rz = m[6] * p.y + rz;
   rx = m[8] * p.z + rx; ry = m[9] * p.z + ry; rz = m[10] * p.z + rz;
   return v3(rx, ry, rz);
@@ -56,16 +67,16 @@ inline V3 transform_vector3(const float* m, V3 p) {
 
 // ---------------------------------------------------------------------------
 // R/math/mod.rs:136-161
-constexpr float kMachineEpsilon = std::numeric_limits<float>::epsilon() * 0.5f;
+constexpr real kMachineEpsilon = std::numeric_limits<real>::epsilon() * 0.5f;
 
-inline float difference_of_products(float a, float b, float c, float d) {
-  float cd = c * d;
-  float dop = std::fma(a, b, -cd);
-  float err = std::fma(-c, d, cd);
+inline real difference_of_products(real a, real b, real c, real d) {
+  real cd = c * d;
+  real dop = std::fma(a, b, -cd);
+  real err = std::fma(-c, d, cd);
   return dop + err;
 }
-inline float gamma_n(int n) {
-  float nf = (float)n;
+inline real gamma_n(int n) {
+  real nf = (real)n;
   return (nf * kMachineEpsilon) / (1.0f - nf * kMachineEpsilon);
 }
 inline int max_component_index(V3 v) {
@@ -73,16 +84,16 @@ inline int max_component_index(V3 v) {
   return v.y > v.z ? 1 : 2;
 }
 inline V3 permute(V3 v, int x, int y, int z) {
-  float a[3] = {v.x, v.y, v.z};
+  real a[3] = {v.x, v.y, v.z};
   return v3(a[x], a[y], a[z]);
 }
 
 struct TriHit {
-  float t, b0, b1, b2;
+  real t, b0, b1, b2;
 };
 
 // R/math/ray.rs:107-209 (pbrt-v4 watertight test, fp64 fallback on e==0).
-bool intersect_triangle(V3 ro, V3 rd, float t_max, V3 v0, V3 v1, V3 v2, TriHit* out) {
+bool intersect_triangle(V3 ro, V3 rd, real t_max, V3 v0, V3 v1, V3 v2, TriHit* out) {
   {
     V3 c = cross(sub(v2, v0), sub(v1, v0));
     if (dot(c, c) == 0.0f) return false;
@@ -98,45 +109,45 @@ bool intersect_triangle(V3 ro, V3 rd, float t_max, V3 v0, V3 v1, V3 v2, TriHit* 
   p0t = permute(p0t, kx, ky, kz);
   p1t = permute(p1t, kx, ky, kz);
   p2t = permute(p2t, kx, ky, kz);
-  float sx = -d.x / d.z;
-  float sy = -d.y / d.z;
-  float sz = 1.0f / d.z;
+  real sx = -d.x / d.z;
+  real sy = -d.y / d.z;
+  real sz = 1.0f / d.z;
   p0t.x += sx * p0t.z;
   p0t.y += sy * p0t.z;
   p1t.x += sx * p1t.z;
   p1t.y += sy * p1t.z;
   p2t.x += sx * p2t.z;
   p2t.y += sy * p2t.z;
-  float e0 = difference_of_products(p1t.x, p2t.y, p1t.y, p2t.x);
-  float e1 = difference_of_products(p2t.x, p0t.y, p2t.y, p0t.x);
-  float e2 = difference_of_products(p0t.x, p1t.y, p0t.y, p1t.x);
+  real e0 = difference_of_products(p1t.x, p2t.y, p1t.y, p2t.x);
+  real e1 = difference_of_products(p2t.x, p0t.y, p2t.y, p0t.x);
+  real e2 = difference_of_products(p0t.x, p1t.y, p0t.y, p1t.x);
   if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
-    e0 = (float)((double)p2t.y * (double)p1t.x - (double)p2t.x * (double)p1t.y);
-    e1 = (float)((double)p0t.y * (double)p2t.x - (double)p0t.x * (double)p2t.y);
-    e2 = (float)((double)p1t.y * (double)p0t.x - (double)p1t.x * (double)p0t.y);
+    e0 = (real)((double)p2t.y * (double)p1t.x - (double)p2t.x * (double)p1t.y);
+    e1 = (real)((double)p0t.y * (double)p2t.x - (double)p0t.x * (double)p2t.y);
+    e2 = (real)((double)p1t.y * (double)p0t.x - (double)p1t.x * (double)p0t.y);
   }
   if ((e0 < 0.0f || e1 < 0.0f || e2 < 0.0f) && (e0 > 0.0f || e1 > 0.0f || e2 > 0.0f)) return false;
-  float det = e0 + e1 + e2;
+  real det = e0 + e1 + e2;
   if (det == 0.0f) return false;
   p0t.z *= sz;
   p1t.z *= sz;
   p2t.z *= sz;
-  float t_scaled = e0 * p0t.z + e1 * p1t.z + e2 * p2t.z;
+  real t_scaled = e0 * p0t.z + e1 * p1t.z + e2 * p2t.z;
   if ((det < 0.0f && (t_scaled >= 0.0f || t_scaled < t_max * det)) ||
       (det > 0.0f && (t_scaled <= 0.0f || t_scaled > t_max * det)))
     return false;
-  float inv_det = 1.0f / det;
-  float b0 = e0 * inv_det, b1 = e1 * inv_det, b2 = e2 * inv_det;
-  float t = t_scaled * inv_det;
-  float max_zt = max_element(vabs(v3(p0t.z, p1t.z, p2t.z)));
-  float delta_z = gamma_n(3) * max_zt;
-  float max_xt = max_element(vabs(v3(p0t.x, p1t.x, p2t.x)));
-  float max_yt = max_element(vabs(v3(p0t.y, p1t.y, p2t.y)));
-  float delta_x = gamma_n(5) * (max_xt + max_zt);
-  float delta_y = gamma_n(5) * (max_yt + max_zt);
-  float delta_e = 2.0f * (gamma_n(2) * max_xt * max_yt + delta_y * max_xt + delta_x * max_yt);
-  float max_e = max_element(vabs(v3(e0, e1, e2)));
-  float delta_t =
+  real inv_det = 1.0f / det;
+  real b0 = e0 * inv_det, b1 = e1 * inv_det, b2 = e2 * inv_det;
+  real t = t_scaled * inv_det;
+  real max_zt = max_element(vabs(v3(p0t.z, p1t.z, p2t.z)));
+  real delta_z = gamma_n(3) * max_zt;
+  real max_xt = max_element(vabs(v3(p0t.x, p1t.x, p2t.x)));
+  real max_yt = max_element(vabs(v3(p0t.y, p1t.y, p2t.y)));
+  real delta_x = gamma_n(5) * (max_xt + max_zt);
+  real delta_y = gamma_n(5) * (max_yt + max_zt);
+  real delta_e = 2.0f * (gamma_n(2) * max_xt * max_yt + delta_y * max_xt + delta_x * max_yt);
+  real max_e = max_element(vabs(v3(e0, e1, e2)));
+  real delta_t =
       3.0f * (gamma_n(3) * max_e * max_zt + delta_e * max_zt + delta_z * max_e) * std::fabs(inv_det);
   if (t <= delta_t) return false;
   out->t = t;
@@ -154,7 +165,7 @@ struct Bounds {
 const Bounds kEmptyBounds = {{kInf, kInf, kInf}, {-kInf, -kInf, -kInf}};
 inline V3 center(const Bounds& b) { return mul(0.5f, add(b.mn, b.mx)); }
 inline V3 extent(const Bounds& b) { return sub(b.mx, b.mn); }
-inline float surface_area(const Bounds& b) {
+inline real surface_area(const Bounds& b) {
   V3 e = vmax(extent(b), v3(0.f, 0.f, 0.f));
   return 2.0f * (e.x * e.y + e.x * e.z + e.y * e.z);
 }
@@ -165,8 +176,8 @@ inline Bounds unite(const Bounds& a, const Bounds& b) { return Bounds{vmin(a.mn,
 constexpr int kBinCount = 16;
 constexpr size_t kTargetLeafPrims = 4;
 constexpr size_t kMaxLeafPrims = 16;
-constexpr float kCostTraverse = 1.0f;
-constexpr float kCostIntersect = 1.0f;
+constexpr real kCostTraverse = 1.0f;
+constexpr real kCostIntersect = 1.0f;
 constexpr uint32_t kLeafBit = 1u << 31;
 constexpr uint32_t kCountShift = 24;
 constexpr uint32_t kCountMask = 0x7F;
@@ -176,7 +187,7 @@ constexpr uint32_t kEmptyChild = kLeafBit;
 inline uint32_t encode_leaf(uint32_t offset, uint32_t count) { return kLeafBit | (count << kCountShift) | offset; }
 
 struct Node {  // QbvhNode, R/qbvh.rs:54-64 (SoA over 4 children)
-  float min_x[4], min_y[4], min_z[4], max_x[4], max_y[4], max_z[4];
+  real min_x[4], min_y[4], min_z[4], max_x[4], max_y[4], max_z[4];
   uint32_t children[4];
   uint8_t valid_mask;
 };
@@ -240,25 +251,25 @@ struct Bin {
 };
 struct BinSplit {
   int axis;
-  float cmin, cmax;
+  real cmin, cmax;
   int bin_split;
-  float cost;
+  real cost;
 };
 
-size_t bin_index_for_centroid(float c, float cmin, float inv) {  // R/qbvh.rs:396-400
-  float raw = (c - cmin) * inv;
+size_t bin_index_for_centroid(real c, real cmin, real inv) {  // R/qbvh.rs:396-400
+  real raw = (c - cmin) * inv;
   // f32::clamp(0, 15): NaN stays NaN; `as usize` maps NaN to 0 and saturates.
-  float cl = raw;
+  real cl = raw;
   if (cl < 0.0f) cl = 0.0f;
-  if (cl > (float)(kBinCount - 1)) cl = (float)(kBinCount - 1);
+  if (cl > (real)(kBinCount - 1)) cl = (real)(kBinCount - 1);
   if (std::isnan(cl)) return 0;
   return (size_t)cl;
 }
 
-void bin_primitives(const BPrim* p, size_t n, int axis, float cmin, float cmax, Bin* bins) {  // :376-394
+void bin_primitives(const BPrim* p, size_t n, int axis, real cmin, real cmax, Bin* bins) {  // :376-394
   for (int i = 0; i < kBinCount; ++i) bins[i] = Bin{kEmptyBounds, 0};
-  float ext = rmax(cmax - cmin, 1.0e-30f);
-  float inv = (float)kBinCount / ext;
+  real ext = rmax(cmax - cmin, 1.0e-30f);
+  real inv = (real)kBinCount / ext;
   for (size_t i = 0; i < n; ++i) {
     size_t bi = bin_index_for_centroid(comp(p[i].centroid, axis), cmin, inv);
     bins[bi].bounds = unite(bins[bi].bounds, p[i].bounds);
@@ -266,7 +277,7 @@ void bin_primitives(const BPrim* p, size_t n, int axis, float cmin, float cmax, 
   }
 }
 
-bool evaluate_bins(const Bin* bins, float parent_area, int* split_out, float* cost_out) {  // :402-448
+bool evaluate_bins(const Bin* bins, real parent_area, int* split_out, real* cost_out) {  // :402-448
   Bounds lb[kBinCount], rb[kBinCount];
   uint32_t lc[kBinCount], rc[kBinCount];
   Bounds acc = kEmptyBounds;
@@ -288,15 +299,15 @@ bool evaluate_bins(const Bin* bins, float parent_area, int* split_out, float* co
   (void)rc;
   uint32_t total = lc[kBinCount - 1];
   if (total <= 1) return false;
-  float best_cost = kInf;
+  real best_cost = kInf;
   int best = -1;
   for (int s = 0; s < kBinCount - 1; ++s) {
     uint32_t nl = lc[s];
     uint32_t nr = total - nl;
     if (nl == 0 || nr == 0) continue;
-    float al = surface_area(lb[s]);
-    float ar = surface_area(rb[s + 1]);
-    float cost = kCostTraverse + kCostIntersect * (al * (float)nl + ar * (float)nr) / parent_area;
+    real al = surface_area(lb[s]);
+    real ar = surface_area(rb[s + 1]);
+    real cost = kCostTraverse + kCostIntersect * (al * (real)nl + ar * (real)nr) / parent_area;
     if (cost < best_cost) {
       best_cost = cost;
       best = s;
@@ -311,15 +322,15 @@ bool evaluate_bins(const Bin* bins, float parent_area, int* split_out, float* co
 bool find_best_binary_split(const BPrim* p, size_t n, const Bounds& parent, BinSplit* out) {  // :318-354
   if (n <= 1) return false;
   Bounds cb = centroid_bounds_of(p, n);
-  float parent_area = rmax(surface_area(parent), 1.0e-30f);
+  real parent_area = rmax(surface_area(parent), 1.0e-30f);
   bool have = false;
   for (int axis = 0; axis < 3; ++axis) {
-    float cmin = comp(cb.mn, axis), cmax = comp(cb.mx, axis);
+    real cmin = comp(cb.mn, axis), cmax = comp(cb.mx, axis);
     if (cmax - cmin <= 0.0f) continue;
     Bin bins[kBinCount];
     bin_primitives(p, n, axis, cmin, cmax, bins);
     int s;
-    float cost;
+    real cost;
     if (!evaluate_bins(bins, parent_area, &s, &cost)) continue;
     if (!have || cost < out->cost) {
       *out = BinSplit{axis, cmin, cmax, s, cost};
@@ -329,11 +340,11 @@ bool find_best_binary_split(const BPrim* p, size_t n, const Bounds& parent, BinS
   return have;
 }
 
-inline float leaf_cost(uint32_t n) { return kCostIntersect * (float)n; }  // :450-452
+inline real leaf_cost(uint32_t n) { return kCostIntersect * (real)n; }  // :450-452
 
 size_t partition_in_place(BPrim* p, size_t n, const BinSplit& s) {  // :454-474
-  float ext = rmax(s.cmax - s.cmin, 1.0e-30f);
-  float inv = (float)kBinCount / ext;
+  real ext = rmax(s.cmax - s.cmin, 1.0e-30f);
+  real inv = (real)kBinCount / ext;
   size_t left = 0, right = n;
   while (left < right) {
     size_t bin = bin_index_for_centroid(comp(p[left].centroid, s.axis), s.cmin, inv);
@@ -457,27 +468,27 @@ bool build_qbvh(const Bounds* pb, size_t n, Qbvh* out) {
 
 // ---------------------------------------------------------------------------
 // Traversal, R/qbvh.rs:540-675.
-inline float safe_inv(float d) { return d == 0.0f ? kInf : 1.0f / d; }  // :574-576
+inline real safe_inv(real d) { return d == 0.0f ? kInf : 1.0f / d; }  // :574-576
 // wide::f32x4::fast_min / fast_max = SSE minps/maxps: (a<b)?a:b and (a>b)?a:b.
-inline float fmin_sse(float a, float b) { return a < b ? a : b; }
-inline float fmax_sse(float a, float b) { return a > b ? a : b; }
+inline real fmin_sse(real a, real b) { return a < b ? a : b; }
+inline real fmax_sse(real a, real b) { return a > b ? a : b; }
 
 struct TraversalStats {
   uint64_t nodes = 0, tris = 0, insts = 0;
 };
 
 // intersect_4_aabbs, R/qbvh.rs:578-607
-uint32_t intersect4(const Node& nd, V3 o, V3 inv, float t_max, float tmin_out[4]) {
+uint32_t intersect4(const Node& nd, V3 o, V3 inv, real t_max, real tmin_out[4]) {
   uint32_t bits = 0;
   for (int i = 0; i < 4; ++i) {
-    float t1x = (nd.min_x[i] - o.x) * inv.x, t2x = (nd.max_x[i] - o.x) * inv.x;
-    float t1y = (nd.min_y[i] - o.y) * inv.y, t2y = (nd.max_y[i] - o.y) * inv.y;
-    float t1z = (nd.min_z[i] - o.z) * inv.z, t2z = (nd.max_z[i] - o.z) * inv.z;
-    float tminx = fmin_sse(t1x, t2x), tmaxx = fmax_sse(t1x, t2x);
-    float tminy = fmin_sse(t1y, t2y), tmaxy = fmax_sse(t1y, t2y);
-    float tminz = fmin_sse(t1z, t2z), tmaxz = fmax_sse(t1z, t2z);
-    float tmin = fmax_sse(fmax_sse(fmax_sse(tminx, tminy), tminz), 0.0f);
-    float tmax = fmin_sse(fmin_sse(fmin_sse(tmaxx, tmaxy), tmaxz), t_max);
+    real t1x = (nd.min_x[i] - o.x) * inv.x, t2x = (nd.max_x[i] - o.x) * inv.x;
+    real t1y = (nd.min_y[i] - o.y) * inv.y, t2y = (nd.max_y[i] - o.y) * inv.y;
+    real t1z = (nd.min_z[i] - o.z) * inv.z, t2z = (nd.max_z[i] - o.z) * inv.z;
+    real tminx = fmin_sse(t1x, t2x), tmaxx = fmax_sse(t1x, t2x);
+    real tminy = fmin_sse(t1y, t2y), tmaxy = fmax_sse(t1y, t2y);
+    real tminz = fmin_sse(t1z, t2z), tmaxz = fmax_sse(t1z, t2z);
+    real tmin = fmax_sse(fmax_sse(fmax_sse(tminx, tminy), tminz), 0.0f);
+    real tmax = fmin_sse(fmin_sse(fmin_sse(tmaxx, tmaxy), tmaxz), t_max);
     if (tmin <= tmax) bits |= 1u << i;  // simd_le: ordered <=
     tmin_out[i] = tmin;
   }
@@ -488,24 +499,24 @@ constexpr int kStackDepth = 128;
 
 // traverse_qbvh, R/qbvh.rs:611-675.  leaf(offset, count, t_max) -> new t_max.
 template <class Leaf>
-float traverse_qbvh(const Qbvh& q, V3 o, V3 d, float t_max0, Leaf&& leaf, TraversalStats* st) {
+real traverse_qbvh(const Qbvh& q, V3 o, V3 d, real t_max0, Leaf&& leaf, TraversalStats* st) {
   if (q.nodes.empty()) {
     if (q.prim.empty()) return t_max0;
     return leaf(0u, (uint32_t)q.prim.size(), t_max0);
   }
   V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
-  float closest = t_max0;
+  real closest = t_max0;
   uint32_t stack[kStackDepth];
   int sp = 0;
   stack[sp++] = 0;
   while (sp > 0) {
     uint32_t ni = stack[--sp];
     const Node& nd = q.nodes[ni];
-    float tmin[4];
+    real tmin[4];
     if (st) st->nodes++;
     uint32_t bits = intersect4(nd, o, inv, closest, tmin);
     if (bits == 0) continue;
-    float ht[4];
+    real ht[4];
     uint32_t hc[4];
     int cnt = 0;
     for (int s = 0; s < 4; ++s) {
@@ -540,7 +551,7 @@ float traverse_qbvh(const Qbvh& q, V3 o, V3 d, float t_max0, Leaf&& leaf, Traver
 // ---------------------------------------------------------------------------
 // Scene, R/scene/mod.rs:62-98 (geometry only).
 struct Mesh {
-  std::vector<float> pos;     // 3 per vertex
+  std::vector<real> pos;     // 3 per vertex
   std::vector<uint32_t> idx;  // 3 per triangle
   Bounds bounds;
   Qbvh qbvh;
@@ -550,7 +561,7 @@ struct Mesh {
 };
 struct Instance {
   uint32_t mesh;
-  float l2w[16], w2l[16];
+  real l2w[16], w2l[16];
   Bounds world_bounds;
 };
 struct Scene {
@@ -560,7 +571,7 @@ struct Scene {
   bool built = false;
 };
 
-Bounds transform_bounds(const Bounds& b, const float* m) {  // R/scene/mod.rs:869-893
+Bounds transform_bounds(const Bounds& b, const real* m) {  // R/scene/mod.rs:869-893
   V3 c[8] = {v3(b.mn.x, b.mn.y, b.mn.z), v3(b.mn.x, b.mn.y, b.mx.z), v3(b.mn.x, b.mx.y, b.mn.z),
              v3(b.mn.x, b.mx.y, b.mx.z), v3(b.mx.x, b.mn.y, b.mn.z), v3(b.mx.x, b.mn.y, b.mx.z),
              v3(b.mx.x, b.mx.y, b.mn.z), v3(b.mx.x, b.mx.y, b.mx.z)};
@@ -576,16 +587,16 @@ Bounds transform_bounds(const Bounds& b, const float* m) {  // R/scene/mod.rs:86
 
 struct SceneHit {
   uint32_t inst, tri;
-  float t, b0, b1, b2;
+  real t, b0, b1, b2;
 };
 
 // closest_mesh_hit_with_filter, R/scene/mod.rs:686-723 (no alpha test)
-bool closest_mesh_hit(const Mesh& m, V3 o, V3 d, float t_max, SceneHit* best, TraversalStats* st) {
+bool closest_mesh_hit(const Mesh& m, V3 o, V3 d, real t_max, SceneHit* best, TraversalStats* st) {
   bool found = false;
   traverse_qbvh(
       m.qbvh, o, d, t_max,
-      [&](uint32_t off, uint32_t count, float cur) {
-        float tm = cur;
+      [&](uint32_t off, uint32_t count, real cur) {
+        real tm = cur;
         for (uint32_t k = off; k < off + count; ++k) {
           uint32_t ti = m.qbvh.prim[k];
           V3 a = m.vert(m.idx[3 * ti]), b = m.vert(m.idx[3 * ti + 1]), c = m.vert(m.idx[3 * ti + 2]);
@@ -609,13 +620,13 @@ bool closest_mesh_hit(const Mesh& m, V3 o, V3 d, float t_max, SceneHit* best, Tr
 
 // Scene::closest_hit, R/scene/mod.rs:340-395.  t_max0 = +inf reproduces the
 // reference exactly; a finite value is this library's extension.
-bool closest_hit(const Scene& s, V3 o, V3 d, float t_max0, SceneHit* out, TraversalStats* st) {
+bool closest_hit(const Scene& s, V3 o, V3 d, real t_max0, SceneHit* out, TraversalStats* st) {
   if (s.inst.empty()) return false;
   bool found = false;
   traverse_qbvh(
       s.tlas, o, d, t_max0,
-      [&](uint32_t off, uint32_t count, float cur) {
-        float tm = cur;
+      [&](uint32_t off, uint32_t count, real cur) {
+        real tm = cur;
         for (uint32_t k = off; k < off + count; ++k) {
           uint32_t ii = s.tlas.prim[k];
           const Instance& in = s.inst[ii];
@@ -641,8 +652,8 @@ bool closest_hit(const Scene& s, V3 o, V3 d, float t_max0, SceneHit* out, Traver
 //   target given      -> unoccluded iff no hit, or the closest hit is the target
 //   distance infinite -> unoccluded iff no hit
 //   otherwise         -> unoccluded iff no hit, or t >= distance * (1 - 1e-3)
-constexpr float kShadowTolerance = 1.0e-3f;
-bool unoccluded_ray(const Scene& s, V3 o, V3 d, float distance, int64_t target_inst, int64_t target_tri,
+constexpr real kShadowTolerance = 1.0e-3f;
+bool unoccluded_ray(const Scene& s, V3 o, V3 d, real distance, int64_t target_inst, int64_t target_tri,
                     TraversalStats* st) {
   SceneHit h;
   if (!closest_hit(s, o, d, kInf, &h, st)) return true;

@@ -26,115 +26,115 @@
 namespace orc {
 namespace lt {
 
-constexpr float kPiF = 3.14159265358979323846f;
-constexpr float kFltMin = 1.17549435e-38f;        // f32::MIN_POSITIVE
-constexpr float kFltEps = 1.1920929e-7f;          // f32::EPSILON
-constexpr float kSgSharpnessMax = 2199023255552.0f;  // 2^41
+constexpr real kPiF = 3.14159265358979323846f;
+constexpr real kFltMin = 1.17549435e-38f;        // f32::MIN_POSITIVE
+constexpr real kFltEps = 1.1920929e-7f;          // f32::EPSILON
+constexpr real kSgSharpnessMax = 2199023255552.0f;  // 2^41
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
-inline float fmaxr(float a, float b) { return std::fmax(a, b); }  // f32::max (NaN-ignoring)
-inline float fminr(float a, float b) { return std::fmin(a, b); }
-inline float clampr(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }  // f32::clamp
-inline V3 vscale(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
-inline V3 vdiv(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+inline real fmaxr(real a, real b) { return std::fmax(a, b); }  // f32::max (NaN-ignoring)
+inline real fminr(real a, real b) { return std::fmin(a, b); }
+inline real clampr(real x, real lo, real hi) { return x < lo ? lo : (x > hi ? hi : x); }  // f32::clamp
+inline V3 vscale(V3 a, real s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline V3 vdiv(V3 a, real s) { return v3(a.x / s, a.y / s, a.z / s); }
 inline V3 vneg(V3 a) { return v3(-a.x, -a.y, -a.z); }
-inline float vlen(V3 a) { return std::sqrt(dot(a, a)); }
+inline real vlen(V3 a) { return std::sqrt(dot(a, a)); }
 inline V3 vnormalize(V3 a) { return vscale(a, 1.0f / vlen(a)); }  // glam normalize
 inline V3 vnormalize_or_zero(V3 a) {
-  float r = 1.0f / vlen(a);
+  real r = 1.0f / vlen(a);
   return (std::isfinite(r) && r > 0.0f) ? vscale(a, r) : v3(0.f, 0.f, 0.f);
 }
-inline float vmax_elem(V3 a) { return fmaxr(a.x, fmaxr(a.y, a.z)); }
-inline float luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }  // R/math/sg.rs:314-316
+inline real vmax_elem(V3 a) { return fmaxr(a.x, fmaxr(a.y, a.z)); }
+inline real luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }  // R/math/sg.rs:314-316
 
 // ---- SG math (R/math/sg.rs) ------------------------------------------------------------
-inline float expm1_over_x(float x) {  // :59-69
-  float u = std::exp(x);
+inline real expm1_over_x(real x) {  // :59-69
+  real u = std::exp(x);
   if (u == 1.0f) return 1.0f;
-  float y = u - 1.0f;
+  real y = u - 1.0f;
   return std::fabs(x) < 1.0f ? y / std::log(u) : y / x;
 }
-inline float erf_approx(float x) {  // :74-107
+inline real erf_approx(real x) {  // :74-107
   if (std::fabs(x) >= 4.0f) return std::copysign(1.0f, x);
   if (std::fabs(x) > 1.0f) {
-    const float a1 = 1.6284595f, a2 = 9.156747e-1f, a3 = 1.5432939e-1f, a4 = -3.5175983e-2f, a5 = 5.6679556e-3f,
+    const real a1 = 1.6284595f, a2 = 9.156747e-1f, a3 = 1.5432939e-1f, a4 = -3.5175983e-2f, a5 = 5.6679556e-3f,
                 a6 = -5.648746e-4f, a7 = 2.5890768e-5f;
-    float a = std::fabs(x);
-    float p = (((((((a7 * a + a6) * a + a5) * a + a4) * a + a3) * a + a2) * a + a1) * a);
-    float y = 1.0f - std::exp2(-p);
+    real a = std::fabs(x);
+    real p = (((((((a7 * a + a6) * a + a5) * a + a4) * a + a3) * a + a2) * a + a1) * a);
+    real y = 1.0f - std::exp2(-p);
     return std::copysign(y, x);
   }
-  const float a1 = 1.12837917f, a2 = -3.76123e-1f, a3 = 1.1279922e-1f, a4 = -2.6703065e-2f, a5 = 4.9073556e-3f,
+  const real a1 = 1.12837917f, a2 = -3.76123e-1f, a3 = 1.1279922e-1f, a4 = -2.6703065e-2f, a5 = 4.9073556e-3f,
               a6 = -5.5885315e-4f;
-  float x2 = x * x;
+  real x2 = x * x;
   return (((((a6 * x2 + a5) * x2 + a4) * x2 + a3) * x2 + a2) * x2 + a1) * x;
 }
-inline float erfc_approx(float x) { return 1.0f - erf_approx(x); }
-inline float sg_integral(float s) { return 4.0f * kPiF * expm1_over_x(-2.0f * s); }  // :124-126
+inline real erfc_approx(real x) { return 1.0f - erf_approx(x); }
+inline real sg_integral(real s) { return 4.0f * kPiF * expm1_over_x(-2.0f * s); }  // :124-126
 
 struct SgLobe {
   V3 axis;
-  float sharpness, log_amplitude;
+  real sharpness, log_amplitude;
 };
-inline SgLobe sg_product(V3 a1, float s1, V3 a2, float s2) {  // :137-156
+inline SgLobe sg_product(V3 a1, real s1, V3 a2, real s2) {  // :137-156
   V3 axis = add(vscale(a1, s1), vscale(a2, s2));
-  float sharp = vlen(axis);
+  real sharp = vlen(axis);
   V3 d = sub(a1, a2);
-  float len2 = dot(d, d);
-  float denom = fmaxr(sharp + s1 + s2, kFltMin);
-  float log_amp = -s1 * s2 * len2 / denom;
+  real len2 = dot(d, d);
+  real denom = fmaxr(sharp + s1 + s2, kFltMin);
+  real log_amp = -s1 * s2 * len2 / denom;
   V3 n = sharp > kFltMin ? vdiv(axis, sharp) : v3(0.f, 0.f, 1.f);
   return SgLobe{n, sharp, log_amp};
 }
-inline float upper_cc_over_2pi(float s) {  // :236-252
+inline real upper_cc_over_2pi(real s) {  // :236-252
   if (s <= 0.5f) {
     return (((((((-1.0f / 362880.0f) * s + 1.0f / 40320.0f) * s - 1.0f / 5040.0f) * s + 1.0f / 720.0f) * s -
               1.0f / 120.0f) * s + 1.0f / 24.0f) * s - 1.0f / 6.0f) * s + 0.5f;
   }
   return (std::expm1(-s) + s) / (s * s);
 }
-inline float lower_cc_over_2pi(float s) {  // :256-275
-  float e = std::exp(-s);
+inline real lower_cc_over_2pi(real s) {  // :256-275
+  real e = std::exp(-s);
   if (s <= 0.5f) {
     return e * (((((((((1.0f / 403200.0f) * s - 1.0f / 45360.0f) * s + 1.0f / 5760.0f) * s - 1.0f / 840.0f) * s +
                     1.0f / 144.0f) * s - 1.0f / 30.0f) * s + 1.0f / 8.0f) * s - 1.0f / 3.0f) * s + 0.5f);
   }
   return e * (-std::expm1(-s) - s * e) / (s * s);
 }
-inline float sg_cc_product_over_pi(float z, float s) {  // :282-306
-  const float A = 2.736083f, B = 17.021297f, C = 4.0100827f, D = 15.219156f, E = 76.0879f;
-  float t = s * std::sqrt(0.5f * ((s + A) * s + B) / (((s + C) * s + D) * s + E));
-  float tz = t * z;
-  const float inv_sqrt_pi = 0.5641896f;
-  const float clamp_min = 0.5f * kFltEps;
-  float lerp = clampr(0.5f * (z * erfc_approx(-tz) + erfc_approx(t)) -
+inline real sg_cc_product_over_pi(real z, real s) {  // :282-306
+  const real A = 2.736083f, B = 17.021297f, C = 4.0100827f, D = 15.219156f, E = 76.0879f;
+  real t = s * std::sqrt(0.5f * ((s + A) * s + B) / (((s + C) * s + D) * s + E));
+  real tz = t * z;
+  const real inv_sqrt_pi = 0.5641896f;
+  const real clamp_min = 0.5f * kFltEps;
+  real lerp = clampr(0.5f * (z * erfc_approx(-tz) + erfc_approx(t)) -
                           0.5f * inv_sqrt_pi * std::exp(-tz * tz) * std::expm1(t * t * (z * z - 1.0f)) / fmaxr(t, kFltMin),
                       clamp_min, 1.0f);
-  float lower = lower_cc_over_2pi(s);
-  float upper = upper_cc_over_2pi(s);
+  real lower = lower_cc_over_2pi(s);
+  real upper = upper_cc_over_2pi(s);
   return 2.0f * (lower + (upper - lower) * lerp);
 }
-inline float vmf_axis_length_to_sharpness(float len) {  // :196-208
-  float r = clampr(len, 0.0f, 1.0f);
+inline real vmf_axis_length_to_sharpness(real len) {  // :196-208
+  real r = clampr(len, 0.0f, 1.0f);
   if (r >= 1.0f) return kSgSharpnessMax;
-  float om = 1.0f - r * r;
+  real om = 1.0f - r * r;
   if (om <= 0.0f) return kSgSharpnessMax;
-  float lambda = r * (3.0f - r * r) / om;
+  real lambda = r * (3.0f - r * r) / om;
   return clampr(lambda, 0.0f, kSgSharpnessMax);
 }
 struct Merged {
   V3 mu;
-  float sigma_s2;
+  real sigma_s2;
   V3 nu_bar;
-  float flux;
+  real flux;
 };
-inline Merged merge_cluster_params(float fl, V3 mul_, float sl, V3 nul, float fr, V3 mur, float sr, V3 nur) {  // :318-346
-  float total = fl + fr;
+inline Merged merge_cluster_params(real fl, V3 mul_, real sl, V3 nul, real fr, V3 mur, real sr, V3 nur) {  // :318-346
+  real total = fl + fr;
   if (total <= 0.0f) return Merged{v3(0.f, 0.f, 0.f), 0.0f, v3(0.f, 0.f, 0.f), 0.0f};
-  float wl = fl / total, wr = fr / total;
+  real wl = fl / total, wr = fr / total;
   V3 mu = add(vscale(mul_, wl), vscale(mur, wr));
   V3 md = sub(mul_, mur);
-  float s2 = wl * sl + wr * sr + wl * wr * dot(md, md);
+  real s2 = wl * sl + wr * sr + wl * wr * dot(md, md);
   V3 nu = add(vscale(nul, wl), vscale(nur, wr));
   return Merged{mu, s2, nu, total};
 }
@@ -142,7 +142,7 @@ inline Merged merge_cluster_params(float fl, V3 mul_, float sl, V3 nul, float fr
 // ---- tree ----------------------------------------------------------------------------
 struct Box {
   V3 mn{INFINITY, INFINITY, INFINITY}, mx{-INFINITY, -INFINITY, -INFINITY};
-  float area() const {  // Bounds::surface_area (R/scene/mesh.rs:31-34)
+  real area() const {  // Bounds::surface_area (R/scene/mesh.rs:31-34)
     V3 e = vmax(sub(mx, mn), v3(0.f, 0.f, 0.f));
     return 2.0f * (e.x * e.y + e.x * e.z + e.y * e.z);
   }
@@ -150,64 +150,64 @@ struct Box {
 };
 
 struct Node {
-  float flux;
+  real flux;
   V3 mu;
-  float sigma_s2;
+  real sigma_s2;
   V3 nu;
-  float lambda;
+  real lambda;
   Box aabb;
-  float radius;
+  real radius;
   V3 cone_axis;
-  float cone_theta_o, cone_theta_e;
+  real cone_theta_o, cone_theta_e;
   bool leaf;
   uint32_t inst, tri;  // triangle leaf
-  float area;          // triangle area (leaf)
+  real area;          // triangle area (leaf)
   uint32_t left, right, parent;
 };
 
 struct Leaf {
   uint32_t inst, tri;
-  float flux;
+  real flux;
   V3 mu;
-  float sigma_s2;
+  real sigma_s2;
   V3 nu_bar;
   Box aabb;
-  float radius;
+  real radius;
   V3 cone_axis;
-  float theta_o, theta_e;
+  real theta_o, theta_e;
   V3 centroid;
-  float area;
+  real area;
 };
 
 struct Cone {  // OrientationCone (R/light_tree/builder.rs:240-338)
   V3 axis{0.f, 0.f, 1.f};
-  float theta_o = 0.0f, theta_e = 0.0f;
+  real theta_o = 0.0f, theta_e = 0.0f;
   bool valid = false;
   static Cone of(const Leaf& l) { return Cone{l.cone_axis, l.theta_o, l.theta_e, true}; }
-  float measure() const {
+  real measure() const {
     if (!valid) return 0.0f;
-    float to = clampr(theta_o, 0.0f, kPiF), te = clampr(theta_e, 0.0f, kPiF * 0.5f);
-    float tw = fminr(to + te, kPiF);
-    float co = std::cos(to), so = std::sin(to);
+    real to = clampr(theta_o, 0.0f, kPiF), te = clampr(theta_e, 0.0f, kPiF * 0.5f);
+    real tw = fminr(to + te, kPiF);
+    real co = std::cos(to), so = std::sin(to);
     return 2.0f * kPiF * (1.0f - co) +
            0.5f * kPiF * (2.0f * tw * so - std::cos(to - 2.0f * tw) - 2.0f * to * so + co);
   }
   static Cone merge(Cone a, Cone b) {
     if (!a.valid) return b;
     if (!b.valid) return a;
-    float te = fmaxr(a.theta_e, b.theta_e);
+    real te = fmaxr(a.theta_e, b.theta_e);
     if (!(a.theta_o >= b.theta_o)) std::swap(a, b);
-    float cd = clampr(dot(a.axis, b.axis), -1.0f, 1.0f);
-    float td = std::acos(cd);
+    real cd = clampr(dot(a.axis, b.axis), -1.0f, 1.0f);
+    real td = std::acos(cd);
     if (fminr(td + b.theta_o, kPiF) <= a.theta_o) return Cone{a.axis, a.theta_o, te, true};
-    float to = fminr((a.theta_o + td + b.theta_o) * 0.5f, kPiF);
+    real to = fminr((a.theta_o + td + b.theta_o) * 0.5f, kPiF);
     if (to >= kPiF - 1e-5f) return Cone{a.axis, kPiF, te, true};
     V3 ra = cross(a.axis, b.axis);
     V3 na;
     if (dot(ra, ra) > 1.0e-12f) {
-      float tr = to - a.theta_o;
+      real tr = to - a.theta_o;
       V3 k = vnormalize(ra);
-      float cr = std::cos(tr), sr = std::sin(tr);
+      real cr = std::cos(tr), sr = std::sin(tr);
       na = vnormalize_or_zero(add(add(vscale(a.axis, cr), vscale(cross(k, a.axis), sr)),
                                   vscale(vscale(k, dot(k, a.axis)), 1.0f - cr)));
     } else {
@@ -218,15 +218,15 @@ struct Cone {  // OrientationCone (R/light_tree/builder.rs:240-338)
 };
 
 struct Cluster {  // AggregatedCluster (R/light_tree/builder.rs:350-437)
-  float flux = 0.0f;
+  real flux = 0.0f;
   V3 mu{0.f, 0.f, 0.f};
-  float sigma_s2 = 0.0f;
+  real sigma_s2 = 0.0f;
   V3 nu_bar{0.f, 0.f, 0.f};
   Box aabb;
-  float radius = 0.0f;
+  real radius = 0.0f;
   Cone cone;
   void add_leaf(const Leaf& l) {
-    float nf = flux + l.flux;
+    real nf = flux + l.flux;
     if (nf <= 0.0f) return;
     if (flux <= 0.0f) {
       flux = l.flux;
@@ -270,17 +270,17 @@ struct Cluster {  // AggregatedCluster (R/light_tree/builder.rs:350-437)
   }
 };
 
-inline float axis_of(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+inline real axis_of(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
 
 // triangle_leaf (R/light_tree/builder.rs:123-185); positions in world space.
-inline bool triangle_leaf(uint32_t inst, uint32_t tri, V3 p0, V3 p1, V3 p2, float max_emission, Leaf* out) {
+inline bool triangle_leaf(uint32_t inst, uint32_t tri, V3 p0, V3 p1, V3 p2, real max_emission, Leaf* out) {
   V3 e1 = sub(p1, p0), e2 = sub(p2, p0);
   V3 gn = cross(e1, e2);
-  float ad = vlen(gn);
+  real ad = vlen(gn);
   if (ad <= 0.0f) return false;
   V3 ng = vdiv(gn, ad);
-  float area = 0.5f * ad;
-  float phi = max_emission * area;
+  real area = 0.5f * ad;
+  real phi = max_emission * area;
   if (phi <= 0.0f) return false;
   Leaf l;
   l.inst = inst;
@@ -300,7 +300,7 @@ inline bool triangle_leaf(uint32_t inst, uint32_t tri, V3 p0, V3 p1, V3 p2, floa
   return true;
 }
 
-inline size_t bin_of(float v) {  // `as usize` saturates; NaN -> 0
+inline size_t bin_of(real v) {  // `as usize` saturates; NaN -> 0
   if (!(v > 0.0f)) return 0;
   if (v >= 12.0f) return 11;
   return std::min<size_t>((size_t)v, 11);
@@ -316,17 +316,17 @@ inline size_t best_split(Leaf* lv, size_t n, const Cluster& total) {
   }
   V3 ext = sub(cmax, cmin);
   if (vmax_elem(ext) <= 0.0f) return 0;
-  float parea = fmaxr(total.aabb.area(), 1.0e-30f);
-  float pcone = fmaxr(total.cone.measure(), 1.0e-30f);
-  float best = INFINITY;
+  real parea = fmaxr(total.aabb.area(), 1.0e-30f);
+  real pcone = fmaxr(total.cone.measure(), 1.0e-30f);
+  real best = INFINITY;
   int best_axis = 0;
   size_t best_at = 0;
   for (int a = 0; a < 3; ++a) {
     if (axis_of(ext, a) <= 0.0f) continue;
     Cluster bins[12];
     uint32_t cnt[12] = {0};
-    float inv = 12.0f / axis_of(ext, a);
-    float lo = axis_of(cmin, a);
+    real inv = 12.0f / axis_of(ext, a);
+    real lo = axis_of(cmin, a);
     for (size_t i = 0; i < n; ++i) {
       size_t b = bin_of((axis_of(lv[i].centroid, a) - lo) * inv);
       bins[b].add_leaf(lv[i]);
@@ -358,9 +358,9 @@ inline size_t best_split(Leaf* lv, size_t n, const Cluster& total) {
       const Cluster& l = pre[s - 1];
       const Cluster& r = suf[s];
       if (pc[s - 1] == 0 || sc[s] == 0) continue;
-      float cost = l.flux * l.aabb.area() * l.cone.measure() + r.flux * r.aabb.area() * r.cone.measure();
+      real cost = l.flux * l.aabb.area() * l.cone.measure() + r.flux * r.aabb.area() * r.cone.measure();
       if (!std::isfinite(cost)) continue;
-      float norm = cost / (parea * pcone);
+      real norm = cost / (parea * pcone);
       if (norm < best) {
         best = norm;
         best_axis = a;
@@ -441,7 +441,7 @@ inline uint32_t build_rec(Leaf* lv, size_t n, std::vector<Node>& nodes) {  // :4
 struct Tree {
   std::vector<Node> nodes;
   uint32_t root = 0;
-  float root_flux() const { return nodes.empty() ? 0.0f : nodes[root].flux; }
+  real root_flux() const { return nodes.empty() ? 0.0f : nodes[root].flux; }
 };
 
 inline Tree build_tree(std::vector<Leaf> leaves) {  // build_light_tree (:58-97)
@@ -463,55 +463,55 @@ inline Tree build_tree(std::vector<Leaf> leaves) {  // build_light_tree (:58-97)
 
 // ---- importance and traversal ------------------------------------------------------------
 // sg_light_for_node (R/light_tree/lobe.rs:233-274)
-inline bool sg_light_for_node(V3 p, V3 n, const Node& nd, float* w, SgLobe* lobe) {
+inline bool sg_light_for_node(V3 p, V3 n, const Node& nd, real* w, SgLobe* lobe) {
   if (nd.flux <= 0.0f) return false;
   V3 d = sub(nd.mu, p);
-  float r2 = dot(d, d);
+  real r2 = dot(d, d);
   if (r2 <= 0.0f) return false;
-  float inv_r = 1.0f / std::sqrt(r2);
-  float cos_below = fmaxr(-(dot(n, d)) * inv_r, 0.0f);
-  float bound_var = 0.5f * nd.radius * nd.radius;
-  float sigma2 = nd.sigma_s2 * (1.0f - cos_below) + bound_var * cos_below;
-  float min_sigma2 = r2 / kSgSharpnessMax;
+  real inv_r = 1.0f / std::sqrt(r2);
+  real cos_below = fmaxr(-(dot(n, d)) * inv_r, 0.0f);
+  real bound_var = 0.5f * nd.radius * nd.radius;
+  real sigma2 = nd.sigma_s2 * (1.0f - cos_below) + bound_var * cos_below;
+  real min_sigma2 = r2 / kSgSharpnessMax;
   if (sigma2 < min_sigma2) sigma2 = min_sigma2;
   V3 axis = vscale(d, inv_r);
-  float sharp = clampr(r2 / sigma2, 0.0f, kSgSharpnessMax);
+  real sharp = clampr(r2 / sigma2, 0.0f, kSgSharpnessMax);
   *lobe = sg_product(vneg(nd.nu), nd.lambda, axis, sharp);
-  float sg_norm = fmaxr(sg_integral(nd.lambda), 1.0e-30f);
-  float w_amp = nd.flux / (2.0f * kPiF * sigma2 * sg_norm);
+  real sg_norm = fmaxr(sg_integral(nd.lambda), 1.0e-30f);
+  real w_amp = nd.flux / (2.0f * kPiF * sigma2 * sg_norm);
   *w = w_amp * std::exp(lobe->log_amplitude);
   return std::isfinite(*w) && *w > 0.0f;
 }
 
 // diffuse_importance (R/light_tree/lobe.rs:276-287)
-inline float diffuse_importance(float rho, V3 n, float w, const SgLobe& lobe) {
+inline real diffuse_importance(real rho, V3 n, real w, const SgLobe& lobe) {
   if (rho <= 0.0f || w <= 0.0f) return 0.0f;
-  float c = clampr(dot(lobe.axis, n), -1.0f, 1.0f);
-  float integral = sg_cc_product_over_pi(c, lobe.sharpness) * kPiF;
+  real c = clampr(dot(lobe.axis, n), -1.0f, 1.0f);
+  real integral = sg_cc_product_over_pi(c, lobe.sharpness) * kPiF;
   return rho * w * integral;
 }
 
 // ---- glossy lobe (R/light_tree/lobe.rs:112-163, 290-341, 396-436) -----------------
 // sg_normalized_hemispherical_integral (R/math/sg.rs:155-171)
-inline float sg_normalized_hemispherical_integral(float cosine, float sharpness) {
-  const float A = 0.65173288f, B = 1.341828f, C = 7.221669f;
-  float steepness = sharpness * std::sqrt((0.5f * sharpness + A) / ((sharpness + B) * sharpness + C));
+inline real sg_normalized_hemispherical_integral(real cosine, real sharpness) {
+  const real A = 0.65173288f, B = 1.341828f, C = 7.221669f;
+  real steepness = sharpness * std::sqrt((0.5f * sharpness + A) / ((sharpness + B) * sharpness + C));
   cosine = clampr(cosine, -1.0f, 1.0f);
-  float denom = erf_approx(steepness);
-  float numer = erf_approx(steepness * cosine);
-  float v = std::fabs(denom) > 0.0f ? 0.5f + 0.5f * (numer / denom) : 0.5f;
+  real denom = erf_approx(steepness);
+  real numer = erf_approx(steepness * cosine);
+  real v = std::fabs(denom) > 0.0f ? 0.5f + 0.5f * (numer / denom) : 0.5f;
   return clampr(v, 0.0f, 1.0f);
 }
 // vmf_hemispherical_integral (R/math/sg.rs:189-194)
-inline float vmf_hemispherical_integral(float cosine, float sharpness) {
-  float v = sg_normalized_hemispherical_integral(cosine, sharpness);
-  float e = std::exp(-sharpness);
-  float num = e + (1.0f - e) * v;
+inline real vmf_hemispherical_integral(real cosine, real sharpness) {
+  real v = sg_normalized_hemispherical_integral(cosine, sharpness);
+  real e = std::exp(-sharpness);
+  real num = e + (1.0f - e) * v;
   return num / (e + 1.0f);
 }
 // glam Mat2: two columns; a * b takes column j of b through a
 struct M2 {
-  float c0x, c0y, c1x, c1y;
+  real c0x, c0y, c1x, c1y;
 };
 inline M2 m2_mul(const M2& a, const M2& b) {
   return M2{a.c0x * b.c0x + a.c1x * b.c0y, a.c0y * b.c0x + a.c1y * b.c0y, a.c0x * b.c1x + a.c1x * b.c1y,
@@ -524,30 +524,30 @@ struct Frame3 {
 };
 inline V3 frame_to_local(const Frame3& f, V3 w) { return v3(dot(w, f.t), dot(w, f.b), dot(w, f.n)); }
 struct GlossyLobe {  // GlossyLobePrecompute (R/light_tree/lobe.rs:36-46)
-  float rho;
+  real rho;
   V3 wi_ts;
   M2 jj;
-  float det_jj4;
-  float beta_x, beta_y;
-  float alpha2_max;
+  real det_jj4;
+  real beta_x, beta_y;
+  real alpha2_max;
   V3 xi_p;
-  float kappa_p;
+  real kappa_p;
 };
 // make_glossy_lobe (R/light_tree/lobe.rs:112-163); false for None
-inline bool make_glossy_lobe(float rho, const Frame3& frame, V3 wo_world, float alpha_x, float alpha_y,
+inline bool make_glossy_lobe(real rho, const Frame3& frame, V3 wo_world, real alpha_x, real alpha_y,
                              GlossyLobe* g) {
   if (rho <= 0.0f) return false;
   V3 wi_ts = frame_to_local(frame, wo_world);
   if (wi_ts.z <= 0.0f) return false;
-  float vlen = std::sqrt(wi_ts.x * wi_ts.x + wi_ts.y * wi_ts.y);
-  float vx = vlen > 0.0f ? wi_ts.x / vlen : 1.0f, vy = vlen > 0.0f ? wi_ts.y / vlen : 0.0f;
+  real vlen = std::sqrt(wi_ts.x * wi_ts.x + wi_ts.y * wi_ts.y);
+  real vx = vlen > 0.0f ? wi_ts.x / vlen : 1.0f, vy = vlen > 0.0f ? wi_ts.y / vlen : 0.0f;
   M2 r{vx, vy, -vy, vx};
   M2 d{0.5f, 0.0f, 0.0f, 0.5f / fmaxr(wi_ts.z, 1.0e-4f)};
   M2 jac = m2_mul(r, d);
   g->jj = m2_mul(jac, m2_transpose(jac));
   g->det_jj4 = 1.0f / (4.0f * wi_ts.z * wi_ts.z);
-  float a2x = clampr(alpha_x * alpha_x, 0.0f, 1.0f - 1.0e-4f);
-  float a2y = clampr(alpha_y * alpha_y, 0.0f, 1.0f - 1.0e-4f);
+  real a2x = clampr(alpha_x * alpha_x, 0.0f, 1.0f - 1.0e-4f);
+  real a2y = clampr(alpha_y * alpha_y, 0.0f, 1.0f - 1.0e-4f);
   g->beta_x = a2x / (1.0f - a2x);
   g->beta_y = a2y / (1.0f - a2y);
   g->alpha2_max = fmaxr(fmaxr(a2x, a2y), 1.0e-6f);
@@ -559,48 +559,48 @@ inline bool make_glossy_lobe(float rho, const Frame3& frame, V3 wo_world, float 
   return true;
 }
 // sggx_reflection_pdf (R/light_tree/lobe.rs:396-419)
-inline float sggx_reflection_pdf(V3 wi, V3 h, const M2& m) {
-  float det = fmaxr(m.c0x * m.c1y - m.c0y * m.c1x, 1.0e-30f);
+inline real sggx_reflection_pdf(V3 wi, V3 h, const M2& m) {
+  real det = fmaxr(m.c0x * m.c1y - m.c0y * m.c1x, 1.0e-30f);
   M2 adj{m.c1y, -m.c0y, -m.c1x, m.c0x};
   // adj * m_xy
-  float ax = adj.c0x * h.x + adj.c1x * h.y, ay = adj.c0y * h.x + adj.c1y * h.y;
-  float length2 = (h.x * ax + h.y * ay) / det + h.z * h.z;
-  float sggx = 1.0f / (kPiF * std::sqrt(det) * (length2 * length2));
-  float inv_norm = std::sqrt(fmaxr(wi.x * wi.x * m.c0x + 2.0f * wi.x * wi.y * m.c0y + wi.y * wi.y * m.c1y + wi.z * wi.z,
+  real ax = adj.c0x * h.x + adj.c1x * h.y, ay = adj.c0y * h.x + adj.c1y * h.y;
+  real length2 = (h.x * ax + h.y * ay) / det + h.z * h.z;
+  real sggx = 1.0f / (kPiF * std::sqrt(det) * (length2 * length2));
+  real inv_norm = std::sqrt(fmaxr(wi.x * wi.x * m.c0x + 2.0f * wi.x * wi.y * m.c0y + wi.y * wi.y * m.c1y + wi.z * wi.z,
                                    0.0f));
   return sggx / (4.0f * fmaxr(inv_norm, 1.0e-30f));
 }
 // glossy_importance (R/light_tree/lobe.rs:290-341)
-inline float glossy_importance(const GlossyLobe& g, const Frame3& frame, V3 n, float w, const SgLobe& lobe) {
+inline real glossy_importance(const GlossyLobe& g, const Frame3& frame, V3 n, real w, const SgLobe& lobe) {
   if (g.rho <= 0.0f || w <= 0.0f) return 0.0f;
-  float inv_kappa = 1.0f / fmaxr(lobe.sharpness, 1.0e-30f);
+  real inv_kappa = 1.0f / fmaxr(lobe.sharpness, 1.0e-30f);
   M2 fp{g.beta_x + g.jj.c0x * inv_kappa, 0.0f + g.jj.c0y * inv_kappa, 0.0f + g.jj.c1x * inv_kappa,
         g.beta_y + g.jj.c1y * inv_kappa};
-  float det = g.beta_x * g.beta_y + 2.0f * inv_kappa * (g.beta_x * g.jj.c1y + g.beta_y * g.jj.c0x) +
+  real det = g.beta_x * g.beta_y + 2.0f * inv_kappa * (g.beta_x * g.jj.c1y + g.beta_y * g.jj.c0x) +
               inv_kappa * inv_kappa * g.det_jj4 / 4.0f;
-  float tr = fp.c0x + fp.c1y;
-  float denom = fmaxr(1.0f + tr + det, 1.0e-12f);
+  real tr = fp.c0x + fp.c1y;
+  real denom = fmaxr(1.0f + tr + det, 1.0e-12f);
   M2 a_bar{(fp.c0x + det) / denom, fp.c0y / denom, fp.c1x / denom, (fp.c1y + det) / denom};
   V3 xi_ts = frame_to_local(frame, lobe.axis);
   V3 hu = add(g.wi_ts, xi_ts);
   if (dot(hu, hu) <= 1.0e-20f) return 0.0f;
   V3 h = vnormalize(hu);
-  float lobe_value = sggx_reflection_pdf(g.wi_ts, h, a_bar);
+  real lobe_value = sggx_reflection_pdf(g.wi_ts, h, a_bar);
   V3 prod = add(vscale(g.xi_p, g.kappa_p), vscale(lobe.axis, lobe.sharpness));
-  float prod_sharp = vlen(prod);
+  real prod_sharp = vlen(prod);
   if (prod_sharp <= 0.0f) return 0.0f;
   V3 prod_dir = vdiv(prod, prod_sharp);
-  float v_vis = vmf_hemispherical_integral(clampr(dot(prod_dir, n), -1.0f, 1.0f), prod_sharp);
-  float sg_full = sg_integral(lobe.sharpness);
+  real v_vis = vmf_hemispherical_integral(clampr(dot(prod_dir, n), -1.0f, 1.0f), prod_sharp);
+  real sg_full = sg_integral(lobe.sharpness);
   return g.rho * w * v_vis * lobe_value * sg_full;
 }
 // merge_glossy_roughness (R/light_tree/lobe.rs:421-436)
-inline void merge_glossy_roughness(float rho_a, float ax_a, float ay_a, float rho_b, float ax_b, float ay_b,
-                                   float* ax, float* ay) {
-  float total = fmaxr(rho_a + rho_b, 1.0e-30f);
-  float wa = rho_a / total, wb = rho_b / total;
-  float a2x = wa * ax_a * ax_a + wb * ax_b * ax_b;
-  float a2y = wa * ay_a * ay_a + wb * ay_b * ay_b;
+inline void merge_glossy_roughness(real rho_a, real ax_a, real ay_a, real rho_b, real ax_b, real ay_b,
+                                   real* ax, real* ay) {
+  real total = fmaxr(rho_a + rho_b, 1.0e-30f);
+  real wa = rho_a / total, wb = rho_b / total;
+  real a2x = wa * ax_a * ax_a + wb * ax_b * ax_b;
+  real a2y = wa * ay_a * ay_a + wb * ay_b * ay_b;
   *ax = std::sqrt(a2x);
   *ay = std::sqrt(a2y);
 }
@@ -609,7 +609,7 @@ inline void merge_glossy_roughness(float rho_a, float ax_a, float ay_a, float rh
 // diffuse lobe (rho > 0) and / or a glossy lobe (has_glossy) in `frame`.
 struct Query {
   V3 p, n;
-  float rho;
+  real rho;
   bool has_glossy = false;
   Frame3 frame{};
   GlossyLobe glossy{};
@@ -618,21 +618,21 @@ struct Query {
 // material's light_tree_importance: the diffuse lobe alone
 // (R/material/normalized_lambert.rs), or the lobes summed and clamped at
 // zero (R/material/standard_surface.rs:700-717).
-inline float node_importance(const Query& q, const Node& nd) {
-  float w;
+inline real node_importance(const Query& q, const Node& nd) {
+  real w;
   SgLobe lobe;
   if (!sg_light_for_node(q.p, q.n, nd, &w, &lobe)) return 0.0f;
-  float imp = diffuse_importance(q.rho, q.n, w, lobe);
+  real imp = diffuse_importance(q.rho, q.n, w, lobe);
   if (!q.has_glossy) return imp;
   return fmaxr(imp + glossy_importance(q.glossy, q.frame, q.n, w, lobe), 0.0f);
 }
 
 // sample_light_tree (R/light_tree/traversal.rs:26-92)
-inline bool sample_tree(const Tree& t, const Query& q, float u, uint32_t* leaf_node, float* pmf_out) {
+inline bool sample_tree(const Tree& t, const Query& q, real u, uint32_t* leaf_node, real* pmf_out) {
   if (t.nodes.empty()) return false;
   u = clampr(u, 0.0f, 1.0f - kFltEps);
   uint32_t ni = t.root;
-  float pmf = 1.0f;
+  real pmf = 1.0f;
   for (;;) {
     const Node& nd = t.nodes[ni];
     if (nd.leaf) {
@@ -642,13 +642,13 @@ inline bool sample_tree(const Tree& t, const Query& q, float u, uint32_t* leaf_n
     }
     const Node& L = t.nodes[nd.left];
     const Node& R = t.nodes[nd.right];
-    float il = node_importance(q, L), ir = node_importance(q, R);
-    float total = il + ir;
+    real il = node_importance(q, L), ir = node_importance(q, R);
+    real total = il + ir;
     if (total <= 0.0f) {
-      float lf = fmaxr(L.flux, 0.0f), rf = fmaxr(R.flux, 0.0f);
-      float tf = lf + rf;
+      real lf = fmaxr(L.flux, 0.0f), rf = fmaxr(R.flux, 0.0f);
+      real tf = lf + rf;
       if (tf <= 0.0f) return false;
-      float pl = lf / tf;
+      real pl = lf / tf;
       if (u < pl) {
         u /= fmaxr(pl, kFltMin);
         pmf *= pl;
@@ -660,7 +660,7 @@ inline bool sample_tree(const Tree& t, const Query& q, float u, uint32_t* leaf_n
       }
       continue;
     }
-    float pl = il / total;
+    real pl = il / total;
     if (u < pl) {
       u /= fmaxr(pl, kFltMin);
       pmf *= pl;
@@ -675,21 +675,21 @@ inline bool sample_tree(const Tree& t, const Query& q, float u, uint32_t* leaf_n
 }
 
 // pdf_for_leaf (R/light_tree/traversal.rs:97-140)
-inline float pdf_leaf(const Tree& t, const Query& q, uint32_t leaf_node) {
+inline real pdf_leaf(const Tree& t, const Query& q, uint32_t leaf_node) {
   if (t.nodes.empty() || leaf_node == kInvalid) return 0.0f;
   if (!t.nodes[leaf_node].leaf) return 0.0f;
-  float pmf = 1.0f;
+  real pmf = 1.0f;
   uint32_t child = leaf_node, parent = t.nodes[leaf_node].parent;
   while (parent != kInvalid) {
     const Node& pn = t.nodes[parent];
     const Node& L = t.nodes[pn.left];
     const Node& R = t.nodes[pn.right];
-    float il = node_importance(q, L), ir = node_importance(q, R);
-    float total = il + ir;
-    float pl;
+    real il = node_importance(q, L), ir = node_importance(q, R);
+    real total = il + ir;
+    real pl;
     if (total <= 0.0f) {
-      float lf = fmaxr(L.flux, 0.0f), rf = fmaxr(R.flux, 0.0f);
-      float tf = lf + rf;
+      real lf = fmaxr(L.flux, 0.0f), rf = fmaxr(R.flux, 0.0f);
+      real tf = lf + rf;
       if (tf <= 0.0f) return 0.0f;
       pl = lf / tf;
     } else {
@@ -706,17 +706,17 @@ inline float pdf_leaf(const Tree& t, const Query& q, uint32_t leaf_node) {
 // ---- environment light (R/light/environment.rs) ----------------------------------------
 namespace env {
 
-constexpr float kOneMinusEps = 1.0f - kFltEps;
-constexpr float kTauF = 6.28318530717958647692f;
+constexpr real kOneMinusEps = 1.0f - kFltEps;
+constexpr real kTauF = 6.28318530717958647692f;
 
 struct Level {
   size_t w, h;
-  std::vector<float> wt;
+  std::vector<real> wt;
 };
 struct Dist {  // HierarchicalDistribution: levels[0] is 1x1, last is the padded leaf
   std::vector<Level> levels;
-  float total = 0.0f;
-  float texel(size_t i, size_t j) const { return levels.back().wt[j * levels.back().w + i]; }
+  real total = 0.0f;
+  real texel(size_t i, size_t j) const { return levels.back().wt[j * levels.back().w + i]; }
 };
 
 inline size_t next_pow2(size_t v) {
@@ -724,13 +724,13 @@ inline size_t next_pow2(size_t v) {
   while (p < v) p <<= 1;
   return p;
 }
-inline float row_sin_theta(size_t j, size_t h) { return std::sin(((float)j + 0.5f) / (float)h * kPiF); }
-inline float positive_luminance(V3 c) { return fmaxr(lt::luminance(c), 0.0f); }
+inline real row_sin_theta(size_t j, size_t h) { return std::sin(((real)j + 0.5f) / (real)h * kPiF); }
+inline real positive_luminance(V3 c) { return fmaxr(lt::luminance(c), 0.0f); }
 
-inline Dist build_distribution(size_t w, size_t h, const std::vector<float>& weights) {  // :426-478
+inline Dist build_distribution(size_t w, size_t h, const std::vector<real>& weights) {  // :426-478
   Dist d;
   const size_t pw = next_pow2(w), ph = next_pow2(h);
-  Level leaf{pw, ph, std::vector<float>(pw * ph, 0.0f)};
+  Level leaf{pw, ph, std::vector<real>(pw * ph, 0.0f)};
   for (size_t j = 0; j < h; ++j)
     for (size_t i = 0; i < w; ++i) leaf.wt[j * pw + i] = fmaxr(weights[j * w + i], 0.0f);
   std::vector<Level> lv{leaf};
@@ -741,7 +741,7 @@ inline Dist build_distribution(size_t w, size_t h, const std::vector<float>& wei
     c.wt.assign(c.w * c.h, 0.0f);
     for (size_t j = 0; j < c.h; ++j)
       for (size_t i = 0; i < c.w; ++i) {
-        float sum = 0.0f;
+        real sum = 0.0f;
         for (size_t dj = 0; dj < sj; ++dj)
           for (size_t di = 0; di < si; ++di) sum += f.wt[(j * sj + dj) * f.w + i * si + di];
         c.wt[j * c.w + i] = sum;
@@ -755,44 +755,44 @@ inline Dist build_distribution(size_t w, size_t h, const std::vector<float>& wei
 
 struct Env {
   size_t w = 0, h = 0;
-  float scale = 1.0f;
-  float rot[9], inv_rot[9];  // Mat3 columns
+  real scale = 1.0f;
+  real rot[9], inv_rot[9];  // Mat3 columns
   std::vector<V3> px;
   Dist dist, dist_mis;
 };
 
-inline void rot_y(float angle, float* m) {  // Mat3::from_rotation_y, columns
-  const float s = std::sin(angle), c = std::cos(angle);
-  const float v[9] = {c, 0.0f, -s, 0.0f, 1.0f, 0.0f, s, 0.0f, c};
+inline void rot_y(real angle, real* m) {  // Mat3::from_rotation_y, columns
+  const real s = std::sin(angle), c = std::cos(angle);
+  const real v[9] = {c, 0.0f, -s, 0.0f, 1.0f, 0.0f, s, 0.0f, c};
   for (int k = 0; k < 9; ++k) m[k] = v[k];
 }
-inline V3 mat3_mul(const float* m, V3 v) {  // Mat3 * Vec3: (x*c0 + y*c1) + z*c2
+inline V3 mat3_mul(const real* m, V3 v) {  // Mat3 * Vec3: (x*c0 + y*c1) + z*c2
   return add(add(vscale(v3(m[0], m[1], m[2]), v.x), vscale(v3(m[3], m[4], m[5]), v.y)), vscale(v3(m[6], m[7], m[8]), v.z));
 }
 
-inline Env make(size_t w, size_t h, std::vector<V3> px, float scale, float rotate_y) {  // from_pixels (:47-83)
+inline Env make(size_t w, size_t h, std::vector<V3> px, real scale, real rotate_y) {  // from_pixels (:47-83)
   Env e;
   e.w = w;
   e.h = h;
   e.scale = scale;
   e.px = std::move(px);
-  std::vector<float> wt(w * h), wm(w * h);
+  std::vector<real> wt(w * h), wm(w * h);
   for (size_t j = 0; j < h; ++j) {  // environment_weights (:362-371)
-    const float st = row_sin_theta(j, h);
+    const real st = row_sin_theta(j, h);
     for (size_t i = 0; i < w; ++i) wt[j * w + i] = positive_luminance(e.px[j * w + i]) * st;
   }
-  float ws = 0.0f, wsum = 0.0f;  // solid_angle_mean_luminance (:393-408)
+  real ws = 0.0f, wsum = 0.0f;  // solid_angle_mean_luminance (:393-408)
   for (size_t j = 0; j < h; ++j) {
-    const float st = row_sin_theta(j, h);
-    wsum += st * (float)w;
+    const real st = row_sin_theta(j, h);
+    wsum += st * (real)w;
     for (size_t i = 0; i < w; ++i) ws += positive_luminance(e.px[j * w + i]) * st;
   }
-  const float mean = wsum > 0.0f ? ws / wsum : 0.0f;
-  const float eps = fmaxr(std::fabs(mean), 1.0f) * 1.0e-6f;  // mis_compensated_environment_weights (:373-391)
+  const real mean = wsum > 0.0f ? ws / wsum : 0.0f;
+  const real eps = fmaxr(std::fabs(mean), 1.0f) * 1.0e-6f;  // mis_compensated_environment_weights (:373-391)
   for (size_t j = 0; j < h; ++j) {
-    const float st = row_sin_theta(j, h);
+    const real st = row_sin_theta(j, h);
     for (size_t i = 0; i < w; ++i) {
-      const float c = positive_luminance(e.px[j * w + i]) - mean;
+      const real c = positive_luminance(e.px[j * w + i]) - mean;
       wm[j * w + i] = c > eps ? c * st : 0.0f;
     }
   }
@@ -803,60 +803,60 @@ inline Env make(size_t w, size_t h, std::vector<V3> px, float scale, float rotat
   return e;
 }
 
-inline float total_power(const Env& e) {  // :129-132
-  const float texel = kTauF * kPiF / ((float)e.w * (float)e.h);
+inline real total_power(const Env& e) {  // :129-132
+  const real texel = kTauF * kPiF / ((real)e.w * (real)e.h);
   return e.dist.total * e.scale * texel;
 }
-inline size_t pixel_coord(float t, size_t n) {  // :326-330
-  const float c = clampr(t, 0.0f, 1.0f);
-  const float f = c * (float)n;
+inline size_t pixel_coord(real t, size_t n) {  // :326-330
+  const real c = clampr(t, 0.0f, 1.0f);
+  const real f = c * (real)n;
   size_t idx = f >= 1.8446744e19f ? ~size_t(0) : (size_t)f;
   return std::min(idx, n - 1);
 }
-inline V3 uv_to_direction(float u, float v) {  // :332-337
-  const float phi = u * kTauF, theta = v * kPiF;
-  const float st = std::sin(theta);
+inline V3 uv_to_direction(real u, real v) {  // :332-337
+  const real phi = u * kTauF, theta = v * kPiF;
+  const real st = std::sin(theta);
   return v3(st * std::sin(phi), std::cos(theta), st * std::cos(phi));
 }
-inline void direction_to_uv(V3 d, float* u, float* v) {  // :339-348
+inline void direction_to_uv(V3 d, real* u, real* v) {  // :339-348
   const V3 dir = vnormalize_or_zero(d);
-  const float y = clampr(dir.y, -1.0f, 1.0f);
-  const float theta = std::acos(y);
-  float phi = std::atan2(dir.x, dir.z);
+  const real y = clampr(dir.y, -1.0f, 1.0f);
+  const real theta = std::acos(y);
+  real phi = std::atan2(dir.x, dir.z);
   if (phi < 0.0f) phi += kTauF;
   *u = clampr(phi / kTauF, 0.0f, 1.0f);
   *v = clampr(theta / kPiF, 0.0f, 1.0f);
 }
 inline V3 radiance(const Env& e, V3 dir) {  // :134-140
-  float u, v;
+  real u, v;
   direction_to_uv(mat3_mul(e.inv_rot, dir), &u, &v);
   const V3 p = e.px[pixel_coord(v, e.h) * e.w + pixel_coord(u, e.w)];
   return vscale(p, e.scale);
 }
-inline float pdf(const Env& e, const Dist& d, V3 dir) {  // pdf_with_distribution (:156-177)
+inline real pdf(const Env& e, const Dist& d, V3 dir) {  // pdf_with_distribution (:156-177)
   if (d.total <= 0.0f) return 0.0f;
-  float u, v;
+  real u, v;
   direction_to_uv(mat3_mul(e.inv_rot, dir), &u, &v);
-  const float st = std::sin(v * kPiF);
+  const real st = std::sin(v * kPiF);
   if (st <= 0.0f) return 0.0f;
-  const float wgt = d.texel(pixel_coord(u, e.w), pixel_coord(v, e.h));
+  const real wgt = d.texel(pixel_coord(u, e.w), pixel_coord(v, e.h));
   if (wgt <= 0.0f) return 0.0f;
-  const float p2d = wgt / d.total * (float)e.w * (float)e.h;
+  const real p2d = wgt / d.total * (real)e.w * (real)e.h;
   return p2d / (kTauF * kPiF * st);
 }
 // hsw_step (:480-530)
-inline bool hsw_step(const Level& parent, const Level& child, size_t* i, size_t* j, float* u, float* v) {
+inline bool hsw_step(const Level& parent, const Level& child, size_t* i, size_t* j, real* u, real* v) {
   const size_t si = child.w / parent.w, sj = child.h / parent.h;
   size_t ci = *i * si, cj = *j * sj;
   if (sj == 2) {
-    float top = 0.0f, bot = 0.0f;
+    real top = 0.0f, bot = 0.0f;
     for (size_t di = 0; di < si; ++di) {
       top += child.wt[cj * child.w + ci + di];
       bot += child.wt[(cj + 1) * child.w + ci + di];
     }
-    const float total = top + bot;
+    const real total = top + bot;
     if (total <= 0.0f) return false;
-    const float pt = top / total;
+    const real pt = top / total;
     if (*v < pt) {
       *v = clampr(*v / pt, 0.0f, kOneMinusEps);
     } else {
@@ -865,10 +865,10 @@ inline bool hsw_step(const Level& parent, const Level& child, size_t* i, size_t*
     }
   }
   if (si == 2) {
-    const float l = child.wt[cj * child.w + ci], r = child.wt[cj * child.w + ci + 1];
-    const float total = l + r;
+    const real l = child.wt[cj * child.w + ci], r = child.wt[cj * child.w + ci + 1];
+    const real total = l + r;
     if (total <= 0.0f) return false;
-    const float pl = l / total;
+    const real pl = l / total;
     if (*u < pl) {
       *u = clampr(*u / pl, 0.0f, kOneMinusEps);
     } else {
@@ -882,24 +882,24 @@ inline bool hsw_step(const Level& parent, const Level& child, size_t* i, size_t*
 }
 struct Sample {
   V3 dir, radiance;
-  float pdf;
+  real pdf;
 };
 // sample_with_distribution (:179-234)
-inline bool sample(const Env& e, const Dist& d, float us0, float us1, Sample* out) {
+inline bool sample(const Env& e, const Dist& d, real us0, real us1, Sample* out) {
   if (d.total <= 0.0f) return false;
-  float u = clampr(us0, 0.0f, 1.0f), v = clampr(us1, 0.0f, 1.0f);
+  real u = clampr(us0, 0.0f, 1.0f), v = clampr(us1, 0.0f, 1.0f);
   size_t i = 0, j = 0;
   for (size_t l = 1; l < d.levels.size(); ++l)
     if (!hsw_step(d.levels[l - 1], d.levels[l], &i, &j, &u, &v)) return false;
   if (i >= e.w || j >= e.h) return false;
-  const float wgt = d.texel(i, j);
+  const real wgt = d.texel(i, j);
   if (wgt <= 0.0f) return false;
-  const float uc = ((float)i + u) / (float)e.w, vc = ((float)j + v) / (float)e.h;
+  const real uc = ((real)i + u) / (real)e.w, vc = ((real)j + v) / (real)e.h;
   const V3 local = uv_to_direction(clampr(uc, 0.0f, kOneMinusEps), clampr(vc, 0.0f, kOneMinusEps));
   out->dir = mat3_mul(e.rot, local);
-  const float st = std::sin(vc * kPiF);
+  const real st = std::sin(vc * kPiF);
   if (st <= 0.0f) return false;
-  const float p2d = wgt / d.total * (float)e.w * (float)e.h;
+  const real p2d = wgt / d.total * (real)e.w * (real)e.h;
   out->pdf = p2d / (kTauF * kPiF * st);
   out->radiance = vscale(e.px[j * e.w + i], e.scale);
   return true;

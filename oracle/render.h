@@ -46,12 +46,12 @@ struct Material {
   OpenPbr pbr{};
   Luts luts;  // OpenPBR: the GGX tables it reads
   V3 color{0.f, 0.f, 0.f};  // Lambert rho, Emissive colour, Mirror colour
-  float strength = 0.0f;    // Emissive
+  real strength = 0.0f;    // Emissive
 };
 
 struct ShadeMesh {
-  std::vector<float> nrm;  // 3 per vertex (may be empty: geometric normal)
-  std::vector<float> uv;   // 2 per vertex (may be empty)
+  std::vector<real> nrm;  // 3 per vertex (may be empty: geometric normal)
+  std::vector<real> uv;   // 2 per vertex (may be empty)
 };
 struct RenderScene {
   const Scene* geo = nullptr;
@@ -70,7 +70,7 @@ struct RenderScene {
   // tint, perpendicular irradiance
   struct DirLight {
     V3 direction, color;
-    float intensity;
+    real intensity;
   };
   std::vector<DirLight> dirs;
   uint32_t nee_policy = 0;  // kNeeReference or kNeeOpenPbrProxy
@@ -87,18 +87,18 @@ struct Vertex {
   uint32_t inst, tri;
   // uv and its screen-space derivatives, the position and normal partials
   // (ShadingVertex, R/material/mod.rs:120-170)
-  float u = 0.0f, v = 0.0f, dudx = 0.0f, dvdx = 0.0f, dudy = 0.0f, dvdy = 0.0f;
+  real u = 0.0f, v = 0.0f, dudx = 0.0f, dvdx = 0.0f, dudy = 0.0f, dvdy = 0.0f;
   V3 dpdx{0.f, 0.f, 0.f}, dpdy{0.f, 0.f, 0.f}, dndu{0.f, 0.f, 0.f}, dndv{0.f, 0.f, 0.f};
 };
 
 // Ray with its optional differential and its cone (R/math/ray.rs:8-80).
 struct RayCone {
-  float width = 0.0f, spread = 0.0f;
+  real width = 0.0f, spread = 0.0f;
 };
-inline RayCone make_cone(float width, float spread) {  // RayCone::new
+inline RayCone make_cone(real width, real spread) {  // RayCone::new
   return RayCone{std::fmax(width, 0.0f), std::fmax(spread, 0.0f)};
 }
-inline float cone_width_at(const RayCone& c, float t) {  // RayCone::width_at
+inline real cone_width_at(const RayCone& c, real t) {  // RayCone::width_at
   if (std::isfinite(t)) return std::fmax(c.width + c.spread * std::fmax(t, 0.0f), 0.0f);
   return c.width;
 }
@@ -110,12 +110,12 @@ struct Ray {
 };
 
 // compute_surface_partials (R/math/mod.rs:112-134) of three values over uvs.
-inline bool surface_partials(V3 a0, V3 a1, V3 a2, const float* uv, V3* dadu, V3* dadv) {
+inline bool surface_partials(V3 a0, V3 a1, V3 a2, const real* uv, V3* dadu, V3* dadv) {
   V3 d1 = sub(a1, a0), d2 = sub(a2, a0);
-  float du1 = uv[2] - uv[0], dv1 = uv[3] - uv[1], du2 = uv[4] - uv[0], dv2 = uv[5] - uv[1];
-  float det = du1 * dv2 - dv1 * du2;
+  real du1 = uv[2] - uv[0], dv1 = uv[3] - uv[1], du2 = uv[4] - uv[0], dv2 = uv[5] - uv[1];
+  real det = du1 * dv2 - dv1 * du2;
   if (std::fabs(det) <= 1.0e-8f) return false;
-  float inv = 1.0f / det;
+  real inv = 1.0f / det;
   V3 a = smul(sub(mul(dv2, d1), mul(dv1, d2)), inv);
   V3 b = smul(add(mul(-du2, d1), mul(du1, d2)), inv);
   if (len_sq(a) == 0.0f || len_sq(b) == 0.0f) return false;
@@ -124,18 +124,18 @@ inline bool surface_partials(V3 a0, V3 a1, V3 a2, const float* uv, V3* dadu, V3*
   return true;
 }
 
-inline float finite_clamped_derivative(float x) {  // R/scene/mod.rs:861-867
+inline real finite_clamped_derivative(real x) {  // R/scene/mod.rs:861-867
   return std::isfinite(x) ? clampf(x, -1.0e8f, 1.0e8f) : 0.0f;
 }
 inline bool v3_finite(V3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
 
 // differentials_from_dp (R/scene/mod.rs:787-819)
 inline bool differentials_from_dp(Vertex& v, V3 dpdu, V3 dpdv, V3 dpdx, V3 dpdy) {
-  float a00 = dot(dpdu, dpdu), a01 = dot(dpdu, dpdv), a11 = dot(dpdv, dpdv);
-  float det = difference_of_products(a00, a11, a01, a01);
+  real a00 = dot(dpdu, dpdu), a01 = dot(dpdu, dpdv), a11 = dot(dpdv, dpdv);
+  real det = difference_of_products(a00, a11, a01, a01);
   if (std::fabs(det) <= 1.0e-12f || !std::isfinite(det)) return false;
-  float inv = 1.0f / det;
-  float b0x = dot(dpdu, dpdx), b1x = dot(dpdv, dpdx), b0y = dot(dpdu, dpdy), b1y = dot(dpdv, dpdy);
+  real inv = 1.0f / det;
+  real b0x = dot(dpdu, dpdx), b1x = dot(dpdv, dpdx), b0y = dot(dpdu, dpdy), b1y = dot(dpdv, dpdy);
   v.dpdx = dpdx;
   v.dpdy = dpdy;
   v.dudx = finite_clamped_derivative(difference_of_products(a11, b0x, a01, b1x) * inv);
@@ -150,11 +150,11 @@ inline bool differentials_from_dp(Vertex& v, V3 dpdu, V3 dpdv, V3 dpdx, V3 dpdy)
 inline void shading_differentials(Vertex& v, V3 n, V3 dpdu, V3 dpdv, const Ray* ray) {
   if (!ray) return;
   if (ray->has_diff) {
-    float plane_d = -dot(n, v.p);
-    float rxden = dot(n, ray->rxd), ryden = dot(n, ray->ryd);
+    real plane_d = -dot(n, v.p);
+    real rxden = dot(n, ray->rxd), ryden = dot(n, ray->ryd);
     if (std::fabs(rxden) > 1.0e-8f && std::fabs(ryden) > 1.0e-8f) {
-      float tx = (-dot(n, ray->rxo) - plane_d) / rxden;
-      float ty = (-dot(n, ray->ryo) - plane_d) / ryden;
+      real tx = (-dot(n, ray->rxo) - plane_d) / rxden;
+      real ty = (-dot(n, ray->ryo) - plane_d) / ryden;
       V3 px = add(ray->rxo, mul(tx, ray->rxd)), py = add(ray->ryo, mul(ty, ray->ryd));
       V3 dpdx = sub(px, v.p), dpdy = sub(py, v.p);
       if (std::isfinite(tx) && std::isfinite(ty) && v3_finite(dpdx) && v3_finite(dpdy) &&
@@ -162,16 +162,16 @@ inline void shading_differentials(Vertex& v, V3 n, V3 dpdu, V3 dpdv, const Ray* 
         return;
     }
   }
-  float dl2 = len_sq(ray->d);
+  real dl2 = len_sq(ray->d);
   if (dl2 <= 0.0f) return;
-  float t = dot(sub(v.p, ray->o), ray->d) / dl2;
+  real t = dot(sub(v.p, ray->o), ray->d) / dl2;
   if (!std::isfinite(t)) return;
-  float width = cone_width_at(ray->cone, t);
+  real width = cone_width_at(ray->cone, t);
   if (width <= 0.0f) return;
-  float cos_theta = std::fmax(std::fabs(dot(normalize_or_zero3(ray->d), n)), 1.0e-3f);
-  float projected = width / cos_theta;
-  float dudx = projected / std::fmax(std::sqrt(len_sq(dpdu)), 1.0e-6f);
-  float dvdy = projected / std::fmax(std::sqrt(len_sq(dpdv)), 1.0e-6f);
+  real cos_theta = std::fmax(std::fabs(dot(normalize_or_zero3(ray->d), n)), 1.0e-3f);
+  real projected = width / cos_theta;
+  real dudx = projected / std::fmax(std::sqrt(len_sq(dpdu)), 1.0e-6f);
+  real dvdy = projected / std::fmax(std::sqrt(len_sq(dpdv)), 1.0e-6f);
   v.dpdx = mul(dudx, dpdu);
   v.dpdy = mul(dvdy, dpdv);
   v.dudx = dudx;
@@ -180,7 +180,7 @@ inline void shading_differentials(Vertex& v, V3 n, V3 dpdu, V3 dpdv, const Ray* 
   v.dvdy = dvdy;
 }
 
-inline V3 mat3_mul_normal(const float* w2l, V3 n) {
+inline V3 mat3_mul_normal(const real* w2l, V3 n) {
   // normal_to_world = Mat3::from_mat4(world_to_local.transpose()): column c is
   // row c of world_to_local's upper 3x3.  mul_vec3 = (x*cx + y*cy) + z*cz.
   V3 c0 = v3(w2l[0], w2l[4], w2l[8]), c1 = v3(w2l[1], w2l[5], w2l[9]), c2 = v3(w2l[2], w2l[6], w2l[10]);
@@ -197,7 +197,7 @@ inline void triangle_world(const RenderScene& rs, uint32_t inst, uint32_t tri, V
 // base_shading_vertex_from_triangle_sample_impl (R/scene/mod.rs:471-544),
 // the fields the integrator reads; `dir` is the incident direction, `ray`
 // the ray that hit (null for a light sample's vertex: zero differentials).
-inline Vertex vertex_at(const RenderScene& rs, uint32_t inst, uint32_t tri, float b0, float b1, float b2, V3 dir,
+inline Vertex vertex_at(const RenderScene& rs, uint32_t inst, uint32_t tri, real b0, real b1, real b2, V3 dir,
                         const Ray* ray = nullptr) {
   const Instance& in = rs.geo->inst[inst];
   const Mesh& m = rs.geo->meshes[in.mesh];
@@ -220,7 +220,7 @@ inline Vertex vertex_at(const RenderScene& rs, uint32_t inst, uint32_t tri, floa
   v.ng = face_forward(gn, nd);
   V3 raw_ns = len_sq(sn) > 0.0f ? sn : v.ng;
   v.ns = face_forward(raw_ns, v.ng);
-  const float uvs[6] = {uvv(i0, 0), uvv(i0, 1), uvv(i1, 0), uvv(i1, 1), uvv(i2, 0), uvv(i2, 1)};
+  const real uvs[6] = {uvv(i0, 0), uvv(i0, 1), uvv(i1, 0), uvv(i1, 1), uvv(i2, 0), uvv(i2, 1)};
   // interpolate_vec2: (b0 uv0 + b1 uv1) + b2 uv2
   v.u = b0 * uvs[0] + b1 * uvs[2] + b2 * uvs[4];
   v.v = b0 * uvs[1] + b1 * uvs[3] + b2 * uvs[5];
@@ -289,8 +289,8 @@ inline bool pbr_le(const OpenPbr& m, const Vertex& v, V3* le) {
 }
 
 // OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548) in radiance mode.
-constexpr float kGeomCosEps = 1.0e-6f;  // R/material/mod.rs:31
-inline bool pbr_sample(const OpenPbr& m, const Luts& luts, const Vertex& v, float u_lobe, float ux, float uy, BSample* out,
+constexpr real kGeomCosEps = 1.0e-6f;  // R/material/mod.rs:31
+inline bool pbr_sample(const OpenPbr& m, const Luts& luts, const Vertex& v, real u_lobe, real ux, real uy, BSample* out,
                        V3* wi_world) {
   if (dot(v.wo, v.ng) <= 0.0f) return false;
   V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
@@ -299,7 +299,11 @@ inline bool pbr_sample(const OpenPbr& m, const Luts& luts, const Vertex& v, floa
   BSample s;
   if (!bsdf_sample(p, wol, u_lobe, ux, uy, &s)) return false;
   V3 wi = onb_to_world(v.frame, s.wi);
-  if (dot(wi, v.ng) <= kGeomCosEps) return false;  // every carried lobe reflects
+  if (s.transmission) {  // a rough refraction must leave through the geometric surface
+    if (dot(wi, v.ng) >= -kGeomCosEps && !s.delta) return false;
+  } else if (dot(wi, v.ng) <= kGeomCosEps) {
+    return false;
+  }
   // cone_spread: 2 * effective specular roughness for GLOSSY samples, else
   // GLOSSY_DIFFUSE_CONE_SPREAD 0.5 (:509-522)
   s.cone_spread = (s.glossy && !s.delta) ? 2.0f * clampf(p.spec_roughness, 0.0f, 1.0f) : 0.5f;
@@ -315,13 +319,13 @@ inline bool mat_may_emit(const Material& m) {
   if (m.kind == kMatOpenPbr) return m.pbr.emission_luminance > 0.0f;  // open_pbr.rs:377-379
   return false;
 }
-inline float mat_max_emission(const RenderScene& rs, const Material& m) {
+inline real mat_max_emission(const RenderScene& rs, const Material& m) {
   if (m.kind == kMatEmissive)  // emissive.rs max_emission, no texture
     return std::fmax(lt::vmax_elem(smul(m.color, m.strength)) * 1.0f, 0.0f);
   if (m.kind == kMatOpenPbr) {  // open_pbr.rs:381-391: texture factor = the emission texture's max_value
     if (m.pbr.emission_luminance <= 0.0f) return 0.0f;
     const uint32_t et = m.pbr.tex[kTexEmission];
-    const float tf = et != kNoTexture ? rs.textures[et].max_value : 1.0f;
+    const real tf = et != kNoTexture ? rs.textures[et].max_value : 1.0f;
     return std::fmax(lt::vmax_elem(m.pbr.emission_color) * m.pbr.emission_luminance * tf, 0.0f);
   }
   return 0.0f;
@@ -336,7 +340,7 @@ inline bool mat_le(const Material& m, const Vertex& v, V3* le) {
 }
 // NormalizedLambertMaterial::sample (normalized_lambert.rs:120-158) and
 // MirrorMaterial::sample (mirror.rs:112-143), radiance mode.
-inline bool mat_sample(const Material& m, const Vertex& v, float u_lobe, float ux, float uy, BSample* out,
+inline bool mat_sample(const Material& m, const Vertex& v, real u_lobe, real ux, real uy, BSample* out,
                        V3* wi_world) {
   if (m.kind == kMatOpenPbr) return pbr_sample(m.pbr, m.luts, v, u_lobe, ux, uy, out, wi_world);
   if (m.kind == kMatEmissive) return false;
@@ -345,7 +349,7 @@ inline bool mat_sample(const Material& m, const Vertex& v, float u_lobe, float u
   BSample s;
   if (m.kind == kMatLambert) {  // NormalizedLambertBsdf::sample (bsdf/normalized_lambert.rs:35-60)
     V3 wi = cosine_hemisphere(ux, uy);
-    float pdf = std::fmax(wi.z, 0.0f) / kPi;
+    real pdf = std::fmax(wi.z, 0.0f) / kPi;
     if (pdf <= 0.0f) return false;
     s.weight = m.color;
     s.wi = wi;
@@ -368,7 +372,7 @@ inline bool mat_sample(const Material& m, const Vertex& v, float u_lobe, float u
 // modified_bsdf_eval (R/material/mod.rs:33-53), radiance mode
 inline V3 shading_normal_correction(const Vertex& v, V3 wi, V3 f) {
   if (len_sq(f) == 0.0f) return v3(0.f, 0.f, 0.f);
-  float cg = std::fabs(dot(wi, v.ng));
+  real cg = std::fabs(dot(wi, v.ng));
   if (cg <= kGeomCosEps) return v3(0.f, 0.f, 0.f);
   return smul(f, std::fabs(dot(wi, v.ns)) / cg);
 }
@@ -393,7 +397,7 @@ inline V3 mat_eval(const Material& m, const Vertex& v, V3 wi) {
   }
   return zero;
 }
-inline float mat_pdf(const Material& m, const Vertex& v, V3 wi) {
+inline real mat_pdf(const Material& m, const Vertex& v, V3 wi) {
   if (m.kind == kMatLambert) {
     if (dot(v.wo, v.ng) <= 0.0f || dot(wi, v.ng) <= 0.0f) return 0.0f;
     V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
@@ -423,24 +427,24 @@ inline float mat_pdf(const Material& m, const Vertex& v, V3 wi) {
 // (alpha_xy), merged in that order by merge_glossy_roughness into one glossy
 // lobe; its frame is (t, normalize(n x t), n) with t the vertex frame's
 // tangent and n the normalised shading normal.
-inline float dielectric_f0(float ior) {
-  float q = (ior - 1.0f) / (ior + 1.0f);
+inline real dielectric_f0(real ior) {
+  real q = (ior - 1.0f) / (ior + 1.0f);
   return q * q;
 }
 inline bool openpbr_proxy_query(const OpenPbr& m, const Vertex& v, lt::Query* q) {
-  float metal = clampf(m.base_metalness, 0.0f, 1.0f), base = clampf(m.base_weight, 0.0f, 1.0f);
+  real metal = clampf(m.base_metalness, 0.0f, 1.0f), base = clampf(m.base_weight, 0.0f, 1.0f);
   V3 under = vlerp(vsplat(1.0f), vclamp01(m.coat_color), clampf(m.coat_weight, 0.0f, 1.0f) * 0.5f);
-  float lb = luminance(vmul(m.base_color, under));
-  float rho_d = std::fmax((1.0f - metal) * (1.0f - clampf(m.transmission_weight, 0.0f, 1.0f)) * base * lb, 0.0f);
-  float rho_metal = metal * base * lb;
-  float rho_spec = (1.0f - metal) * std::fmax(m.specular_weight, 0.0f) * luminance(vmul(vclamp01(m.specular_color), under)) *
+  real lb = luminance(vmul(m.base_color, under));
+  real rho_d = std::fmax((1.0f - metal) * (1.0f - clampf(m.transmission_weight, 0.0f, 1.0f)) * base * lb, 0.0f);
+  real rho_metal = metal * base * lb;
+  real rho_spec = (1.0f - metal) * std::fmax(m.specular_weight, 0.0f) * luminance(vmul(vclamp01(m.specular_color), under)) *
                    dielectric_f0(std::fmax(m.specular_ior, 1.0e-4f));
-  float rho_coat = clampf(m.coat_weight, 0.0f, 1.0f) * dielectric_f0(std::fmax(m.coat_ior, 1.0e-4f));
-  float sx, sy, cx, cy;
+  real rho_coat = clampf(m.coat_weight, 0.0f, 1.0f) * dielectric_f0(std::fmax(m.coat_ior, 1.0e-4f));
+  real sx, sy, cx, cy;
   alpha_xy(clampf(m.specular_roughness, 0.0f, 1.0f), m.specular_roughness_anisotropy, &sx, &sy);
   alpha_xy(clampf(m.coat_roughness, 0.0f, 1.0f), m.coat_roughness_anisotropy, &cx, &cy);
-  float rho = 0.0f, ax = 1.0f, ay = 1.0f;
-  const float rl[3] = {rho_metal, rho_spec, rho_coat}, xl[3] = {sx, sx, cx}, yl[3] = {sy, sy, cy};
+  real rho = 0.0f, ax = 1.0f, ay = 1.0f;
+  const real rl[3] = {rho_metal, rho_spec, rho_coat}, xl[3] = {sx, sx, cx}, yl[3] = {sy, sy, cy};
   for (int i = 0; i < 3; ++i) {
     if (!(rl[i] > 0.0f)) continue;
     if (rho > 0.0f) {
@@ -466,7 +470,7 @@ inline bool openpbr_proxy_query(const OpenPbr& m, const Vertex& v, lt::Query* q)
 inline bool mat_query(const RenderScene& rs, const Material& m, const Vertex& v, lt::Query* q) {
   if (m.kind == kMatOpenPbr && rs.nee_policy == kNeeOpenPbrProxy) return openpbr_proxy_query(m.pbr, v, q);
   if (m.kind != kMatLambert) return false;
-  float rho = lt::luminance(m.color);
+  real rho = lt::luminance(m.color);
   if (rho <= 0.0f) return false;
   *q = lt::Query{v.p, v.ns, rho};
   return true;
@@ -482,7 +486,7 @@ inline void build_lights(RenderScene& rs) {
   for (uint32_t i = 0; i < (uint32_t)g.inst.size(); ++i) {
     const Material& m = rs.materials[rs.inst_material[i]];
     if (!mat_may_emit(m)) continue;
-    float me = mat_max_emission(rs, m);
+    real me = mat_max_emission(rs, m);
     if (me <= 0.0f) continue;
     const Mesh& mesh = g.meshes[g.inst[i].mesh];
     const uint32_t ntri = (uint32_t)(mesh.idx.size() / 3);
@@ -491,7 +495,7 @@ inline void build_lights(RenderScene& rs) {
     for (uint32_t t = 0; t < ntri; ++t) {
       V3 p[3];
       triangle_world(rs, i, t, p);
-      float area = 0.5f * lt::vlen(cross(sub(p[1], p[0]), sub(p[2], p[0])));
+      real area = 0.5f * lt::vlen(cross(sub(p[1], p[0]), sub(p[2], p[0])));
       if (area * me <= 0.0f) continue;
       lt::Leaf l;
       if (lt::triangle_leaf(i, t, p[0], p[1], p[2], me, &l)) leaves.push_back(l);
@@ -504,13 +508,13 @@ inline void build_lights(RenderScene& rs) {
   }
 }
 
-inline float balance_heuristic(float a, float b) {  // R/math/mod.rs:163-171
-  float sum = a + b;
+inline real balance_heuristic(real a, real b) {  // R/math/mod.rs:163-171
+  real sum = a + b;
   if (a <= 0.0f || sum <= 0.0f) return 0.0f;
   return a / sum;
 }
 
-constexpr float kRayEpsilon = 1.0e-4f;  // R/integrator/mod.rs:49
+constexpr real kRayEpsilon = 1.0e-4f;  // R/integrator/mod.rs:49
 inline V3 offset_origin(V3 p, V3 ng, V3 dir) {
   V3 off = dot(dir, ng) >= 0.0f ? ng : neg(ng);
   return add(p, mul(kRayEpsilon, off));
@@ -523,41 +527,41 @@ inline V3 offset_origin(V3 p, V3 ng, V3 dir) {
 // directional light i.
 struct Category {
   int kind = -1;
-  float pmf = 0.0f;
+  real pmf = 0.0f;
 };
 struct LightSampler {
   std::vector<int> kind;
-  std::vector<float> weight, cdf;
-  float total = 0.0f;
+  std::vector<real> weight, cdf;
+  real total = 0.0f;
 };
 inline LightSampler light_sampler(const RenderScene& rs) {
   LightSampler ls;
-  float wt = std::fmax(rs.tree.root_flux(), 0.0f);
+  real wt = std::fmax(rs.tree.root_flux(), 0.0f);
   if (rs.tree.nodes.empty()) wt = 0.0f;
   if (wt > 0.0f) { ls.kind.push_back(0); ls.weight.push_back(wt); }
   if (rs.has_env) {
-    float we = std::fmax(lt::env::total_power(rs.env), 0.0f);
+    real we = std::fmax(lt::env::total_power(rs.env), 0.0f);
     if (we > 0.0f) { ls.kind.push_back(1); ls.weight.push_back(we); }
   }
   for (size_t i = 0; i < rs.dirs.size(); ++i) {
-    float w = std::fmax(lt::luminance(rs.dirs[i].color) * rs.dirs[i].intensity, 0.0f);
+    real w = std::fmax(lt::luminance(rs.dirs[i].color) * rs.dirs[i].intensity, 0.0f);
     if (w > 0.0f) { ls.kind.push_back(2 + (int)i); ls.weight.push_back(w); }
   }
   ls.cdf.push_back(0.0f);
-  float total = 0.0f;
-  for (float w : ls.weight) {
+  real total = 0.0f;
+  for (real w : ls.weight) {
     total += w;
     ls.cdf.push_back(total);
   }
   if (total > 0.0f) {
-    float inv = 1.0f / total;
-    for (float& c : ls.cdf) c *= inv;
+    real inv = 1.0f / total;
+    for (real& c : ls.cdf) c *= inv;
   }
   ls.total = total;
   return ls;
 }
 // sample_category (R/light/mod.rs:197-213): partition_point(c <= u) - 1
-inline bool sample_category(const RenderScene& rs, float u, Category* out) {
+inline bool sample_category(const RenderScene& rs, real u, Category* out) {
   LightSampler ls = light_sampler(rs);
   const int n = (int)ls.kind.size();
   if (n == 0 || ls.total <= 0.0f) return false;
@@ -570,7 +574,7 @@ inline bool sample_category(const RenderScene& rs, float u, Category* out) {
   out->pmf = ls.weight[idx] / ls.total;
   return true;
 }
-inline float category_pmf(const RenderScene& rs, int kind) {  // :215-224
+inline real category_pmf(const RenderScene& rs, int kind) {  // :215-224
   LightSampler ls = light_sampler(rs);
   if (ls.total <= 0.0f) return 0.0f;
   for (size_t i = 0; i < ls.kind.size(); ++i)
@@ -582,43 +586,43 @@ inline float category_pmf(const RenderScene& rs, int kind) {  // :215-224
 // sample_light_mis_compensated_lazy (R/light/mod.rs:289-331),
 // sample_li_for_triangle (R/light/area.rs:12-54) and the environment's
 // MIS-compensated sample (R/light/environment.rs:260-276).
-inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& vtx, float u_root, float u_tree,
-                       float u_aux, float us0, float us1) {
+inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& vtx, real u_root, real u_tree,
+                       real u_aux, real us0, real us1) {
   (void)u_aux;
   const V3 zero = v3(0.f, 0.f, 0.f);
   Category cat;
   if (!sample_category(rs, u_root, &cat)) return zero;
   V3 le, wi;
-  float li_pdf, selection;
+  real li_pdf, selection;
   uint32_t t_inst = lt::kInvalid, t_tri = lt::kInvalid;
   if (cat.kind == 0) {
     lt::Query q;
     if (!mat_query(rs, m, vtx, &q)) return zero;
     uint32_t leaf;
-    float leaf_pmf;
+    real leaf_pmf;
     if (!lt::sample_tree(rs.tree, q, u_tree, &leaf, &leaf_pmf)) return zero;
     const lt::Node& ln = rs.tree.nodes[leaf];
     // sample_triangle_point (R/scene/mod.rs:616-633)
     V3 p[3];
     triangle_world(rs, ln.inst, ln.tri, p);
-    float su0 = std::sqrt(clampf(us0, 0.0f, 1.0f));
-    float u1 = clampf(us1, 0.0f, 1.0f);
-    float b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
+    real su0 = std::sqrt(clampf(us0, 0.0f, 1.0f));
+    real u1 = clampf(us1, 0.0f, 1.0f);
+    real b0 = 1.0f - su0, b1 = u1 * su0, b2 = (1.0f - u1) * su0;
     V3 lp = interp3(b0, b1, b2, p[0], p[1], p[2]);
-    float area = 0.5f * lt::vlen(cross(sub(p[1], p[0]), sub(p[2], p[0])));
-    float pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
+    real area = 0.5f * lt::vlen(cross(sub(p[1], p[0]), sub(p[2], p[0])));
+    real pdf_area = area > 0.0f ? 1.0f / area : 0.0f;
     if (pdf_area <= 0.0f) return zero;
     V3 to_light = sub(lp, vtx.p);
-    float d2 = len_sq(to_light);
+    real d2 = len_sq(to_light);
     if (d2 <= 0.0f) return zero;
-    float dist = std::sqrt(d2);
+    real dist = std::sqrt(d2);
     wi = sdiv(to_light, dist);
     const Material& lm0 = rs.materials[rs.inst_material[ln.inst]];
     if (!mat_may_emit(lm0)) return zero;
     Vertex lv = vertex_at(rs, ln.inst, ln.tri, b0, b1, b2, wi);
     const Material lm = material_at(rs, lm0, lv);
     if (!mat_le(lm, lv, &le)) return zero;
-    float cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
+    real cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
     if (cos_light <= 0.0f) return zero;
     li_pdf = pdf_area * d2 / cos_light;
     selection = cat.pmf * leaf_pmf;
@@ -641,9 +645,9 @@ inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& v
   }
   if (li_pdf <= 0.0f || selection <= 0.0f) return zero;
   V3 f = mat_eval(m, vtx, wi);
-  float bsdf_pdf = len_sq(f) == 0.0f ? 0.0f : mat_pdf(m, vtx, wi);
+  real bsdf_pdf = len_sq(f) == 0.0f ? 0.0f : mat_pdf(m, vtx, wi);
   if (len_sq(f) == 0.0f) return zero;
-  float cos_surface = std::fabs(dot(vtx.ng, wi));
+  real cos_surface = std::fabs(dot(vtx.ng, wi));
   if (cos_surface <= 0.0f) return zero;
   // unoccluded_ray (R/integrator/mod.rs:239-265): a light triangle target, or
   // an infinite light that any hit blocks
@@ -651,45 +655,45 @@ inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& v
   if (closest_hit(*rs.geo, offset_origin(vtx.p, vtx.ng, wi), wi, INFINITY, &sh, nullptr) &&
       !(t_inst != lt::kInvalid && sh.inst == t_inst && sh.tri == t_tri))
     return zero;
-  float pdf_total = selection * li_pdf;
+  real pdf_total = selection * li_pdf;
   if (pdf_total <= 0.0f) return zero;
   if (cat.kind >= 2) return smul(vmul(le, f), cos_surface / pdf_total);  // delta light: no MIS
-  float mis = balance_heuristic(pdf_total, bsdf_pdf);
+  real mis = balance_heuristic(pdf_total, bsdf_pdf);
   return smul(vmul(le, f), cos_surface * mis / pdf_total);
 }
 
 // pdf_for_triangle_hit (R/light/mod.rs:407-429) with
 // Scene::area_light_pdf_solid_angle (R/scene/mod.rs:587-611).
-inline float light_pdf_for_hit(const RenderScene& rs, const Material& shading_m, const Vertex& vtx, const Vertex& lv) {
+inline real light_pdf_for_hit(const RenderScene& rs, const Material& shading_m, const Vertex& vtx, const Vertex& lv) {
   if (rs.tree.nodes.empty()) return 0.0f;
   lt::Query q;
   if (!mat_query(rs, shading_m, vtx, &q)) return 0.0f;
   uint32_t leaf = rs.leaf_node(lv.inst, lv.tri);
-  float leaf_pmf = lt::pdf_leaf(rs.tree, q, leaf);
+  real leaf_pmf = lt::pdf_leaf(rs.tree, q, leaf);
   if (leaf_pmf <= 0.0f) return 0.0f;
-  float cat_pmf = category_pmf(rs, 0);
-  const float area = rs.tree.nodes[leaf].area;
+  real cat_pmf = category_pmf(rs, 0);
+  const real area = rs.tree.nodes[leaf].area;
   if (area <= 0.0f) return 0.0f;
-  float pdf_area = 1.0f / area;
+  real pdf_area = 1.0f / area;
   V3 to_light = sub(lv.p, vtx.p);
-  float d2 = len_sq(to_light);
+  real d2 = len_sq(to_light);
   if (d2 <= 0.0f) return 0.0f;
-  float dist = std::sqrt(d2);
+  real dist = std::sqrt(d2);
   V3 wi = sdiv(to_light, dist);
-  float cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
+  real cos_light = std::fmax(dot(lv.ng, neg(wi)), 0.0f);
   if (cos_light <= 0.0f) return 0.0f;
   return cat_pmf * leaf_pmf * (pdf_area * d2 / cos_light);
 }
 
 // emitted_radiance_from_bsdf_sample_area (R/integrator/mis.rs:212-234)
-inline V3 emitted_from_bsdf_sample(const RenderScene& rs, V3 throughput, float bsdf_pdf, bool delta,
+inline V3 emitted_from_bsdf_sample(const RenderScene& rs, V3 throughput, real bsdf_pdf, bool delta,
                                    const Material& shading_m, const Vertex& vtx, const Vertex& lv,
                                    const Material& light_m) {
   V3 le;
   if (!mat_le(light_m, lv, &le)) return v3(0.f, 0.f, 0.f);
   if (delta) return vmul(throughput, le);
-  float light_pdf = light_pdf_for_hit(rs, shading_m, vtx, lv);
-  float mis = balance_heuristic(bsdf_pdf, light_pdf);
+  real light_pdf = light_pdf_for_hit(rs, shading_m, vtx, lv);
+  real mis = balance_heuristic(bsdf_pdf, light_pdf);
   return smul(vmul(throughput, le), mis);
 }
 
@@ -699,19 +703,20 @@ inline V3 env_le(const RenderScene& rs, V3 dir) {
 }
 // emitted_radiance_from_bsdf_sample_infinite (R/integrator/mis.rs:236-257)
 // with pdf_for_environment_hit (R/light/mod.rs:431-443), MIS-compensated.
-inline V3 emitted_infinite(const RenderScene& rs, V3 throughput, float bsdf_pdf, bool delta, V3 dir) {
+inline V3 emitted_infinite(const RenderScene& rs, V3 throughput, real bsdf_pdf, bool delta, V3 dir) {
   V3 le = env_le(rs, dir);
   if (le.x == 0.0f && le.y == 0.0f && le.z == 0.0f) return v3(0.f, 0.f, 0.f);
   if (delta) return vmul(throughput, le);
-  float cat = category_pmf(rs, 1);
-  float light_pdf = cat <= 0.0f ? 0.0f : cat * lt::env::pdf(rs.env, rs.env.dist_mis, dir);
+  real cat = category_pmf(rs, 1);
+  real light_pdf = cat <= 0.0f ? 0.0f : cat * lt::env::pdf(rs.env, rs.env.dist_mis, dir);
   return smul(vmul(throughput, le), balance_heuristic(bsdf_pdf, light_pdf));
 }
 
-// specular_ray_differential (R/integrator/mod.rs:88-138) for a reflection
-// (every delta lobe carried reflects) with reflected_differential_directions
-// (:140-157) and differential_is_reasonable (:208-219).
-inline bool specular_differential(const Ray& in, const Vertex& v, V3 wi_world, Ray* out) {
+// specular_ray_differential (R/integrator/mod.rs:88-138) with
+// reflected_differential_directions (:140-157) for reflections,
+// transmitted_differential_directions (:159-206) for refractions and
+// differential_is_reasonable (:208-219).
+inline bool specular_differential(const Ray& in, const Vertex& v, V3 wi_world, const BSample& s, Ray* out) {
   if (!in.has_diff) return false;
   V3 wo = v.wo, wi = normalize_or_zero3(wi_world);
   V3 dndx = add(mul(v.dudx, v.dndu), mul(v.dvdx, v.dndv));
@@ -719,11 +724,36 @@ inline bool specular_differential(const Ray& in, const Vertex& v, V3 wi_world, R
   V3 dwodx = sub(neg(in.rxd), wo), dwody = sub(neg(in.ryd), wo);
   V3 rxo = offset_origin(add(v.p, v.dpdx), v.ng, wi), ryo = offset_origin(add(v.p, v.dpdy), v.ng, wi);
   V3 n = v.ns;
-  float dwn_dx = dot(dwodx, n) + dot(wo, dndx), dwn_dy = dot(dwody, n) + dot(wo, dndy);
-  float wn = dot(wo, n);
-  V3 rxd = add(sub(wi, dwodx), mul(2.0f, add(mul(wn, dndx), mul(dwn_dx, n))));
-  V3 ryd = add(sub(wi, dwody), mul(2.0f, add(mul(wn, dndy), mul(dwn_dy, n))));
-  const float kMax = 1.0e16f;
+  V3 rxd, ryd;
+  if (!s.transmission) {
+    real dwn_dx = dot(dwodx, n) + dot(wo, dndx), dwn_dy = dot(dwody, n) + dot(wo, dndy);
+    real wn = dot(wo, n);
+    rxd = add(sub(wi, dwodx), mul(2.0f, add(mul(wn, dndx), mul(dwn_dx, n))));
+    ryd = add(sub(wi, dwody), mul(2.0f, add(mul(wn, dndy), mul(dwn_dy, n))));
+  } else {
+    const real eta = s.eta;
+    if (std::fabs(eta - 1.0f) <= 1.0e-6f) {
+      rxd = sub(wi, dwodx);
+      ryd = sub(wi, dwody);
+    } else {
+      if (eta <= 0.0f) return false;
+      if (dot(wo, n) < 0.0f) {
+        n = neg(n);
+        dndx = neg(dndx);
+        dndy = neg(dndy);
+      }
+      real win = dot(wi, n);
+      if (std::fabs(win) <= 1.0e-6f) return false;
+      real dwn_dx = dot(dwodx, n) + dot(wo, dndx), dwn_dy = dot(dwody, n) + dot(wo, dndy);
+      real wn = dot(wo, n);
+      real mu = wn * eta - std::fabs(win);
+      real dmu_scale = eta + eta * eta * wn / win;
+      real dmudx = dwn_dx * dmu_scale, dmudy = dwn_dy * dmu_scale;
+      rxd = add(add(sub(wi, mul(eta, dwodx)), mul(mu, dndx)), mul(dmudx, n));
+      ryd = add(add(sub(wi, mul(eta, dwody)), mul(mu, dndy)), mul(dmudy, n));
+    }
+  }
+  const real kMax = 1.0e16f;
   if (!v3_finite(rxo) || !v3_finite(ryo) || !v3_finite(rxd) || !v3_finite(ryd) || len_sq(rxo) > kMax ||
       len_sq(ryo) > kMax || len_sq(rxd) > kMax || len_sq(ryd) > kMax)
     return false;
@@ -734,12 +764,12 @@ inline bool specular_differential(const Ray& in, const Vertex& v, V3 wi_world, R
   return true;
 }
 // spawn_scattered_ray (R/integrator/mod.rs:58-75)
-inline Ray spawn_scattered(const Ray& in, float hit_t, const Vertex& v, const BSample& s, V3 wi) {
+inline Ray spawn_scattered(const Ray& in, real hit_t, const Vertex& v, const BSample& s, V3 wi) {
   Ray r;
   r.o = offset_origin(v.p, v.ng, wi);
   r.d = wi;
   r.cone = make_cone(cone_width_at(in.cone, hit_t), in.cone.spread + s.cone_spread);
-  if (s.delta) r.has_diff = specular_differential(in, v, wi, &r);
+  if (s.delta) r.has_diff = specular_differential(in, v, wi, s, &r);
   return r;
 }
 
@@ -750,7 +780,7 @@ inline V3 trace_radiance(const RenderScene& rs, const Ray& camera_ray, const Pat
   SceneHit h;
   if (!closest_hit(*rs.geo, camera_ray.o, camera_ray.d, INFINITY, &h, nullptr)) return env_le(rs, camera_ray.d);
   Ray ray = camera_ray;
-  float hit_t = h.t;
+  real hit_t = h.t;
   Vertex vtx = shading_vertex(rs, h, ray);
   Material mat = material_at(rs, rs.materials[vtx.material], vtx);
   V3 le;
@@ -772,7 +802,7 @@ inline V3 trace_radiance(const RenderScene& rs, const Ray& camera_ray, const Pat
     const bool delta = s.delta;
     throughput = vmul(throughput, s.weight);
     if (depth + 1 >= rr_start) {
-      float q = clampf(rmax(throughput.x, rmax(throughput.y, throughput.z)), 0.05f, 1.0f);
+      real q = clampf(rmax(throughput.x, rmax(throughput.y, throughput.z)), 0.05f, 1.0f);
       if (ps.s1(base + 13) > q) break;
       throughput = sdiv(throughput, q);
     }

@@ -77,12 +77,14 @@ void orc_qbvh_export(void* h, float* bounds, uint32_t* children, uint8_t* valid,
   Qbvh* q = (Qbvh*)h;
   for (size_t i = 0; i < q->nodes.size(); ++i) {
     const Node& n = q->nodes[i];
-    std::memcpy(bounds + 24 * i + 0, n.min_x, 16);
-    std::memcpy(bounds + 24 * i + 4, n.min_y, 16);
-    std::memcpy(bounds + 24 * i + 8, n.min_z, 16);
-    std::memcpy(bounds + 24 * i + 12, n.max_x, 16);
-    std::memcpy(bounds + 24 * i + 16, n.max_y, 16);
-    std::memcpy(bounds + 24 * i + 20, n.max_z, 16);
+    for (int k = 0; k < 4; ++k) {
+      bounds[24 * i + 0 + k] = n.min_x[k];
+      bounds[24 * i + 4 + k] = n.min_y[k];
+      bounds[24 * i + 8 + k] = n.min_z[k];
+      bounds[24 * i + 12 + k] = n.max_x[k];
+      bounds[24 * i + 16 + k] = n.max_y[k];
+      bounds[24 * i + 20 + k] = n.max_z[k];
+    }
     std::memcpy(children + 4 * i, n.children, 16);
     valid[i] = n.valid_mask;
   }
@@ -173,9 +175,11 @@ int orc_scene_add_instance(void* h, uint32_t mesh, const float* l2w, const float
   if (mesh >= s->meshes.size()) return -1;
   Instance in;
   in.mesh = mesh;
-  std::memcpy(in.l2w, l2w, 64);
-  std::memcpy(in.w2l, w2l, 64);
-  in.world_bounds = transform_bounds(s->meshes[mesh].bounds, l2w);
+  for (int k = 0; k < 16; ++k) {
+    in.l2w[k] = l2w[k];
+    in.w2l[k] = w2l[k];
+  }
+  in.world_bounds = transform_bounds(s->meshes[mesh].bounds, in.l2w);
   s->inst.push_back(in);
   s->built = false;
   return (int)s->inst.size() - 1;
@@ -218,7 +222,7 @@ int orc_closest_hit(void* h, const OrcRay* rays, size_t n, OrcHit* hits, uint32_
     if (ok) {
       hits[i] = OrcHit{sh.inst, sh.tri, sh.t, sh.b0, sh.b1, sh.b2};
     } else {
-      hits[i] = OrcHit{~0u, ~0u, kInf, 0.f, 0.f, 0.f};
+      hits[i] = OrcHit{~0u, ~0u, (float)kInf, 0.f, 0.f, 0.f};
     }
     if (stats) {
       stats[3 * i] = (uint32_t)st.nodes;
@@ -373,7 +377,7 @@ extern "C" {
 // then the kind's parameters (TptOpenPbr for OpenPBR).
 struct OrcMaterial {
   uint32_t kind, pad;
-  float raw[40];
+  float raw[48];
 };
 static OpenPbr parse_open_pbr(const OrcMaterial* m);
 
@@ -469,7 +473,12 @@ int orc_env_sample(void* h, float u, float v, int mis, float* out) {
   return 1;
 }
 float orc_env_total_power(void* h) { return lt::env::total_power(((RenderScene*)h)->env); }
-void orc_env_uv(const float* d, float* uv) { lt::env::direction_to_uv(v3(d[0], d[1], d[2]), &uv[0], &uv[1]); }
+void orc_env_uv(const float* d, float* uv) {
+  real u, v;
+  lt::env::direction_to_uv(v3(d[0], d[1], d[2]), &u, &v);
+  uv[0] = u;
+  uv[1] = v;
+}
 void orc_env_dir(float u, float v, float* d) {
   V3 r = lt::env::uv_to_direction(u, v);
   d[0] = r.x;
@@ -506,7 +515,10 @@ uint32_t orc_render_light_tree(void* h, float* out, uint32_t cap) {
 // descent (leaf node + pmf, false if none) and the reverse leaf pmf.
 int orc_render_tree_sample(void* h, const float* p, const float* n, float rho, float u, uint32_t* leaf, float* pmf) {
   lt::Query q{v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]), rho};
-  return lt::sample_tree(((RenderScene*)h)->tree, q, u, leaf, pmf) ? 1 : 0;
+  real pm = 0;
+  const bool ok = lt::sample_tree(((RenderScene*)h)->tree, q, u, leaf, &pm);
+  *pmf = (float)pm;
+  return ok ? 1 : 0;
 }
 float orc_render_tree_pdf(void* h, const float* p, const float* n, float rho, uint32_t leaf) {
   lt::Query q{v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]), rho};
@@ -525,7 +537,10 @@ int orc_render_vertex_tree_sample(void* h, uint32_t inst, uint32_t tri, const fl
   Vertex v = vertex_at(rs, inst, tri, b[0], b[1], b[2], v3(dir[0], dir[1], dir[2]));
   lt::Query q;
   if (!mat_query(rs, material_at(rs, rs.materials[v.material], v), v, &q)) return 0;
-  return lt::sample_tree(rs.tree, q, u, leaf, pmf) ? 1 : 0;
+  real pm = 0;
+  const bool ok = lt::sample_tree(rs.tree, q, u, leaf, &pm);
+  *pmf = (float)pm;
+  return ok ? 1 : 0;
 }
 float orc_render_vertex_tree_pdf(void* h, uint32_t inst, uint32_t tri, const float* b, const float* dir,
                                  uint32_t leaf) {
@@ -599,7 +614,10 @@ float orc_sggx_reflection_pdf(const float* wi, const float* h, const float* m) {
 }
 // merge_glossy_roughness: out = (alpha_x, alpha_y)
 void orc_merge_glossy_roughness(float ra, float axa, float aya, float rb, float axb, float ayb, float* out) {
-  lt::merge_glossy_roughness(ra, axa, aya, rb, axb, ayb, &out[0], &out[1]);
+  real ax, ay;
+  lt::merge_glossy_roughness(ra, axa, aya, rb, axb, ayb, &ax, &ay);
+  out[0] = ax;
+  out[1] = ay;
 }
 // make_glossy_lobe(rho, frame = (t, b, n) in frame[9], wo, ax, ay), then
 // glossy_importance against the SG light (w, axis, sharpness) with normal n;
@@ -625,15 +643,15 @@ void orc_sg_product(const float* a1, float s1, const float* a2, float s2, float*
   out[4] = l.log_amplitude;
 }
 
-// The pixel loop of R/main.rs:98-126 over rows [y0, y1): PathSampler,
-// camera ray, trace_radiance, running mean, exposure.  rgb: 3 floats per
-// pixel of the full frame (only the rows rendered are written).
+}  // extern "C"
+// The pixel loop of R/main.rs:98-126 (PathSampler, camera ray,
+// trace_radiance, running mean, exposure) over pixels [x0, x1) x [y0, y1) of
+// a w x hgt frame (every pixel's samples depend on the full frame size only),
+// written row-major into a (y1-y0) x (x1-x0) crop of 3 values per pixel.
 // cam: eye, forward, right, image_up, tan_half_fov_y, pixel_spread, exposure.
-// Pixels [x0, x1) x [y0, y1) of a w x hgt frame (the pixel loop of
-// R/main.rs:98-126 restricted to a rectangle; every pixel's samples depend on
-// the full frame size only), written row-major into a (y1-y0) x (x1-x0) crop.
-void orc_render_rect(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth,
-                     uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, float* rgb, int threads) {
+template <class Out>
+static void render_rect(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth,
+                        uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, Out* rgb, int threads) {
   const RenderScene& rs = *(RenderScene*)h;
   CF3 eye{cam[0], cam[1], cam[2]}, fwd{cam[3], cam[4], cam[5]}, right{cam[6], cam[7], cam[8]},
       up{cam[9], cam[10], cam[11]};
@@ -674,11 +692,22 @@ void orc_render_rect(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32
       const float cnt = (float)(si + 1);
       color = add(color, sdiv(sub(s, color), cnt));
     }
-    float* o = rgb + 3 * ((size_t)(y - y0) * cw + (x - x0));
+    Out* o = rgb + 3 * ((size_t)(y - y0) * cw + (x - x0));
     o[0] = color.x * exposure;
     o[1] = color.y * exposure;
     o[2] = color.z * exposure;
   });
+}
+extern "C" {
+void orc_render_rect(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth,
+                     uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, float* rgb, int threads) {
+  render_rect(h, cam, w, hgt, spp, max_depth, x0, x1, y0, y1, rgb, threads);
+}
+// The same with the running means written unrounded (double): the FP64
+// build's image for the gradient reference.
+void orc_render_rect_f64(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth,
+                         uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, double* rgb, int threads) {
+  render_rect(h, cam, w, hgt, spp, max_depth, x0, x1, y0, y1, rgb, threads);
 }
 
 // Rows [y0, y1) of the frame into the full-frame buffer rgb (w * hgt * 3).
@@ -720,6 +749,11 @@ static OpenPbr parse_open_pbr(const OrcMaterial* m) {
   o.fuzz_color = v3(f[29], f[30], f[31]);
   o.fuzz_roughness = f[32];
   std::memcpy(o.tex, &f[33], sizeof(o.tex));
+  o.transmission_color = v3(f[39], f[40], f[41]);
+  o.transmission_dispersion_scale = f[42];
+  o.transmission_dispersion_abbe_number = f[43];
+  o.thin_film_thickness = f[44];
+  o.thin_film_ior = f[45];
   return o;
 }
 // Table sizes: albedo (per eta), conductor e / e_avg, dielectric e / e_avg.
@@ -730,19 +764,19 @@ void orc_lut_sizes(uint32_t* out) {
   out[3] = kDielCos * kDielRough * kDielEta;
   out[4] = kDielRough * kDielEta;
 }
-void orc_lut_albedo(float eta, float* out) {
-  const AlbedoLut& l = albedo_lut_for(eta);
-  std::memcpy(out, l.v.data(), 4 * l.v.size());
+static void copy_out(const std::vector<real>& v, float* out) {
+  for (size_t i = 0; i < v.size(); ++i) out[i] = (float)v[i];
 }
+void orc_lut_albedo(float eta, float* out) { copy_out(albedo_lut_for(eta).v, out); }
 void orc_lut_cond(float* e, float* eavg) {
   const CondLut& l = cond_lut();
-  std::memcpy(e, l.e.data(), 4 * l.e.size());
-  std::memcpy(eavg, l.eavg.data(), 4 * l.eavg.size());
+  copy_out(l.e, e);
+  copy_out(l.eavg, eavg);
 }
 void orc_lut_diel(float* e, float* eavg) {
   const DielLut& l = diel_lut();
-  std::memcpy(e, l.e.data(), 4 * l.e.size());
-  std::memcpy(eavg, l.eavg.data(), 4 * l.eavg.size());
+  copy_out(l.e, e);
+  copy_out(l.eavg, eavg);
 }
 // which: 0 albedo(eta; w, roughness), 1 conductor e(cos, r), 2 conductor e_avg(r),
 // 3 dielectric e(cos, r, eta), 4 dielectric e_avg(r, eta)
@@ -773,7 +807,7 @@ void orc_openpbr_eval(const OrcMaterial* m, int front_face, const float* wo, con
   }
 }
 // OpenPbrBsdf::sample for n (u_lobe, u0, u1) triples; out per sample:
-// ok, weight[3], wi[3], pdf, delta (9 floats).
+// ok, weight[3], wi[3], pdf, delta, transmission, eta (11 floats).
 void orc_openpbr_sample(const OrcMaterial* m, int front_face, const float* wo, const float* u, size_t n, float* out) {
   const OpenPbr o = parse_open_pbr(m);
   const V3 w = v3(wo[0], wo[1], wo[2]);
@@ -781,7 +815,7 @@ void orc_openpbr_sample(const OrcMaterial* m, int front_face, const float* wo, c
   for (size_t i = 0; i < n; ++i) {
     BSample s{};
     const bool ok = bsdf_sample(p, w, u[3 * i], u[3 * i + 1], u[3 * i + 2], &s);
-    float* r = out + 9 * i;
+    float* r = out + 11 * i;
     r[0] = ok ? 1.0f : 0.0f;
     r[1] = ok ? s.weight.x : 0.0f;
     r[2] = ok ? s.weight.y : 0.0f;
@@ -791,12 +825,15 @@ void orc_openpbr_sample(const OrcMaterial* m, int front_face, const float* wo, c
     r[6] = ok ? s.wi.z : 0.0f;
     r[7] = ok ? s.pdf : 0.0f;
     r[8] = ok && s.delta ? 1.0f : 0.0f;
+    r[9] = ok && s.transmission ? 1.0f : 0.0f;
+    r[10] = ok ? s.eta : 0.0f;
   }
 }
 // Pieces of the BSDF the reference's own tests check: 0 metal Fresnel at cos
 // (3), 1 metal Fresnel average (3), 2 emission exit tint at wo (3), 3 coat
 // darkening factor (3), 4 dielectric ms params (eta_o, ratio_r, ome_o) for
-// (alpha = a, eta = b).
+// (alpha = a, eta = b), 5 / 6 thin-wall reflectance / transmittance at
+// cos_o = a (3), 7 thin-wall transmission alphas (2).
 void orc_openpbr_probe(const OrcMaterial* m, int which, const float* wo, float a, float b, float* out) {
   const OpenPbr o = parse_open_pbr(m);
   const V3 w = v3(wo[0], wo[1], wo[2]);
@@ -811,6 +848,16 @@ void orc_openpbr_probe(const OrcMaterial* m, int which, const float* wo, float a
     q.luts.diel = &diel_lut();
     const MsParams s = ms_params(q, a, a, b);
     r = v3(s.eta_o, s.ratio_r, s.ome_o);
+  }
+  if (which == 5 || which == 6) {
+    V3 rr, tt;
+    thin_wall_coefficients(p, a, &rr, &tt);
+    r = which == 5 ? rr : tt;
+  }
+  if (which == 7) {
+    real ax, ay;
+    thin_wall_alpha(p, &ax, &ay);
+    r = v3(ax, ay, 0.0f);
   }
   out[0] = r.x;
   out[1] = r.y;

@@ -1,5 +1,9 @@
 """ctypes wrapper of the oracle (oracle/liboracle.so), the CPU restatement of
 the reference ray-query path.  Test infrastructure: used only as the checker.
+
+liboracle64.so is the same restatement compiled with every arithmetic value
+widened to double (ORC_FP64): the FP64 reference renderer of the gradient
+checks (OracleScene / OracleRenderer with fp64=True).
 """
 from __future__ import annotations
 
@@ -13,16 +17,18 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent.parent
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+ORACLE_LIB64 = ORACLE_DIR / "liboracle64.so"
 
-_lib = None
+_libs = {}
 
 
-def lib() -> ctypes.CDLL:
-    global _lib
-    if _lib is None:
-        if not ORACLE_LIB.exists():
+def lib(fp64: bool = False) -> ctypes.CDLL:
+    """The oracle library: the f32 restatement, or with fp64 its FP64 build."""
+    if fp64 not in _libs:
+        path = ORACLE_LIB64 if fp64 else ORACLE_LIB
+        if not path.exists():
             subprocess.run(["make", "-C", os.fspath(ORACLE_DIR)], check=True, capture_output=True)
-        L = ctypes.CDLL(os.fspath(ORACLE_LIB))
+        L = ctypes.CDLL(os.fspath(path))
         vp, sz, f32p = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p
         L.orc_intersect_triangle.argtypes = [f32p, ctypes.c_float, f32p, f32p]
         L.orc_intersect_triangle.restype = ctypes.c_int
@@ -127,8 +133,9 @@ def lib() -> ctypes.CDLL:
         L.orc_closest_hit.restype = ctypes.c_int
         L.orc_unoccluded.argtypes = [vp, vp, sz, vp, vp, vp, vp, ctypes.c_int]
         L.orc_unoccluded.restype = ctypes.c_int
-        _lib = L
-    return _lib
+        L.orc_render_rect_f64.argtypes = [vp, vp] + [ctypes.c_uint32] * 8 + [vp, ctypes.c_int]
+        _libs[fp64] = L
+    return _libs[fp64]
 
 
 def env_uv(d):
@@ -201,13 +208,14 @@ def openpbr_eval(material, wo, wi, front_face=True):
 
 def openpbr_sample(material, wo, u, front_face=True):
     """OpenPbrBsdf sample for rows (u_lobe, u0, u1) -> dict of ok, weight,
-    wi, pdf, delta."""
+    wi, pdf, delta, transmission, eta."""
     m = _mat_buf(material)
     woa = np.asarray(wo, np.float32)
     ua = np.ascontiguousarray(np.asarray(u, np.float32).reshape(-1, 3))
-    out = np.zeros((len(ua), 9), np.float32)
+    out = np.zeros((len(ua), 11), np.float32)
     lib().orc_openpbr_sample(_p(m), int(front_face), _p(woa), _p(ua), len(ua), _p(out))
-    return dict(ok=out[:, 0] > 0, weight=out[:, 1:4], wi=out[:, 4:7], pdf=out[:, 7], delta=out[:, 8] > 0)
+    return dict(ok=out[:, 0] > 0, weight=out[:, 1:4], wi=out[:, 4:7], pdf=out[:, 7], delta=out[:, 8] > 0,
+                transmission=out[:, 9] > 0, eta=out[:, 10])
 
 
 def openpbr_probe(material, which, wo=(0.0, 0.0, 1.0), a=0.0, b=0.0):
@@ -320,36 +328,38 @@ class OracleQbvh:
 
 
 class OracleScene:
-    """Oracle scene: meshes + instances (math-convention 4x4 matrices)."""
+    """Oracle scene: meshes + instances (math-convention 4x4 matrices); fp64:
+    in the FP64 build of the oracle."""
 
-    def __init__(self):
-        self.h = lib().orc_scene_new()
+    def __init__(self, fp64: bool = False):
+        self._L = lib(fp64)
+        self.h = self._L.orc_scene_new()
         self._keep = []
 
     def add_mesh(self, positions, indices) -> int:
         p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
-        return lib().orc_scene_add_mesh(self.h, _p(p), len(p), _p(i), len(i))
+        return self._L.orc_scene_add_mesh(self.h, _p(p), len(p), _p(i), len(i))
 
     def add_instance(self, mesh, local_to_world, world_to_local) -> int:
         l2w = np.ascontiguousarray(np.asarray(local_to_world, np.float32).T).reshape(16)
         w2l = np.ascontiguousarray(np.asarray(world_to_local, np.float32).T).reshape(16)
-        return lib().orc_scene_add_instance(self.h, mesh, _p(l2w), _p(w2l))
+        return self._L.orc_scene_add_instance(self.h, mesh, _p(l2w), _p(w2l))
 
     def build(self):
-        lib().orc_scene_build(self.h)
+        self._L.orc_scene_build(self.h)
 
     def mesh_qbvh(self, mesh: int):
-        return _export(lib().orc_scene_mesh_qbvh(self.h, mesh))
+        return _export(self._L.orc_scene_mesh_qbvh(self.h, mesh))
 
     def tlas(self):
-        return _export(lib().orc_scene_tlas(self.h))
+        return _export(self._L.orc_scene_tlas(self.h))
 
     def closest_hit(self, rays, threads=8, stats=False):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.zeros(len(rays), HIT_DTYPE)
         st = np.zeros((len(rays), 3), np.uint32) if stats else None
-        rc = lib().orc_closest_hit(self.h, _p(rays), len(rays), _p(hits), _p(st), threads)
+        rc = self._L.orc_closest_hit(self.h, _p(rays), len(rays), _p(hits), _p(st), threads)
         if rc != 0:
             raise RuntimeError("QbvhNotBuilt")
         return (hits, st) if stats else hits
@@ -360,19 +370,19 @@ class OracleScene:
         tg = None if target is None else np.ascontiguousarray(target, np.uint32).reshape(-1, 2)
         ds = None if distance is None else np.ascontiguousarray(distance, np.float32)
         st = np.zeros((len(rays), 3), np.uint32) if stats else None
-        rc = lib().orc_unoccluded(self.h, _p(rays), len(rays), _p(tg), _p(ds), _p(out), _p(st), threads)
+        rc = self._L.orc_unoccluded(self.h, _p(rays), len(rays), _p(tg), _p(ds), _p(out), _p(st), threads)
         if rc != 0:
             raise RuntimeError("QbvhNotBuilt")
         return (out.astype(bool), st) if stats else out.astype(bool)
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_scene_free(self.h)
+            self._L.orc_scene_free(self.h)
 
 
-def oracle_scene_from(scene) -> OracleScene:
+def oracle_scene_from(scene, fp64: bool = False) -> OracleScene:
     """An oracle copy of a tpt.Scene (same f32 arrays and matrices), built."""
-    o = OracleScene()
+    o = OracleScene(fp64)
     for m in scene.meshes:
         o.add_mesh(m.positions, m.indices)
     for inst in scene.instances:
@@ -383,88 +393,90 @@ def oracle_scene_from(scene) -> OracleScene:
 
 class OracleRenderer:
     """The oracle's MIS path tracer over a tpt.Scene and its materials
-    (tpt.material records): light tree built from the emissive instances."""
+    (tpt.material records): light tree built from the emissive instances.
+    fp64: the FP64 build (render_rect_f64 returns its image in float64)."""
 
-    def __init__(self, scene, materials, nee_policy: str = "reference"):
+    def __init__(self, scene, materials, nee_policy: str = "reference", fp64: bool = False):
         from tpt.material import records
-        self.geo = oracle_scene_from(scene)
-        self.h = lib().orc_render_scene_new(self.geo.h, len(scene.meshes))
+        self._L = lib(fp64)
+        self.geo = oracle_scene_from(scene, fp64)
+        self.h = self._L.orc_render_scene_new(self.geo.h, len(scene.meshes))
         self._keep = []
         for i, m in enumerate(scene.meshes):
             n = None if m.normals is None else np.ascontiguousarray(m.normals, np.float32)
             uv = None if m.uvs is None else np.ascontiguousarray(m.uvs, np.float32)
             self._keep += [n, uv]
-            lib().orc_render_set_mesh(self.h, i, _p(n), _p(uv), len(m.positions))
+            self._L.orc_render_set_mesh(self.h, i, _p(n), _p(uv), len(m.positions))
         for i, inst in enumerate(scene.instances):
-            lib().orc_render_set_instance_material(self.h, i, inst.material_index)
+            self._L.orc_render_set_instance_material(self.h, i, inst.material_index)
         self.scene_meshes = scene.meshes
         for t in getattr(scene, "textures", []):  # before the materials: the light tree reads them
             px = np.ascontiguousarray(t.pixels, np.float32)
-            lib().orc_render_add_texture(self.h, px.shape[1], px.shape[0], 3 if px.ndim == 3 else 1, _p(px))
+            self._L.orc_render_add_texture(self.h, px.shape[1], px.shape[0], 3 if px.ndim == 3 else 1, _p(px))
         self.recs = records(materials)
-        lib().orc_render_set_materials(self.h, ctypes.addressof(self.recs), len(materials))
+        self._L.orc_render_set_materials(self.h, ctypes.addressof(self.recs), len(materials))
         for dl in getattr(scene, "directional_lights", []):
             d, c = np.asarray(dl.direction, np.float32), np.asarray(dl.color, np.float32)
-            lib().orc_render_add_directional(self.h, _p(d), _p(c), dl.intensity)
+            self._L.orc_render_add_directional(self.h, _p(d), _p(c), dl.intensity)
         env = getattr(scene, "environment", None)
         if env is not None:
             px = np.ascontiguousarray(env.pixels, np.float32)
-            lib().orc_render_set_environment(self.h, _p(px), px.shape[1], px.shape[0], env.scale, env.rotate_y)
+            self._L.orc_render_set_environment(self.h, _p(px), px.shape[1], px.shape[0], env.scale, env.rotate_y)
         self.set_nee_policy(nee_policy)
 
     def set_nee_policy(self, policy: str) -> None:
         """"reference" (OpenPBR builds no light-tree query) or
         "openpbr_proxy" (the OpenPBR proxy query)."""
-        lib().orc_render_set_nee_policy(self.h, {"reference": 0, "openpbr_proxy": 1}[policy])
+        self._L.orc_render_set_nee_policy(self.h, {"reference": 0, "openpbr_proxy": 1}[policy])
 
     def vertex_tree_sample(self, inst, tri, bary, direction, u):
         """The stochastic descent with the query of a shading vertex under the
         renderer's policy: (leaf node, pmf) or None."""
         leaf, pmf = ctypes.c_uint32(0), ctypes.c_float(0)
         b, d = np.asarray(bary, np.float32), np.asarray(direction, np.float32)
-        ok = lib().orc_render_vertex_tree_sample(self.h, inst, tri, _p(b), _p(d), u, ctypes.byref(leaf),
+        ok = self._L.orc_render_vertex_tree_sample(self.h, inst, tri, _p(b), _p(d), u, ctypes.byref(leaf),
                                                  ctypes.byref(pmf))
         return (leaf.value, pmf.value) if ok else None
 
     def vertex_tree_pdf(self, inst, tri, bary, direction, leaf):
         b, d = np.asarray(bary, np.float32), np.asarray(direction, np.float32)
-        return lib().orc_render_vertex_tree_pdf(self.h, inst, tri, _p(b), _p(d), leaf)
+        return self._L.orc_render_vertex_tree_pdf(self.h, inst, tri, _p(b), _p(d), leaf)
 
     def light_categories(self):
         """The light sampler's categories: [(kind, pmf)], kind 0 tree, 1
         environment, 2 + i directional light i."""
         kind, pmf = np.zeros(32, np.int32), np.zeros(32, np.float32)
-        n = lib().orc_render_light_categories(self.h, _p(kind), _p(pmf), 32)
+        n = self._L.orc_render_light_categories(self.h, _p(kind), _p(pmf), 32)
         return [(int(kind[i]), float(pmf[i])) for i in range(min(n, 32))]
 
     # environment light queries (R/light/environment.rs)
     def env_pdf(self, d, mis=False):
         da = np.asarray(d, np.float32)
-        return lib().orc_env_pdf(self.h, _p(da), int(mis))
+        return self._L.orc_env_pdf(self.h, _p(da), int(mis))
 
     def env_radiance(self, d):
         da, out = np.asarray(d, np.float32), np.zeros(3, np.float32)
-        lib().orc_env_radiance(self.h, _p(da), _p(out))
+        self._L.orc_env_radiance(self.h, _p(da), _p(out))
         return out
 
     def env_sample(self, u, v, mis=False):
         out = np.zeros(7, np.float32)
-        ok = lib().orc_env_sample(self.h, u, v, int(mis), _p(out))
+        ok = self._L.orc_env_sample(self.h, u, v, int(mis), _p(out))
         return (out[:3], out[3:6], float(out[6])) if ok else None
 
     def env_total_power(self):
-        return lib().orc_env_total_power(self.h)
+        return self._L.orc_env_total_power(self.h)
 
     def emitted_infinite(self, throughput, bsdf_pdf, delta, direction):
         t, d, out = np.asarray(throughput, np.float32), np.asarray(direction, np.float32), np.zeros(3, np.float32)
-        lib().orc_render_emitted_infinite(self.h, _p(t), bsdf_pdf, int(delta), _p(d), _p(out))
+        self._L.orc_render_emitted_infinite(self.h, _p(t), bsdf_pdf, int(delta), _p(d), _p(out))
         return out
 
     def light_tree(self):
         """Light-tree nodes: structured array (see orc_render_light_tree)."""
         cap = 2 * sum(len(m.indices) for m in self.scene_meshes) + 1
         raw = np.zeros((cap, 20), np.float32)
-        n = lib().orc_render_light_tree(self.h, _p(raw), cap)
+        n = self._L.orc_render_light_tree(self.h, _p(raw), cap)
         raw = raw[:n]
         u = raw.view(np.uint32)
         return dict(flux=raw[:, 0], mu=raw[:, 1:4], sigma_s2=raw[:, 4], nu=raw[:, 5:8], **{"lambda": raw[:, 8]},
@@ -476,17 +488,17 @@ class OracleRenderer:
         leaf = ctypes.c_uint32(0)
         pmf = ctypes.c_float(0)
         pa, na = np.asarray(p, np.float32), np.asarray(n, np.float32)
-        ok = lib().orc_render_tree_sample(self.h, _p(pa), _p(na), rho, u, ctypes.byref(leaf), ctypes.byref(pmf))
+        ok = self._L.orc_render_tree_sample(self.h, _p(pa), _p(na), rho, u, ctypes.byref(leaf), ctypes.byref(pmf))
         return (leaf.value, pmf.value) if ok else None
 
     def tree_pdf(self, p, n, rho, leaf):
         pa, na = np.asarray(p, np.float32), np.asarray(n, np.float32)
-        return lib().orc_render_tree_pdf(self.h, _p(pa), _p(na), rho, leaf)
+        return self._L.orc_render_tree_pdf(self.h, _p(pa), _p(na), rho, leaf)
 
     def direct_light(self, inst, tri, bary, direction, u_root, u_tree, u_aux, us):
         out = np.zeros(3, np.float32)
         b, d, u = (np.asarray(a, np.float32) for a in (bary, direction, us))
-        lib().orc_render_direct_light(self.h, inst, tri, _p(b), _p(d), u_root, u_tree, u_aux, _p(u), _p(out))
+        self._L.orc_render_direct_light(self.h, inst, tri, _p(b), _p(d), u_root, u_tree, u_aux, _p(u), _p(out))
         return out
 
     def emitted(self, vertex, light_vertex, throughput, bsdf_pdf, delta):
@@ -494,14 +506,14 @@ class OracleRenderer:
         (i, t, b, d), (li, lt_, lb, ld) = vertex, light_vertex
         b, d, lb, ld, th = (np.asarray(a, np.float32) for a in (b, d, lb, ld, throughput))
         out = np.zeros(3, np.float32)
-        lib().orc_render_emitted(self.h, i, t, _p(b), _p(d), li, lt_, _p(lb), _p(ld), _p(th), bsdf_pdf, int(delta),
+        self._L.orc_render_emitted(self.h, i, t, _p(b), _p(d), li, lt_, _p(lb), _p(ld), _p(th), bsdf_pdf, int(delta),
                                  _p(out))
         return out
 
     def trace(self, origin, direction, frame, spp, pixel, sample_index, max_depth):
         out = np.zeros(3, np.float32)
         o, d = np.asarray(origin, np.float32), np.asarray(direction, np.float32)
-        lib().orc_render_trace(self.h, _p(o), _p(d), frame[0], frame[1], spp, pixel[0], pixel[1], sample_index,
+        self._L.orc_render_trace(self.h, _p(o), _p(d), frame[0], frame[1], spp, pixel[0], pixel[1], sample_index,
                                max_depth, _p(out))
         return out
 
@@ -511,7 +523,7 @@ class OracleRenderer:
                      + [cam.tan_half_fov_y, cam.pixel_spread, cam.exposure], np.float32)
         out = np.zeros((height, width, 3), np.float32)
         y0, y1 = rows if rows else (0, height)
-        lib().orc_render(self.h, _p(c), width, height, spp, max_depth, y0, y1, _p(out), threads)
+        self._L.orc_render(self.h, _p(c), width, height, spp, max_depth, y0, y1, _p(out), threads)
         return out
 
     def render_rect(self, cam, width, height, spp, max_depth, x0, x1, y0, y1, threads=8):
@@ -519,9 +531,18 @@ class OracleRenderer:
         c = np.array(list(cam.eye) + list(cam.forward) + list(cam.right) + list(cam.image_up)
                      + [cam.tan_half_fov_y, cam.pixel_spread, cam.exposure], np.float32)
         out = np.zeros((y1 - y0, x1 - x0, 3), np.float32)
-        lib().orc_render_rect(self.h, _p(c), width, height, spp, max_depth, x0, x1, y0, y1, _p(out), threads)
+        self._L.orc_render_rect(self.h, _p(c), width, height, spp, max_depth, x0, x1, y0, y1, _p(out), threads)
+        return out
+
+    def render_rect_f64(self, cam, width, height, spp, max_depth, x0, x1, y0, y1, threads=8):
+        """render_rect with the pixels' running means returned unrounded
+        (float64; the FP64 build accumulates in double)."""
+        c = np.array(list(cam.eye) + list(cam.forward) + list(cam.right) + list(cam.image_up)
+                     + [cam.tan_half_fov_y, cam.pixel_spread, cam.exposure], np.float32)
+        out = np.zeros((y1 - y0, x1 - x0, 3), np.float64)
+        self._L.orc_render_rect_f64(self.h, _p(c), width, height, spp, max_depth, x0, x1, y0, y1, _p(out), threads)
         return out
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_render_scene_free(self.h)
+            self._L.orc_render_scene_free(self.h)

@@ -1,7 +1,7 @@
 """OpenPBR BSDF on the GPU (tpt_material_bsdf) against the oracle's
 restatement of the reference (oracle/bsdf.h), query by query: eval, pdf and
-sample for layered, metallic, coated, fuzzy and smooth materials, front and
-back faces.
+sample for layered, metallic, coated, fuzzy, smooth and transmissive (rough
+and smooth refraction, thin walls) materials, front and back faces.
 
 The GGX tables are built on the host by the product and by the oracle with
 the same estimators, so they agree; the device's transcendental functions
@@ -37,6 +37,17 @@ MATERIALS = [
     OpenPbr(base_color=(0.3, 0.3, 0.3), specular_weight=0.0),  # the reference's zero-weight specular lobe
     OpenPbr(base_color=(0.6, 0.2, 0.9), specular_weight=0.4, specular_ior=2.2, base_diffuse_roughness=1.0),
     NormalizedLambert((0.6, 0.5, 0.4)),
+    # transmission: rough glass, tinted rough glass under a coat, smooth glass,
+    # half-transmissive plastic, thin walls (rough and smooth)
+    OpenPbr(base_weight=0.0, transmission_weight=1.0, specular_roughness=0.3),
+    OpenPbr(base_color=(0.5, 0.5, 0.5), transmission_weight=0.8, transmission_color=(0.9, 0.6, 0.3),
+            specular_roughness=0.55, specular_ior=1.7, coat_weight=0.4, coat_roughness=0.2),
+    OpenPbr(base_weight=0.0, transmission_weight=1.0, specular_roughness=0.0, specular_ior=1.33),
+    OpenPbr(base_color=(0.2, 0.7, 0.3), transmission_weight=0.5, specular_roughness=0.4, base_metalness=0.2),
+    OpenPbr(base_color=(0.8, 0.8, 0.2), transmission_weight=0.7, transmission_color=(0.4, 0.8, 0.9),
+            specular_roughness=0.45, thin_walled=True),
+    OpenPbr(base_weight=0.3, transmission_weight=1.0, transmission_color=(0.9, 0.5, 0.5), specular_roughness=0.0,
+            thin_walled=True),
 ]
 
 
@@ -92,6 +103,7 @@ def test_bsdf_matches_oracle(dscene, mi, front):
         mismatched += int((ok_g != s["ok"]).sum())
         total += n_q
         assert np.array_equal(((g["flags"] & 2) > 0)[both], s["delta"][both])
+        assert np.array_equal(((g["flags"] & 4) > 0)[both], s["transmission"][both])
         close_dir = np.all(np.abs(g["wi"][both] - s["wi"][both]) < 1e-4, axis=1)
         # a sample whose lobe or branch choice sits on a last-place difference
         # may go elsewhere: count it with the mismatches
@@ -104,7 +116,8 @@ def test_bsdf_matches_oracle(dscene, mi, front):
 
 def test_unsupported_openpbr_parameters_are_rejected(device):
     cam_scene = scenes.cornell(build=False)
-    for bad, what in [(dict(thin_film_weight=0.3), "thin_film_weight"), (dict(thin_walled=True), "thin_walled")]:
+    for bad, what in [(dict(thin_film_weight=0.3), "thin_film_weight"),
+                      (dict(transmission_weight=1.0, transmission_dispersion_scale=0.2), "dispersion")]:
         cs = scenes.cornell(build=False)
         cs.scene.materials[1] = OpenPbr.new((0.8, 0.1, 0.1)).with_(**bad)
         cs.scene.build_qbvh()

@@ -79,11 +79,13 @@ def test_render_is_deterministic_and_running_mean(cornell):
 
 def test_unsupported_material_is_rejected(device):
     cs = scenes.cornell(build=False)
-    cs.scene.materials[0] = OpenPbr.new((0.8, 0.8, 0.8)).with_(transmission_weight=0.5)  # no BTDF lobes
+    # dispersion (the spectral BTDF) is not carried
+    cs.scene.materials[0] = OpenPbr.new((0.8, 0.8, 0.8)).with_(transmission_weight=0.5,
+                                                                 transmission_dispersion_scale=0.5)
     cs.scene.build_qbvh()
     ds = cs.scene.upload(device)
     cam = camera.pinhole(**cs.camera, width=8, height=8)
-    with pytest.raises(tpt._lib.TptError, match="transmission_weight"):
+    with pytest.raises(tpt._lib.TptError, match="transmission_dispersion_scale"):
         ds.render(cam, 8, 8, 1, 8)
     ds.close()
 

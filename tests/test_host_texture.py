@@ -42,7 +42,7 @@ def test_checkerboard_cells():
 
 def test_material_record_carries_texture_slots():
     r = OpenPbr(base_color_texture=2, emission_color_texture=0).open_pbr_record()
-    assert ctypes.sizeof(r) == 160 and ctypes.sizeof(MaterialRecord) == 168
+    assert ctypes.sizeof(r) == 192 and ctypes.sizeof(MaterialRecord) == 200
     assert r.base_color_texture == 2 and r.emission_color_texture == 0
     assert r.base_metalness_texture == NO_TEXTURE and r.fuzz_roughness_texture == NO_TEXTURE
     # the slot fields sit where include/tpt.h puts them: after fuzz_roughness

@@ -169,7 +169,9 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
         const int first = lin ? 3 : 0;  // an affine base colour takes the exact ratio below
         // one rebuild of the BSDF per parameter and side serves both the light
         // direction and the sampled direction
-        if (use_nee || use_smp) bsdf_param_ratios2(sv.tab, m, vtx, use_nee, wn, rn, use_smp, s.wi, rs, first);
+        const bool thin_lobe = use_smp && s.trans && p0.thin_walled;  // the thin-wall BTDF's own estimate
+        if (use_nee || use_smp)
+          bsdf_param_ratios2(sv.tab, m, vtx, use_nee, wn, rn, use_smp, s.wi, rs, first, thin_lobe);
         if (use_dlt) bsdf_delta_param_ratios(sv.tab, m, vtx, s.lobe, rs, first);
         float gk[kParamRows];
         for (int k = 0; k < kParamRows; ++k)
@@ -246,8 +248,16 @@ __device__ __forceinline__ RayDiff camera_ray(const CamView& cam, const smp::Pat
   return r;
 }
 
+// The one-thread-per-pixel kernels keep a whole path in registers: without a
+// bound their textured / two-level instantiations take 256 VGPRs plus a few
+// AGPRs, one wave per SIMD.  Two waves per SIMD caps them at 256 registers
+// (a handful of spills, measured with -Rpass-analysis=kernel-resource-usage)
+// and doubles the waves that hide traversal latency; three waves would spill
+// 90-250 registers per lane.
+constexpr int kPathWaves = 2;
+
 template <bool kTwoLevel, bool kTex>
-__global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
+__global__ __launch_bounds__(kBlock, kPathWaves) void render_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
                                                         uint32_t h, uint32_t spp, uint32_t max_depth,
                                                         float* __restrict__ rgb, uint32_t* __restrict__ status) {
   __shared__ uint32_t lds[kLdsDepth * kBlock];
@@ -283,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView sc, ShadeView 
 // (its radiance) and a replay pass that accumulates dLoss/d(albedo) into
 // the block's LDS sums; each block writes its sums to `partials`.
 template <bool kTwoLevel, bool kTex, typename Acc>
-__global__ __launch_bounds__(kBlock) void adjoint_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
+__global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
                                                          uint32_t h, uint32_t spp, uint32_t max_depth,
                                                          const float* __restrict__ dimg, Acc* __restrict__ partials,
                                                          uint32_t n_grad, uint32_t rows, float* __restrict__ texgrad,
@@ -378,7 +388,7 @@ __device__ __forceinline__ float two_sum(float a, float b, float* e) {
 // second kernel adds in FP64.  kFp64: FP64 LDS sums and FP64 texel
 // gradients instead (the reference accumulation of the error report).
 template <bool kTwoLevel, bool kTex, bool kFp64>
-__global__ __launch_bounds__(kBlock) void adjoint_batch_kernel(SceneView sc, ShadeView sv,
+__global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_batch_kernel(SceneView sc, ShadeView sv,
                                                                const AdjFrame* __restrict__ frames, uint32_t nframes,
                                                                uint32_t total, uint32_t max_spp,
                                                                void* __restrict__ partials, uint32_t n_grad,
@@ -483,8 +493,8 @@ using namespace tpt;
 
 namespace {
 
-// The materials the path tracer implements: OpenPBR without transmission,
-// thin film or thin walls, NormalizedLambert, Emissive and Mirror.
+// The materials the path tracer implements: OpenPBR without thin film or
+// dispersion, NormalizedLambert, Emissive and Mirror.
 bool supported(const TptMaterial& mat, std::string* why) {
   if (mat.kind == TPT_MATERIAL_NORMALIZED_LAMBERT || mat.kind == TPT_MATERIAL_EMISSIVE ||
       mat.kind == TPT_MATERIAL_MIRROR)
@@ -494,9 +504,8 @@ bool supported(const TptMaterial& mat, std::string* why) {
     return false;
   }
   const TptOpenPbr& m = mat.u.open_pbr;
-  if (m.transmission_weight != 0.0f) *why = "transmission_weight must be 0";
-  else if (m.thin_film_weight != 0.0f) *why = "thin_film_weight must be 0";
-  else if (m.thin_walled) *why = "thin_walled must be false";
+  if (m.thin_film_weight != 0.0f) *why = "thin_film_weight must be 0";
+  else if (m.transmission_dispersion_scale > 0.0f) *why = "transmission_dispersion_scale must be 0";
   else return true;
   return false;
 }
@@ -562,7 +571,7 @@ __global__ __launch_bounds__(256) void bsdf_query_kernel(pt::Tables tb, const Tp
         r.wi[1] = smp.wi.y;
         r.wi[2] = smp.wi.z;
         r.sample_pdf = smp.pdf;
-        r.flags = 1u | (smp.delta ? 2u : 0u);
+        r.flags = 1u | (smp.delta ? 2u : 0u) | (smp.trans ? 4u : 0u);
       }
     }
     out[i] = r;

@@ -394,12 +394,13 @@ __device__ __forceinline__ float zeltner_b(float ndv, float rough) {
 
 struct MsParams {
   float eta_o, req, e_avg_o, ome_o;
+  float eta_rel, ratio_r, ome_t;  // the transmission terms (ms_params with trans)
 };
 struct TwoLobe {
   float ss, ms, total;
 };
 struct Probs {
-  float coat, metal, spec, fuzz, diff, total;
+  float coat, metal, spec, btdf, fuzz, diff, total;
 };
 struct Bsdf {
   F3 base_color, spec_color, coat_color, coat_dark, fuzz_color;
@@ -407,6 +408,11 @@ struct Bsdf {
   float coat, cax, cay, coat_eta;
   float fuzz, fuzz_rough, diff_rough;
   float spec_rough;  // effective specular roughness (a glossy sample's cone spread)
+  // transmission: weight, colour (the BTDF reads the specular alphas and
+  // eta, R/material/open_pbr.rs:436, 451-452), thin walls
+  float trans;
+  F3 trans_color;
+  bool thin;
   // directional albedos at the make_bsdf wo (they depend on wo only, and
   // every query of one Bsdf uses that wo): fuzz (Zeltner), coat and specular
   // (GGX tables with multiple-scattering compensation)
@@ -417,7 +423,11 @@ struct Bsdf {
   // average Fresnel and multiple-scattering colour; the lobe probabilities
   MsParams ms_s, ms_c;
   float msf_s, msf_c, eo_s, eo_c;
-  TwoLobe tl_s, tl_c, tl_m;
+  TwoLobe tl_s, tl_c, tl_m, tl_t;
+  // thin walls: the slab's reflectance and transmittance at wo and the
+  // transmission lobe's alphas
+  F3 thin_r, thin_t;
+  float tax, tay;
   float m_req, m_eo, m_omea;
   F3 m_fms;
   Probs pr;
@@ -506,6 +516,9 @@ __device__ __forceinline__ Bsdf make_bsdf_params(const Tables& tb, uint32_t mi, 
   p.fuzz_color = clamp01(f3(m.fuzz_color[0], m.fuzz_color[1], m.fuzz_color[2]));
   p.fuzz_rough = fr;
   p.diff_rough = clampf(m.base_diffuse_roughness, 0.0f, 1.0f);
+  p.trans = clampf(m.transmission_weight, 0.0f, 1.0f);
+  p.trans_color = clamp01(f3(m.transmission_color[0], m.transmission_color[1], m.transmission_color[2]));
+  p.thin = m.thin_walled != 0u;
   p.front = front;
   p.spec_tab = tb.albedo + (size_t)tb.mat[2 * mi] * kAlbLen;
   p.coat_tab = tb.albedo + (size_t)tb.mat[2 * mi + 1] * kAlbLen;
@@ -529,14 +542,28 @@ __device__ __forceinline__ float rough_proxy(float ax, float ay) { return clampf
 __device__ __forceinline__ bool coat_smooth(const Bsdf& p) { return p.coat <= 0.0f || fmaxf(p.cax, p.cay) < kSmoothAlpha; }
 __device__ __forceinline__ bool metal_smooth(const Bsdf& p) { return p.metal <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
 __device__ __forceinline__ bool spec_smooth(const Bsdf& p) { return p.specular <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
+__device__ __forceinline__ bool trans_smooth(const Bsdf& p) { return p.trans <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
 
-__device__ __forceinline__ MsParams ms_params(const Bsdf& p, float ax, float ay, float eta) {  // :593-625
+// dielectric_ms_params (:593-625); the transmission terms (eta_rel, ratio_r,
+// ome_t) only with `trans`
+__device__ __forceinline__ MsParams ms_params(const Bsdf& p, float ax, float ay, float eta, bool trans = false) {
   MsParams s;
   s.eta_o = p.front ? fmaxf(eta, 1.0e-4f) : 1.0f / fmaxf(eta, 1.0e-4f);
   s.req = powf(ax * ay, 0.25f);
   s.e_avg_o = diel_eavg(p.diel, s.req, s.eta_o);
   s.ome_o = fmaxf(1.0f - s.e_avg_o, kMsEps);
-  return s;  // the transmission ratio is not needed without transmission
+  s.eta_rel = 1.0f / s.eta_o;
+  s.ratio_r = s.ome_t = 0.0f;
+  if (trans) {
+    const float e_avg_t = diel_eavg(p.diel, s.req, s.eta_rel);
+    s.ome_t = fmaxf(1.0f - e_avg_t, kMsEps);
+    const float fo = f_avg_dielectric(s.eta_o), ft = f_avg_dielectric(s.eta_rel);
+    const float a = (1.0f - fo) / s.ome_o;
+    const float b = (1.0f - ft) * s.eta_o * s.eta_o / s.ome_t;
+    const float x = a + b > kMsEps ? b / (a + b) : 0.5f;
+    s.ratio_r = clampf(1.0f - x * (1.0f - fo), 0.0f, 1.0f);
+  }
+  return s;
 }
 __device__ __forceinline__ float ms_factor(const MsParams& s) {  // :573-577
   const float fa = clampf(f_avg_dielectric(s.eta_o), 0.0f, 1.0f);
@@ -547,6 +574,11 @@ __device__ __forceinline__ TwoLobe refl_lobes(const MsParams& s) {  // :627-636
   const float fa = clampf(f_avg_dielectric(s.eta_o), 0.0f, 1.0f);
   const float ss = clampf(fa * s.e_avg_o, 0.0f, 1.0f);
   const float ms = fmaxf(1.0f - s.e_avg_o, 0.0f) * ms_factor(s);
+  return TwoLobe{ss, ms, fmaxf(ss + ms, 1.0e-8f)};
+}
+__device__ __forceinline__ TwoLobe trans_lobes(const MsParams& s) {  // dielectric_transmission_lobe_weights :638-646
+  const float ss = clampf(s.e_avg_o, 0.0f, 1.0f);
+  const float ms = fmaxf(1.0f - s.e_avg_o, 0.0f) * clampf(1.0f - s.ratio_r, 0.0f, 1.0f);
   return TwoLobe{ss, ms, fmaxf(ss + ms, 1.0e-8f)};
 }
 __device__ __forceinline__ TwoLobe cond_lobes(const Bsdf& p) {  // :579-591
@@ -566,10 +598,11 @@ __device__ __forceinline__ F3 coat_tint(const Bsdf& p, F3 wo, F3 wi) {  // :483-
 }
 struct LayerW {
   float coat_amp;
-  F3 metal, spec, fuzz, diff;
+  F3 metal, spec, btdf, fuzz, diff;
 };
-// layer_weights (:333-390) without transmission, subsurface, thin walls; wo
-// is the make_bsdf wo (its albedos come from p)
+// layer_weights (:333-390); subsurface is 0 (make_bsdf forces it,
+// R/material/open_pbr.rs:455), so the diffuse BTDF's weight is zero and is
+// left out; wo is the make_bsdf wo (its albedos come from p)
 __device__ __forceinline__ LayerW layer_weights(const Bsdf& p, F3 wo, const F3* wi) {
   const float bfs = fmaxf(1.0f - p.fuzz * p.e_fuzz, 0.0f);
   F3 under = f3s(bfs);
@@ -584,7 +617,9 @@ __device__ __forceinline__ LayerW layer_weights(const Bsdf& p, F3 wo, const F3* 
   const float e_spec = p.e_spec;
   w.spec = hadamard(below_metal, p.spec_color);
   const F3 leak = vmaxs(sub(f3s(1.0f), muls(p.spec_color, e_spec)), 0.0f);
-  const F3 below_trans = muls(hadamard(below_metal, leak), 1.0f - 0.0f);
+  const F3 below_spec = hadamard(below_metal, leak);
+  w.btdf = hadamard(muls(below_spec, p.trans), p.thin ? f3s(1.0f) : p.trans_color);
+  const F3 below_trans = muls(below_spec, 1.0f - p.trans);
   w.diff = hadamard(muls(below_trans, (1.0f - 0.0f) * p.base), p.base_color);
   w.coat_amp = p.coat * bfs;
   w.fuzz = muls(p.fuzz_color, p.fuzz);
@@ -596,10 +631,11 @@ __device__ __forceinline__ Probs lobe_probs(const Bsdf& p, F3 wo) {  // :392-422
   r.coat = p.coat > 0.0f ? w.coat_amp : 0.0f;
   r.metal = luminance(hadamard(w.metal, add(f3s(0.5f), muls(p.base_color, 0.5f))));
   r.spec = luminance(w.spec);
+  r.btdf = p.trans > 0.0f ? fmaxf(luminance(w.btdf), 0.0f) : 0.0f;
   r.fuzz = fmaxf(luminance(w.fuzz), 0.0f);
   r.diff = fmaxf(luminance(w.diff), 0.0f);
   const float zero = fmaxf(luminance(f3s(0.0f)), 0.0f);
-  r.total = r.coat + r.metal + r.spec + 0.0f + r.fuzz + r.diff + zero;
+  r.total = r.coat + r.metal + r.spec + r.btdf + r.fuzz + r.diff + zero;
   return r;
 }
 // refl_ms_eval (:648-670) with the lobe's cached parameters, factor and table value at wo
@@ -666,6 +702,8 @@ __device__ __forceinline__ void metal_constants(Bsdf& p, F3 wo) {
   const F3 fms0 = muls(hadamard(fa, fa), ea);
   p.m_fms = f3(fms0.x / ds.x, fms0.y / ds.y, fms0.z / ds.z);
 }
+__device__ __forceinline__ void thin_wall_coefficients(const Bsdf& p, float cos_theta_o, F3* refl, F3* trans);
+__device__ __forceinline__ void thin_wall_alpha(const Bsdf& p, float* ax, float* ay);
 // make_bsdf: the parameters plus every wo-only term that layer_weights, the
 // lobe evaluations, pdf and sample read (the same expressions, evaluated once
 // per vertex under the conditions their readers test).
@@ -675,10 +713,18 @@ __device__ __forceinline__ void make_bsdf(const Tables& tb, uint32_t mi, const T
   p = make_bsdf_params(tb, mi, m, front, wol);
   const bool rough_s = !(fmaxf(p.ax, p.ay) < kSmoothAlpha), rough_c = !(fmaxf(p.cax, p.cay) < kSmoothAlpha);
   if (rough_s) {
-    p.ms_s = ms_params(p, p.ax, p.ay, p.spec_eta);
+    // the transmission lobe's multiple-scattering terms read the same table
+    // at the same (roughness, eta) (eval_spec_btdf passes specular_eta, the
+    // pdf transmission_eta: the same value here)
+    p.ms_s = ms_params(p, p.ax, p.ay, p.spec_eta, p.trans > 0.0f && !p.thin);
     p.msf_s = ms_factor(p.ms_s);
     p.tl_s = refl_lobes(p.ms_s);
+    p.tl_t = trans_lobes(p.ms_s);
     if (wol.z > 0.0f) p.eo_s = diel_e(p.diel, wol.z, p.ms_s.req, p.ms_s.eta_o);
+  }
+  if (p.thin) {
+    thin_wall_coefficients(p, fabsf(wol.z), &p.thin_r, &p.thin_t);
+    thin_wall_alpha(p, &p.tax, &p.tay);
   }
   if (p.coat > 0.0f && rough_c) {
     p.ms_c = ms_params(p, p.cax, p.cay, p.coat_eta);
@@ -695,9 +741,12 @@ __device__ __forceinline__ void make_bsdf(const Tables& tb, uint32_t mi, const T
                  ? albedo_ms(wol, p.cax, p.cay, p.msf_c, p.eo_c,
                              albedo_lookup(p.coat_tab, p.coat_eta, wol, rough_proxy(p.cax, p.cay)))
                  : 0.0f;
-  p.e_spec = wol.z <= 0.0f ? 0.0f
-                           : albedo_ms(wol, p.ax, p.ay, p.msf_s, p.eo_s,
-                                       albedo_lookup(p.spec_tab, p.spec_eta, wol, rough_proxy(p.ax, p.ay)));
+  if (wol.z <= 0.0f) {
+    p.e_spec = 0.0f;
+  } else {
+    const float ss = albedo_lookup(p.spec_tab, p.spec_eta, wol, rough_proxy(p.ax, p.ay));
+    p.e_spec = p.thin ? ss : albedo_ms(wol, p.ax, p.ay, p.msf_s, p.eo_s, ss);  // lookup_spec_albedo :439-454
+  }
   if (wol.z > 0.0f) p.pr = lobe_probs(p, wol);
 }
 // The same as a call: the Bsdf lives in memory anyway (eval, pdf and sample
@@ -729,14 +778,49 @@ __device__ __forceinline__ float base_fresnel(const Bsdf& p, float c) {  // base
                            : weighted_fresnel_ratio(c, 1.0f / fmaxf(a, 1.0e-4f), 1.0f / fmaxf(b, 1.0e-4f));
   return lerpf(un, co, coat);
 }
-__device__ __forceinline__ float spec_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_spec_brdf :841-867
-  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+// thin_wall_coefficients (:931-959): the slab's total reflectance and
+// transmittance at an outgoing cosine, the interior absorbing
+// transmission_color^(1/cos_i) per pass, the internal reflections summed
+__device__ __forceinline__ void thin_wall_coefficients(const Bsdf& p, float cos_theta_o, F3* refl, F3* trans) {
+  const float eta = fmaxf(p.trans_eta, 1.0e-4f), eta_prime = fmaxf(p.spec_eta, 1.0e-4f);
+  const float cos_o = clampf(cos_theta_o, 0.0f, 1.0f);
+  const float sin2_o = fmaxf(1.0f - cos_o * cos_o, 0.0f);
+  const float sin2_i = clampf(sin2_o / (eta * eta), 0.0f, 1.0f);
+  const float cos_i = fmaxf(sqrtf(1.0f - sin2_i), 1.0e-4f);
+  const float f_o = clampf(base_fresnel(p, cos_o), 0.0f, 1.0f);  // spec_brdf_fresnel without a thin film
+  // thin_wall_internal_fresnel (:961-979) without a thin film
+  const float f_i =
+      clampf(weighted_fresnel_ratio(cos_i, 1.0f / fmaxf(eta, 1.0e-4f), 1.0f / fmaxf(eta_prime, 1.0e-4f)), 0.0f, 1.0f);
+  const F3 a = pow3(clamp01(p.trans_color), 1.0f / cos_i);
+  const F3 a2 = hadamard(a, a);
+  const F3 fi2a2 = muls(muls(a2, f_i), f_i);
+  const F3 den = vmaxs(sub(f3s(1.0f), fi2a2), 1.0e-6f);
+  const float omfo = fmaxf(1.0f - f_o, 0.0f), omfi = fmaxf(1.0f - f_i, 0.0f);
+  const F3 rn = muls(a2, omfo * omfi * f_i);
+  const F3 r = add(f3s(f_o), f3(rn.x / den.x, rn.y / den.y, rn.z / den.z));
+  const F3 t = hadamard(muls(a, omfo * omfi), add(f3s(1.0f), f3(fi2a2.x / den.x, fi2a2.y / den.y, fi2a2.z / den.z)));
+  *refl = clamp01(r);
+  *trans = clamp01(t);
+}
+// thin_wall_transmission_alpha_xy (:553-563)
+__device__ __forceinline__ void thin_wall_alpha(const Bsdf& p, float* ax, float* ay) {
+  const float eta = fmaxf(p.trans_eta, 1.0f);
+  const float e5 = eta - 0.5f;
+  const float scale = sqrtf(fmaxf(3.7f * (eta - 1.0f) * (e5 * e5) / (eta * (eta * eta)), 0.0f));
+  const float alpha_scale = scale * scale;
+  *ax = clampf(p.ax * alpha_scale, kMinAlpha, 1.0f);
+  *ay = clampf(p.ay * alpha_scale, kMinAlpha, 1.0f);
+}
+__device__ __forceinline__ F3 spec_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_spec_brdf :841-867
+  const F3 zero = f3s(0.0f);
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return zero;
   F3 wm;
-  if (!half_vector(wo, wi, &wm)) return 0.0f;
+  if (!half_vector(wo, wi, &wm)) return zero;
   const float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
-  if (d <= 0.0f || g <= 0.0f) return 0.0f;
+  if (d <= 0.0f || g <= 0.0f) return zero;
+  if (p.thin) return muls(p.thin_r, d * g / (4.0f * wo.z * wi.z));
   const float ss = base_fresnel(p, fabsf(dot(wo, wm))) * (d * g / (4.0f * wo.z * wi.z));
-  return ss + refl_ms_eval(p, wo, wi, p.ms_s, p.msf_s, p.eo_s);
+  return f3s(ss + refl_ms_eval(p, wo, wi, p.ms_s, p.msf_s, p.eo_s));
 }
 __device__ __forceinline__ float spec_ss_pdf(const Bsdf& p, F3 wo, F3 wi) {  // :981-994
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
@@ -745,6 +829,100 @@ __device__ __forceinline__ float spec_ss_pdf(const Bsdf& p, F3 wo, F3 wi) {  // 
   const float den = 4.0f * fabsf(dot(wo, wm));
   if (den <= 0.0f) return 0.0f;
   return pdf_bounded_vndf(wo, wm, p.ax, p.ay) / den;
+}
+
+// ---- specular BTDF (:638-697, 1024-1222), radiance mode, no dispersion --------------------
+// pdf_wm_vndf (R/bsdf/smith_ggx.rs:62-75)
+__device__ __forceinline__ float ggx_g1(F3 w, float ax, float ay) {  // :45-51
+  const float l = ggx_lambda(w, ax, ay);
+  return __builtin_isfinite(l) ? 1.0f / (1.0f + l) : 0.0f;
+}
+__device__ __forceinline__ float pdf_vndf(F3 wo, F3 wm, float ax, float ay) {
+  if (!(wo.z > 0.0f) || wm.z <= 0.0f) return 0.0f;
+  const float d = ggx_d(wm, ax, ay), g1 = ggx_g1(wo, ax, ay), c = fmaxf(dot(wo, wm), 0.0f);
+  if (d <= 0.0f || g1 <= 0.0f || c <= 0.0f) return 0.0f;
+  return d * g1 * c / wo.z;
+}
+// transmission_eta_rel (:1245-1248)
+__device__ __forceinline__ float trans_eta_rel(const Bsdf& p) { return p.front ? 1.0f / p.trans_eta : p.trans_eta; }
+// the generalised half vector of a refraction pair (:1081-1096, 1194-1209):
+// normalize(eta_rel wo + wi), flipped up; false where undefined
+__device__ __forceinline__ bool refraction_half_vector(F3 wo, F3 wi, float eta_rel, F3* wm, float* co, float* ci,
+                                                       float* dn) {
+  const F3 u = add(muls(wo, eta_rel), wi);
+  if (len2(u) < 1.0e-12f) return false;
+  F3 m = normalize(u);
+  if (m.z < 0.0f) m = neg(m);
+  const float cwo = dot(wo, m);
+  if (cwo <= 0.0f) return false;
+  const float cwi = dot(wi, m);
+  const float d = cwi + eta_rel * cwo;
+  if (fabsf(d) < 1.0e-6f) return false;
+  *wm = m;
+  *co = cwo;
+  *ci = cwi;
+  *dn = d;
+  return true;
+}
+// eval_dielectric_transmission_ms (:672-696) with the cached lobe terms
+__device__ __forceinline__ float trans_ms_eval(const Bsdf& p, F3 wo, F3 wi) {
+  if (wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
+  const MsParams& s = p.ms_s;
+  const float ei = diel_e(p.diel, fabsf(wi.z), s.req, s.eta_rel);
+  const float rs = 1.0f / (s.eta_rel * s.eta_rel);  // TransportMode::Radiance.transmission_scale
+  return (1.0f - s.ratio_r) * (1.0f - p.eo_s) * (1.0f - ei) * rs / (kPi * s.ome_t);
+}
+// eval_thin_wall_spec_btdf (:1053-1072): the reflection lobe at the thin-wall
+// alphas mirrored through the surface, times the slab transmittance
+__device__ __forceinline__ F3 thin_btdf_eval(const Bsdf& p, F3 wo, F3 wi) {
+  const F3 zero = f3s(0.0f);
+  if (wo.z <= 0.0f || wi.z >= 0.0f || trans_smooth(p)) return zero;
+  const F3 wim = f3(wi.x, wi.y, -wi.z);
+  F3 wm;
+  if (!half_vector(wo, wim, &wm)) return zero;
+  const float d = ggx_d(wm, p.tax, p.tay), g = ggx_g2(wo, wim, p.tax, p.tay);
+  if (d <= 0.0f || g <= 0.0f) return zero;
+  return muls(p.thin_t, d * g / (4.0f * wo.z * fabsf(wi.z)));
+}
+__device__ __forceinline__ F3 btdf_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_spec_btdf :1024-1051
+  if (p.thin) return thin_btdf_eval(p, wo, wi);
+  const F3 zero = f3s(0.0f);
+  if (wo.z <= 0.0f || wi.z >= 0.0f) return zero;
+  const float eta_rel = trans_eta_rel(p);
+  F3 wm;
+  float co, ci, dn;
+  if (!refraction_half_vector(wo, wi, eta_rel, &wm, &co, &ci, &dn)) return zero;
+  // eval_spec_btdf_scalar_with_eta (:1074-1118)
+  const float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
+  if (d <= 0.0f || g <= 0.0f) return zero;
+  const float rs = 1.0f / (eta_rel * eta_rel);
+  const float sc = d * g * fabsf(ci * co);
+  const float denom = dn * dn * fabsf(wo.z) * fabsf(wi.z);
+  if (denom <= 0.0f) return zero;
+  const float scalar = sc * rs / denom;
+  const float ss = fmaxf(1.0f - base_fresnel(p, co), 0.0f) * scalar;  // spec_btdf_fresnel_with_eta, no thin film
+  return f3s(ss + trans_ms_eval(p, wo, wi));
+}
+__device__ __forceinline__ float btdf_pdf(const Bsdf& p, F3 wo, F3 wi) {  // pdf_specular_btdf :1154-1221
+  if (wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
+  if (p.thin) {
+    if (trans_smooth(p)) return 0.0f;
+    const F3 wim = f3(wi.x, wi.y, -wi.z);
+    F3 wm;
+    if (!half_vector(wo, wim, &wm)) return 0.0f;
+    const float pw = pdf_bounded_vndf(wo, wm, p.tax, p.tay);
+    const float den = 4.0f * fabsf(dot(wo, wm));
+    return den > 0.0f ? pw / den : 0.0f;
+  }
+  float pss = 0.0f;
+  F3 wm;
+  float co, ci, dn;
+  if (refraction_half_vector(wo, wi, trans_eta_rel(p), &wm, &co, &ci, &dn)) {
+    const float pw = pdf_vndf(wo, wm, p.ax, p.ay);
+    if (pw > 0.0f) pss = pw * fabsf(ci) / (dn * dn);
+  }
+  const TwoLobe& w = p.tl_t;
+  return (w.ss * pss + w.ms * cos_pdf(-wi.z)) / w.total;
 }
 __device__ __forceinline__ float phi_of(F3 v) {  // :2060-2063
   const float ph = atan2f(v.y, v.x);
@@ -792,7 +970,8 @@ __device__ __noinline__ F3 bsdf_eval(const Bsdf& p, F3 wo, F3 wi) {
   const LayerW w = layer_weights(p, wo, &wi);
   if (w.coat_amp > 0.0f && !coat_smooth(p)) total = add(total, muls(f3s(coat_eval(p, wo, wi)), w.coat_amp));
   if (len2(w.metal) > 0.0f && !metal_smooth(p)) total = add(total, hadamard(w.metal, metal_eval(p, wo, wi)));
-  if (len2(w.spec) > 0.0f && !spec_smooth(p)) total = add(total, hadamard(w.spec, f3s(spec_eval(p, wo, wi))));
+  if (len2(w.spec) > 0.0f && !spec_smooth(p)) total = add(total, hadamard(w.spec, spec_eval(p, wo, wi)));
+  if (len2(w.btdf) > 0.0f && !trans_smooth(p)) total = add(total, hadamard(w.btdf, btdf_eval(p, wo, wi)));
   if (len2(w.fuzz) > 0.0f) {
     float pd;
     total = add(total, hadamard(w.fuzz, f3s(fuzz_eval(p, wo, wi, &pd))));
@@ -835,10 +1014,11 @@ __device__ __noinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {
     float ps = 0.0f;
     if (wo.z > 0.0f && wi.z > 0.0f) {
       const TwoLobe& w = p.tl_s;
-      ps = (w.ss * spec_ss_pdf(p, wo, wi) + w.ms * cos_pdf(wi.z)) / w.total;
+      ps = p.thin ? spec_ss_pdf(p, wo, wi) : (w.ss * spec_ss_pdf(p, wo, wi) + w.ms * cos_pdf(wi.z)) / w.total;
     }
     pdf += (pr.spec / pr.total) * ps;
   }
+  if (pr.btdf > 0.0f && !trans_smooth(p)) pdf += (pr.btdf / pr.total) * btdf_pdf(p, wo, wi);
   if (pr.fuzz > 0.0f) {
     float pd;
     fuzz_eval(p, wo, wi, &pd);
@@ -855,9 +1035,11 @@ struct Sample {
   F3 weight, wi;
   float pdf;
   bool delta;
-  bool glossy;  // BsdfFlags::GLOSSY: a rough coat, metal, specular or fuzz lobe (read with !delta)
+  bool glossy;  // BsdfFlags::GLOSSY: a rough coat, metal, specular, transmission or fuzz lobe (read with !delta)
+  bool trans;   // BsdfFlags::TRANSMISSION
+  float eta;    // BsdfSample::eta: the relative IOR of a refraction, else 1
   float cone;   // MaterialSample::cone_spread
-  int lobe;     // OpenPBR: the lobe picked (0 coat, 1 metal, 2 specular, 4 fuzz, 5 diffuse)
+  int lobe;     // OpenPBR: the lobe picked (0 coat, 1 metal, 2 specular, 3 transmission, 4 fuzz, 5 diffuse)
 };
 __device__ __forceinline__ bool finalize(const Bsdf& p, F3 wo, F3 wi, Sample* out) {  // :1958-1999
   const float pdf = bsdf_pdf(p, wo, wi);
@@ -902,6 +1084,52 @@ __device__ __forceinline__ bool sample_bounded_vndf(F3 wo, float ax, float ay, f
 __device__ __forceinline__ F3 reflect_local(F3 wo, F3 wm) {  // R/bsdf/smith_ggx.rs:12-14
   return normalize_or_zero(add(neg(wo), muls(wm, 2.0f * dot(wo, wm))));
 }
+// sample_wm_vndf (Heitz 2018, R/bsdf/smith_ggx.rs:78-119)
+__device__ __forceinline__ bool sample_vndf(F3 wo, float ax, float ay, float u0, float u1, F3* out) {
+  if (wo.z <= 0.0f) return false;
+  ax = fmaxf(ax, kMinAlpha);
+  ay = fmaxf(ay, kMinAlpha);
+  const F3 ws = normalize_or_zero(f3(ax * wo.x, ay * wo.y, wo.z));
+  if (len2(ws) == 0.0f) return false;
+  const float lensq = ws.x * ws.x + ws.y * ws.y;
+  const F3 t1 = lensq > 0.0f ? divs(f3(-ws.y, ws.x, 0.0f), sqrtf(lensq)) : f3(1.0f, 0.0f, 0.0f);
+  const F3 t2 = cross(ws, t1);
+  const float r = sqrtf(clampf(u0, 0.0f, 1.0f));
+  const float phi = kTau * u1;
+  const float p1 = r * cosf(phi);
+  float p2 = r * sinf(phi);
+  const float s = 0.5f * (1.0f + ws.z);
+  p2 = (1.0f - s) * sqrtf(fmaxf(1.0f - p1 * p1, 0.0f)) + s * p2;
+  const F3 nh = add(add(muls(t1, p1), muls(t2, p2)), muls(ws, sqrtf(fmaxf(1.0f - p1 * p1 - p2 * p2, 0.0f))));
+  const F3 wm = normalize_or_zero(f3(ax * nh.x, ay * nh.y, fmaxf(nh.z, 0.0f)));
+  if (len2(wm) == 0.0f || wm.z <= 0.0f) return false;
+  *out = wm;
+  return true;
+}
+// refract_about (R/bsdf/open_pbr.rs:2043-2058)
+__device__ __forceinline__ bool refract_about(F3 wo, F3 wm, float eta, F3* out) {
+  const float co = dot(wo, wm);
+  if (co <= 0.0f) return false;
+  const float s2t = eta * eta * fmaxf(1.0f - co * co, 0.0f);
+  if (s2t >= 1.0f) return false;
+  const float ct = sqrtf(fmaxf(1.0f - s2t, 0.0f));
+  const F3 wi = normalize_or_zero(add(muls(wo, -eta), muls(wm, eta * co - ct)));
+  if (len2(wi) == 0.0f) return false;
+  *out = wi;
+  return true;
+}
+// refract (R/math/mod.rs:83-102): the smooth refraction of wo (shading frame)
+__device__ __forceinline__ bool refract_smooth(F3 wo, float eta, F3* out) {
+  if (wo.z <= 0.0f) return false;
+  const float s2o = fmaxf(1.0f - wo.z * wo.z, 0.0f);
+  const float s2t = eta * eta * s2o;
+  if (s2t >= 1.0f) return false;
+  const float ct = sqrtf(fmaxf(1.0f - s2t, 0.0f));
+  const F3 wi = normalize_or_zero(f3(-eta * wo.x, -eta * wo.y, -ct));
+  if (len2(wi) == 0.0f) return false;
+  *out = wi;
+  return true;
+}
 __device__ __forceinline__ bool sample_two_lobe(const Bsdf& p, F3 wo, float ax, float ay, TwoLobe w, float ux, float uy,
                                                 Sample* out) {
   const float ssp = w.ss / w.total;
@@ -919,11 +1147,26 @@ __device__ __forceinline__ bool sample_two_lobe(const Bsdf& p, F3 wo, float ax, 
 // is this over the lobe probability): the lobe's layer weight at the mirror
 // direction times its Fresnel term there (sample_coat / sample_metal /
 // sample_spec_brdf for smooth lobes, R/bsdf/open_pbr.rs:1467-1668).
+// Smooth transmission (lobe 3, sample_spec_btdf :1670-1781): the layer weight
+// at the refracted direction (the pass-through -wo on thin walls) times
+// (1 - Fresnel) / eta_rel^2, or the slab transmittance on thin walls.
 __device__ __forceinline__ F3 delta_numerator(const Bsdf& p, F3 wo, int lobe) {
   const F3 mirror = f3(-wo.x, -wo.y, wo.z);
   if (lobe == 0) return f3s(layer_weights(p, wo, &mirror).coat_amp * fresnel_dielectric(fabsf(mirror.z), 1.0f, p.coat_eta));
   if (lobe == 1) return hadamard(layer_weights(p, wo, &mirror).metal, metal_fresnel(p, fabsf(mirror.z)));
-  if (lobe == 2) return hadamard(layer_weights(p, wo, &mirror).spec, f3s(base_fresnel(p, fabsf(mirror.z))));
+  if (lobe == 2)
+    return hadamard(layer_weights(p, wo, &mirror).spec, p.thin ? p.thin_r : f3s(base_fresnel(p, fabsf(mirror.z))));
+  if (lobe == 3) {
+    if (p.thin) {
+      const F3 through = neg(wo);
+      return hadamard(layer_weights(p, wo, &through).btdf, p.thin_t);
+    }
+    const float eta_rel = trans_eta_rel(p);
+    F3 wi;
+    if (!refract_smooth(wo, eta_rel, &wi)) return f3s(0.0f);
+    const float f = base_fresnel(p, fabsf(wo.z));
+    return muls(muls(hadamard(layer_weights(p, wo, &wi).btdf, f3s(1.0f)), 1.0f - f), 1.0f / (eta_rel * eta_rel));
+  }
   return f3s(0.0f);
 }
 __device__ __forceinline__ bool delta_sample(F3 weight, F3 wi, float p_lobe, Sample* out) {
@@ -952,10 +1195,10 @@ __device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, flo
   } else if ((u -= pr.metal) < pr.spec) {
     lobe = 2;
     plv = pr.spec;
-  } else if ((u -= pr.spec) < 0.0f) {
+  } else if ((u -= pr.spec) < pr.btdf) {
     lobe = 3;
-    plv = 0.0f;
-  } else if ((u -= 0.0f) < pr.fuzz) {
+    plv = pr.btdf;
+  } else if ((u -= pr.btdf) < pr.fuzz) {
     lobe = 4;
     plv = pr.fuzz;
   } else if ((u -= pr.fuzz) < pr.diff) {
@@ -968,6 +1211,8 @@ __device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, flo
   const float p_lobe = plv / pr.total;
   if (p_lobe <= 0.0f) return false;
   out->glossy = lobe != 5;
+  out->trans = false;
+  out->eta = 1.0f;
   out->lobe = lobe;
   const F3 mirror = f3(-wo.x, -wo.y, wo.z);
   if (lobe == 0) {
@@ -980,7 +1225,56 @@ __device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, flo
   }
   if (lobe == 2) {
     if (spec_smooth(p)) return delta_sample(divs(delta_numerator(p, wo, 2), p_lobe), mirror, p_lobe, out);
+    if (p.thin) {  // thin walls: single scattering only
+      F3 wm;
+      if (!sample_bounded_vndf(wo, p.ax, p.ay, ux, uy, &wm)) return false;
+      const F3 wi = reflect_local(wo, wm);
+      if (wi.z <= 0.0f) return false;
+      return finalize(p, wo, wi, out);
+    }
     return sample_two_lobe(p, wo, p.ax, p.ay, p.tl_s, ux, uy, out);
+  }
+  if (lobe == 3) {  // sample_spec_btdf (:1670-1837), no dispersion
+    out->trans = true;
+    if (p.thin) {
+      if (trans_smooth(p)) return delta_sample(divs(delta_numerator(p, wo, 3), p_lobe), neg(wo), p_lobe, out);
+      F3 wm;
+      if (!sample_bounded_vndf(wo, p.tax, p.tay, ux, uy, &wm)) return false;
+      const F3 wr = reflect_local(wo, wm);
+      if (wr.z <= 0.0f) return false;
+      const F3 wi = f3(wr.x, wr.y, -wr.z);
+      // the lobe's own pdf and value (not the whole BSDF's)
+      const float pdf_total = p_lobe * btdf_pdf(p, wo, wi);
+      if (pdf_total <= 0.0f) return false;
+      const F3 bw = layer_weights(p, wo, &wi).btdf;
+      out->weight = divs(muls(hadamard(bw, thin_btdf_eval(p, wo, wi)), fabsf(wi.z)), pdf_total);
+      out->wi = wi;
+      out->pdf = pdf_total;
+      out->delta = false;
+      return true;
+    }
+    const float eta_rel = trans_eta_rel(p);
+    out->eta = eta_rel;
+    if (trans_smooth(p)) {
+      F3 wi;
+      if (!refract_smooth(wo, eta_rel, &wi)) return false;
+      return delta_sample(divs(delta_numerator(p, wo, 3), p_lobe), wi, p_lobe, out);
+    }
+    const TwoLobe& w = p.tl_t;
+    const float ssp = w.ss / w.total;
+    if (ux >= ssp) {
+      const F3 up = cosine_hemisphere(ssp < 1.0f ? (ux - ssp) / (1.0f - ssp) : 0.0f, uy);
+      return finalize(p, wo, f3(up.x, up.y, -up.z), out);
+    }
+    F3 wm;
+    if (!sample_vndf(wo, p.ax, p.ay, ssp > 0.0f ? ux / ssp : 0.0f, uy, &wm)) return false;
+    const float co = dot(wo, wm);
+    if (co <= 0.0f) return false;
+    F3 wi;
+    if (!refract_about(wo, wm, eta_rel, &wi)) return false;
+    if (wi.z >= 0.0f) return false;
+    if (fabsf(dot(wi, wm) + eta_rel * co) < 1.0e-6f) return false;
+    return finalize(p, wo, wi, out);
   }
   if (lobe == 4) {
     F3 wi;
@@ -1278,7 +1572,13 @@ __device__ __forceinline__ bool pbr_sample(const Tables& tb, const TptOpenPbr& m
   bsdf_at<kOutline>(tb, v.material, m, v.front, wol, &p);
   if (!bsdf_sample(p, wol, u_lobe, ux, uy, s)) return false;
   const F3 wi = to_world(v.frame, s->wi);
-  if (dot(wi, v.ng) <= 1.0e-6f) return false;  // GEOMETRIC_NORMAL_COS_EPSILON; every carried lobe reflects
+  // GEOMETRIC_NORMAL_COS_EPSILON: a reflection must leave above the geometric
+  // surface, a rough refraction below it
+  if (s->trans) {
+    if (dot(wi, v.ng) >= -1.0e-6f && !s->delta) return false;
+  } else if (dot(wi, v.ng) <= 1.0e-6f) {
+    return false;
+  }
   // cone_spread: 2 * effective specular roughness for glossy samples, else
   // GLOSSY_DIFFUSE_CONE_SPREAD (:21, :509-522)
   s->cone = (s->glossy && !s->delta) ? 2.0f * clampf(p.spec_rough, 0.0f, 1.0f) : 0.5f;
@@ -1312,6 +1612,8 @@ __device__ __forceinline__ bool mat_sample(const Tables& tb, const TptMaterial& 
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
   F3 wl;
+  s->trans = false;
+  s->eta = 1.0f;
   if (m.kind == TPT_MATERIAL_NORMALIZED_LAMBERT) {  // R/bsdf/normalized_lambert.rs:35-60
     wl = cosine_hemisphere(ux, uy);
     const float pdf = fmaxf(wl.z, 0.0f) / kPi;
@@ -1411,9 +1713,16 @@ __device__ __forceinline__ float& pbr_param(TptOpenPbr& p, int k) {
 // ra[k] / rb[k] receive the ratios of parameter k at directions wia / wib
 // (each only when na / nb), for k from `first` on (the others stay 0).  Each
 // rebuilt BSDF serves both directions, so a vertex with a light sample and a
-// BSDF sample builds it once per parameter and side.
+// BSDF sample builds it once per parameter and side.  thin_b: the sample at
+// wib came from the thin-walled rough BTDF, whose weight is that lobe's own
+// layer weight times value over its own density (sample_spec_btdf
+// :1700-1727), not the whole BSDF's: the ratio is taken on that lobe.
+__device__ __forceinline__ F3 sampled_value(const Bsdf& b, F3 wo, F3 wi, bool thin_lobe) {
+  return thin_lobe ? hadamard(layer_weights(b, wo, &wi).btdf, thin_btdf_eval(b, wo, wi)) : bsdf_eval(b, wo, wi);
+}
 __device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMaterial& m, const Vtx& v, bool na, F3 wia,
-                                                F3 ra[kParamRows], bool nb, F3 wib, F3 rb[kParamRows], int first) {
+                                                F3 ra[kParamRows], bool nb, F3 wib, F3 rb[kParamRows], int first,
+                                                bool thin_b = false) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
   for (int k = 0; k < kParamRows; ++k) ra[k] = rb[k] = zero;
   if (dot(v.wo, v.ng) <= 0.0f) return;
@@ -1421,7 +1730,7 @@ __device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMater
   if (wol.z <= 0.0f) return;
   Bsdf b;
   make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
-  const F3 fa0 = na ? bsdf_eval(b, wol, wia) : zero, fb0 = nb ? bsdf_eval(b, wol, wib) : zero;
+  const F3 fa0 = na ? bsdf_eval(b, wol, wia) : zero, fb0 = nb ? sampled_value(b, wol, wib, thin_b) : zero;
   na = na && ((fa0.x > 0.0f) || (fa0.y > 0.0f) || (fa0.z > 0.0f));
   nb = nb && ((fb0.x > 0.0f) || (fb0.y > 0.0f) || (fb0.z > 0.0f));
   if (!na && !nb) return;
@@ -1437,9 +1746,9 @@ __device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMater
     pbr_param(a, k) = hi;
     pbr_param(c, k) = lo;
     make_bsdf_call(tb, v.material, a, v.front, wol, &b);
-    const F3 fah = na ? bsdf_eval(b, wol, wia) : zero, fbh = nb ? bsdf_eval(b, wol, wib) : zero;
+    const F3 fah = na ? bsdf_eval(b, wol, wia) : zero, fbh = nb ? sampled_value(b, wol, wib, thin_b) : zero;
     make_bsdf_call(tb, v.material, c, v.front, wol, &b);
-    const F3 fal = na ? bsdf_eval(b, wol, wia) : zero, fbl = nb ? bsdf_eval(b, wol, wib) : zero;
+    const F3 fal = na ? bsdf_eval(b, wol, wia) : zero, fbl = nb ? sampled_value(b, wol, wib, thin_b) : zero;
     if (na) ra[k] = ratio(divs(sub(fah, fal), hi - lo), fa0);
     if (nb) rb[k] = ratio(divs(sub(fbh, fbl), hi - lo), fb0);
   }
@@ -1930,22 +2239,46 @@ __device__ __forceinline__ F3 offset_origin(F3 p, F3 ng, F3 d) {  // R/integrato
   return add(p, scl(1.0e-4f, dot(d, ng) >= 0.0f ? ng : neg(ng)));
 }
 
-// specular_ray_differential (R/integrator/mod.rs:88-138) for a reflection
-// (every delta lobe carried reflects), reflected_differential_directions
-// (:140-157), differential_is_reasonable (:208-219).
+// specular_ray_differential (R/integrator/mod.rs:88-138) with
+// reflected_differential_directions (:140-157) for reflections,
+// transmitted_differential_directions (:159-206) for refractions, and
+// differential_is_reasonable (:208-219).
 __device__ __forceinline__ bool specular_differential(const RayDiff& in, const Vtx& v, const VtxTex& t, F3 wi_world,
-                                                      RayDiff* out) {
+                                                      const Sample& s, RayDiff* out) {
   if (!in.has) return false;
   const F3 wo = v.wo, wi = normalize_or_zero(wi_world);
-  const F3 dndx = add(scl(t.dudx, t.dndu), scl(t.dvdx, t.dndv));
-  const F3 dndy = add(scl(t.dudy, t.dndu), scl(t.dvdy, t.dndv));
+  F3 dndx = add(scl(t.dudx, t.dndu), scl(t.dvdx, t.dndv));
+  F3 dndy = add(scl(t.dudy, t.dndu), scl(t.dvdy, t.dndv));
   const F3 dwodx = sub(neg(in.rxd), wo), dwody = sub(neg(in.ryd), wo);
   const F3 rxo = offset_origin(add(v.p, t.dpdx), v.ng, wi), ryo = offset_origin(add(v.p, t.dpdy), v.ng, wi);
-  const F3 n = v.ns;
-  const float dwn_dx = dot(dwodx, n) + dot(wo, dndx), dwn_dy = dot(dwody, n) + dot(wo, dndy);
-  const float wn = dot(wo, n);
-  const F3 rxd = add(sub(wi, dwodx), scl(2.0f, add(scl(wn, dndx), scl(dwn_dx, n))));
-  const F3 ryd = add(sub(wi, dwody), scl(2.0f, add(scl(wn, dndy), scl(dwn_dy, n))));
+  F3 n = v.ns;
+  F3 rxd, ryd;
+  if (!s.trans) {
+    const float dwn_dx = dot(dwodx, n) + dot(wo, dndx), dwn_dy = dot(dwody, n) + dot(wo, dndy);
+    const float wn = dot(wo, n);
+    rxd = add(sub(wi, dwodx), scl(2.0f, add(scl(wn, dndx), scl(dwn_dx, n))));
+    ryd = add(sub(wi, dwody), scl(2.0f, add(scl(wn, dndy), scl(dwn_dy, n))));
+  } else if (fabsf(s.eta - 1.0f) <= 1.0e-6f) {
+    rxd = sub(wi, dwodx);
+    ryd = sub(wi, dwody);
+  } else {
+    const float eta = s.eta;
+    if (eta <= 0.0f) return false;
+    if (dot(wo, n) < 0.0f) {
+      n = neg(n);
+      dndx = neg(dndx);
+      dndy = neg(dndy);
+    }
+    const float win = dot(wi, n);
+    if (fabsf(win) <= 1.0e-6f) return false;
+    const float dwn_dx = dot(dwodx, n) + dot(wo, dndx), dwn_dy = dot(dwody, n) + dot(wo, dndy);
+    const float wn = dot(wo, n);
+    const float mu = wn * eta - fabsf(win);
+    const float dmu_scale = eta + eta * eta * wn / win;
+    const float dmudx = dwn_dx * dmu_scale, dmudy = dwn_dy * dmu_scale;
+    rxd = add(add(sub(wi, scl(eta, dwodx)), scl(mu, dndx)), scl(dmudx, n));
+    ryd = add(add(sub(wi, scl(eta, dwody)), scl(mu, dndy)), scl(dmudy, n));
+  }
   const float kMax = 1.0e16f;
   if (!finite3(rxo) || !finite3(ryo) || !finite3(rxd) || !finite3(ryd) || len2(rxo) > kMax || len2(ryo) > kMax ||
       len2(rxd) > kMax || len2(ryd) > kMax)
@@ -1966,7 +2299,7 @@ __device__ __forceinline__ RayDiff spawn_scattered(const RayDiff& in, float hit_
   r.d = wi;
   r.cw = fmaxf(cone_width_at(in.cw, in.cs, hit_t), 0.0f);
   r.cs = fmaxf(in.cs + s.cone, 0.0f);
-  r.has = s.delta && specular_differential(in, v, t, wi, &r);
+  r.has = s.delta && specular_differential(in, v, t, wi, s, &r);
   return r;
 }
 

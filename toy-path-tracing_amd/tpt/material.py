@@ -31,11 +31,14 @@ class OpenPbrRecord(ctypes.Structure):
         ("base_color_texture", ctypes.c_uint32), ("base_metalness_texture", ctypes.c_uint32),
         ("specular_roughness_texture", ctypes.c_uint32), ("fuzz_weight_texture", ctypes.c_uint32),
         ("fuzz_roughness_texture", ctypes.c_uint32), ("emission_color_texture", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32),
+        ("transmission_color", ctypes.c_float * 3), ("transmission_dispersion_scale", ctypes.c_float),
+        ("transmission_dispersion_abbe_number", ctypes.c_float),
+        ("thin_film_thickness", ctypes.c_float), ("thin_film_ior", ctypes.c_float),
+        ("pad", ctypes.c_uint32 * 2),
     ]
 
 
-assert ctypes.sizeof(OpenPbrRecord) == 160
+assert ctypes.sizeof(OpenPbrRecord) == 192
 NO_TEXTURE = 0xFFFFFFFF
 TEXTURE_FIELDS = ("base_color_texture", "base_metalness_texture", "specular_roughness_texture",
                   "fuzz_weight_texture", "fuzz_roughness_texture", "emission_color_texture")
@@ -45,12 +48,12 @@ KIND_OPEN_PBR, KIND_NORMALIZED_LAMBERT, KIND_EMISSIVE, KIND_MIRROR = 0, 1, 2, 3
 
 class MaterialRecord(ctypes.Structure):
     """TptMaterial: kind, pad, then the kind's parameters (OpenPBR: the
-    160-byte TptOpenPbr; Lambert: rho[3]; Emissive: color[3], strength;
+    192-byte TptOpenPbr; Lambert: rho[3]; Emissive: color[3], strength;
     Mirror: color[3])."""
-    _fields_ = [("kind", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("params", ctypes.c_float * 40)]
+    _fields_ = [("kind", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("params", ctypes.c_float * 48)]
 
 
-assert ctypes.sizeof(MaterialRecord) == 168
+assert ctypes.sizeof(MaterialRecord) == 200
 
 
 @dataclass(frozen=True)
@@ -78,6 +81,11 @@ class OpenPbr:
     coat_darkening: float = 1.0
     fuzz_color: tuple = (1.0, 1.0, 1.0)
     fuzz_roughness: float = 0.5
+    transmission_color: tuple = (1.0, 1.0, 1.0)
+    transmission_dispersion_scale: float = 0.0
+    transmission_dispersion_abbe_number: float = 20.0
+    thin_film_thickness: float = 0.5
+    thin_film_ior: float = 1.4
     # indices into the scene's texture table (Scene.add_texture), None: no
     # texture (R/material/open_pbr.rs:66-72)
     base_color_texture: int | None = None

@@ -475,6 +475,72 @@ def cornell_textured(build: bool = True) -> CornellScene:
     return CornellScene(s, mats, li, cam)
 
 
+def cornell_glass(build: bool = True, textured: bool = False, light_size: float = 0.2,
+                  light_radiance: float = 15.0) -> CornellScene:
+    """The Cornell box with OpenPBR transmission: the short box is smooth
+    glass (IOR 1.5: delta refraction and reflection, total internal
+    reflection inside), the tall box rough tinted glass (roughness 0.35,
+    IOR 1.33, transmission colour (0.9, 0.75, 0.5): the GGX BTDF with its
+    multiple-scattering lobe), and a thin-walled frosted pane (roughness
+    0.3, transmission weight 0.9 over a white base) hangs in front of the
+    back wall.  The walls carry the reference's OpenPbrMaterial::new
+    defaults (glossy); the light is OpenPBR emission 15.  textured: the
+    floor's base colour is a tiled checkerboard (64 x 64, uv x 4), so the
+    path tracers run their textured instantiation, whose ray differentials
+    refract through the smooth glass.  Materials: 0 walls, 1 red, 2 green,
+    3 smooth glass, 4 rough glass, 5 pane, 6 light, 7 floor.  light_size /
+    light_radiance: the ceiling light's side and radiance (a large dim light
+    keeps caustics through the glass from being rare fireflies, which the
+    finite-difference checks of the adjoint need)."""
+    from .material import OpenPbr
+    from .texture import checkerboard
+
+    s = Scene()
+    floor_m = flat_quads([[(0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)]])
+    floor_m.uvs = floor_m.uvs * np.float32(4.0)
+    walls = s.add_mesh(flat_quads([
+        [(0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)],  # ceiling y=1, normal -y
+        [(0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)],  # back wall z=1, normal -z
+    ]))
+    floor = s.add_mesh(floor_m)
+    red = s.add_mesh(flat_quads([[(0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)]]))
+    green = s.add_mesh(flat_quads([[(1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)]]))
+    box = s.add_mesh(unit_box())
+    pane = s.add_mesh(flat_quads([[(0.55, 0.35, 0.9), (0.55, 0.8, 0.9), (0.95, 0.8, 0.9), (0.95, 0.35, 0.9)]]))
+    lo, hi = 0.5 - 0.5 * light_size, 0.5 + 0.5 * light_size
+    light = s.add_mesh(flat_quads([[(lo, 0.999, lo), (hi, 0.999, lo), (hi, 0.999, hi), (lo, 0.999, hi)]]))
+    eye4 = np.eye(4)
+    s.add_instance(walls, 0, eye4)
+    s.add_instance(floor, 7, eye4)
+    s.add_instance(red, 1, eye4)
+    s.add_instance(green, 2, eye4)
+    s.add_instance(box, 3, _translate(0.68, 0.15, 0.36) @ _rot_y(-18.0) @ _scale(0.3, 0.3, 0.3))
+    s.add_instance(box, 4, _translate(0.33, 0.3, 0.64) @ _rot_y(15.0) @ _scale(0.3, 0.6, 0.3))
+    s.add_instance(pane, 5, eye4)
+    li = s.add_instance(light, 6, eye4)
+    tex = None
+    if textured:
+        tex = s.add_texture(checkerboard(64, 64, 8, (0.9, 0.85, 0.8), (0.15, 0.2, 0.3)))
+    if build:
+        s.build_qbvh()
+    mats = [
+        OpenPbr.new((0.8, 0.8, 0.8)),
+        OpenPbr.new((0.8, 0.1, 0.1)),
+        OpenPbr.new((0.1, 0.8, 0.1)),
+        OpenPbr(base_weight=0.0, transmission_weight=1.0, specular_roughness=0.0, specular_ior=1.5),
+        OpenPbr(base_weight=0.0, transmission_weight=1.0, specular_roughness=0.35, specular_ior=1.33,
+                transmission_color=(0.9, 0.75, 0.5)),
+        OpenPbr(base_color=(0.9, 0.9, 0.9), transmission_weight=0.9, specular_roughness=0.3, thin_walled=True),
+        OpenPbr.new((0.0, 0.0, 0.0)).with_(base_weight=0.0, specular_weight=0.0,
+                                           emission_luminance=float(light_radiance), emission_color=(1.0, 1.0, 1.0)),
+        OpenPbr.new((1.0, 1.0, 1.0) if textured else (0.8, 0.8, 0.8)).with_(base_color_texture=tex),
+    ]
+    for m in mats:
+        s.add_material(m)
+    cam = dict(eye=(0.5, 0.5, -1.4), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov_y_deg=40.0)
+    return CornellScene(s, mats, li, cam)
+
+
 def cornell_inverse(build: bool = True, seed: int = 0, tex_size: int = 512, glossy: bool = False) -> CornellScene:
     """The inverse-rendering Cornell box: the Cornell geometry split into 16
     OpenPBR materials (five walls, the six faces of the short box, five faces
