@@ -386,8 +386,9 @@ def render_bench(dev, stream, args):
 
 
 def relmse(x, ref):
-    """Mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2)."""
-    return float(((x - ref) ** 2 / (ref * ref + 1e-2)).mean())
+    """Mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2) (tpt.imageio.relmse)."""
+    from tpt.imageio import relmse as _relmse
+    return _relmse(x, ref)
 
 
 def policy_comparison(rds, stream, frame, render, ref_spp):

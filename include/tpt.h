@@ -185,7 +185,7 @@ typedef struct TptMaterial {
     struct { float rho[3]; } lambert;
     struct { float color[3]; float strength; } emissive;
     struct { float color[3]; } mirror;
-    float raw[40];
+    float raw[48];
   } u;
 } TptMaterial; /* 200 bytes */
 
@@ -300,12 +300,18 @@ int32_t tpt_device_synchronize(TptDevice* device);
  *                          lanes when fewer than this many lanes are busy
  *   "blocks_per_cu" 0..64  cap on resident 256-thread blocks per CU (0: the
  *                          occupancy limit)
+ *   "lds_depth"     4..32  traversal stack entries per lane kept in LDS (the
+ *                          rest, up to the reference's 128, in scratch;
+ *                          default 16; tpt_trace_noalloc keeps 16)
  * and of the wavefront path tracer (tpt_render_batch):
  *   "wf_paths"  65536..2^26  paths in flight per pass (default 2^21)
  *   "wf_graph"  0 or 1       capture one pass as a HIP graph and replay it
- *   "wf_sort"   0 or 1       shade each bounce's hits grouped by material
- *                            (a counting sort of the hit queue; scenes of at
- *                            most 255 materials; the image does not change) */
+ *   "wf_tile"   128, 256, 512  the shade queue's tile: threads per generate /
+ *                            shade block, one queue entry each (default 256)
+ *   "wf_sort"   0..64        shade each bounce's hits grouped by material:
+ *                            0 in queue order, g sorted by material index / g
+ *                            (a counting sort of the hit queue into at most
+ *                            256 keys; the image does not change) */
 int32_t tpt_device_set_param(TptDevice* device, const char* name, int64_t value);
 
 /* ---- sampler and camera --------------------------------------------------- */
@@ -359,6 +365,19 @@ int32_t tpt_light_tree_build(const TptSceneDesc* desc, TptLightNode* out, uint32
  * Scene::build_light_tree does (R/scene/mod.rs:300-303; area-light table in
  * instance order, R/light_tree/builder.rs:58-121). */
 TptScene* tpt_scene_create(TptDevice* device, const TptSceneDesc* desc);
+
+/* The scene description as an on-disk cache (no reference counterpart; the
+ * reference rebuilds its scene each run, R/main.rs:60-97): save writes every
+ * array of desc (meshes with their built QBVHs, instances, TLAS, materials,
+ * environment, textures, directional lights) to one file with a format
+ * version, the record sizes and a checksum; load reads it back into a
+ * descriptor that tpt_scene_create accepts, owning one allocation that
+ * tpt_scene_desc_free releases.  Load returns NULL (tpt_last_error says
+ * why) for a missing, truncated or corrupt file, another format version or
+ * other record layouts. */
+int32_t tpt_scene_desc_save(const TptSceneDesc* desc, const char* path);
+TptSceneDesc* tpt_scene_desc_load(const char* path);
+void tpt_scene_desc_free(TptSceneDesc* desc);
 void tpt_scene_release(TptScene* scene);
 /* Device bytes held by the scene (nodes, quantised nodes, triangles, instances). */
 size_t tpt_scene_device_bytes(const TptScene* scene);
@@ -520,7 +539,10 @@ typedef struct TptFrameDesc {
 int32_t tpt_render_batch(TptScene* scene, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags);
 
 /* Autotuner of tpt_render_batch for a workload: coordinate descent over the
- * device's "wf_paths", "blocks_per_cu", "refill_below" and "wf_sort" parameters, each
+ * device's "wf_paths", "blocks_per_cu", "refill_below", "lds_depth" (the
+ * traversal stack's entries in LDS), "wf_tile" (the shade queue's tile:
+ * threads per generate / shade block) and "wf_sort" (the material sort's
+ * granularity: 0 off, g materials per sort key) parameters, each
  * candidate timed with HIP events on the given frames (samples per pixel
  * capped at tune_spp, 0: as given; outputs go to an internal buffer).  A
  * candidate replaces the current setting only when at least 1% faster, and
@@ -536,6 +558,8 @@ typedef struct TptTuneResult {
   uint32_t wf_sort;
   float ms_default;  /* the starting setting */
   float ms_best;
+  uint32_t lds_depth;
+  uint32_t wf_tile;
 } TptTuneResult;
 int32_t tpt_autotune_render(TptScene* scene, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags,
                             uint32_t tune_spp, TptTuneResult* out);

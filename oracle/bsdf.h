@@ -282,6 +282,8 @@ struct BSample {
   bool transmission = false;  // BsdfFlags::TRANSMISSION
   real eta = 1.0f;           // BsdfSample::eta (relative IOR of a refraction, 1 otherwise)
   real cone_spread = 0.0f;  // MaterialSample::cone_spread (set by the material)
+  int lobe = -1;            // OpenPBR: the picked lobe (bsdf_sample's numbering) and its probability
+  real p_lobe = 0.0f;
 };
 // sample_cosine_weighted_hemisphere, R/math/mod.rs:26-36
 inline V3 cosine_hemisphere(real u, real v) {
@@ -1517,6 +1519,7 @@ inline bool sample_two_lobe(const PbrBsdf& p, V3 wo, real ax, real ay, TwoLobe w
   if (wi.z <= 0.0f) return false;
   return finalize(p, wo, wi, out);
 }
+inline bool sample_lobe(const PbrBsdf& p, V3 wo, int lobe, real p_lobe, real ux, real uy, BSample* out);
 // OpenPbrBsdf::sample (:295-331) with pick_lobe (:2016-2041)
 inline bool bsdf_sample(const PbrBsdf& p, V3 wo, real u_lobe, real ux, real uy, BSample* out) {
   if (!(wo.z > 0.0f)) return false;
@@ -1549,6 +1552,12 @@ inline bool bsdf_sample(const PbrBsdf& p, V3 wo, real u_lobe, real ux, real uy, 
   }
   real p_lobe = plv / pr.total;
   if (p_lobe <= 0.0f) return false;
+  return sample_lobe(p, wo, lobe, p_lobe, ux, uy, out);
+}
+// The picked lobe's sample (the lobe branches of OpenPbrBsdf::sample, :295-331).
+inline bool sample_lobe(const PbrBsdf& p, V3 wo, int lobe, real p_lobe, real ux, real uy, BSample* out) {
+  out->lobe = lobe;
+  out->p_lobe = p_lobe;
   out->glossy = lobe != 5 && lobe != 6;  // delta samples carry DELTA instead (read with !delta)
   out->transmission = false;
   out->eta = 1.0f;

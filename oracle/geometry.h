@@ -664,12 +664,14 @@ bool unoccluded_ray(const Scene& s, V3 o, V3 d, real distance, int64_t target_in
 
 template <class F>
 void parallel_for(size_t n, int threads, F&& f) {
-  if (threads <= 1 || n < 1024) {
+  if (threads <= 1 || n < 2) {
     for (size_t i = 0; i < n; ++i) f(i);
     return;
   }
   std::atomic<size_t> next{0};
-  const size_t chunk = 256;
+  // chunks of up to 256 items, at least eight per thread (a small crop of
+  // heavy pixels still spreads over every thread)
+  const size_t chunk = std::max<size_t>(1, std::min<size_t>(256, n / (8 * (size_t)threads)));
   std::vector<std::thread> pool;
   for (int t = 0; t < threads; ++t)
     pool.emplace_back([&]() {

@@ -586,8 +586,11 @@ inline real category_pmf(const RenderScene& rs, int kind) {  // :215-224
 // sample_light_mis_compensated_lazy (R/light/mod.rs:289-331),
 // sample_li_for_triangle (R/light/area.rs:12-54) and the environment's
 // MIS-compensated sample (R/light/environment.rs:260-276).
+// em (optional): the material whose BSDF value the contribution carries,
+// every decision and density staying those of m (detached evaluation, see
+// trace_radiance).
 inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& vtx, real u_root, real u_tree,
-                       real u_aux, real us0, real us1) {
+                       real u_aux, real us0, real us1, const Material* em = nullptr) {
   (void)u_aux;
   const V3 zero = v3(0.f, 0.f, 0.f);
   Category cat;
@@ -657,6 +660,7 @@ inline V3 direct_light(const RenderScene& rs, const Material& m, const Vertex& v
     return zero;
   real pdf_total = selection * li_pdf;
   if (pdf_total <= 0.0f) return zero;
+  if (em) f = mat_eval(*em, vtx, wi);
   if (cat.kind >= 2) return smul(vmul(le, f), cos_surface / pdf_total);  // delta light: no MIS
   real mis = balance_heuristic(pdf_total, bsdf_pdf);
   return smul(vmul(le, f), cos_surface * mis / pdf_total);
@@ -773,26 +777,64 @@ inline Ray spawn_scattered(const Ray& in, real hit_t, const Vertex& v, const BSa
   return r;
 }
 
+// The sampled direction's weight with the BSDF value taken from another
+// material mb and the sampling density of the path's own sample s: mb's
+// colour for NormalizedLambert and Mirror (whose weights are their colour);
+// for OpenPBR f_b(wi) |cos| / s.pdf on the rough lobes (finalize's weight),
+// the lobe-only value of a rough thin-walled transmission over s.pdf (that
+// sample's weight), and on a delta lobe mb's numerator of the same lobe over
+// s's lobe probability.  A delta lobe that is not delta under mb yields NaN.
+inline V3 detached_weight(const Material& mb, const Vertex& v, const BSample& s) {
+  if (mb.kind == kMatLambert || mb.kind == kMatMirror) return mb.color;
+  if (mb.kind != kMatOpenPbr) return vsplat((real)NAN);
+  V3 wol = normalize_or_zero3(onb_to_local(v.frame, v.wo));
+  const PbrBsdf pb = make_bsdf(mb.pbr, v.front_face, wol, mb.luts);
+  if (s.delta) {
+    BSample t;
+    if (!sample_lobe(pb, wol, s.lobe, s.p_lobe, 0.0f, 0.0f, &t) || !t.delta) return vsplat((real)NAN);
+    return t.weight;
+  }
+  if (s.transmission && pb.thin_walled) {
+    V3 wi = s.wi;
+    LayerW w = layer_weights(pb, wol, &wi);
+    return sdiv(smul(vmul(w.spec_btdf, eval_thin_btdf(pb, wol, s.wi)), std::fabs(s.wi.z)), s.pdf);
+  }
+  return smul(bsdf_eval(pb, wol, s.wi), std::fabs(s.wi.z) / s.pdf);
+}
+
 // mis::trace_radiance (R/integrator/mis.rs:19-139) for a camera ray with its
 // differential and cone.
-inline V3 trace_radiance(const RenderScene& rs, const Ray& camera_ray, const PathSampler& ps, uint32_t max_depth) {
+//
+// ev (optional): detached evaluation, the estimator a path-replay adjoint
+// differentiates.  ev is the same geometry with other materials and
+// textures; every sampling decision, density, MIS weight and Russian-roulette
+// probability is that of rs's path, and the BSDF values (the sampled
+// directions' weights, next-event contributions) and the emission are ev's.
+// With ev's materials equal to rs's the result is rs's radiance.  A central
+// difference over ev's parameters of this estimator is the derivative of the
+// path's contribution with the sampling held fixed.
+inline V3 trace_radiance(const RenderScene& rs, const Ray& camera_ray, const PathSampler& ps, uint32_t max_depth,
+                         const RenderScene* ev = nullptr) {
   V3 radiance = v3(0.f, 0.f, 0.f), throughput = v3(1.f, 1.f, 1.f);
+  V3 value = throughput;  // ev's throughput (throughput itself without ev)
   SceneHit h;
   if (!closest_hit(*rs.geo, camera_ray.o, camera_ray.d, INFINITY, &h, nullptr)) return env_le(rs, camera_ray.d);
   Ray ray = camera_ray;
   real hit_t = h.t;
   Vertex vtx = shading_vertex(rs, h, ray);
   Material mat = material_at(rs, rs.materials[vtx.material], vtx);
+  Material emat = ev ? material_at(*ev, ev->materials[vtx.material], vtx) : mat;
   V3 le;
-  if (mat_le(mat, vtx, &le)) radiance = add(radiance, le);
+  if (mat_le(emat, vtx, &le)) radiance = add(radiance, le);
   const uint32_t rr_start = 4;
   for (uint32_t depth = 0; depth < max_depth; ++depth) {
     const uint32_t base = 2u + depth * 16u;
     if (!mat_is_pure_emitter(mat)) {
       float us[2];
       ps.s2(base + 3, us);
-      V3 dl = direct_light(rs, mat, vtx, ps.s1(base + 0), ps.s1(base + 1), ps.s1(base + 2), us[0], us[1]);
-      radiance = add(radiance, vmul(throughput, dl));
+      V3 dl = direct_light(rs, mat, vtx, ps.s1(base + 0), ps.s1(base + 1), ps.s1(base + 2), us[0], us[1],
+                           ev ? &emat : nullptr);
+      radiance = add(radiance, vmul(ev ? value : throughput, dl));
     }
     BSample s;
     V3 wi;
@@ -801,22 +843,27 @@ inline V3 trace_radiance(const RenderScene& rs, const Ray& camera_ray, const Pat
     if (!mat_sample(mat, vtx, ps.s1(base + 5), udir[0], udir[1], &s, &wi)) break;
     const bool delta = s.delta;
     throughput = vmul(throughput, s.weight);
+    if (ev) value = vmul(value, detached_weight(emat, vtx, s));
     if (depth + 1 >= rr_start) {
       real q = clampf(rmax(throughput.x, rmax(throughput.y, throughput.z)), 0.05f, 1.0f);
       if (ps.s1(base + 13) > q) break;
       throughput = sdiv(throughput, q);
+      value = sdiv(value, q);
     }
+    const V3 carried = ev ? value : throughput;
     const Ray next = spawn_scattered(ray, hit_t, vtx, s, wi);
     SceneHit nh;
     if (!closest_hit(*rs.geo, next.o, next.d, INFINITY, &nh, nullptr)) {
-      radiance = add(radiance, emitted_infinite(rs, throughput, s.pdf, delta, wi));
+      radiance = add(radiance, emitted_infinite(rs, carried, s.pdf, delta, wi));
       break;
     }
     Vertex nv = shading_vertex(rs, nh, next);
     const Material nm = material_at(rs, rs.materials[nv.material], nv);
-    radiance = add(radiance, emitted_from_bsdf_sample(rs, throughput, s.pdf, delta, mat, vtx, nv, nm));
+    const Material enm = ev ? material_at(*ev, ev->materials[nv.material], nv) : nm;
+    radiance = add(radiance, emitted_from_bsdf_sample(rs, carried, s.pdf, delta, mat, vtx, nv, enm));
     vtx = nv;
     mat = nm;
+    emat = enm;
     ray = next;
     hit_t = nh.t;
   }

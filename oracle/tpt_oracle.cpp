@@ -649,10 +649,14 @@ void orc_sg_product(const float* a1, float s1, const float* a2, float s2, float*
 // a w x hgt frame (every pixel's samples depend on the full frame size only),
 // written row-major into a (y1-y0) x (x1-x0) crop of 3 values per pixel.
 // cam: eye, forward, right, image_up, tan_half_fov_y, pixel_spread, exposure.
+// ev (optional): a renderer over the same geometry whose materials and
+// textures give the BSDF values (trace_radiance's detached evaluation).
 template <class Out>
 static void render_rect(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth,
-                        uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, Out* rgb, int threads) {
+                        uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, Out* rgb, int threads,
+                        const void* ev = nullptr) {
   const RenderScene& rs = *(RenderScene*)h;
+  const RenderScene* evs = (const RenderScene*)ev;
   CF3 eye{cam[0], cam[1], cam[2]}, fwd{cam[3], cam[4], cam[5]}, right{cam[6], cam[7], cam[8]},
       up{cam[9], cam[10], cam[11]};
   const float th = cam[12], spread = cam[13], exposure = cam[14];
@@ -688,7 +692,7 @@ static void render_rect(void* h, const float* cam, uint32_t w, uint32_t hgt, uin
       ray.rxd = v3(rxd.x, rxd.y, rxd.z);
       ray.ryd = v3(ryd.x, ryd.y, ryd.z);
       ray.cone = make_cone(0.0f, spread * sc);
-      V3 s = trace_radiance(rs, ray, ps, max_depth);
+      V3 s = trace_radiance(rs, ray, ps, max_depth, evs);
       const float cnt = (float)(si + 1);
       color = add(color, sdiv(sub(s, color), cnt));
     }
@@ -708,6 +712,16 @@ void orc_render_rect(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32
 void orc_render_rect_f64(void* h, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp, uint32_t max_depth,
                          uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, double* rgb, int threads) {
   render_rect(h, cam, w, hgt, spp, max_depth, x0, x1, y0, y1, rgb, threads);
+}
+// render_rect_f64 with detached evaluation: h's paths (every decision and
+// density), ev's BSDF values and emission.  ev must hold the same geometry
+// (the caller builds both renderers from one scene, with other materials
+// or textures); a delta or thin-walled transmission sample of an OpenPBR
+// material makes the pixel NaN.
+void orc_render_rect_detached_f64(void* h, const void* ev, const float* cam, uint32_t w, uint32_t hgt, uint32_t spp,
+                                  uint32_t max_depth, uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1,
+                                  double* rgb, int threads) {
+  render_rect(h, cam, w, hgt, spp, max_depth, x0, x1, y0, y1, rgb, threads, ev);
 }
 
 // Rows [y0, y1) of the frame into the full-frame buffer rgb (w * hgt * 3).

@@ -8,13 +8,9 @@ import numpy as np
 
 import oracle
 from tpt import camera
+from tpt.imageio import relmse
 
 PATH = Path(__file__).resolve().parent / "golden" / "reference_images.npz"
-
-
-def relmse(x, ref):
-    """Mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2)."""
-    return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
 
 
 def reference(name, scene):

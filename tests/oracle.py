@@ -134,6 +134,7 @@ def lib(fp64: bool = False) -> ctypes.CDLL:
         L.orc_unoccluded.argtypes = [vp, vp, sz, vp, vp, vp, vp, ctypes.c_int]
         L.orc_unoccluded.restype = ctypes.c_int
         L.orc_render_rect_f64.argtypes = [vp, vp] + [ctypes.c_uint32] * 8 + [vp, ctypes.c_int]
+        L.orc_render_rect_detached_f64.argtypes = [vp, vp, vp] + [ctypes.c_uint32] * 8 + [vp, ctypes.c_int]
         _libs[fp64] = L
     return _libs[fp64]
 
@@ -399,6 +400,7 @@ class OracleRenderer:
     def __init__(self, scene, materials, nee_policy: str = "reference", fp64: bool = False):
         from tpt.material import records
         self._L = lib(fp64)
+        self.fp64 = fp64
         self.geo = oracle_scene_from(scene, fp64)
         self.h = self._L.orc_render_scene_new(self.geo.h, len(scene.meshes))
         self._keep = []
@@ -541,6 +543,23 @@ class OracleRenderer:
                      + [cam.tan_half_fov_y, cam.pixel_spread, cam.exposure], np.float32)
         out = np.zeros((y1 - y0, x1 - x0, 3), np.float64)
         self._L.orc_render_rect_f64(self.h, _p(c), width, height, spp, max_depth, x0, x1, y0, y1, _p(out), threads)
+        return out
+
+    def render_rect_detached_f64(self, evaluate, cam, width, height, spp, max_depth, x0, x1, y0, y1, threads=8):
+        """render_rect_f64 with detached evaluation: this renderer's paths
+        (sampling decisions, densities, MIS weights, roulette) with the BSDF
+        values and emission of `evaluate`, an OracleRenderer of the same
+        build over the same geometry with other materials or textures.  Its
+        central differences over `evaluate`'s parameters are the derivative a
+        path-replay adjoint computes (sampling held at this renderer's
+        parameters)."""
+        if evaluate.fp64 != self.fp64:
+            raise ValueError("both renderers must come from the same oracle build")
+        c = np.array(list(cam.eye) + list(cam.forward) + list(cam.right) + list(cam.image_up)
+                     + [cam.tan_half_fov_y, cam.pixel_spread, cam.exposure], np.float32)
+        out = np.zeros((max(0, y1 - y0), max(0, x1 - x0), 3), np.float64)
+        self._L.orc_render_rect_detached_f64(self.h, evaluate.h, _p(c), width, height, spp, max_depth, x0, x1, y0, y1,
+                                             _p(out), threads)
         return out
 
     def __del__(self):

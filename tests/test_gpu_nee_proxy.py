@@ -27,12 +27,9 @@ import golden_images
 import oracle
 from tpt import camera, scenes
 from tpt.autograd import with_params
+from tpt.imageio import relmse
 
 pytestmark = pytest.mark.gpu
-
-
-def relmse(x, ref):
-    return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
 
 
 def bits_equal(a, b):

@@ -22,12 +22,9 @@ import oracle
 import tpt
 from tpt import camera, scenes
 from tpt.material import OpenPbr
+from tpt.imageio import relmse
 
 pytestmark = pytest.mark.gpu
-
-
-def relmse(x, ref):
-    return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
 
 
 @pytest.fixture(scope="module", params=[0.0, 1.0], ids=["diffuse", "glossy"])

@@ -19,6 +19,7 @@ from tpt import _lib, camera, scenes
 from tpt.material import OpenPbr
 from tpt.scene import Scene
 from tpt.texture import Texture
+from tpt.imageio import relmse
 
 pytestmark = pytest.mark.gpu
 
@@ -125,8 +126,6 @@ def test_invalid_texture_slots_are_rejected(device, bad, what):
 
 
 # ---- the textured path tracer ------------------------------------------------------------------
-def relmse(x, ref):
-    return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
 
 
 @pytest.fixture(scope="module")

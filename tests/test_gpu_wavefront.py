@@ -126,11 +126,37 @@ def test_autotuner_picks_a_setting_no_slower_and_keeps_the_image(device, scene):
         assert res["ms_best"] <= res["ms_default"] and res["trials"] >= 10
         assert res["wf_paths"] in (1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
         assert res["blocks_per_cu"] in (0, 2, 3, 4, 6, 8) and res["refill_below"] in (8, 16, 24, 32, 48)
-        assert res["wf_sort"] in (0, 1)
+        assert res["wf_sort"] in (0, 1, 2, 4, 8)
+        assert res["lds_depth"] in (8, 12, 16, 24, 32) and res["wf_tile"] in (128, 256, 512)
         (img,) = ds.render_batch([(cam, w, h, 8, 8)], exact_shadow=True)
         assert bits_equal(img, ref)
     finally:
-        device.set_param("wf_paths", 1 << 21)
-        device.set_param("blocks_per_cu", 4)
-        device.set_param("refill_below", 24)
-        device.set_param("wf_sort", 0)
+        for name, value in DEFAULTS.items():
+            device.set_param(name, value)
+
+
+DEFAULTS = dict(wf_paths=1 << 21, blocks_per_cu=0, refill_below=40, wf_sort=0, lds_depth=16, wf_tile=256)
+
+
+@pytest.mark.parametrize("name,value", [("lds_depth", 4), ("lds_depth", 32), ("wf_tile", 128), ("wf_tile", 512),
+                                        ("wf_sort", 2), ("wf_sort", 8)])
+def test_tuning_knobs_do_not_change_the_image(device, scene, name, value):
+    """The traversal stack's LDS depth, the shade queue's tile and the sort's
+    granularity change only how the work is scheduled: the image stays the
+    megakernel's bit for bit."""
+    cs, ds = scene
+    w = h = 80
+    cam = camera.pinhole(**cs.camera, width=w, height=h)
+    ref = ds.render(cam, w, h, 12, 8)
+    device.set_param(name, value)
+    try:
+        (img,) = ds.render_batch([(cam, w, h, 12, 8)], exact_shadow=True)
+    finally:
+        device.set_param(name, DEFAULTS[name])
+    assert bits_equal(img, ref)
+
+
+def test_tuning_knob_ranges_are_checked(device):
+    for name, value in [("lds_depth", 3), ("lds_depth", 33), ("wf_tile", 64), ("wf_tile", 300), ("wf_sort", 65)]:
+        with pytest.raises(tpt._lib.TptError, match="out of range"):
+            device.set_param(name, value)

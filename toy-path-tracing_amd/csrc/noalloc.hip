@@ -47,6 +47,11 @@ __global__ __launch_bounds__(256) void noalloc_clear(uint32_t* p, uint32_t words
   for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) p[i] = 0u;
 }
 
+// The traversal stack keeps the fixed kLdsDepth entries per lane in LDS here
+// (not the device's tuned depth), so the resident blocks looked up at set-up
+// stay valid.
+constexpr size_t kNoallocLds = (size_t)kLdsDepth * kBlock * 4;
+
 // Kernel variants: query 0 closest, 1 early-exit shadow, 2 exact-rule shadow;
 // FP32 or 8-bit nodes; one or two levels.
 constexpr int kVariants = 12;
@@ -82,7 +87,7 @@ bool noalloc_setup(TptDevice* d) {
   d->noalloc_scratch = ptr;
   for (int v = 0; v < kVariants; ++v) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_kernel(v), kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_kernel(v), kBlock, kNoallocLds) != hipSuccess ||
         per_cu < 1)
       per_cu = 1;
     d->noalloc_per_cu[v] = per_cu;
@@ -156,13 +161,14 @@ extern "C" int32_t tpt_trace_noalloc(TptScene* s, uint32_t query, const TptRay* 
     qr.refill_below = d->refill_below;
     qr.grab = d->refill_grab;
     qr.regions = d->regions;
+    qr.lds_depth = kLdsDepth;
     int per_cu = d->noalloc_per_cu[v];
     if (d->blocks_per_cu && (int)d->blocks_per_cu < per_cu) per_cu = (int)d->blocks_per_cu;
     const size_t want = (n + kBlock - 1) / kBlock;
     const size_t cap = (size_t)per_cu * (size_t)(d->cu_count > 0 ? d->cu_count : 1);
     const unsigned blocks = (unsigned)(want < cap ? want : cap);
     void* args[] = {(void*)&sv, (void*)&rays, (void*)&qr};
-    if (!hip_ok(hipLaunchKernel(variant_kernel(v), dim3(blocks), dim3(kBlock), args, 0, d->stream),
+    if (!hip_ok(hipLaunchKernel(variant_kernel(v), dim3(blocks), dim3(kBlock), args, kNoallocLds, d->stream),
                 "hipLaunchKernel(trace)"))
       return TPT_ERROR_DEVICE;
     return hip_ok(hipGetLastError(), "kernel launch") ? TPT_OK : TPT_ERROR_DEVICE;

@@ -268,6 +268,7 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void render_kernel(SceneView sc
   st.lds = lds + threadIdx.x;
   st.spill = spill;
   st.sp = 0;
+  st.lds_depth = kLdsDepth;
   st.overflow = false;
   Lane L;
   bool overflow = false;
@@ -310,6 +311,7 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_kernel(SceneView s
     st.lds = lds + threadIdx.x;
     st.spill = spill;
     st.sp = 0;
+    st.lds_depth = kLdsDepth;
     st.overflow = false;
     Lane L;
     bool overflow = false;
@@ -322,7 +324,10 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_kernel(SceneView s
     adj.texgrad = texgrad;
     adj.rows = rows;
     const Adjoint<Acc> none{};
-    for (uint32_t si = 0; si < spp; ++si) {
+    // a pixel whose image gradient is zero contributes nothing (a loss over
+    // a crop or a mask replays only the pixels it reads)
+    const uint32_t n_samples = (adj.w.x != 0.0f || adj.w.y != 0.0f || adj.w.z != 0.0f) ? spp : 0u;
+    for (uint32_t si = 0; si < n_samples; ++si) {
       const smp::PathSampler ps{zf, x, y, si};
       const RayDiff ray = camera_ray<kTex>(cam, ps, x, y, w, h, spp);
       adj.total = trace_path<kTwoLevel, false, kTex>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
@@ -412,6 +417,7 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_batch_kernel(Scene
   st.lds = lds + threadIdx.x;
   st.spill = spill;
   st.sp = 0;
+  st.lds_depth = kLdsDepth;
   st.overflow = false;
   Lane L;
   bool overflow = false;
@@ -430,8 +436,10 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_batch_kernel(Scene
     adj.w = f3(fr.dimg[3 * lp] * scale, fr.dimg[3 * lp + 1] * scale, fr.dimg[3 * lp + 2] * scale);
   }
   const Adjoint<Acc, Acc> none{};
+  // a pixel whose image gradient is zero contributes nothing
+  const bool replay = live && (adj.w.x != 0.0f || adj.w.y != 0.0f || adj.w.z != 0.0f);
   for (uint32_t si = 0; si < max_spp; ++si) {
-    if (live && si < fr.spp) {
+    if (replay && si < fr.spp) {
       const smp::PathSampler ps{fr.zf, x, y, si};
       const RayDiff ray = camera_ray<kTex>(fr.cam, ps, x, y, fr.w, fr.h, fr.spp);
       adj.total = trace_path<kTwoLevel, false, kTex>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, none);

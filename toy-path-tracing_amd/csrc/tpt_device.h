@@ -125,11 +125,16 @@ struct TptDevice {
   uint32_t refill_grab = 0;           // ray ids a wave takes per work-counter atomic (at least its idle lanes)
   uint32_t regions = 8;               // work regions (multiple of the 8 XCDs keeps block b on region b's XCD)
   uint32_t blocks_per_cu = 0;         // resident blocks per CU (0: occupancy limit)
+  uint32_t lds_depth = 16;            // traversal stack entries per lane kept in LDS (the rest in scratch)
   // wavefront path tracer: paths in flight per pass, graph capture of a pass,
-  // and its buffers (grow-only)
+  // the shade queue's tile (threads per generate / shade block, one queue
+  // entry each), the material sort's granularity (0: shade in queue order;
+  // g: the hits sorted by material index / g, misses last), and its buffers
+  // (grow-only)
   uint32_t wf_paths = 1u << 21;
   uint32_t wf_graph = 1;
-  uint32_t wf_sort = 0;  // shade each bounce's hits grouped by material (counting sort of the queue)
+  uint32_t wf_tile = 256;
+  uint32_t wf_sort = 0;
   tpt::DeviceBuffer wf_buf;
   // tpt_trace_noalloc: this device's address of the file-scope scratch array
   // (looked up once by noalloc_setup) and the resident blocks per CU of its

@@ -325,7 +325,11 @@ int32_t tpt_device_set_param(TptDevice* d, const char* name, int64_t value) {
     d->wf_paths = (uint32_t)value;
   } else if (n == "wf_graph" && (value == 0 || value == 1)) {
     d->wf_graph = (uint32_t)value;
-  } else if (n == "wf_sort" && (value == 0 || value == 1)) {
+  } else if (n == "lds_depth" && value >= kMinLdsDepth && value <= kMaxLdsDepth) {
+    d->lds_depth = (uint32_t)value;
+  } else if (n == "wf_tile" && (value == 128 || value == 256 || value == 512)) {
+    d->wf_tile = (uint32_t)value;
+  } else if (n == "wf_sort" && value >= 0 && value <= 64) {
     d->wf_sort = (uint32_t)value;
   } else {
     set_error("tpt_device_set_param: unknown parameter or value out of range: " + n);
@@ -704,19 +708,20 @@ const void* pick_kernel(int query, bool q8, bool exact, bool cnt, bool two) {
 }
 
 // Persistent grid: enough resident blocks to fill every CU (or the tuned
-// blocks per CU), never more blocks than the rays need.
-unsigned grid_blocks(TptDevice* d, const void* kern, size_t n) {
+// blocks per CU), never more blocks than the rays need.  lds: the launch's
+// dynamic LDS bytes (the traversal stack).
+unsigned grid_blocks(TptDevice* d, const void* kern, size_t n, size_t lds) {
   static std::mutex mu;
-  static std::vector<std::pair<const void*, int>> cache;
+  static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
   int per_cu = 0;
   {
     std::lock_guard<std::mutex> lock(mu);
     for (auto& e : cache)
-      if (e.first == kern) per_cu = e.second;
+      if (e.first.first == kern && e.first.second == lds) per_cu = e.second;
     if (per_cu == 0) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess || per_cu < 1)
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
-      cache.emplace_back(kern, per_cu);
+      cache.emplace_back(std::make_pair(kern, lds), per_cu);
     }
   }
   if (d->blocks_per_cu && (int)d->blocks_per_cu < per_cu) per_cu = (int)d->blocks_per_cu;
@@ -803,12 +808,14 @@ int32_t trace_impl(TptScene* s, int query, const TptRay* rays, size_t n, const u
   q.refill_below = d->refill_below;
   q.grab = d->refill_grab;
   q.regions = d->regions;
+  q.lds_depth = d->lds_depth;
+  const size_t lds = (size_t)q.lds_depth * kBlock * 4;
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const bool cnt = dcounts != nullptr;
   const void* kern = pick_kernel(query, q8, exact, cnt, two);
-  const unsigned blocks = grid_blocks(d, kern, n);
+  const unsigned blocks = grid_blocks(d, kern, n, lds);
   void* args[] = {(void*)&v, (void*)&drays, (void*)&q};
-  if (!hip_ok(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, 0, d->stream), "hipLaunchKernel"))
+  if (!hip_ok(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, lds, d->stream), "hipLaunchKernel"))
     return TPT_ERROR_DEVICE;
   hipStream_t st = d->stream;
   if (!hip_ok(hipGetLastError(), "kernel launch")) return TPT_ERROR_DEVICE;
@@ -852,11 +859,13 @@ bool launch_trace_queue(TptScene* s, int query, bool q8, bool exact, const TptRa
   q.refill_below = d->refill_below;
   q.grab = d->refill_grab;
   q.regions = d->regions;
+  q.lds_depth = d->lds_depth;
+  const size_t lds = (size_t)q.lds_depth * kBlock * 4;
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const void* kern = pick_kernel(query, q8, exact, false, two);
-  const unsigned blocks = grid_blocks(d, kern, n_max);
+  const unsigned blocks = grid_blocks(d, kern, n_max, lds);
   void* args[] = {(void*)&v, (void*)&rays, (void*)&q};
-  return hip_ok(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, 0, st), "hipLaunchKernel(trace queue)");
+  return hip_ok(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, lds, st), "hipLaunchKernel(trace queue)");
 }
 
 }  // namespace tpt

@@ -32,6 +32,7 @@ struct Query {
   uint32_t refill_below;  // refill when fewer lanes than this are busy
   uint32_t grab;          // ray ids taken per work-counter atomic (at least the idle lanes)
   uint32_t regions;       // work regions, 1..kMaxRegions
+  uint32_t lds_depth;     // LDS stack entries per lane (the launch's dynamic LDS is lds_depth * kBlock words)
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
@@ -123,12 +124,13 @@ __device__ __forceinline__ void finish_ray(const Query& q, uint32_t rid, const L
 // rays, until all rays are done.  Exits are wave-uniform.
 template <bool kQ8, int kMode, bool kExact, bool kCount, bool kTwoLevel>
 __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView sc, const TptRay* __restrict__ rays, Query q) {
-  __shared__ uint32_t lds[kLdsDepth * kBlock];
-  uint32_t spill[kStackDepth - kLdsDepth];
+  extern __shared__ uint32_t lds[];  // q.lds_depth * kBlock entries
+  uint32_t spill[kStackDepth - kMinLdsDepth];
   LaneStack st;
   st.lds = lds + threadIdx.x;
   st.spill = spill;
   st.sp = 0;
+  st.lds_depth = (int)q.lds_depth;
   st.overflow = false;
   Lane L;
   Counters cnt = {0, 0, 0};

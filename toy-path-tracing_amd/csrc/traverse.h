@@ -20,7 +20,9 @@
 namespace tpt {
 
 constexpr int kBlock = 256;
-constexpr int kLdsDepth = 16;
+constexpr int kLdsDepth = 16;     // LDS stack entries per lane of the one-thread-per-pixel kernels
+constexpr int kMinLdsDepth = 4;   // the range of the trace kernels' tunable LDS depth
+constexpr int kMaxLdsDepth = 32;
 constexpr int kStackDepth = 128;
 constexpr uint32_t kEmptyChild = kChildLeafBit;
 
@@ -396,22 +398,25 @@ __device__ __forceinline__ uint32_t slab4(const GpuNode* __restrict__ nodes, con
   return bits;
 }
 
-// Per-lane traversal stack: kLdsDepth entries in LDS laid out [depth][thread]
-// (the bank is the thread, so lanes at different depths never conflict),
-// continued in a private array up to the reference's 128 entries
-// (R/qbvh.rs:609).  The stack pointer is a register of its own: it must not
-// share an aggregate with the dynamically indexed spill array, or it would be
-// kept in scratch memory.
+// Per-lane traversal stack: lds_depth entries in LDS laid out
+// [depth][thread] (the bank is the thread, so lanes at different depths never
+// conflict), continued in a private array up to the reference's 128 entries
+// (R/qbvh.rs:609).  lds_depth is kLdsDepth in the path kernels and the
+// device's tuned "lds_depth" in the trace kernels (whose LDS is sized at
+// launch).  The stack pointer is a register of its own: it must not share an
+// aggregate with the dynamically indexed spill array, or it would be kept in
+// scratch memory.
 struct LaneStack {
   uint32_t* lds;    // lds[d * kBlock] is this thread's entry d
-  uint32_t* spill;  // private array of kStackDepth - kLdsDepth entries
+  uint32_t* spill;  // private array of at least kStackDepth - lds_depth entries
   int sp;
+  int lds_depth;
   bool overflow;
   __device__ __forceinline__ void push(uint32_t v) {
-    if (sp < kLdsDepth) {
+    if (sp < lds_depth) {
       lds[sp * kBlock] = v;
     } else if (sp < kStackDepth) {
-      spill[sp - kLdsDepth] = v;
+      spill[sp - lds_depth] = v;
     } else {
       overflow = true;  // the reference panics here; the entry is dropped
       return;
@@ -420,7 +425,7 @@ struct LaneStack {
   }
   __device__ __forceinline__ uint32_t pop() {
     --sp;
-    return sp < kLdsDepth ? lds[sp * kBlock] : spill[sp - kLdsDepth];
+    return sp < lds_depth ? lds[sp * kBlock] : spill[sp - lds_depth];
   }
 };
 

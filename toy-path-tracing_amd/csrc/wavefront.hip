@@ -109,13 +109,14 @@ __device__ __forceinline__ uint32_t lane_id() {
 // calls it (block-uniform control flow); threads with want_a (want_b) get
 // consecutive slots of queue a (b).  One global atomic per queue per block
 // call: the queue counters are single words updated by every block on all
-// XCDs, and appends of one atomic per wave serialise on them.
-constexpr uint32_t kAppendBlock = 256;
+// XCDs, and appends of one atomic per wave serialise on them.  kTile: the
+// block's threads (the queue tile: one queue entry per thread).
 struct AppendSlots {
   uint32_t a, b;
 };
+template <uint32_t kTile>
 __device__ __forceinline__ AppendSlots block_append(uint32_t* ca, bool want_a, uint32_t* cb, bool want_b) {
-  constexpr uint32_t kWaves = kAppendBlock / 64;
+  constexpr uint32_t kWaves = kTile / 64;
   __shared__ uint32_t cnt[2][kWaves];
   __shared__ uint32_t base[2];
   const uint64_t ma = __ballot(want_a), mb = __ballot(want_b);
@@ -241,8 +242,8 @@ __device__ __forceinline__ void load_diff(const State& st, uint32_t P, uint32_t 
 
 // Camera rays of the pass's samples (PathSampler camera dimension,
 // PinholeCamera::generate_ray_differential), state init, queue 0 append.
-template <bool kTex>
-__global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Queue q, Counters* c,
+template <bool kTex, uint32_t kTile>
+__global__ __launch_bounds__(kTile) void wf_generate(Pass ps, State st, Queue q, Counters* c,
                                                             const uint8_t* __restrict__ ztab, uint32_t ztab_dims) {
   // One path slot per thread (the host launches ceil(paths / block) blocks):
   // no thread carries state from one slot to another.
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Q
         want = true;
       }
     }
-    const uint32_t slot = block_append(&c->rays[0].v, want, &c->rays[0].v, false).a;
+    const uint32_t slot = block_append<kTile>(&c->rays[0].v, want, &c->rays[0].v, false).a;
     if (want) {
       put_ray(q.rays, slot, o, d);
       q.path[slot] = path;
@@ -284,41 +285,42 @@ __global__ __launch_bounds__(kAppendBlock) void wf_generate(Pass ps, State st, Q
   }
 }
 
-// One bounce of mis::trace_radiance for every path of the input queue.
-// With kTex the ray differentials and cones follow the paths and the
-// materials' textures are applied at every vertex.
 // ---- material sort of a bounce's hit queue ----------------------------------
-// Counting sort by the hit's material (misses in the last bin): wf_sort_count
-// counts the keys, wf_sort_scatter gives every block a range per key and
-// writes the queue entries' indices in key order.  Entries of one key keep no
-// particular order (blocks reserve their ranges atomically); the shade results
-// of a path do not depend on the order paths are shaded in.
+// Counting sort by the hit's material index / group (misses in the last
+// bin): wf_sort_count counts the keys, wf_sort_scatter gives every block a
+// range per key and writes the queue entries' indices in key order.  Entries
+// of one key keep no particular order (blocks reserve their ranges
+// atomically); the shade results of a path do not depend on the order paths
+// are shaded in.  group 1 sorts by material; larger groups give fewer bins
+// (less sorting work, more divergence left in a wave).
 constexpr uint32_t kMaxSortBins = 256;
 struct SortBuf {
   uint32_t* hist;    // kMaxSortBins counts
   uint32_t* cursor;  // kMaxSortBins reservation cursors
   uint32_t* order;   // queue indices in material order (P)
 };
-__device__ __forceinline__ uint32_t sort_key(const TptHit& h, const GpuShade* inst, uint32_t bins) {
-  return h.instance == TPT_INVALID_INDEX ? bins - 1u : inst[h.instance].material;
+__device__ __forceinline__ uint32_t sort_key(const TptHit& h, const GpuShade* inst, uint32_t bins, uint32_t group) {
+  return h.instance == TPT_INVALID_INDEX ? bins - 1u : inst[h.instance].material / group;
 }
 __global__ __launch_bounds__(256) void wf_sort_zero(SortBuf sb) {
   sb.hist[threadIdx.x] = 0u;
   sb.cursor[threadIdx.x] = 0u;
 }
 __global__ __launch_bounds__(256) void wf_sort_count(const TptHit* __restrict__ hits, const GpuShade* inst,
-                                                     const Counters* c, uint32_t in_idx, uint32_t bins, SortBuf sb) {
+                                                     const Counters* c, uint32_t in_idx, uint32_t bins, uint32_t group,
+                                                     SortBuf sb) {
   __shared__ uint32_t cnt[kMaxSortBins];
   cnt[threadIdx.x] = 0u;
   __syncthreads();
   const uint32_t n = c->rays[in_idx].v;
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i < n) atomicAdd(&cnt[sort_key(hits[i], inst, bins)], 1u);
+  if (i < n) atomicAdd(&cnt[sort_key(hits[i], inst, bins, group)], 1u);
   __syncthreads();
   if (threadIdx.x < bins && cnt[threadIdx.x]) atomicAdd(&sb.hist[threadIdx.x], cnt[threadIdx.x]);
 }
 __global__ __launch_bounds__(256) void wf_sort_scatter(const TptHit* __restrict__ hits, const GpuShade* inst,
-                                                       const Counters* c, uint32_t in_idx, uint32_t bins, SortBuf sb) {
+                                                       const Counters* c, uint32_t in_idx, uint32_t bins,
+                                                       uint32_t group, SortBuf sb) {
   __shared__ uint32_t cnt[kMaxSortBins], first[kMaxSortBins];
   const uint32_t n = c->rays[in_idx].v;
   if (blockIdx.x * 256u >= n) return;  // uniform per block
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(256) void wf_sort_scatter(const TptHit* __restrict_
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   uint32_t key = 0, rank = 0;
   if (i < n) {
-    key = sort_key(hits[i], inst, bins);
+    key = sort_key(hits[i], inst, bins, group);
     rank = atomicAdd(&cnt[key], 1u);
   }
   __syncthreads();
@@ -347,9 +349,12 @@ __global__ __launch_bounds__(256) void wf_sort_scatter(const TptHit* __restrict_
   if (i < n) sb.order[first[key] + rank] = i;
 }
 
-// order: the queue indices in material order (nullptr: queue order)
-template <bool kTex>
-__global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
+// One bounce of mis::trace_radiance for every path of the input queue.
+// With kTex the ray differentials and cones follow the paths and the
+// materials' textures are applied at every vertex.  order: the queue indices
+// in material order (nullptr: queue order).
+template <bool kTex, uint32_t kTile>
+__global__ __launch_bounds__(kTile) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
                                                 const TptHit* __restrict__ hits, Queue qout, ShadowQ sq,
                                                 Counters* c, uint32_t in_idx, uint32_t bounce,
                                                 const uint8_t* __restrict__ ztab, uint32_t ztab_dims,
@@ -480,7 +485,7 @@ __global__ __launch_bounds__(kAppendBlock) void wf_shade(Pass ps, pt::ShadeView 
         store3(st.rad, path, rad);
       }
     }
-    const AppendSlots slots = block_append(&c->rays[in_idx ^ 1u].v, want_ray, &c->shadow.v, want_shadow);
+    const AppendSlots slots = block_append<kTile>(&c->rays[in_idx ^ 1u].v, want_ray, &c->shadow.v, want_shadow);
     const uint32_t rs = slots.a, ss = slots.b;
     if (want_ray) {
       put_ray(qout.rays, rs, no, nd);
@@ -583,30 +588,34 @@ struct Plan {
   bool exact_shadow;
   bool tex;  // the scene has textures: ray differentials and textured materials
   wf::SortBuf sort;
-  uint32_t sort_bins;  // material count + 1 (misses); 0: shade in queue order
+  uint32_t sort_bins;   // ceil(material count / group) + 1 (misses); 0: shade in queue order
+  uint32_t sort_group;  // materials per sort key
+  uint32_t tile;        // threads per generate / shade block (one queue entry each): 128, 256 or 512
 };
 
-// The kernels of one pass, all reading their counts on the device.
-bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream_t st) {
+// The kernels of one pass, all reading their counts on the device; kTile:
+// the generate / shade blocks' threads.
+template <uint32_t kTile>
+bool enqueue_pass_tile(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream_t st) {
   TptDevice* d = s->dev;
-  auto gen = p.tex ? wf::wf_generate<true> : wf::wf_generate<false>;
-  auto shade = p.tex ? wf::wf_shade<true> : wf::wf_shade<false>;
+  auto gen = p.tex ? wf::wf_generate<true, kTile> : wf::wf_generate<false, kTile>;
+  auto shade = p.tex ? wf::wf_shade<true, kTile> : wf::wf_shade<false, kTile>;
   // One thread per path slot: generate and shade take exactly one slot per
   // thread (they have no slot loop).  An earlier form looped over slots with
   // the shade grid capped at the resident capacity; there the untextured
   // build gave wrong OpenPBR paths on a thread's second slot, differently
   // from run to run (the cause was not found), so no launch may hand a
   // thread more than one slot.  Blocks past the queue length exit at once.
-  const unsigned g_gen = (unsigned)((p.ps.paths + wf::kAppendBlock - 1) / wf::kAppendBlock);
+  const unsigned g_gen = (unsigned)((p.ps.paths + kTile - 1) / kTile);
   const unsigned g_shade = g_gen;
-  if ((uint64_t)g_gen * wf::kAppendBlock < p.ps.paths) {
+  if ((uint64_t)g_gen * kTile < p.ps.paths) {
     set_error("tpt_render_batch: generate / shade grid smaller than the paths in flight");
     return false;
   }
   const unsigned g_add = resident_blocks(d, wf::wf_shadow_add, p.ps.paths);
   const unsigned g_fold = resident_blocks(d, wf::wf_fold, p.ps.total_pixels);
   hipLaunchKernelGGL(wf::wf_reset, dim3(1), dim3(64), 0, st, p.c, 0u);
-  hipLaunchKernelGGL(gen, dim3(g_gen), dim3(wf::kAppendBlock), 0, st, p.ps, p.st, p.q[0], p.c,
+  hipLaunchKernelGGL(gen, dim3(g_gen), dim3(kTile), 0, st, p.ps, p.st, p.q[0], p.c,
                      (const uint8_t*)p.ztab, p.ztab_dims);
   for (uint32_t b = 0; b <= p.max_depth; ++b) {
     const uint32_t a = b & 1u;
@@ -618,12 +627,12 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
       const unsigned g_sort = (unsigned)((p.ps.paths + 255u) / 256u);
       hipLaunchKernelGGL(wf::wf_sort_zero, dim3(1), dim3(256), 0, st, p.sort);
       hipLaunchKernelGGL(wf::wf_sort_count, dim3(g_sort), dim3(256), 0, st, (const TptHit*)p.hits, sv.inst,
-                         (const wf::Counters*)p.c, a, p.sort_bins, p.sort);
+                         (const wf::Counters*)p.c, a, p.sort_bins, p.sort_group, p.sort);
       hipLaunchKernelGGL(wf::wf_sort_scatter, dim3(g_sort), dim3(256), 0, st, (const TptHit*)p.hits, sv.inst,
-                         (const wf::Counters*)p.c, a, p.sort_bins, p.sort);
+                         (const wf::Counters*)p.c, a, p.sort_bins, p.sort_group, p.sort);
       order = p.sort.order;
     }
-    hipLaunchKernelGGL(shade, dim3(g_shade), dim3(wf::kAppendBlock), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
+    hipLaunchKernelGGL(shade, dim3(g_shade), dim3(kTile), 0, st, p.ps, sv, p.st, p.q[a], (const TptHit*)p.hits,
                        p.q[a ^ 1u], p.sq, p.c, a, b, (const uint8_t*)p.ztab, p.ztab_dims, order);
     if (b < p.max_depth && s->host_cats.count) {  // the light sampler has a category: NEE can run
       if (!launch_trace_queue(s, 1, p.q8, p.exact_shadow, p.sq.rays, &p.c->shadow.v, p.ps.paths, nullptr, p.sq.target,
@@ -635,6 +644,11 @@ bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream
   hipLaunchKernelGGL(wf::wf_fold, dim3(g_fold), dim3(256), 0, st, p.ps, p.st, p.color, (const wf::Counters*)p.c);
   hipLaunchKernelGGL(wf::wf_next_pass, dim3(1), dim3(64), 0, st, p.c);
   return hip_ok(hipGetLastError(), "wavefront launch");
+}
+bool enqueue_pass(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream_t st) {
+  if (p.tile == 128) return enqueue_pass_tile<128>(s, p, sv, st);
+  if (p.tile == 512) return enqueue_pass_tile<512>(s, p, sv, st);
+  return enqueue_pass_tile<256>(s, p, sv, st);
 }
 
 }  // namespace
@@ -694,7 +708,9 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     const size_t o_su = take(P);
     const bool tex = s->texture_count > 0;
     const size_t o_diff = tex ? take(12 * 4 * P) : 0, o_dflag = tex ? take(4 * P) : 0, o_cone = tex ? take(8 * P) : 0;
-    const uint32_t sort_bins = (d->wf_sort && s->material_count + 1 <= wf::kMaxSortBins) ? s->material_count + 1 : 0;
+    const uint32_t sort_group = d->wf_sort;
+    const uint32_t sort_keys = sort_group ? (s->material_count + sort_group - 1) / sort_group + 1 : 0;
+    const uint32_t sort_bins = sort_keys <= wf::kMaxSortBins ? sort_keys : 0;
     const size_t o_sort = sort_bins ? take(4 * (2 * wf::kMaxSortBins + P)) : 0;
     // digit-permutation rows of every sampler dimension a path of up to 256 bounces reads
     // (deeper bounces compute their rows in the kernel)
@@ -753,6 +769,8 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.exact_shadow = flags & TPT_FLAG_ANY_EXACT;
     p.tex = tex;
     p.sort_bins = sort_bins;
+    p.sort_group = sort_group ? sort_group : 1u;
+    p.tile = d->wf_tile;
     p.sort = sort_bins ? wf::SortBuf{(uint32_t*)(base + o_sort), (uint32_t*)(base + o_sort) + wf::kMaxSortBins,
                                      (uint32_t*)(base + o_sort) + 2 * wf::kMaxSortBins}
                        : wf::SortBuf{nullptr, nullptr, nullptr};
@@ -842,11 +860,15 @@ bool time_batch(TptScene* s, std::vector<TptFrameDesc>& frames, uint32_t flags, 
 // from the starting one.
 constexpr float kTuneMargin = 0.99f;
 // Autotuner of the wavefront path tracer: coordinate descent over paths in
-// flight, resident trace blocks per CU, the trace kernels' refill threshold and
-// the material sort of the shade queue, each candidate timed (HIP events) on the caller's frames at
-// tune_spp samples per pixel; the fastest setting is left on the device.  A
-// paths-in-flight candidate larger than the tuning batch's path count runs
-// the batch in one pass, like the batch size itself, so it is not tried.
+// flight, resident trace blocks per CU, the trace kernels' refill threshold,
+// their traversal stack's LDS depth, the shade queue's tile (threads per
+// generate / shade block) and the material sort's granularity (off, or
+// materials per sort key), each candidate timed (HIP events) on the
+// caller's frames at tune_spp samples per pixel; the fastest setting is left
+// on the device.  A paths-in-flight candidate larger than the tuning batch's
+// path count runs the batch in one pass, like the batch size itself, so it
+// is not tried; sort granularities whose key count exceeds the sort's 256
+// bins are not tried.
 extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, uint32_t frame_count, uint32_t flags,
                                        uint32_t tune_spp, TptTuneResult* out) {
   try {
@@ -867,7 +889,6 @@ extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, 
       off += (size_t)f.render.width * f.render.height;
       if (tune_spp) f.render.samples_per_pixel = std::min(f.render.samples_per_pixel, tune_spp);
     }
-    const uint32_t keep_paths = d->wf_paths, keep_bpc = d->blocks_per_cu, keep_refill = d->refill_below;
     uint32_t trials = 0;
     float best = 0.0f;
     if (!time_batch(s, fr, flags, 2, &best)) return TPT_ERROR_DEVICE;
@@ -892,6 +913,9 @@ extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, 
     const uint32_t paths[] = {1u << 19, 1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24};
     const uint32_t bpcs[] = {0u, 2u, 3u, 4u, 6u, 8u};
     const uint32_t refills[] = {8u, 16u, 24u, 32u, 48u};
+    const uint32_t depths[] = {8u, 12u, 16u, 24u, 32u};
+    const uint32_t tiles[] = {128u, 256u, 512u};
+    const uint32_t groups[] = {0u, 1u, 2u, 4u, 8u};
     size_t batch_paths = 0;
     for (const auto& f : fr) batch_paths += (size_t)f.render.width * f.render.height * f.render.samples_per_pixel;
     for (uint32_t v : paths)
@@ -900,7 +924,14 @@ extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, 
       if (v != d->blocks_per_cu && tryv(&d->blocks_per_cu, v) != TPT_OK) return TPT_ERROR_DEVICE;
     for (uint32_t v : refills)
       if (v != d->refill_below && tryv(&d->refill_below, v) != TPT_OK) return TPT_ERROR_DEVICE;
-    if (tryv(&d->wf_sort, d->wf_sort ^ 1u) != TPT_OK) return TPT_ERROR_DEVICE;
+    for (uint32_t v : depths)
+      if (v != d->lds_depth && tryv(&d->lds_depth, v) != TPT_OK) return TPT_ERROR_DEVICE;
+    for (uint32_t v : tiles)
+      if (v != d->wf_tile && tryv(&d->wf_tile, v) != TPT_OK) return TPT_ERROR_DEVICE;
+    for (uint32_t v : groups) {
+      const uint32_t keys = v ? (s->material_count + v - 1) / v + 1 : 0;
+      if (v != d->wf_sort && keys <= wf::kMaxSortBins && tryv(&d->wf_sort, v) != TPT_OK) return TPT_ERROR_DEVICE;
+    }
     out->wf_paths = d->wf_paths;
     out->blocks_per_cu = d->blocks_per_cu;
     out->refill_below = d->refill_below;
@@ -908,9 +939,8 @@ extern "C" int32_t tpt_autotune_render(TptScene* s, const TptFrameDesc* frames, 
     out->wf_sort = d->wf_sort;
     out->ms_default = base_ms;
     out->ms_best = best;
-    (void)keep_paths;
-    (void)keep_bpc;
-    (void)keep_refill;
+    out->lds_depth = d->lds_depth;
+    out->wf_tile = d->wf_tile;
     return TPT_OK;
   } catch (const std::exception& e) {
     set_error(std::string("tpt_autotune_render: ") + e.what());

@@ -160,7 +160,7 @@ assert BSDF_QUERY_DTYPE.itemsize == 48 and BSDF_RESULT_DTYPE.itemsize == 48
 class TuneResult(ctypes.Structure):
     _fields_ = [("wf_paths", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("refill_below", ctypes.c_uint32),
                 ("trials", ctypes.c_uint32), ("wf_sort", ctypes.c_uint32), ("ms_default", ctypes.c_float),
-                ("ms_best", ctypes.c_float)]
+                ("ms_best", ctypes.c_float), ("lds_depth", ctypes.c_uint32), ("wf_tile", ctypes.c_uint32)]
 
 
 class FrameDesc(ctypes.Structure):
@@ -204,6 +204,9 @@ _SIGNATURES = [
     ("tpt_qbvh_export", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("tpt_qbvh_quantize", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("tpt_scene_create", ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("tpt_scene_desc_save", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p]),
+    ("tpt_scene_desc_load", ctypes.c_void_p, [ctypes.c_char_p]),
+    ("tpt_scene_desc_free", None, [ctypes.c_void_p]),
     ("tpt_scene_release", None, [ctypes.c_void_p]),
     ("tpt_scene_device_bytes", ctypes.c_size_t, [ctypes.c_void_p]),
     ("tpt_scene_light_tree", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),

@@ -99,9 +99,9 @@ def render_albedo(albedo: torch.Tensor, device_scene, materials, camera, width: 
 
 def _texel_offsets(device_scene) -> list:
     offs, o = [], 0
-    for t in device_scene.textures:
+    for shape in device_scene.texture_shapes:
         offs.append(o)
-        o += int(np.prod(t.pixels.shape))
+        o += int(np.prod(shape))
     return offs + [o]
 
 
@@ -125,7 +125,7 @@ class TextureRender(torch.autograd.Function):
         d_alb = torch.zeros((device_scene.material_count, 3), dtype=torch.float32, device=g_img.device)
         d_tex = torch.zeros(max(1, offs[-1]), dtype=torch.float32, device=g_img.device)
         device_scene.render_adjoint_texels_device(camera, width, height, spp, max_depth, g_img, d_alb, d_tex)
-        g = d_tex[offs[texture]:offs[texture + 1]].reshape(device_scene.textures[texture].pixels.shape)
+        g = d_tex[offs[texture]:offs[texture + 1]].reshape(device_scene.texture_shapes[texture])
         return g.to(grad_image.device), None, None, None, None, None, None, None
 
 
@@ -193,8 +193,8 @@ class ViewsRender(torch.autograd.Function):
         tx = None
         if texels is not None:
             tx = texels.detach().to(_dev(device_scene), torch.float32).contiguous()
-            for t in range(len(device_scene.textures)):
-                device_scene.set_texture_device(t, tx[offs[t]:offs[t + 1]].view(device_scene.textures[t].pixels.shape))
+            for t in range(len(device_scene.texture_shapes)):
+                device_scene.set_texture_device(t, tx[offs[t]:offs[t + 1]].view(device_scene.texture_shapes[t]))
         outs = [torch.empty((h, w, 3), dtype=torch.float32, device=_dev(device_scene)) for _, w, h, _, _ in views]
         device_scene.render_batch_device(list(views), outs)
         ctx.call = (device_scene, materials, p, tx, views, texels is not None)
@@ -207,8 +207,8 @@ class ViewsRender(torch.autograd.Function):
         device_scene.set_materials(with_params(materials, p))  # the forward's parameters
         offs = _texel_offsets(device_scene)
         if tx is not None:
-            for t in range(len(device_scene.textures)):
-                device_scene.set_texture_device(t, tx[offs[t]:offs[t + 1]].view(device_scene.textures[t].pixels.shape))
+            for t in range(len(device_scene.texture_shapes)):
+                device_scene.set_texture_device(t, tx[offs[t]:offs[t + 1]].view(device_scene.texture_shapes[t]))
         dev = _dev(device_scene)
         gi = [(g if g is not None else torch.zeros((h, w, 3), device=dev)).detach().to(dev, torch.float32).contiguous()
               for g, (_, w, h, _, _) in zip(grad_images, views)]

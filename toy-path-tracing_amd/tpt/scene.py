@@ -12,6 +12,7 @@ column-major order.
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass, field
 from enum import Enum
 
@@ -244,6 +245,17 @@ class Scene:
             raise ClosestHitError(ClosestHitError.Kind.QbvhNotBuilt)
         return DeviceScene(device, self)
 
+    def save_cache(self, path) -> None:
+        """Writes the built scene (meshes with their QBVHs, instances, TLAS,
+        materials, environment, textures, directional lights) to one file
+        (tpt_scene_desc_save), which DeviceScene.from_cache uploads without a
+        build."""
+        if not self.built:
+            raise ClosestHitError(ClosestHitError.Kind.QbvhNotBuilt)
+        desc, keep = scene_desc(self)
+        check(lib().tpt_scene_desc_save(ctypes.byref(desc), os.fsencode(os.fspath(path))))
+        del keep
+
 
 class Device:
     """A GPU (tpt_device_create); trace calls go on its stream."""
@@ -376,7 +388,37 @@ class DeviceScene:
             raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
         self.instance_count = len(scene.instances)
         self.material_count = len(scene.materials)
-        self.textures = list(scene.textures)  # host copies (shapes of the gradient outputs)
+        # the textures' (height, width[, channels]) shapes: the gradient outputs'
+        self.texture_shapes = [tuple(t.pixels.shape) for t in scene.textures]
+
+    @classmethod
+    def from_cache(cls, device: Device, path) -> "DeviceScene":
+        """Uploads a scene saved by Scene.save_cache (tpt_scene_desc_load +
+        tpt_scene_create): no mesh or QBVH build, the file holds the built
+        arrays.  Raises TptError for a missing, truncated or corrupt file or
+        one written with another format version or record layout."""
+        L = lib()
+        d = L.tpt_scene_desc_load(os.fsencode(os.fspath(path)))
+        if not d:
+            raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
+        try:
+            desc = _lib.SceneDesc.from_address(d)
+            self = cls.__new__(cls)
+            self.device = device
+            self.handle = L.tpt_scene_create(device.handle, d)
+            if not self.handle:
+                raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
+            self.instance_count = desc.instance_count
+            self.material_count = desc.material_count
+            shapes = []
+            if desc.texture_count:
+                tex = (_lib.TextureDesc * desc.texture_count).from_address(desc.textures)
+                for t in tex:
+                    shapes.append((t.height, t.width) if t.channels == 1 else (t.height, t.width, t.channels))
+            self.texture_shapes = shapes
+        finally:
+            L.tpt_scene_desc_free(d)
+        return self
 
     @property
     def device_bytes(self) -> int:
@@ -494,7 +536,7 @@ class DeviceScene:
         d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
         g = np.ascontiguousarray(dloss_dimage, dtype=np.float32).reshape(height, width, 3)
         out = np.zeros((self.material_count, 3), np.float32)
-        shapes = [t.pixels.shape for t in self.textures]
+        shapes = self.texture_shapes
         flat = np.zeros(max(1, sum(int(np.prod(sh)) for sh in shapes)), np.float32)
         check(lib().tpt_render_adjoint_texels(self.handle, ctypes.byref(d), ptr(g), ptr(out), ptr(flat),
                                               _lib.TPT_FLAG_GRAD_FP64 if fp64 else 0))
@@ -515,7 +557,7 @@ class DeviceScene:
         d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
         g = np.ascontiguousarray(dloss_dimage, dtype=np.float32).reshape(height, width, 3)
         out = np.zeros((self.material_count, 5), np.float32)
-        shapes = [t.pixels.shape for t in self.textures]
+        shapes = self.texture_shapes
         flat = np.zeros(max(1, sum(int(np.prod(sh)) for sh in shapes)), np.float32)
         check(lib().tpt_render_adjoint_params(self.handle, ctypes.byref(d), ptr(g), ptr(out),
                                               ptr(flat) if shapes else None,
@@ -563,8 +605,8 @@ class DeviceScene:
     def set_texture(self, texture: int, pixels) -> None:
         """tpt_scene_set_texture: new level-0 texels (same shape) for a texture."""
         px = np.ascontiguousarray(pixels, np.float32)
-        if px.shape != self.textures[texture].pixels.shape:
-            raise ValueError("set_texture: the texture's shape must stay " + str(self.textures[texture].pixels.shape))
+        if tuple(px.shape) != self.texture_shapes[texture]:
+            raise ValueError("set_texture: the texture's shape must stay " + str(self.texture_shapes[texture]))
         check(lib().tpt_scene_set_texture(self.handle, texture, ptr(px), 0))
 
     def render_adjoint_batch(self, views, fp64: bool = False, texels: bool = True):
@@ -582,7 +624,7 @@ class DeviceScene:
             keep.append(g)
             arr[i].dloss_dimage = g.ctypes.data
         out = np.zeros((self.material_count, 5), np.float32)
-        shapes = [t.pixels.shape for t in self.textures]
+        shapes = self.texture_shapes
         flat = np.zeros(max(1, sum(int(np.prod(sh)) for sh in shapes)), np.float32) if texels and shapes else None
         check(lib().tpt_render_adjoint_batch(self.handle, arr, len(views), ptr(out), ptr(flat),
                                              _lib.TPT_FLAG_GRAD_FP64 if fp64 else 0))
@@ -612,8 +654,8 @@ class DeviceScene:
     def set_texture_device(self, texture: int, pixels_dev) -> None:
         """tpt_scene_set_texture from a device buffer (a torch tensor of the
         texture's level-0 shape); synchronous."""
-        if tuple(pixels_dev.shape) != tuple(self.textures[texture].pixels.shape):
-            raise ValueError("set_texture: the texture's shape must stay " + str(self.textures[texture].pixels.shape))
+        if tuple(pixels_dev.shape) != self.texture_shapes[texture]:
+            raise ValueError("set_texture: the texture's shape must stay " + str(self.texture_shapes[texture]))
         check(lib().tpt_scene_set_texture(self.handle, texture, ptr(pixels_dev), _lib.TPT_FLAG_DEVICE_POINTERS))
 
     def render_batch(self, frames, quantized: bool = False, exact_shadow: bool = False) -> list:

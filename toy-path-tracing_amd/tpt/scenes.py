@@ -685,8 +685,8 @@ def city(grid: int = 256, rocks: int = 400_000, seed: int = 0, cell: float = 4.0
     ground and the emitters, each under an identity instance.
 
     Materials: 0 diffuse, 1-3 metal at specular_roughness 0.05 / 0.3 / 0.8,
-    4 coat 0.5, 5 smooth dielectric of IOR 1.5 (reflection: the backend
-    carries no transmission lobe), 6 sheen (fuzz), 7 base-colour textured,
+    4 coat 0.5, 5 transmission with IOR 1.5 (smooth glass: transmission
+    weight 1, no diffuse base), 6 sheen (fuzz), 7 base-colour textured,
     8 the lamp emitter.  Every surface is wound outward ((p1 - p0) x (p2 - p0)
     points out of the tower, up from the ground, down from a lamp)."""
     from .material import OpenPbr
@@ -742,7 +742,8 @@ def city(grid: int = 256, rocks: int = 400_000, seed: int = 0, cell: float = 4.0
         OpenPbr.new((0.9, 0.9, 0.92)).with_(base_metalness=1.0, specular_roughness=0.3),
         OpenPbr.new((0.75, 0.45, 0.35)).with_(base_metalness=1.0, specular_roughness=0.8),
         OpenPbr.new((0.55, 0.08, 0.06)).with_(coat_weight=0.5),
-        OpenPbr.new((0.02, 0.02, 0.02)).with_(specular_ior=1.5, specular_roughness=0.0),
+        OpenPbr.new((0.8, 0.8, 0.8)).with_(base_weight=0.0, transmission_weight=1.0, specular_ior=1.5,
+                                           specular_roughness=0.0),
         OpenPbr.new((0.35, 0.33, 0.3)).with_(specular_weight=0.0, fuzz_weight=1.0, fuzz_roughness=0.5,
                                              fuzz_color=(0.8, 0.75, 0.6)),
         OpenPbr.new((1.0, 1.0, 1.0)).with_(base_color_texture=tex, specular_roughness=0.5),
