@@ -255,6 +255,7 @@ def main():
             "q8_wavefront_vs_fp32_megakernel": city["wavefront_q8"]["speedup_vs_megakernel"],
             "fp32_wavefront_vs_fp32_megakernel": city["wavefront"]["speedup_vs_megakernel"],
         }
+        summary["city_qbvh_build"] = city["qbvh_build"]
     if nee_proxy:
         summary["nee_proxy_relmse_gain_at_equal_time"] = {k: v["relmse_gain_at_equal_time"]
                                                           for k, v in nee_proxy.items()}
@@ -386,9 +387,9 @@ def render_bench(dev, stream, args):
 
 
 def relmse(x, ref):
-    """Mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2) (tpt.imageio.relmse)."""
-    from tpt.imageio import relmse as _relmse
-    return _relmse(x, ref)
+    """Mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2) (tpt.imageio.relmse's
+    definition, here on device tensors)."""
+    return float(((x - ref) ** 2 / (ref * ref + 1e-2)).mean())
 
 
 def policy_comparison(rds, stream, frame, render, ref_spp):
@@ -476,11 +477,30 @@ def city_bench(dev, stream, args):
 
     spp, depth = args.city_spp, 16
     t = time.perf_counter()
-    cs = scenes.city()
-    t_build = time.perf_counter() - t
+    cs = scenes.city(build=False)
+    t_gen = time.perf_counter() - t
+    # the QBVH build of every mesh and of the instances: host threads, then
+    # the device build (tpt_qbvh_build_device), which must give the same trees
+    t = time.perf_counter()
+    cs.scene.build_qbvh()
+    t_host = time.perf_counter() - t
+    host_trees = [(m.qbvh.nodes, m.qbvh.primitive_indices) for m in cs.scene.meshes]
+    host_trees.append((cs.scene.qbvh.nodes, cs.scene.qbvh.primitive_indices))
+    t = time.perf_counter()
+    cs.scene.build_qbvh(device=dev)
+    t_gpu = time.perf_counter() - t
+    gpu_trees = [(m.qbvh.nodes, m.qbvh.primitive_indices) for m in cs.scene.meshes]
+    gpu_trees.append((cs.scene.qbvh.nodes, cs.scene.qbvh.primitive_indices))
+    same = all(a[0].tobytes() == b[0].tobytes() and np.array_equal(a[1], b[1]) for a, b in zip(host_trees, gpu_trees))
+    t_build = t_gen + t_host
     tris = sum(m.triangle_count() for m in cs.scene.meshes)
-    log(f"city: {tris} triangles, {cs.emissive_triangles} emissive, generated and built in {t_build:.1f}s")
+    log(f"city: {tris} triangles, {cs.emissive_triangles} emissive, generated in {t_gen:.1f}s; QBVH build "
+        f"host {t_host * 1e3:.0f} ms, GPU {t_gpu * 1e3:.0f} ms (bounds in host memory, trees copied back), "
+        f"identical trees: {same}")
+    t = time.perf_counter()
     rds = cs.scene.upload(dev)
+    t_upload = time.perf_counter() - t
+    log(f"city upload (layout, 8-bit nodes encoded on the device, light tree, textures): {t_upload:.2f}s")
     frames = [(camera.pinhole(**v, width=w, height=h), w, h, spp, depth) for v, w, h in cs.cameras]
     pixels = sum(w * h for _, w, h, _, _ in frames)
     samples = pixels * spp
@@ -508,6 +528,9 @@ def city_bench(dev, stream, args):
                     f"with sun, 8 OpenPBR variants; 4 frames (1920x1080, 1280x720, 640x360, 3840x2160), "
                     f"{spp} spp, max depth {depth}",
         "scene_build_s": round(t_build, 2),
+        "qbvh_build": {"host_ms": round(t_host * 1e3, 1), "gpu_ms": round(t_gpu * 1e3, 1),
+                       "speedup": round(t_host / t_gpu, 2), "identical_trees": bool(same)},
+        "upload_s": round(t_upload, 2),
         "megakernel": {"ms_per_batch": round(ms, 3), "samples_per_s": round(samples / (ms / 1e3), 1)},
     }
     log(f"city megakernel fp32: {ms:.1f} ms per 4 frames = {samples / (ms / 1e3) / 1e6:.1f} Msamples/s")

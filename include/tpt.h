@@ -338,7 +338,26 @@ int32_t tpt_path_randoms(TptDevice* device, uint32_t width, uint32_t height, uin
  * Returns NULL when n == 0 (build_qbvh returns None) or on error.
  * threads <= 0 uses the hardware concurrency; the tree does not depend on it. */
 TptQbvh* tpt_qbvh_build(const float* bounds_min, const float* bounds_max, size_t n, int32_t threads);
+
+/* build_qbvh (R/qbvh.rs:144-516) on the device of `device`: the same nodes
+ * and primitive order as tpt_qbvh_build (and the reference), from the
+ * primitives' bounds (3 floats each) in host memory, or in device memory
+ * with TPT_FLAG_DEVICE_POINTERS.  Ranges of more than 2048 primitives are
+ * split level by level over blocks, smaller ones by one block each.  NULL
+ * with tpt_last_error for no primitives, more than 2^31 - 1, a leaf beyond
+ * the 24-bit child encoding, or a forced median split whose centroids do not
+ * coincide (the reference then sorts them; the host build reproduces it). */
+TptQbvh* tpt_qbvh_build_device(TptDevice* device, const float* bounds_min, const float* bounds_max, size_t n,
+                               uint32_t flags);
 void tpt_qbvh_release(TptQbvh* qbvh);
+/* The scene's 8-bit quantised nodes as the device holds them (64 bytes each,
+ * the layout tpt_qbvh_quantize writes; BLAS nodes of every mesh in mesh
+ * order, interior children rebased into that array, or with tlas != 0 the
+ * TLAS nodes): *count receives the node count, and with out non-NULL and
+ * capacity >= *count the nodes are copied.  tpt_scene_create encodes them on
+ * the device from the FP32 nodes with the encoder tpt_qbvh_quantize runs on
+ * the host.  No reference counterpart (an inspection call). */
+int32_t tpt_scene_q8_nodes(const TptScene* scene, uint32_t tlas, void* out, size_t capacity, size_t* count);
 size_t tpt_qbvh_node_count(const TptQbvh* qbvh);
 size_t tpt_qbvh_primitive_count(const TptQbvh* qbvh);
 /* Copies the nodes (node_count entries) and primitive_indices out. */

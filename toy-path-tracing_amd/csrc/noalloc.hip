@@ -47,10 +47,10 @@ __global__ __launch_bounds__(256) void noalloc_clear(uint32_t* p, uint32_t words
   for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) p[i] = 0u;
 }
 
-// The traversal stack keeps the fixed kLdsDepth entries per lane in LDS here
-// (not the device's tuned depth), so the resident blocks looked up at set-up
-// stay valid.
-constexpr size_t kNoallocLds = (size_t)kLdsDepth * kBlock * 4;
+// The traversal stack keeps the fixed kLdsDepth entries per lane in the
+// kernels' static LDS here (not the device's tuned depth), so the resident
+// blocks looked up at set-up stay valid; no launch-sized LDS.
+constexpr size_t kNoallocLds = 0;
 
 // Kernel variants: query 0 closest, 1 early-exit shadow, 2 exact-rule shadow;
 // FP32 or 8-bit nodes; one or two levels.
@@ -60,7 +60,7 @@ int variant_index(int query, bool exact, bool q8, bool two) {
   return qm * 4 + (q8 ? 2 : 0) + (two ? 1 : 0);
 }
 const void* variant_kernel(int v) {
-#define TPT_NK(Q8, M, E, T) reinterpret_cast<const void*>(&trace_kernel<Q8, M, E, false, T>)
+#define TPT_NK(Q8, M, E, T) reinterpret_cast<const void*>(&trace_kernel<Q8, M, E, false, T, kLdsDepth>)
   static const void* const table[kVariants] = {
       TPT_NK(false, kClosest, false, false), TPT_NK(false, kClosest, false, true),
       TPT_NK(true, kClosest, false, false),  TPT_NK(true, kClosest, false, true),

@@ -136,6 +136,8 @@ struct TptDevice {
   uint32_t wf_tile = 256;
   uint32_t wf_sort = 0;
   tpt::DeviceBuffer wf_buf;
+  // tpt_qbvh_build_device's workspace (grow-only: about 420 bytes per primitive)
+  tpt::DeviceBuffer qbvh_buf;
   // tpt_trace_noalloc: this device's address of the file-scope scratch array
   // (looked up once by noalloc_setup) and the resident blocks per CU of its
   // kernel variants
@@ -144,6 +146,12 @@ struct TptDevice {
 };
 
 namespace tpt {
+// The QBVH build on the device (qbvh_build_gpu.hip): the host builder's
+// output from bounds in host or device memory.  Returns 0, -1 (too many
+// primitives), -2 (child encoding overflow), -3 (a forced split it does not
+// reorder) or -4 (device error, set_error holds it).
+int build_qbvh_device(TptDevice* d, const float* bmin, const float* bmax, size_t n, bool device_pointers,
+                      HostQbvh* out);
 // Looks up the allocation-free entry point's scratch on the current device
 // (tpt_device_create calls it once per device handle).
 bool noalloc_setup(TptDevice* d);
@@ -163,6 +171,7 @@ struct TptScene {
   uint32_t* tlas_prim = nullptr;
   uint32_t instance_count = 0;
   uint32_t tlas_node_count = 0;
+  uint32_t node_count = 0;  // BLAS nodes of every mesh (nodes / qnodes entries)
   // shading (path tracer): per-instance records, mesh vertex arrays in
   // original order, indices, materials
   tpt::GpuShade* shade = nullptr;

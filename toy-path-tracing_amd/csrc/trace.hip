@@ -29,6 +29,7 @@
 
 #include "environment.h"
 #include "ggx_tables.h"
+#include "q8_encode.h"
 #include "tpt_device.h"
 #include "tpt_internal.h"
 #include "trace_kernel.h"
@@ -60,6 +61,22 @@ struct TptQbvh {
 
 namespace tpt {
 namespace {
+
+// The 8-bit copies of n FP32 nodes, encoded on the device (q8_encode.h).
+__global__ void encode_q8_kernel(const GpuNode* __restrict__ nodes, uint32_t n, GpuNodeQ8* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) quantize_node_impl(nodes[i], &out[i]);
+}
+bool encode_q8(TptDevice* d, const GpuNode* nodes, size_t n, GpuNodeQ8** dst, size_t* bytes) {
+  *dst = nullptr;
+  const size_t nb = (n ? n : 1) * sizeof(GpuNodeQ8);
+  if (!hip_ok(hipMalloc((void**)dst, nb), "hipMalloc")) return false;
+  *bytes += nb;
+  if (!n) return true;
+  hipLaunchKernelGGL(encode_q8_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, d->stream, nodes, (uint32_t)n,
+                     *dst);
+  return hip_ok(hipGetLastError(), "encode_q8_kernel") && hip_ok(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+}
 
 template <class T>
 bool upload(const std::vector<T>& v, T** dst, size_t* bytes) {
@@ -380,7 +397,60 @@ TptQbvh* tpt_qbvh_build(const float* bmin, const float* bmax, size_t n, int32_t 
   }
 }
 
+TptQbvh* tpt_qbvh_build_device(TptDevice* d, const float* bmin, const float* bmax, size_t n, uint32_t flags) {
+  try {
+    if (!d) {
+      set_error("tpt_qbvh_build_device: null device");
+      return nullptr;
+    }
+    if (n == 0) {
+      set_error("tpt_qbvh_build_device: no primitives (build_qbvh returns None)");
+      return nullptr;
+    }
+    if (!bmin || !bmax) {
+      set_error("tpt_qbvh_build_device: null bounds");
+      return nullptr;
+    }
+    if (!hip_ok(hipSetDevice(d->index), "hipSetDevice")) return nullptr;
+    TptQbvh* q = new TptQbvh();
+    const int rc = build_qbvh_device(d, bmin, bmax, n, (flags & TPT_FLAG_DEVICE_POINTERS) != 0, &q->q);
+    if (rc != 0) {
+      if (rc == -2) set_error("tpt_qbvh_build_device: leaf offset exceeds the 24-bit child encoding");
+      else if (rc == -3)
+        set_error("tpt_qbvh_build_device: a forced median split over non-coincident centroids (the reference "
+                  "sorts them; use tpt_qbvh_build)");
+      else if (rc == -1) set_error("tpt_qbvh_build_device: more than 2^31 - 1 primitives");
+      delete q;
+      return nullptr;
+    }
+    return q;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return nullptr;
+  }
+}
+
 void tpt_qbvh_release(TptQbvh* q) { delete q; }
+
+int32_t tpt_scene_q8_nodes(const TptScene* s, uint32_t tlas, void* out, size_t capacity, size_t* count) {
+  if (!s || !count) {
+    set_error("tpt_scene_q8_nodes: null argument");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  const size_t n = tlas ? s->tlas_node_count : s->node_count;
+  *count = n;
+  if (!out) return TPT_OK;
+  if (capacity < n) {
+    set_error("tpt_scene_q8_nodes: capacity below the node count");
+    return TPT_ERROR_BAD_ARGUMENT;
+  }
+  if (!n) return TPT_OK;
+  if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice") ||
+      !hip_ok(hipStreamSynchronize(s->dev->stream), "hipStreamSynchronize") ||
+      !hip_ok(hipMemcpy(out, tlas ? s->qtlas : s->qnodes, n * sizeof(GpuNodeQ8), hipMemcpyDeviceToHost), "hipMemcpy"))
+    return TPT_ERROR_DEVICE;
+  return TPT_OK;
+}
 size_t tpt_qbvh_node_count(const TptQbvh* q) { return q ? q->q.nodes.size() : 0; }
 size_t tpt_qbvh_primitive_count(const TptQbvh* q) { return q ? q->q.prim.size() : 0; }
 int32_t tpt_qbvh_export(const TptQbvh* q, TptQbvhNode* nodes, uint32_t* prim) {
@@ -416,7 +486,6 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
     }
     if (!hip_ok(hipSetDevice(dev->index), "hipSetDevice")) return nullptr;
     std::vector<GpuNode> nodes;
-    std::vector<GpuNodeQ8> qnodes;
     std::vector<GpuTri> tris;
     std::vector<uint32_t> rank;
     std::vector<uint32_t> mesh_node_base(desc->mesh_count), mesh_tri_base(desc->mesh_count);
@@ -452,9 +521,6 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
           }
         }
         nodes.push_back(to_gpu_node(mn[k], nb));
-        GpuNodeQ8 qn;
-        quantize_node(nodes.back(), &qn);
-        qnodes.push_back(qn);
       }
       blas_stack = std::max(blas_stack, stack_bound(mn, 0, 0) + 1);
       const size_t tb = tris.size();
@@ -504,14 +570,8 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
       g.tri_count = desc->meshes[id.mesh].triangle_count;
     }
     std::vector<GpuNode> tlas;
-    std::vector<GpuNodeQ8> qtlas;
     std::vector<TptQbvhNode> tn(desc->tlas_nodes, desc->tlas_nodes + desc->tlas_node_count);
-    for (uint32_t k = 0; k < desc->tlas_node_count; ++k) {
-      tlas.push_back(to_gpu_node(tn[k], 0));
-      GpuNodeQ8 qn;
-      quantize_node(tlas.back(), &qn);
-      qtlas.push_back(qn);
-    }
+    for (uint32_t k = 0; k < desc->tlas_node_count; ++k) tlas.push_back(to_gpu_node(tn[k], 0));
     std::vector<uint32_t> tprim(desc->tlas_primitive_indices, desc->tlas_primitive_indices + desc->instance_count);
     for (uint32_t v : tprim)
       if (v >= desc->instance_count) {
@@ -601,10 +661,12 @@ TptScene* tpt_scene_create(TptDevice* dev, const TptSceneDesc* desc) {
     s->w_tree = lnodes.empty() ? 0.0f : std::fmax(lnodes[0].flux, 0.0f);
     s->light_nodes = std::move(lnodes);
     s->host_mats = mats;
-    if (!upload(nodes, &s->nodes, &s->bytes) || !upload(qnodes, &s->qnodes, &s->bytes) ||
+    s->node_count = (uint32_t)nodes.size();
+    // the 8-bit nodes are encoded on the device from the uploaded FP32 nodes
+    if (!upload(nodes, &s->nodes, &s->bytes) || !encode_q8(dev, s->nodes, nodes.size(), &s->qnodes, &s->bytes) ||
         !upload(tris, &s->tris, &s->bytes) || !upload(rank, &s->tri_rank, &s->bytes) ||
         !upload(inst, &s->inst, &s->bytes) || !upload(tlas, &s->tlas, &s->bytes) ||
-        !upload(qtlas, &s->qtlas, &s->bytes) || !upload(tprim, &s->tlas_prim, &s->bytes) ||
+        !encode_q8(dev, s->tlas, tlas.size(), &s->qtlas, &s->bytes) || !upload(tprim, &s->tlas_prim, &s->bytes) ||
         !upload(shade, &s->shade, &s->bytes) || !upload(vpos, &s->vpos, &s->bytes) ||
         !upload(vnrm, &s->vnrm, &s->bytes) || !upload(vuv, &s->vuv, &s->bytes) ||
         !upload(vidx, &s->vidx, &s->bytes) || !upload(mats, &s->mats, &s->bytes) ||
@@ -694,9 +756,12 @@ int32_t tpt_scene_light_tree(const TptScene* s, TptLightNode* out, uint32_t capa
 namespace tpt {
 namespace {
 
-// Kernel variant for (query, node format, exact shadow rule, counters, levels).
-const void* pick_kernel(int query, bool q8, bool exact, bool cnt, bool two) {
-#define TPT_K(Q8, M, E, C, T) reinterpret_cast<const void*>(&trace_kernel<Q8, M, E, C, T>)
+// Kernel variant for (query, node format, exact shadow rule, counters,
+// levels, the default LDS stack depth as a constant).
+const void* pick_kernel(int query, bool q8, bool exact, bool cnt, bool two, bool fixed_lds) {
+#define TPT_K(Q8, M, E, C, T)                                                         \
+  (fixed_lds ? reinterpret_cast<const void*>(&trace_kernel<Q8, M, E, C, T, kLdsDepth>) \
+             : reinterpret_cast<const void*>(&trace_kernel<Q8, M, E, C, T, 0>))
 #define TPT_BY_LEVEL(Q8, M, E, C) (two ? TPT_K(Q8, M, E, C, true) : TPT_K(Q8, M, E, C, false))
 #define TPT_BY_COUNT(Q8, M, E) (cnt ? TPT_BY_LEVEL(Q8, M, E, true) : TPT_BY_LEVEL(Q8, M, E, false))
   if (query == 0) return q8 ? TPT_BY_COUNT(true, kClosest, false) : TPT_BY_COUNT(false, kClosest, false);
@@ -809,10 +874,11 @@ int32_t trace_impl(TptScene* s, int query, const TptRay* rays, size_t n, const u
   q.grab = d->refill_grab;
   q.regions = d->regions;
   q.lds_depth = d->lds_depth;
-  const size_t lds = (size_t)q.lds_depth * kBlock * 4;
+  const bool fixed_lds = q.lds_depth == (uint32_t)kLdsDepth;  // static LDS, no launch-sized LDS
+  const size_t lds = fixed_lds ? 0 : (size_t)q.lds_depth * kBlock * 4;
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const bool cnt = dcounts != nullptr;
-  const void* kern = pick_kernel(query, q8, exact, cnt, two);
+  const void* kern = pick_kernel(query, q8, exact, cnt, two, fixed_lds);
   const unsigned blocks = grid_blocks(d, kern, n, lds);
   void* args[] = {(void*)&v, (void*)&drays, (void*)&q};
   if (!hip_ok(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, lds, d->stream), "hipLaunchKernel"))
@@ -860,9 +926,10 @@ bool launch_trace_queue(TptScene* s, int query, bool q8, bool exact, const TptRa
   q.grab = d->refill_grab;
   q.regions = d->regions;
   q.lds_depth = d->lds_depth;
-  const size_t lds = (size_t)q.lds_depth * kBlock * 4;
+  const bool fixed_lds = q.lds_depth == (uint32_t)kLdsDepth;  // static LDS, no launch-sized LDS
+  const size_t lds = fixed_lds ? 0 : (size_t)q.lds_depth * kBlock * 4;
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
-  const void* kern = pick_kernel(query, q8, exact, false, two);
+  const void* kern = pick_kernel(query, q8, exact, false, two, fixed_lds);
   const unsigned blocks = grid_blocks(d, kern, n_max, lds);
   void* args[] = {(void*)&v, (void*)&rays, (void*)&q};
   return hip_ok(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, lds, st), "hipLaunchKernel(trace queue)");

@@ -37,9 +37,9 @@ struct Query {
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
-template <bool kQ8, int kMode, bool kExact, bool kCount, bool kTwoLevel>
+template <bool kQ8, int kMode, bool kExact, bool kCount, bool kTwoLevel, class Stack>
 __device__ __forceinline__ void begin_ray(const SceneView& sc, const TptRay* rays, const Query& q, uint32_t rid,
-                                          Lane& L, LaneStack& st) {
+                                          Lane& L, Stack& st) {
   F3 o, d;
   float tmax;
   load_ray(rays, rid, &o, &d, &tmax);
@@ -122,15 +122,25 @@ __device__ __forceinline__ void finish_ray(const Query& q, uint32_t rid, const L
 
 // Persistent trace kernel: every wave loops, refilling idle lanes with new
 // rays, until all rays are done.  Exits are wave-uniform.
-template <bool kQ8, int kMode, bool kExact, bool kCount, bool kTwoLevel>
+// kLds: the LDS stack depth as a constant with a static LDS array (the
+// default depth), or 0 for the tuned depth q.lds_depth in LDS sized at
+// launch.
+template <bool kQ8, int kMode, bool kExact, bool kCount, bool kTwoLevel, int kLds>
 __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView sc, const TptRay* __restrict__ rays, Query q) {
-  extern __shared__ uint32_t lds[];  // q.lds_depth * kBlock entries
-  uint32_t spill[kStackDepth - kMinLdsDepth];
-  LaneStack st;
+  uint32_t* lds;
+  if constexpr (kLds > 0) {
+    __shared__ uint32_t static_lds[kLds * kBlock];
+    lds = static_lds;
+  } else {
+    extern __shared__ uint32_t dynamic_lds[];  // q.lds_depth * kBlock entries
+    lds = dynamic_lds;
+  }
+  uint32_t spill[kStackDepth - (kLds > 0 ? kLds : kMinLdsDepth)];
+  LaneStackT<kLds> st;
   st.lds = lds + threadIdx.x;
   st.spill = spill;
   st.sp = 0;
-  st.lds_depth = (int)q.lds_depth;
+  st.lds_depth = kLds > 0 ? kLds : (int)q.lds_depth;
   st.overflow = false;
   Lane L;
   Counters cnt = {0, 0, 0};

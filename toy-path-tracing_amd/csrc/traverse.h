@@ -398,25 +398,32 @@ __device__ __forceinline__ uint32_t slab4(const GpuNode* __restrict__ nodes, con
   return bits;
 }
 
-// Per-lane traversal stack: lds_depth entries in LDS laid out
-// [depth][thread] (the bank is the thread, so lanes at different depths never
-// conflict), continued in a private array up to the reference's 128 entries
-// (R/qbvh.rs:609).  lds_depth is kLdsDepth in the path kernels and the
-// device's tuned "lds_depth" in the trace kernels (whose LDS is sized at
-// launch).  The stack pointer is a register of its own: it must not share an
-// aggregate with the dynamically indexed spill array, or it would be kept in
-// scratch memory.
-struct LaneStack {
+// Per-lane traversal stack: depth() entries in LDS laid out [depth][thread]
+// (the bank is the thread, so lanes at different depths never conflict),
+// continued in a private array up to the reference's 128 entries
+// (R/qbvh.rs:609).  kDepth > 0 fixes the LDS depth at compile time (the
+// path kernels and the trace kernels' default); kDepth == 0 reads it from
+// lds_depth (the trace kernels at a tuned depth, LDS sized at launch).  A
+// runtime depth costs the shadow kernel two VGPRs (82 against 80: five
+// waves per SIMD instead of six), so the default stays a constant.  The
+// stack pointer is a register of its own: it must not share an aggregate
+// with the dynamically indexed spill array, or it would be kept in scratch.
+template <int kDepth>
+struct LaneStackT {
   uint32_t* lds;    // lds[d * kBlock] is this thread's entry d
-  uint32_t* spill;  // private array of at least kStackDepth - lds_depth entries
+  uint32_t* spill;  // private array of at least kStackDepth - depth() entries
   int sp;
-  int lds_depth;
+  int lds_depth;    // read only when kDepth == 0
   bool overflow;
+  __device__ __forceinline__ int depth() const {
+    if constexpr (kDepth > 0) return kDepth;
+    else return lds_depth;
+  }
   __device__ __forceinline__ void push(uint32_t v) {
-    if (sp < lds_depth) {
+    if (sp < depth()) {
       lds[sp * kBlock] = v;
     } else if (sp < kStackDepth) {
-      spill[sp - lds_depth] = v;
+      spill[sp - depth()] = v;
     } else {
       overflow = true;  // the reference panics here; the entry is dropped
       return;
@@ -425,9 +432,10 @@ struct LaneStack {
   }
   __device__ __forceinline__ uint32_t pop() {
     --sp;
-    return sp < lds_depth ? lds[sp * kBlock] : spill[sp - lds_depth];
+    return sp < depth() ? lds[sp * kBlock] : spill[sp - depth()];
   }
 };
+using LaneStack = LaneStackT<kLdsDepth>;
 
 // Leaves of the last visited node, in processing order (q0 first).
 struct LeafQueue {
@@ -477,9 +485,9 @@ enum VisitOrder { kVisitReference = 0, kVisitNearLast = 1, kVisitSlot = 2 };
 // its conservative entry distances (an order that can differ from the
 // reference's only between children whose entry distances tie within
 // rounding).
-template <bool kQ8, int kOrder = kVisitReference, bool kCons = false>
+template <bool kQ8, int kOrder = kVisitReference, bool kCons = false, class Stack = LaneStack>
 __device__ __forceinline__ void visit_node(const GpuNode* nodes, const GpuNodeQ8* qnodes, uint32_t ni,
-                                           const Ray3& r, float t_max, LaneStack& st, LeafQueue& lq) {
+                                           const Ray3& r, float t_max, Stack& st, LeafQueue& lq) {
   float tmin[4];
   uint32_t child[4];
   uint32_t bits;
@@ -571,7 +579,8 @@ struct Lane {
   LeafQueue tq;
 };
 
-__device__ __forceinline__ void enter_instance(const SceneView& sc, Lane& L, LaneStack& st, uint32_t ii,
+template <class Stack>
+__device__ __forceinline__ void enter_instance(const SceneView& sc, Lane& L, Stack& st, uint32_t ii,
                                                const F3& wo, const F3& wd) {
   const GpuInstance in = sc.inst[ii];
   to_local(in, wo, wd, &L.r);
@@ -589,8 +598,8 @@ __device__ __forceinline__ void enter_instance(const SceneView& sc, Lane& L, Lan
   L.level = 1;
 }
 
-template <bool kTwoLevel>
-__device__ __forceinline__ void lane_begin(const SceneView& sc, Lane& L, LaneStack& st, F3 o, F3 d, float t_max,
+template <bool kTwoLevel, class Stack>
+__device__ __forceinline__ void lane_begin(const SceneView& sc, Lane& L, Stack& st, F3 o, F3 d, float t_max,
                                            float strict, uint32_t skip_inst, uint32_t skip_tri) {
   L.closest = t_max;
   L.strict = strict;
@@ -623,8 +632,8 @@ __device__ __forceinline__ void lane_begin(const SceneView& sc, Lane& L, LaneSta
 // One iteration: at most one node visit and one triangle test (the "if-if"
 // form: a lane that pops a node with leaf children tests a triangle in the
 // same iteration).  Returns true when the lane's query is finished.
-template <bool kQ8, int kMode, bool kCount, bool kTwoLevel>
-__device__ __forceinline__ bool lane_step(const SceneView& sc, Lane& L, LaneStack& st, Counters& cnt) {
+template <bool kQ8, int kMode, bool kCount, bool kTwoLevel, class Stack>
+__device__ __forceinline__ bool lane_step(const SceneView& sc, Lane& L, Stack& st, Counters& cnt) {
   if (kTwoLevel && L.level == 0) {
     if (L.inst_cur >= L.inst_end) {
       if (L.tq.n == 0) {

@@ -204,6 +204,10 @@ _SIGNATURES = [
     ("tpt_qbvh_export", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("tpt_qbvh_quantize", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("tpt_scene_create", ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("tpt_qbvh_build_device", ctypes.c_void_p,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32]),
+    ("tpt_scene_q8_nodes", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("tpt_scene_desc_save", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p]),
     ("tpt_scene_desc_load", ctypes.c_void_p, [ctypes.c_char_p]),
     ("tpt_scene_desc_free", None, [ctypes.c_void_p]),

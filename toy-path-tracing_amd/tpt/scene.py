@@ -41,14 +41,19 @@ class Qbvh:
     primitive_indices: np.ndarray  # uint32
 
 
-def build_qbvh(bounds_min: np.ndarray, bounds_max: np.ndarray, threads: int = 0) -> Qbvh | None:
-    """build_qbvh (R/qbvh.rs:144-166): None for no primitives."""
+def build_qbvh(bounds_min: np.ndarray, bounds_max: np.ndarray, threads: int = 0,
+               device: "Device | None" = None) -> Qbvh | None:
+    """build_qbvh (R/qbvh.rs:144-166): None for no primitives.  With a
+    device the build runs on that GPU (tpt_qbvh_build_device; the same tree)."""
     bmin = np.ascontiguousarray(bounds_min, dtype=np.float32).reshape(-1, 3)
     bmax = np.ascontiguousarray(bounds_max, dtype=np.float32).reshape(-1, 3)
     n = bmin.shape[0]
     if n == 0:
         return None
-    h = lib().tpt_qbvh_build(ptr(bmin), ptr(bmax), n, threads)
+    if device is not None:
+        h = lib().tpt_qbvh_build_device(device.handle, ptr(bmin), ptr(bmax), n, 0)
+    else:
+        h = lib().tpt_qbvh_build(ptr(bmin), ptr(bmax), n, threads)
     if not h:
         raise _lib.TptError(_lib.TPT_ERROR, _lib.last_error())
     try:
@@ -101,10 +106,10 @@ class Mesh:
         hi = np.fmax(np.fmax(v[:, 0], v[:, 1]), v[:, 2])
         return lo, hi
 
-    def build_qbvh(self, threads: int = 0) -> None:
-        """Mesh::build_qbvh (R/scene/mesh.rs:114-119)."""
+    def build_qbvh(self, threads: int = 0, device: "Device | None" = None) -> None:
+        """Mesh::build_qbvh (R/scene/mesh.rs:114-119); on `device` when given."""
         lo, hi = self.triangle_bounds()
-        self.qbvh = build_qbvh(lo, hi, threads)
+        self.qbvh = build_qbvh(lo, hi, threads, device)
 
 
 def glam_cols(m: np.ndarray) -> np.ndarray:
@@ -228,14 +233,15 @@ class Scene:
         self.built = False
         return len(self.instances) - 1
 
-    def build_qbvh(self, threads: int = 0) -> None:
-        """Scene::build_qbvh (R/scene/mod.rs:305-316)."""
+    def build_qbvh(self, threads: int = 0, device: "Device | None" = None) -> None:
+        """Scene::build_qbvh (R/scene/mod.rs:305-316); the meshes' and the
+        instances' QBVHs built on `device` when given (the same trees)."""
         for m in self.meshes:
-            m.build_qbvh(threads)
+            m.build_qbvh(threads, device)
         if self.instances:
             lo = np.stack([i.world_bounds[0] for i in self.instances])
             hi = np.stack([i.world_bounds[1] for i in self.instances])
-            self.qbvh = build_qbvh(lo, hi, threads)
+            self.qbvh = build_qbvh(lo, hi, threads, device)
         else:
             self.qbvh = None
         self.built = True
