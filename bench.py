@@ -534,7 +534,9 @@ def city_bench(dev, stream, args):
         "megakernel": {"ms_per_batch": round(ms, 3), "samples_per_s": round(samples / (ms / 1e3), 1)},
     }
     log(f"city megakernel fp32: {ms:.1f} ms per 4 frames = {samples / (ms / 1e3) / 1e6:.1f} Msamples/s")
-    tune = rds.autotune(frames, quantized=True, tune_spp=spp)  # the workload itself: 46 M paths
+    # the workload itself (46 M paths at 4 spp); at more samples per pixel the
+    # trials run at 4 spp, the same per-sample work
+    tune = rds.autotune(frames, quantized=True, tune_spp=min(spp, 4))
     res["autotune"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()}
     log(f"city autotune: {res['autotune']}")
     wf = [torch.empty_like(o) for o in mega]

@@ -108,9 +108,10 @@ typedef struct TptInstanceDesc {
 /* OpenPbrMaterial parameters (R/material/open_pbr.rs:23-82) as the renderer
  * holds them after prepare_color (rendering-space colours): base, metal,
  * specular, transmission (the specular BTDF, rough or smooth, and thin
- * walls), coat, fuzz and emission.  Materials with a thin film
- * (thin_film_weight != 0) or dispersion (transmission_dispersion_scale > 0)
- * are rejected with TPT_ERROR_BAD_ARGUMENT (normal maps, opacity, subsurface
+ * walls), coat, fuzz, thin film (thin_film_weight, thin_film_thickness,
+ * thin_film_ior; no thickness texture) and emission.  Materials with
+ * dispersion (transmission_dispersion_scale > 0) are rejected with
+ * TPT_ERROR_BAD_ARGUMENT (normal maps, opacity, subsurface
  * and the interior medium are not part of the record; the reference
  * forces the last two off, validate_and_warn :303-321).  The fields after
  * emission_color_texture carry transmission_color (default 1),

@@ -879,7 +879,174 @@ struct PbrBsdf {
   bool front_face;
   Luts luts;
   real spec_roughness;  // effective_specular_roughness (the sample's cone spread reads it)
+  // thin film (OpenPbrBsdfParams :58-65): weight in [0, 1], thickness in nm
+  // (0 without a film), film IOR; the metal's complex IOR
+  // (artist_friendly_complex_ior of base_color and specular_color)
+  real film_weight = 0.0f, film_thickness = 0.0f, film_ior = 1.4f;
+  V3 metal_n = {0.0f, 0.0f, 0.0f}, metal_k = {0.0f, 0.0f, 0.0f};
 };
+
+// ---- thin film (R/bsdf/thin_film.rs, R/bsdf/conductor_complex.rs:132-147) -----------------
+// artist_friendly_complex_ior (conductor_complex.rs:132-147): Gulbrandsen's
+// n, k from the face (normal-incidence) and edge colours
+inline void artist_friendly_complex_ior(V3 face, V3 edge, V3* n_out, V3* k_out) {
+  const real c[3] = {face.x, face.y, face.z}, e[3] = {edge.x, edge.y, edge.z};
+  real n[3], k[3];
+  for (int i = 0; i < 3; ++i) {
+    const real r = clampf(c[i], 0.0f, 0.999f), g = clampf(e[i], 0.0f, 1.0f);
+    const real rs = std::sqrt(std::fmax(r, 0.0f));
+    const real n_max = (1.0f + rs) / std::fmax(1.0f - rs, 1.0e-6f);
+    const real n_min = (1.0f - r) / (1.0f + r);
+    n[i] = g * n_min + (1.0f - g) * n_max;
+    const real np1 = n[i] + 1.0f, nm1 = n[i] - 1.0f;
+    const real k2 = std::fmax((np1 * np1 * r - nm1 * nm1) / std::fmax(1.0f - r, 1.0e-6f), 0.0f);
+    k[i] = std::sqrt(std::fmax(k2, 0.0f));
+  }
+  *n_out = v3(n[0], n[1], n[2]);
+  *k_out = v3(k[0], k[1], k[2]);
+}
+// fresnel_dielectric_polarized (thin_film.rs:143-156): |r_s|, |r_p| and
+// their phases (pi where the amplitude is negative); total internal
+// reflection gives (1, 1) with phase 0
+inline void film_fresnel_dielectric(real cos_i, real eta_i, real eta_t, real r[2], real phi[2]) {
+  cos_i = clampf(cos_i, 0.0f, 1.0f);
+  const real sin_i_sq = std::fmax(1.0f - cos_i * cos_i, 0.0f);
+  const real ratio = eta_i / eta_t;
+  const real sin_t_sq = ratio * ratio * sin_i_sq;
+  if (sin_t_sq >= 1.0f) {
+    r[0] = r[1] = 1.0f;
+    phi[0] = phi[1] = 0.0f;
+    return;
+  }
+  const real cos_t = std::sqrt(std::fmax(1.0f - sin_t_sq, 0.0f));
+  const real rs = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
+  const real rp = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
+  phi[0] = rs < 0.0f ? kPi : 0.0f;
+  phi[1] = rp < 0.0f ? kPi : 0.0f;
+  r[0] = std::fabs(rs);
+  r[1] = std::fabs(rp);
+}
+// fresnel_complex_polarized (thin_film.rs:193-244): film (eta_i) into a
+// base of complex IOR eta_t + i kappa_t: amplitudes and phases per
+// polarisation
+inline void film_fresnel_complex(real cos_i, real eta_i, real eta_t, real kappa_t, real* rs_out, real* rp_out,
+                                 real* phs_out, real* php_out) {
+  cos_i = clampf(cos_i, 0.0f, 1.0f);
+  const real cos_i2 = cos_i * cos_i;
+  const real sin_i2 = std::fmax(1.0f - cos_i2, 0.0f);
+  const real eta_r = eta_t / std::fmax(eta_i, 1.0e-6f);
+  const real kappa_r = kappa_t / std::fmax(eta_i, 1.0e-6f);
+  const real eta_r2 = eta_r * eta_r, kappa_r2 = kappa_r * kappa_r;
+  const real inner = eta_r2 - kappa_r2 - sin_i2;
+  const real radicand = std::fmax(inner * inner + 4.0f * eta_r2 * kappa_r2, 0.0f);
+  const real a2_plus_b2 = std::sqrt(radicand);
+  const real a2 = std::fmax(0.5f * (a2_plus_b2 + inner), 0.0f);
+  const real b2 = std::fmax(0.5f * (a2_plus_b2 - inner), 0.0f);
+  const real a = std::sqrt(a2);
+  const real t1 = a2_plus_b2 + cos_i2, t2 = 2.0f * a * cos_i;
+  const real r_s2 = (t1 - t2) / std::fmax(t1 + t2, 1.0e-12f);
+  const real t3 = a2_plus_b2 * cos_i2 + sin_i2 * sin_i2, t4 = t2 * sin_i2;
+  const real r_p2 = r_s2 * (t3 - t4) / std::fmax(t3 + t4, 1.0e-12f);
+  *rs_out = std::sqrt(std::fmax(r_s2, 0.0f));
+  *rp_out = std::sqrt(std::fmax(r_p2, 0.0f));
+  *phs_out = std::atan2(2.0f * std::sqrt(b2) * cos_i, cos_i2 - a2 - b2);
+  if (r_p2 > 0.0f) {
+    const real p_num = 2.0f * cos_i * (std::sqrt(b2) * (eta_r2 + kappa_r2) - 2.0f * a * eta_r * kappa_r);
+    const real p_den = (eta_r2 + kappa_r2) * (eta_r2 + kappa_r2) * cos_i2 - (a2 + b2);
+    *php_out = std::atan2(p_num, p_den);
+  } else {
+    *php_out = 0.0f;
+  }
+}
+// eval_sensitivity (thin_film.rs:111-141): the Fourier transform of the
+// CIE XYZ matching functions (Gaussian fits) at the optical path
+// difference, with a phase shift per channel, in linear Rec.709
+inline V3 film_sensitivity(real opd, V3 shift) {
+  const real phase = 2.0f * kPi * opd * 1.0e-9f;
+  const real val[3] = {5.4856e-13f, 4.4201e-13f, 5.2481e-13f};
+  const real pos[3] = {1.6810e+06f, 1.7953e+06f, 2.2084e+06f};
+  const real var[3] = {4.3278e+09f, 9.3046e+09f, 6.6121e+09f};
+  const real sh[3] = {shift.x, shift.y, shift.z};
+  real xyz[3];
+  for (int i = 0; i < 3; ++i) {
+    const real amp = std::sqrt(std::fmax(var[i] * (2.0f * kPi), 0.0f));
+    xyz[i] = val[i] * amp * std::cos(pos[i] * phase + sh[i]) * std::exp(-phase * phase * var[i]);
+  }
+  const real extra_var = 4.5282e+09f;
+  xyz[0] += 9.7470e-14f * std::sqrt(2.0f * kPi * extra_var) * std::cos(2.2399e+06f * phase + sh[0]) *
+            std::exp(-extra_var * phase * phase);
+  for (int i = 0; i < 3; ++i) xyz[i] /= 1.0685e-7f;
+  // XYZ_TO_REC709 (thin_film.rs:5-9), columns applied in order
+  const real m[3][3] = {{3.2404542f, -0.969266f, 0.0556434f},
+                        {-1.5371385f, 1.8750108f, -0.2040259f},
+                        {-0.4985314f, 0.041556f, 1.0572252f}};
+  real out[3];
+  for (int r = 0; r < 3; ++r) out[r] = m[0][r] * xyz[0] + m[1][r] * xyz[1] + m[2][r] * xyz[2];
+  return v3(out[0], out[1], out[2]);
+}
+// eval_thin_film_inner (thin_film.rs:47-109): the film's reflectance for an
+// incident cosine from the outside medium, the film IOR, the base's
+// (complex) IOR per channel and the thickness in nm: the incoherent
+// two-interface sum plus the first two interference orders, per
+// polarisation, averaged and clamped to [0, 1].
+inline V3 thin_film(real cos_theta_i, real eta_outside, real eta_film, V3 n_base, V3 k_base, real thickness_nm) {
+  const real cos1 = clampf(cos_theta_i, 0.0f, 1.0f);
+  eta_outside = std::fmax(eta_outside, 1.0e-3f);
+  eta_film = std::fmax(eta_film, 1.0e-3f);
+  const real eta_ratio = eta_outside / eta_film;
+  const real sin1_sq = std::fmax(1.0f - cos1 * cos1, 0.0f);
+  const real sin2_sq = eta_ratio * eta_ratio * sin1_sq;
+  if (sin2_sq >= 1.0f) return vsplat(1.0f);
+  const real cos2 = std::sqrt(std::fmax(1.0f - sin2_sq, 0.0f));
+  real r12[2], phi12[2];
+  film_fresnel_dielectric(cos1, eta_outside, eta_film, r12, phi12);
+  const real phi21[2] = {kPi - phi12[0], kPi - phi12[1]};
+  const real r12_sq[2] = {r12[0] * r12[0], r12[1] * r12[1]};
+  const real t121_sq[2] = {std::fmax(1.0f - r12_sq[0], 0.0f), std::fmax(1.0f - r12_sq[1], 0.0f)};
+  const real nb[3] = {n_base.x, n_base.y, n_base.z}, kb[3] = {k_base.x, k_base.y, k_base.z};
+  real r23_sq[2][3], phi23[2][3];
+  for (int c = 0; c < 3; ++c) {
+    real rs, rp, ps, pp;
+    film_fresnel_complex(cos2, eta_film, nb[c], kb[c], &rs, &rp, &ps, &pp);
+    r23_sq[0][c] = rs * rs;
+    r23_sq[1][c] = rp * rp;
+    phi23[0][c] = ps;
+    phi23[1][c] = pp;
+  }
+  const real opd = 2.0f * eta_film * thickness_nm * cos2;
+  real res[2][3];
+  for (int pol = 0; pol < 2; ++pol) {
+    const V3 phi2 = v3(phi21[pol] + phi23[pol][0], phi21[pol] + phi23[pol][1], phi21[pol] + phi23[pol][2]);
+    real i_c[3], cm[3], r_geo[3];
+    for (int c = 0; c < 3; ++c) {
+      r_geo[c] = std::sqrt(std::fmax(r12_sq[pol] * r23_sq[pol][c], 0.0f));
+      i_c[c] = r12_sq[pol] + t121_sq[pol] * r23_sq[pol][c] / std::fmax(1.0f - r12_sq[pol] * r23_sq[pol][c], 1.0e-6f);
+      cm[c] = t121_sq[pol] * r23_sq[pol][c];
+    }
+    for (int m = 1; m <= 2; ++m) {
+      const V3 sens = film_sensitivity((real)m * opd, smul(phi2, (real)m));
+      const real sv[3] = {sens.x, sens.y, sens.z};
+      for (int c = 0; c < 3; ++c) {
+        i_c[c] += 2.0f * cm[c] * sv[c];
+        cm[c] *= r_geo[c];
+      }
+    }
+    for (int c = 0; c < 3; ++c) res[pol][c] = i_c[c];
+  }
+  return v3(clampf(0.5f * (res[0][0] + res[1][0]), 0.0f, 1.0f), clampf(0.5f * (res[0][1] + res[1][1]), 0.0f, 1.0f),
+            clampf(0.5f * (res[0][2] + res[1][2]), 0.0f, 1.0f));
+}
+// the film over a base seen through the outside medium or, with a coat,
+// blended with the film under the coat by the coat weight (the common form
+// of metal_fresnel :761-790, spec_brdf_fresnel :869-900 and
+// spec_btdf_fresnel_with_eta :1120-1152)
+inline V3 film_coated(const PbrBsdf& p, real c, V3 n_base, V3 k_base) {
+  const V3 uncoated = thin_film(c, 1.0f, p.film_ior, n_base, k_base, p.film_thickness);
+  if (p.coat <= 0.0f) return uncoated;
+  const V3 coated = thin_film(c, p.coat_eta, p.film_ior, n_base, k_base, p.film_thickness);
+  return vlerp(uncoated, coated, clampf(p.coat, 0.0f, 1.0f));
+}
+inline bool film_active(const PbrBsdf& p) { return p.film_weight > 0.0f && p.film_thickness > 0.0f; }
 
 // apply_fuzz_roughening / effective_*_roughness (:736-778)
 inline real fuzz_roughening(const OpenPbr& m, real r, real fw, real fr) {
@@ -954,6 +1121,11 @@ inline PbrBsdf make_bsdf(const OpenPbr& m, bool front_face, V3 wo_local, const L
   p.thin_walled = m.thin_walled != 0;
   p.front_face = front_face;
   p.luts = luts;
+  // thin_film_thickness_nm_at (R/material/open_pbr.rs:792-813), untextured
+  p.film_weight = clampf(m.thin_film_weight, 0.0f, 1.0f);
+  p.film_thickness = m.thin_film_weight <= 0.0f ? 0.0f : std::fmax(m.thin_film_thickness, 0.0f) * 1000.0f;
+  p.film_ior = std::fmax(m.thin_film_ior, 1.0e-4f);
+  artist_friendly_complex_ior(vclamp01(base_color), vclamp01(m.specular_color), &p.metal_n, &p.metal_k);
   return p;
 }
 
@@ -1162,7 +1334,7 @@ inline real pdf_coat(const PbrBsdf& p, V3 wo, V3 wi) {  // :724-742
   if (den <= 0.0f) return w.ms * pms / w.total;
   return (w.ss * (pw / den) + w.ms * pms) / w.total;
 }
-inline V3 metal_fresnel(const PbrBsdf& p, real c) {  // metal_f82_tint_fresnel :792-802
+inline V3 metal_f82_fresnel(const PbrBsdf& p, real c) {  // metal_f82_tint_fresnel :792-802
   V3 f0 = vmax(smul(p.base_color, p.base), vsplat(0.0f));
   V3 fs = schlick_vec3(f0, c), f82 = schlick_vec3(f0, kCos82);
   V3 ft = vmul(vclamp01(p.specular_color), f82);
@@ -1172,7 +1344,15 @@ inline V3 metal_fresnel(const PbrBsdf& p, real c) {  // metal_f82_tint_fresnel :
   V3 r = sub(fs, smul(sub(f82, ft), corr));
   return vclamp01(smul(r, p.specular));
 }
-inline V3 metal_fresnel_avg(const PbrBsdf& p) {  // metal_f82_tint_fresnel_avg :832-839
+// metal_fresnel (:761-790): the F82-tint Fresnel, blended with the film
+// over the metal's complex IOR
+inline V3 metal_fresnel(const PbrBsdf& p, real c) {
+  const V3 no_film = metal_f82_fresnel(p, c);
+  if (!film_active(p)) return no_film;
+  return vlerp(no_film, film_coated(p, c, p.metal_n, p.metal_k), p.film_weight);
+}
+inline V3 metal_fresnel_avg(const PbrBsdf& p);
+inline V3 metal_f82_fresnel_avg(const PbrBsdf& p) {  // metal_f82_tint_fresnel_avg :832-839
   V3 f0 = vmax(smul(p.base_color, p.base), vsplat(0.0f));
   V3 f82 = schlick_vec3(f0, kCos82);
   V3 ft = vmul(vclamp01(p.specular_color), f82);
@@ -1180,6 +1360,17 @@ inline V3 metal_fresnel_avg(const PbrBsdf& p) {  // metal_f82_tint_fresnel_avg :
   V3 b = sdiv(sub(f82, ft), std::fmax(den, 1.0e-8f));
   V3 r = sub(add(f0, sdiv(sub(vsplat(1.0f), f0), 21.0f)), sdiv(b, 126.0f));
   return vclamp01(smul(r, p.specular));
+}
+// metal_fresnel_avg (:818-830): with a film, the cosine-weighted mean of
+// metal_fresnel over 16 midpoints
+inline V3 metal_fresnel_avg(const PbrBsdf& p) {
+  if (p.film_weight <= 0.0f || p.film_thickness <= 0.0f) return metal_f82_fresnel_avg(p);
+  V3 sum = vsplat(0.0f);
+  for (int i = 0; i < 16; ++i) {
+    const real mu = ((real)i + 0.5f) / 16.0f;
+    sum = add(sum, smul(metal_fresnel(p, mu), 2.0f * mu));
+  }
+  return sdiv(sum, 16.0f);
 }
 inline V3 eval_metal(const PbrBsdf& p, V3 wo, V3 wi) {  // :744-759, 804-816
   const V3 zero = vsplat(0.0f);
@@ -1222,9 +1413,19 @@ inline real base_dielectric_fresnel(const PbrBsdf& p, real c, real ep, real epr)
                           : weighted_fresnel_ratio(c, 1.0f / std::fmax(a, 1.0e-4f), 1.0f / std::fmax(b, 1.0e-4f));
   return lerpf(un, co, coat);
 }
-// spec_brdf_fresnel (:869-900) without a thin film
-inline real spec_brdf_fresnel(const PbrBsdf& p, real c) {
-  return base_dielectric_fresnel(p, c, p.transmission_eta, p.specular_eta);
+// spec_brdf_fresnel (:869-900): the base's dielectric Fresnel, blended with
+// the film over a dielectric base of IOR specular_eta
+inline V3 spec_brdf_fresnel(const PbrBsdf& p, real c) {
+  const V3 no_film = vsplat(base_dielectric_fresnel(p, c, p.transmission_eta, p.specular_eta));
+  if (!film_active(p)) return no_film;
+  return vlerp(no_film, film_coated(p, c, vsplat(p.specular_eta), vsplat(0.0f)), p.film_weight);
+}
+// spec_btdf_fresnel_with_eta (:1120-1152): the same for the refraction
+// Fresnel (no film on a thin wall)
+inline V3 spec_btdf_fresnel(const PbrBsdf& p, real c, real eta_physical, real eta_prime) {
+  const V3 no_film = vsplat(base_dielectric_fresnel(p, c, eta_physical, eta_prime));
+  if (!film_active(p) || p.thin_walled) return no_film;
+  return vlerp(no_film, film_coated(p, c, vsplat(eta_prime), vsplat(0.0f)), p.film_weight);
 }
 // thin_wall_coefficients (:931-959): the slab's total reflectance and
 // transmittance for an outgoing cosine, with the interior absorption
@@ -1236,17 +1437,21 @@ inline void thin_wall_coefficients(const PbrBsdf& p, real cos_theta_o, V3* refl,
   const real sin2_o = std::fmax(1.0f - cos_o * cos_o, 0.0f);
   const real sin2_i = clampf(sin2_o / (eta * eta), 0.0f, 1.0f);
   const real cos_i = std::fmax(std::sqrt(1.0f - sin2_i), 1.0e-4f);
-  const real f_o = clampf(spec_brdf_fresnel(p, cos_o), 0.0f, 1.0f);
-  // thin_wall_internal_fresnel (:961-979) without a thin film
-  const real f_i = clampf(weighted_fresnel_ratio(cos_i, 1.0f / std::fmax(eta, 1.0e-4f), 1.0f / std::fmax(eta_prime, 1.0e-4f)),
-                           0.0f, 1.0f);
+  const V3 f_o = vclamp01(spec_brdf_fresnel(p, cos_o));
+  // thin_wall_internal_fresnel (:961-979): the film seen from inside the
+  // slab (outside medium eta, base 1)
+  V3 f_i = vsplat(weighted_fresnel_ratio(cos_i, 1.0f / std::fmax(eta, 1.0e-4f), 1.0f / std::fmax(eta_prime, 1.0e-4f)));
+  if (film_active(p))
+    f_i = vlerp(f_i, thin_film(cos_i, std::fmax(eta, 1.0e-4f), p.film_ior, vsplat(1.0f), vsplat(0.0f), p.film_thickness),
+                p.film_weight);
+  f_i = vclamp01(f_i);
   const V3 a = vec3_powf(vclamp01(p.transmission_color), 1.0f / cos_i);
   const V3 a2 = vmul(a, a);
-  const V3 fi2a2 = smul(smul(a2, f_i), f_i);
+  const V3 fi2a2 = vmul(vmul(a2, f_i), f_i);
   const V3 denom = vmax(sub(vsplat(1.0f), fi2a2), vsplat(1.0e-6f));
-  const real omfo = std::fmax(1.0f - f_o, 0.0f), omfi = std::fmax(1.0f - f_i, 0.0f);
-  const V3 r = add(vsplat(f_o), vdiv3(smul(a2, omfo * omfi * f_i), denom));
-  const V3 t = vmul(smul(a, omfo * omfi), add(vsplat(1.0f), vdiv3(fi2a2, denom)));
+  const V3 omfo = vmax(sub(vsplat(1.0f), f_o), vsplat(0.0f)), omfi = vmax(sub(vsplat(1.0f), f_i), vsplat(0.0f));
+  const V3 r = add(f_o, vdiv3(vmul(vmul(vmul(omfo, omfi), f_i), a2), denom));
+  const V3 t = vmul(vmul(vmul(omfo, omfi), a), add(vsplat(1.0f), vdiv3(fi2a2, denom)));
   *refl = vclamp01(r);
   *trans = vclamp01(t);
 }
@@ -1270,9 +1475,9 @@ inline V3 eval_spec(const PbrBsdf& p, V3 wo, V3 wi) {  // eval_spec_brdf :841-86
     thin_wall_coefficients(p, std::fabs(wo.z), &r, &t);
     return smul(r, d * g / (4.0f * wo.z * wi.z));
   }
-  real f = spec_brdf_fresnel(p, std::fabs(dot(wo, wm)));
-  real ss = f * (d * g / (4.0f * wo.z * wi.z));
-  return vsplat(ss + eval_refl_ms(p, wo, wi, p.ax, p.ay, p.specular_eta));
+  V3 f = spec_brdf_fresnel(p, std::fabs(dot(wo, wm)));
+  V3 ss = smul(f, d * g / (4.0f * wo.z * wi.z));
+  return add(ss, vsplat(eval_refl_ms(p, wo, wi, p.ax, p.ay, p.specular_eta)));
 }
 inline real pdf_spec_ss(const PbrBsdf& p, V3 wo, V3 wi) {  // :981-994
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
@@ -1368,10 +1573,9 @@ inline V3 eval_spec_btdf(const PbrBsdf& p, V3 wo, V3 wi) {  // eval_spec_btdf :1
   if (wo.z <= 0.0f || wi.z >= 0.0f) return zero;
   real sc, co;
   if (!eval_btdf_scalar(p, wo, wi, trans_eta_rel(p), &sc, &co)) return zero;
-  // spec_btdf_fresnel_with_eta (:1120-1152) without a thin film
-  real f = base_dielectric_fresnel(p, co, p.transmission_eta, p.specular_eta);
-  real ss = std::fmax(1.0f - f, 0.0f) * sc;
-  return vsplat(ss + eval_trans_ms(p, wo, wi, p.ax, p.ay, p.specular_eta));
+  V3 f = spec_btdf_fresnel(p, co, p.transmission_eta, p.specular_eta);
+  V3 ss = smul(vmax(sub(vsplat(1.0f), f), vsplat(0.0f)), sc);
+  return add(ss, vsplat(eval_trans_ms(p, wo, wi, p.ax, p.ay, p.specular_eta)));
 }
 inline real pdf_btdf_ss(const PbrBsdf& p, V3 wo, V3 wi, real eta_rel) {  // pdf_specular_btdf_ss_with_eta :1193-1221
   V3 wm;
@@ -1587,7 +1791,7 @@ inline bool sample_lobe(const PbrBsdf& p, V3 wo, int lobe, real p_lobe, real ux,
         V3 t;
         thin_wall_coefficients(p, std::fabs(wo.z), &f, &t);
       } else {
-        f = vsplat(spec_brdf_fresnel(p, std::fabs(wi.z)));
+        f = spec_brdf_fresnel(p, std::fabs(wi.z));
       }
       LayerW w = layer_weights(p, wo, &wi);
       return delta_sample(sdiv(vmul(w.spec_brdf, f), p_lobe), wi, p_lobe, out);

@@ -634,6 +634,21 @@ float orc_glossy_importance(float rho, const float* frame, const float* wo, floa
   return lt::glossy_importance(g, f, v3(n[0], n[1], n[2]), w, l);
 }
 // sg_product: out = axis[3], sharpness, log_amplitude
+// eval_thin_film_inner (R/bsdf/thin_film.rs:47-109): out = RGB reflectance
+void orc_thin_film(float cos_theta_i, float eta_outside, float eta_film, const float* n, const float* k,
+                   float thickness_nm, float* out) {
+  V3 r = thin_film(cos_theta_i, eta_outside, eta_film, v3(n[0], n[1], n[2]), v3(k[0], k[1], k[2]), thickness_nm);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+// artist_friendly_complex_ior (R/bsdf/conductor_complex.rs:132-147): out = n[3], k[3]
+void orc_artist_ior(const float* face, const float* edge, float* out) {
+  V3 n, k;
+  artist_friendly_complex_ior(v3(face[0], face[1], face[2]), v3(edge[0], edge[1], edge[2]), &n, &k);
+  const real v[6] = {n.x, n.y, n.z, k.x, k.y, k.z};
+  for (int i = 0; i < 6; ++i) out[i] = (float)v[i];
+}
 void orc_sg_product(const float* a1, float s1, const float* a2, float s2, float* out) {
   lt::SgLobe l = lt::sg_product(v3(a1[0], a1[1], a1[2]), s1, v3(a2[0], a2[1], a2[2]), s2);
   out[0] = l.axis.x;

@@ -105,6 +105,8 @@ def lib(fp64: bool = False) -> ctypes.CDLL:
         L.orc_sg.argtypes = [ctypes.c_int, f32, f32]
         L.orc_sg.restype = f32
         L.orc_sg_product.argtypes = [vp, f32, vp, f32, vp]
+        L.orc_thin_film.argtypes = [f32, f32, f32, vp, vp, f32, vp]
+        L.orc_artist_ior.argtypes = [vp, vp, vp]
         L.orc_lut_sizes.argtypes = [vp]
         L.orc_lut_albedo.argtypes = [f32, vp]
         L.orc_lut_cond.argtypes = [vp, vp]
@@ -565,3 +567,21 @@ class OracleRenderer:
     def __del__(self):
         if getattr(self, "h", None):
             self._L.orc_render_scene_free(self.h)
+
+
+def thin_film(cos_theta_i, eta_outside, eta_film, n_base, k_base, thickness_nm):
+    """eval_thin_film_inner (R/bsdf/thin_film.rs:47-109) -> RGB reflectance."""
+    n = np.asarray(n_base, np.float32).reshape(3)
+    k = np.asarray(k_base, np.float32).reshape(3)
+    out = np.zeros(3, np.float32)
+    lib().orc_thin_film(cos_theta_i, eta_outside, eta_film, _p(n), _p(k), thickness_nm, _p(out))
+    return out
+
+
+def artist_ior(face, edge):
+    """artist_friendly_complex_ior (R/bsdf/conductor_complex.rs:132-147) -> (n, k)."""
+    f = np.asarray(face, np.float32).reshape(3)
+    e = np.asarray(edge, np.float32).reshape(3)
+    out = np.zeros(6, np.float32)
+    lib().orc_artist_ior(_p(f), _p(e), _p(out))
+    return out[:3], out[3:]

@@ -48,6 +48,18 @@ MATERIALS = [
             specular_roughness=0.45, thin_walled=True),
     OpenPbr(base_weight=0.3, transmission_weight=1.0, transmission_color=(0.9, 0.5, 0.5), specular_roughness=0.0,
             thin_walled=True),
+    # thin films: on a dielectric, on a coated metal, on a smooth specular
+    # (delta lobe), on rough glass, on a thin wall
+    OpenPbr(base_color=(0.4, 0.5, 0.6), specular_roughness=0.3, thin_film_weight=1.0, thin_film_thickness=0.4,
+            thin_film_ior=1.5),
+    OpenPbr(base_metalness=1.0, base_color=(0.9, 0.6, 0.3), specular_roughness=0.25, thin_film_weight=0.7,
+            thin_film_thickness=0.3, thin_film_ior=2.0, coat_weight=0.5, coat_roughness=0.2),
+    OpenPbr(base_color=(0.2, 0.2, 0.2), specular_roughness=0.0, thin_film_weight=1.0, thin_film_thickness=0.25,
+            thin_film_ior=1.8),
+    OpenPbr(base_weight=0.0, transmission_weight=1.0, specular_roughness=0.3, thin_film_weight=0.8,
+            thin_film_thickness=0.5, thin_film_ior=1.3),
+    OpenPbr(base_weight=0.2, transmission_weight=0.8, specular_roughness=0.4, thin_walled=True, thin_film_weight=1.0,
+            thin_film_thickness=0.35, thin_film_ior=1.6),
 ]
 
 
@@ -116,8 +128,7 @@ def test_bsdf_matches_oracle(dscene, mi, front):
 
 def test_unsupported_openpbr_parameters_are_rejected(device):
     cam_scene = scenes.cornell(build=False)
-    for bad, what in [(dict(thin_film_weight=0.3), "thin_film_weight"),
-                      (dict(transmission_weight=1.0, transmission_dispersion_scale=0.2), "dispersion")]:
+    for bad, what in [(dict(transmission_weight=1.0, transmission_dispersion_scale=0.2), "dispersion")]:
         cs = scenes.cornell(build=False)
         cs.scene.materials[1] = OpenPbr.new((0.8, 0.1, 0.1)).with_(**bad)
         cs.scene.build_qbvh()

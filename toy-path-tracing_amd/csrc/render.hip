@@ -58,12 +58,12 @@ __device__ __forceinline__ bool closest(const SceneView& sc, F3 o, F3 d, Lane& L
 
 // direct_light_mis_contribution with the shadow ray traced by the lane
 // traversal (closest hit, then the target test).
-template <bool kTwoLevel, bool kTex>
+template <bool kTwoLevel, bool kTex, int kMode>
 __device__ __forceinline__ F3 direct_light(const SceneView& sc, const ShadeView& sv, const TptMaterial& m,
                                            const Vtx& vtx, const TreeQuery& tq, float u_root, float u_tree,
                                            float us0, float us1, Lane& L, LaneStack& st, F3* dir = nullptr) {
   NeeSample ns;
-  if (!nee_sample<kTex, true>(sv, m, vtx, tq, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
+  if (!nee_sample<kTex, kMode>(sv, m, vtx, tq, u_root, u_tree, us0, us1, &ns)) return f3(0.0f, 0.0f, 0.0f);
   if (dir) *dir = ns.dir;
   uint32_t hi, ht;
   float tt, hb0, hb1, hb2;
@@ -103,7 +103,7 @@ struct Adjoint {
 // continuation) at the sampled direction, sampling and MIS weights held
 // fixed; c = parameter * texture value, so dL/dparameter = dL/dc * texture
 // value and dL/dtexel = dL/dc * parameter * d(lookup)/d(texel).
-template <bool kTwoLevel, bool kAdj, bool kTex, typename Acc, typename TexAcc = float>
+template <bool kTwoLevel, bool kAdj, bool kTex, int kMode, typename Acc, typename TexAcc = float>
 __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& sv, const RayDiff& camera_ray,
                                          const smp::PathSampler& ps, uint32_t max_depth, Lane& L, LaneStack& st,
                                          bool& overflow, const Adjoint<Acc, TexAcc>& adj) {
@@ -138,7 +138,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     if (m.kind != TPT_MATERIAL_EMISSIVE) {  // should_sample_direct_light: not a pure emitter
       float us[2];
       ps.s2(base + 3, us);
-      const F3 dl = direct_light<kTwoLevel, kTex>(sc, sv, m, vtx, tq, ps.s1(base + 0), ps.s1(base + 1), us[0], us[1],
+      const F3 dl = direct_light<kTwoLevel, kTex, kMode>(sc, sv, m, vtx, tq, ps.s1(base + 0), ps.s1(base + 1), us[0], us[1],
                                                   L, st, kAdj ? &nee_dir : nullptr);
       overflow |= st.overflow;
       nee_c = hadamard(thr, dl);
@@ -148,7 +148,7 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
     F3 wi;
     float ud[2];
     ps.s2(base + 7, ud);
-    const bool sampled = mat_sample<true>(sv.tab, m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi);
+    const bool sampled = mat_sample<kMode>(sv.tab, m, vtx, ps.s1(base + 5), ud[0], ud[1], &s, &wi);
     const bool lin = kAdj && base_color_linear(m);
     if (kAdj && (lin || (adj.rows == kParamRows && m.kind == TPT_MATERIAL_OPEN_PBR))) {
       const F3 rest = sub(adj.total, radiance);  // collected after this vertex's NEE
@@ -256,7 +256,9 @@ __device__ __forceinline__ RayDiff camera_ray(const CamView& cam, const smp::Pat
 // 90-250 registers per lane.
 constexpr int kPathWaves = 2;
 
-template <bool kTwoLevel, bool kTex>
+// kFilm: some material has a thin film (the film-free instantiation leaves
+// the film's interference code out)
+template <bool kTwoLevel, bool kTex, bool kFilm>
 __global__ __launch_bounds__(kBlock, kPathWaves) void render_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
                                                         uint32_t h, uint32_t spp, uint32_t max_depth,
                                                         float* __restrict__ rgb, uint32_t* __restrict__ status) {
@@ -279,7 +281,8 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void render_kernel(SceneView sc
   for (uint32_t si = 0; si < spp; ++si) {
     const smp::PathSampler ps{zf, x, y, si};
     const RayDiff ray = camera_ray<kTex>(cam, ps, x, y, w, h, spp);
-    const F3 radiance = trace_path<kTwoLevel, false, kTex>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
+    constexpr int kMode = kShadeOutline | (kFilm ? 0 : kShadeNoFilm);
+    const F3 radiance = trace_path<kTwoLevel, false, kTex, kMode>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
     const float cnt = (float)(si + 1);
     color = add(color, divs(sub(radiance, color), cnt));
   }
@@ -330,8 +333,8 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_kernel(SceneView s
     for (uint32_t si = 0; si < n_samples; ++si) {
       const smp::PathSampler ps{zf, x, y, si};
       const RayDiff ray = camera_ray<kTex>(cam, ps, x, y, w, h, spp);
-      adj.total = trace_path<kTwoLevel, false, kTex>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
-      (void)trace_path<kTwoLevel, true, kTex>(sc, sv, ray, ps, max_depth, L, st, overflow, adj);
+      adj.total = trace_path<kTwoLevel, false, kTex, kShadeOutline>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
+      (void)trace_path<kTwoLevel, true, kTex, kShadeOutline>(sc, sv, ray, ps, max_depth, L, st, overflow, adj);
     }
     if (overflow) atomicOr(status, 1u);
   }
@@ -442,8 +445,8 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_batch_kernel(Scene
     if (replay && si < fr.spp) {
       const smp::PathSampler ps{fr.zf, x, y, si};
       const RayDiff ray = camera_ray<kTex>(fr.cam, ps, x, y, fr.w, fr.h, fr.spp);
-      adj.total = trace_path<kTwoLevel, false, kTex>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, none);
-      (void)trace_path<kTwoLevel, true, kTex>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, adj);
+      adj.total = trace_path<kTwoLevel, false, kTex, kShadeOutline>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, none);
+      (void)trace_path<kTwoLevel, true, kTex, kShadeOutline>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, adj);
     }
     if (!kFp64) {  // fold the sample's epoch into the compensated sums
       __syncthreads();
@@ -501,8 +504,8 @@ using namespace tpt;
 
 namespace {
 
-// The materials the path tracer implements: OpenPBR without thin film or
-// dispersion, NormalizedLambert, Emissive and Mirror.
+// The materials the path tracer implements: OpenPBR without dispersion,
+// NormalizedLambert, Emissive and Mirror.
 bool supported(const TptMaterial& mat, std::string* why) {
   if (mat.kind == TPT_MATERIAL_NORMALIZED_LAMBERT || mat.kind == TPT_MATERIAL_EMISSIVE ||
       mat.kind == TPT_MATERIAL_MIRROR)
@@ -512,10 +515,11 @@ bool supported(const TptMaterial& mat, std::string* why) {
     return false;
   }
   const TptOpenPbr& m = mat.u.open_pbr;
-  if (m.thin_film_weight != 0.0f) *why = "thin_film_weight must be 0";
-  else if (m.transmission_dispersion_scale > 0.0f) *why = "transmission_dispersion_scale must be 0";
-  else return true;
-  return false;
+  if (m.transmission_dispersion_scale > 0.0f) {
+    *why = "transmission_dispersion_scale must be 0";
+    return false;
+  }
+  return true;
 }
 
 // The inputs of a material's GGX tables (see upload_ggx_tables).
@@ -589,7 +593,7 @@ __global__ __launch_bounds__(256) void bsdf_query_kernel(pt::Tables tb, const Tp
 }  // namespace
 
 namespace tpt {
-bool render_materials_ok(TptScene* s, const char* fn) {
+bool render_materials_ok(TptScene* s, const char* fn, bool* film) {
   std::vector<TptMaterial> mats(s->material_count);
   std::vector<GpuShade> shade(s->instance_count);
   if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return false;
@@ -601,6 +605,12 @@ bool render_materials_ok(TptScene* s, const char* fn) {
       !hip_ok(hipMemcpy(shade.data(), s->shade, shade.size() * sizeof(GpuShade), hipMemcpyDeviceToHost),
               "hipMemcpy(instances)"))
     return false;
+  if (film) {  // thin_film_thickness_nm_at is non-zero (NaN counts: the film path handles every value)
+    *film = false;
+    for (const TptMaterial& m : mats)
+      *film |= m.kind == TPT_MATERIAL_OPEN_PBR && !(m.u.open_pbr.thin_film_weight <= 0.0f) &&
+               !(m.u.open_pbr.thin_film_thickness <= 0.0f);
+  }
   for (uint32_t i = 0; i < s->instance_count; ++i) {
     if (shade[i].material >= s->material_count) {
       set_error(std::string(fn) + ": instance " + std::to_string(i) + " has no material");
@@ -628,7 +638,8 @@ extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb
       set_error("tpt_render: frame too large or samples_per_pixel == 0");
       return TPT_ERROR_BAD_ARGUMENT;
     }
-    if (!render_materials_ok(s, "tpt_render")) return TPT_ERROR_BAD_ARGUMENT;
+    bool film = true;
+    if (!render_materials_ok(s, "tpt_render", &film)) return TPT_ERROR_BAD_ARGUMENT;
     TptDevice* d = s->dev;
     pt::CamView cam;
     const TptCamera& c = desc->camera;
@@ -653,8 +664,10 @@ extern "C" int32_t tpt_render(TptScene* s, const TptRenderDesc* desc, float* rgb
     const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
     const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
     const bool tex = s->texture_count > 0;
-    auto kernel = two ? (tex ? pt::render_kernel<true, true> : pt::render_kernel<true, false>)
-                      : (tex ? pt::render_kernel<false, true> : pt::render_kernel<false, false>);
+    auto kernel = film ? (two ? (tex ? pt::render_kernel<true, true, true> : pt::render_kernel<true, false, true>)
+                              : (tex ? pt::render_kernel<false, true, true> : pt::render_kernel<false, false, true>))
+                       : (two ? (tex ? pt::render_kernel<true, true, false> : pt::render_kernel<true, false, false>)
+                              : (tex ? pt::render_kernel<false, true, false> : pt::render_kernel<false, false, false>));
     hipLaunchKernelGGL(kernel, grid, block, 0, d->stream, v, sv, cam, w, h, desc->samples_per_pixel, desc->max_depth,
                        drgb, d->status);
     if (!hip_ok(hipGetLastError(), "render_kernel launch")) return TPT_ERROR_DEVICE;

@@ -244,9 +244,11 @@ __device__ __forceinline__ F3 sample_eon(F3 wo, float r, float u1, float u2, flo
 
 // ---- OpenPBR (R/material/open_pbr.rs, R/bsdf/open_pbr.rs) ----------------------------------
 // Carried: base (EON diffuse), F82 metal, dielectric specular and coat (GGX
-// with multiple-scattering compensation), fuzz (LTC); transmission, thin
-// film, thin walls, subsurface, textures and normal maps are rejected on the
-// host.  The GGX tables (ggx_tables.h) are read through Tables.
+// with multiple-scattering compensation), fuzz (LTC), the specular BTDF,
+// thin walls, thin film and the material's textures; dispersion is rejected
+// on the host, subsurface and the interior medium are off in the reference,
+// normal maps are not carried.  The GGX tables (ggx_tables.h) are read
+// through Tables.
 struct Tables {
   const float* albedo;     // directional-albedo tables, ggx::kAlbedoLen each; table 0 is all zero
   const uint32_t* mat;     // per material: spec table, coat table
@@ -402,7 +404,11 @@ struct TwoLobe {
 struct Probs {
   float coat, metal, spec, btdf, fuzz, diff, total;
 };
-struct Bsdf {
+// kF: the thin-film lobe is compiled in (BsdfT<false>: materials known to
+// have no film; film_active is then a compile-time false, and the film's
+// interference code stays out of the kernels that shade such scenes)
+template <bool kF>
+struct BsdfT {
   F3 base_color, spec_color, coat_color, coat_dark, fuzz_color;
   float base, specular, ax, ay, spec_eta, trans_eta, metal;
   float coat, cax, cay, coat_eta;
@@ -436,7 +442,10 @@ struct Bsdf {
   const float* coat_tab;  // of coat_eta
   const float* cond;
   const float* diel;
+  // thin film: weight, thickness in nm (0: none), film IOR
+  float film_w, film_t, film_ior;
 };
+using Bsdf = BsdfT<true>;
 
 __device__ __forceinline__ float modulated_eta(float eta, float w) {  // R/material/open_pbr.rs:871-878
   if (fabsf(w - 1.0f) < 1.0e-6f) return eta;
@@ -490,9 +499,160 @@ __device__ __forceinline__ F3 coat_darkening(const TptOpenPbr& m, F3 base_color,
   return clamp01(lerp3(f3s(1.0f), f3(omk / den.x, omk / den.y, omk / den.z), cw * delta));
 }
 // OpenPbrMaterial::make_bsdf (:393-487) for material `mi`: the parameters
-__device__ __forceinline__ Bsdf make_bsdf_params(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front,
+// ---- thin film (R/bsdf/thin_film.rs; conductor_complex.rs:132-147) --------------------
+// artist_friendly_complex_ior: Gulbrandsen's n, k from the face and edge colours
+__device__ __forceinline__ void artist_complex_ior(F3 face, F3 edge, F3* n_out, F3* k_out) {
+  const float c[3] = {face.x, face.y, face.z}, e[3] = {edge.x, edge.y, edge.z};
+  float n[3], k[3];
+  for (int i = 0; i < 3; ++i) {
+    const float r = clampf(c[i], 0.0f, 0.999f), g = clampf(e[i], 0.0f, 1.0f);
+    const float rs = sqrtf(fmaxf(r, 0.0f));
+    const float n_max = (1.0f + rs) / fmaxf(1.0f - rs, 1.0e-6f);
+    const float n_min = (1.0f - r) / (1.0f + r);
+    n[i] = g * n_min + (1.0f - g) * n_max;
+    const float np1 = n[i] + 1.0f, nm1 = n[i] - 1.0f;
+    const float k2 = fmaxf((np1 * np1 * r - nm1 * nm1) / fmaxf(1.0f - r, 1.0e-6f), 0.0f);
+    k[i] = sqrtf(fmaxf(k2, 0.0f));
+  }
+  *n_out = f3(n[0], n[1], n[2]);
+  *k_out = f3(k[0], k[1], k[2]);
+}
+// fresnel_dielectric_polarized (thin_film.rs:143-156): amplitudes and phases
+__device__ __forceinline__ void film_fresnel_dielectric(float cos_i, float eta_i, float eta_t, float r[2],
+                                                        float phi[2]) {
+  cos_i = clampf(cos_i, 0.0f, 1.0f);
+  const float sin_i_sq = fmaxf(1.0f - cos_i * cos_i, 0.0f);
+  const float ratio = eta_i / eta_t;
+  const float sin_t_sq = ratio * ratio * sin_i_sq;
+  if (sin_t_sq >= 1.0f) {
+    r[0] = r[1] = 1.0f;
+    phi[0] = phi[1] = 0.0f;
+    return;
+  }
+  const float cos_t = sqrtf(fmaxf(1.0f - sin_t_sq, 0.0f));
+  const float rs = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
+  const float rp = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
+  phi[0] = rs < 0.0f ? kPi : 0.0f;
+  phi[1] = rp < 0.0f ? kPi : 0.0f;
+  r[0] = fabsf(rs);
+  r[1] = fabsf(rp);
+}
+// fresnel_complex_polarized (thin_film.rs:193-244)
+__device__ __forceinline__ void film_fresnel_complex(float cos_i, float eta_i, float eta_t, float kappa_t, float* rs_out,
+                                                     float* rp_out, float* phs_out, float* php_out) {
+  cos_i = clampf(cos_i, 0.0f, 1.0f);
+  const float cos_i2 = cos_i * cos_i;
+  const float sin_i2 = fmaxf(1.0f - cos_i2, 0.0f);
+  const float eta_r = eta_t / fmaxf(eta_i, 1.0e-6f), kappa_r = kappa_t / fmaxf(eta_i, 1.0e-6f);
+  const float eta_r2 = eta_r * eta_r, kappa_r2 = kappa_r * kappa_r;
+  const float inner = eta_r2 - kappa_r2 - sin_i2;
+  const float a2_plus_b2 = sqrtf(fmaxf(inner * inner + 4.0f * eta_r2 * kappa_r2, 0.0f));
+  const float a2 = fmaxf(0.5f * (a2_plus_b2 + inner), 0.0f);
+  const float b2 = fmaxf(0.5f * (a2_plus_b2 - inner), 0.0f);
+  const float a = sqrtf(a2);
+  const float t1 = a2_plus_b2 + cos_i2, t2 = 2.0f * a * cos_i;
+  const float r_s2 = (t1 - t2) / fmaxf(t1 + t2, 1.0e-12f);
+  const float t3 = a2_plus_b2 * cos_i2 + sin_i2 * sin_i2, t4 = t2 * sin_i2;
+  const float r_p2 = r_s2 * (t3 - t4) / fmaxf(t3 + t4, 1.0e-12f);
+  *rs_out = sqrtf(fmaxf(r_s2, 0.0f));
+  *rp_out = sqrtf(fmaxf(r_p2, 0.0f));
+  *phs_out = atan2f(2.0f * sqrtf(b2) * cos_i, cos_i2 - a2 - b2);
+  if (r_p2 > 0.0f) {
+    const float p_num = 2.0f * cos_i * (sqrtf(b2) * (eta_r2 + kappa_r2) - 2.0f * a * eta_r * kappa_r);
+    const float p_den = (eta_r2 + kappa_r2) * (eta_r2 + kappa_r2) * cos_i2 - (a2 + b2);
+    *php_out = atan2f(p_num, p_den);
+  } else {
+    *php_out = 0.0f;
+  }
+}
+// eval_sensitivity (thin_film.rs:111-141): the XYZ matching functions'
+// Gaussian-fit Fourier transform at the optical path difference, phase
+// shifted per channel, in linear Rec.709
+__device__ __forceinline__ F3 film_sensitivity(float opd, F3 shift) {
+  const float phase = 2.0f * kPi * opd * 1.0e-9f;
+  const float val[3] = {5.4856e-13f, 4.4201e-13f, 5.2481e-13f};
+  const float pos[3] = {1.6810e+06f, 1.7953e+06f, 2.2084e+06f};
+  const float var[3] = {4.3278e+09f, 9.3046e+09f, 6.6121e+09f};
+  const float sh[3] = {shift.x, shift.y, shift.z};
+  float xyz[3];
+  for (int i = 0; i < 3; ++i) {
+    const float amp = sqrtf(fmaxf(var[i] * (2.0f * kPi), 0.0f));
+    xyz[i] = val[i] * amp * cosf(pos[i] * phase + sh[i]) * expf(-phase * phase * var[i]);
+  }
+  const float extra_var = 4.5282e+09f;
+  xyz[0] += 9.7470e-14f * sqrtf(2.0f * kPi * extra_var) * cosf(2.2399e+06f * phase + sh[0]) *
+            expf(-extra_var * phase * phase);
+  for (int i = 0; i < 3; ++i) xyz[i] /= 1.0685e-7f;
+  const float m[3][3] = {{3.2404542f, -0.969266f, 0.0556434f},
+                         {-1.5371385f, 1.8750108f, -0.2040259f},
+                         {-0.4985314f, 0.041556f, 1.0572252f}};  // XYZ_TO_REC709 columns
+  return f3(m[0][0] * xyz[0] + m[1][0] * xyz[1] + m[2][0] * xyz[2],
+            m[0][1] * xyz[0] + m[1][1] * xyz[1] + m[2][1] * xyz[2],
+            m[0][2] * xyz[0] + m[1][2] * xyz[1] + m[2][2] * xyz[2]);
+}
+// eval_thin_film_inner (thin_film.rs:47-109): the film's reflectance from
+// the outside medium over a base of complex IOR (per channel), thickness in
+// nm: incoherent two-interface sum plus two interference orders per
+// polarisation, averaged and clamped.  One out-of-line copy serves every
+// caller, and the polarisation and order loops stay rolled: inlined at each
+// Fresnel site the film's 16 cosines, 16 exponentials and 6 arctangents made
+// the BSDF functions three times larger than the instruction cache holds.
+__device__ __noinline__ F3 thin_film(float cos_theta_i, float eta_outside, float eta_film, F3 n_base, F3 k_base,
+                                     float thickness_nm) {
+  const float cos1 = clampf(cos_theta_i, 0.0f, 1.0f);
+  eta_outside = fmaxf(eta_outside, 1.0e-3f);
+  eta_film = fmaxf(eta_film, 1.0e-3f);
+  const float eta_ratio = eta_outside / eta_film;
+  const float sin1_sq = fmaxf(1.0f - cos1 * cos1, 0.0f);
+  const float sin2_sq = eta_ratio * eta_ratio * sin1_sq;
+  if (sin2_sq >= 1.0f) return f3s(1.0f);
+  const float cos2 = sqrtf(fmaxf(1.0f - sin2_sq, 0.0f));
+  float r12[2], phi12[2];
+  film_fresnel_dielectric(cos1, eta_outside, eta_film, r12, phi12);
+  const float nb[3] = {n_base.x, n_base.y, n_base.z}, kb[3] = {k_base.x, k_base.y, k_base.z};
+  float r23_s[3], r23_p[3], phi23_s[3], phi23_p[3];
+  for (int c = 0; c < 3; ++c) {
+    float rs, rp, ps, pp;
+    film_fresnel_complex(cos2, eta_film, nb[c], kb[c], &rs, &rp, &ps, &pp);
+    r23_s[c] = rs * rs;
+    r23_p[c] = rp * rp;
+    phi23_s[c] = ps;
+    phi23_p[c] = pp;
+  }
+  const float opd = 2.0f * eta_film * thickness_nm * cos2;
+  float res[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+  for (int pol = 0; pol < 2; ++pol) {
+    const float r12_pol = pol ? r12[1] : r12[0];
+    const float r12_sq = r12_pol * r12_pol;
+    const float t121_sq = fmaxf(1.0f - r12_sq, 0.0f);
+    const float phi21 = kPi - (pol ? phi12[1] : phi12[0]);
+    float r23_sq[3], phi2[3], i_c[3], cm[3], r_geo[3];
+    for (int c = 0; c < 3; ++c) {
+      r23_sq[c] = pol ? r23_p[c] : r23_s[c];
+      phi2[c] = phi21 + (pol ? phi23_p[c] : phi23_s[c]);
+      r_geo[c] = sqrtf(fmaxf(r12_sq * r23_sq[c], 0.0f));
+      i_c[c] = r12_sq + t121_sq * r23_sq[c] / fmaxf(1.0f - r12_sq * r23_sq[c], 1.0e-6f);
+      cm[c] = t121_sq * r23_sq[c];
+    }
+#pragma unroll 1
+    for (int m = 1; m <= 2; ++m) {
+      const F3 sens = film_sensitivity((float)m * opd, muls(f3(phi2[0], phi2[1], phi2[2]), (float)m));
+      const float sv[3] = {sens.x, sens.y, sens.z};
+      for (int c = 0; c < 3; ++c) {
+        i_c[c] += 2.0f * cm[c] * sv[c];
+        cm[c] *= r_geo[c];
+      }
+    }
+    for (int c = 0; c < 3; ++c) res[c] += i_c[c];
+  }
+  return f3(clampf(0.5f * res[0], 0.0f, 1.0f), clampf(0.5f * res[1], 0.0f, 1.0f), clampf(0.5f * res[2], 0.0f, 1.0f));
+}
+
+template <bool kFilm = true>
+__device__ __forceinline__ BsdfT<kFilm> make_bsdf_params(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front,
                                                  F3 wol) {
-  Bsdf p;
+  BsdfT<kFilm> p;
   p.base_color = hadamard(f3(m.base_color[0], m.base_color[1], m.base_color[2]), f3s(1.0f));
   p.metal = clampf(m.base_metalness * 1.0f, 0.0f, 1.0f);
   const float fw = clampf(m.fuzz_weight * 1.0f, 0.0f, 1.0f), fr = clampf(m.fuzz_roughness * 1.0f, 0.0f, 1.0f);
@@ -518,6 +678,12 @@ __device__ __forceinline__ Bsdf make_bsdf_params(const Tables& tb, uint32_t mi, 
   p.diff_rough = clampf(m.base_diffuse_roughness, 0.0f, 1.0f);
   p.trans = clampf(m.transmission_weight, 0.0f, 1.0f);
   p.trans_color = clamp01(f3(m.transmission_color[0], m.transmission_color[1], m.transmission_color[2]));
+  // thin_film_thickness_nm_at (R/material/open_pbr.rs:792-813), untextured
+  // (kFilm false: a caller that knows no material has a film makes the
+  // fields constants, and every film branch folds away)
+  p.film_w = kFilm ? clampf(m.thin_film_weight, 0.0f, 1.0f) : 0.0f;
+  p.film_t = (!kFilm || m.thin_film_weight <= 0.0f) ? 0.0f : fmaxf(m.thin_film_thickness, 0.0f) * 1000.0f;
+  p.film_ior = kFilm ? fmaxf(m.thin_film_ior, 1.0e-4f) : 1.0f;
   p.thin = m.thin_walled != 0u;
   p.front = front;
   p.spec_tab = tb.albedo + (size_t)tb.mat[2 * mi] * kAlbLen;
@@ -539,14 +705,19 @@ __device__ __forceinline__ float refracted_cos(float mu, float eta) {  // :2174-
   return sqrtf(fmaxf(1.0f - s2, 0.0f));
 }
 __device__ __forceinline__ float rough_proxy(float ax, float ay) { return clampf(sqrtf(fmaxf(ax, ay)), 0.0f, 1.0f); }
-__device__ __forceinline__ bool coat_smooth(const Bsdf& p) { return p.coat <= 0.0f || fmaxf(p.cax, p.cay) < kSmoothAlpha; }
-__device__ __forceinline__ bool metal_smooth(const Bsdf& p) { return p.metal <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
-__device__ __forceinline__ bool spec_smooth(const Bsdf& p) { return p.specular <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
-__device__ __forceinline__ bool trans_smooth(const Bsdf& p) { return p.trans <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
+template <bool kF>
+__device__ __forceinline__ bool coat_smooth(const BsdfT<kF>& p) { return p.coat <= 0.0f || fmaxf(p.cax, p.cay) < kSmoothAlpha; }
+template <bool kF>
+__device__ __forceinline__ bool metal_smooth(const BsdfT<kF>& p) { return p.metal <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
+template <bool kF>
+__device__ __forceinline__ bool spec_smooth(const BsdfT<kF>& p) { return p.specular <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
+template <bool kF>
+__device__ __forceinline__ bool trans_smooth(const BsdfT<kF>& p) { return p.trans <= 0.0f || fmaxf(p.ax, p.ay) < kSmoothAlpha; }
 
 // dielectric_ms_params (:593-625); the transmission terms (eta_rel, ratio_r,
 // ome_t) only with `trans`
-__device__ __forceinline__ MsParams ms_params(const Bsdf& p, float ax, float ay, float eta, bool trans = false) {
+template <bool kF>
+__device__ __forceinline__ MsParams ms_params(const BsdfT<kF>& p, float ax, float ay, float eta, bool trans = false) {
   MsParams s;
   s.eta_o = p.front ? fmaxf(eta, 1.0e-4f) : 1.0f / fmaxf(eta, 1.0e-4f);
   s.req = powf(ax * ay, 0.25f);
@@ -581,7 +752,8 @@ __device__ __forceinline__ TwoLobe trans_lobes(const MsParams& s) {  // dielectr
   const float ms = fmaxf(1.0f - s.e_avg_o, 0.0f) * clampf(1.0f - s.ratio_r, 0.0f, 1.0f);
   return TwoLobe{ss, ms, fmaxf(ss + ms, 1.0e-8f)};
 }
-__device__ __forceinline__ TwoLobe cond_lobes(const Bsdf& p) {  // :579-591
+template <bool kF>
+__device__ __forceinline__ TwoLobe cond_lobes(const BsdfT<kF>& p) {  // :579-591
   const float e = clampf(cond_eavg(p.cond, powf(p.ax * p.ay, 0.25f)), 0.0f, 1.0f);
   const float ms = fmaxf(1.0f - e, 0.0f);
   return TwoLobe{e, ms, fmaxf(e + ms, 1.0e-8f)};
@@ -591,7 +763,8 @@ __device__ __forceinline__ float albedo_ms(F3 wo, float ax, float ay, float msf,
   if (wo.z <= 0.0f || fmaxf(ax, ay) < kSmoothAlpha) return clampf(ss, 0.0f, 1.0f);
   return clampf(ss + msf * fmaxf(1.0f - eo, 0.0f), 0.0f, 1.0f);
 }
-__device__ __forceinline__ F3 coat_tint(const Bsdf& p, F3 wo, F3 wi) {  // :483-492
+template <bool kF>
+__device__ __forceinline__ F3 coat_tint(const BsdfT<kF>& p, F3 wo, F3 wi) {  // :483-492
   if (wo.z <= 0.0f || wi.z <= 0.0f) return f3s(1.0f);
   const float mo = refracted_cos(wo.z, p.coat_eta), mi = refracted_cos(wi.z, p.coat_eta);
   return pow3(p.coat_color, 0.5f * (1.0f / fmaxf(mi, 1.0e-4f) + 1.0f / fmaxf(mo, 1.0e-4f)));
@@ -603,7 +776,8 @@ struct LayerW {
 // layer_weights (:333-390); subsurface is 0 (make_bsdf forces it,
 // R/material/open_pbr.rs:455), so the diffuse BTDF's weight is zero and is
 // left out; wo is the make_bsdf wo (its albedos come from p)
-__device__ __forceinline__ LayerW layer_weights(const Bsdf& p, F3 wo, const F3* wi) {
+template <bool kF>
+__device__ __forceinline__ LayerW layer_weights(const BsdfT<kF>& p, F3 wo, const F3* wi) {
   const float bfs = fmaxf(1.0f - p.fuzz * p.e_fuzz, 0.0f);
   F3 under = f3s(bfs);
   if (p.coat > 0.0f) {
@@ -625,7 +799,8 @@ __device__ __forceinline__ LayerW layer_weights(const Bsdf& p, F3 wo, const F3* 
   w.fuzz = muls(p.fuzz_color, p.fuzz);
   return w;
 }
-__device__ __forceinline__ Probs lobe_probs(const Bsdf& p, F3 wo) {  // :392-422
+template <bool kF>
+__device__ __forceinline__ Probs lobe_probs(const BsdfT<kF>& p, F3 wo) {  // :392-422
   const LayerW w = layer_weights(p, wo, nullptr);
   Probs r;
   r.coat = p.coat > 0.0f ? w.coat_amp : 0.0f;
@@ -639,12 +814,14 @@ __device__ __forceinline__ Probs lobe_probs(const Bsdf& p, F3 wo) {  // :392-422
   return r;
 }
 // refl_ms_eval (:648-670) with the lobe's cached parameters, factor and table value at wo
-__device__ __forceinline__ float refl_ms_eval(const Bsdf& p, F3 wo, F3 wi, const MsParams& s, float msf, float eo) {
+template <bool kF>
+__device__ __forceinline__ float refl_ms_eval(const BsdfT<kF>& p, F3 wo, F3 wi, const MsParams& s, float msf, float eo) {
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
   const float ei = diel_e(p.diel, wi.z, s.req, s.eta_o);
   return msf * (1.0f - eo) * (1.0f - ei) / (kPi * s.ome_o);
 }
-__device__ __forceinline__ float coat_eval(const Bsdf& p, F3 wo, F3 wi) {  // :698-722
+template <bool kF>
+__device__ __forceinline__ float coat_eval(const BsdfT<kF>& p, F3 wo, F3 wi) {  // :698-722
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
   F3 wm;
   if (!half_vector(wo, wi, &wm)) return 0.0f;
@@ -663,7 +840,19 @@ __device__ __forceinline__ float two_lobe_pdf(TwoLobe w, F3 wo, F3 wi, float ax,
   if (den <= 0.0f) return early_ms ? w.ms * pms / w.total : (w.ss * 0.0f + w.ms * pms) / w.total;
   return (w.ss * (pw / den) + w.ms * pms) / w.total;
 }
-__device__ __forceinline__ F3 metal_fresnel(const Bsdf& p, float c) {  // :792-802
+template <bool kF>
+__device__ __forceinline__ bool film_active(const BsdfT<kF>& p) { return kF && p.film_w > 0.0f && p.film_t > 0.0f; }
+// the film from the outside medium or, with a coat, blended with the film
+// under the coat (the common form of metal_fresnel :761-790,
+// spec_brdf_fresnel :869-900, spec_btdf_fresnel_with_eta :1120-1152)
+template <bool kF>
+__device__ __forceinline__ F3 film_coated(const BsdfT<kF>& p, float c, F3 n_base, F3 k_base) {
+  const F3 uncoated = thin_film(c, 1.0f, p.film_ior, n_base, k_base, p.film_t);
+  if (p.coat <= 0.0f) return uncoated;
+  return lerp3(uncoated, thin_film(c, p.coat_eta, p.film_ior, n_base, k_base, p.film_t), clampf(p.coat, 0.0f, 1.0f));
+}
+template <bool kF>
+__device__ __forceinline__ F3 metal_f82_fresnel(const BsdfT<kF>& p, float c) {  // metal_f82_tint_fresnel :792-802
   const F3 f0 = vmaxs(muls(p.base_color, p.base), 0.0f);
   const F3 f82 = schlick3(f0, kCos82);
   const F3 ft = hadamard(p.spec_color, f82);
@@ -671,13 +860,45 @@ __device__ __forceinline__ F3 metal_fresnel(const Bsdf& p, float c) {  // :792-8
   const float corr = cc * pow6i(1.0f - cc) / fmaxf(kF82Den, 1.0e-8f);
   return clamp01(muls(sub(schlick3(f0, c), muls(sub(f82, ft), corr)), p.specular));
 }
-__device__ __forceinline__ F3 metal_fresnel_avg(const Bsdf& p) {  // :832-839
+// metal_fresnel (:761-790): F82-tint, blended with the film over the metal,
+// whose complex IOR n, k is artist_friendly_complex_ior of base_color and
+// specular_color (computed where a film needs it, not carried in Bsdf)
+template <bool kF>
+__device__ __forceinline__ F3 metal_fresnel_film(const BsdfT<kF>& p, float c, F3 n, F3 k) {
+  return lerp3(metal_f82_fresnel(p, c), film_coated(p, c, n, k), p.film_w);
+}
+template <bool kF>
+__device__ __forceinline__ F3 metal_fresnel(const BsdfT<kF>& p, float c) {
+  if (!film_active(p)) return metal_f82_fresnel(p, c);
+  F3 n, k;
+  artist_complex_ior(clamp01(p.base_color), p.spec_color, &n, &k);
+  return metal_fresnel_film(p, c, n, k);
+}
+template <bool kF>
+__device__ __forceinline__ F3 metal_f82_fresnel_avg(const BsdfT<kF>& p) {  // metal_f82_tint_fresnel_avg :832-839
   const F3 f0 = vmaxs(muls(p.base_color, p.base), 0.0f);
   const F3 f82 = schlick3(f0, kCos82);
   const F3 b = divs(sub(f82, hadamard(p.spec_color, f82)), fmaxf(kF82Den, 1.0e-8f));
   return clamp01(muls(sub(add(f0, divs(sub(f3s(1.0f), f0), 21.0f)), divs(b, 126.0f)), p.specular));
 }
-__device__ __forceinline__ F3 metal_eval(const Bsdf& p, F3 wo, F3 wi) {  // :744-759, 804-816, 2137-2146
+// metal_fresnel_avg (:818-830): with a film, the cosine-weighted mean over 16 midpoints
+template <bool kF>
+__device__ __forceinline__ F3 metal_fresnel_avg_film(const BsdfT<kF>& p) {
+  F3 n, k, sum = f3s(0.0f);
+  artist_complex_ior(clamp01(p.base_color), p.spec_color, &n, &k);
+#pragma unroll 1
+  for (int i = 0; i < 16; ++i) {
+    const float mu = ((float)i + 0.5f) / 16.0f;
+    sum = add(sum, muls(metal_fresnel_film(p, mu, n, k), 2.0f * mu));
+  }
+  return divs(sum, 16.0f);
+}
+template <bool kF>
+__device__ __forceinline__ F3 metal_fresnel_avg(const BsdfT<kF>& p) {
+  return film_active(p) ? metal_fresnel_avg_film(p) : metal_f82_fresnel_avg(p);
+}
+template <bool kF>
+__device__ __forceinline__ F3 metal_eval(const BsdfT<kF>& p, F3 wo, F3 wi) {  // :744-759, 804-816, 2137-2146
   const F3 zero = f3s(0.0f);
   if (wo.z <= 0.0f || wi.z <= 0.0f) return zero;
   F3 wm;
@@ -690,7 +911,8 @@ __device__ __forceinline__ F3 metal_eval(const Bsdf& p, F3 wo, F3 wi) {  // :744
 }
 // the conductor's per-vertex terms of metal_eval: equivalent roughness, table
 // values at wo, 1 - energy average and multiple-scattering colour
-__device__ __forceinline__ void metal_constants(Bsdf& p, F3 wo) {
+template <bool kF>
+__device__ __forceinline__ void metal_constants(BsdfT<kF>& p, F3 wo) {
   const float req = powf(p.ax * p.ay, 0.25f);
   p.m_req = req;
   p.m_eo = cond_e(p.cond, wo.z, req);
@@ -702,15 +924,18 @@ __device__ __forceinline__ void metal_constants(Bsdf& p, F3 wo) {
   const F3 fms0 = muls(hadamard(fa, fa), ea);
   p.m_fms = f3(fms0.x / ds.x, fms0.y / ds.y, fms0.z / ds.z);
 }
-__device__ __forceinline__ void thin_wall_coefficients(const Bsdf& p, float cos_theta_o, F3* refl, F3* trans);
-__device__ __forceinline__ void thin_wall_alpha(const Bsdf& p, float* ax, float* ay);
+template <bool kF>
+__device__ __forceinline__ void thin_wall_coefficients(const BsdfT<kF>& p, float cos_theta_o, F3* refl, F3* trans);
+template <bool kF>
+__device__ __forceinline__ void thin_wall_alpha(const BsdfT<kF>& p, float* ax, float* ay);
 // make_bsdf: the parameters plus every wo-only term that layer_weights, the
 // lobe evaluations, pdf and sample read (the same expressions, evaluated once
 // per vertex under the conditions their readers test).
+template <bool kFilm = true>
 __device__ __forceinline__ void make_bsdf(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol,
-                                          Bsdf* out) {
-  Bsdf& p = *out;
-  p = make_bsdf_params(tb, mi, m, front, wol);
+                                          BsdfT<kFilm>* out) {
+  BsdfT<kFilm>& p = *out;
+  p = make_bsdf_params<kFilm>(tb, mi, m, front, wol);
   const bool rough_s = !(fmaxf(p.ax, p.ay) < kSmoothAlpha), rough_c = !(fmaxf(p.cax, p.cay) < kSmoothAlpha);
   if (rough_s) {
     // the transmission lobe's multiple-scattering terms read the same table
@@ -753,19 +978,31 @@ __device__ __forceinline__ void make_bsdf(const Tables& tb, uint32_t mi, const T
 // take it by reference), and a caller that keeps a whole path in registers
 // (the one-thread-per-pixel kernels) keeps its register budget for the path
 // loop.  The wavefront shade kernel, whose live state is small, inlines it.
+template <bool kFilm = true>
 __device__ __noinline__ void make_bsdf_call(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol,
-                                            Bsdf* out) {
-  make_bsdf(tb, mi, m, front, wol, out);
+                                            BsdfT<kFilm>* out) {
+  make_bsdf<kFilm>(tb, mi, m, front, wol, out);
 }
-template <bool kOutline>
+// Shading modes of the path kernels (a bit set, template argument kMode):
+// kShadeOutline builds the OpenPBR Bsdf in a call (make_bsdf_call);
+// kShadeNoFilm is for scenes whose materials have no thin film (the host
+// checks the table before the launch): the Bsdf is a BsdfT<false>, so the
+// film's interference code (about 50 VGPRs at its peak) is not compiled into
+// the kernel and does not cost it occupancy.
+constexpr int kShadeOutline = 1, kShadeNoFilm = 2;
+template <int kMode>
+using ModeBsdf = BsdfT<(kMode & kShadeNoFilm) == 0>;
+template <int kMode>
 __device__ __forceinline__ void bsdf_at(const Tables& tb, uint32_t mi, const TptOpenPbr& m, bool front, F3 wol,
-                                        Bsdf* out) {
-  if constexpr (kOutline)
-    make_bsdf_call(tb, mi, m, front, wol, out);
+                                        ModeBsdf<kMode>* out) {
+  constexpr bool kFilm = (kMode & kShadeNoFilm) == 0;
+  if constexpr ((kMode & kShadeOutline) != 0)
+    make_bsdf_call<kFilm>(tb, mi, m, front, wol, out);
   else
-    make_bsdf(tb, mi, m, front, wol, out);
+    make_bsdf<kFilm>(tb, mi, m, front, wol, out);
 }
-__device__ __forceinline__ float base_fresnel(const Bsdf& p, float c) {  // base_dielectric_fresnel :902-929
+template <bool kF>
+__device__ __forceinline__ float base_fresnel(const BsdfT<kF>& p, float c) {  // base_dielectric_fresnel :902-929
   const float ep = p.trans_eta, epr = p.spec_eta;
   const float un = p.front ? weighted_fresnel_ratio(c, ep, epr) : weighted_fresnel_ratio(c, 1.0f / ep, 1.0f / epr);
   const float coat = clampf(p.coat, 0.0f, 1.0f);
@@ -778,17 +1015,62 @@ __device__ __forceinline__ float base_fresnel(const Bsdf& p, float c) {  // base
                            : weighted_fresnel_ratio(c, 1.0f / fmaxf(a, 1.0e-4f), 1.0f / fmaxf(b, 1.0e-4f));
   return lerpf(un, co, coat);
 }
+// spec_brdf_fresnel (:869-900) / spec_btdf_fresnel_with_eta (:1120-1152,
+// no film on thin walls): the base Fresnel, blended with the film over a
+// dielectric base of IOR specular_eta (out of line: filmed materials only)
+template <bool kF>
+__device__ __forceinline__ F3 dielectric_fresnel_film(const BsdfT<kF>& p, float c) {
+  return lerp3(f3s(base_fresnel(p, c)), film_coated(p, c, f3s(p.spec_eta), f3s(0.0f)), p.film_w);
+}
+template <bool kF>
+__device__ __forceinline__ F3 spec_fresnel3(const BsdfT<kF>& p, float c) {
+  return film_active(p) ? dielectric_fresnel_film(p, c) : f3s(base_fresnel(p, c));
+}
+template <bool kF>
+__device__ __forceinline__ F3 btdf_fresnel3(const BsdfT<kF>& p, float c) {
+  return (film_active(p) && !p.thin) ? dielectric_fresnel_film(p, c) : f3s(base_fresnel(p, c));
+}
 // thin_wall_coefficients (:931-959): the slab's total reflectance and
 // transmittance at an outgoing cosine, the interior absorbing
 // transmission_color^(1/cos_i) per pass, the internal reflections summed
-__device__ __forceinline__ void thin_wall_coefficients(const Bsdf& p, float cos_theta_o, F3* refl, F3* trans) {
+// with a film: the Fresnel terms per channel (out of line)
+template <bool kF>
+__device__ __forceinline__ void thin_wall_coefficients_film(const BsdfT<kF>& p, float cos_theta_o, F3* refl, F3* trans) {
   const float eta = fmaxf(p.trans_eta, 1.0e-4f), eta_prime = fmaxf(p.spec_eta, 1.0e-4f);
   const float cos_o = clampf(cos_theta_o, 0.0f, 1.0f);
   const float sin2_o = fmaxf(1.0f - cos_o * cos_o, 0.0f);
   const float sin2_i = clampf(sin2_o / (eta * eta), 0.0f, 1.0f);
   const float cos_i = fmaxf(sqrtf(1.0f - sin2_i), 1.0e-4f);
-  const float f_o = clampf(base_fresnel(p, cos_o), 0.0f, 1.0f);  // spec_brdf_fresnel without a thin film
-  // thin_wall_internal_fresnel (:961-979) without a thin film
+  const F3 f_o = clamp01(spec_fresnel3(p, cos_o));
+  // thin_wall_internal_fresnel (:961-979): the film seen from inside the
+  // slab (outside medium eta, base 1)
+  F3 f_i = f3s(weighted_fresnel_ratio(cos_i, 1.0f / fmaxf(eta, 1.0e-4f), 1.0f / fmaxf(eta_prime, 1.0e-4f)));
+  if (film_active(p))
+    f_i = lerp3(f_i, thin_film(cos_i, fmaxf(eta, 1.0e-4f), p.film_ior, f3s(1.0f), f3s(0.0f), p.film_t), p.film_w);
+  f_i = clamp01(f_i);
+  const F3 a = pow3(clamp01(p.trans_color), 1.0f / cos_i);
+  const F3 a2 = hadamard(a, a);
+  const F3 fi2a2 = hadamard(hadamard(a2, f_i), f_i);
+  const F3 den = vmaxs(sub(f3s(1.0f), fi2a2), 1.0e-6f);
+  const F3 omfo = vmaxs(sub(f3s(1.0f), f_o), 0.0f), omfi = vmaxs(sub(f3s(1.0f), f_i), 0.0f);
+  const F3 rn = hadamard(hadamard(hadamard(omfo, omfi), f_i), a2);
+  const F3 r = add(f_o, f3(rn.x / den.x, rn.y / den.y, rn.z / den.z));
+  const F3 t = hadamard(hadamard(hadamard(omfo, omfi), a), add(f3s(1.0f), f3(fi2a2.x / den.x, fi2a2.y / den.y, fi2a2.z / den.z)));
+  *refl = clamp01(r);
+  *trans = clamp01(t);
+}
+template <bool kF>
+__device__ __forceinline__ void thin_wall_coefficients(const BsdfT<kF>& p, float cos_theta_o, F3* refl, F3* trans) {
+  if (film_active(p)) {
+    thin_wall_coefficients_film(p, cos_theta_o, refl, trans);
+    return;
+  }
+  const float eta = fmaxf(p.trans_eta, 1.0e-4f), eta_prime = fmaxf(p.spec_eta, 1.0e-4f);
+  const float cos_o = clampf(cos_theta_o, 0.0f, 1.0f);
+  const float sin2_o = fmaxf(1.0f - cos_o * cos_o, 0.0f);
+  const float sin2_i = clampf(sin2_o / (eta * eta), 0.0f, 1.0f);
+  const float cos_i = fmaxf(sqrtf(1.0f - sin2_i), 1.0e-4f);
+  const float f_o = clampf(base_fresnel(p, cos_o), 0.0f, 1.0f);
   const float f_i =
       clampf(weighted_fresnel_ratio(cos_i, 1.0f / fmaxf(eta, 1.0e-4f), 1.0f / fmaxf(eta_prime, 1.0e-4f)), 0.0f, 1.0f);
   const F3 a = pow3(clamp01(p.trans_color), 1.0f / cos_i);
@@ -803,7 +1085,8 @@ __device__ __forceinline__ void thin_wall_coefficients(const Bsdf& p, float cos_
   *trans = clamp01(t);
 }
 // thin_wall_transmission_alpha_xy (:553-563)
-__device__ __forceinline__ void thin_wall_alpha(const Bsdf& p, float* ax, float* ay) {
+template <bool kF>
+__device__ __forceinline__ void thin_wall_alpha(const BsdfT<kF>& p, float* ax, float* ay) {
   const float eta = fmaxf(p.trans_eta, 1.0f);
   const float e5 = eta - 0.5f;
   const float scale = sqrtf(fmaxf(3.7f * (eta - 1.0f) * (e5 * e5) / (eta * (eta * eta)), 0.0f));
@@ -811,7 +1094,8 @@ __device__ __forceinline__ void thin_wall_alpha(const Bsdf& p, float* ax, float*
   *ax = clampf(p.ax * alpha_scale, kMinAlpha, 1.0f);
   *ay = clampf(p.ay * alpha_scale, kMinAlpha, 1.0f);
 }
-__device__ __forceinline__ F3 spec_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_spec_brdf :841-867
+template <bool kF>
+__device__ __forceinline__ F3 spec_eval(const BsdfT<kF>& p, F3 wo, F3 wi) {  // eval_spec_brdf :841-867
   const F3 zero = f3s(0.0f);
   if (wo.z <= 0.0f || wi.z <= 0.0f) return zero;
   F3 wm;
@@ -819,10 +1103,11 @@ __device__ __forceinline__ F3 spec_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_
   const float d = ggx_d(wm, p.ax, p.ay), g = ggx_g2(wo, wi, p.ax, p.ay);
   if (d <= 0.0f || g <= 0.0f) return zero;
   if (p.thin) return muls(p.thin_r, d * g / (4.0f * wo.z * wi.z));
-  const float ss = base_fresnel(p, fabsf(dot(wo, wm))) * (d * g / (4.0f * wo.z * wi.z));
-  return f3s(ss + refl_ms_eval(p, wo, wi, p.ms_s, p.msf_s, p.eo_s));
+  const F3 ss = muls(spec_fresnel3(p, fabsf(dot(wo, wm))), d * g / (4.0f * wo.z * wi.z));
+  return add(ss, f3s(refl_ms_eval(p, wo, wi, p.ms_s, p.msf_s, p.eo_s)));
 }
-__device__ __forceinline__ float spec_ss_pdf(const Bsdf& p, F3 wo, F3 wi) {  // :981-994
+template <bool kF>
+__device__ __forceinline__ float spec_ss_pdf(const BsdfT<kF>& p, F3 wo, F3 wi) {  // :981-994
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
   F3 wm;
   if (!half_vector(wo, wi, &wm)) return 0.0f;
@@ -844,7 +1129,8 @@ __device__ __forceinline__ float pdf_vndf(F3 wo, F3 wm, float ax, float ay) {
   return d * g1 * c / wo.z;
 }
 // transmission_eta_rel (:1245-1248)
-__device__ __forceinline__ float trans_eta_rel(const Bsdf& p) { return p.front ? 1.0f / p.trans_eta : p.trans_eta; }
+template <bool kF>
+__device__ __forceinline__ float trans_eta_rel(const BsdfT<kF>& p) { return p.front ? 1.0f / p.trans_eta : p.trans_eta; }
 // the generalised half vector of a refraction pair (:1081-1096, 1194-1209):
 // normalize(eta_rel wo + wi), flipped up; false where undefined
 __device__ __forceinline__ bool refraction_half_vector(F3 wo, F3 wi, float eta_rel, F3* wm, float* co, float* ci,
@@ -865,7 +1151,8 @@ __device__ __forceinline__ bool refraction_half_vector(F3 wo, F3 wi, float eta_r
   return true;
 }
 // eval_dielectric_transmission_ms (:672-696) with the cached lobe terms
-__device__ __forceinline__ float trans_ms_eval(const Bsdf& p, F3 wo, F3 wi) {
+template <bool kF>
+__device__ __forceinline__ float trans_ms_eval(const BsdfT<kF>& p, F3 wo, F3 wi) {
   if (wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
   const MsParams& s = p.ms_s;
   const float ei = diel_e(p.diel, fabsf(wi.z), s.req, s.eta_rel);
@@ -874,7 +1161,8 @@ __device__ __forceinline__ float trans_ms_eval(const Bsdf& p, F3 wo, F3 wi) {
 }
 // eval_thin_wall_spec_btdf (:1053-1072): the reflection lobe at the thin-wall
 // alphas mirrored through the surface, times the slab transmittance
-__device__ __forceinline__ F3 thin_btdf_eval(const Bsdf& p, F3 wo, F3 wi) {
+template <bool kF>
+__device__ __forceinline__ F3 thin_btdf_eval(const BsdfT<kF>& p, F3 wo, F3 wi) {
   const F3 zero = f3s(0.0f);
   if (wo.z <= 0.0f || wi.z >= 0.0f || trans_smooth(p)) return zero;
   const F3 wim = f3(wi.x, wi.y, -wi.z);
@@ -884,7 +1172,8 @@ __device__ __forceinline__ F3 thin_btdf_eval(const Bsdf& p, F3 wo, F3 wi) {
   if (d <= 0.0f || g <= 0.0f) return zero;
   return muls(p.thin_t, d * g / (4.0f * wo.z * fabsf(wi.z)));
 }
-__device__ __forceinline__ F3 btdf_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_spec_btdf :1024-1051
+template <bool kF>
+__device__ __forceinline__ F3 btdf_eval(const BsdfT<kF>& p, F3 wo, F3 wi) {  // eval_spec_btdf :1024-1051
   if (p.thin) return thin_btdf_eval(p, wo, wi);
   const F3 zero = f3s(0.0f);
   if (wo.z <= 0.0f || wi.z >= 0.0f) return zero;
@@ -900,10 +1189,11 @@ __device__ __forceinline__ F3 btdf_eval(const Bsdf& p, F3 wo, F3 wi) {  // eval_
   const float denom = dn * dn * fabsf(wo.z) * fabsf(wi.z);
   if (denom <= 0.0f) return zero;
   const float scalar = sc * rs / denom;
-  const float ss = fmaxf(1.0f - base_fresnel(p, co), 0.0f) * scalar;  // spec_btdf_fresnel_with_eta, no thin film
-  return f3s(ss + trans_ms_eval(p, wo, wi));
+  const F3 ss = muls(vmaxs(sub(f3s(1.0f), btdf_fresnel3(p, co)), 0.0f), scalar);
+  return add(ss, f3s(trans_ms_eval(p, wo, wi)));
 }
-__device__ __forceinline__ float btdf_pdf(const Bsdf& p, F3 wo, F3 wi) {  // pdf_specular_btdf :1154-1221
+template <bool kF>
+__device__ __forceinline__ float btdf_pdf(const BsdfT<kF>& p, F3 wo, F3 wi) {  // pdf_specular_btdf :1154-1221
   if (wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
   if (p.thin) {
     if (trans_smooth(p)) return 0.0f;
@@ -933,7 +1223,8 @@ __device__ __forceinline__ F3 rotate_z(F3 v, float angle) {  // :2065-2068
   return f3(c * v.x - s * v.y, s * v.x + c * v.y, v.z);
 }
 // OpenPbrFuzzBsdf eval (one channel) and pdf (:159-182), LTC :2097-2110
-__device__ __forceinline__ float fuzz_eval(const Bsdf& p, F3 wo, F3 wi, float* pdf) {
+template <bool kF>
+__device__ __forceinline__ float fuzz_eval(const BsdfT<kF>& p, F3 wo, F3 wi, float* pdf) {
   *pdf = 0.0f;
   if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
   const float r = clampf(p.fuzz_rough, 0.01f, 1.0f);
@@ -948,7 +1239,8 @@ __device__ __forceinline__ float fuzz_eval(const Bsdf& p, F3 wo, F3 wi, float* p
   *pdf = v;
   return zeltner_albedo(wo.z, r) * v / fmaxf(wi.z, 1.0e-6f);
 }
-__device__ __forceinline__ bool fuzz_sample(const Bsdf& p, F3 wo, float u0, float u1, F3* wi) {  // :184-200, 2112-2123
+template <bool kF>
+__device__ __forceinline__ bool fuzz_sample(const BsdfT<kF>& p, F3 wo, float u0, float u1, F3* wi) {  // :184-200, 2112-2123
   if (wo.z <= 0.0f) return false;
   const float r = clampf(p.fuzz_rough, 0.01f, 1.0f);
   const float a = zeltner_a(wo.z, r), b = zeltner_b(wo.z, r);
@@ -964,7 +1256,8 @@ __device__ __forceinline__ bool fuzz_sample(const Bsdf& p, F3 wo, float u0, floa
 }
 
 // OpenPbrBsdf::eval (:224-259), radiance mode
-__device__ __noinline__ F3 bsdf_eval(const Bsdf& p, F3 wo, F3 wi) {
+template <bool kF>
+__device__ __noinline__ F3 bsdf_eval(const BsdfT<kF>& p, F3 wo, F3 wi) {
   F3 total = f3s(0.0f);
   if (!(wo.z > 0.0f)) return total;
   const LayerW w = layer_weights(p, wo, &wi);
@@ -987,15 +1280,17 @@ __device__ __noinline__ F3 bsdf_eval(const Bsdf& p, F3 wo, F3 wi) {
 // of the base colour enters eval only through the diffuse lobe's weight
 // below_transmission * base * base_color (layer_weights) times the unit-albedo
 // EON lobe, so the derivative is that weight at base colour 1 times the lobe.
-__device__ __noinline__ F3 bsdf_eval_dbase(const Bsdf& p, F3 wo, F3 wi) {
+template <bool kF>
+__device__ __noinline__ F3 bsdf_eval_dbase(const BsdfT<kF>& p, F3 wo, F3 wi) {
   if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) return f3s(0.0f);
-  Bsdf q = p;
+  BsdfT<kF> q = p;
   q.base_color = f3s(1.0f);
   const LayerW w = layer_weights(q, wo, &wi);
   return muls(w.diff, f_eon1(p.diff_rough, wo, wi));
 }
 // OpenPbrBsdf::pdf (:261-293)
-__device__ __noinline__ float bsdf_pdf(const Bsdf& p, F3 wo, F3 wi) {
+template <bool kF>
+__device__ __noinline__ float bsdf_pdf(const BsdfT<kF>& p, F3 wo, F3 wi) {
   if (!(wo.z > 0.0f)) return 0.0f;
   const Probs& pr = p.pr;
   if (pr.total <= 0.0f) return 0.0f;
@@ -1041,7 +1336,8 @@ struct Sample {
   float cone;   // MaterialSample::cone_spread
   int lobe;     // OpenPBR: the lobe picked (0 coat, 1 metal, 2 specular, 3 transmission, 4 fuzz, 5 diffuse)
 };
-__device__ __forceinline__ bool finalize(const Bsdf& p, F3 wo, F3 wi, Sample* out) {  // :1958-1999
+template <bool kF>
+__device__ __forceinline__ bool finalize(const BsdfT<kF>& p, F3 wo, F3 wi, Sample* out) {  // :1958-1999
   const float pdf = bsdf_pdf(p, wo, wi);
   if (pdf <= 0.0f) return false;
   const F3 f = bsdf_eval(p, wo, wi);
@@ -1130,7 +1426,8 @@ __device__ __forceinline__ bool refract_smooth(F3 wo, float eta, F3* out) {
   *out = wi;
   return true;
 }
-__device__ __forceinline__ bool sample_two_lobe(const Bsdf& p, F3 wo, float ax, float ay, TwoLobe w, float ux, float uy,
+template <bool kF>
+__device__ __forceinline__ bool sample_two_lobe(const BsdfT<kF>& p, F3 wo, float ax, float ay, TwoLobe w, float ux, float uy,
                                                 Sample* out) {
   const float ssp = w.ss / w.total;
   if (ux >= ssp) {
@@ -1150,12 +1447,13 @@ __device__ __forceinline__ bool sample_two_lobe(const Bsdf& p, F3 wo, float ax, 
 // Smooth transmission (lobe 3, sample_spec_btdf :1670-1781): the layer weight
 // at the refracted direction (the pass-through -wo on thin walls) times
 // (1 - Fresnel) / eta_rel^2, or the slab transmittance on thin walls.
-__device__ __forceinline__ F3 delta_numerator(const Bsdf& p, F3 wo, int lobe) {
+template <bool kF>
+__device__ __forceinline__ F3 delta_numerator(const BsdfT<kF>& p, F3 wo, int lobe) {
   const F3 mirror = f3(-wo.x, -wo.y, wo.z);
   if (lobe == 0) return f3s(layer_weights(p, wo, &mirror).coat_amp * fresnel_dielectric(fabsf(mirror.z), 1.0f, p.coat_eta));
   if (lobe == 1) return hadamard(layer_weights(p, wo, &mirror).metal, metal_fresnel(p, fabsf(mirror.z)));
   if (lobe == 2)
-    return hadamard(layer_weights(p, wo, &mirror).spec, p.thin ? p.thin_r : f3s(base_fresnel(p, fabsf(mirror.z))));
+    return hadamard(layer_weights(p, wo, &mirror).spec, p.thin ? p.thin_r : spec_fresnel3(p, fabsf(mirror.z)));
   if (lobe == 3) {
     if (p.thin) {
       const F3 through = neg(wo);
@@ -1179,7 +1477,8 @@ __device__ __forceinline__ bool delta_sample(F3 weight, F3 wi, float p_lobe, Sam
 // OpenPbrBsdf::sample (:295-331), pick_lobe (:2016-2041), sample_coat /
 // sample_metal / sample_spec_brdf / sample_fuzz / sample_diff_brdf
 // (:1467-1668, 1929-1940)
-__device__ __noinline__ bool bsdf_sample(const Bsdf& p, F3 wo, float u_lobe, float ux, float uy, Sample* out) {
+template <bool kF>
+__device__ __noinline__ bool bsdf_sample(const BsdfT<kF>& p, F3 wo, float u_lobe, float ux, float uy, Sample* out) {
   if (!(wo.z > 0.0f)) return false;
   const Probs& pr = p.pr;
   if (pr.total <= 0.0f) return false;
@@ -1562,14 +1861,14 @@ __device__ __forceinline__ bool pbr_le(const TptOpenPbr& m, const Vtx& v, F3* le
 }
 
 // OpenPbrMaterial::sample (R/material/open_pbr.rs:489-548), radiance mode.
-template <bool kOutline>
+template <int kMode>
 __device__ __forceinline__ bool pbr_sample(const Tables& tb, const TptOpenPbr& m, const Vtx& v, float u_lobe, float ux,
                                            float uy, Sample* s, F3* wi_world) {
   if (dot(v.wo, v.ng) <= 0.0f) return false;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
-  Bsdf p;
-  bsdf_at<kOutline>(tb, v.material, m, v.front, wol, &p);
+  ModeBsdf<kMode> p;
+  bsdf_at<kMode>(tb, v.material, m, v.front, wol, &p);
   if (!bsdf_sample(p, wol, u_lobe, ux, uy, s)) return false;
   const F3 wi = to_world(v.frame, s->wi);
   // GEOMETRIC_NORMAL_COS_EPSILON: a reflection must leave above the geometric
@@ -1603,11 +1902,11 @@ __device__ __forceinline__ bool mat_le(const TptMaterial& m, const Vtx& v, F3* l
 }
 // NormalizedLambertMaterial::sample (R/material/normalized_lambert.rs:120-158)
 // and MirrorMaterial::sample (R/material/mirror.rs:112-143), radiance mode.
-// kOutline: build the OpenPBR Bsdf in a call (bsdf_at)
-template <bool kOutline = false>
+// kMode: the shading mode (kShadeOutline, kShadeNoFilm) of bsdf_at
+template <int kMode = 0>
 __device__ __forceinline__ bool mat_sample(const Tables& tb, const TptMaterial& m, const Vtx& v, float u_lobe, float ux,
                                            float uy, Sample* s, F3* wi_world) {
-  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_sample<kOutline>(tb, m.u.open_pbr, v, u_lobe, ux, uy, s, wi_world);
+  if (m.kind == TPT_MATERIAL_OPEN_PBR) return pbr_sample<kMode>(tb, m.u.open_pbr, v, u_lobe, ux, uy, s, wi_world);
   if (m.kind == TPT_MATERIAL_EMISSIVE) return false;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return false;
@@ -1640,7 +1939,7 @@ __device__ __forceinline__ bool mat_sample(const Tables& tb, const TptMaterial& 
 // and OpenPBR (R/material/open_pbr.rs:549-590), each with modified_bsdf_eval
 // (R/material/mod.rs:33-53); Mirror and Emissive evaluate to zero.  *pdf is
 // 0 when f is 0 (Material::eval_pdf).
-template <bool kOutline = false>
+template <int kMode = 0>
 __device__ __forceinline__ F3 mat_eval_pdf(const Tables& tb, const TptMaterial& m, const Vtx& v, F3 wi, float* pdf) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
   *pdf = 0.0f;
@@ -1658,8 +1957,8 @@ __device__ __forceinline__ F3 mat_eval_pdf(const Tables& tb, const TptMaterial& 
     const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
     const F3 wil = normalize_or_zero(to_frame(v.frame, wi));
     if (wol.z <= 0.0f) return zero;
-    Bsdf b;
-    bsdf_at<kOutline>(tb, v.material, m.u.open_pbr, v.front, wol, &b);
+    ModeBsdf<kMode> b;
+    bsdf_at<kMode>(tb, v.material, m.u.open_pbr, v.front, wol, &b);
     f = bsdf_eval(b, wol, wil);
     if (len2(f) == 0.0f) return zero;
     p = bsdf_pdf(b, wol, wil);
@@ -1717,7 +2016,8 @@ __device__ __forceinline__ float& pbr_param(TptOpenPbr& p, int k) {
 // wib came from the thin-walled rough BTDF, whose weight is that lobe's own
 // layer weight times value over its own density (sample_spec_btdf
 // :1700-1727), not the whole BSDF's: the ratio is taken on that lobe.
-__device__ __forceinline__ F3 sampled_value(const Bsdf& b, F3 wo, F3 wi, bool thin_lobe) {
+template <bool kF>
+__device__ __forceinline__ F3 sampled_value(const BsdfT<kF>& b, F3 wo, F3 wi, bool thin_lobe) {
   return thin_lobe ? hadamard(layer_weights(b, wo, &wi).btdf, thin_btdf_eval(b, wo, wi)) : bsdf_eval(b, wo, wi);
 }
 __device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMaterial& m, const Vtx& v, bool na, F3 wia,
@@ -2316,7 +2616,7 @@ struct NeeSample {
   F3 origin, dir, value;
   uint32_t inst, tri;
 };
-template <bool kTex = false, bool kOutline = false>
+template <bool kTex = false, int kMode = 0>
 __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMaterial& m, const Vtx& vtx,
                                            const TreeQuery& tq, float u_root, float u_tree, float us0, float us1,
                                            NeeSample* ns) {
@@ -2384,7 +2684,7 @@ __device__ __forceinline__ bool nee_sample(const ShadeView& sv, const TptMateria
   }
   if (li_pdf <= 0.0f || selection <= 0.0f) return false;
   float bsdf_pdf;
-  const F3 f = mat_eval_pdf<kOutline>(sv.tab, m, vtx, wi, &bsdf_pdf);
+  const F3 f = mat_eval_pdf<kMode>(sv.tab, m, vtx, wi, &bsdf_pdf);
   if (len2(f) == 0.0f) return false;
   const float cos_surface = fabsf(dot(vtx.ng, wi));
   if (cos_surface <= 0.0f) return false;

@@ -244,7 +244,8 @@ bool texel_grads_to_level0(TptScene* s, T* g, float* out, hipStream_t st);
 // Checks the materials' texture indices against the scene's texture table
 // (colour slots need RGB, scalar slots one channel); "" when valid.
 std::string material_texture_error(const TptMaterial& m, uint32_t index, const std::vector<pt::GpuTexture>& tex);
-bool render_materials_ok(TptScene* s, const char* fn);
+// film (optional): set to whether some material has a thin film
+bool render_materials_ok(TptScene* s, const char* fn, bool* film = nullptr);
 
 inline SceneView view_of(const TptScene* s) {
   SceneView v;

@@ -353,7 +353,9 @@ __global__ __launch_bounds__(256) void wf_sort_scatter(const TptHit* __restrict_
 // With kTex the ray differentials and cones follow the paths and the
 // materials' textures are applied at every vertex.  order: the queue indices
 // in material order (nullptr: queue order).
-template <bool kTex, uint32_t kTile>
+// kFilm: some material has a thin film (the film-free instantiation leaves
+// the film's interference code out)
+template <bool kTex, uint32_t kTile, bool kFilm>
 __global__ __launch_bounds__(kTile) void wf_shade(Pass ps, pt::ShadeView sv, State st, Queue qin,
                                                 const TptHit* __restrict__ hits, Queue qout, ShadowQ sq,
                                                 Counters* c, uint32_t in_idx, uint32_t bounce,
@@ -444,7 +446,8 @@ __global__ __launch_bounds__(kTile) void wf_shade(Pass ps, pt::ShadeView sv, Sta
           if (m.kind != TPT_MATERIAL_EMISSIVE) {
             float us[2];
             smpl.s2(bd + 3, us, zrows[2]);
-            if (pt::nee_sample<kTex>(sv, m, vtx, tq, smpl.s1(bd + 0, zrows[0]), smpl.s1(bd + 1, zrows[1]), us[0], us[1],
+            constexpr int kMode = kFilm ? 0 : pt::kShadeNoFilm;
+            if (pt::nee_sample<kTex, kMode>(sv, m, vtx, tq, smpl.s1(bd + 0, zrows[0]), smpl.s1(bd + 1, zrows[1]), us[0], us[1],
                                      &ns)) {
               want_shadow = true;
               nee_value = pt::hadamard(thr, ns.value);
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(kTile) void wf_shade(Pass ps, pt::ShadeView sv, Sta
           F3 wi;
           float ud[2];
           smpl.s2(bd + 7, ud, zrows[4]);
-          if (pt::mat_sample(sv.tab, m, vtx, smpl.s1(bd + 5, zrows[3]), ud[0], ud[1], &smp_, &wi)) {
+          if (pt::mat_sample<kFilm ? 0 : pt::kShadeNoFilm>(sv.tab, m, vtx, smpl.s1(bd + 5, zrows[3]), ud[0], ud[1], &smp_, &wi)) {
             thr = pt::hadamard(thr, smp_.weight);
             bool live = true;
             if (bounce + 1 >= 4) {
@@ -587,6 +590,7 @@ struct Plan {
   bool q8;
   bool exact_shadow;
   bool tex;  // the scene has textures: ray differentials and textured materials
+  bool film;  // some material has a thin film
   wf::SortBuf sort;
   uint32_t sort_bins;   // ceil(material count / group) + 1 (misses); 0: shade in queue order
   uint32_t sort_group;  // materials per sort key
@@ -599,7 +603,8 @@ template <uint32_t kTile>
 bool enqueue_pass_tile(TptScene* s, const Plan& p, const pt::ShadeView& sv, hipStream_t st) {
   TptDevice* d = s->dev;
   auto gen = p.tex ? wf::wf_generate<true, kTile> : wf::wf_generate<false, kTile>;
-  auto shade = p.tex ? wf::wf_shade<true, kTile> : wf::wf_shade<false, kTile>;
+  auto shade = p.film ? (p.tex ? wf::wf_shade<true, kTile, true> : wf::wf_shade<false, kTile, true>)
+                      : (p.tex ? wf::wf_shade<true, kTile, false> : wf::wf_shade<false, kTile, false>);
   // One thread per path slot: generate and shade take exactly one slot per
   // thread (they have no slot loop).  An earlier form looped over slots with
   // the shade grid capped at the resident capacity; there the untextured
@@ -676,7 +681,8 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
       set_error("tpt_render_batch: too many pixels for one call");
       return TPT_ERROR_BAD_ARGUMENT;
     }
-    if (!render_materials_ok(s, "tpt_render_batch")) return TPT_ERROR_BAD_ARGUMENT;
+    bool film = true;
+    if (!render_materials_ok(s, "tpt_render_batch", &film)) return TPT_ERROR_BAD_ARGUMENT;
     TptDevice* d = s->dev;
     if (!hip_ok(hipSetDevice(d->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
     const hipStream_t st = d->stream;
@@ -768,6 +774,7 @@ extern "C" int32_t tpt_render_batch(TptScene* s, const TptFrameDesc* frames, uin
     p.q8 = flags & TPT_FLAG_QUANTIZED;
     p.exact_shadow = flags & TPT_FLAG_ANY_EXACT;
     p.tex = tex;
+    p.film = film;
     p.sort_bins = sort_bins;
     p.sort_group = sort_group ? sort_group : 1u;
     p.tile = d->wf_tile;

@@ -541,6 +541,33 @@ def cornell_glass(build: bool = True, textured: bool = False, light_size: float 
     return CornellScene(s, mats, li, cam)
 
 
+def cornell_film(build: bool = True, textured: bool = False, light_size: float = 0.2,
+                 light_radiance: float = 15.0) -> CornellScene:
+    """The Cornell box with OpenPBR thin films (R/bsdf/thin_film.rs: two-beam
+    interference over the base, per polarisation): the tall box is rough
+    metal (roughness 0.3, a 0.45 um film of IOR 1.45 over a copper-like base
+    colour), the short box a smooth coated dielectric (coat 0.6, a 0.3 um
+    film of IOR 1.3 under the coat: the delta lobes' Fresnel carries the
+    film), and a thin-walled frosted pane with a 0.6 um film (the slab's
+    internal reflections see the film from inside) hangs in front of the
+    back wall.  Walls, light and floor as cornell_glass.  Materials: 0 walls,
+    1 red, 2 green, 3 coated film box, 4 metal film box, 5 film pane, 6 light,
+    7 floor."""
+    cs = cornell_glass(build=build, textured=textured, light_size=light_size, light_radiance=light_radiance)
+    from .material import OpenPbr
+    mats = list(cs.materials)
+    mats[3] = OpenPbr.new((0.7, 0.75, 0.8)).with_(specular_roughness=0.0, coat_weight=0.6, thin_film_weight=1.0,
+                                                  thin_film_thickness=0.3, thin_film_ior=1.3)
+    mats[4] = OpenPbr.new((0.95, 0.64, 0.54)).with_(base_metalness=1.0, specular_roughness=0.3,
+                                                    thin_film_weight=1.0, thin_film_thickness=0.45,
+                                                    thin_film_ior=1.45)
+    mats[5] = mats[5].with_(thin_film_weight=0.8, thin_film_thickness=0.6, thin_film_ior=1.5)
+    cs.scene.materials.clear()
+    for m in mats:
+        cs.scene.add_material(m)
+    return CornellScene(cs.scene, mats, cs.light_instance, cs.camera)
+
+
 def cornell_inverse(build: bool = True, seed: int = 0, tex_size: int = 512, glossy: bool = False) -> CornellScene:
     """The inverse-rendering Cornell box: the Cornell geometry split into 16
     OpenPBR materials (five walls, the six faces of the short box, five faces
