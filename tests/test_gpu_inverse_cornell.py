@@ -85,19 +85,33 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_inverse_cornell_adjoint_matches_central_differences(setup):
+@pytest.fixture(scope="module")
+def adjoint(setup):
+    """The adjoint of the loss at the seed-0 parameters (computed once for
+    the three tests below, which keeps each test's run well under the
+    time a silent GPU step may take)."""
     cs, ds, cam, target, texels = setup
     theta = theta_of(cs.materials)
     img = render(ds, cs, theta, cam)
     print("inverse Cornell: adjoint at 16,384 spp", flush=True)
     d_params, d_tex = ds.render_adjoint_params(cam, W, H, SPP, DEPTH, (2.0 * (img - target)).astype(np.float32))
+    return theta, d_params, d_tex
+
+
+def test_inverse_cornell_adjoint_rows(setup, adjoint):
+    _, d_params, d_tex = adjoint
     assert d_params.shape == (17, 5)
+    assert len(d_tex) == 1 and np.all(np.isfinite(d_tex[0]))
     # the lamp (row 16: base and specular weight 0) scatters only through its
     # zero-weight specular lobe, whose mirror sample carries the IOR-1 Fresnel
     # term (zero up to rounding) times the lobe weight (1 - metalness): its
     # row is that rounding-level derivative, not a learnable signal
     assert np.abs(d_params[16]).max() <= 1e-6 * np.abs(d_params[:16]).max(), d_params[16]
 
+
+def test_inverse_cornell_parameters_match_central_differences(setup, adjoint):
+    cs, ds, cam, target, texels = setup
+    theta, d_params, _ = adjoint
     # seeded materials: four base-colour channels, three metalness and three
     # roughness entries
     rng = np.random.default_rng(42)
@@ -114,7 +128,22 @@ def test_inverse_cornell_adjoint_matches_central_differences(setup):
         ad_p.append(float(d_params[i, c]))
         print(f"  parameter ({i}, {c}): adjoint {ad_p[-1]:.5g} fd {fd_p[-1]:.5g}", flush=True)
     ds.set_materials(with_params(cs.materials, theta))
+    fd_p, ad_p = np.array(fd_p), np.array(ad_p)
+    ep, er = rel_l2(ad_p[:7], fd_p[:7]), rel_l2(ad_p[7:], fd_p[7:])
+    print(f"inverse Cornell, 256^2, depth {DEPTH}, {SPP} spp, h {STEP_PARAMS}: parameters {picks}\n"
+          f"  adjoint {ad_p}\n  fd      {fd_p}\n  rel L2 base colour + metalness {ep:.4f}, roughness {er:.4f}",
+          flush=True)
+    assert ep < 0.05, ep
+    # roughness moves the sampled directions, so its differences also jump
+    # wherever a moved direction crosses a visibility edge: reported above,
+    # held only to the sign and order of magnitude of the adjoint
+    big = np.abs(fd_p[7:]) > 0.2 * np.abs(fd_p[7:]).max()
+    assert np.all(np.sign(ad_p[7:][big]) == np.sign(fd_p[7:][big])), (ad_p[7:], fd_p[7:])
 
+
+def test_inverse_cornell_texel_patches_match_central_differences(setup, adjoint):
+    cs, ds, cam, target, texels = setup
+    _, _, d_tex = adjoint
     # texels: directional derivatives along 16 x 16 patches (one channel) at
     # the largest adjoint gradients.  A single texel moves a pixel's base
     # colour only by its EWA weight times the step, so hardly any sample's
@@ -146,18 +175,8 @@ def test_inverse_cornell_adjoint_matches_central_differences(setup):
         ad_t.append(float(g[ys, xs, ch].sum()))
         print(f"  texel patch ({y}, {x}, {ch}): adjoint {ad_t[-1]:.5g} fd {fd_t[-1]:.5g}", flush=True)
     ds.set_texture(0, texels)
-
-    fd_p, ad_p, fd_t, ad_t = map(np.array, (fd_p, ad_p, fd_t, ad_t))
-    ep, et = rel_l2(ad_p[:7], fd_p[:7]), rel_l2(ad_t, fd_t)
-    er = rel_l2(ad_p[7:], fd_p[7:])
-    print(f"inverse Cornell, 256^2, depth {DEPTH}, {SPP} spp, h {STEP_PARAMS} (parameters), {STEP} (texels): "
-          f"parameters {picks}\n"
-          f"  adjoint {ad_p}\n  fd      {fd_p}\n  rel L2 base colour + metalness {ep:.4f}, roughness {er:.4f}\n"
-          f"texel patches {centres}\n  adjoint {ad_t}\n  fd      {fd_t}\n  rel L2 {et:.4f}", flush=True)
-    assert ep < 0.05, ep
+    fd_t, ad_t = np.array(fd_t), np.array(ad_t)
+    et = rel_l2(ad_t, fd_t)
+    print(f"inverse Cornell, 256^2, depth {DEPTH}, {SPP} spp, h {STEP} (texels): texel patches {centres}\n"
+          f"  adjoint {ad_t}\n  fd      {fd_t}\n  rel L2 {et:.4f}", flush=True)
     assert et < 0.05, et
-    # roughness moves the sampled directions, so its differences also jump
-    # wherever a moved direction crosses a visibility edge: reported above,
-    # held only to the sign and order of magnitude of the adjoint
-    big = np.abs(fd_p[7:]) > 0.2 * np.abs(fd_p[7:]).max()
-    assert np.all(np.sign(ad_p[7:][big]) == np.sign(fd_p[7:][big])), (ad_p[7:], fd_p[7:])
