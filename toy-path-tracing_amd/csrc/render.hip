@@ -107,6 +107,7 @@ template <bool kTwoLevel, bool kAdj, bool kTex, int kMode, typename Acc, typenam
 __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& sv, const RayDiff& camera_ray,
                                          const smp::PathSampler& ps, uint32_t max_depth, Lane& L, LaneStack& st,
                                          bool& overflow, const Adjoint<Acc, TexAcc>& adj) {
+  constexpr bool kFilm = (kMode & kShadeNoFilm) == 0;
   F3 radiance = f3(0.0f, 0.0f, 0.0f), thr = f3(1.0f, 1.0f, 1.0f);
   uint32_t hi, ht;
   float tt, hb0, hb1, hb2;
@@ -171,14 +172,14 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
         // direction and the sampled direction
         const bool thin_lobe = use_smp && s.trans && p0.thin_walled;  // the thin-wall BTDF's own estimate
         if (use_nee || use_smp)
-          bsdf_param_ratios2(sv.tab, m, vtx, use_nee, wn, rn, use_smp, s.wi, rs, first, thin_lobe);
-        if (use_dlt) bsdf_delta_param_ratios(sv.tab, m, vtx, s.lobe, rs, first);
+          bsdf_param_ratios2<kFilm>(sv.tab, m, vtx, use_nee, wn, rn, use_smp, s.wi, rs, first, thin_lobe);
+        if (use_dlt) bsdf_delta_param_ratios<kFilm>(sv.tab, m, vtx, s.lobe, rs, first);
         float gk[kParamRows];
         for (int k = 0; k < kParamRows; ++k)
           gk[k] = dot(adj.w, add(hadamard(nee_c, rn[k]), hadamard(rest, rs[k])));
         if (lin) {  // the base colour exactly (the ratio is diagonal and exact for affine BSDFs)
-          if (use_nee) g = hadamard(nee_c, base_color_ratio(sv.tab, m, vtx, wn));
-          if (use_smp) g = add(g, hadamard(rest, base_color_ratio(sv.tab, m, vtx, s.wi)));
+          if (use_nee) g = hadamard(nee_c, base_color_ratio<kFilm>(sv.tab, m, vtx, wn));
+          if (use_smp) g = add(g, hadamard(rest, base_color_ratio<kFilm>(sv.tab, m, vtx, s.wi)));
           g = hadamard(g, adj.w);
         } else {
           g = f3(gk[0], gk[1], gk[2]);
@@ -188,8 +189,8 @@ __device__ __forceinline__ F3 trace_path(const SceneView& sc, const ShadeView& s
         if (p0.specular_roughness_texture == TPT_INVALID_INDEX && gk[3] != 0.0f) atomicAdd(&gm[3], (Acc)gk[3]);
         if (p0.base_metalness_texture == TPT_INVALID_INDEX && gk[4] != 0.0f) atomicAdd(&gm[4], (Acc)gk[4]);
       } else {
-        if (use_nee) g = hadamard(nee_c, base_color_ratio(sv.tab, m, vtx, wn));
-        if (use_smp) g = add(g, hadamard(rest, base_color_ratio(sv.tab, m, vtx, s.wi)));
+        if (use_nee) g = hadamard(nee_c, base_color_ratio<kFilm>(sv.tab, m, vtx, wn));
+        if (use_smp) g = add(g, hadamard(rest, base_color_ratio<kFilm>(sv.tab, m, vtx, s.wi)));
         g = hadamard(g, adj.w);  // dL/dc of this sample
       }
       F3 tv = f3(1.0f, 1.0f, 1.0f);
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void render_kernel(SceneView sc
 // Path-replay adjoint of the render loop: per pixel sample a primal pass
 // (its radiance) and a replay pass that accumulates dLoss/d(albedo) into
 // the block's LDS sums; each block writes its sums to `partials`.
-template <bool kTwoLevel, bool kTex, typename Acc>
+template <bool kTwoLevel, bool kTex, bool kFilm, typename Acc>
 __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_kernel(SceneView sc, ShadeView sv, CamView cam, uint32_t w,
                                                          uint32_t h, uint32_t spp, uint32_t max_depth,
                                                          const float* __restrict__ dimg, Acc* __restrict__ partials,
@@ -333,8 +334,9 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_kernel(SceneView s
     for (uint32_t si = 0; si < n_samples; ++si) {
       const smp::PathSampler ps{zf, x, y, si};
       const RayDiff ray = camera_ray<kTex>(cam, ps, x, y, w, h, spp);
-      adj.total = trace_path<kTwoLevel, false, kTex, kShadeOutline>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
-      (void)trace_path<kTwoLevel, true, kTex, kShadeOutline>(sc, sv, ray, ps, max_depth, L, st, overflow, adj);
+      constexpr int kMode = kShadeOutline | (kFilm ? 0 : kShadeNoFilm);
+      adj.total = trace_path<kTwoLevel, false, kTex, kMode>(sc, sv, ray, ps, max_depth, L, st, overflow, none);
+      (void)trace_path<kTwoLevel, true, kTex, kMode>(sc, sv, ray, ps, max_depth, L, st, overflow, adj);
     }
     if (overflow) atomicOr(status, 1u);
   }
@@ -395,7 +397,7 @@ __device__ __forceinline__ float two_sum(float a, float b, float* e) {
 // carries no rounding drift; each block writes sum and compensation, which a
 // second kernel adds in FP64.  kFp64: FP64 LDS sums and FP64 texel
 // gradients instead (the reference accumulation of the error report).
-template <bool kTwoLevel, bool kTex, bool kFp64>
+template <bool kTwoLevel, bool kTex, bool kFilm, bool kFp64>
 __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_batch_kernel(SceneView sc, ShadeView sv,
                                                                const AdjFrame* __restrict__ frames, uint32_t nframes,
                                                                uint32_t total, uint32_t max_spp,
@@ -445,8 +447,9 @@ __global__ __launch_bounds__(kBlock, kPathWaves) void adjoint_batch_kernel(Scene
     if (replay && si < fr.spp) {
       const smp::PathSampler ps{fr.zf, x, y, si};
       const RayDiff ray = camera_ray<kTex>(fr.cam, ps, x, y, fr.w, fr.h, fr.spp);
-      adj.total = trace_path<kTwoLevel, false, kTex, kShadeOutline>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, none);
-      (void)trace_path<kTwoLevel, true, kTex, kShadeOutline>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, adj);
+      constexpr int kMode = kShadeOutline | (kFilm ? 0 : kShadeNoFilm);
+      adj.total = trace_path<kTwoLevel, false, kTex, kMode>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, none);
+      (void)trace_path<kTwoLevel, true, kTex, kMode>(sc, sv, ray, ps, fr.max_depth, L, st, overflow, adj);
     }
     if (!kFp64) {  // fold the sample's epoch into the compensated sums
       __syncthreads();
@@ -719,7 +722,7 @@ pt::CamView cam_view(const TptCamera& c) {
 
 template <typename Acc>
 int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, float* d_albedo, float* d_texels,
-                     bool dev_ptr, uint32_t rows) {
+                     bool dev_ptr, uint32_t rows, bool film) {
   TptDevice* d = s->dev;
   const uint32_t w = desc->width, h = desc->height;
   const size_t n = (size_t)w * h;
@@ -755,8 +758,11 @@ int32_t adjoint_impl(TptScene* s, const TptRenderDesc* desc, const float* dimg, 
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const size_t lds_bytes = (size_t)n_grad * sizeof(Acc);
   const bool tex = s->texture_count > 0;
-  auto kernel = two ? (tex ? pt::adjoint_kernel<true, true, Acc> : pt::adjoint_kernel<true, false, Acc>)
-                    : (tex ? pt::adjoint_kernel<false, true, Acc> : pt::adjoint_kernel<false, false, Acc>);
+  auto kernel = film ? (two ? (tex ? pt::adjoint_kernel<true, true, true, Acc> : pt::adjoint_kernel<true, false, true, Acc>)
+                            : (tex ? pt::adjoint_kernel<false, true, true, Acc> : pt::adjoint_kernel<false, false, true, Acc>))
+                     : (two ? (tex ? pt::adjoint_kernel<true, true, false, Acc> : pt::adjoint_kernel<true, false, false, Acc>)
+                            : (tex ? pt::adjoint_kernel<false, true, false, Acc>
+                                   : pt::adjoint_kernel<false, false, false, Acc>));
   hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), lds_bytes, d->stream, v, sv, cam, w, h,
                      desc->samples_per_pixel, desc->max_depth, ddimg, partials, n_grad, rows, tbuf, d->status);
   hipLaunchKernelGGL((pt::adjoint_reduce<Acc>), dim3((n_grad + 255) / 256), dim3(256), 0, d->stream,
@@ -799,7 +805,8 @@ int32_t render_adjoint(TptScene* s, const TptRenderDesc* desc, const float* dlos
       set_error("tpt_render_adjoint: needs 1.." + std::to_string(max_mats) + " materials");
       return TPT_ERROR_BAD_ARGUMENT;
     }
-    if (!render_materials_ok(s, "tpt_render_adjoint")) return TPT_ERROR_BAD_ARGUMENT;
+    bool film = true;
+    if (!render_materials_ok(s, "tpt_render_adjoint", &film)) return TPT_ERROR_BAD_ARGUMENT;
     if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
     if (rows == pt::kParamRows && !s->cond_tables) {
       // the metalness derivative reads the conductor tables even where a
@@ -821,8 +828,9 @@ int32_t render_adjoint(TptScene* s, const TptRenderDesc* desc, const float* dlos
       return TPT_OK;
     }
     const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
-    return (flags & TPT_FLAG_GRAD_FP64) ? adjoint_impl<double>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr, rows)
-                                        : adjoint_impl<float>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr, rows);
+    return (flags & TPT_FLAG_GRAD_FP64)
+               ? adjoint_impl<double>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr, rows, film)
+               : adjoint_impl<float>(s, desc, dloss_dimage, d_albedo, d_texels, dev_ptr, rows, film);
   } catch (const std::exception& e) {
     set_error(std::string("tpt_render_adjoint: ") + e.what());
     return TPT_ERROR;
@@ -854,7 +862,7 @@ namespace {
 // texel gradients.
 template <bool kFp64>
 int32_t adjoint_batch_impl(TptScene* s, const TptAdjointFrameDesc* frames, uint32_t nframes, float* d_params,
-                           float* d_texels, bool dev_ptr) {
+                           float* d_texels, bool dev_ptr, bool film) {
   using Acc = typename std::conditional<kFp64, double, float>::type;
   TptDevice* d = s->dev;
   const uint32_t n_grad = pt::kParamRows * s->material_count;
@@ -918,8 +926,14 @@ int32_t adjoint_batch_impl(TptScene* s, const TptAdjointFrameDesc* frames, uint3
   const bool two = !(s->instance_count == 1 && s->tlas_node_count == 0);
   const bool tex = s->texture_count > 0;
   const size_t lds_bytes = (size_t)n_grad * (kFp64 ? sizeof(double) : 3 * sizeof(float));
-  auto kernel = two ? (tex ? pt::adjoint_batch_kernel<true, true, kFp64> : pt::adjoint_batch_kernel<true, false, kFp64>)
-                    : (tex ? pt::adjoint_batch_kernel<false, true, kFp64> : pt::adjoint_batch_kernel<false, false, kFp64>);
+  auto kernel =
+      film ? (two ? (tex ? pt::adjoint_batch_kernel<true, true, true, kFp64> : pt::adjoint_batch_kernel<true, false, true, kFp64>)
+                  : (tex ? pt::adjoint_batch_kernel<false, true, true, kFp64>
+                         : pt::adjoint_batch_kernel<false, false, true, kFp64>))
+           : (two ? (tex ? pt::adjoint_batch_kernel<true, true, false, kFp64>
+                         : pt::adjoint_batch_kernel<true, false, false, kFp64>)
+                  : (tex ? pt::adjoint_batch_kernel<false, true, false, kFp64>
+                         : pt::adjoint_batch_kernel<false, false, false, kFp64>));
   hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), lds_bytes, st, v, sv, (const pt::AdjFrame*)dframes, nframes,
                      (uint32_t)total, max_spp, partials, n_grad, (void*)tbuf, d->status);
   hipLaunchKernelGGL((pt::adjoint_batch_reduce<kFp64>), dim3((n_grad + 255) / 256), dim3(256), 0, st,
@@ -970,15 +984,17 @@ extern "C" int32_t tpt_render_adjoint_batch(TptScene* s, const TptAdjointFrameDe
       set_error("tpt_render_adjoint_batch: needs 1..1024 materials");
       return TPT_ERROR_BAD_ARGUMENT;
     }
-    if (!render_materials_ok(s, "tpt_render_adjoint_batch")) return TPT_ERROR_BAD_ARGUMENT;
+    bool film = true;
+    if (!render_materials_ok(s, "tpt_render_adjoint_batch", &film)) return TPT_ERROR_BAD_ARGUMENT;
     if (!hip_ok(hipSetDevice(s->dev->index), "hipSetDevice")) return TPT_ERROR_DEVICE;
     if (!s->cond_tables) {  // the metalness derivative reads the conductor tables
       s->force_cond = true;
       if (!upload_ggx_tables(s, s->host_mats)) return TPT_ERROR_DEVICE;
     }
     const bool dev_ptr = flags & TPT_FLAG_DEVICE_POINTERS;
-    return (flags & TPT_FLAG_GRAD_FP64) ? adjoint_batch_impl<true>(s, frames, frame_count, d_params, d_texels, dev_ptr)
-                                        : adjoint_batch_impl<false>(s, frames, frame_count, d_params, d_texels, dev_ptr);
+    return (flags & TPT_FLAG_GRAD_FP64)
+               ? adjoint_batch_impl<true>(s, frames, frame_count, d_params, d_texels, dev_ptr, film)
+               : adjoint_batch_impl<false>(s, frames, frame_count, d_params, d_texels, dev_ptr, film);
   } catch (const std::exception& e) {
     set_error(std::string("tpt_render_adjoint_batch: ") + e.what());
     return TPT_ERROR;

@@ -1987,13 +1987,14 @@ __device__ __host__ __forceinline__ bool base_color_linear(const TptMaterial& m)
 // colour.  The shading-normal correction is common to both and cancels.
 // wil: the shading-frame direction (a BSDF sample's own, or a light
 // direction brought into the frame as mat_eval_pdf does).
+template <bool kFilm = true>
 __device__ __forceinline__ F3 base_color_ratio(const Tables& tb, const TptMaterial& m, const Vtx& v, F3 wil) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
   if (dot(v.wo, v.ng) <= 0.0f) return zero;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return zero;
-  Bsdf b;
-  make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
+  BsdfT<kFilm> b;
+  make_bsdf_call<kFilm>(tb, v.material, m.u.open_pbr, v.front, wol, &b);
   const F3 f = bsdf_eval(b, wol, wil), d = bsdf_eval_dbase(b, wol, wil);
   return f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
 }
@@ -2020,6 +2021,7 @@ template <bool kF>
 __device__ __forceinline__ F3 sampled_value(const BsdfT<kF>& b, F3 wo, F3 wi, bool thin_lobe) {
   return thin_lobe ? hadamard(layer_weights(b, wo, &wi).btdf, thin_btdf_eval(b, wo, wi)) : bsdf_eval(b, wo, wi);
 }
+template <bool kFilm = true>
 __device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMaterial& m, const Vtx& v, bool na, F3 wia,
                                                 F3 ra[kParamRows], bool nb, F3 wib, F3 rb[kParamRows], int first,
                                                 bool thin_b = false) {
@@ -2028,8 +2030,8 @@ __device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMater
   if (dot(v.wo, v.ng) <= 0.0f) return;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return;
-  Bsdf b;
-  make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
+  BsdfT<kFilm> b;
+  make_bsdf_call<kFilm>(tb, v.material, m.u.open_pbr, v.front, wol, &b);
   const F3 fa0 = na ? bsdf_eval(b, wol, wia) : zero, fb0 = nb ? sampled_value(b, wol, wib, thin_b) : zero;
   na = na && ((fa0.x > 0.0f) || (fa0.y > 0.0f) || (fa0.z > 0.0f));
   nb = nb && ((fb0.x > 0.0f) || (fb0.y > 0.0f) || (fb0.z > 0.0f));
@@ -2045,9 +2047,9 @@ __device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMater
     if (!(hi > lo)) continue;
     pbr_param(a, k) = hi;
     pbr_param(c, k) = lo;
-    make_bsdf_call(tb, v.material, a, v.front, wol, &b);
+    make_bsdf_call<kFilm>(tb, v.material, a, v.front, wol, &b);
     const F3 fah = na ? bsdf_eval(b, wol, wia) : zero, fbh = nb ? sampled_value(b, wol, wib, thin_b) : zero;
-    make_bsdf_call(tb, v.material, c, v.front, wol, &b);
+    make_bsdf_call<kFilm>(tb, v.material, c, v.front, wol, &b);
     const F3 fal = na ? bsdf_eval(b, wol, wia) : zero, fbl = nb ? sampled_value(b, wol, wib, thin_b) : zero;
     if (na) ra[k] = ratio(divs(sub(fah, fal), hi - lo), fa0);
     if (nb) rb[k] = ratio(divs(sub(fbh, fbl), hi - lo), fb0);
@@ -2061,6 +2063,7 @@ __device__ __noinline__ void bsdf_param_ratios2(const Tables& tb, const TptMater
 // over its lobe probability, which the adjoint holds fixed with the rest of
 // the sampling, so a contribution that went through it changes by
 // (d numerator / d theta_k) / numerator per unit of theta_k.
+template <bool kFilm = true>
 __device__ __noinline__ void bsdf_delta_param_ratios(const Tables& tb, const TptMaterial& m, const Vtx& v, int lobe,
                                                      F3 r[kParamRows], int first) {
   const F3 zero = f3(0.0f, 0.0f, 0.0f);
@@ -2068,8 +2071,8 @@ __device__ __noinline__ void bsdf_delta_param_ratios(const Tables& tb, const Tpt
   if (dot(v.wo, v.ng) <= 0.0f) return;
   const F3 wol = normalize_or_zero(to_frame(v.frame, v.wo));
   if (wol.z <= 0.0f) return;
-  Bsdf b;
-  make_bsdf_call(tb, v.material, m.u.open_pbr, v.front, wol, &b);
+  BsdfT<kFilm> b;
+  make_bsdf_call<kFilm>(tb, v.material, m.u.open_pbr, v.front, wol, &b);
   const F3 f = delta_numerator(b, wol, lobe);
   if (!(f.x > 0.0f) && !(f.y > 0.0f) && !(f.z > 0.0f)) return;
   for (int k = first; k < kParamRows; ++k) {
@@ -2080,9 +2083,9 @@ __device__ __noinline__ void bsdf_delta_param_ratios(const Tables& tb, const Tpt
     if (!(hi > lo)) continue;
     pbr_param(a, k) = hi;
     pbr_param(c, k) = lo;
-    make_bsdf_call(tb, v.material, a, v.front, wol, &b);
+    make_bsdf_call<kFilm>(tb, v.material, a, v.front, wol, &b);
     const F3 fa = delta_numerator(b, wol, lobe);
-    make_bsdf_call(tb, v.material, c, v.front, wol, &b);
+    make_bsdf_call<kFilm>(tb, v.material, c, v.front, wol, &b);
     const F3 fc = delta_numerator(b, wol, lobe);
     const F3 d = divs(sub(fa, fc), hi - lo);
     r[k] = f3(f.x > 0.0f ? d.x / f.x : 0.0f, f.y > 0.0f ? d.y / f.y : 0.0f, f.z > 0.0f ? d.z / f.z : 0.0f);
