@@ -13,7 +13,8 @@ Each entry records the scene generator's geometry digest, so a changed
 generator makes the CPU suite fail until the images are regenerated.
 
 Run from the repository root after building the oracle (about 5 minutes on
-8 host threads for the first six cases, 10 more for the two glass boxes):
+8 host threads for the first six cases, 10 more for the two glass boxes,
+about 7 for the thin-film box):
     python tests/golden/make_reference_images.py
 Entries whose scene digest still matches are kept; --all renders every case.
 """
@@ -45,6 +46,7 @@ CASES = {
     "outdoor_lambert": (lambda: scenes.outdoor("lambert"), 24, 6),
     "cornell_glass": (lambda: scenes.cornell_glass(), 48, 8),
     "cornell_glass_textured": (lambda: scenes.cornell_glass(textured=True), 32, 8),
+    "cornell_film": (lambda: scenes.cornell_film(), 48, 8),
 }
 
 

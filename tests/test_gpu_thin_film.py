@@ -7,6 +7,8 @@ restatement of the reference renderer.
 * Pixel by pixel against the oracle at equal samples: most pixels
   bit-identical (a last-place difference of a transcendental can send a
   path down another branch; such a pixel differs by that path).
+* Against the CPU oracle's 16,384-spp image of the film box the GPU
+  image's relMSE at 64 spp is no worse than the CPU image's.
 * The wavefront batch equals the megakernel bit for bit.
 * Scenes without a film launch kernels built without the film code (the
   host checks the material table); a film-free scene renders the same bits
@@ -22,6 +24,7 @@ import time
 import numpy as np
 import pytest
 
+import golden_images
 import oracle
 from tpt import camera, scenes
 from tpt.material import OpenPbr
@@ -49,6 +52,18 @@ def test_film_render_matches_oracle_pixel_by_pixel(film):
     print(f"bit-identical pixels: {same.mean():.4f}, within 1e-5: {close.mean():.4f}")
     assert same.mean() > 0.9
     assert close.mean() > 0.95
+
+
+def test_film_relmse_no_worse_than_cpu(device):
+    """Against the CPU oracle's 16,384-spp image of the film box
+    (tests/golden/reference_images.npz) the GPU image's relMSE at 64 spp is
+    no worse than the CPU image's at 64 spp."""
+    cs = scenes.cornell_film()
+    ds = cs.scene.upload(device)
+    e_gpu, e_cpu = golden_images.gpu_and_cpu_relmse("cornell_film", cs, ds, 64)
+    ds.close()
+    print(f"cornell_film relMSE at 64 spp against the CPU reference: gpu {e_gpu:.5f} cpu {e_cpu:.5f}")
+    assert e_gpu <= 1.1 * e_cpu + 1e-6
 
 
 def test_film_wavefront_equals_megakernel(film):
