@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--rays", type=int, default=2_000_000, help="bounce and shadow ray counts")
     ap.add_argument("--render", type=int, default=1, help="also time the Cornell path tracer (1) or not (0)")
     ap.add_argument("--render-scenes", default="openpbr,openpbr_glossy,lambert_nee,textured",
-                    help="comma-separated render lines to time (openpbr, openpbr_glossy, lambert_nee, textured)")
+                    help="comma-separated render lines to time (openpbr, openpbr_glossy, lambert_nee, textured; "
+                         "also glass, film)")
     ap.add_argument("--adjoint", type=int, default=1, help="also time the path-replay adjoint (1) or not (0)")
     ap.add_argument("--city", type=int, default=1,
                     help="also time the four-frame city batch (1) or not (0): megakernel vs wavefront fp32 / q8")
@@ -315,6 +316,11 @@ def render_bench(dev, stream, args):
         ("textured", scenes.cornell_textured,
          "OpenPBR with base-colour, roughness, metalness, fuzz and emission textures (EWA over mip pyramids, "
          "ray differentials and cones), a Mirror box and a NormalizedLambert box with light-tree NEE"),
+        ("glass", scenes.cornell_glass,
+         "OpenPBR transmission: smooth glass, rough tinted glass, a thin-walled frosted pane (not run by default)"),
+        ("film", scenes.cornell_film,
+         "OpenPBR thin film: rough metal, smooth coated dielectric and a thin-walled pane under films "
+         "(not run by default)"),
     ):
         if key not in args.render_scenes.split(","):
             continue
