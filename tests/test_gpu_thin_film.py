@@ -112,7 +112,7 @@ def test_params_adjoint_through_film_matches_detached_differences(device):
     derivative the path-replay adjoint accumulates sample by sample.
     Central differences of plain renders with common random numbers
     estimate the same gradient through moved samples and are far noisier
-    on the roughness rows (measured on this box at 256 x 256, 1,024 spp:
+    on the roughness rows (measured on MI355X on the film Cornell box at 256 x 256, 1,024 spp:
     30-80 % apart from the adjoint there, with or without the films, and
     10-50 % apart between steps 5e-3 and 2e-2)."""
     cs = scenes.cornell_film(light_size=0.9, light_radiance=2.0)
